@@ -1,0 +1,199 @@
+"""Seeded workload generators for the step benchmarks and tests.
+
+One project PRNG (xoshiro256** seeded through SplitMix64) so every host
+produces the same scene from the same seed.  Scene builders use the host
+mirror (chipmunk2d_amd.space) exactly as a Chipmunk program would use the C
+API, and return a `Space` (or a list of them) ready for `pack()`.
+
+Workloads (sizes as named):
+  pyramid_210       20-row pyramid of 210 boxes on a ground segment, slop 0.5
+  random_world      circles + convex polys falling on a static container with
+                    sinusoidal terrain; default 50k circles, 40k polys, 10k segments
+  batch_worlds      many independent spaces of log-uniform body counts, each a
+                    pyramid or a random pile on its own ground
+"""
+import math
+
+from .space import Space, Body, Circle, Segment, Poly, Box, moment_for_box, PivotJoint, DampedSpring, \
+    SlideJoint, GearJoint
+
+MASK64 = (1 << 64) - 1
+
+SEED_PYRAMID = 0xC4100001
+SEED_RANDOM_WORLD = 0xC4100002
+SEED_BATCH = 0xC4100003
+SEED_GRAD = 0xC4100004
+SEED_CHAINS = 0xC4100005
+
+
+def _rotl(x, k):
+    return ((x << k) | (x >> (64 - k))) & MASK64
+
+
+class Rng:
+    """xoshiro256** (Blackman & Vigna), seeded through SplitMix64."""
+
+    def __init__(self, seed):
+        s = seed & MASK64
+        self.s = []
+        for _ in range(4):
+            s = (s + 0x9E3779B97F4A7C15) & MASK64
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+            self.s.append(z ^ (z >> 31))
+
+    def next_u64(self):
+        s = self.s
+        result = (_rotl((s[1] * 5) & MASK64, 7) * 9) & MASK64
+        t = (s[1] << 17) & MASK64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = _rotl(s[3], 45)
+        return result
+
+    def uniform(self, lo=0.0, hi=1.0):
+        u = (self.next_u64() >> 11) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+    def randint(self, lo, hi):
+        """Integer in [lo, hi] inclusive."""
+        return lo + self.next_u64() % (hi - lo + 1)
+
+    def normal(self, mu=0.0, sigma=1.0):
+        u1 = self.uniform()
+        u2 = self.uniform()
+        r = math.sqrt(-2.0 * math.log(1.0 - u1))
+        return mu + sigma * r * math.cos(2.0 * math.pi * u2)
+
+
+def pyramid_210(rows=20, ground_friction=0.6, x_offset=0.0, space=None):
+    """20-row pyramid of boxes (side 30, bevel 0.5, mass 1, friction 0.6) on a
+    static ground segment from (-500,-240) to (500,-240); gravity (0,-100),
+    10 iterations, collision slop 0.5.  Row r holds rows-r boxes spaced 32."""
+    if space is None:
+        space = Space()
+        space.iterations = 10
+        space.gravity = (0.0, -100.0)
+        space.collision_slop = 0.5
+    ground = space.add(Segment(space.static_body, (x_offset - 500.0, -240.0), (x_offset + 500.0, -240.0), 0.0))
+    ground.elasticity = 0.0
+    ground.friction = ground_friction
+    for r in range(rows):
+        for j in range(rows - r):
+            body = space.add(Body(1.0, moment_for_box(1.0, 30.0, 30.0)))
+            body.position = (x_offset + (j - (rows - 1 - r) / 2.0) * 32.0, -240.0 + 15.5 + 31.0 * r)
+            shape = space.add(Box(body, 30.0, 30.0, 0.5))
+            shape.elasticity = 0.0
+            shape.friction = 0.6
+    return space
+
+
+def _random_poly_verts(rng, k, radius):
+    verts = []
+    for _ in range(k):
+        ang = rng.uniform(0.0, 2.0 * math.pi)
+        verts.append((radius * math.cos(ang), radius * math.sin(ang)))
+    return verts
+
+
+def random_world(n_circles=50000, n_polys=40000, n_segments=10000, extent=2000.0, seed=SEED_RANDOM_WORLD,
+                 terrain_amplitude=20.0, terrain_period=200.0, segment_radius=0.5, vel_sigma=5.0):
+    """Circles (radius U[0.5,1.5]) and convex polys (hull of 3-8 points at a
+    radius U[1,2], bevel 0.05), density 1, friction U[0.3,0.9], elasticity 0,
+    positions uniform over an extent x extent box, velocities N(0, vel_sigma);
+    a static container: 10% of the segments form the two side walls, the rest
+    a sinusoidal terrain along the bottom.  Gravity (0,-100), 10 iterations."""
+    rng = Rng(seed)
+    space = Space()
+    space.iterations = 10
+    space.gravity = (0.0, -100.0)
+    half = extent / 2.0
+    n_wall = max(2, n_segments // 10) if n_segments else 0
+    n_terrain = n_segments - n_wall
+    sb = space.static_body
+    if n_wall:
+        per_side = n_wall // 2
+        for side, x in ((0, -half - 10.0), (1, half + 10.0)):
+            cnt = per_side if side == 0 else n_wall - per_side
+            y0, y1 = -half - 50.0, half + 50.0
+            for i in range(cnt):
+                a = (x, y0 + (y1 - y0) * i / cnt)
+                b = (x, y0 + (y1 - y0) * (i + 1) / cnt)
+                s = space.add(Segment(sb, a, b, segment_radius))
+                s.friction = 0.7
+    if n_terrain:
+        x0, x1 = -half - 10.0, half + 10.0
+        ybase = -half - 30.0
+
+        def ty(x):
+            return ybase + terrain_amplitude * math.sin(2.0 * math.pi * x / terrain_period)
+
+        for i in range(n_terrain):
+            xa = x0 + (x1 - x0) * i / n_terrain
+            xb = x0 + (x1 - x0) * (i + 1) / n_terrain
+            s = space.add(Segment(sb, (xa, ty(xa)), (xb, ty(xb)), segment_radius))
+            s.friction = 0.7
+    # dynamic bodies interleaved so circles and polys share the index range
+    kinds = [0] * n_circles + [1] * n_polys
+    for i in range(len(kinds) - 1, 0, -1):
+        j = rng.randint(0, i)
+        kinds[i], kinds[j] = kinds[j], kinds[i]
+    for kind in kinds:
+        body = space.add(Body(0.0, 0.0))
+        if kind == 0:
+            shape = space.add(Circle(body, rng.uniform(0.5, 1.5)))
+        else:
+            k = rng.randint(3, 8)
+            verts = _random_poly_verts(rng, k, rng.uniform(1.0, 2.0))
+            shape = space.add(Poly(body, verts, radius=0.05))
+        shape.density = 1.0
+        shape.friction = rng.uniform(0.3, 0.9)
+        shape.elasticity = 0.0
+        body.position = (rng.uniform(-half, half), rng.uniform(-half, half))
+        body.velocity = (rng.normal(0.0, vel_sigma), rng.normal(0.0, vel_sigma))
+    return space
+
+
+def random_pile(n_bodies, rng, width=None, x_offset=0.0):
+    """A random circle/poly pile of n_bodies falling on its own ground segment."""
+    space = Space()
+    space.iterations = 10
+    space.gravity = (0.0, -100.0)
+    if width is None:
+        width = max(40.0, 4.0 * math.sqrt(n_bodies) * 2.0)
+    half = width / 2.0
+    ground = space.add(Segment(space.static_body, (x_offset - half - 20.0, 0.0), (x_offset + half + 20.0, 0.0), 0.5))
+    ground.friction = 0.7
+    height = 4.0 * n_bodies * 4.0 / width + 20.0
+    for _ in range(n_bodies):
+        body = space.add(Body(0.0, 0.0))
+        if rng.uniform() < 0.5:
+            shape = space.add(Circle(body, rng.uniform(0.5, 1.5)))
+        else:
+            k = rng.randint(3, 8)
+            shape = space.add(Poly(body, _random_poly_verts(rng, k, rng.uniform(1.0, 2.0)), radius=0.05))
+        shape.density = 1.0
+        shape.friction = rng.uniform(0.3, 0.9)
+        body.position = (x_offset + rng.uniform(-half, half), rng.uniform(3.0, 3.0 + height))
+        body.velocity = (rng.normal(0.0, 5.0), rng.normal(0.0, 5.0))
+    return space
+
+
+def batch_worlds(n_worlds=4096, lo=16, hi=2048, seed=SEED_BATCH, pyramid_fraction=0.5):
+    """Independent spaces with body counts log-uniform in [lo, hi]; each is a
+    pyramid stack (rows chosen so rows*(rows+1)/2 is near the count) or a
+    random pile, on its own ground."""
+    rng = Rng(seed)
+    spaces = []
+    for _ in range(n_worlds):
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        if rng.uniform() < pyramid_fraction:
+            rows = max(1, int((math.sqrt(8 * n + 1) - 1) / 2))
+            spaces.append(pyramid_210(rows=rows))
+        else:
+            spaces.append(random_pile(n, rng))
+    return spaces

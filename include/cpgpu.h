@@ -1,0 +1,221 @@
+/*
+ * cpgpu.h -- C ABI of the MI355X (gfx950) step library for Chipmunk2D spaces.
+ *
+ * The library replaces the body of Chipmunk2D 7.0.3's cpSpaceStep
+ * (src/cpSpaceStep.c:335-445) with hand-written HIP kernels.  It does not
+ * know Chipmunk's structs: a caller packs its space into the plain
+ * structure-of-arrays description below (doubles, integers, plain pointers
+ * and sizes) and gets body state back the same way.  The packing code a
+ * Chipmunk maintainer adds on the reference side (cpGPUSpaceNew / Step /
+ * SyncHost over cpSpace, modelled on cpHastySpace) is described in
+ * INTEGRATION.md; the Python host mirror in chipmunk2d_amd/ packs its own
+ * Space objects into the same description.
+ *
+ * Several independent spaces ("worlds") can be packed into one description;
+ * they are stepped together, each with its own parameters, and never interact.
+ *
+ * Errors: every entry point returns CPGPU_OK (0) or a negative code and
+ * records a message retrievable with cpgpuGetLastError().  The reference
+ * aborts through cpAssertHard (include/chipmunk/chipmunk.h:48-58); this
+ * library returns the same condition text as an error instead, so the caller
+ * (for instance the cpGPUSpaceStep binding) decides whether to abort.
+ */
+#ifndef CPGPU_H
+#define CPGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPGPU_OK 0
+#define CPGPU_ERR_INVALID -1      /* bad argument or unsupported feature in the description */
+#define CPGPU_ERR_HIP -2          /* a HIP runtime call failed */
+#define CPGPU_ERR_CAPACITY -3     /* a fixed device capacity was exceeded */
+#define CPGPU_ERR_DEVICE -4       /* the device-side error word was raised during a step */
+
+/* Body types, same numbering as cpBodyType (include/chipmunk/cpBody.h:30-45). */
+#define CPGPU_BODY_DYNAMIC 0
+#define CPGPU_BODY_KINEMATIC 1
+#define CPGPU_BODY_STATIC 2
+
+/* Shape types, same numbering as cpShapeType (include/chipmunk/chipmunk_structs.h:166-171). */
+#define CPGPU_SHAPE_CIRCLE 0
+#define CPGPU_SHAPE_SEGMENT 1
+#define CPGPU_SHAPE_POLY 2
+
+/* Constraint types (the joint classes of src/cpPivotJoint.c, cpDampedSpring.c,
+ * cpSlideJoint.c, cpGearJoint.c). */
+#define CPGPU_JOINT_PIVOT 0
+#define CPGPU_JOINT_DAMPED_SPRING 1
+#define CPGPU_JOINT_SLIDE 2
+#define CPGPU_JOINT_GEAR 3
+
+/* Arbiter states, same numbering as enum cpArbiterState (chipmunk_structs.h:83-95). */
+#define CPGPU_ARB_FIRST_COLLISION 0
+#define CPGPU_ARB_NORMAL 1
+#define CPGPU_ARB_IGNORE 2
+#define CPGPU_ARB_CACHED 3
+
+/* Arithmetic of the device state. */
+#define CPGPU_FP64 0
+#define CPGPU_FP32 1
+
+/* Per-space parameters: the cpSpace fields read by cpSpaceStep
+ * (chipmunk_structs.h:396-442; defaults set in src/cpSpace.c:131-156). */
+typedef struct cpgpuSpaceParams {
+	int32_t iterations;            /* cpSpace.iterations (default 10) */
+	uint32_t collision_persistence;/* cpSpace.collisionPersistence (default 3) */
+	double gravity_x, gravity_y;   /* cpSpace.gravity */
+	double damping;                /* cpSpace.damping (default 1) */
+	double collision_slop;         /* cpSpace.collisionSlop (default 0.1) */
+	double collision_bias;         /* cpSpace.collisionBias (default pow(0.9, 60)) */
+	double curr_dt;                /* cpSpace.curr_dt: dt of the previous step, 0 for a fresh space */
+	uint32_t stamp;                /* cpSpace.stamp */
+	uint32_t _pad;
+} cpgpuSpaceParams;
+
+/* Packed description of one or more spaces.  Arrays marked [n][2] hold x,y
+ * pairs; [n][6] transforms are cpTransform {a, b, c, d, tx, ty}
+ * (include/chipmunk/chipmunk_types.h:257-259).  Bodies, shapes and
+ * constraints of world w occupy the index ranges
+ * [world_*_start[w], world_*_start[w+1]). */
+typedef struct cpgpuWorldDesc {
+	int32_t n_worlds;
+	const cpgpuSpaceParams *space;       /* [n_worlds] */
+	const int32_t *world_body_start;     /* [n_worlds+1] */
+	const int32_t *world_shape_start;    /* [n_worlds+1] */
+	const int32_t *world_constraint_start; /* [n_worlds+1]; may be NULL when n_constraints == 0 */
+
+	/* Bodies (struct cpBody, chipmunk_structs.h:35-81). */
+	int32_t n_bodies;
+	const int32_t *body_type;            /* CPGPU_BODY_* */
+	const double *body_m, *body_m_inv;   /* mass and inverse (INFINITY / 0 for static and kinematic) */
+	const double *body_i, *body_i_inv;   /* moment and inverse */
+	const double *body_cog;              /* [n][2] centre of gravity, body-local */
+	const double *body_p;                /* [n][2] position of the centre of gravity */
+	const double *body_v;                /* [n][2] */
+	const double *body_f;                /* [n][2] force accumulated for the next step */
+	const double *body_a, *body_w, *body_t; /* angle, angular velocity, torque */
+	const double *body_v_bias;           /* [n][2] pseudo-velocity */
+	const double *body_w_bias;
+	const double *body_transform;        /* [n][6] */
+
+	/* Shapes (struct cpShape and subclasses, chipmunk_structs.h:177-236). */
+	int32_t n_shapes;
+	const int32_t *shape_type;           /* CPGPU_SHAPE_* */
+	const int32_t *shape_body;           /* global body index */
+	const uint64_t *shape_hashid;        /* cpShape.hashid (assigned by cpSpaceAddShape, src/cpSpace.c:432) */
+	const double *shape_e, *shape_u;     /* elasticity, friction */
+	const double *shape_surface_v;       /* [n][2] */
+	const uint64_t *shape_group;         /* cpShapeFilter.group */
+	const uint32_t *shape_categories;    /* cpShapeFilter.categories */
+	const uint32_t *shape_mask;          /* cpShapeFilter.mask */
+	const double *shape_radius;          /* circle r, segment r, poly bevel r */
+	/* [n][10] body-local geometry: circle {c.x, c.y}; segment {a.x, a.y, b.x, b.y,
+	 * n.x, n.y, a_tangent.x, a_tangent.y, b_tangent.x, b_tangent.y}; unused for polys. */
+	const double *shape_geom;
+	/* Polygons: vertices and edge normals in body-local space, in the order
+	 * cpPolyShape stores them (planes[i+count] of src/cpPolyShape.c:147-165). */
+	const int32_t *shape_poly_start;     /* [n] first vertex of the shape in poly_verts */
+	const int32_t *shape_poly_count;     /* [n] vertex count (0 for non-polys), at most CPGPU_MAX_POLY_VERTS */
+	int32_t n_poly_verts;
+	const double *poly_verts;            /* [n_poly_verts][2] */
+	const double *poly_normals;          /* [n_poly_verts][2] */
+
+	/* Constraints (struct cpConstraint and joints, chipmunk_structs.h:250-391). */
+	int32_t n_constraints;
+	const int32_t *cons_type;            /* CPGPU_JOINT_* */
+	const int32_t *cons_a, *cons_b;      /* global body indices */
+	const double *cons_max_force, *cons_error_bias, *cons_max_bias;
+	const int32_t *cons_collide_bodies;  /* 0 or 1 */
+	/* [n][8] joint parameters:
+	 *   pivot  {anchorA.x, anchorA.y, anchorB.x, anchorB.y}
+	 *   spring {anchorA.x, anchorA.y, anchorB.x, anchorB.y, restLength, stiffness, damping}
+	 *   slide  {anchorA.x, anchorA.y, anchorB.x, anchorB.y, min, max}
+	 *   gear   {phase, ratio, ratio_inv} */
+	const double *cons_params;
+	/* [n][2] accumulated impulse carried between steps (pivot jAcc.x/.y,
+	 * spring jAcc, slide jnAcc, gear jAcc); NULL means zero. */
+	const double *cons_jacc;
+} cpgpuWorldDesc;
+
+#define CPGPU_MAX_POLY_VERTS 64
+
+typedef struct cpgpuWorld cpgpuWorld;
+
+/* Step statistics of the most recent step, summed over worlds. */
+typedef struct cpgpuStats {
+	int64_t candidate_pairs;   /* pairs that passed the exact QueryReject predicate */
+	int64_t arbiters;          /* arbiters solved this step (cpSpace.arbiters) */
+	int64_t contacts;          /* contacts solved this step */
+	int64_t cached_arbiters;   /* arbiters held in the persistence table */
+	int32_t arbiter_colours;   /* colours of the arbiter graph */
+	int32_t constraint_colours;/* colours of the constraint graph */
+	int64_t steps;             /* steps taken since creation */
+} cpgpuStats;
+
+/* ---- lifecycle (replaces cpHastySpaceNew / cpHastySpaceFree, src/cpHastySpace.c:560-592) ---- */
+
+/* Upload a packed description to the current HIP device.  precision is
+ * CPGPU_FP64 (bit-for-bit the reference's double arithmetic) or CPGPU_FP32. */
+int cpgpuWorldCreate(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld **out);
+void cpgpuWorldDestroy(cpgpuWorld *world);
+
+/* ---- the step (replaces cpSpaceStep, src/cpSpaceStep.c:335-445, and
+ * cpHastySpaceStep, src/cpHastySpace.c:594-700) ---- */
+
+/* Advance every world by dt.  Asynchronous: returns once the work is queued. */
+int cpgpuWorldStep(cpgpuWorld *world, double dt);
+/* Advance every world by dt, n_steps times. */
+int cpgpuWorldStepN(cpgpuWorld *world, double dt, int32_t n_steps);
+/* Wait for queued steps and check the device error word. */
+int cpgpuWorldSync(cpgpuWorld *world);
+
+/* ---- state readback (the role of the cpBody getters after cpSpaceStep) ---- */
+
+/* Copy body state to host arrays sized like the description; any pointer may be NULL. */
+int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, double *w,
+                         double *v_bias, double *w_bias, double *transform);
+/* Overwrite body velocities / forces from host arrays (the cpBodySetVelocity /
+ * cpBodySetForce setters); any pointer may be NULL. */
+int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w,
+                          const double *f, const double *t);
+int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out);
+
+/* Copy the arbiters solved in the last step (cpSpace.arbiters, in solve
+ * order: by colour, then by key).  Arrays have room for `cap` arbiters;
+ * returns the number of arbiters through *n_out.  Any pointer may be NULL.
+ *   shapes  [cap][2] global shape indices (a, b) in narrowphase orientation
+ *   colour  [cap]    solve colour
+ *   state   [cap]    CPGPU_ARB_*
+ *   count   [cap]    contact count (1 or 2)
+ *   normal  [cap][2]
+ *   contact [cap][2][12] per contact {r1.x, r1.y, r2.x, r2.y, nMass, tMass,
+ *                    bounce, jnAcc, jtAcc, jBias, bias, 0}
+ *   hash    [cap][2] contact feature hashes */
+int cpgpuWorldReadArbiters(cpgpuWorld *world, int64_t cap, int64_t *n_out,
+                           int32_t *shapes, int32_t *colour, int32_t *state, int32_t *count,
+                           double *normal, double *contact, uint64_t *hash);
+/* Copy constraint accumulated impulses ([n][2], see cons_jacc) and colours. */
+int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour);
+
+/* ---- profiling ---- */
+
+/* Record HIP events around every kernel launch (0 = off, the default). */
+int cpgpuWorldSetTiming(cpgpuWorld *world, int32_t enable);
+/* Per-kernel accumulated device time in ms and launch counts since timing was
+ * enabled.  names is filled with a '\n'-separated list of kernel names.
+ * Returns the number of kernels through *n_out. */
+int cpgpuWorldGetTiming(cpgpuWorld *world, int32_t cap, int32_t *n_out, double *ms,
+                        int64_t *launches, char *names, int32_t names_cap);
+
+const char *cpgpuGetLastError(void);
+const char *cpgpuVersion(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPGPU_H */

@@ -1,0 +1,1742 @@
+/*
+ * cpstep_oracle.c -- CPU restatement of Chipmunk2D 7.0.3's cpSpaceStep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the HIP step library; see the
+ * header).  Compiled with gcc -O2 -ffp-contract=off -fno-fast-math so every
+ * + - * / sqrt is a single IEEE double operation, the same as the reference's
+ * strict build and the device library.
+ *
+ * Every routine cites the reference file:line it restates.  How each part is
+ * pinned (tests/test_oracle_pins.py, against the reference compiled from
+ * /root/reference into oracle/_ref by oracle/Makefile):
+ *   integration, shape caching   bit-exact vs cpBodyUpdatePosition/Velocity and
+ *                                cpShapeUpdate of the compiled reference, plus the
+ *                                closed form of BodyTest.m:147-184
+ *   candidate pairs              equal as a set to the reference's own
+ *                                cpSpatialIndexReindexQuery + QueryReject, and to
+ *                                O(n^2) brute force on small worlds
+ *   narrowphase                  bit-exact vs the reference's cpCollide on the same
+ *                                shapes, orientation and GJK id (all 6 pair types)
+ *   one full step                bit-exact vs cpSpaceStep when replayed in the
+ *                                reference's arbiter order with libm trig
+ *   joints                       bit-exact vs one cpSpaceStep with joints replayed
+ *                                in the reference's constraint order
+ *   cp_sincos / cp_exp           within 1 ulp of glibc over dense sweeps
+ * The deterministic graph colouring (the device solve order) has no reference
+ * counterpart; it is pinned by its defining properties (a proper colouring,
+ * every arbiter coloured, identical to an independent Python restatement).
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "cpstep_oracle.h"
+
+/* ------------------------------------------------------------------ */
+/* scalar and vector helpers: include/chipmunk/chipmunk_types.h:96-147,
+ * cpVect.h, cpBB.h, cpTransform.h                                     */
+
+typedef struct { double x, y; } V;
+
+static inline V v2(double x, double y) { V r = {x, y}; return r; }
+static inline V vadd(V a, V b) { return v2(a.x + b.x, a.y + b.y); }
+static inline V vsub(V a, V b) { return v2(a.x - b.x, a.y - b.y); }
+static inline V vneg(V a) { return v2(-a.x, -a.y); }
+static inline V vmult(V a, double s) { return v2(a.x * s, a.y * s); }
+static inline double vdot(V a, V b) { return a.x * b.x + a.y * b.y; }
+static inline double vcross(V a, V b) { return a.x * b.y - a.y * b.x; }
+static inline V vperp(V a) { return v2(-a.y, a.x); }
+static inline V vrperp(V a) { return v2(a.y, -a.x); }
+static inline V vrotate(V a, V b) { return v2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+static inline double vlengthsq(V a) { return vdot(a, a); }
+static inline double vlength(V a) { return sqrt(vdot(a, a)); }
+static inline V vnormalize(V a) { return vmult(a, 1.0 / (vlength(a) + DBL_MIN)); }
+static inline V vlerp(V a, V b, double t) { return vadd(vmult(a, 1.0 - t), vmult(b, t)); }
+static inline int veql(V a, V b) { return a.x == b.x && a.y == b.y; }
+static inline V vclamp(V v, double len) { return (vdot(v, v) > len * len) ? vmult(vnormalize(v), len) : v; }
+static inline double fmaxc(double a, double b) { return (a > b) ? a : b; }
+static inline double fminc(double a, double b) { return (a < b) ? a : b; }
+static inline double fclampc(double f, double lo, double hi) { return fminc(fmaxc(f, lo), hi); }
+static inline double fclamp01(double f) { return fmaxc(0.0, fminc(f, 1.0)); }
+
+typedef struct { double a, b, c, d, tx, ty; } XF;
+static inline V xpoint(XF t, V p) { return v2(t.a * p.x + t.c * p.y + t.tx, t.b * p.x + t.d * p.y + t.ty); }
+static inline V xvect(XF t, V v) { return v2(t.a * v.x + t.c * v.y, t.b * v.x + t.d * v.y); }
+
+/* CP_HASH_PAIR, include/chipmunk/chipmunk_private.h:28-29 (cpHashValue is 64-bit uintptr_t). */
+#define HASH_COEF 3344921057ull
+static inline uint64_t hash_pair(uint64_t a, uint64_t b) { return a * HASH_COEF ^ b * HASH_COEF; }
+
+/* ------------------------------------------------------------------ */
+/* Deterministic transcendentals shared (as an algorithm) with the device
+ * library.  sin/cos: Cody-Waite reduction by pi/2 in three FMA steps, then
+ * the Taylor series of sin and cos on [-pi/4, pi/4] (coefficients 1/n!
+ * rounded to double), with the 1 - z/2 rounding error of cos carried
+ * separately.  exp: reduction by ln 2 and the Taylor series of exp on
+ * [-ln2/2, ln2/2], scaled by ldexp.  Pure IEEE operations: identical bits
+ * on every IEEE-754 double machine. */
+
+#define PIO2_1 0x1.921fb54442d18p+0
+#define PIO2_2 0x1.1a62633145c07p-54
+#define PIO2_3 -0x1.f1976b7ed8fbcp-110
+#define TWO_OVER_PI 0x1.45f306dc9c883p-1
+
+void cp_sincos_oracle(double x, double *s_out, double *c_out)
+{
+	if (!(fabs(x) < 1e15)) { *s_out = x - x; *c_out = x - x; return; }
+	double k = rint(x * TWO_OVER_PI);
+	double r = fma(-k, PIO2_1, x);
+	r = fma(-k, PIO2_2, r);
+	r = fma(-k, PIO2_3, r);
+	double z = r * r;
+	/* sin r = r + r z (-1/3! + z (1/5! - ...)), through r^19 */
+	double ps = -0x1.2f49b46814157p-57;          /* -1/19! */
+	ps = ps * z + 0x1.952c77030ad4ap-49;         /*  1/17! */
+	ps = ps * z - 0x1.ae7f3e733b81fp-41;         /* -1/15! */
+	ps = ps * z + 0x1.6124613a86d09p-33;         /*  1/13! */
+	ps = ps * z - 0x1.ae64567f544e4p-26;         /* -1/11! */
+	ps = ps * z + 0x1.71de3a556c734p-19;         /*  1/9!  */
+	ps = ps * z - 0x1.a01a01a01a01ap-13;         /* -1/7!  */
+	ps = ps * z + 0x1.1111111111111p-7;          /*  1/5!  */
+	ps = ps * z - 0x1.5555555555555p-3;          /* -1/3!  */
+	double sn = r + r * (z * ps);
+	/* cos r = 1 - z/2 + z^2 (1/4! - z/6! + ...), through r^18 */
+	double qc = -0x1.6827863b97d97p-53;          /* -1/18! */
+	qc = qc * z + 0x1.ae7f3e733b81fp-45;         /*  1/16! */
+	qc = qc * z - 0x1.93974a8c07c9dp-37;         /* -1/14! */
+	qc = qc * z + 0x1.1eed8eff8d898p-29;         /*  1/12! */
+	qc = qc * z - 0x1.27e4fb7789f5cp-22;         /* -1/10! */
+	qc = qc * z + 0x1.a01a01a01a01ap-16;         /*  1/8!  */
+	qc = qc * z - 0x1.6c16c16c16c17p-10;         /* -1/6!  */
+	qc = qc * z + 0x1.5555555555555p-5;          /*  1/4!  */
+	double hz = 0.5 * z;
+	double wc = 1.0 - hz;
+	double cs = wc + (((1.0 - wc) - hz) + (z * z) * qc);
+	long q = (long)k & 3;
+	if (q == 0) { *s_out = sn; *c_out = cs; }
+	else if (q == 1) { *s_out = cs; *c_out = -sn; }
+	else if (q == 2) { *s_out = -sn; *c_out = -cs; }
+	else { *s_out = -cs; *c_out = sn; }
+}
+
+#define LN2_HI 0x1.62e42fefa39efp-1
+#define LN2_LO 0x1.abc9e3b39803fp-56
+#define LOG2E 0x1.71547652b82fep+0
+
+double cp_exp_oracle(double x)
+{
+	if (x != x) return x;
+	if (x > 709.782712893384) return INFINITY;
+	if (x < -745.2) return 0.0;
+	double k = rint(x * LOG2E);
+	double r = fma(-k, LN2_HI, x);
+	r = fma(-k, LN2_LO, r);
+	double p = 0x1.6124613a86d09p-33;            /* 1/13! */
+	p = p * r + 0x1.1eed8eff8d898p-29;           /* 1/12! */
+	p = p * r + 0x1.ae64567f544e4p-26;           /* 1/11! */
+	p = p * r + 0x1.27e4fb7789f5cp-22;           /* 1/10! */
+	p = p * r + 0x1.71de3a556c734p-19;           /* 1/9!  */
+	p = p * r + 0x1.a01a01a01a01ap-16;           /* 1/8!  */
+	p = p * r + 0x1.a01a01a01a01ap-13;           /* 1/7!  */
+	p = p * r + 0x1.6c16c16c16c17p-10;           /* 1/6!  */
+	p = p * r + 0x1.1111111111111p-7;            /* 1/5!  */
+	p = p * r + 0x1.5555555555555p-5;            /* 1/4!  */
+	p = p * r + 0x1.5555555555555p-3;            /* 1/3!  */
+	p = p * r + 0.5;
+	double e = 1.0 + (r + (r * r) * p);
+	return ldexp(e, (int)k);
+}
+
+/* ------------------------------------------------------------------ */
+/* state                                                               */
+
+typedef struct {
+	int type, world;
+	double m, m_inv, i, i_inv;
+	V cog, p, v, f;
+	double a, w, t;
+	XF T;
+	V v_bias;
+	double w_bias;
+} OBody;
+
+typedef struct {
+	int type, body, world;
+	uint64_t hashid;
+	double e, u;
+	V sv;
+	uint64_t group;
+	uint32_t cats, mask;
+	double r;
+	V c, tc;                                 /* circle */
+	V a, b, n, ta, tb, tn, a_tan, b_tan;     /* segment */
+	int count;                               /* poly */
+	const V *lv, *ln;
+	V *wv, *wn;
+	double bb[4];                            /* l, b, r, t */
+	double fat[4];                           /* velocity-fattened leaf box of the dynamic BBTree */
+	uint32_t rebuild_stamp;                  /* step of the last leaf rebuild */
+} OShape;
+
+typedef struct {
+	V r1, r2;
+	double nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
+	uint64_t hash;
+} OContact;
+
+typedef struct {
+	int32_t sa, sb, ba, bb;
+	uint64_t key, prio;
+	double e, u;
+	V svr, n;
+	int count;
+	OContact con[2];
+	uint32_t id, stamp;
+	int state, colour;
+} OArb;
+
+typedef struct {
+	int type, a, b;
+	double maxForce, errorBias, maxBias;
+	int collide;
+	double P[8];
+	/* solver state (chipmunk_structs.h:270-391) */
+	V r1, r2, n, bias2, jAcc2;
+	double k11, k12, k21, k22;
+	double nMass, bias, jAcc, jnAcc, target_vrn, v_coef, iSum;
+	int colour;
+	uint64_t prio;
+} OJoint;
+
+typedef struct {
+	int b0, b1, s0, s1, c0, c1;
+	cpgpuSpaceParams sp;
+	OArb *arbs;
+	int64_t narb, arbcap;
+	int64_t *tab;
+	int64_t tabmask;
+	int64_t *active;       /* indices into arbs, in solve order */
+	int64_t nactive;
+	int32_t *jorder;       /* constraint solve order */
+	int ncol_arb, ncol_cons;
+	uint64_t *nocollide;   /* sorted body-pair keys of collideBodies == false constraints */
+	int64_t n_nocollide;
+	int64_t npairs, ncontacts;
+	/* GJK warm-start ids of collided pairs (the Pair.id cache of cpBBTree.c:69-78) */
+	uint64_t *pid_key;
+	uint32_t *pid_id, *pid_stamp;
+	int64_t pid_n, pid_mask;
+} OWorldPart;
+
+struct oworld {
+	int nw;
+	OWorldPart *W;
+	int nb, ns, nc;
+	OBody *B;
+	OShape *S;
+	OJoint *J;
+	V *polymem;
+	int libm_trig;
+	int64_t next_n;
+	int32_t *next_pairs;
+	int64_t ncoll;
+	int32_t *coll;      /* replayed collisions {a, b, id}, sorted by pair key */
+	uint64_t *coll_key;
+	int joint_list_order;
+	int64_t steps;
+};
+
+/* splitmix64 finalizer: a bijection on 64-bit integers; the colouring priority. */
+static inline uint64_t mix64(uint64_t z)
+{
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+
+static inline uint64_t pair_key(int32_t a, int32_t b)
+{
+	uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+	return ((uint64_t)lo << 32) | hi;
+}
+
+/* ------------------------------------------------------------------ */
+/* construction                                                        */
+
+static XF xf_from(const double *t) { XF x = {t[0], t[1], t[2], t[3], t[4], t[5]}; return x; }
+
+static int cmp_u64(const void *a, const void *b)
+{
+	uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+	return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static void cache_shape(oworld *w, OShape *s);
+static void fat_bb(oworld *w, OShape *s);
+
+oworld *ow_new(const cpgpuWorldDesc *d)
+{
+	oworld *w = (oworld *)calloc(1, sizeof(oworld));
+	w->nw = d->n_worlds;
+	w->nb = d->n_bodies;
+	w->ns = d->n_shapes;
+	w->nc = d->n_constraints;
+	w->B = (OBody *)calloc((size_t)(w->nb ? w->nb : 1), sizeof(OBody));
+	w->S = (OShape *)calloc((size_t)(w->ns ? w->ns : 1), sizeof(OShape));
+	w->J = (OJoint *)calloc((size_t)(w->nc ? w->nc : 1), sizeof(OJoint));
+	w->W = (OWorldPart *)calloc((size_t)w->nw, sizeof(OWorldPart));
+	w->polymem = (V *)calloc((size_t)(4 * (d->n_poly_verts + 1)), sizeof(V));
+
+	for (int wi = 0; wi < w->nw; wi++) {
+		OWorldPart *P = &w->W[wi];
+		P->b0 = d->world_body_start[wi]; P->b1 = d->world_body_start[wi + 1];
+		P->s0 = d->world_shape_start[wi]; P->s1 = d->world_shape_start[wi + 1];
+		P->c0 = d->world_constraint_start ? d->world_constraint_start[wi] : 0;
+		P->c1 = d->world_constraint_start ? d->world_constraint_start[wi + 1] : 0;
+		P->sp = d->space[wi];
+		for (int b = P->b0; b < P->b1; b++) w->B[b].world = wi;
+	}
+	for (int i = 0; i < w->nb; i++) {
+		OBody *b = &w->B[i];
+		b->type = d->body_type[i];
+		b->m = d->body_m[i]; b->m_inv = d->body_m_inv[i];
+		b->i = d->body_i[i]; b->i_inv = d->body_i_inv[i];
+		b->cog = v2(d->body_cog[2 * i], d->body_cog[2 * i + 1]);
+		b->p = v2(d->body_p[2 * i], d->body_p[2 * i + 1]);
+		b->v = v2(d->body_v[2 * i], d->body_v[2 * i + 1]);
+		b->f = v2(d->body_f[2 * i], d->body_f[2 * i + 1]);
+		b->a = d->body_a[i]; b->w = d->body_w[i]; b->t = d->body_t[i];
+		b->v_bias = v2(d->body_v_bias[2 * i], d->body_v_bias[2 * i + 1]);
+		b->w_bias = d->body_w_bias[i];
+		b->T = xf_from(d->body_transform + 6 * i);
+	}
+	V *lvbase = w->polymem, *lnbase = w->polymem + d->n_poly_verts;
+	V *wvbase = w->polymem + 2 * d->n_poly_verts, *wnbase = w->polymem + 3 * d->n_poly_verts;
+	for (int k = 0; k < d->n_poly_verts; k++) {
+		lvbase[k] = v2(d->poly_verts[2 * k], d->poly_verts[2 * k + 1]);
+		lnbase[k] = v2(d->poly_normals[2 * k], d->poly_normals[2 * k + 1]);
+	}
+	for (int i = 0; i < w->ns; i++) {
+		OShape *s = &w->S[i];
+		const double *g = d->shape_geom + 10 * i;
+		s->type = d->shape_type[i];
+		s->body = d->shape_body[i];
+		s->world = w->B[s->body].world;
+		s->hashid = d->shape_hashid[i];
+		s->e = d->shape_e[i]; s->u = d->shape_u[i];
+		s->sv = v2(d->shape_surface_v[2 * i], d->shape_surface_v[2 * i + 1]);
+		s->group = d->shape_group[i]; s->cats = d->shape_categories[i]; s->mask = d->shape_mask[i];
+		s->r = d->shape_radius[i];
+		if (s->type == CPGPU_SHAPE_CIRCLE) {
+			s->c = v2(g[0], g[1]);
+		} else if (s->type == CPGPU_SHAPE_SEGMENT) {
+			s->a = v2(g[0], g[1]); s->b = v2(g[2], g[3]); s->n = v2(g[4], g[5]);
+			s->a_tan = v2(g[6], g[7]); s->b_tan = v2(g[8], g[9]);
+		} else {
+			int st = d->shape_poly_start[i];
+			s->count = d->shape_poly_count[i];
+			s->lv = lvbase + st; s->ln = lnbase + st;
+			s->wv = wvbase + st; s->wn = wnbase + st;
+		}
+		/* cpSpaceAddShape caches every shape once with its body's transform (cpSpace.c:433)
+		 * and inserts it into the dynamic index with a fattened box (cpBBTree.c:605-616). */
+		cache_shape(w, s);
+		if (w->B[s->body].type != CPGPU_BODY_STATIC) fat_bb(w, s);
+	}
+	for (int i = 0; i < w->nc; i++) {
+		OJoint *j = &w->J[i];
+		j->type = d->cons_type[i];
+		j->a = d->cons_a[i]; j->b = d->cons_b[i];
+		j->maxForce = d->cons_max_force[i]; j->errorBias = d->cons_error_bias[i]; j->maxBias = d->cons_max_bias[i];
+		j->collide = d->cons_collide_bodies[i];
+		memcpy(j->P, d->cons_params + 8 * i, 8 * sizeof(double));
+		double j0 = d->cons_jacc ? d->cons_jacc[2 * i] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * i + 1] : 0.0;
+		j->jAcc2 = v2(j0, j1);
+		j->jAcc = j0; j->jnAcc = j0;
+		j->prio = mix64((uint64_t)(uint32_t)i | 0x8000000000000000ull);
+	}
+	for (int wi = 0; wi < w->nw; wi++) {
+		OWorldPart *P = &w->W[wi];
+		int64_t n = 0;
+		for (int c = P->c0; c < P->c1; c++) if (!w->J[c].collide) n++;
+		P->nocollide = (uint64_t *)malloc((size_t)(n ? n : 1) * sizeof(uint64_t));
+		n = 0;
+		for (int c = P->c0; c < P->c1; c++) if (!w->J[c].collide) P->nocollide[n++] = pair_key(w->J[c].a, w->J[c].b);
+		qsort(P->nocollide, (size_t)n, sizeof(uint64_t), cmp_u64);
+		P->n_nocollide = n;
+		P->arbcap = 64;
+		P->arbs = (OArb *)calloc((size_t)P->arbcap, sizeof(OArb));
+		P->active = (int64_t *)calloc((size_t)P->arbcap, sizeof(int64_t));
+		P->tabmask = 127;
+		P->tab = (int64_t *)malloc(128 * sizeof(int64_t));
+		for (int k = 0; k < 128; k++) P->tab[k] = -1;
+		P->jorder = (int32_t *)calloc((size_t)(P->c1 - P->c0 + 1), sizeof(int32_t));
+		for (int c = P->c0; c < P->c1; c++) P->jorder[c - P->c0] = c;
+	}
+	return w;
+}
+
+void ow_free(oworld *w)
+{
+	if (!w) return;
+	for (int wi = 0; wi < w->nw; wi++) {
+		free(w->W[wi].arbs); free(w->W[wi].active); free(w->W[wi].tab);
+		free(w->W[wi].nocollide); free(w->W[wi].jorder);
+		free(w->W[wi].pid_key); free(w->W[wi].pid_id); free(w->W[wi].pid_stamp);
+	}
+	free(w->W); free(w->B); free(w->S); free(w->J); free(w->polymem); free(w->next_pairs);
+	free(w->coll); free(w->coll_key);
+	free(w);
+}
+
+void ow_set_libm_trig(oworld *w, int enable) { w->libm_trig = enable; }
+void ow_set_joint_list_order(oworld *w, int enable) { w->joint_list_order = enable; }
+
+static int64_t *g_sort_coll_idx;
+static const uint64_t *g_coll_keys;
+static int cmp_coll(const void *x, const void *y)
+{
+	uint64_t a = g_coll_keys[*(const int64_t *)x], b = g_coll_keys[*(const int64_t *)y];
+	return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+void ow_set_next_collisions(oworld *w, int64_t n, const int32_t *coll)
+{
+	free(w->coll); free(w->coll_key);
+	uint64_t *keys = (uint64_t *)malloc((size_t)(n + 1) * sizeof(uint64_t));
+	int64_t *idx = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
+	for (int64_t k = 0; k < n; k++) { keys[k] = pair_key(coll[3 * k], coll[3 * k + 1]); idx[k] = k; }
+	g_coll_keys = keys;
+	g_sort_coll_idx = idx;
+	qsort(idx, (size_t)n, sizeof(int64_t), cmp_coll);
+	w->coll = (int32_t *)malloc((size_t)(3 * n + 3) * sizeof(int32_t));
+	w->coll_key = (uint64_t *)malloc((size_t)(n + 1) * sizeof(uint64_t));
+	for (int64_t k = 0; k < n; k++) {
+		memcpy(w->coll + 3 * k, coll + 3 * idx[k], 3 * sizeof(int32_t));
+		w->coll_key[k] = keys[idx[k]];
+	}
+	w->ncoll = n;
+	free(keys); free(idx);
+}
+
+static const int32_t *coll_find(oworld *w, uint64_t key)
+{
+	int64_t lo = 0, hi = w->ncoll;
+	while (lo < hi) {
+		int64_t mid = (lo + hi) / 2;
+		if (w->coll_key[mid] < key) lo = mid + 1; else hi = mid;
+	}
+	return (lo < w->ncoll && w->coll_key[lo] == key) ? w->coll + 3 * lo : NULL;
+}
+
+void ow_set_next_order(oworld *w, int64_t n, const int32_t *pairs)
+{
+	free(w->next_pairs);
+	w->next_pairs = (int32_t *)malloc((size_t)(2 * n + 2) * sizeof(int32_t));
+	memcpy(w->next_pairs, pairs, (size_t)(2 * n) * sizeof(int32_t));
+	w->next_n = n;
+}
+
+/* ------------------------------------------------------------------ */
+/* integration: cpBodyUpdatePosition / SetTransform (cpBody.c:511-522, 347-357)
+ * and cpBodyUpdateVelocity (cpBody.c:493-509)                          */
+
+static void update_position(oworld *w, OBody *b, double dt)
+{
+	b->p = vadd(b->p, vmult(vadd(b->v, b->v_bias), dt));
+	b->a = b->a + (b->w + b->w_bias) * dt;
+	double sn, cs;
+	if (w->libm_trig) { cs = cos(b->a); sn = sin(b->a); }
+	else cp_sincos_oracle(b->a, &sn, &cs);
+	V c = b->cog;
+	/* cpTransformNewTranspose(rot.x, -rot.y, tx, rot.y, rot.x, ty) */
+	b->T.a = cs; b->T.b = sn; b->T.c = -sn; b->T.d = cs;
+	b->T.tx = b->p.x - (c.x * cs - c.y * sn);
+	b->T.ty = b->p.y - (c.x * sn + c.y * cs);
+	b->v_bias = v2(0.0, 0.0);
+	b->w_bias = 0.0;
+}
+
+static void update_velocity(OBody *b, V g, double damping, double dt)
+{
+	if (b->type == CPGPU_BODY_KINEMATIC) return;
+	b->v = vadd(vmult(b->v, damping), vmult(vadd(g, vmult(b->f, b->m_inv)), dt));
+	b->w = b->w * damping + b->t * b->i_inv * dt;
+	b->f = v2(0.0, 0.0);
+	b->t = 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* shape caching: cpCircleShapeCacheData (cpShape.c:291-296),
+ * cpSegmentShapeCacheData (cpShape.c:378-405), cpPolyShapeCacheData
+ * (cpPolyShape.c:39-64)                                                 */
+
+static void cache_shape(oworld *w, OShape *s)
+{
+	XF T = w->B[s->body].T;
+	if (s->type == CPGPU_SHAPE_CIRCLE) {
+		V c = s->tc = xpoint(T, s->c);
+		s->bb[0] = c.x - s->r; s->bb[1] = c.y - s->r; s->bb[2] = c.x + s->r; s->bb[3] = c.y + s->r;
+	} else if (s->type == CPGPU_SHAPE_SEGMENT) {
+		s->ta = xpoint(T, s->a); s->tb = xpoint(T, s->b); s->tn = xvect(T, s->n);
+		double l, r, bt, tp;
+		if (s->ta.x < s->tb.x) { l = s->ta.x; r = s->tb.x; } else { l = s->tb.x; r = s->ta.x; }
+		if (s->ta.y < s->tb.y) { bt = s->ta.y; tp = s->tb.y; } else { bt = s->tb.y; tp = s->ta.y; }
+		double rad = s->r;
+		s->bb[0] = l - rad; s->bb[1] = bt - rad; s->bb[2] = r + rad; s->bb[3] = tp + rad;
+	} else {
+		double l = INFINITY, r = -INFINITY, bt = INFINITY, tp = -INFINITY;
+		for (int i = 0; i < s->count; i++) {
+			V v = xpoint(T, s->lv[i]);
+			V n = xvect(T, s->ln[i]);
+			s->wv[i] = v; s->wn[i] = n;
+			l = fminc(l, v.x); r = fmaxc(r, v.x); bt = fminc(bt, v.y); tp = fmaxc(tp, v.y);
+		}
+		double rad = s->r;
+		s->bb[0] = l - rad; s->bb[1] = bt - rad; s->bb[2] = r + rad; s->bb[3] = tp + rad;
+	}
+}
+
+/* GetBB (cpBBTree.c:82-98) with ShapeVelocityFunc (cpSpace.c:106): the leaf
+ * box of a dynamic shape, and LeafUpdate's containment test (cpBBTree.c:
+ * 498-517).  A rebuilt leaf clears its cached pairs, so the warm-start GJK
+ * id of every pair of that shape restarts at 0 (cpBBTree.c:211,434). */
+static void fat_bb(oworld *w, OShape *s)
+{
+	const double *bb = s->bb;
+	V v = vmult(w->B[s->body].v, 0.1);
+	double x = (bb[2] - bb[0]) * 0.1;
+	double y = (bb[3] - bb[1]) * 0.1;
+	s->fat[0] = bb[0] + fminc(-x, v.x);
+	s->fat[1] = bb[1] + fminc(-y, v.y);
+	s->fat[2] = bb[2] + fmaxc(x, v.x);
+	s->fat[3] = bb[3] + fmaxc(y, v.y);
+}
+
+static void leaf_update(oworld *w, OShape *s)
+{
+	const double *f = s->fat, *b = s->bb;
+	int contains = f[0] <= b[0] && f[2] >= b[2] && f[1] <= b[1] && f[3] >= b[3];
+	if (!contains) {
+		fat_bb(w, s);
+		s->rebuild_stamp = w->W[s->world].sp.stamp;
+	}
+}
+
+void ow_cache_all_shapes(oworld *w) { for (int i = 0; i < w->ns; i++) cache_shape(w, &w->S[i]); }
+
+void ow_read_shape_bbs(oworld *w, double *bb)
+{
+	for (int i = 0; i < w->ns; i++) memcpy(bb + 4 * i, w->S[i].bb, 4 * sizeof(double));
+}
+
+void ow_set_transforms(oworld *w, const double *t)
+{
+	for (int i = 0; i < w->nb; i++) w->B[i].T = xf_from(t + 6 * i);
+}
+
+/* ------------------------------------------------------------------ */
+/* narrowphase: src/cpCollision.c                                      */
+
+typedef struct { V p; uint32_t index; } SupportPt;                  /* :81-91 */
+typedef struct { V a, b, ab; uint32_t id; } MinkPt;                  /* :117-132 */
+typedef struct { const OShape *s1, *s2; } Ctx;                       /* :134-137 */
+typedef struct { V p; uint64_t hash; } EdgePt;                       /* :147-151 */
+typedef struct { EdgePt a, b; double r; V n; } Edge;                 /* :154-158 */
+typedef struct { V a, b, n; double d; uint32_t id; } ClosestPts;     /* :211-221 */
+typedef struct { int count; uint32_t id; V n; V p1[2], p2[2]; uint64_t hash[2]; } CollInfo;
+
+static inline int poly_support_index(const OShape *s, V n)            /* :62-77 */
+{
+	double max = -INFINITY;
+	int index = 0;
+	for (int i = 0; i < s->count; i++) {
+		double d = vdot(s->wv[i], n);
+		if (d > max) { max = d; index = i; }
+	}
+	return index;
+}
+
+static inline SupportPt shape_support(const OShape *s, V n)          /* :95-114 */
+{
+	SupportPt r;
+	if (s->type == CPGPU_SHAPE_CIRCLE) { r.p = s->tc; r.index = 0; }
+	else if (s->type == CPGPU_SHAPE_SEGMENT) {
+		if (vdot(s->ta, n) > vdot(s->tb, n)) { r.p = s->ta; r.index = 0; }
+		else { r.p = s->tb; r.index = 1; }
+	} else {
+		int i = poly_support_index(s, n);
+		r.p = s->wv[i]; r.index = (uint32_t)i;
+	}
+	return r;
+}
+
+static inline MinkPt mink_new(SupportPt a, SupportPt b)               /* :127-132 */
+{
+	MinkPt m;
+	m.a = a.p; m.b = b.p; m.ab = vsub(b.p, a.p);
+	m.id = ((a.index & 0xFF) << 8) | (b.index & 0xFF);
+	return m;
+}
+
+static inline MinkPt support(const Ctx *ctx, V n)                     /* :140-145 */
+{
+	SupportPt a = shape_support(ctx->s1, vneg(n));
+	SupportPt b = shape_support(ctx->s2, n);
+	return mink_new(a, b);
+}
+
+static inline int check_point_greater(V a, V b, V c)                 /* cpRobust.c:4-8 */
+{
+	return (b.y - a.y) * (a.x + b.x - 2 * c.x) > (b.x - a.x) * (a.y + b.y - 2 * c.y);
+}
+
+static inline int check_axis(V v0, V v1, V p, V n)                    /* cpRobust.c:10-13 */
+{
+	return vdot(p, n) <= fmaxc(vdot(v0, n), vdot(v1, n));
+}
+
+static inline double closest_t(V a, V b)                              /* :193-199 */
+{
+	V delta = vsub(b, a);
+	return -fclampc(vdot(delta, vadd(a, b)) / (vlengthsq(delta) + DBL_MIN), -1.0, 1.0);
+}
+
+static inline V lerp_t(V a, V b, double t)                            /* :201-207 */
+{
+	double ht = 0.5 * t;
+	return vadd(vmult(a, 0.5 - ht), vmult(b, 0.5 + ht));
+}
+
+static ClosestPts closest_points_new(MinkPt v0, MinkPt v1)            /* :223-258 */
+{
+	double t = closest_t(v0.ab, v1.ab);
+	V p = lerp_t(v0.ab, v1.ab, t);
+	V pa = lerp_t(v0.a, v1.a, t);
+	V pb = lerp_t(v0.b, v1.b, t);
+	uint32_t id = ((v0.id & 0xFFFF) << 16) | (v1.id & 0xFFFF);
+	V delta = vsub(v1.ab, v0.ab);
+	V n = vnormalize(vrperp(delta));
+	double d = vdot(n, p);
+	ClosestPts r;
+	r.a = pa; r.b = pb; r.id = id;
+	if (d <= 0.0 || (-1.0 < t && t < 1.0)) { r.n = n; r.d = d; }
+	else {
+		double d2 = vlength(p);
+		r.n = vmult(p, 1.0 / (d2 + DBL_MIN));
+		r.d = d2;
+	}
+	return r;
+}
+
+static inline double closest_dist(V v0, V v1)                          /* :262-266 */
+{
+	return vlengthsq(lerp_t(v0, v1, closest_t(v0, v1)));
+}
+
+#define MAX_GJK_ITERATIONS 30
+#define MAX_EPA_ITERATIONS 30
+#define EPA_HULL_MAX 40
+
+static ClosestPts epa(const Ctx *ctx, MinkPt v0, MinkPt v1, MinkPt v2)  /* :270-343 (loop form) */
+{
+	MinkPt hullA[EPA_HULL_MAX], hullB[EPA_HULL_MAX];
+	MinkPt *hull = hullA, *hull2 = hullB;
+	int count = 3;
+	hull[0] = v0; hull[1] = v1; hull[2] = v2;
+	for (int iteration = 1;; iteration++) {
+		int mini = 0;
+		double minDist = INFINITY;
+		for (int j = 0, i = count - 1; j < count; i = j, j++) {
+			double d = closest_dist(hull[i].ab, hull[j].ab);
+			if (d < minDist) { minDist = d; mini = i; }
+		}
+		MinkPt e0 = hull[mini];
+		MinkPt e1 = hull[(mini + 1) % count];
+		MinkPt p = support(ctx, vperp(vsub(e1.ab, e0.ab)));
+		int duplicate = (p.id == e0.id || p.id == e1.id);
+		if (!duplicate && check_point_greater(e0.ab, e1.ab, p.ab) && iteration < MAX_EPA_ITERATIONS) {
+			int count2 = 1;
+			hull2[0] = p;
+			for (int i = 0; i < count; i++) {
+				int index = (mini + 1 + i) % count;
+				V h0 = hull2[count2 - 1].ab;
+				V h1 = hull[index].ab;
+				V h2 = (i + 1 < count ? hull[(index + 1) % count] : p).ab;
+				if (check_point_greater(h0, h2, h1)) hull2[count2++] = hull[index];
+			}
+			MinkPt *tmp = hull; hull = hull2; hull2 = tmp;
+			count = count2;
+		} else {
+			return closest_points_new(e0, e1);
+		}
+	}
+}
+
+static ClosestPts gjk_recurse(const Ctx *ctx, MinkPt v0, MinkPt v1)    /* :348-392 (loop form) */
+{
+	int iteration = 1;
+	for (;;) {
+		if (iteration > MAX_GJK_ITERATIONS) return closest_points_new(v0, v1);
+		if (check_point_greater(v1.ab, v0.ab, v2(0.0, 0.0))) {
+			MinkPt tmp = v0; v0 = v1; v1 = tmp;
+			continue;
+		}
+		double t = closest_t(v0.ab, v1.ab);
+		V n = (-1.0 < t && t < 1.0) ? vperp(vsub(v1.ab, v0.ab)) : vneg(lerp_t(v0.ab, v1.ab, t));
+		MinkPt p = support(ctx, n);
+		if (check_point_greater(p.ab, v0.ab, v2(0.0, 0.0)) && check_point_greater(v1.ab, p.ab, v2(0.0, 0.0))) {
+			return epa(ctx, v0, p, v1);
+		}
+		if (check_axis(v0.ab, v1.ab, p.ab, n)) return closest_points_new(v0, v1);
+		if (closest_dist(v0.ab, p.ab) < closest_dist(p.ab, v1.ab)) v1 = p;
+		else v0 = p;
+		iteration++;
+	}
+}
+
+static SupportPt shape_point(const OShape *s, int i)                   /* :395-413 */
+{
+	SupportPt r;
+	if (s->type == CPGPU_SHAPE_CIRCLE) { r.p = s->tc; r.index = 0; }
+	else if (s->type == CPGPU_SHAPE_SEGMENT) { r.p = (i == 0 ? s->ta : s->tb); r.index = (uint32_t)i; }
+	else {
+		int index = (i < s->count ? i : 0);
+		r.p = s->wv[index]; r.index = (uint32_t)index;
+	}
+	return r;
+}
+
+static inline V bb_center(const double *bb) { return vlerp(v2(bb[0], bb[1]), v2(bb[2], bb[3]), 0.5); }
+
+static ClosestPts gjk(const Ctx *ctx, uint32_t *id)                    /* :416-472 */
+{
+	MinkPt v0, v1;
+	if (*id) {
+		v0 = mink_new(shape_point(ctx->s1, (*id >> 24) & 0xFF), shape_point(ctx->s2, (*id >> 16) & 0xFF));
+		v1 = mink_new(shape_point(ctx->s1, (*id >> 8) & 0xFF), shape_point(ctx->s2, (*id) & 0xFF));
+	} else {
+		V axis = vperp(vsub(bb_center(ctx->s1->bb), bb_center(ctx->s2->bb)));
+		v0 = support(ctx, axis);
+		v1 = support(ctx, vneg(axis));
+	}
+	ClosestPts points = gjk_recurse(ctx, v0, v1);
+	*id = points.id;
+	return points;
+}
+
+static inline void push_contact(CollInfo *info, V p1, V p2, uint64_t hash)  /* :44-55 */
+{
+	info->p1[info->count] = p1;
+	info->p2[info->count] = p2;
+	info->hash[info->count] = hash;
+	info->count++;
+}
+
+static void contact_points(Edge e1, Edge e2, ClosestPts points, CollInfo *info)  /* :477-518 */
+{
+	double mindist = e1.r + e2.r;
+	if (points.d <= mindist) {
+		V n = info->n = points.n;
+		double d_e1_a = vcross(e1.a.p, n);
+		double d_e1_b = vcross(e1.b.p, n);
+		double d_e2_a = vcross(e2.a.p, n);
+		double d_e2_b = vcross(e2.b.p, n);
+		double e1_denom = 1.0 / (d_e1_b - d_e1_a + DBL_MIN);
+		double e2_denom = 1.0 / (d_e2_b - d_e2_a + DBL_MIN);
+		{
+			V p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_b - d_e1_a) * e1_denom)));
+			V p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_a - d_e2_a) * e2_denom)));
+			double dist = vdot(vsub(p2, p1), n);
+			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.a.hash, e2.b.hash));
+		}
+		{
+			V p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_a - d_e1_a) * e1_denom)));
+			V p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_b - d_e2_a) * e2_denom)));
+			double dist = vdot(vsub(p2, p1), n);
+			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.b.hash, e2.a.hash));
+		}
+	}
+}
+
+static Edge support_edge_poly(const OShape *s, V n)                    /* :160-178 */
+{
+	int count = s->count;
+	int i1 = poly_support_index(s, n);
+	int i0 = (i1 - 1 + count) % count;
+	int i2 = (i1 + 1) % count;
+	uint64_t h = s->hashid;
+	Edge e;
+	e.r = s->r;
+	if (vdot(n, s->wn[i1]) > vdot(n, s->wn[i2])) {
+		e.a.p = s->wv[i0]; e.a.hash = hash_pair(h, (uint64_t)i0);
+		e.b.p = s->wv[i1]; e.b.hash = hash_pair(h, (uint64_t)i1);
+		e.n = s->wn[i1];
+	} else {
+		e.a.p = s->wv[i1]; e.a.hash = hash_pair(h, (uint64_t)i1);
+		e.b.p = s->wv[i2]; e.b.hash = hash_pair(h, (uint64_t)i2);
+		e.n = s->wn[i2];
+	}
+	return e;
+}
+
+static Edge support_edge_segment(const OShape *s, V n)                 /* :180-191 */
+{
+	uint64_t h = s->hashid;
+	Edge e;
+	e.r = s->r;
+	if (vdot(s->tn, n) > 0.0) {
+		e.a.p = s->ta; e.a.hash = hash_pair(h, 0);
+		e.b.p = s->tb; e.b.hash = hash_pair(h, 1);
+		e.n = s->tn;
+	} else {
+		e.a.p = s->tb; e.a.hash = hash_pair(h, 1);
+		e.b.p = s->ta; e.b.hash = hash_pair(h, 0);
+		e.n = vneg(s->tn);
+	}
+	return e;
+}
+
+static inline V body_rot(oworld *w, const OShape *s) { XF T = w->B[s->body].T; return v2(T.a, T.b); }
+
+static void circle_circle(const OShape *c1, const OShape *c2, CollInfo *info)  /* :525-537 */
+{
+	double mindist = c1->r + c2->r;
+	V delta = vsub(c2->tc, c1->tc);
+	double distsq = vlengthsq(delta);
+	if (distsq < mindist * mindist) {
+		double dist = sqrt(distsq);
+		V n = info->n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : v2(1.0, 0.0));
+		push_contact(info, vadd(c1->tc, vmult(n, c1->r)), vadd(c2->tc, vmult(n, -c2->r)), 0);
+	}
+}
+
+static void circle_segment(oworld *w, const OShape *circle, const OShape *seg, CollInfo *info)  /* :539-569 */
+{
+	V seg_a = seg->ta, seg_b = seg->tb, center = circle->tc;
+	V seg_delta = vsub(seg_b, seg_a);
+	double closest_tt = fclamp01(vdot(seg_delta, vsub(center, seg_a)) / vlengthsq(seg_delta));
+	V closest = vadd(seg_a, vmult(seg_delta, closest_tt));
+	double mindist = circle->r + seg->r;
+	V delta = vsub(closest, center);
+	double distsq = vlengthsq(delta);
+	if (distsq < mindist * mindist) {
+		double dist = sqrt(distsq);
+		V n = info->n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : seg->tn);
+		V rot = body_rot(w, seg);
+		if ((closest_tt != 0.0 || vdot(n, vrotate(seg->a_tan, rot)) >= 0.0) &&
+		    (closest_tt != 1.0 || vdot(n, vrotate(seg->b_tan, rot)) >= 0.0)) {
+			push_contact(info, vadd(center, vmult(n, circle->r)), vadd(closest, vmult(n, -seg->r)), 0);
+		}
+	}
+}
+
+static void segment_segment(oworld *w, const OShape *s1, const OShape *s2, CollInfo *info)  /* :571-604 */
+{
+	Ctx ctx = {s1, s2};
+	ClosestPts points = gjk(&ctx, &info->id);
+	V n = points.n;
+	V rot1 = body_rot(w, s1), rot2 = body_rot(w, s2);
+	if (points.d <= (s1->r + s2->r) &&
+	    (!veql(points.a, s1->ta) || vdot(n, vrotate(s1->a_tan, rot1)) <= 0.0) &&
+	    (!veql(points.a, s1->tb) || vdot(n, vrotate(s1->b_tan, rot1)) <= 0.0) &&
+	    (!veql(points.b, s2->ta) || vdot(n, vrotate(s2->a_tan, rot2)) >= 0.0) &&
+	    (!veql(points.b, s2->tb) || vdot(n, vrotate(s2->b_tan, rot2)) >= 0.0)) {
+		contact_points(support_edge_segment(s1, n), support_edge_segment(s2, vneg(n)), points, info);
+	}
+}
+
+static void poly_poly(const OShape *p1, const OShape *p2, CollInfo *info)  /* :606-627 */
+{
+	Ctx ctx = {p1, p2};
+	ClosestPts points = gjk(&ctx, &info->id);
+	if (points.d - p1->r - p2->r <= 0.0) {
+		contact_points(support_edge_poly(p1, points.n), support_edge_poly(p2, vneg(points.n)), points, info);
+	}
+}
+
+static void segment_poly(oworld *w, const OShape *seg, const OShape *poly, CollInfo *info)  /* :629-659 */
+{
+	Ctx ctx = {seg, poly};
+	ClosestPts points = gjk(&ctx, &info->id);
+	V n = points.n;
+	V rot = body_rot(w, seg);
+	if (points.d - seg->r - poly->r <= 0.0 &&
+	    (!veql(points.a, seg->ta) || vdot(n, vrotate(seg->a_tan, rot)) <= 0.0) &&
+	    (!veql(points.a, seg->tb) || vdot(n, vrotate(seg->b_tan, rot)) <= 0.0)) {
+		contact_points(support_edge_segment(seg, n), support_edge_poly(poly, vneg(n)), points, info);
+	}
+}
+
+static void circle_poly(const OShape *circle, const OShape *poly, CollInfo *info)  /* :661-679 */
+{
+	Ctx ctx = {circle, poly};
+	ClosestPts points = gjk(&ctx, &info->id);
+	if (points.d <= circle->r + poly->r) {
+		V n = info->n = points.n;
+		push_contact(info, vadd(points.a, vmult(n, circle->r)), vadd(points.b, vmult(n, -poly->r)), 0);
+	}
+}
+
+/* cpCollide (cpCollision.c:701-726): a and b already type-ordered by the caller. */
+static void collide(oworld *w, int32_t ia, int32_t ib, uint32_t id, CollInfo *info)
+{
+	const OShape *a = &w->S[ia], *b = &w->S[ib];
+	info->count = 0;
+	info->id = id;
+	info->n = v2(0.0, 0.0);
+	int code = a->type + b->type * 3;
+	switch (code) {
+	case 0: circle_circle(a, b, info); break;
+	case 3: circle_segment(w, a, b, info); break;
+	case 4: segment_segment(w, a, b, info); break;
+	case 6: circle_poly(a, b, info); break;
+	case 7: segment_poly(w, a, b, info); break;
+	case 8: poly_poly(a, b, info); break;
+	default: fprintf(stderr, "oracle: shape types not sorted\n"); abort();
+	}
+}
+
+int ow_collide(oworld *w, int32_t a, int32_t b, uint32_t id, int32_t *ab_out, uint32_t *id_out,
+               double *n_out, double *points_out, uint64_t *hash_out)
+{
+	if (w->S[a].type > w->S[b].type) { int32_t t = a; a = b; b = t; }
+	CollInfo info;
+	collide(w, a, b, id, &info);
+	ab_out[0] = a; ab_out[1] = b;
+	*id_out = info.id;
+	n_out[0] = info.n.x; n_out[1] = info.n.y;
+	for (int i = 0; i < info.count; i++) {
+		points_out[4 * i + 0] = info.p1[i].x; points_out[4 * i + 1] = info.p1[i].y;
+		points_out[4 * i + 2] = info.p2[i].x; points_out[4 * i + 3] = info.p2[i].y;
+		hash_out[i] = info.hash[i];
+	}
+	return info.count;
+}
+
+/* ------------------------------------------------------------------ */
+/* broadphase semantics: the exact QueryReject predicate
+ * (cpSpaceStep.c:204-232, cpShapeFilterReject chipmunk_private.h:144-155)
+ * over dynamic x dynamic and dynamic x static shapes (cpBBTree.c:638-654,
+ * cpSpatialIndex.c:61-68), by sort-and-sweep on the left edge.          */
+
+static inline int bb_intersects(const double *a, const double *b)   /* cpBB.h:58-61 */
+{
+	return a[0] <= b[2] && b[0] <= a[2] && a[1] <= b[3] && b[1] <= a[3];
+}
+
+static inline int filter_reject(const OShape *a, const OShape *b)
+{
+	return (a->group != 0 && a->group == b->group) || (a->cats & b->mask) == 0 || (b->cats & a->mask) == 0;
+}
+
+static int nocollide_has(const OWorldPart *P, int ba, int bb)
+{
+	uint64_t k = pair_key(ba, bb);
+	int64_t lo = 0, hi = P->n_nocollide;
+	while (lo < hi) {
+		int64_t mid = (lo + hi) / 2;
+		if (P->nocollide[mid] < k) lo = mid + 1; else hi = mid;
+	}
+	return lo < P->n_nocollide && P->nocollide[lo] == k;
+}
+
+/* Pair admitted by the device library: QueryReject passes and not both
+ * bodies of infinite mass (cpSpaceStep.c:281; such pairs never reach the
+ * solver, so they are dropped before the narrowphase). */
+static int pair_ok(oworld *w, const OWorldPart *P, int i, int j)
+{
+	const OShape *a = &w->S[i], *b = &w->S[j];
+	const OBody *ba = &w->B[a->body], *bb = &w->B[b->body];
+	if (ba->type == CPGPU_BODY_STATIC && bb->type == CPGPU_BODY_STATIC) return 0;
+	if (!bb_intersects(a->bb, b->bb)) return 0;
+	if (a->body == b->body) return 0;
+	if (filter_reject(a, b)) return 0;
+	if (nocollide_has(P, a->body, b->body)) return 0;
+	if (ba->m == INFINITY && bb->m == INFINITY) return 0;
+	return 1;
+}
+
+static const double *g_sort_bbs;
+static int cmp_by_left(const void *x, const void *y)
+{
+	int i = *(const int *)x, j = *(const int *)y;
+	double a = g_sort_bbs[4 * i], b = g_sort_bbs[4 * j];
+	if (a < b) return -1;
+	if (a > b) return 1;
+	return i < j ? -1 : (i > j ? 1 : 0);
+}
+
+typedef struct { int32_t a, b; } IPair;
+static int cmp_pair(const void *x, const void *y)
+{
+	const IPair *p = (const IPair *)x, *q = (const IPair *)y;
+	if (p->a != q->a) return p->a < q->a ? -1 : 1;
+	if (p->b != q->b) return p->b < q->b ? -1 : 1;
+	return 0;
+}
+
+/* Candidate pairs of one world, (a < b), sorted. */
+static IPair *world_pairs(oworld *w, OWorldPart *P, int64_t *n_out)
+{
+	int n = P->s1 - P->s0;
+	double *bbs = (double *)malloc((size_t)(4 * (n ? n : 1)) * sizeof(double));
+	int *idx = (int *)malloc((size_t)(n ? n : 1) * sizeof(int));
+	for (int k = 0; k < n; k++) { memcpy(bbs + 4 * k, w->S[P->s0 + k].bb, 4 * sizeof(double)); idx[k] = k; }
+	g_sort_bbs = bbs;
+	qsort(idx, (size_t)n, sizeof(int), cmp_by_left);
+	int64_t cap = 1024, np = 0;
+	IPair *pairs = (IPair *)malloc((size_t)cap * sizeof(IPair));
+	for (int x = 0; x < n; x++) {
+		int i = idx[x];
+		double right = bbs[4 * i + 2];
+		for (int y = x + 1; y < n; y++) {
+			int j = idx[y];
+			if (!(bbs[4 * j] <= right)) break;
+			int gi = P->s0 + i, gj = P->s0 + j;
+			if (!pair_ok(w, P, gi, gj)) continue;
+			if (np == cap) { cap *= 2; pairs = (IPair *)realloc(pairs, (size_t)cap * sizeof(IPair)); }
+			pairs[np].a = gi < gj ? gi : gj;
+			pairs[np].b = gi < gj ? gj : gi;
+			np++;
+		}
+	}
+	qsort(pairs, (size_t)np, sizeof(IPair), cmp_pair);
+	free(bbs); free(idx);
+	*n_out = np;
+	return pairs;
+}
+
+int64_t ow_candidate_pairs(oworld *w, int64_t cap, int32_t *out)
+{
+	int64_t total = 0;
+	for (int wi = 0; wi < w->nw; wi++) {
+		int64_t np;
+		IPair *p = world_pairs(w, &w->W[wi], &np);
+		for (int64_t k = 0; k < np; k++, total++) {
+			if (total < cap) { out[2 * total] = p[k].a; out[2 * total + 1] = p[k].b; }
+		}
+		free(p);
+	}
+	return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* arbiter persistence: the cachedArbiters hash set of cpSpace
+ * (cpSpaceStep.c:248-251, cpHashSet.c:152-176) keyed by the unordered
+ * shape pair, eviction by cpSpaceArbiterSetFilter (cpSpaceStep.c:292-325). */
+
+static void tab_rebuild(OWorldPart *P)
+{
+	int64_t need = 2 * P->narb + 16, cap = 128;
+	while (cap < need) cap *= 2;
+	if (cap - 1 != P->tabmask) { free(P->tab); P->tab = (int64_t *)malloc((size_t)cap * sizeof(int64_t)); P->tabmask = cap - 1; }
+	for (int64_t k = 0; k < cap; k++) P->tab[k] = -1;
+	for (int64_t i = 0; i < P->narb; i++) {
+		uint64_t h = mix64(P->arbs[i].key) & (uint64_t)P->tabmask;
+		while (P->tab[h] >= 0) h = (h + 1) & (uint64_t)P->tabmask;
+		P->tab[h] = i;
+	}
+}
+
+static int64_t tab_find(OWorldPart *P, uint64_t key)
+{
+	uint64_t h = mix64(key) & (uint64_t)P->tabmask;
+	while (P->tab[h] >= 0) {
+		if (P->arbs[P->tab[h]].key == key) return P->tab[h];
+		h = (h + 1) & (uint64_t)P->tabmask;
+	}
+	return -1;
+}
+
+static int64_t arb_append(OWorldPart *P)
+{
+	if (P->narb == P->arbcap) {
+		P->arbcap *= 2;
+		P->arbs = (OArb *)realloc(P->arbs, (size_t)P->arbcap * sizeof(OArb));
+		P->active = (int64_t *)realloc(P->active, (size_t)P->arbcap * sizeof(int64_t));
+	}
+	memset(&P->arbs[P->narb], 0, sizeof(OArb));
+	return P->narb++;
+}
+
+/* Warm-start ids.  The reference caches the id returned by cpCollide in the
+ * BBTree Pair of the two leaves; a leaf rebuilt by LeafUpdate drops its
+ * pairs, so the pair restarts from id 0 (cpBBTree.c:206-223,434,504-511).
+ * Here an id is kept per collided pair with the step it was written, and is
+ * valid while neither shape's leaf has been rebuilt since. */
+static void pid_rebuild(oworld *w, OWorldPart *P, int keep_all)
+{
+	int64_t n = 0;
+	for (int64_t k = 0; k <= P->pid_mask && P->pid_key; k++) if (P->pid_key[k] != UINT64_MAX) n++;
+	int64_t cap = 256;
+	while (cap < 2 * n + 64) cap *= 2;
+	uint64_t *ok = P->pid_key; uint32_t *oi = P->pid_id, *os = P->pid_stamp;
+	int64_t omask = P->pid_mask;
+	P->pid_key = (uint64_t *)malloc((size_t)cap * sizeof(uint64_t));
+	P->pid_id = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
+	P->pid_stamp = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
+	for (int64_t k = 0; k < cap; k++) P->pid_key[k] = UINT64_MAX;
+	P->pid_mask = cap - 1;
+	P->pid_n = 0;
+	for (int64_t k = 0; ok && k <= omask; k++) {
+		if (ok[k] == UINT64_MAX) continue;
+		int32_t a = (int32_t)(ok[k] >> 32), b = (int32_t)(ok[k] & 0xffffffffu);
+		if (!keep_all && (os[k] < w->S[a].rebuild_stamp || os[k] < w->S[b].rebuild_stamp)) continue;
+		uint64_t h = mix64(ok[k]) & (uint64_t)P->pid_mask;
+		while (P->pid_key[h] != UINT64_MAX) h = (h + 1) & (uint64_t)P->pid_mask;
+		P->pid_key[h] = ok[k]; P->pid_id[h] = oi[k]; P->pid_stamp[h] = os[k];
+		P->pid_n++;
+	}
+	free(ok); free(oi); free(os);
+}
+
+static uint32_t pid_get(oworld *w, OWorldPart *P, uint64_t key, int32_t a, int32_t b)
+{
+	if (!P->pid_key) return 0;
+	uint64_t h = mix64(key) & (uint64_t)P->pid_mask;
+	while (P->pid_key[h] != UINT64_MAX) {
+		if (P->pid_key[h] == key) {
+			uint32_t st = P->pid_stamp[h];
+			if (st < w->S[a].rebuild_stamp || st < w->S[b].rebuild_stamp) return 0;
+			return P->pid_id[h];
+		}
+		h = (h + 1) & (uint64_t)P->pid_mask;
+	}
+	return 0;
+}
+
+static void pid_put(OWorldPart *P, uint64_t key, uint32_t id)
+{
+	uint64_t h = mix64(key) & (uint64_t)P->pid_mask;
+	while (P->pid_key[h] != UINT64_MAX && P->pid_key[h] != key) h = (h + 1) & (uint64_t)P->pid_mask;
+	if (P->pid_key[h] == UINT64_MAX) P->pid_n++;
+	P->pid_key[h] = key; P->pid_id[h] = id; P->pid_stamp[h] = P->sp.stamp;
+}
+
+/* ------------------------------------------------------------------ */
+/* deterministic graph colouring (the device solve order).
+ * Conflict nodes are dynamic bodies only (static and kinematic bodies have
+ * zero inverse mass and are never written).  Colour c is built in
+ * COLOUR_SUBROUNDS claim rounds: every uncoloured element none of whose
+ * dynamic bodies is taken by colour c claims those bodies with an atomic
+ * min of its priority; elements holding all their claims take colour c and
+ * mark their bodies taken.  The result is independent of processing order. */
+
+#define COLOUR_SUBROUNDS 3
+
+static int colour_elements(oworld *w, int64_t n, const int32_t *ba, const int32_t *bb, const uint64_t *prio,
+                           int32_t *colour)
+{
+	if (n == 0) return 0;
+	uint64_t *claim = (uint64_t *)malloc((size_t)w->nb * sizeof(uint64_t));
+	int32_t *taken = (int32_t *)malloc((size_t)w->nb * sizeof(int32_t));
+	for (int i = 0; i < w->nb; i++) taken[i] = -1;
+	for (int64_t k = 0; k < n; k++) colour[k] = -1;
+	int64_t left = n;
+	int c = 0;
+	while (left > 0) {
+		for (int r = 0; r < COLOUR_SUBROUNDS; r++) {
+			for (int64_t k = 0; k < n; k++) {
+				if (ba[k] >= 0) claim[ba[k]] = UINT64_MAX;
+				if (bb[k] >= 0) claim[bb[k]] = UINT64_MAX;
+			}
+			for (int64_t k = 0; k < n; k++) {
+				if (colour[k] >= 0) continue;
+				if ((ba[k] >= 0 && taken[ba[k]] == c) || (bb[k] >= 0 && taken[bb[k]] == c)) continue;
+				if (ba[k] >= 0 && prio[k] < claim[ba[k]]) claim[ba[k]] = prio[k];
+				if (bb[k] >= 0 && prio[k] < claim[bb[k]]) claim[bb[k]] = prio[k];
+			}
+			for (int64_t k = 0; k < n; k++) {
+				if (colour[k] >= 0) continue;
+				if ((ba[k] >= 0 && taken[ba[k]] == c) || (bb[k] >= 0 && taken[bb[k]] == c)) continue;
+				if ((ba[k] < 0 || claim[ba[k]] == prio[k]) && (bb[k] < 0 || claim[bb[k]] == prio[k])) {
+					colour[k] = c;
+					left--;
+				}
+			}
+			for (int64_t k = 0; k < n; k++) {
+				if (colour[k] != c) continue;
+				if (ba[k] >= 0) taken[ba[k]] = c;
+				if (bb[k] >= 0) taken[bb[k]] = c;
+			}
+		}
+		c++;
+	}
+	free(claim); free(taken);
+	return c;
+}
+
+static OArb *g_sort_arbs;
+static int cmp_solve(const void *x, const void *y)
+{
+	const OArb *a = &g_sort_arbs[*(const int64_t *)x], *b = &g_sort_arbs[*(const int64_t *)y];
+	if (a->colour != b->colour) return a->colour < b->colour ? -1 : 1;
+	return a->key < b->key ? -1 : (a->key > b->key ? 1 : 0);
+}
+static OJoint *g_sort_joints;
+static int cmp_jsolve(const void *x, const void *y)
+{
+	int32_t i = *(const int32_t *)x, j = *(const int32_t *)y;
+	const OJoint *a = &g_sort_joints[i], *b = &g_sort_joints[j];
+	if (a->colour != b->colour) return a->colour < b->colour ? -1 : 1;
+	return i < j ? -1 : (i > j ? 1 : 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* contact solver: cpArbiterPreStep / ApplyCachedImpulse / ApplyImpulse
+ * (cpArbiter.c:416-498) with the helpers of chipmunk_private.h:172-226   */
+
+static inline V rel_vel(const OBody *a, const OBody *b, V r1, V r2)   /* :172-178 */
+{
+	V v1 = vadd(a->v, vmult(vperp(r1), a->w));
+	V v2s = vadd(b->v, vmult(vperp(r2), b->w));
+	return vsub(v2s, v1);
+}
+
+static inline void apply_impulse(OBody *body, V j, V r)                /* :186-190 */
+{
+	body->v = vadd(body->v, vmult(j, body->m_inv));
+	body->w += body->i_inv * vcross(r, j);
+}
+
+static inline void apply_impulses(OBody *a, OBody *b, V r1, V r2, V j)  /* :192-197 */
+{
+	apply_impulse(a, vneg(j), r1);
+	apply_impulse(b, j, r2);
+}
+
+static inline void apply_bias_impulse(OBody *body, V j, V r)           /* :199-203 */
+{
+	body->v_bias = vadd(body->v_bias, vmult(j, body->m_inv));
+	body->w_bias += body->i_inv * vcross(r, j);
+}
+
+static inline double k_scalar_body(const OBody *b, V r, V n)           /* :212-217 */
+{
+	double rcn = vcross(r, n);
+	return b->m_inv + b->i_inv * rcn * rcn;
+}
+
+static inline double k_scalar(const OBody *a, const OBody *b, V r1, V r2, V n)  /* :219-226 */
+{
+	return k_scalar_body(a, r1, n) + k_scalar_body(b, r2, n);
+}
+
+static void arb_prestep(oworld *w, OArb *arb, double dt, double slop, double bias)
+{
+	const OBody *a = &w->B[arb->ba], *b = &w->B[arb->bb];
+	V n = arb->n;
+	V body_delta = vsub(b->p, a->p);
+	for (int i = 0; i < arb->count; i++) {
+		OContact *con = &arb->con[i];
+		con->nMass = 1.0 / k_scalar(a, b, con->r1, con->r2, n);
+		con->tMass = 1.0 / k_scalar(a, b, con->r1, con->r2, vperp(n));
+		double dist = vdot(vadd(vsub(con->r2, con->r1), body_delta), n);
+		con->bias = -bias * fminc(0.0, dist + slop) / dt;
+		con->jBias = 0.0;
+		con->bounce = vdot(rel_vel(a, b, con->r1, con->r2), n) * arb->e;
+	}
+}
+
+static void arb_apply_cached(oworld *w, OArb *arb, double dt_coef)
+{
+	if (arb->state == CPGPU_ARB_FIRST_COLLISION) return;
+	OBody *a = &w->B[arb->ba], *b = &w->B[arb->bb];
+	V n = arb->n;
+	for (int i = 0; i < arb->count; i++) {
+		OContact *con = &arb->con[i];
+		V j = vrotate(n, v2(con->jnAcc, con->jtAcc));
+		apply_impulses(a, b, con->r1, con->r2, vmult(j, dt_coef));
+	}
+}
+
+static void arb_apply_impulse(oworld *w, OArb *arb)
+{
+	OBody *a = &w->B[arb->ba], *b = &w->B[arb->bb];
+	V n = arb->n;
+	V surface_vr = arb->svr;
+	double friction = arb->u;
+	for (int i = 0; i < arb->count; i++) {
+		OContact *con = &arb->con[i];
+		double nMass = con->nMass;
+		V r1 = con->r1, r2 = con->r2;
+		V vb1 = vadd(a->v_bias, vmult(vperp(r1), a->w_bias));
+		V vb2 = vadd(b->v_bias, vmult(vperp(r2), b->w_bias));
+		V vr = vadd(rel_vel(a, b, r1, r2), surface_vr);
+		double vbn = vdot(vsub(vb2, vb1), n);
+		double vrn = vdot(vr, n);
+		double vrt = vdot(vr, vperp(n));
+		double jbn = (con->bias - vbn) * nMass;
+		double jbnOld = con->jBias;
+		con->jBias = fmaxc(jbnOld + jbn, 0.0);
+		double jn = -(con->bounce + vrn) * nMass;
+		double jnOld = con->jnAcc;
+		con->jnAcc = fmaxc(jnOld + jn, 0.0);
+		double jtMax = friction * con->jnAcc;
+		double jt = -vrt * con->tMass;
+		double jtOld = con->jtAcc;
+		con->jtAcc = fclampc(jtOld + jt, -jtMax, jtMax);
+		V jb = vmult(n, con->jBias - jbnOld);
+		apply_bias_impulse(a, vneg(jb), r1);
+		apply_bias_impulse(b, jb, r2);
+		apply_impulses(a, b, r1, r2, vrotate(n, v2(con->jnAcc - jnOld, con->jtAcc - jtOld)));
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* joints: src/cpPivotJoint.c:24-70, cpDampedSpring.c:24-77,
+ * cpSlideJoint.c:24-89, cpGearJoint.c:24-69; bias_coef
+ * chipmunk_private.h:264-268 (libm pow, evaluated on the host);
+ * k_tensor chipmunk_private.h:228-262                                   */
+
+static double bias_coef_host(double errorBias, double dt) { return 1.0 - pow(errorBias, dt); }
+
+static void joint_prestep(oworld *w, OJoint *j, double dt)
+{
+	OBody *a = &w->B[j->a], *b = &w->B[j->b];
+	V anchorA = v2(j->P[0], j->P[1]), anchorB = v2(j->P[2], j->P[3]);
+	if (j->type == CPGPU_JOINT_PIVOT) {
+		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
+		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
+		/* k_tensor */
+		double m_sum = a->m_inv + b->m_inv;
+		double k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
+		double a_i_inv = a->i_inv;
+		double r1xsq = j->r1.x * j->r1.x * a_i_inv;
+		double r1ysq = j->r1.y * j->r1.y * a_i_inv;
+		double r1nxy = -j->r1.x * j->r1.y * a_i_inv;
+		k11 += r1ysq; k12 += r1nxy; k21 += r1nxy; k22 += r1xsq;
+		double b_i_inv = b->i_inv;
+		double r2xsq = j->r2.x * j->r2.x * b_i_inv;
+		double r2ysq = j->r2.y * j->r2.y * b_i_inv;
+		double r2nxy = -j->r2.x * j->r2.y * b_i_inv;
+		k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
+		double det = k11 * k22 - k12 * k21;
+		double det_inv = 1.0 / det;
+		j->k11 = k22 * det_inv; j->k12 = -k12 * det_inv;
+		j->k21 = -k21 * det_inv; j->k22 = k11 * det_inv;
+		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
+		j->bias2 = vclamp(vmult(delta, -bias_coef_host(j->errorBias, dt) / dt), j->maxBias);
+	} else if (j->type == CPGPU_JOINT_DAMPED_SPRING) {
+		double restLength = j->P[4], stiffness = j->P[5], damping = j->P[6];
+		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
+		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
+		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
+		double dist = vlength(delta);
+		j->n = vmult(delta, 1.0 / (dist != 0.0 ? dist : INFINITY));
+		double k = k_scalar(a, b, j->r1, j->r2, j->n);
+		j->nMass = 1.0 / k;
+		j->target_vrn = 0.0;
+		j->v_coef = 1.0 - cp_exp_oracle(-damping * dt * k);
+		double f_spring = (restLength - dist) * stiffness;
+		double j_spring = j->jAcc = f_spring * dt;
+		apply_impulses(a, b, j->r1, j->r2, vmult(j->n, j_spring));
+	} else if (j->type == CPGPU_JOINT_SLIDE) {
+		double mn = j->P[4], mx = j->P[5];
+		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
+		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
+		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
+		double dist = vlength(delta);
+		double pdist = 0.0;
+		if (dist > mx) { pdist = dist - mx; j->n = vnormalize(delta); }
+		else if (dist < mn) { pdist = mn - dist; j->n = vneg(vnormalize(delta)); }
+		else { j->n = v2(0.0, 0.0); j->jnAcc = 0.0; }
+		j->nMass = 1.0 / k_scalar(a, b, j->r1, j->r2, j->n);
+		double maxBias = j->maxBias;
+		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * pdist / dt, -maxBias, maxBias);
+	} else {
+		double phase = j->P[0], ratio = j->P[1], ratio_inv = j->P[2];
+		j->iSum = 1.0 / (a->i_inv * ratio_inv + ratio * b->i_inv);
+		double maxBias = j->maxBias;
+		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * (b->a * ratio - a->a - phase) / dt, -maxBias, maxBias);
+	}
+}
+
+static void joint_apply_cached(oworld *w, OJoint *j, double dt_coef)
+{
+	OBody *a = &w->B[j->a], *b = &w->B[j->b];
+	if (j->type == CPGPU_JOINT_PIVOT) {
+		apply_impulses(a, b, j->r1, j->r2, vmult(j->jAcc2, dt_coef));
+	} else if (j->type == CPGPU_JOINT_SLIDE) {
+		V jj = vmult(j->n, j->jnAcc * dt_coef);
+		apply_impulses(a, b, j->r1, j->r2, jj);
+	} else if (j->type == CPGPU_JOINT_GEAR) {
+		double ratio_inv = j->P[2];
+		double jj = j->jAcc * dt_coef;
+		a->w -= jj * a->i_inv * ratio_inv;
+		b->w += jj * b->i_inv;
+	}
+}
+
+static void joint_apply_impulse(oworld *w, OJoint *j, double dt)
+{
+	OBody *a = &w->B[j->a], *b = &w->B[j->b];
+	if (j->type == CPGPU_JOINT_PIVOT) {
+		V r1 = j->r1, r2 = j->r2;
+		V vr = rel_vel(a, b, r1, r2);
+		V d = vsub(j->bias2, vr);
+		V jj = v2(d.x * j->k11 + d.y * j->k12, d.x * j->k21 + d.y * j->k22);
+		V jOld = j->jAcc2;
+		j->jAcc2 = vclamp(vadd(j->jAcc2, jj), j->maxForce * dt);
+		jj = vsub(j->jAcc2, jOld);
+		apply_impulses(a, b, j->r1, j->r2, jj);
+	} else if (j->type == CPGPU_JOINT_DAMPED_SPRING) {
+		V n = j->n;
+		double vrn = vdot(rel_vel(a, b, j->r1, j->r2), n);
+		double v_damp = (j->target_vrn - vrn) * j->v_coef;
+		j->target_vrn = vrn + v_damp;
+		double j_damp = v_damp * j->nMass;
+		j->jAcc += j_damp;
+		apply_impulses(a, b, j->r1, j->r2, vmult(j->n, j_damp));
+	} else if (j->type == CPGPU_JOINT_SLIDE) {
+		if (veql(j->n, v2(0.0, 0.0))) return;
+		V n = j->n;
+		V vr = rel_vel(a, b, j->r1, j->r2);
+		double vrn = vdot(vr, n);
+		double jn = (j->bias - vrn) * j->nMass;
+		double jnOld = j->jnAcc;
+		j->jnAcc = fclampc(jnOld + jn, -j->maxForce * dt, 0.0);
+		jn = j->jnAcc - jnOld;
+		apply_impulses(a, b, j->r1, j->r2, vmult(n, jn));
+	} else {
+		double ratio = j->P[1], ratio_inv = j->P[2];
+		double wr = b->w * ratio - a->w;
+		double jMax = j->maxForce * dt;
+		double jj = (j->bias - wr) * j->iSum;
+		double jOld = j->jAcc;
+		j->jAcc = fclampc(jOld + jj, -jMax, jMax);
+		jj = j->jAcc - jOld;
+		a->w -= jj * a->i_inv * ratio_inv;
+		b->w += jj * b->i_inv;
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* the step: cpSpaceStep (cpSpaceStep.c:335-445)                       */
+
+static void step_world(oworld *w, OWorldPart *P, double dt, const int32_t *order, int64_t order_n)
+{
+	cpgpuSpaceParams *sp = &P->sp;
+	sp->stamp++;
+	double prev_dt = sp->curr_dt;
+	sp->curr_dt = dt;
+
+	/* :351-360 reset the arbiters solved last step */
+	for (int64_t k = 0; k < P->nactive; k++) P->arbs[P->active[k]].state = CPGPU_ARB_NORMAL;
+	P->nactive = 0;
+
+	/* :364-367 integrate positions of dynamic and kinematic bodies */
+	for (int b = P->b0; b < P->b1; b++) if (w->B[b].type != CPGPU_BODY_STATIC) update_position(w, &w->B[b], dt);
+	/* :371 cache the dynamic-index shapes */
+	for (int s = P->s0; s < P->s1; s++) if (w->B[w->S[s].body].type != CPGPU_BODY_STATIC) cache_shape(w, &w->S[s]);
+	/* :372 cpBBTreeReindexQuery first updates every leaf (cpBBTree.c:644) */
+	for (int s = P->s0; s < P->s1; s++) if (w->B[w->S[s].body].type != CPGPU_BODY_STATIC) leaf_update(w, &w->S[s]);
+
+	/* :372 broadphase + cpSpaceCollideShapes (:235-289) */
+	int64_t np;
+	IPair *pairs = world_pairs(w, P, &np);
+	P->npairs = np;
+	P->ncontacts = 0;
+	pid_rebuild(w, P, 0);
+	while (2 * (P->pid_n + np) + 64 > P->pid_mask + 1) {
+		int64_t need = 2 * (P->pid_n + np) + 64, cap = P->pid_mask + 1;
+		while (cap < need) cap *= 2;
+		/* grow: reinsert into a larger table */
+		uint64_t *ok = P->pid_key; uint32_t *oi = P->pid_id, *os = P->pid_stamp;
+		int64_t omask = P->pid_mask;
+		P->pid_key = (uint64_t *)malloc((size_t)cap * sizeof(uint64_t));
+		P->pid_id = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
+		P->pid_stamp = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
+		for (int64_t k = 0; k < cap; k++) P->pid_key[k] = UINT64_MAX;
+		P->pid_mask = cap - 1;
+		P->pid_n = 0;
+		for (int64_t k = 0; k <= omask; k++) if (ok[k] != UINT64_MAX) {
+			uint64_t h = mix64(ok[k]) & (uint64_t)P->pid_mask;
+			while (P->pid_key[h] != UINT64_MAX) h = (h + 1) & (uint64_t)P->pid_mask;
+			P->pid_key[h] = ok[k]; P->pid_id[h] = oi[k]; P->pid_stamp[h] = os[k];
+			P->pid_n++;
+		}
+		free(ok); free(oi); free(os);
+	}
+	for (int64_t k = 0; k < np; k++) {
+		int32_t ia = pairs[k].a, ib = pairs[k].b;
+		/* narrowphase orientation: cpCollide sorts by type (cpCollision.c:707-710);
+		 * same-type pairs take the lower shape index first */
+		if (w->S[ia].type > w->S[ib].type) { int32_t t = ia; ia = ib; ib = t; }
+		if (order) {
+			for (int64_t q = 0; q < order_n; q++) {
+				if (order[2 * q] == ib && order[2 * q + 1] == ia && w->S[ia].type == w->S[ib].type) {
+					int32_t t = ia; ia = ib; ib = t;
+					break;
+				}
+			}
+		}
+		uint64_t key = pair_key(ia, ib);
+		int64_t ai = tab_find(P, key);
+		uint32_t id = pid_get(w, P, key, ia, ib);
+		if (w->ncoll > 0) {
+			const int32_t *c = coll_find(w, key);
+			if (c) { ia = c[0]; ib = c[1]; id = (uint32_t)c[2]; }
+		}
+		CollInfo info;
+		collide(w, ia, ib, id, &info);
+		pid_put(P, key, info.id);   /* cpSpaceCollideShapes returns the new id to the tree (:245) */
+		if (info.count == 0) continue;
+		P->ncontacts += info.count;
+		if (ai < 0) {
+			/* cpArbiterInit (cpArbiter.c:316-350) */
+			ai = arb_append(P);
+			OArb *na = &P->arbs[ai];
+			na->key = key;
+			na->prio = mix64(key);
+			na->count = 0;
+			na->state = CPGPU_ARB_FIRST_COLLISION;
+			na->stamp = 0;
+		}
+		OArb *arb = &P->arbs[ai];
+		/* cpArbiterUpdate (cpArbiter.c:356-414) */
+		const OShape *sa = &w->S[ia], *sb = &w->S[ib];
+		arb->sa = ia; arb->sb = ib; arb->ba = sa->body; arb->bb = sb->body;
+		OContact nc[2];
+		for (int i = 0; i < info.count; i++) {
+			OContact *con = &nc[i];
+			memset(con, 0, sizeof(*con));
+			con->r1 = vsub(info.p1[i], w->B[sa->body].p);
+			con->r2 = vsub(info.p2[i], w->B[sb->body].p);
+			con->hash = info.hash[i];
+			con->jnAcc = con->jtAcc = 0.0;
+			for (int j = 0; j < arb->count; j++) {
+				OContact *old = &arb->con[j];
+				if (con->hash == old->hash) { con->jnAcc = old->jnAcc; con->jtAcc = old->jtAcc; }
+			}
+		}
+		arb->con[0] = nc[0];
+		if (info.count > 1) arb->con[1] = nc[1];
+		arb->count = info.count;
+		arb->n = info.n;
+		arb->e = sa->e * sb->e;
+		arb->u = sa->u * sb->u;
+		V surface_vr = vsub(sb->sv, sa->sv);
+		arb->svr = vsub(surface_vr, vmult(info.n, vdot(surface_vr, info.n)));
+		if (arb->state == CPGPU_ARB_CACHED) arb->state = CPGPU_ARB_FIRST_COLLISION;
+		/* :280 admit; :287 stamp */
+		P->active[P->nactive++] = ai;
+		arb->stamp = sp->stamp;
+		/* keep the lookup index current for arbiters created this step */
+		if (P->narb * 2 + 16 > P->tabmask + 1) tab_rebuild(P);
+		else if (tab_find(P, key) < 0) {
+			uint64_t h = mix64(key) & (uint64_t)P->tabmask;
+			while (P->tab[h] >= 0) h = (h + 1) & (uint64_t)P->tabmask;
+			P->tab[h] = ai;
+		}
+	}
+	free(pairs);
+
+	/* :380 cpHashSetFilter(cachedArbiters, cpSpaceArbiterSetFilter) */
+	{
+		int64_t *remap = (int64_t *)malloc((size_t)(P->narb ? P->narb : 1) * sizeof(int64_t));
+		int64_t keep = 0;
+		for (int64_t i = 0; i < P->narb; i++) {
+			OArb *arb = &P->arbs[i];
+			uint32_t ticks = sp->stamp - arb->stamp;
+			const OBody *a = &w->B[arb->ba], *b = &w->B[arb->bb];
+			int evict = 0;
+			if (a->type == CPGPU_BODY_STATIC && b->type == CPGPU_BODY_STATIC) {
+				evict = 0;
+			} else {
+				if (ticks >= 1 && arb->state != CPGPU_ARB_CACHED) arb->state = CPGPU_ARB_CACHED;
+				if (ticks >= sp->collision_persistence) evict = 1;
+			}
+			if (evict) { remap[i] = -1; continue; }
+			remap[i] = keep;
+			if (keep != i) P->arbs[keep] = *arb;
+			keep++;
+		}
+		P->narb = keep;
+		for (int64_t k = 0; k < P->nactive; k++) P->active[k] = remap[P->active[k]];
+		free(remap);
+		tab_rebuild(P);
+	}
+
+	/* solve order: deterministic colours (or an explicit replay order) */
+	if (order) {
+		int64_t n = 0;
+		for (int64_t q = 0; q < order_n; q++) {
+			uint64_t key = pair_key(order[2 * q], order[2 * q + 1]);
+			int64_t ai = tab_find(P, key);
+			if (ai >= 0 && P->arbs[ai].stamp == sp->stamp) { P->arbs[ai].colour = (int)q; P->active[n++] = ai; }
+		}
+		P->nactive = n;
+		P->ncol_arb = (int)n;
+	} else {
+		int64_t n = P->nactive;
+		int32_t *ba = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+		int32_t *bb = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+		uint64_t *pr = (uint64_t *)malloc((size_t)(n + 1) * sizeof(uint64_t));
+		int32_t *col = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+		for (int64_t k = 0; k < n; k++) {
+			OArb *arb = &P->arbs[P->active[k]];
+			ba[k] = w->B[arb->ba].type == CPGPU_BODY_DYNAMIC ? arb->ba : -1;
+			bb[k] = w->B[arb->bb].type == CPGPU_BODY_DYNAMIC ? arb->bb : -1;
+			pr[k] = arb->prio;
+		}
+		P->ncol_arb = colour_elements(w, n, ba, bb, pr, col);
+		for (int64_t k = 0; k < n; k++) P->arbs[P->active[k]].colour = col[k];
+		g_sort_arbs = P->arbs;
+		qsort(P->active, (size_t)n, sizeof(int64_t), cmp_solve);
+		free(ba); free(bb); free(pr); free(col);
+	}
+	{
+		int64_t n = P->c1 - P->c0;
+		int32_t *ba = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+		int32_t *bb = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+		uint64_t *pr = (uint64_t *)malloc((size_t)(n + 1) * sizeof(uint64_t));
+		int32_t *col = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+		for (int64_t k = 0; k < n; k++) {
+			OJoint *j = &w->J[P->c0 + k];
+			ba[k] = w->B[j->a].type == CPGPU_BODY_DYNAMIC ? j->a : -1;
+			bb[k] = w->B[j->b].type == CPGPU_BODY_DYNAMIC ? j->b : -1;
+			pr[k] = j->prio;
+		}
+		if (w->joint_list_order) {
+			/* replay the reference's constraint list order */
+			for (int64_t k = 0; k < n; k++) { w->J[P->c0 + k].colour = (int)k; P->jorder[k] = P->c0 + (int32_t)k; }
+			P->ncol_cons = (int)n;
+		} else {
+			P->ncol_cons = colour_elements(w, n, ba, bb, pr, col);
+			for (int64_t k = 0; k < n; k++) { w->J[P->c0 + k].colour = col[k]; P->jorder[k] = P->c0 + (int32_t)k; }
+			g_sort_joints = w->J;
+			qsort(P->jorder, (size_t)n, sizeof(int32_t), cmp_jsolve);
+		}
+		free(ba); free(bb); free(pr); free(col);
+	}
+
+	/* :383-387 prestep the arbiters */
+	double slop = sp->collision_slop;
+	double biasCoef = 1.0 - pow(sp->collision_bias, dt);
+	for (int64_t k = 0; k < P->nactive; k++) arb_prestep(w, &P->arbs[P->active[k]], dt, slop, biasCoef);
+	/* :389-396 prestep the constraints (in solve order) */
+	int64_t nj = P->c1 - P->c0;
+	for (int64_t k = 0; k < nj; k++) joint_prestep(w, &w->J[P->jorder[k]], dt);
+	/* :399-404 integrate velocities */
+	double damping = pow(sp->damping, dt);
+	V g = v2(sp->gravity_x, sp->gravity_y);
+	for (int b = P->b0; b < P->b1; b++) if (w->B[b].type != CPGPU_BODY_STATIC) update_velocity(&w->B[b], g, damping, dt);
+	/* :407-415 cached impulses */
+	double dt_coef = (prev_dt == 0.0 ? 0.0 : dt / prev_dt);
+	for (int64_t k = 0; k < P->nactive; k++) arb_apply_cached(w, &P->arbs[P->active[k]], dt_coef);
+	for (int64_t k = 0; k < nj; k++) joint_apply_cached(w, &w->J[P->jorder[k]], dt_coef);
+	/* :418-427 the impulse solver */
+	for (int it = 0; it < sp->iterations; it++) {
+		for (int64_t k = 0; k < P->nactive; k++) arb_apply_impulse(w, &P->arbs[P->active[k]]);
+		for (int64_t k = 0; k < nj; k++) joint_apply_impulse(w, &w->J[P->jorder[k]], dt);
+	}
+}
+
+void ow_step(oworld *w, double dt)
+{
+	if (dt == 0.0) return;   /* cpSpaceStep.c:339 */
+	for (int wi = 0; wi < w->nw; wi++) {
+		step_world(w, &w->W[wi], dt, w->next_n > 0 ? w->next_pairs : NULL, w->next_n > 0 ? w->next_n : 0);
+	}
+	w->next_n = 0;
+	w->ncoll = 0;
+	w->steps++;
+}
+
+/* ------------------------------------------------------------------ */
+/* readback                                                            */
+
+int ow_read_bodies(oworld *w, double *p, double *v, double *a, double *wv,
+                   double *v_bias, double *w_bias, double *transform)
+{
+	for (int i = 0; i < w->nb; i++) {
+		OBody *b = &w->B[i];
+		if (p) { p[2 * i] = b->p.x; p[2 * i + 1] = b->p.y; }
+		if (v) { v[2 * i] = b->v.x; v[2 * i + 1] = b->v.y; }
+		if (a) a[i] = b->a;
+		if (wv) wv[i] = b->w;
+		if (v_bias) { v_bias[2 * i] = b->v_bias.x; v_bias[2 * i + 1] = b->v_bias.y; }
+		if (w_bias) w_bias[i] = b->w_bias;
+		if (transform) {
+			double *t = transform + 6 * i;
+			t[0] = b->T.a; t[1] = b->T.b; t[2] = b->T.c; t[3] = b->T.d; t[4] = b->T.tx; t[5] = b->T.ty;
+		}
+	}
+	return 0;
+}
+
+int ow_write_bodies(oworld *w, const double *v, const double *wv, const double *f, const double *t)
+{
+	for (int i = 0; i < w->nb; i++) {
+		OBody *b = &w->B[i];
+		if (v) b->v = v2(v[2 * i], v[2 * i + 1]);
+		if (wv) b->w = wv[i];
+		if (f) b->f = v2(f[2 * i], f[2 * i + 1]);
+		if (t) b->t = t[i];
+	}
+	return 0;
+}
+
+int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colour, int32_t *state,
+                         int32_t *count, double *normal, double *contact, uint64_t *hash)
+{
+	int64_t n = 0;
+	for (int wi = 0; wi < w->nw; wi++) {
+		OWorldPart *P = &w->W[wi];
+		for (int64_t k = 0; k < P->nactive; k++, n++) {
+			if (n >= cap) continue;
+			OArb *arb = &P->arbs[P->active[k]];
+			if (shapes) { shapes[2 * n] = arb->sa; shapes[2 * n + 1] = arb->sb; }
+			if (colour) colour[n] = arb->colour;
+			if (state) state[n] = arb->state;
+			if (count) count[n] = arb->count;
+			if (normal) { normal[2 * n] = arb->n.x; normal[2 * n + 1] = arb->n.y; }
+			for (int i = 0; i < 2; i++) {
+				OContact *c = &arb->con[i];
+				int valid = i < arb->count;
+				if (contact) {
+					double *o = contact + 24 * n + 12 * i;
+					double vals[12] = {c->r1.x, c->r1.y, c->r2.x, c->r2.y, c->nMass, c->tMass, c->bounce,
+					                   c->jnAcc, c->jtAcc, c->jBias, c->bias, 0.0};
+					for (int q = 0; q < 12; q++) o[q] = valid ? vals[q] : 0.0;
+				}
+				if (hash) hash[2 * n + i] = valid ? c->hash : 0;
+			}
+		}
+	}
+	return n;
+}
+
+int ow_read_constraints(oworld *w, double *jacc, int32_t *colour)
+{
+	for (int i = 0; i < w->nc; i++) {
+		OJoint *j = &w->J[i];
+		if (jacc) {
+			if (j->type == CPGPU_JOINT_PIVOT) { jacc[2 * i] = j->jAcc2.x; jacc[2 * i + 1] = j->jAcc2.y; }
+			else if (j->type == CPGPU_JOINT_SLIDE) { jacc[2 * i] = j->jnAcc; jacc[2 * i + 1] = 0.0; }
+			else { jacc[2 * i] = j->jAcc; jacc[2 * i + 1] = 0.0; }
+		}
+		if (colour) colour[i] = j->colour;
+	}
+	return 0;
+}
+
+void ow_stats(oworld *w, cpgpuStats *out)
+{
+	memset(out, 0, sizeof(*out));
+	for (int wi = 0; wi < w->nw; wi++) {
+		OWorldPart *P = &w->W[wi];
+		out->candidate_pairs += P->npairs;
+		out->arbiters += P->nactive;
+		out->contacts += P->ncontacts;
+		out->cached_arbiters += P->narb;
+		if (P->ncol_arb > out->arbiter_colours) out->arbiter_colours = P->ncol_arb;
+		if (P->ncol_cons > out->constraint_colours) out->constraint_colours = P->ncol_cons;
+	}
+	out->steps = w->steps;
+}

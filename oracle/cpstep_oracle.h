@@ -1,0 +1,82 @@
+/*
+ * cpstep_oracle.h -- CPU restatement of Chipmunk2D 7.0.3's cpSpaceStep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP step library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  Nothing in chipmunk2d_amd/ links, imports or calls it.
+ *
+ * It steps the same packed description (include/cpgpu.h) the device library
+ * takes, in plain C double arithmetic compiled with -ffp-contract=off, and
+ * reproduces the device library's deterministic solve order (graph colours),
+ * so device results can be compared bit for bit.  See cpstep_oracle.c for
+ * the reference file:line each function follows and how it is pinned.
+ */
+#ifndef CPSTEP_ORACLE_H
+#define CPSTEP_ORACLE_H
+
+#include <stdint.h>
+#include "../include/cpgpu.h"
+
+typedef struct oworld oworld;
+
+oworld *ow_new(const cpgpuWorldDesc *desc);
+void ow_free(oworld *w);
+
+/* One cpSpaceStep over every world. */
+void ow_step(oworld *w, double dt);
+
+/* 1: body rotations use libm cos/sin exactly as the reference does
+ * (cpBody.c:350 via cpVect.h:104-107).  0 (default): the deterministic
+ * cp_sincos that the device library also uses. */
+void ow_set_libm_trig(oworld *w, int enable);
+
+/* 1: solve constraints in description order instead of colour order
+ * (the reference's cpSpace.constraints order). */
+void ow_set_joint_list_order(oworld *w, int enable);
+
+/* Replace the colour order for the next step only: the arbiters are solved in
+ * exactly this order and same-type pairs collide in this orientation.  pairs
+ * holds n (shape a, shape b) global indices.  Used to replay the reference's
+ * own order. */
+void ow_set_next_order(oworld *w, int64_t n, const int32_t *pairs);
+
+/* For the next step only: collide these pairs in the given orientation with
+ * the given warm-start id ({a, b, id} triples), as the reference did. */
+void ow_set_next_collisions(oworld *w, int64_t n, const int32_t *coll);
+
+int ow_read_bodies(oworld *w, double *p, double *v, double *a, double *wv,
+                   double *v_bias, double *w_bias, double *transform);
+int ow_write_bodies(oworld *w, const double *v, const double *wv, const double *f, const double *t);
+
+/* Same layout as cpgpuWorldReadArbiters. */
+int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colour, int32_t *state,
+                         int32_t *count, double *normal, double *contact, uint64_t *hash);
+int ow_read_constraints(oworld *w, double *jacc, int32_t *colour);
+void ow_stats(oworld *w, cpgpuStats *out);
+
+/* ---- pieces exposed for the pinning tests ---- */
+
+/* Recompute the world-space geometry of every shape from the current body
+ * transforms (cpShapeUpdate for all shapes, cpShape.c:216-220). */
+void ow_cache_all_shapes(oworld *w);
+/* Shape bounding boxes [n_shapes][4] (l, b, r, t) as last cached. */
+void ow_read_shape_bbs(oworld *w, double *bb);
+/* Set body transforms directly ([n][6]), as cpBody.transform. */
+void ow_set_transforms(oworld *w, const double *transform);
+
+/* cpCollide(a, b, id) (cpCollision.c:701-726) on cached shapes.  Writes the
+ * oriented shapes, the returned id, the normal, count and per contact
+ * {r1.x, r1.y, r2.x, r2.y} (absolute points) and hashes.  Returns count. */
+int ow_collide(oworld *w, int32_t a, int32_t b, uint32_t id, int32_t *ab_out, uint32_t *id_out,
+               double *n_out, double *points_out, uint64_t *hash_out);
+
+/* Candidate pairs of the exact QueryReject predicate (cpSpaceStep.c:204-232)
+ * for the current cached geometry: writes up to cap (a, b) pairs with a < b,
+ * sorted; returns the total count. */
+int64_t ow_candidate_pairs(oworld *w, int64_t cap, int32_t *pairs);
+
+/* The deterministic sin/cos shared with the device library. */
+void cp_sincos_oracle(double x, double *s, double *c);
+double cp_exp_oracle(double x);
+
+#endif

@@ -1,0 +1,381 @@
+/*
+ * ref_bridge.c -- test-only bridge into the compiled reference (Chipmunk2D 7.0.3).
+ *
+ * TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
+ * the unmodified reference sources under /root/reference/src into
+ * oracle/_ref/libcpref.so.  The tests use it to pin the oracle
+ * (cpstep_oracle.c) against the real cpSpaceStep, cpCollide, cpBBTree and
+ * body integrators, and bench.py times the reference's cpSpaceStep /
+ * cpHastySpaceStep through it as the CPU baseline.  It builds a real cpSpace
+ * from the same packed description (include/cpgpu.h) through the public API,
+ * then writes the few private fields (chipmunk_structs.h) that the public
+ * setters would recompute through libm (body transform, inverse masses).
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+#include "chipmunk/chipmunk_private.h"
+#include "chipmunk/cpHastySpace.h"
+#include "../include/cpgpu.h"
+
+typedef struct rbWorld {
+	cpSpace *space;
+	int hasty;
+	int nb, ns, nc;
+	int b0, s0, c0;
+	cpBody **bodies;
+	cpShape **shapes;
+	cpConstraint **cons;
+	int *ctype;
+} rbWorld;
+
+static cpTransform xf(const double *t)
+{
+	cpTransform x = {t[0], t[1], t[2], t[3], t[4], t[5]};
+	return x;
+}
+
+/* Build world `wi` of the description as a cpSpace (hasty != 0: cpHastySpace
+ * with that many threads). */
+rbWorld *rb_new(const cpgpuWorldDesc *d, int wi, int hasty)
+{
+	rbWorld *w = (rbWorld *)calloc(1, sizeof(rbWorld));
+	const cpgpuSpaceParams *sp = &d->space[wi];
+	w->hasty = hasty;
+	if (hasty) {
+		w->space = cpHastySpaceNew();
+		cpHastySpaceSetThreads(w->space, (unsigned long)hasty);
+	} else {
+		w->space = cpSpaceNew();
+	}
+	cpSpace *space = w->space;
+	cpSpaceSetIterations(space, sp->iterations);
+	cpSpaceSetGravity(space, cpv(sp->gravity_x, sp->gravity_y));
+	cpSpaceSetDamping(space, sp->damping);
+	cpSpaceSetCollisionSlop(space, sp->collision_slop);
+	cpSpaceSetCollisionBias(space, sp->collision_bias);
+	cpSpaceSetCollisionPersistence(space, sp->collision_persistence);
+	space->curr_dt = sp->curr_dt;
+	space->stamp = sp->stamp;
+
+	w->b0 = d->world_body_start[wi];
+	w->nb = d->world_body_start[wi + 1] - w->b0;
+	w->s0 = d->world_shape_start[wi];
+	w->ns = d->world_shape_start[wi + 1] - w->s0;
+	w->c0 = d->world_constraint_start ? d->world_constraint_start[wi] : 0;
+	w->nc = d->world_constraint_start ? d->world_constraint_start[wi + 1] - w->c0 : 0;
+	w->bodies = (cpBody **)calloc((size_t)w->nb + 1, sizeof(cpBody *));
+	w->shapes = (cpShape **)calloc((size_t)w->ns + 1, sizeof(cpShape *));
+	w->cons = (cpConstraint **)calloc((size_t)w->nc + 1, sizeof(cpConstraint *));
+	w->ctype = (int *)calloc((size_t)w->nc + 1, sizeof(int));
+
+	for (int k = 0; k < w->nb; k++) {
+		int i = w->b0 + k;
+		cpBody *b;
+		if (d->body_type[i] == CPGPU_BODY_STATIC) b = cpBodyNewStatic();
+		else if (d->body_type[i] == CPGPU_BODY_KINEMATIC) b = cpBodyNewKinematic();
+		else b = cpBodyNew(d->body_m[i], d->body_i[i]);
+		b->m = d->body_m[i]; b->m_inv = d->body_m_inv[i];
+		b->i = d->body_i[i]; b->i_inv = d->body_i_inv[i];
+		b->cog = cpv(d->body_cog[2 * i], d->body_cog[2 * i + 1]);
+		b->p = cpv(d->body_p[2 * i], d->body_p[2 * i + 1]);
+		b->v = cpv(d->body_v[2 * i], d->body_v[2 * i + 1]);
+		b->f = cpv(d->body_f[2 * i], d->body_f[2 * i + 1]);
+		b->a = d->body_a[i]; b->w = d->body_w[i]; b->t = d->body_t[i];
+		b->v_bias = cpv(d->body_v_bias[2 * i], d->body_v_bias[2 * i + 1]);
+		b->w_bias = d->body_w_bias[i];
+		b->transform = xf(d->body_transform + 6 * i);
+		b->userData = (cpDataPointer)(intptr_t)i;
+		cpSpaceAddBody(space, b);
+		w->bodies[k] = b;
+	}
+	for (int k = 0; k < w->ns; k++) {
+		int i = w->s0 + k;
+		cpBody *body = w->bodies[d->shape_body[i] - w->b0];
+		const double *g = d->shape_geom + 10 * i;
+		double r = d->shape_radius[i];
+		cpShape *s;
+		if (d->shape_type[i] == CPGPU_SHAPE_CIRCLE) {
+			s = cpCircleShapeNew(body, r, cpv(g[0], g[1]));
+		} else if (d->shape_type[i] == CPGPU_SHAPE_SEGMENT) {
+			s = cpSegmentShapeNew(body, cpv(g[0], g[1]), cpv(g[2], g[3]), r);
+			cpSegmentShape *seg = (cpSegmentShape *)s;
+			seg->n = cpv(g[4], g[5]);
+			seg->a_tangent = cpv(g[6], g[7]);
+			seg->b_tangent = cpv(g[8], g[9]);
+		} else {
+			int st = d->shape_poly_start[i], cnt = d->shape_poly_count[i];
+			cpVect *verts = (cpVect *)malloc((size_t)cnt * sizeof(cpVect));
+			for (int q = 0; q < cnt; q++) verts[q] = cpv(d->poly_verts[2 * (st + q)], d->poly_verts[2 * (st + q) + 1]);
+			s = cpPolyShapeNewRaw(body, cnt, verts, r);
+			cpPolyShape *poly = (cpPolyShape *)s;
+			for (int q = 0; q < cnt; q++) {
+				poly->planes[q + cnt].n = cpv(d->poly_normals[2 * (st + q)], d->poly_normals[2 * (st + q) + 1]);
+			}
+			free(verts);
+		}
+		cpShapeSetElasticity(s, d->shape_e[i]);
+		cpShapeSetFriction(s, d->shape_u[i]);
+		cpShapeSetSurfaceVelocity(s, cpv(d->shape_surface_v[2 * i], d->shape_surface_v[2 * i + 1]));
+		cpShapeSetFilter(s, cpShapeFilterNew((cpGroup)d->shape_group[i], d->shape_categories[i], d->shape_mask[i]));
+		s->userData = (cpDataPointer)(intptr_t)i;
+		cpSpaceAddShape(space, s);
+		if (s->hashid != d->shape_hashid[i]) {
+			fprintf(stderr, "ref_bridge: shape %d hashid %lu != description %lu\n", i,
+			        (unsigned long)s->hashid, (unsigned long)d->shape_hashid[i]);
+			abort();
+		}
+		w->shapes[k] = s;
+	}
+	/* cpSpaceAddShape caches shapes with the body transform; the body-type
+	 * mass accumulation is bypassed (shapes carry zero mass), so restore the
+	 * description's mass fields in case a public call touched them. */
+	for (int k = 0; k < w->nb; k++) {
+		int i = w->b0 + k;
+		cpBody *b = w->bodies[k];
+		b->m = d->body_m[i]; b->m_inv = d->body_m_inv[i];
+		b->i = d->body_i[i]; b->i_inv = d->body_i_inv[i];
+		b->cog = cpv(d->body_cog[2 * i], d->body_cog[2 * i + 1]);
+		b->p = cpv(d->body_p[2 * i], d->body_p[2 * i + 1]);
+		b->transform = xf(d->body_transform + 6 * i);
+	}
+	for (int k = 0; k < w->nc; k++) {
+		int i = w->c0 + k;
+		cpBody *a = w->bodies[d->cons_a[i] - w->b0], *b = w->bodies[d->cons_b[i] - w->b0];
+		const double *P = d->cons_params + 8 * i;
+		cpConstraint *c;
+		int t = d->cons_type[i];
+		if (t == CPGPU_JOINT_PIVOT) c = cpPivotJointNew2(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]));
+		else if (t == CPGPU_JOINT_DAMPED_SPRING) c = cpDampedSpringNew(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]), P[4], P[5], P[6]);
+		else if (t == CPGPU_JOINT_SLIDE) c = cpSlideJointNew(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]), P[4], P[5]);
+		else {
+			c = cpGearJointNew(a, b, P[0], P[1]);
+			((cpGearJoint *)c)->ratio_inv = P[2];
+		}
+		cpConstraintSetMaxForce(c, d->cons_max_force[i]);
+		cpConstraintSetErrorBias(c, d->cons_error_bias[i]);
+		cpConstraintSetMaxBias(c, d->cons_max_bias[i]);
+		cpConstraintSetCollideBodies(c, (cpBool)d->cons_collide_bodies[i]);
+		if (d->cons_jacc) {
+			double j0 = d->cons_jacc[2 * i], j1 = d->cons_jacc[2 * i + 1];
+			if (t == CPGPU_JOINT_PIVOT) ((cpPivotJoint *)c)->jAcc = cpv(j0, j1);
+			else if (t == CPGPU_JOINT_DAMPED_SPRING) ((cpDampedSpring *)c)->jAcc = j0;
+			else if (t == CPGPU_JOINT_SLIDE) ((cpSlideJoint *)c)->jnAcc = j0;
+			else ((cpGearJoint *)c)->jAcc = j0;
+		}
+		c->userData = (cpDataPointer)(intptr_t)i;
+		cpSpaceAddConstraint(space, c);
+		w->cons[k] = c;
+		w->ctype[k] = t;
+	}
+	return w;
+}
+
+void rb_free(rbWorld *w)
+{
+	if (!w) return;
+	for (int k = 0; k < w->nc; k++) { cpSpaceRemoveConstraint(w->space, w->cons[k]); cpConstraintFree(w->cons[k]); }
+	for (int k = 0; k < w->ns; k++) { cpSpaceRemoveShape(w->space, w->shapes[k]); cpShapeFree(w->shapes[k]); }
+	for (int k = 0; k < w->nb; k++) { cpSpaceRemoveBody(w->space, w->bodies[k]); cpBodyFree(w->bodies[k]); }
+	if (w->hasty) cpHastySpaceFree(w->space); else cpSpaceFree(w->space);
+	free(w->bodies); free(w->shapes); free(w->cons); free(w->ctype);
+	free(w);
+}
+
+void rb_step(rbWorld *w, double dt, int n)
+{
+	for (int k = 0; k < n; k++) {
+		if (w->hasty) cpHastySpaceStep(w->space, dt);
+		else cpSpaceStep(w->space, dt);
+	}
+}
+
+/* Body state of this world, arrays indexed by local body index. */
+void rb_read_bodies(rbWorld *w, double *p, double *v, double *a, double *wv, double *v_bias, double *w_bias,
+                    double *transform)
+{
+	for (int k = 0; k < w->nb; k++) {
+		cpBody *b = w->bodies[k];
+		if (p) { p[2 * k] = b->p.x; p[2 * k + 1] = b->p.y; }
+		if (v) { v[2 * k] = b->v.x; v[2 * k + 1] = b->v.y; }
+		if (a) a[k] = b->a;
+		if (wv) wv[k] = b->w;
+		if (v_bias) { v_bias[2 * k] = b->v_bias.x; v_bias[2 * k + 1] = b->v_bias.y; }
+		if (w_bias) w_bias[k] = b->w_bias;
+		if (transform) {
+			cpTransform t = b->transform;
+			double *o = transform + 6 * k;
+			o[0] = t.a; o[1] = t.b; o[2] = t.c; o[3] = t.d; o[4] = t.tx; o[5] = t.ty;
+		}
+	}
+}
+
+/* cpSpace.arbiters after the last step, in solve order; same layout as
+ * cpgpuWorldReadArbiters (shape indices are global description indices). */
+int64_t rb_read_arbiters(rbWorld *w, int64_t cap, int32_t *shapes, int32_t *state, int32_t *count,
+                         double *normal, double *contact, uint64_t *hash)
+{
+	cpArray *arr = w->space->arbiters;
+	int64_t n = arr->num;
+	for (int64_t k = 0; k < n && k < cap; k++) {
+		cpArbiter *arb = (cpArbiter *)arr->arr[k];
+		if (shapes) {
+			shapes[2 * k] = (int32_t)(intptr_t)arb->a->userData;
+			shapes[2 * k + 1] = (int32_t)(intptr_t)arb->b->userData;
+		}
+		if (state) state[k] = (int32_t)arb->state;
+		if (count) count[k] = arb->count;
+		if (normal) { normal[2 * k] = arb->n.x; normal[2 * k + 1] = arb->n.y; }
+		for (int i = 0; i < 2; i++) {
+			int valid = i < arb->count;
+			struct cpContact *c = valid ? &arb->contacts[i] : NULL;
+			if (contact) {
+				double *o = contact + 24 * k + 12 * i;
+				if (valid) {
+					double vals[12] = {c->r1.x, c->r1.y, c->r2.x, c->r2.y, c->nMass, c->tMass, c->bounce,
+					                   c->jnAcc, c->jtAcc, c->jBias, c->bias, 0.0};
+					memcpy(o, vals, sizeof(vals));
+				} else memset(o, 0, 12 * sizeof(double));
+			}
+			if (hash) hash[2 * k + i] = valid ? (uint64_t)c->hash : 0;
+		}
+	}
+	return n;
+}
+
+int64_t rb_num_cached_arbiters(rbWorld *w) { return cpHashSetCount(w->space->cachedArbiters); }
+
+/* Constraint accumulated impulses, [nc][2], description order. */
+void rb_read_constraints(rbWorld *w, double *jacc)
+{
+	for (int k = 0; k < w->nc; k++) {
+		cpConstraint *c = w->cons[k];
+		double j0 = 0, j1 = 0;
+		int t = w->ctype[k];
+		if (t == CPGPU_JOINT_PIVOT) { j0 = ((cpPivotJoint *)c)->jAcc.x; j1 = ((cpPivotJoint *)c)->jAcc.y; }
+		else if (t == CPGPU_JOINT_DAMPED_SPRING) j0 = ((cpDampedSpring *)c)->jAcc;
+		else if (t == CPGPU_JOINT_SLIDE) j0 = ((cpSlideJoint *)c)->jnAcc;
+		else j0 = ((cpGearJoint *)c)->jAcc;
+		jacc[2 * k] = j0; jacc[2 * k + 1] = j1;
+	}
+}
+
+/* Set body transforms (local indices) and re-cache every shape with them
+ * through the public cpShapeUpdate (cpShape.c:216-220). */
+void rb_set_transforms(rbWorld *w, const double *transform)
+{
+	for (int k = 0; k < w->nb; k++) w->bodies[k]->transform = xf(transform + 6 * k);
+	for (int k = 0; k < w->ns; k++) cpShapeUpdate(w->shapes[k], w->shapes[k]->body->transform);
+}
+
+void rb_read_shape_bbs(rbWorld *w, double *bb)
+{
+	for (int k = 0; k < w->ns; k++) {
+		cpBB b = w->shapes[k]->bb;
+		bb[4 * k] = b.l; bb[4 * k + 1] = b.b; bb[4 * k + 2] = b.r; bb[4 * k + 3] = b.t;
+	}
+}
+
+/* cpCollide on two shapes (local indices) as they are cached. */
+int rb_collide(rbWorld *w, int32_t a, int32_t b, uint32_t id, int32_t *ab_out, uint32_t *id_out, double *n_out,
+               double *points_out, uint64_t *hash_out)
+{
+	struct cpContact contacts[CP_MAX_CONTACTS_PER_ARBITER];
+	struct cpCollisionInfo info = cpCollide(w->shapes[a], w->shapes[b], id, contacts);
+	ab_out[0] = (int32_t)(intptr_t)info.a->userData;
+	ab_out[1] = (int32_t)(intptr_t)info.b->userData;
+	*id_out = info.id;
+	n_out[0] = info.n.x; n_out[1] = info.n.y;
+	for (int i = 0; i < info.count; i++) {
+		points_out[4 * i + 0] = contacts[i].r1.x; points_out[4 * i + 1] = contacts[i].r1.y;
+		points_out[4 * i + 2] = contacts[i].r2.x; points_out[4 * i + 3] = contacts[i].r2.y;
+		hash_out[i] = (uint64_t)contacts[i].hash;
+	}
+	return info.count;
+}
+
+/* The reference broadphase: cpSpatialIndexReindexQuery over the dynamic
+ * BBTree (cpBBTree.c:638-654, which also collides against the static tree)
+ * filtered by the same QueryReject predicate cpSpaceCollideShapes applies
+ * (cpSpaceStep.c:204-232).  Shapes must have been cached first.  Writes
+ * (min, max) global index pairs; returns the count. */
+typedef struct { rbWorld *w; int64_t cap, n; int32_t *out; } PairCtx;
+
+static cpBool reject_constraint(cpBody *a, cpBody *b)
+{
+	CP_BODY_FOREACH_CONSTRAINT(a, constraint) {
+		if (!constraint->collideBodies &&
+		    ((constraint->a == a && constraint->b == b) || (constraint->a == b && constraint->b == a))) return cpTrue;
+	}
+	return cpFalse;
+}
+
+static cpCollisionID pair_recorder(cpShape *a, cpShape *b, cpCollisionID id, void *data)
+{
+	PairCtx *ctx = (PairCtx *)data;
+	if (!cpBBIntersects(a->bb, b->bb) || a->body == b->body || cpShapeFilterReject(a->filter, b->filter) ||
+	    reject_constraint(a->body, b->body)) return id;
+	if (a->body->m == INFINITY && b->body->m == INFINITY) return id;
+	int32_t ia = (int32_t)(intptr_t)a->userData, ib = (int32_t)(intptr_t)b->userData;
+	if (ctx->n < ctx->cap) {
+		ctx->out[2 * ctx->n] = ia < ib ? ia : ib;
+		ctx->out[2 * ctx->n + 1] = ia < ib ? ib : ia;
+	}
+	ctx->n++;
+	return id;
+}
+
+int64_t rb_candidate_pairs(rbWorld *w, int64_t cap, int32_t *out)
+{
+	PairCtx ctx = {w, cap, 0, out};
+	cpSpatialIndexReindexQuery(w->space->dynamicShapes, (cpSpatialIndexQueryFunc)pair_recorder, &ctx);
+	return ctx.n;
+}
+
+/* The reference integrators on one body (local index) with libm trig. */
+void rb_body_update_position(rbWorld *w, int k, double dt) { cpBodyUpdatePosition(w->bodies[k], dt); }
+void rb_body_update_velocity(rbWorld *w, int k, double gx, double gy, double damping, double dt)
+{
+	cpBodyUpdateVelocity(w->bodies[k], cpv(gx, gy), damping, dt);
+}
+
+int64_t rb_space_stamp(rbWorld *w) { return (int64_t)w->space->stamp; }
+
+/* ---- collision log: every cpCollide made by cpSpaceCollideShapes
+ * (cpSpaceStep.c:242), linked through -Wl,--wrap=cpCollide.  Records the
+ * oriented shapes (after cpCollide's type sort), the warm-start id the
+ * BBTree pair supplied, and the id returned. */
+struct cpCollisionInfo __real_cpCollide(const cpShape *a, const cpShape *b, cpCollisionID id, struct cpContact *contacts);
+
+static int g_log_on;
+static int64_t g_log_n, g_log_cap;
+static int32_t *g_log;
+
+struct cpCollisionInfo __wrap_cpCollide(const cpShape *a, const cpShape *b, cpCollisionID id, struct cpContact *contacts)
+{
+	struct cpCollisionInfo info = __real_cpCollide(a, b, id, contacts);
+	if (g_log_on) {
+		if (g_log_n == g_log_cap) {
+			g_log_cap = g_log_cap ? 2 * g_log_cap : 4096;
+			g_log = (int32_t *)realloc(g_log, (size_t)g_log_cap * 4 * sizeof(int32_t));
+		}
+		int32_t *e = g_log + 4 * g_log_n++;
+		e[0] = (int32_t)(intptr_t)info.a->userData;
+		e[1] = (int32_t)(intptr_t)info.b->userData;
+		e[2] = (int32_t)id;
+		e[3] = (int32_t)info.id;
+	}
+	return info;
+}
+
+void rb_log_collisions(int on) { g_log_on = on; g_log_n = 0; }
+
+/* Copies up to cap entries of {a, b, id_in, id_out}; returns the count. */
+int64_t rb_read_collision_log(int64_t cap, int32_t *out)
+{
+	int64_t n = g_log_n < cap ? g_log_n : cap;
+	if (out && n) memcpy(out, g_log, (size_t)n * 4 * sizeof(int32_t));
+	return g_log_n;
+}

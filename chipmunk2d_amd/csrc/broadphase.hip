@@ -1,0 +1,501 @@
+// broadphase.hip -- candidate pairs for the narrowphase.
+//
+// Replaces the reference's cpBBTree reindex query (src/cpBBTree.c:638-654)
+// with a linear BVH rebuilt every step:
+//   Morton keys of the shapes' box centres, prefixed by the world index, so
+//   every world is one subtree of a single hierarchy;
+//   an LSD radix sort whose block-local ranking uses LDS counters and
+//   wavefront ballots (stable, 8-bit digits, digits that do not vary skipped);
+//   the Karras (2012) radix-tree construction, one internal node per lane;
+//   a bottom-up refit of node boxes;
+//   one traversal lane per dynamic shape against the dynamic tree (pairs
+//   reported once, by the lower leaf) and the static tree.
+// Node boxes are unions (min/max) of the shapes' tight boxes, so culling
+// with the reference's inclusive cpBBIntersects test loses no pair; every
+// reported pair satisfies the exact QueryReject predicate
+// (src/cpSpaceStep.c:204-232).
+#include "cpgpu_internal.h"
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 4;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int MORTON_BITS = 22;
+constexpr int WORLD_SHIFT = 44;
+constexpr int TRAVERSAL_STACK = 64;
+
+__device__ __forceinline__ unsigned long long ord_of(double x)
+{
+	unsigned long long u = (unsigned long long)__double_as_longlong(x);
+	return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double double_of(unsigned long long u)
+{
+	u = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+	return __longlong_as_double((long long)u);
+}
+
+__global__ void k_bounds_reset(unsigned long long *wb, int n_worlds)
+{
+	const int w = blockIdx.x * blockDim.x + threadIdx.x;
+	if (w >= n_worlds) return;
+	wb[4 * w + 0] = ~0ull;
+	wb[4 * w + 1] = ~0ull;
+	wb[4 * w + 2] = 0ull;
+	wb[4 * w + 3] = 0ull;
+}
+
+__device__ __forceinline__ V2 box_centre(double4 b) { return v2(0.5 * (b.x + b.z), 0.5 * (b.y + b.w)); }
+
+__global__ void k_bounds(const DevWorld W, const int32_t *shapes, int n, unsigned long long *wb)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int s = shapes[i];
+	const V2 c = box_centre(W.s_bb[s]);
+	const int w = W.s_world[s];
+	atomicMin(&wb[4 * w + 0], ord_of(c.x));
+	atomicMin(&wb[4 * w + 1], ord_of(c.y));
+	atomicMax(&wb[4 * w + 2], ord_of(c.x));
+	atomicMax(&wb[4 * w + 3], ord_of(c.y));
+}
+
+__device__ __forceinline__ uint64_t spread22(uint32_t x)
+{
+	uint64_t v = x & 0x3fffffull;
+	v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+	v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+	v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+	v = (v | (v << 2)) & 0x3333333333333333ull;
+	v = (v | (v << 1)) & 0x5555555555555555ull;
+	return v;
+}
+
+__device__ __forceinline__ uint32_t quantise(double x, double lo, double hi)
+{
+	const double scale = (hi > lo) ? (double)((1u << MORTON_BITS) - 1) / (hi - lo) : 0.0;
+	double q = (x - lo) * scale;
+	if (!(q > 0.0)) q = 0.0;
+	if (q > (double)((1u << MORTON_BITS) - 1)) q = (double)((1u << MORTON_BITS) - 1);
+	return (uint32_t)q;
+}
+
+__global__ void k_keys(const DevWorld W, const int32_t *shapes, int n, const unsigned long long *wb, uint64_t *keys,
+                       int32_t *vals, unsigned long long *orand)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t key = 0;
+	bool valid = i < n;
+	if (valid) {
+		const int s = shapes[i];
+		const int w = W.s_world[s];
+		const V2 c = box_centre(W.s_bb[s]);
+		const uint32_t qx = quantise(c.x, double_of(wb[4 * w + 0]), double_of(wb[4 * w + 2]));
+		const uint32_t qy = quantise(c.y, double_of(wb[4 * w + 1]), double_of(wb[4 * w + 3]));
+		key = ((uint64_t)w << WORLD_SHIFT) | spread22(qx) | (spread22(qy) << 1);
+		keys[i] = key;
+		vals[i] = s;
+	}
+	// OR / AND of all keys: which radix digits vary
+	unsigned long long o = valid ? key : 0ull, a = valid ? key : ~0ull;
+	for (int off = 32; off > 0; off >>= 1) {
+		o |= __shfl_xor(o, off);
+		a &= __shfl_xor(a, off);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		atomicOr(&orand[0], (unsigned long long)o);
+		atomicAnd(&orand[1], (unsigned long long)a);
+	}
+}
+
+// ---- radix sort ----
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, int n, int shift, uint32_t *hist,
+                                                      int nblocks)
+{
+	__shared__ uint32_t h[256];
+	const int tid = threadIdx.x;
+	h[tid] = 0;
+	__syncthreads();
+	for (int r = 0; r < RS_ITEMS; r++) {
+		const int i = blockIdx.x * RS_TILE + r * RS_THREADS + tid;
+		if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+	}
+	__syncthreads();
+	hist[tid * nblocks + blockIdx.x] = h[tid];
+}
+
+// exclusive scan of `total` counters in one workgroup of 1024 lanes
+__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t *hist, int total)
+{
+	__shared__ uint32_t sums[1024];
+	const int tid = threadIdx.x;
+	const int chunk = (total + 1023) / 1024;
+	const int lo = tid * chunk;
+	const int hi = min(lo + chunk, total);
+	uint32_t s = 0;
+	for (int i = lo; i < hi; i++) s += hist[i];
+	sums[tid] = s;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1) {
+		const uint32_t v = tid >= off ? sums[tid - off] : 0u;
+		__syncthreads();
+		sums[tid] += v;
+		__syncthreads();
+	}
+	uint32_t run = tid > 0 ? sums[tid - 1] : 0u;
+	for (int i = lo; i < hi; i++) {
+		const uint32_t c = hist[i];
+		hist[i] = run;
+		run += c;
+	}
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *kin, const int32_t *vin, uint64_t *kout,
+                                                         int32_t *vout, int n, int shift, const uint32_t *offsets,
+                                                         int nblocks)
+{
+	__shared__ uint32_t base[256];
+	__shared__ uint32_t run[256];
+	__shared__ uint32_t wcnt[RS_THREADS / 64][256];
+	const int tid = threadIdx.x;
+	const int wave = tid >> 6, lane = tid & 63;
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	base[tid] = offsets[tid * nblocks + blockIdx.x];
+	run[tid] = 0;
+	for (int r = 0; r < RS_ITEMS; r++) {
+		const int i = blockIdx.x * RS_TILE + r * RS_THREADS + tid;
+		const bool valid = i < n;
+		uint64_t k = 0;
+		int32_t v = 0;
+		uint32_t digit = 0;
+		if (valid) { k = kin[i]; v = vin[i]; digit = (uint32_t)((k >> shift) & 255); }
+		for (int q = 0; q < RS_THREADS / 64; q++) wcnt[q][tid] = 0;
+		__syncthreads();
+		// lanes of this wave holding the same digit
+		uint64_t peers = __ballot(valid);
+		for (int bit = 0; bit < 8; bit++) {
+			const bool b = (digit >> bit) & 1;
+			const uint64_t m = __ballot(b);
+			peers &= b ? m : ~m;
+		}
+		const uint32_t rank = (uint32_t)__popcll(peers & lt);
+		if (valid && rank == 0) wcnt[wave][digit] = (uint32_t)__popcll(peers);
+		__syncthreads();
+		if (valid) {
+			uint32_t pre = run[digit];
+			for (int q = 0; q < wave; q++) pre += wcnt[q][digit];
+			const uint32_t pos = base[digit] + pre + rank;
+			kout[pos] = k;
+			vout[pos] = v;
+		}
+		__syncthreads();
+		uint32_t add = 0;
+		for (int q = 0; q < RS_THREADS / 64; q++) add += wcnt[q][tid];
+		run[tid] += add;
+		__syncthreads();
+	}
+}
+
+// ---- Karras hierarchy ----
+
+__device__ __forceinline__ int kdelta(const uint64_t *k, int n, int i, int j)
+{
+	if (j < 0 || j >= n) return -1;
+	const uint64_t a = k[i], b = k[j];
+	if (a == b) return 64 + __clz((unsigned int)(i ^ j));
+	return __clzll((long long)(a ^ b));
+}
+
+__global__ void k_world_ranges(const uint64_t *keys, int n, int32_t *first, int32_t *last, int32_t *root)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int w = (int)(keys[i] >> WORLD_SHIFT);
+	const bool f = (i == 0) || (int)(keys[i - 1] >> WORLD_SHIFT) != w;
+	const bool l = (i == n - 1) || (int)(keys[i + 1] >> WORLD_SHIFT) != w;
+	if (f) first[w] = i;
+	if (l) last[w] = i;
+	if (f && l) root[w] = (n - 1) + i;
+}
+
+__global__ void k_karras(const uint64_t *keys, int n, int32_t *child, int32_t *parent, int32_t *range,
+                         const int32_t *first, const int32_t *last, int32_t *root)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n - 1) return;
+	const int d = kdelta(keys, n, i, i + 1) > kdelta(keys, n, i, i - 1) ? 1 : -1;
+	const int dmin = kdelta(keys, n, i, i - d);
+	int lmax = 2;
+	while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+	int l = 0;
+	for (int t = lmax >> 1; t >= 1; t >>= 1)
+		if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+	const int j = i + l * d;
+	const int dnode = kdelta(keys, n, i, j);
+	int s = 0, t = l;
+	do {
+		t = (t + 1) >> 1;
+		if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+	} while (t > 1);
+	const int gamma = i + s * d + min(d, 0);
+	const int lo = min(i, j), hi = max(i, j);
+	const int left = (lo == gamma) ? (n - 1) + gamma : gamma;
+	const int right = (hi == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
+	child[2 * i] = left;
+	child[2 * i + 1] = right;
+	parent[left] = i;
+	parent[right] = i;
+	range[2 * i] = lo;
+	range[2 * i + 1] = hi;
+	const int w = (int)(keys[lo] >> WORLD_SHIFT);
+	if (first[w] == lo && last[w] == hi) root[w] = i;
+}
+
+__device__ __forceinline__ double4 box_union(double4 a, double4 b)
+{
+	return make_double4(fminc(a.x, b.x), fminc(a.y, b.y), fmaxc(a.z, b.z), fmaxc(a.w, b.w));
+}
+
+__device__ __forceinline__ double4 load_box_coherent(const double4 *p)
+{
+	const volatile double *q = (const volatile double *)p;
+	return make_double4(q[0], q[1], q[2], q[3]);
+}
+
+__global__ void k_refit(const DevWorld W, int n, const int32_t *leaf_shape, const int32_t *child,
+                        const int32_t *parent, double4 *box, uint32_t *visits)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int node = (n - 1) + i;
+	box[node] = W.s_bb[leaf_shape[i]];
+	int p = parent[node];
+	while (p >= 0) {
+		__threadfence();
+		if (atomicAdd(&visits[p], 1u) == 0u) return;
+		__threadfence();
+		box[p] = box_union(load_box_coherent(&box[child[2 * p]]), load_box_coherent(&box[child[2 * p + 1]]));
+		node = p;
+		p = parent[p];
+	}
+}
+
+// ---- traversal ----
+
+__device__ __forceinline__ bool box_overlap(double4 a, double4 b)
+{
+	return a.x <= b.z && b.x <= a.z && a.y <= b.w && b.y <= a.w;
+}
+
+// QueryReject without the box test (cpSpaceStep.c:219-232), plus the
+// both-infinite-mass rejection of cpSpaceCollideShapes (:281).
+__device__ __forceinline__ bool pair_ok(const DevWorld &W, int s, int t)
+{
+	const int ba = W.s_body[s], bb = W.s_body[t];
+	if (ba == bb) return false;
+	const uint64_t ga = W.s_group[s], gb = W.s_group[t];
+	if ((ga != 0 && ga == gb) || (W.s_cats[s] & W.s_mask[t]) == 0 || (W.s_cats[t] & W.s_mask[s]) == 0) return false;
+	for (int k = W.nc_start[ba]; k < W.nc_start[ba + 1]; k++)
+		if (W.nc_list[k] == bb) return false;
+	if (W.b_m[ba] == __builtin_inf() && W.b_m[bb] == __builtin_inf()) return false;
+	return true;
+}
+
+__device__ __forceinline__ void emit_pair(const DevWorld &W, int s, int t, int2 *pairs, int cap, int32_t *count)
+{
+	// orientation of cpCollide (cpCollision.c:707-710); same types: lower index first
+	const int ts = W.s_type[s], tt = W.s_type[t];
+	int a = s, b = t;
+	if (ts > tt || (ts == tt && s > t)) { a = t; b = s; }
+	const uint64_t active = __ballot(1);
+	const int lane = __lane_id();
+	const int leader = __ffsll((long long)active) - 1;
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	const int rank = __popcll(active & lt);
+	int base = 0;
+	if (lane == leader) base = atomicAdd(count, __popcll(active));
+	base = __shfl(base, leader);
+	const int idx = base + rank;
+	if (idx < cap) pairs[idx] = make_int2(a, b);
+}
+
+__global__ void __launch_bounds__(256) k_pairs(const DevWorld W, int nd, const uint64_t *dkeys,
+                                               const int32_t *dshape, const int32_t *dchild,
+                                               const int32_t *drange, const double4 *dbox, const int32_t *droot,
+                                               int ns, const int32_t *sshape, const int32_t *schild,
+                                               const double4 *sbox, const int32_t *sroot, int2 *pairs, int cap,
+                                               int32_t *count)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= nd) return;
+	const int s = dshape[q];
+	const double4 bb = W.s_bb[s];
+	const int w = (int)(dkeys[q] >> WORLD_SHIFT);
+	int stack[TRAVERSAL_STACK];
+	int sp = 0;
+	// dynamic tree: leaves after q only
+	stack[sp++] = droot[w];
+	while (sp > 0) {
+		const int node = stack[--sp];
+		if (node >= nd - 1) {
+			const int p = node - (nd - 1);
+			if (p <= q) continue;
+			const int t = dshape[p];
+			if (box_overlap(bb, W.s_bb[t]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
+		} else {
+			if (drange[2 * node + 1] <= q) continue;
+			if (!box_overlap(bb, dbox[node])) continue;
+			if (sp + 2 > TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
+			stack[sp++] = dchild[2 * node];
+			stack[sp++] = dchild[2 * node + 1];
+		}
+	}
+	// static tree
+	if (ns > 0 && sroot[w] >= 0) {
+		stack[sp++] = sroot[w];
+		while (sp > 0) {
+			const int node = stack[--sp];
+			if (node >= ns - 1) {
+				const int t = sshape[node - (ns - 1)];
+				if (box_overlap(bb, W.s_bb[t]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
+			} else {
+				if (!box_overlap(bb, sbox[node])) continue;
+				if (sp + 2 > TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
+				stack[sp++] = schild[2 * node];
+				stack[sp++] = schild[2 * node + 1];
+			}
+		}
+	}
+}
+
+inline int blocks_for(int n, int t) { return (n + t - 1) / t; }
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+
+void radix_sort_reserve(RadixSortBuffers &b, int n)
+{
+	if (n <= b.cap) return;
+	radix_sort_free(b);
+	int cap = 1024;
+	while (cap < n) cap *= 2;
+	const int nblocks = blocks_for(cap, RS_TILE);
+	(void)hipMalloc(&b.keys_alt, sizeof(uint64_t) * cap);
+	(void)hipMalloc(&b.vals_alt, sizeof(int32_t) * cap);
+	(void)hipMalloc(&b.block_hist, sizeof(uint32_t) * 256 * nblocks);
+	(void)hipMalloc(&b.orand, sizeof(uint64_t) * 2);
+	b.cap = cap;
+}
+
+void radix_sort_free(RadixSortBuffers &b)
+{
+	if (b.keys_alt) (void)hipFree(b.keys_alt);
+	if (b.vals_alt) (void)hipFree(b.vals_alt);
+	if (b.block_hist) (void)hipFree(b.block_hist);
+	if (b.orand) (void)hipFree(b.orand);
+	b = RadixSortBuffers();
+}
+
+hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n, RadixSortBuffers &b, hipStream_t stream,
+                            KernelTimer *timer)
+{
+	if (n <= 1) return hipSuccess;
+	uint64_t orand[2];
+	HIP_TRY(hipMemcpyAsync(orand, b.orand, sizeof(orand), hipMemcpyDeviceToHost, stream));
+	HIP_TRY(hipStreamSynchronize(stream));
+	const uint64_t vary = orand[0] ^ orand[1];
+	const int nblocks = blocks_for(n, RS_TILE);
+	uint64_t *kin = keys, *kout = b.keys_alt;
+	int32_t *vin = vals, *vout = b.vals_alt;
+	for (int d = 0; d < 8; d++) {
+		if (((vary >> (8 * d)) & 0xFF) == 0) continue;
+		const int shift = 8 * d;
+		TIMED(timer, "radix_sort", stream, {
+			k_rs_hist<<<nblocks, RS_THREADS, 0, stream>>>(kin, n, shift, b.block_hist, nblocks);
+			k_rs_scan<<<1, 1024, 0, stream>>>(b.block_hist, 256 * nblocks);
+			k_rs_scatter<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, n, shift, b.block_hist, nblocks);
+		});
+		HIP_TRY(hipGetLastError());
+		uint64_t *tk = kin; kin = kout; kout = tk;
+		int32_t *tv = vin; vin = vout; vout = tv;
+	}
+	if (kin != keys) {
+		HIP_TRY(hipMemcpyAsync(keys, kin, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, stream));
+		HIP_TRY(hipMemcpyAsync(vals, vin, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, stream));
+	}
+	return hipSuccess;
+}
+
+void lbvh_reserve(Lbvh &t, int n, int n_worlds)
+{
+	if (n <= t.cap && t.world_root) return;
+	lbvh_free(t);
+	int cap = 1024;
+	while (cap < n) cap *= 2;
+	(void)hipMalloc(&t.keys, sizeof(uint64_t) * cap);
+	(void)hipMalloc(&t.leaf_shape, sizeof(int32_t) * cap);
+	(void)hipMalloc(&t.child, sizeof(int32_t) * 2 * cap);
+	(void)hipMalloc(&t.parent, sizeof(int32_t) * 2 * cap);
+	(void)hipMalloc(&t.range, sizeof(int32_t) * 2 * cap);
+	(void)hipMalloc(&t.box, sizeof(double4) * 2 * cap);
+	(void)hipMalloc(&t.visits, sizeof(uint32_t) * cap);
+	(void)hipMalloc(&t.world_root, sizeof(int32_t) * n_worlds);
+	(void)hipMalloc(&t.world_first, sizeof(int32_t) * n_worlds);
+	(void)hipMalloc(&t.world_last, sizeof(int32_t) * n_worlds);
+	t.cap = cap;
+}
+
+void lbvh_free(Lbvh &t)
+{
+	void *ptrs[] = {t.keys, t.leaf_shape, t.child, t.parent, t.range, t.box, t.visits, t.world_root,
+	                t.world_first, t.world_last};
+	for (void *p : ptrs) if (p) (void)hipFree(p);
+	t = Lbvh();
+}
+
+hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, int n_worlds,
+                      RadixSortBuffers &sortbuf, double4 *world_bounds, hipStream_t stream, KernelTimer *timer)
+{
+	t.n = n;
+	HIP_TRY(hipMemsetAsync(t.world_root, 0xFF, sizeof(int32_t) * n_worlds, stream));
+	if (n == 0) return hipSuccess;
+	unsigned long long *wb = (unsigned long long *)world_bounds;
+	radix_sort_reserve(sortbuf, n);
+	TIMED(timer, "bvh_keys", stream, {
+		k_bounds_reset<<<blocks_for(n_worlds, 256), 256, 0, stream>>>(wb, n_worlds);
+		k_bounds<<<blocks_for(n, 256), 256, 0, stream>>>(W, shapes, n, wb);
+		(void)hipMemsetAsync(sortbuf.orand, 0, sizeof(uint64_t), stream);
+		(void)hipMemsetAsync(sortbuf.orand + 1, 0xFF, sizeof(uint64_t), stream);
+		k_keys<<<blocks_for(n, 256), 256, 0, stream>>>(W, shapes, n, wb, t.keys, t.leaf_shape,
+		                                                (unsigned long long *)sortbuf.orand);
+	});
+	HIP_TRY(hipGetLastError());
+	HIP_TRY(radix_sort_pairs(t.keys, t.leaf_shape, n, sortbuf, stream, timer));
+	TIMED(timer, "bvh_build", stream, {
+		(void)hipMemsetAsync(t.parent, 0xFF, sizeof(int32_t) * 1, stream);
+		(void)hipMemsetAsync(t.visits, 0, sizeof(uint32_t) * (n > 1 ? n - 1 : 1), stream);
+		k_world_ranges<<<blocks_for(n, 256), 256, 0, stream>>>(t.keys, n, t.world_first, t.world_last,
+		                                                        t.world_root);
+		if (n > 1)
+			k_karras<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.child, t.parent, t.range,
+			                                                      t.world_first, t.world_last, t.world_root);
+		k_refit<<<blocks_for(n, 256), 256, 0, stream>>>(W, n, t.leaf_shape, t.child, t.parent, t.box, t.visits);
+	});
+	return hipGetLastError();
+}
+
+hipError_t broadphase_pairs(const Lbvh &dyn, const Lbvh &stat, const DevWorld &W, int2 *pairs, int cap,
+                            int32_t *count, hipStream_t stream, KernelTimer *timer)
+{
+	HIP_TRY(hipMemsetAsync(count, 0, sizeof(int32_t), stream));
+	if (dyn.n == 0) return hipSuccess;
+	TIMED(timer, "bvh_pairs", stream, {
+		k_pairs<<<blocks_for(dyn.n, 256), 256, 0, stream>>>(W, dyn.n, dyn.keys, dyn.leaf_shape, dyn.child, dyn.range,
+		                                                    dyn.box, dyn.world_root, stat.n, stat.leaf_shape,
+		                                                    stat.child, stat.box, stat.world_root, pairs, cap, count);
+	});
+	return hipGetLastError();
+}

@@ -1,0 +1,233 @@
+// cpgpu_device.h -- device-side math and state layout of the gfx950 step library.
+//
+// All kernels are compiled with -ffp-contract=off: every + - * / sqrt below is
+// one IEEE double operation, the same sequence the reference performs, so
+// results are bit-identical to the reference's strict build.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cpgpu.h"
+
+#define CPD __device__ __forceinline__
+
+// ---------------------------------------------------------------------------
+// vectors (include/chipmunk/cpVect.h, chipmunk_types.h:96-147)
+
+struct V2 { double x, y; };
+
+CPD V2 v2(double x, double y) { V2 r; r.x = x; r.y = y; return r; }
+CPD V2 vadd(V2 a, V2 b) { return v2(a.x + b.x, a.y + b.y); }
+CPD V2 vsub(V2 a, V2 b) { return v2(a.x - b.x, a.y - b.y); }
+CPD V2 vneg(V2 a) { return v2(-a.x, -a.y); }
+CPD V2 vmult(V2 a, double s) { return v2(a.x * s, a.y * s); }
+CPD double vdot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+CPD double vcross(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
+CPD V2 vperp(V2 a) { return v2(-a.y, a.x); }
+CPD V2 vrperp(V2 a) { return v2(a.y, -a.x); }
+CPD V2 vrotate(V2 a, V2 b) { return v2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+CPD double vlengthsq(V2 a) { return vdot(a, a); }
+CPD double vlength(V2 a) { return __builtin_sqrt(vdot(a, a)); }
+#define CP_DBL_MIN 2.2250738585072014e-308
+CPD V2 vnormalize(V2 a) { return vmult(a, 1.0 / (vlength(a) + CP_DBL_MIN)); }
+CPD V2 vlerp(V2 a, V2 b, double t) { return vadd(vmult(a, 1.0 - t), vmult(b, t)); }
+CPD bool veql(V2 a, V2 b) { return a.x == b.x && a.y == b.y; }
+CPD V2 vclamp(V2 v, double len) { return (vdot(v, v) > len * len) ? vmult(vnormalize(v), len) : v; }
+CPD double fmaxc(double a, double b) { return (a > b) ? a : b; }
+CPD double fminc(double a, double b) { return (a < b) ? a : b; }
+CPD double fclampc(double f, double lo, double hi) { return fminc(fmaxc(f, lo), hi); }
+CPD double fclamp01(double f) { return fmaxc(0.0, fminc(f, 1.0)); }
+
+// cpTransform {a, b, c, d, tx, ty} (cpTransform.h)
+struct XF { double a, b, c, d, tx, ty; };
+CPD V2 xpoint(const XF &t, V2 p) { return v2(t.a * p.x + t.c * p.y + t.tx, t.b * p.x + t.d * p.y + t.ty); }
+CPD V2 xvect(const XF &t, V2 v) { return v2(t.a * v.x + t.c * v.y, t.b * v.x + t.d * v.y); }
+
+// CP_HASH_PAIR (chipmunk_private.h:28-29), 64-bit cpHashValue
+#define CP_HASH_COEF 3344921057ull
+CPD uint64_t hash_pair(uint64_t a, uint64_t b) { return a * CP_HASH_COEF ^ b * CP_HASH_COEF; }
+
+// splitmix64 finalizer: a bijection, used for hashing keys and colouring priorities
+CPD uint64_t mix64(uint64_t z)
+{
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+
+CPD uint64_t pair_key(int32_t a, int32_t b)
+{
+	uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+	return ((uint64_t)lo << 32) | hi;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic sin/cos and exp (bit-identical on the host oracle): Cody-Waite
+// reduction in three FMA steps, Taylor series with 1/n! coefficients.
+
+CPD void cp_sincos(double x, double &s_out, double &c_out)
+{
+	if (!(__builtin_fabs(x) < 1e15)) { s_out = x - x; c_out = x - x; return; }
+	const double k = __builtin_rint(x * 0x1.45f306dc9c883p-1);
+	double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, x);
+	r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
+	r = __builtin_fma(-k, -0x1.f1976b7ed8fbcp-110, r);
+	const double z = r * r;
+	double ps = -0x1.2f49b46814157p-57;
+	ps = ps * z + 0x1.952c77030ad4ap-49;
+	ps = ps * z - 0x1.ae7f3e733b81fp-41;
+	ps = ps * z + 0x1.6124613a86d09p-33;
+	ps = ps * z - 0x1.ae64567f544e4p-26;
+	ps = ps * z + 0x1.71de3a556c734p-19;
+	ps = ps * z - 0x1.a01a01a01a01ap-13;
+	ps = ps * z + 0x1.1111111111111p-7;
+	ps = ps * z - 0x1.5555555555555p-3;
+	const double sn = r + r * (z * ps);
+	double qc = -0x1.6827863b97d97p-53;
+	qc = qc * z + 0x1.ae7f3e733b81fp-45;
+	qc = qc * z - 0x1.93974a8c07c9dp-37;
+	qc = qc * z + 0x1.1eed8eff8d898p-29;
+	qc = qc * z - 0x1.27e4fb7789f5cp-22;
+	qc = qc * z + 0x1.a01a01a01a01ap-16;
+	qc = qc * z - 0x1.6c16c16c16c17p-10;
+	qc = qc * z + 0x1.5555555555555p-5;
+	const double hz = 0.5 * z;
+	const double wc = 1.0 - hz;
+	const double cs = wc + (((1.0 - wc) - hz) + (z * z) * qc);
+	const long q = (long)k & 3;
+	if (q == 0) { s_out = sn; c_out = cs; }
+	else if (q == 1) { s_out = cs; c_out = -sn; }
+	else if (q == 2) { s_out = -sn; c_out = -cs; }
+	else { s_out = -cs; c_out = sn; }
+}
+
+CPD double cp_exp(double x)
+{
+	if (x != x) return x;
+	if (x > 709.782712893384) return __builtin_inf();
+	if (x < -745.2) return 0.0;
+	const double k = __builtin_rint(x * 0x1.71547652b82fep+0);
+	double r = __builtin_fma(-k, 0x1.62e42fefa39efp-1, x);
+	r = __builtin_fma(-k, 0x1.abc9e3b39803fp-56, r);
+	double p = 0x1.6124613a86d09p-33;
+	p = p * r + 0x1.1eed8eff8d898p-29;
+	p = p * r + 0x1.ae64567f544e4p-26;
+	p = p * r + 0x1.27e4fb7789f5cp-22;
+	p = p * r + 0x1.71de3a556c734p-19;
+	p = p * r + 0x1.a01a01a01a01ap-16;
+	p = p * r + 0x1.a01a01a01a01ap-13;
+	p = p * r + 0x1.6c16c16c16c17p-10;
+	p = p * r + 0x1.1111111111111p-7;
+	p = p * r + 0x1.5555555555555p-5;
+	p = p * r + 0x1.5555555555555p-3;
+	p = p * r + 0.5;
+	const double e = 1.0 + (r + (r * r) * p);
+	return __builtin_ldexp(e, (int)k);
+}
+
+// ---------------------------------------------------------------------------
+// device state
+
+// Per-world step parameters, refreshed by the host every step.
+struct WorldStep {
+	double dt, dt_coef, damping, bias_coef, slop;
+	double gx, gy;
+	int32_t iterations;
+	uint32_t stamp;
+	uint32_t persistence;
+	int32_t _pad;
+};
+
+// Velocity record of a body: the solver's working set (64 bytes).
+struct BodyVel {
+	double vx, vy, w, vbx, vby, wb, m_inv, i_inv;
+};
+
+// A contact of an arbiter (struct cpContact, chipmunk_structs.h:101-111).
+struct Contact {
+	double r1x, r1y, r2x, r2y;
+	double nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
+	uint64_t hash;
+};
+
+// An arbiter (struct cpArbiter, chipmunk_structs.h:124-145), device form.
+struct Arbiter {
+	uint64_t key;
+	int32_t sa, sb, ba, bb;
+	int32_t count, state;
+	uint32_t stamp;
+	int32_t world;
+	double nx, ny, e, u, svx, svy;
+	Contact c[2];
+};
+
+// Persistent pair record: the warm-start GJK id of a collided pair and the
+// arbiter it owns (index into the arbiter array, -1 for none).
+struct PairRec {
+	uint64_t key;
+	uint32_t id, id_stamp;
+	int32_t arb;
+	int32_t _pad;
+};
+
+// Joint (struct cpConstraint + joint subclasses), device form.
+struct Joint {
+	int32_t type, a, b, world;
+	double maxForce, errorBias, maxBias;
+	double P[8];
+	double r1x, r1y, r2x, r2y, nx, ny;
+	double k11, k12, k21, k22;
+	double bx, by, bias, nMass, jAx, jAy, jAcc, jnAcc, target_vrn, v_coef, iSum;
+	double bias_coef;
+};
+
+struct DevWorld {
+	int32_t n_worlds, n_bodies, n_shapes, n_joints;
+	// bodies
+	const int32_t *b_type;
+	const int32_t *b_world;
+	const double *b_m;           // mass (INFINITY marks static/kinematic)
+	double2 *b_p;                // position
+	double *b_a;                 // angle
+	double2 *b_f;
+	double *b_t;
+	double2 *b_cog;
+	XF *b_T;                     // transform
+	BodyVel *b_vel;
+	// shapes
+	const int32_t *s_type, *s_body, *s_world;
+	const uint64_t *s_hashid;
+	const double *s_e, *s_u;
+	const double2 *s_sv;
+	const uint64_t *s_group;
+	const uint32_t *s_cats, *s_mask;
+	const double *s_r;
+	const double *s_geom;        // [n][10] body-local geometry
+	const int32_t *s_pstart, *s_pcount;
+	const double2 *pv_local, *pn_local;
+	double2 *pv_world, *pn_world;
+	double *s_wgeom;             // [n][6]: circle tc; segment ta, tb, tn
+	double4 *s_bb;               // l, b, r, t
+	double4 *s_fat;              // dynamic-tree leaf boxes
+	uint32_t *s_rebuild;         // step of the last leaf rebuild
+	const uint8_t *s_static;     // shape belongs to a static body
+	// no-collide constraint partners (CSR over bodies)
+	const int32_t *nc_start, *nc_list;
+	// per-world step parameters
+	const WorldStep *ws;
+	// joints
+	Joint *joints;
+	// device error word
+	int32_t *err;
+};
+
+#define CPGPU_ERR_PAIRS_OVERFLOW 1
+#define CPGPU_ERR_ARBITERS_OVERFLOW 2
+#define CPGPU_ERR_RECORDS_OVERFLOW 4
+#define CPGPU_ERR_TRAVERSAL_STACK 8
+#define CPGPU_ERR_BARRIER_TIMEOUT 16
+#define CPGPU_ERR_TABLE_FULL 32
+#define CPGPU_ERR_EPA_HULL 64
+
+CPD void raise_err(int32_t *err, int32_t code) { atomicOr(err, code); }

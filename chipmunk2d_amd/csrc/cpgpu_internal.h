@@ -1,0 +1,102 @@
+// cpgpu_internal.h -- host-side launchers shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cpgpu_device.h"
+
+// ---- per-kernel device timing with HIP events on the launch stream ----
+struct KernelTimer {
+	bool enabled = false;
+	struct Rec { const char *name; hipEvent_t a, b; };
+	std::vector<Rec> pending;
+	std::vector<hipEvent_t> pool;
+	std::map<std::string, std::pair<double, long>> acc;   // name -> (ms, launches)
+	hipEvent_t take() {
+		if (pool.empty()) { hipEvent_t e; (void)hipEventCreate(&e); return e; }
+		hipEvent_t e = pool.back(); pool.pop_back(); return e;
+	}
+	void begin(const char *name, hipStream_t s) {
+		if (!enabled) return;
+		Rec r; r.name = name; r.a = take(); r.b = take();
+		(void)hipEventRecord(r.a, s);
+		pending.push_back(r);
+	}
+	void end(hipStream_t s) {
+		if (!enabled || pending.empty()) return;
+		(void)hipEventRecord(pending.back().b, s);
+	}
+	// after a stream sync: fold finished records into the totals
+	void collect() {
+		for (auto &r : pending) {
+			float ms = 0.0f;
+			if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+				auto &x = acc[r.name];
+				x.first += ms; x.second += 1;
+			}
+			pool.push_back(r.a); pool.push_back(r.b);
+		}
+		pending.clear();
+	}
+	~KernelTimer() {
+		for (auto &r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+		for (auto e : pool) (void)hipEventDestroy(e);
+	}
+};
+
+#define TIMED(timer, name, stream, ...) do { \
+	if (timer) (timer)->begin(name, stream); \
+	__VA_ARGS__; \
+	if (timer) (timer)->end(stream); \
+} while (0)
+
+// ---- radix sort (broadphase.hip) ----
+struct RadixSortBuffers {
+	uint64_t *keys_alt = nullptr;
+	int32_t *vals_alt = nullptr;
+	uint32_t *block_hist = nullptr;   // [256][n_blocks]
+	uint64_t *orand = nullptr;        // [2]: OR and AND of all keys
+	int cap = 0;
+};
+void radix_sort_reserve(RadixSortBuffers &b, int n);
+void radix_sort_free(RadixSortBuffers &b);
+// Sorts (keys, vals) ascending by key, stable.  Reads back the varying-digit
+// mask (one device sync).  The result ends in keys/vals.
+hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n, RadixSortBuffers &b, hipStream_t stream,
+                            KernelTimer *timer);
+
+// ---- LBVH (broadphase.hip) ----
+struct Lbvh {
+	int n = 0;                      // leaves
+	int cap = 0;
+	uint64_t *keys = nullptr;       // sorted keys
+	int32_t *leaf_shape = nullptr;  // sorted leaf -> shape index
+	int32_t *child = nullptr;       // [n-1][2] children (>= n-1: leaf n-1+i)
+	int32_t *parent = nullptr;      // [2n-1]
+	int32_t *range = nullptr;       // [n-1][2] leaf range of internal nodes
+	double4 *box = nullptr;         // [2n-1]
+	uint32_t *visits = nullptr;     // [n-1] refit counters
+	int32_t *world_root = nullptr;  // [n_worlds] root node per world (-1: none)
+	int32_t *world_first = nullptr; // [n_worlds]
+	int32_t *world_last = nullptr;  // [n_worlds]
+};
+void lbvh_reserve(Lbvh &t, int n, int n_worlds);
+void lbvh_free(Lbvh &t);
+// Build over the shapes in `shapes` (n entries): keys from bb centres
+// quantised in per-world bounds, sorted, Karras hierarchy, boxes refitted.
+hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, int n_worlds,
+                      RadixSortBuffers &sortbuf, double4 *world_bounds, hipStream_t stream, KernelTimer *timer);
+
+// Candidate pairs of the exact QueryReject predicate: dynamic x dynamic
+// (each pair once) and dynamic x static.  Pairs are written oriented for
+// cpCollide (lower shape type first, then lower index) into pairs[cap][2];
+// the total count (possibly > cap) goes to *count.
+hipError_t broadphase_pairs(const Lbvh &dyn, const Lbvh &stat, const DevWorld &W, int2 *pairs, int cap,
+                            int32_t *count, hipStream_t stream, KernelTimer *timer);
+
+#define HIP_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)

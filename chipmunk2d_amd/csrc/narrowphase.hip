@@ -1,0 +1,233 @@
+// narrowphase.hip -- collision and arbiter persistence, one candidate pair per lane.
+//
+// The reference keeps two caches between steps: the BBTree pairs, which hold
+// the GJK warm-start id of every collided pair until one of the two leaves is
+// rebuilt (src/cpBBTree.c:69-78,206-223,498-517), and the cachedArbiters
+// hash set (src/cpSpaceStep.c:248-251, src/cpHashSet.c), which holds the
+// arbiters with their contacts' accumulated impulses for
+// collisionPersistence steps (cpSpaceArbiterSetFilter, cpSpaceStep.c:292-325).
+// Here both live in one table of pair records keyed by the shape pair:
+//   {key, GJK id, step the id was written, arbiter reference}.
+// Each step the records of the previous step are hashed (open addressing,
+// linear probing), every candidate pair looks its record up, collides,
+// and appends a new record; arbiters touched this step go to the solver's
+// array, untouched ones within persistence to a cached array.
+#include "cpgpu_internal.h"
+#include "narrowphase.h"
+#include "cpgpu_step.h"
+
+namespace {
+
+constexpr uint64_t EMPTY_KEY = ~0ull;
+constexpr int32_t CACHED_TAG = 0x40000000;
+
+__global__ void k_table_clear(uint64_t *tkeys, int32_t *tvals, int tcap)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= tcap) return;
+	tkeys[i] = EMPTY_KEY;
+	tvals[i] = -1;
+}
+
+__global__ void k_table_insert(const PairRec *recs, const int32_t *n_recs_ptr, uint64_t *tkeys, int32_t *tvals,
+                               int tmask, uint8_t *seen, int32_t *err)
+{
+	const int n = *n_recs_ptr;
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+		seen[r] = 0;
+		const uint64_t key = recs[r].key;
+		uint64_t h = mix64(key) & (uint64_t)tmask;
+		bool done = false;
+		for (int probe = 0; probe <= tmask; probe++) {
+			const unsigned long long prev = atomicCAS((unsigned long long *)&tkeys[h], (unsigned long long)EMPTY_KEY,
+			                                          (unsigned long long)key);
+			if (prev == EMPTY_KEY || prev == key) { tvals[h] = r; done = true; break; }
+			h = (h + 1) & (uint64_t)tmask;
+		}
+		if (!done) raise_err(err, CPGPU_ERR_TABLE_FULL);
+	}
+}
+
+__device__ __forceinline__ int table_find(const uint64_t *tkeys, const int32_t *tvals, int tmask, uint64_t key)
+{
+	uint64_t h = mix64(key) & (uint64_t)tmask;
+	for (int probe = 0; probe <= tmask; probe++) {
+		const uint64_t k = tkeys[h];
+		if (k == key) return tvals[h];
+		if (k == EMPTY_KEY) return -1;
+		h = (h + 1) & (uint64_t)tmask;
+	}
+	return -1;
+}
+
+// wave-aggregated append: one atomic per wave for the lanes that call it
+__device__ __forceinline__ int append_index(int32_t *counter)
+{
+	const uint64_t active = __ballot(1);
+	const int lane = __lane_id();
+	const int leader = __ffsll((long long)active) - 1;
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	int base = 0;
+	if (lane == leader) base = atomicAdd(counter, __popcll(active));
+	base = __shfl(base, leader);
+	return base + __popcll(active & lt);
+}
+
+__device__ __forceinline__ const Arbiter *arb_ref(const StepBuffers &B, int32_t ref)
+{
+	return (ref & CACHED_TAG) ? &B.prev_cached[ref & ~CACHED_TAG] : &B.prev_act[ref];
+}
+
+// Keep an untouched arbiter for collisionPersistence steps as CACHED
+// (cpSpaceArbiterSetFilter, cpSpaceStep.c:292-325).  Returns the new
+// reference or -1 when evicted.
+__device__ __forceinline__ int32_t carry_arbiter(const StepBuffers &B, const DevWorld &W, int32_t ref)
+{
+	const Arbiter *old = arb_ref(B, ref);
+	const WorldStep ws = W.ws[old->world];
+	const uint32_t ticks = ws.stamp - old->stamp;
+	if (ticks >= ws.persistence) return -1;
+	const int m = append_index(B.n_cached);
+	if (m >= B.cap_cached) { raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); return -1; }
+	Arbiter a = *old;
+	a.state = CPGPU_ARB_CACHED;
+	B.cached[m] = a;
+	return m | CACHED_TAG;
+}
+
+__device__ __forceinline__ void write_record(const StepBuffers &B, const DevWorld &W, uint64_t key, uint32_t id,
+                                             uint32_t id_stamp, int32_t arb)
+{
+	const int k = append_index(B.n_recs);
+	if (k >= B.cap_recs) { raise_err(W.err, CPGPU_ERR_RECORDS_OVERFLOW); return; }
+	PairRec r;
+	r.key = key; r.id = id; r.id_stamp = id_stamp; r.arb = arb; r._pad = 0;
+	B.recs[k] = r;
+}
+
+__global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const StepBuffers B, const int2 *pairs,
+                                                     int n_pairs)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n_pairs) return;
+	const int2 pr = pairs[i];
+	const int ia = pr.x, ib = pr.y;
+	const uint64_t key = pair_key(ia, ib);
+	const int world = W.s_world[ia];
+	const WorldStep ws = W.ws[world];
+
+	// the record of the previous step: warm-start id and arbiter
+	uint32_t id = 0;
+	int32_t old = -1;
+	const int r = table_find(B.tkeys, B.tvals, B.tmask, key);
+	if (r >= 0) {
+		const PairRec rec = B.prev_recs[r];
+		B.seen[r] = 1;
+		if (rec.id_stamp >= W.s_rebuild[ia] && rec.id_stamp >= W.s_rebuild[ib]) id = rec.id;
+		old = rec.arb;
+	}
+
+	const ShapeView sa = load_shape(W, ia);
+	const ShapeView sb = load_shape(W, ib);
+	CollInfo info;
+	collide(sa, sb, id, info, W.err);
+
+	int32_t arb_out = -1;
+	if (info.count > 0) {
+		// cpArbiterUpdate (cpArbiter.c:356-414) on a new or persisting arbiter
+		const int m = append_index(B.n_act);
+		if (m >= B.cap_act) {
+			raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW);
+		} else {
+			const Arbiter *po = old >= 0 ? arb_ref(B, old) : nullptr;
+			Arbiter a;
+			a.key = key;
+			a.sa = ia; a.sb = ib;
+			a.ba = W.s_body[ia]; a.bb = W.s_body[ib];
+			a.world = world;
+			a.stamp = ws.stamp;
+			// state: an arbiter solved last step was reset to NORMAL at the start of
+			// this step (cpSpaceStep.c:351-360); a cached one becomes FIRST_COLLISION
+			if (po == nullptr) a.state = CPGPU_ARB_FIRST_COLLISION;
+			else a.state = (po->stamp == ws.stamp - 1u) ? CPGPU_ARB_NORMAL : CPGPU_ARB_FIRST_COLLISION;
+			const double2 pA = W.b_p[a.ba], pB = W.b_p[a.bb];
+			const int old_count = po ? po->count : 0;
+			for (int c = 0; c < 2; c++) {
+				Contact con;
+				if (c < info.count) {
+					con.r1x = info.p1[c].x - pA.x; con.r1y = info.p1[c].y - pA.y;
+					con.r2x = info.p2[c].x - pB.x; con.r2y = info.p2[c].y - pB.y;
+					con.hash = info.hash[c];
+					con.jnAcc = 0.0; con.jtAcc = 0.0;
+					for (int j = 0; j < old_count; j++) {
+						if (con.hash == po->c[j].hash) { con.jnAcc = po->c[j].jnAcc; con.jtAcc = po->c[j].jtAcc; }
+					}
+				} else {
+					con.r1x = con.r1y = con.r2x = con.r2y = 0.0;
+					con.hash = 0; con.jnAcc = 0.0; con.jtAcc = 0.0;
+				}
+				con.nMass = con.tMass = con.bounce = con.jBias = con.bias = 0.0;
+				a.c[c] = con;
+			}
+			a.count = info.count;
+			a.nx = info.n.x; a.ny = info.n.y;
+			a.e = W.s_e[ia] * W.s_e[ib];
+			a.u = W.s_u[ia] * W.s_u[ib];
+			const double2 svA = W.s_sv[ia], svB = W.s_sv[ib];
+			const V2 surface_vr = v2(svB.x - svA.x, svB.y - svA.y);
+			const V2 svr = vsub(surface_vr, vmult(info.n, vdot(surface_vr, info.n)));
+			a.svx = svr.x; a.svy = svr.y;
+			B.act[m] = a;
+			arb_out = m;
+		}
+	} else if (old >= 0) {
+		arb_out = carry_arbiter(B, W, old);
+	}
+	write_record(B, W, key, info.id, ws.stamp, arb_out);
+}
+
+// Records of the previous step whose pair was not a candidate this step.
+__global__ void k_carry(const DevWorld W, const StepBuffers B)
+{
+	const int n = *B.n_prev_recs;
+	for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+		if (B.seen[r]) continue;
+		const PairRec rec = B.prev_recs[r];
+		const int32_t ia = (int32_t)(rec.key >> 32), ib = (int32_t)(rec.key & 0xffffffffu);
+		const bool id_ok = rec.id_stamp >= W.s_rebuild[ia] && rec.id_stamp >= W.s_rebuild[ib];
+		int32_t arb = -1;
+		if (rec.arb >= 0) arb = carry_arbiter(B, W, rec.arb);
+		if (arb < 0 && !id_ok) continue;
+		write_record(B, W, rec.key, rec.id, rec.id_stamp, arb);
+	}
+}
+
+inline int blocks_for(int n, int t) { return (n + t - 1) / t; }
+
+} // namespace
+
+hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 *pairs, const int32_t *n_pairs_dev,
+                            int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer)
+{
+	HIP_TRY(hipMemsetAsync(B.n_recs, 0, sizeof(int32_t), stream));
+	HIP_TRY(hipMemsetAsync(B.n_act, 0, sizeof(int32_t), stream));
+	HIP_TRY(hipMemsetAsync(B.n_cached, 0, sizeof(int32_t), stream));
+	TIMED(timer, "pair_table", stream, {
+		k_table_clear<<<blocks_for(B.tmask + 1, 256), 256, 0, stream>>>(B.tkeys, B.tvals, B.tmask + 1);
+		if (n_prev_recs_host > 0)
+			k_table_insert<<<blocks_for(n_prev_recs_host, 256), 256, 0, stream>>>(B.prev_recs, B.n_prev_recs, B.tkeys,
+			                                                                      B.tvals, B.tmask, B.seen, W.err);
+	});
+	HIP_TRY(hipGetLastError());
+	if (n_pairs_host > 0) {
+		TIMED(timer, "narrowphase", stream,
+		      k_narrowphase<<<blocks_for(n_pairs_host, 256), 256, 0, stream>>>(W, B, pairs, n_pairs_host));
+		HIP_TRY(hipGetLastError());
+	}
+	if (n_prev_recs_host > 0) {
+		TIMED(timer, "arbiter_carry", stream,
+		      k_carry<<<blocks_for(n_prev_recs_host, 256), 256, 0, stream>>>(W, B));
+		HIP_TRY(hipGetLastError());
+	}
+	return hipSuccess;
+}

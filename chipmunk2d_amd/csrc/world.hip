@@ -1,0 +1,902 @@
+// world.hip -- the C ABI of include/cpgpu.h: device state, the step driver,
+// the integration and shape-caching kernels, and readback.
+//
+// Step order follows cpSpaceStep (src/cpSpaceStep.c:335-445):
+//   integrate positions (cpBodyUpdatePosition, cpBody.c:511-522)
+//   cache shape geometry (cpShapeCacheBB, cpShape.c:210-220) and update the
+//     dynamic leaves' fattened boxes (cpBBTree.c:498-517)
+//   broadphase (broadphase.hip), narrowphase + arbiter persistence
+//     (narrowphase.hip)
+//   solver: colouring, pre-steps, velocity integration, cached impulses,
+//     iterations (solver.hip)
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "cpgpu_step.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+	g_last_error = msg;
+	return code;
+}
+
+#define HIP_OR_FAIL(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
+	return fail(CPGPU_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } while (0)
+
+inline int blocks_for(long n, int t) { return (int)((n + t - 1) / t); }
+
+template <typename T>
+hipError_t dalloc(T *&p, size_t n)
+{
+	p = nullptr;
+	return hipMalloc((void **)&p, sizeof(T) * (n ? n : 1));
+}
+
+template <typename T>
+hipError_t upload(T *&p, const T *src, size_t n)
+{
+	hipError_t e = dalloc(p, n);
+	if (e != hipSuccess) return e;
+	if (n && src) return hipMemcpy(p, src, sizeof(T) * n, hipMemcpyHostToDevice);
+	if (n) return hipMemset(p, 0, sizeof(T) * n);
+	return hipSuccess;
+}
+
+// ---- kernels: integration and shape caching ----
+
+// cpBodyUpdatePosition + SetTransform (cpBody.c:511-522, 347-357)
+__global__ void k_update_positions(const DevWorld W, const int32_t *bodies, int n)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= n) return;
+	const int b = bodies[q];
+	const double dt = W.ws[W.b_world[b]].dt;
+	BodyVel bv = W.b_vel[b];
+	const double2 p0 = W.b_p[b];
+	const V2 p = vadd(v2(p0.x, p0.y), vmult(vadd(v2(bv.vx, bv.vy), v2(bv.vbx, bv.vby)), dt));
+	const double a = W.b_a[b] + (bv.w + bv.wb) * dt;
+	double sn, cs;
+	cp_sincos(a, sn, cs);
+	const double2 c = W.b_cog[b];
+	XF T;
+	T.a = cs; T.b = sn; T.c = -sn; T.d = cs;
+	T.tx = p.x - (c.x * cs - c.y * sn);
+	T.ty = p.y - (c.x * sn + c.y * cs);
+	W.b_p[b] = make_double2(p.x, p.y);
+	W.b_a[b] = a;
+	W.b_T[b] = T;
+	bv.vbx = 0.0; bv.vby = 0.0; bv.wb = 0.0;
+	W.b_vel[b] = bv;
+}
+
+// cacheData of the three shape classes (cpShape.c:291-296, 378-405; cpPolyShape.c:39-64)
+__device__ void cache_shape(const DevWorld &W, int s)
+{
+	const XF T = W.b_T[W.s_body[s]];
+	const int type = W.s_type[s];
+	const double r = W.s_r[s];
+	const double *g = W.s_geom + 10 * (size_t)s;
+	double *wg = W.s_wgeom + 6 * (size_t)s;
+	double4 bb;
+	if (type == CPGPU_SHAPE_CIRCLE) {
+		const V2 c = xpoint(T, v2(g[0], g[1]));
+		wg[0] = c.x; wg[1] = c.y;
+		bb = make_double4(c.x - r, c.y - r, c.x + r, c.y + r);
+	} else if (type == CPGPU_SHAPE_SEGMENT) {
+		const V2 ta = xpoint(T, v2(g[0], g[1]));
+		const V2 tb = xpoint(T, v2(g[2], g[3]));
+		const V2 tn = xvect(T, v2(g[4], g[5]));
+		wg[0] = ta.x; wg[1] = ta.y; wg[2] = tb.x; wg[3] = tb.y; wg[4] = tn.x; wg[5] = tn.y;
+		double l, rr, bt, tp;
+		if (ta.x < tb.x) { l = ta.x; rr = tb.x; } else { l = tb.x; rr = ta.x; }
+		if (ta.y < tb.y) { bt = ta.y; tp = tb.y; } else { bt = tb.y; tp = ta.y; }
+		bb = make_double4(l - r, bt - r, rr + r, tp + r);
+	} else {
+		const int st = W.s_pstart[s], cnt = W.s_pcount[s];
+		double l = __builtin_inf(), rr = -__builtin_inf(), bt = __builtin_inf(), tp = -__builtin_inf();
+		for (int i = 0; i < cnt; i++) {
+			const double2 lv = W.pv_local[st + i], ln = W.pn_local[st + i];
+			const V2 v = xpoint(T, v2(lv.x, lv.y));
+			const V2 nn = xvect(T, v2(ln.x, ln.y));
+			W.pv_world[st + i] = make_double2(v.x, v.y);
+			W.pn_world[st + i] = make_double2(nn.x, nn.y);
+			l = fminc(l, v.x); rr = fmaxc(rr, v.x); bt = fminc(bt, v.y); tp = fmaxc(tp, v.y);
+		}
+		bb = make_double4(l - r, bt - r, rr + r, tp + r);
+	}
+	W.s_bb[s] = bb;
+}
+
+// GetBB (cpBBTree.c:82-98) with the body velocity (cpSpace.c:106)
+__device__ double4 fat_box(const DevWorld &W, int s, double4 bb)
+{
+	const BodyVel bv = W.b_vel[W.s_body[s]];
+	const V2 v = vmult(v2(bv.vx, bv.vy), 0.1);
+	const double x = (bb.z - bb.x) * 0.1;
+	const double y = (bb.w - bb.y) * 0.1;
+	return make_double4(bb.x + fminc(-x, v.x), bb.y + fminc(-y, v.y), bb.z + fmaxc(x, v.x), bb.w + fmaxc(y, v.y));
+}
+
+__global__ void k_cache_shapes(const DevWorld W, const int32_t *shapes, int n, int leaf_update, int init_fat)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= n) return;
+	const int s = shapes[q];
+	cache_shape(W, s);
+	if (init_fat) {
+		W.s_fat[s] = fat_box(W, s, W.s_bb[s]);
+	} else if (leaf_update) {
+		// LeafUpdate (cpBBTree.c:498-517): rebuild the leaf box when the shape escaped it
+		const double4 f = W.s_fat[s], b = W.s_bb[s];
+		const bool contains = f.x <= b.x && f.z >= b.z && f.y <= b.y && f.w >= b.w;
+		if (!contains) {
+			W.s_fat[s] = fat_box(W, s, b);
+			W.s_rebuild[s] = W.ws[W.s_world[s]].stamp;
+		}
+	}
+}
+
+// ---- host-side deterministic colouring of the joints (same rule as the device arbiters) ----
+
+uint64_t mix64_host(uint64_t z)
+{
+	z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+	z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+	return z ^ (z >> 31);
+}
+
+int colour_host(int n_bodies, const std::vector<int> &ba, const std::vector<int> &bb, const std::vector<uint64_t> &prio,
+                std::vector<int> &colour)
+{
+	const size_t n = ba.size();
+	colour.assign(n, -1);
+	if (n == 0) return 0;
+	std::vector<uint64_t> claim(n_bodies, ~0ull);
+	std::vector<int> taken(n_bodies, -1);
+	size_t left = n;
+	int c = 0;
+	while (left > 0) {
+		for (int r = 0; r < COLOUR_SUBROUNDS; r++) {
+			for (size_t k = 0; k < n; k++) {
+				if (ba[k] >= 0) claim[ba[k]] = ~0ull;
+				if (bb[k] >= 0) claim[bb[k]] = ~0ull;
+			}
+			auto eligible = [&](size_t k) {
+				return colour[k] < 0 && !(ba[k] >= 0 && taken[ba[k]] == c) && !(bb[k] >= 0 && taken[bb[k]] == c);
+			};
+			for (size_t k = 0; k < n; k++) {
+				if (!eligible(k)) continue;
+				if (ba[k] >= 0) claim[ba[k]] = std::min(claim[ba[k]], prio[k]);
+				if (bb[k] >= 0) claim[bb[k]] = std::min(claim[bb[k]], prio[k]);
+			}
+			std::vector<size_t> won;
+			for (size_t k = 0; k < n; k++) {
+				if (!eligible(k)) continue;
+				if ((ba[k] < 0 || claim[ba[k]] == prio[k]) && (bb[k] < 0 || claim[bb[k]] == prio[k])) won.push_back(k);
+			}
+			for (size_t k : won) {
+				colour[k] = c;
+				if (ba[k] >= 0) taken[ba[k]] = c;
+				if (bb[k] >= 0) taken[bb[k]] = c;
+				left--;
+			}
+		}
+		c++;
+	}
+	return c;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+
+struct cpgpuWorld {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int precision = CPGPU_FP64;
+	int nw = 0, nb = 0, ns = 0, nc = 0, npv = 0;
+	std::vector<cpgpuSpaceParams> sp;
+	std::vector<int32_t> body_type;
+	DevWorld W{};
+	// owned device arrays (freed at destroy)
+	std::vector<void *> owned;
+	// index lists
+	int32_t *d_dyn_bodies = nullptr;
+	int n_dyn_bodies = 0;
+	int32_t *d_dyn_shapes = nullptr;
+	int n_dyn_shapes = 0;
+	int32_t *d_static_shapes = nullptr;
+	int n_static_shapes = 0;
+	int32_t *d_all_shapes = nullptr;
+	WorldStep *d_ws = nullptr;
+	WorldStep *h_ws = nullptr;          // pinned
+	double *d_joint_bias_coef = nullptr;
+	double last_joint_dt = -1.0;
+	std::vector<double> joint_error_bias;
+	// broadphase
+	Lbvh dyn_tree, static_tree;
+	RadixSortBuffers sortbuf;
+	double4 *d_world_bounds = nullptr;
+	int2 *d_pairs = nullptr;
+	int pair_cap = 0;
+	int32_t *d_counters = nullptr;      // [0] n_pairs
+	// persistence (double-buffered)
+	PairRec *recs[2] = {nullptr, nullptr};
+	int rec_cap[2] = {0, 0};
+	Arbiter *act[2] = {nullptr, nullptr};
+	int act_cap[2] = {0, 0};
+	Arbiter *cached[2] = {nullptr, nullptr};
+	int cached_cap[2] = {0, 0};
+	int32_t *d_step_counts = nullptr;   // [2][3]: n_recs, n_act, n_cached per slot
+	int cur = 0;
+	uint64_t *tkeys = nullptr;
+	int32_t *tvals = nullptr;
+	int tcap = 0;
+	uint8_t *seen = nullptr;
+	int seen_cap = 0;
+	// solver
+	int32_t *colour = nullptr, *order = nullptr;
+	int colour_cap = 0;
+	int32_t *col_off = nullptr, *col_cursor = nullptr, *n_colours = nullptr, *left = nullptr;
+	unsigned long long *claim = nullptr;
+	int32_t *taken = nullptr;
+	int32_t *d_jorder = nullptr, *d_jcol_off = nullptr;
+	int n_jcolours = 0;
+	std::vector<int32_t> h_jcolour;
+	GridBarrier *bar = nullptr;
+	int solver_grid = 0;
+	int max_iterations = 0;
+	// host mirror of counters after the last sync
+	int32_t *h_counts = nullptr;        // pinned: [0..5] step counts, [6] n_pairs, [7] err, [8] n_colours
+	int64_t steps = 0;
+	int64_t last_pairs = 0;
+	bool pending = false;
+	KernelTimer timer;
+};
+
+namespace {
+
+template <typename T>
+hipError_t own_upload(cpgpuWorld *w, T *&p, const T *src, size_t n)
+{
+	hipError_t e = upload(p, src, n);
+	if (e == hipSuccess) w->owned.push_back((void *)p);
+	return e;
+}
+
+template <typename T>
+hipError_t own_alloc(cpgpuWorld *w, T *&p, size_t n)
+{
+	hipError_t e = dalloc(p, n);
+	if (e == hipSuccess) w->owned.push_back((void *)p);
+	return e;
+}
+
+int pow2_at_least(long n)
+{
+	long c = 64;
+	while (c < n) c *= 2;
+	return (int)c;
+}
+
+template <typename T>
+hipError_t regrow(T *&p, int &cap, long need)
+{
+	if (need <= cap && p) return hipSuccess;
+	if (p) (void)hipFree(p);
+	long c = std::max<long>(need + need / 2, 1024);
+	cap = (int)c;
+	return dalloc(p, (size_t)c);
+}
+
+int validate(const cpgpuWorldDesc *d)
+{
+	if (!d) return fail(CPGPU_ERR_INVALID, "null description");
+	if (d->n_worlds < 1) return fail(CPGPU_ERR_INVALID, "n_worlds must be >= 1");
+	if (!d->space || !d->world_body_start || !d->world_shape_start)
+		return fail(CPGPU_ERR_INVALID, "missing space parameters or world ranges");
+	if (d->world_body_start[0] != 0 || d->world_body_start[d->n_worlds] != d->n_bodies)
+		return fail(CPGPU_ERR_INVALID, "world_body_start must run from 0 to n_bodies");
+	if (d->world_shape_start[0] != 0 || d->world_shape_start[d->n_worlds] != d->n_shapes)
+		return fail(CPGPU_ERR_INVALID, "world_shape_start must run from 0 to n_shapes");
+	if (d->n_constraints > 0 && (!d->world_constraint_start || d->world_constraint_start[0] != 0 ||
+	                             d->world_constraint_start[d->n_worlds] != d->n_constraints))
+		return fail(CPGPU_ERR_INVALID, "world_constraint_start must run from 0 to n_constraints");
+	for (int w = 0; w < d->n_worlds; w++) {
+		if (d->world_body_start[w + 1] < d->world_body_start[w] || d->world_shape_start[w + 1] < d->world_shape_start[w])
+			return fail(CPGPU_ERR_INVALID, "world ranges must be non-decreasing");
+		if (d->space[w].iterations < 0) return fail(CPGPU_ERR_INVALID, "iterations must be >= 0");
+	}
+	if (d->n_worlds >= (1 << 20)) return fail(CPGPU_ERR_INVALID, "at most 2^20 - 1 worlds");
+	std::vector<int> body_world(d->n_bodies);
+	for (int w = 0; w < d->n_worlds; w++)
+		for (int b = d->world_body_start[w]; b < d->world_body_start[w + 1]; b++) body_world[b] = w;
+	for (int b = 0; b < d->n_bodies; b++) {
+		const int t = d->body_type[b];
+		if (t < 0 || t > 2) return fail(CPGPU_ERR_INVALID, "body type out of range");
+	}
+	for (int s = 0; s < d->n_shapes; s++) {
+		const int t = d->shape_type[s];
+		if (t < 0 || t > 2) return fail(CPGPU_ERR_INVALID, "shape type out of range");
+		const int b = d->shape_body[s];
+		if (b < 0 || b >= d->n_bodies) return fail(CPGPU_ERR_INVALID, "shape body index out of range");
+		int w = 0;
+		while (!(d->world_shape_start[w] <= s && s < d->world_shape_start[w + 1])) w++;
+		if (body_world[b] != w) return fail(CPGPU_ERR_INVALID, "shape and its body belong to different worlds");
+		if (t == CPGPU_SHAPE_POLY) {
+			const int c = d->shape_poly_count[s], st = d->shape_poly_start[s];
+			if (c < 1 || c > CPGPU_MAX_POLY_VERTS)
+				return fail(CPGPU_ERR_INVALID, "polygon vertex count must be in [1, CPGPU_MAX_POLY_VERTS]");
+			if (st < 0 || st + c > d->n_poly_verts) return fail(CPGPU_ERR_INVALID, "polygon vertex range out of range");
+		}
+	}
+	for (int c = 0; c < d->n_constraints; c++) {
+		const int t = d->cons_type[c];
+		if (t < 0 || t > 3) return fail(CPGPU_ERR_INVALID, "constraint type out of range");
+		if (d->cons_a[c] < 0 || d->cons_a[c] >= d->n_bodies || d->cons_b[c] < 0 || d->cons_b[c] >= d->n_bodies)
+			return fail(CPGPU_ERR_INVALID, "constraint body index out of range");
+		if (body_world[d->cons_a[c]] != body_world[d->cons_b[c]])
+			return fail(CPGPU_ERR_INVALID, "constraint joins bodies of different worlds");
+	}
+	return CPGPU_OK;
+}
+
+int check_device_error(cpgpuWorld *w)
+{
+	const int32_t e = w->h_counts[7];
+	if (!e) return CPGPU_OK;
+	std::string m = "device error during step:";
+	if (e & CPGPU_ERR_PAIRS_OVERFLOW) m += " pair buffer overflow;";
+	if (e & CPGPU_ERR_ARBITERS_OVERFLOW) m += " arbiter buffer or colour overflow;";
+	if (e & CPGPU_ERR_RECORDS_OVERFLOW) m += " pair-record overflow;";
+	if (e & CPGPU_ERR_TRAVERSAL_STACK) m += " BVH traversal stack overflow;";
+	if (e & CPGPU_ERR_BARRIER_TIMEOUT) m += " solver grid barrier timeout;";
+	if (e & CPGPU_ERR_TABLE_FULL) m += " pair table full;";
+	if (e & CPGPU_ERR_EPA_HULL) m += " EPA hull overflow;";
+	return fail(CPGPU_ERR_DEVICE, m);
+}
+
+// queue a copy of the step counters and the error word into pinned memory
+void queue_counters(cpgpuWorld *w)
+{
+	(void)hipMemcpyAsync(w->h_counts, w->d_step_counts, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
+	(void)hipMemcpyAsync(w->h_counts + 6, w->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
+	(void)hipMemcpyAsync(w->h_counts + 7, w->W.err, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
+	(void)hipMemcpyAsync(w->h_counts + 8, w->n_colours, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
+}
+
+int sync_world(cpgpuWorld *w)
+{
+	queue_counters(w);
+	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+	w->timer.collect();
+	w->pending = false;
+	return check_device_error(w);
+}
+
+} // namespace
+
+extern "C" {
+
+const char *cpgpuGetLastError(void) { return g_last_error.c_str(); }
+const char *cpgpuVersion(void) { return "cpgpu 0.1 (gfx950; Chipmunk2D 7.0.3 cpSpaceStep semantics)"; }
+
+int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **out)
+{
+	if (!out) return fail(CPGPU_ERR_INVALID, "null output pointer");
+	*out = nullptr;
+	int rc = validate(d);
+	if (rc != CPGPU_OK) return rc;
+	if (precision != CPGPU_FP64) return fail(CPGPU_ERR_INVALID, "only CPGPU_FP64 is implemented");
+	int ndev = 0;
+	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CPGPU_ERR_HIP, "no HIP device available");
+
+	cpgpuWorld *w = new cpgpuWorld();
+	w->precision = precision;
+	HIP_OR_FAIL(hipGetDevice(&w->device));
+	HIP_OR_FAIL(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+	w->nw = d->n_worlds; w->nb = d->n_bodies; w->ns = d->n_shapes; w->nc = d->n_constraints;
+	w->npv = d->n_poly_verts;
+	w->sp.assign(d->space, d->space + d->n_worlds);
+	w->body_type.assign(d->body_type, d->body_type + d->n_bodies);
+	DevWorld &W = w->W;
+	W.n_worlds = w->nw; W.n_bodies = w->nb; W.n_shapes = w->ns; W.n_joints = w->nc;
+	const int nb = w->nb, ns = w->ns, nc = w->nc;
+
+	// bodies
+	std::vector<int32_t> body_world(nb);
+	for (int k = 0; k < w->nw; k++)
+		for (int b = d->world_body_start[k]; b < d->world_body_start[k + 1]; b++) body_world[b] = k;
+	std::vector<BodyVel> vel(nb);
+	std::vector<XF> T(nb);
+	for (int b = 0; b < nb; b++) {
+		vel[b] = {d->body_v[2 * b], d->body_v[2 * b + 1], d->body_w[b], d->body_v_bias[2 * b],
+		          d->body_v_bias[2 * b + 1], d->body_w_bias[b], d->body_m_inv[b], d->body_i_inv[b]};
+		const double *t = d->body_transform + 6 * b;
+		T[b] = {t[0], t[1], t[2], t[3], t[4], t[5]};
+	}
+	int32_t *p_btype, *p_bworld;
+	double *p_bm;
+	HIP_OR_FAIL(own_upload(w, p_btype, d->body_type, nb)); W.b_type = p_btype;
+	HIP_OR_FAIL(own_upload(w, p_bworld, body_world.data(), nb)); W.b_world = p_bworld;
+	HIP_OR_FAIL(own_upload(w, p_bm, d->body_m, nb)); W.b_m = p_bm;
+	HIP_OR_FAIL(own_upload(w, W.b_p, (const double2 *)d->body_p, nb));
+	HIP_OR_FAIL(own_upload(w, W.b_a, d->body_a, nb));
+	HIP_OR_FAIL(own_upload(w, W.b_f, (const double2 *)d->body_f, nb));
+	HIP_OR_FAIL(own_upload(w, W.b_t, d->body_t, nb));
+	HIP_OR_FAIL(own_upload(w, W.b_cog, (const double2 *)d->body_cog, nb));
+	HIP_OR_FAIL(own_upload(w, W.b_T, T.data(), nb));
+	HIP_OR_FAIL(own_upload(w, W.b_vel, vel.data(), nb));
+
+	// shapes
+	std::vector<int32_t> shape_world(ns);
+	std::vector<uint8_t> shape_static(ns);
+	std::vector<int32_t> dyn_shapes, static_shapes, all_shapes(ns);
+	for (int s = 0; s < ns; s++) {
+		shape_world[s] = body_world[d->shape_body[s]];
+		shape_static[s] = d->body_type[d->shape_body[s]] == CPGPU_BODY_STATIC;
+		(shape_static[s] ? static_shapes : dyn_shapes).push_back(s);
+		all_shapes[s] = s;
+	}
+	{
+		int32_t *pt, *pb, *pw, *pps, *ppc;
+		uint64_t *ph, *pg;
+		double *pe, *pu, *pr, *pgeom;
+		double2 *psv;
+		uint32_t *pcat, *pmask;
+		uint8_t *pst;
+		HIP_OR_FAIL(own_upload(w, pt, d->shape_type, ns)); W.s_type = pt;
+		HIP_OR_FAIL(own_upload(w, pb, d->shape_body, ns)); W.s_body = pb;
+		HIP_OR_FAIL(own_upload(w, pw, shape_world.data(), ns)); W.s_world = pw;
+		HIP_OR_FAIL(own_upload(w, ph, d->shape_hashid, ns)); W.s_hashid = ph;
+		HIP_OR_FAIL(own_upload(w, pe, d->shape_e, ns)); W.s_e = pe;
+		HIP_OR_FAIL(own_upload(w, pu, d->shape_u, ns)); W.s_u = pu;
+		HIP_OR_FAIL(own_upload(w, psv, (const double2 *)d->shape_surface_v, ns)); W.s_sv = psv;
+		HIP_OR_FAIL(own_upload(w, pg, d->shape_group, ns)); W.s_group = pg;
+		HIP_OR_FAIL(own_upload(w, pcat, d->shape_categories, ns)); W.s_cats = pcat;
+		HIP_OR_FAIL(own_upload(w, pmask, d->shape_mask, ns)); W.s_mask = pmask;
+		HIP_OR_FAIL(own_upload(w, pr, d->shape_radius, ns)); W.s_r = pr;
+		HIP_OR_FAIL(own_upload(w, pgeom, d->shape_geom, 10 * (size_t)ns)); W.s_geom = pgeom;
+		std::vector<int32_t> pstart(ns, 0), pcount(ns, 0);
+		for (int s = 0; s < ns; s++)
+			if (d->shape_type[s] == CPGPU_SHAPE_POLY) { pstart[s] = d->shape_poly_start[s]; pcount[s] = d->shape_poly_count[s]; }
+		HIP_OR_FAIL(own_upload(w, pps, pstart.data(), ns)); W.s_pstart = pps;
+		HIP_OR_FAIL(own_upload(w, ppc, pcount.data(), ns)); W.s_pcount = ppc;
+		HIP_OR_FAIL(own_upload(w, pst, shape_static.data(), ns)); W.s_static = pst;
+		double2 *pvl, *pnl;
+		HIP_OR_FAIL(own_upload(w, pvl, (const double2 *)d->poly_verts, w->npv)); W.pv_local = pvl;
+		HIP_OR_FAIL(own_upload(w, pnl, (const double2 *)d->poly_normals, w->npv)); W.pn_local = pnl;
+		HIP_OR_FAIL(own_alloc(w, W.pv_world, w->npv));
+		HIP_OR_FAIL(own_alloc(w, W.pn_world, w->npv));
+		HIP_OR_FAIL(own_alloc(w, W.s_wgeom, 6 * (size_t)ns));
+		HIP_OR_FAIL(own_alloc(w, W.s_bb, ns));
+		HIP_OR_FAIL(own_alloc(w, W.s_fat, ns));
+		HIP_OR_FAIL(own_upload(w, W.s_rebuild, (const uint32_t *)nullptr, ns));
+	}
+	w->n_dyn_shapes = (int)dyn_shapes.size();
+	w->n_static_shapes = (int)static_shapes.size();
+	HIP_OR_FAIL(own_upload(w, w->d_dyn_shapes, dyn_shapes.data(), dyn_shapes.size()));
+	HIP_OR_FAIL(own_upload(w, w->d_static_shapes, static_shapes.data(), static_shapes.size()));
+	HIP_OR_FAIL(own_upload(w, w->d_all_shapes, all_shapes.data(), all_shapes.size()));
+	std::vector<int32_t> dyn_bodies;
+	for (int b = 0; b < nb; b++) if (d->body_type[b] != CPGPU_BODY_STATIC) dyn_bodies.push_back(b);
+	w->n_dyn_bodies = (int)dyn_bodies.size();
+	HIP_OR_FAIL(own_upload(w, w->d_dyn_bodies, dyn_bodies.data(), dyn_bodies.size()));
+
+	// no-collide partners (QueryRejectConstraint, cpSpaceStep.c:204-216)
+	{
+		std::vector<std::vector<int32_t>> part(nb);
+		for (int c = 0; c < nc; c++) {
+			if (d->cons_collide_bodies[c]) continue;
+			part[d->cons_a[c]].push_back(d->cons_b[c]);
+			part[d->cons_b[c]].push_back(d->cons_a[c]);
+		}
+		std::vector<int32_t> start(nb + 1, 0), list;
+		for (int b = 0; b < nb; b++) {
+			start[b] = (int32_t)list.size();
+			list.insert(list.end(), part[b].begin(), part[b].end());
+		}
+		start[nb] = (int32_t)list.size();
+		int32_t *pst, *pl;
+		HIP_OR_FAIL(own_upload(w, pst, start.data(), nb + 1)); W.nc_start = pst;
+		HIP_OR_FAIL(own_upload(w, pl, list.data(), list.size())); W.nc_list = pl;
+	}
+
+	// joints, coloured once
+	{
+		std::vector<Joint> J(nc);
+		std::vector<int> ba(nc), bbv(nc);
+		std::vector<uint64_t> prio(nc);
+		w->joint_error_bias.resize(nc);
+		for (int c = 0; c < nc; c++) {
+			Joint &j = J[c];
+			memset(&j, 0, sizeof(j));
+			j.type = d->cons_type[c]; j.a = d->cons_a[c]; j.b = d->cons_b[c]; j.world = body_world[j.a];
+			j.maxForce = d->cons_max_force[c]; j.errorBias = d->cons_error_bias[c]; j.maxBias = d->cons_max_bias[c];
+			memcpy(j.P, d->cons_params + 8 * c, 8 * sizeof(double));
+			const double j0 = d->cons_jacc ? d->cons_jacc[2 * c] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * c + 1] : 0.0;
+			j.jAx = j0; j.jAy = j1; j.jAcc = j0; j.jnAcc = j0;
+			w->joint_error_bias[c] = j.errorBias;
+			ba[c] = d->body_type[j.a] == CPGPU_BODY_DYNAMIC ? j.a : -1;
+			bbv[c] = d->body_type[j.b] == CPGPU_BODY_DYNAMIC ? j.b : -1;
+			prio[c] = mix64_host((uint64_t)(uint32_t)c | 0x8000000000000000ull);
+		}
+		w->n_jcolours = colour_host(nb, ba, bbv, prio, w->h_jcolour);
+		std::vector<int32_t> jorder(nc), joff(w->n_jcolours + 1, 0);
+		for (int c = 0; c < nc; c++) joff[w->h_jcolour[c] + 1]++;
+		for (int q = 0; q < w->n_jcolours; q++) joff[q + 1] += joff[q];
+		std::vector<int32_t> cursor(joff.begin(), joff.end() - 1);
+		for (int c = 0; c < nc; c++) jorder[cursor[w->h_jcolour[c]]++] = c;
+		HIP_OR_FAIL(own_upload(w, W.joints, J.data(), nc));
+		HIP_OR_FAIL(own_upload(w, w->d_jorder, jorder.data(), nc));
+		HIP_OR_FAIL(own_upload(w, w->d_jcol_off, joff.data(), joff.size()));
+	}
+
+	// per-world step parameters, counters, error word
+	HIP_OR_FAIL(own_alloc(w, w->d_ws, w->nw));
+	W.ws = w->d_ws;
+	HIP_OR_FAIL(hipHostMalloc((void **)&w->h_ws, sizeof(WorldStep) * w->nw, hipHostMallocDefault));
+	HIP_OR_FAIL(hipHostMalloc((void **)&w->h_counts, sizeof(int32_t) * 16, hipHostMallocDefault));
+	memset(w->h_counts, 0, sizeof(int32_t) * 16);
+	HIP_OR_FAIL(own_upload(w, W.err, (const int32_t *)nullptr, 1));
+	HIP_OR_FAIL(own_upload(w, w->d_counters, (const int32_t *)nullptr, 4));
+	HIP_OR_FAIL(own_upload(w, w->d_step_counts, (const int32_t *)nullptr, 6));
+	HIP_OR_FAIL(own_alloc(w, w->d_world_bounds, w->nw));
+	HIP_OR_FAIL(own_alloc(w, w->d_joint_bias_coef, nc));
+	HIP_OR_FAIL(own_upload(w, w->bar, (const GridBarrier *)nullptr, 1));
+	HIP_OR_FAIL(own_upload(w, w->col_off, (const int32_t *)nullptr, CPGPU_MAX_COLOURS + 1));
+	HIP_OR_FAIL(own_upload(w, w->col_cursor, (const int32_t *)nullptr, CPGPU_MAX_COLOURS));
+	HIP_OR_FAIL(own_upload(w, w->n_colours, (const int32_t *)nullptr, 1));
+	HIP_OR_FAIL(own_upload(w, w->left, (const int32_t *)nullptr, 1));
+	HIP_OR_FAIL(own_alloc(w, w->claim, 2 * (size_t)nb));
+	HIP_OR_FAIL(own_alloc(w, w->taken, nb));
+	w->max_iterations = 0;
+	for (auto &s : w->sp) w->max_iterations = std::max(w->max_iterations, (int)s.iterations);
+
+	// initial cache of every shape with its body's transform (cpSpaceAddShape, cpSpace.c:433)
+	// and the dynamic leaves' first fattened boxes (cpBBTreeInsert, cpBBTree.c:605-616)
+	if (ns > 0)
+		k_cache_shapes<<<blocks_for(ns, 256), 256, 0, w->stream>>>(W, w->d_all_shapes, ns, 0, 0);
+	if (w->n_dyn_shapes > 0)
+		k_cache_shapes<<<blocks_for(w->n_dyn_shapes, 256), 256, 0, w->stream>>>(W, w->d_dyn_shapes, w->n_dyn_shapes, 0, 1);
+	HIP_OR_FAIL(hipGetLastError());
+	// the static tree, built once
+	if (w->n_static_shapes > 0) {
+		lbvh_reserve(w->static_tree, w->n_static_shapes, w->nw);
+		HIP_OR_FAIL(lbvh_build(w->static_tree, W, w->d_static_shapes, w->n_static_shapes, w->nw, w->sortbuf,
+		                       w->d_world_bounds, w->stream, nullptr));
+	}
+	lbvh_reserve(w->dyn_tree, std::max(w->n_dyn_shapes, 1), w->nw);
+	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+
+	// initial capacities
+	const long base = std::max<long>(4L * w->n_dyn_shapes, 4096);
+	HIP_OR_FAIL(regrow(w->d_pairs, w->pair_cap, base));
+	for (int k = 0; k < 2; k++) {
+		HIP_OR_FAIL(regrow(w->recs[k], w->rec_cap[k], 2 * base));
+		HIP_OR_FAIL(regrow(w->act[k], w->act_cap[k], base));
+		HIP_OR_FAIL(regrow(w->cached[k], w->cached_cap[k], base));
+	}
+	w->solver_grid = std::min(solver_max_grid(), 1024);
+	if (w->solver_grid < 1) w->solver_grid = 1;
+	*out = w;
+	return CPGPU_OK;
+}
+
+void cpgpuWorldDestroy(cpgpuWorld *w)
+{
+	if (!w) return;
+	(void)hipStreamSynchronize(w->stream);
+	for (void *p : w->owned) (void)hipFree(p);
+	lbvh_free(w->dyn_tree);
+	lbvh_free(w->static_tree);
+	radix_sort_free(w->sortbuf);
+	void *extra[] = {w->d_pairs, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
+	                 w->tkeys, w->tvals, w->seen, w->colour, w->order};
+	for (void *p : extra) if (p) (void)hipFree(p);
+	if (w->h_ws) (void)hipHostFree(w->h_ws);
+	if (w->h_counts) (void)hipHostFree(w->h_counts);
+	(void)hipStreamDestroy(w->stream);
+	delete w;
+}
+
+int cpgpuWorldStep(cpgpuWorld *w, double dt)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (dt == 0.0) return CPGPU_OK;   // cpSpaceStep.c:339
+	DevWorld &W = w->W;
+	KernelTimer *timer = w->timer.enabled ? &w->timer : nullptr;
+
+	// per-world scalars of this step (cpSpaceStep.c:341-344, 383-384, 399, 407); pow on the host
+	for (int k = 0; k < w->nw; k++) {
+		cpgpuSpaceParams &s = w->sp[k];
+		s.stamp++;
+		const double prev_dt = s.curr_dt;
+		s.curr_dt = dt;
+		WorldStep &ws = w->h_ws[k];
+		ws.dt = dt;
+		ws.dt_coef = (prev_dt == 0.0 ? 0.0 : dt / prev_dt);
+		ws.damping = pow(s.damping, dt);
+		ws.bias_coef = 1.0 - pow(s.collision_bias, dt);
+		ws.slop = s.collision_slop;
+		ws.gx = s.gravity_x; ws.gy = s.gravity_y;
+		ws.iterations = s.iterations;
+		ws.stamp = s.stamp;
+		ws.persistence = s.collision_persistence;
+	}
+	HIP_OR_FAIL(hipMemcpyAsync(w->d_ws, w->h_ws, sizeof(WorldStep) * w->nw, hipMemcpyHostToDevice, w->stream));
+	if (w->nc > 0 && dt != w->last_joint_dt) {
+		// bias_coef (chipmunk_private.h:264-268) per joint, libm pow on the host
+		std::vector<double> bc(w->nc);
+		for (int c = 0; c < w->nc; c++) bc[c] = 1.0 - pow(w->joint_error_bias[c], dt);
+		HIP_OR_FAIL(hipMemcpy2DAsync(&W.joints[0].bias_coef, sizeof(Joint), bc.data(), sizeof(double), sizeof(double),
+		                             w->nc, hipMemcpyHostToDevice, w->stream));
+		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+		w->last_joint_dt = dt;
+	}
+
+	// integrate positions, cache the dynamic shapes, update leaf boxes
+	if (w->n_dyn_bodies > 0)
+		TIMED(timer, "integrate_positions", w->stream,
+		      k_update_positions<<<blocks_for(w->n_dyn_bodies, 256), 256, 0, w->stream>>>(W, w->d_dyn_bodies,
+		                                                                                 w->n_dyn_bodies));
+	if (w->n_dyn_shapes > 0)
+		TIMED(timer, "cache_shapes", w->stream,
+		      k_cache_shapes<<<blocks_for(w->n_dyn_shapes, 256), 256, 0, w->stream>>>(W, w->d_dyn_shapes,
+		                                                                             w->n_dyn_shapes, 1, 0));
+	HIP_OR_FAIL(hipGetLastError());
+
+	// broadphase (one sync inside the radix sort)
+	HIP_OR_FAIL(lbvh_build(w->dyn_tree, W, w->d_dyn_shapes, w->n_dyn_shapes, w->nw, w->sortbuf, w->d_world_bounds,
+	                       w->stream, timer));
+	HIP_OR_FAIL(broadphase_pairs(w->dyn_tree, w->static_tree, W, w->d_pairs, w->pair_cap, w->d_counters, w->stream,
+	                             timer));
+	int rc = sync_world(w);
+	if (rc != CPGPU_OK) return rc;
+	int n_pairs = w->h_counts[6];
+	if (n_pairs > w->pair_cap) {
+		HIP_OR_FAIL(regrow(w->d_pairs, w->pair_cap, n_pairs));
+		HIP_OR_FAIL(broadphase_pairs(w->dyn_tree, w->static_tree, W, w->d_pairs, w->pair_cap, w->d_counters,
+		                             w->stream, timer));
+		rc = sync_world(w);
+		if (rc != CPGPU_OK) return rc;
+		n_pairs = w->h_counts[6];
+	}
+	w->last_pairs = n_pairs;
+
+	// capacities of this step's outputs from last step's counts
+	const int prev = w->cur ^ 1, cur = w->cur;
+	const int n_prev_recs = w->steps > 0 ? w->h_counts[3 * prev + 0] : 0;
+	const int n_prev_act = w->steps > 0 ? w->h_counts[3 * prev + 1] : 0;
+	const int n_prev_cached = w->steps > 0 ? w->h_counts[3 * prev + 2] : 0;
+	HIP_OR_FAIL(regrow(w->recs[cur], w->rec_cap[cur], (long)n_pairs + n_prev_recs + 1));
+	HIP_OR_FAIL(regrow(w->act[cur], w->act_cap[cur], (long)n_pairs + 1));
+	HIP_OR_FAIL(regrow(w->cached[cur], w->cached_cap[cur], (long)n_prev_act + n_prev_cached + 1));
+	const int tcap = pow2_at_least(2L * n_prev_recs + 64);
+	if (tcap > w->tcap) {
+		if (w->tkeys) (void)hipFree(w->tkeys);
+		if (w->tvals) (void)hipFree(w->tvals);
+		HIP_OR_FAIL(dalloc(w->tkeys, tcap));
+		HIP_OR_FAIL(dalloc(w->tvals, tcap));
+		w->tcap = tcap;
+	}
+	HIP_OR_FAIL(regrow(w->seen, w->seen_cap, (long)n_prev_recs + 1));
+	if (w->act_cap[cur] > w->colour_cap) {
+		if (w->colour) (void)hipFree(w->colour);
+		if (w->order) (void)hipFree(w->order);
+		w->colour_cap = w->act_cap[cur];
+		HIP_OR_FAIL(dalloc(w->colour, w->colour_cap));
+		HIP_OR_FAIL(dalloc(w->order, w->colour_cap));
+	}
+	if (w->steps == 0) HIP_OR_FAIL(hipMemsetAsync(w->d_step_counts + 3 * prev, 0, 3 * sizeof(int32_t), w->stream));
+
+	StepBuffers B;
+	B.tkeys = w->tkeys; B.tvals = w->tvals; B.tmask = w->tcap - 1;
+	B.prev_recs = w->recs[prev]; B.n_prev_recs = w->d_step_counts + 3 * prev + 0;
+	B.seen = w->seen;
+	B.prev_act = w->act[prev]; B.prev_cached = w->cached[prev];
+	B.recs = w->recs[cur]; B.n_recs = w->d_step_counts + 3 * cur + 0; B.cap_recs = w->rec_cap[cur];
+	B.act = w->act[cur]; B.n_act = w->d_step_counts + 3 * cur + 1; B.cap_act = w->act_cap[cur];
+	B.cached = w->cached[cur]; B.n_cached = w->d_step_counts + 3 * cur + 2; B.cap_cached = w->cached_cap[cur];
+	HIP_OR_FAIL(narrowphase_step(W, B, w->d_pairs, w->d_counters, n_pairs, n_prev_recs, w->stream, timer));
+
+	SolveBuffers S;
+	S.act = w->act[cur]; S.n_act = B.n_act;
+	S.colour = w->colour; S.order = w->order;
+	S.col_off = w->col_off; S.col_cursor = w->col_cursor; S.n_colours = w->n_colours; S.left = w->left;
+	S.claim = w->claim; S.taken = w->taken;
+	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
+	S.bar = w->bar;
+	S.max_iterations = w->max_iterations;
+	S.n_bodies = w->nb;
+	S.dyn_bodies = w->d_dyn_bodies; S.n_dyn_bodies = w->n_dyn_bodies;
+	HIP_OR_FAIL(solver_step(W, S, w->solver_grid, w->stream, timer));
+
+	w->cur ^= 1;
+	w->steps++;
+	w->pending = true;
+	return CPGPU_OK;
+}
+
+int cpgpuWorldStepN(cpgpuWorld *w, double dt, int32_t n_steps)
+{
+	for (int k = 0; k < n_steps; k++) {
+		const int rc = cpgpuWorldStep(w, dt);
+		if (rc != CPGPU_OK) return rc;
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldSync(cpgpuWorld *w)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	return sync_world(w);
+}
+
+int cpgpuWorldReadBodies(cpgpuWorld *w, double *p, double *v, double *a, double *wv, double *v_bias, double *w_bias,
+                         double *transform)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int nb = w->nb;
+	if (p) HIP_OR_FAIL(hipMemcpy(p, w->W.b_p, sizeof(double2) * nb, hipMemcpyDeviceToHost));
+	if (a) HIP_OR_FAIL(hipMemcpy(a, w->W.b_a, sizeof(double) * nb, hipMemcpyDeviceToHost));
+	if (transform) HIP_OR_FAIL(hipMemcpy(transform, w->W.b_T, sizeof(XF) * nb, hipMemcpyDeviceToHost));
+	if (v || wv || v_bias || w_bias) {
+		std::vector<BodyVel> vel(nb);
+		HIP_OR_FAIL(hipMemcpy(vel.data(), w->W.b_vel, sizeof(BodyVel) * nb, hipMemcpyDeviceToHost));
+		for (int b = 0; b < nb; b++) {
+			if (v) { v[2 * b] = vel[b].vx; v[2 * b + 1] = vel[b].vy; }
+			if (wv) wv[b] = vel[b].w;
+			if (v_bias) { v_bias[2 * b] = vel[b].vbx; v_bias[2 * b + 1] = vel[b].vby; }
+			if (w_bias) w_bias[b] = vel[b].wb;
+		}
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldWriteBodies(cpgpuWorld *w, const double *v, const double *wv, const double *f, const double *t)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int nb = w->nb;
+	if (v || wv) {
+		std::vector<BodyVel> vel(nb);
+		HIP_OR_FAIL(hipMemcpy(vel.data(), w->W.b_vel, sizeof(BodyVel) * nb, hipMemcpyDeviceToHost));
+		for (int b = 0; b < nb; b++) {
+			if (v) { vel[b].vx = v[2 * b]; vel[b].vy = v[2 * b + 1]; }
+			if (wv) vel[b].w = wv[b];
+		}
+		HIP_OR_FAIL(hipMemcpy(w->W.b_vel, vel.data(), sizeof(BodyVel) * nb, hipMemcpyHostToDevice));
+	}
+	if (f) HIP_OR_FAIL(hipMemcpy(w->W.b_f, f, sizeof(double2) * nb, hipMemcpyHostToDevice));
+	if (t) HIP_OR_FAIL(hipMemcpy(w->W.b_t, t, sizeof(double) * nb, hipMemcpyHostToDevice));
+	return CPGPU_OK;
+}
+
+int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int last = w->cur ^ 1;
+	memset(out, 0, sizeof(*out));
+	out->candidate_pairs = w->last_pairs;
+	if (w->steps > 0) {
+		out->arbiters = w->h_counts[3 * last + 1];
+		out->cached_arbiters = w->h_counts[3 * last + 1] + w->h_counts[3 * last + 2];
+		out->arbiter_colours = w->h_counts[8];
+		// contacts of the solved arbiters
+		std::vector<Arbiter> arbs(out->arbiters);
+		if (out->arbiters)
+			HIP_OR_FAIL(hipMemcpy(arbs.data(), w->act[last], sizeof(Arbiter) * out->arbiters, hipMemcpyDeviceToHost));
+		for (auto &a : arbs) out->contacts += a.count;
+	}
+	out->constraint_colours = w->n_jcolours;
+	out->steps = w->steps;
+	return CPGPU_OK;
+}
+
+int cpgpuWorldReadArbiters(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *shapes, int32_t *colour,
+                           int32_t *state, int32_t *count, double *normal, double *contact, uint64_t *hash)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int last = w->cur ^ 1;
+	const int n = w->steps > 0 ? w->h_counts[3 * last + 1] : 0;
+	if (n_out) *n_out = n;
+	if (n == 0 || cap <= 0) return CPGPU_OK;
+	std::vector<Arbiter> arbs(n);
+	std::vector<int32_t> col(n);
+	HIP_OR_FAIL(hipMemcpy(arbs.data(), w->act[last], sizeof(Arbiter) * n, hipMemcpyDeviceToHost));
+	HIP_OR_FAIL(hipMemcpy(col.data(), w->colour, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+	std::vector<int> idx(n);
+	for (int k = 0; k < n; k++) idx[k] = k;
+	std::sort(idx.begin(), idx.end(), [&](int x, int y) {
+		if (col[x] != col[y]) return col[x] < col[y];
+		return arbs[x].key < arbs[y].key;
+	});
+	for (int q = 0; q < n && q < cap; q++) {
+		const Arbiter &a = arbs[idx[q]];
+		if (shapes) { shapes[2 * q] = a.sa; shapes[2 * q + 1] = a.sb; }
+		if (colour) colour[q] = col[idx[q]];
+		if (state) state[q] = a.state;
+		if (count) count[q] = a.count;
+		if (normal) { normal[2 * q] = a.nx; normal[2 * q + 1] = a.ny; }
+		for (int i = 0; i < 2; i++) {
+			const Contact &c = a.c[i];
+			const bool valid = i < a.count;
+			if (contact) {
+				double *o = contact + 24 * q + 12 * i;
+				const double vals[12] = {c.r1x, c.r1y, c.r2x, c.r2y, c.nMass, c.tMass, c.bounce,
+				                         c.jnAcc, c.jtAcc, c.jBias, c.bias, 0.0};
+				for (int k = 0; k < 12; k++) o[k] = valid ? vals[k] : 0.0;
+			}
+			if (hash) hash[2 * q + i] = valid ? c.hash : 0;
+		}
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldReadConstraints(cpgpuWorld *w, double *jacc, int32_t *colour)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	std::vector<Joint> J(w->nc);
+	if (w->nc) HIP_OR_FAIL(hipMemcpy(J.data(), w->W.joints, sizeof(Joint) * w->nc, hipMemcpyDeviceToHost));
+	for (int c = 0; c < w->nc; c++) {
+		const Joint &j = J[c];
+		if (jacc) {
+			if (j.type == CPGPU_JOINT_PIVOT) { jacc[2 * c] = j.jAx; jacc[2 * c + 1] = j.jAy; }
+			else if (j.type == CPGPU_JOINT_SLIDE) { jacc[2 * c] = j.jnAcc; jacc[2 * c + 1] = 0.0; }
+			else { jacc[2 * c] = j.jAcc; jacc[2 * c + 1] = 0.0; }
+		}
+		if (colour) colour[c] = w->h_jcolour[c];
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldSetTiming(cpgpuWorld *w, int32_t enable)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	w->timer.enabled = enable != 0;
+	w->timer.acc.clear();
+	return CPGPU_OK;
+}
+
+int cpgpuWorldGetTiming(cpgpuWorld *w, int32_t cap, int32_t *n_out, double *ms, int64_t *launches, char *names,
+                        int32_t names_cap)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	std::string all;
+	int k = 0;
+	for (auto &kv : w->timer.acc) {
+		if (k < cap) {
+			if (ms) ms[k] = kv.second.first;
+			if (launches) launches[k] = kv.second.second;
+		}
+		all += kv.first;
+		all += "\n";
+		k++;
+	}
+	if (n_out) *n_out = k;
+	if (names && names_cap > 0) {
+		const size_t m = std::min<size_t>(all.size(), (size_t)names_cap - 1);
+		memcpy(names, all.data(), m);
+		names[m] = 0;
+	}
+	return CPGPU_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,153 @@
+"""ctypes binding of the gfx950 step library (include/cpgpu.h).
+
+The library is built in-tree (chipmunk2d_amd/libcpgpu.so, see build.py).  If
+it is missing this module raises on import of `lib()`: there is no CPU
+fallback for the step.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .cabi import Stats, WorldDesc, ptr, FP64
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpgpu.so")
+
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+
+EXPORTS = [
+    "cpgpuWorldCreate", "cpgpuWorldDestroy", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
+    "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
+    "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuGetLastError", "cpgpuVersion",
+]
+
+
+class CpgpuError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded step library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CpgpuError("%s is missing: build it with `python -m chipmunk2d_amd.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.cpgpuWorldCreate.argtypes = [ctypes.POINTER(WorldDesc), ctypes.c_int32, ctypes.POINTER(_vp)]
+        L.cpgpuWorldDestroy.argtypes = [_vp]
+        L.cpgpuWorldDestroy.restype = None
+        L.cpgpuWorldStep.argtypes = [_vp, ctypes.c_double]
+        L.cpgpuWorldStepN.argtypes = [_vp, ctypes.c_double, ctypes.c_int32]
+        L.cpgpuWorldSync.argtypes = [_vp]
+        L.cpgpuWorldReadBodies.argtypes = [_vp] + [_dp] * 7
+        L.cpgpuWorldWriteBodies.argtypes = [_vp] + [_dp] * 4
+        L.cpgpuWorldGetStats.argtypes = [_vp, ctypes.POINTER(Stats)]
+        L.cpgpuWorldReadArbiters.argtypes = [_vp, ctypes.c_int64, _i64p, _ip, _ip, _ip, _ip, _dp, _dp, _u64p]
+        L.cpgpuWorldReadConstraints.argtypes = [_vp, _dp, _ip]
+        L.cpgpuWorldSetTiming.argtypes = [_vp, ctypes.c_int32]
+        L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
+        L.cpgpuGetLastError.restype = ctypes.c_char_p
+        L.cpgpuVersion.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise CpgpuError("cpgpu error %d: %s" % (rc, lib().cpgpuGetLastError().decode()))
+
+
+class GPUWorld:
+    """One or more packed spaces resident on the current HIP device."""
+
+    def __init__(self, packed, precision=FP64):
+        self.L = lib()
+        self.packed = packed
+        self.desc = packed.desc()
+        h = _vp()
+        _check(self.L.cpgpuWorldCreate(ctypes.byref(self.desc), precision, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cpgpuWorldDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, dt, n=1):
+        if n == 1:
+            _check(self.L.cpgpuWorldStep(self.h, dt))
+        else:
+            _check(self.L.cpgpuWorldStepN(self.h, dt, n))
+
+    def sync(self):
+        _check(self.L.cpgpuWorldSync(self.h))
+
+    def bodies(self):
+        from types import SimpleNamespace
+        n = self.packed.n_bodies
+        s = SimpleNamespace(p=np.zeros((n, 2)), v=np.zeros((n, 2)), a=np.zeros(n), w=np.zeros(n),
+                            v_bias=np.zeros((n, 2)), w_bias=np.zeros(n), transform=np.zeros((n, 6)))
+        s.FIELDS = ("p", "v", "a", "w", "v_bias", "w_bias", "transform")
+        _check(self.L.cpgpuWorldReadBodies(self.h, *[ptr(getattr(s, f), ctypes.c_double) for f in s.FIELDS]))
+        return s
+
+    def write_bodies(self, v=None, w=None, f=None, t=None):
+        args = [ptr(np.ascontiguousarray(x, dtype=np.float64), ctypes.c_double) if x is not None else None
+                for x in (v, w, f, t)]
+        _check(self.L.cpgpuWorldWriteBodies(self.h, *args))
+
+    def stats(self):
+        st = Stats()
+        _check(self.L.cpgpuWorldGetStats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def arbiters(self):
+        from types import SimpleNamespace
+        n = ctypes.c_int64()
+        _check(self.L.cpgpuWorldReadArbiters(self.h, 0, ctypes.byref(n), None, None, None, None, None, None, None))
+        n = n.value
+        A = SimpleNamespace(n=n, shapes=np.zeros((n, 2), np.int32), colour=np.zeros(n, np.int32),
+                            state=np.zeros(n, np.int32), count=np.zeros(n, np.int32), normal=np.zeros((n, 2)),
+                            contact=np.zeros((n, 2, 12)), hash=np.zeros((n, 2), np.uint64))
+        m = ctypes.c_int64()
+        _check(self.L.cpgpuWorldReadArbiters(self.h, n, ctypes.byref(m), ptr(A.shapes, ctypes.c_int32),
+                                             ptr(A.colour, ctypes.c_int32), ptr(A.state, ctypes.c_int32),
+                                             ptr(A.count, ctypes.c_int32), ptr(A.normal, ctypes.c_double),
+                                             ptr(A.contact, ctypes.c_double), ptr(A.hash, ctypes.c_uint64)))
+        return A
+
+    def constraints(self):
+        n = self.packed.n_constraints
+        jacc = np.zeros((n, 2))
+        col = np.zeros(n, np.int32)
+        _check(self.L.cpgpuWorldReadConstraints(self.h, ptr(jacc, ctypes.c_double), ptr(col, ctypes.c_int32)))
+        return jacc, col
+
+    def set_timing(self, on=True):
+        _check(self.L.cpgpuWorldSetTiming(self.h, 1 if on else 0))
+
+    def timing(self):
+        """dict kernel name -> (total ms, launches) since timing was enabled."""
+        cap = 64
+        n = ctypes.c_int32()
+        ms = np.zeros(cap)
+        launches = np.zeros(cap, np.int64)
+        names = ctypes.create_string_buffer(8192)
+        _check(self.L.cpgpuWorldGetTiming(self.h, cap, ctypes.byref(n), ptr(ms, ctypes.c_double),
+                                          ptr(launches, ctypes.c_int64), names, 8192))
+        keys = names.value.decode().split("\n")[:n.value]
+        return {k: (float(ms[i]), int(launches[i])) for i, k in enumerate(keys)}
+
+
+def version():
+    return lib().cpgpuVersion().decode()

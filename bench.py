@@ -1,0 +1,193 @@
+#!/usr/bin/env python
+"""Benchmark of the gfx950 cpSpaceStep on one MI355X.
+
+Workload (default): the single 100k-shape world -- 50k circles, 40k convex
+polygons, 10k static segments (container + sinusoidal terrain), seeded,
+10 solver iterations, dt 1/60 -- stepped for the configured 300-step rollout
+from its initial state.  A "step" is one cpSpaceStep of the whole world.
+
+    python bench.py [--steps K] [--warmup W] [--workload random_world_100k|batch_4096]
+
+W warm-up steps run on a separate copy of the world (first-launch costs),
+then exactly K steps are timed on a fresh copy, so the timed region is
+steps 0..K-1 of the rollout.  The step library synchronises its own stream
+at both ends of the timed region.  Prints one JSON line.
+
+roofline: the dominant kernel's algorithmic bytes per launch (streaming
+model: every phase reads and writes the state the reference's dataflow
+touches, FP64) over its average launch time from HIP events recorded on
+the library's stream during the timed region, against 8 TB/s HBM.
+cpu_baseline: the reference's own cpSpaceStep (compiled from its sources,
+strict IEEE build, one thread) on the first steps of the same rollout.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "body-steps/s @10 iters: 100k-shape world & 4096-world batch; fwd+adjoint steps/s"
+HBM_PEAK_GBS = 8000.0
+S = 8  # bytes per FP64 scalar
+
+
+def build_workload(name):
+    from chipmunk2d_amd import scenes
+    from chipmunk2d_amd.cabi import concat
+    if name == "random_world_100k":
+        sp = scenes.random_world()
+        return sp.pack(), {"workload": "random_world_100k", "circles": 50000, "polys": 40000,
+                           "static_segments": 10000, "extent": 2000.0, "iterations": 10, "dt": 1.0 / 60.0,
+                           "seed": hex(scenes.SEED_RANDOM_WORLD), "rollout_steps": 300}
+    if name == "batch_4096":
+        spaces = scenes.batch_worlds()
+        pw = concat([s.pack() for s in spaces])
+        return pw, {"workload": "batch_4096", "worlds": len(spaces), "bodies_log_uniform": [16, 2048],
+                    "iterations": 10, "dt": 1.0 / 60.0, "seed": hex(scenes.SEED_BATCH), "rollout_steps": 240}
+    raise SystemExit("unknown workload " + name)
+
+
+def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations):
+    """Streaming-model bytes of one launch of `kernel`, from the per-step averages."""
+    c, a, p = per_step["contacts"], per_step["arbiters"], per_step["pairs"]
+    if kernel == "solver":
+        # contact pre-step 26s + cached impulse 24s, then 47s per contact and iteration;
+        # velocity integration reads 8s and writes 6s per dynamic body
+        return c * (26 + 24) * S + c * iterations * 47 * S + n_dyn_bodies * 14 * S
+    if kernel == "narrowphase":
+        # per pair: two shapes' world geometry (~8s each for the mix), material 4s + 8 B each,
+        # warm-start record 24 B; per colliding pair its arbiter header 6s + 8 B, per contact 4s + 8 B
+        return p * (2 * 8 * S + 2 * (4 * S + 8) + 24) + a * (6 * S + 8) + c * (4 * S + 8)
+    if kernel == "bvh_pairs":
+        return n_dyn_shapes * (4 * S + 20) + p * 8
+    if kernel == "integrate_positions":
+        return n_dyn_bodies * (11 + 12) * S
+    if kernel == "cache_shapes":
+        return n_dyn_shapes * 15 * S
+    return None
+
+
+def run_gpu(pw, steps, warmup, dt):
+    from chipmunk2d_amd.gpu import GPUWorld
+    if warmup > 0:
+        g = GPUWorld(pw)
+        g.step(dt, warmup)
+        g.sync()
+        g.close()
+    g = GPUWorld(pw)
+    g.sync()
+    g.set_timing(True)
+    t_step = []
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step(dt)
+        t_step.append(time.perf_counter())
+    g.sync()
+    t1 = time.perf_counter()
+    timing = g.timing()
+    stats = g.stats()
+    g.close()
+    return t1 - t0, [t - t0 for t in t_step], timing, stats
+
+
+def run_cpu_reference(pw, sample_steps, dt):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from checkers import RefWorld, ref_lib, REF_SO
+    if not os.path.exists(REF_SO):
+        return None
+    r = RefWorld(pw)
+    t0 = time.perf_counter()
+    r.step(dt, sample_steps)
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="random_world_100k", choices=["random_world_100k", "batch_4096"])
+    ap.add_argument("--cpu-sample-steps", type=int, default=120,
+                    help="steps of the same rollout the reference CPU step is timed on")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would step its own "
+              "slice of worlds independently); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    dt = 1.0 / 60.0
+    pw, config = build_workload(args.workload)
+    n_dyn = pw.dynamic_body_count()
+    import numpy as np
+    body_type = pw.a["body_type"]
+    n_dyn_shapes = int(np.count_nonzero(body_type[pw.a["shape_body"]] != 2))
+    iterations = max(sp["iterations"] for sp in pw.a["space"])
+    config.update({"bodies": n_dyn, "shapes": pw.n_shapes, "worlds": pw.n_worlds})
+
+    wall, t_steps, timing, stats = run_gpu(pw, args.steps, args.warmup, dt)
+    value = n_dyn * args.steps / wall
+    per_step = {"contacts": stats["total_contacts"] / args.steps, "arbiters": stats["total_arbiters"] / args.steps,
+                "pairs": stats["total_pairs"] / args.steps}
+
+    kernels = {}
+    for name, (ms, launches) in timing.items():
+        avg_ms = ms / max(launches, 1)
+        b = algorithmic_bytes(name, per_step, n_dyn, n_dyn_shapes, iterations)
+        kernels[name] = {"total_ms": round(ms, 3), "launches": launches, "avg_us": round(avg_ms * 1e3, 2)}
+        if b is not None:
+            kernels[name]["bytes_per_launch"] = int(b)
+            kernels[name]["GBps"] = round(b / (avg_ms * 1e-3) / 1e9, 1)
+    dominant = max(timing.items(), key=lambda kv: kv[1][0])[0] if timing else None
+    roofline = None
+    if dominant and "GBps" in kernels.get(dominant, {}):
+        ach = kernels[dominant]["GBps"]
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                    "bytes_per_launch": kernels[dominant]["bytes_per_launch"],
+                    "avg_launch_us": kernels[dominant]["avg_us"]}
+
+    cpu = None
+    if not args.no_cpu_baseline and args.workload == "random_world_100k":
+        k = min(args.cpu_sample_steps, args.steps)
+        t_cpu = run_cpu_reference(pw, k, dt)
+        if t_cpu is not None:
+            gpu_same = n_dyn * k / t_steps[k - 1]
+            cpu = {"value": round(n_dyn * k / t_cpu, 1), "unit": "body-steps/s", "cores": 1, "kind": "reference",
+                   "sample": "steps 0-%d of the same 300-step rollout (the reference's cpSpaceStep, strict build "
+                             "-O2 -ffp-contract=off, one thread; later steps cost more as piles form)" % (k - 1),
+                   "seconds": round(t_cpu, 2), "gpu_same_sample": round(gpu_same, 1),
+                   "gpu_over_cpu_same_sample": round(gpu_same / (n_dyn * k / t_cpu), 1),
+                   "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded scene generator, xoshiro256**)",
+        "config": config,
+        "contacts_per_s": round(stats["total_contacts"] / wall, 1),
+        "avg_contacts_per_step": round(per_step["contacts"], 1),
+        "avg_arbiters_per_step": round(per_step["arbiters"], 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,9 @@ class Stats(ctypes.Structure):
         ("arbiter_colours", ctypes.c_int32),
         ("constraint_colours", ctypes.c_int32),
         ("steps", ctypes.c_int64),
+        ("total_arbiters", ctypes.c_int64),
+        ("total_contacts", ctypes.c_int64),
+        ("total_pairs", ctypes.c_int64),
     ]
 
     def as_dict(self):

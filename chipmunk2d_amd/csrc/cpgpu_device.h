@@ -220,6 +220,8 @@ struct DevWorld {
 	Joint *joints;
 	// device error word
 	int32_t *err;
+	// cumulative counters: [0] arbiters solved, [1] contacts solved
+	unsigned long long *totals;
 };
 
 #define CPGPU_ERR_PAIRS_OVERFLOW 1

@@ -184,6 +184,14 @@ __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const Ste
 		arb_out = carry_arbiter(B, W, old);
 	}
 	write_record(B, W, key, info.id, ws.stamp, arb_out);
+
+	// cumulative arbiter / contact counts (count is 0, 1 or 2), one atomic per wave
+	const unsigned long long na = (unsigned long long)__popcll(__ballot(info.count > 0));
+	const unsigned long long nc = na + (unsigned long long)__popcll(__ballot(info.count > 1));
+	if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && na > 0) {
+		atomicAdd(&W.totals[0], na);
+		atomicAdd(&W.totals[1], nc);
+	}
 }
 
 // Records of the previous step whose pair was not a candidate this step.

@@ -259,6 +259,7 @@ struct cpgpuWorld {
 	int32_t *h_counts = nullptr;        // pinned: [0..5] step counts, [6] n_pairs, [7] err, [8] n_colours
 	int64_t steps = 0;
 	int64_t last_pairs = 0;
+	int64_t total_pairs = 0;
 	bool pending = false;
 	KernelTimer timer;
 };
@@ -548,6 +549,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(hipHostMalloc((void **)&w->h_counts, sizeof(int32_t) * 16, hipHostMallocDefault));
 	memset(w->h_counts, 0, sizeof(int32_t) * 16);
 	HIP_OR_FAIL(own_upload(w, W.err, (const int32_t *)nullptr, 1));
+	HIP_OR_FAIL(own_upload(w, W.totals, (const unsigned long long *)nullptr, 2));
 	HIP_OR_FAIL(own_upload(w, w->d_counters, (const int32_t *)nullptr, 4));
 	HIP_OR_FAIL(own_upload(w, w->d_step_counts, (const int32_t *)nullptr, 6));
 	HIP_OR_FAIL(own_alloc(w, w->d_world_bounds, w->nw));
@@ -672,6 +674,7 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		n_pairs = w->h_counts[6];
 	}
 	w->last_pairs = n_pairs;
+	w->total_pairs += n_pairs;
 
 	// capacities of this step's outputs from last step's counts
 	const int prev = w->cur ^ 1, cur = w->cur;
@@ -802,6 +805,11 @@ int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
 	}
 	out->constraint_colours = w->n_jcolours;
 	out->steps = w->steps;
+	unsigned long long totals[2];
+	HIP_OR_FAIL(hipMemcpy(totals, w->W.totals, sizeof(totals), hipMemcpyDeviceToHost));
+	out->total_arbiters = (int64_t)totals[0];
+	out->total_contacts = (int64_t)totals[1];
+	out->total_pairs = w->total_pairs;
 	return CPGPU_OK;
 }
 

@@ -61,6 +61,7 @@ class Rng:
 
     def randint(self, lo, hi):
         """Integer in [lo, hi] inclusive."""
+        lo, hi = int(lo), int(hi)
         return lo + self.next_u64() % (hi - lo + 1)
 
     def normal(self, mu=0.0, sigma=1.0):
@@ -180,6 +181,45 @@ def random_pile(n_bodies, rng, width=None, x_offset=0.0):
         shape.friction = rng.uniform(0.3, 0.9)
         body.position = (x_offset + rng.uniform(-half, half), rng.uniform(3.0, 3.0 + height))
         body.velocity = (rng.normal(0.0, 5.0), rng.normal(0.0, 5.0))
+    return space
+
+
+def chain_world(n_links=200, n_circles=300, seed=SEED_CHAINS, rng=None):
+    """A chain of n_links boxes (4 x 1, mass 1) joined by pivot joints
+    (collideBodies off) and hung from a static body, with a damped spring
+    (stiffness U[50,500], damping U[1,10]), a slide joint and a gear joint
+    every 10 links, and n_circles circles (radius U[0.5,1.5]) falling on it.
+    Gravity (0,-100), 10 iterations."""
+    if rng is None:
+        rng = Rng(seed)
+    space = Space()
+    space.iterations = 10
+    space.gravity = (0.0, -100.0)
+    x0, y0 = -2.0 * n_links, 0.0
+    links = []
+    prev = space.static_body
+    for i in range(n_links):
+        body = space.add(Body(1.0, moment_for_box(1.0, 4.0, 1.0)))
+        body.position = (x0 + 4.0 * i + 2.0, y0)
+        shape = space.add(Box(body, 4.0, 1.0, 0.0))
+        shape.friction = 0.6
+        pivot = (x0 + 4.0 * i, y0)
+        j = space.add(PivotJoint(prev, body, pivot))
+        j.collide_bodies = False
+        links.append(body)
+        prev = body
+    for i in range(0, n_links - 10, 10):
+        a, b = links[i], links[i + 10]
+        space.add(DampedSpring(a, b, (0.0, 0.0), (0.0, 0.0), 40.0, rng.uniform(50.0, 500.0), rng.uniform(1.0, 10.0)))
+        if i + 2 < n_links:
+            space.add(SlideJoint(links[i], links[i + 2], (0.0, 0.0), (0.0, 0.0), 4.0, 8.5))
+        space.add(GearJoint(links[i], links[i + 1], 0.0, 1.0))
+    for _ in range(n_circles):
+        body = space.add(Body(0.0, 0.0))
+        shape = space.add(Circle(body, rng.uniform(0.5, 1.5)))
+        shape.density = 1.0
+        shape.friction = 0.5
+        body.position = (rng.uniform(x0, -x0), rng.uniform(10.0, 10.0 + 0.5 * n_links))
     return space
 
 

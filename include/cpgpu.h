@@ -154,6 +154,9 @@ typedef struct cpgpuStats {
 	int32_t arbiter_colours;   /* colours of the arbiter graph */
 	int32_t constraint_colours;/* colours of the constraint graph */
 	int64_t steps;             /* steps taken since creation */
+	int64_t total_arbiters;    /* arbiters solved, summed over all steps since creation */
+	int64_t total_contacts;    /* contacts solved, summed over all steps since creation */
+	int64_t total_pairs;       /* candidate pairs, summed over all steps since creation */
 } cpgpuStats;
 
 /* ---- lifecycle (replaces cpHastySpaceNew / cpHastySpaceFree, src/cpHastySpace.c:560-592) ---- */

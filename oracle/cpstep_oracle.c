@@ -1331,7 +1331,7 @@ static void joint_prestep(oworld *w, OJoint *j, double dt)
 		double k = k_scalar(a, b, j->r1, j->r2, j->n);
 		j->nMass = 1.0 / k;
 		j->target_vrn = 0.0;
-		j->v_coef = 1.0 - cp_exp_oracle(-damping * dt * k);
+		j->v_coef = 1.0 - (w->libm_trig ? exp(-damping * dt * k) : cp_exp_oracle(-damping * dt * k));
 		double f_spring = (restLength - dist) * stiffness;
 		double j_spring = j->jAcc = f_spring * dt;
 		apply_impulses(a, b, j->r1, j->r2, vmult(j->n, j_spring));

@@ -25,9 +25,10 @@ void ow_free(oworld *w);
 /* One cpSpaceStep over every world. */
 void ow_step(oworld *w, double dt);
 
-/* 1: body rotations use libm cos/sin exactly as the reference does
- * (cpBody.c:350 via cpVect.h:104-107).  0 (default): the deterministic
- * cp_sincos that the device library also uses. */
+/* 1: body rotations use libm cos/sin (cpBody.c:350 via cpVect.h:104-107) and
+ * the damped spring libm exp (cpDampedSpring.c:47) exactly as the reference
+ * does.  0 (default): the deterministic cp_sincos / cp_exp that the device
+ * library also uses. */
 void ow_set_libm_trig(oworld *w, int enable);
 
 /* 1: solve constraints in description order instead of colour order

@@ -1,0 +1,253 @@
+"""Parity of the HIP step (chipmunk2d_amd/libcpgpu.so, through its C ABI)
+with the oracle -- needs an MI355X.
+
+The device step and the oracle perform the same IEEE double operations in
+the same order (the oracle solves in the device's colour order), so every
+comparison below is bit-exact: body state after every step, the arbiter set
+with its contacts, impulses, feature hashes, states and colours, and the
+joints' accumulated impulses.
+"""
+import numpy as np
+import pytest
+
+from chipmunk2d_amd.cabi import concat
+from chipmunk2d_amd.gpu import GPUWorld
+from chipmunk2d_amd.scenes import pyramid_210, random_world, random_pile, chain_world, Rng
+from chipmunk2d_amd.space import Space, Body, Circle, Segment, Box, moment_for_circle
+from checkers import OracleWorld
+import test_oracle_pins as pins
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 60.0
+
+
+def assert_bodies_equal(g, o, label=""):
+    gb, ob = g.bodies(), o.bodies()
+    for f in ob.FIELDS:
+        x, y = getattr(gb, f), getattr(ob, f)
+        if not np.array_equal(x, y):
+            diff = np.abs(x - y)
+            raise AssertionError("%s: body field %s differs (max %g at %s)" % (label, f, np.nanmax(diff),
+                                                                            np.unravel_index(np.nanargmax(diff), diff.shape)))
+
+
+def assert_arbiters_equal(g, o):
+    ga, oa = g.arbiters(), o.arbiters()
+    assert ga.n == oa.n
+    gk = {(min(a, b), max(a, b)): k for k, (a, b) in enumerate(ga.shapes)}
+    ok = oa.by_key()
+    assert set(gk) == set(ok)
+    idx_g = np.array([gk[k] for k in sorted(gk)], dtype=np.int64)
+    idx_o = np.array([ok[k] for k in sorted(ok)], dtype=np.int64)
+    assert np.array_equal(ga.shapes[idx_g], oa.shapes[idx_o])
+    assert np.array_equal(ga.colour[idx_g], oa.colour[idx_o])
+    assert np.array_equal(ga.state[idx_g], oa.state[idx_o])
+    assert np.array_equal(ga.count[idx_g], oa.count[idx_o])
+    assert np.array_equal(ga.normal[idx_g], oa.normal[idx_o])
+    assert np.array_equal(ga.contact[idx_g], oa.contact[idx_o])
+    assert np.array_equal(ga.hash[idx_g], oa.hash[idx_o])
+
+
+def run_lockstep(pw, steps, check_every=1, arbiters_every=0):
+    g = GPUWorld(pw)
+    o = OracleWorld(pw)
+    for s in range(steps):
+        g.step(DT)
+        o.step(DT)
+        if (s + 1) % check_every == 0 or s == steps - 1:
+            assert_bodies_equal(g, o, "step %d" % s)
+        if arbiters_every and (s + 1) % arbiters_every == 0:
+            assert_arbiters_equal(g, o)
+    gs, os_ = g.stats(), o.stats()
+    for k in ("candidate_pairs", "arbiters", "contacts", "cached_arbiters", "arbiter_colours"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+    return g, o
+
+
+def test_pyramid_bit_exact():
+    g, o = run_lockstep(pyramid_210().pack(), 300, check_every=1, arbiters_every=50)
+    assert_arbiters_equal(g, o)
+    # the stack settles and stays up (the reference scene's purpose)
+    b = g.bodies()
+    assert b.p[1:, 1].max() > 300.0
+
+
+@pytest.mark.parametrize("seed", [7, 8])
+def test_random_world_bit_exact(seed):
+    pw = random_world(n_circles=500, n_polys=500, n_segments=200, extent=120.0, seed=seed).pack()
+    g, o = run_lockstep(pw, 150, check_every=1, arbiters_every=25)
+    assert g.stats()["arbiters"] > 500
+
+
+def test_chain_with_joints_bit_exact():
+    pw = chain_world(n_links=80, n_circles=120, seed=42).pack()
+    g, o = run_lockstep(pw, 150, check_every=5, arbiters_every=50)
+    jg, cg = g.constraints()
+    jo, co = o.constraints()
+    assert np.array_equal(cg, co)
+    assert np.array_equal(jg, jo)
+    assert np.count_nonzero(jg) > 50
+
+
+def test_batch_of_worlds_bit_exact():
+    rng = Rng(0xBA7C4)
+    spaces = [pyramid_210(rows=r).pack() for r in (3, 6, 9)]
+    spaces += [random_pile(n, rng).pack() for n in (16, 100, 300)]
+    spaces += [chain_world(n_links=20, n_circles=10, seed=5).pack()]
+    pw = concat(spaces)
+    assert pw.n_worlds == 7
+    run_lockstep(pw, 120, check_every=10, arbiters_every=40)
+
+
+def test_worlds_in_a_batch_step_as_when_alone():
+    """A space stepped inside a batch ends bit-identical to the same space stepped alone."""
+    rng = Rng(77)
+    a = random_pile(150, rng).pack()
+    b = pyramid_210(rows=7).pack()
+    g_batch = GPUWorld(concat([a, b]))
+    g_alone = GPUWorld(b)
+    for _ in range(60):
+        g_batch.step(DT)
+        g_alone.step(DT)
+    nb_a = a.n_bodies
+    x, y = g_batch.bodies(), g_alone.bodies()
+    assert np.array_equal(x.p[nb_a:], y.p) and np.array_equal(x.v[nb_a:], y.v) and np.array_equal(x.a[nb_a:], y.a)
+
+
+def test_repeatable_run_to_run():
+    pw = random_world(n_circles=800, n_polys=800, n_segments=200, extent=150.0, seed=19).pack()
+    g1, g2 = GPUWorld(pw), GPUWorld(pw)
+    for _ in range(80):
+        g1.step(DT)
+        g2.step(DT)
+    b1, b2 = g1.bodies(), g2.bodies()
+    assert np.array_equal(b1.p, b2.p) and np.array_equal(b1.v, b2.v) and np.array_equal(b1.a, b2.a)
+
+
+def test_filters_groups_masks_and_no_collide_joints():
+    sp = random_world(n_circles=300, n_polys=300, n_segments=80, extent=60.0, seed=23)
+    for k, s in enumerate(sp.shapes):
+        if k % 5 == 0:
+            s.set_filter(group=7)
+        if k % 9 == 0:
+            s.set_filter(categories=1, mask=2)
+    run_lockstep(sp.pack(), 80, check_every=4, arbiters_every=40)
+
+
+# ---- known answers of the reference's own tests, on the device ----
+
+def test_time_stepping_known_answer_gpu():
+    pins.check_time_stepping(GPUWorld)
+
+
+def test_basic_simulation_gpu():
+    pins.check_basic_simulation(GPUWorld)
+
+
+def test_ball_rests_on_bar_gpu():
+    """CallbacksTest.m:257-292 without the removed bar: a ball (r 1) on a bar of
+    radius 1 at y 0 rests at y 2 within 1.1 slop."""
+    space = Space()
+    space.gravity = (0.0, -100.0)
+    space.add(Segment(space.static_body, (-10.0, 0.0), (10.0, 0.0), 1.0))
+    ball = space.add(Body(1.0, moment_for_circle(1.0, 0.0, 1.0)))
+    ball.position = (0.0, 10.0)
+    space.add(Circle(ball, 1.0))
+    g = GPUWorld(space.pack())
+    for _ in range(100):
+        g.step(0.01)
+    assert abs(g.bodies().p[1][1] - 2.0) < 1.1 * space.collision_slop
+
+
+# ---- edge cases ----
+
+def test_empty_and_shapeless_worlds():
+    space = Space()
+    space.gravity = (0.0, -10.0)
+    b = space.add(Body(2.0, 3.0))
+    b.velocity = (1.0, 2.0)
+    run_lockstep(space.pack(), 10)
+    space = Space()
+    space.add(Segment(space.static_body, (0.0, 0.0), (1.0, 0.0), 0.0))
+    run_lockstep(space.pack(), 5)
+
+
+def test_zero_dt_is_a_no_op():
+    pw = pyramid_210(rows=4).pack()
+    g = GPUWorld(pw)
+    before = g.bodies()
+    g.step(0.0)
+    after = g.bodies()
+    assert np.array_equal(before.p, after.p) and g.stats()["steps"] == 0
+
+
+def test_coincident_circles():
+    """Zero distance: cpCollide's fallback normal (1, 0) (cpCollision.c:533)."""
+    space = Space()
+    for _ in range(2):
+        body = space.add(Body(1.0, 1.0))
+        body.position = (3.0, 4.0)
+        space.add(Circle(body, 1.0))
+    g, o = run_lockstep(space.pack(), 3)
+    assert_arbiters_equal(g, o)
+
+
+def test_kinematic_body_pushes_dynamic():
+    space = Space()
+    kin = space.add(Body.kinematic())
+    kin.velocity = (5.0, 0.0)
+    space.add(Box(kin, 4.0, 4.0, 0.0))
+    for k in range(5):
+        body = space.add(Body(1.0, 1.0))
+        body.position = (5.0 + 2.5 * k, 0.0)
+        space.add(Circle(body, 1.0))
+    g, o = run_lockstep(space.pack(), 120, check_every=1)
+    assert g.bodies().p[1][0] > 10.0
+
+
+def test_variable_dt_cached_impulse_scaling():
+    """dt_coef = dt / prev_dt scales the warm start (cpSpaceStep.c:407)."""
+    pw = pyramid_210(rows=6).pack()
+    g, o = GPUWorld(pw), OracleWorld(pw)
+    for k in range(60):
+        dt = DT if k % 3 else DT * 0.5
+        g.step(dt)
+        o.step(dt)
+    assert_bodies_equal(g, o)
+
+
+def test_velocity_and_force_writes():
+    pw = pyramid_210(rows=5).pack()
+    g, o = GPUWorld(pw), OracleWorld(pw)
+    for k in range(30):
+        g.step(DT)
+        o.step(DT)
+    n = pw.n_bodies
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(n, 2))
+    f = rng.normal(size=(n, 2)) * 10
+    v[0] = 0.0
+    f[0] = 0.0
+    g.write_bodies(v=v, f=f)
+    o.write_bodies(v=np.ascontiguousarray(v), f=np.ascontiguousarray(f))
+    for k in range(30):
+        g.step(DT)
+        o.step(DT)
+    assert_bodies_equal(g, o)
+
+
+# ---- the full-size workloads: size-independent properties ----
+
+def test_random_world_100k_first_steps_bit_exact():
+    """The 100k-shape world (50k circles, 40k polys, 10k static segments):
+    the first steps are small enough for the oracle."""
+    pw = random_world().pack()
+    assert pw.n_shapes == 100000
+    g, o = GPUWorld(pw), OracleWorld(pw)
+    for s in range(3):
+        g.step(DT)
+        o.step(DT)
+        assert_bodies_equal(g, o, "step %d" % s)
+    assert_arbiters_equal(g, o)
+    assert g.stats()["candidate_pairs"] == o.stats()["candidate_pairs"]

@@ -44,9 +44,9 @@ def build_workload(name):
                            "static_segments": 10000, "extent": 2000.0, "iterations": 10, "dt": 1.0 / 60.0,
                            "seed": hex(scenes.SEED_RANDOM_WORLD), "rollout_steps": 300}
     if name == "batch_4096":
-        spaces = scenes.batch_worlds()
-        pw = concat([s.pack() for s in spaces])
-        return pw, {"workload": "batch_4096", "worlds": len(spaces), "bodies_log_uniform": [16, 2048],
+        pw = scenes.batch_packed(n_worlds=4096, processes=min(16, os.cpu_count() or 1))
+        return pw, {"workload": "batch_4096", "worlds": pw.n_worlds, "bodies_log_uniform": [16, 2048],
+                    "kinds": "pyramid stack or random circle/poly pile (1:1), each on its own ground",
                     "iterations": 10, "dt": 1.0 / 60.0, "seed": hex(scenes.SEED_BATCH), "rollout_steps": 240}
     raise SystemExit("unknown workload " + name)
 
@@ -54,7 +54,7 @@ def build_workload(name):
 def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations):
     """Streaming-model bytes of one launch of `kernel`, from the per-step averages."""
     c, a, p = per_step["contacts"], per_step["arbiters"], per_step["pairs"]
-    if kernel == "solver":
+    if kernel in ("solver", "solver_worlds"):
         # contact pre-step 26s + cached impulse 24s, then 47s per contact and iteration;
         # velocity integration reads 8s and writes 6s per dynamic body
         return c * (26 + 24) * S + c * iterations * 47 * S + n_dyn_bodies * 14 * S

@@ -51,14 +51,38 @@ __device__ __forceinline__ V2 box_centre(double4 b) { return v2(0.5 * (b.x + b.z
 __global__ void k_bounds(const DevWorld W, const int32_t *shapes, int n, unsigned long long *wb)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const int s = shapes[i];
-	const V2 c = box_centre(W.s_bb[s]);
-	const int w = W.s_world[s];
-	atomicMin(&wb[4 * w + 0], ord_of(c.x));
-	atomicMin(&wb[4 * w + 1], ord_of(c.y));
-	atomicMax(&wb[4 * w + 2], ord_of(c.x));
-	atomicMax(&wb[4 * w + 3], ord_of(c.y));
+	const bool valid = i < n;
+	int w = -1;
+	unsigned long long x0 = ~0ull, y0 = ~0ull, x1 = 0ull, y1 = 0ull;
+	if (valid) {
+		const int s = shapes[i];
+		const V2 c = box_centre(W.s_bb[s]);
+		w = W.s_world[s];
+		x0 = x1 = ord_of(c.x);
+		y0 = y1 = ord_of(c.y);
+	}
+	// one wave usually holds one world: reduce across the wave, one atomic per bound
+	const int w_first = __shfl(w, 0);
+	const bool uniform = __ballot(valid && w != w_first) == 0ull;
+	if (uniform) {
+		for (int off = 32; off > 0; off >>= 1) {
+			x0 = min(x0, (unsigned long long)__shfl_xor(x0, off));
+			y0 = min(y0, (unsigned long long)__shfl_xor(y0, off));
+			x1 = max(x1, (unsigned long long)__shfl_xor(x1, off));
+			y1 = max(y1, (unsigned long long)__shfl_xor(y1, off));
+		}
+		if ((threadIdx.x & 63) == 0 && w_first >= 0) {
+			atomicMin(&wb[4 * w_first + 0], x0);
+			atomicMin(&wb[4 * w_first + 1], y0);
+			atomicMax(&wb[4 * w_first + 2], x1);
+			atomicMax(&wb[4 * w_first + 3], y1);
+		}
+	} else if (valid) {
+		atomicMin(&wb[4 * w + 0], x0);
+		atomicMin(&wb[4 * w + 1], y0);
+		atomicMax(&wb[4 * w + 2], x1);
+		atomicMax(&wb[4 * w + 3], y1);
+	}
 }
 
 __device__ __forceinline__ uint64_t spread22(uint32_t x)
@@ -264,19 +288,30 @@ __device__ __forceinline__ double4 load_box_coherent(const double4 *p)
 	return make_double4(q[0], q[1], q[2], q[3]);
 }
 
+// Bottom-up box refit: the second lane to reach a node combines its children.
+// A node whose leaves all belong to this workgroup's chunk was completed by
+// lanes of this workgroup, so workgroup-scope ordering suffices; only nodes
+// spanning chunks need device-scope fences (which write back / invalidate L2).
 __global__ void k_refit(const DevWorld W, int n, const int32_t *leaf_shape, const int32_t *child,
-                        const int32_t *parent, double4 *box, uint32_t *visits)
+                        const int32_t *parent, const int32_t *range, double4 *box, uint32_t *visits)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
+	const int chunk_lo = blockIdx.x * blockDim.x, chunk_hi = chunk_lo + blockDim.x - 1;
 	int node = (n - 1) + i;
 	box[node] = W.s_bb[leaf_shape[i]];
 	int p = parent[node];
 	while (p >= 0) {
-		__threadfence();
-		if (atomicAdd(&visits[p], 1u) == 0u) return;
-		__threadfence();
-		box[p] = box_union(load_box_coherent(&box[child[2 * p]]), load_box_coherent(&box[child[2 * p + 1]]));
+		const bool local = range[2 * p] >= chunk_lo && range[2 * p + 1] <= chunk_hi;
+		if (local) {
+			if (__hip_atomic_fetch_add(&visits[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
+			box[p] = box_union(box[child[2 * p]], box[child[2 * p + 1]]);
+		} else {
+			__threadfence();
+			if (atomicAdd(&visits[p], 1u) == 0u) return;
+			__threadfence();
+			box[p] = box_union(load_box_coherent(&box[child[2 * p]]), load_box_coherent(&box[child[2 * p + 1]]));
+		}
 		node = p;
 		p = parent[p];
 	}
@@ -482,7 +517,8 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 		if (n > 1)
 			k_karras<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.child, t.parent, t.range,
 			                                                      t.world_first, t.world_last, t.world_root);
-		k_refit<<<blocks_for(n, 256), 256, 0, stream>>>(W, n, t.leaf_shape, t.child, t.parent, t.box, t.visits);
+		k_refit<<<blocks_for(n, 256), 256, 0, stream>>>(W, n, t.leaf_shape, t.child, t.parent, t.range, t.box,
+		                                                 t.visits);
 	});
 	return hipGetLastError();
 }

@@ -214,6 +214,9 @@ struct DevWorld {
 	const uint8_t *s_static;     // shape belongs to a static body
 	// no-collide constraint partners (CSR over bodies)
 	const int32_t *nc_start, *nc_list;
+	// first shape / body index of every world ([n_worlds + 1])
+	const int32_t *w_shape0;
+	const int32_t *w_body0;
 	// per-world step parameters
 	const WorldStep *ws;
 	// joints

@@ -36,30 +36,52 @@ struct GridBarrier {
 	unsigned int gen;
 };
 
+// Worlds are solved in one of two ways, chosen per world by size:
+//   small worlds: one workgroup each (k_solve_worlds), every synchronisation
+//     a workgroup barrier;
+//   large worlds: together on a co-resident grid (k_solve_grid) with grid
+//     barriers between colours.
 struct SolveBuffers {
 	Arbiter *act;
 	const int32_t *n_act;
 	int32_t *colour;           // [cap] colour of each active arbiter
-	int32_t *order;            // [cap] active arbiters sorted by colour
-	int32_t *col_off;          // [CPGPU_MAX_COLOURS + 1] counts, then offsets
+	int32_t *order;            // [cap] active arbiters in colour order (per small world: its own segment)
+	int32_t *arb_by_world;     // [cap] active arbiters sorted by (large, world); nullptr for a single space
+	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world
+	int32_t *large_first;      // [1] first sorted position of a large world's arbiter
+	const uint8_t *world_small;
+	const int32_t *small_worlds;   // the small worlds (one workgroup each)
+	int32_t *col_off;          // [CPGPU_MAX_COLOURS + 1] grid solve: counts, then offsets
 	int32_t *col_cursor;       // [CPGPU_MAX_COLOURS]
-	int32_t *n_colours;        // [1]
-	int32_t *left;             // [1] uncoloured arbiters
+	int32_t *n_colours;        // [1] grid solve colour count
+	int32_t *left;             // [1] uncoloured arbiters (grid solve)
 	unsigned long long *claim; // [2][n_bodies]
 	int32_t *taken;            // [n_bodies]
-	// joints, coloured once at creation
-	const int32_t *jorder;     // joints in colour order
+	// joints, coloured once at creation: large worlds together ...
+	const int32_t *jorder;     // joints of large worlds in colour order
 	const int32_t *jcol_off;   // [n_jcolours + 1]
 	int n_jcolours;
+	// ... and every small world on its own
+	const int32_t *jw_order;   // joints sorted by (world, colour)
+	const int32_t *jw_col;     // per world: n colours + 1 offsets into jw_order
+	const int32_t *jw_cbase;   // [n_worlds] first entry of the world's offsets in jw_col
+	const int32_t *jw_ncol;    // [n_worlds]
 	GridBarrier *bar;
-	int max_iterations;
+	int max_iterations;        // over the large worlds
 	int n_bodies;
-	const int32_t *dyn_bodies; // non-static bodies
-	int n_dyn_bodies;
+	const int32_t *large_bodies;   // non-static bodies of large worlds
+	int n_large_bodies;
 };
 
-// Launch the persistent solver: colouring, colour sort, pre-steps,
-// velocity integration, cached impulses and the impulse iterations.
-hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, int grid, hipStream_t stream, KernelTimer *timer);
-// Largest co-resident grid of the solver kernel on the current device.
+// Sort the solved arbiters by (large, world) and find the small worlds'
+// ranges (skipped for a single space).
+hipError_t solver_sort_by_world(const SolveBuffers &S, int n_act, int n_worlds, RadixSortBuffers &sortbuf,
+                                uint64_t *keys, hipStream_t stream, KernelTimer *timer);
+// Launch the solvers: colouring, colour sort, pre-steps, velocity
+// integration, cached impulses and the impulse iterations.  n_small small
+// worlds get a workgroup each; grid > 0 launches the grid solve of the large
+// worlds with that many workgroups.
+hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, int n_small, int grid, hipStream_t stream,
+                       KernelTimer *timer);
+// Largest co-resident grid of the grid solve on the current device.
 int solver_max_grid();

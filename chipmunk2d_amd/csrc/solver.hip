@@ -21,7 +21,8 @@
 
 namespace {
 
-constexpr int SOLVER_THREADS = 256;
+constexpr int WPW_THREADS = 256;    // workgroup per world
+constexpr int GRID_THREADS = 1024;  // grid solve: few, large workgroups keep the barrier cheap
 
 // ---- grid barrier (all blocks co-resident: cooperative launch) ----
 __device__ void grid_sync(GridBarrier *bar, int32_t *err)
@@ -328,54 +329,97 @@ __device__ void joint_apply_impulse(const DevWorld &W, Joint &j)
 	if (is_dynamic(W, j.b)) store_bv(W, j.b, b);
 }
 
-// ---- the persistent kernel ----
+// ---- the solve routine, shared by the workgroup-per-world and grid kernels ----
 
 __device__ __forceinline__ int dyn_or_neg(const DevWorld &W, int b) { return is_dynamic(W, b) ? b : -1; }
 
-__global__ void __launch_bounds__(SOLVER_THREADS) k_solve(const DevWorld W, const SolveBuffers S)
+// colouring priority: a bijective hash of the shape pair's indices within its
+// own space, so a space is coloured (and stepped) identically alone or in a batch
+__device__ __forceinline__ unsigned long long arbiter_priority(const DevWorld &W, const Arbiter &arb)
 {
-	const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-	const int stride = gridDim.x * blockDim.x;
-	const int n = *S.n_act;
+	const int s0 = W.w_shape0[arb.world];
+	return mix64(pair_key(arb.sa - s0, arb.sb - s0));
+}
+
+// Synchronisation of the threads taking part: one workgroup, or the whole
+// co-resident grid.
+struct BlockSync {
+	int tid, stride;
+	__device__ void sync() { __syncthreads(); }
+};
+struct GridSync {
+	int tid, stride;
+	GridBarrier *bar;
+	int32_t *err;
+	__device__ void sync() { grid_sync(bar, err); }
+};
+
+// The elements one solve covers.  Arbiters are list[0..n) (indices into
+// S.act; list == nullptr means 0..n-1).  col_off / col_cursor / left live in
+// LDS for a workgroup solve and in global memory for a grid solve.
+struct SolveSet {
+	const int32_t *list;
+	int n;
+	int32_t *order;            // output: the n arbiters in colour order
+	int32_t *col_off;          // [CPGPU_MAX_COLOURS + 1]
+	int32_t *col_cursor;       // [CPGPU_MAX_COLOURS]
+	int32_t *left;
+	int32_t *n_colours_out;    // may be nullptr
+	const int32_t *bodies;     // bodies to integrate: list, or nullptr for the range [body0, body0 + nb)
+	int body0, nb;
+	const int32_t *jorder;     // joints in colour order
+	const int32_t *jcol_off;   // [njcol + 1]
+	int njcol;
+	int iterations;            // sweeps (elements of worlds with fewer sweeps skip the extra ones)
+};
+
+template <class Sync>
+__device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy)
+{
+	const int tid = sy.tid, stride = sy.stride, n = T.n;
 	unsigned long long *claim0 = S.claim, *claim1 = S.claim + S.n_bodies;
+	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
 
 	// ---- colouring ----
 	for (int k = tid; k < n; k += stride) {
-		S.colour[k] = -1;
-		const int ba = dyn_or_neg(W, S.act[k].ba), bb = dyn_or_neg(W, S.act[k].bb);
+		const int a = arb_index(k);
+		S.colour[a] = -1;
+		const int ba = dyn_or_neg(W, S.act[a].ba), bb = dyn_or_neg(W, S.act[a].bb);
 		if (ba >= 0) { claim0[ba] = ~0ull; claim1[ba] = ~0ull; S.taken[ba] = -1; }
 		if (bb >= 0) { claim0[bb] = ~0ull; claim1[bb] = ~0ull; S.taken[bb] = -1; }
 	}
-	for (int k = tid; k <= CPGPU_MAX_COLOURS; k += stride) S.col_off[k] = 0;
-	if (tid == 0) *S.left = n;
-	grid_sync(S.bar, W.err);
+	for (int k = tid; k <= CPGPU_MAX_COLOURS; k += stride) T.col_off[k] = 0;
+	if (tid == 0) *T.left = n;
+	sy.sync();
 
 	int c = 0, sub = 0;
 	while (true) {
-		if (__hip_atomic_load(S.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+		if (__hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
 		if (c >= CPGPU_MAX_COLOURS) { if (tid == 0) raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); break; }
 		unsigned long long *cl = (sub & 1) ? claim1 : claim0;
 		unsigned long long *other = (sub & 1) ? claim0 : claim1;
 		for (int k = tid; k < n; k += stride) {
-			if (S.colour[k] >= 0) continue;
-			const Arbiter &arb = S.act[k];
+			const int a = arb_index(k);
+			if (S.colour[a] >= 0) continue;
+			const Arbiter &arb = S.act[a];
 			const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
 			if ((ba >= 0 && S.taken[ba] == c) || (bb >= 0 && S.taken[bb] == c)) continue;
-			const unsigned long long prio = mix64(arb.key);
+			const unsigned long long prio = arbiter_priority(W, arb);
 			if (ba >= 0) atomicMin(&cl[ba], prio);
 			if (bb >= 0) atomicMin(&cl[bb], prio);
 		}
-		grid_sync(S.bar, W.err);
+		sy.sync();
 		int won = 0;
 		for (int k = tid; k < n; k += stride) {
-			if (S.colour[k] >= 0) continue;
-			const Arbiter &arb = S.act[k];
+			const int a = arb_index(k);
+			if (S.colour[a] >= 0) continue;
+			const Arbiter &arb = S.act[a];
 			const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
-			const unsigned long long prio = mix64(arb.key);
+			const unsigned long long prio = arbiter_priority(W, arb);
 			const bool win = (ba < 0 || __hip_atomic_load(&cl[ba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio) &&
 			                 (bb < 0 || __hip_atomic_load(&cl[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio);
 			if (win) {
-				S.colour[k] = c;
+				S.colour[a] = c;
 				if (ba >= 0) S.taken[ba] = c;
 				if (bb >= 0) S.taken[bb] = c;
 				won++;
@@ -384,44 +428,48 @@ __global__ void __launch_bounds__(SOLVER_THREADS) k_solve(const DevWorld W, cons
 				if (bb >= 0) other[bb] = ~0ull;
 			}
 		}
-		if (won) atomicSub(S.left, won);
-		grid_sync(S.bar, W.err);
+		if (won) atomicSub(T.left, won);
+		sy.sync();
 		if (++sub % COLOUR_SUBROUNDS == 0) c++;
 	}
 	const int n_colours = (sub % COLOUR_SUBROUNDS == 0) ? c : c + 1;
 
 	// ---- colour sort ----
-	for (int k = tid; k < n; k += stride)
-		if (S.colour[k] >= 0) atomicAdd(&S.col_off[S.colour[k]], 1);
-	grid_sync(S.bar, W.err);
+	for (int k = tid; k < n; k += stride) {
+		const int col = S.colour[arb_index(k)];
+		if (col >= 0) atomicAdd(&T.col_off[col], 1);
+	}
+	sy.sync();
 	if (tid == 0) {
 		int run = 0;
 		for (int q = 0; q < n_colours; q++) {
-			const int cnt = S.col_off[q];
-			S.col_off[q] = run;
-			S.col_cursor[q] = run;
+			const int cnt = T.col_off[q];
+			T.col_off[q] = run;
+			T.col_cursor[q] = run;
 			run += cnt;
 		}
-		S.col_off[n_colours] = run;
-		*S.n_colours = n_colours;
+		T.col_off[n_colours] = run;
+		if (T.n_colours_out) *T.n_colours_out = n_colours;
 	}
-	grid_sync(S.bar, W.err);
+	sy.sync();
 	for (int k = tid; k < n; k += stride) {
-		if (S.colour[k] < 0) continue;
-		const int pos = atomicAdd(&S.col_cursor[S.colour[k]], 1);
-		S.order[pos] = k;
+		const int a = arb_index(k);
+		const int col = S.colour[a];
+		if (col < 0) continue;
+		const int pos = atomicAdd(&T.col_cursor[col], 1);
+		T.order[pos] = a;
 	}
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----
-	for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[k]);
-	grid_sync(S.bar, W.err);
-	for (int jc = 0; jc < S.n_jcolours; jc++) {
-		for (int k = S.jcol_off[jc] + tid; k < S.jcol_off[jc + 1]; k += stride) joint_prestep(W, W.joints[S.jorder[k]]);
-		grid_sync(S.bar, W.err);
+	for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);
+	sy.sync();
+	for (int jc = 0; jc < T.njcol; jc++) {
+		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) joint_prestep(W, W.joints[T.jorder[k]]);
+		sy.sync();
 	}
 	// ---- velocity integration (cpSpaceStep.c:399-404, cpBody.c:493-509) ----
-	for (int q = tid; q < S.n_dyn_bodies; q += stride) {
-		const int b = S.dyn_bodies[q];
-		if (W.b_type[b] != CPGPU_BODY_DYNAMIC) continue;   // kinematic bodies keep their velocity
+	for (int q = tid; q < T.nb; q += stride) {
+		const int b = T.bodies ? T.bodies[q] : T.body0 + q;
+		if (W.b_type[b] != CPGPU_BODY_DYNAMIC) continue;   // kinematic and static bodies keep their velocity
 		const WorldStep ws = W.ws[W.b_world[b]];
 		BodyVel bv = W.b_vel[b];
 		const double2 f = W.b_f[b];
@@ -433,34 +481,133 @@ __global__ void __launch_bounds__(SOLVER_THREADS) k_solve(const DevWorld W, cons
 		W.b_f[b] = make_double2(0.0, 0.0);
 		W.b_t[b] = 0.0;
 	}
-	grid_sync(S.bar, W.err);
+	sy.sync();
 	// ---- cached impulses (cpSpaceStep.c:407-415) ----
 	for (int q = 0; q < n_colours; q++) {
-		for (int k = S.col_off[q] + tid; k < S.col_off[q + 1]; k += stride) arb_apply_cached(W, S.act[S.order[k]]);
-		grid_sync(S.bar, W.err);
+		for (int k = T.col_off[q] + tid; k < T.col_off[q + 1]; k += stride) arb_apply_cached(W, S.act[T.order[k]]);
+		sy.sync();
 	}
-	for (int jc = 0; jc < S.n_jcolours; jc++) {
-		for (int k = S.jcol_off[jc] + tid; k < S.jcol_off[jc + 1]; k += stride)
-			joint_apply_cached(W, W.joints[S.jorder[k]]);
-		grid_sync(S.bar, W.err);
+	for (int jc = 0; jc < T.njcol; jc++) {
+		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) joint_apply_cached(W, W.joints[T.jorder[k]]);
+		sy.sync();
 	}
 	// ---- the impulse iterations (cpSpaceStep.c:418-427) ----
-	for (int it = 0; it < S.max_iterations; it++) {
+	for (int it = 0; it < T.iterations; it++) {
 		for (int q = 0; q < n_colours; q++) {
-			for (int k = S.col_off[q] + tid; k < S.col_off[q + 1]; k += stride) {
-				Arbiter &arb = S.act[S.order[k]];
+			for (int k = T.col_off[q] + tid; k < T.col_off[q + 1]; k += stride) {
+				Arbiter &arb = S.act[T.order[k]];
 				if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, arb);
 			}
-			grid_sync(S.bar, W.err);
+			sy.sync();
 		}
-		for (int jc = 0; jc < S.n_jcolours; jc++) {
-			for (int k = S.jcol_off[jc] + tid; k < S.jcol_off[jc + 1]; k += stride) {
-				Joint &j = W.joints[S.jorder[k]];
+		for (int jc = 0; jc < T.njcol; jc++) {
+			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
+				Joint &j = W.joints[T.jorder[k]];
 				if (it < W.ws[j.world].iterations) joint_apply_impulse(W, j);
 			}
-			grid_sync(S.bar, W.err);
+			sy.sync();
 		}
 	}
+}
+
+// One workgroup per small world: every synchronisation is a workgroup barrier.
+__global__ void __launch_bounds__(WPW_THREADS) k_solve_worlds(const DevWorld W, const SolveBuffers S)
+{
+	__shared__ int32_t col_off[CPGPU_MAX_COLOURS + 1];
+	__shared__ int32_t col_cursor[CPGPU_MAX_COLOURS];
+	__shared__ int32_t left;
+	const int w = S.small_worlds[blockIdx.x];
+	SolveSet T;
+	if (S.arb_by_world) {
+		T.list = S.arb_by_world + S.warb_off[w];
+		T.n = S.warb_off[w + 1] - S.warb_off[w];
+		T.order = S.order + S.warb_off[w];
+	} else {   // a single space: every solved arbiter is its own
+		T.list = nullptr;
+		T.n = *S.n_act;
+		T.order = S.order;
+	}
+	T.col_off = col_off;
+	T.col_cursor = col_cursor;
+	T.left = &left;
+	T.n_colours_out = nullptr;
+	T.bodies = nullptr;
+	T.body0 = W.w_body0[w];
+	T.nb = W.w_body0[w + 1] - W.w_body0[w];
+	T.jorder = S.jw_order;
+	T.jcol_off = S.jw_col + S.jw_cbase[w];
+	T.njcol = S.jw_ncol[w];
+	T.iterations = W.ws[w].iterations;
+	BlockSync sy{(int)threadIdx.x, (int)blockDim.x};
+	solve_set(W, S, T, sy);
+}
+
+// All large worlds together on the co-resident grid.
+__global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, const SolveBuffers S)
+{
+	SolveSet T;
+	const int n_act = *S.n_act;
+	T.list = S.arb_by_world ? S.arb_by_world + S.large_first[0] : nullptr;
+	T.n = S.arb_by_world ? n_act - S.large_first[0] : n_act;
+	T.order = S.order + (S.arb_by_world ? S.large_first[0] : 0);
+	T.col_off = S.col_off;
+	T.col_cursor = S.col_cursor;
+	T.left = S.left;
+	T.n_colours_out = S.n_colours;
+	T.bodies = S.large_bodies;
+	T.body0 = 0;
+	T.nb = S.n_large_bodies;
+	T.jorder = S.jorder;
+	T.jcol_off = S.jcol_off;
+	T.njcol = S.n_jcolours;
+	T.iterations = S.max_iterations;
+	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err};
+	solve_set(W, S, T, sy);
+}
+
+// ---- sorting the solved arbiters by world: keys (large << 20 | world) ----
+
+__global__ void k_world_keys(const SolveBuffers S, uint64_t *keys, int32_t *vals, unsigned long long *orand)
+{
+	const int n = *S.n_act;
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	const bool valid = k < n;
+	unsigned long long key = 0;
+	if (valid) {
+		const int w = S.act[k].world;
+		key = ((unsigned long long)(S.world_small[w] ? 0 : 1) << 20) | (unsigned long long)w;
+		keys[k] = key;
+		vals[k] = k;
+	}
+	unsigned long long o = valid ? key : 0ull, a = valid ? key : ~0ull;
+	for (int off = 32; off > 0; off >>= 1) {
+		o |= __shfl_xor(o, off);
+		a &= __shfl_xor(a, off);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		atomicOr(&orand[0], o);
+		atomicAnd(&orand[1], a);
+	}
+}
+
+__device__ __forceinline__ int lower_bound_u64(const uint64_t *keys, int n, uint64_t key)
+{
+	int lo = 0, hi = n;
+	while (lo < hi) {
+		const int mid = (lo + hi) >> 1;
+		if (keys[mid] < key) lo = mid + 1; else hi = mid;
+	}
+	return lo;
+}
+
+// warb_off[w] = first sorted position of small world w (n_worlds + 1 entries);
+// large_first = first position of a large world's arbiter
+__global__ void k_world_offsets(const SolveBuffers S, const uint64_t *keys, int n_worlds)
+{
+	const int n = *S.n_act;
+	const int w = blockIdx.x * blockDim.x + threadIdx.x;
+	if (w <= n_worlds) S.warb_off[w] = lower_bound_u64(keys, n, (uint64_t)w);
+	if (w == n_worlds + 1) S.large_first[0] = lower_bound_u64(keys, n, 1ull << 20);
 }
 
 } // namespace
@@ -471,18 +618,44 @@ int solver_max_grid()
 	hipDeviceProp_t prop;
 	(void)hipGetDevice(&dev);
 	(void)hipGetDeviceProperties(&prop, dev);
-	(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_solve),
-	                                                   SOLVER_THREADS, 0);
+	(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_solve_grid),
+	                                                   GRID_THREADS, 0);
 	if (per_cu < 1) per_cu = 1;
 	return per_cu * prop.multiProcessorCount;
 }
 
-hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, int grid, hipStream_t stream, KernelTimer *timer)
+hipError_t solver_sort_by_world(const SolveBuffers &S, int n_act, int n_worlds, RadixSortBuffers &sortbuf,
+                                uint64_t *keys, hipStream_t stream, KernelTimer *timer)
 {
-	void *args[] = {(void *)&W, (void *)&S};
-	hipError_t e = hipSuccess;
-	TIMED(timer, "solver", stream,
-	      e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve), dim3(grid), dim3(SOLVER_THREADS),
-	                                     args, 0, stream));
-	return e;
+	if (n_act > 0) {
+		radix_sort_reserve(sortbuf, n_act);
+		HIP_TRY(hipMemsetAsync(sortbuf.orand, 0, sizeof(uint64_t), stream));
+		HIP_TRY(hipMemsetAsync(sortbuf.orand + 1, 0xFF, sizeof(uint64_t), stream));
+		TIMED(timer, "world_sort", stream,
+		      k_world_keys<<<(n_act + 255) / 256, 256, 0, stream>>>(S, keys, S.arb_by_world,
+		                                                             (unsigned long long *)sortbuf.orand));
+		HIP_TRY(hipGetLastError());
+		HIP_TRY(radix_sort_pairs(keys, S.arb_by_world, n_act, sortbuf, stream, timer));
+	}
+	k_world_offsets<<<(n_worlds + 2 + 255) / 256, 256, 0, stream>>>(S, keys, n_worlds);
+	return hipGetLastError();
+}
+
+hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, int n_small, int grid, hipStream_t stream,
+                       KernelTimer *timer)
+{
+	if (n_small > 0) {
+		TIMED(timer, "solver_worlds", stream,
+		      k_solve_worlds<<<n_small, WPW_THREADS, 0, stream>>>(W, S));
+		HIP_TRY(hipGetLastError());
+	}
+	if (grid > 0) {
+		void *args[] = {(void *)&W, (void *)&S};
+		hipError_t e = hipSuccess;
+		TIMED(timer, "solver", stream,
+		      e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_grid), dim3(grid),
+		                                     dim3(GRID_THREADS), args, 0, stream));
+		HIP_TRY(e);
+	}
+	return hipSuccess;
 }

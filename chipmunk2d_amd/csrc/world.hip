@@ -250,6 +250,18 @@ struct cpgpuWorld {
 	unsigned long long *claim = nullptr;
 	int32_t *taken = nullptr;
 	int32_t *d_jorder = nullptr, *d_jcol_off = nullptr;
+	int32_t *d_jw_order = nullptr, *d_jw_col = nullptr, *d_jw_cbase = nullptr, *d_jw_ncol = nullptr;
+	// small / large worlds
+	int small_world_max_bodies = 4096;
+	uint8_t *d_world_small = nullptr;
+	int32_t *d_small_worlds = nullptr;
+	int n_small = 0, n_large_worlds = 0;
+	int32_t *d_large_bodies = nullptr;
+	int n_large_bodies = 0;
+	int max_iterations_large = 0;
+	int32_t *arb_by_world = nullptr;
+	uint64_t *world_keys = nullptr;
+	int32_t *warb_off = nullptr, *large_first = nullptr;
 	int n_jcolours = 0;
 	std::vector<int32_t> h_jcolour;
 	GridBarrier *bar = nullptr;
@@ -510,6 +522,9 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		int32_t *pst, *pl;
 		HIP_OR_FAIL(own_upload(w, pst, start.data(), nb + 1)); W.nc_start = pst;
 		HIP_OR_FAIL(own_upload(w, pl, list.data(), list.size())); W.nc_list = pl;
+		int32_t *pw0, *pb0;
+		HIP_OR_FAIL(own_upload(w, pw0, d->world_shape_start, w->nw + 1)); W.w_shape0 = pw0;
+		HIP_OR_FAIL(own_upload(w, pb0, d->world_body_start, w->nw + 1)); W.w_body0 = pb0;
 	}
 
 	// joints, coloured once
@@ -529,17 +544,76 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 			w->joint_error_bias[c] = j.errorBias;
 			ba[c] = d->body_type[j.a] == CPGPU_BODY_DYNAMIC ? j.a : -1;
 			bbv[c] = d->body_type[j.b] == CPGPU_BODY_DYNAMIC ? j.b : -1;
-			prio[c] = mix64_host((uint64_t)(uint32_t)c | 0x8000000000000000ull);
+			prio[c] = mix64_host((uint64_t)(uint32_t)(c - d->world_constraint_start[j.world]) | 0x8000000000000000ull);
 		}
-		w->n_jcolours = colour_host(nb, ba, bbv, prio, w->h_jcolour);
-		std::vector<int32_t> jorder(nc), joff(w->n_jcolours + 1, 0);
-		for (int c = 0; c < nc; c++) joff[w->h_jcolour[c] + 1]++;
-		for (int q = 0; q < w->n_jcolours; q++) joff[q + 1] += joff[q];
-		std::vector<int32_t> cursor(joff.begin(), joff.end() - 1);
-		for (int c = 0; c < nc; c++) jorder[cursor[w->h_jcolour[c]]++] = c;
+		// worlds are coloured independently (disjoint bodies), so one pass colours them all
+		const int max_col = colour_host(nb, ba, bbv, prio, w->h_jcolour);
+		(void)max_col;
+
+		// small worlds (one workgroup each) and large worlds (the grid solve)
+		std::vector<uint8_t> small(w->nw);
+		std::vector<int32_t> small_list, large_bodies;
+		for (int k = 0; k < w->nw; k++) {
+			int ndyn = 0;
+			for (int b = d->world_body_start[k]; b < d->world_body_start[k + 1]; b++)
+				ndyn += d->body_type[b] != CPGPU_BODY_STATIC;
+			small[k] = ndyn <= w->small_world_max_bodies;
+			if (small[k]) small_list.push_back(k);
+		}
+		for (int b = 0; b < nb; b++)
+			if (d->body_type[b] != CPGPU_BODY_STATIC && !small[body_world[b]]) large_bodies.push_back(b);
+		w->n_small = (int)small_list.size();
+		w->n_large_worlds = w->nw - w->n_small;
+		w->n_large_bodies = (int)large_bodies.size();
+		w->max_iterations_large = 0;
+		for (int k = 0; k < w->nw; k++)
+			if (!small[k]) w->max_iterations_large = std::max(w->max_iterations_large, (int)w->sp[k].iterations);
+		HIP_OR_FAIL(own_upload(w, w->d_world_small, small.data(), small.size()));
+		HIP_OR_FAIL(own_upload(w, w->d_small_worlds, small_list.data(), small_list.size()));
+		HIP_OR_FAIL(own_upload(w, w->d_large_bodies, large_bodies.data(), large_bodies.size()));
+
+		// large worlds' joints: by colour
+		int ncol_large = 0;
+		for (int c = 0; c < nc; c++)
+			if (!small[J[c].world]) ncol_large = std::max(ncol_large, w->h_jcolour[c] + 1);
+		std::vector<int32_t> jorder, joff(ncol_large + 1, 0);
+		for (int c = 0; c < nc; c++)
+			if (!small[J[c].world]) joff[w->h_jcolour[c] + 1]++;
+		for (int q = 0; q < ncol_large; q++) joff[q + 1] += joff[q];
+		jorder.assign(joff[ncol_large], 0);
+		{
+			std::vector<int32_t> cursor(joff.begin(), joff.end() - 1);
+			for (int c = 0; c < nc; c++)
+				if (!small[J[c].world]) jorder[cursor[w->h_jcolour[c]]++] = c;
+		}
+		w->n_jcolours = ncol_large;
+		// every world's joints: by (world, colour), with per-world colour offsets
+		std::vector<int32_t> jw_order(nc), jw_col, jw_cbase(w->nw), jw_ncol(w->nw);
+		{
+			int pos = 0;
+			for (int k = 0; k < w->nw; k++) {
+				const int c0 = d->world_constraint_start ? d->world_constraint_start[k] : 0;
+				const int c1 = d->world_constraint_start ? d->world_constraint_start[k + 1] : 0;
+				int ncol = 0;
+				for (int c = c0; c < c1; c++) ncol = std::max(ncol, w->h_jcolour[c] + 1);
+				jw_ncol[k] = ncol;
+				jw_cbase[k] = (int32_t)jw_col.size();
+				std::vector<int32_t> cnt(ncol + 1, 0);
+				for (int c = c0; c < c1; c++) cnt[w->h_jcolour[c] + 1]++;
+				for (int q = 0; q < ncol; q++) cnt[q + 1] += cnt[q];
+				for (int q = 0; q <= ncol; q++) jw_col.push_back(pos + cnt[q]);
+				std::vector<int32_t> cursor(cnt.begin(), cnt.end() - 1);
+				for (int c = c0; c < c1; c++) jw_order[pos + cursor[w->h_jcolour[c]]++] = c;
+				pos += c1 - c0;
+			}
+		}
 		HIP_OR_FAIL(own_upload(w, W.joints, J.data(), nc));
-		HIP_OR_FAIL(own_upload(w, w->d_jorder, jorder.data(), nc));
+		HIP_OR_FAIL(own_upload(w, w->d_jorder, jorder.data(), jorder.size()));
 		HIP_OR_FAIL(own_upload(w, w->d_jcol_off, joff.data(), joff.size()));
+		HIP_OR_FAIL(own_upload(w, w->d_jw_order, jw_order.data(), jw_order.size()));
+		HIP_OR_FAIL(own_upload(w, w->d_jw_col, jw_col.data(), jw_col.size()));
+		HIP_OR_FAIL(own_upload(w, w->d_jw_cbase, jw_cbase.data(), jw_cbase.size()));
+		HIP_OR_FAIL(own_upload(w, w->d_jw_ncol, jw_ncol.data(), jw_ncol.size()));
 	}
 
 	// per-world step parameters, counters, error word
@@ -559,6 +633,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_upload(w, w->col_cursor, (const int32_t *)nullptr, CPGPU_MAX_COLOURS));
 	HIP_OR_FAIL(own_upload(w, w->n_colours, (const int32_t *)nullptr, 1));
 	HIP_OR_FAIL(own_upload(w, w->left, (const int32_t *)nullptr, 1));
+	HIP_OR_FAIL(own_upload(w, w->warb_off, (const int32_t *)nullptr, w->nw + 2));
+	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 1));
 	HIP_OR_FAIL(own_alloc(w, w->claim, 2 * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->taken, nb));
 	w->max_iterations = 0;
@@ -588,7 +664,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		HIP_OR_FAIL(regrow(w->act[k], w->act_cap[k], base));
 		HIP_OR_FAIL(regrow(w->cached[k], w->cached_cap[k], base));
 	}
-	w->solver_grid = std::min(solver_max_grid(), 1024);
+	// the grid solve: few large workgroups keep its barriers cheap
+	w->solver_grid = std::min(solver_max_grid(), 64);
 	if (w->solver_grid < 1) w->solver_grid = 1;
 	*out = w;
 	return CPGPU_OK;
@@ -603,7 +680,7 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	lbvh_free(w->static_tree);
 	radix_sort_free(w->sortbuf);
 	void *extra[] = {w->d_pairs, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
-	                 w->tkeys, w->tvals, w->seen, w->colour, w->order};
+	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys};
 	for (void *p : extra) if (p) (void)hipFree(p);
 	if (w->h_ws) (void)hipHostFree(w->h_ws);
 	if (w->h_counts) (void)hipHostFree(w->h_counts);
@@ -694,11 +771,13 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	}
 	HIP_OR_FAIL(regrow(w->seen, w->seen_cap, (long)n_prev_recs + 1));
 	if (w->act_cap[cur] > w->colour_cap) {
-		if (w->colour) (void)hipFree(w->colour);
-		if (w->order) (void)hipFree(w->order);
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys};
+		for (void *p : old) if (p) (void)hipFree(p);
 		w->colour_cap = w->act_cap[cur];
 		HIP_OR_FAIL(dalloc(w->colour, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->order, w->colour_cap));
+		HIP_OR_FAIL(dalloc(w->arb_by_world, w->colour_cap));
+		HIP_OR_FAIL(dalloc(w->world_keys, w->colour_cap));
 	}
 	if (w->steps == 0) HIP_OR_FAIL(hipMemsetAsync(w->d_step_counts + 3 * prev, 0, 3 * sizeof(int32_t), w->stream));
 
@@ -715,14 +794,25 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	SolveBuffers S;
 	S.act = w->act[cur]; S.n_act = B.n_act;
 	S.colour = w->colour; S.order = w->order;
+	S.arb_by_world = w->nw > 1 ? w->arb_by_world : nullptr;
+	S.warb_off = w->warb_off; S.large_first = w->large_first;
+	S.world_small = w->d_world_small; S.small_worlds = w->d_small_worlds;
 	S.col_off = w->col_off; S.col_cursor = w->col_cursor; S.n_colours = w->n_colours; S.left = w->left;
 	S.claim = w->claim; S.taken = w->taken;
 	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
+	S.jw_order = w->d_jw_order; S.jw_col = w->d_jw_col; S.jw_cbase = w->d_jw_cbase; S.jw_ncol = w->d_jw_ncol;
 	S.bar = w->bar;
-	S.max_iterations = w->max_iterations;
+	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;
-	S.dyn_bodies = w->d_dyn_bodies; S.n_dyn_bodies = w->n_dyn_bodies;
-	HIP_OR_FAIL(solver_step(W, S, w->solver_grid, w->stream, timer));
+	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
+	if (w->nw > 1) {
+		// the solved arbiters of each world (one sync for the count)
+		HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, B.n_act, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
+		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+		w->timer.collect();
+		HIP_OR_FAIL(solver_sort_by_world(S, w->h_counts[10], w->nw, w->sortbuf, w->world_keys, w->stream, timer));
+	}
+	HIP_OR_FAIL(solver_step(W, S, w->n_small, w->n_large_worlds > 0 ? w->solver_grid : 0, w->stream, timer));
 
 	w->cur ^= 1;
 	w->steps++;
@@ -796,12 +886,15 @@ int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
 	if (w->steps > 0) {
 		out->arbiters = w->h_counts[3 * last + 1];
 		out->cached_arbiters = w->h_counts[3 * last + 1] + w->h_counts[3 * last + 2];
-		out->arbiter_colours = w->h_counts[8];
-		// contacts of the solved arbiters
+		// contacts and colours of the solved arbiters (colours count per space; the largest is reported)
 		std::vector<Arbiter> arbs(out->arbiters);
-		if (out->arbiters)
+		std::vector<int32_t> col(out->arbiters);
+		if (out->arbiters) {
 			HIP_OR_FAIL(hipMemcpy(arbs.data(), w->act[last], sizeof(Arbiter) * out->arbiters, hipMemcpyDeviceToHost));
+			HIP_OR_FAIL(hipMemcpy(col.data(), w->colour, sizeof(int32_t) * out->arbiters, hipMemcpyDeviceToHost));
+		}
 		for (auto &a : arbs) out->contacts += a.count;
+		for (int32_t c : col) out->arbiter_colours = std::max(out->arbiter_colours, c + 1);
 	}
 	out->constraint_colours = w->n_jcolours;
 	out->steps = w->steps;

@@ -223,17 +223,46 @@ def chain_world(n_links=200, n_circles=300, seed=SEED_CHAINS, rng=None):
     return space
 
 
-def batch_worlds(n_worlds=4096, lo=16, hi=2048, seed=SEED_BATCH, pyramid_fraction=0.5):
-    """Independent spaces with body counts log-uniform in [lo, hi]; each is a
-    pyramid stack (rows chosen so rows*(rows+1)/2 is near the count) or a
-    random pile, on its own ground."""
+def batch_specs(n_worlds=4096, lo=16, hi=2048, seed=SEED_BATCH, pyramid_fraction=0.5):
+    """(kind, body count, world seed) of each space of a batch: counts
+    log-uniform in [lo, hi], pyramid or random pile, each with its own seed
+    drawn from the batch seed (so worlds can be built in any order)."""
     rng = Rng(seed)
-    spaces = []
+    specs = []
     for _ in range(n_worlds):
         n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
-        if rng.uniform() < pyramid_fraction:
-            rows = max(1, int((math.sqrt(8 * n + 1) - 1) / 2))
-            spaces.append(pyramid_210(rows=rows))
-        else:
-            spaces.append(random_pile(n, rng))
-    return spaces
+        kind = "pyramid" if rng.uniform() < pyramid_fraction else "pile"
+        specs.append((kind, n, rng.next_u64()))
+    return specs
+
+
+def build_batch_world(spec):
+    """One space of a batch: a pyramid stack (rows chosen so rows*(rows+1)/2
+    is near the count) or a random pile, on its own ground."""
+    kind, n, wseed = spec
+    if kind == "pyramid":
+        rows = max(1, int((math.sqrt(8 * n + 1) - 1) / 2))
+        return pyramid_210(rows=rows)
+    return random_pile(n, Rng(wseed))
+
+
+def batch_worlds(n_worlds=4096, lo=16, hi=2048, seed=SEED_BATCH, pyramid_fraction=0.5):
+    """Independent spaces with body counts log-uniform in [lo, hi]."""
+    return [build_batch_world(s) for s in batch_specs(n_worlds, lo, hi, seed, pyramid_fraction)]
+
+
+def _pack_spec(spec):
+    return build_batch_world(spec).pack()
+
+
+def batch_packed(n_worlds=4096, lo=16, hi=2048, seed=SEED_BATCH, pyramid_fraction=0.5, processes=8):
+    """The batch packed into one description, worlds built in worker processes."""
+    from .cabi import concat
+    specs = batch_specs(n_worlds, lo, hi, seed, pyramid_fraction)
+    if processes and processes > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(processes) as pool:
+            packed = pool.map(_pack_spec, specs, chunksize=16)
+    else:
+        packed = [_pack_spec(s) for s in specs]
+    return concat(packed)

@@ -356,7 +356,8 @@ oworld *ow_new(const cpgpuWorldDesc *d)
 		double j0 = d->cons_jacc ? d->cons_jacc[2 * i] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * i + 1] : 0.0;
 		j->jAcc2 = v2(j0, j1);
 		j->jAcc = j0; j->jnAcc = j0;
-		j->prio = mix64((uint64_t)(uint32_t)i | 0x8000000000000000ull);
+		/* colouring priority from the constraint's index within its own space */
+		j->prio = mix64((uint64_t)(uint32_t)(i - w->W[w->B[j->a].world].c0) | 0x8000000000000000ull);
 	}
 	for (int wi = 0; wi < w->nw; wi++) {
 		OWorldPart *P = &w->W[wi];
@@ -1492,7 +1493,9 @@ static void step_world(oworld *w, OWorldPart *P, double dt, const int32_t *order
 			ai = arb_append(P);
 			OArb *na = &P->arbs[ai];
 			na->key = key;
-			na->prio = mix64(key);
+			/* colouring priority from the shape indices within the space, so a
+			 * space colours the same alone or inside a batch */
+			na->prio = mix64(pair_key(ia - P->s0, ib - P->s0));
 			na->count = 0;
 			na->state = CPGPU_ARB_FIRST_COLLISION;
 			na->stamp = 0;

@@ -100,6 +100,23 @@ def test_batch_of_worlds_bit_exact():
     run_lockstep(pw, 120, check_every=10, arbiters_every=40)
 
 
+def test_grid_solve_mid_size_world_bit_exact():
+    """More dynamic bodies than one workgroup takes: the co-resident grid solve."""
+    pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=260.0, seed=12).pack()
+    assert pw.dynamic_body_count() > 4096
+    g, o = run_lockstep(pw, 60, check_every=2, arbiters_every=30)
+    assert g.stats()["arbiters"] > 300
+
+
+def test_mixed_batch_grid_and_workgroup_solves_bit_exact():
+    """One large space (grid solve) and small spaces (a workgroup each) in one step."""
+    rng = Rng(0x1A46E)
+    big = random_world(n_circles=2400, n_polys=2400, n_segments=300, extent=250.0, seed=13).pack()
+    spaces = [pyramid_210(rows=5).pack(), big, random_pile(120, rng).pack(), pyramid_210(rows=8).pack()]
+    pw = concat(spaces)
+    run_lockstep(pw, 60, check_every=5, arbiters_every=30)
+
+
 def test_worlds_in_a_batch_step_as_when_alone():
     """A space stepped inside a batch ends bit-identical to the same space stepped alone."""
     rng = Rng(77)

@@ -23,7 +23,7 @@ constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int MORTON_BITS = 22;
 constexpr int WORLD_SHIFT = 44;
-constexpr int TRAVERSAL_STACK = 64;
+constexpr int TRAVERSAL_STACK = 48;
 
 __device__ __forceinline__ unsigned long long ord_of(double x)
 {
@@ -150,35 +150,57 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, in
 	hist[tid * nblocks + blockIdx.x] = h[tid];
 }
 
-// exclusive scan of `total` counters in one workgroup of 1024 lanes
-__global__ void __launch_bounds__(1024) k_rs_scan(uint32_t *hist, int total)
+// exclusive scan of one value per lane over a 256-lane workgroup; *total gets the sum
+__device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t *lds4, uint32_t *total)
 {
-	__shared__ uint32_t sums[1024];
-	const int tid = threadIdx.x;
-	const int chunk = (total + 1023) / 1024;
-	const int lo = tid * chunk;
-	const int hi = min(lo + chunk, total);
-	uint32_t s = 0;
-	for (int i = lo; i < hi; i++) s += hist[i];
-	sums[tid] = s;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	uint32_t x = v;
+	for (int off = 1; off < 64; off <<= 1) {
+		const uint32_t t = __shfl_up(x, off);
+		if (lane >= off) x += t;
+	}
+	if (lane == 63) lds4[wave] = x;
 	__syncthreads();
-	for (int off = 1; off < 1024; off <<= 1) {
-		const uint32_t v = tid >= off ? sums[tid - off] : 0u;
-		__syncthreads();
-		sums[tid] += v;
-		__syncthreads();
+	uint32_t pre = 0, tot = 0;
+	for (int q = 0; q < 4; q++) {
+		if (q < wave) pre += lds4[q];
+		tot += lds4[q];
 	}
-	uint32_t run = tid > 0 ? sums[tid - 1] : 0u;
-	for (int i = lo; i < hi; i++) {
-		const uint32_t c = hist[i];
-		hist[i] = run;
-		run += c;
+	__syncthreads();
+	*total = tot;
+	return pre + x - v;
+}
+
+// exclusive scan of each digit's row of block counts (one workgroup per digit);
+// the row total goes to bin_total[digit]
+__global__ void __launch_bounds__(256) k_rs_scan_rows(uint32_t *hist, int nblocks, uint32_t *bin_total)
+{
+	__shared__ uint32_t lds4[4];
+	uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
+	uint32_t run = 0;
+	for (int base = 0; base < nblocks; base += 256) {
+		const int i = base + threadIdx.x;
+		const uint32_t v = i < nblocks ? row[i] : 0u;
+		uint32_t tot;
+		const uint32_t ex = block_exclusive_scan256(v, lds4, &tot);
+		if (i < nblocks) row[i] = run + ex;
+		run += tot;
 	}
+	if (threadIdx.x == 0) bin_total[blockIdx.x] = run;
+}
+
+// exclusive scan of the 256 digit totals, in place
+__global__ void __launch_bounds__(256) k_rs_scan_bins(uint32_t *bin_total)
+{
+	__shared__ uint32_t lds4[4];
+	uint32_t tot;
+	const uint32_t ex = block_exclusive_scan256(bin_total[threadIdx.x], lds4, &tot);
+	bin_total[threadIdx.x] = ex;
 }
 
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *kin, const int32_t *vin, uint64_t *kout,
                                                          int32_t *vout, int n, int shift, const uint32_t *offsets,
-                                                         int nblocks)
+                                                         int nblocks, const uint32_t *bin_base)
 {
 	__shared__ uint32_t base[256];
 	__shared__ uint32_t run[256];
@@ -186,7 +208,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *kin, 
 	const int tid = threadIdx.x;
 	const int wave = tid >> 6, lane = tid & 63;
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-	base[tid] = offsets[tid * nblocks + blockIdx.x];
+	base[tid] = offsets[tid * nblocks + blockIdx.x] + bin_base[tid];
 	run[tid] = 0;
 	for (int r = 0; r < RS_ITEMS; r++) {
 		const int i = blockIdx.x * RS_TILE + r * RS_THREADS + tid;
@@ -292,15 +314,18 @@ __device__ __forceinline__ double4 load_box_coherent(const double4 *p)
 // A node whose leaves all belong to this workgroup's chunk was completed by
 // lanes of this workgroup, so workgroup-scope ordering suffices; only nodes
 // spanning chunks need device-scope fences (which write back / invalidate L2).
-__global__ void k_refit(const DevWorld W, int n, const int32_t *leaf_shape, const int32_t *child,
-                        const int32_t *parent, const int32_t *range, double4 *box, uint32_t *visits)
+__global__ void k_refit(const DevWorld W, int n, const uint64_t *keys, const int32_t *leaf_shape,
+                        const int32_t *child, const int32_t *parent, const int32_t *range, const int32_t *world_root,
+                        double4 *box, uint32_t *visits)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const int chunk_lo = blockIdx.x * blockDim.x, chunk_hi = chunk_lo + blockDim.x - 1;
+	// traversals start at the world roots: nodes above them are never read
+	const int root = world_root[(int)(keys[i] >> WORLD_SHIFT)];
 	int node = (n - 1) + i;
 	box[node] = W.s_bb[leaf_shape[i]];
-	int p = parent[node];
+	int p = node == root ? -1 : parent[node];
 	while (p >= 0) {
 		const bool local = range[2 * p] >= chunk_lo && range[2 * p + 1] <= chunk_hi;
 		if (local) {
@@ -313,7 +338,7 @@ __global__ void k_refit(const DevWorld W, int n, const int32_t *leaf_shape, cons
 			box[p] = box_union(load_box_coherent(&box[child[2 * p]]), load_box_coherent(&box[child[2 * p + 1]]));
 		}
 		node = p;
-		p = parent[p];
+		p = node == root ? -1 : parent[p];
 	}
 }
 
@@ -368,38 +393,54 @@ __global__ void __launch_bounds__(256) k_pairs(const DevWorld W, int nd, const u
 	const int s = dshape[q];
 	const double4 bb = W.s_bb[s];
 	const int w = (int)(dkeys[q] >> WORLD_SHIFT);
-	int stack[TRAVERSAL_STACK];
+	// per-lane stack in LDS, laid out [depth][lane] so lanes at one depth hit distinct banks
+	__shared__ int32_t stk[TRAVERSAL_STACK][256];
+	const int lane = threadIdx.x;
 	int sp = 0;
-	// dynamic tree: leaves after q only
-	stack[sp++] = droot[w];
+	// dynamic tree: each pair is reported by its lower leaf, so only leaves after q
+	{
+		const int root = droot[w];
+		if (root >= nd - 1) {
+			// a one-leaf world: nothing after q
+		} else if (drange[2 * root + 1] > q && box_overlap(bb, dbox[root])) {
+			stk[sp++][lane] = root;
+		}
+	}
 	while (sp > 0) {
-		const int node = stack[--sp];
-		if (node >= nd - 1) {
-			const int p = node - (nd - 1);
-			if (p <= q) continue;
-			const int t = dshape[p];
-			if (box_overlap(bb, W.s_bb[t]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
-		} else {
-			if (drange[2 * node + 1] <= q) continue;
-			if (!box_overlap(bb, dbox[node])) continue;
-			if (sp + 2 > TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
-			stack[sp++] = dchild[2 * node];
-			stack[sp++] = dchild[2 * node + 1];
+		const int node = stk[--sp][lane];
+		for (int side = 0; side < 2; side++) {
+			const int c = dchild[2 * node + side];
+			if (c >= nd - 1) {
+				const int p = c - (nd - 1);
+				if (p <= q) continue;
+				const int t = dshape[p];
+				if (box_overlap(bb, dbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
+			} else if (drange[2 * c + 1] > q && box_overlap(bb, dbox[c])) {
+				if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
+				stk[sp++][lane] = c;
+			}
 		}
 	}
 	// static tree
 	if (ns > 0 && sroot[w] >= 0) {
-		stack[sp++] = sroot[w];
+		const int root = sroot[w];
+		if (root >= ns - 1) {
+			const int t = sshape[root - (ns - 1)];
+			if (box_overlap(bb, sbox[root]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
+		} else if (box_overlap(bb, sbox[root])) {
+			stk[sp++][lane] = root;
+		}
 		while (sp > 0) {
-			const int node = stack[--sp];
-			if (node >= ns - 1) {
-				const int t = sshape[node - (ns - 1)];
-				if (box_overlap(bb, W.s_bb[t]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
-			} else {
-				if (!box_overlap(bb, sbox[node])) continue;
-				if (sp + 2 > TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
-				stack[sp++] = schild[2 * node];
-				stack[sp++] = schild[2 * node + 1];
+			const int node = stk[--sp][lane];
+			for (int side = 0; side < 2; side++) {
+				const int c = schild[2 * node + side];
+				if (c >= ns - 1) {
+					const int t = sshape[c - (ns - 1)];
+					if (box_overlap(bb, sbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
+				} else if (box_overlap(bb, sbox[c])) {
+					if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
+					stk[sp++][lane] = c;
+				}
 			}
 		}
 	}
@@ -422,6 +463,7 @@ void radix_sort_reserve(RadixSortBuffers &b, int n)
 	(void)hipMalloc(&b.vals_alt, sizeof(int32_t) * cap);
 	(void)hipMalloc(&b.block_hist, sizeof(uint32_t) * 256 * nblocks);
 	(void)hipMalloc(&b.orand, sizeof(uint64_t) * 2);
+	(void)hipMalloc(&b.bin_total, sizeof(uint32_t) * 256);
 	b.cap = cap;
 }
 
@@ -431,6 +473,7 @@ void radix_sort_free(RadixSortBuffers &b)
 	if (b.vals_alt) (void)hipFree(b.vals_alt);
 	if (b.block_hist) (void)hipFree(b.block_hist);
 	if (b.orand) (void)hipFree(b.orand);
+	if (b.bin_total) (void)hipFree(b.bin_total);
 	b = RadixSortBuffers();
 }
 
@@ -450,8 +493,10 @@ hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n, RadixSortBuffe
 		const int shift = 8 * d;
 		TIMED(timer, "radix_sort", stream, {
 			k_rs_hist<<<nblocks, RS_THREADS, 0, stream>>>(kin, n, shift, b.block_hist, nblocks);
-			k_rs_scan<<<1, 1024, 0, stream>>>(b.block_hist, 256 * nblocks);
-			k_rs_scatter<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, n, shift, b.block_hist, nblocks);
+			k_rs_scan_rows<<<256, 256, 0, stream>>>(b.block_hist, nblocks, b.bin_total);
+			k_rs_scan_bins<<<1, 256, 0, stream>>>(b.bin_total);
+			k_rs_scatter<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, n, shift, b.block_hist, nblocks,
+			                                                 b.bin_total);
 		});
 		HIP_TRY(hipGetLastError());
 		uint64_t *tk = kin; kin = kout; kout = tk;
@@ -517,8 +562,8 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 		if (n > 1)
 			k_karras<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.child, t.parent, t.range,
 			                                                      t.world_first, t.world_last, t.world_root);
-		k_refit<<<blocks_for(n, 256), 256, 0, stream>>>(W, n, t.leaf_shape, t.child, t.parent, t.range, t.box,
-		                                                 t.visits);
+		k_refit<<<blocks_for(n, 256), 256, 0, stream>>>(W, n, t.keys, t.leaf_shape, t.child, t.parent, t.range,
+		                                                 t.world_root, t.box, t.visits);
 	});
 	return hipGetLastError();
 }

@@ -61,6 +61,7 @@ struct RadixSortBuffers {
 	int32_t *vals_alt = nullptr;
 	uint32_t *block_hist = nullptr;   // [256][n_blocks]
 	uint64_t *orand = nullptr;        // [2]: OR and AND of all keys
+	uint32_t *bin_total = nullptr;    // [256] per-digit totals, then their exclusive scan
 	int cap = 0;
 };
 void radix_sort_reserve(RadixSortBuffers &b, int n);

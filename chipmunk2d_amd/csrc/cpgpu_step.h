@@ -67,6 +67,13 @@ struct SolveBuffers {
 	const int32_t *jw_cbase;   // [n_worlds] first entry of the world's offsets in jw_col
 	const int32_t *jw_ncol;    // [n_worlds]
 	GridBarrier *bar;
+	// grid solve: colours (and the last uncoloured arbiters) with few arbiters
+	// are handled by workgroup 0 alone instead of paying a grid barrier each
+	int32_t *tail_list;        // [colour_tail_max]
+	int32_t *tail_n;           // [1]
+	int32_t *tail_state;       // [2] colour and sub-round after the tail colouring
+	int colour_tail_max;
+	int solve_tail_max;
 	int max_iterations;        // over the large worlds
 	int n_bodies;
 	const int32_t *large_bodies;   // non-static bodies of large worlds

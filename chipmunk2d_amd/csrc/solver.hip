@@ -344,10 +344,12 @@ __device__ __forceinline__ unsigned long long arbiter_priority(const DevWorld &W
 // Synchronisation of the threads taking part: one workgroup, or the whole
 // co-resident grid.
 struct BlockSync {
+	static constexpr bool grid = false;
 	int tid, stride;
 	__device__ void sync() { __syncthreads(); }
 };
 struct GridSync {
+	static constexpr bool grid = true;
 	int tid, stride;
 	GridBarrier *bar;
 	int32_t *err;
@@ -373,6 +375,69 @@ struct SolveSet {
 	int iterations;            // sweeps (elements of worlds with fewer sweeps skip the extra ones)
 };
 
+// One colouring claim round over arbiters list[0..n) (see the file comment).
+// Returns the number of arbiters that took colour c.
+__device__ int colour_round(const DevWorld &W, const SolveBuffers &S, const int32_t *list, int n, int tid,
+                            int stride, int c, unsigned long long *cl, unsigned long long *other, bool claim_phase)
+{
+	int won = 0;
+	for (int k = tid; k < n; k += stride) {
+		const int a = list ? list[k] : k;
+		if (S.colour[a] >= 0) continue;
+		const Arbiter &arb = S.act[a];
+		const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
+		const unsigned long long prio = arbiter_priority(W, arb);
+		if (claim_phase) {
+			if ((ba >= 0 && S.taken[ba] == c) || (bb >= 0 && S.taken[bb] == c)) continue;
+			if (ba >= 0) atomicMin(&cl[ba], prio);
+			if (bb >= 0) atomicMin(&cl[bb], prio);
+		} else {
+			const bool win = (ba < 0 || __hip_atomic_load(&cl[ba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio) &&
+			                 (bb < 0 || __hip_atomic_load(&cl[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio);
+			if (win) {
+				S.colour[a] = c;
+				if (ba >= 0) S.taken[ba] = c;
+				if (bb >= 0) S.taken[bb] = c;
+				won++;
+			} else {
+				if (ba >= 0) other[ba] = ~0ull;
+				if (bb >= 0) other[bb] = ~0ull;
+			}
+		}
+	}
+	return won;
+}
+
+// Run f(k) for every position k of colours [0, n_colours) in colour order,
+// synchronising between colours.  On the grid, a run of colours with at most
+// tail_max arbiters each is handled by workgroup 0 alone, with workgroup
+// barriers between those colours and one grid barrier after the run: the
+// sequential order is unchanged, the grid barriers of small colours go away.
+template <class Sync, class F>
+__device__ void for_colours(Sync &sy, const int32_t *col_off, int n_colours, int tail_max, F f)
+{
+	int q = 0;
+	while (q < n_colours) {
+		const int cnt = col_off[q + 1] - col_off[q];
+		if (!Sync::grid || cnt > tail_max) {
+			for (int k = col_off[q] + sy.tid; k < col_off[q + 1]; k += sy.stride) f(k);
+			sy.sync();
+			q++;
+		} else {
+			int q1 = q + 1;
+			while (q1 < n_colours && col_off[q1 + 1] - col_off[q1] <= tail_max) q1++;
+			if (blockIdx.x == 0) {
+				for (int r = q; r < q1; r++) {
+					for (int k = col_off[r] + (int)threadIdx.x; k < col_off[r + 1]; k += (int)blockDim.x) f(k);
+					__syncthreads();
+				}
+			}
+			sy.sync();
+			q = q1;
+		}
+	}
+}
+
 template <class Sync>
 __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy)
 {
@@ -389,45 +454,48 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		if (bb >= 0) { claim0[bb] = ~0ull; claim1[bb] = ~0ull; S.taken[bb] = -1; }
 	}
 	for (int k = tid; k <= CPGPU_MAX_COLOURS; k += stride) T.col_off[k] = 0;
-	if (tid == 0) *T.left = n;
+	if (tid == 0) {
+		*T.left = n;
+		if (Sync::grid) *S.tail_n = 0;
+	}
 	sy.sync();
 
 	int c = 0, sub = 0;
 	while (true) {
-		if (__hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+		const int left = __hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (left == 0) break;
 		if (c >= CPGPU_MAX_COLOURS) { if (tid == 0) raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); break; }
+		if (Sync::grid && left <= S.colour_tail_max) {
+			// the last few arbiters: gather them, then workgroup 0 finishes the colouring alone
+			for (int k = tid; k < n; k += stride) {
+				const int a = arb_index(k);
+				if (S.colour[a] < 0) S.tail_list[atomicAdd(S.tail_n, 1)] = a;
+			}
+			sy.sync();
+			if (blockIdx.x == 0) {
+				const int nt = __hip_atomic_load(S.tail_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				while (__hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0 && c < CPGPU_MAX_COLOURS) {
+					unsigned long long *cl = (sub & 1) ? claim1 : claim0;
+					unsigned long long *other = (sub & 1) ? claim0 : claim1;
+					colour_round(W, S, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other, true);
+					__syncthreads();
+					const int won = colour_round(W, S, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other, false);
+					if (won) atomicSub(T.left, won);
+					__syncthreads();
+					if (++sub % COLOUR_SUBROUNDS == 0) c++;
+				}
+				if (threadIdx.x == 0) { S.tail_state[0] = c; S.tail_state[1] = sub; }
+			}
+			sy.sync();
+			c = __hip_atomic_load(&S.tail_state[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			sub = __hip_atomic_load(&S.tail_state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			break;
+		}
 		unsigned long long *cl = (sub & 1) ? claim1 : claim0;
 		unsigned long long *other = (sub & 1) ? claim0 : claim1;
-		for (int k = tid; k < n; k += stride) {
-			const int a = arb_index(k);
-			if (S.colour[a] >= 0) continue;
-			const Arbiter &arb = S.act[a];
-			const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
-			if ((ba >= 0 && S.taken[ba] == c) || (bb >= 0 && S.taken[bb] == c)) continue;
-			const unsigned long long prio = arbiter_priority(W, arb);
-			if (ba >= 0) atomicMin(&cl[ba], prio);
-			if (bb >= 0) atomicMin(&cl[bb], prio);
-		}
+		colour_round(W, S, T.list, n, tid, stride, c, cl, other, true);
 		sy.sync();
-		int won = 0;
-		for (int k = tid; k < n; k += stride) {
-			const int a = arb_index(k);
-			if (S.colour[a] >= 0) continue;
-			const Arbiter &arb = S.act[a];
-			const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
-			const unsigned long long prio = arbiter_priority(W, arb);
-			const bool win = (ba < 0 || __hip_atomic_load(&cl[ba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio) &&
-			                 (bb < 0 || __hip_atomic_load(&cl[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio);
-			if (win) {
-				S.colour[a] = c;
-				if (ba >= 0) S.taken[ba] = c;
-				if (bb >= 0) S.taken[bb] = c;
-				won++;
-			} else {
-				if (ba >= 0) other[ba] = ~0ull;
-				if (bb >= 0) other[bb] = ~0ull;
-			}
-		}
+		const int won = colour_round(W, S, T.list, n, tid, stride, c, cl, other, false);
 		if (won) atomicSub(T.left, won);
 		sy.sync();
 		if (++sub % COLOUR_SUBROUNDS == 0) c++;
@@ -483,23 +551,18 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	}
 	sy.sync();
 	// ---- cached impulses (cpSpaceStep.c:407-415) ----
-	for (int q = 0; q < n_colours; q++) {
-		for (int k = T.col_off[q] + tid; k < T.col_off[q + 1]; k += stride) arb_apply_cached(W, S.act[T.order[k]]);
-		sy.sync();
-	}
+	const int tail = S.solve_tail_max;
+	for_colours(sy, T.col_off, n_colours, tail, [&](int k) { arb_apply_cached(W, S.act[T.order[k]]); });
 	for (int jc = 0; jc < T.njcol; jc++) {
 		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) joint_apply_cached(W, W.joints[T.jorder[k]]);
 		sy.sync();
 	}
 	// ---- the impulse iterations (cpSpaceStep.c:418-427) ----
 	for (int it = 0; it < T.iterations; it++) {
-		for (int q = 0; q < n_colours; q++) {
-			for (int k = T.col_off[q] + tid; k < T.col_off[q + 1]; k += stride) {
-				Arbiter &arb = S.act[T.order[k]];
-				if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, arb);
-			}
-			sy.sync();
-		}
+		for_colours(sy, T.col_off, n_colours, tail, [&](int k) {
+			Arbiter &arb = S.act[T.order[k]];
+			if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, arb);
+		});
 		for (int jc = 0; jc < T.njcol; jc++) {
 			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
 				Joint &j = W.joints[T.jorder[k]];

@@ -262,6 +262,9 @@ struct cpgpuWorld {
 	int32_t *arb_by_world = nullptr;
 	uint64_t *world_keys = nullptr;
 	int32_t *warb_off = nullptr, *large_first = nullptr;
+	// grid-solve tails (see SolveBuffers)
+	int colour_tail_max = 4096, solve_tail_max = 2048;
+	int32_t *tail_list = nullptr, *tail_n = nullptr;
 	int n_jcolours = 0;
 	std::vector<int32_t> h_jcolour;
 	GridBarrier *bar = nullptr;
@@ -634,6 +637,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_upload(w, w->n_colours, (const int32_t *)nullptr, 1));
 	HIP_OR_FAIL(own_upload(w, w->left, (const int32_t *)nullptr, 1));
 	HIP_OR_FAIL(own_upload(w, w->warb_off, (const int32_t *)nullptr, w->nw + 2));
+	HIP_OR_FAIL(own_alloc(w, w->tail_list, w->colour_tail_max));
+	HIP_OR_FAIL(own_upload(w, w->tail_n, (const int32_t *)nullptr, 4));
 	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 1));
 	HIP_OR_FAIL(own_alloc(w, w->claim, 2 * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->taken, nb));
@@ -802,6 +807,8 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
 	S.jw_order = w->d_jw_order; S.jw_col = w->d_jw_col; S.jw_cbase = w->d_jw_cbase; S.jw_ncol = w->d_jw_ncol;
 	S.bar = w->bar;
+	S.tail_list = w->tail_list; S.tail_n = w->tail_n; S.tail_state = w->tail_n + 2;
+	S.colour_tail_max = w->colour_tail_max; S.solve_tail_max = w->solve_tail_max;
 	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;
 	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;

@@ -51,13 +51,20 @@ def build_workload(name):
     raise SystemExit("unknown workload " + name)
 
 
-def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations):
-    """Streaming-model bytes of one launch of `kernel`, from the per-step averages."""
+def solver_bytes(contacts, bodies, iterations):
+    """Streaming-model bytes of solving `contacts` contacts and integrating `bodies` bodies:
+    contact pre-step 26s + cached impulse 24s, then 47s per contact and iteration;
+    velocity integration reads 8s and writes 6s per dynamic body."""
+    return contacts * (26 + 24) * S + contacts * iterations * 47 * S + bodies * 14 * S
+
+
+def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations, launches, solve_counts):
+    """Streaming-model bytes of one launch of `kernel`, from the per-step averages (solver kernels: from the
+    contacts and bodies each solve unit handled, counted on the device)."""
     c, a, p = per_step["contacts"], per_step["arbiters"], per_step["pairs"]
-    if kernel in ("solver", "solver_worlds"):
-        # contact pre-step 26s + cached impulse 24s, then 47s per contact and iteration;
-        # velocity integration reads 8s and writes 6s per dynamic body
-        return c * (26 + 24) * S + c * iterations * 47 * S + n_dyn_bodies * 14 * S
+    if kernel in solve_counts:
+        uc, ub = solve_counts[kernel]
+        return solver_bytes(uc / max(launches, 1), ub / max(launches, 1), iterations)
     if kernel == "narrowphase":
         # per pair: two shapes' world geometry (~8s each for the mix), material 4s + 8 B each,
         # warm-start record 24 B; per colliding pair its arbiter header 6s + 8 B, per contact 4s + 8 B
@@ -91,8 +98,9 @@ def run_gpu(pw, steps, warmup, dt):
     t1 = time.perf_counter()
     timing = g.timing()
     stats = g.stats()
+    counts = g.solve_counts()
     g.close()
-    return t1 - t0, [t - t0 for t in t_step], timing, stats
+    return t1 - t0, [t - t0 for t in t_step], timing, stats, counts
 
 
 def run_cpu_reference(pw, sample_steps, dt):
@@ -130,7 +138,7 @@ def main():
     iterations = max(sp["iterations"] for sp in pw.a["space"])
     config.update({"bodies": n_dyn, "shapes": pw.n_shapes, "worlds": pw.n_worlds})
 
-    wall, t_steps, timing, stats = run_gpu(pw, args.steps, args.warmup, dt)
+    wall, t_steps, timing, stats, solve_counts = run_gpu(pw, args.steps, args.warmup, dt)
     value = n_dyn * args.steps / wall
     per_step = {"contacts": stats["total_contacts"] / args.steps, "arbiters": stats["total_arbiters"] / args.steps,
                 "pairs": stats["total_pairs"] / args.steps}
@@ -138,7 +146,7 @@ def main():
     kernels = {}
     for name, (ms, launches) in timing.items():
         avg_ms = ms / max(launches, 1)
-        b = algorithmic_bytes(name, per_step, n_dyn, n_dyn_shapes, iterations)
+        b = algorithmic_bytes(name, per_step, n_dyn, n_dyn_shapes, iterations, launches, solve_counts)
         kernels[name] = {"total_ms": round(ms, 3), "launches": launches, "avg_us": round(avg_ms * 1e3, 2)}
         if b is not None:
             kernels[name]["bytes_per_launch"] = int(b)

@@ -48,8 +48,13 @@ __global__ void k_bounds_reset(unsigned long long *wb, int n_worlds)
 
 __device__ __forceinline__ V2 box_centre(double4 b) { return v2(0.5 * (b.x + b.z), 0.5 * (b.y + b.w)); }
 
-__global__ void k_bounds(const DevWorld W, const int32_t *shapes, int n, unsigned long long *wb)
+// Per-world bounds of the box centres (order-preserving integer images of the
+// doubles, so atomic min / max give the exact extremes).  Waves and then
+// workgroups whose lanes share one world combine before the atomics.
+__global__ void __launch_bounds__(256) k_bounds(const DevWorld W, const int32_t *shapes, int n, unsigned long long *wb)
 {
+	__shared__ int ww[4];
+	__shared__ unsigned long long wv[4][4];
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	const bool valid = i < n;
 	int w = -1;
@@ -61,7 +66,7 @@ __global__ void k_bounds(const DevWorld W, const int32_t *shapes, int n, unsigne
 		x0 = x1 = ord_of(c.x);
 		y0 = y1 = ord_of(c.y);
 	}
-	// one wave usually holds one world: reduce across the wave, one atomic per bound
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int w_first = __shfl(w, 0);
 	const bool uniform = __ballot(valid && w != w_first) == 0ull;
 	if (uniform) {
@@ -71,7 +76,26 @@ __global__ void k_bounds(const DevWorld W, const int32_t *shapes, int n, unsigne
 			x1 = max(x1, (unsigned long long)__shfl_xor(x1, off));
 			y1 = max(y1, (unsigned long long)__shfl_xor(y1, off));
 		}
-		if ((threadIdx.x & 63) == 0 && w_first >= 0) {
+	}
+	if (lane == 0) {
+		ww[wave] = uniform ? w_first : -2;
+		wv[wave][0] = x0; wv[wave][1] = y0; wv[wave][2] = x1; wv[wave][3] = y1;
+	}
+	__syncthreads();
+	const bool block_uniform = ww[0] >= 0 && ww[1] == ww[0] && ww[2] == ww[0] && ww[3] == ww[0];
+	if (block_uniform) {
+		if (threadIdx.x == 0) {
+			for (int q = 1; q < 4; q++) {
+				x0 = min(x0, wv[q][0]); y0 = min(y0, wv[q][1]);
+				x1 = max(x1, wv[q][2]); y1 = max(y1, wv[q][3]);
+			}
+			atomicMin(&wb[4 * w_first + 0], x0);
+			atomicMin(&wb[4 * w_first + 1], y0);
+			atomicMax(&wb[4 * w_first + 2], x1);
+			atomicMax(&wb[4 * w_first + 3], y1);
+		}
+	} else if (uniform) {
+		if (lane == 0 && w_first >= 0) {
 			atomicMin(&wb[4 * w_first + 0], x0);
 			atomicMin(&wb[4 * w_first + 1], y0);
 			atomicMax(&wb[4 * w_first + 2], x1);
@@ -105,31 +129,40 @@ __device__ __forceinline__ uint32_t quantise(double x, double lo, double hi)
 	return (uint32_t)q;
 }
 
-__global__ void k_keys(const DevWorld W, const int32_t *shapes, int n, const unsigned long long *wb, uint64_t *keys,
-                       int32_t *vals, unsigned long long *orand)
+constexpr int KEYS_BLOCKS = 1024;
+
+// Morton keys prefixed by the world index (grid-stride), and the OR / AND of
+// all keys, which tell the radix sort which digits vary: one atomic pair per
+// workgroup.
+__global__ void __launch_bounds__(256) k_keys(const DevWorld W, const int32_t *shapes, int n,
+                                             const unsigned long long *wb, uint64_t *keys, int32_t *vals,
+                                             unsigned long long *orand)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	uint64_t key = 0;
-	bool valid = i < n;
-	if (valid) {
+	__shared__ unsigned long long red[2][4];
+	unsigned long long o = 0ull, a = ~0ull;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
 		const int s = shapes[i];
 		const int w = W.s_world[s];
 		const V2 c = box_centre(W.s_bb[s]);
 		const uint32_t qx = quantise(c.x, double_of(wb[4 * w + 0]), double_of(wb[4 * w + 2]));
 		const uint32_t qy = quantise(c.y, double_of(wb[4 * w + 1]), double_of(wb[4 * w + 3]));
-		key = ((uint64_t)w << WORLD_SHIFT) | spread22(qx) | (spread22(qy) << 1);
+		const uint64_t key = ((uint64_t)w << WORLD_SHIFT) | spread22(qx) | (spread22(qy) << 1);
 		keys[i] = key;
 		vals[i] = s;
+		o |= key;
+		a &= key;
 	}
-	// OR / AND of all keys: which radix digits vary
-	unsigned long long o = valid ? key : 0ull, a = valid ? key : ~0ull;
 	for (int off = 32; off > 0; off >>= 1) {
 		o |= __shfl_xor(o, off);
 		a &= __shfl_xor(a, off);
 	}
-	if ((threadIdx.x & 63) == 0) {
-		atomicOr(&orand[0], (unsigned long long)o);
-		atomicAnd(&orand[1], (unsigned long long)a);
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (lane == 0) { red[0][wave] = o; red[1][wave] = a; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int q = 1; q < 4; q++) { o |= red[0][q]; a &= red[1][q]; }
+		atomicOr(&orand[0], o);
+		atomicAnd(&orand[1], a);
 	}
 }
 
@@ -304,42 +337,87 @@ __device__ __forceinline__ double4 box_union(double4 a, double4 b)
 	return make_double4(fminc(a.x, b.x), fminc(a.y, b.y), fmaxc(a.z, b.z), fmaxc(a.w, b.w));
 }
 
-__device__ __forceinline__ double4 load_box_coherent(const double4 *p)
+__device__ __forceinline__ double4 shfl_up4(double4 v, int off)
 {
-	const volatile double *q = (const volatile double *)p;
-	return make_double4(q[0], q[1], q[2], q[3]);
+	return make_double4(__shfl_up(v.x, off), __shfl_up(v.y, off), __shfl_up(v.z, off), __shfl_up(v.w, off));
+}
+__device__ __forceinline__ double4 shfl_down4(double4 v, int off)
+{
+	return make_double4(__shfl_down(v.x, off), __shfl_down(v.y, off), __shfl_down(v.z, off), __shfl_down(v.w, off));
 }
 
-// Bottom-up box refit: the second lane to reach a node combines its children.
-// A node whose leaves all belong to this workgroup's chunk was completed by
-// lanes of this workgroup, so workgroup-scope ordering suffices; only nodes
-// spanning chunks need device-scope fences (which write back / invalidate L2).
-__global__ void k_refit(const DevWorld W, int n, const uint64_t *keys, const int32_t *leaf_shape,
-                        const int32_t *child, const int32_t *parent, const int32_t *range, const int32_t *world_root,
-                        double4 *box, uint32_t *visits)
+// Box refit in two fence-free passes over chunks of REFIT_CHUNK sorted leaves.
+// Pass 1 (one workgroup per chunk): leaf boxes into LDS; the nodes whose leaf
+// range lies inside the chunk are refitted bottom-up in LDS (the second lane
+// to reach a node combines its children, workgroup-scope ordering); the
+// chunk's inclusive prefix and suffix unions of leaf boxes and its total box
+// are written out.
+constexpr int REFIT_CHUNK = 256;
+
+__global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, int n, const int32_t *leaf_shape,
+                                                           const int32_t *child, const int32_t *parent,
+                                                           const int32_t *range, double4 *box, double4 *pre,
+                                                           double4 *suf, double4 *cbox)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n) return;
-	const int chunk_lo = blockIdx.x * blockDim.x, chunk_hi = chunk_lo + blockDim.x - 1;
-	// traversals start at the world roots: nodes above them are never read
-	const int root = world_root[(int)(keys[i] >> WORLD_SHIFT)];
-	int node = (n - 1) + i;
-	box[node] = W.s_bb[leaf_shape[i]];
-	int p = node == root ? -1 : parent[node];
-	while (p >= 0) {
-		const bool local = range[2 * p] >= chunk_lo && range[2 * p + 1] <= chunk_hi;
-		if (local) {
-			if (__hip_atomic_fetch_add(&visits[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
-			box[p] = box_union(box[child[2 * p]], box[child[2 * p + 1]]);
-		} else {
-			__threadfence();
-			if (atomicAdd(&visits[p], 1u) == 0u) return;
-			__threadfence();
-			box[p] = box_union(load_box_coherent(&box[child[2 * p]]), load_box_coherent(&box[child[2 * p + 1]]));
-		}
-		node = p;
-		p = node == root ? -1 : parent[p];
+	__shared__ double4 lb[REFIT_CHUNK];
+	__shared__ double4 ib[REFIT_CHUNK];
+	__shared__ uint32_t vis[REFIT_CHUNK];
+	__shared__ double4 wtot[REFIT_CHUNK / 64];
+	const int t = threadIdx.x, lo = blockIdx.x * REFIT_CHUNK, i = lo + t;
+	const int hi = min(lo + REFIT_CHUNK - 1, n - 1);
+	const bool valid = i < n;
+	const double inf = __builtin_inf();
+	const double4 b = valid ? W.s_bb[leaf_shape[i]] : make_double4(inf, inf, -inf, -inf);
+	lb[t] = b;
+	vis[t] = 0;
+	if (valid) box[(n - 1) + i] = b;
+	const int lane = t & 63, wave = t >> 6;
+	double4 pf = b, sf = b;
+	for (int off = 1; off < 64; off <<= 1) {
+		const double4 u = shfl_up4(pf, off);
+		const double4 d = shfl_down4(sf, off);
+		if (lane >= off) pf = box_union(pf, u);
+		if (lane + off < 64) sf = box_union(sf, d);
 	}
+	if (lane == 63) wtot[wave] = pf;
+	__syncthreads();
+	for (int q = 0; q < REFIT_CHUNK / 64; q++) {
+		if (q < wave) pf = box_union(pf, wtot[q]);
+		if (q > wave) sf = box_union(sf, wtot[q]);
+	}
+	if (valid) { pre[i] = pf; suf[i] = sf; }
+	if (t == REFIT_CHUNK - 1) cbox[blockIdx.x] = pf;
+	if (!valid || n < 2) return;
+	// a node's children lie inside its leaf range, and Karras node k is an end of its own range
+	int p = parent[(n - 1) + i];
+	while (p >= 0) {
+		if (range[2 * p] < lo || range[2 * p + 1] > hi) break;
+		if (__hip_atomic_fetch_add(&vis[p - lo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) break;
+		const int c0 = child[2 * p], c1 = child[2 * p + 1];
+		const double4 b0 = c0 >= n - 1 ? lb[c0 - (n - 1) - lo] : ib[c0 - lo];
+		const double4 b1 = c1 >= n - 1 ? lb[c1 - (n - 1) - lo] : ib[c1 - lo];
+		const double4 u = box_union(b0, b1);
+		ib[p - lo] = u;
+		box[p] = u;
+		p = parent[p];
+	}
+}
+
+// Pass 2: a node spanning chunks gets the union of the suffix of its first
+// chunk, the whole chunks between, and the prefix of its last chunk.  Nodes
+// above the world roots are never read and are skipped.
+__global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, const double4 *pre,
+                             const double4 *suf, const double4 *cbox, double4 *box)
+{
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= n - 1) return;
+	const int lo = range[2 * p], hi = range[2 * p + 1];
+	const int cl = lo / REFIT_CHUNK, ch = hi / REFIT_CHUNK;
+	if (cl == ch) return;
+	if ((keys[lo] >> WORLD_SHIFT) != (keys[hi] >> WORLD_SHIFT)) return;
+	double4 u = box_union(suf[lo], pre[hi]);
+	for (int c = cl + 1; c < ch; c++) u = box_union(u, cbox[c]);
+	box[p] = u;
 }
 
 // ---- traversal ----
@@ -521,7 +599,9 @@ void lbvh_reserve(Lbvh &t, int n, int n_worlds)
 	(void)hipMalloc(&t.parent, sizeof(int32_t) * 2 * cap);
 	(void)hipMalloc(&t.range, sizeof(int32_t) * 2 * cap);
 	(void)hipMalloc(&t.box, sizeof(double4) * 2 * cap);
-	(void)hipMalloc(&t.visits, sizeof(uint32_t) * cap);
+	(void)hipMalloc(&t.pre, sizeof(double4) * cap);
+	(void)hipMalloc(&t.suf, sizeof(double4) * cap);
+	(void)hipMalloc(&t.cbox, sizeof(double4) * (cap / REFIT_CHUNK + 1));
 	(void)hipMalloc(&t.world_root, sizeof(int32_t) * n_worlds);
 	(void)hipMalloc(&t.world_first, sizeof(int32_t) * n_worlds);
 	(void)hipMalloc(&t.world_last, sizeof(int32_t) * n_worlds);
@@ -530,7 +610,7 @@ void lbvh_reserve(Lbvh &t, int n, int n_worlds)
 
 void lbvh_free(Lbvh &t)
 {
-	void *ptrs[] = {t.keys, t.leaf_shape, t.child, t.parent, t.range, t.box, t.visits, t.world_root,
+	void *ptrs[] = {t.keys, t.leaf_shape, t.child, t.parent, t.range, t.box, t.pre, t.suf, t.cbox, t.world_root,
 	                t.world_first, t.world_last};
 	for (void *p : ptrs) if (p) (void)hipFree(p);
 	t = Lbvh();
@@ -549,21 +629,22 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 		k_bounds<<<blocks_for(n, 256), 256, 0, stream>>>(W, shapes, n, wb);
 		(void)hipMemsetAsync(sortbuf.orand, 0, sizeof(uint64_t), stream);
 		(void)hipMemsetAsync(sortbuf.orand + 1, 0xFF, sizeof(uint64_t), stream);
-		k_keys<<<blocks_for(n, 256), 256, 0, stream>>>(W, shapes, n, wb, t.keys, t.leaf_shape,
+		k_keys<<<min(blocks_for(n, 256), KEYS_BLOCKS), 256, 0, stream>>>(W, shapes, n, wb, t.keys, t.leaf_shape,
 		                                                (unsigned long long *)sortbuf.orand);
 	});
 	HIP_TRY(hipGetLastError());
 	HIP_TRY(radix_sort_pairs(t.keys, t.leaf_shape, n, sortbuf, stream, timer));
 	TIMED(timer, "bvh_build", stream, {
 		(void)hipMemsetAsync(t.parent, 0xFF, sizeof(int32_t) * 1, stream);
-		(void)hipMemsetAsync(t.visits, 0, sizeof(uint32_t) * (n > 1 ? n - 1 : 1), stream);
 		k_world_ranges<<<blocks_for(n, 256), 256, 0, stream>>>(t.keys, n, t.world_first, t.world_last,
 		                                                        t.world_root);
 		if (n > 1)
 			k_karras<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.child, t.parent, t.range,
 			                                                      t.world_first, t.world_last, t.world_root);
-		k_refit<<<blocks_for(n, 256), 256, 0, stream>>>(W, n, t.keys, t.leaf_shape, t.child, t.parent, t.range,
-		                                                 t.world_root, t.box, t.visits);
+		k_refit_local<<<blocks_for(n, REFIT_CHUNK), REFIT_CHUNK, 0, stream>>>(W, n, t.leaf_shape, t.child, t.parent,
+		                                                                    t.range, t.box, t.pre, t.suf, t.cbox);
+		if (n > REFIT_CHUNK)
+			k_refit_span<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.range, t.pre, t.suf, t.cbox, t.box);
 	});
 	return hipGetLastError();
 }

@@ -81,7 +81,9 @@ struct Lbvh {
 	int32_t *parent = nullptr;      // [2n-1]
 	int32_t *range = nullptr;       // [n-1][2] leaf range of internal nodes
 	double4 *box = nullptr;         // [2n-1]
-	uint32_t *visits = nullptr;     // [n-1] refit counters
+	double4 *pre = nullptr;         // [n] refit: union of the chunk's leaf boxes up to each leaf
+	double4 *suf = nullptr;         // [n] ... and from each leaf to the chunk's end
+	double4 *cbox = nullptr;        // [n / chunk + 1] union of each chunk's leaf boxes
 	int32_t *world_root = nullptr;  // [n_worlds] root node per world (-1: none)
 	int32_t *world_first = nullptr; // [n_worlds]
 	int32_t *world_last = nullptr;  // [n_worlds]

@@ -39,18 +39,36 @@ struct GridBarrier {
 // Worlds are solved in one of two ways, chosen per world by size:
 //   small worlds: one workgroup each (k_solve_worlds), every synchronisation
 //     a workgroup barrier;
-//   large worlds: together on a co-resident grid (k_solve_grid) with grid
-//     barriers between colours.
+//   large worlds: split each step into islands, the connected components of
+//     their dynamic bodies under this step's arbiters and the joints.
+//     Islands are body-disjoint, so each can be solved on its own in its
+//     colour order with the same result as the whole set.  Joint-free
+//     islands of at most island_max arbiters take a workgroup each
+//     (k_solve_worlds again); the rest are solved together on a co-resident
+//     grid (k_solve_grid) with grid barriers between colours.
+// Sorted arbiter keys: small world w -> w; large-world island with root
+// body r -> 1 << 40 | r; grid set -> 2 << 40.
+#define ISLAND_KEY (1ull << 40)
+#define GRID_KEY (2ull << 40)
+
 struct SolveBuffers {
 	Arbiter *act;
 	const int32_t *n_act;
 	int32_t *colour;           // [cap] colour of each active arbiter
-	int32_t *order;            // [cap] active arbiters in colour order (per small world: its own segment)
-	int32_t *arb_by_world;     // [cap] active arbiters sorted by (large, world); nullptr for a single space
-	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world
-	int32_t *large_first;      // [1] first sorted position of a large world's arbiter
+	int32_t *order;            // [cap] active arbiters in colour order (per workgroup solve: its own segment)
+	int32_t *arb_by_world;     // [cap] active arbiters sorted by key; nullptr for a single small space
+	uint64_t *keys;            // [cap] the sorted keys
+	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world (the last: islands)
+	int32_t *large_first;      // [1] first sorted position of the grid set
+	// islands of the large worlds
+	int32_t *label;            // [n_bodies] union-find parent, then island root (the least body index)
+	int32_t *icount;           // [n_bodies] per root: arbiters of the island
+	int32_t *ijoint;           // [n_bodies] per root: the island holds a joint
+	int32_t *groups;           // [cap] first sorted position of each workgroup-solved island
+	int32_t *n_groups;         // [2] island count, work counter
+	int island_max;
+	int n_large_joints;        // joints of large worlds (jorder[0..n))
 	const uint8_t *world_small;
-	const int32_t *small_worlds;   // the small worlds (one workgroup each)
 	int32_t *col_off;          // [CPGPU_MAX_COLOURS + 1] grid solve: counts, then offsets
 	int32_t *col_cursor;       // [CPGPU_MAX_COLOURS]
 	int32_t *n_colours;        // [1] grid solve colour count
@@ -80,15 +98,36 @@ struct SolveBuffers {
 	int n_large_bodies;
 };
 
-// Sort the solved arbiters by (large, world) and find the small worlds'
-// ranges (skipped for a single space).
-hipError_t solver_sort_by_world(const SolveBuffers &S, int n_act, int n_worlds, RadixSortBuffers &sortbuf,
-                                uint64_t *keys, hipStream_t stream, KernelTimer *timer);
+// Group the solved arbiters: islands of the large worlds (union-find), the
+// pre-steps and velocity integration of the workgroup-solved islands, then
+// the arbiters sorted by key with the small worlds' ranges and the island
+// list.  Skipped for a single small space.
+hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
+                        RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer);
+// Small worlds are launched in size buckets, each with its own workgroup
+// size and room for the largest of its worlds' bodies in LDS (0: the bodies
+// stay in global memory).  Worlds are listed largest first.
+#define CPGPU_MAX_BUCKETS 4
+// solve units counted in DevWorld::totals: the buckets, the islands, the grid
+static_assert(CPGPU_SOLVE_UNITS == CPGPU_MAX_BUCKETS + 2, "solve units: buckets, islands, grid");
+struct WorldBucket {
+	int max_bodies;            // worlds with at most this many bodies (static included)
+	int threads;               // 64, 128, 256, 512 or 1024
+	int lds_cap;
+	const int32_t *worlds;     // device list
+	int n;
+};
 // Launch the solvers: colouring, colour sort, pre-steps, velocity
-// integration, cached impulses and the impulse iterations.  n_small small
-// worlds get a workgroup each; grid > 0 launches the grid solve of the large
-// worlds with that many workgroups.
-hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, int n_small, int grid, hipStream_t stream,
-                       KernelTimer *timer);
+// integration, cached impulses and the impulse iterations.  Each bucket of
+// small worlds runs on its own auxiliary stream (all independent worlds);
+// grid > 0 launches the grid
+// solve with that many workgroups, on the main stream, while island_blocks
+// workgroups take the islands on aux[CPGPU_MAX_BUCKETS]; the main stream
+// waits for all of them.  events: CPGPU_MAX_BUCKETS + 2.
+hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBucket *buckets, int n_buckets,
+                       int island_blocks, int grid, hipStream_t stream, const hipStream_t *aux,
+                       const hipEvent_t *events, KernelTimer *timer);
+// Largest dynamic LDS a small-world workgroup may take next to its static LDS.
+int solver_max_lds_bytes();
 // Largest co-resident grid of the grid solve on the current device.
 int solver_max_grid();

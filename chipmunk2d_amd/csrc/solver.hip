@@ -21,7 +21,6 @@
 
 namespace {
 
-constexpr int WPW_THREADS = 256;    // workgroup per world
 constexpr int GRID_THREADS = 1024;  // grid solve: few, large workgroups keep the barrier cheap
 
 // ---- grid barrier (all blocks co-resident: cooperative launch) ----
@@ -78,6 +77,30 @@ __device__ __forceinline__ void store_bv(const DevWorld &W, int b, const BV &r)
 	q.i_inv = r.i_inv;
 	W.b_vel[b] = q;
 }
+
+// Where the impulse loops read and write body velocities: global memory, or
+// a copy of one world's contiguous body range staged in LDS.
+struct GlobalBodies {
+	const DevWorld *W;
+	__device__ __forceinline__ BV load(int b) const { return load_bv(*W, b); }
+	__device__ __forceinline__ void store(int b, const BV &r) const { store_bv(*W, b, r); }
+};
+struct LdsBodies {
+	BodyVel *v;
+	int base;
+	__device__ __forceinline__ BV load(int b) const
+	{
+		const BodyVel &q = v[b - base];
+		BV r;
+		r.v = v2(q.vx, q.vy); r.w = q.w; r.vb = v2(q.vbx, q.vby); r.wb = q.wb; r.m_inv = q.m_inv; r.i_inv = q.i_inv;
+		return r;
+	}
+	__device__ __forceinline__ void store(int b, const BV &r) const
+	{
+		BodyVel &q = v[b - base];
+		q.vx = r.v.x; q.vy = r.v.y; q.w = r.w; q.vbx = r.vb.x; q.vby = r.vb.y; q.wb = r.wb;
+	}
+};
 
 __device__ __forceinline__ V2 rel_vel(const BV &a, const BV &b, V2 r1, V2 r2)
 {
@@ -136,24 +159,26 @@ __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
 	}
 }
 
-__device__ void arb_apply_cached(const DevWorld &W, const Arbiter &arb)
+template <class BA>
+__device__ void arb_apply_cached(const DevWorld &W, const BA &bodies, const Arbiter &arb)
 {
 	if (arb.state == CPGPU_ARB_FIRST_COLLISION) return;
 	const double dt_coef = W.ws[arb.world].dt_coef;
-	BV a = load_bv(W, arb.ba), b = load_bv(W, arb.bb);
+	BV a = bodies.load(arb.ba), b = bodies.load(arb.bb);
 	const V2 n = v2(arb.nx, arb.ny);
 	for (int i = 0; i < arb.count; i++) {
 		const Contact &con = arb.c[i];
 		const V2 j = vrotate(n, v2(con.jnAcc, con.jtAcc));
 		apply_impulses(a, b, v2(con.r1x, con.r1y), v2(con.r2x, con.r2y), vmult(j, dt_coef));
 	}
-	if (is_dynamic(W, arb.ba)) store_bv(W, arb.ba, a);
-	if (is_dynamic(W, arb.bb)) store_bv(W, arb.bb, b);
+	if (is_dynamic(W, arb.ba)) bodies.store(arb.ba, a);
+	if (is_dynamic(W, arb.bb)) bodies.store(arb.bb, b);
 }
 
-__device__ void arb_apply_impulse(const DevWorld &W, Arbiter &arb)
+template <class BA>
+__device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &arb)
 {
-	BV a = load_bv(W, arb.ba), b = load_bv(W, arb.bb);
+	BV a = bodies.load(arb.ba), b = bodies.load(arb.bb);
 	const V2 n = v2(arb.nx, arb.ny);
 	const V2 surface_vr = v2(arb.svx, arb.svy);
 	const double friction = arb.u;
@@ -182,8 +207,8 @@ __device__ void arb_apply_impulse(const DevWorld &W, Arbiter &arb)
 		apply_bias_impulse(b, jb, r2);
 		apply_impulses(a, b, r1, r2, vrotate(n, v2(con.jnAcc - jnOld, con.jtAcc - jtOld)));
 	}
-	if (is_dynamic(W, arb.ba)) store_bv(W, arb.ba, a);
-	if (is_dynamic(W, arb.bb)) store_bv(W, arb.bb, b);
+	if (is_dynamic(W, arb.ba)) bodies.store(arb.ba, a);
+	if (is_dynamic(W, arb.bb)) bodies.store(arb.bb, b);
 }
 
 // ---- joints (cpPivotJoint.c:24-70, cpDampedSpring.c:24-77, cpSlideJoint.c:24-89, cpGearJoint.c:24-69) ----
@@ -262,11 +287,12 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 	}
 }
 
-__device__ void joint_apply_cached(const DevWorld &W, const Joint &j)
+template <class BA>
+__device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Joint &j)
 {
 	if (j.type == CPGPU_JOINT_DAMPED_SPRING) return;
 	const double dt_coef = W.ws[j.world].dt_coef;
-	BV a = load_bv(W, j.a), b = load_bv(W, j.b);
+	BV a = bodies.load(j.a), b = bodies.load(j.b);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
 		apply_impulses(a, b, r1, r2, vmult(v2(j.jAx, j.jAy), dt_coef));
@@ -278,14 +304,15 @@ __device__ void joint_apply_cached(const DevWorld &W, const Joint &j)
 		a.w -= jj * a.i_inv * ratio_inv;
 		b.w += jj * b.i_inv;
 	}
-	if (is_dynamic(W, j.a)) store_bv(W, j.a, a);
-	if (is_dynamic(W, j.b)) store_bv(W, j.b, b);
+	if (is_dynamic(W, j.a)) bodies.store(j.a, a);
+	if (is_dynamic(W, j.b)) bodies.store(j.b, b);
 }
 
-__device__ void joint_apply_impulse(const DevWorld &W, Joint &j)
+template <class BA>
+__device__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &j)
 {
 	const double dt = W.ws[j.world].dt;
-	BV a = load_bv(W, j.a), b = load_bv(W, j.b);
+	BV a = bodies.load(j.a), b = bodies.load(j.b);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
 		const V2 vr = rel_vel(a, b, r1, r2);
@@ -325,13 +352,36 @@ __device__ void joint_apply_impulse(const DevWorld &W, Joint &j)
 		a.w -= jj * a.i_inv * ratio_inv;
 		b.w += jj * b.i_inv;
 	}
-	if (is_dynamic(W, j.a)) store_bv(W, j.a, a);
-	if (is_dynamic(W, j.b)) store_bv(W, j.b, b);
+	if (is_dynamic(W, j.a)) bodies.store(j.a, a);
+	if (is_dynamic(W, j.b)) bodies.store(j.b, b);
 }
 
 // ---- the solve routine, shared by the workgroup-per-world and grid kernels ----
 
 __device__ __forceinline__ int dyn_or_neg(const DevWorld &W, int b) { return is_dynamic(W, b) ? b : -1; }
+
+// cpBodyUpdateVelocity (cpBody.c:493-509) with the space's damping and
+// gravity (cpSpaceStep.c:399-404); kinematic and static bodies keep their velocity
+__device__ __forceinline__ void integrate_velocity(const DevWorld &W, int b)
+{
+	if (W.b_type[b] != CPGPU_BODY_DYNAMIC) return;
+	const WorldStep ws = W.ws[W.b_world[b]];
+	BodyVel bv = W.b_vel[b];
+	const double2 f = W.b_f[b];
+	const V2 v = vadd(vmult(v2(bv.vx, bv.vy), ws.damping),
+	                  vmult(vadd(v2(ws.gx, ws.gy), vmult(v2(f.x, f.y), bv.m_inv)), ws.dt));
+	bv.vx = v.x; bv.vy = v.y;
+	bv.w = bv.w * ws.damping + W.b_t[b] * bv.i_inv * ws.dt;
+	W.b_vel[b] = bv;
+	W.b_f[b] = make_double2(0.0, 0.0);
+	W.b_t[b] = 0.0;
+}
+
+// an island of a large world is solved by a workgroup (else on the grid)
+__device__ __forceinline__ bool island_on_workgroup(const SolveBuffers &S, int root)
+{
+	return S.icount[root] <= S.island_max && S.ijoint[root] == 0;
+}
 
 // colouring priority: a bijective hash of the shape pair's indices within its
 // own space, so a space is coloured (and stepped) identically alone or in a batch
@@ -373,7 +423,19 @@ struct SolveSet {
 	const int32_t *jcol_off;   // [njcol + 1]
 	int njcol;
 	int iterations;            // sweeps (elements of worlds with fewer sweeps skip the extra ones)
+	bool prestep;              // arbiter pre-steps here (else done before, with the bodies' integration)
+	bool island_filter;        // integrate only bodies whose island is not solved by a workgroup
+	BodyVel *lds_bodies;       // LDS room for lds_cap bodies, or nullptr
+	int lds_cap;
+	int unit;                  // solve unit whose contacts / bodies are counted (W.totals[2 + 2 unit ...])
 };
+
+// a wave-reduced 64-bit add to a counter
+__device__ __forceinline__ void wave_add(unsigned long long *counter, unsigned long long v)
+{
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+	if (__lane_id() == 0 && v) atomicAdd(counter, v);
+}
 
 // One colouring claim round over arbiters list[0..n) (see the file comment).
 // Returns the number of arbiters that took colour c.
@@ -434,6 +496,35 @@ __device__ void for_colours(Sync &sy, const int32_t *col_off, int n_colours, int
 			}
 			sy.sync();
 			q = q1;
+		}
+	}
+}
+
+// cached impulses (cpSpaceStep.c:407-415), then the impulse iterations
+// (cpSpaceStep.c:418-427): arbiters colour by colour, then joints colour by colour
+template <class Sync, class BA>
+__device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy, int n_colours,
+                               const BA &bodies)
+{
+	const int tid = sy.tid, stride = sy.stride;
+	const int tail = S.solve_tail_max;
+	for_colours(sy, T.col_off, n_colours, tail, [&](int k) { arb_apply_cached(W, bodies, S.act[T.order[k]]); });
+	for (int jc = 0; jc < T.njcol; jc++) {
+		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride)
+			joint_apply_cached(W, bodies, W.joints[T.jorder[k]]);
+		sy.sync();
+	}
+	for (int it = 0; it < T.iterations; it++) {
+		for_colours(sy, T.col_off, n_colours, tail, [&](int k) {
+			Arbiter &arb = S.act[T.order[k]];
+			if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, bodies, arb);
+		});
+		for (int jc = 0; jc < T.njcol; jc++) {
+			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
+				Joint &j = W.joints[T.jorder[k]];
+				if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);
+			}
+			sy.sync();
 		}
 	}
 }
@@ -503,10 +594,14 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	const int n_colours = (sub % COLOUR_SUBROUNDS == 0) ? c : c + 1;
 
 	// ---- colour sort ----
+	unsigned long long n_contacts = 0;
 	for (int k = tid; k < n; k += stride) {
-		const int col = S.colour[arb_index(k)];
+		const int a = arb_index(k);
+		const int col = S.colour[a];
 		if (col >= 0) atomicAdd(&T.col_off[col], 1);
+		n_contacts += S.act[a].count;
 	}
+	wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
 	sy.sync();
 	if (tid == 0) {
 		int run = 0;
@@ -528,81 +623,113 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		T.order[pos] = a;
 	}
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----
-	for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);
+	if (T.prestep)
+		for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);
 	sy.sync();
 	for (int jc = 0; jc < T.njcol; jc++) {
 		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) joint_prestep(W, W.joints[T.jorder[k]]);
 		sy.sync();
 	}
 	// ---- velocity integration (cpSpaceStep.c:399-404, cpBody.c:493-509) ----
+	unsigned long long n_int = 0;
 	for (int q = tid; q < T.nb; q += stride) {
 		const int b = T.bodies ? T.bodies[q] : T.body0 + q;
-		if (W.b_type[b] != CPGPU_BODY_DYNAMIC) continue;   // kinematic and static bodies keep their velocity
-		const WorldStep ws = W.ws[W.b_world[b]];
-		BodyVel bv = W.b_vel[b];
-		const double2 f = W.b_f[b];
-		const V2 v = vadd(vmult(v2(bv.vx, bv.vy), ws.damping),
-		                  vmult(vadd(v2(ws.gx, ws.gy), vmult(v2(f.x, f.y), bv.m_inv)), ws.dt));
-		bv.vx = v.x; bv.vy = v.y;
-		bv.w = bv.w * ws.damping + W.b_t[b] * bv.i_inv * ws.dt;
-		W.b_vel[b] = bv;
-		W.b_f[b] = make_double2(0.0, 0.0);
-		W.b_t[b] = 0.0;
+		if (T.island_filter && island_on_workgroup(S, S.label[b])) continue;
+		integrate_velocity(W, b);
+		n_int++;
 	}
+	wave_add(&W.totals[3 + 2 * T.unit], n_int);
 	sy.sync();
-	// ---- cached impulses (cpSpaceStep.c:407-415) ----
-	const int tail = S.solve_tail_max;
-	for_colours(sy, T.col_off, n_colours, tail, [&](int k) { arb_apply_cached(W, S.act[T.order[k]]); });
-	for (int jc = 0; jc < T.njcol; jc++) {
-		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) joint_apply_cached(W, W.joints[T.jorder[k]]);
+	// ---- impulses: on a copy of the bodies in LDS when the set is one world's range that fits ----
+	if (T.lds_bodies && !T.bodies && T.nb <= T.lds_cap) {
+		for (int q = tid; q < T.nb; q += stride) T.lds_bodies[q] = W.b_vel[T.body0 + q];
 		sy.sync();
-	}
-	// ---- the impulse iterations (cpSpaceStep.c:418-427) ----
-	for (int it = 0; it < T.iterations; it++) {
-		for_colours(sy, T.col_off, n_colours, tail, [&](int k) {
-			Arbiter &arb = S.act[T.order[k]];
-			if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, arb);
-		});
-		for (int jc = 0; jc < T.njcol; jc++) {
-			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
-				Joint &j = W.joints[T.jorder[k]];
-				if (it < W.ws[j.world].iterations) joint_apply_impulse(W, j);
-			}
-			sy.sync();
-		}
+		solve_impulses(W, S, T, sy, n_colours, LdsBodies{T.lds_bodies, T.body0});
+		for (int q = tid; q < T.nb; q += stride)
+			if (is_dynamic(W, T.body0 + q)) W.b_vel[T.body0 + q] = T.lds_bodies[q];
+		sy.sync();
+	} else {
+		solve_impulses(W, S, T, sy, n_colours, GlobalBodies{&W});
 	}
 }
 
-// One workgroup per small world: every synchronisation is a workgroup barrier.
-__global__ void __launch_bounds__(WPW_THREADS) k_solve_worlds(const DevWorld W, const SolveBuffers S)
+__device__ __forceinline__ int lower_bound_u64(const uint64_t *keys, int lo, int hi, uint64_t key)
 {
+	while (lo < hi) {
+		const int mid = (lo + hi) >> 1;
+		if (keys[mid] < key) lo = mid + 1; else hi = mid;
+	}
+	return lo;
+}
+
+// One workgroup per small world, then the remaining workgroups take the
+// islands of the large worlds from a work counter: every synchronisation is a
+// workgroup barrier.
+template <int THREADS>
+__global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, const SolveBuffers S,
+                                                            const int32_t *worlds, int n_small, int lds_cap,
+                                                            int unit)
+{
+	extern __shared__ BodyVel lds_bodies[];
 	__shared__ int32_t col_off[CPGPU_MAX_COLOURS + 1];
 	__shared__ int32_t col_cursor[CPGPU_MAX_COLOURS];
 	__shared__ int32_t left;
-	const int w = S.small_worlds[blockIdx.x];
+	__shared__ int32_t next;
 	SolveSet T;
-	if (S.arb_by_world) {
-		T.list = S.arb_by_world + S.warb_off[w];
-		T.n = S.warb_off[w + 1] - S.warb_off[w];
-		T.order = S.order + S.warb_off[w];
-	} else {   // a single space: every solved arbiter is its own
-		T.list = nullptr;
-		T.n = *S.n_act;
-		T.order = S.order;
-	}
 	T.col_off = col_off;
 	T.col_cursor = col_cursor;
 	T.left = &left;
 	T.n_colours_out = nullptr;
 	T.bodies = nullptr;
-	T.body0 = W.w_body0[w];
-	T.nb = W.w_body0[w + 1] - W.w_body0[w];
-	T.jorder = S.jw_order;
-	T.jcol_off = S.jw_col + S.jw_cbase[w];
-	T.njcol = S.jw_ncol[w];
-	T.iterations = W.ws[w].iterations;
+	T.island_filter = false;
+	T.lds_bodies = lds_cap > 0 ? lds_bodies : nullptr;
+	T.lds_cap = lds_cap;
+	T.unit = unit;
 	BlockSync sy{(int)threadIdx.x, (int)blockDim.x};
-	solve_set(W, S, T, sy);
+	if ((int)blockIdx.x < n_small) {
+		const int w = worlds[blockIdx.x];
+		if (S.arb_by_world) {
+			T.list = S.arb_by_world + S.warb_off[w];
+			T.n = S.warb_off[w + 1] - S.warb_off[w];
+			T.order = S.order + S.warb_off[w];
+		} else {   // a single space: every solved arbiter is its own
+			T.list = nullptr;
+			T.n = *S.n_act;
+			T.order = S.order;
+		}
+		T.body0 = W.w_body0[w];
+		T.nb = W.w_body0[w + 1] - W.w_body0[w];
+		T.jorder = S.jw_order;
+		T.jcol_off = S.jw_col + S.jw_cbase[w];
+		T.njcol = S.jw_ncol[w];
+		T.iterations = W.ws[w].iterations;
+		T.prestep = true;
+		solve_set(W, S, T, sy);
+		return;
+	}
+	// islands: pre-steps and velocity integration were done before
+	const int n_groups = S.n_groups[0];
+	const int island_end = S.large_first[0];
+	T.body0 = 0;
+	T.nb = 0;
+	T.jorder = nullptr;
+	T.jcol_off = nullptr;
+	T.njcol = 0;
+	T.prestep = false;
+	while (true) {
+		if (threadIdx.x == 0) next = atomicAdd(&S.n_groups[1], 1);
+		__syncthreads();
+		const int g = next;
+		__syncthreads();
+		if (g >= n_groups) break;
+		const int start = S.groups[g];
+		const int end = lower_bound_u64(S.keys, start, island_end, S.keys[start] + 1);
+		T.list = S.arb_by_world + start;
+		T.n = end - start;
+		T.order = S.order + start;
+		T.iterations = W.ws[S.act[T.list[0]].world].iterations;
+		solve_set(W, S, T, sy);
+	}
 }
 
 // All large worlds together on the co-resident grid.
@@ -624,53 +751,209 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	T.jcol_off = S.jcol_off;
 	T.njcol = S.n_jcolours;
 	T.iterations = S.max_iterations;
+	T.prestep = true;
+	T.island_filter = true;
+	T.lds_bodies = nullptr;
+	T.lds_cap = 0;
+	T.unit = CPGPU_MAX_BUCKETS + 1;
 	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err};
 	solve_set(W, S, T, sy);
 }
 
-// ---- sorting the solved arbiters by world: keys (large << 20 | world) ----
+// ---- grouping the solved arbiters: islands and keys ----
 
-__global__ void k_world_keys(const SolveBuffers S, uint64_t *keys, int32_t *vals, unsigned long long *orand)
+constexpr int GROUP_BLOCKS = 1024;
+
+// union-find over body indices; a root is hooked under the smaller root, so
+// every island's root is its least body index
+__device__ __forceinline__ int uf_find(int32_t *parent, int x)
+{
+	while (true) {
+		const int p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (p == x) return x;
+		const int gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		// path halving: x is no root, so only this pointer to an ancestor changes
+		if (gp != p) __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		x = gp;
+	}
+}
+
+__device__ void uf_unite(int32_t *parent, int a, int b)
+{
+	while (true) {
+		a = uf_find(parent, a);
+		b = uf_find(parent, b);
+		if (a == b) return;
+		if (a < b) { const int t = a; a = b; b = t; }
+		int expected = a;
+		if (__hip_atomic_compare_exchange_strong(&parent[a], &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+		                                         __HIP_MEMORY_SCOPE_AGENT))
+			return;
+		a = expected;   // a was hooked meanwhile: continue from its parent
+	}
+}
+
+__global__ void k_uf_init(const SolveBuffers S)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q == 0) { S.n_groups[0] = 0; S.n_groups[1] = 0; }
+	if (q >= S.n_large_bodies) return;
+	const int b = S.large_bodies[q];
+	S.label[b] = b;
+	S.icount[b] = 0;
+	S.ijoint[b] = 0;
+}
+
+// edges: arbiters and joints of large worlds joining two dynamic bodies
+__global__ void k_uf_link(const DevWorld W, const SolveBuffers S)
 {
 	const int n = *S.n_act;
-	const int k = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool valid = k < n;
-	unsigned long long key = 0;
-	if (valid) {
-		const int w = S.act[k].world;
-		key = ((unsigned long long)(S.world_small[w] ? 0 : 1) << 20) | (unsigned long long)w;
-		keys[k] = key;
-		vals[k] = k;
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n + S.n_large_joints; k += gridDim.x * blockDim.x) {
+		int a, b;
+		if (k < n) {
+			const Arbiter &arb = S.act[k];
+			if (S.world_small[arb.world]) continue;
+			a = arb.ba; b = arb.bb;
+		} else {
+			const Joint &j = W.joints[S.jorder[k - n]];
+			a = j.a; b = j.b;
+		}
+		if (is_dynamic(W, a) && is_dynamic(W, b)) uf_unite(S.label, a, b);
 	}
-	unsigned long long o = valid ? key : 0ull, a = valid ? key : ~0ull;
+}
+
+// Every body's label becomes its root.  The walk only reads: a path-halving
+// store here could overwrite a label another lane has just set to its root
+// with a non-root ancestor.  Concurrent label writes are roots, so every
+// walk still ends at the root.
+__global__ void k_uf_flatten(const DevWorld W, const SolveBuffers S)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= S.n_large_bodies) return;
+	const int b = S.large_bodies[q];
+	int x = b;
+	while (true) {
+		const int p = __hip_atomic_load(&S.label[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (p == x) break;
+		x = p;
+	}
+	__hip_atomic_store(&S.label[b], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int island_of(const DevWorld &W, const SolveBuffers &S, int a, int b)
+{
+	return S.label[is_dynamic(W, a) ? a : b];
+}
+
+// add 1 to icount[root] for every lane with root >= 0: lanes of a wave with
+// the same root (one large island: most of them) combine into one atomic
+__device__ __forceinline__ void count_by_root(int32_t *icount, int root)
+{
+	uint64_t todo = __ballot(root >= 0);
+	while (todo) {
+		const int leader = __ffsll((long long)todo) - 1;
+		const int r = __shfl(root, leader);
+		const uint64_t same = __ballot(root == r) & todo;
+		if (__lane_id() == leader) atomicAdd(&icount[r], (int)__popcll(same));
+		todo &= ~same;
+	}
+}
+
+__global__ void k_island_count(const DevWorld W, const SolveBuffers S)
+{
+	const int n = *S.n_act;
+	const int total = n + S.n_large_joints;
+	// every lane of a wave runs the same number of passes (the wave-wide ballots)
+	const int span = gridDim.x * blockDim.x;
+	const int passes = (total + span - 1) / span;
+	for (int p = 0; p < passes; p++) {
+		const int k = p * span + blockIdx.x * blockDim.x + threadIdx.x;
+		int root = -1;
+		if (k < n) {
+			const Arbiter &arb = S.act[k];
+			if (!S.world_small[arb.world]) {
+				root = island_of(W, S, arb.ba, arb.bb);
+				// both bodies of an arbiter must share the island: islands are solved concurrently
+				if (is_dynamic(W, arb.ba) && is_dynamic(W, arb.bb) && S.label[arb.ba] != S.label[arb.bb])
+					raise_err(W.err, CPGPU_ERR_ISLANDS);
+			}
+		}
+		count_by_root(S.icount, root);
+		if (k >= n && k < total) {
+			const Joint &j = W.joints[S.jorder[k - n]];
+			if (is_dynamic(W, j.a) || is_dynamic(W, j.b)) S.ijoint[island_of(W, S, j.a, j.b)] = 1;
+		}
+	}
+}
+
+// Sort keys of the solved arbiters; the pre-steps of the workgroup-solved
+// islands' arbiters (they only read velocities); OR / AND of the keys for the
+// radix sort.
+__global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveBuffers S, unsigned long long *orand)
+{
+	__shared__ unsigned long long red[2][4];
+	const int n = *S.n_act;
+	unsigned long long o = 0ull, a = ~0ull;
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+		Arbiter &arb = S.act[k];
+		const int w = arb.world;
+		unsigned long long key;
+		if (S.world_small[w]) {
+			key = (unsigned long long)w;
+		} else {
+			const int r = island_of(W, S, arb.ba, arb.bb);
+			if (island_on_workgroup(S, r)) {
+				key = ISLAND_KEY | (unsigned long long)r;
+				arb_prestep(W, arb);
+			} else {
+				key = GRID_KEY;
+			}
+		}
+		S.keys[k] = key;
+		S.arb_by_world[k] = k;
+		o |= key;
+		a &= key;
+	}
 	for (int off = 32; off > 0; off >>= 1) {
 		o |= __shfl_xor(o, off);
 		a &= __shfl_xor(a, off);
 	}
-	if ((threadIdx.x & 63) == 0) {
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	if (lane == 0) { red[0][wave] = o; red[1][wave] = a; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int q = 1; q < 4; q++) { o |= red[0][q]; a &= red[1][q]; }
 		atomicOr(&orand[0], o);
 		atomicAnd(&orand[1], a);
 	}
 }
 
-__device__ __forceinline__ int lower_bound_u64(const uint64_t *keys, int n, uint64_t key)
+// velocity integration of the bodies not solved on the grid (after the
+// pre-steps above, before any impulse)
+__global__ void k_island_integrate(const DevWorld W, const SolveBuffers S)
 {
-	int lo = 0, hi = n;
-	while (lo < hi) {
-		const int mid = (lo + hi) >> 1;
-		if (keys[mid] < key) lo = mid + 1; else hi = mid;
-	}
-	return lo;
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= S.n_large_bodies) return;
+	const int b = S.large_bodies[q];
+	if (island_on_workgroup(S, S.label[b])) integrate_velocity(W, b);
 }
 
 // warb_off[w] = first sorted position of small world w (n_worlds + 1 entries);
-// large_first = first position of a large world's arbiter
-__global__ void k_world_offsets(const SolveBuffers S, const uint64_t *keys, int n_worlds)
+// large_first = first position of the grid set
+__global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 {
 	const int n = *S.n_act;
 	const int w = blockIdx.x * blockDim.x + threadIdx.x;
-	if (w <= n_worlds) S.warb_off[w] = lower_bound_u64(keys, n, (uint64_t)w);
-	if (w == n_worlds + 1) S.large_first[0] = lower_bound_u64(keys, n, 1ull << 20);
+	if (w <= n_worlds) S.warb_off[w] = lower_bound_u64(S.keys, 0, n, (uint64_t)w);
+	if (w == n_worlds + 1) S.large_first[0] = lower_bound_u64(S.keys, 0, n, GRID_KEY);
+}
+
+// the first sorted position of every workgroup-solved island
+__global__ void k_island_groups(const SolveBuffers S, int n_worlds)
+{
+	const int lo = S.warb_off[n_worlds], hi = S.large_first[0];
+	for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x)
+		if (i == lo || S.keys[i] != S.keys[i - 1]) S.groups[atomicAdd(&S.n_groups[0], 1)] = i;
 }
 
 } // namespace
@@ -687,30 +970,89 @@ int solver_max_grid()
 	return per_cu * prop.multiProcessorCount;
 }
 
-hipError_t solver_sort_by_world(const SolveBuffers &S, int n_act, int n_worlds, RadixSortBuffers &sortbuf,
-                                uint64_t *keys, hipStream_t stream, KernelTimer *timer)
+hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
+                        RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer)
 {
+	const int nb = S.n_large_bodies;
+	const int edge_blocks = std::max(1, std::min((n_act + S.n_large_joints + 255) / 256, GROUP_BLOCKS));
+	if (large) {
+		TIMED(timer, "island_union", stream, {
+			k_uf_init<<<std::max(1, (nb + 255) / 256), 256, 0, stream>>>(S);
+			k_uf_link<<<edge_blocks, 256, 0, stream>>>(W, S);
+			if (nb > 0) k_uf_flatten<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
+		});
+		TIMED(timer, "island_count", stream, k_island_count<<<edge_blocks, 256, 0, stream>>>(W, S));
+		HIP_TRY(hipGetLastError());
+	}
 	if (n_act > 0) {
 		radix_sort_reserve(sortbuf, n_act);
 		HIP_TRY(hipMemsetAsync(sortbuf.orand, 0, sizeof(uint64_t), stream));
 		HIP_TRY(hipMemsetAsync(sortbuf.orand + 1, 0xFF, sizeof(uint64_t), stream));
 		TIMED(timer, "world_sort", stream,
-		      k_world_keys<<<(n_act + 255) / 256, 256, 0, stream>>>(S, keys, S.arb_by_world,
-		                                                             (unsigned long long *)sortbuf.orand));
+		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
+		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
-		HIP_TRY(radix_sort_pairs(keys, S.arb_by_world, n_act, sortbuf, stream, timer));
+		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, sortbuf, stream, timer));
 	}
-	k_world_offsets<<<(n_worlds + 2 + 255) / 256, 256, 0, stream>>>(S, keys, n_worlds);
+	if (large && nb > 0) {
+		TIMED(timer, "island_integrate", stream,
+		      k_island_integrate<<<(nb + 255) / 256, 256, 0, stream>>>(W, S));
+	}
+	k_world_offsets<<<(n_worlds + 2 + 255) / 256, 256, 0, stream>>>(S, n_worlds);
+	if (large && n_act > 0)
+		k_island_groups<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(S, n_worlds);
 	return hipGetLastError();
 }
 
-hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, int n_small, int grid, hipStream_t stream,
-                       KernelTimer *timer)
+template <int THREADS>
+static void launch_worlds(const DevWorld &W, const SolveBuffers &S, const int32_t *worlds, int blocks, int n_small,
+                          int lds_cap, int unit, hipStream_t st)
 {
-	if (n_small > 0) {
-		TIMED(timer, "solver_worlds", stream,
-		      k_solve_worlds<<<n_small, WPW_THREADS, 0, stream>>>(W, S));
+	k_solve_worlds<THREADS><<<blocks, THREADS, (size_t)lds_cap * sizeof(BodyVel), st>>>(W, S, worlds, n_small,
+	                                                                                    lds_cap, unit);
+}
+
+static const char *bucket_names[CPGPU_MAX_BUCKETS] = {"solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2",
+                                                      "solver_worlds_b3"};
+
+int solver_max_lds_bytes()
+{
+	int dev = 0, lds = 0;
+	(void)hipGetDevice(&dev);
+	(void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+	// the static LDS of k_solve_worlds: colour offsets and cursors
+	return lds - (int)(sizeof(int32_t) * (2 * CPGPU_MAX_COLOURS + 4));
+}
+
+hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBucket *buckets, int n_buckets,
+                       int island_blocks, int grid, hipStream_t stream, const hipStream_t *aux,
+                       const hipEvent_t *events, KernelTimer *timer)
+{
+	// fork: the buckets' streams start after everything queued so far
+	HIP_TRY(hipEventRecord(events[0], stream));
+	for (int q = 0; q < n_buckets; q++) {
+		const WorldBucket &B = buckets[q];
+		if (B.n == 0) continue;
+		HIP_TRY(hipStreamWaitEvent(aux[q], events[0], 0));
+		TIMED(timer, bucket_names[q], aux[q], {
+			switch (B.threads) {
+			case 64: launch_worlds<64>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
+			case 128: launch_worlds<128>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
+			case 256: launch_worlds<256>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
+			case 512: launch_worlds<512>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
+			default: launch_worlds<1024>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
+			}
+		});
 		HIP_TRY(hipGetLastError());
+	}
+	if (island_blocks > 0) {
+		// n_small = 0: every workgroup takes islands from the work counter; beside the grid solve
+		hipStream_t is = aux[CPGPU_MAX_BUCKETS];
+		HIP_TRY(hipStreamWaitEvent(is, events[0], 0));
+		TIMED(timer, "solver_islands", is, launch_worlds<256>(W, S, nullptr, island_blocks, 0, 0, CPGPU_MAX_BUCKETS, is));
+		HIP_TRY(hipGetLastError());
+		HIP_TRY(hipEventRecord(events[1 + CPGPU_MAX_BUCKETS], is));
+		HIP_TRY(hipStreamWaitEvent(stream, events[1 + CPGPU_MAX_BUCKETS], 0));
 	}
 	if (grid > 0) {
 		void *args[] = {(void *)&W, (void *)&S};
@@ -719,6 +1061,12 @@ hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, int n_small, in
 		      e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_grid), dim3(grid),
 		                                     dim3(GRID_THREADS), args, 0, stream));
 		HIP_TRY(e);
+	}
+	// join
+	for (int q = 0; q < n_buckets; q++) {
+		if (buckets[q].n == 0) continue;
+		HIP_TRY(hipEventRecord(events[1 + q], aux[q]));
+		HIP_TRY(hipStreamWaitEvent(stream, events[1 + q], 0));
 	}
 	return hipSuccess;
 }

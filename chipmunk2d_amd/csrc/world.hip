@@ -254,14 +254,25 @@ struct cpgpuWorld {
 	// small / large worlds
 	int small_world_max_bodies = 4096;
 	uint8_t *d_world_small = nullptr;
-	int32_t *d_small_worlds = nullptr;
 	int n_small = 0, n_large_worlds = 0;
+	std::vector<int32_t> h_world_nb;      // bodies per world (static included)
+	std::vector<uint8_t> h_small;
+	// small worlds in size buckets (see WorldBucket), each on its own stream
+	int n_buckets = 0;
+	WorldBucket buckets[CPGPU_MAX_BUCKETS] = {};
+	int32_t *bucket_lists[CPGPU_MAX_BUCKETS] = {};
+	hipStream_t aux[CPGPU_MAX_BUCKETS + 1] = {};   // the last: the islands' solve
+	hipEvent_t fork_join[CPGPU_MAX_BUCKETS + 2] = {};
 	int32_t *d_large_bodies = nullptr;
 	int n_large_bodies = 0;
 	int max_iterations_large = 0;
 	int32_t *arb_by_world = nullptr;
 	uint64_t *world_keys = nullptr;
 	int32_t *warb_off = nullptr, *large_first = nullptr;
+	// islands of the large worlds (see SolveBuffers)
+	int32_t *label = nullptr, *icount = nullptr, *ijoint = nullptr, *groups = nullptr, *n_groups = nullptr;
+	int island_max = 2048, island_blocks_max = 2048;
+	int n_large_joints = 0;
 	// grid-solve tails (see SolveBuffers)
 	int colour_tail_max = 4096, solve_tail_max = 2048;
 	int32_t *tail_list = nullptr, *tail_n = nullptr;
@@ -378,6 +389,7 @@ int check_device_error(cpgpuWorld *w)
 	if (e & CPGPU_ERR_BARRIER_TIMEOUT) m += " solver grid barrier timeout;";
 	if (e & CPGPU_ERR_TABLE_FULL) m += " pair table full;";
 	if (e & CPGPU_ERR_EPA_HULL) m += " EPA hull overflow;";
+	if (e & CPGPU_ERR_ISLANDS) m += " inconsistent island labels;";
 	return fail(CPGPU_ERR_DEVICE, m);
 }
 
@@ -406,6 +418,38 @@ extern "C" {
 const char *cpgpuGetLastError(void) { return g_last_error.c_str(); }
 const char *cpgpuVersion(void) { return "cpgpu 0.1 (gfx950; Chipmunk2D 7.0.3 cpSpaceStep semantics)"; }
 
+// Distribute the small worlds over the size buckets (first bucket whose
+// max_bodies holds the world), largest first within a bucket, and size each
+// bucket's LDS for its largest world (worlds beyond the device limit keep
+// their bodies in global memory).
+static hipError_t build_buckets(cpgpuWorld *w)
+{
+	const int max_lds = solver_max_lds_bytes();
+	std::vector<std::vector<int32_t>> lists(w->n_buckets);
+	for (int k = 0; k < w->nw; k++) {
+		if (!w->h_small[k]) continue;
+		int q = 0;
+		while (q < w->n_buckets - 1 && w->h_world_nb[k] > w->buckets[q].max_bodies) q++;
+		lists[q].push_back(k);
+	}
+	for (int q = 0; q < w->n_buckets; q++) {
+		auto &L = lists[q];
+		std::stable_sort(L.begin(), L.end(), [&](int a, int b) { return w->h_world_nb[a] > w->h_world_nb[b]; });
+		if (w->bucket_lists[q]) { (void)hipFree(w->bucket_lists[q]); w->bucket_lists[q] = nullptr; }
+		WorldBucket &B = w->buckets[q];
+		B.n = (int)L.size();
+		B.lds_cap = 0;
+		B.worlds = nullptr;
+		if (L.empty()) continue;
+		HIP_TRY(hipMalloc(&w->bucket_lists[q], sizeof(int32_t) * L.size()));
+		HIP_TRY(hipMemcpy(w->bucket_lists[q], L.data(), sizeof(int32_t) * L.size(), hipMemcpyHostToDevice));
+		B.worlds = w->bucket_lists[q];
+		const int largest = w->h_world_nb[L[0]];
+		B.lds_cap = std::min(largest, max_lds / (int)sizeof(BodyVel));
+	}
+	return hipSuccess;
+}
+
 int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **out)
 {
 	if (!out) return fail(CPGPU_ERR_INVALID, "null output pointer");
@@ -420,6 +464,9 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	w->precision = precision;
 	HIP_OR_FAIL(hipGetDevice(&w->device));
 	HIP_OR_FAIL(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+	for (int q = 0; q <= CPGPU_MAX_BUCKETS; q++) HIP_OR_FAIL(hipStreamCreateWithFlags(&w->aux[q], hipStreamNonBlocking));
+	for (int q = 0; q < CPGPU_MAX_BUCKETS + 2; q++)
+		HIP_OR_FAIL(hipEventCreateWithFlags(&w->fork_join[q], hipEventDisableTiming));
 	w->nw = d->n_worlds; w->nb = d->n_bodies; w->ns = d->n_shapes; w->nc = d->n_constraints;
 	w->npv = d->n_poly_verts;
 	w->sp.assign(d->space, d->space + d->n_worlds);
@@ -553,26 +600,35 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		const int max_col = colour_host(nb, ba, bbv, prio, w->h_jcolour);
 		(void)max_col;
 
-		// small worlds (one workgroup each) and large worlds (the grid solve)
+		// small worlds (one workgroup each) and large worlds (islands and the grid solve)
 		std::vector<uint8_t> small(w->nw);
-		std::vector<int32_t> small_list, large_bodies;
+		std::vector<int32_t> large_bodies;
+		w->h_world_nb.assign(w->nw, 0);
+		w->n_small = 0;
 		for (int k = 0; k < w->nw; k++) {
 			int ndyn = 0;
 			for (int b = d->world_body_start[k]; b < d->world_body_start[k + 1]; b++)
 				ndyn += d->body_type[b] != CPGPU_BODY_STATIC;
 			small[k] = ndyn <= w->small_world_max_bodies;
-			if (small[k]) small_list.push_back(k);
+			w->n_small += small[k];
+			w->h_world_nb[k] = d->world_body_start[k + 1] - d->world_body_start[k];
 		}
+		w->h_small = small;
 		for (int b = 0; b < nb; b++)
 			if (d->body_type[b] != CPGPU_BODY_STATIC && !small[body_world[b]]) large_bodies.push_back(b);
-		w->n_small = (int)small_list.size();
 		w->n_large_worlds = w->nw - w->n_small;
 		w->n_large_bodies = (int)large_bodies.size();
 		w->max_iterations_large = 0;
 		for (int k = 0; k < w->nw; k++)
 			if (!small[k]) w->max_iterations_large = std::max(w->max_iterations_large, (int)w->sp[k].iterations);
 		HIP_OR_FAIL(own_upload(w, w->d_world_small, small.data(), small.size()));
-		HIP_OR_FAIL(own_upload(w, w->d_small_worlds, small_list.data(), small_list.size()));
+		const int default_buckets[4][2] = {{64, 64}, {256, 128}, {1024, 256}, {1 << 30, 512}};
+		w->n_buckets = 4;
+		for (int q = 0; q < 4; q++) {
+			w->buckets[q].max_bodies = default_buckets[q][0];
+			w->buckets[q].threads = default_buckets[q][1];
+		}
+		HIP_OR_FAIL(build_buckets(w));
 		HIP_OR_FAIL(own_upload(w, w->d_large_bodies, large_bodies.data(), large_bodies.size()));
 
 		// large worlds' joints: by colour
@@ -590,6 +646,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 				if (!small[J[c].world]) jorder[cursor[w->h_jcolour[c]]++] = c;
 		}
 		w->n_jcolours = ncol_large;
+		w->n_large_joints = (int)jorder.size();
 		// every world's joints: by (world, colour), with per-world colour offsets
 		std::vector<int32_t> jw_order(nc), jw_col, jw_cbase(w->nw), jw_ncol(w->nw);
 		{
@@ -626,7 +683,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(hipHostMalloc((void **)&w->h_counts, sizeof(int32_t) * 16, hipHostMallocDefault));
 	memset(w->h_counts, 0, sizeof(int32_t) * 16);
 	HIP_OR_FAIL(own_upload(w, W.err, (const int32_t *)nullptr, 1));
-	HIP_OR_FAIL(own_upload(w, W.totals, (const unsigned long long *)nullptr, 2));
+	HIP_OR_FAIL(own_upload(w, W.totals, (const unsigned long long *)nullptr, 2 + 2 * CPGPU_SOLVE_UNITS));
 	HIP_OR_FAIL(own_upload(w, w->d_counters, (const int32_t *)nullptr, 4));
 	HIP_OR_FAIL(own_upload(w, w->d_step_counts, (const int32_t *)nullptr, 6));
 	HIP_OR_FAIL(own_alloc(w, w->d_world_bounds, w->nw));
@@ -642,6 +699,10 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 1));
 	HIP_OR_FAIL(own_alloc(w, w->claim, 2 * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->taken, nb));
+	HIP_OR_FAIL(own_alloc(w, w->label, nb));
+	HIP_OR_FAIL(own_alloc(w, w->icount, nb));
+	HIP_OR_FAIL(own_alloc(w, w->ijoint, nb));
+	HIP_OR_FAIL(own_upload(w, w->n_groups, (const int32_t *)nullptr, 2));
 	w->max_iterations = 0;
 	for (auto &s : w->sp) w->max_iterations = std::max(w->max_iterations, (int)s.iterations);
 
@@ -685,10 +746,16 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	lbvh_free(w->static_tree);
 	radix_sort_free(w->sortbuf);
 	void *extra[] = {w->d_pairs, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
-	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys};
+	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys, w->groups};
 	for (void *p : extra) if (p) (void)hipFree(p);
 	if (w->h_ws) (void)hipHostFree(w->h_ws);
 	if (w->h_counts) (void)hipHostFree(w->h_counts);
+	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++)
+		if (w->bucket_lists[q]) (void)hipFree(w->bucket_lists[q]);
+	for (int q = 0; q <= CPGPU_MAX_BUCKETS; q++)
+		if (w->aux[q]) (void)hipStreamDestroy(w->aux[q]);
+	for (int q = 0; q < CPGPU_MAX_BUCKETS + 2; q++)
+		if (w->fork_join[q]) (void)hipEventDestroy(w->fork_join[q]);
 	(void)hipStreamDestroy(w->stream);
 	delete w;
 }
@@ -776,13 +843,14 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	}
 	HIP_OR_FAIL(regrow(w->seen, w->seen_cap, (long)n_prev_recs + 1));
 	if (w->act_cap[cur] > w->colour_cap) {
-		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys};
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups};
 		for (void *p : old) if (p) (void)hipFree(p);
 		w->colour_cap = w->act_cap[cur];
 		HIP_OR_FAIL(dalloc(w->colour, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->order, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->arb_by_world, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->world_keys, w->colour_cap));
+		HIP_OR_FAIL(dalloc(w->groups, w->colour_cap));
 	}
 	if (w->steps == 0) HIP_OR_FAIL(hipMemsetAsync(w->d_step_counts + 3 * prev, 0, 3 * sizeof(int32_t), w->stream));
 
@@ -799,9 +867,15 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	SolveBuffers S;
 	S.act = w->act[cur]; S.n_act = B.n_act;
 	S.colour = w->colour; S.order = w->order;
-	S.arb_by_world = w->nw > 1 ? w->arb_by_world : nullptr;
+	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
+	S.arb_by_world = grouped ? w->arb_by_world : nullptr;
+	S.keys = w->world_keys;
+	S.label = w->label; S.icount = w->icount; S.ijoint = w->ijoint;
+	S.groups = w->groups; S.n_groups = w->n_groups;
+	S.island_max = w->island_max;
+	S.n_large_joints = w->n_large_joints;
 	S.warb_off = w->warb_off; S.large_first = w->large_first;
-	S.world_small = w->d_world_small; S.small_worlds = w->d_small_worlds;
+	S.world_small = w->d_world_small;
 	S.col_off = w->col_off; S.col_cursor = w->col_cursor; S.n_colours = w->n_colours; S.left = w->left;
 	S.claim = w->claim; S.taken = w->taken;
 	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
@@ -812,14 +886,18 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;
 	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
-	if (w->nw > 1) {
-		// the solved arbiters of each world (one sync for the count)
+	int island_blocks = 0;
+	if (grouped) {
+		// the solved arbiters by world and island (one sync for the count)
 		HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, B.n_act, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
 		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
 		w->timer.collect();
-		HIP_OR_FAIL(solver_sort_by_world(S, w->h_counts[10], w->nw, w->sortbuf, w->world_keys, w->stream, timer));
+		const int n_act = w->h_counts[10];
+		HIP_OR_FAIL(solver_group(W, S, n_act, w->nw, w->n_large_worlds > 0, w->sortbuf, w->stream, timer));
+		if (w->n_large_worlds > 0) island_blocks = std::min(n_act, w->island_blocks_max);
 	}
-	HIP_OR_FAIL(solver_step(W, S, w->n_small, w->n_large_worlds > 0 ? w->solver_grid : 0, w->stream, timer));
+	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, island_blocks, w->n_large_worlds > 0 ? w->solver_grid : 0,
+	                        w->stream, w->aux, w->fork_join, timer));
 
 	w->cur ^= 1;
 	w->steps++;
@@ -979,6 +1057,19 @@ int cpgpuWorldSetTiming(cpgpuWorld *w, int32_t enable)
 	if (rc != CPGPU_OK) return rc;
 	w->timer.enabled = enable != 0;
 	w->timer.acc.clear();
+	return CPGPU_OK;
+}
+
+int cpgpuWorldGetSolveCounts(cpgpuWorld *w, int64_t *contacts, int64_t *bodies)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	unsigned long long t[2 + 2 * CPGPU_SOLVE_UNITS];
+	HIP_OR_FAIL(hipMemcpy(t, w->W.totals, sizeof(t), hipMemcpyDeviceToHost));
+	for (int u = 0; u < CPGPU_SOLVE_UNITS; u++) {
+		if (contacts) contacts[u] = (int64_t)t[2 + 2 * u];
+		if (bodies) bodies[u] = (int64_t)t[3 + 2 * u];
+	}
 	return CPGPU_OK;
 }
 

@@ -25,7 +25,8 @@ _vp = ctypes.c_void_p
 EXPORTS = [
     "cpgpuWorldCreate", "cpgpuWorldDestroy", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
-    "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuGetLastError", "cpgpuVersion",
+    "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
+    "cpgpuGetLastError", "cpgpuVersion",
 ]
 
 
@@ -53,6 +54,7 @@ def lib():
         L.cpgpuWorldReadConstraints.argtypes = [_vp, _dp, _ip]
         L.cpgpuWorldSetTiming.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
+        L.cpgpuWorldGetSolveCounts.argtypes = [_vp, _i64p, _i64p]
         L.cpgpuGetLastError.restype = ctypes.c_char_p
         L.cpgpuVersion.restype = ctypes.c_char_p
         _lib = L
@@ -147,6 +149,16 @@ class GPUWorld:
                                           ptr(launches, ctypes.c_int64), names, 8192))
         keys = names.value.decode().split("\n")[:n.value]
         return {k: (float(ms[i]), int(launches[i])) for i, k in enumerate(keys)}
+
+    SOLVE_UNITS = ("solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2", "solver_worlds_b3", "solver_islands",
+                   "solver")
+
+    def solve_counts(self):
+        """dict solver kernel name -> (contacts solved, bodies integrated) since creation."""
+        c = np.zeros(len(self.SOLVE_UNITS), np.int64)
+        b = np.zeros(len(self.SOLVE_UNITS), np.int64)
+        _check(self.L.cpgpuWorldGetSolveCounts(self.h, ptr(c, ctypes.c_int64), ptr(b, ctypes.c_int64)))
+        return {k: (int(c[i]), int(b[i])) for i, k in enumerate(self.SOLVE_UNITS)}
 
 
 def version():

@@ -214,6 +214,14 @@ int cpgpuWorldSetTiming(cpgpuWorld *world, int32_t enable);
 int cpgpuWorldGetTiming(cpgpuWorld *world, int32_t cap, int32_t *n_out, double *ms,
                         int64_t *launches, char *names, int32_t names_cap);
 
+/* Contacts solved and bodies velocity-integrated since creation by each
+ * solve unit, in the order of the timing names solver_worlds_b0 ..
+ * solver_worlds_b3 (size buckets of small worlds), solver_islands,
+ * solver (grid): CPGPU_SOLVE_UNITS entries each (instrumentation for
+ * per-kernel byte counts; no counterpart in the reference). */
+#define CPGPU_SOLVE_UNITS 6
+int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodies);
+
 const char *cpgpuGetLastError(void);
 const char *cpgpuVersion(void);
 

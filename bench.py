@@ -78,14 +78,25 @@ def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations, 
     return None
 
 
-def run_gpu(pw, steps, warmup, dt):
+def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60):
+    """Warm-up (and, for batches, the autotuner, once the copy has run `autotune_at` steps so that piles have
+    formed) on a throwaway copy of the world, then `steps` timed steps on a fresh copy with the tuned tiling."""
     from chipmunk2d_amd.gpu import GPUWorld
-    if warmup > 0:
+    tiling = None
+    if warmup > 0 or autotune_steps > 0:
         g = GPUWorld(pw)
-        g.step(dt, warmup)
+        if warmup > 0:
+            g.step(dt, warmup)
+        if autotune_steps > 0:
+            g.step(dt, max(autotune_at - warmup, 0))
+            g.autotune(dt, autotune_steps)
         g.sync()
+        tiling = g.buckets()
         g.close()
     g = GPUWorld(pw)
+    if autotune_steps > 0:
+        g.set_buckets([b[0] for b in tiling], [b[1] for b in tiling])
+        tiling = g.buckets()
     g.sync()
     g.set_timing(True)
     t_step = []
@@ -100,7 +111,7 @@ def run_gpu(pw, steps, warmup, dt):
     stats = g.stats()
     counts = g.solve_counts()
     g.close()
-    return t1 - t0, [t - t0 for t in t_step], timing, stats, counts
+    return t1 - t0, [t - t0 for t in t_step], timing, stats, counts, tiling
 
 
 def run_cpu_reference(pw, sample_steps, dt):
@@ -123,6 +134,8 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=120,
                     help="steps of the same rollout the reference CPU step is timed on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune-steps", type=int, default=3,
+                    help="steps per trial of the batch tiling autotuner (on the warm-up copy; 0: default tiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would step its own "
@@ -138,7 +151,12 @@ def main():
     iterations = max(sp["iterations"] for sp in pw.a["space"])
     config.update({"bodies": n_dyn, "shapes": pw.n_shapes, "worlds": pw.n_worlds})
 
-    wall, t_steps, timing, stats, solve_counts = run_gpu(pw, args.steps, args.warmup, dt)
+    autotune = args.autotune_steps if args.workload == "batch_4096" else 0
+    wall, t_steps, timing, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt, autotune)
+    if autotune:
+        config["tiling"] = {"autotuned": True, "trial_steps": autotune, "tuned_after_steps": 60,
+                            "buckets": [{"max_bodies": b[0], "threads": b[1], "worlds": b[2], "lds_bodies": b[3]}
+                                        for b in tiling if b[2] > 0]}
     value = n_dyn * args.steps / wall
     per_step = {"contacts": stats["total_contacts"] / args.steps, "arbiters": stats["total_arbiters"] / args.steps,
                 "pairs": stats["total_pairs"] / args.steps}

@@ -15,34 +15,55 @@
 // ---------------------------------------------------------------------------
 // vectors (include/chipmunk/cpVect.h, chipmunk_types.h:96-147)
 
-struct V2 { double x, y; };
+// Generic over the scalar: double for the step, a forward-mode dual number
+// (cpgpu_dual.h) where the adjoint needs contact-geometry derivatives.  The
+// value part of every operation is the same IEEE double operation.
+template <class T> struct V2T { T x, y; };
+using V2 = V2T<double>;
+
+CPD double csqrt(double x) { return __builtin_sqrt(x); }
 
 CPD V2 v2(double x, double y) { V2 r; r.x = x; r.y = y; return r; }
-CPD V2 vadd(V2 a, V2 b) { return v2(a.x + b.x, a.y + b.y); }
-CPD V2 vsub(V2 a, V2 b) { return v2(a.x - b.x, a.y - b.y); }
-CPD V2 vneg(V2 a) { return v2(-a.x, -a.y); }
-CPD V2 vmult(V2 a, double s) { return v2(a.x * s, a.y * s); }
-CPD double vdot(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
-CPD double vcross(V2 a, V2 b) { return a.x * b.y - a.y * b.x; }
-CPD V2 vperp(V2 a) { return v2(-a.y, a.x); }
-CPD V2 vrperp(V2 a) { return v2(a.y, -a.x); }
-CPD V2 vrotate(V2 a, V2 b) { return v2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-CPD double vlengthsq(V2 a) { return vdot(a, a); }
-CPD double vlength(V2 a) { return __builtin_sqrt(vdot(a, a)); }
+template <class T> CPD V2T<T> v2t(T x, T y) { V2T<T> r; r.x = x; r.y = y; return r; }
+template <class T> CPD V2T<T> vcast(V2 a) { return v2t<T>(T(a.x), T(a.y)); }
+template <class T> CPD V2T<T> vadd(V2T<T> a, V2T<T> b) { return v2t<T>(a.x + b.x, a.y + b.y); }
+template <class T> CPD V2T<T> vsub(V2T<T> a, V2T<T> b) { return v2t<T>(a.x - b.x, a.y - b.y); }
+template <class T> CPD V2T<T> vneg(V2T<T> a) { return v2t<T>(-a.x, -a.y); }
+template <class T, class S> CPD V2T<T> vmult(V2T<T> a, S s) { return v2t<T>(a.x * s, a.y * s); }
+template <class T> CPD T vdot(V2T<T> a, V2T<T> b) { return a.x * b.x + a.y * b.y; }
+template <class T> CPD T vcross(V2T<T> a, V2T<T> b) { return a.x * b.y - a.y * b.x; }
+template <class T> CPD V2T<T> vperp(V2T<T> a) { return v2t<T>(-a.y, a.x); }
+template <class T> CPD V2T<T> vrperp(V2T<T> a) { return v2t<T>(a.y, -a.x); }
+template <class T> CPD V2T<T> vrotate(V2T<T> a, V2T<T> b)
+{
+	return v2t<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <class T> CPD T vlengthsq(V2T<T> a) { return vdot(a, a); }
+template <class T> CPD T vlength(V2T<T> a) { return csqrt(vdot(a, a)); }
 #define CP_DBL_MIN 2.2250738585072014e-308
-CPD V2 vnormalize(V2 a) { return vmult(a, 1.0 / (vlength(a) + CP_DBL_MIN)); }
-CPD V2 vlerp(V2 a, V2 b, double t) { return vadd(vmult(a, 1.0 - t), vmult(b, t)); }
-CPD bool veql(V2 a, V2 b) { return a.x == b.x && a.y == b.y; }
-CPD V2 vclamp(V2 v, double len) { return (vdot(v, v) > len * len) ? vmult(vnormalize(v), len) : v; }
-CPD double fmaxc(double a, double b) { return (a > b) ? a : b; }
-CPD double fminc(double a, double b) { return (a < b) ? a : b; }
-CPD double fclampc(double f, double lo, double hi) { return fminc(fmaxc(f, lo), hi); }
-CPD double fclamp01(double f) { return fmaxc(0.0, fminc(f, 1.0)); }
+template <class T> CPD V2T<T> vnormalize(V2T<T> a) { return vmult(a, 1.0 / (vlength(a) + CP_DBL_MIN)); }
+template <class T, class S> CPD V2T<T> vlerp(V2T<T> a, V2T<T> b, S t)
+{
+	return vadd(vmult(a, 1.0 - t), vmult(b, t));
+}
+template <class T> CPD bool veql(V2T<T> a, V2T<T> b) { return a.x == b.x && a.y == b.y; }
+template <class T, class S> CPD V2T<T> vclamp(V2T<T> v, S len)
+{
+	return (vdot(v, v) > len * len) ? vmult(vnormalize(v), len) : v;
+}
+template <class T> CPD T fmaxc(T a, T b) { return (a > b) ? a : b; }
+template <class T> CPD T fminc(T a, T b) { return (a < b) ? a : b; }
+template <class T, class L, class H> CPD T fclampc(T f, L lo, H hi) { return fminc(fmaxc(f, T(lo)), T(hi)); }
+template <class T> CPD T fclamp01(T f) { return fmaxc(T(0.0), fminc(f, T(1.0))); }
 
 // cpTransform {a, b, c, d, tx, ty} (cpTransform.h)
-struct XF { double a, b, c, d, tx, ty; };
-CPD V2 xpoint(const XF &t, V2 p) { return v2(t.a * p.x + t.c * p.y + t.tx, t.b * p.x + t.d * p.y + t.ty); }
-CPD V2 xvect(const XF &t, V2 v) { return v2(t.a * v.x + t.c * v.y, t.b * v.x + t.d * v.y); }
+template <class T> struct XFT { T a, b, c, d, tx, ty; };
+using XF = XFT<double>;
+template <class T> CPD V2T<T> xpoint(const XFT<T> &t, V2 p)
+{
+	return v2t<T>(t.a * p.x + t.c * p.y + t.tx, t.b * p.x + t.d * p.y + t.ty);
+}
+template <class T> CPD V2T<T> xvect(const XFT<T> &t, V2 v) { return v2t<T>(t.a * v.x + t.c * v.y, t.b * v.x + t.d * v.y); }
 
 // CP_HASH_PAIR (chipmunk_private.h:28-29), 64-bit cpHashValue
 #define CP_HASH_COEF 3344921057ull

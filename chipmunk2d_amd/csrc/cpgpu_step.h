@@ -107,7 +107,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 // Small worlds are launched in size buckets, each with its own workgroup
 // size and room for the largest of its worlds' bodies in LDS (0: the bodies
 // stay in global memory).  Worlds are listed largest first.
-#define CPGPU_MAX_BUCKETS 4
+static_assert(CPGPU_MAX_BUCKETS == 4, "bucket count of the C ABI");
 // solve units counted in DevWorld::totals: the buckets, the islands, the grid
 static_assert(CPGPU_SOLVE_UNITS == CPGPU_MAX_BUCKETS + 2, "solve units: buckets, islands, grid");
 struct WorldBucket {

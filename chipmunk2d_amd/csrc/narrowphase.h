@@ -8,6 +8,12 @@
 // keeps 16-bit Minkowski point ids (vertex index pairs) instead of points,
 // re-deriving a point from its id (the id determines the support vertices,
 // so the recomputed point is bit-identical).
+//
+// Every routine is generic over the scalar T: double for the step, and the
+// forward-mode Dual of cpgpu_dual.h for the adjoint, which re-runs a pair's
+// collision with the step's GJK id to differentiate its contacts with
+// respect to the two bodies' positions and angles (same branches, so the
+// contact features are the step's).
 #pragma once
 
 #include "cpgpu_device.h"
@@ -16,17 +22,19 @@
 #define NP_MAX_EPA_ITERATIONS 30
 #define NP_EPA_HULL_MAX 36
 
-struct ShapeView {
+template <class T>
+struct ShapeViewT {
 	int type, count;
 	double r;
 	uint64_t hashid;
-	V2 tc;                 // circle centre (world)
-	V2 ta, tb, tn;         // segment (world)
-	V2 a_tan, b_tan;       // segment neighbour tangents (body-local)
-	V2 rot;                // body rotation (transform.a, transform.b)
-	const double2 *wv, *wn;
+	V2T<T> tc;                 // circle centre (world)
+	V2T<T> ta, tb, tn;         // segment (world)
+	V2 a_tan, b_tan;           // segment neighbour tangents (body-local)
+	V2T<T> rot;                // body rotation (transform.a, transform.b)
+	const V2T<T> *wv, *wn;     // polygon vertices and edge normals (world)
 	double bb[4];
 };
+using ShapeView = ShapeViewT<double>;
 
 CPD ShapeView load_shape(const DevWorld &W, int s)
 {
@@ -50,121 +58,124 @@ CPD ShapeView load_shape(const DevWorld &W, int s)
 	} else {
 		const int st = W.s_pstart[s];
 		v.count = W.s_pcount[s];
-		v.wv = W.pv_world + st;
-		v.wn = W.pn_world + st;
+		// double2 and V2 share their layout {x, y}
+		v.wv = reinterpret_cast<const V2 *>(W.pv_world + st);
+		v.wn = reinterpret_cast<const V2 *>(W.pn_world + st);
 	}
 	return v;
 }
 
-CPD V2 ld2(const double2 *p, int i) { const double2 q = p[i]; return v2(q.x, q.y); }
-
-struct SupportPt { V2 p; uint32_t index; };
-struct MinkPt { V2 a, b, ab; uint32_t id; };
-struct ClosestPts { V2 a, b, n; double d; uint32_t id; };
-struct EdgePt { V2 p; uint64_t hash; };
-struct Edge { EdgePt a, b; double r; V2 n; };
-struct CollInfo {
+template <class T> struct SupportPtT { V2T<T> p; uint32_t index; };
+template <class T> struct MinkPtT { V2T<T> a, b, ab; uint32_t id; };
+template <class T> struct ClosestPtsT { V2T<T> a, b, n; T d; uint32_t id; };
+template <class T> struct EdgePtT { V2T<T> p; uint64_t hash; };
+template <class T> struct EdgeT { EdgePtT<T> a, b; double r; V2T<T> n; };
+template <class T> struct CollInfoT {
 	int count;
 	uint32_t id;
-	V2 n;
-	V2 p1[2], p2[2];
+	V2T<T> n;
+	V2T<T> p1[2], p2[2];
 	uint64_t hash[2];
 };
+using CollInfo = CollInfoT<double>;
 
 // PolySupportPointIndex (cpCollision.c:62-77)
-CPD int poly_support_index(const ShapeView &s, V2 n)
+template <class T> CPD int poly_support_index(const ShapeViewT<T> &s, V2T<T> n)
 {
-	double max = -__builtin_inf();
+	T max = T(-__builtin_inf());
 	int index = 0;
 	for (int i = 0; i < s.count; i++) {
-		const double d = vdot(ld2(s.wv, i), n);
+		const T d = vdot(s.wv[i], n);
 		if (d > max) { max = d; index = i; }
 	}
 	return index;
 }
 
 // Circle/Segment/PolySupportPoint (cpCollision.c:95-114)
-CPD SupportPt shape_support(const ShapeView &s, V2 n)
+template <class T> CPD SupportPtT<T> shape_support(const ShapeViewT<T> &s, V2T<T> n)
 {
-	SupportPt r;
+	SupportPtT<T> r;
 	if (s.type == CPGPU_SHAPE_CIRCLE) { r.p = s.tc; r.index = 0; }
 	else if (s.type == CPGPU_SHAPE_SEGMENT) {
 		if (vdot(s.ta, n) > vdot(s.tb, n)) { r.p = s.ta; r.index = 0; }
 		else { r.p = s.tb; r.index = 1; }
 	} else {
 		const int i = poly_support_index(s, n);
-		r.p = ld2(s.wv, i); r.index = (uint32_t)i;
+		r.p = s.wv[i]; r.index = (uint32_t)i;
 	}
 	return r;
 }
 
 // ShapePoint (cpCollision.c:395-413)
-CPD SupportPt shape_point(const ShapeView &s, int i)
+template <class T> CPD SupportPtT<T> shape_point(const ShapeViewT<T> &s, int i)
 {
-	SupportPt r;
+	SupportPtT<T> r;
 	if (s.type == CPGPU_SHAPE_CIRCLE) { r.p = s.tc; r.index = 0; }
 	else if (s.type == CPGPU_SHAPE_SEGMENT) { r.p = (i == 0 ? s.ta : s.tb); r.index = (uint32_t)i; }
 	else {
 		const int index = (i < s.count ? i : 0);
-		r.p = ld2(s.wv, index); r.index = (uint32_t)index;
+		r.p = s.wv[index]; r.index = (uint32_t)index;
 	}
 	return r;
 }
 
 // MinkowskiPointNew (cpCollision.c:127-132)
-CPD MinkPt mink_new(SupportPt a, SupportPt b)
+template <class T> CPD MinkPtT<T> mink_new(SupportPtT<T> a, SupportPtT<T> b)
 {
-	MinkPt m;
+	MinkPtT<T> m;
 	m.a = a.p; m.b = b.p; m.ab = vsub(b.p, a.p);
 	m.id = ((a.index & 0xFF) << 8) | (b.index & 0xFF);
 	return m;
 }
 
-CPD MinkPt mink_from_id(const ShapeView &s1, const ShapeView &s2, uint32_t id)
+template <class T> CPD MinkPtT<T> mink_from_id(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2, uint32_t id)
 {
 	return mink_new(shape_point(s1, (int)((id >> 8) & 0xFF)), shape_point(s2, (int)(id & 0xFF)));
 }
 
 // Support (cpCollision.c:140-145)
-CPD MinkPt support(const ShapeView &s1, const ShapeView &s2, V2 n)
+template <class T> CPD MinkPtT<T> support(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2, V2T<T> n)
 {
 	return mink_new(shape_support(s1, vneg(n)), shape_support(s2, n));
 }
 
 // cpCheckPointGreater / cpCheckAxis (cpRobust.c:4-13)
-CPD bool check_point_greater(V2 a, V2 b, V2 c)
+template <class T> CPD bool check_point_greater(V2T<T> a, V2T<T> b, V2T<T> c)
 {
-	return (b.y - a.y) * (a.x + b.x - 2 * c.x) > (b.x - a.x) * (a.y + b.y - 2 * c.y);
+	return (b.y - a.y) * (a.x + b.x - 2.0 * c.x) > (b.x - a.x) * (a.y + b.y - 2.0 * c.y);
 }
-CPD bool check_axis(V2 v0, V2 v1, V2 p, V2 n) { return vdot(p, n) <= fmaxc(vdot(v0, n), vdot(v1, n)); }
+template <class T> CPD bool check_axis(V2T<T> v0, V2T<T> v1, V2T<T> p, V2T<T> n)
+{
+	return vdot(p, n) <= fmaxc(vdot(v0, n), vdot(v1, n));
+}
 
 // ClosestT / LerpT (cpCollision.c:193-207)
-CPD double closest_t(V2 a, V2 b)
+template <class T> CPD T closest_t(V2T<T> a, V2T<T> b)
 {
-	const V2 delta = vsub(b, a);
+	const V2T<T> delta = vsub(b, a);
 	return -fclampc(vdot(delta, vadd(a, b)) / (vlengthsq(delta) + CP_DBL_MIN), -1.0, 1.0);
 }
-CPD V2 lerp_t(V2 a, V2 b, double t)
+template <class T> CPD V2T<T> lerp_t(V2T<T> a, V2T<T> b, T t)
 {
-	const double ht = 0.5 * t;
+	const T ht = 0.5 * t;
 	return vadd(vmult(a, 0.5 - ht), vmult(b, 0.5 + ht));
 }
 
 // ClosestPointsNew (cpCollision.c:223-258)
-CPD ClosestPts closest_points_new(const MinkPt &v0, const MinkPt &v1)
+template <class T> CPD ClosestPtsT<T> closest_points_new(const MinkPtT<T> &v0, const MinkPtT<T> &v1)
 {
-	const double t = closest_t(v0.ab, v1.ab);
-	const V2 p = lerp_t(v0.ab, v1.ab, t);
-	ClosestPts r;
+	const T t = closest_t(v0.ab, v1.ab);
+	const V2T<T> p = lerp_t(v0.ab, v1.ab, t);
+	ClosestPtsT<T> r;
 	r.a = lerp_t(v0.a, v1.a, t);
 	r.b = lerp_t(v0.b, v1.b, t);
 	r.id = ((v0.id & 0xFFFF) << 16) | (v1.id & 0xFFFF);
-	const V2 delta = vsub(v1.ab, v0.ab);
-	const V2 n = vnormalize(vrperp(delta));
-	const double d = vdot(n, p);
+	const V2T<T> delta = vsub(v1.ab, v0.ab);
+	const V2T<T> n = vnormalize(vrperp(delta));
+	const T d = vdot(n, p);
 	if (d <= 0.0 || (-1.0 < t && t < 1.0)) { r.n = n; r.d = d; }
 	else {
-		const double d2 = vlength(p);
+		const T d2 = vlength(p);
 		r.n = vmult(p, 1.0 / (d2 + CP_DBL_MIN));
 		r.d = d2;
 	}
@@ -172,11 +183,12 @@ CPD ClosestPts closest_points_new(const MinkPt &v0, const MinkPt &v1)
 }
 
 // ClosestDist (cpCollision.c:262-266)
-CPD double closest_dist(V2 v0, V2 v1) { return vlengthsq(lerp_t(v0, v1, closest_t(v0, v1))); }
+template <class T> CPD T closest_dist(V2T<T> v0, V2T<T> v1) { return vlengthsq(lerp_t(v0, v1, closest_t(v0, v1))); }
 
 // EPA / EPARecurse (cpCollision.c:270-343) as a loop over id-only hulls.
-CPD ClosestPts epa(const ShapeView &s1, const ShapeView &s2, const MinkPt &v0, const MinkPt &v1, const MinkPt &v2,
-                   int32_t *err)
+template <class T>
+CPD ClosestPtsT<T> epa(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2, const MinkPtT<T> &v0, const MinkPtT<T> &v1,
+                       const MinkPtT<T> &v2, int32_t *err)
 {
 	uint32_t hullA[NP_EPA_HULL_MAX], hullB[NP_EPA_HULL_MAX];
 	uint32_t *hull = hullA, *hull2 = hullB;
@@ -184,17 +196,17 @@ CPD ClosestPts epa(const ShapeView &s1, const ShapeView &s2, const MinkPt &v0, c
 	hull[0] = v0.id; hull[1] = v1.id; hull[2] = v2.id;
 	for (int iteration = 1;; iteration++) {
 		int mini = 0;
-		double minDist = __builtin_inf();
-		MinkPt hi = mink_from_id(s1, s2, hull[count - 1]);
+		T minDist = T(__builtin_inf());
+		MinkPtT<T> hi = mink_from_id(s1, s2, hull[count - 1]);
 		for (int j = 0; j < count; j++) {
-			const MinkPt hj = mink_from_id(s1, s2, hull[j]);
-			const double d = closest_dist(hi.ab, hj.ab);
+			const MinkPtT<T> hj = mink_from_id(s1, s2, hull[j]);
+			const T d = closest_dist(hi.ab, hj.ab);
 			if (d < minDist) { minDist = d; mini = (j == 0 ? count - 1 : j - 1); }
 			hi = hj;
 		}
-		const MinkPt e0 = mink_from_id(s1, s2, hull[mini]);
-		const MinkPt e1 = mink_from_id(s1, s2, hull[(mini + 1) % count]);
-		const MinkPt p = support(s1, s2, vperp(vsub(e1.ab, e0.ab)));
+		const MinkPtT<T> e0 = mink_from_id(s1, s2, hull[mini]);
+		const MinkPtT<T> e1 = mink_from_id(s1, s2, hull[(mini + 1) % count]);
+		const MinkPtT<T> p = support(s1, s2, vperp(vsub(e1.ab, e0.ab)));
 		const bool duplicate = (p.id == e0.id || p.id == e1.id);
 		if (!duplicate && check_point_greater(e0.ab, e1.ab, p.ab) && iteration < NP_MAX_EPA_ITERATIONS) {
 			if (count + 1 > NP_EPA_HULL_MAX) { raise_err(err, CPGPU_ERR_EPA_HULL); return closest_points_new(e0, e1); }
@@ -202,9 +214,9 @@ CPD ClosestPts epa(const ShapeView &s1, const ShapeView &s2, const MinkPt &v0, c
 			hull2[0] = p.id;
 			for (int i = 0; i < count; i++) {
 				const int index = (mini + 1 + i) % count;
-				const V2 h0 = mink_from_id(s1, s2, hull2[count2 - 1]).ab;
-				const V2 h1 = mink_from_id(s1, s2, hull[index]).ab;
-				const V2 h2 = (i + 1 < count ? mink_from_id(s1, s2, hull[(index + 1) % count]) : p).ab;
+				const V2T<T> h0 = mink_from_id(s1, s2, hull2[count2 - 1]).ab;
+				const V2T<T> h1 = mink_from_id(s1, s2, hull[index]).ab;
+				const V2T<T> h2 = (i + 1 < count ? mink_from_id(s1, s2, hull[(index + 1) % count]) : p).ab;
 				if (check_point_greater(h0, h2, h1)) hull2[count2++] = hull[index];
 			}
 			uint32_t *tmp = hull; hull = hull2; hull2 = tmp;
@@ -216,19 +228,22 @@ CPD ClosestPts epa(const ShapeView &s1, const ShapeView &s2, const MinkPt &v0, c
 }
 
 // GJKRecurse (cpCollision.c:348-392) as a loop
-CPD ClosestPts gjk_recurse(const ShapeView &s1, const ShapeView &s2, MinkPt v0, MinkPt v1, int32_t *err)
+template <class T>
+CPD ClosestPtsT<T> gjk_recurse(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2, MinkPtT<T> v0, MinkPtT<T> v1,
+                               int32_t *err)
 {
+	const V2T<T> origin = v2t<T>(T(0.0), T(0.0));
 	int iteration = 1;
 	for (;;) {
 		if (iteration > NP_MAX_GJK_ITERATIONS) return closest_points_new(v0, v1);
-		if (check_point_greater(v1.ab, v0.ab, v2(0.0, 0.0))) {
-			const MinkPt tmp = v0; v0 = v1; v1 = tmp;
+		if (check_point_greater(v1.ab, v0.ab, origin)) {
+			const MinkPtT<T> tmp = v0; v0 = v1; v1 = tmp;
 			continue;
 		}
-		const double t = closest_t(v0.ab, v1.ab);
-		const V2 n = (-1.0 < t && t < 1.0) ? vperp(vsub(v1.ab, v0.ab)) : vneg(lerp_t(v0.ab, v1.ab, t));
-		const MinkPt p = support(s1, s2, n);
-		if (check_point_greater(p.ab, v0.ab, v2(0.0, 0.0)) && check_point_greater(v1.ab, p.ab, v2(0.0, 0.0))) {
+		const T t = closest_t(v0.ab, v1.ab);
+		const V2T<T> n = (-1.0 < t && t < 1.0) ? vperp(vsub(v1.ab, v0.ab)) : vneg(lerp_t(v0.ab, v1.ab, t));
+		const MinkPtT<T> p = support(s1, s2, n);
+		if (check_point_greater(p.ab, v0.ab, origin) && check_point_greater(v1.ab, p.ab, origin)) {
 			return epa(s1, s2, v0, p, v1, err);
 		}
 		if (check_axis(v0.ab, v1.ab, p.ab, n)) return closest_points_new(v0, v1);
@@ -241,24 +256,25 @@ CPD ClosestPts gjk_recurse(const ShapeView &s1, const ShapeView &s2, MinkPt v0, 
 CPD V2 bb_center(const double *bb) { return vlerp(v2(bb[0], bb[1]), v2(bb[2], bb[3]), 0.5); }
 
 // GJK (cpCollision.c:416-472)
-CPD ClosestPts gjk(const ShapeView &s1, const ShapeView &s2, uint32_t &id, int32_t *err)
+template <class T>
+CPD ClosestPtsT<T> gjk(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2, uint32_t &id, int32_t *err)
 {
-	MinkPt v0, v1;
+	MinkPtT<T> v0, v1;
 	if (id) {
 		v0 = mink_new(shape_point(s1, (int)((id >> 24) & 0xFF)), shape_point(s2, (int)((id >> 16) & 0xFF)));
 		v1 = mink_new(shape_point(s1, (int)((id >> 8) & 0xFF)), shape_point(s2, (int)(id & 0xFF)));
 	} else {
-		const V2 axis = vperp(vsub(bb_center(s1.bb), bb_center(s2.bb)));
+		const V2T<T> axis = vcast<T>(vperp(vsub(bb_center(s1.bb), bb_center(s2.bb))));
 		v0 = support(s1, s2, axis);
 		v1 = support(s1, s2, vneg(axis));
 	}
-	const ClosestPts points = gjk_recurse(s1, s2, v0, v1, err);
+	const ClosestPtsT<T> points = gjk_recurse(s1, s2, v0, v1, err);
 	id = points.id;
 	return points;
 }
 
 // cpCollisionInfoPushContact (cpCollision.c:44-55)
-CPD void push_contact(CollInfo &info, V2 p1, V2 p2, uint64_t hash)
+template <class T> CPD void push_contact(CollInfoT<T> &info, V2T<T> p1, V2T<T> p2, uint64_t hash)
 {
 	info.p1[info.count] = p1;
 	info.p2[info.count] = p2;
@@ -267,59 +283,60 @@ CPD void push_contact(CollInfo &info, V2 p1, V2 p2, uint64_t hash)
 }
 
 // ContactPoints (cpCollision.c:477-518)
-CPD void contact_points(const Edge &e1, const Edge &e2, const ClosestPts &points, CollInfo &info)
+template <class T>
+CPD void contact_points(const EdgeT<T> &e1, const EdgeT<T> &e2, const ClosestPtsT<T> &points, CollInfoT<T> &info)
 {
 	const double mindist = e1.r + e2.r;
 	if (points.d <= mindist) {
-		const V2 n = info.n = points.n;
-		const double d_e1_a = vcross(e1.a.p, n);
-		const double d_e1_b = vcross(e1.b.p, n);
-		const double d_e2_a = vcross(e2.a.p, n);
-		const double d_e2_b = vcross(e2.b.p, n);
-		const double e1_denom = 1.0 / (d_e1_b - d_e1_a + CP_DBL_MIN);
-		const double e2_denom = 1.0 / (d_e2_b - d_e2_a + CP_DBL_MIN);
+		const V2T<T> n = info.n = points.n;
+		const T d_e1_a = vcross(e1.a.p, n);
+		const T d_e1_b = vcross(e1.b.p, n);
+		const T d_e2_a = vcross(e2.a.p, n);
+		const T d_e2_b = vcross(e2.b.p, n);
+		const T e1_denom = 1.0 / (d_e1_b - d_e1_a + CP_DBL_MIN);
+		const T e2_denom = 1.0 / (d_e2_b - d_e2_a + CP_DBL_MIN);
 		{
-			const V2 p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_b - d_e1_a) * e1_denom)));
-			const V2 p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_a - d_e2_a) * e2_denom)));
-			const double dist = vdot(vsub(p2, p1), n);
+			const V2T<T> p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_b - d_e1_a) * e1_denom)));
+			const V2T<T> p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_a - d_e2_a) * e2_denom)));
+			const T dist = vdot(vsub(p2, p1), n);
 			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.a.hash, e2.b.hash));
 		}
 		{
-			const V2 p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_a - d_e1_a) * e1_denom)));
-			const V2 p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_b - d_e2_a) * e2_denom)));
-			const double dist = vdot(vsub(p2, p1), n);
+			const V2T<T> p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_a - d_e1_a) * e1_denom)));
+			const V2T<T> p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_b - d_e2_a) * e2_denom)));
+			const T dist = vdot(vsub(p2, p1), n);
 			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.b.hash, e2.a.hash));
 		}
 	}
 }
 
 // SupportEdgeForPoly (cpCollision.c:160-178)
-CPD Edge support_edge_poly(const ShapeView &s, V2 n)
+template <class T> CPD EdgeT<T> support_edge_poly(const ShapeViewT<T> &s, V2T<T> n)
 {
 	const int count = s.count;
 	const int i1 = poly_support_index(s, n);
 	const int i0 = (i1 - 1 + count) % count;
 	const int i2 = (i1 + 1) % count;
 	const uint64_t h = s.hashid;
-	Edge e;
+	EdgeT<T> e;
 	e.r = s.r;
-	if (vdot(n, ld2(s.wn, i1)) > vdot(n, ld2(s.wn, i2))) {
-		e.a.p = ld2(s.wv, i0); e.a.hash = hash_pair(h, (uint64_t)i0);
-		e.b.p = ld2(s.wv, i1); e.b.hash = hash_pair(h, (uint64_t)i1);
-		e.n = ld2(s.wn, i1);
+	if (vdot(n, s.wn[i1]) > vdot(n, s.wn[i2])) {
+		e.a.p = s.wv[i0]; e.a.hash = hash_pair(h, (uint64_t)i0);
+		e.b.p = s.wv[i1]; e.b.hash = hash_pair(h, (uint64_t)i1);
+		e.n = s.wn[i1];
 	} else {
-		e.a.p = ld2(s.wv, i1); e.a.hash = hash_pair(h, (uint64_t)i1);
-		e.b.p = ld2(s.wv, i2); e.b.hash = hash_pair(h, (uint64_t)i2);
-		e.n = ld2(s.wn, i2);
+		e.a.p = s.wv[i1]; e.a.hash = hash_pair(h, (uint64_t)i1);
+		e.b.p = s.wv[i2]; e.b.hash = hash_pair(h, (uint64_t)i2);
+		e.n = s.wn[i2];
 	}
 	return e;
 }
 
 // SupportEdgeForSegment (cpCollision.c:180-191)
-CPD Edge support_edge_segment(const ShapeView &s, V2 n)
+template <class T> CPD EdgeT<T> support_edge_segment(const ShapeViewT<T> &s, V2T<T> n)
 {
 	const uint64_t h = s.hashid;
-	Edge e;
+	EdgeT<T> e;
 	e.r = s.r;
 	if (vdot(s.tn, n) > 0.0) {
 		e.a.p = s.ta; e.a.hash = hash_pair(h, 0);
@@ -334,92 +351,97 @@ CPD Edge support_edge_segment(const ShapeView &s, V2 n)
 }
 
 // CircleToCircle (cpCollision.c:525-537)
-CPD void circle_circle(const ShapeView &c1, const ShapeView &c2, CollInfo &info)
+template <class T> CPD void circle_circle(const ShapeViewT<T> &c1, const ShapeViewT<T> &c2, CollInfoT<T> &info)
 {
 	const double mindist = c1.r + c2.r;
-	const V2 delta = vsub(c2.tc, c1.tc);
-	const double distsq = vlengthsq(delta);
+	const V2T<T> delta = vsub(c2.tc, c1.tc);
+	const T distsq = vlengthsq(delta);
 	if (distsq < mindist * mindist) {
-		const double dist = __builtin_sqrt(distsq);
-		const V2 n = info.n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : v2(1.0, 0.0));
+		const T dist = csqrt(distsq);
+		const V2T<T> n = info.n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : v2t<T>(T(1.0), T(0.0)));
 		push_contact(info, vadd(c1.tc, vmult(n, c1.r)), vadd(c2.tc, vmult(n, -c2.r)), 0);
 	}
 }
 
 // CircleToSegment (cpCollision.c:539-569)
-CPD void circle_segment(const ShapeView &circle, const ShapeView &seg, CollInfo &info)
+template <class T> CPD void circle_segment(const ShapeViewT<T> &circle, const ShapeViewT<T> &seg, CollInfoT<T> &info)
 {
-	const V2 seg_a = seg.ta, seg_b = seg.tb, center = circle.tc;
-	const V2 seg_delta = vsub(seg_b, seg_a);
-	const double closest_tt = fclamp01(vdot(seg_delta, vsub(center, seg_a)) / vlengthsq(seg_delta));
-	const V2 closest = vadd(seg_a, vmult(seg_delta, closest_tt));
+	const V2T<T> seg_a = seg.ta, seg_b = seg.tb, center = circle.tc;
+	const V2T<T> seg_delta = vsub(seg_b, seg_a);
+	const T closest_tt = fclamp01(vdot(seg_delta, vsub(center, seg_a)) / vlengthsq(seg_delta));
+	const V2T<T> closest = vadd(seg_a, vmult(seg_delta, closest_tt));
 	const double mindist = circle.r + seg.r;
-	const V2 delta = vsub(closest, center);
-	const double distsq = vlengthsq(delta);
+	const V2T<T> delta = vsub(closest, center);
+	const T distsq = vlengthsq(delta);
 	if (distsq < mindist * mindist) {
-		const double dist = __builtin_sqrt(distsq);
-		const V2 n = info.n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : seg.tn);
-		const V2 rot = seg.rot;
-		if ((closest_tt != 0.0 || vdot(n, vrotate(seg.a_tan, rot)) >= 0.0) &&
-		    (closest_tt != 1.0 || vdot(n, vrotate(seg.b_tan, rot)) >= 0.0)) {
+		const T dist = csqrt(distsq);
+		const V2T<T> n = info.n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : seg.tn);
+		const V2T<T> rot = seg.rot;
+		if ((closest_tt != 0.0 || vdot(n, vrotate(vcast<T>(seg.a_tan), rot)) >= 0.0) &&
+		    (closest_tt != 1.0 || vdot(n, vrotate(vcast<T>(seg.b_tan), rot)) >= 0.0)) {
 			push_contact(info, vadd(center, vmult(n, circle.r)), vadd(closest, vmult(n, -seg.r)), 0);
 		}
 	}
 }
 
 // SegmentToSegment (cpCollision.c:571-604)
-CPD void segment_segment(const ShapeView &s1, const ShapeView &s2, CollInfo &info, int32_t *err)
+template <class T>
+CPD void segment_segment(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2, CollInfoT<T> &info, int32_t *err)
 {
-	const ClosestPts points = gjk(s1, s2, info.id, err);
-	const V2 n = points.n;
-	const V2 rot1 = s1.rot, rot2 = s2.rot;
+	const ClosestPtsT<T> points = gjk(s1, s2, info.id, err);
+	const V2T<T> n = points.n;
+	const V2T<T> rot1 = s1.rot, rot2 = s2.rot;
 	if (points.d <= (s1.r + s2.r) &&
-	    (!veql(points.a, s1.ta) || vdot(n, vrotate(s1.a_tan, rot1)) <= 0.0) &&
-	    (!veql(points.a, s1.tb) || vdot(n, vrotate(s1.b_tan, rot1)) <= 0.0) &&
-	    (!veql(points.b, s2.ta) || vdot(n, vrotate(s2.a_tan, rot2)) >= 0.0) &&
-	    (!veql(points.b, s2.tb) || vdot(n, vrotate(s2.b_tan, rot2)) >= 0.0)) {
+	    (!veql(points.a, s1.ta) || vdot(n, vrotate(vcast<T>(s1.a_tan), rot1)) <= 0.0) &&
+	    (!veql(points.a, s1.tb) || vdot(n, vrotate(vcast<T>(s1.b_tan), rot1)) <= 0.0) &&
+	    (!veql(points.b, s2.ta) || vdot(n, vrotate(vcast<T>(s2.a_tan), rot2)) >= 0.0) &&
+	    (!veql(points.b, s2.tb) || vdot(n, vrotate(vcast<T>(s2.b_tan), rot2)) >= 0.0)) {
 		contact_points(support_edge_segment(s1, n), support_edge_segment(s2, vneg(n)), points, info);
 	}
 }
 
 // PolyToPoly (cpCollision.c:606-627)
-CPD void poly_poly(const ShapeView &p1, const ShapeView &p2, CollInfo &info, int32_t *err)
+template <class T>
+CPD void poly_poly(const ShapeViewT<T> &p1, const ShapeViewT<T> &p2, CollInfoT<T> &info, int32_t *err)
 {
-	const ClosestPts points = gjk(p1, p2, info.id, err);
+	const ClosestPtsT<T> points = gjk(p1, p2, info.id, err);
 	if (points.d - p1.r - p2.r <= 0.0) {
 		contact_points(support_edge_poly(p1, points.n), support_edge_poly(p2, vneg(points.n)), points, info);
 	}
 }
 
 // SegmentToPoly (cpCollision.c:629-659)
-CPD void segment_poly(const ShapeView &seg, const ShapeView &poly, CollInfo &info, int32_t *err)
+template <class T>
+CPD void segment_poly(const ShapeViewT<T> &seg, const ShapeViewT<T> &poly, CollInfoT<T> &info, int32_t *err)
 {
-	const ClosestPts points = gjk(seg, poly, info.id, err);
-	const V2 n = points.n;
-	const V2 rot = seg.rot;
+	const ClosestPtsT<T> points = gjk(seg, poly, info.id, err);
+	const V2T<T> n = points.n;
+	const V2T<T> rot = seg.rot;
 	if (points.d - seg.r - poly.r <= 0.0 &&
-	    (!veql(points.a, seg.ta) || vdot(n, vrotate(seg.a_tan, rot)) <= 0.0) &&
-	    (!veql(points.a, seg.tb) || vdot(n, vrotate(seg.b_tan, rot)) <= 0.0)) {
+	    (!veql(points.a, seg.ta) || vdot(n, vrotate(vcast<T>(seg.a_tan), rot)) <= 0.0) &&
+	    (!veql(points.a, seg.tb) || vdot(n, vrotate(vcast<T>(seg.b_tan), rot)) <= 0.0)) {
 		contact_points(support_edge_segment(seg, n), support_edge_poly(poly, vneg(n)), points, info);
 	}
 }
 
 // CircleToPoly (cpCollision.c:661-679)
-CPD void circle_poly(const ShapeView &circle, const ShapeView &poly, CollInfo &info, int32_t *err)
+template <class T>
+CPD void circle_poly(const ShapeViewT<T> &circle, const ShapeViewT<T> &poly, CollInfoT<T> &info, int32_t *err)
 {
-	const ClosestPts points = gjk(circle, poly, info.id, err);
+	const ClosestPtsT<T> points = gjk(circle, poly, info.id, err);
 	if (points.d <= circle.r + poly.r) {
-		const V2 n = info.n = points.n;
+		const V2T<T> n = info.n = points.n;
 		push_contact(info, vadd(points.a, vmult(n, circle.r)), vadd(points.b, vmult(n, -poly.r)), 0);
 	}
 }
 
 // cpCollide dispatch (cpCollision.c:688-726); a.type <= b.type
-CPD void collide(const ShapeView &a, const ShapeView &b, uint32_t id, CollInfo &info, int32_t *err)
+template <class T>
+CPD void collide(const ShapeViewT<T> &a, const ShapeViewT<T> &b, uint32_t id, CollInfoT<T> &info, int32_t *err)
 {
 	info.count = 0;
 	info.id = id;
-	info.n = v2(0.0, 0.0);
+	info.n = v2t<T>(T(0.0), T(0.0));
 	switch (a.type + b.type * 3) {
 	case 0: circle_circle(a, b, info); break;
 	case 3: circle_segment(a, b, info); break;

@@ -24,20 +24,26 @@ namespace {
 constexpr int GRID_THREADS = 1024;  // grid solve: few, large workgroups keep the barrier cheap
 
 // ---- grid barrier (all blocks co-resident: cooperative launch) ----
+// One lane per workgroup: a release fence (writes back this XCD's L2) before
+// arriving, a relaxed spin on the generation word, one acquire fence
+// (invalidates L2) after it changes.  Acquire loads inside the spin would
+// invalidate L2 on every poll while other workgroups still work.
 __device__ void grid_sync(GridBarrier *bar, int32_t *err)
 {
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		const unsigned int nblocks = gridDim.x;
-		const unsigned int g = __hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-		__threadfence();
-		const unsigned int arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+		const unsigned int g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+		const unsigned int arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (arrived == nblocks - 1) {
+			// every other workgroup has arrived: its writes precede its arrival
+			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
 			__hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			__hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+			__hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		} else {
 			long spins = 0;
-			while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+			while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
 				__builtin_amdgcn_s_sleep(1);
 				if (++spins > (1L << 25)) {
 					// a barrier that never completes ends the kernel instead of hanging it
@@ -49,7 +55,7 @@ __device__ void grid_sync(GridBarrier *bar, int32_t *err)
 					break;
 			}
 		}
-		__threadfence();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 	}
 	__syncthreads();
 }

@@ -1060,6 +1060,80 @@ int cpgpuWorldSetTiming(cpgpuWorld *w, int32_t enable)
 	return CPGPU_OK;
 }
 
+int cpgpuWorldGetBuckets(cpgpuWorld *w, int32_t *max_bodies, int32_t *threads, int32_t *n_worlds, int32_t *lds_bodies)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++) {
+		const bool on = q < w->n_buckets;
+		if (max_bodies) max_bodies[q] = on ? w->buckets[q].max_bodies : 0;
+		if (threads) threads[q] = on ? w->buckets[q].threads : 0;
+		if (n_worlds) n_worlds[q] = on ? w->buckets[q].n : 0;
+		if (lds_bodies) lds_bodies[q] = on ? w->buckets[q].lds_cap : 0;
+	}
+	return CPGPU_OK;
+}
+
+static bool valid_threads(int t) { return t == 64 || t == 128 || t == 256 || t == 512 || t == 1024; }
+
+int cpgpuWorldSetBuckets(cpgpuWorld *w, int32_t n, const int32_t *max_bodies, const int32_t *threads)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (n < 1 || n > CPGPU_MAX_BUCKETS) return fail(CPGPU_ERR_INVALID, "bucket count out of range");
+	for (int q = 0; q < n; q++) {
+		if (!valid_threads(threads[q])) return fail(CPGPU_ERR_INVALID, "bucket threads must be 64..1024, a power of 2");
+		if (q > 0 && max_bodies[q] <= max_bodies[q - 1]) return fail(CPGPU_ERR_INVALID, "bucket limits must rise");
+	}
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	w->n_buckets = n;
+	for (int q = 0; q < n; q++) {
+		w->buckets[q].max_bodies = max_bodies[q];
+		w->buckets[q].threads = threads[q];
+	}
+	HIP_OR_FAIL(build_buckets(w));
+	return CPGPU_OK;
+}
+
+int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *threads_out)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (trial_steps < 1) return fail(CPGPU_ERR_INVALID, "trial_steps must be positive");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	static const int candidates[] = {64, 128, 256, 512, 1024};
+	static const char *names[CPGPU_MAX_BUCKETS] = {"solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2",
+	                                               "solver_worlds_b3"};
+	const bool was_timing = w->timer.enabled;
+	auto saved = w->timer.acc;
+	double best[CPGPU_MAX_BUCKETS];
+	int best_t[CPGPU_MAX_BUCKETS];
+	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++) { best[q] = 1e300; best_t[q] = q < w->n_buckets ? w->buckets[q].threads : 0; }
+	w->timer.enabled = true;
+	for (int t : candidates) {
+		// every bucket tries the same workgroup size at once (the buckets run concurrently)
+		for (int q = 0; q < w->n_buckets; q++) w->buckets[q].threads = t;
+		w->timer.acc.clear();
+		for (int k = 0; k < trial_steps; k++) {
+			rc = cpgpuWorldStep(w, dt);
+			if (rc != CPGPU_OK) { w->timer.enabled = was_timing; return rc; }
+		}
+		rc = cpgpuWorldSync(w);
+		if (rc != CPGPU_OK) { w->timer.enabled = was_timing; return rc; }
+		for (int q = 0; q < w->n_buckets; q++) {
+			auto it = w->timer.acc.find(names[q]);
+			if (it == w->timer.acc.end() || it->second.second == 0) continue;
+			const double per = it->second.first / it->second.second;
+			if (per < best[q]) { best[q] = per; best_t[q] = t; }
+		}
+	}
+	for (int q = 0; q < w->n_buckets; q++) w->buckets[q].threads = best_t[q];
+	if (threads_out)
+		for (int q = 0; q < CPGPU_MAX_BUCKETS; q++) threads_out[q] = q < w->n_buckets ? best_t[q] : 0;
+	w->timer.acc = saved;
+	w->timer.enabled = was_timing;
+	return CPGPU_OK;
+}
+
 int cpgpuWorldGetSolveCounts(cpgpuWorld *w, int64_t *contacts, int64_t *bodies)
 {
 	int rc = cpgpuWorldSync(w);

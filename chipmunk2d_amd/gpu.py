@@ -26,7 +26,7 @@ EXPORTS = [
     "cpgpuWorldCreate", "cpgpuWorldDestroy", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
     "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
-    "cpgpuGetLastError", "cpgpuVersion",
+    "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuGetLastError", "cpgpuVersion",
 ]
 
 
@@ -55,6 +55,9 @@ def lib():
         L.cpgpuWorldSetTiming.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
         L.cpgpuWorldGetSolveCounts.argtypes = [_vp, _i64p, _i64p]
+        L.cpgpuWorldGetBuckets.argtypes = [_vp, _ip, _ip, _ip, _ip]
+        L.cpgpuWorldSetBuckets.argtypes = [_vp, ctypes.c_int32, _ip, _ip]
+        L.cpgpuWorldAutotune.argtypes = [_vp, ctypes.c_double, ctypes.c_int32, _ip]
         L.cpgpuGetLastError.restype = ctypes.c_char_p
         L.cpgpuVersion.restype = ctypes.c_char_p
         _lib = L
@@ -149,6 +152,23 @@ class GPUWorld:
                                           ptr(launches, ctypes.c_int64), names, 8192))
         keys = names.value.decode().split("\n")[:n.value]
         return {k: (float(ms[i]), int(launches[i])) for i, k in enumerate(keys)}
+
+    def buckets(self):
+        """list of (max_bodies, threads, n_worlds, lds_bodies) per size bucket."""
+        a = [np.zeros(4, np.int32) for _ in range(4)]
+        _check(self.L.cpgpuWorldGetBuckets(self.h, *[ptr(x, ctypes.c_int32) for x in a]))
+        return [tuple(int(x[q]) for x in a) for q in range(4)]
+
+    def set_buckets(self, max_bodies, threads):
+        mb = np.ascontiguousarray(max_bodies, np.int32)
+        th = np.ascontiguousarray(threads, np.int32)
+        _check(self.L.cpgpuWorldSetBuckets(self.h, len(mb), ptr(mb, ctypes.c_int32), ptr(th, ctypes.c_int32)))
+
+    def autotune(self, dt, trial_steps=3):
+        """Pick each size bucket's workgroup size by timing (advances the world 5 x trial_steps steps)."""
+        out = np.zeros(4, np.int32)
+        _check(self.L.cpgpuWorldAutotune(self.h, dt, trial_steps, ptr(out, ctypes.c_int32)))
+        return [int(x) for x in out]
 
     SOLVE_UNITS = ("solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2", "solver_worlds_b3", "solver_islands",
                    "solver")

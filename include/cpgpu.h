@@ -214,6 +214,24 @@ int cpgpuWorldSetTiming(cpgpuWorld *world, int32_t enable);
 int cpgpuWorldGetTiming(cpgpuWorld *world, int32_t cap, int32_t *n_out, double *ms,
                         int64_t *launches, char *names, int32_t names_cap);
 
+/* ---- batch tiling (no counterpart in the reference: a batch of spaces is
+ * this library's own form).  Spaces of at most 4096 dynamic bodies are
+ * solved one workgroup each, in up to CPGPU_MAX_BUCKETS size buckets
+ * (bucket q takes the spaces with more bodies than bucket q-1's limit and at
+ * most max_bodies[q], static bodies included; the last takes the rest).
+ * Each bucket has its own workgroup size (64, 128, 256, 512 or 1024
+ * threads) and stages its spaces' bodies in LDS when they fit; the buckets
+ * run concurrently.  Defaults: limits 64, 256, 1024, unbounded with 64,
+ * 128, 256, 512 threads. */
+#define CPGPU_MAX_BUCKETS 4
+int cpgpuWorldGetBuckets(cpgpuWorld *world, int32_t *max_bodies, int32_t *threads, int32_t *n_worlds,
+                         int32_t *lds_bodies);
+int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies, const int32_t *threads);
+/* Autotuner: steps the world 5 x trial_steps times (the state advances),
+ * timing every bucket's solve at each workgroup size, and keeps the fastest
+ * size per bucket (reported in threads_out[CPGPU_MAX_BUCKETS]). */
+int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out);
+
 /* Contacts solved and bodies velocity-integrated since creation by each
  * solve unit, in the order of the timing names solver_worlds_b0 ..
  * solver_worlds_b3 (size buckets of small worlds), solver_islands,

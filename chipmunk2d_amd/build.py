@@ -13,7 +13,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcpgpu.so")
-SOURCES = ["world.hip", "broadphase.hip", "narrowphase.hip", "solver.hip"]
+SOURCES = ["world.hip", "broadphase.hip", "narrowphase.hip", "solver.hip", "adjoint.hip"]
 ARCH = os.environ.get("CPGPU_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -1,0 +1,649 @@
+// adjoint.hip -- reverse-mode derivative of a taped rollout of cpSpaceStep.
+//
+// The reference has no derivative; this is the library's own extension.
+// A taped rollout records, for every step, the solved arbiters after the
+// solve (their contacts, with the recorded contact each warm start came
+// from) and, every few steps, the bodies' state (checkpoints).  The backward
+// pass walks the checkpoint segments from the last: it replays a segment's
+// steps from its checkpoint with the step's own kernels (the replayed values
+// are bit-identical to the rollout's), recording each arbiter's pre-state in
+// every sweep, then reverses the segment's steps one by one with the kernels
+// below, one workgroup per space:
+//   impulse sweeps in reverse (cpArbiterApplyImpulse, cpArbiter.c:459-498),
+//   cached impulses (cpArbiterApplyCachedImpulse, :441-455),
+//   velocity integration (cpBodyUpdateVelocity, cpBody.c:493-509),
+//   arbiter pre-step (cpArbiterPreStep, cpArbiter.c:416-439),
+//   contact geometry: the pair's collision re-run on dual numbers with the
+//     rollout's GJK id (same branches, so the contact features stay those of
+//     the rollout) gives the derivatives of normal and contact points with
+//     respect to both bodies' positions and angles,
+//   position integration (cpBodyUpdatePosition, cpBody.c:511-522).
+// Warm starts carry adjoints from a contact back to the recorded contact its
+// accumulated impulses came from.  Derivatives at the clamps of the
+// projected Gauss-Seidel follow the branch the rollout took.
+#include "cpgpu_adjoint.h"
+#include "cpgpu_dual.h"
+#include "narrowphase.h"
+
+namespace {
+
+constexpr int ADJ_THREADS = 256;
+constexpr int DUAL_MAX_VERTS = 8;
+
+__device__ __forceinline__ bool dyn(const DevWorld &W, int b) { return W.b_type[b] == CPGPU_BODY_DYNAMIC; }
+
+// ---- tape records ----
+
+__global__ void k_tape_record(Arbiter *arbs, const int32_t *n_ptr, int n_host, Arbiter *tape, int off)
+{
+	const int n = n_ptr ? *n_ptr : n_host;
+	for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < n; m += gridDim.x * blockDim.x) {
+		Arbiter a = arbs[m];
+		for (int c = 0; c < 2; c++) a.c[c].tape = 2 * (off + m) + c;
+		arbs[m].c[0].tape = a.c[0].tape;
+		arbs[m].c[1].tape = a.c[1].tape;
+		tape[off + m] = a;
+	}
+}
+
+// the rollout's arbiters of one step with their accumulated impulses reset to
+// the warm start they started from
+__global__ void k_replay_prepare(const Arbiter *tape, int off, int n, Arbiter *act)
+{
+	const int m = blockIdx.x * blockDim.x + threadIdx.x;
+	if (m >= n) return;
+	Arbiter a = tape[off + m];
+	for (int c = 0; c < 2; c++) {
+		const int src = a.c[c].src;
+		if (c < a.count && src >= 0) {
+			a.c[c].jnAcc = tape[src >> 1].c[src & 1].jnAcc;
+			a.c[c].jtAcc = tape[src >> 1].c[src & 1].jtAcc;
+		} else {
+			a.c[c].jnAcc = 0.0;
+			a.c[c].jtAcc = 0.0;
+		}
+	}
+	act[m] = a;
+}
+
+__global__ void k_state_save(const DevWorld W, BodyState *out)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= W.n_bodies) return;
+	BodyState s;
+	s.p = W.b_p[b]; s.a = W.b_a[b]; s.vel = W.b_vel[b]; s.f = W.b_f[b]; s.t = W.b_t[b]; s.T = W.b_T[b];
+	out[b] = s;
+}
+
+__global__ void k_state_load(const DevWorld W, const BodyState *in)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= W.n_bodies) return;
+	const BodyState s = in[b];
+	W.b_p[b] = s.p; W.b_a[b] = s.a; W.b_vel[b] = s.vel; W.b_f[b] = s.f; W.b_t[b] = s.t; W.b_T[b] = s.T;
+}
+
+// ---- the reverse of one contact's impulse (cpArbiter.c:459-498) ----
+
+struct BodyV {
+	V2 v, vb;
+	double w, wb;
+};
+
+struct ContactAdj {
+	V2 r1, r2;
+	double nMass, tMass, bias, bounce;
+};
+
+// per arbiter: adjoints of its parameters, accumulated over the sweeps, and
+// of its contacts' accumulated impulses
+struct ArbAdj {
+	ContactAdj c[2];
+	V2 n, svr;
+	double u;
+	double jn[2], jt[2], jb[2];
+};
+
+__device__ __forceinline__ BodyV bodyv_from(const double *r)
+{
+	BodyV b;
+	b.v = v2(r[0], r[1]); b.w = r[2]; b.vb = v2(r[3], r[4]); b.wb = r[5];
+	return b;
+}
+
+// The forward of contact i of arb on (a, b) with accumulators (jn, jt, jb):
+// the same operations as arb_apply_impulse.
+__device__ void contact_forward(const Arbiter &arb, int i, double mA, double iA, double mB, double iB, BodyV &a,
+                                BodyV &b, double &jnAcc, double &jtAcc, double &jBias)
+{
+	const Contact &con = arb.c[i];
+	const V2 n = v2(arb.nx, arb.ny);
+	const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
+	const V2 vb1 = vadd(a.vb, vmult(vperp(r1), a.wb));
+	const V2 vb2 = vadd(b.vb, vmult(vperp(r2), b.wb));
+	const V2 vr = vadd(vsub(vadd(b.v, vmult(vperp(r2), b.w)), vadd(a.v, vmult(vperp(r1), a.w))), v2(arb.svx, arb.svy));
+	const double vbn = vdot(vsub(vb2, vb1), n);
+	const double vrn = vdot(vr, n);
+	const double vrt = vdot(vr, vperp(n));
+	const double jbn = (con.bias - vbn) * con.nMass;
+	const double jbnOld = jBias;
+	jBias = fmaxc(jbnOld + jbn, 0.0);
+	const double jn = -(con.bounce + vrn) * con.nMass;
+	const double jnOld = jnAcc;
+	jnAcc = fmaxc(jnOld + jn, 0.0);
+	const double jtMax = arb.u * jnAcc;
+	const double jt = -vrt * con.tMass;
+	const double jtOld = jtAcc;
+	jtAcc = fclampc(jtOld + jt, -jtMax, jtMax);
+	const V2 jb = vmult(n, jBias - jbnOld);
+	a.vb = vadd(a.vb, vmult(vneg(jb), mA)); a.wb += iA * vcross(r1, vneg(jb));
+	b.vb = vadd(b.vb, vmult(jb, mB)); b.wb += iB * vcross(r2, jb);
+	const V2 j = vrotate(n, v2(jnAcc - jnOld, jtAcc - jtOld));
+	a.v = vadd(a.v, vmult(vneg(j), mA)); a.w += iA * vcross(r1, vneg(j));
+	b.v = vadd(b.v, vmult(j, mB)); b.w += iB * vcross(r2, j);
+}
+
+// Reverse of contact_forward.  (a, b, jnOld, jtOld, jbOld): the pre-state;
+// ga, gb: adjoints of the bodies after the contact on entry, before it on
+// exit; G.jn/jt/jb[i]: adjoints of the contact's accumulators after it on
+// entry, before it on exit.  Parameter adjoints add into G and G.c[i], the
+// bodies' inverse mass and moment adjoints into gm (mA, iA, mB, iB).
+__device__ void contact_reverse(const Arbiter &arb, int i, double mA, double iA, double mB, double iB,
+                                const BodyV &a, const BodyV &b, double jnOld, double jtOld, double jbOld, BodyV &ga,
+                                BodyV &gb, ArbAdj &G, double gm[4])
+{
+	const Contact &con = arb.c[i];
+	ContactAdj &gc = G.c[i];
+	const V2 n = v2(arb.nx, arb.ny);
+	const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
+	// forward intermediates
+	const V2 vb1 = vadd(a.vb, vmult(vperp(r1), a.wb));
+	const V2 vb2 = vadd(b.vb, vmult(vperp(r2), b.wb));
+	const V2 vr = vadd(vsub(vadd(b.v, vmult(vperp(r2), b.w)), vadd(a.v, vmult(vperp(r1), a.w))), v2(arb.svx, arb.svy));
+	const double vbn = vdot(vsub(vb2, vb1), n);
+	const double vrn = vdot(vr, n);
+	const double vrt = vdot(vr, vperp(n));
+	const double jbn = (con.bias - vbn) * con.nMass;
+	const double z = jbOld + jbn;
+	const double jbNew = fmaxc(z, 0.0);
+	const double jn = -(con.bounce + vrn) * con.nMass;
+	const double y = jnOld + jn;
+	const double jnNew = fmaxc(y, 0.0);
+	const double M = arb.u * jnNew;
+	const double jt = -vrt * con.tMass;
+	const double x = jtOld + jt;
+	const double t1 = (x > -M) ? x : -M;
+	const double jtNew = (t1 < M) ? t1 : M;
+	const double db = jbNew - jbOld;
+	const V2 jbv = vmult(n, db);
+	const V2 d = v2(jnNew - jnOld, jtNew - jtOld);
+	const V2 J = vrotate(n, d);
+	V2 gn = v2(0.0, 0.0);
+	// apply_impulses
+	const V2 gJ = vadd(vsub(vmult(gb.v, mB), vmult(ga.v, mA)),
+	                   vsub(vmult(vperp(r2), iB * gb.w), vmult(vperp(r1), iA * ga.w)));
+	gc.r1 = vsub(gc.r1, vmult(vrperp(J), iA * ga.w));
+	gc.r2 = vadd(gc.r2, vmult(vrperp(J), iB * gb.w));
+	gm[0] -= vdot(ga.v, J); gm[1] -= ga.w * vcross(r1, J);
+	gm[2] += vdot(gb.v, J); gm[3] += gb.w * vcross(r2, J);
+	const V2 gd = v2(gJ.x * n.x + gJ.y * n.y, -gJ.x * n.y + gJ.y * n.x);
+	gn = vadd(gn, v2(gJ.x * d.x + gJ.y * d.y, -gJ.x * d.y + gJ.y * d.x));
+	// bias impulses
+	const V2 gjbv = vadd(vsub(vmult(gb.vb, mB), vmult(ga.vb, mA)),
+	                     vsub(vmult(vperp(r2), iB * gb.wb), vmult(vperp(r1), iA * ga.wb)));
+	gc.r1 = vsub(gc.r1, vmult(vrperp(jbv), iA * ga.wb));
+	gc.r2 = vadd(gc.r2, vmult(vrperp(jbv), iB * gb.wb));
+	gm[0] -= vdot(ga.vb, jbv); gm[1] -= ga.wb * vcross(r1, jbv);
+	gm[2] += vdot(gb.vb, jbv); gm[3] += gb.wb * vcross(r2, jbv);
+	const double gdb = vdot(gjbv, n);
+	gn = vadd(gn, vmult(gjbv, db));
+	// adjoints of the new accumulators: later uses plus this contact's own
+	const double g_jtNew = G.jt[i] + gd.y;
+	double gx = 0.0, gM = 0.0;
+	if (t1 < M) { if (x > -M) gx = g_jtNew; else gM -= g_jtNew; }
+	else gM += g_jtNew;
+	const double gjt = gx;
+	const double gvrt = -gjt * con.tMass;
+	gc.tMass += -gjt * vrt;
+	G.u += gM * jnNew;
+	const double g_jnNew = G.jn[i] + gd.x + gM * arb.u;
+	const double gy = (y > 0.0) ? g_jnNew : 0.0;
+	const double gjn = gy;
+	gc.bounce += -gjn * con.nMass;
+	const double gvrn = -gjn * con.nMass;
+	gc.nMass += -gjn * (con.bounce + vrn);
+	const double g_jbNew = G.jb[i] + gdb;
+	const double gz = (z > 0.0) ? g_jbNew : 0.0;
+	const double gjbn = gz;
+	gc.bias += gjbn * con.nMass;
+	const double gvbn = -gjbn * con.nMass;
+	gc.nMass += gjbn * (con.bias - vbn);
+	// accumulators before the contact
+	G.jn[i] = gy - gd.x;
+	G.jt[i] = gx - gd.y;
+	G.jb[i] = gz - gdb;
+	// relative velocities
+	const V2 gvr = vadd(vmult(n, gvrn), vmult(vperp(n), gvrt));
+	gn = vadd(gn, vadd(vmult(vr, gvrn), vmult(vrperp(vr), gvrt)));
+	const V2 gvb2 = vmult(n, gvbn), gvb1 = vmult(n, -gvbn);
+	gn = vadd(gn, vmult(vsub(vb2, vb1), gvbn));
+	gb.v = vadd(gb.v, gvr); gb.w += vdot(gvr, vperp(r2)); gc.r2 = vadd(gc.r2, vmult(vrperp(gvr), b.w));
+	ga.v = vsub(ga.v, gvr); ga.w -= vdot(gvr, vperp(r1)); gc.r1 = vsub(gc.r1, vmult(vrperp(gvr), a.w));
+	G.svr = vadd(G.svr, gvr);
+	ga.vb = vadd(ga.vb, gvb1); ga.wb += vdot(gvb1, vperp(r1)); gc.r1 = vadd(gc.r1, vmult(vrperp(gvb1), a.wb));
+	gb.vb = vadd(gb.vb, gvb2); gb.wb += vdot(gvb2, vperp(r2)); gc.r2 = vadd(gc.r2, vmult(vrperp(gvb2), b.wb));
+	G.n = vadd(G.n, gn);
+}
+
+// ---- contact geometry on dual numbers ----
+
+struct DualShape {
+	ShapeViewT<Dual> v;
+	V2T<Dual> wv[DUAL_MAX_VERTS], wn[DUAL_MAX_VERTS];
+};
+
+// body position and angle after the step's position update (cpBody.c:511-522, the
+// operations of k_update_positions), from the step's start state
+__device__ __forceinline__ void updated_pose(const BodyState &s, double dt, V2 &p, double &a)
+{
+	p = vadd(v2(s.p.x, s.p.y), vmult(vadd(v2(s.vel.vx, s.vel.vy), v2(s.vel.vbx, s.vel.vby)), dt));
+	a = s.a + (s.vel.w + s.vel.wb) * dt;
+}
+
+// the body transform on duals; seed k0 >= 0 makes (p.x, p.y, a) the inputs k0..k0+2
+__device__ XFT<Dual> dual_transform(const DevWorld &W, const BodyState *start, int b, double dt, int k0)
+{
+	XFT<Dual> T;
+	if (W.b_type[b] == CPGPU_BODY_STATIC) {
+		const XF t = W.b_T[b];
+		T.a = Dual(t.a); T.b = Dual(t.b); T.c = Dual(t.c); T.d = Dual(t.d); T.tx = Dual(t.tx); T.ty = Dual(t.ty);
+		return T;
+	}
+	V2 p;
+	double a;
+	updated_pose(start[b], dt, p, a);
+	double sn, cs;
+	cp_sincos(a, sn, cs);
+	Dual px(p.x), py(p.y), dsn(sn), dcs(cs);
+	if (k0 >= 0) {
+		px.d[k0] = 1.0;
+		py.d[k0 + 1] = 1.0;
+		dsn.d[k0 + 2] = cs;
+		dcs.d[k0 + 2] = -sn;
+	}
+	const double2 c = W.b_cog[b];
+	T.a = dcs; T.b = dsn; T.c = -dsn; T.d = dcs;
+	T.tx = px - (c.x * dcs - c.y * dsn);
+	T.ty = py - (c.x * dsn + c.y * dcs);
+	return T;
+}
+
+// cacheData on duals (cpShape.c:291-296, 378-405; cpPolyShape.c:39-64), with the
+// box of k_cache_shapes from the value parts (it seeds GJK when the id is 0)
+__device__ bool dual_shape(const DevWorld &W, int s, const XFT<Dual> &T, DualShape &out)
+{
+	ShapeViewT<Dual> &v = out.v;
+	v.type = W.s_type[s];
+	v.r = W.s_r[s];
+	v.hashid = W.s_hashid[s];
+	v.rot = v2t<Dual>(T.a, T.b);
+	v.count = 0;
+	v.wv = out.wv; v.wn = out.wn;
+	const double r = v.r;
+	const double *g = W.s_geom + 10 * (size_t)s;
+	if (v.type == CPGPU_SHAPE_CIRCLE) {
+		v.tc = xpoint(T, v2(g[0], g[1]));
+		const double cx = v.tc.x.v, cy = v.tc.y.v;
+		v.bb[0] = cx - r; v.bb[1] = cy - r; v.bb[2] = cx + r; v.bb[3] = cy + r;
+	} else if (v.type == CPGPU_SHAPE_SEGMENT) {
+		v.ta = xpoint(T, v2(g[0], g[1]));
+		v.tb = xpoint(T, v2(g[2], g[3]));
+		v.tn = xvect(T, v2(g[4], g[5]));
+		v.a_tan = v2(g[6], g[7]); v.b_tan = v2(g[8], g[9]);
+		const double ax = v.ta.x.v, ay = v.ta.y.v, bx = v.tb.x.v, by = v.tb.y.v;
+		double l, rr, bt, tp;
+		if (ax < bx) { l = ax; rr = bx; } else { l = bx; rr = ax; }
+		if (ay < by) { bt = ay; tp = by; } else { bt = by; tp = ay; }
+		v.bb[0] = l - r; v.bb[1] = bt - r; v.bb[2] = rr + r; v.bb[3] = tp + r;
+	} else {
+		const int st = W.s_pstart[s], cnt = W.s_pcount[s];
+		if (cnt > DUAL_MAX_VERTS) return false;
+		v.count = cnt;
+		double l = __builtin_inf(), rr = -__builtin_inf(), bt = __builtin_inf(), tp = -__builtin_inf();
+		for (int i = 0; i < cnt; i++) {
+			const double2 lv = W.pv_local[st + i], ln = W.pn_local[st + i];
+			out.wv[i] = xpoint(T, v2(lv.x, lv.y));
+			out.wn[i] = xvect(T, v2(ln.x, ln.y));
+			const double x = out.wv[i].x.v, y = out.wv[i].y.v;
+			l = fminc(l, x); rr = fmaxc(rr, x); bt = fminc(bt, y); tp = fmaxc(tp, y);
+		}
+		v.bb[0] = l - r; v.bb[1] = bt - r; v.bb[2] = rr + r; v.bb[3] = tp + r;
+	}
+	return true;
+}
+
+// ---- adjoint state ----
+
+__device__ __forceinline__ void add2(double *p, V2 v) { p[0] += v.x; p[1] += v.y; }
+
+// The reverse of one step for one small space: one workgroup.
+__global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, const AdjStep A)
+{
+	__shared__ int32_t col_start[CPGPU_MAX_COLOURS + 1];
+	__shared__ int32_t n_col;
+	const int w = blockIdx.x;
+	const int tid = threadIdx.x, stride = blockDim.x;
+	ArbAdj *garb = reinterpret_cast<ArbAdj *>(A.garb);
+	const int lo = A.warb_off ? A.warb_off[w] : 0;
+	const int hi = A.warb_off ? A.warb_off[w + 1] : A.n;
+	const WorldStep ws = W.ws[w];
+	const int b0 = W.w_body0[w], b1 = W.w_body0[w + 1];
+	// colour boundaries of the space's colour order
+	if (tid == 0) n_col = hi > lo ? A.colour[A.order[hi - 1]] + 1 : 0;
+	for (int pos = lo + tid; pos < hi; pos += stride) {
+		const int c = A.colour[A.order[pos]];
+		if (pos == lo || A.colour[A.order[pos - 1]] != c) col_start[c] = pos;
+	}
+	__syncthreads();
+	if (tid == 0) col_start[n_col] = hi;
+	// per-arbiter adjoints: the final accumulators' adjoints come from later warm starts
+	for (int pos = lo + tid; pos < hi; pos += stride) {
+		const Arbiter &arb = A.act[A.order[pos]];
+		ArbAdj G;
+		for (int i = 0; i < 2; i++) {
+			G.c[i].r1 = v2(0.0, 0.0); G.c[i].r2 = v2(0.0, 0.0);
+			G.c[i].nMass = G.c[i].tMass = G.c[i].bias = G.c[i].bounce = 0.0;
+			const bool on = i < arb.count && arb.c[i].tape >= 0;
+			G.jn[i] = on ? A.gJ[arb.c[i].tape].x : 0.0;
+			G.jt[i] = on ? A.gJ[arb.c[i].tape].y : 0.0;
+			G.jb[i] = 0.0;
+		}
+		G.n = v2(0.0, 0.0); G.svr = v2(0.0, 0.0); G.u = 0.0;
+		garb[pos] = G;
+	}
+	__syncthreads();
+	const int ncol = n_col;
+
+	// ---- impulse sweeps, last first ----
+	for (int it = ws.iterations - 1; it >= 0; it--) {
+		for (int c = ncol - 1; c >= 0; c--) {
+			for (int pos = col_start[c] + tid; pos < col_start[c + 1]; pos += stride) {
+				const Arbiter &arb = A.act[A.order[pos]];
+				const ArbRec &rec = A.rec[(size_t)it * A.n + pos];
+				const int ba = arb.ba, bb = arb.bb;
+				const double mA = W.b_vel[ba].m_inv, iA = W.b_vel[ba].i_inv;
+				const double mB = W.b_vel[bb].m_inv, iB = W.b_vel[bb].i_inv;
+				BodyAdj &GA = A.gb[ba];
+				BodyAdj &GB = A.gb[bb];
+				BodyV ga, gb;
+				ga.v = v2(GA.v[0], GA.v[1]); ga.w = GA.w; ga.vb = v2(GA.vb[0], GA.vb[1]); ga.wb = GA.wb;
+				gb.v = v2(GB.v[0], GB.v[1]); gb.w = GB.w; gb.vb = v2(GB.vb[0], GB.vb[1]); gb.wb = GB.wb;
+				ArbAdj G = garb[pos];
+				double gm[4] = {0.0, 0.0, 0.0, 0.0};
+				// pre-states: before contact 0 (recorded) and before contact 1 (recomputed)
+				const BodyV a0 = bodyv_from(rec.a), b0v = bodyv_from(rec.b);
+				if (arb.count > 1) {
+					BodyV a1 = a0, b1v = b0v;
+					double jn = rec.acc[0][0], jt = rec.acc[0][1], jb = rec.acc[0][2];
+					contact_forward(arb, 0, mA, iA, mB, iB, a1, b1v, jn, jt, jb);
+					contact_reverse(arb, 1, mA, iA, mB, iB, a1, b1v, rec.acc[1][0], rec.acc[1][1], rec.acc[1][2], ga,
+					                gb, G, gm);
+				}
+				if (arb.count > 0)
+					contact_reverse(arb, 0, mA, iA, mB, iB, a0, b0v, rec.acc[0][0], rec.acc[0][1], rec.acc[0][2], ga,
+					                gb, G, gm);
+				garb[pos] = G;
+				if (dyn(W, ba)) {
+					GA.v[0] = ga.v.x; GA.v[1] = ga.v.y; GA.w = ga.w; GA.vb[0] = ga.vb.x; GA.vb[1] = ga.vb.y; GA.wb = ga.wb;
+					A.gminv[ba] += gm[0]; A.giinv[ba] += gm[1];
+				}
+				if (dyn(W, bb)) {
+					GB.v[0] = gb.v.x; GB.v[1] = gb.v.y; GB.w = gb.w; GB.vb[0] = gb.vb.x; GB.vb[1] = gb.vb.y; GB.wb = gb.wb;
+					A.gminv[bb] += gm[2]; A.giinv[bb] += gm[3];
+				}
+			}
+			__syncthreads();
+		}
+	}
+
+	// ---- cached impulses (cpArbiterApplyCachedImpulse), skipped on a first collision ----
+	for (int c = ncol - 1; c >= 0; c--) {
+		for (int pos = col_start[c] + tid; pos < col_start[c + 1]; pos += stride) {
+			const Arbiter &arb = A.act[A.order[pos]];
+			ArbAdj G = garb[pos];
+			if (arb.state != CPGPU_ARB_FIRST_COLLISION) {
+				const int ba = arb.ba, bb = arb.bb;
+				const double mA = W.b_vel[ba].m_inv, iA = W.b_vel[ba].i_inv;
+				const double mB = W.b_vel[bb].m_inv, iB = W.b_vel[bb].i_inv;
+				BodyAdj &GA = A.gb[ba];
+				BodyAdj &GB = A.gb[bb];
+				const V2 gva = v2(GA.v[0], GA.v[1]), gvb = v2(GB.v[0], GB.v[1]);
+				const double gwa = GA.w, gwb = GB.w;
+				const V2 n = v2(arb.nx, arb.ny);
+				double gm[4] = {0.0, 0.0, 0.0, 0.0};
+				for (int i = 0; i < arb.count; i++) {
+					const Contact &con = arb.c[i];
+					const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
+					const int src = con.src;
+					const double jn0 = src >= 0 ? A.tape[src >> 1].c[src & 1].jnAcc : 0.0;
+					const double jt0 = src >= 0 ? A.tape[src >> 1].c[src & 1].jtAcc : 0.0;
+					const V2 q = v2(jn0, jt0);
+					const V2 J = vmult(vrotate(n, q), ws.dt_coef);
+					const V2 gJ = vadd(vsub(vmult(gvb, mB), vmult(gva, mA)),
+					                   vsub(vmult(vperp(r2), iB * gwb), vmult(vperp(r1), iA * gwa)));
+					G.c[i].r1 = vsub(G.c[i].r1, vmult(vrperp(J), iA * gwa));
+					G.c[i].r2 = vadd(G.c[i].r2, vmult(vrperp(J), iB * gwb));
+					gm[0] -= vdot(gva, J); gm[1] -= gwa * vcross(r1, J);
+					gm[2] += vdot(gvb, J); gm[3] += gwb * vcross(r2, J);
+					const V2 gR = vmult(gJ, ws.dt_coef);
+					G.jn[i] += gR.x * n.x + gR.y * n.y;
+					G.jt[i] += -gR.x * n.y + gR.y * n.x;
+					G.n = vadd(G.n, v2(gR.x * q.x + gR.y * q.y, -gR.x * q.y + gR.y * q.x));
+				}
+				if (dyn(W, ba)) { A.gminv[ba] += gm[0]; A.giinv[ba] += gm[1]; }
+				if (dyn(W, bb)) { A.gminv[bb] += gm[2]; A.giinv[bb] += gm[3]; }
+			}
+			garb[pos] = G;
+		}
+		__syncthreads();
+	}
+	// In a fixed order (one lane; runs are reproducible bit for bit): the warm
+	// starts send the adjoints of the initial accumulators to the contacts they
+	// came from, and the friction adjoint goes to the two shapes (u = uA uB).
+	if (tid == 0) {
+		for (int pos = lo; pos < hi; pos++) {
+			const Arbiter &arb = A.act[A.order[pos]];
+			const ArbAdj &G = garb[pos];
+			for (int i = 0; i < arb.count; i++) {
+				const int src = arb.c[i].src;
+				if (src >= 0) {
+					A.gJ[src].x += G.jn[i];
+					A.gJ[src].y += G.jt[i];
+				}
+			}
+			A.gu[arb.sa] += G.u * W.s_u[arb.sb];
+			A.gu[arb.sb] += G.u * W.s_u[arb.sa];
+		}
+	}
+	__syncthreads();
+
+	// ---- velocity integration (cpBodyUpdateVelocity): v = v damping + (g + f m_inv) dt ----
+	for (int b = b0 + tid; b < b1; b += stride) {
+		if (!dyn(W, b)) continue;
+		BodyAdj &GB = A.gb[b];
+		const BodyState &s = A.start[b];
+		A.gminv[b] += (GB.v[0] * s.f.x + GB.v[1] * s.f.y) * ws.dt;
+		A.giinv[b] += GB.w * s.t * ws.dt;
+		GB.v[0] *= ws.damping; GB.v[1] *= ws.damping;
+		GB.w *= ws.damping;
+	}
+	__syncthreads();
+
+	// ---- pre-step and contact geometry, colour by colour (bodies of one colour are distinct) ----
+	for (int c = ncol - 1; c >= 0; c--) {
+		for (int pos = col_start[c] + tid; pos < col_start[c + 1]; pos += stride) {
+			const Arbiter &arb = A.act[A.order[pos]];
+			ArbAdj G = garb[pos];
+			const int ba = arb.ba, bb = arb.bb;
+			const BodyState &sa = A.start[ba], &sb = A.start[bb];
+			const double iA = sa.vel.i_inv, iB = sb.vel.i_inv;
+			const V2 n = v2(arb.nx, arb.ny);
+			V2 pA, pB;
+			double angA, angB;
+			if (W.b_type[ba] == CPGPU_BODY_STATIC) pA = v2(sa.p.x, sa.p.y); else updated_pose(sa, ws.dt, pA, angA);
+			if (W.b_type[bb] == CPGPU_BODY_STATIC) pB = v2(sb.p.x, sb.p.y); else updated_pose(sb, ws.dt, pB, angB);
+			const V2 body_delta = vsub(pB, pA);
+			// velocities the pre-step saw (before the velocity integration)
+			const V2 vA = v2(sa.vel.vx, sa.vel.vy), vB = v2(sb.vel.vx, sb.vel.vy);
+			const double wA = sa.vel.w, wB = sb.vel.w;
+			double gm[4] = {0.0, 0.0, 0.0, 0.0};
+			V2 gpA = v2(0.0, 0.0), gpB = v2(0.0, 0.0);
+			V2 gvA = v2(0.0, 0.0), gvB = v2(0.0, 0.0);
+			double gwA = 0.0, gwB = 0.0;
+			for (int i = 0; i < arb.count; i++) {
+				const Contact &con = arb.c[i];
+				ContactAdj &gc = G.c[i];
+				const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
+				// nMass = 1 / k(n), tMass = 1 / k(perp n)  (k_scalar, chipmunk_private.h:212-226)
+				const double kn_bar = -gc.nMass * con.nMass * con.nMass;
+				const double rcn1 = vcross(r1, n), rcn2 = vcross(r2, n);
+				gm[0] += kn_bar; gm[1] += kn_bar * rcn1 * rcn1;
+				gm[2] += kn_bar; gm[3] += kn_bar * rcn2 * rcn2;
+				gc.r1 = vadd(gc.r1, vmult(vrperp(n), kn_bar * 2.0 * iA * rcn1));
+				gc.r2 = vadd(gc.r2, vmult(vrperp(n), kn_bar * 2.0 * iB * rcn2));
+				G.n = vadd(G.n, vadd(vmult(vperp(r1), kn_bar * 2.0 * iA * rcn1), vmult(vperp(r2), kn_bar * 2.0 * iB * rcn2)));
+				const V2 t = vperp(n);
+				const double kt_bar = -gc.tMass * con.tMass * con.tMass;
+				const double rct1 = vcross(r1, t), rct2 = vcross(r2, t);
+				gm[0] += kt_bar; gm[1] += kt_bar * rct1 * rct1;
+				gm[2] += kt_bar; gm[3] += kt_bar * rct2 * rct2;
+				gc.r1 = vadd(gc.r1, vmult(vrperp(t), kt_bar * 2.0 * iA * rct1));
+				gc.r2 = vadd(gc.r2, vmult(vrperp(t), kt_bar * 2.0 * iB * rct2));
+				const V2 gt = vadd(vmult(vperp(r1), kt_bar * 2.0 * iA * rct1), vmult(vperp(r2), kt_bar * 2.0 * iB * rct2));
+				G.n = vadd(G.n, v2(gt.y, -gt.x));
+				// bias = -bias_coef min(0, dist + slop) / dt, dist = (r2 - r1 + pB - pA) . n
+				const V2 dvec = vadd(vsub(r2, r1), body_delta);
+				const double dist = vdot(dvec, n);
+				const double xb = dist + ws.slop;
+				if (!(0.0 < xb)) {
+					const double gdist = gc.bias * (-ws.bias_coef / ws.dt);
+					gc.r2 = vadd(gc.r2, vmult(n, gdist));
+					gc.r1 = vsub(gc.r1, vmult(n, gdist));
+					gpB = vadd(gpB, vmult(n, gdist));
+					gpA = vsub(gpA, vmult(n, gdist));
+					G.n = vadd(G.n, vmult(dvec, gdist));
+				}
+				// bounce = (relative velocity . n) e
+				const double gvrn = gc.bounce * arb.e;
+				if (gvrn != 0.0) {
+					const V2 rv = vsub(vadd(vB, vmult(vperp(r2), wB)), vadd(vA, vmult(vperp(r1), wA)));
+					gvB = vadd(gvB, vmult(n, gvrn)); gwB += gvrn * vdot(vperp(r2), n);
+					gc.r2 = vadd(gc.r2, vmult(vrperp(n), wB * gvrn));
+					gvA = vsub(gvA, vmult(n, gvrn)); gwA -= gvrn * vdot(vperp(r1), n);
+					gc.r1 = vsub(gc.r1, vmult(vrperp(n), wA * gvrn));
+					G.n = vadd(G.n, vmult(rv, gvrn));
+				}
+			}
+			// surface velocity: svr = s - n (s . n) with s the shapes' surface velocity difference
+			{
+				const double2 svA = W.s_sv[arb.sa], svB = W.s_sv[arb.sb];
+				const V2 sv = v2(svB.x - svA.x, svB.y - svA.y);
+				G.n = vsub(G.n, vadd(vmult(G.svr, vdot(sv, n)), vmult(sv, vdot(n, G.svr))));
+			}
+			// contact geometry: r1 = p1 - pA, r2 = p2 - pB with (n, p1, p2) from the collision on duals
+			{
+				DualShape dsa, dsb;
+				const XFT<Dual> TA = dual_transform(W, A.start, ba, ws.dt, dyn(W, ba) ? 0 : -1);
+				const XFT<Dual> TB = dual_transform(W, A.start, bb, ws.dt, dyn(W, bb) ? 3 : -1);
+				const bool ok = dual_shape(W, arb.sa, TA, dsa) && dual_shape(W, arb.sb, TB, dsb);
+				CollInfoT<Dual> info;
+				if (ok) collide(dsa.v, dsb.v, arb.id_in, info, W.err);
+				bool same = ok && info.count == arb.count && info.n.x.v == arb.nx && info.n.y.v == arb.ny;
+				for (int i = 0; same && i < arb.count; i++) {
+					same = info.p1[i].x.v - pA.x == arb.c[i].r1x && info.p1[i].y.v - pA.y == arb.c[i].r1y &&
+					       info.p2[i].x.v - pB.x == arb.c[i].r2x && info.p2[i].y.v - pB.y == arb.c[i].r2y;
+				}
+				if (!same) {
+					raise_err(W.err, CPGPU_ERR_ADJOINT);
+				} else {
+					double gin[DUAL_N];
+					for (int k = 0; k < DUAL_N; k++) gin[k] = G.n.x * info.n.x.d[k] + G.n.y * info.n.y.d[k];
+					for (int i = 0; i < arb.count; i++) {
+						const ContactAdj &gc = G.c[i];
+						for (int k = 0; k < DUAL_N; k++)
+							gin[k] += gc.r1.x * info.p1[i].x.d[k] + gc.r1.y * info.p1[i].y.d[k] +
+							          gc.r2.x * info.p2[i].x.d[k] + gc.r2.y * info.p2[i].y.d[k];
+						gpA = vsub(gpA, gc.r1);
+						gpB = vsub(gpB, gc.r2);
+					}
+					if (dyn(W, ba)) {
+						BodyAdj &GA = A.gb[ba];
+						GA.p[0] += gpA.x + gin[0]; GA.p[1] += gpA.y + gin[1]; GA.a += gin[2];
+						GA.v[0] += gvA.x; GA.v[1] += gvA.y; GA.w += gwA;
+						A.gminv[ba] += gm[0]; A.giinv[ba] += gm[1];
+					}
+					if (dyn(W, bb)) {
+						BodyAdj &GB = A.gb[bb];
+						GB.p[0] += gpB.x + gin[3]; GB.p[1] += gpB.y + gin[4]; GB.a += gin[5];
+						GB.v[0] += gvB.x; GB.v[1] += gvB.y; GB.w += gwB;
+						A.gminv[bb] += gm[2]; A.giinv[bb] += gm[3];
+					}
+				}
+			}
+		}
+		__syncthreads();
+	}
+
+	// ---- position integration (cpBodyUpdatePosition): p' = p + (v + v_bias) dt, a' = a + (w + w_bias) dt ----
+	// The bias velocities the solve started from were zero (reset by the update), so
+	// the adjoint of the step's incoming bias velocities is the update's alone.
+	for (int b = b0 + tid; b < b1; b += stride) {
+		if (!dyn(W, b)) continue;
+		BodyAdj &GB = A.gb[b];
+		GB.v[0] += GB.p[0] * ws.dt; GB.v[1] += GB.p[1] * ws.dt;
+		GB.vb[0] = GB.p[0] * ws.dt; GB.vb[1] = GB.p[1] * ws.dt;
+		GB.w += GB.a * ws.dt;
+		GB.wb = GB.a * ws.dt;
+	}
+}
+
+inline int blocks_for(int n, int t) { return (n + t - 1) / t; }
+
+} // namespace
+
+hipError_t adjoint_tape_record(Arbiter *arbs, const int32_t *n_dev, int n_host, Arbiter *tape, int off,
+                               hipStream_t stream)
+{
+	if (n_host <= 0) return hipSuccess;
+	k_tape_record<<<std::min(blocks_for(n_host, 256), 1024), 256, 0, stream>>>(arbs, n_dev, n_host, tape, off);
+	return hipGetLastError();
+}
+
+hipError_t adjoint_replay_prepare(const Arbiter *tape, int off, int n, Arbiter *act, hipStream_t stream)
+{
+	if (n <= 0) return hipSuccess;
+	k_replay_prepare<<<blocks_for(n, 256), 256, 0, stream>>>(tape, off, n, act);
+	return hipGetLastError();
+}
+
+hipError_t adjoint_state_save(const DevWorld &W, BodyState *out, hipStream_t stream)
+{
+	if (W.n_bodies <= 0) return hipSuccess;
+	k_state_save<<<blocks_for(W.n_bodies, 256), 256, 0, stream>>>(W, out);
+	return hipGetLastError();
+}
+
+hipError_t adjoint_state_load(const DevWorld &W, const BodyState *in, hipStream_t stream)
+{
+	if (W.n_bodies <= 0) return hipSuccess;
+	k_state_load<<<blocks_for(W.n_bodies, 256), 256, 0, stream>>>(W, in);
+	return hipGetLastError();
+}
+
+size_t adjoint_arb_scratch_bytes() { return sizeof(ArbAdj); }
+
+hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream)
+{
+	k_reverse_step<<<n_worlds, ADJ_THREADS, 0, stream>>>(W, A);
+	return hipGetLastError();
+}

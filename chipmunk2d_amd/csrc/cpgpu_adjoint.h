@@ -1,0 +1,45 @@
+// cpgpu_adjoint.h -- rollout tape and reverse-step launchers (adjoint.hip).
+#pragma once
+
+#include "cpgpu_step.h"
+
+// One body's state at the start of a step (checkpoints, segment replay).
+struct BodyState {
+	double2 p;
+	double a;
+	BodyVel vel;
+	double2 f;
+	double t;
+	XF T;
+};
+
+// Adjoint of one body's step state.
+struct BodyAdj {
+	double p[2], a, v[2], w, vb[2], wb;
+};
+
+// Everything the reverse of one replayed step reads and writes.
+struct AdjStep {
+	int n;                      // solved arbiters of the step
+	const Arbiter *act;         // after the replayed solve
+	const int32_t *order;       // colour order (per space: its own segment)
+	const int32_t *colour;      // colour of each arbiter
+	const int32_t *warb_off;    // first position of each space, nullptr for a single space
+	const ArbRec *rec;          // [iteration][position] pre-states
+	const BodyState *start;     // bodies at the start of the step
+	const Arbiter *tape;        // every recorded arbiter of the rollout
+	void *garb;                 // [n] per-arbiter adjoint scratch (adjoint_arb_scratch_bytes each)
+	BodyAdj *gb;                // [n_bodies] adjoint state, updated in place
+	double *gminv, *giinv;      // [n_bodies] adjoints of 1/m and 1/I
+	double *gu;                 // [n_shapes] adjoint of the friction coefficient
+	double2 *gJ;                // [recorded contacts] adjoints of the accumulated impulses (jn, jt)
+};
+
+hipError_t adjoint_tape_record(Arbiter *arbs, const int32_t *n_dev, int n_host, Arbiter *tape, int off,
+                               hipStream_t stream);
+hipError_t adjoint_replay_prepare(const Arbiter *tape, int off, int n, Arbiter *act, hipStream_t stream);
+hipError_t adjoint_state_save(const DevWorld &W, BodyState *out, hipStream_t stream);
+hipError_t adjoint_state_load(const DevWorld &W, const BodyState *in, hipStream_t stream);
+hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream);
+// bytes of the per-arbiter adjoint scratch (AdjStep::garb_raw) per solved arbiter
+size_t adjoint_arb_scratch_bytes();

@@ -170,6 +170,10 @@ struct Contact {
 	double r1x, r1y, r2x, r2y;
 	double nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
 	uint64_t hash;
+	// rollout tape (adjoint): where this contact's accumulated impulses were
+	// recorded after its last solve, and the recorded contact its warm start
+	// came from (-1: none)
+	int32_t tape, src;
 };
 
 // An arbiter (struct cpArbiter, chipmunk_structs.h:124-145), device form.
@@ -179,6 +183,8 @@ struct Arbiter {
 	int32_t count, state;
 	uint32_t stamp;
 	int32_t world;
+	uint32_t id_in;            // GJK warm-start id the collision started from
+	int32_t _pad;
 	double nx, ny, e, u, svx, svy;
 	Contact c[2];
 };
@@ -256,5 +262,6 @@ struct DevWorld {
 #define CPGPU_ERR_TABLE_FULL 32
 #define CPGPU_ERR_EPA_HULL 64
 #define CPGPU_ERR_ISLANDS 128
+#define CPGPU_ERR_ADJOINT 256
 
 CPD void raise_err(int32_t *err, int32_t code) { atomicOr(err, code); }

@@ -51,6 +51,14 @@ struct GridBarrier {
 #define ISLAND_KEY (1ull << 40)
 #define GRID_KEY (2ull << 40)
 
+// Pre-state of one arbiter's impulse application in one sweep (rollout
+// tape): both bodies' velocities and bias velocities, and each contact's
+// accumulated impulses before the application.
+struct ArbRec {
+	double a[6], b[6];         // v.x, v.y, w, v_bias.x, v_bias.y, w_bias
+	double acc[2][3];          // jnAcc, jtAcc, jBias
+};
+
 struct SolveBuffers {
 	Arbiter *act;
 	const int32_t *n_act;
@@ -96,6 +104,9 @@ struct SolveBuffers {
 	int n_bodies;
 	const int32_t *large_bodies;   // non-static bodies of large worlds
 	int n_large_bodies;
+	// recording (adjoint replay): rec[it * rec_stride + position in order], or nullptr
+	ArbRec *rec;
+	int rec_stride;
 };
 
 // Group the solved arbiters: islands of the large worlds (union-find), the

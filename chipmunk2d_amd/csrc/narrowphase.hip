@@ -146,6 +146,8 @@ __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const Ste
 			a.ba = W.s_body[ia]; a.bb = W.s_body[ib];
 			a.world = world;
 			a.stamp = ws.stamp;
+			a.id_in = id;
+			a._pad = 0;
 			// state: an arbiter solved last step was reset to NORMAL at the start of
 			// this step (cpSpaceStep.c:351-360); a cached one becomes FIRST_COLLISION
 			if (po == nullptr) a.state = CPGPU_ARB_FIRST_COLLISION;
@@ -159,13 +161,19 @@ __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const Ste
 					con.r2x = info.p2[c].x - pB.x; con.r2y = info.p2[c].y - pB.y;
 					con.hash = info.hash[c];
 					con.jnAcc = 0.0; con.jtAcc = 0.0;
+					con.src = -1;
 					for (int j = 0; j < old_count; j++) {
-						if (con.hash == po->c[j].hash) { con.jnAcc = po->c[j].jnAcc; con.jtAcc = po->c[j].jtAcc; }
+						if (con.hash == po->c[j].hash) {
+							con.jnAcc = po->c[j].jnAcc; con.jtAcc = po->c[j].jtAcc;
+							con.src = po->c[j].tape;
+						}
 					}
 				} else {
 					con.r1x = con.r1y = con.r2x = con.r2y = 0.0;
 					con.hash = 0; con.jnAcc = 0.0; con.jtAcc = 0.0;
+					con.src = -1;
 				}
+				con.tape = -1;
 				con.nMass = con.tMass = con.bounce = con.jBias = con.bias = 0.0;
 				a.c[c] = con;
 			}

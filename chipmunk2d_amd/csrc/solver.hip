@@ -434,6 +434,7 @@ struct SolveSet {
 	BodyVel *lds_bodies;       // LDS room for lds_cap bodies, or nullptr
 	int lds_cap;
 	int unit;                  // solve unit whose contacts / bodies are counted (W.totals[2 + 2 unit ...])
+	int order_base;            // position of order[0] in S.order (records are indexed by it)
 };
 
 // a wave-reduced 64-bit add to a counter
@@ -506,6 +507,18 @@ __device__ void for_colours(Sync &sy, const int32_t *col_off, int n_colours, int
 	}
 }
 
+template <class BA> __device__ __forceinline__ void record_arbiter(ArbRec *r, const BA &bodies, const Arbiter &arb)
+{
+	const BV a = bodies.load(arb.ba), b = bodies.load(arb.bb);
+	r->a[0] = a.v.x; r->a[1] = a.v.y; r->a[2] = a.w; r->a[3] = a.vb.x; r->a[4] = a.vb.y; r->a[5] = a.wb;
+	r->b[0] = b.v.x; r->b[1] = b.v.y; r->b[2] = b.w; r->b[3] = b.vb.x; r->b[4] = b.vb.y; r->b[5] = b.wb;
+	for (int i = 0; i < 2; i++) {
+		r->acc[i][0] = arb.c[i].jnAcc;
+		r->acc[i][1] = arb.c[i].jtAcc;
+		r->acc[i][2] = arb.c[i].jBias;
+	}
+}
+
 // cached impulses (cpSpaceStep.c:407-415), then the impulse iterations
 // (cpSpaceStep.c:418-427): arbiters colour by colour, then joints colour by colour
 template <class Sync, class BA>
@@ -523,7 +536,10 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 	for (int it = 0; it < T.iterations; it++) {
 		for_colours(sy, T.col_off, n_colours, tail, [&](int k) {
 			Arbiter &arb = S.act[T.order[k]];
-			if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, bodies, arb);
+			if (it < W.ws[arb.world].iterations) {
+				if (S.rec) record_arbiter(S.rec + (size_t)it * S.rec_stride + T.order_base + k, bodies, arb);
+				arb_apply_impulse(W, bodies, arb);
+			}
 		});
 		for (int jc = 0; jc < T.njcol; jc++) {
 			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
@@ -697,12 +713,13 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 		if (S.arb_by_world) {
 			T.list = S.arb_by_world + S.warb_off[w];
 			T.n = S.warb_off[w + 1] - S.warb_off[w];
-			T.order = S.order + S.warb_off[w];
+			T.order_base = S.warb_off[w];
 		} else {   // a single space: every solved arbiter is its own
 			T.list = nullptr;
 			T.n = *S.n_act;
-			T.order = S.order;
+			T.order_base = 0;
 		}
+		T.order = S.order + T.order_base;
 		T.body0 = W.w_body0[w];
 		T.nb = W.w_body0[w + 1] - W.w_body0[w];
 		T.jorder = S.jw_order;
@@ -733,6 +750,7 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 		T.list = S.arb_by_world + start;
 		T.n = end - start;
 		T.order = S.order + start;
+		T.order_base = start;
 		T.iterations = W.ws[S.act[T.list[0]].world].iterations;
 		solve_set(W, S, T, sy);
 	}
@@ -745,7 +763,8 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	const int n_act = *S.n_act;
 	T.list = S.arb_by_world ? S.arb_by_world + S.large_first[0] : nullptr;
 	T.n = S.arb_by_world ? n_act - S.large_first[0] : n_act;
-	T.order = S.order + (S.arb_by_world ? S.large_first[0] : 0);
+	T.order_base = S.arb_by_world ? S.large_first[0] : 0;
+	T.order = S.order + T.order_base;
 	T.col_off = S.col_off;
 	T.col_cursor = S.col_cursor;
 	T.left = S.left;

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "cpgpu_step.h"
+#include "cpgpu_adjoint.h"
 
 namespace {
 
@@ -273,6 +274,19 @@ struct cpgpuWorld {
 	int32_t *label = nullptr, *icount = nullptr, *ijoint = nullptr, *groups = nullptr, *n_groups = nullptr;
 	int island_max = 2048, island_blocks_max = 2048;
 	int n_large_joints = 0;
+	// rollout tape for the adjoint (cpgpuWorldTapeBegin)
+	struct Tape {
+		bool on = false;
+		int every = 10;
+		int steps = 0;
+		Arbiter *arb = nullptr;         // every recorded arbiter: those alive at the start, then per step
+		long cap = 0, n = 0;
+		std::vector<long> off;          // per step: first recorded arbiter of the step
+		std::vector<int> count;         // per step: solved arbiters
+		std::vector<std::vector<WorldStep>> ws;
+		std::vector<BodyState *> ckpt;  // bodies at steps 0, every, 2 every, ...
+	} tape;
+	int32_t *d_replay_n = nullptr;
 	// grid-solve tails (see SolveBuffers)
 	int colour_tail_max = 4096, solve_tail_max = 2048;
 	int32_t *tail_list = nullptr, *tail_n = nullptr;
@@ -390,6 +404,7 @@ int check_device_error(cpgpuWorld *w)
 	if (e & CPGPU_ERR_TABLE_FULL) m += " pair table full;";
 	if (e & CPGPU_ERR_EPA_HULL) m += " EPA hull overflow;";
 	if (e & CPGPU_ERR_ISLANDS) m += " inconsistent island labels;";
+	if (e & CPGPU_ERR_ADJOINT) m += " adjoint: a contact's collision re-run differs from the rollout's;";
 	return fail(CPGPU_ERR_DEVICE, m);
 }
 
@@ -737,6 +752,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	return CPGPU_OK;
 }
 
+static void tape_clear(cpgpuWorld *w);
+
 void cpgpuWorldDestroy(cpgpuWorld *w)
 {
 	if (!w) return;
@@ -750,6 +767,8 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	for (void *p : extra) if (p) (void)hipFree(p);
 	if (w->h_ws) (void)hipHostFree(w->h_ws);
 	if (w->h_counts) (void)hipHostFree(w->h_counts);
+	tape_clear(w);
+	if (w->d_replay_n) (void)hipFree(w->d_replay_n);
 	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++)
 		if (w->bucket_lists[q]) (void)hipFree(w->bucket_lists[q]);
 	for (int q = 0; q <= CPGPU_MAX_BUCKETS; q++)
@@ -758,6 +777,60 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 		if (w->fork_join[q]) (void)hipEventDestroy(w->fork_join[q]);
 	(void)hipStreamDestroy(w->stream);
 	delete w;
+}
+
+// The solver's buffers over an arbiter array (the step's, or a replay's).
+static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_act, int32_t *colour, int32_t *order,
+                                  int32_t *warb_off)
+{
+	SolveBuffers S;
+	S.act = act; S.n_act = n_act;
+	S.colour = colour; S.order = order;
+	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
+	S.arb_by_world = grouped ? w->arb_by_world : nullptr;
+	S.keys = w->world_keys;
+	S.label = w->label; S.icount = w->icount; S.ijoint = w->ijoint;
+	S.groups = w->groups; S.n_groups = w->n_groups;
+	S.island_max = w->island_max;
+	S.n_large_joints = w->n_large_joints;
+	S.warb_off = warb_off; S.large_first = w->large_first;
+	S.world_small = w->d_world_small;
+	S.col_off = w->col_off; S.col_cursor = w->col_cursor; S.n_colours = w->n_colours; S.left = w->left;
+	S.claim = w->claim; S.taken = w->taken;
+	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
+	S.jw_order = w->d_jw_order; S.jw_col = w->d_jw_col; S.jw_cbase = w->d_jw_cbase; S.jw_ncol = w->d_jw_ncol;
+	S.bar = w->bar;
+	S.tail_list = w->tail_list; S.tail_n = w->tail_n; S.tail_state = w->tail_n + 2;
+	S.colour_tail_max = w->colour_tail_max; S.solve_tail_max = w->solve_tail_max;
+	S.max_iterations = w->max_iterations_large;
+	S.n_bodies = w->nb;
+	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
+	S.rec = nullptr; S.rec_stride = 0;
+	return S;
+}
+
+static hipError_t tape_reserve(cpgpuWorld *w, long n)
+{
+	auto &T = w->tape;
+	if (n <= T.cap) return hipSuccess;
+	long cap = std::max<long>(1024, T.cap);
+	while (cap < n) cap *= 2;
+	Arbiter *p = nullptr;
+	HIP_TRY(hipMalloc(&p, sizeof(Arbiter) * (size_t)cap));
+	if (T.n > 0) HIP_TRY(hipMemcpyAsync(p, T.arb, sizeof(Arbiter) * (size_t)T.n, hipMemcpyDeviceToDevice, w->stream));
+	HIP_TRY(hipStreamSynchronize(w->stream));
+	if (T.arb) (void)hipFree(T.arb);
+	T.arb = p;
+	T.cap = cap;
+	return hipSuccess;
+}
+
+static void tape_clear(cpgpuWorld *w)
+{
+	auto &T = w->tape;
+	if (T.arb) (void)hipFree(T.arb);
+	for (BodyState *c : T.ckpt) (void)hipFree(c);
+	T = cpgpuWorld::Tape();
 }
 
 int cpgpuWorldStep(cpgpuWorld *w, double dt)
@@ -785,6 +858,15 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		ws.persistence = s.collision_persistence;
 	}
 	HIP_OR_FAIL(hipMemcpyAsync(w->d_ws, w->h_ws, sizeof(WorldStep) * w->nw, hipMemcpyHostToDevice, w->stream));
+	if (w->tape.on) {
+		if (w->tape.steps > 0 && w->tape.steps % w->tape.every == 0) {
+			BodyState *ck = nullptr;
+			HIP_OR_FAIL(hipMalloc(&ck, sizeof(BodyState) * (size_t)w->nb));
+			w->tape.ckpt.push_back(ck);
+			HIP_OR_FAIL(adjoint_state_save(W, ck, w->stream));
+		}
+		w->tape.ws.emplace_back(w->h_ws, w->h_ws + w->nw);
+	}
 	if (w->nc > 0 && dt != w->last_joint_dt) {
 		// bias_coef (chipmunk_private.h:264-268) per joint, libm pow on the host
 		std::vector<double> bc(w->nc);
@@ -864,28 +946,8 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	B.cached = w->cached[cur]; B.n_cached = w->d_step_counts + 3 * cur + 2; B.cap_cached = w->cached_cap[cur];
 	HIP_OR_FAIL(narrowphase_step(W, B, w->d_pairs, w->d_counters, n_pairs, n_prev_recs, w->stream, timer));
 
-	SolveBuffers S;
-	S.act = w->act[cur]; S.n_act = B.n_act;
-	S.colour = w->colour; S.order = w->order;
 	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
-	S.arb_by_world = grouped ? w->arb_by_world : nullptr;
-	S.keys = w->world_keys;
-	S.label = w->label; S.icount = w->icount; S.ijoint = w->ijoint;
-	S.groups = w->groups; S.n_groups = w->n_groups;
-	S.island_max = w->island_max;
-	S.n_large_joints = w->n_large_joints;
-	S.warb_off = w->warb_off; S.large_first = w->large_first;
-	S.world_small = w->d_world_small;
-	S.col_off = w->col_off; S.col_cursor = w->col_cursor; S.n_colours = w->n_colours; S.left = w->left;
-	S.claim = w->claim; S.taken = w->taken;
-	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
-	S.jw_order = w->d_jw_order; S.jw_col = w->d_jw_col; S.jw_cbase = w->d_jw_cbase; S.jw_ncol = w->d_jw_ncol;
-	S.bar = w->bar;
-	S.tail_list = w->tail_list; S.tail_n = w->tail_n; S.tail_state = w->tail_n + 2;
-	S.colour_tail_max = w->colour_tail_max; S.solve_tail_max = w->solve_tail_max;
-	S.max_iterations = w->max_iterations_large;
-	S.n_bodies = w->nb;
-	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
+	SolveBuffers S = solve_buffers(w, w->act[cur], B.n_act, w->colour, w->order, w->warb_off);
 	int island_blocks = 0;
 	if (grouped) {
 		// the solved arbiters by world and island (one sync for the count)
@@ -898,6 +960,24 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	}
 	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, island_blocks, w->n_large_worlds > 0 ? w->solver_grid : 0,
 	                        w->stream, w->aux, w->fork_join, timer));
+
+	if (w->tape.on) {
+		// record the solved arbiters (their contacts learn where they were recorded)
+		int n_act;
+		if (grouped) {
+			n_act = w->h_counts[10];
+		} else {
+			HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, B.n_act, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
+			HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+			n_act = w->h_counts[10];
+		}
+		HIP_OR_FAIL(tape_reserve(w, w->tape.n + n_act));
+		HIP_OR_FAIL(adjoint_tape_record(w->act[cur], nullptr, n_act, w->tape.arb, (int)w->tape.n, w->stream));
+		w->tape.off.push_back(w->tape.n);
+		w->tape.count.push_back(n_act);
+		w->tape.n += n_act;
+		w->tape.steps++;
+	}
 
 	w->cur ^= 1;
 	w->steps++;
@@ -1132,6 +1212,205 @@ int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *t
 	w->timer.acc = saved;
 	w->timer.enabled = was_timing;
 	return CPGPU_OK;
+}
+
+int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (checkpoint_every < 1) return fail(CPGPU_ERR_INVALID, "checkpoint_every must be positive");
+	if (w->n_large_worlds > 0)
+		return fail(CPGPU_ERR_INVALID, "the adjoint supports spaces solved one workgroup each (at most " +
+		                                   std::to_string(w->small_world_max_bodies) + " dynamic bodies)");
+	if (w->nc > 0) return fail(CPGPU_ERR_INVALID, "the adjoint does not support constraints");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	tape_clear(w);
+	auto &T = w->tape;
+	T.every = checkpoint_every;
+	// the arbiters alive now (last step's solved and cached ones) are recorded
+	// first, so that warm starts from them find a recorded source
+	if (w->steps > 0) {
+		const int prev = w->cur ^ 1;
+		const int n_act = w->h_counts[3 * prev + 1], n_cached = w->h_counts[3 * prev + 2];
+		HIP_OR_FAIL(tape_reserve(w, (long)n_act + n_cached));
+		HIP_OR_FAIL(adjoint_tape_record(w->act[prev], nullptr, n_act, T.arb, 0, w->stream));
+		HIP_OR_FAIL(adjoint_tape_record(w->cached[prev], nullptr, n_cached, T.arb, n_act, w->stream));
+		T.n = (long)n_act + n_cached;
+	}
+	BodyState *ck = nullptr;
+	HIP_OR_FAIL(hipMalloc(&ck, sizeof(BodyState) * (size_t)w->nb));
+	T.ckpt.push_back(ck);
+	HIP_OR_FAIL(adjoint_state_save(w->W, ck, w->stream));
+	if (!w->d_replay_n) HIP_OR_FAIL(hipMalloc(&w->d_replay_n, sizeof(int32_t)));
+	T.on = true;
+	return cpgpuWorldSync(w);
+}
+
+int cpgpuWorldTapeEnd(cpgpuWorld *w)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	tape_clear(w);
+	return CPGPU_OK;
+}
+
+// Replay step t of the tape from the bodies now in W: position update, the
+// recorded arbiters with their warm starts, the solve with pre-state records.
+static int replay_step(cpgpuWorld *w, int t, Arbiter *act, int32_t *colour, int32_t *order, int32_t *warb_off,
+                       ArbRec *rec)
+{
+	auto &T = w->tape;
+	DevWorld &W = w->W;
+	HIP_OR_FAIL(hipMemcpyAsync(w->d_ws, T.ws[t].data(), sizeof(WorldStep) * w->nw, hipMemcpyHostToDevice,
+	                           w->stream));
+	if (w->n_dyn_bodies > 0)
+		k_update_positions<<<blocks_for(w->n_dyn_bodies, 256), 256, 0, w->stream>>>(W, w->d_dyn_bodies,
+		                                                                           w->n_dyn_bodies);
+	const int n = T.count[t];
+	HIP_OR_FAIL(adjoint_replay_prepare(T.arb, (int)T.off[t], n, act, w->stream));
+	HIP_OR_FAIL(hipMemcpyAsync(w->d_replay_n, &T.count[t], sizeof(int32_t), hipMemcpyHostToDevice, w->stream));
+	SolveBuffers S = solve_buffers(w, act, w->d_replay_n, colour, order, warb_off);
+	S.rec = rec;
+	S.rec_stride = n;
+	if (w->nw > 1) HIP_OR_FAIL(solver_group(W, S, n, w->nw, false, w->sortbuf, w->stream, nullptr));
+	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, 0, 0, w->stream, w->aux, w->fork_join, nullptr));
+	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+	return CPGPU_OK;
+}
+
+int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, const double *g_v, const double *g_w,
+                       double *d_p0, double *d_a0, double *d_v0, double *d_w0, double *d_m_inv, double *d_i_inv,
+                       double *d_u)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	auto &T = w->tape;
+	if (!T.on) return fail(CPGPU_ERR_INVALID, "no tape: call cpgpuWorldTapeBegin before the rollout");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	DevWorld &W = w->W;
+	const int nb = w->nb, ns = w->ns;
+	int max_n = 1, max_it = 1;
+	for (int c : T.count) max_n = std::max(max_n, c);
+	for (auto &s : w->sp) max_it = std::max(max_it, (int)s.iterations);
+	const int K = T.every;
+	// device buffers of the pass
+	BodyAdj *gb = nullptr;
+	double *gminv = nullptr, *giinv = nullptr, *gu = nullptr;
+	double2 *gJ = nullptr;
+	BodyState *live = nullptr, *start = nullptr;
+	Arbiter *act = nullptr;
+	int32_t *colour = nullptr, *order = nullptr, *warb = nullptr;
+	ArbRec *rec = nullptr;
+	void *garb = nullptr;
+	unsigned long long *totals_saved = nullptr;
+	std::vector<void *> bufs;
+	auto grab = [&](void **p, size_t bytes) -> hipError_t {
+		hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+		if (e == hipSuccess) bufs.push_back(*p);
+		return e;
+	};
+	auto release = [&]() { for (void *p : bufs) (void)hipFree(p); bufs.clear(); };
+#define ADJ_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { release(); \
+	return fail(CPGPU_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); } } while (0)
+	ADJ_TRY(grab((void **)&gb, sizeof(BodyAdj) * nb));
+	ADJ_TRY(grab((void **)&gminv, sizeof(double) * nb));
+	ADJ_TRY(grab((void **)&giinv, sizeof(double) * nb));
+	ADJ_TRY(grab((void **)&gu, sizeof(double) * ns));
+	ADJ_TRY(grab((void **)&gJ, sizeof(double2) * 2 * (size_t)std::max<long>(T.n, 1)));
+	ADJ_TRY(grab((void **)&live, sizeof(BodyState) * nb));
+	ADJ_TRY(grab((void **)&start, sizeof(BodyState) * nb * (size_t)K));
+	ADJ_TRY(grab((void **)&act, sizeof(Arbiter) * (size_t)max_n * K));
+	ADJ_TRY(grab((void **)&colour, sizeof(int32_t) * (size_t)max_n * K));
+	ADJ_TRY(grab((void **)&order, sizeof(int32_t) * (size_t)max_n * K));
+	ADJ_TRY(grab((void **)&warb, sizeof(int32_t) * (size_t)(w->nw + 2) * K));
+	ADJ_TRY(grab((void **)&rec, sizeof(ArbRec) * (size_t)max_n * max_it * K));
+	ADJ_TRY(grab(&garb, adjoint_arb_scratch_bytes() * (size_t)max_n));
+	ADJ_TRY(grab((void **)&totals_saved, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS)));
+	// replay scratch sized for the largest step
+	if (max_n > w->colour_cap) {
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups};
+		for (void *p : old) if (p) (void)hipFree(p);
+		w->colour_cap = max_n;
+		ADJ_TRY(dalloc(w->colour, w->colour_cap));
+		ADJ_TRY(dalloc(w->order, w->colour_cap));
+		ADJ_TRY(dalloc(w->arb_by_world, w->colour_cap));
+		ADJ_TRY(dalloc(w->world_keys, w->colour_cap));
+		ADJ_TRY(dalloc(w->groups, w->colour_cap));
+	}
+	// seeds: the adjoint of the state after the last step
+	{
+		std::vector<BodyAdj> h(nb);
+		for (int b = 0; b < nb; b++) {
+			BodyAdj &x = h[b];
+			x.p[0] = g_p ? g_p[2 * b] : 0.0; x.p[1] = g_p ? g_p[2 * b + 1] : 0.0;
+			x.a = g_a ? g_a[b] : 0.0;
+			x.v[0] = g_v ? g_v[2 * b] : 0.0; x.v[1] = g_v ? g_v[2 * b + 1] : 0.0;
+			x.w = g_w ? g_w[b] : 0.0;
+			x.vb[0] = x.vb[1] = x.wb = 0.0;
+		}
+		ADJ_TRY(hipMemcpy(gb, h.data(), sizeof(BodyAdj) * nb, hipMemcpyHostToDevice));
+	}
+	ADJ_TRY(hipMemset(gminv, 0, sizeof(double) * nb));
+	ADJ_TRY(hipMemset(giinv, 0, sizeof(double) * nb));
+	ADJ_TRY(hipMemset(gu, 0, sizeof(double) * ns));
+	ADJ_TRY(hipMemset(gJ, 0, sizeof(double2) * 2 * (size_t)std::max<long>(T.n, 1)));
+	ADJ_TRY(adjoint_state_save(W, live, w->stream));
+	ADJ_TRY(hipMemcpyAsync(totals_saved, W.totals, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS),
+	                       hipMemcpyDeviceToDevice, w->stream));
+	const int n_seg = (T.steps + K - 1) / K;
+	int result = CPGPU_OK;
+	for (int seg = n_seg - 1; seg >= 0 && result == CPGPU_OK; seg--) {
+		const int t0 = seg * K, t1 = std::min(t0 + K, T.steps);
+		ADJ_TRY(adjoint_state_load(W, T.ckpt[seg], w->stream));
+		for (int t = t0; t < t1 && result == CPGPU_OK; t++) {
+			const int q = t - t0;
+			ADJ_TRY(adjoint_state_save(W, start + (size_t)nb * q, w->stream));
+			result = replay_step(w, t, act + (size_t)max_n * q, colour + (size_t)max_n * q, order + (size_t)max_n * q,
+			                     warb + (size_t)(w->nw + 2) * q, rec + (size_t)max_n * max_it * q);
+		}
+		for (int t = t1 - 1; t >= t0 && result == CPGPU_OK; t--) {
+			const int q = t - t0;
+			// stream-ordered: the previous step's reverse may still be reading the parameters
+			ADJ_TRY(hipMemcpyAsync(w->d_ws, T.ws[t].data(), sizeof(WorldStep) * w->nw, hipMemcpyHostToDevice,
+			                       w->stream));
+			AdjStep A;
+			A.n = T.count[t];
+			A.act = act + (size_t)max_n * q;
+			A.order = order + (size_t)max_n * q;
+			A.colour = colour + (size_t)max_n * q;
+			A.warb_off = w->nw > 1 ? warb + (size_t)(w->nw + 2) * q : nullptr;
+			A.rec = rec + (size_t)max_n * max_it * q;
+			A.start = start + (size_t)nb * q;
+			A.tape = T.arb;
+			A.garb = garb;
+			A.gb = gb; A.gminv = gminv; A.giinv = giinv; A.gu = gu; A.gJ = gJ;
+			ADJ_TRY(adjoint_reverse_step(W, A, w->nw, w->stream));
+		}
+		if (result == CPGPU_OK) result = sync_world(w);
+	}
+	// results, then the world as it was
+	if (result == CPGPU_OK) {
+		std::vector<BodyAdj> h(nb);
+		ADJ_TRY(hipMemcpy(h.data(), gb, sizeof(BodyAdj) * nb, hipMemcpyDeviceToHost));
+		for (int b = 0; b < nb; b++) {
+			if (d_p0) { d_p0[2 * b] = h[b].p[0]; d_p0[2 * b + 1] = h[b].p[1]; }
+			if (d_a0) d_a0[b] = h[b].a;
+			if (d_v0) { d_v0[2 * b] = h[b].v[0]; d_v0[2 * b + 1] = h[b].v[1]; }
+			if (d_w0) d_w0[b] = h[b].w;
+		}
+		if (d_m_inv) ADJ_TRY(hipMemcpy(d_m_inv, gminv, sizeof(double) * nb, hipMemcpyDeviceToHost));
+		if (d_i_inv) ADJ_TRY(hipMemcpy(d_i_inv, giinv, sizeof(double) * nb, hipMemcpyDeviceToHost));
+		if (d_u) ADJ_TRY(hipMemcpy(d_u, gu, sizeof(double) * ns, hipMemcpyDeviceToHost));
+	}
+	ADJ_TRY(adjoint_state_load(W, live, w->stream));
+	ADJ_TRY(hipMemcpyAsync(W.totals, totals_saved, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS),
+	                       hipMemcpyDeviceToDevice, w->stream));
+	ADJ_TRY(hipMemcpyAsync(w->d_ws, w->h_ws, sizeof(WorldStep) * w->nw, hipMemcpyHostToDevice, w->stream));
+	ADJ_TRY(hipStreamSynchronize(w->stream));
+	release();
+#undef ADJ_TRY
+	return result;
 }
 
 int cpgpuWorldGetSolveCounts(cpgpuWorld *w, int64_t *contacts, int64_t *bodies)

@@ -26,7 +26,8 @@ EXPORTS = [
     "cpgpuWorldCreate", "cpgpuWorldDestroy", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
     "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
-    "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuGetLastError", "cpgpuVersion",
+    "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
+    "cpgpuWorldBackward", "cpgpuGetLastError", "cpgpuVersion",
 ]
 
 
@@ -58,6 +59,9 @@ def lib():
         L.cpgpuWorldGetBuckets.argtypes = [_vp, _ip, _ip, _ip, _ip]
         L.cpgpuWorldSetBuckets.argtypes = [_vp, ctypes.c_int32, _ip, _ip]
         L.cpgpuWorldAutotune.argtypes = [_vp, ctypes.c_double, ctypes.c_int32, _ip]
+        L.cpgpuWorldTapeBegin.argtypes = [_vp, ctypes.c_int32]
+        L.cpgpuWorldTapeEnd.argtypes = [_vp]
+        L.cpgpuWorldBackward.argtypes = [_vp] + [_dp] * 11
         L.cpgpuGetLastError.restype = ctypes.c_char_p
         L.cpgpuVersion.restype = ctypes.c_char_p
         _lib = L
@@ -169,6 +173,31 @@ class GPUWorld:
         out = np.zeros(4, np.int32)
         _check(self.L.cpgpuWorldAutotune(self.h, dt, trial_steps, ptr(out, ctypes.c_int32)))
         return [int(x) for x in out]
+
+    def tape_begin(self, checkpoint_every=10):
+        """Record the rollout from here on for backward()."""
+        _check(self.L.cpgpuWorldTapeBegin(self.h, checkpoint_every))
+
+    def tape_end(self):
+        _check(self.L.cpgpuWorldTapeEnd(self.h))
+
+    def backward(self, g_p=None, g_a=None, g_v=None, g_w=None):
+        """Adjoint of the taped rollout.  g_*: dL/d of the final positions [n,2], angles [n],
+        velocities [n,2], angular velocities [n] (None: zero).  Returns a dict of dL/d of the
+        initial state ('p', 'a', 'v', 'w') and of the parameters ('m_inv', 'i_inv' per body,
+        'u' per shape)."""
+        nb, ns = self.packed.n_bodies, self.packed.n_shapes
+
+        def inp(x, shape):
+            return None if x is None else np.ascontiguousarray(np.broadcast_to(x, shape), np.float64)
+
+        seeds = [inp(g_p, (nb, 2)), inp(g_a, (nb,)), inp(g_v, (nb, 2)), inp(g_w, (nb,))]
+        out = {"p": np.zeros((nb, 2)), "a": np.zeros(nb), "v": np.zeros((nb, 2)), "w": np.zeros(nb),
+               "m_inv": np.zeros(nb), "i_inv": np.zeros(nb), "u": np.zeros(ns)}
+        args = [None if x is None else ptr(x, ctypes.c_double) for x in seeds]
+        args += [ptr(out[k], ctypes.c_double) for k in ("p", "a", "v", "w", "m_inv", "i_inv", "u")]
+        _check(self.L.cpgpuWorldBackward(self.h, *args))
+        return out
 
     SOLVE_UNITS = ("solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2", "solver_worlds_b3", "solver_islands",
                    "solver")

@@ -11,11 +11,17 @@ Workloads (sizes as named):
                     sinusoidal terrain; default 50k circles, 40k polys, 10k segments
   batch_worlds      many independent spaces of log-uniform body counts, each a
                     pyramid or a random pile on its own ground
+  grad_batch        many spaces of 64 circles / boxes launched onto a ground
+                    segment, with target positions (differentiation)
+  chain_world       a pivot-joint chain with springs, slides and gears, and
+                    falling circles
 """
 import math
 
-from .space import Space, Body, Circle, Segment, Poly, Box, moment_for_box, PivotJoint, DampedSpring, \
-    SlideJoint, GearJoint
+import numpy as np
+
+from .space import Space, Body, Circle, Segment, Poly, Box, moment_for_box, moment_for_circle, PivotJoint, \
+    DampedSpring, SlideJoint, GearJoint
 
 MASK64 = (1 << 64) - 1
 
@@ -182,6 +188,50 @@ def random_pile(n_bodies, rng, width=None, x_offset=0.0):
         body.position = (x_offset + rng.uniform(-half, half), rng.uniform(3.0, 3.0 + height))
         body.velocity = (rng.normal(0.0, 5.0), rng.normal(0.0, 5.0))
     return space
+
+
+def grad_world(rng, n_bodies=64, vel_sigma=20.0):
+    """One space of the differentiation workload: n_bodies circles or boxes
+    (radius or half-side U[0.5,2], mass U[0.5,2], friction U[0.2,0.8],
+    elasticity 0) on an 8-wide grid 6 apart above a ground segment (friction
+    0.7), launched with velocities N(0, vel_sigma); gravity (0,-100), 10
+    iterations.  Returns (space, targets [n_bodies, 2]) with seeded target
+    positions U[-40,40] x U[0,30].  The moment follows the mass (i = k m for
+    the body's geometry)."""
+    space = Space()
+    space.iterations = 10
+    space.gravity = (0.0, -100.0)
+    ground = space.add(Segment(space.static_body, (-80.0, 0.0), (80.0, 0.0), 0.5))
+    ground.friction = 0.7
+    for k in range(n_bodies):
+        size = rng.uniform(0.5, 2.0)
+        m = rng.uniform(0.5, 2.0)
+        if rng.uniform() < 0.5:
+            body = space.add(Body(m, moment_for_circle(m, 0.0, size)))
+            shape = space.add(Circle(body, size))
+        else:
+            body = space.add(Body(m, moment_for_box(m, 2.0 * size, 2.0 * size)))
+            shape = space.add(Box(body, 2.0 * size, 2.0 * size, 0.0))
+        shape.friction = rng.uniform(0.2, 0.8)
+        shape.elasticity = 0.0
+        body.position = (-21.0 + 6.0 * (k % 8), 5.0 + 6.0 * (k // 8))
+        body.velocity = (rng.normal(0.0, vel_sigma), rng.normal(0.0, vel_sigma))
+    targets = np.array([(rng.uniform(-40.0, 40.0), rng.uniform(0.0, 30.0)) for _ in range(n_bodies)])
+    return space, targets
+
+
+def grad_batch(n_worlds=1024, n_bodies=64, seed=SEED_GRAD):
+    """The differentiation batch: n_worlds spaces of grad_world.  Returns
+    (packed description, targets [n_bodies_total, 2] with zeros for static
+    bodies)."""
+    from .cabi import concat
+    rng = Rng(seed)
+    packs, targets = [], []
+    for _ in range(n_worlds):
+        sp, tg = grad_world(rng, n_bodies)
+        packs.append(sp.pack())
+        targets.append(np.vstack([np.zeros((1, 2)), tg]))   # the static body comes first
+    return concat(packs), np.vstack(targets)
 
 
 def chain_world(n_links=200, n_circles=300, seed=SEED_CHAINS, rng=None):

@@ -232,6 +232,28 @@ int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies
  * size per bucket (reported in threads_out[CPGPU_MAX_BUCKETS]). */
 int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out);
 
+/* ---- differentiation (no counterpart in the reference: cpFloat is a plain
+ * double there).  cpgpuWorldTapeBegin starts recording the rollout: every
+ * later cpgpuWorldStep records its solved arbiters, and the bodies' state is
+ * checkpointed every checkpoint_every steps.  cpgpuWorldBackward then takes
+ * the adjoint (dL/d state) of the state after the last recorded step --
+ * positions g_p [n][2], angles g_a, velocities g_v [n][2], angular velocities
+ * g_w (NULL: zero) -- and returns dL/d of the state at the tape's start
+ * (d_p0, d_a0, d_v0, d_w0) and of the parameters over the rollout: each
+ * body's 1/m and 1/I (d_m_inv, d_i_inv) and each shape's friction
+ * coefficient (d_u [n_shapes]).  Any output may be NULL.  The backward pass
+ * replays each checkpoint segment with the step's own kernels and runs the
+ * hand-written reverse kernels; the world is left as it was.  The contact
+ * set and the clamp branches of the rollout are held fixed (derivatives
+ * where they change are one-sided).  Spaces must be solved one workgroup each
+ * and have no constraints; polygons of at most 8 vertices; no velocity or
+ * force writes during a taped rollout. */
+int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every);
+int cpgpuWorldTapeEnd(cpgpuWorld *world);
+int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v,
+                       const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0,
+                       double *d_m_inv, double *d_i_inv, double *d_u);
+
 /* Contacts solved and bodies velocity-integrated since creation by each
  * solve unit, in the order of the timing names solver_worlds_b0 ..
  * solver_worlds_b3 (size buckets of small worlds), solver_islands,

@@ -6,12 +6,18 @@ polygons, 10k static segments (container + sinusoidal terrain), seeded,
 10 solver iterations, dt 1/60 -- stepped for the configured 300-step rollout
 from its initial state.  A "step" is one cpSpaceStep of the whole world.
 
-    python bench.py [--steps K] [--warmup W] [--workload random_world_100k|batch_4096]
+    python bench.py [--steps K] [--warmup W] [--workload random_world_100k|batch_4096|chains_2048|grad_1024]
 
 W warm-up steps run on a separate copy of the world (first-launch costs),
 then exactly K steps are timed on a fresh copy, so the timed region is
 steps 0..K-1 of the rollout.  The step library synchronises its own stream
 at both ends of the timed region.  Prints one JSON line.
+
+Other workloads: batch_4096 (4096 spaces, log-uniform 16..2048 bodies, 240
+steps), chains_2048 (2048 spaces of a 200-link jointed chain and 300
+circles, 300 steps), grad_1024 (1024 spaces of 64 circles / boxes: a taped
+120-step rollout and its adjoint; a step is one forward step and its share
+of the backward pass, so value is forward+adjoint body-steps/s).
 
 roofline: the dominant kernel's algorithmic bytes per launch (streaming
 model: every phase reads and writes the state the reference's dataflow
@@ -26,6 +32,8 @@ import os
 import platform
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -43,6 +51,20 @@ def build_workload(name):
         return sp.pack(), {"workload": "random_world_100k", "circles": 50000, "polys": 40000,
                            "static_segments": 10000, "extent": 2000.0, "iterations": 10, "dt": 1.0 / 60.0,
                            "seed": hex(scenes.SEED_RANDOM_WORLD), "rollout_steps": 300}
+    if name == "chains_2048":
+        pw = scenes.chains_packed(n_worlds=2048, processes=min(16, os.cpu_count() or 1))
+        return pw, {"workload": "chains_2048", "worlds": pw.n_worlds, "links": 200, "circles": 300,
+                    "constraints_per_world": pw.n_constraints // pw.n_worlds,
+                    "joints": "pivot chain (collideBodies off), damped spring and slide joint to link i-2, "
+                              "gear every 10 links", "iterations": 10, "dt": 1.0 / 60.0,
+                    "seed": hex(scenes.SEED_CHAINS), "rollout_steps": 300}
+    if name == "grad_1024":
+        pw, targets = scenes.grad_batch(n_worlds=1024, n_bodies=64)
+        pw.targets = targets
+        return pw, {"workload": "grad_1024", "worlds": pw.n_worlds, "bodies_per_world": 64,
+                    "loss": "sum |p_T - target|^2", "grads": "initial velocities, masses, friction",
+                    "checkpoint_every": 10, "iterations": 10, "dt": 1.0 / 60.0, "seed": hex(scenes.SEED_GRAD),
+                    "rollout_steps": 120}
     if name == "batch_4096":
         pw = scenes.batch_packed(n_worlds=4096, processes=min(16, os.cpu_count() or 1))
         return pw, {"workload": "batch_4096", "worlds": pw.n_worlds, "bodies_log_uniform": [16, 2048],
@@ -65,6 +87,10 @@ def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations, 
     if kernel in solve_counts:
         uc, ub = solve_counts[kernel]
         return solver_bytes(uc / max(launches, 1), ub / max(launches, 1), iterations)
+    if kernel == "reverse_step":
+        # reads the forward solve's state and writes as many adjoints (twice the forward's bytes), plus
+        # the recorded pre-state of every arbiter application (144 B)
+        return 2 * solver_bytes(c, n_dyn_bodies, iterations) + a * iterations * 144
     if kernel == "narrowphase":
         # per pair: two shapes' world geometry (~8s each for the mix), material 4s + 8 B each,
         # warm-start record 24 B; per colliding pair its arbiter header 6s + 8 B, per contact 4s + 8 B
@@ -114,6 +140,36 @@ def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60):
     return t1 - t0, [t - t0 for t in t_step], timing, stats, counts, tiling
 
 
+def run_grad(pw, steps, warmup, dt, checkpoint_every=10):
+    """A taped rollout of `steps` steps and its backward pass: (forward seconds, backward seconds, timing,
+    stats, loss)."""
+    from chipmunk2d_amd.gpu import GPUWorld
+    from chipmunk2d_amd import grad as G
+    dyn = (pw.a["body_type"] == G.DYNAMIC).astype(float)
+    if warmup > 0:
+        g = GPUWorld(pw)
+        g.tape_begin(checkpoint_every)
+        g.step(dt, warmup)
+        g.backward(g_p=np.ones((pw.n_bodies, 2)))
+        g.close()
+    g = GPUWorld(pw)
+    g.sync()
+    g.set_timing(True)
+    g.tape_begin(checkpoint_every)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step(dt)
+    g.sync()
+    t1 = time.perf_counter()
+    L, gp = G.position_loss(g.bodies().p, pw.targets, dyn)
+    t2 = time.perf_counter()
+    g.backward(g_p=gp)
+    t3 = time.perf_counter()
+    timing, stats, counts = g.timing(), g.stats(), g.solve_counts()
+    g.close()
+    return t1 - t0, t3 - t2, timing, stats, counts, L
+
+
 def run_cpu_reference(pw, sample_steps, dt):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from checkers import RefWorld, ref_lib, REF_SO
@@ -128,9 +184,10 @@ def run_cpu_reference(pw, sample_steps, dt):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's rollout length)")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="random_world_100k", choices=["random_world_100k", "batch_4096"])
+    ap.add_argument("--workload", default="random_world_100k",
+                    choices=["random_world_100k", "batch_4096", "chains_2048", "grad_1024"])
     ap.add_argument("--cpu-sample-steps", type=int, default=120,
                     help="steps of the same rollout the reference CPU step is timed on")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -144,15 +201,26 @@ def main():
 
     dt = 1.0 / 60.0
     pw, config = build_workload(args.workload)
+    if args.steps is None:
+        args.steps = config["rollout_steps"]
     n_dyn = pw.dynamic_body_count()
-    import numpy as np
     body_type = pw.a["body_type"]
     n_dyn_shapes = int(np.count_nonzero(body_type[pw.a["shape_body"]] != 2))
     iterations = max(sp["iterations"] for sp in pw.a["space"])
     config.update({"bodies": n_dyn, "shapes": pw.n_shapes, "worlds": pw.n_worlds})
 
-    autotune = args.autotune_steps if args.workload == "batch_4096" else 0
-    wall, t_steps, timing, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt, autotune)
+    autotune = args.autotune_steps if args.workload in ("batch_4096", "chains_2048") else 0
+    grad_extra = None
+    if args.workload == "grad_1024":
+        t_fwd, t_bwd, timing, stats, solve_counts, loss = run_grad(pw, args.steps, args.warmup, dt)
+        wall = t_fwd + t_bwd
+        t_steps = None
+        tiling = None
+        grad_extra = {"forward_s": round(t_fwd, 4), "backward_s": round(t_bwd, 4), "loss": loss,
+                      "fwd_adjoint_steps_per_s": round(args.steps / wall, 2),
+                      "forward_only_body_steps_per_s": round(n_dyn * args.steps / t_fwd, 1)}
+    else:
+        wall, t_steps, timing, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt, autotune)
     if autotune:
         config["tiling"] = {"autotuned": True, "trial_steps": autotune, "tuned_after_steps": 60,
                             "buckets": [{"max_bodies": b[0], "threads": b[1], "worlds": b[2], "lds_bodies": b[3]}
@@ -212,6 +280,9 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    if grad_extra:
+        out["metric_part"] = "fwd+adjoint"
+        out.update(grad_extra)
     print(json.dumps(out))
 
 

@@ -79,6 +79,12 @@ __global__ void k_update_positions(const DevWorld W, const int32_t *bodies, int 
 	W.b_vel[b] = bv;
 }
 
+__global__ void k_set_joint_bias(Joint *joints, const double *bc, int n)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < n) joints[c].bias_coef = bc[c];
+}
+
 // cacheData of the three shape classes (cpShape.c:291-296, 378-405; cpPolyShape.c:39-64)
 __device__ void cache_shape(const DevWorld &W, int s)
 {
@@ -868,12 +874,19 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		w->tape.ws.emplace_back(w->h_ws, w->h_ws + w->nw);
 	}
 	if (w->nc > 0 && dt != w->last_joint_dt) {
-		// bias_coef (chipmunk_private.h:264-268) per joint, libm pow on the host
+		// bias_coef (chipmunk_private.h:264-268) per joint, libm pow on the host (once per
+		// distinct errorBias), then scattered into the joints on the device
 		std::vector<double> bc(w->nc);
-		for (int c = 0; c < w->nc; c++) bc[c] = 1.0 - pow(w->joint_error_bias[c], dt);
-		HIP_OR_FAIL(hipMemcpy2DAsync(&W.joints[0].bias_coef, sizeof(Joint), bc.data(), sizeof(double), sizeof(double),
-		                             w->nc, hipMemcpyHostToDevice, w->stream));
-		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+		double last_eb = 0.0, last_bc = 0.0;
+		bool have = false;
+		for (int c = 0; c < w->nc; c++) {
+			const double eb = w->joint_error_bias[c];
+			if (!have || eb != last_eb) { last_eb = eb; last_bc = 1.0 - pow(eb, dt); have = true; }
+			bc[c] = last_bc;
+		}
+		HIP_OR_FAIL(hipMemcpy(w->d_joint_bias_coef, bc.data(), sizeof(double) * w->nc, hipMemcpyHostToDevice));
+		k_set_joint_bias<<<blocks_for(w->nc, 256), 256, 0, w->stream>>>(W.joints, w->d_joint_bias_coef, w->nc);
+		HIP_OR_FAIL(hipGetLastError());
 		w->last_joint_dt = dt;
 	}
 
@@ -1258,7 +1271,7 @@ int cpgpuWorldTapeEnd(cpgpuWorld *w)
 // Replay step t of the tape from the bodies now in W: position update, the
 // recorded arbiters with their warm starts, the solve with pre-state records.
 static int replay_step(cpgpuWorld *w, int t, Arbiter *act, int32_t *colour, int32_t *order, int32_t *warb_off,
-                       ArbRec *rec)
+                       ArbRec *rec, KernelTimer *timer)
 {
 	auto &T = w->tape;
 	DevWorld &W = w->W;
@@ -1274,7 +1287,8 @@ static int replay_step(cpgpuWorld *w, int t, Arbiter *act, int32_t *colour, int3
 	S.rec = rec;
 	S.rec_stride = n;
 	if (w->nw > 1) HIP_OR_FAIL(solver_group(W, S, n, w->nw, false, w->sortbuf, w->stream, nullptr));
-	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, 0, 0, w->stream, w->aux, w->fork_join, nullptr));
+	TIMED(timer, "replay_solve", w->stream,
+	      HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, 0, 0, w->stream, w->aux, w->fork_join, nullptr)));
 	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
 	return CPGPU_OK;
 }
@@ -1289,6 +1303,7 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	int rc = cpgpuWorldSync(w);
 	if (rc != CPGPU_OK) return rc;
 	DevWorld &W = w->W;
+	KernelTimer *timer = w->timer.enabled ? &w->timer : nullptr;
 	const int nb = w->nb, ns = w->ns;
 	int max_n = 1, max_it = 1;
 	for (int c : T.count) max_n = std::max(max_n, c);
@@ -1367,7 +1382,7 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 			const int q = t - t0;
 			ADJ_TRY(adjoint_state_save(W, start + (size_t)nb * q, w->stream));
 			result = replay_step(w, t, act + (size_t)max_n * q, colour + (size_t)max_n * q, order + (size_t)max_n * q,
-			                     warb + (size_t)(w->nw + 2) * q, rec + (size_t)max_n * max_it * q);
+			                     warb + (size_t)(w->nw + 2) * q, rec + (size_t)max_n * max_it * q, timer);
 		}
 		for (int t = t1 - 1; t >= t0 && result == CPGPU_OK; t--) {
 			const int q = t - t0;
@@ -1385,7 +1400,7 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 			A.tape = T.arb;
 			A.garb = garb;
 			A.gb = gb; A.gminv = gminv; A.giinv = giinv; A.gu = gu; A.gJ = gJ;
-			ADJ_TRY(adjoint_reverse_step(W, A, w->nw, w->stream));
+			TIMED(timer, "reverse_step", w->stream, ADJ_TRY(adjoint_reverse_step(W, A, w->nw, w->stream)));
 		}
 		if (result == CPGPU_OK) result = sync_world(w);
 	}

@@ -236,10 +236,12 @@ def grad_batch(n_worlds=1024, n_bodies=64, seed=SEED_GRAD):
 
 def chain_world(n_links=200, n_circles=300, seed=SEED_CHAINS, rng=None):
     """A chain of n_links boxes (4 x 1, mass 1) joined by pivot joints
-    (collideBodies off) and hung from a static body, with a damped spring
-    (stiffness U[50,500], damping U[1,10]), a slide joint and a gear joint
-    every 10 links, and n_circles circles (radius U[0.5,1.5]) falling on it.
-    Gravity (0,-100), 10 iterations."""
+    (collideBodies off) and hung from a static body; every link i >= 2 also
+    has a damped spring to link i-2 (rest length 8, stiffness U[50,500],
+    damping U[1,10]) and a slide joint to it (limits [6, 8]); a gear joint
+    (ratio 1) joins every 10th link to the next; n_circles circles (radius
+    U[0.5,1.5], density 1) fall on the chain.  Gravity (0,-100), 10
+    iterations."""
     if rng is None:
         rng = Rng(seed)
     space = Space()
@@ -258,11 +260,11 @@ def chain_world(n_links=200, n_circles=300, seed=SEED_CHAINS, rng=None):
         j.collide_bodies = False
         links.append(body)
         prev = body
-    for i in range(0, n_links - 10, 10):
-        a, b = links[i], links[i + 10]
-        space.add(DampedSpring(a, b, (0.0, 0.0), (0.0, 0.0), 40.0, rng.uniform(50.0, 500.0), rng.uniform(1.0, 10.0)))
-        if i + 2 < n_links:
-            space.add(SlideJoint(links[i], links[i + 2], (0.0, 0.0), (0.0, 0.0), 4.0, 8.5))
+    for i in range(2, n_links):
+        a, b = links[i - 2], links[i]
+        space.add(DampedSpring(a, b, (0.0, 0.0), (0.0, 0.0), 8.0, rng.uniform(50.0, 500.0), rng.uniform(1.0, 10.0)))
+        space.add(SlideJoint(a, b, (0.0, 0.0), (0.0, 0.0), 6.0, 8.0))
+    for i in range(0, n_links - 1, 10):
         space.add(GearJoint(links[i], links[i + 1], 0.0, 1.0))
     for _ in range(n_circles):
         body = space.add(Body(0.0, 0.0))
@@ -271,6 +273,26 @@ def chain_world(n_links=200, n_circles=300, seed=SEED_CHAINS, rng=None):
         shape.friction = 0.5
         body.position = (rng.uniform(x0, -x0), rng.uniform(10.0, 10.0 + 0.5 * n_links))
     return space
+
+
+def _pack_chain(seed):
+    return chain_world(rng=Rng(seed)).pack()
+
+
+def chains_packed(n_worlds=2048, seed=SEED_CHAINS, processes=8):
+    """The constraints-and-contacts batch: n_worlds chain_world spaces (200
+    links, 300 circles), each seeded from the batch seed, built in worker
+    processes and packed into one description."""
+    from .cabi import concat
+    rng = Rng(seed)
+    seeds = [rng.next_u64() for _ in range(n_worlds)]
+    if processes and processes > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(processes) as pool:
+            packed = pool.map(_pack_chain, seeds, chunksize=16)
+    else:
+        packed = [_pack_chain(s) for s in seeds]
+    return concat(packed)
 
 
 def batch_specs(n_worlds=4096, lo=16, hi=2048, seed=SEED_BATCH, pyramid_fraction=0.5):

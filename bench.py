@@ -104,6 +104,18 @@ def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations, 
     return None
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (profiles/traffic_<workload>.txt, calibrated on a device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bench_kernel=%s " % kernel):
+            return int(line.split("hbm_bytes_per_launch=")[1].split()[0])
+    return None
+
+
 def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60):
     """Warm-up (and, for batches, the autotuner, once the copy has run `autotune_at` steps so that piles have
     formed) on a throwaway copy of the world, then `steps` timed steps on a fresh copy with the tuned tiling."""
@@ -170,15 +182,40 @@ def run_grad(pw, steps, warmup, dt, checkpoint_every=10):
     return t1 - t0, t3 - t2, timing, stats, counts, L
 
 
-def run_cpu_reference(pw, sample_steps, dt):
+def run_cpu_reference(pw, sample_steps, dt, hasty_threads=0, release=False):
+    """Seconds for the reference's own cpSpaceStep (or cpHastySpaceStep with hasty_threads) to step world 0
+    of pw sample_steps times."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from checkers import RefWorld, ref_lib, REF_SO
+    from checkers import RefWorld, REF_SO
     if not os.path.exists(REF_SO):
         return None
-    r = RefWorld(pw)
+    r = RefWorld(pw, hasty_threads=hasty_threads, release=release)
     t0 = time.perf_counter()
     r.step(dt, sample_steps)
     return time.perf_counter() - t0
+
+
+def run_cpu_reference_batch(pw, sample_steps, dt, threads):
+    """Seconds for the reference's cpSpaceStep to step every world of pw sample_steps times, the worlds
+    split into contiguous slices over `threads` host threads (each slice stepped one step at a time,
+    all slices joined after every step)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from checkers import RefWorld, REF_SO
+    if not os.path.exists(REF_SO):
+        return None
+    from concurrent.futures import ThreadPoolExecutor
+    worlds = [RefWorld(pw, w) for w in range(pw.n_worlds)]
+    slices = [worlds[k::threads] for k in range(threads)]
+
+    def step_slice(sl):
+        for r in sl:
+            r.step(dt)
+
+    with ThreadPoolExecutor(threads) as ex:
+        t0 = time.perf_counter()
+        for _ in range(sample_steps):
+            list(ex.map(step_slice, slices))
+        return time.perf_counter() - t0
 
 
 def main():
@@ -190,6 +227,8 @@ def main():
                     choices=["random_world_100k", "batch_4096", "chains_2048", "grad_1024"])
     ap.add_argument("--cpu-sample-steps", type=int, default=120,
                     help="steps of the same rollout the reference CPU step is timed on")
+    ap.add_argument("--cpu-batch-sample-steps", type=int, default=20,
+                    help="steps of a batch rollout the reference CPU step is timed on (all worlds)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune-steps", type=int, default=3,
                     help="steps per trial of the batch tiling autotuner (on the warm-up copy; 0: default tiling)")
@@ -242,7 +281,7 @@ def main():
     if dominant and "GBps" in kernels.get(dominant, {}):
         ach = kernels[dominant]["GBps"]
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None,
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dominant),
                     "bytes_per_launch": kernels[dominant]["bytes_per_launch"],
                     "avg_launch_us": kernels[dominant]["avg_us"]}
 
@@ -255,6 +294,27 @@ def main():
             cpu = {"value": round(n_dyn * k / t_cpu, 1), "unit": "body-steps/s", "cores": 1, "kind": "reference",
                    "sample": "steps 0-%d of the same 300-step rollout (the reference's cpSpaceStep, strict build "
                              "-O2 -ffp-contract=off, one thread; later steps cost more as piles form)" % (k - 1),
+                   "seconds": round(t_cpu, 2), "gpu_same_sample": round(gpu_same, 1),
+                   "gpu_over_cpu_same_sample": round(gpu_same / (n_dyn * k / t_cpu), 1),
+                   "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+            # the reference's own Release flags (-O3 -ffast-math) and cpHastySpace, same sample
+            t_rel = run_cpu_reference(pw, k, dt, release=True)
+            t_hasty = run_cpu_reference(pw, k, dt, hasty_threads=os.cpu_count() or 1)
+            cpu["also"] = {
+                "reference_release_1_core": round(n_dyn * k / t_rel, 1),
+                "hasty_space": {"value": round(n_dyn * k / t_hasty, 1), "threads_requested": os.cpu_count(),
+                                "note": "cpHastySpace clamps its solver to 2 threads (cpHastySpace.c:387,534)"},
+            }
+    elif not args.no_cpu_baseline and args.workload in ("batch_4096", "chains_2048"):
+        k = min(args.cpu_batch_sample_steps, args.steps)
+        threads = min(16, os.cpu_count() or 1)
+        t_cpu = run_cpu_reference_batch(pw, k, dt, threads)
+        if t_cpu is not None:
+            gpu_same = n_dyn * k / t_steps[k - 1]
+            cpu = {"value": round(n_dyn * k / t_cpu, 1), "unit": "body-steps/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": "steps 0-%d of the same rollout, every world, the reference's cpSpaceStep (strict "
+                             "build) with the worlds split over %d host threads, joined every step" % (k - 1, threads),
                    "seconds": round(t_cpu, 2), "gpu_same_sample": round(gpu_same, 1),
                    "gpu_over_cpu_same_sample": round(gpu_same / (n_dyn * k / t_cpu), 1),
                    "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}

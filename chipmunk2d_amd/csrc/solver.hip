@@ -616,33 +616,76 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	const int n_colours = (sub % COLOUR_SUBROUNDS == 0) ? c : c + 1;
 
 	// ---- colour sort ----
+	// (on the grid each workgroup counts its arbiters per colour in LDS and
+	// reserves its ranges with one atomic per colour; positions inside a
+	// colour do not matter, its arbiters share no dynamic body)
 	unsigned long long n_contacts = 0;
-	for (int k = tid; k < n; k += stride) {
-		const int a = arb_index(k);
-		const int col = S.colour[a];
-		if (col >= 0) atomicAdd(&T.col_off[col], 1);
-		n_contacts += S.act[a].count;
-	}
-	wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
-	sy.sync();
-	if (tid == 0) {
-		int run = 0;
-		for (int q = 0; q < n_colours; q++) {
-			const int cnt = T.col_off[q];
-			T.col_off[q] = run;
-			T.col_cursor[q] = run;
-			run += cnt;
+	if constexpr (Sync::grid) {
+		__shared__ int32_t lcount[CPGPU_MAX_COLOURS], lbase[CPGPU_MAX_COLOURS];
+		for (int q = threadIdx.x; q < CPGPU_MAX_COLOURS; q += blockDim.x) lcount[q] = 0;
+		__syncthreads();
+		for (int k = tid; k < n; k += stride) {
+			const int a = arb_index(k);
+			const int col = S.colour[a];
+			if (col >= 0) atomicAdd(&lcount[col], 1);
+			n_contacts += S.act[a].count;
 		}
-		T.col_off[n_colours] = run;
-		if (T.n_colours_out) *T.n_colours_out = n_colours;
-	}
-	sy.sync();
-	for (int k = tid; k < n; k += stride) {
-		const int a = arb_index(k);
-		const int col = S.colour[a];
-		if (col < 0) continue;
-		const int pos = atomicAdd(&T.col_cursor[col], 1);
-		T.order[pos] = a;
+		wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
+		__syncthreads();
+		for (int q = threadIdx.x; q < n_colours; q += blockDim.x)
+			if (lcount[q]) atomicAdd(&T.col_off[q], lcount[q]);
+		sy.sync();
+		if (tid == 0) {
+			int run = 0;
+			for (int q = 0; q < n_colours; q++) {
+				const int cnt = T.col_off[q];
+				T.col_off[q] = run;
+				T.col_cursor[q] = run;
+				run += cnt;
+			}
+			T.col_off[n_colours] = run;
+			if (T.n_colours_out) *T.n_colours_out = n_colours;
+		}
+		sy.sync();
+		for (int q = threadIdx.x; q < n_colours; q += blockDim.x) {
+			lbase[q] = lcount[q] ? atomicAdd(&T.col_cursor[q], lcount[q]) : 0;
+			lcount[q] = 0;
+		}
+		__syncthreads();
+		for (int k = tid; k < n; k += stride) {
+			const int a = arb_index(k);
+			const int col = S.colour[a];
+			if (col < 0) continue;
+			T.order[lbase[col] + atomicAdd(&lcount[col], 1)] = a;
+		}
+	} else {
+		for (int k = tid; k < n; k += stride) {
+			const int a = arb_index(k);
+			const int col = S.colour[a];
+			if (col >= 0) atomicAdd(&T.col_off[col], 1);
+			n_contacts += S.act[a].count;
+		}
+		wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
+		sy.sync();
+		if (tid == 0) {
+			int run = 0;
+			for (int q = 0; q < n_colours; q++) {
+				const int cnt = T.col_off[q];
+				T.col_off[q] = run;
+				T.col_cursor[q] = run;
+				run += cnt;
+			}
+			T.col_off[n_colours] = run;
+			if (T.n_colours_out) *T.n_colours_out = n_colours;
+		}
+		sy.sync();
+		for (int k = tid; k < n; k += stride) {
+			const int a = arb_index(k);
+			const int col = S.colour[a];
+			if (col < 0) continue;
+			const int pos = atomicAdd(&T.col_cursor[col], 1);
+			T.order[pos] = a;
+		}
 	}
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----
 	if (T.prestep)
@@ -769,14 +812,15 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	T.col_cursor = S.col_cursor;
 	T.left = S.left;
 	T.n_colours_out = S.n_colours;
+	// pre-steps (and, without joints, the velocity integration) ran before, on the whole device
 	T.bodies = S.large_bodies;
 	T.body0 = 0;
-	T.nb = S.n_large_bodies;
+	T.nb = S.n_large_joints > 0 ? S.n_large_bodies : 0;
 	T.jorder = S.jorder;
 	T.jcol_off = S.jcol_off;
 	T.njcol = S.n_jcolours;
 	T.iterations = S.max_iterations;
-	T.prestep = true;
+	T.prestep = false;
 	T.island_filter = true;
 	T.lds_bodies = nullptr;
 	T.lds_cap = 0;
@@ -911,8 +955,9 @@ __global__ void k_island_count(const DevWorld W, const SolveBuffers S)
 	}
 }
 
-// Sort keys of the solved arbiters; the pre-steps of the workgroup-solved
-// islands' arbiters (they only read velocities); OR / AND of the keys for the
+// Sort keys of the solved arbiters; the pre-steps of the large worlds'
+// arbiters (they only read velocities, before the joints' pre-steps and the
+// velocity integration, as in cpSpaceStep); OR / AND of the keys for the
 // radix sort.
 __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveBuffers S, unsigned long long *orand)
 {
@@ -927,12 +972,8 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 			key = (unsigned long long)w;
 		} else {
 			const int r = island_of(W, S, arb.ba, arb.bb);
-			if (island_on_workgroup(S, r)) {
-				key = ISLAND_KEY | (unsigned long long)r;
-				arb_prestep(W, arb);
-			} else {
-				key = GRID_KEY;
-			}
+			key = island_on_workgroup(S, r) ? (ISLAND_KEY | (unsigned long long)r) : GRID_KEY;
+			arb_prestep(W, arb);
 		}
 		S.keys[k] = key;
 		S.arb_by_world[k] = k;
@@ -953,14 +994,15 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 	}
 }
 
-// velocity integration of the bodies not solved on the grid (after the
-// pre-steps above, before any impulse)
+// velocity integration of the large worlds' bodies that no joint pre-step
+// touches first: all of them without joints, else those of the workgroup-solved
+// islands (after the pre-steps above, before any impulse)
 __global__ void k_island_integrate(const DevWorld W, const SolveBuffers S)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= S.n_large_bodies) return;
 	const int b = S.large_bodies[q];
-	if (island_on_workgroup(S, S.label[b])) integrate_velocity(W, b);
+	if (S.n_large_joints == 0 || island_on_workgroup(S, S.label[b])) integrate_velocity(W, b);
 }
 
 // warb_off[w] = first sorted position of small world w (n_worlds + 1 entries);

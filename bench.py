@@ -7,6 +7,7 @@ polygons, 10k static segments (container + sinusoidal terrain), seeded,
 from its initial state.  A "step" is one cpSpaceStep of the whole world.
 
     python bench.py [--steps K] [--warmup W] [--workload random_world_100k|batch_4096|chains_2048|grad_1024]
+                    [--precision fp64|fp32]
 
 W warm-up steps run on a separate copy of the world (first-launch costs),
 then exactly K steps are timed on a fresh copy, so the timed region is
@@ -25,6 +26,11 @@ touches, FP64) over its average launch time from HIP events recorded on
 the library's stream during the timed region, against 8 TB/s HBM.
 cpu_baseline: the reference's own cpSpaceStep (compiled from its sources,
 strict IEEE build, one thread) on the first steps of the same rollout.
+
+--precision fp32 steps CPGPU_FP32 worlds (float state and arithmetic,
+4-byte scalars in the byte model) and adds a "drift" object: the FP32
+state against the FP64 step (bit-for-bit the oracle) after 1, 10 and 60
+steps of the same workload, measured outside the timed region.
 """
 import argparse
 import json
@@ -40,7 +46,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "body-steps/s @10 iters: 100k-shape world & 4096-world batch; fwd+adjoint steps/s"
 HBM_PEAK_GBS = 8000.0
-S = 8  # bytes per FP64 scalar
+S = 8  # bytes per scalar: 8 for FP64 worlds, 4 for FP32 (set from --precision)
 
 
 def build_workload(name):
@@ -104,10 +110,11 @@ def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations, 
     return None
 
 
-def measured_traffic(workload, kernel):
+def measured_traffic(workload, kernel, precision="fp64"):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
-    (profiles/traffic_<workload>.txt, calibrated on a device copy), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % workload)
+    (profiles/traffic_<workload>.txt, or traffic_<workload>_fp32.txt for FP32 worlds; calibrated on a device
+    copy), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s%s.txt" % (workload, "" if precision == "fp64" else "_fp32"))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -116,13 +123,13 @@ def measured_traffic(workload, kernel):
     return None
 
 
-def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60):
+def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60, precision=0):
     """Warm-up (and, for batches, the autotuner, once the copy has run `autotune_at` steps so that piles have
     formed) on a throwaway copy of the world, then `steps` timed steps on a fresh copy with the tuned tiling."""
     from chipmunk2d_amd.gpu import GPUWorld
     tiling = None
     if warmup > 0 or autotune_steps > 0:
-        g = GPUWorld(pw)
+        g = GPUWorld(pw, precision)
         if warmup > 0:
             g.step(dt, warmup)
         if autotune_steps > 0:
@@ -131,7 +138,7 @@ def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60):
         g.sync()
         tiling = g.buckets()
         g.close()
-    g = GPUWorld(pw)
+    g = GPUWorld(pw, precision)
     if autotune_steps > 0:
         g.set_buckets([b[0] for b in tiling], [b[1] for b in tiling])
         tiling = g.buckets()
@@ -232,7 +239,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune-steps", type=int, default=3,
                     help="steps per trial of the batch tiling autotuner (on the warm-up copy; 0: default tiling)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
+                    help="world precision (CPGPU_FP64: bit-for-bit the reference's double build; CPGPU_FP32)")
     args = ap.parse_args()
+    global S
+    S = 8 if args.precision == "fp64" else 4
+    from chipmunk2d_amd.cabi import FP32, FP64
+    precision = FP64 if args.precision == "fp64" else FP32
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would step its own "
               "slice of worlds independently); run with --gpus 1", file=sys.stderr)
@@ -251,6 +264,9 @@ def main():
     autotune = args.autotune_steps if args.workload in ("batch_4096", "chains_2048") else 0
     grad_extra = None
     if args.workload == "grad_1024":
+        if args.precision != "fp64":
+            print("the adjoint runs on FP64 worlds only", file=sys.stderr)
+            sys.exit(2)
         t_fwd, t_bwd, timing, stats, solve_counts, loss = run_grad(pw, args.steps, args.warmup, dt)
         wall = t_fwd + t_bwd
         t_steps = None
@@ -259,7 +275,8 @@ def main():
                       "fwd_adjoint_steps_per_s": round(args.steps / wall, 2),
                       "forward_only_body_steps_per_s": round(n_dyn * args.steps / t_fwd, 1)}
     else:
-        wall, t_steps, timing, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt, autotune)
+        wall, t_steps, timing, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt, autotune,
+                                                                      precision=precision)
     if autotune:
         config["tiling"] = {"autotuned": True, "trial_steps": autotune, "tuned_after_steps": 60,
                             "buckets": [{"max_bodies": b[0], "threads": b[1], "worlds": b[2], "lds_bodies": b[3]}
@@ -281,7 +298,7 @@ def main():
     if dominant and "GBps" in kernels.get(dominant, {}):
         ach = kernels[dominant]["GBps"]
         roofline = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dominant),
+                    "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dominant, args.precision),
                     "bytes_per_launch": kernels[dominant]["bytes_per_launch"],
                     "avg_launch_us": kernels[dominant]["avg_us"]}
 
@@ -319,6 +336,8 @@ def main():
                    "gpu_over_cpu_same_sample": round(gpu_same / (n_dyn * k / t_cpu), 1),
                    "host": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
 
+    if cpu is not None:
+        cpu["precision"] = "f64 (the reference's default CP_USE_DOUBLES build)"
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -330,7 +349,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded scene generator, xoshiro256**)",
         "config": config,
         "contacts_per_s": round(stats["total_contacts"] / wall, 1),
@@ -343,6 +362,10 @@ def main():
     if grad_extra:
         out["metric_part"] = "fwd+adjoint"
         out.update(grad_extra)
+    if args.precision == "fp32":
+        from chipmunk2d_amd.drift import drift_report
+        out["drift"] = {"against": "FP64 step of the same workload (bit-for-bit the oracle)",
+                        "at_steps": drift_report(pw, dt, checkpoints=(1, 10, 60))}
     print(json.dumps(out))
 
 

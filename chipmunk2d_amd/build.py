@@ -2,8 +2,12 @@
 
 Every translation unit is compiled with -ffp-contract=off (one IEEE
 operation per + - * /, as in the reference's strict build) and with packed
-FP32 instructions disabled.  Run as `python -m chipmunk2d_amd.build` or via
-__graft_entry__.build().
+FP32 instructions disabled.  The step sources are compiled twice: once with
+double arithmetic into namespace cpgpu64 (CPGPU_FP64 worlds, bit-for-bit the
+reference's double build) and once with float into cpgpu32 (CPGPU_FP32
+worlds); dispatch.hip holds the extern "C" entry points, which forward each
+call to the build the world was created with.  Run as
+`python -m chipmunk2d_amd.build` or via __graft_entry__.build().
 """
 import os
 import subprocess
@@ -14,6 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcpgpu.so")
 SOURCES = ["world.hip", "broadphase.hip", "narrowphase.hip", "solver.hip", "adjoint.hip"]
+VARIANTS = {"64": ["-DCPGPU_NS=cpgpu64", "-DCPGPU_REAL=double"],
+            "32": ["-DCPGPU_NS=cpgpu32", "-DCPGPU_REAL=float"]}
+UNITS = [(src, v) for v in VARIANTS for src in SOURCES] + [("dispatch.hip", None)]
 ARCH = os.environ.get("CPGPU_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -27,22 +34,24 @@ HIPCC_FLAGS = [
 ]
 
 
-def _obj(src):
-    return os.path.join(CSRC, "_build", os.path.splitext(src)[0] + ".o")
+def _obj(unit):
+    src, v = unit
+    base = os.path.splitext(src)[0]
+    return os.path.join(CSRC, "_build", base + ("_fp" + v if v else "") + ".o")
 
 
-def _compile(src):
-    obj = _obj(src)
-    cmd = ["hipcc"] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _compile(unit):
+    src, v = unit
+    cmd = ["hipcc"] + HIPCC_FLAGS + (VARIANTS[v] if v else []) + ["-c", os.path.join(CSRC, src), "-o", _obj(unit)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     msgs = "\n".join(line for line in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in line)
-    return src, r.returncode, msgs
+    return "%s%s" % (src, " (fp%s)" % v if v else ""), r.returncode, msgs
 
 
 def build(verbose=True):
     os.makedirs(os.path.join(CSRC, "_build"), exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+    with ThreadPoolExecutor(max_workers=min(len(UNITS), os.cpu_count() or 4)) as ex:
+        results = list(ex.map(_compile, UNITS))
     failed = False
     for src, rc, msgs in results:
         if msgs.strip() and verbose:
@@ -52,7 +61,7 @@ def build(verbose=True):
     if failed:
         raise RuntimeError("hipcc failed; see messages above")
     tmp = OUT + ".tmp"
-    cmd = ["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [_obj(s) for s in SOURCES]
+    cmd = ["hipcc", "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [_obj(u) for u in UNITS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)

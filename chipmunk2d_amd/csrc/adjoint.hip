@@ -25,6 +25,8 @@
 #include "cpgpu_dual.h"
 #include "narrowphase.h"
 
+namespace CPGPU_NS {
+
 namespace {
 
 constexpr int ADJ_THREADS = 256;
@@ -104,7 +106,7 @@ struct ArbAdj {
 	double jn[2], jt[2], jb[2];
 };
 
-__device__ __forceinline__ BodyV bodyv_from(const double *r)
+__device__ __forceinline__ BodyV bodyv_from(const real *r)
 {
 	BodyV b;
 	b.v = v2(r[0], r[1]); b.w = r[2]; b.vb = v2(r[3], r[4]); b.wb = r[5];
@@ -271,7 +273,7 @@ __device__ XFT<Dual> dual_transform(const DevWorld &W, const BodyState *start, i
 		dsn.d[k0 + 2] = cs;
 		dcs.d[k0 + 2] = -sn;
 	}
-	const double2 c = W.b_cog[b];
+	const real2 c = W.b_cog[b];
 	T.a = dcs; T.b = dsn; T.c = -dsn; T.d = dcs;
 	T.tx = px - (c.x * dcs - c.y * dsn);
 	T.ty = py - (c.x * dsn + c.y * dcs);
@@ -290,7 +292,7 @@ __device__ bool dual_shape(const DevWorld &W, int s, const XFT<Dual> &T, DualSha
 	v.count = 0;
 	v.wv = out.wv; v.wn = out.wn;
 	const double r = v.r;
-	const double *g = W.s_geom + 10 * (size_t)s;
+	const real *g = W.s_geom + 10 * (size_t)s;
 	if (v.type == CPGPU_SHAPE_CIRCLE) {
 		v.tc = xpoint(T, v2(g[0], g[1]));
 		const double cx = v.tc.x.v, cy = v.tc.y.v;
@@ -311,7 +313,7 @@ __device__ bool dual_shape(const DevWorld &W, int s, const XFT<Dual> &T, DualSha
 		v.count = cnt;
 		double l = __builtin_inf(), rr = -__builtin_inf(), bt = __builtin_inf(), tp = -__builtin_inf();
 		for (int i = 0; i < cnt; i++) {
-			const double2 lv = W.pv_local[st + i], ln = W.pn_local[st + i];
+			const real2 lv = W.pv_local[st + i], ln = W.pn_local[st + i];
 			out.wv[i] = xpoint(T, v2(lv.x, lv.y));
 			out.wn[i] = xvect(T, v2(ln.x, ln.y));
 			const double x = out.wv[i].x.v, y = out.wv[i].y.v;
@@ -546,7 +548,7 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, 
 			}
 			// surface velocity: svr = s - n (s . n) with s the shapes' surface velocity difference
 			{
-				const double2 svA = W.s_sv[arb.sa], svB = W.s_sv[arb.sb];
+				const real2 svA = W.s_sv[arb.sa], svB = W.s_sv[arb.sb];
 				const V2 sv = v2(svB.x - svA.x, svB.y - svA.y);
 				G.n = vsub(G.n, vadd(vmult(G.svr, vdot(sv, n)), vmult(sv, vdot(n, G.svr))));
 			}
@@ -647,3 +649,5 @@ hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_world
 	k_reverse_step<<<n_worlds, ADJ_THREADS, 0, stream>>>(W, A);
 	return hipGetLastError();
 }
+
+} // namespace CPGPU_NS

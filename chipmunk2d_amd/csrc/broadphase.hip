@@ -16,6 +16,8 @@
 // (src/cpSpaceStep.c:204-232).
 #include "cpgpu_internal.h"
 
+namespace CPGPU_NS {
+
 namespace {
 
 constexpr int RS_THREADS = 256;
@@ -25,6 +27,7 @@ constexpr int MORTON_BITS = 22;
 constexpr int WORLD_SHIFT = 44;
 constexpr int TRAVERSAL_STACK = 48;
 
+// order-preserving integer image of a box coordinate (as a double in both builds)
 __device__ __forceinline__ unsigned long long ord_of(double x)
 {
 	unsigned long long u = (unsigned long long)__double_as_longlong(x);
@@ -46,7 +49,7 @@ __global__ void k_bounds_reset(unsigned long long *wb, int n_worlds)
 	wb[4 * w + 3] = 0ull;
 }
 
-__device__ __forceinline__ V2 box_centre(double4 b) { return v2(0.5 * (b.x + b.z), 0.5 * (b.y + b.w)); }
+__device__ __forceinline__ V2 box_centre(real4 b) { return v2(0.5 * (b.x + b.z), 0.5 * (b.y + b.w)); }
 
 // Per-world bounds of the box centres (order-preserving integer images of the
 // doubles, so atomic min / max give the exact extremes).  Waves and then
@@ -332,18 +335,18 @@ __global__ void k_karras(const uint64_t *keys, int n, int32_t *child, int32_t *p
 	if (first[w] == lo && last[w] == hi) root[w] = i;
 }
 
-__device__ __forceinline__ double4 box_union(double4 a, double4 b)
+__device__ __forceinline__ real4 box_union(real4 a, real4 b)
 {
-	return make_double4(fminc(a.x, b.x), fminc(a.y, b.y), fmaxc(a.z, b.z), fmaxc(a.w, b.w));
+	return make_real4(fminc(a.x, b.x), fminc(a.y, b.y), fmaxc(a.z, b.z), fmaxc(a.w, b.w));
 }
 
-__device__ __forceinline__ double4 shfl_up4(double4 v, int off)
+__device__ __forceinline__ real4 shfl_up4(real4 v, int off)
 {
-	return make_double4(__shfl_up(v.x, off), __shfl_up(v.y, off), __shfl_up(v.z, off), __shfl_up(v.w, off));
+	return make_real4(__shfl_up(v.x, off), __shfl_up(v.y, off), __shfl_up(v.z, off), __shfl_up(v.w, off));
 }
-__device__ __forceinline__ double4 shfl_down4(double4 v, int off)
+__device__ __forceinline__ real4 shfl_down4(real4 v, int off)
 {
-	return make_double4(__shfl_down(v.x, off), __shfl_down(v.y, off), __shfl_down(v.z, off), __shfl_down(v.w, off));
+	return make_real4(__shfl_down(v.x, off), __shfl_down(v.y, off), __shfl_down(v.z, off), __shfl_down(v.w, off));
 }
 
 // Box refit in two fence-free passes over chunks of REFIT_CHUNK sorted leaves.
@@ -356,26 +359,26 @@ constexpr int REFIT_CHUNK = 256;
 
 __global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, int n, const int32_t *leaf_shape,
                                                            const int32_t *child, const int32_t *parent,
-                                                           const int32_t *range, double4 *box, double4 *pre,
-                                                           double4 *suf, double4 *cbox)
+                                                           const int32_t *range, real4 *box, real4 *pre,
+                                                           real4 *suf, real4 *cbox)
 {
-	__shared__ double4 lb[REFIT_CHUNK];
-	__shared__ double4 ib[REFIT_CHUNK];
+	__shared__ real4 lb[REFIT_CHUNK];
+	__shared__ real4 ib[REFIT_CHUNK];
 	__shared__ uint32_t vis[REFIT_CHUNK];
-	__shared__ double4 wtot[REFIT_CHUNK / 64];
+	__shared__ real4 wtot[REFIT_CHUNK / 64];
 	const int t = threadIdx.x, lo = blockIdx.x * REFIT_CHUNK, i = lo + t;
 	const int hi = min(lo + REFIT_CHUNK - 1, n - 1);
 	const bool valid = i < n;
-	const double inf = __builtin_inf();
-	const double4 b = valid ? W.s_bb[leaf_shape[i]] : make_double4(inf, inf, -inf, -inf);
+	const real inf = __builtin_inf();
+	const real4 b = valid ? W.s_bb[leaf_shape[i]] : make_real4(inf, inf, -inf, -inf);
 	lb[t] = b;
 	vis[t] = 0;
 	if (valid) box[(n - 1) + i] = b;
 	const int lane = t & 63, wave = t >> 6;
-	double4 pf = b, sf = b;
+	real4 pf = b, sf = b;
 	for (int off = 1; off < 64; off <<= 1) {
-		const double4 u = shfl_up4(pf, off);
-		const double4 d = shfl_down4(sf, off);
+		const real4 u = shfl_up4(pf, off);
+		const real4 d = shfl_down4(sf, off);
 		if (lane >= off) pf = box_union(pf, u);
 		if (lane + off < 64) sf = box_union(sf, d);
 	}
@@ -394,9 +397,9 @@ __global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, i
 		if (range[2 * p] < lo || range[2 * p + 1] > hi) break;
 		if (__hip_atomic_fetch_add(&vis[p - lo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) break;
 		const int c0 = child[2 * p], c1 = child[2 * p + 1];
-		const double4 b0 = c0 >= n - 1 ? lb[c0 - (n - 1) - lo] : ib[c0 - lo];
-		const double4 b1 = c1 >= n - 1 ? lb[c1 - (n - 1) - lo] : ib[c1 - lo];
-		const double4 u = box_union(b0, b1);
+		const real4 b0 = c0 >= n - 1 ? lb[c0 - (n - 1) - lo] : ib[c0 - lo];
+		const real4 b1 = c1 >= n - 1 ? lb[c1 - (n - 1) - lo] : ib[c1 - lo];
+		const real4 u = box_union(b0, b1);
 		ib[p - lo] = u;
 		box[p] = u;
 		p = parent[p];
@@ -406,8 +409,8 @@ __global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, i
 // Pass 2: a node spanning chunks gets the union of the suffix of its first
 // chunk, the whole chunks between, and the prefix of its last chunk.  Nodes
 // above the world roots are never read and are skipped.
-__global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, const double4 *pre,
-                             const double4 *suf, const double4 *cbox, double4 *box)
+__global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, const real4 *pre,
+                             const real4 *suf, const real4 *cbox, real4 *box)
 {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= n - 1) return;
@@ -415,14 +418,14 @@ __global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, 
 	const int cl = lo / REFIT_CHUNK, ch = hi / REFIT_CHUNK;
 	if (cl == ch) return;
 	if ((keys[lo] >> WORLD_SHIFT) != (keys[hi] >> WORLD_SHIFT)) return;
-	double4 u = box_union(suf[lo], pre[hi]);
+	real4 u = box_union(suf[lo], pre[hi]);
 	for (int c = cl + 1; c < ch; c++) u = box_union(u, cbox[c]);
 	box[p] = u;
 }
 
 // ---- traversal ----
 
-__device__ __forceinline__ bool box_overlap(double4 a, double4 b)
+__device__ __forceinline__ bool box_overlap(real4 a, real4 b)
 {
 	return a.x <= b.z && b.x <= a.z && a.y <= b.w && b.y <= a.w;
 }
@@ -461,15 +464,15 @@ __device__ __forceinline__ void emit_pair(const DevWorld &W, int s, int t, int2 
 
 __global__ void __launch_bounds__(256) k_pairs(const DevWorld W, int nd, const uint64_t *dkeys,
                                                const int32_t *dshape, const int32_t *dchild,
-                                               const int32_t *drange, const double4 *dbox, const int32_t *droot,
+                                               const int32_t *drange, const real4 *dbox, const int32_t *droot,
                                                int ns, const int32_t *sshape, const int32_t *schild,
-                                               const double4 *sbox, const int32_t *sroot, int2 *pairs, int cap,
+                                               const real4 *sbox, const int32_t *sroot, int2 *pairs, int cap,
                                                int32_t *count)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= nd) return;
 	const int s = dshape[q];
-	const double4 bb = W.s_bb[s];
+	const real4 bb = W.s_bb[s];
 	const int w = (int)(dkeys[q] >> WORLD_SHIFT);
 	// per-lane stack in LDS, laid out [depth][lane] so lanes at one depth hit distinct banks
 	__shared__ int32_t stk[TRAVERSAL_STACK][256];
@@ -598,10 +601,10 @@ void lbvh_reserve(Lbvh &t, int n, int n_worlds)
 	(void)hipMalloc(&t.child, sizeof(int32_t) * 2 * cap);
 	(void)hipMalloc(&t.parent, sizeof(int32_t) * 2 * cap);
 	(void)hipMalloc(&t.range, sizeof(int32_t) * 2 * cap);
-	(void)hipMalloc(&t.box, sizeof(double4) * 2 * cap);
-	(void)hipMalloc(&t.pre, sizeof(double4) * cap);
-	(void)hipMalloc(&t.suf, sizeof(double4) * cap);
-	(void)hipMalloc(&t.cbox, sizeof(double4) * (cap / REFIT_CHUNK + 1));
+	(void)hipMalloc(&t.box, sizeof(real4) * 2 * cap);
+	(void)hipMalloc(&t.pre, sizeof(real4) * cap);
+	(void)hipMalloc(&t.suf, sizeof(real4) * cap);
+	(void)hipMalloc(&t.cbox, sizeof(real4) * (cap / REFIT_CHUNK + 1));
 	(void)hipMalloc(&t.world_root, sizeof(int32_t) * n_worlds);
 	(void)hipMalloc(&t.world_first, sizeof(int32_t) * n_worlds);
 	(void)hipMalloc(&t.world_last, sizeof(int32_t) * n_worlds);
@@ -617,7 +620,7 @@ void lbvh_free(Lbvh &t)
 }
 
 hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, int n_worlds,
-                      RadixSortBuffers &sortbuf, double4 *world_bounds, hipStream_t stream, KernelTimer *timer)
+                      RadixSortBuffers &sortbuf, real4 *world_bounds, hipStream_t stream, KernelTimer *timer)
 {
 	t.n = n;
 	HIP_TRY(hipMemsetAsync(t.world_root, 0xFF, sizeof(int32_t) * n_worlds, stream));
@@ -661,3 +664,5 @@ hipError_t broadphase_pairs(const Lbvh &dyn, const Lbvh &stat, const DevWorld &W
 	});
 	return hipGetLastError();
 }
+
+} // namespace CPGPU_NS

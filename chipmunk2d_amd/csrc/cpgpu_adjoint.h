@@ -3,13 +3,15 @@
 
 #include "cpgpu_step.h"
 
+namespace CPGPU_NS {
+
 // One body's state at the start of a step (checkpoints, segment replay).
 struct BodyState {
-	double2 p;
-	double a;
+	real2 p;
+	real a;
 	BodyVel vel;
-	double2 f;
-	double t;
+	real2 f;
+	real t;
 	XF T;
 };
 
@@ -43,3 +45,5 @@ hipError_t adjoint_state_load(const DevWorld &W, const BodyState *in, hipStream_
 hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream);
 // bytes of the per-arbiter adjoint scratch (AdjStep::garb_raw) per solved arbiter
 size_t adjoint_arb_scratch_bytes();
+
+} // namespace CPGPU_NS

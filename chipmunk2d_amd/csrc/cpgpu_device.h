@@ -1,14 +1,47 @@
 // cpgpu_device.h -- device-side math and state layout of the gfx950 step library.
 //
 // All kernels are compiled with -ffp-contract=off: every + - * / sqrt below is
-// one IEEE double operation, the same sequence the reference performs, so
-// results are bit-identical to the reference's strict build.
+// one IEEE operation of the build's real type.  In the FP64 build (real =
+// double) that is the sequence the reference performs, so results are
+// bit-identical to the reference's strict build; the FP32 build (real =
+// float) runs the same sequence in single precision.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/cpgpu.h"
+
+namespace CPGPU_NS {
+
+// The arithmetic type of the device state and the step.
+typedef CPGPU_REAL real;
+typedef HIP_vector_type<real, 2> real2;
+typedef HIP_vector_type<real, 4> real4;
+__host__ __device__ inline real2 make_real2(real x, real y)
+{
+	real2 r;
+	r.x = x; r.y = y;
+	return r;
+}
+__host__ __device__ inline real4 make_real4(real x, real y, real z, real w)
+{
+	real4 r;
+	r.x = x; r.y = y; r.z = z; r.w = w;
+	return r;
+}
+// smallest positive normal value (CPFLOAT_MIN, chipmunk_types.h)
+#define CP_REAL_MIN (sizeof(real) == 4 ? (real)1.17549435e-38f : (real)2.2250738585072014e-308)
+
+// The library is compiled twice: the FP64 step (namespace cpgpu64, real =
+// double) and the FP32 step (namespace cpgpu32, real = float); dispatch.hip
+// routes each C ABI call to the world's own build.
+#ifndef CPGPU_NS
+#define CPGPU_NS cpgpu64
+#endif
+#ifndef CPGPU_REAL
+#define CPGPU_REAL double
+#endif
 
 #define CPD __device__ __forceinline__
 
@@ -19,13 +52,14 @@
 // (cpgpu_dual.h) where the adjoint needs contact-geometry derivatives.  The
 // value part of every operation is the same IEEE double operation.
 template <class T> struct V2T { T x, y; };
-using V2 = V2T<double>;
+using V2 = V2T<real>;
 
 CPD double csqrt(double x) { return __builtin_sqrt(x); }
+CPD float csqrt(float x) { return __builtin_sqrtf(x); }
 
-CPD V2 v2(double x, double y) { V2 r; r.x = x; r.y = y; return r; }
+CPD V2 v2(real x, real y) { V2 r; r.x = x; r.y = y; return r; }
 template <class T> CPD V2T<T> v2t(T x, T y) { V2T<T> r; r.x = x; r.y = y; return r; }
-template <class T> CPD V2T<T> vcast(V2 a) { return v2t<T>(T(a.x), T(a.y)); }
+template <class T, class S> CPD V2T<T> vcast(V2T<S> a) { return v2t<T>(T(a.x), T(a.y)); }
 template <class T> CPD V2T<T> vadd(V2T<T> a, V2T<T> b) { return v2t<T>(a.x + b.x, a.y + b.y); }
 template <class T> CPD V2T<T> vsub(V2T<T> a, V2T<T> b) { return v2t<T>(a.x - b.x, a.y - b.y); }
 template <class T> CPD V2T<T> vneg(V2T<T> a) { return v2t<T>(-a.x, -a.y); }
@@ -40,11 +74,10 @@ template <class T> CPD V2T<T> vrotate(V2T<T> a, V2T<T> b)
 }
 template <class T> CPD T vlengthsq(V2T<T> a) { return vdot(a, a); }
 template <class T> CPD T vlength(V2T<T> a) { return csqrt(vdot(a, a)); }
-#define CP_DBL_MIN 2.2250738585072014e-308
-template <class T> CPD V2T<T> vnormalize(V2T<T> a) { return vmult(a, 1.0 / (vlength(a) + CP_DBL_MIN)); }
+template <class T> CPD V2T<T> vnormalize(V2T<T> a) { return vmult(a, T(1.0) / (vlength(a) + CP_REAL_MIN)); }
 template <class T, class S> CPD V2T<T> vlerp(V2T<T> a, V2T<T> b, S t)
 {
-	return vadd(vmult(a, 1.0 - t), vmult(b, t));
+	return vadd(vmult(a, S(1.0) - t), vmult(b, t));
 }
 template <class T> CPD bool veql(V2T<T> a, V2T<T> b) { return a.x == b.x && a.y == b.y; }
 template <class T, class S> CPD V2T<T> vclamp(V2T<T> v, S len)
@@ -58,12 +91,15 @@ template <class T> CPD T fclamp01(T f) { return fmaxc(T(0.0), fminc(f, T(1.0)));
 
 // cpTransform {a, b, c, d, tx, ty} (cpTransform.h)
 template <class T> struct XFT { T a, b, c, d, tx, ty; };
-using XF = XFT<double>;
-template <class T> CPD V2T<T> xpoint(const XFT<T> &t, V2 p)
+using XF = XFT<real>;
+template <class T, class S> CPD V2T<T> xpoint(const XFT<T> &t, V2T<S> p)
 {
 	return v2t<T>(t.a * p.x + t.c * p.y + t.tx, t.b * p.x + t.d * p.y + t.ty);
 }
-template <class T> CPD V2T<T> xvect(const XFT<T> &t, V2 v) { return v2t<T>(t.a * v.x + t.c * v.y, t.b * v.x + t.d * v.y); }
+template <class T, class S> CPD V2T<T> xvect(const XFT<T> &t, V2T<S> v)
+{
+	return v2t<T>(t.a * v.x + t.c * v.y, t.b * v.x + t.d * v.y);
+}
 
 // CP_HASH_PAIR (chipmunk_private.h:28-29), 64-bit cpHashValue
 #define CP_HASH_COEF 3344921057ull
@@ -152,8 +188,8 @@ CPD double cp_exp(double x)
 
 // Per-world step parameters, refreshed by the host every step.
 struct WorldStep {
-	double dt, dt_coef, damping, bias_coef, slop;
-	double gx, gy;
+	real dt, dt_coef, damping, bias_coef, slop;
+	real gx, gy;
 	int32_t iterations;
 	uint32_t stamp;
 	uint32_t persistence;
@@ -162,13 +198,13 @@ struct WorldStep {
 
 // Velocity record of a body: the solver's working set (64 bytes).
 struct BodyVel {
-	double vx, vy, w, vbx, vby, wb, m_inv, i_inv;
+	real vx, vy, w, vbx, vby, wb, m_inv, i_inv;
 };
 
 // A contact of an arbiter (struct cpContact, chipmunk_structs.h:101-111).
 struct Contact {
-	double r1x, r1y, r2x, r2y;
-	double nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
+	real r1x, r1y, r2x, r2y;
+	real nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
 	uint64_t hash;
 	// rollout tape (adjoint): where this contact's accumulated impulses were
 	// recorded after its last solve, and the recorded contact its warm start
@@ -185,7 +221,7 @@ struct Arbiter {
 	int32_t world;
 	uint32_t id_in;            // GJK warm-start id the collision started from
 	int32_t _pad;
-	double nx, ny, e, u, svx, svy;
+	real nx, ny, e, u, svx, svy;
 	Contact c[2];
 };
 
@@ -201,12 +237,12 @@ struct PairRec {
 // Joint (struct cpConstraint + joint subclasses), device form.
 struct Joint {
 	int32_t type, a, b, world;
-	double maxForce, errorBias, maxBias;
-	double P[8];
-	double r1x, r1y, r2x, r2y, nx, ny;
-	double k11, k12, k21, k22;
-	double bx, by, bias, nMass, jAx, jAy, jAcc, jnAcc, target_vrn, v_coef, iSum;
-	double bias_coef;
+	real maxForce, errorBias, maxBias;
+	real P[8];
+	real r1x, r1y, r2x, r2y, nx, ny;
+	real k11, k12, k21, k22;
+	real bx, by, bias, nMass, jAx, jAy, jAcc, jnAcc, target_vrn, v_coef, iSum;
+	real bias_coef;
 };
 
 struct DevWorld {
@@ -214,29 +250,29 @@ struct DevWorld {
 	// bodies
 	const int32_t *b_type;
 	const int32_t *b_world;
-	const double *b_m;           // mass (INFINITY marks static/kinematic)
-	double2 *b_p;                // position
-	double *b_a;                 // angle
-	double2 *b_f;
-	double *b_t;
-	double2 *b_cog;
+	const real *b_m;             // mass (INFINITY marks static/kinematic)
+	real2 *b_p;                  // position
+	real *b_a;                   // angle
+	real2 *b_f;
+	real *b_t;
+	real2 *b_cog;
 	XF *b_T;                     // transform
 	BodyVel *b_vel;
 	// shapes
 	const int32_t *s_type, *s_body, *s_world;
 	const uint64_t *s_hashid;
-	const double *s_e, *s_u;
-	const double2 *s_sv;
+	const real *s_e, *s_u;
+	const real2 *s_sv;
 	const uint64_t *s_group;
 	const uint32_t *s_cats, *s_mask;
-	const double *s_r;
-	const double *s_geom;        // [n][10] body-local geometry
+	const real *s_r;
+	const real *s_geom;          // [n][10] body-local geometry
 	const int32_t *s_pstart, *s_pcount;
-	const double2 *pv_local, *pn_local;
-	double2 *pv_world, *pn_world;
-	double *s_wgeom;             // [n][6]: circle tc; segment ta, tb, tn
-	double4 *s_bb;               // l, b, r, t
-	double4 *s_fat;              // dynamic-tree leaf boxes
+	const real2 *pv_local, *pn_local;
+	real2 *pv_world, *pn_world;
+	real *s_wgeom;               // [n][6]: circle tc; segment ta, tb, tn
+	real4 *s_bb;                 // l, b, r, t
+	real4 *s_fat;                // dynamic-tree leaf boxes
 	uint32_t *s_rebuild;         // step of the last leaf rebuild
 	const uint8_t *s_static;     // shape belongs to a static body
 	// no-collide constraint partners (CSR over bodies)
@@ -265,3 +301,5 @@ struct DevWorld {
 #define CPGPU_ERR_ADJOINT 256
 
 CPD void raise_err(int32_t *err, int32_t code) { atomicOr(err, code); }
+
+} // namespace CPGPU_NS

@@ -11,6 +11,8 @@
 
 #include "cpgpu_device.h"
 
+namespace CPGPU_NS {
+
 #define DUAL_N 6
 
 struct Dual {
@@ -135,3 +137,5 @@ CPD Dual csqrt(const Dual &a)
 
 CPD double dval(double x) { return x; }
 CPD double dval(const Dual &x) { return x.v; }
+
+} // namespace CPGPU_NS

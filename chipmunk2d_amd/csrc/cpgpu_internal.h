@@ -10,13 +10,15 @@
 
 #include "cpgpu_device.h"
 
+namespace CPGPU_NS {
+
 // ---- per-kernel device timing with HIP events on the launch stream ----
 struct KernelTimer {
 	bool enabled = false;
 	struct Rec { const char *name; hipEvent_t a, b; };
 	std::vector<Rec> pending;
 	std::vector<hipEvent_t> pool;
-	std::map<std::string, std::pair<double, long>> acc;   // name -> (ms, launches)
+	std::map<std::string, std::pair<real, long>> acc;   // name -> (ms, launches)
 	hipEvent_t take() {
 		if (pool.empty()) { hipEvent_t e; (void)hipEventCreate(&e); return e; }
 		hipEvent_t e = pool.back(); pool.pop_back(); return e;
@@ -80,10 +82,10 @@ struct Lbvh {
 	int32_t *child = nullptr;       // [n-1][2] children (>= n-1: leaf n-1+i)
 	int32_t *parent = nullptr;      // [2n-1]
 	int32_t *range = nullptr;       // [n-1][2] leaf range of internal nodes
-	double4 *box = nullptr;         // [2n-1]
-	double4 *pre = nullptr;         // [n] refit: union of the chunk's leaf boxes up to each leaf
-	double4 *suf = nullptr;         // [n] ... and from each leaf to the chunk's end
-	double4 *cbox = nullptr;        // [n / chunk + 1] union of each chunk's leaf boxes
+	real4 *box = nullptr;         // [2n-1]
+	real4 *pre = nullptr;         // [n] refit: union of the chunk's leaf boxes up to each leaf
+	real4 *suf = nullptr;         // [n] ... and from each leaf to the chunk's end
+	real4 *cbox = nullptr;        // [n / chunk + 1] union of each chunk's leaf boxes
 	int32_t *world_root = nullptr;  // [n_worlds] root node per world (-1: none)
 	int32_t *world_first = nullptr; // [n_worlds]
 	int32_t *world_last = nullptr;  // [n_worlds]
@@ -93,7 +95,7 @@ void lbvh_free(Lbvh &t);
 // Build over the shapes in `shapes` (n entries): keys from bb centres
 // quantised in per-world bounds, sorted, Karras hierarchy, boxes refitted.
 hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, int n_worlds,
-                      RadixSortBuffers &sortbuf, double4 *world_bounds, hipStream_t stream, KernelTimer *timer);
+                      RadixSortBuffers &sortbuf, real4 *world_bounds, hipStream_t stream, KernelTimer *timer);
 
 // Candidate pairs of the exact QueryReject predicate: dynamic x dynamic
 // (each pair once) and dynamic x static.  Pairs are written oriented for
@@ -103,3 +105,5 @@ hipError_t broadphase_pairs(const Lbvh &dyn, const Lbvh &stat, const DevWorld &W
                             int32_t *count, hipStream_t stream, KernelTimer *timer);
 
 #define HIP_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
+
+} // namespace CPGPU_NS

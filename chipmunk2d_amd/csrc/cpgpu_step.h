@@ -3,6 +3,8 @@
 
 #include "cpgpu_internal.h"
 
+namespace CPGPU_NS {
+
 // Buffers of the persistence table and arbiter arrays for one step.  "prev"
 // arrays are last step's output; the others are written this step.
 struct StepBuffers {
@@ -55,8 +57,8 @@ struct GridBarrier {
 // tape): both bodies' velocities and bias velocities, and each contact's
 // accumulated impulses before the application.
 struct ArbRec {
-	double a[6], b[6];         // v.x, v.y, w, v_bias.x, v_bias.y, w_bias
-	double acc[2][3];          // jnAcc, jtAcc, jBias
+	real a[6], b[6];         // v.x, v.y, w, v_bias.x, v_bias.y, w_bias
+	real acc[2][3];          // jnAcc, jtAcc, jBias
 };
 
 struct SolveBuffers {
@@ -142,3 +144,5 @@ hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBuck
 int solver_max_lds_bytes();
 // Largest co-resident grid of the grid solve on the current device.
 int solver_max_grid();
+
+} // namespace CPGPU_NS

@@ -1,0 +1,194 @@
+// dispatch.hip -- the C ABI of include/cpgpu.h over the library's two builds.
+//
+// Every device source is compiled twice: namespace cpgpu64 (real = double,
+// the bit-exact FP64 step) and namespace cpgpu32 (real = float, the FP32
+// mode).  A world remembers its precision in its first field; each entry
+// point forwards to the build the world was created with.
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/cpgpu.h"
+#include "cpgpu_errors.h"
+
+thread_local std::string cpgpu_last_error;
+
+#define CPGPU_DECLARE_API \
+	int cpgpuWorldCreate(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld **out); \
+	void cpgpuWorldDestroy(cpgpuWorld *world); \
+	int cpgpuWorldStep(cpgpuWorld *world, double dt); \
+	int cpgpuWorldStepN(cpgpuWorld *world, double dt, int32_t n_steps); \
+	int cpgpuWorldSync(cpgpuWorld *world); \
+	int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, double *w, double *v_bias, double *w_bias, double *transform); \
+	int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w, const double *f, const double *t); \
+	int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out); \
+	int cpgpuWorldReadArbiters(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *shapes, int32_t *colour, int32_t *state, int32_t *count, double *normal, double *contact, uint64_t *hash); \
+	int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour); \
+	int cpgpuWorldSetTiming(cpgpuWorld *world, int32_t enable); \
+	int cpgpuWorldGetTiming(cpgpuWorld *world, int32_t cap, int32_t *n_out, double *ms, int64_t *launches, char *names, int32_t names_cap); \
+	int cpgpuWorldGetBuckets(cpgpuWorld *world, int32_t *max_bodies, int32_t *threads, int32_t *n_worlds, int32_t *lds_bodies); \
+	int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies, const int32_t *threads); \
+	int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out); \
+	int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every); \
+	int cpgpuWorldTapeEnd(cpgpuWorld *world); \
+	int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v, const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0, double *d_m_inv, double *d_i_inv, double *d_u); \
+	int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodies);
+
+namespace cpgpu64 {
+struct cpgpuWorld;
+CPGPU_DECLARE_API
+} // namespace cpgpu64
+namespace cpgpu32 {
+struct cpgpuWorld;
+CPGPU_DECLARE_API
+} // namespace cpgpu32
+
+namespace {
+// both builds' world structs start with their precision (CPGPU_FP64 / CPGPU_FP32)
+inline bool is_fp32(const cpgpuWorld *w) { return *reinterpret_cast<const int32_t *>(w) == CPGPU_FP32; }
+inline int null_world()
+{
+	cpgpu_last_error = "null world";
+	return CPGPU_ERR_INVALID;
+}
+} // namespace
+
+extern "C" {
+
+int cpgpuWorldCreate(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld **out)
+{
+	if (precision == CPGPU_FP32)
+		return cpgpu32::cpgpuWorldCreate(desc, precision, reinterpret_cast<cpgpu32::cpgpuWorld **>(out));
+	return cpgpu64::cpgpuWorldCreate(desc, precision, reinterpret_cast<cpgpu64::cpgpuWorld **>(out));
+}
+
+void cpgpuWorldDestroy(cpgpuWorld *world)
+{
+	if (!world) return;
+	if (is_fp32(world)) cpgpu32::cpgpuWorldDestroy(reinterpret_cast<cpgpu32::cpgpuWorld *>(world));
+	else cpgpu64::cpgpuWorldDestroy(reinterpret_cast<cpgpu64::cpgpuWorld *>(world));
+}
+
+int cpgpuWorldStep(cpgpuWorld *world, double dt)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldStep(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), dt);
+	return cpgpu64::cpgpuWorldStep(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), dt);
+}
+
+int cpgpuWorldStepN(cpgpuWorld *world, double dt, int32_t n_steps)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldStepN(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), dt, n_steps);
+	return cpgpu64::cpgpuWorldStepN(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), dt, n_steps);
+}
+
+int cpgpuWorldSync(cpgpuWorld *world)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldSync(reinterpret_cast<cpgpu32::cpgpuWorld *>(world));
+	return cpgpu64::cpgpuWorldSync(reinterpret_cast<cpgpu64::cpgpuWorld *>(world));
+}
+
+int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, double *w, double *v_bias, double *w_bias, double *transform)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldReadBodies(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), p, v, a, w, v_bias, w_bias, transform);
+	return cpgpu64::cpgpuWorldReadBodies(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), p, v, a, w, v_bias, w_bias, transform);
+}
+
+int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w, const double *f, const double *t)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldWriteBodies(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), v, w, f, t);
+	return cpgpu64::cpgpuWorldWriteBodies(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), v, w, f, t);
+}
+
+int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetStats(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), out);
+	return cpgpu64::cpgpuWorldGetStats(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), out);
+}
+
+int cpgpuWorldReadArbiters(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *shapes, int32_t *colour, int32_t *state, int32_t *count, double *normal, double *contact, uint64_t *hash)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldReadArbiters(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), cap, n_out, shapes, colour, state, count, normal, contact, hash);
+	return cpgpu64::cpgpuWorldReadArbiters(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), cap, n_out, shapes, colour, state, count, normal, contact, hash);
+}
+
+int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldReadConstraints(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), jacc, colour);
+	return cpgpu64::cpgpuWorldReadConstraints(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), jacc, colour);
+}
+
+int cpgpuWorldSetTiming(cpgpuWorld *world, int32_t enable)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldSetTiming(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), enable);
+	return cpgpu64::cpgpuWorldSetTiming(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), enable);
+}
+
+int cpgpuWorldGetTiming(cpgpuWorld *world, int32_t cap, int32_t *n_out, double *ms, int64_t *launches, char *names, int32_t names_cap)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetTiming(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), cap, n_out, ms, launches, names, names_cap);
+	return cpgpu64::cpgpuWorldGetTiming(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), cap, n_out, ms, launches, names, names_cap);
+}
+
+int cpgpuWorldGetBuckets(cpgpuWorld *world, int32_t *max_bodies, int32_t *threads, int32_t *n_worlds, int32_t *lds_bodies)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetBuckets(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), max_bodies, threads, n_worlds, lds_bodies);
+	return cpgpu64::cpgpuWorldGetBuckets(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), max_bodies, threads, n_worlds, lds_bodies);
+}
+
+int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies, const int32_t *threads)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldSetBuckets(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), n, max_bodies, threads);
+	return cpgpu64::cpgpuWorldSetBuckets(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), n, max_bodies, threads);
+}
+
+int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldAutotune(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), dt, trial_steps, threads_out);
+	return cpgpu64::cpgpuWorldAutotune(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), dt, trial_steps, threads_out);
+}
+
+int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldTapeBegin(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), checkpoint_every);
+	return cpgpu64::cpgpuWorldTapeBegin(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), checkpoint_every);
+}
+
+int cpgpuWorldTapeEnd(cpgpuWorld *world)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldTapeEnd(reinterpret_cast<cpgpu32::cpgpuWorld *>(world));
+	return cpgpu64::cpgpuWorldTapeEnd(reinterpret_cast<cpgpu64::cpgpuWorld *>(world));
+}
+
+int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v, const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0, double *d_m_inv, double *d_i_inv, double *d_u)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldBackward(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), g_p, g_a, g_v, g_w, d_p0, d_a0, d_v0, d_w0, d_m_inv, d_i_inv, d_u);
+	return cpgpu64::cpgpuWorldBackward(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), g_p, g_a, g_v, g_w, d_p0, d_a0, d_v0, d_w0, d_m_inv, d_i_inv, d_u);
+}
+
+int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodies)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetSolveCounts(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), contacts, bodies);
+	return cpgpu64::cpgpuWorldGetSolveCounts(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), contacts, bodies);
+}
+
+const char *cpgpuGetLastError(void) { return cpgpu_last_error.c_str(); }
+const char *cpgpuVersion(void) { return "cpgpu 0.2 (gfx950; Chipmunk2D 7.0.3 cpSpaceStep semantics; FP64 and FP32 builds)"; }
+
+} // extern "C"

@@ -9,7 +9,7 @@
 // re-deriving a point from its id (the id determines the support vertices,
 // so the recomputed point is bit-identical).
 //
-// Every routine is generic over the scalar T: double for the step, and the
+// Every routine is generic over the scalar T: the build's real for the step, and the
 // forward-mode Dual of cpgpu_dual.h for the adjoint, which re-runs a pair's
 // collision with the step's GJK id to differentiate its contacts with
 // respect to the two bodies' positions and angles (same branches, so the
@@ -18,6 +18,8 @@
 
 #include "cpgpu_device.h"
 
+namespace CPGPU_NS {
+
 #define NP_MAX_GJK_ITERATIONS 30
 #define NP_MAX_EPA_ITERATIONS 30
 #define NP_EPA_HULL_MAX 36
@@ -25,16 +27,16 @@
 template <class T>
 struct ShapeViewT {
 	int type, count;
-	double r;
+	real r;
 	uint64_t hashid;
 	V2T<T> tc;                 // circle centre (world)
 	V2T<T> ta, tb, tn;         // segment (world)
 	V2 a_tan, b_tan;           // segment neighbour tangents (body-local)
 	V2T<T> rot;                // body rotation (transform.a, transform.b)
 	const V2T<T> *wv, *wn;     // polygon vertices and edge normals (world)
-	double bb[4];
+	real bb[4];
 };
-using ShapeView = ShapeViewT<double>;
+using ShapeView = ShapeViewT<real>;
 
 CPD ShapeView load_shape(const DevWorld &W, int s)
 {
@@ -44,8 +46,8 @@ CPD ShapeView load_shape(const DevWorld &W, int s)
 	v.hashid = W.s_hashid[s];
 	const XF T = W.b_T[W.s_body[s]];
 	v.rot = v2(T.a, T.b);
-	const double *g = W.s_wgeom + 6 * (size_t)s;
-	const double4 bb = W.s_bb[s];
+	const real *g = W.s_wgeom + 6 * (size_t)s;
+	const real4 bb = W.s_bb[s];
 	v.bb[0] = bb.x; v.bb[1] = bb.y; v.bb[2] = bb.z; v.bb[3] = bb.w;
 	v.count = 0;
 	v.wv = nullptr; v.wn = nullptr;
@@ -53,12 +55,12 @@ CPD ShapeView load_shape(const DevWorld &W, int s)
 		v.tc = v2(g[0], g[1]);
 	} else if (v.type == CPGPU_SHAPE_SEGMENT) {
 		v.ta = v2(g[0], g[1]); v.tb = v2(g[2], g[3]); v.tn = v2(g[4], g[5]);
-		const double *lg = W.s_geom + 10 * (size_t)s;
+		const real *lg = W.s_geom + 10 * (size_t)s;
 		v.a_tan = v2(lg[6], lg[7]); v.b_tan = v2(lg[8], lg[9]);
 	} else {
 		const int st = W.s_pstart[s];
 		v.count = W.s_pcount[s];
-		// double2 and V2 share their layout {x, y}
+		// real2 and V2 share their layout {x, y}
 		v.wv = reinterpret_cast<const V2 *>(W.pv_world + st);
 		v.wn = reinterpret_cast<const V2 *>(W.pn_world + st);
 	}
@@ -69,7 +71,7 @@ template <class T> struct SupportPtT { V2T<T> p; uint32_t index; };
 template <class T> struct MinkPtT { V2T<T> a, b, ab; uint32_t id; };
 template <class T> struct ClosestPtsT { V2T<T> a, b, n; T d; uint32_t id; };
 template <class T> struct EdgePtT { V2T<T> p; uint64_t hash; };
-template <class T> struct EdgeT { EdgePtT<T> a, b; double r; V2T<T> n; };
+template <class T> struct EdgeT { EdgePtT<T> a, b; real r; V2T<T> n; };
 template <class T> struct CollInfoT {
 	int count;
 	uint32_t id;
@@ -77,7 +79,7 @@ template <class T> struct CollInfoT {
 	V2T<T> p1[2], p2[2];
 	uint64_t hash[2];
 };
-using CollInfo = CollInfoT<double>;
+using CollInfo = CollInfoT<real>;
 
 // PolySupportPointIndex (cpCollision.c:62-77)
 template <class T> CPD int poly_support_index(const ShapeViewT<T> &s, V2T<T> n)
@@ -142,7 +144,7 @@ template <class T> CPD MinkPtT<T> support(const ShapeViewT<T> &s1, const ShapeVi
 // cpCheckPointGreater / cpCheckAxis (cpRobust.c:4-13)
 template <class T> CPD bool check_point_greater(V2T<T> a, V2T<T> b, V2T<T> c)
 {
-	return (b.y - a.y) * (a.x + b.x - 2.0 * c.x) > (b.x - a.x) * (a.y + b.y - 2.0 * c.y);
+	return (b.y - a.y) * (a.x + b.x - T(2.0) * c.x) > (b.x - a.x) * (a.y + b.y - T(2.0) * c.y);
 }
 template <class T> CPD bool check_axis(V2T<T> v0, V2T<T> v1, V2T<T> p, V2T<T> n)
 {
@@ -153,12 +155,12 @@ template <class T> CPD bool check_axis(V2T<T> v0, V2T<T> v1, V2T<T> p, V2T<T> n)
 template <class T> CPD T closest_t(V2T<T> a, V2T<T> b)
 {
 	const V2T<T> delta = vsub(b, a);
-	return -fclampc(vdot(delta, vadd(a, b)) / (vlengthsq(delta) + CP_DBL_MIN), -1.0, 1.0);
+	return -fclampc(vdot(delta, vadd(a, b)) / (vlengthsq(delta) + CP_REAL_MIN), -1.0, 1.0);
 }
 template <class T> CPD V2T<T> lerp_t(V2T<T> a, V2T<T> b, T t)
 {
-	const T ht = 0.5 * t;
-	return vadd(vmult(a, 0.5 - ht), vmult(b, 0.5 + ht));
+	const T ht = T(0.5) * t;
+	return vadd(vmult(a, T(0.5) - ht), vmult(b, T(0.5) + ht));
 }
 
 // ClosestPointsNew (cpCollision.c:223-258)
@@ -173,10 +175,10 @@ template <class T> CPD ClosestPtsT<T> closest_points_new(const MinkPtT<T> &v0, c
 	const V2T<T> delta = vsub(v1.ab, v0.ab);
 	const V2T<T> n = vnormalize(vrperp(delta));
 	const T d = vdot(n, p);
-	if (d <= 0.0 || (-1.0 < t && t < 1.0)) { r.n = n; r.d = d; }
+	if (d <= T(0.0) || (T(-1.0) < t && t < T(1.0))) { r.n = n; r.d = d; }
 	else {
 		const T d2 = vlength(p);
-		r.n = vmult(p, 1.0 / (d2 + CP_DBL_MIN));
+		r.n = vmult(p, T(1.0) / (d2 + CP_REAL_MIN));
 		r.d = d2;
 	}
 	return r;
@@ -241,7 +243,7 @@ CPD ClosestPtsT<T> gjk_recurse(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2,
 			continue;
 		}
 		const T t = closest_t(v0.ab, v1.ab);
-		const V2T<T> n = (-1.0 < t && t < 1.0) ? vperp(vsub(v1.ab, v0.ab)) : vneg(lerp_t(v0.ab, v1.ab, t));
+		const V2T<T> n = (T(-1.0) < t && t < T(1.0)) ? vperp(vsub(v1.ab, v0.ab)) : vneg(lerp_t(v0.ab, v1.ab, t));
 		const MinkPtT<T> p = support(s1, s2, n);
 		if (check_point_greater(p.ab, v0.ab, origin) && check_point_greater(v1.ab, p.ab, origin)) {
 			return epa(s1, s2, v0, p, v1, err);
@@ -253,7 +255,7 @@ CPD ClosestPtsT<T> gjk_recurse(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2,
 	}
 }
 
-CPD V2 bb_center(const double *bb) { return vlerp(v2(bb[0], bb[1]), v2(bb[2], bb[3]), 0.5); }
+CPD V2 bb_center(const real *bb) { return vlerp(v2(bb[0], bb[1]), v2(bb[2], bb[3]), real(0.5)); }
 
 // GJK (cpCollision.c:416-472)
 template <class T>
@@ -286,26 +288,26 @@ template <class T> CPD void push_contact(CollInfoT<T> &info, V2T<T> p1, V2T<T> p
 template <class T>
 CPD void contact_points(const EdgeT<T> &e1, const EdgeT<T> &e2, const ClosestPtsT<T> &points, CollInfoT<T> &info)
 {
-	const double mindist = e1.r + e2.r;
+	const real mindist = e1.r + e2.r;
 	if (points.d <= mindist) {
 		const V2T<T> n = info.n = points.n;
 		const T d_e1_a = vcross(e1.a.p, n);
 		const T d_e1_b = vcross(e1.b.p, n);
 		const T d_e2_a = vcross(e2.a.p, n);
 		const T d_e2_b = vcross(e2.b.p, n);
-		const T e1_denom = 1.0 / (d_e1_b - d_e1_a + CP_DBL_MIN);
-		const T e2_denom = 1.0 / (d_e2_b - d_e2_a + CP_DBL_MIN);
+		const T e1_denom = T(1.0) / (d_e1_b - d_e1_a + CP_REAL_MIN);
+		const T e2_denom = T(1.0) / (d_e2_b - d_e2_a + CP_REAL_MIN);
 		{
 			const V2T<T> p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_b - d_e1_a) * e1_denom)));
 			const V2T<T> p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_a - d_e2_a) * e2_denom)));
 			const T dist = vdot(vsub(p2, p1), n);
-			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.a.hash, e2.b.hash));
+			if (dist <= T(0.0)) push_contact(info, p1, p2, hash_pair(e1.a.hash, e2.b.hash));
 		}
 		{
 			const V2T<T> p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_a - d_e1_a) * e1_denom)));
 			const V2T<T> p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_b - d_e2_a) * e2_denom)));
 			const T dist = vdot(vsub(p2, p1), n);
-			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.b.hash, e2.a.hash));
+			if (dist <= T(0.0)) push_contact(info, p1, p2, hash_pair(e1.b.hash, e2.a.hash));
 		}
 	}
 }
@@ -338,7 +340,7 @@ template <class T> CPD EdgeT<T> support_edge_segment(const ShapeViewT<T> &s, V2T
 	const uint64_t h = s.hashid;
 	EdgeT<T> e;
 	e.r = s.r;
-	if (vdot(s.tn, n) > 0.0) {
+	if (vdot(s.tn, n) > T(0.0)) {
 		e.a.p = s.ta; e.a.hash = hash_pair(h, 0);
 		e.b.p = s.tb; e.b.hash = hash_pair(h, 1);
 		e.n = s.tn;
@@ -353,12 +355,12 @@ template <class T> CPD EdgeT<T> support_edge_segment(const ShapeViewT<T> &s, V2T
 // CircleToCircle (cpCollision.c:525-537)
 template <class T> CPD void circle_circle(const ShapeViewT<T> &c1, const ShapeViewT<T> &c2, CollInfoT<T> &info)
 {
-	const double mindist = c1.r + c2.r;
+	const real mindist = c1.r + c2.r;
 	const V2T<T> delta = vsub(c2.tc, c1.tc);
 	const T distsq = vlengthsq(delta);
 	if (distsq < mindist * mindist) {
 		const T dist = csqrt(distsq);
-		const V2T<T> n = info.n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : v2t<T>(T(1.0), T(0.0)));
+		const V2T<T> n = info.n = (dist != T(0.0) ? vmult(delta, T(1.0) / dist) : v2t<T>(T(1.0), T(0.0)));
 		push_contact(info, vadd(c1.tc, vmult(n, c1.r)), vadd(c2.tc, vmult(n, -c2.r)), 0);
 	}
 }
@@ -370,15 +372,15 @@ template <class T> CPD void circle_segment(const ShapeViewT<T> &circle, const Sh
 	const V2T<T> seg_delta = vsub(seg_b, seg_a);
 	const T closest_tt = fclamp01(vdot(seg_delta, vsub(center, seg_a)) / vlengthsq(seg_delta));
 	const V2T<T> closest = vadd(seg_a, vmult(seg_delta, closest_tt));
-	const double mindist = circle.r + seg.r;
+	const real mindist = circle.r + seg.r;
 	const V2T<T> delta = vsub(closest, center);
 	const T distsq = vlengthsq(delta);
 	if (distsq < mindist * mindist) {
 		const T dist = csqrt(distsq);
-		const V2T<T> n = info.n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : seg.tn);
+		const V2T<T> n = info.n = (dist != T(0.0) ? vmult(delta, T(1.0) / dist) : seg.tn);
 		const V2T<T> rot = seg.rot;
-		if ((closest_tt != 0.0 || vdot(n, vrotate(vcast<T>(seg.a_tan), rot)) >= 0.0) &&
-		    (closest_tt != 1.0 || vdot(n, vrotate(vcast<T>(seg.b_tan), rot)) >= 0.0)) {
+		if ((closest_tt != T(0.0) || vdot(n, vrotate(vcast<T>(seg.a_tan), rot)) >= T(0.0)) &&
+		    (closest_tt != T(1.0) || vdot(n, vrotate(vcast<T>(seg.b_tan), rot)) >= T(0.0))) {
 			push_contact(info, vadd(center, vmult(n, circle.r)), vadd(closest, vmult(n, -seg.r)), 0);
 		}
 	}
@@ -392,10 +394,10 @@ CPD void segment_segment(const ShapeViewT<T> &s1, const ShapeViewT<T> &s2, CollI
 	const V2T<T> n = points.n;
 	const V2T<T> rot1 = s1.rot, rot2 = s2.rot;
 	if (points.d <= (s1.r + s2.r) &&
-	    (!veql(points.a, s1.ta) || vdot(n, vrotate(vcast<T>(s1.a_tan), rot1)) <= 0.0) &&
-	    (!veql(points.a, s1.tb) || vdot(n, vrotate(vcast<T>(s1.b_tan), rot1)) <= 0.0) &&
-	    (!veql(points.b, s2.ta) || vdot(n, vrotate(vcast<T>(s2.a_tan), rot2)) >= 0.0) &&
-	    (!veql(points.b, s2.tb) || vdot(n, vrotate(vcast<T>(s2.b_tan), rot2)) >= 0.0)) {
+	    (!veql(points.a, s1.ta) || vdot(n, vrotate(vcast<T>(s1.a_tan), rot1)) <= T(0.0)) &&
+	    (!veql(points.a, s1.tb) || vdot(n, vrotate(vcast<T>(s1.b_tan), rot1)) <= T(0.0)) &&
+	    (!veql(points.b, s2.ta) || vdot(n, vrotate(vcast<T>(s2.a_tan), rot2)) >= T(0.0)) &&
+	    (!veql(points.b, s2.tb) || vdot(n, vrotate(vcast<T>(s2.b_tan), rot2)) >= T(0.0))) {
 		contact_points(support_edge_segment(s1, n), support_edge_segment(s2, vneg(n)), points, info);
 	}
 }
@@ -405,7 +407,7 @@ template <class T>
 CPD void poly_poly(const ShapeViewT<T> &p1, const ShapeViewT<T> &p2, CollInfoT<T> &info, int32_t *err)
 {
 	const ClosestPtsT<T> points = gjk(p1, p2, info.id, err);
-	if (points.d - p1.r - p2.r <= 0.0) {
+	if (points.d - p1.r - p2.r <= T(0.0)) {
 		contact_points(support_edge_poly(p1, points.n), support_edge_poly(p2, vneg(points.n)), points, info);
 	}
 }
@@ -417,9 +419,9 @@ CPD void segment_poly(const ShapeViewT<T> &seg, const ShapeViewT<T> &poly, CollI
 	const ClosestPtsT<T> points = gjk(seg, poly, info.id, err);
 	const V2T<T> n = points.n;
 	const V2T<T> rot = seg.rot;
-	if (points.d - seg.r - poly.r <= 0.0 &&
-	    (!veql(points.a, seg.ta) || vdot(n, vrotate(vcast<T>(seg.a_tan), rot)) <= 0.0) &&
-	    (!veql(points.a, seg.tb) || vdot(n, vrotate(vcast<T>(seg.b_tan), rot)) <= 0.0)) {
+	if (points.d - seg.r - poly.r <= T(0.0) &&
+	    (!veql(points.a, seg.ta) || vdot(n, vrotate(vcast<T>(seg.a_tan), rot)) <= T(0.0)) &&
+	    (!veql(points.a, seg.tb) || vdot(n, vrotate(vcast<T>(seg.b_tan), rot)) <= T(0.0))) {
 		contact_points(support_edge_segment(seg, n), support_edge_poly(poly, vneg(n)), points, info);
 	}
 }
@@ -452,3 +454,5 @@ CPD void collide(const ShapeViewT<T> &a, const ShapeViewT<T> &b, uint32_t id, Co
 	default: break;
 	}
 }
+
+} // namespace CPGPU_NS

@@ -16,6 +16,8 @@
 #include "narrowphase.h"
 #include "cpgpu_step.h"
 
+namespace CPGPU_NS {
+
 namespace {
 
 constexpr uint64_t EMPTY_KEY = ~0ull;
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const Ste
 			// this step (cpSpaceStep.c:351-360); a cached one becomes FIRST_COLLISION
 			if (po == nullptr) a.state = CPGPU_ARB_FIRST_COLLISION;
 			else a.state = (po->stamp == ws.stamp - 1u) ? CPGPU_ARB_NORMAL : CPGPU_ARB_FIRST_COLLISION;
-			const double2 pA = W.b_p[a.ba], pB = W.b_p[a.bb];
+			const real2 pA = W.b_p[a.ba], pB = W.b_p[a.bb];
 			const int old_count = po ? po->count : 0;
 			for (int c = 0; c < 2; c++) {
 				Contact con;
@@ -181,7 +183,7 @@ __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const Ste
 			a.nx = info.n.x; a.ny = info.n.y;
 			a.e = W.s_e[ia] * W.s_e[ib];
 			a.u = W.s_u[ia] * W.s_u[ib];
-			const double2 svA = W.s_sv[ia], svB = W.s_sv[ib];
+			const real2 svA = W.s_sv[ia], svB = W.s_sv[ib];
 			const V2 surface_vr = v2(svB.x - svA.x, svB.y - svA.y);
 			const V2 svr = vsub(surface_vr, vmult(info.n, vdot(surface_vr, info.n)));
 			a.svx = svr.x; a.svy = svr.y;
@@ -247,3 +249,5 @@ hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 
 	}
 	return hipSuccess;
 }
+
+} // namespace CPGPU_NS

@@ -19,6 +19,8 @@
 // all their claims take colour c.
 #include "cpgpu_step.h"
 
+namespace CPGPU_NS {
+
 namespace {
 
 constexpr int GRID_THREADS = 1024;  // grid solve: few, large workgroups keep the barrier cheap
@@ -65,7 +67,7 @@ __device__ __forceinline__ bool is_dynamic(const DevWorld &W, int b) { return W.
 // ---- body helpers (chipmunk_private.h:172-226) ----
 struct BV {
 	V2 v, vb;
-	double w, wb, m_inv, i_inv;
+	real w, wb, m_inv, i_inv;
 };
 
 __device__ __forceinline__ BV load_bv(const DevWorld &W, int b)
@@ -133,13 +135,13 @@ __device__ __forceinline__ void apply_bias_impulse(BV &body, V2 j, V2 r)
 	body.wb += body.i_inv * vcross(r, j);
 }
 
-__device__ __forceinline__ double k_scalar_body(const BV &b, V2 r, V2 n)
+__device__ __forceinline__ real k_scalar_body(const BV &b, V2 r, V2 n)
 {
-	const double rcn = vcross(r, n);
+	const real rcn = vcross(r, n);
 	return b.m_inv + b.i_inv * rcn * rcn;
 }
 
-__device__ __forceinline__ double k_scalar(const BV &a, const BV &b, V2 r1, V2 r2, V2 n)
+__device__ __forceinline__ real k_scalar(const BV &a, const BV &b, V2 r1, V2 r2, V2 n)
 {
 	return k_scalar_body(a, r1, n) + k_scalar_body(b, r2, n);
 }
@@ -150,7 +152,7 @@ __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
 {
 	const WorldStep ws = W.ws[arb.world];
 	const BV a = load_bv(W, arb.ba), b = load_bv(W, arb.bb);
-	const double2 pa = W.b_p[arb.ba], pb = W.b_p[arb.bb];
+	const real2 pa = W.b_p[arb.ba], pb = W.b_p[arb.bb];
 	const V2 n = v2(arb.nx, arb.ny);
 	const V2 body_delta = v2(pb.x - pa.x, pb.y - pa.y);
 	for (int i = 0; i < arb.count; i++) {
@@ -158,8 +160,8 @@ __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
 		const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
 		con.nMass = 1.0 / k_scalar(a, b, r1, r2, n);
 		con.tMass = 1.0 / k_scalar(a, b, r1, r2, vperp(n));
-		const double dist = vdot(vadd(vsub(r2, r1), body_delta), n);
-		con.bias = -ws.bias_coef * fminc(0.0, dist + ws.slop) / ws.dt;
+		const real dist = vdot(vadd(vsub(r2, r1), body_delta), n);
+		con.bias = -ws.bias_coef * fminc(real(0.0), dist + ws.slop) / ws.dt;
 		con.jBias = 0.0;
 		con.bounce = vdot(rel_vel(a, b, r1, r2), n) * arb.e;
 	}
@@ -169,7 +171,7 @@ template <class BA>
 __device__ void arb_apply_cached(const DevWorld &W, const BA &bodies, const Arbiter &arb)
 {
 	if (arb.state == CPGPU_ARB_FIRST_COLLISION) return;
-	const double dt_coef = W.ws[arb.world].dt_coef;
+	const real dt_coef = W.ws[arb.world].dt_coef;
 	BV a = bodies.load(arb.ba), b = bodies.load(arb.bb);
 	const V2 n = v2(arb.nx, arb.ny);
 	for (int i = 0; i < arb.count; i++) {
@@ -187,26 +189,26 @@ __device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &
 	BV a = bodies.load(arb.ba), b = bodies.load(arb.bb);
 	const V2 n = v2(arb.nx, arb.ny);
 	const V2 surface_vr = v2(arb.svx, arb.svy);
-	const double friction = arb.u;
+	const real friction = arb.u;
 	for (int i = 0; i < arb.count; i++) {
 		Contact &con = arb.c[i];
-		const double nMass = con.nMass;
+		const real nMass = con.nMass;
 		const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
 		const V2 vb1 = vadd(a.vb, vmult(vperp(r1), a.wb));
 		const V2 vb2 = vadd(b.vb, vmult(vperp(r2), b.wb));
 		const V2 vr = vadd(rel_vel(a, b, r1, r2), surface_vr);
-		const double vbn = vdot(vsub(vb2, vb1), n);
-		const double vrn = vdot(vr, n);
-		const double vrt = vdot(vr, vperp(n));
-		const double jbn = (con.bias - vbn) * nMass;
-		const double jbnOld = con.jBias;
-		con.jBias = fmaxc(jbnOld + jbn, 0.0);
-		const double jn = -(con.bounce + vrn) * nMass;
-		const double jnOld = con.jnAcc;
-		con.jnAcc = fmaxc(jnOld + jn, 0.0);
-		const double jtMax = friction * con.jnAcc;
-		const double jt = -vrt * con.tMass;
-		const double jtOld = con.jtAcc;
+		const real vbn = vdot(vsub(vb2, vb1), n);
+		const real vrn = vdot(vr, n);
+		const real vrt = vdot(vr, vperp(n));
+		const real jbn = (con.bias - vbn) * nMass;
+		const real jbnOld = con.jBias;
+		con.jBias = fmaxc(jbnOld + jbn, real(0.0));
+		const real jn = -(con.bounce + vrn) * nMass;
+		const real jnOld = con.jnAcc;
+		con.jnAcc = fmaxc(jnOld + jn, real(0.0));
+		const real jtMax = friction * con.jnAcc;
+		const real jt = -vrt * con.tMass;
+		const real jtOld = con.jtAcc;
 		con.jtAcc = fclampc(jtOld + jt, -jtMax, jtMax);
 		const V2 jb = vmult(n, con.jBias - jbnOld);
 		apply_bias_impulse(a, vneg(jb), r1);
@@ -221,74 +223,74 @@ __device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &
 
 __device__ void joint_prestep(const DevWorld &W, Joint &j)
 {
-	const double dt = W.ws[j.world].dt;
+	const real dt = W.ws[j.world].dt;
 	BV a = load_bv(W, j.a), b = load_bv(W, j.b);
 	const XF Ta = W.b_T[j.a], Tb = W.b_T[j.b];
-	const double2 cogA = W.b_cog[j.a], cogB = W.b_cog[j.b];
-	const double2 pA = W.b_p[j.a], pB = W.b_p[j.b];
+	const real2 cogA = W.b_cog[j.a], cogB = W.b_cog[j.b];
+	const real2 pA = W.b_p[j.a], pB = W.b_p[j.b];
 	const V2 anchorA = v2(j.P[0], j.P[1]), anchorB = v2(j.P[2], j.P[3]);
-	const double bc = j.bias_coef;
+	const real bc = j.bias_coef;
 	if (j.type == CPGPU_JOINT_PIVOT) {
 		const V2 r1 = xvect(Ta, vsub(anchorA, v2(cogA.x, cogA.y)));
 		const V2 r2 = xvect(Tb, vsub(anchorB, v2(cogB.x, cogB.y)));
 		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
-		const double m_sum = a.m_inv + b.m_inv;
-		double k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
-		const double a_i_inv = a.i_inv;
-		const double r1xsq = r1.x * r1.x * a_i_inv;
-		const double r1ysq = r1.y * r1.y * a_i_inv;
-		const double r1nxy = -r1.x * r1.y * a_i_inv;
+		const real m_sum = a.m_inv + b.m_inv;
+		real k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
+		const real a_i_inv = a.i_inv;
+		const real r1xsq = r1.x * r1.x * a_i_inv;
+		const real r1ysq = r1.y * r1.y * a_i_inv;
+		const real r1nxy = -r1.x * r1.y * a_i_inv;
 		k11 += r1ysq; k12 += r1nxy; k21 += r1nxy; k22 += r1xsq;
-		const double b_i_inv = b.i_inv;
-		const double r2xsq = r2.x * r2.x * b_i_inv;
-		const double r2ysq = r2.y * r2.y * b_i_inv;
-		const double r2nxy = -r2.x * r2.y * b_i_inv;
+		const real b_i_inv = b.i_inv;
+		const real r2xsq = r2.x * r2.x * b_i_inv;
+		const real r2ysq = r2.y * r2.y * b_i_inv;
+		const real r2nxy = -r2.x * r2.y * b_i_inv;
 		k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
-		const double det = k11 * k22 - k12 * k21;
-		const double det_inv = 1.0 / det;
+		const real det = k11 * k22 - k12 * k21;
+		const real det_inv = 1.0 / det;
 		j.k11 = k22 * det_inv; j.k12 = -k12 * det_inv;
 		j.k21 = -k21 * det_inv; j.k22 = k11 * det_inv;
 		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(v2(pA.x, pA.y), r1));
 		const V2 bias = vclamp(vmult(delta, -bc / dt), j.maxBias);
 		j.bx = bias.x; j.by = bias.y;
 	} else if (j.type == CPGPU_JOINT_DAMPED_SPRING) {
-		const double restLength = j.P[4], stiffness = j.P[5], damping = j.P[6];
+		const real restLength = j.P[4], stiffness = j.P[5], damping = j.P[6];
 		const V2 r1 = xvect(Ta, vsub(anchorA, v2(cogA.x, cogA.y)));
 		const V2 r2 = xvect(Tb, vsub(anchorB, v2(cogB.x, cogB.y)));
 		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
 		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(v2(pA.x, pA.y), r1));
-		const double dist = vlength(delta);
+		const real dist = vlength(delta);
 		const V2 n = vmult(delta, 1.0 / (dist != 0.0 ? dist : __builtin_inf()));
 		j.nx = n.x; j.ny = n.y;
-		const double k = k_scalar(a, b, r1, r2, n);
+		const real k = k_scalar(a, b, r1, r2, n);
 		j.nMass = 1.0 / k;
 		j.target_vrn = 0.0;
 		j.v_coef = 1.0 - cp_exp(-damping * dt * k);
-		const double f_spring = (restLength - dist) * stiffness;
-		const double j_spring = j.jAcc = f_spring * dt;
+		const real f_spring = (restLength - dist) * stiffness;
+		const real j_spring = j.jAcc = f_spring * dt;
 		apply_impulses(a, b, r1, r2, vmult(n, j_spring));
 		if (is_dynamic(W, j.a)) store_bv(W, j.a, a);
 		if (is_dynamic(W, j.b)) store_bv(W, j.b, b);
 	} else if (j.type == CPGPU_JOINT_SLIDE) {
-		const double mn = j.P[4], mx = j.P[5];
+		const real mn = j.P[4], mx = j.P[5];
 		const V2 r1 = xvect(Ta, vsub(anchorA, v2(cogA.x, cogA.y)));
 		const V2 r2 = xvect(Tb, vsub(anchorB, v2(cogB.x, cogB.y)));
 		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
 		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(v2(pA.x, pA.y), r1));
-		const double dist = vlength(delta);
-		double pdist = 0.0;
+		const real dist = vlength(delta);
+		real pdist = 0.0;
 		V2 n;
 		if (dist > mx) { pdist = dist - mx; n = vnormalize(delta); }
 		else if (dist < mn) { pdist = mn - dist; n = vneg(vnormalize(delta)); }
 		else { n = v2(0.0, 0.0); j.jnAcc = 0.0; }
 		j.nx = n.x; j.ny = n.y;
 		j.nMass = 1.0 / k_scalar(a, b, r1, r2, n);
-		const double maxBias = j.maxBias;
+		const real maxBias = j.maxBias;
 		j.bias = fclampc(-bc * pdist / dt, -maxBias, maxBias);
 	} else {
-		const double phase = j.P[0], ratio = j.P[1], ratio_inv = j.P[2];
+		const real phase = j.P[0], ratio = j.P[1], ratio_inv = j.P[2];
 		j.iSum = 1.0 / (a.i_inv * ratio_inv + ratio * b.i_inv);
-		const double maxBias = j.maxBias;
+		const real maxBias = j.maxBias;
 		j.bias = fclampc(-bc * (W.b_a[j.b] * ratio - W.b_a[j.a] - phase) / dt, -maxBias, maxBias);
 	}
 }
@@ -297,7 +299,7 @@ template <class BA>
 __device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Joint &j)
 {
 	if (j.type == CPGPU_JOINT_DAMPED_SPRING) return;
-	const double dt_coef = W.ws[j.world].dt_coef;
+	const real dt_coef = W.ws[j.world].dt_coef;
 	BV a = bodies.load(j.a), b = bodies.load(j.b);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
@@ -305,8 +307,8 @@ __device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Jo
 	} else if (j.type == CPGPU_JOINT_SLIDE) {
 		apply_impulses(a, b, r1, r2, vmult(v2(j.nx, j.ny), j.jnAcc * dt_coef));
 	} else {
-		const double ratio_inv = j.P[2];
-		const double jj = j.jAcc * dt_coef;
+		const real ratio_inv = j.P[2];
+		const real jj = j.jAcc * dt_coef;
 		a.w -= jj * a.i_inv * ratio_inv;
 		b.w += jj * b.i_inv;
 	}
@@ -317,7 +319,7 @@ __device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Jo
 template <class BA>
 __device__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &j)
 {
-	const double dt = W.ws[j.world].dt;
+	const real dt = W.ws[j.world].dt;
 	BV a = bodies.load(j.a), b = bodies.load(j.b);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
@@ -331,28 +333,28 @@ __device__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &
 		apply_impulses(a, b, r1, r2, jj);
 	} else if (j.type == CPGPU_JOINT_DAMPED_SPRING) {
 		const V2 n = v2(j.nx, j.ny);
-		const double vrn = vdot(rel_vel(a, b, r1, r2), n);
-		const double v_damp = (j.target_vrn - vrn) * j.v_coef;
+		const real vrn = vdot(rel_vel(a, b, r1, r2), n);
+		const real v_damp = (j.target_vrn - vrn) * j.v_coef;
 		j.target_vrn = vrn + v_damp;
-		const double j_damp = v_damp * j.nMass;
+		const real j_damp = v_damp * j.nMass;
 		j.jAcc += j_damp;
 		apply_impulses(a, b, r1, r2, vmult(n, j_damp));
 	} else if (j.type == CPGPU_JOINT_SLIDE) {
 		const V2 n = v2(j.nx, j.ny);
 		if (veql(n, v2(0.0, 0.0))) return;
 		const V2 vr = rel_vel(a, b, r1, r2);
-		const double vrn = vdot(vr, n);
-		double jn = (j.bias - vrn) * j.nMass;
-		const double jnOld = j.jnAcc;
+		const real vrn = vdot(vr, n);
+		real jn = (j.bias - vrn) * j.nMass;
+		const real jnOld = j.jnAcc;
 		j.jnAcc = fclampc(jnOld + jn, -j.maxForce * dt, 0.0);
 		jn = j.jnAcc - jnOld;
 		apply_impulses(a, b, r1, r2, vmult(n, jn));
 	} else {
-		const double ratio = j.P[1], ratio_inv = j.P[2];
-		const double wr = b.w * ratio - a.w;
-		const double jMax = j.maxForce * dt;
-		double jj = (j.bias - wr) * j.iSum;
-		const double jOld = j.jAcc;
+		const real ratio = j.P[1], ratio_inv = j.P[2];
+		const real wr = b.w * ratio - a.w;
+		const real jMax = j.maxForce * dt;
+		real jj = (j.bias - wr) * j.iSum;
+		const real jOld = j.jAcc;
 		j.jAcc = fclampc(jOld + jj, -jMax, jMax);
 		jj = j.jAcc - jOld;
 		a.w -= jj * a.i_inv * ratio_inv;
@@ -373,13 +375,13 @@ __device__ __forceinline__ void integrate_velocity(const DevWorld &W, int b)
 	if (W.b_type[b] != CPGPU_BODY_DYNAMIC) return;
 	const WorldStep ws = W.ws[W.b_world[b]];
 	BodyVel bv = W.b_vel[b];
-	const double2 f = W.b_f[b];
+	const real2 f = W.b_f[b];
 	const V2 v = vadd(vmult(v2(bv.vx, bv.vy), ws.damping),
 	                  vmult(vadd(v2(ws.gx, ws.gy), vmult(v2(f.x, f.y), bv.m_inv)), ws.dt));
 	bv.vx = v.x; bv.vy = v.y;
 	bv.w = bv.w * ws.damping + W.b_t[b] * bv.i_inv * ws.dt;
 	W.b_vel[b] = bv;
-	W.b_f[b] = make_double2(0.0, 0.0);
+	W.b_f[b] = make_real2(0.0, 0.0);
 	W.b_t[b] = 0.0;
 }
 
@@ -1137,3 +1139,5 @@ hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBuck
 	}
 	return hipSuccess;
 }
+
+} // namespace CPGPU_NS

@@ -17,16 +17,17 @@
 #include <string>
 #include <vector>
 
+#include "cpgpu_errors.h"
 #include "cpgpu_step.h"
 #include "cpgpu_adjoint.h"
 
-namespace {
+namespace CPGPU_NS {
 
-thread_local std::string g_last_error;
+namespace {
 
 int fail(int code, const std::string &msg)
 {
-	g_last_error = msg;
+	cpgpu_last_error = msg;
 	return code;
 }
 
@@ -60,26 +61,27 @@ __global__ void k_update_positions(const DevWorld W, const int32_t *bodies, int 
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= n) return;
 	const int b = bodies[q];
-	const double dt = W.ws[W.b_world[b]].dt;
+	const real dt = W.ws[W.b_world[b]].dt;
 	BodyVel bv = W.b_vel[b];
-	const double2 p0 = W.b_p[b];
+	const real2 p0 = W.b_p[b];
 	const V2 p = vadd(v2(p0.x, p0.y), vmult(vadd(v2(bv.vx, bv.vy), v2(bv.vbx, bv.vby)), dt));
-	const double a = W.b_a[b] + (bv.w + bv.wb) * dt;
-	double sn, cs;
-	cp_sincos(a, sn, cs);
-	const double2 c = W.b_cog[b];
+	const real a = W.b_a[b] + (bv.w + bv.wb) * dt;
+	double sn_d, cs_d;
+	cp_sincos(a, sn_d, cs_d);
+	const real sn = (real)sn_d, cs = (real)cs_d;
+	const real2 c = W.b_cog[b];
 	XF T;
 	T.a = cs; T.b = sn; T.c = -sn; T.d = cs;
 	T.tx = p.x - (c.x * cs - c.y * sn);
 	T.ty = p.y - (c.x * sn + c.y * cs);
-	W.b_p[b] = make_double2(p.x, p.y);
+	W.b_p[b] = make_real2(p.x, p.y);
 	W.b_a[b] = a;
 	W.b_T[b] = T;
-	bv.vbx = 0.0; bv.vby = 0.0; bv.wb = 0.0;
+	bv.vbx = real(0.0); bv.vby = real(0.0); bv.wb = real(0.0);
 	W.b_vel[b] = bv;
 }
 
-__global__ void k_set_joint_bias(Joint *joints, const double *bc, int n)
+__global__ void k_set_joint_bias(Joint *joints, const real *bc, int n)
 {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c < n) joints[c].bias_coef = bc[c];
@@ -90,47 +92,47 @@ __device__ void cache_shape(const DevWorld &W, int s)
 {
 	const XF T = W.b_T[W.s_body[s]];
 	const int type = W.s_type[s];
-	const double r = W.s_r[s];
-	const double *g = W.s_geom + 10 * (size_t)s;
-	double *wg = W.s_wgeom + 6 * (size_t)s;
-	double4 bb;
+	const real r = W.s_r[s];
+	const real *g = W.s_geom + 10 * (size_t)s;
+	real *wg = W.s_wgeom + 6 * (size_t)s;
+	real4 bb;
 	if (type == CPGPU_SHAPE_CIRCLE) {
 		const V2 c = xpoint(T, v2(g[0], g[1]));
 		wg[0] = c.x; wg[1] = c.y;
-		bb = make_double4(c.x - r, c.y - r, c.x + r, c.y + r);
+		bb = make_real4(c.x - r, c.y - r, c.x + r, c.y + r);
 	} else if (type == CPGPU_SHAPE_SEGMENT) {
 		const V2 ta = xpoint(T, v2(g[0], g[1]));
 		const V2 tb = xpoint(T, v2(g[2], g[3]));
 		const V2 tn = xvect(T, v2(g[4], g[5]));
 		wg[0] = ta.x; wg[1] = ta.y; wg[2] = tb.x; wg[3] = tb.y; wg[4] = tn.x; wg[5] = tn.y;
-		double l, rr, bt, tp;
+		real l, rr, bt, tp;
 		if (ta.x < tb.x) { l = ta.x; rr = tb.x; } else { l = tb.x; rr = ta.x; }
 		if (ta.y < tb.y) { bt = ta.y; tp = tb.y; } else { bt = tb.y; tp = ta.y; }
-		bb = make_double4(l - r, bt - r, rr + r, tp + r);
+		bb = make_real4(l - r, bt - r, rr + r, tp + r);
 	} else {
 		const int st = W.s_pstart[s], cnt = W.s_pcount[s];
-		double l = __builtin_inf(), rr = -__builtin_inf(), bt = __builtin_inf(), tp = -__builtin_inf();
+		real l = real(__builtin_inf()), rr = real(-__builtin_inf()), bt = real(__builtin_inf()), tp = real(-__builtin_inf());
 		for (int i = 0; i < cnt; i++) {
-			const double2 lv = W.pv_local[st + i], ln = W.pn_local[st + i];
+			const real2 lv = W.pv_local[st + i], ln = W.pn_local[st + i];
 			const V2 v = xpoint(T, v2(lv.x, lv.y));
 			const V2 nn = xvect(T, v2(ln.x, ln.y));
-			W.pv_world[st + i] = make_double2(v.x, v.y);
-			W.pn_world[st + i] = make_double2(nn.x, nn.y);
+			W.pv_world[st + i] = make_real2(v.x, v.y);
+			W.pn_world[st + i] = make_real2(nn.x, nn.y);
 			l = fminc(l, v.x); rr = fmaxc(rr, v.x); bt = fminc(bt, v.y); tp = fmaxc(tp, v.y);
 		}
-		bb = make_double4(l - r, bt - r, rr + r, tp + r);
+		bb = make_real4(l - r, bt - r, rr + r, tp + r);
 	}
 	W.s_bb[s] = bb;
 }
 
 // GetBB (cpBBTree.c:82-98) with the body velocity (cpSpace.c:106)
-__device__ double4 fat_box(const DevWorld &W, int s, double4 bb)
+__device__ real4 fat_box(const DevWorld &W, int s, real4 bb)
 {
 	const BodyVel bv = W.b_vel[W.s_body[s]];
-	const V2 v = vmult(v2(bv.vx, bv.vy), 0.1);
-	const double x = (bb.z - bb.x) * 0.1;
-	const double y = (bb.w - bb.y) * 0.1;
-	return make_double4(bb.x + fminc(-x, v.x), bb.y + fminc(-y, v.y), bb.z + fmaxc(x, v.x), bb.w + fmaxc(y, v.y));
+	const V2 v = vmult(v2(bv.vx, bv.vy), real(0.1));
+	const real x = (bb.z - bb.x) * real(0.1);
+	const real y = (bb.w - bb.y) * real(0.1);
+	return make_real4(bb.x + fminc(-x, v.x), bb.y + fminc(-y, v.y), bb.z + fmaxc(x, v.x), bb.w + fmaxc(y, v.y));
 }
 
 __global__ void k_cache_shapes(const DevWorld W, const int32_t *shapes, int n, int leaf_update, int init_fat)
@@ -143,7 +145,7 @@ __global__ void k_cache_shapes(const DevWorld W, const int32_t *shapes, int n, i
 		W.s_fat[s] = fat_box(W, s, W.s_bb[s]);
 	} else if (leaf_update) {
 		// LeafUpdate (cpBBTree.c:498-517): rebuild the leaf box when the shape escaped it
-		const double4 f = W.s_fat[s], b = W.s_bb[s];
+		const real4 f = W.s_fat[s], b = W.s_bb[s];
 		const bool contains = f.x <= b.x && f.z >= b.z && f.y <= b.y && f.w >= b.w;
 		if (!contains) {
 			W.s_fat[s] = fat_box(W, s, b);
@@ -207,9 +209,9 @@ int colour_host(int n_bodies, const std::vector<int> &ba, const std::vector<int>
 // ---------------------------------------------------------------------------
 
 struct cpgpuWorld {
+	int32_t precision = sizeof(real) == 4 ? CPGPU_FP32 : CPGPU_FP64;   // first: read by the C ABI dispatch
 	int device = 0;
 	hipStream_t stream = nullptr;
-	int precision = CPGPU_FP64;
 	int nw = 0, nb = 0, ns = 0, nc = 0, npv = 0;
 	std::vector<cpgpuSpaceParams> sp;
 	std::vector<int32_t> body_type;
@@ -226,13 +228,13 @@ struct cpgpuWorld {
 	int32_t *d_all_shapes = nullptr;
 	WorldStep *d_ws = nullptr;
 	WorldStep *h_ws = nullptr;          // pinned
-	double *d_joint_bias_coef = nullptr;
+	real *d_joint_bias_coef = nullptr;
 	double last_joint_dt = -1.0;
 	std::vector<double> joint_error_bias;
 	// broadphase
 	Lbvh dyn_tree, static_tree;
 	RadixSortBuffers sortbuf;
-	double4 *d_world_bounds = nullptr;
+	real4 *d_world_bounds = nullptr;
 	int2 *d_pairs = nullptr;
 	int pair_cap = 0;
 	int32_t *d_counters = nullptr;      // [0] n_pairs
@@ -318,6 +320,32 @@ hipError_t own_upload(cpgpuWorld *w, T *&p, const T *src, size_t n)
 	hipError_t e = upload(p, src, n);
 	if (e == hipSuccess) w->owned.push_back((void *)p);
 	return e;
+}
+
+// Upload n_scalars doubles of the description as the build's real type (D is
+// real or real2; both builds read the same double description).
+template <typename D>
+hipError_t own_upload_real(cpgpuWorld *w, D *&p, const double *src, size_t n_scalars)
+{
+	std::vector<real> h(n_scalars);
+	for (size_t i = 0; i < n_scalars; i++) h[i] = (real)src[i];
+	return own_upload(w, p, reinterpret_cast<const D *>(h.data()), n_scalars * sizeof(real) / sizeof(D));
+}
+
+// Download n device reals as doubles.
+static hipError_t download_real(double *dst, const real *src, size_t n)
+{
+	std::vector<real> h(n);
+	hipError_t e = hipMemcpy(h.data(), src, sizeof(real) * n, hipMemcpyDeviceToHost);
+	for (size_t i = 0; e == hipSuccess && i < n; i++) dst[i] = (double)h[i];
+	return e;
+}
+
+static hipError_t upload_real_into(real *dst, const double *src, size_t n)
+{
+	std::vector<real> h(n);
+	for (size_t i = 0; i < n; i++) h[i] = (real)src[i];
+	return hipMemcpy(dst, h.data(), sizeof(real) * n, hipMemcpyHostToDevice);
 }
 
 template <typename T>
@@ -434,10 +462,7 @@ int sync_world(cpgpuWorld *w)
 
 } // namespace
 
-extern "C" {
-
-const char *cpgpuGetLastError(void) { return g_last_error.c_str(); }
-const char *cpgpuVersion(void) { return "cpgpu 0.1 (gfx950; Chipmunk2D 7.0.3 cpSpaceStep semantics)"; }
+// ---- the C ABI of this build (dispatch.hip forwards each call) ----
 
 // Distribute the small worlds over the size buckets (first bucket whose
 // max_bodies holds the world), largest first within a bucket, and size each
@@ -477,7 +502,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	*out = nullptr;
 	int rc = validate(d);
 	if (rc != CPGPU_OK) return rc;
-	if (precision != CPGPU_FP64) return fail(CPGPU_ERR_INVALID, "only CPGPU_FP64 is implemented");
+	if (precision != (sizeof(real) == 4 ? CPGPU_FP32 : CPGPU_FP64))
+		return fail(CPGPU_ERR_INVALID, "precision must be CPGPU_FP64 or CPGPU_FP32");
 	int ndev = 0;
 	if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(CPGPU_ERR_HIP, "no HIP device available");
 
@@ -503,21 +529,22 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	std::vector<BodyVel> vel(nb);
 	std::vector<XF> T(nb);
 	for (int b = 0; b < nb; b++) {
-		vel[b] = {d->body_v[2 * b], d->body_v[2 * b + 1], d->body_w[b], d->body_v_bias[2 * b],
-		          d->body_v_bias[2 * b + 1], d->body_w_bias[b], d->body_m_inv[b], d->body_i_inv[b]};
+		vel[b] = {(real)d->body_v[2 * b], (real)d->body_v[2 * b + 1], (real)d->body_w[b], (real)d->body_v_bias[2 * b],
+		          (real)d->body_v_bias[2 * b + 1], (real)d->body_w_bias[b], (real)d->body_m_inv[b],
+		          (real)d->body_i_inv[b]};
 		const double *t = d->body_transform + 6 * b;
-		T[b] = {t[0], t[1], t[2], t[3], t[4], t[5]};
+		T[b] = {(real)t[0], (real)t[1], (real)t[2], (real)t[3], (real)t[4], (real)t[5]};
 	}
 	int32_t *p_btype, *p_bworld;
-	double *p_bm;
+	real *p_bm;
 	HIP_OR_FAIL(own_upload(w, p_btype, d->body_type, nb)); W.b_type = p_btype;
 	HIP_OR_FAIL(own_upload(w, p_bworld, body_world.data(), nb)); W.b_world = p_bworld;
-	HIP_OR_FAIL(own_upload(w, p_bm, d->body_m, nb)); W.b_m = p_bm;
-	HIP_OR_FAIL(own_upload(w, W.b_p, (const double2 *)d->body_p, nb));
-	HIP_OR_FAIL(own_upload(w, W.b_a, d->body_a, nb));
-	HIP_OR_FAIL(own_upload(w, W.b_f, (const double2 *)d->body_f, nb));
-	HIP_OR_FAIL(own_upload(w, W.b_t, d->body_t, nb));
-	HIP_OR_FAIL(own_upload(w, W.b_cog, (const double2 *)d->body_cog, nb));
+	HIP_OR_FAIL(own_upload_real(w, p_bm, d->body_m, nb)); W.b_m = p_bm;
+	HIP_OR_FAIL(own_upload_real(w, W.b_p, d->body_p, 2 * (size_t)nb));
+	HIP_OR_FAIL(own_upload_real(w, W.b_a, d->body_a, nb));
+	HIP_OR_FAIL(own_upload_real(w, W.b_f, d->body_f, 2 * (size_t)nb));
+	HIP_OR_FAIL(own_upload_real(w, W.b_t, d->body_t, nb));
+	HIP_OR_FAIL(own_upload_real(w, W.b_cog, d->body_cog, 2 * (size_t)nb));
 	HIP_OR_FAIL(own_upload(w, W.b_T, T.data(), nb));
 	HIP_OR_FAIL(own_upload(w, W.b_vel, vel.data(), nb));
 
@@ -534,31 +561,31 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	{
 		int32_t *pt, *pb, *pw, *pps, *ppc;
 		uint64_t *ph, *pg;
-		double *pe, *pu, *pr, *pgeom;
-		double2 *psv;
+		real *pe, *pu, *pr, *pgeom;
+		real2 *psv;
 		uint32_t *pcat, *pmask;
 		uint8_t *pst;
 		HIP_OR_FAIL(own_upload(w, pt, d->shape_type, ns)); W.s_type = pt;
 		HIP_OR_FAIL(own_upload(w, pb, d->shape_body, ns)); W.s_body = pb;
 		HIP_OR_FAIL(own_upload(w, pw, shape_world.data(), ns)); W.s_world = pw;
 		HIP_OR_FAIL(own_upload(w, ph, d->shape_hashid, ns)); W.s_hashid = ph;
-		HIP_OR_FAIL(own_upload(w, pe, d->shape_e, ns)); W.s_e = pe;
-		HIP_OR_FAIL(own_upload(w, pu, d->shape_u, ns)); W.s_u = pu;
-		HIP_OR_FAIL(own_upload(w, psv, (const double2 *)d->shape_surface_v, ns)); W.s_sv = psv;
+		HIP_OR_FAIL(own_upload_real(w, pe, d->shape_e, ns)); W.s_e = pe;
+		HIP_OR_FAIL(own_upload_real(w, pu, d->shape_u, ns)); W.s_u = pu;
+		HIP_OR_FAIL(own_upload_real(w, psv, d->shape_surface_v, 2 * (size_t)ns)); W.s_sv = psv;
 		HIP_OR_FAIL(own_upload(w, pg, d->shape_group, ns)); W.s_group = pg;
 		HIP_OR_FAIL(own_upload(w, pcat, d->shape_categories, ns)); W.s_cats = pcat;
 		HIP_OR_FAIL(own_upload(w, pmask, d->shape_mask, ns)); W.s_mask = pmask;
-		HIP_OR_FAIL(own_upload(w, pr, d->shape_radius, ns)); W.s_r = pr;
-		HIP_OR_FAIL(own_upload(w, pgeom, d->shape_geom, 10 * (size_t)ns)); W.s_geom = pgeom;
+		HIP_OR_FAIL(own_upload_real(w, pr, d->shape_radius, ns)); W.s_r = pr;
+		HIP_OR_FAIL(own_upload_real(w, pgeom, d->shape_geom, 10 * (size_t)ns)); W.s_geom = pgeom;
 		std::vector<int32_t> pstart(ns, 0), pcount(ns, 0);
 		for (int s = 0; s < ns; s++)
 			if (d->shape_type[s] == CPGPU_SHAPE_POLY) { pstart[s] = d->shape_poly_start[s]; pcount[s] = d->shape_poly_count[s]; }
 		HIP_OR_FAIL(own_upload(w, pps, pstart.data(), ns)); W.s_pstart = pps;
 		HIP_OR_FAIL(own_upload(w, ppc, pcount.data(), ns)); W.s_pcount = ppc;
 		HIP_OR_FAIL(own_upload(w, pst, shape_static.data(), ns)); W.s_static = pst;
-		double2 *pvl, *pnl;
-		HIP_OR_FAIL(own_upload(w, pvl, (const double2 *)d->poly_verts, w->npv)); W.pv_local = pvl;
-		HIP_OR_FAIL(own_upload(w, pnl, (const double2 *)d->poly_normals, w->npv)); W.pn_local = pnl;
+		real2 *pvl, *pnl;
+		HIP_OR_FAIL(own_upload_real(w, pvl, d->poly_verts, 2 * (size_t)w->npv)); W.pv_local = pvl;
+		HIP_OR_FAIL(own_upload_real(w, pnl, d->poly_normals, 2 * (size_t)w->npv)); W.pn_local = pnl;
 		HIP_OR_FAIL(own_alloc(w, W.pv_world, w->npv));
 		HIP_OR_FAIL(own_alloc(w, W.pn_world, w->npv));
 		HIP_OR_FAIL(own_alloc(w, W.s_wgeom, 6 * (size_t)ns));
@@ -608,11 +635,12 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 			Joint &j = J[c];
 			memset(&j, 0, sizeof(j));
 			j.type = d->cons_type[c]; j.a = d->cons_a[c]; j.b = d->cons_b[c]; j.world = body_world[j.a];
-			j.maxForce = d->cons_max_force[c]; j.errorBias = d->cons_error_bias[c]; j.maxBias = d->cons_max_bias[c];
-			memcpy(j.P, d->cons_params + 8 * c, 8 * sizeof(double));
+			j.maxForce = (real)d->cons_max_force[c]; j.errorBias = (real)d->cons_error_bias[c];
+			j.maxBias = (real)d->cons_max_bias[c];
+			for (int k = 0; k < 8; k++) j.P[k] = (real)d->cons_params[8 * c + k];
 			const double j0 = d->cons_jacc ? d->cons_jacc[2 * c] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * c + 1] : 0.0;
-			j.jAx = j0; j.jAy = j1; j.jAcc = j0; j.jnAcc = j0;
-			w->joint_error_bias[c] = j.errorBias;
+			j.jAx = (real)j0; j.jAy = (real)j1; j.jAcc = (real)j0; j.jnAcc = (real)j0;
+			w->joint_error_bias[c] = d->cons_error_bias[c];
 			ba[c] = d->body_type[j.a] == CPGPU_BODY_DYNAMIC ? j.a : -1;
 			bbv[c] = d->body_type[j.b] == CPGPU_BODY_DYNAMIC ? j.b : -1;
 			prio[c] = mix64_host((uint64_t)(uint32_t)(c - d->world_constraint_start[j.world]) | 0x8000000000000000ull);
@@ -876,15 +904,16 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	if (w->nc > 0 && dt != w->last_joint_dt) {
 		// bias_coef (chipmunk_private.h:264-268) per joint, libm pow on the host (once per
 		// distinct errorBias), then scattered into the joints on the device
-		std::vector<double> bc(w->nc);
-		double last_eb = 0.0, last_bc = 0.0;
+		std::vector<real> bc(w->nc);
+		double last_eb = 0.0;
+		real last_bc = 0.0;
 		bool have = false;
 		for (int c = 0; c < w->nc; c++) {
 			const double eb = w->joint_error_bias[c];
-			if (!have || eb != last_eb) { last_eb = eb; last_bc = 1.0 - pow(eb, dt); have = true; }
+			if (!have || eb != last_eb) { last_eb = eb; last_bc = (real)(1.0 - pow(eb, dt)); have = true; }
 			bc[c] = last_bc;
 		}
-		HIP_OR_FAIL(hipMemcpy(w->d_joint_bias_coef, bc.data(), sizeof(double) * w->nc, hipMemcpyHostToDevice));
+		HIP_OR_FAIL(hipMemcpy(w->d_joint_bias_coef, bc.data(), sizeof(real) * w->nc, hipMemcpyHostToDevice));
 		k_set_joint_bias<<<blocks_for(w->nc, 256), 256, 0, w->stream>>>(W.joints, w->d_joint_bias_coef, w->nc);
 		HIP_OR_FAIL(hipGetLastError());
 		w->last_joint_dt = dt;
@@ -1019,9 +1048,9 @@ int cpgpuWorldReadBodies(cpgpuWorld *w, double *p, double *v, double *a, double 
 	int rc = cpgpuWorldSync(w);
 	if (rc != CPGPU_OK) return rc;
 	const int nb = w->nb;
-	if (p) HIP_OR_FAIL(hipMemcpy(p, w->W.b_p, sizeof(double2) * nb, hipMemcpyDeviceToHost));
-	if (a) HIP_OR_FAIL(hipMemcpy(a, w->W.b_a, sizeof(double) * nb, hipMemcpyDeviceToHost));
-	if (transform) HIP_OR_FAIL(hipMemcpy(transform, w->W.b_T, sizeof(XF) * nb, hipMemcpyDeviceToHost));
+	if (p) HIP_OR_FAIL(download_real(p, reinterpret_cast<const real *>(w->W.b_p), 2 * (size_t)nb));
+	if (a) HIP_OR_FAIL(download_real(a, w->W.b_a, nb));
+	if (transform) HIP_OR_FAIL(download_real(transform, reinterpret_cast<const real *>(w->W.b_T), 6 * (size_t)nb));
 	if (v || wv || v_bias || w_bias) {
 		std::vector<BodyVel> vel(nb);
 		HIP_OR_FAIL(hipMemcpy(vel.data(), w->W.b_vel, sizeof(BodyVel) * nb, hipMemcpyDeviceToHost));
@@ -1044,13 +1073,13 @@ int cpgpuWorldWriteBodies(cpgpuWorld *w, const double *v, const double *wv, cons
 		std::vector<BodyVel> vel(nb);
 		HIP_OR_FAIL(hipMemcpy(vel.data(), w->W.b_vel, sizeof(BodyVel) * nb, hipMemcpyDeviceToHost));
 		for (int b = 0; b < nb; b++) {
-			if (v) { vel[b].vx = v[2 * b]; vel[b].vy = v[2 * b + 1]; }
-			if (wv) vel[b].w = wv[b];
+			if (v) { vel[b].vx = (real)v[2 * b]; vel[b].vy = (real)v[2 * b + 1]; }
+			if (wv) vel[b].w = (real)wv[b];
 		}
 		HIP_OR_FAIL(hipMemcpy(w->W.b_vel, vel.data(), sizeof(BodyVel) * nb, hipMemcpyHostToDevice));
 	}
-	if (f) HIP_OR_FAIL(hipMemcpy(w->W.b_f, f, sizeof(double2) * nb, hipMemcpyHostToDevice));
-	if (t) HIP_OR_FAIL(hipMemcpy(w->W.b_t, t, sizeof(double) * nb, hipMemcpyHostToDevice));
+	if (f) HIP_OR_FAIL(upload_real_into(reinterpret_cast<real *>(w->W.b_f), f, 2 * (size_t)nb));
+	if (t) HIP_OR_FAIL(upload_real_into(w->W.b_t, t, nb));
 	return CPGPU_OK;
 }
 
@@ -1231,6 +1260,7 @@ int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	if (checkpoint_every < 1) return fail(CPGPU_ERR_INVALID, "checkpoint_every must be positive");
+	if (sizeof(real) != 8) return fail(CPGPU_ERR_INVALID, "the adjoint needs a CPGPU_FP64 world");
 	if (w->n_large_worlds > 0)
 		return fail(CPGPU_ERR_INVALID, "the adjoint supports spaces solved one workgroup each (at most " +
 		                                   std::to_string(w->small_world_max_bodies) + " dynamic bodies)");
@@ -1466,4 +1496,4 @@ int cpgpuWorldGetTiming(cpgpuWorld *w, int32_t cap, int32_t *n_out, double *ms, 
 	return CPGPU_OK;
 }
 
-} // extern "C"
+} // namespace CPGPU_NS

@@ -444,7 +444,37 @@ __device__ __forceinline__ bool pair_ok(const DevWorld &W, int s, int t)
 	return true;
 }
 
-__device__ __forceinline__ void emit_pair(const DevWorld &W, int s, int t, int2 *pairs, int cap, int32_t *count)
+// Candidate pairs are staged per wave in LDS and written out PAIR_STAGE at a
+// time with one atomic on the global count.  Pairs are found inside divergent
+// traversal loops, a few lanes at a time: an atomic per find puts millions
+// of same-address atomics per step on one counter.
+constexpr int PAIR_STAGE = 128;
+
+struct PairStage {
+	int2 *buf;              // this wave's PAIR_STAGE slots
+	volatile int32_t *n;    // pairs staged (read by every lane, written by one)
+	int2 *pairs;
+	int cap;
+	int32_t *count;
+};
+
+// the active lanes copy the staged pairs out; one atomic reserves their range
+__device__ __forceinline__ void stage_flush(const PairStage &st, uint64_t active, int staged)
+{
+	const int lane = __lane_id();
+	const int leader = __ffsll((long long)active) - 1;
+	const int n_act = __popcll(active);
+	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+	const int rank = __popcll(active & lt);
+	int base = 0;
+	if (lane == leader) base = atomicAdd(st.count, staged);
+	base = __shfl(base, leader);
+	for (int i = rank; i < staged; i += n_act)
+		if (base + i < st.cap) st.pairs[base + i] = st.buf[i];
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+__device__ __forceinline__ void emit_pair(const DevWorld &W, int s, int t, const PairStage &st)
 {
 	// orientation of cpCollide (cpCollision.c:707-710); same types: lower index first
 	const int ts = W.s_type[s], tt = W.s_type[t];
@@ -455,11 +485,78 @@ __device__ __forceinline__ void emit_pair(const DevWorld &W, int s, int t, int2 
 	const int leader = __ffsll((long long)active) - 1;
 	const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 	const int rank = __popcll(active & lt);
-	int base = 0;
-	if (lane == leader) base = atomicAdd(count, __popcll(active));
-	base = __shfl(base, leader);
-	const int idx = base + rank;
-	if (idx < cap) pairs[idx] = make_int2(a, b);
+	const int n_act = __popcll(active);
+	int staged = *st.n;
+	if (staged + n_act > PAIR_STAGE) {
+		stage_flush(st, active, staged);
+		staged = 0;
+	}
+	st.buf[staged + rank] = make_int2(a, b);
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+	if (lane == leader) *st.n = staged + n_act;
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+
+// one dynamic shape's queries: pairs with later leaves of its world's dynamic
+// tree and with its world's static tree
+__device__ __forceinline__ void traverse_pairs(const DevWorld &W, int q, int nd, const uint64_t *dkeys,
+                                               const int32_t *dshape, const int32_t *dchild, const int32_t *drange,
+                                               const real4 *dbox, const int32_t *droot, int ns, const int32_t *sshape,
+                                               const int32_t *schild, const real4 *sbox, const int32_t *sroot,
+                                               int32_t *stk, const PairStage &st)
+{
+	const int s = dshape[q];
+	const real4 bb = W.s_bb[s];
+	const int w = (int)(dkeys[q] >> WORLD_SHIFT);
+	int sp = 0;
+	// dynamic tree: each pair is reported by its lower leaf, so only leaves after q
+	{
+		const int root = droot[w];
+		if (root >= nd - 1) {
+			// a one-leaf world: nothing after q
+		} else if (drange[2 * root + 1] > q && box_overlap(bb, dbox[root])) {
+			stk[256 * sp++] = root;
+		}
+	}
+	while (sp > 0) {
+		const int node = stk[256 * --sp];
+		for (int side = 0; side < 2; side++) {
+			const int c = dchild[2 * node + side];
+			if (c >= nd - 1) {
+				const int p = c - (nd - 1);
+				if (p <= q) continue;
+				const int t = dshape[p];
+				if (box_overlap(bb, dbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, st);
+			} else if (drange[2 * c + 1] > q && box_overlap(bb, dbox[c])) {
+				if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
+				stk[256 * sp++] = c;
+			}
+		}
+	}
+	// static tree
+	if (ns > 0 && sroot[w] >= 0) {
+		const int root = sroot[w];
+		if (root >= ns - 1) {
+			const int t = sshape[root - (ns - 1)];
+			if (box_overlap(bb, sbox[root]) && pair_ok(W, s, t)) emit_pair(W, s, t, st);
+		} else if (box_overlap(bb, sbox[root])) {
+			stk[256 * sp++] = root;
+		}
+		while (sp > 0) {
+			const int node = stk[256 * --sp];
+			for (int side = 0; side < 2; side++) {
+				const int c = schild[2 * node + side];
+				if (c >= ns - 1) {
+					const int t = sshape[c - (ns - 1)];
+					if (box_overlap(bb, sbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, st);
+				} else if (box_overlap(bb, sbox[c])) {
+					if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
+					stk[256 * sp++] = c;
+				}
+			}
+		}
+	}
 }
 
 __global__ void __launch_bounds__(256) k_pairs(const DevWorld W, int nd, const uint64_t *dkeys,
@@ -470,61 +567,20 @@ __global__ void __launch_bounds__(256) k_pairs(const DevWorld W, int nd, const u
                                                int32_t *count)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q >= nd) return;
-	const int s = dshape[q];
-	const real4 bb = W.s_bb[s];
-	const int w = (int)(dkeys[q] >> WORLD_SHIFT);
 	// per-lane stack in LDS, laid out [depth][lane] so lanes at one depth hit distinct banks
 	__shared__ int32_t stk[TRAVERSAL_STACK][256];
+	__shared__ int2 stage_buf[256 / 64][PAIR_STAGE];
+	__shared__ int32_t stage_n[256 / 64];
 	const int lane = threadIdx.x;
-	int sp = 0;
-	// dynamic tree: each pair is reported by its lower leaf, so only leaves after q
-	{
-		const int root = droot[w];
-		if (root >= nd - 1) {
-			// a one-leaf world: nothing after q
-		} else if (drange[2 * root + 1] > q && box_overlap(bb, dbox[root])) {
-			stk[sp++][lane] = root;
-		}
-	}
-	while (sp > 0) {
-		const int node = stk[--sp][lane];
-		for (int side = 0; side < 2; side++) {
-			const int c = dchild[2 * node + side];
-			if (c >= nd - 1) {
-				const int p = c - (nd - 1);
-				if (p <= q) continue;
-				const int t = dshape[p];
-				if (box_overlap(bb, dbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
-			} else if (drange[2 * c + 1] > q && box_overlap(bb, dbox[c])) {
-				if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
-				stk[sp++][lane] = c;
-			}
-		}
-	}
-	// static tree
-	if (ns > 0 && sroot[w] >= 0) {
-		const int root = sroot[w];
-		if (root >= ns - 1) {
-			const int t = sshape[root - (ns - 1)];
-			if (box_overlap(bb, sbox[root]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
-		} else if (box_overlap(bb, sbox[root])) {
-			stk[sp++][lane] = root;
-		}
-		while (sp > 0) {
-			const int node = stk[--sp][lane];
-			for (int side = 0; side < 2; side++) {
-				const int c = schild[2 * node + side];
-				if (c >= ns - 1) {
-					const int t = sshape[c - (ns - 1)];
-					if (box_overlap(bb, sbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, pairs, cap, count);
-				} else if (box_overlap(bb, sbox[c])) {
-					if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
-					stk[sp++][lane] = c;
-				}
-			}
-		}
-	}
+	const int wave = threadIdx.x >> 6;
+	if (__lane_id() == 0) stage_n[wave] = 0;
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+	const PairStage st{stage_buf[wave], &stage_n[wave], pairs, cap, count};
+	if (q < nd) traverse_pairs(W, q, nd, dkeys, dshape, dchild, drange, dbox, droot, ns, sshape, schild, sbox, sroot,
+	                           &stk[0][lane], st);
+	// every lane of the wave is here again: write out what is left
+	const int staged = *st.n;
+	if (staged > 0) stage_flush(st, __ballot(1), staged);
 }
 
 inline int blocks_for(int n, int t) { return (n + t - 1) / t; }

@@ -12,6 +12,16 @@
 
 #include "../../include/cpgpu.h"
 
+// The library is compiled twice: the FP64 step (namespace cpgpu64, real =
+// double) and the FP32 step (namespace cpgpu32, real = float); dispatch.hip
+// routes each C ABI call to the world's own build.
+#ifndef CPGPU_NS
+#define CPGPU_NS cpgpu64
+#endif
+#ifndef CPGPU_REAL
+#define CPGPU_REAL double
+#endif
+
 namespace CPGPU_NS {
 
 // The arithmetic type of the device state and the step.
@@ -32,16 +42,6 @@ __host__ __device__ inline real4 make_real4(real x, real y, real z, real w)
 }
 // smallest positive normal value (CPFLOAT_MIN, chipmunk_types.h)
 #define CP_REAL_MIN (sizeof(real) == 4 ? (real)1.17549435e-38f : (real)2.2250738585072014e-308)
-
-// The library is compiled twice: the FP64 step (namespace cpgpu64, real =
-// double) and the FP32 step (namespace cpgpu32, real = float); dispatch.hip
-// routes each C ABI call to the world's own build.
-#ifndef CPGPU_NS
-#define CPGPU_NS cpgpu64
-#endif
-#ifndef CPGPU_REAL
-#define CPGPU_REAL double
-#endif
 
 #define CPD __device__ __forceinline__
 

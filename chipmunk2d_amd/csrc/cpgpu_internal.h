@@ -18,7 +18,7 @@ struct KernelTimer {
 	struct Rec { const char *name; hipEvent_t a, b; };
 	std::vector<Rec> pending;
 	std::vector<hipEvent_t> pool;
-	std::map<std::string, std::pair<real, long>> acc;   // name -> (ms, launches)
+	std::map<std::string, std::pair<double, long>> acc;   // name -> (ms, launches)
 	hipEvent_t take() {
 		if (pool.empty()) { hipEvent_t e; (void)hipEventCreate(&e); return e; }
 		hipEvent_t e = pool.back(); pool.pop_back(); return e;

@@ -65,6 +65,10 @@ struct SolveBuffers {
 	Arbiter *act;
 	const int32_t *n_act;
 	int32_t *colour;           // [cap] colour of each active arbiter
+	// colouring scratch, indexed by position in arb_by_world (solve-set position + order_base):
+	int2 *cbody;               // [cap] the arbiter's dynamic bodies, -1 for a static or kinematic one
+	unsigned long long *cprio; // [cap] its colouring priority
+	int32_t *ccol;             // [cap] its colour while colouring
 	int32_t *order;            // [cap] active arbiters in colour order (per workgroup solve: its own segment)
 	int32_t *arb_by_world;     // [cap] active arbiters sorted by key; nullptr for a single small space
 	uint64_t *keys;            // [cap] the sorted keys

@@ -25,6 +25,10 @@ namespace {
 
 constexpr int GRID_THREADS = 1024;  // grid solve: few, large workgroups keep the barrier cheap
 
+// s_waitcnt vmcnt(0): every vector memory access this wave issued has
+// completed (gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15)
+__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---- grid barrier (all blocks co-resident: cooperative launch) ----
 // One lane per workgroup: a release fence (writes back this XCD's L2) before
 // arriving, a relaxed spin on the generation word, one acquire fence
@@ -42,6 +46,7 @@ __device__ void grid_sync(GridBarrier *bar, int32_t *err)
 			// every other workgroup has arrived: its writes precede its arrival
 			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
 			__hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			wait_vm();   // the reset lands before any workgroup can see the new generation and arrive again
 			__hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		} else {
 			long spins = 0;
@@ -401,10 +406,16 @@ __device__ __forceinline__ unsigned long long arbiter_priority(const DevWorld &W
 
 // Synchronisation of the threads taking part: one workgroup, or the whole
 // co-resident grid.
+// sweep_tid() numbers the threads for the impulse sweeps.  On the grid it
+// interleaves the workgroups (thread t of workgroup g is g + t * gridDim.x),
+// so the arbiters of a colour, however few, are spread over every workgroup
+// instead of filling the first ones: a sweep is bound by each CU's scattered
+// arbiter and body loads.
 struct BlockSync {
 	static constexpr bool grid = false;
 	int tid, stride;
 	__device__ void sync() { __syncthreads(); }
+	__device__ int sweep_tid() const { return tid; }
 };
 struct GridSync {
 	static constexpr bool grid = true;
@@ -412,6 +423,7 @@ struct GridSync {
 	GridBarrier *bar;
 	int32_t *err;
 	__device__ void sync() { grid_sync(bar, err); }
+	__device__ int sweep_tid() const { return (int)(blockIdx.x + threadIdx.x * gridDim.x); }
 };
 
 // The elements one solve covers.  Arbiters are list[0..n) (indices into
@@ -446,18 +458,23 @@ __device__ __forceinline__ void wave_add(unsigned long long *counter, unsigned l
 	if (__lane_id() == 0 && v) atomicAdd(counter, v);
 }
 
-// One colouring claim round over arbiters list[0..n) (see the file comment).
-// Returns the number of arbiters that took colour c.
-__device__ int colour_round(const DevWorld &W, const SolveBuffers &S, const int32_t *list, int n, int tid,
-                            int stride, int c, unsigned long long *cl, unsigned long long *other, bool claim_phase)
+// One colouring claim round (see the file comment) over the arbiters at
+// positions idx[0..n) of the solve set (idx == nullptr: 0..n-1).  Each
+// arbiter's dynamic bodies (-1 for none), priority and colour were gathered
+// into the position-indexed arrays cbody / cprio / ccol, so a round reads
+// them coalesced instead of scattered arbiter records.  Returns the number of
+// arbiters that took colour c.
+__device__ int colour_round(const SolveBuffers &S, const int2 *cbody, const unsigned long long *cprio, int32_t *ccol,
+                            const int32_t *idx, int n, int tid, int stride, int c, unsigned long long *cl,
+                            unsigned long long *other, bool claim_phase)
 {
 	int won = 0;
-	for (int k = tid; k < n; k += stride) {
-		const int a = list ? list[k] : k;
-		if (S.colour[a] >= 0) continue;
-		const Arbiter &arb = S.act[a];
-		const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
-		const unsigned long long prio = arbiter_priority(W, arb);
+	for (int j = tid; j < n; j += stride) {
+		const int k = idx ? idx[j] : j;
+		if (ccol[k] >= 0) continue;
+		const int2 bd = cbody[k];
+		const int ba = bd.x, bb = bd.y;
+		const unsigned long long prio = cprio[k];
 		if (claim_phase) {
 			if ((ba >= 0 && S.taken[ba] == c) || (bb >= 0 && S.taken[bb] == c)) continue;
 			if (ba >= 0) atomicMin(&cl[ba], prio);
@@ -466,7 +483,7 @@ __device__ int colour_round(const DevWorld &W, const SolveBuffers &S, const int3
 			const bool win = (ba < 0 || __hip_atomic_load(&cl[ba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio) &&
 			                 (bb < 0 || __hip_atomic_load(&cl[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio);
 			if (win) {
-				S.colour[a] = c;
+				ccol[k] = c;
 				if (ba >= 0) S.taken[ba] = c;
 				if (bb >= 0) S.taken[bb] = c;
 				won++;
@@ -491,7 +508,7 @@ __device__ void for_colours(Sync &sy, const int32_t *col_off, int n_colours, int
 	while (q < n_colours) {
 		const int cnt = col_off[q + 1] - col_off[q];
 		if (!Sync::grid || cnt > tail_max) {
-			for (int k = col_off[q] + sy.tid; k < col_off[q + 1]; k += sy.stride) f(k);
+			for (int k = col_off[q] + sy.sweep_tid(); k < col_off[q + 1]; k += sy.stride) f(k);
 			sy.sync();
 			q++;
 		} else {
@@ -527,7 +544,7 @@ template <class Sync, class BA>
 __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy, int n_colours,
                                const BA &bodies)
 {
-	const int tid = sy.tid, stride = sy.stride;
+	const int tid = sy.sweep_tid(), stride = sy.stride;
 	const int tail = S.solve_tail_max;
 	for_colours(sy, T.col_off, n_colours, tail, [&](int k) { arb_apply_cached(W, bodies, S.act[T.order[k]]); });
 	for (int jc = 0; jc < T.njcol; jc++) {
@@ -559,12 +576,17 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	const int tid = sy.tid, stride = sy.stride, n = T.n;
 	unsigned long long *claim0 = S.claim, *claim1 = S.claim + S.n_bodies;
 	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
+	int2 *cbody = S.cbody + T.order_base;
+	unsigned long long *cprio = S.cprio + T.order_base;
+	int32_t *ccol = S.ccol + T.order_base;
 
 	// ---- colouring ----
 	for (int k = tid; k < n; k += stride) {
-		const int a = arb_index(k);
-		S.colour[a] = -1;
-		const int ba = dyn_or_neg(W, S.act[a].ba), bb = dyn_or_neg(W, S.act[a].bb);
+		const Arbiter &arb = S.act[arb_index(k)];
+		const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
+		cbody[k] = make_int2(ba, bb);
+		cprio[k] = arbiter_priority(W, arb);
+		ccol[k] = -1;
 		if (ba >= 0) { claim0[ba] = ~0ull; claim1[ba] = ~0ull; S.taken[ba] = -1; }
 		if (bb >= 0) { claim0[bb] = ~0ull; claim1[bb] = ~0ull; S.taken[bb] = -1; }
 	}
@@ -582,19 +604,18 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		if (c >= CPGPU_MAX_COLOURS) { if (tid == 0) raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); break; }
 		if (Sync::grid && left <= S.colour_tail_max) {
 			// the last few arbiters: gather them, then workgroup 0 finishes the colouring alone
-			for (int k = tid; k < n; k += stride) {
-				const int a = arb_index(k);
-				if (S.colour[a] < 0) S.tail_list[atomicAdd(S.tail_n, 1)] = a;
-			}
+			for (int k = tid; k < n; k += stride)
+				if (ccol[k] < 0) S.tail_list[atomicAdd(S.tail_n, 1)] = k;
 			sy.sync();
 			if (blockIdx.x == 0) {
 				const int nt = __hip_atomic_load(S.tail_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 				while (__hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0 && c < CPGPU_MAX_COLOURS) {
 					unsigned long long *cl = (sub & 1) ? claim1 : claim0;
 					unsigned long long *other = (sub & 1) ? claim0 : claim1;
-					colour_round(W, S, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other, true);
+					colour_round(S, cbody, cprio, ccol, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other, true);
 					__syncthreads();
-					const int won = colour_round(W, S, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other, false);
+					const int won = colour_round(S, cbody, cprio, ccol, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other,
+					                             false);
 					if (won) atomicSub(T.left, won);
 					__syncthreads();
 					if (++sub % COLOUR_SUBROUNDS == 0) c++;
@@ -608,9 +629,9 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		}
 		unsigned long long *cl = (sub & 1) ? claim1 : claim0;
 		unsigned long long *other = (sub & 1) ? claim0 : claim1;
-		colour_round(W, S, T.list, n, tid, stride, c, cl, other, true);
+		colour_round(S, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, true);
 		sy.sync();
-		const int won = colour_round(W, S, T.list, n, tid, stride, c, cl, other, false);
+		const int won = colour_round(S, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, false);
 		if (won) atomicSub(T.left, won);
 		sy.sync();
 		if (++sub % COLOUR_SUBROUNDS == 0) c++;
@@ -628,7 +649,8 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		__syncthreads();
 		for (int k = tid; k < n; k += stride) {
 			const int a = arb_index(k);
-			const int col = S.colour[a];
+			const int col = ccol[k];
+			S.colour[a] = col;
 			if (col >= 0) atomicAdd(&lcount[col], 1);
 			n_contacts += S.act[a].count;
 		}
@@ -655,15 +677,15 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		}
 		__syncthreads();
 		for (int k = tid; k < n; k += stride) {
-			const int a = arb_index(k);
-			const int col = S.colour[a];
+			const int col = ccol[k];
 			if (col < 0) continue;
-			T.order[lbase[col] + atomicAdd(&lcount[col], 1)] = a;
+			T.order[lbase[col] + atomicAdd(&lcount[col], 1)] = arb_index(k);
 		}
 	} else {
 		for (int k = tid; k < n; k += stride) {
 			const int a = arb_index(k);
-			const int col = S.colour[a];
+			const int col = ccol[k];
+			S.colour[a] = col;
 			if (col >= 0) atomicAdd(&T.col_off[col], 1);
 			n_contacts += S.act[a].count;
 		}
@@ -682,11 +704,10 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		}
 		sy.sync();
 		for (int k = tid; k < n; k += stride) {
-			const int a = arb_index(k);
-			const int col = S.colour[a];
+			const int col = ccol[k];
 			if (col < 0) continue;
 			const int pos = atomicAdd(&T.col_cursor[col], 1);
-			T.order[pos] = a;
+			T.order[pos] = arb_index(k);
 		}
 	}
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----

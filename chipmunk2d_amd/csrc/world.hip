@@ -19,6 +19,13 @@
 
 #include "cpgpu_errors.h"
 #include "cpgpu_step.h"
+
+#ifndef CPGPU_COLOUR_TAIL_MAX
+#define CPGPU_COLOUR_TAIL_MAX 4096
+#endif
+#ifndef CPGPU_SOLVE_TAIL_MAX
+#define CPGPU_SOLVE_TAIL_MAX 256
+#endif
 #include "cpgpu_adjoint.h"
 
 namespace CPGPU_NS {
@@ -254,6 +261,9 @@ struct cpgpuWorld {
 	int seen_cap = 0;
 	// solver
 	int32_t *colour = nullptr, *order = nullptr;
+	int2 *cbody = nullptr;              // colouring scratch (SolveBuffers::cbody / cprio / ccol)
+	unsigned long long *cprio = nullptr;
+	int32_t *ccol = nullptr;
 	int colour_cap = 0;
 	int32_t *col_off = nullptr, *col_cursor = nullptr, *n_colours = nullptr, *left = nullptr;
 	unsigned long long *claim = nullptr;
@@ -296,7 +306,9 @@ struct cpgpuWorld {
 	} tape;
 	int32_t *d_replay_n = nullptr;
 	// grid-solve tails (see SolveBuffers)
-	int colour_tail_max = 4096, solve_tail_max = 2048;
+	// colouring finishes on one workgroup once at most colour_tail_max arbiters are left; runs of colours
+	// of at most solve_tail_max arbiters are swept by one workgroup between two grid barriers
+	int colour_tail_max = CPGPU_COLOUR_TAIL_MAX, solve_tail_max = CPGPU_SOLVE_TAIL_MAX;
 	int32_t *tail_list = nullptr, *tail_n = nullptr;
 	int n_jcolours = 0;
 	std::vector<int32_t> h_jcolour;
@@ -797,7 +809,8 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	lbvh_free(w->static_tree);
 	radix_sort_free(w->sortbuf);
 	void *extra[] = {w->d_pairs, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
-	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys, w->groups};
+	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys, w->groups,
+	                 w->cbody, w->cprio, w->ccol};
 	for (void *p : extra) if (p) (void)hipFree(p);
 	if (w->h_ws) (void)hipHostFree(w->h_ws);
 	if (w->h_counts) (void)hipHostFree(w->h_counts);
@@ -820,6 +833,7 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	SolveBuffers S;
 	S.act = act; S.n_act = n_act;
 	S.colour = colour; S.order = order;
+	S.cbody = w->cbody; S.cprio = w->cprio; S.ccol = w->ccol;
 	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
 	S.arb_by_world = grouped ? w->arb_by_world : nullptr;
 	S.keys = w->world_keys;
@@ -967,7 +981,7 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	}
 	HIP_OR_FAIL(regrow(w->seen, w->seen_cap, (long)n_prev_recs + 1));
 	if (w->act_cap[cur] > w->colour_cap) {
-		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups};
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups, w->cbody, w->cprio, w->ccol};
 		for (void *p : old) if (p) (void)hipFree(p);
 		w->colour_cap = w->act_cap[cur];
 		HIP_OR_FAIL(dalloc(w->colour, w->colour_cap));
@@ -975,6 +989,9 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		HIP_OR_FAIL(dalloc(w->arb_by_world, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->world_keys, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->groups, w->colour_cap));
+		HIP_OR_FAIL(dalloc(w->cbody, w->colour_cap));
+		HIP_OR_FAIL(dalloc(w->cprio, w->colour_cap));
+		HIP_OR_FAIL(dalloc(w->ccol, w->colour_cap));
 	}
 	if (w->steps == 0) HIP_OR_FAIL(hipMemsetAsync(w->d_step_counts + 3 * prev, 0, 3 * sizeof(int32_t), w->stream));
 
@@ -1374,7 +1391,7 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	ADJ_TRY(grab((void **)&totals_saved, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS)));
 	// replay scratch sized for the largest step
 	if (max_n > w->colour_cap) {
-		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups};
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups, w->cbody, w->cprio, w->ccol};
 		for (void *p : old) if (p) (void)hipFree(p);
 		w->colour_cap = max_n;
 		ADJ_TRY(dalloc(w->colour, w->colour_cap));
@@ -1382,6 +1399,9 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 		ADJ_TRY(dalloc(w->arb_by_world, w->colour_cap));
 		ADJ_TRY(dalloc(w->world_keys, w->colour_cap));
 		ADJ_TRY(dalloc(w->groups, w->colour_cap));
+		ADJ_TRY(dalloc(w->cbody, w->colour_cap));
+		ADJ_TRY(dalloc(w->cprio, w->colour_cap));
+		ADJ_TRY(dalloc(w->ccol, w->colour_cap));
 	}
 	// seeds: the adjoint of the state after the last step
 	{

@@ -324,6 +324,319 @@ __device__ bool dual_shape(const DevWorld &W, int s, const XFT<Dual> &T, DualSha
 	return true;
 }
 
+// ---- the reverse of the joints (cpPivotJoint.c:24-70, cpDampedSpring.c:24-77,
+// cpSlideJoint.c:24-89, cpGearJoint.c:24-69) ----
+
+// per joint: adjoints of its pre-step results, accumulated over the sweeps,
+// and of its accumulators (pivot jAcc; spring x: target_vrn; slide, gear x)
+struct JointAdj {
+	V2 r1, r2, n, bias;        // bias: pivot (bx, by); slide and gear in x
+	double k[4];               // pivot k11 k12 k21 k22
+	double nMass, v_coef, iSum;
+	V2 acc;
+};
+
+struct JV {
+	V2 v;
+	double w;
+};
+
+__device__ __forceinline__ JV jv_from(const real *r)
+{
+	JV x;
+	x.v = v2(r[0], r[1]); x.w = r[2];
+	return x;
+}
+
+__device__ __forceinline__ V2 jv_relvel(const JV &a, const JV &b, V2 r1, V2 r2)
+{
+	return vsub(vadd(b.v, vmult(vperp(r2), b.w)), vadd(a.v, vmult(vperp(r1), a.w)));
+}
+
+// Adjoint of apply_impulses(a, b, r1, r2, J) (chipmunk_private.h:187-199):
+// returns dL/dJ; adds to the anchors' and inverse masses' adjoints.  The
+// velocity adjoints pass through unchanged (the impulse is added).
+__device__ __forceinline__ V2 impulse_adj(const JV &ga, const JV &gb, double mA, double iA, double mB, double iB,
+                                          V2 r1, V2 r2, V2 J, V2 &g_r1, V2 &g_r2, double gm[4])
+{
+	gm[0] -= vdot(ga.v, J); gm[1] -= ga.w * vcross(r1, J);
+	gm[2] += vdot(gb.v, J); gm[3] += gb.w * vcross(r2, J);
+	g_r1 = vsub(g_r1, vmult(vrperp(J), iA * ga.w));
+	g_r2 = vadd(g_r2, vmult(vrperp(J), iB * gb.w));
+	return vadd(vsub(vmult(gb.v, mB), vmult(ga.v, mA)), vsub(vmult(vperp(r2), iB * gb.w), vmult(vperp(r1), iA * ga.w)));
+}
+
+// adjoint of vr = (vB + perp(r2) wB) - (vA + perp(r1) wA) for dL/dvr = gvr
+__device__ __forceinline__ void relvel_adj(const JV &a, const JV &b, V2 r1, V2 r2, V2 gvr, JV &ga, JV &gb, V2 &g_r1,
+                                           V2 &g_r2)
+{
+	gb.v = vadd(gb.v, gvr); gb.w += vdot(gvr, vperp(r2)); g_r2 = vadd(g_r2, vmult(vrperp(gvr), b.w));
+	ga.v = vsub(ga.v, gvr); ga.w -= vdot(gvr, vperp(r1)); g_r1 = vsub(g_r1, vmult(vrperp(gvr), a.w));
+}
+
+// y = vclamp(x, L) (cpVect.h): the identity, or x L / |x| when |x| > L
+__device__ __forceinline__ V2 vclamp_adj(V2 x, double L, V2 gy)
+{
+	if (!(vdot(x, x) > L * L)) return gy;
+	const double len = csqrt(vdot(x, x));
+	const V2 xh = vmult(x, 1.0 / len);
+	return vmult(vsub(gy, vmult(xh, vdot(xh, gy))), L / len);
+}
+
+// d fclampc(f, lo, hi) / df along the branch taken
+__device__ __forceinline__ double fclamp_d(double f, double lo, double hi)
+{
+	const double m = (f > lo) ? f : lo;
+	return (f > lo && m < hi) ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ double k_scalar_body_d(double m_inv, double i_inv, V2 r, V2 n)
+{
+	const double rcn = vcross(r, n);
+	return m_inv + i_inv * rcn * rcn;
+}
+
+__device__ __forceinline__ JV gb_load(const BodyAdj &g)
+{
+	JV x;
+	x.v = v2(g.v[0], g.v[1]); x.w = g.w;
+	return x;
+}
+
+__device__ __forceinline__ void gb_store(BodyAdj &g, const JV &x)
+{
+	g.v[0] = x.v.x; g.v[1] = x.v.y; g.w = x.w;
+}
+
+// Reverse of joint_apply_impulse from its pre-state R.  ga, gb: adjoints of
+// the bodies after the application on entry, before it on exit; G.acc: of
+// the accumulators after it on entry, before it on exit.
+__device__ void joint_impulse_reverse(const Joint &j, double dt, const JointRec &R, double mA, double iA, double mB,
+                                      double iB, JV &ga, JV &gb, JointAdj &G, double gm[4])
+{
+	const JV a = jv_from(R.a), b = jv_from(R.b);
+	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
+	if (j.type == CPGPU_JOINT_PIVOT) {
+		const V2 jOld = v2(R.acc[0], R.acc[1]);
+		const V2 d = vsub(v2(j.bx, j.by), jv_relvel(a, b, r1, r2));
+		const V2 jj = v2(d.x * j.k11 + d.y * j.k12, d.x * j.k21 + d.y * j.k22);
+		const V2 x = vadd(jOld, jj);
+		const double L = j.maxForce * dt;
+		const V2 D = vsub(vclamp(x, L), jOld);
+		const V2 gD = impulse_adj(ga, gb, mA, iA, mB, iB, r1, r2, D, G.r1, G.r2, gm);
+		const V2 gx = vclamp_adj(x, L, vadd(G.acc, gD));
+		const V2 gd = v2(gx.x * j.k11 + gx.y * j.k21, gx.x * j.k12 + gx.y * j.k22);
+		G.k[0] += gx.x * d.x; G.k[1] += gx.x * d.y; G.k[2] += gx.y * d.x; G.k[3] += gx.y * d.y;
+		G.bias = vadd(G.bias, gd);
+		relvel_adj(a, b, r1, r2, vneg(gd), ga, gb, G.r1, G.r2);
+		G.acc = vsub(gx, gD);
+	} else if (j.type == CPGPU_JOINT_DAMPED_SPRING) {
+		const V2 n = v2(j.nx, j.ny);
+		const double tOld = R.acc[0];
+		const V2 vr = jv_relvel(a, b, r1, r2);
+		const double vrn = vdot(vr, n);
+		const double v_damp = (tOld - vrn) * j.v_coef;
+		const double j_damp = v_damp * j.nMass;
+		const V2 gJ = impulse_adj(ga, gb, mA, iA, mB, iB, r1, r2, vmult(n, j_damp), G.r1, G.r2, gm);
+		const double g_jd = vdot(gJ, n);
+		G.n = vadd(G.n, vmult(gJ, j_damp));
+		// target_vrn' = vrn + v_damp, j_damp = v_damp nMass, v_damp = (target_vrn - vrn) v_coef
+		const double g_vd = g_jd * j.nMass + G.acc.x;
+		G.nMass += g_jd * v_damp;
+		const double g_vrn = G.acc.x - g_vd * j.v_coef;
+		G.v_coef += g_vd * (tOld - vrn);
+		G.n = vadd(G.n, vmult(vr, g_vrn));
+		relvel_adj(a, b, r1, r2, vmult(n, g_vrn), ga, gb, G.r1, G.r2);
+		G.acc = v2(g_vd * j.v_coef, 0.0);
+	} else if (j.type == CPGPU_JOINT_SLIDE) {
+		const V2 n = v2(j.nx, j.ny);
+		if (n.x == 0.0 && n.y == 0.0) return;
+		const double jnOld = R.acc[0];
+		const V2 vr = jv_relvel(a, b, r1, r2);
+		const double vrn = vdot(vr, n);
+		const double jn = (j.bias - vrn) * j.nMass;
+		const double x = jnOld + jn;
+		const double lo = -j.maxForce * dt;
+		const double D = fclampc(x, lo, 0.0) - jnOld;
+		const V2 gJ = impulse_adj(ga, gb, mA, iA, mB, iB, r1, r2, vmult(n, D), G.r1, G.r2, gm);
+		const double gD = vdot(gJ, n);
+		G.n = vadd(G.n, vmult(gJ, D));
+		const double gx = (G.acc.x + gD) * fclamp_d(x, lo, 0.0);
+		G.bias.x += gx * j.nMass;
+		G.nMass += gx * (j.bias - vrn);
+		const double g_vrn = -gx * j.nMass;
+		G.n = vadd(G.n, vmult(vr, g_vrn));
+		relvel_adj(a, b, r1, r2, vmult(n, g_vrn), ga, gb, G.r1, G.r2);
+		G.acc.x = gx - gD;
+	} else {
+		const double ratio = j.P[1], ratio_inv = j.P[2];
+		const double jOld = R.acc[0];
+		const double wr = b.w * ratio - a.w;
+		const double jMax = j.maxForce * dt;
+		const double x = jOld + (j.bias - wr) * j.iSum;
+		const double D = fclampc(x, -jMax, jMax) - jOld;
+		// a.w -= D iA ratio_inv, b.w += D iB
+		const double gD = -ga.w * iA * ratio_inv + gb.w * iB;
+		gm[1] -= ga.w * D * ratio_inv; gm[3] += gb.w * D;
+		const double gx = (G.acc.x + gD) * fclamp_d(x, -jMax, jMax);
+		G.bias.x += gx * j.iSum;
+		G.iSum += gx * (j.bias - wr);
+		const double g_wr = -gx * j.iSum;
+		gb.w += g_wr * ratio;
+		ga.w -= g_wr;
+		G.acc.x = gx - gD;
+	}
+}
+
+// Reverse of joint_apply_cached with the warm-start accumulators acc.
+__device__ void joint_cached_reverse(const Joint &j, double dt_coef, const real *acc, double mA, double iA, double mB,
+                                     double iB, const JV &ga, const JV &gb, JointAdj &G, double gm[4])
+{
+	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
+	if (j.type == CPGPU_JOINT_PIVOT) {
+		const V2 gJ = impulse_adj(ga, gb, mA, iA, mB, iB, r1, r2, vmult(v2(acc[0], acc[1]), dt_coef), G.r1, G.r2, gm);
+		G.acc = vadd(G.acc, vmult(gJ, dt_coef));
+	} else if (j.type == CPGPU_JOINT_SLIDE) {
+		const V2 n = v2(j.nx, j.ny);
+		const double s = acc[0] * dt_coef;
+		const V2 gJ = impulse_adj(ga, gb, mA, iA, mB, iB, r1, r2, vmult(n, s), G.r1, G.r2, gm);
+		G.acc.x += vdot(gJ, n) * dt_coef;
+		G.n = vadd(G.n, vmult(gJ, s));
+	} else if (j.type == CPGPU_JOINT_GEAR) {
+		const double ratio_inv = j.P[2];
+		const double jj = acc[0] * dt_coef;
+		const double g_jj = -ga.w * iA * ratio_inv + gb.w * iB;
+		gm[1] -= ga.w * jj * ratio_inv; gm[3] += gb.w * jj;
+		G.acc.x += g_jj * dt_coef;
+	}
+}
+
+// rotation (cos, sin) of body b after the step's position update
+__device__ __forceinline__ void updated_rot(const DevWorld &W, const BodyState &s, int b, double dt, double &cs,
+                                            double &sn)
+{
+	if (W.b_type[b] == CPGPU_BODY_STATIC) {
+		const XF t = W.b_T[b];
+		cs = t.a; sn = t.b;
+		return;
+	}
+	V2 p;
+	double a;
+	updated_pose(s, dt, p, a);
+	cp_sincos(a, sn, cs);
+}
+
+// Reverse of joint_prestep (the spring's impulse included): the adjoints of
+// its results in G go to the bodies' updated positions and angles, their
+// inverse masses, the parameters gP[8] and, through the spring impulse, read
+// the velocity adjoints after it.
+__device__ void joint_prestep_reverse(const DevWorld &W, const AdjStep &A, const WorldStep &ws, const Joint &j,
+                                      JointAdj &G, double gP[8], V2 &gpA, double &gaA, V2 &gpB, double &gaB,
+                                      double gm[4])
+{
+	const BodyState &sa = A.start[j.a], &sb = A.start[j.b];
+	const double mA = sa.vel.m_inv, iA = sa.vel.i_inv, mB = sb.vel.m_inv, iB = sb.vel.i_inv;
+	const double dt = ws.dt, bc = j.bias_coef;
+	V2 pA, pB;
+	double angA = 0.0, angB = 0.0;
+	if (W.b_type[j.a] == CPGPU_BODY_STATIC) pA = v2(sa.p.x, sa.p.y); else updated_pose(sa, dt, pA, angA);
+	if (W.b_type[j.b] == CPGPU_BODY_STATIC) pB = v2(sb.p.x, sb.p.y); else updated_pose(sb, dt, pB, angB);
+	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
+	const V2 delta = vsub(vadd(pB, r2), vadd(pA, r1));
+	// k_scalar(a, b, r1, r2, n) (chipmunk_private.h:212-226) with adjoint g_k
+	auto k_scalar_adj = [&](V2 n, double g_k) {
+		const double rcn1 = vcross(r1, n), rcn2 = vcross(r2, n);
+		gm[0] += g_k; gm[1] += g_k * rcn1 * rcn1;
+		gm[2] += g_k; gm[3] += g_k * rcn2 * rcn2;
+		G.r1 = vadd(G.r1, vmult(vrperp(n), g_k * 2.0 * iA * rcn1));
+		G.r2 = vadd(G.r2, vmult(vrperp(n), g_k * 2.0 * iB * rcn2));
+		G.n = vadd(G.n, vadd(vmult(vperp(r1), g_k * 2.0 * iA * rcn1), vmult(vperp(r2), g_k * 2.0 * iB * rcn2)));
+	};
+	V2 gdelta = v2(0.0, 0.0);
+	if (j.type == CPGPU_JOINT_PIVOT) {
+		// bias = vclamp(delta (-bias_coef / dt), maxBias)
+		gdelta = vmult(vclamp_adj(vmult(delta, -bc / dt), j.maxBias, G.bias), -bc / dt);
+		// (k11 k12; k21 k22) = M^-1 (k_tensor, chipmunk_private.h:228-262): dM = -K^T dK K^T
+		const double K[4] = {j.k11, j.k12, j.k21, j.k22};
+		double P[4];
+		for (int r = 0; r < 2; r++)
+			for (int q = 0; q < 2; q++) P[2 * r + q] = K[r] * G.k[q] + K[2 + r] * G.k[2 + q];
+		double gM[4];
+		for (int r = 0; r < 2; r++)
+			for (int q = 0; q < 2; q++) gM[2 * r + q] = -(P[2 * r] * K[2 * q] + P[2 * r + 1] * K[2 * q + 1]);
+		const double g12 = gM[1] + gM[2];
+		gm[0] += gM[0] + gM[3]; gm[2] += gM[0] + gM[3];
+		gm[1] += gM[0] * r1.y * r1.y - g12 * r1.x * r1.y + gM[3] * r1.x * r1.x;
+		gm[3] += gM[0] * r2.y * r2.y - g12 * r2.x * r2.y + gM[3] * r2.x * r2.x;
+		G.r1 = vadd(G.r1, v2(iA * (2.0 * gM[3] * r1.x - g12 * r1.y), iA * (2.0 * gM[0] * r1.y - g12 * r1.x)));
+		G.r2 = vadd(G.r2, v2(iB * (2.0 * gM[3] * r2.x - g12 * r2.y), iB * (2.0 * gM[0] * r2.y - g12 * r2.x)));
+	} else if (j.type == CPGPU_JOINT_DAMPED_SPRING) {
+		const double rest = j.P[4], stiffness = j.P[5], damping = j.P[6];
+		const V2 n = v2(j.nx, j.ny);
+		const double dist = vlength(delta);
+		const double k = k_scalar_body_d(mA, iA, r1, n) + k_scalar_body_d(mB, iB, r2, n);
+		// the spring impulse n (rest - dist) stiffness dt, applied to the velocities
+		const double js = (rest - dist) * stiffness * dt;
+		const JV ga = gb_load(A.gb[j.a]), gb = gb_load(A.gb[j.b]);
+		const V2 gJ = impulse_adj(ga, gb, mA, iA, mB, iB, r1, r2, vmult(n, js), G.r1, G.r2, gm);
+		const double g_f = vdot(gJ, n) * dt;
+		G.n = vadd(G.n, vmult(gJ, js));
+		gP[4] += g_f * stiffness;
+		gP[5] += g_f * (rest - dist);
+		double g_dist = -g_f * stiffness;
+		// v_coef = 1 - exp(-damping dt k), nMass = 1 / k
+		const double e = cp_exp(-damping * dt * k);
+		const double g_arg = -G.v_coef * e;
+		gP[6] += g_arg * (-dt * k);
+		const double g_k = g_arg * (-damping * dt) - G.nMass / (k * k);
+		k_scalar_adj(n, g_k);
+		// n = delta / dist
+		if (dist != 0.0) gdelta = vadd(vmult(vsub(G.n, vmult(n, vdot(n, G.n))), 1.0 / dist), vmult(n, g_dist));
+	} else if (j.type == CPGPU_JOINT_SLIDE) {
+		const V2 n = v2(j.nx, j.ny);
+		if (!(n.x == 0.0 && n.y == 0.0)) {
+			const double mn = j.P[4], mx = j.P[5];
+			const double dist = vlength(delta);
+			const double s = dist > mx ? 1.0 : -1.0;
+			const double pdist = s > 0.0 ? dist - mx : mn - dist;
+			// bias = clamp(-bias_coef pdist / dt, +-maxBias)
+			const double xb = -bc * pdist / dt;
+			const double g_pdist = G.bias.x * fclamp_d(xb, -j.maxBias, j.maxBias) * (-bc / dt);
+			if (s > 0.0) gP[5] -= g_pdist; else gP[4] += g_pdist;
+			const double g_dist = s * g_pdist;
+			const double k = k_scalar_body_d(mA, iA, r1, n) + k_scalar_body_d(mB, iB, r2, n);
+			k_scalar_adj(n, -G.nMass / (k * k));
+			// n = s delta / |delta|
+			const V2 nh = vmult(n, s);
+			gdelta = vadd(vmult(vsub(G.n, vmult(nh, vdot(nh, G.n))), s / dist), vmult(nh, g_dist));
+		}
+	} else {
+		// iSum = 1 / (iA ratio_inv + ratio iB); bias = clamp(-bias_coef (aB ratio - aA - phase) / dt, +-maxBias)
+		const double ratio = j.P[1], ratio_inv = j.P[2];
+		const double g_den = -G.iSum * j.iSum * j.iSum;
+		gm[1] += g_den * ratio_inv; gm[3] += g_den * ratio;
+		const double aA = W.b_type[j.a] == CPGPU_BODY_STATIC ? (double)sa.a : angA;
+		const double aB = W.b_type[j.b] == CPGPU_BODY_STATIC ? (double)sb.a : angB;
+		const double xb = -bc * (aB * ratio - aA - j.P[0]) / dt;
+		const double g_x = G.bias.x * fclamp_d(xb, -j.maxBias, j.maxBias);
+		gaB += g_x * (-bc / dt) * ratio;
+		gaA += g_x * (bc / dt);
+		gP[0] += g_x * (bc / dt);
+		return;
+	}
+	// delta = (pB + r2) - (pA + r1)
+	gpB = vadd(gpB, gdelta); gpA = vsub(gpA, gdelta);
+	G.r2 = vadd(G.r2, gdelta); G.r1 = vsub(G.r1, gdelta);
+	// r = R(angle) (anchor - cog): dr/dangle = perp(r), dr/danchor = R
+	gaA += vdot(G.r1, vperp(r1));
+	gaB += vdot(G.r2, vperp(r2));
+	double cs, sn;
+	updated_rot(W, sa, j.a, dt, cs, sn);
+	gP[0] += cs * G.r1.x + sn * G.r1.y; gP[1] += -sn * G.r1.x + cs * G.r1.y;
+	updated_rot(W, sb, j.b, dt, cs, sn);
+	gP[2] += cs * G.r2.x + sn * G.r2.y; gP[3] += -sn * G.r2.x + cs * G.r2.y;
+}
+
 // ---- adjoint state ----
 
 __device__ __forceinline__ void add2(double *p, V2 v) { p[0] += v.x; p[1] += v.y; }
@@ -363,11 +676,47 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, 
 		G.n = v2(0.0, 0.0); G.svr = v2(0.0, 0.0); G.u = 0.0;
 		garb[pos] = G;
 	}
+	// the space's joints by colour (positions in jw_order); their accumulators' adjoints come from the
+	// next step's warm start (a spring's accumulator does not carry over)
+	const int32_t *jcol = A.nc > 0 ? A.jw_col + A.jw_cbase[w] : nullptr;
+	const int njc = A.nc > 0 ? A.jw_ncol[w] : 0;
+	JointAdj *gjt = reinterpret_cast<JointAdj *>(A.gjoint);
+	if (njc > 0)
+		for (int pos = jcol[0] + tid; pos < jcol[njc]; pos += stride) {
+			const int c = A.jw_order[pos];
+			JointAdj G;
+			G.r1 = G.r2 = G.n = G.bias = v2(0.0, 0.0);
+			G.k[0] = G.k[1] = G.k[2] = G.k[3] = 0.0;
+			G.nMass = G.v_coef = G.iSum = 0.0;
+			G.acc = A.joints[c].type == CPGPU_JOINT_DAMPED_SPRING ? v2(0.0, 0.0) : v2(A.gJacc[c].x, A.gJacc[c].y);
+			gjt[pos] = G;
+		}
 	__syncthreads();
 	const int ncol = n_col;
+	// one joint application reversed (sweep slot: 0 cached, it + 1 sweep it)
+	auto joint_step_reverse = [&](int pos, int slot) {
+		const Joint &j = A.joints[A.jw_order[pos]];
+		const double mA = W.b_vel[j.a].m_inv, iA = W.b_vel[j.a].i_inv;
+		const double mB = W.b_vel[j.b].m_inv, iB = W.b_vel[j.b].i_inv;
+		BodyAdj &GA = A.gb[j.a];
+		BodyAdj &GB = A.gb[j.b];
+		JV ga = gb_load(GA), gb = gb_load(GB);
+		JointAdj G = gjt[pos];
+		double gm[4] = {0.0, 0.0, 0.0, 0.0};
+		const JointRec &R = A.jrec[(size_t)slot * A.nc + pos];
+		if (slot == 0) joint_cached_reverse(j, ws.dt_coef, R.acc, mA, iA, mB, iB, ga, gb, G, gm);
+		else joint_impulse_reverse(j, ws.dt, R, mA, iA, mB, iB, ga, gb, G, gm);
+		gjt[pos] = G;
+		if (dyn(W, j.a)) { gb_store(GA, ga); A.gminv[j.a] += gm[0]; A.giinv[j.a] += gm[1]; }
+		if (dyn(W, j.b)) { gb_store(GB, gb); A.gminv[j.b] += gm[2]; A.giinv[j.b] += gm[3]; }
+	};
 
-	// ---- impulse sweeps, last first ----
+	// ---- impulse sweeps, last first (in a sweep the joints follow the arbiters) ----
 	for (int it = ws.iterations - 1; it >= 0; it--) {
+		for (int jc = njc - 1; jc >= 0; jc--) {
+			for (int pos = jcol[jc] + tid; pos < jcol[jc + 1]; pos += stride) joint_step_reverse(pos, it + 1);
+			__syncthreads();
+		}
 		for (int c = ncol - 1; c >= 0; c--) {
 			for (int pos = col_start[c] + tid; pos < col_start[c + 1]; pos += stride) {
 				const Arbiter &arb = A.act[A.order[pos]];
@@ -408,7 +757,23 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, 
 		}
 	}
 
-	// ---- cached impulses (cpArbiterApplyCachedImpulse), skipped on a first collision ----
+	// ---- cached impulses: the joints' (after the arbiters'), then the arbiters' (cpArbiterApplyCachedImpulse,
+	// skipped on a first collision) ----
+	for (int jc = njc - 1; jc >= 0; jc--) {
+		for (int pos = jcol[jc] + tid; pos < jcol[jc + 1]; pos += stride) joint_step_reverse(pos, 0);
+		__syncthreads();
+	}
+	// the joints' warm starts: to the accumulators at the end of the step before (a spring's pre-step
+	// overwrites its accumulator, an inactive slide's clears it)
+	if (njc > 0)
+		for (int pos = jcol[0] + tid; pos < jcol[njc]; pos += stride) {
+			const int c = A.jw_order[pos];
+			const Joint &j = A.joints[c];
+			const bool carried = j.type == CPGPU_JOINT_PIVOT || j.type == CPGPU_JOINT_GEAR ||
+			                     (j.type == CPGPU_JOINT_SLIDE && !(j.nx == 0.0 && j.ny == 0.0));
+			const V2 g = carried ? gjt[pos].acc : v2(0.0, 0.0);
+			A.gJacc[c] = make_double2(g.x, g.y);
+		}
 	for (int c = ncol - 1; c >= 0; c--) {
 		for (int pos = col_start[c] + tid; pos < col_start[c + 1]; pos += stride) {
 			const Arbiter &arb = A.act[A.order[pos]];
@@ -480,6 +845,32 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, 
 		GB.w *= ws.damping;
 	}
 	__syncthreads();
+
+	// ---- joint pre-steps (after the arbiters' pre-steps, before the velocity integration) ----
+	for (int jc = njc - 1; jc >= 0; jc--) {
+		for (int pos = jcol[jc] + tid; pos < jcol[jc + 1]; pos += stride) {
+			const int c = A.jw_order[pos];
+			const Joint &j = A.joints[c];
+			JointAdj G = gjt[pos];
+			double gP[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+			double gm[4] = {0.0, 0.0, 0.0, 0.0};
+			V2 gpA = v2(0.0, 0.0), gpB = v2(0.0, 0.0);
+			double gaA = 0.0, gaB = 0.0;
+			joint_prestep_reverse(W, A, ws, j, G, gP, gpA, gaA, gpB, gaB, gm);
+			for (int k = 0; k < 8; k++) A.gP[8 * (size_t)c + k] += gP[k];
+			if (dyn(W, j.a)) {
+				BodyAdj &GA = A.gb[j.a];
+				GA.p[0] += gpA.x; GA.p[1] += gpA.y; GA.a += gaA;
+				A.gminv[j.a] += gm[0]; A.giinv[j.a] += gm[1];
+			}
+			if (dyn(W, j.b)) {
+				BodyAdj &GB = A.gb[j.b];
+				GB.p[0] += gpB.x; GB.p[1] += gpB.y; GB.a += gaB;
+				A.gminv[j.b] += gm[2]; A.giinv[j.b] += gm[3];
+			}
+		}
+		__syncthreads();
+	}
 
 	// ---- pre-step and contact geometry, colour by colour (bodies of one colour are distinct) ----
 	for (int c = ncol - 1; c >= 0; c--) {
@@ -643,6 +1034,7 @@ hipError_t adjoint_state_load(const DevWorld &W, const BodyState *in, hipStream_
 }
 
 size_t adjoint_arb_scratch_bytes() { return sizeof(ArbAdj); }
+size_t adjoint_joint_scratch_bytes() { return sizeof(JointAdj); }
 
 hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream)
 {

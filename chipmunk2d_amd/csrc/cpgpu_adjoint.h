@@ -35,6 +35,15 @@ struct AdjStep {
 	double *gminv, *giinv;      // [n_bodies] adjoints of 1/m and 1/I
 	double *gu;                 // [n_shapes] adjoint of the friction coefficient
 	double2 *gJ;                // [recorded contacts] adjoints of the accumulated impulses (jn, jt)
+	// joints (nc 0: none)
+	int nc;
+	const Joint *joints;        // the joints after the replayed step (pre-step results, parameters)
+	const JointRec *jrec;       // [slot][position in jw_order] pre-states, slot 0 before the cached impulses
+	const int32_t *jw_order, *jw_col, *jw_cbase, *jw_ncol;   // per space: joints by colour
+	void *gjoint;               // [nc] per-joint adjoint scratch (adjoint_joint_scratch_bytes each)
+	double2 *gJacc;             // [nc] adjoint of each joint's accumulators at the end of the step on entry,
+	                            //      at the end of the step before on exit
+	double *gP;                 // [nc * 8] adjoints of the joints' parameters (cpgpuWorldDesc::cons_params)
 };
 
 hipError_t adjoint_tape_record(Arbiter *arbs, const int32_t *n_dev, int n_host, Arbiter *tape, int off,
@@ -43,7 +52,8 @@ hipError_t adjoint_replay_prepare(const Arbiter *tape, int off, int n, Arbiter *
 hipError_t adjoint_state_save(const DevWorld &W, BodyState *out, hipStream_t stream);
 hipError_t adjoint_state_load(const DevWorld &W, const BodyState *in, hipStream_t stream);
 hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream);
-// bytes of the per-arbiter adjoint scratch (AdjStep::garb_raw) per solved arbiter
+// bytes of the per-arbiter / per-joint adjoint scratch (AdjStep::garb, gjoint) per element
 size_t adjoint_arb_scratch_bytes();
+size_t adjoint_joint_scratch_bytes();
 
 } // namespace CPGPU_NS

@@ -61,6 +61,14 @@ struct ArbRec {
 	real acc[2][3];          // jnAcc, jtAcc, jBias
 };
 
+// Pre-state of one joint's impulse application (rollout tape): both bodies'
+// velocities and the joint's accumulators before it -- pivot (jAx, jAy),
+// damped spring (target_vrn, jAcc), slide (jnAcc, 0), gear (jAcc, 0).
+struct JointRec {
+	real a[3], b[3];         // v.x, v.y, w
+	real acc[2];
+};
+
 struct SolveBuffers {
 	Arbiter *act;
 	const int32_t *n_act;
@@ -113,6 +121,9 @@ struct SolveBuffers {
 	// recording (adjoint replay): rec[it * rec_stride + position in order], or nullptr
 	ArbRec *rec;
 	int rec_stride;
+	// joints: jrec[slot * n_joints + position in jw_order], slot 0 before the cached impulses,
+	// slot it + 1 before sweep it; or nullptr
+	JointRec *jrec;
 };
 
 // Group the solved arbiters: islands of the large worlds (union-find), the

@@ -32,6 +32,7 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every); \
 	int cpgpuWorldTapeEnd(cpgpuWorld *world); \
 	int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v, const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0, double *d_m_inv, double *d_i_inv, double *d_u); \
+	int cpgpuWorldGetJointGrads(cpgpuWorld *world, double *d_params); \
 	int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodies);
 
 namespace cpgpu64 {
@@ -179,6 +180,13 @@ int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, 
 	if (!world) return null_world();
 	if (is_fp32(world)) return cpgpu32::cpgpuWorldBackward(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), g_p, g_a, g_v, g_w, d_p0, d_a0, d_v0, d_w0, d_m_inv, d_i_inv, d_u);
 	return cpgpu64::cpgpuWorldBackward(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), g_p, g_a, g_v, g_w, d_p0, d_a0, d_v0, d_w0, d_m_inv, d_i_inv, d_u);
+}
+
+int cpgpuWorldGetJointGrads(cpgpuWorld *world, double *d_params)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetJointGrads(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), d_params);
+	return cpgpu64::cpgpuWorldGetJointGrads(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), d_params);
 }
 
 int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodies)

@@ -538,6 +538,17 @@ template <class BA> __device__ __forceinline__ void record_arbiter(ArbRec *r, co
 	}
 }
 
+template <class BA> __device__ __forceinline__ void record_joint(JointRec *r, const BA &bodies, const Joint &j)
+{
+	const BV a = bodies.load(j.a), b = bodies.load(j.b);
+	r->a[0] = a.v.x; r->a[1] = a.v.y; r->a[2] = a.w;
+	r->b[0] = b.v.x; r->b[1] = b.v.y; r->b[2] = b.w;
+	if (j.type == CPGPU_JOINT_PIVOT) { r->acc[0] = j.jAx; r->acc[1] = j.jAy; }
+	else if (j.type == CPGPU_JOINT_DAMPED_SPRING) { r->acc[0] = j.target_vrn; r->acc[1] = j.jAcc; }
+	else if (j.type == CPGPU_JOINT_SLIDE) { r->acc[0] = j.jnAcc; r->acc[1] = 0.0; }
+	else { r->acc[0] = j.jAcc; r->acc[1] = 0.0; }
+}
+
 // cached impulses (cpSpaceStep.c:407-415), then the impulse iterations
 // (cpSpaceStep.c:418-427): arbiters colour by colour, then joints colour by colour
 template <class Sync, class BA>
@@ -548,8 +559,10 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 	const int tail = S.solve_tail_max;
 	for_colours(sy, T.col_off, n_colours, tail, [&](int k) { arb_apply_cached(W, bodies, S.act[T.order[k]]); });
 	for (int jc = 0; jc < T.njcol; jc++) {
-		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride)
+		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
+			if (S.jrec) record_joint(S.jrec + k, bodies, W.joints[T.jorder[k]]);
 			joint_apply_cached(W, bodies, W.joints[T.jorder[k]]);
+		}
 		sy.sync();
 	}
 	for (int it = 0; it < T.iterations; it++) {
@@ -563,7 +576,10 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 		for (int jc = 0; jc < T.njcol; jc++) {
 			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
 				Joint &j = W.joints[T.jorder[k]];
-				if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);
+				if (it < W.ws[j.world].iterations) {
+					if (S.jrec) record_joint(S.jrec + (size_t)(it + 1) * W.n_joints + k, bodies, j);
+					joint_apply_impulse(W, bodies, j);
+				}
 			}
 			sy.sync();
 		}

@@ -303,7 +303,9 @@ struct cpgpuWorld {
 		std::vector<int> count;         // per step: solved arbiters
 		std::vector<std::vector<WorldStep>> ws;
 		std::vector<BodyState *> ckpt;  // bodies at steps 0, every, 2 every, ...
+		std::vector<Joint *> jckpt;     // joints at the same steps
 	} tape;
+	std::vector<double> h_joint_grads;  // [nc * 8] parameter adjoints of the last backward pass
 	int32_t *d_replay_n = nullptr;
 	// grid-solve tails (see SolveBuffers)
 	// colouring finishes on one workgroup once at most colour_tail_max arbiters are left; runs of colours
@@ -853,7 +855,7 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;
 	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
-	S.rec = nullptr; S.rec_stride = 0;
+	S.rec = nullptr; S.rec_stride = 0; S.jrec = nullptr;
 	return S;
 }
 
@@ -878,7 +880,30 @@ static void tape_clear(cpgpuWorld *w)
 	auto &T = w->tape;
 	if (T.arb) (void)hipFree(T.arb);
 	for (BodyState *c : T.ckpt) (void)hipFree(c);
+	for (Joint *c : T.jckpt) (void)hipFree(c);
 	T = cpgpuWorld::Tape();
+}
+
+// bias_coef (chipmunk_private.h:264-268) per joint for step dt, libm pow on the
+// host (once per distinct errorBias), then scattered into the joints on the device
+static hipError_t set_joint_bias(cpgpuWorld *w, double dt)
+{
+	std::vector<real> bc(w->nc);
+	double last_eb = 0.0;
+	real last_bc = 0.0;
+	bool have = false;
+	for (int c = 0; c < w->nc; c++) {
+		const double eb = w->joint_error_bias[c];
+		if (!have || eb != last_eb) { last_eb = eb; last_bc = (real)(1.0 - pow(eb, dt)); have = true; }
+		bc[c] = last_bc;
+	}
+	HIP_TRY(hipMemcpyAsync(w->d_joint_bias_coef, bc.data(), sizeof(real) * w->nc, hipMemcpyHostToDevice, w->stream));
+	k_set_joint_bias<<<blocks_for(w->nc, 256), 256, 0, w->stream>>>(w->W.joints, w->d_joint_bias_coef, w->nc);
+	HIP_TRY(hipGetLastError());
+	// the host vector must outlive the copy
+	HIP_TRY(hipStreamSynchronize(w->stream));
+	w->last_joint_dt = dt;
+	return hipSuccess;
 }
 
 int cpgpuWorldStep(cpgpuWorld *w, double dt)
@@ -915,22 +940,13 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		}
 		w->tape.ws.emplace_back(w->h_ws, w->h_ws + w->nw);
 	}
-	if (w->nc > 0 && dt != w->last_joint_dt) {
-		// bias_coef (chipmunk_private.h:264-268) per joint, libm pow on the host (once per
-		// distinct errorBias), then scattered into the joints on the device
-		std::vector<real> bc(w->nc);
-		double last_eb = 0.0;
-		real last_bc = 0.0;
-		bool have = false;
-		for (int c = 0; c < w->nc; c++) {
-			const double eb = w->joint_error_bias[c];
-			if (!have || eb != last_eb) { last_eb = eb; last_bc = (real)(1.0 - pow(eb, dt)); have = true; }
-			bc[c] = last_bc;
-		}
-		HIP_OR_FAIL(hipMemcpy(w->d_joint_bias_coef, bc.data(), sizeof(real) * w->nc, hipMemcpyHostToDevice));
-		k_set_joint_bias<<<blocks_for(w->nc, 256), 256, 0, w->stream>>>(W.joints, w->d_joint_bias_coef, w->nc);
-		HIP_OR_FAIL(hipGetLastError());
-		w->last_joint_dt = dt;
+	if (w->nc > 0 && dt != w->last_joint_dt) HIP_OR_FAIL(set_joint_bias(w, dt));
+	if (w->tape.on && w->nc > 0 && w->tape.steps % w->tape.every == 0) {
+		// the joints at the checkpoint, with this step's bias coefficients
+		Joint *jk = nullptr;
+		HIP_OR_FAIL(hipMalloc(&jk, sizeof(Joint) * (size_t)w->nc));
+		w->tape.jckpt.push_back(jk);
+		HIP_OR_FAIL(hipMemcpyAsync(jk, W.joints, sizeof(Joint) * (size_t)w->nc, hipMemcpyDeviceToDevice, w->stream));
 	}
 
 	// integrate positions, cache the dynamic shapes, update leaf boxes
@@ -1281,7 +1297,6 @@ int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
 	if (w->n_large_worlds > 0)
 		return fail(CPGPU_ERR_INVALID, "the adjoint supports spaces solved one workgroup each (at most " +
 		                                   std::to_string(w->small_world_max_bodies) + " dynamic bodies)");
-	if (w->nc > 0) return fail(CPGPU_ERR_INVALID, "the adjoint does not support constraints");
 	int rc = cpgpuWorldSync(w);
 	if (rc != CPGPU_OK) return rc;
 	tape_clear(w);
@@ -1318,7 +1333,7 @@ int cpgpuWorldTapeEnd(cpgpuWorld *w)
 // Replay step t of the tape from the bodies now in W: position update, the
 // recorded arbiters with their warm starts, the solve with pre-state records.
 static int replay_step(cpgpuWorld *w, int t, Arbiter *act, int32_t *colour, int32_t *order, int32_t *warb_off,
-                       ArbRec *rec, KernelTimer *timer)
+                       ArbRec *rec, JointRec *jrec, KernelTimer *timer)
 {
 	auto &T = w->tape;
 	DevWorld &W = w->W;
@@ -1333,6 +1348,7 @@ static int replay_step(cpgpuWorld *w, int t, Arbiter *act, int32_t *colour, int3
 	SolveBuffers S = solve_buffers(w, act, w->d_replay_n, colour, order, warb_off);
 	S.rec = rec;
 	S.rec_stride = n;
+	S.jrec = jrec;
 	if (w->nw > 1) HIP_OR_FAIL(solver_group(W, S, n, w->nw, false, w->sortbuf, w->stream, nullptr));
 	TIMED(timer, "replay_solve", w->stream,
 	      HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, 0, 0, w->stream, w->aux, w->fork_join, nullptr)));
@@ -1365,6 +1381,13 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	int32_t *colour = nullptr, *order = nullptr, *warb = nullptr;
 	ArbRec *rec = nullptr;
 	void *garb = nullptr;
+	const int nc = w->nc;
+	JointRec *jrec = nullptr;
+	Joint *jsnap = nullptr, *live_joints = nullptr;
+	void *gjoint = nullptr;
+	double2 *gJacc = nullptr;
+	double *gP = nullptr;
+	const double live_joint_dt = w->last_joint_dt;
 	unsigned long long *totals_saved = nullptr;
 	std::vector<void *> bufs;
 	auto grab = [&](void **p, size_t bytes) -> hipError_t {
@@ -1388,6 +1411,12 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	ADJ_TRY(grab((void **)&warb, sizeof(int32_t) * (size_t)(w->nw + 2) * K));
 	ADJ_TRY(grab((void **)&rec, sizeof(ArbRec) * (size_t)max_n * max_it * K));
 	ADJ_TRY(grab(&garb, adjoint_arb_scratch_bytes() * (size_t)max_n));
+	ADJ_TRY(grab((void **)&jrec, sizeof(JointRec) * (size_t)nc * (max_it + 1) * K));
+	ADJ_TRY(grab((void **)&jsnap, sizeof(Joint) * (size_t)nc * K));
+	ADJ_TRY(grab((void **)&live_joints, sizeof(Joint) * (size_t)nc));
+	ADJ_TRY(grab(&gjoint, adjoint_joint_scratch_bytes() * (size_t)nc));
+	ADJ_TRY(grab((void **)&gJacc, sizeof(double2) * (size_t)std::max(nc, 1)));
+	ADJ_TRY(grab((void **)&gP, sizeof(double) * 8 * (size_t)std::max(nc, 1)));
 	ADJ_TRY(grab((void **)&totals_saved, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS)));
 	// replay scratch sized for the largest step
 	if (max_n > w->colour_cap) {
@@ -1420,6 +1449,10 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	ADJ_TRY(hipMemset(giinv, 0, sizeof(double) * nb));
 	ADJ_TRY(hipMemset(gu, 0, sizeof(double) * ns));
 	ADJ_TRY(hipMemset(gJ, 0, sizeof(double2) * 2 * (size_t)std::max<long>(T.n, 1)));
+	ADJ_TRY(hipMemset(gJacc, 0, sizeof(double2) * (size_t)std::max(nc, 1)));
+	ADJ_TRY(hipMemset(gP, 0, sizeof(double) * 8 * (size_t)std::max(nc, 1)));
+	if (nc > 0)
+		ADJ_TRY(hipMemcpyAsync(live_joints, W.joints, sizeof(Joint) * nc, hipMemcpyDeviceToDevice, w->stream));
 	ADJ_TRY(adjoint_state_save(W, live, w->stream));
 	ADJ_TRY(hipMemcpyAsync(totals_saved, W.totals, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS),
 	                       hipMemcpyDeviceToDevice, w->stream));
@@ -1428,11 +1461,22 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	for (int seg = n_seg - 1; seg >= 0 && result == CPGPU_OK; seg--) {
 		const int t0 = seg * K, t1 = std::min(t0 + K, T.steps);
 		ADJ_TRY(adjoint_state_load(W, T.ckpt[seg], w->stream));
+		double replay_dt = T.ws[t0][0].dt;   // the joint checkpoint holds this step's bias coefficients
+		if (nc > 0)
+			ADJ_TRY(hipMemcpyAsync(W.joints, T.jckpt[seg], sizeof(Joint) * nc, hipMemcpyDeviceToDevice, w->stream));
 		for (int t = t0; t < t1 && result == CPGPU_OK; t++) {
 			const int q = t - t0;
+			if (nc > 0 && T.ws[t][0].dt != replay_dt) {
+				replay_dt = T.ws[t][0].dt;
+				ADJ_TRY(set_joint_bias(w, replay_dt));
+			}
 			ADJ_TRY(adjoint_state_save(W, start + (size_t)nb * q, w->stream));
 			result = replay_step(w, t, act + (size_t)max_n * q, colour + (size_t)max_n * q, order + (size_t)max_n * q,
-			                     warb + (size_t)(w->nw + 2) * q, rec + (size_t)max_n * max_it * q, timer);
+			                     warb + (size_t)(w->nw + 2) * q, rec + (size_t)max_n * max_it * q,
+			                     nc > 0 ? jrec + (size_t)nc * (max_it + 1) * q : nullptr, timer);
+			if (nc > 0)
+				ADJ_TRY(hipMemcpyAsync(jsnap + (size_t)nc * q, W.joints, sizeof(Joint) * nc, hipMemcpyDeviceToDevice,
+				                       w->stream));
 		}
 		for (int t = t1 - 1; t >= t0 && result == CPGPU_OK; t--) {
 			const int q = t - t0;
@@ -1450,6 +1494,11 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 			A.tape = T.arb;
 			A.garb = garb;
 			A.gb = gb; A.gminv = gminv; A.giinv = giinv; A.gu = gu; A.gJ = gJ;
+			A.nc = nc;
+			A.joints = jsnap + (size_t)nc * q;
+			A.jrec = jrec + (size_t)nc * (max_it + 1) * q;
+			A.jw_order = w->d_jw_order; A.jw_col = w->d_jw_col; A.jw_cbase = w->d_jw_cbase; A.jw_ncol = w->d_jw_ncol;
+			A.gjoint = gjoint; A.gJacc = gJacc; A.gP = gP;
 			TIMED(timer, "reverse_step", w->stream, ADJ_TRY(adjoint_reverse_step(W, A, w->nw, w->stream)));
 		}
 		if (result == CPGPU_OK) result = sync_world(w);
@@ -1467,6 +1516,13 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 		if (d_m_inv) ADJ_TRY(hipMemcpy(d_m_inv, gminv, sizeof(double) * nb, hipMemcpyDeviceToHost));
 		if (d_i_inv) ADJ_TRY(hipMemcpy(d_i_inv, giinv, sizeof(double) * nb, hipMemcpyDeviceToHost));
 		if (d_u) ADJ_TRY(hipMemcpy(d_u, gu, sizeof(double) * ns, hipMemcpyDeviceToHost));
+		w->h_joint_grads.assign(8 * (size_t)nc, 0.0);
+		if (nc > 0)
+			ADJ_TRY(hipMemcpy(w->h_joint_grads.data(), gP, sizeof(double) * 8 * nc, hipMemcpyDeviceToHost));
+	}
+	if (nc > 0) {
+		ADJ_TRY(hipMemcpyAsync(W.joints, live_joints, sizeof(Joint) * nc, hipMemcpyDeviceToDevice, w->stream));
+		w->last_joint_dt = live_joint_dt;
 	}
 	ADJ_TRY(adjoint_state_load(W, live, w->stream));
 	ADJ_TRY(hipMemcpyAsync(W.totals, totals_saved, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS),
@@ -1488,6 +1544,15 @@ int cpgpuWorldGetSolveCounts(cpgpuWorld *w, int64_t *contacts, int64_t *bodies)
 		if (contacts) contacts[u] = (int64_t)t[2 + 2 * u];
 		if (bodies) bodies[u] = (int64_t)t[3 + 2 * u];
 	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldGetJointGrads(cpgpuWorld *w, double *d_params)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (w->h_joint_grads.size() != 8 * (size_t)w->nc)
+		return fail(CPGPU_ERR_INVALID, "no joint gradients: call cpgpuWorldBackward first");
+	if (d_params) std::copy(w->h_joint_grads.begin(), w->h_joint_grads.end(), d_params);
 	return CPGPU_OK;
 }
 

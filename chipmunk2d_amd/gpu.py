@@ -27,7 +27,7 @@ EXPORTS = [
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
     "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
     "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
-    "cpgpuWorldBackward", "cpgpuGetLastError", "cpgpuVersion",
+    "cpgpuWorldBackward", "cpgpuWorldGetJointGrads", "cpgpuGetLastError", "cpgpuVersion",
 ]
 
 
@@ -62,6 +62,7 @@ def lib():
         L.cpgpuWorldTapeBegin.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldTapeEnd.argtypes = [_vp]
         L.cpgpuWorldBackward.argtypes = [_vp] + [_dp] * 11
+        L.cpgpuWorldGetJointGrads.argtypes = [_vp, _dp]
         L.cpgpuGetLastError.restype = ctypes.c_char_p
         L.cpgpuVersion.restype = ctypes.c_char_p
         _lib = L
@@ -185,7 +186,7 @@ class GPUWorld:
         """Adjoint of the taped rollout.  g_*: dL/d of the final positions [n,2], angles [n],
         velocities [n,2], angular velocities [n] (None: zero).  Returns a dict of dL/d of the
         initial state ('p', 'a', 'v', 'w') and of the parameters ('m_inv', 'i_inv' per body,
-        'u' per shape)."""
+        'u' per shape, 'joint' [n_constraints, 8] in the layout of cons_params)."""
         nb, ns = self.packed.n_bodies, self.packed.n_shapes
 
         def inp(x, shape):
@@ -197,6 +198,8 @@ class GPUWorld:
         args = [None if x is None else ptr(x, ctypes.c_double) for x in seeds]
         args += [ptr(out[k], ctypes.c_double) for k in ("p", "a", "v", "w", "m_inv", "i_inv", "u")]
         _check(self.L.cpgpuWorldBackward(self.h, *args))
+        out["joint"] = np.zeros((self.packed.n_constraints, 8))
+        _check(self.L.cpgpuWorldGetJointGrads(self.h, ptr(out["joint"], ctypes.c_double)))
         return out
 
     SOLVE_UNITS = ("solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2", "solver_worlds_b3", "solver_islands",

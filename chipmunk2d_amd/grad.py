@@ -14,12 +14,14 @@ from .cabi import PackedWorld
 from .gpu import GPUWorld
 
 DYNAMIC = 0
+SPRING = 1   # CPGPU_JOINT_DAMPED_SPRING
 
 
-def with_params(pw, v0=None, m=None, u=None):
+def with_params(pw, v0=None, m=None, u=None, joint=None):
     """A copy of the packed description with initial velocities v0 [n, 2],
-    masses m [n] (dynamic bodies; others ignored) and friction coefficients
-    u [n_shapes] replaced."""
+    masses m [n] (dynamic bodies; others ignored), friction coefficients
+    u [n_shapes] and constraint parameters joint [n_constraints, 8]
+    (cons_params) replaced."""
     a = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in pw.a.items()}
     if v0 is not None:
         a["body_v"] = np.ascontiguousarray(v0, dtype=np.float64).reshape(-1, 2)
@@ -34,6 +36,8 @@ def with_params(pw, v0=None, m=None, u=None):
         a["body_i_inv"][ch] = 1.0 / (k * m[ch])
     if u is not None:
         a["shape_u"] = np.ascontiguousarray(u, dtype=np.float64)
+    if joint is not None:
+        a["cons_params"] = np.ascontiguousarray(joint, dtype=np.float64).reshape(-1, 8)
     return PackedWorld(a)
 
 
@@ -54,7 +58,13 @@ def param_grads(pw, adj):
     dm = np.zeros(len(m))
     # 1/m and 1/(k m): d/dm = -1/m^2 and -k/(k m)^2 = -(i/m)/i^2
     dm[dyn] = -adj["m_inv"][dyn] / (m[dyn] * m[dyn]) - adj["i_inv"][dyn] * (i[dyn] / m[dyn]) / (i[dyn] * i[dyn])
-    return {"v0": adj["v"], "w0": adj["w"], "p0": adj["p"], "a0": adj["a"], "m": dm, "u": adj["u"]}
+    out = {"v0": adj["v"], "w0": adj["w"], "p0": adj["p"], "a0": adj["a"], "m": dm, "u": adj["u"],
+           "joint": adj["joint"]}
+    # damped springs: stiffness and damping (cons_params columns 5 and 6)
+    spring = pw.a["cons_type"] == SPRING if pw.n_constraints else np.zeros(0, bool)
+    out["spring_k"] = adj["joint"][spring, 5] if pw.n_constraints else np.zeros(0)
+    out["spring_c"] = adj["joint"][spring, 6] if pw.n_constraints else np.zeros(0)
+    return out
 
 
 def position_loss(p, targets, dyn):

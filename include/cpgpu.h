@@ -245,14 +245,22 @@ int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_
  * replays each checkpoint segment with the step's own kernels and runs the
  * hand-written reverse kernels; the world is left as it was.  The contact
  * set and the clamp branches of the rollout are held fixed (derivatives
- * where they change are one-sided).  Spaces must be solved one workgroup each
- * and have no constraints; polygons of at most 8 vertices; no velocity or
- * force writes during a taped rollout. */
+ * where they change are one-sided).  Joints (pivot, damped spring, slide,
+ * gear) are differentiated too; their accumulated impulses carry adjoints
+ * between steps like the contacts' warm starts.  Spaces must be solved one
+ * workgroup each; polygons of at most 8 vertices; CPGPU_FP64 worlds; no
+ * velocity or force writes during a taped rollout. */
 int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every);
 int cpgpuWorldTapeEnd(cpgpuWorld *world);
 int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v,
                        const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0,
                        double *d_m_inv, double *d_i_inv, double *d_u);
+/* After cpgpuWorldBackward: dL/d of each constraint's parameters over the
+ * rollout, [n_constraints][8] in the layout of cpgpuWorldDesc::cons_params
+ * (spring: anchors, restLength, stiffness, damping; slide: anchors, min, max;
+ * pivot: anchors; gear: phase -- ratio entries are 0).  No counterpart in the
+ * reference. */
+int cpgpuWorldGetJointGrads(cpgpuWorld *world, double *d_params);
 
 /* Contacts solved and bodies velocity-integrated since creation by each
  * solve unit, in the order of the timing names solver_worlds_b0 ..

@@ -18,7 +18,10 @@ Other workloads: batch_4096 (4096 spaces, log-uniform 16..2048 bodies, 240
 steps), chains_2048 (2048 spaces of a 200-link jointed chain and 300
 circles, 300 steps), grad_1024 (1024 spaces of 64 circles / boxes: a taped
 120-step rollout and its adjoint; a step is one forward step and its share
-of the backward pass, so value is forward+adjoint body-steps/s).
+of the backward pass, so value is forward+adjoint body-steps/s),
+grad_chains_2048 (the chains_2048 spaces: a taped 300-step rollout and its
+adjoint for a loss on the final link positions, gradients w.r.t. every
+spring's stiffness and damping).
 
 roofline: the dominant kernel's algorithmic bytes per launch (streaming
 model: every phase reads and writes the state the reference's dataflow
@@ -71,6 +74,20 @@ def build_workload(name):
                     "loss": "sum |p_T - target|^2", "grads": "initial velocities, masses, friction",
                     "checkpoint_every": 10, "iterations": 10, "dt": 1.0 / 60.0, "seed": hex(scenes.SEED_GRAD),
                     "rollout_steps": 120}
+    if name == "grad_chains_2048":
+        pw = scenes.chains_packed(n_worlds=2048, processes=min(16, os.cpu_count() or 1))
+        links = np.zeros(pw.n_bodies, bool)
+        links[pw.a["cons_a"]] = True
+        links[pw.a["cons_b"]] = True
+        links &= pw.a["body_type"] == 0
+        # system identification: final link positions against those of the chains at rest, the
+        # gradients taken w.r.t. every spring's stiffness and damping
+        pw.targets = pw.a["body_p"].copy()
+        pw.loss_mask = links.astype(float)
+        return pw, {"workload": "grad_chains_2048", "worlds": pw.n_worlds, "links": 200, "circles": 300,
+                    "loss": "sum over links |p_T - p_0|^2", "grads": "spring stiffness and damping (and every "
+                    "joint parameter, initial velocities, masses, friction)", "checkpoint_every": 10,
+                    "iterations": 10, "dt": 1.0 / 60.0, "seed": hex(scenes.SEED_CHAINS), "rollout_steps": 300}
     if name == "batch_4096":
         pw = scenes.batch_packed(n_worlds=4096, processes=min(16, os.cpu_count() or 1))
         return pw, {"workload": "batch_4096", "worlds": pw.n_worlds, "bodies_log_uniform": [16, 2048],
@@ -164,7 +181,7 @@ def run_grad(pw, steps, warmup, dt, checkpoint_every=10):
     stats, loss)."""
     from chipmunk2d_amd.gpu import GPUWorld
     from chipmunk2d_amd import grad as G
-    dyn = (pw.a["body_type"] == G.DYNAMIC).astype(float)
+    dyn = getattr(pw, "loss_mask", (pw.a["body_type"] == G.DYNAMIC).astype(float))
     if warmup > 0:
         g = GPUWorld(pw)
         g.tape_begin(checkpoint_every)
@@ -231,7 +248,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's rollout length)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="random_world_100k",
-                    choices=["random_world_100k", "batch_4096", "chains_2048", "grad_1024"])
+                    choices=["random_world_100k", "batch_4096", "chains_2048", "grad_1024", "grad_chains_2048"])
     ap.add_argument("--cpu-sample-steps", type=int, default=120,
                     help="steps of the same rollout the reference CPU step is timed on")
     ap.add_argument("--cpu-batch-sample-steps", type=int, default=20,
@@ -263,7 +280,7 @@ def main():
 
     autotune = args.autotune_steps if args.workload in ("batch_4096", "chains_2048") else 0
     grad_extra = None
-    if args.workload == "grad_1024":
+    if args.workload in ("grad_1024", "grad_chains_2048"):
         if args.precision != "fp64":
             print("the adjoint runs on FP64 worlds only", file=sys.stderr)
             sys.exit(2)

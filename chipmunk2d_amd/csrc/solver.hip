@@ -25,32 +25,24 @@ namespace {
 
 constexpr int GRID_THREADS = 1024;  // grid solve: few, large workgroups keep the barrier cheap
 
-// s_waitcnt vmcnt(0): every vector memory access this wave issued has
-// completed (gfx9 encoding: vmcnt 0, expcnt 7, lgkmcnt 15)
-__device__ __forceinline__ void wait_vm() { __builtin_amdgcn_s_waitcnt(0x0F70); }
-
 // ---- grid barrier (all blocks co-resident: cooperative launch) ----
-// One lane per workgroup: a release fence (writes back this XCD's L2) before
-// arriving, a relaxed spin on the generation word, one acquire fence
-// (invalidates L2) after it changes.  Acquire loads inside the spin would
-// invalidate L2 on every poll while other workgroups still work.
-__device__ void grid_sync(GridBarrier *bar, int32_t *err)
+// The arrival count starts at 0 with each launch (k_bar_reset) and only
+// grows: barrier e of the launch completes when it reaches e * gridDim.x, so
+// the last arrival has nothing to reset or publish.  One lane per workgroup:
+// a release fence (writes back this XCD's L2) before arriving, a relaxed
+// spin on the count, one acquire fence (invalidates L2) when it is reached.
+// Acquire loads inside the spin would invalidate L2 on every poll while
+// other workgroups still work.
+__device__ void grid_sync(GridBarrier *bar, int32_t *err, unsigned int &epoch)
 {
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		const unsigned int nblocks = gridDim.x;
-		const unsigned int g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const unsigned int target = (++epoch) * gridDim.x;
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 		const unsigned int arrived = __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		if (arrived == nblocks - 1) {
-			// every other workgroup has arrived: its writes precede its arrival
-			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-			__hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			wait_vm();   // the reset lands before any workgroup can see the new generation and arrive again
-			__hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		} else {
+		if (arrived + 1 != target) {
 			long spins = 0;
-			while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+			while (__hip_atomic_load(&bar->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
 				__builtin_amdgcn_s_sleep(1);
 				if (++spins > (1L << 25)) {
 					// a barrier that never completes ends the kernel instead of hanging it
@@ -65,6 +57,12 @@ __device__ void grid_sync(GridBarrier *bar, int32_t *err)
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 	}
 	__syncthreads();
+}
+
+__global__ void k_bar_reset(GridBarrier *bar)
+{
+	bar->count = 0;
+	bar->gen = 0;
 }
 
 __device__ __forceinline__ bool is_dynamic(const DevWorld &W, int b) { return W.b_type[b] == CPGPU_BODY_DYNAMIC; }
@@ -422,7 +420,8 @@ struct GridSync {
 	int tid, stride;
 	GridBarrier *bar;
 	int32_t *err;
-	__device__ void sync() { grid_sync(bar, err); }
+	unsigned int epoch;   // barriers passed (used by lane 0)
+	__device__ void sync() { grid_sync(bar, err, epoch); }
 	__device__ int sweep_tid() const { return (int)(blockIdx.x + threadIdx.x * gridDim.x); }
 };
 
@@ -864,7 +863,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	T.lds_bodies = nullptr;
 	T.lds_cap = 0;
 	T.unit = CPGPU_MAX_BUCKETS + 1;
-	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err};
+	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err, 0u};
 	solve_set(W, S, T, sy);
 }
 
@@ -1163,6 +1162,7 @@ hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBuck
 	if (grid > 0) {
 		void *args[] = {(void *)&W, (void *)&S};
 		hipError_t e = hipSuccess;
+		k_bar_reset<<<1, 1, 0, stream>>>(S.bar);
 		TIMED(timer, "solver", stream,
 		      e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_grid), dim3(grid),
 		                                     dim3(GRID_THREADS), args, 0, stream));

@@ -872,15 +872,22 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 constexpr int GROUP_BLOCKS = 1024;
 
 // union-find over body indices; a root is hooked under the smaller root, so
-// every island's root is its least body index
+// every island's root is its least body index.
+//
+// The walks read and halve paths through this XCD's caches (workgroup-scope
+// relaxed atomics): a stale parent is still an ancestor, since a pointer only
+// ever moves to an ancestor, so a walk still ends at a body that was a root.
+// Only the hook is a device-scope compare-and-swap; when it finds that the
+// root was hooked meanwhile it returns the fresh parent and the walk goes on
+// from there.  Device-scope loads would go past L2 on every step of a walk.
 __device__ __forceinline__ int uf_find(int32_t *parent, int x)
 {
 	while (true) {
-		const int p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const int p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 		if (p == x) return x;
-		const int gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const int gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 		// path halving: x is no root, so only this pointer to an ancestor changes
-		if (gp != p) __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (gp != p) __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 		x = gp;
 	}
 }
@@ -940,11 +947,11 @@ __global__ void k_uf_flatten(const DevWorld W, const SolveBuffers S)
 	const int b = S.large_bodies[q];
 	int x = b;
 	while (true) {
-		const int p = __hip_atomic_load(&S.label[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const int p = __hip_atomic_load(&S.label[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 		if (p == x) break;
 		x = p;
 	}
-	__hip_atomic_store(&S.label[b], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__hip_atomic_store(&S.label[b], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ int island_of(const DevWorld &W, const SolveBuffers &S, int a, int b)

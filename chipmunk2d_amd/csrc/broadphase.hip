@@ -170,10 +170,26 @@ __global__ void __launch_bounds__(256) k_keys(const DevWorld W, const int32_t *s
 }
 
 // ---- radix sort ----
+// LSD passes over the bytes the key width can hold; the element count may
+// come from the device (n_dev), the launch is sized for the capacity.  A
+// pass whose digit is the same for every key (the keys' OR and AND, reduced
+// on the device while the keys were made, agree on it) is the identity
+// permutation of a stable sort: it becomes a copy, so the ping-pong of the
+// buffers stays fixed and no host round trip is needed.
 
-__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, int n, int shift, uint32_t *hist,
-                                                      int nblocks)
+__device__ __forceinline__ bool rs_digit_constant(const unsigned long long *orand, int shift)
 {
+	return (((orand[0] ^ orand[1]) >> shift) & 255ull) == 0ull;
+}
+
+__device__ __forceinline__ int rs_count(const int32_t *n_dev, int n) { return n_dev ? *n_dev : n; }
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, const int32_t *n_dev, int n_cap,
+                                                      int shift, uint32_t *hist, int nblocks,
+                                                      const unsigned long long *orand)
+{
+	if (rs_digit_constant(orand, shift)) return;
+	const int n = rs_count(n_dev, n_cap);
 	__shared__ uint32_t h[256];
 	const int tid = threadIdx.x;
 	h[tid] = 0;
@@ -209,8 +225,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan256(uint32_t v, uint32_t
 
 // exclusive scan of each digit's row of block counts (one workgroup per digit);
 // the row total goes to bin_total[digit]
-__global__ void __launch_bounds__(256) k_rs_scan_rows(uint32_t *hist, int nblocks, uint32_t *bin_total)
+__global__ void __launch_bounds__(256) k_rs_scan_rows(uint32_t *hist, int nblocks, uint32_t *bin_total, int shift,
+                                                     const unsigned long long *orand)
 {
+	if (rs_digit_constant(orand, shift)) return;
 	__shared__ uint32_t lds4[4];
 	uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
 	uint32_t run = 0;
@@ -226,8 +244,9 @@ __global__ void __launch_bounds__(256) k_rs_scan_rows(uint32_t *hist, int nblock
 }
 
 // exclusive scan of the 256 digit totals, in place
-__global__ void __launch_bounds__(256) k_rs_scan_bins(uint32_t *bin_total)
+__global__ void __launch_bounds__(256) k_rs_scan_bins(uint32_t *bin_total, int shift, const unsigned long long *orand)
 {
+	if (rs_digit_constant(orand, shift)) return;
 	__shared__ uint32_t lds4[4];
 	uint32_t tot;
 	const uint32_t ex = block_exclusive_scan256(bin_total[threadIdx.x], lds4, &tot);
@@ -235,9 +254,18 @@ __global__ void __launch_bounds__(256) k_rs_scan_bins(uint32_t *bin_total)
 }
 
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *kin, const int32_t *vin, uint64_t *kout,
-                                                         int32_t *vout, int n, int shift, const uint32_t *offsets,
-                                                         int nblocks, const uint32_t *bin_base)
+                                                         int32_t *vout, const int32_t *n_dev, int n_cap, int shift,
+                                                         const uint32_t *offsets, int nblocks,
+                                                         const uint32_t *bin_base, const unsigned long long *orand)
 {
+	const int n = rs_count(n_dev, n_cap);
+	if (rs_digit_constant(orand, shift)) {
+		for (int r = 0; r < RS_ITEMS; r++) {
+			const int i = blockIdx.x * RS_TILE + r * RS_THREADS + threadIdx.x;
+			if (i < n) { kout[i] = kin[i]; vout[i] = vin[i]; }
+		}
+		return;
+	}
 	__shared__ uint32_t base[256];
 	__shared__ uint32_t run[256];
 	__shared__ uint32_t wcnt[RS_THREADS / 64][256];
@@ -614,34 +642,29 @@ void radix_sort_free(RadixSortBuffers &b)
 	b = RadixSortBuffers();
 }
 
-hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n, RadixSortBuffers &b, hipStream_t stream,
-                            KernelTimer *timer)
+hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n_cap, const int32_t *n_dev, int key_bits,
+                            RadixSortBuffers &b, hipStream_t stream, KernelTimer *timer)
 {
-	if (n <= 1) return hipSuccess;
-	uint64_t orand[2];
-	HIP_TRY(hipMemcpyAsync(orand, b.orand, sizeof(orand), hipMemcpyDeviceToHost, stream));
-	HIP_TRY(hipStreamSynchronize(stream));
-	const uint64_t vary = orand[0] ^ orand[1];
-	const int nblocks = blocks_for(n, RS_TILE);
+	if (n_cap <= 1) return hipSuccess;
+	const int nblocks = blocks_for(n_cap, RS_TILE);
+	const unsigned long long *orand = (const unsigned long long *)b.orand;
 	uint64_t *kin = keys, *kout = b.keys_alt;
 	int32_t *vin = vals, *vout = b.vals_alt;
-	for (int d = 0; d < 8; d++) {
-		if (((vary >> (8 * d)) & 0xFF) == 0) continue;
-		const int shift = 8 * d;
+	for (int shift = 0; shift < key_bits; shift += 8) {
 		TIMED(timer, "radix_sort", stream, {
-			k_rs_hist<<<nblocks, RS_THREADS, 0, stream>>>(kin, n, shift, b.block_hist, nblocks);
-			k_rs_scan_rows<<<256, 256, 0, stream>>>(b.block_hist, nblocks, b.bin_total);
-			k_rs_scan_bins<<<1, 256, 0, stream>>>(b.bin_total);
-			k_rs_scatter<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, n, shift, b.block_hist, nblocks,
-			                                                 b.bin_total);
+			k_rs_hist<<<nblocks, RS_THREADS, 0, stream>>>(kin, n_dev, n_cap, shift, b.block_hist, nblocks, orand);
+			k_rs_scan_rows<<<256, 256, 0, stream>>>(b.block_hist, nblocks, b.bin_total, shift, orand);
+			k_rs_scan_bins<<<1, 256, 0, stream>>>(b.bin_total, shift, orand);
+			k_rs_scatter<<<nblocks, RS_THREADS, 0, stream>>>(kin, vin, kout, vout, n_dev, n_cap, shift, b.block_hist,
+			                                                 nblocks, b.bin_total, orand);
 		});
 		HIP_TRY(hipGetLastError());
 		uint64_t *tk = kin; kin = kout; kout = tk;
 		int32_t *tv = vin; vin = vout; vout = tv;
 	}
 	if (kin != keys) {
-		HIP_TRY(hipMemcpyAsync(keys, kin, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, stream));
-		HIP_TRY(hipMemcpyAsync(vals, vin, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, stream));
+		HIP_TRY(hipMemcpyAsync(keys, kin, sizeof(uint64_t) * n_cap, hipMemcpyDeviceToDevice, stream));
+		HIP_TRY(hipMemcpyAsync(vals, vin, sizeof(int32_t) * n_cap, hipMemcpyDeviceToDevice, stream));
 	}
 	return hipSuccess;
 }
@@ -692,7 +715,10 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 		                                                (unsigned long long *)sortbuf.orand);
 	});
 	HIP_TRY(hipGetLastError());
-	HIP_TRY(radix_sort_pairs(t.keys, t.leaf_shape, n, sortbuf, stream, timer));
+	// world index above the 44 Morton bits
+	int world_bits = 0;
+	while ((1 << world_bits) < n_worlds) world_bits++;
+	HIP_TRY(radix_sort_pairs(t.keys, t.leaf_shape, n, nullptr, WORLD_SHIFT + world_bits, sortbuf, stream, timer));
 	TIMED(timer, "bvh_build", stream, {
 		(void)hipMemsetAsync(t.parent, 0xFF, sizeof(int32_t) * 1, stream);
 		k_world_ranges<<<blocks_for(n, 256), 256, 0, stream>>>(t.keys, n, t.world_first, t.world_last,

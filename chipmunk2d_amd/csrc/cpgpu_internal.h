@@ -68,10 +68,13 @@ struct RadixSortBuffers {
 };
 void radix_sort_reserve(RadixSortBuffers &b, int n);
 void radix_sort_free(RadixSortBuffers &b);
-// Sorts (keys, vals) ascending by key, stable.  Reads back the varying-digit
-// mask (one device sync).  The result ends in keys/vals.
-hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n, RadixSortBuffers &b, hipStream_t stream,
-                            KernelTimer *timer);
+// Sorts (keys, vals) ascending by key, stable, over the low key_bits bits:
+// *n_dev elements (n_dev == nullptr: n_cap), at most n_cap.  b.orand holds
+// the keys' OR and AND (reduced on the device when they were made); digits
+// they agree on are copied, not sorted.  No host round trip; the result ends
+// in keys/vals.
+hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n_cap, const int32_t *n_dev, int key_bits,
+                            RadixSortBuffers &b, hipStream_t stream, KernelTimer *timer);
 
 // ---- LBVH (broadphase.hip) ----
 struct Lbvh {

@@ -129,7 +129,8 @@ struct SolveBuffers {
 // Group the solved arbiters: islands of the large worlds (union-find), the
 // pre-steps and velocity integration of the workgroup-solved islands, then
 // the arbiters sorted by key with the small worlds' ranges and the island
-// list.  Skipped for a single small space.
+// list.  Skipped for a single small space.  n_act bounds the solved arbiters
+// from above (launch sizes); the kernels read the count *S.n_act.
 hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
                         RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer);
 // Small worlds are launched in size buckets, each with its own workgroup

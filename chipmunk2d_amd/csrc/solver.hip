@@ -1085,6 +1085,7 @@ int solver_max_grid()
 hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
                         RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer)
 {
+	// n_act: an upper bound of *S.n_act (the kernels read the count)
 	const int nb = S.n_large_bodies;
 	const int edge_blocks = std::max(1, std::min((n_act + S.n_large_joints + 255) / 256, GROUP_BLOCKS));
 	if (large) {
@@ -1104,7 +1105,8 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
 		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
-		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, sortbuf, stream, timer));
+		// keys: a small world's index, or GRID_KEY, or ISLAND_KEY | island root (below bit 42)
+		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, 42, sortbuf, stream, timer));
 	}
 	if (large && nb > 0) {
 		TIMED(timer, "island_integrate", stream,

@@ -1025,27 +1025,20 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	SolveBuffers S = solve_buffers(w, w->act[cur], B.n_act, w->colour, w->order, w->warb_off);
 	int island_blocks = 0;
 	if (grouped) {
-		// the solved arbiters by world and island (one sync for the count)
-		HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, B.n_act, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
-		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
-		w->timer.collect();
-		const int n_act = w->h_counts[10];
-		HIP_OR_FAIL(solver_group(W, S, n_act, w->nw, w->n_large_worlds > 0, w->sortbuf, w->stream, timer));
-		if (w->n_large_worlds > 0) island_blocks = std::min(n_act, w->island_blocks_max);
+		// the solved arbiters by world and island; launches sized for the capacity (at most one solved
+		// arbiter per candidate pair), the kernels read the count
+		const int n_act_cap = std::max(n_pairs, 1);
+		HIP_OR_FAIL(solver_group(W, S, n_act_cap, w->nw, w->n_large_worlds > 0, w->sortbuf, w->stream, timer));
+		if (w->n_large_worlds > 0) island_blocks = std::min(n_act_cap, w->island_blocks_max);
 	}
 	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, island_blocks, w->n_large_worlds > 0 ? w->solver_grid : 0,
 	                        w->stream, w->aux, w->fork_join, timer));
 
 	if (w->tape.on) {
 		// record the solved arbiters (their contacts learn where they were recorded)
-		int n_act;
-		if (grouped) {
-			n_act = w->h_counts[10];
-		} else {
-			HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, B.n_act, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
-			HIP_OR_FAIL(hipStreamSynchronize(w->stream));
-			n_act = w->h_counts[10];
-		}
+		HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, B.n_act, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
+		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+		const int n_act = w->h_counts[10];
 		HIP_OR_FAIL(tape_reserve(w, w->tape.n + n_act));
 		HIP_OR_FAIL(adjoint_tape_record(w->act[cur], nullptr, n_act, w->tape.arb, (int)w->tape.n, w->stream));
 		w->tape.off.push_back(w->tape.n);

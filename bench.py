@@ -110,10 +110,14 @@ def algorithmic_bytes(kernel, per_step, n_dyn_bodies, n_dyn_shapes, iterations, 
     if kernel in solve_counts:
         uc, ub = solve_counts[kernel]
         return solver_bytes(uc / max(launches, 1), ub / max(launches, 1), iterations)
-    if kernel == "reverse_step":
+    if kernel == "reverse_solve":
         # reads the forward solve's state and writes as many adjoints (twice the forward's bytes), plus
-        # the recorded pre-state of every arbiter application (144 B)
-        return 2 * solver_bytes(c, n_dyn_bodies, iterations) + a * iterations * 144
+        # the recorded pre-state of every arbiter application (18 scalars)
+        return 2 * solver_bytes(c, n_dyn_bodies, iterations) + a * iterations * 18 * S
+    if kernel == "reverse_geometry":
+        # per arbiter: both shapes' geometry and poses, the arbiter and its adjoints (read), the bodies'
+        # position adjoints (written): the narrowphase's bytes twice over
+        return 2 * (a * (2 * 8 * S + 2 * (4 * S + 8) + 24) + a * (6 * S + 8) + c * (4 * S + 8))
     if kernel == "narrowphase":
         # per pair: two shapes' world geometry (~8s each for the mix), material 4s + 8 B each,
         # warm-start record 24 B; per colliding pair its arbiter header 6s + 8 B, per contact 4s + 8 B

@@ -31,6 +31,7 @@ namespace {
 
 constexpr int ADJ_THREADS = 256;
 constexpr int DUAL_MAX_VERTS = 8;
+constexpr int ADJ_GU_LDS = 4096;   // shapes of a space whose friction adjoints are summed in LDS
 
 __device__ __forceinline__ bool dyn(const DevWorld &W, int b) { return W.b_type[b] == CPGPU_BODY_DYNAMIC; }
 
@@ -104,6 +105,8 @@ struct ArbAdj {
 	V2 n, svr;
 	double u;
 	double jn[2], jt[2], jb[2];
+	int gu_s[2];          // the two shapes (space-local) and their friction adjoint terms
+	double gu_v[2];
 };
 
 __device__ __forceinline__ BodyV bodyv_from(const real *r)
@@ -641,8 +644,28 @@ __device__ void joint_prestep_reverse(const DevWorld &W, const AdjStep &A, const
 
 __device__ __forceinline__ void add2(double *p, V2 v) { p[0] += v.x; p[1] += v.y; }
 
-// The reverse of one step for one small space: one workgroup.
-__global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, const AdjStep A)
+// colour boundaries of space w's colour order, positions [lo, hi); returns
+// the colour count (col_start[0..n] in LDS)
+__device__ int reverse_colours(const AdjStep &A, int lo, int hi, int32_t *col_start, int32_t *n_col)
+{
+	const int tid = threadIdx.x, stride = blockDim.x;
+	if (tid == 0) *n_col = hi > lo ? A.colour[A.order[hi - 1]] + 1 : 0;
+	for (int pos = lo + tid; pos < hi; pos += stride) {
+		const int c = A.colour[A.order[pos]];
+		if (pos == lo || A.colour[A.order[pos - 1]] != c) col_start[c] = pos;
+	}
+	__syncthreads();
+	if (tid == 0) col_start[*n_col] = hi;
+	__syncthreads();
+	return *n_col;
+}
+
+// The reverse of one step for one small space (one workgroup), in two
+// kernels: the impulses (sweeps, cached impulses, velocity integration) here,
+// with few registers; the pre-steps and the contact geometry on dual numbers
+// (k_reverse_geometry) after it.  The per-arbiter and per-joint adjoints pass
+// between them in garb / gjoint.
+__global__ void __launch_bounds__(ADJ_THREADS) k_reverse_solve(const DevWorld W, const AdjStep A)
 {
 	__shared__ int32_t col_start[CPGPU_MAX_COLOURS + 1];
 	__shared__ int32_t n_col;
@@ -653,14 +676,7 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, 
 	const int hi = A.warb_off ? A.warb_off[w + 1] : A.n;
 	const WorldStep ws = W.ws[w];
 	const int b0 = W.w_body0[w], b1 = W.w_body0[w + 1];
-	// colour boundaries of the space's colour order
-	if (tid == 0) n_col = hi > lo ? A.colour[A.order[hi - 1]] + 1 : 0;
-	for (int pos = lo + tid; pos < hi; pos += stride) {
-		const int c = A.colour[A.order[pos]];
-		if (pos == lo || A.colour[A.order[pos - 1]] != c) col_start[c] = pos;
-	}
-	__syncthreads();
-	if (tid == 0) col_start[n_col] = hi;
+	reverse_colours(A, lo, hi, col_start, &n_col);
 	// per-arbiter adjoints: the final accumulators' adjoints come from later warm starts
 	for (int pos = lo + tid; pos < hi; pos += stride) {
 		const Arbiter &arb = A.act[A.order[pos]];
@@ -814,24 +830,55 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, 
 		}
 		__syncthreads();
 	}
-	// In a fixed order (one lane; runs are reproducible bit for bit): the warm
-	// starts send the adjoints of the initial accumulators to the contacts they
-	// came from, and the friction adjoint goes to the two shapes (u = uA uB).
-	if (tid == 0) {
-		for (int pos = lo; pos < hi; pos++) {
-			const Arbiter &arb = A.act[A.order[pos]];
-			const ArbAdj &G = garb[pos];
-			for (int i = 0; i < arb.count; i++) {
-				const int src = arb.c[i].src;
-				if (src >= 0) {
-					A.gJ[src].x += G.jn[i];
-					A.gJ[src].y += G.jt[i];
-				}
+	// The warm starts send the adjoints of the initial accumulators to the
+	// contacts they came from.  An arbiter's sources are contacts of its own
+	// shape pair's earlier arbiter, so each arbiter's lane adds them alone.
+	// The friction adjoint goes to both shapes (u = uA uB); a shape has many
+	// arbiters, so those sums run in a fixed order on one lane (gradients are
+	// reproducible bit for bit), into LDS when the space's shapes fit.
+	__shared__ double lgu[ADJ_GU_LDS];
+	const int s0 = W.w_shape0[w], ns_w = W.w_shape0[w + 1] - s0;
+	const bool gu_lds = ns_w <= ADJ_GU_LDS;
+	for (int pos = lo + tid; pos < hi; pos += stride) {
+		const Arbiter &arb = A.act[A.order[pos]];
+		ArbAdj &G = garb[pos];
+		for (int i = 0; i < arb.count; i++) {
+			const int src = arb.c[i].src;
+			if (src >= 0) {
+				A.gJ[src].x += G.jn[i];
+				A.gJ[src].y += G.jt[i];
 			}
-			A.gu[arb.sa] += G.u * W.s_u[arb.sb];
-			A.gu[arb.sb] += G.u * W.s_u[arb.sa];
+		}
+		G.gu_s[0] = arb.sa - s0; G.gu_s[1] = arb.sb - s0;
+		G.gu_v[0] = G.u * W.s_u[arb.sb];
+		G.gu_v[1] = G.u * W.s_u[arb.sa];
+	}
+	if (gu_lds)
+		for (int q = tid; q < ns_w; q += stride) lgu[q] = 0.0;
+	__syncthreads();
+	if (tid == 0) {
+		double *gu = gu_lds ? lgu : A.gu + s0;
+		int pos = lo;
+		// four arbiters' terms are loaded ahead of their sums
+		for (; pos + 4 <= hi; pos += 4) {
+			int sh[8];
+			double v[8];
+			for (int q = 0; q < 4; q++) {
+				const ArbAdj &G = garb[pos + q];
+				sh[2 * q] = G.gu_s[0]; sh[2 * q + 1] = G.gu_s[1];
+				v[2 * q] = G.gu_v[0]; v[2 * q + 1] = G.gu_v[1];
+			}
+			for (int q = 0; q < 8; q++) gu[sh[q]] += v[q];
+		}
+		for (; pos < hi; pos++) {
+			const ArbAdj &G = garb[pos];
+			gu[G.gu_s[0]] += G.gu_v[0];
+			gu[G.gu_s[1]] += G.gu_v[1];
 		}
 	}
+	__syncthreads();
+	if (gu_lds)
+		for (int q = tid; q < ns_w; q += stride) A.gu[s0 + q] += lgu[q];
 	__syncthreads();
 
 	// ---- velocity integration (cpBodyUpdateVelocity): v = v damping + (g + f m_inv) dt ----
@@ -845,6 +892,24 @@ __global__ void __launch_bounds__(ADJ_THREADS) k_reverse_step(const DevWorld W, 
 		GB.w *= ws.damping;
 	}
 	__syncthreads();
+
+}
+
+__global__ void __launch_bounds__(ADJ_THREADS) k_reverse_geometry(const DevWorld W, const AdjStep A)
+{
+	__shared__ int32_t col_start[CPGPU_MAX_COLOURS + 1];
+	__shared__ int32_t n_col;
+	const int w = blockIdx.x;
+	const int tid = threadIdx.x, stride = blockDim.x;
+	ArbAdj *garb = reinterpret_cast<ArbAdj *>(A.garb);
+	const int lo = A.warb_off ? A.warb_off[w] : 0;
+	const int hi = A.warb_off ? A.warb_off[w + 1] : A.n;
+	const WorldStep ws = W.ws[w];
+	const int b0 = W.w_body0[w], b1 = W.w_body0[w + 1];
+	const int ncol = reverse_colours(A, lo, hi, col_start, &n_col);
+	const int32_t *jcol = A.nc > 0 ? A.jw_col + A.jw_cbase[w] : nullptr;
+	const int njc = A.nc > 0 ? A.jw_ncol[w] : 0;
+	JointAdj *gjt = reinterpret_cast<JointAdj *>(A.gjoint);
 
 	// ---- joint pre-steps (after the arbiters' pre-steps, before the velocity integration) ----
 	for (int jc = njc - 1; jc >= 0; jc--) {
@@ -1036,9 +1101,15 @@ hipError_t adjoint_state_load(const DevWorld &W, const BodyState *in, hipStream_
 size_t adjoint_arb_scratch_bytes() { return sizeof(ArbAdj); }
 size_t adjoint_joint_scratch_bytes() { return sizeof(JointAdj); }
 
-hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream)
+hipError_t adjoint_reverse_solve(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream)
 {
-	k_reverse_step<<<n_worlds, ADJ_THREADS, 0, stream>>>(W, A);
+	k_reverse_solve<<<n_worlds, ADJ_THREADS, 0, stream>>>(W, A);
+	return hipGetLastError();
+}
+
+hipError_t adjoint_reverse_geometry(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream)
+{
+	k_reverse_geometry<<<n_worlds, ADJ_THREADS, 0, stream>>>(W, A);
 	return hipGetLastError();
 }
 

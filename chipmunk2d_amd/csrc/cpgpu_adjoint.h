@@ -51,7 +51,9 @@ hipError_t adjoint_tape_record(Arbiter *arbs, const int32_t *n_dev, int n_host, 
 hipError_t adjoint_replay_prepare(const Arbiter *tape, int off, int n, Arbiter *act, hipStream_t stream);
 hipError_t adjoint_state_save(const DevWorld &W, BodyState *out, hipStream_t stream);
 hipError_t adjoint_state_load(const DevWorld &W, const BodyState *in, hipStream_t stream);
-hipError_t adjoint_reverse_step(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream);
+// the reverse of one replayed step: the impulses, then the pre-steps and contact geometry
+hipError_t adjoint_reverse_solve(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream);
+hipError_t adjoint_reverse_geometry(const DevWorld &W, const AdjStep &A, int n_worlds, hipStream_t stream);
 // bytes of the per-arbiter / per-joint adjoint scratch (AdjStep::garb, gjoint) per element
 size_t adjoint_arb_scratch_bytes();
 size_t adjoint_joint_scratch_bytes();

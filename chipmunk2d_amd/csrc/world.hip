@@ -1492,7 +1492,8 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 			A.jrec = jrec + (size_t)nc * (max_it + 1) * q;
 			A.jw_order = w->d_jw_order; A.jw_col = w->d_jw_col; A.jw_cbase = w->d_jw_cbase; A.jw_ncol = w->d_jw_ncol;
 			A.gjoint = gjoint; A.gJacc = gJacc; A.gP = gP;
-			TIMED(timer, "reverse_step", w->stream, ADJ_TRY(adjoint_reverse_step(W, A, w->nw, w->stream)));
+			TIMED(timer, "reverse_solve", w->stream, ADJ_TRY(adjoint_reverse_solve(W, A, w->nw, w->stream)));
+			TIMED(timer, "reverse_geometry", w->stream, ADJ_TRY(adjoint_reverse_geometry(W, A, w->nw, w->stream)));
 		}
 		if (result == CPGPU_OK) result = sync_world(w);
 	}

@@ -162,7 +162,7 @@ def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60, precision=0
     g = GPUWorld(pw, precision)
     if autotune_steps > 0:
         g.set_buckets([b[0] for b in tiling], [b[1] for b in tiling])
-        tiling = g.buckets()
+    tiling = g.buckets()
     g.sync()
     g.set_timing(True)
     t_step = []
@@ -298,10 +298,12 @@ def main():
     else:
         wall, t_steps, timing, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt, autotune,
                                                                       precision=precision)
-    if autotune:
-        config["tiling"] = {"autotuned": True, "trial_steps": autotune, "tuned_after_steps": 60,
-                            "buckets": [{"max_bodies": b[0], "threads": b[1], "worlds": b[2], "lds_bodies": b[3]}
-                                        for b in tiling if b[2] > 0]}
+    if tiling:
+        config["tiling"] = {"autotuned": bool(autotune), "buckets": [
+            {"bucket": i, "max_bodies": b[0], "threads": b[1], "worlds": b[2], "lds_bodies": b[3]}
+            for i, b in enumerate(tiling) if b[2] > 0]}
+        if autotune:
+            config["tiling"].update({"trial_steps": autotune, "tuned_after_steps": 60})
     value = n_dyn * args.steps / wall
     per_step = {"contacts": stats["total_contacts"] / args.steps, "arbiters": stats["total_arbiters"] / args.steps,
                 "pairs": stats["total_pairs"] / args.steps}

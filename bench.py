@@ -29,6 +29,8 @@ touches, FP64) over its average launch time from HIP events recorded on
 the library's stream during the timed region, against 8 TB/s HBM.
 cpu_baseline: the reference's own cpSpaceStep (compiled from its sources,
 strict IEEE build, one thread) on the first steps of the same rollout.
+plain_per_space_gpu (batch_4096): a sample of the batch's spaces stepped one
+at a time by the plain single-space GPU step, beside the batched rate.
 
 --precision fp32 steps CPGPU_FP32 worlds (float state and arithmetic,
 4-byte scalars in the byte model) and adds a "drift" object: the FP32
@@ -223,6 +225,30 @@ def run_cpu_reference(pw, sample_steps, dt, hasty_threads=0, release=False):
     return time.perf_counter() - t0
 
 
+def run_plain_per_space(pw, n_sample, steps, dt):
+    """The batch's spaces stepped one at a time by the plain single-space GPU step: every
+    (n_worlds / n_sample)-th space of the seeded batch rebuilt alone, `steps` steps each after one
+    untimed step.  Returns (body-steps/s, bodies in the sample, seconds)."""
+    from chipmunk2d_amd import scenes
+    from chipmunk2d_amd.gpu import GPUWorld
+    specs = scenes.batch_specs(n_worlds=pw.n_worlds)
+    stride = max(1, pw.n_worlds // n_sample)
+    spaces = [scenes.build_batch_world(specs[k]).pack() for k in range(0, pw.n_worlds, stride)][:n_sample]
+    worlds = [GPUWorld(sp) for sp in spaces]
+    for g in worlds:
+        g.step(dt)
+        g.sync()
+    t0 = time.perf_counter()
+    for g in worlds:
+        g.step(dt, steps)
+        g.sync()
+    t = time.perf_counter() - t0
+    bodies = sum(sp.dynamic_body_count() for sp in spaces)
+    for g in worlds:
+        g.close()
+    return bodies * steps / t, bodies, t
+
+
 def run_cpu_reference_batch(pw, sample_steps, dt, threads):
     """Seconds for the reference's cpSpaceStep to step every world of pw sample_steps times, the worlds
     split into contiguous slices over `threads` host threads (each slice stepped one step at a time,
@@ -345,7 +371,15 @@ def main():
                 "hasty_space": {"value": round(n_dyn * k / t_hasty, 1), "threads_requested": os.cpu_count(),
                                 "note": "cpHastySpace clamps its solver to 2 threads (cpHastySpace.c:387,534)"},
             }
-    elif not args.no_cpu_baseline and args.workload in ("batch_4096", "chains_2048"):
+    plain = None
+    if args.workload == "batch_4096":
+        # the same spaces stepped one at a time by the plain per-space GPU step (each its own world)
+        rate, bodies, secs = run_plain_per_space(pw, 64, 20, dt)
+        plain = {"value": round(rate, 1), "unit": "body-steps/s",
+                 "sample": "every 64th of the 4096 spaces (%d dynamic bodies), each alone, 20 steps after one "
+                           "untimed step, stepped one space after another" % bodies,
+                 "seconds": round(secs, 3), "batched_over_plain": round(value / rate, 1)}
+    if not args.no_cpu_baseline and args.workload in ("batch_4096", "chains_2048"):
         k = min(args.cpu_batch_sample_steps, args.steps)
         threads = min(16, os.cpu_count() or 1)
         t_cpu = run_cpu_reference_batch(pw, k, dt, threads)
@@ -380,6 +414,7 @@ def main():
         "avg_arbiters_per_step": round(per_step["arbiters"], 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "plain_per_space_gpu": plain,
         "kernels": kernels,
     }
     if grad_extra:

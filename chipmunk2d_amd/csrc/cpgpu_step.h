@@ -25,7 +25,12 @@ struct StepBuffers {
 	Arbiter *cached;           // untouched arbiters within persistence
 	int32_t *n_cached;
 	int cap_cached;
+	// candidate pairs grouped by their shapes' type pair (the narrowphase's input order)
+	int2 *pairs_by_type;       // [pair capacity]
+	int32_t *type_counts;      // [2 * PAIR_CLASSES]: counts, then cursors
 };
+
+#define PAIR_CLASSES 9         // type_a * 3 + type_b (type_a <= type_b)
 
 hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 *pairs, const int32_t *n_pairs_dev,
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer);

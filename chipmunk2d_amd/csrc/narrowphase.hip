@@ -220,6 +220,71 @@ __global__ void k_carry(const DevWorld W, const StepBuffers B)
 	}
 }
 
+__global__ void k_fill_i32(int32_t *p, int n, int32_t v)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = v;
+}
+
+// ---- grouping the candidate pairs by shape types ----
+// One lane per pair runs the whole cpCollide dispatch; a wave pays for every
+// type path any of its lanes takes.  Pairs grouped by (type_a, type_b) before
+// the narrowphase fill waves with one path each.  The order of the pairs does
+// not change any result: every arbiter is found by its shape pair's key.
+
+__device__ __forceinline__ int pair_class(const DevWorld &W, int2 pr)
+{
+	return W.s_type[pr.x] * 3 + W.s_type[pr.y];
+}
+
+__global__ void __launch_bounds__(256) k_pair_class_count(const DevWorld W, const int2 *pairs, int n,
+                                                          int32_t *counts)
+{
+	__shared__ int32_t h[PAIR_CLASSES];
+	if (threadIdx.x < PAIR_CLASSES) h[threadIdx.x] = 0;
+	__syncthreads();
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+		atomicAdd(&h[pair_class(W, pairs[i])], 1);
+	__syncthreads();
+	if (threadIdx.x < PAIR_CLASSES && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
+}
+
+// counts -> class starts (cursors)
+__global__ void k_pair_class_scan(int32_t *counts)
+{
+	if (threadIdx.x != 0) return;
+	int run = 0;
+	for (int c = 0; c < PAIR_CLASSES; c++) {
+		const int k = counts[c];
+		counts[PAIR_CLASSES + c] = run;
+		run += k;
+	}
+}
+
+// each workgroup counts its pairs per class in LDS, reserves one range per
+// class, then writes its pairs into them
+__global__ void __launch_bounds__(256) k_pair_class_scatter(const DevWorld W, const int2 *pairs, int n,
+                                                            int32_t *cursors, int2 *out)
+{
+	__shared__ int32_t h[PAIR_CLASSES], base[PAIR_CLASSES];
+	const int per_block = (n + gridDim.x - 1) / gridDim.x;
+	const int lo = blockIdx.x * per_block, hi = min(n, lo + per_block);
+	if (threadIdx.x < PAIR_CLASSES) h[threadIdx.x] = 0;
+	__syncthreads();
+	for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[pair_class(W, pairs[i])], 1);
+	__syncthreads();
+	if (threadIdx.x < PAIR_CLASSES) {
+		base[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&cursors[threadIdx.x], h[threadIdx.x]) : 0;
+		h[threadIdx.x] = 0;
+	}
+	__syncthreads();
+	for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+		const int2 pr = pairs[i];
+		const int c = pair_class(W, pr);
+		out[base[c] + atomicAdd(&h[c], 1)] = pr;
+	}
+}
+
 inline int blocks_for(int n, int t) { return (n + t - 1) / t; }
 
 } // namespace
@@ -238,8 +303,16 @@ hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 
 	});
 	HIP_TRY(hipGetLastError());
 	if (n_pairs_host > 0) {
+		const int g = std::min(blocks_for(n_pairs_host, 256), 1024);
+		TIMED(timer, "pair_classes", stream, {
+			k_fill_i32<<<1, 2 * PAIR_CLASSES, 0, stream>>>(B.type_counts, 2 * PAIR_CLASSES, 0);
+			k_pair_class_count<<<g, 256, 0, stream>>>(W, pairs, n_pairs_host, B.type_counts);
+			k_pair_class_scan<<<1, 64, 0, stream>>>(B.type_counts);
+			k_pair_class_scatter<<<g, 256, 0, stream>>>(W, pairs, n_pairs_host, B.type_counts + PAIR_CLASSES,
+			                                            B.pairs_by_type);
+		});
 		TIMED(timer, "narrowphase", stream,
-		      k_narrowphase<<<blocks_for(n_pairs_host, 256), 256, 0, stream>>>(W, B, pairs, n_pairs_host));
+		      k_narrowphase<<<blocks_for(n_pairs_host, 256), 256, 0, stream>>>(W, B, B.pairs_by_type, n_pairs_host));
 		HIP_TRY(hipGetLastError());
 	}
 	if (n_prev_recs_host > 0) {

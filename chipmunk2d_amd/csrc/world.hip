@@ -244,6 +244,9 @@ struct cpgpuWorld {
 	real4 *d_world_bounds = nullptr;
 	int2 *d_pairs = nullptr;
 	int pair_cap = 0;
+	int2 *d_pairs_by_type = nullptr;    // the pairs grouped by shape types (StepBuffers::pairs_by_type)
+	int pairs_by_type_cap = 0;
+	int32_t *d_type_counts = nullptr;
 	int32_t *d_counters = nullptr;      // [0] n_pairs
 	// persistence (double-buffered)
 	PairRec *recs[2] = {nullptr, nullptr};
@@ -788,6 +791,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	// initial capacities
 	const long base = std::max<long>(4L * w->n_dyn_shapes, 4096);
 	HIP_OR_FAIL(regrow(w->d_pairs, w->pair_cap, base));
+	HIP_OR_FAIL(own_alloc(w, w->d_type_counts, 2 * PAIR_CLASSES));
 	for (int k = 0; k < 2; k++) {
 		HIP_OR_FAIL(regrow(w->recs[k], w->rec_cap[k], 2 * base));
 		HIP_OR_FAIL(regrow(w->act[k], w->act_cap[k], base));
@@ -810,7 +814,7 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	lbvh_free(w->dyn_tree);
 	lbvh_free(w->static_tree);
 	radix_sort_free(w->sortbuf);
-	void *extra[] = {w->d_pairs, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
+	void *extra[] = {w->d_pairs, w->d_pairs_by_type, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
 	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys, w->groups,
 	                 w->cbody, w->cprio, w->ccol};
 	for (void *p : extra) if (p) (void)hipFree(p);
@@ -1019,6 +1023,9 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	B.recs = w->recs[cur]; B.n_recs = w->d_step_counts + 3 * cur + 0; B.cap_recs = w->rec_cap[cur];
 	B.act = w->act[cur]; B.n_act = w->d_step_counts + 3 * cur + 1; B.cap_act = w->act_cap[cur];
 	B.cached = w->cached[cur]; B.n_cached = w->d_step_counts + 3 * cur + 2; B.cap_cached = w->cached_cap[cur];
+	HIP_OR_FAIL(regrow(w->d_pairs_by_type, w->pairs_by_type_cap, (long)n_pairs + 1));
+	B.pairs_by_type = w->d_pairs_by_type;
+	B.type_counts = w->d_type_counts;
 	HIP_OR_FAIL(narrowphase_step(W, B, w->d_pairs, w->d_counters, n_pairs, n_prev_recs, w->stream, timer));
 
 	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;

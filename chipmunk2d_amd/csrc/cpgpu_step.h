@@ -51,12 +51,25 @@ struct GridBarrier {
 //     Islands are body-disjoint, so each can be solved on its own in its
 //     colour order with the same result as the whole set.  Joint-free
 //     islands of at most island_max arbiters take a workgroup each
-//     (k_solve_worlds again); the rest are solved together on a co-resident
-//     grid (k_solve_grid) with grid barriers between colours.
+//     (k_solve_worlds again); the rest form the grid set, solved together
+//     on a co-resident grid (k_solve_grid).
+// The grid set of each large world is cut into GRID_PARTS strips along x
+// with equal numbers of its dynamic bodies (quantiles of a PART_BINS-bin
+// histogram of their x between the set's least and greatest x).  An arbiter
+// whose dynamic bodies lie in one strip is interior to it; the others cross
+// strips (the boundary).  Strips share no dynamic body, so each workgroup
+// colours and sweeps its strips with workgroup barriers only; the boundary
+// is coloured on its own and swept after every strip, with grid barriers.
+// The solve order is therefore: each strip in its colour order, then the
+// boundary in its colour order.
 // Sorted arbiter keys: small world w -> w; large-world island with root
-// body r -> 1 << 40 | r; grid set -> 2 << 40.
+// body r -> 1 << 40 | r; strip q of the l-th large world -> 2 << 40 | l << 8 | q;
+// boundary -> BOUNDARY_KEY, the greatest key.
 #define ISLAND_KEY (1ull << 40)
 #define GRID_KEY (2ull << 40)
+#define BOUNDARY_KEY (GRID_KEY | ((1ull << 40) - 1))
+#define GRID_PARTS 64
+#define PART_BINS 1024
 
 // Pre-state of one arbiter's impulse application in one sweep (rollout
 // tape): both bodies' velocities and bias velocities, and each contact's
@@ -86,7 +99,7 @@ struct SolveBuffers {
 	int32_t *arb_by_world;     // [cap] active arbiters sorted by key; nullptr for a single small space
 	uint64_t *keys;            // [cap] the sorted keys
 	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world (the last: islands)
-	int32_t *large_first;      // [1] first sorted position of the grid set
+	int32_t *large_first;      // [2] first sorted position of the grid set, of the boundary
 	// islands of the large worlds
 	int32_t *label;            // [n_bodies] union-find parent, then island root (the least body index)
 	int32_t *icount;           // [n_bodies] per root: arbiters of the island
@@ -123,6 +136,15 @@ struct SolveBuffers {
 	int n_bodies;
 	const int32_t *large_bodies;   // non-static bodies of large worlds
 	int n_large_bodies;
+	// strips of the grid set (n_parts = GRID_PARTS per large world)
+	const int32_t *large_index;    // [n_worlds] ordinal among the large worlds, -1 for a small world
+	int n_parts;
+	unsigned long long *pbound;    // [2 per large world] least and greatest x of the grid set (ordered keys)
+	int32_t *phist;                // [PART_BINS per large world] bin counts, then each bin's strip
+	int32_t *bpart;                // [n_bodies] strip of each grid-set body
+	int32_t *pcol;                 // [n_parts][CPGPU_MAX_COLOURS + 1] colour offsets of each strip
+	int2 *pinfo;                   // [n_parts] first sorted position and colour count of each strip
+	int boundary_tail_max;         // boundary colours of at most this many arbiters: workgroup 0 alone
 	// recording (adjoint replay): rec[it * rec_stride + position in order], or nullptr
 	ArbRec *rec;
 	int rec_stride;

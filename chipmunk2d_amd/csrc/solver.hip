@@ -585,8 +585,10 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 	}
 }
 
+// Colour the set's arbiters and sort them into colour order (T.order,
+// T.col_off); returns the colour count.
 template <class Sync>
-__device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy)
+__device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy)
 {
 	const int tid = sy.tid, stride = sy.stride, n = T.n;
 	unsigned long long *claim0 = S.claim, *claim1 = S.claim + S.n_bodies;
@@ -725,6 +727,15 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 			T.order[pos] = arb_index(k);
 		}
 	}
+	return n_colours;
+}
+
+template <class Sync>
+__device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy)
+{
+	const int tid = sy.tid, stride = sy.stride, n = T.n;
+	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
+	const int n_colours = colour_set(W, S, T, sy);
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----
 	if (T.prestep)
 		for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);
@@ -837,34 +848,149 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 	}
 }
 
-// All large worlds together on the co-resident grid.
+// The grid set of all large worlds on the co-resident grid.  Each workgroup
+// colours and sweeps its strips with workgroup barriers; the boundary is
+// coloured on the whole grid and swept after every strip sweep; joints follow
+// the arbiters in every sweep, as in cpSpaceStep.
 __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, const SolveBuffers S)
 {
-	SolveSet T;
+	__shared__ int32_t lcol_off[CPGPU_MAX_COLOURS + 1];
+	__shared__ int32_t lcol_cursor[CPGPU_MAX_COLOURS];
+	__shared__ int32_t lleft;
 	const int n_act = *S.n_act;
-	T.list = S.arb_by_world ? S.arb_by_world + S.large_first[0] : nullptr;
-	T.n = S.arb_by_world ? n_act - S.large_first[0] : n_act;
-	T.order_base = S.arb_by_world ? S.large_first[0] : 0;
-	T.order = S.order + T.order_base;
+	const int grid_first = S.large_first[0], bound_first = S.large_first[1];
+	const int unit = CPGPU_MAX_BUCKETS + 1;
+	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err, 0u};
+	BlockSync bs{(int)threadIdx.x, (int)blockDim.x};
+
+	// ---- each strip coloured by its workgroup ----
+	for (int u = blockIdx.x; u < S.n_parts; u += gridDim.x) {
+		const uint64_t key = GRID_KEY | ((uint64_t)(u / GRID_PARTS) << 8) | (uint64_t)(u % GRID_PARTS);
+		const int start = lower_bound_u64(S.keys, grid_first, bound_first, key);
+		const int end = lower_bound_u64(S.keys, start, bound_first, key + 1);
+		int nc = 0;
+		if (end > start) {
+			SolveSet P = {};
+			P.list = S.arb_by_world + start;
+			P.n = end - start;
+			P.order = S.order + start;
+			P.order_base = start;
+			P.col_off = lcol_off;
+			P.col_cursor = lcol_cursor;
+			P.left = &lleft;
+			P.unit = unit;
+			nc = colour_set(W, S, P, bs);
+			__syncthreads();
+		}
+		int32_t *pc = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
+		for (int q = threadIdx.x; q <= nc; q += blockDim.x) pc[q] = lcol_off[q];
+		if (threadIdx.x == 0) S.pinfo[u] = make_int2(start, nc);
+		__syncthreads();
+	}
+	sy.sync();
+
+	// ---- the boundary, coloured on the whole grid ----
+	SolveSet T = {};
+	T.list = S.arb_by_world + bound_first;
+	T.n = n_act - bound_first;
+	T.order = S.order + bound_first;
+	T.order_base = bound_first;
 	T.col_off = S.col_off;
 	T.col_cursor = S.col_cursor;
 	T.left = S.left;
 	T.n_colours_out = S.n_colours;
-	// pre-steps (and, without joints, the velocity integration) ran before, on the whole device
-	T.bodies = S.large_bodies;
-	T.body0 = 0;
-	T.nb = S.n_large_joints > 0 ? S.n_large_bodies : 0;
-	T.jorder = S.jorder;
-	T.jcol_off = S.jcol_off;
-	T.njcol = S.n_jcolours;
-	T.iterations = S.max_iterations;
-	T.prestep = false;
-	T.island_filter = true;
-	T.lds_bodies = nullptr;
-	T.lds_cap = 0;
-	T.unit = CPGPU_MAX_BUCKETS + 1;
-	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err, 0u};
-	solve_set(W, S, T, sy);
+	T.unit = unit;
+	const int n_bcol = colour_set(W, S, T, sy);
+	sy.sync();
+
+	// ---- joint pre-steps (cpSpaceStep.c:389-396) and, with joints, the velocity
+	// integration of the grid set (cpSpaceStep.c:399-404; without joints it ran before) ----
+	for (int jc = 0; jc < S.n_jcolours; jc++) {
+		for (int k = S.jcol_off[jc] + sy.tid; k < S.jcol_off[jc + 1]; k += sy.stride)
+			joint_prestep(W, W.joints[S.jorder[k]]);
+		sy.sync();
+	}
+	if (S.n_large_joints > 0) {
+		unsigned long long n_int = 0;
+		for (int q = sy.tid; q < S.n_large_bodies; q += sy.stride) {
+			const int b = S.large_bodies[q];
+			if (island_on_workgroup(S, S.label[b])) continue;
+			integrate_velocity(W, b);
+			n_int++;
+		}
+		wave_add(&W.totals[3 + 2 * unit], n_int);
+		sy.sync();
+	}
+
+	// ---- sweeps: the strips, then the boundary, then the joints ----
+	const GlobalBodies bodies{&W};
+	int loaded = -1;   // the strip whose colour offsets are in lcol_off
+	auto strips = [&](auto f) {
+		for (int u = blockIdx.x; u < S.n_parts; u += gridDim.x) {
+			const int2 info = S.pinfo[u];
+			if (loaded != u) {
+				const int32_t *pc = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
+				for (int q = threadIdx.x; q <= info.y; q += blockDim.x) lcol_off[q] = pc[q];
+				loaded = u;
+				__syncthreads();
+			}
+			for (int q = 0; q < info.y; q++) {
+				for (int k = lcol_off[q] + (int)threadIdx.x; k < lcol_off[q + 1]; k += (int)blockDim.x) f(info.x + k);
+				__syncthreads();
+			}
+		}
+		sy.sync();
+	};
+	auto joints = [&](auto f) {
+		for (int jc = 0; jc < S.n_jcolours; jc++) {
+			for (int k = S.jcol_off[jc] + sy.sweep_tid(); k < S.jcol_off[jc + 1]; k += sy.stride) f(W.joints[S.jorder[k]]);
+			sy.sync();
+		}
+	};
+	// cached impulses (cpSpaceStep.c:407-415)
+	auto cached = [&](int pos) { arb_apply_cached(W, bodies, S.act[S.order[pos]]); };
+	strips(cached);
+	for_colours(sy, T.col_off, n_bcol, S.boundary_tail_max, [&](int k) { cached(bound_first + k); });
+	joints([&](Joint &j) { joint_apply_cached(W, bodies, j); });
+	// the impulse iterations (cpSpaceStep.c:418-427)
+	for (int it = 0; it < S.max_iterations; it++) {
+		auto apply = [&](int pos) {
+			Arbiter &arb = S.act[S.order[pos]];
+			if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, bodies, arb);
+		};
+		strips(apply);
+		for_colours(sy, T.col_off, n_bcol, S.boundary_tail_max, [&](int k) { apply(bound_first + k); });
+		joints([&](Joint &j) {
+			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);
+		});
+	}
+}
+
+// ---- strips of the grid set ----
+
+// a double's bits as an unsigned key with the same order (x + 0.0: -0.0 becomes +0.0)
+__device__ __forceinline__ unsigned long long x_key(double x)
+{
+	const unsigned long long u = (unsigned long long)__double_as_longlong(x + 0.0);
+	return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key_x(unsigned long long k)
+{
+	return __longlong_as_double((long long)((k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k));
+}
+
+// histogram bin of x in [lo, hi]
+__device__ __forceinline__ int part_bin(double x, double lo, double hi)
+{
+	if (!(hi > lo)) return 0;
+	const double t = (x - lo) * ((double)PART_BINS / (hi - lo));
+	if (!(t >= 0.0)) return 0;
+	return t >= (double)PART_BINS ? PART_BINS - 1 : (int)t;
+}
+
+__device__ __forceinline__ bool in_grid_set(const DevWorld &W, const SolveBuffers &S, int b)
+{
+	return is_dynamic(W, b) && !island_on_workgroup(S, S.label[b]);
 }
 
 // ---- grouping the solved arbiters: islands and keys ----
@@ -911,6 +1037,11 @@ __global__ void k_uf_init(const SolveBuffers S)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q == 0) { S.n_groups[0] = 0; S.n_groups[1] = 0; }
+	const int n_large = S.n_parts / GRID_PARTS;
+	for (int i = q; i < n_large * PART_BINS; i += gridDim.x * blockDim.x) {
+		S.phist[i] = 0;
+		if (i < n_large) { S.pbound[2 * i] = ~0ull; S.pbound[2 * i + 1] = 0ull; }
+	}
 	if (q >= S.n_large_bodies) return;
 	const int b = S.large_bodies[q];
 	S.label[b] = b;
@@ -1000,6 +1131,82 @@ __global__ void k_island_count(const DevWorld W, const SolveBuffers S)
 	}
 }
 
+// The grid set's least and greatest x per large world.  Lanes of one world
+// (nearly always the whole wave) combine their keys before the atomics.
+__global__ void k_part_bounds(const DevWorld W, const SolveBuffers S)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	int l = -1;
+	unsigned long long k = 0;
+	if (q < S.n_large_bodies) {
+		const int b = S.large_bodies[q];
+		if (in_grid_set(W, S, b)) {
+			l = S.large_index[W.b_world[b]];
+			k = x_key((double)W.b_p[b].x);
+		}
+	}
+	uint64_t todo = __ballot(l >= 0);
+	while (todo) {
+		const int leader = __ffsll((long long)todo) - 1;
+		const int r = __shfl(l, leader);
+		const bool mine = l == r;
+		unsigned long long mn = mine ? k : ~0ull, mx = mine ? k : 0ull;
+		for (int off = 32; off > 0; off >>= 1) {
+			const unsigned long long a = __shfl_xor(mn, off), b = __shfl_xor(mx, off);
+			mn = a < mn ? a : mn;
+			mx = b > mx ? b : mx;
+		}
+		if (__lane_id() == leader) {
+			atomicMin(&S.pbound[2 * r], mn);
+			atomicMax(&S.pbound[2 * r + 1], mx);
+		}
+		todo &= ~__ballot(mine);
+	}
+}
+
+// histogram of the grid set's x per large world
+__global__ void k_part_hist(const DevWorld W, const SolveBuffers S)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= S.n_large_bodies) return;
+	const int b = S.large_bodies[q];
+	if (!in_grid_set(W, S, b)) return;
+	const int l = S.large_index[W.b_world[b]];
+	const int bin = part_bin((double)W.b_p[b].x, key_x(S.pbound[2 * l]), key_x(S.pbound[2 * l + 1]));
+	atomicAdd(&S.phist[l * PART_BINS + bin], 1);
+}
+
+// one workgroup (PART_BINS threads) per large world: each bin's strip is
+// GRID_PARTS times the fraction of the set's bodies in the bins before it
+__global__ void __launch_bounds__(PART_BINS) k_part_scan(const SolveBuffers S)
+{
+	__shared__ int32_t sc[PART_BINS];
+	int32_t *h = S.phist + blockIdx.x * PART_BINS;
+	const int i = threadIdx.x;
+	const int own = h[i];
+	sc[i] = own;
+	__syncthreads();
+	for (int off = 1; off < PART_BINS; off <<= 1) {
+		const int add = i >= off ? sc[i - off] : 0;
+		__syncthreads();
+		sc[i] += add;
+		__syncthreads();
+	}
+	const long long total = sc[PART_BINS - 1], before = sc[i] - own;
+	h[i] = total > 0 ? (int)(before * GRID_PARTS / total) : 0;
+}
+
+__global__ void k_part_assign(const DevWorld W, const SolveBuffers S)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= S.n_large_bodies) return;
+	const int b = S.large_bodies[q];
+	if (!in_grid_set(W, S, b)) return;
+	const int l = S.large_index[W.b_world[b]];
+	const int bin = part_bin((double)W.b_p[b].x, key_x(S.pbound[2 * l]), key_x(S.pbound[2 * l + 1]));
+	S.bpart[b] = S.phist[l * PART_BINS + bin];
+}
+
 // Sort keys of the solved arbiters; the pre-steps of the large worlds'
 // arbiters (they only read velocities, before the joints' pre-steps and the
 // velocity integration, as in cpSpaceStep); OR / AND of the keys for the
@@ -1017,7 +1224,16 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 			key = (unsigned long long)w;
 		} else {
 			const int r = island_of(W, S, arb.ba, arb.bb);
-			key = island_on_workgroup(S, r) ? (ISLAND_KEY | (unsigned long long)r) : GRID_KEY;
+			if (island_on_workgroup(S, r)) {
+				key = ISLAND_KEY | (unsigned long long)r;
+			} else {
+				// the strip of its dynamic bodies, or the boundary when they lie in two strips
+				const int pa = is_dynamic(W, arb.ba) ? S.bpart[arb.ba] : -1;
+				const int pb = is_dynamic(W, arb.bb) ? S.bpart[arb.bb] : -1;
+				key = (pa >= 0 && pb >= 0 && pa != pb)
+				          ? BOUNDARY_KEY
+				          : GRID_KEY | ((unsigned long long)S.large_index[w] << 8) | (unsigned long long)max(pa, pb);
+			}
 			arb_prestep(W, arb);
 		}
 		S.keys[k] = key;
@@ -1051,13 +1267,16 @@ __global__ void k_island_integrate(const DevWorld W, const SolveBuffers S)
 }
 
 // warb_off[w] = first sorted position of small world w (n_worlds + 1 entries);
-// large_first = first position of the grid set
+// large_first = first position of the grid set, of the boundary
 __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 {
 	const int n = *S.n_act;
 	const int w = blockIdx.x * blockDim.x + threadIdx.x;
 	if (w <= n_worlds) S.warb_off[w] = lower_bound_u64(S.keys, 0, n, (uint64_t)w);
-	if (w == n_worlds + 1) S.large_first[0] = lower_bound_u64(S.keys, 0, n, GRID_KEY);
+	if (w == n_worlds + 1) {
+		S.large_first[0] = lower_bound_u64(S.keys, 0, n, GRID_KEY);
+		S.large_first[1] = lower_bound_u64(S.keys, 0, n, BOUNDARY_KEY);
+	}
 }
 
 // the first sorted position of every workgroup-solved island
@@ -1095,6 +1314,13 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 			if (nb > 0) k_uf_flatten<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 		});
 		TIMED(timer, "island_count", stream, k_island_count<<<edge_blocks, 256, 0, stream>>>(W, S));
+		if (nb > 0)
+			TIMED(timer, "strips", stream, {
+				k_part_bounds<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
+				k_part_hist<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
+				k_part_scan<<<S.n_parts / GRID_PARTS, PART_BINS, 0, stream>>>(S);
+				k_part_assign<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
+			});
 		HIP_TRY(hipGetLastError());
 	}
 	if (n_act > 0) {
@@ -1105,7 +1331,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
 		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
-		// keys: a small world's index, or GRID_KEY, or ISLAND_KEY | island root (below bit 42)
+		// keys: a small world's index, ISLAND_KEY | island root, a strip's or the boundary key (below bit 42)
 		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, 42, sortbuf, stream, timer));
 	}
 	if (large && nb > 0) {

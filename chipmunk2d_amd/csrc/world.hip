@@ -26,6 +26,9 @@
 #ifndef CPGPU_SOLVE_TAIL_MAX
 #define CPGPU_SOLVE_TAIL_MAX 256
 #endif
+#ifndef CPGPU_BOUNDARY_TAIL_MAX
+#define CPGPU_BOUNDARY_TAIL_MAX 4096
+#endif
 #include "cpgpu_adjoint.h"
 
 namespace CPGPU_NS {
@@ -314,6 +317,11 @@ struct cpgpuWorld {
 	// colouring finishes on one workgroup once at most colour_tail_max arbiters are left; runs of colours
 	// of at most solve_tail_max arbiters are swept by one workgroup between two grid barriers
 	int colour_tail_max = CPGPU_COLOUR_TAIL_MAX, solve_tail_max = CPGPU_SOLVE_TAIL_MAX;
+	int boundary_tail_max = CPGPU_BOUNDARY_TAIL_MAX;
+	// strips of the grid set
+	int32_t *d_large_index = nullptr, *phist = nullptr, *bpart = nullptr, *pcol = nullptr;
+	unsigned long long *pbound = nullptr;
+	int2 *pinfo = nullptr;
 	int32_t *tail_list = nullptr, *tail_n = nullptr;
 	int n_jcolours = 0;
 	std::vector<int32_t> h_jcolour;
@@ -696,6 +704,15 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		}
 		HIP_OR_FAIL(build_buckets(w));
 		HIP_OR_FAIL(own_upload(w, w->d_large_bodies, large_bodies.data(), large_bodies.size()));
+		std::vector<int32_t> large_index(w->nw, -1);
+		for (int k = 0, l = 0; k < w->nw; k++)
+			if (!small[k]) large_index[k] = l++;
+		HIP_OR_FAIL(own_upload(w, w->d_large_index, large_index.data(), large_index.size()));
+		const int n_large = std::max(w->n_large_worlds, 1);
+		HIP_OR_FAIL(own_alloc(w, w->pbound, 2 * (size_t)n_large));
+		HIP_OR_FAIL(own_alloc(w, w->phist, (size_t)PART_BINS * n_large));
+		HIP_OR_FAIL(own_alloc(w, w->pcol, (size_t)GRID_PARTS * n_large * (CPGPU_MAX_COLOURS + 1)));
+		HIP_OR_FAIL(own_alloc(w, w->pinfo, (size_t)GRID_PARTS * n_large));
 
 		// large worlds' joints: by colour
 		int ncol_large = 0;
@@ -762,7 +779,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_upload(w, w->warb_off, (const int32_t *)nullptr, w->nw + 2));
 	HIP_OR_FAIL(own_alloc(w, w->tail_list, w->colour_tail_max));
 	HIP_OR_FAIL(own_upload(w, w->tail_n, (const int32_t *)nullptr, 4));
-	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 1));
+	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 2));
+	HIP_OR_FAIL(own_alloc(w, w->bpart, nb));
 	HIP_OR_FAIL(own_alloc(w, w->claim, 2 * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->taken, nb));
 	HIP_OR_FAIL(own_alloc(w, w->label, nb));
@@ -859,6 +877,9 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;
 	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
+	S.large_index = w->d_large_index; S.n_parts = GRID_PARTS * w->n_large_worlds;
+	S.pbound = w->pbound; S.phist = w->phist; S.bpart = w->bpart; S.pcol = w->pcol; S.pinfo = w->pinfo;
+	S.boundary_tail_max = w->boundary_tail_max;
 	S.rec = nullptr; S.rec_stride = 0; S.jrec = nullptr;
 	return S;
 }

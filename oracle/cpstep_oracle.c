@@ -196,6 +196,7 @@ typedef struct {
 	OContact con[2];
 	uint32_t id, stamp;
 	int state, colour;
+	int unit;   /* solve unit of a large space (grid_units) */
 } OArb;
 
 typedef struct {
@@ -1175,6 +1176,7 @@ static OArb *g_sort_arbs;
 static int cmp_solve(const void *x, const void *y)
 {
 	const OArb *a = &g_sort_arbs[*(const int64_t *)x], *b = &g_sort_arbs[*(const int64_t *)y];
+	if (a->unit != b->unit) return a->unit < b->unit ? -1 : 1;
 	if (a->colour != b->colour) return a->colour < b->colour ? -1 : 1;
 	return a->key < b->key ? -1 : (a->key > b->key ? 1 : 0);
 }
@@ -1185,6 +1187,129 @@ static int cmp_jsolve(const void *x, const void *y)
 	const OJoint *a = &g_sort_joints[i], *b = &g_sort_joints[j];
 	if (a->colour != b->colour) return a->colour < b->colour ? -1 : 1;
 	return i < j ? -1 : (i > j ? 1 : 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* the device's solve units of a large space (no reference counterpart).
+ * A space with more than SMALL_WORLD_MAX_BODIES non-static bodies is split
+ * into islands: connected components of its dynamic bodies under the
+ * step's arbiters and its joints.  Islands with more than ISLAND_MAX
+ * arbiters or with a joint form the grid set.  The grid set's dynamic
+ * bodies are cut into GRID_PARTS strips along x holding equal shares of
+ * them: a PART_BINS-bin histogram of x between the set's least and greatest
+ * x, each bin going to strip floor(GRID_PARTS * (bodies in earlier bins) /
+ * (bodies)).  An arbiter of the grid set is interior to the strip of its
+ * dynamic bodies, or on the boundary when they lie in two strips.
+ * Each unit is coloured on its own and the space is solved unit by unit:
+ * the other islands (unit 0, together: they share no body), the strips in
+ * order (1 + q), the boundary (1 + GRID_PARTS).                          */
+
+#define SMALL_WORLD_MAX_BODIES 4096
+#define ISLAND_MAX 2048
+#define GRID_PARTS 64
+#define PART_BINS 1024
+
+static int uf_root(int32_t *parent, int x)
+{
+	while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; }
+	return x;
+}
+
+/* a double's bits as an unsigned key of the same order (x + 0.0 turns -0.0 into +0.0) */
+static uint64_t x_key(double x)
+{
+	double y = x + 0.0;
+	uint64_t u;
+	memcpy(&u, &y, sizeof u);
+	return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+static double key_x(uint64_t k)
+{
+	uint64_t u = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+	double x;
+	memcpy(&x, &u, sizeof x);
+	return x;
+}
+static int part_bin(double x, double lo, double hi)
+{
+	if (!(hi > lo)) return 0;
+	double t = (x - lo) * ((double)PART_BINS / (hi - lo));
+	if (!(t >= 0.0)) return 0;
+	return t >= (double)PART_BINS ? PART_BINS - 1 : (int)t;
+}
+
+/* unit of every active arbiter (OArb.unit); returns 0 for a small space (all units 0) */
+static int grid_units(oworld *w, OWorldPart *P)
+{
+	int n_ns = 0;
+	for (int b = P->b0; b < P->b1; b++) n_ns += w->B[b].type != CPGPU_BODY_STATIC;
+	for (int64_t k = 0; k < P->nactive; k++) P->arbs[P->active[k]].unit = 0;
+	if (n_ns <= SMALL_WORLD_MAX_BODIES) return 0;
+	const int nbw = P->b1 - P->b0;
+	int32_t *parent = (int32_t *)malloc((size_t)nbw * sizeof(int32_t));
+	int32_t *icount = (int32_t *)calloc((size_t)nbw, sizeof(int32_t));
+	int32_t *ijoint = (int32_t *)calloc((size_t)nbw, sizeof(int32_t));
+	int32_t *part = (int32_t *)malloc((size_t)nbw * sizeof(int32_t));
+	int32_t hist[PART_BINS];
+#define DYN(b) (w->B[b].type == CPGPU_BODY_DYNAMIC)
+	for (int i = 0; i < nbw; i++) parent[i] = i;
+	for (int64_t k = 0; k < P->nactive; k++) {
+		const OArb *arb = &P->arbs[P->active[k]];
+		if (DYN(arb->ba) && DYN(arb->bb)) {
+			int ra = uf_root(parent, arb->ba - P->b0), rb = uf_root(parent, arb->bb - P->b0);
+			if (ra != rb) parent[ra > rb ? ra : rb] = ra < rb ? ra : rb;
+		}
+	}
+	for (int c = P->c0; c < P->c1; c++) {
+		const OJoint *j = &w->J[c];
+		if (DYN(j->a) && DYN(j->b)) {
+			int ra = uf_root(parent, j->a - P->b0), rb = uf_root(parent, j->b - P->b0);
+			if (ra != rb) parent[ra > rb ? ra : rb] = ra < rb ? ra : rb;
+		}
+	}
+	for (int64_t k = 0; k < P->nactive; k++) {
+		const OArb *arb = &P->arbs[P->active[k]];
+		icount[uf_root(parent, (DYN(arb->ba) ? arb->ba : arb->bb) - P->b0)]++;
+	}
+	for (int c = P->c0; c < P->c1; c++) {
+		const OJoint *j = &w->J[c];
+		if (DYN(j->a) || DYN(j->b)) ijoint[uf_root(parent, (DYN(j->a) ? j->a : j->b) - P->b0)] = 1;
+	}
+	/* the grid set's bodies and their strips */
+	uint64_t lo_k = UINT64_MAX, hi_k = 0;
+	int n_grid = 0;
+	for (int i = 0; i < nbw; i++) {
+		part[i] = -1;
+		if (!DYN(P->b0 + i)) continue;
+		int r = uf_root(parent, i);
+		if (icount[r] <= ISLAND_MAX && !ijoint[r]) continue;
+		part[i] = 0;
+		uint64_t key = x_key(w->B[P->b0 + i].p.x);
+		if (key < lo_k) lo_k = key;
+		if (key > hi_k) hi_k = key;
+		n_grid++;
+	}
+	double lo = key_x(lo_k), hi = key_x(hi_k);
+	memset(hist, 0, sizeof hist);
+	for (int i = 0; i < nbw; i++)
+		if (part[i] >= 0) hist[part_bin(w->B[P->b0 + i].p.x, lo, hi)]++;
+	for (int q = 0, before = 0; q < PART_BINS; q++) {
+		int cnt = hist[q];
+		hist[q] = n_grid > 0 ? (int)((int64_t)before * GRID_PARTS / n_grid) : 0;
+		before += cnt;
+	}
+	for (int i = 0; i < nbw; i++)
+		if (part[i] >= 0) part[i] = hist[part_bin(w->B[P->b0 + i].p.x, lo, hi)];
+	for (int64_t k = 0; k < P->nactive; k++) {
+		OArb *arb = &P->arbs[P->active[k]];
+		int r = uf_root(parent, (DYN(arb->ba) ? arb->ba : arb->bb) - P->b0);
+		if (icount[r] <= ISLAND_MAX && !ijoint[r]) continue;
+		int pa = DYN(arb->ba) ? part[arb->ba - P->b0] : -1, pb = DYN(arb->bb) ? part[arb->bb - P->b0] : -1;
+		arb->unit = (pa >= 0 && pb >= 0 && pa != pb) ? 1 + GRID_PARTS : 1 + (pa > pb ? pa : pb);
+	}
+#undef DYN
+	free(parent); free(icount); free(ijoint); free(part);
+	return 1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1587,8 +1712,26 @@ static void step_world(oworld *w, OWorldPart *P, double dt, const int32_t *order
 			bb[k] = w->B[arb->bb].type == CPGPU_BODY_DYNAMIC ? arb->bb : -1;
 			pr[k] = arb->prio;
 		}
-		P->ncol_arb = colour_elements(w, n, ba, bb, pr, col);
-		for (int64_t k = 0; k < n; k++) P->arbs[P->active[k]].colour = col[k];
+		if (!grid_units(w, P)) {
+			P->ncol_arb = colour_elements(w, n, ba, bb, pr, col);
+			for (int64_t k = 0; k < n; k++) P->arbs[P->active[k]].colour = col[k];
+		} else {
+			/* each unit coloured on its own */
+			int64_t *idx = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
+			int32_t *uba = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+			int32_t *ubb = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+			uint64_t *upr = (uint64_t *)malloc((size_t)(n + 1) * sizeof(uint64_t));
+			P->ncol_arb = 0;
+			for (int u = 0; u <= 1 + GRID_PARTS; u++) {
+				int64_t m = 0;
+				for (int64_t k = 0; k < n; k++)
+					if (P->arbs[P->active[k]].unit == u) { idx[m] = k; uba[m] = ba[k]; ubb[m] = bb[k]; upr[m] = pr[k]; m++; }
+				int nc = colour_elements(w, m, uba, ubb, upr, col);
+				if (nc > P->ncol_arb) P->ncol_arb = nc;
+				for (int64_t q = 0; q < m; q++) P->arbs[P->active[idx[q]]].colour = col[q];
+			}
+			free(idx); free(uba); free(ubb); free(upr);
+		}
 		g_sort_arbs = P->arbs;
 		qsort(P->active, (size_t)n, sizeof(int64_t), cmp_solve);
 		free(ba); free(bb); free(pr); free(col);
@@ -1711,6 +1854,18 @@ int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colou
 				if (hash) hash[2 * n + i] = valid ? c->hash : 0;
 			}
 		}
+	}
+	return n;
+}
+
+/* solve unit of every arbiter of the last step, in ow_read_arbiters' order */
+int64_t ow_read_units(oworld *w, int64_t cap, int32_t *unit)
+{
+	int64_t n = 0;
+	for (int wi = 0; wi < w->nw; wi++) {
+		OWorldPart *P = &w->W[wi];
+		for (int64_t k = 0; k < P->nactive; k++, n++)
+			if (n < cap) unit[n] = P->arbs[P->active[k]].unit;
 	}
 	return n;
 }

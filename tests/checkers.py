@@ -44,6 +44,8 @@ def oracle_lib():
         lib.ow_read_arbiters.restype = ctypes.c_int64
         lib.ow_read_arbiters.argtypes = [_vp, ctypes.c_int64, _ip, _ip, _ip, _ip, _dp, _dp, _u64p]
         lib.ow_read_constraints.argtypes = [_vp, _dp, _ip]
+        lib.ow_read_units.restype = ctypes.c_int64
+        lib.ow_read_units.argtypes = [_vp, ctypes.c_int64, _ip]
         lib.ow_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
         lib.ow_cache_all_shapes.argtypes = [_vp]
         lib.ow_read_shape_bbs.argtypes = [_vp, _dp]
@@ -197,6 +199,13 @@ class OracleWorld:
                                   ptr(A.normal, ctypes.c_double), ptr(A.contact, ctypes.c_double),
                                   ptr(A.hash, ctypes.c_uint64))
         return A
+
+    def units(self):
+        """Solve unit of every arbiter of the last step (arbiters() order)."""
+        n = self.lib.ow_read_units(self.h, 0, None)
+        u = _arr(n, np.int32)
+        self.lib.ow_read_units(self.h, n, ptr(u, ctypes.c_int32))
+        return u
 
     def constraints(self):
         n = self.packed.n_constraints

@@ -100,21 +100,53 @@ def test_batch_of_worlds_bit_exact():
     run_lockstep(pw, 120, check_every=10, arbiters_every=40)
 
 
+def _strips_seen(o):
+    """Strips holding arbiters and boundary arbiters in the oracle's last step."""
+    u = o.units()
+    return len(np.unique(u[(u > 0) & (u <= 64)])), int(np.count_nonzero(u == 65))
+
+
 def test_grid_solve_mid_size_world_bit_exact():
-    """More dynamic bodies than one workgroup takes: the co-resident grid solve."""
+    """More dynamic bodies than one workgroup takes, in small islands: each
+    island on a workgroup beside the (empty) grid solve."""
     pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=260.0, seed=12).pack()
     assert pw.dynamic_body_count() > 4096
     g, o = run_lockstep(pw, 60, check_every=2, arbiters_every=30)
     assert g.stats()["arbiters"] > 300
 
 
+def test_grid_strips_dense_world_bit_exact():
+    """A dense large space: one island of ~10k arbiters, solved on the grid in
+    64 strips and a boundary."""
+    pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
+    g, o = run_lockstep(pw, 60, check_every=1, arbiters_every=20)
+    strips, boundary = _strips_seen(o)
+    assert strips == 64 and boundary > 1000, (strips, boundary)
+
+
+def test_grid_strips_chain_with_joints_bit_exact():
+    """A large space whose chain island (pivots, springs, slides, gears) is on
+    the grid: strips of the links and the circles on them, then the joints."""
+    pw = chain_world(n_links=300, n_circles=3900, seed=9).pack()
+    g, o = run_lockstep(pw, 100, check_every=2, arbiters_every=50)
+    strips, boundary = _strips_seen(o)
+    assert strips >= 16 and boundary > 0, (strips, boundary)
+    jg, cg = g.constraints()
+    jo, co = o.constraints()
+    assert np.array_equal(cg, co)
+    assert np.array_equal(jg, jo)
+
+
 def test_mixed_batch_grid_and_workgroup_solves_bit_exact():
-    """One large space (grid solve) and small spaces (a workgroup each) in one step."""
+    """Two large spaces (grid sets in strips) and small spaces (a workgroup each) in one step."""
     rng = Rng(0x1A46E)
-    big = random_world(n_circles=2400, n_polys=2400, n_segments=300, extent=250.0, seed=13).pack()
-    spaces = [pyramid_210(rows=5).pack(), big, random_pile(120, rng).pack(), pyramid_210(rows=8).pack()]
+    big = random_world(n_circles=2400, n_polys=2400, n_segments=300, extent=130.0, seed=13).pack()
+    big2 = random_world(n_circles=2300, n_polys=2300, n_segments=200, extent=125.0, seed=14).pack()
+    spaces = [pyramid_210(rows=5).pack(), big, random_pile(120, rng).pack(), big2, pyramid_210(rows=8).pack()]
     pw = concat(spaces)
-    run_lockstep(pw, 60, check_every=5, arbiters_every=30)
+    g, o = run_lockstep(pw, 40, check_every=5, arbiters_every=20)
+    strips, boundary = _strips_seen(o)
+    assert strips == 64 and boundary > 1000, (strips, boundary)
 
 
 def test_worlds_in_a_batch_step_as_when_alone():

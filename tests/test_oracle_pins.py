@@ -498,6 +498,92 @@ def test_colouring_is_proper_and_matches_restatement():
     assert _python_colouring(ba, bb, prio, pw.n_bodies) == list(A.colour)
 
 
+def _strip_units(pw, p, shapes, small_max=4096, island_max=2048, parts=64, bins=1024):
+    """Independent restatement of a large space's solve units (one space):
+    islands by scipy's connected components, strips by numpy quantiles."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    bt = pw.a["body_type"]
+    sb = pw.a["shape_body"]
+    nb = pw.n_bodies
+    if np.count_nonzero(bt != 2) <= small_max:
+        return np.zeros(len(shapes), np.int32)
+    a, b = sb[shapes[:, 0]].astype(np.int64), sb[shapes[:, 1]].astype(np.int64)
+    ja, jb = pw.a["cons_a"].astype(np.int64), pw.a["cons_b"].astype(np.int64)
+    dyn = bt == BODY_DYNAMIC
+    ea = np.concatenate([a[dyn[a] & dyn[b]], ja[dyn[ja] & dyn[jb]]])
+    eb = np.concatenate([b[dyn[a] & dyn[b]], jb[dyn[ja] & dyn[jb]]])
+    g = coo_matrix((np.ones(len(ea)), (ea, eb)), shape=(nb, nb))
+    _, comp = connected_components(g, directed=False)
+    arb_comp = comp[np.where(dyn[a], a, b)]
+    icount = np.bincount(arb_comp, minlength=nb)
+    ijoint = np.zeros(nb, bool)
+    jdyn = dyn[ja] | dyn[jb]
+    ijoint[comp[np.where(dyn[ja], ja, jb)[jdyn]]] = True
+    grid_comp = (icount > island_max) | ijoint
+    grid_body = dyn & grid_comp[comp]
+    x = p[grid_body, 0] + 0.0
+    lo, hi = x.min(), x.max()
+    if hi > lo:
+        t = (x - lo) * (float(bins) / (hi - lo))
+        binv = np.minimum(np.floor(t), bins - 1).astype(np.int64)
+    else:
+        binv = np.zeros(len(x), np.int64)
+    hist = np.bincount(binv, minlength=bins)
+    before = np.concatenate([[0], np.cumsum(hist)[:-1]])
+    part = np.full(nb, -1, np.int64)
+    part[grid_body] = (before * parts // len(x))[binv]
+    unit = np.zeros(len(shapes), np.int32)
+    on_grid = grid_comp[arb_comp]
+    pa = np.where(dyn[a], part[a], -1)
+    pb = np.where(dyn[b], part[b], -1)
+    cross = (pa >= 0) & (pb >= 0) & (pa != pb)
+    unit[on_grid] = np.where(cross, 1 + parts, 1 + np.maximum(pa, pb))[on_grid]
+    return unit
+
+
+@pytest.mark.parametrize("scene", ["dense", "chain"])
+def test_large_space_strips_match_restatement(scene):
+    """A large space's solve units (islands, strips along x, the boundary)
+    against an independent restatement; each unit coloured on its own (the
+    Python claim rounds) and solved unit by unit, by colour inside a unit."""
+    from chipmunk2d_amd.scenes import chain_world
+    if scene == "dense":
+        pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
+        steps = 6
+    else:
+        # a 300-link chain (pivots, springs, slides, gears) under 3900 circles: its island is
+        # on the grid for its joints; circles landing on it put arbiters into the strips
+        pw = chain_world(n_links=300, n_circles=3900, seed=9).pack()
+        steps = 90
+    o = OracleWorld(pw)
+    for _ in range(steps):
+        o.step(1 / 60)
+    A = o.arbiters()
+    units = o.units()
+    want = _strip_units(pw, o.bodies().p, A.shapes)
+    assert np.array_equal(units, want)
+    strips = np.unique(units[(units > 0) & (units <= 64)])
+    assert len(strips) >= (32 if scene == "dense" else 16), len(strips)
+    assert np.count_nonzero(units == 65) > 0
+    # solve order: by unit, then by colour
+    order = units.astype(np.int64) * 4096 + A.colour
+    assert np.all(np.diff(order) >= 0)
+    btype = pw.a["body_type"]
+    sb = pw.a["shape_body"]
+    ba = [int(sb[x]) if btype[sb[x]] == BODY_DYNAMIC else -1 for x in A.shapes[:, 0]]
+    bb = [int(sb[x]) if btype[sb[x]] == BODY_DYNAMIC else -1 for x in A.shapes[:, 1]]
+    keys = [(min(int(x), int(y)) << 32) | max(int(x), int(y)) for x, y in A.shapes]
+    prio = [_mix64(k) for k in keys]
+    n_col = 0
+    for u in np.unique(units):
+        idx = np.nonzero(units == u)[0]
+        col = _python_colouring([ba[k] for k in idx], [bb[k] for k in idx], [prio[k] for k in idx], pw.n_bodies)
+        assert col == list(A.colour[idx]), u
+        n_col = max(n_col, max(col) + 1)
+    assert o.stats()["arbiter_colours"] == n_col
+
+
 # ---------------------------------------------------------------------------
 # joints: one step at a time against the reference, constraints in its list order
 

@@ -56,20 +56,29 @@ struct GridBarrier {
 // The grid set of each large world is cut into GRID_PARTS strips along x
 // with equal numbers of its dynamic bodies (quantiles of a PART_BINS-bin
 // histogram of their x between the set's least and greatest x).  An arbiter
-// whose dynamic bodies lie in one strip is interior to it; the others cross
-// strips (the boundary).  Strips share no dynamic body, so each workgroup
-// colours and sweeps its strips with workgroup barriers only; the boundary
-// is coloured on its own and swept after every strip, with grid barriers.
-// The solve order is therefore: each strip in its colour order, then the
-// boundary in its colour order.
+// whose dynamic bodies lie in one strip belongs to that strip; one joining
+// strips q and q + 1 to seam q; one joining strips further apart to the wide
+// set.  Strips share no dynamic body, and neither do even seams (0, 2, ...)
+// nor odd ones, so the solve units of the grid set are swept in four
+// phases: all strips, all even seams, all odd seams, the wide set.  Each
+// unit is coloured on its own and swept by one workgroup with workgroup
+// barriers; grid barriers separate the phases.  The solve order is
+// therefore: the strips, the even seams, the odd seams (each in its colour
+// order), then the wide set in its colour order.
 // Sorted arbiter keys: small world w -> w; large-world island with root
 // body r -> 1 << 40 | r; strip q of the l-th large world -> 2 << 40 | l << 8 | q;
-// boundary -> BOUNDARY_KEY, the greatest key.
+// its seam q -> 2 << 40 | l << 8 | (GRID_PARTS + q); wide set -> WIDE_KEY, the greatest key.
 #define ISLAND_KEY (1ull << 40)
 #define GRID_KEY (2ull << 40)
-#define BOUNDARY_KEY (GRID_KEY | ((1ull << 40) - 1))
+#define WIDE_KEY (GRID_KEY | ((1ull << 40) - 1))
 #define GRID_PARTS 64
+#define GRID_SEAMS (GRID_PARTS - 1)
+#define EVEN_SEAMS ((GRID_SEAMS + 1) / 2)
+#define ODD_SEAMS (GRID_SEAMS / 2)
 #define PART_BINS 1024
+// Claim arrays for concurrent colourings: strips and workgroup-solved units,
+// even seams, odd seams, the wide set.
+#define CLAIM_SETS 4
 
 // Pre-state of one arbiter's impulse application in one sweep (rollout
 // tape): both bodies' velocities and bias velocities, and each contact's
@@ -99,7 +108,7 @@ struct SolveBuffers {
 	int32_t *arb_by_world;     // [cap] active arbiters sorted by key; nullptr for a single small space
 	uint64_t *keys;            // [cap] the sorted keys
 	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world (the last: islands)
-	int32_t *large_first;      // [2] first sorted position of the grid set, of the boundary
+	int32_t *large_first;      // [2] first sorted position of the grid set, of the wide set
 	// islands of the large worlds
 	int32_t *label;            // [n_bodies] union-find parent, then island root (the least body index)
 	int32_t *icount;           // [n_bodies] per root: arbiters of the island
@@ -109,12 +118,8 @@ struct SolveBuffers {
 	int island_max;
 	int n_large_joints;        // joints of large worlds (jorder[0..n))
 	const uint8_t *world_small;
-	int32_t *col_off;          // [CPGPU_MAX_COLOURS + 1] grid solve: counts, then offsets
-	int32_t *col_cursor;       // [CPGPU_MAX_COLOURS]
-	int32_t *n_colours;        // [1] grid solve colour count
-	int32_t *left;             // [1] uncoloured arbiters (grid solve)
-	unsigned long long *claim; // [2][n_bodies]
-	int32_t *taken;            // [n_bodies]
+	unsigned long long *claim; // [CLAIM_SETS][2][n_bodies]
+	int32_t *taken;            // [CLAIM_SETS][n_bodies]
 	// joints, coloured once at creation: large worlds together ...
 	const int32_t *jorder;     // joints of large worlds in colour order
 	const int32_t *jcol_off;   // [n_jcolours + 1]
@@ -125,26 +130,18 @@ struct SolveBuffers {
 	const int32_t *jw_cbase;   // [n_worlds] first entry of the world's offsets in jw_col
 	const int32_t *jw_ncol;    // [n_worlds]
 	GridBarrier *bar;
-	// grid solve: colours (and the last uncoloured arbiters) with few arbiters
-	// are handled by workgroup 0 alone instead of paying a grid barrier each
-	int32_t *tail_list;        // [colour_tail_max]
-	int32_t *tail_n;           // [1]
-	int32_t *tail_state;       // [2] colour and sub-round after the tail colouring
-	int colour_tail_max;
-	int solve_tail_max;
 	int max_iterations;        // over the large worlds
 	int n_bodies;
 	const int32_t *large_bodies;   // non-static bodies of large worlds
 	int n_large_bodies;
-	// strips of the grid set (n_parts = GRID_PARTS per large world)
+	// solve units of the grid set
 	const int32_t *large_index;    // [n_worlds] ordinal among the large worlds, -1 for a small world
-	int n_parts;
+	int n_large_worlds;
 	unsigned long long *pbound;    // [2 per large world] least and greatest x of the grid set (ordered keys)
 	int32_t *phist;                // [PART_BINS per large world] bin counts, then each bin's strip
 	int32_t *bpart;                // [n_bodies] strip of each grid-set body
-	int32_t *pcol;                 // [n_parts][CPGPU_MAX_COLOURS + 1] colour offsets of each strip
-	int2 *pinfo;                   // [n_parts] first sorted position and colour count of each strip
-	int boundary_tail_max;         // boundary colours of at most this many arbiters: workgroup 0 alone
+	int32_t *pcol;                 // [grid_units][CPGPU_MAX_COLOURS + 1] colour offsets of each unit
+	int2 *pinfo;                   // [grid_units] first sorted position and colour count of each unit
 	// recording (adjoint replay): rec[it * rec_stride + position in order], or nullptr
 	ArbRec *rec;
 	int rec_stride;
@@ -160,6 +157,13 @@ struct SolveBuffers {
 // from above (launch sizes); the kernels read the count *S.n_act.
 hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
                         RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer);
+// solve units of the grid set: the strips, even seams and odd seams of every
+// large world, then the wide set
+__host__ __device__ inline int grid_units(int n_large_worlds)
+{
+	return n_large_worlds * (GRID_PARTS + GRID_SEAMS) + 1;
+}
+
 // Small worlds are launched in size buckets, each with its own workgroup
 // size and room for the largest of its worlds' bodies in LDS (0: the bodies
 // stay in global memory).  Worlds are listed largest first.

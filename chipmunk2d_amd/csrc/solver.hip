@@ -1,16 +1,18 @@
-// solver.hip -- the velocity solver of cpSpaceStep as one persistent kernel.
+// solver.hip -- the velocity solver of cpSpaceStep: persistent kernels.
 //
 // Reference: the second half of cpSpaceStep (src/cpSpaceStep.c:380-427):
 // arbiter pre-step, constraint pre-step, velocity integration, cached
 // impulses, then `iterations` sweeps of cpArbiterApplyImpulse and the joints'
 // applyImpulse.  The reference sweeps its arbiter list sequentially
-// (Gauss-Seidel).  Here the arbiters of one step are coloured so that no two
-// arbiters of a colour share a dynamic body; a sweep runs colour after colour
-// with a grid barrier between colours, so the result is exactly the
-// sequential sweep in colour order (arbiters of one colour touch disjoint
-// bodies, so their order inside the colour does not matter).  Static and
-// kinematic bodies have zero inverse mass, are never written, and do not
-// constrain the colouring.
+// (Gauss-Seidel).  Here the arbiters of one solve unit (a small world, an
+// island, a strip, seam or the wide set of a large world's grid set) are
+// coloured so that no two arbiters of a colour share a dynamic body; a sweep
+// runs colour after colour with a barrier between colours, so the result is
+// exactly the sequential sweep in colour order (arbiters of one colour touch
+// disjoint bodies, so their order inside the colour does not matter).
+// Units that run concurrently share no dynamic body.  Static and kinematic
+// bodies have zero inverse mass, are never written, and do not constrain
+// the colouring.
 //
 // The colouring is deterministic (independent of thread timing): colour c is
 // built in COLOUR_SUBROUNDS rounds in which every uncoloured arbiter whose
@@ -404,19 +406,17 @@ __device__ __forceinline__ unsigned long long arbiter_priority(const DevWorld &W
 
 // Synchronisation of the threads taking part: one workgroup, or the whole
 // co-resident grid.
-// sweep_tid() numbers the threads for the impulse sweeps.  On the grid it
+// sweep_tid() numbers the threads for the joint sweeps.  On the grid it
 // interleaves the workgroups (thread t of workgroup g is g + t * gridDim.x),
-// so the arbiters of a colour, however few, are spread over every workgroup
+// so the joints of a colour, however few, are spread over every workgroup
 // instead of filling the first ones: a sweep is bound by each CU's scattered
-// arbiter and body loads.
+// joint and body loads.
 struct BlockSync {
-	static constexpr bool grid = false;
 	int tid, stride;
 	__device__ void sync() { __syncthreads(); }
 	__device__ int sweep_tid() const { return tid; }
 };
 struct GridSync {
-	static constexpr bool grid = true;
 	int tid, stride;
 	GridBarrier *bar;
 	int32_t *err;
@@ -427,7 +427,7 @@ struct GridSync {
 
 // The elements one solve covers.  Arbiters are list[0..n) (indices into
 // S.act; list == nullptr means 0..n-1).  col_off / col_cursor / left live in
-// LDS for a workgroup solve and in global memory for a grid solve.
+// LDS.
 struct SolveSet {
 	const int32_t *list;
 	int n;
@@ -435,7 +435,6 @@ struct SolveSet {
 	int32_t *col_off;          // [CPGPU_MAX_COLOURS + 1]
 	int32_t *col_cursor;       // [CPGPU_MAX_COLOURS]
 	int32_t *left;
-	int32_t *n_colours_out;    // may be nullptr
 	const int32_t *bodies;     // bodies to integrate: list, or nullptr for the range [body0, body0 + nb)
 	int body0, nb;
 	const int32_t *jorder;     // joints in colour order
@@ -448,6 +447,8 @@ struct SolveSet {
 	int lds_cap;
 	int unit;                  // solve unit whose contacts / bodies are counted (W.totals[2 + 2 unit ...])
 	int order_base;            // position of order[0] in S.order (records are indexed by it)
+	int cset;                  // claim arrays used by the colouring (units coloured concurrently
+	                           // with shared bodies use different ones)
 };
 
 // a wave-reduced 64-bit add to a counter
@@ -463,7 +464,7 @@ __device__ __forceinline__ void wave_add(unsigned long long *counter, unsigned l
 // into the position-indexed arrays cbody / cprio / ccol, so a round reads
 // them coalesced instead of scattered arbiter records.  Returns the number of
 // arbiters that took colour c.
-__device__ int colour_round(const SolveBuffers &S, const int2 *cbody, const unsigned long long *cprio, int32_t *ccol,
+__device__ int colour_round(int32_t *taken, const int2 *cbody, const unsigned long long *cprio, int32_t *ccol,
                             const int32_t *idx, int n, int tid, int stride, int c, unsigned long long *cl,
                             unsigned long long *other, bool claim_phase)
 {
@@ -475,7 +476,7 @@ __device__ int colour_round(const SolveBuffers &S, const int2 *cbody, const unsi
 		const int ba = bd.x, bb = bd.y;
 		const unsigned long long prio = cprio[k];
 		if (claim_phase) {
-			if ((ba >= 0 && S.taken[ba] == c) || (bb >= 0 && S.taken[bb] == c)) continue;
+			if ((ba >= 0 && taken[ba] == c) || (bb >= 0 && taken[bb] == c)) continue;
 			if (ba >= 0) atomicMin(&cl[ba], prio);
 			if (bb >= 0) atomicMin(&cl[bb], prio);
 		} else {
@@ -483,8 +484,8 @@ __device__ int colour_round(const SolveBuffers &S, const int2 *cbody, const unsi
 			                 (bb < 0 || __hip_atomic_load(&cl[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio);
 			if (win) {
 				ccol[k] = c;
-				if (ba >= 0) S.taken[ba] = c;
-				if (bb >= 0) S.taken[bb] = c;
+				if (ba >= 0) taken[ba] = c;
+				if (bb >= 0) taken[bb] = c;
 				won++;
 			} else {
 				if (ba >= 0) other[ba] = ~0ull;
@@ -496,32 +497,13 @@ __device__ int colour_round(const SolveBuffers &S, const int2 *cbody, const unsi
 }
 
 // Run f(k) for every position k of colours [0, n_colours) in colour order,
-// synchronising between colours.  On the grid, a run of colours with at most
-// tail_max arbiters each is handled by workgroup 0 alone, with workgroup
-// barriers between those colours and one grid barrier after the run: the
-// sequential order is unchanged, the grid barriers of small colours go away.
+// synchronising between colours.
 template <class Sync, class F>
-__device__ void for_colours(Sync &sy, const int32_t *col_off, int n_colours, int tail_max, F f)
+__device__ void for_colours(Sync &sy, const int32_t *col_off, int n_colours, F f)
 {
-	int q = 0;
-	while (q < n_colours) {
-		const int cnt = col_off[q + 1] - col_off[q];
-		if (!Sync::grid || cnt > tail_max) {
-			for (int k = col_off[q] + sy.sweep_tid(); k < col_off[q + 1]; k += sy.stride) f(k);
-			sy.sync();
-			q++;
-		} else {
-			int q1 = q + 1;
-			while (q1 < n_colours && col_off[q1 + 1] - col_off[q1] <= tail_max) q1++;
-			if (blockIdx.x == 0) {
-				for (int r = q; r < q1; r++) {
-					for (int k = col_off[r] + (int)threadIdx.x; k < col_off[r + 1]; k += (int)blockDim.x) f(k);
-					__syncthreads();
-				}
-			}
-			sy.sync();
-			q = q1;
-		}
+	for (int q = 0; q < n_colours; q++) {
+		for (int k = col_off[q] + sy.sweep_tid(); k < col_off[q + 1]; k += sy.stride) f(k);
+		sy.sync();
 	}
 }
 
@@ -555,8 +537,7 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
                                const BA &bodies)
 {
 	const int tid = sy.sweep_tid(), stride = sy.stride;
-	const int tail = S.solve_tail_max;
-	for_colours(sy, T.col_off, n_colours, tail, [&](int k) { arb_apply_cached(W, bodies, S.act[T.order[k]]); });
+	for_colours(sy, T.col_off, n_colours, [&](int k) { arb_apply_cached(W, bodies, S.act[T.order[k]]); });
 	for (int jc = 0; jc < T.njcol; jc++) {
 		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
 			if (S.jrec) record_joint(S.jrec + k, bodies, W.joints[T.jorder[k]]);
@@ -565,7 +546,7 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 		sy.sync();
 	}
 	for (int it = 0; it < T.iterations; it++) {
-		for_colours(sy, T.col_off, n_colours, tail, [&](int k) {
+		for_colours(sy, T.col_off, n_colours, [&](int k) {
 			Arbiter &arb = S.act[T.order[k]];
 			if (it < W.ws[arb.world].iterations) {
 				if (S.rec) record_arbiter(S.rec + (size_t)it * S.rec_stride + T.order_base + k, bodies, arb);
@@ -586,12 +567,12 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 }
 
 // Colour the set's arbiters and sort them into colour order (T.order,
-// T.col_off); returns the colour count.
-template <class Sync>
-__device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy)
+// T.col_off) on one workgroup; returns the colour count.
+__device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T)
 {
-	const int tid = sy.tid, stride = sy.stride, n = T.n;
-	unsigned long long *claim0 = S.claim, *claim1 = S.claim + S.n_bodies;
+	const int tid = threadIdx.x, stride = blockDim.x, n = T.n;
+	unsigned long long *claim0 = S.claim + (size_t)T.cset * 2 * S.n_bodies, *claim1 = claim0 + S.n_bodies;
+	int32_t *taken = S.taken + (size_t)T.cset * S.n_bodies;
 	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
 	int2 *cbody = S.cbody + T.order_base;
 	unsigned long long *cprio = S.cprio + T.order_base;
@@ -604,128 +585,56 @@ __device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		cbody[k] = make_int2(ba, bb);
 		cprio[k] = arbiter_priority(W, arb);
 		ccol[k] = -1;
-		if (ba >= 0) { claim0[ba] = ~0ull; claim1[ba] = ~0ull; S.taken[ba] = -1; }
-		if (bb >= 0) { claim0[bb] = ~0ull; claim1[bb] = ~0ull; S.taken[bb] = -1; }
+		if (ba >= 0) { claim0[ba] = ~0ull; claim1[ba] = ~0ull; taken[ba] = -1; }
+		if (bb >= 0) { claim0[bb] = ~0ull; claim1[bb] = ~0ull; taken[bb] = -1; }
 	}
 	for (int k = tid; k <= CPGPU_MAX_COLOURS; k += stride) T.col_off[k] = 0;
-	if (tid == 0) {
-		*T.left = n;
-		if (Sync::grid) *S.tail_n = 0;
-	}
-	sy.sync();
+	if (tid == 0) *T.left = n;
+	__syncthreads();
 
 	int c = 0, sub = 0;
 	while (true) {
-		const int left = __hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const int left = __hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 		if (left == 0) break;
 		if (c >= CPGPU_MAX_COLOURS) { if (tid == 0) raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); break; }
-		if (Sync::grid && left <= S.colour_tail_max) {
-			// the last few arbiters: gather them, then workgroup 0 finishes the colouring alone
-			for (int k = tid; k < n; k += stride)
-				if (ccol[k] < 0) S.tail_list[atomicAdd(S.tail_n, 1)] = k;
-			sy.sync();
-			if (blockIdx.x == 0) {
-				const int nt = __hip_atomic_load(S.tail_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-				while (__hip_atomic_load(T.left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0 && c < CPGPU_MAX_COLOURS) {
-					unsigned long long *cl = (sub & 1) ? claim1 : claim0;
-					unsigned long long *other = (sub & 1) ? claim0 : claim1;
-					colour_round(S, cbody, cprio, ccol, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other, true);
-					__syncthreads();
-					const int won = colour_round(S, cbody, cprio, ccol, S.tail_list, nt, threadIdx.x, blockDim.x, c, cl, other,
-					                             false);
-					if (won) atomicSub(T.left, won);
-					__syncthreads();
-					if (++sub % COLOUR_SUBROUNDS == 0) c++;
-				}
-				if (threadIdx.x == 0) { S.tail_state[0] = c; S.tail_state[1] = sub; }
-			}
-			sy.sync();
-			c = __hip_atomic_load(&S.tail_state[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			sub = __hip_atomic_load(&S.tail_state[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			break;
-		}
 		unsigned long long *cl = (sub & 1) ? claim1 : claim0;
 		unsigned long long *other = (sub & 1) ? claim0 : claim1;
-		colour_round(S, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, true);
-		sy.sync();
-		const int won = colour_round(S, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, false);
+		colour_round(taken, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, true);
+		__syncthreads();
+		const int won = colour_round(taken, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, false);
 		if (won) atomicSub(T.left, won);
-		sy.sync();
+		__syncthreads();
 		if (++sub % COLOUR_SUBROUNDS == 0) c++;
 	}
 	const int n_colours = (sub % COLOUR_SUBROUNDS == 0) ? c : c + 1;
 
 	// ---- colour sort ----
-	// (on the grid each workgroup counts its arbiters per colour in LDS and
-	// reserves its ranges with one atomic per colour; positions inside a
-	// colour do not matter, its arbiters share no dynamic body)
 	unsigned long long n_contacts = 0;
-	if constexpr (Sync::grid) {
-		__shared__ int32_t lcount[CPGPU_MAX_COLOURS], lbase[CPGPU_MAX_COLOURS];
-		for (int q = threadIdx.x; q < CPGPU_MAX_COLOURS; q += blockDim.x) lcount[q] = 0;
-		__syncthreads();
-		for (int k = tid; k < n; k += stride) {
-			const int a = arb_index(k);
-			const int col = ccol[k];
-			S.colour[a] = col;
-			if (col >= 0) atomicAdd(&lcount[col], 1);
-			n_contacts += S.act[a].count;
+	for (int k = tid; k < n; k += stride) {
+		const int a = arb_index(k);
+		const int col = ccol[k];
+		S.colour[a] = col;
+		if (col >= 0) atomicAdd(&T.col_off[col], 1);
+		n_contacts += S.act[a].count;
+	}
+	wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
+	__syncthreads();
+	if (tid == 0) {
+		int run = 0;
+		for (int q = 0; q < n_colours; q++) {
+			const int cnt = T.col_off[q];
+			T.col_off[q] = run;
+			T.col_cursor[q] = run;
+			run += cnt;
 		}
-		wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
-		__syncthreads();
-		for (int q = threadIdx.x; q < n_colours; q += blockDim.x)
-			if (lcount[q]) atomicAdd(&T.col_off[q], lcount[q]);
-		sy.sync();
-		if (tid == 0) {
-			int run = 0;
-			for (int q = 0; q < n_colours; q++) {
-				const int cnt = T.col_off[q];
-				T.col_off[q] = run;
-				T.col_cursor[q] = run;
-				run += cnt;
-			}
-			T.col_off[n_colours] = run;
-			if (T.n_colours_out) *T.n_colours_out = n_colours;
-		}
-		sy.sync();
-		for (int q = threadIdx.x; q < n_colours; q += blockDim.x) {
-			lbase[q] = lcount[q] ? atomicAdd(&T.col_cursor[q], lcount[q]) : 0;
-			lcount[q] = 0;
-		}
-		__syncthreads();
-		for (int k = tid; k < n; k += stride) {
-			const int col = ccol[k];
-			if (col < 0) continue;
-			T.order[lbase[col] + atomicAdd(&lcount[col], 1)] = arb_index(k);
-		}
-	} else {
-		for (int k = tid; k < n; k += stride) {
-			const int a = arb_index(k);
-			const int col = ccol[k];
-			S.colour[a] = col;
-			if (col >= 0) atomicAdd(&T.col_off[col], 1);
-			n_contacts += S.act[a].count;
-		}
-		wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
-		sy.sync();
-		if (tid == 0) {
-			int run = 0;
-			for (int q = 0; q < n_colours; q++) {
-				const int cnt = T.col_off[q];
-				T.col_off[q] = run;
-				T.col_cursor[q] = run;
-				run += cnt;
-			}
-			T.col_off[n_colours] = run;
-			if (T.n_colours_out) *T.n_colours_out = n_colours;
-		}
-		sy.sync();
-		for (int k = tid; k < n; k += stride) {
-			const int col = ccol[k];
-			if (col < 0) continue;
-			const int pos = atomicAdd(&T.col_cursor[col], 1);
-			T.order[pos] = arb_index(k);
-		}
+		T.col_off[n_colours] = run;
+	}
+	__syncthreads();
+	for (int k = tid; k < n; k += stride) {
+		const int col = ccol[k];
+		if (col < 0) continue;
+		const int pos = atomicAdd(&T.col_cursor[col], 1);
+		T.order[pos] = arb_index(k);
 	}
 	return n_colours;
 }
@@ -735,7 +644,7 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 {
 	const int tid = sy.tid, stride = sy.stride, n = T.n;
 	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
-	const int n_colours = colour_set(W, S, T, sy);
+	const int n_colours = colour_set(W, S, T);
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----
 	if (T.prestep)
 		for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);
@@ -793,7 +702,7 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 	T.col_off = col_off;
 	T.col_cursor = col_cursor;
 	T.left = &left;
-	T.n_colours_out = nullptr;
+	T.cset = 0;
 	T.bodies = nullptr;
 	T.island_filter = false;
 	T.lds_bodies = lds_cap > 0 ? lds_bodies : nullptr;
@@ -848,26 +757,53 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 	}
 }
 
-// The grid set of all large worlds on the co-resident grid.  Each workgroup
-// colours and sweeps its strips with workgroup barriers; the boundary is
-// coloured on the whole grid and swept after every strip sweep; joints follow
-// the arbiters in every sweep, as in cpSpaceStep.
+// Key of grid-set unit u (the strips, even seams and odd seams of every large
+// world, then the wide set) and the claim arrays of its colouring.
+__device__ __forceinline__ uint64_t unit_key(int u, int n_large)
+{
+	if (u < n_large * GRID_PARTS) return GRID_KEY | ((uint64_t)(u / GRID_PARTS) << 8) | (uint64_t)(u % GRID_PARTS);
+	u -= n_large * GRID_PARTS;
+	if (u < n_large * EVEN_SEAMS)
+		return GRID_KEY | ((uint64_t)(u / EVEN_SEAMS) << 8) | (uint64_t)(GRID_PARTS + 2 * (u % EVEN_SEAMS));
+	u -= n_large * EVEN_SEAMS;
+	if (u < n_large * ODD_SEAMS)
+		return GRID_KEY | ((uint64_t)(u / ODD_SEAMS) << 8) | (uint64_t)(GRID_PARTS + 1 + 2 * (u % ODD_SEAMS));
+	return WIDE_KEY;
+}
+__device__ __forceinline__ int unit_claims(int u, int n_large)
+{
+	if (u < n_large * GRID_PARTS) return 0;
+	if (u < n_large * (GRID_PARTS + EVEN_SEAMS)) return 1;
+	if (u < n_large * (GRID_PARTS + GRID_SEAMS)) return 2;
+	return 3;
+}
+
+constexpr int GRID_LDS_UNITS = 4;   // units per workgroup whose colour offsets stay in LDS (the last slot is scratch)
+
+// The grid set of all large worlds on the co-resident grid.  Workgroup g
+// colours the units g, g + gridDim.x, ... and sweeps them phase by phase:
+// strips, even seams, odd seams, the wide set, each phase ending in a grid
+// barrier; joints follow the arbiters in every sweep, as in cpSpaceStep.
 __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, const SolveBuffers S)
 {
-	__shared__ int32_t lcol_off[CPGPU_MAX_COLOURS + 1];
-	__shared__ int32_t lcol_cursor[CPGPU_MAX_COLOURS];
+	__shared__ int32_t lcol[GRID_LDS_UNITS][CPGPU_MAX_COLOURS + 1];
+	__shared__ int2 linfo[GRID_LDS_UNITS];
+	__shared__ int32_t lcursor[CPGPU_MAX_COLOURS];
 	__shared__ int32_t lleft;
 	const int n_act = *S.n_act;
-	const int grid_first = S.large_first[0], bound_first = S.large_first[1];
-	const int unit = CPGPU_MAX_BUCKETS + 1;
+	const int grid_first = S.large_first[0], wide_first = S.large_first[1];
+	const int n_large = S.n_large_worlds;
+	const int n_units = grid_units(n_large);
+	const int stat = CPGPU_MAX_BUCKETS + 1;
+	const int g = blockIdx.x, G = gridDim.x;
 	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err, 0u};
-	BlockSync bs{(int)threadIdx.x, (int)blockDim.x};
 
-	// ---- each strip coloured by its workgroup ----
-	for (int u = blockIdx.x; u < S.n_parts; u += gridDim.x) {
-		const uint64_t key = GRID_KEY | ((uint64_t)(u / GRID_PARTS) << 8) | (uint64_t)(u % GRID_PARTS);
-		const int start = lower_bound_u64(S.keys, grid_first, bound_first, key);
-		const int end = lower_bound_u64(S.keys, start, bound_first, key + 1);
+	// ---- every unit coloured by its workgroup ----
+	for (int u = g, i = 0; u < n_units; u += G, i++) {
+		const uint64_t key = unit_key(u, n_large);
+		const int start = lower_bound_u64(S.keys, grid_first, n_act, key);
+		const int end = key == WIDE_KEY ? n_act : lower_bound_u64(S.keys, start, n_act, key + 1);
+		int32_t *col = lcol[min(i, GRID_LDS_UNITS - 1)];
 		int nc = 0;
 		if (end > start) {
 			SolveSet P = {};
@@ -875,32 +811,23 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			P.n = end - start;
 			P.order = S.order + start;
 			P.order_base = start;
-			P.col_off = lcol_off;
-			P.col_cursor = lcol_cursor;
+			P.col_off = col;
+			P.col_cursor = lcursor;
 			P.left = &lleft;
-			P.unit = unit;
-			nc = colour_set(W, S, P, bs);
+			P.unit = stat;
+			P.cset = unit_claims(u, n_large);
+			nc = colour_set(W, S, P);
 			__syncthreads();
 		}
-		int32_t *pc = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
-		for (int q = threadIdx.x; q <= nc; q += blockDim.x) pc[q] = lcol_off[q];
-		if (threadIdx.x == 0) S.pinfo[u] = make_int2(start, nc);
+		if (i >= GRID_LDS_UNITS - 1) {
+			int32_t *pc = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
+			for (int q = threadIdx.x; q <= nc; q += blockDim.x) pc[q] = col[q];
+			if (threadIdx.x == 0) S.pinfo[u] = make_int2(start, nc);
+		} else if (threadIdx.x == 0) {
+			linfo[i] = make_int2(start, nc);
+		}
 		__syncthreads();
 	}
-	sy.sync();
-
-	// ---- the boundary, coloured on the whole grid ----
-	SolveSet T = {};
-	T.list = S.arb_by_world + bound_first;
-	T.n = n_act - bound_first;
-	T.order = S.order + bound_first;
-	T.order_base = bound_first;
-	T.col_off = S.col_off;
-	T.col_cursor = S.col_cursor;
-	T.left = S.left;
-	T.n_colours_out = S.n_colours;
-	T.unit = unit;
-	const int n_bcol = colour_set(W, S, T, sy);
 	sy.sync();
 
 	// ---- joint pre-steps (cpSpaceStep.c:389-396) and, with joints, the velocity
@@ -918,49 +845,47 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			integrate_velocity(W, b);
 			n_int++;
 		}
-		wave_add(&W.totals[3 + 2 * unit], n_int);
+		wave_add(&W.totals[3 + 2 * stat], n_int);
 		sy.sync();
 	}
 
-	// ---- sweeps: the strips, then the boundary, then the joints ----
+	// ---- sweeps ----
 	const GlobalBodies bodies{&W};
-	int loaded = -1;   // the strip whose colour offsets are in lcol_off
-	auto strips = [&](auto f) {
-		for (int u = blockIdx.x; u < S.n_parts; u += gridDim.x) {
-			const int2 info = S.pinfo[u];
-			if (loaded != u) {
-				const int32_t *pc = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
-				for (int q = threadIdx.x; q <= info.y; q += blockDim.x) lcol_off[q] = pc[q];
-				loaded = u;
-				__syncthreads();
-			}
+	// this workgroup's units among [u0, u1), colour by colour, then a grid barrier
+	auto phase = [&](int u0, int u1, auto f) {
+		for (int u = u0 + ((g - u0) % G + G) % G; u < u1; u += G) {
+			const int i = (u - g) / G;
+			const bool lds = i < GRID_LDS_UNITS - 1;
+			const int2 info = lds ? linfo[i] : S.pinfo[u];
+			const int32_t *col = lds ? lcol[i] : S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
 			for (int q = 0; q < info.y; q++) {
-				for (int k = lcol_off[q] + (int)threadIdx.x; k < lcol_off[q + 1]; k += (int)blockDim.x) f(info.x + k);
+				const int c1 = col[q + 1];
+				for (int k = col[q] + (int)threadIdx.x; k < c1; k += (int)blockDim.x) f(info.x + k);
 				__syncthreads();
 			}
 		}
 		sy.sync();
 	};
-	auto joints = [&](auto f) {
+	const int e0 = n_large * GRID_PARTS, o0 = e0 + n_large * EVEN_SEAMS, w0 = o0 + n_large * ODD_SEAMS;
+	auto sweep = [&](auto f, auto fj) {
+		phase(0, e0, f);
+		phase(e0, o0, f);
+		phase(o0, w0, f);
+		if (wide_first < n_act) phase(w0, n_units, f);
 		for (int jc = 0; jc < S.n_jcolours; jc++) {
-			for (int k = S.jcol_off[jc] + sy.sweep_tid(); k < S.jcol_off[jc + 1]; k += sy.stride) f(W.joints[S.jorder[k]]);
+			for (int k = S.jcol_off[jc] + sy.sweep_tid(); k < S.jcol_off[jc + 1]; k += sy.stride) fj(W.joints[S.jorder[k]]);
 			sy.sync();
 		}
 	};
 	// cached impulses (cpSpaceStep.c:407-415)
-	auto cached = [&](int pos) { arb_apply_cached(W, bodies, S.act[S.order[pos]]); };
-	strips(cached);
-	for_colours(sy, T.col_off, n_bcol, S.boundary_tail_max, [&](int k) { cached(bound_first + k); });
-	joints([&](Joint &j) { joint_apply_cached(W, bodies, j); });
+	sweep([&](int pos) { arb_apply_cached(W, bodies, S.act[S.order[pos]]); },
+	      [&](Joint &j) { joint_apply_cached(W, bodies, j); });
 	// the impulse iterations (cpSpaceStep.c:418-427)
 	for (int it = 0; it < S.max_iterations; it++) {
-		auto apply = [&](int pos) {
+		sweep([&](int pos) {
 			Arbiter &arb = S.act[S.order[pos]];
 			if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, bodies, arb);
-		};
-		strips(apply);
-		for_colours(sy, T.col_off, n_bcol, S.boundary_tail_max, [&](int k) { apply(bound_first + k); });
-		joints([&](Joint &j) {
+		}, [&](Joint &j) {
 			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);
 		});
 	}
@@ -1037,7 +962,7 @@ __global__ void k_uf_init(const SolveBuffers S)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q == 0) { S.n_groups[0] = 0; S.n_groups[1] = 0; }
-	const int n_large = S.n_parts / GRID_PARTS;
+	const int n_large = S.n_large_worlds;
 	for (int i = q; i < n_large * PART_BINS; i += gridDim.x * blockDim.x) {
 		S.phist[i] = 0;
 		if (i < n_large) { S.pbound[2 * i] = ~0ull; S.pbound[2 * i + 1] = 0ull; }
@@ -1227,12 +1152,13 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 			if (island_on_workgroup(S, r)) {
 				key = ISLAND_KEY | (unsigned long long)r;
 			} else {
-				// the strip of its dynamic bodies, or the boundary when they lie in two strips
+				// the strip of its dynamic bodies, the seam of two neighbouring strips, or the wide set
 				const int pa = is_dynamic(W, arb.ba) ? S.bpart[arb.ba] : -1;
 				const int pb = is_dynamic(W, arb.bb) ? S.bpart[arb.bb] : -1;
-				key = (pa >= 0 && pb >= 0 && pa != pb)
-				          ? BOUNDARY_KEY
-				          : GRID_KEY | ((unsigned long long)S.large_index[w] << 8) | (unsigned long long)max(pa, pb);
+				const unsigned long long l = (unsigned long long)S.large_index[w] << 8;
+				if (pa < 0 || pb < 0 || pa == pb) key = GRID_KEY | l | (unsigned long long)max(pa, pb);
+				else if (abs(pa - pb) == 1) key = GRID_KEY | l | (unsigned long long)(GRID_PARTS + min(pa, pb));
+				else key = WIDE_KEY;
 			}
 			arb_prestep(W, arb);
 		}
@@ -1267,7 +1193,7 @@ __global__ void k_island_integrate(const DevWorld W, const SolveBuffers S)
 }
 
 // warb_off[w] = first sorted position of small world w (n_worlds + 1 entries);
-// large_first = first position of the grid set, of the boundary
+// large_first = first position of the grid set, of the wide set
 __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 {
 	const int n = *S.n_act;
@@ -1275,7 +1201,7 @@ __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 	if (w <= n_worlds) S.warb_off[w] = lower_bound_u64(S.keys, 0, n, (uint64_t)w);
 	if (w == n_worlds + 1) {
 		S.large_first[0] = lower_bound_u64(S.keys, 0, n, GRID_KEY);
-		S.large_first[1] = lower_bound_u64(S.keys, 0, n, BOUNDARY_KEY);
+		S.large_first[1] = lower_bound_u64(S.keys, 0, n, WIDE_KEY);
 	}
 }
 
@@ -1318,7 +1244,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 			TIMED(timer, "strips", stream, {
 				k_part_bounds<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 				k_part_hist<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
-				k_part_scan<<<S.n_parts / GRID_PARTS, PART_BINS, 0, stream>>>(S);
+				k_part_scan<<<S.n_large_worlds, PART_BINS, 0, stream>>>(S);
 				k_part_assign<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 			});
 		HIP_TRY(hipGetLastError());
@@ -1331,7 +1257,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
 		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
-		// keys: a small world's index, ISLAND_KEY | island root, a strip's or the boundary key (below bit 42)
+		// keys: a small world's index, ISLAND_KEY | island root, a grid-set unit's key (below bit 42)
 		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, 42, sortbuf, stream, timer));
 	}
 	if (large && nb > 0) {

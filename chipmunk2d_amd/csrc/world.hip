@@ -20,15 +20,6 @@
 #include "cpgpu_errors.h"
 #include "cpgpu_step.h"
 
-#ifndef CPGPU_COLOUR_TAIL_MAX
-#define CPGPU_COLOUR_TAIL_MAX 4096
-#endif
-#ifndef CPGPU_SOLVE_TAIL_MAX
-#define CPGPU_SOLVE_TAIL_MAX 256
-#endif
-#ifndef CPGPU_BOUNDARY_TAIL_MAX
-#define CPGPU_BOUNDARY_TAIL_MAX 4096
-#endif
 #include "cpgpu_adjoint.h"
 
 namespace CPGPU_NS {
@@ -271,7 +262,6 @@ struct cpgpuWorld {
 	unsigned long long *cprio = nullptr;
 	int32_t *ccol = nullptr;
 	int colour_cap = 0;
-	int32_t *col_off = nullptr, *col_cursor = nullptr, *n_colours = nullptr, *left = nullptr;
 	unsigned long long *claim = nullptr;
 	int32_t *taken = nullptr;
 	int32_t *d_jorder = nullptr, *d_jcol_off = nullptr;
@@ -313,23 +303,17 @@ struct cpgpuWorld {
 	} tape;
 	std::vector<double> h_joint_grads;  // [nc * 8] parameter adjoints of the last backward pass
 	int32_t *d_replay_n = nullptr;
-	// grid-solve tails (see SolveBuffers)
-	// colouring finishes on one workgroup once at most colour_tail_max arbiters are left; runs of colours
-	// of at most solve_tail_max arbiters are swept by one workgroup between two grid barriers
-	int colour_tail_max = CPGPU_COLOUR_TAIL_MAX, solve_tail_max = CPGPU_SOLVE_TAIL_MAX;
-	int boundary_tail_max = CPGPU_BOUNDARY_TAIL_MAX;
-	// strips of the grid set
+	// solve units of the grid set
 	int32_t *d_large_index = nullptr, *phist = nullptr, *bpart = nullptr, *pcol = nullptr;
 	unsigned long long *pbound = nullptr;
 	int2 *pinfo = nullptr;
-	int32_t *tail_list = nullptr, *tail_n = nullptr;
 	int n_jcolours = 0;
 	std::vector<int32_t> h_jcolour;
 	GridBarrier *bar = nullptr;
 	int solver_grid = 0;
 	int max_iterations = 0;
 	// host mirror of counters after the last sync
-	int32_t *h_counts = nullptr;        // pinned: [0..5] step counts, [6] n_pairs, [7] err, [8] n_colours
+	int32_t *h_counts = nullptr;        // pinned: [0..5] step counts, [6] n_pairs, [7] err
 	int64_t steps = 0;
 	int64_t last_pairs = 0;
 	int64_t total_pairs = 0;
@@ -473,7 +457,6 @@ void queue_counters(cpgpuWorld *w)
 	(void)hipMemcpyAsync(w->h_counts, w->d_step_counts, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
 	(void)hipMemcpyAsync(w->h_counts + 6, w->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
 	(void)hipMemcpyAsync(w->h_counts + 7, w->W.err, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
-	(void)hipMemcpyAsync(w->h_counts + 8, w->n_colours, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
 }
 
 int sync_world(cpgpuWorld *w)
@@ -711,8 +694,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		const int n_large = std::max(w->n_large_worlds, 1);
 		HIP_OR_FAIL(own_alloc(w, w->pbound, 2 * (size_t)n_large));
 		HIP_OR_FAIL(own_alloc(w, w->phist, (size_t)PART_BINS * n_large));
-		HIP_OR_FAIL(own_alloc(w, w->pcol, (size_t)GRID_PARTS * n_large * (CPGPU_MAX_COLOURS + 1)));
-		HIP_OR_FAIL(own_alloc(w, w->pinfo, (size_t)GRID_PARTS * n_large));
+		HIP_OR_FAIL(own_alloc(w, w->pcol, (size_t)grid_units(n_large) * (CPGPU_MAX_COLOURS + 1)));
+		HIP_OR_FAIL(own_alloc(w, w->pinfo, (size_t)grid_units(n_large)));
 
 		// large worlds' joints: by colour
 		int ncol_large = 0;
@@ -772,17 +755,11 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_alloc(w, w->d_world_bounds, w->nw));
 	HIP_OR_FAIL(own_alloc(w, w->d_joint_bias_coef, nc));
 	HIP_OR_FAIL(own_upload(w, w->bar, (const GridBarrier *)nullptr, 1));
-	HIP_OR_FAIL(own_upload(w, w->col_off, (const int32_t *)nullptr, CPGPU_MAX_COLOURS + 1));
-	HIP_OR_FAIL(own_upload(w, w->col_cursor, (const int32_t *)nullptr, CPGPU_MAX_COLOURS));
-	HIP_OR_FAIL(own_upload(w, w->n_colours, (const int32_t *)nullptr, 1));
-	HIP_OR_FAIL(own_upload(w, w->left, (const int32_t *)nullptr, 1));
 	HIP_OR_FAIL(own_upload(w, w->warb_off, (const int32_t *)nullptr, w->nw + 2));
-	HIP_OR_FAIL(own_alloc(w, w->tail_list, w->colour_tail_max));
-	HIP_OR_FAIL(own_upload(w, w->tail_n, (const int32_t *)nullptr, 4));
 	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 2));
 	HIP_OR_FAIL(own_alloc(w, w->bpart, nb));
-	HIP_OR_FAIL(own_alloc(w, w->claim, 2 * (size_t)nb));
-	HIP_OR_FAIL(own_alloc(w, w->taken, nb));
+	HIP_OR_FAIL(own_alloc(w, w->claim, CLAIM_SETS * 2 * (size_t)nb));
+	HIP_OR_FAIL(own_alloc(w, w->taken, CLAIM_SETS * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->label, nb));
 	HIP_OR_FAIL(own_alloc(w, w->icount, nb));
 	HIP_OR_FAIL(own_alloc(w, w->ijoint, nb));
@@ -867,19 +844,15 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.n_large_joints = w->n_large_joints;
 	S.warb_off = warb_off; S.large_first = w->large_first;
 	S.world_small = w->d_world_small;
-	S.col_off = w->col_off; S.col_cursor = w->col_cursor; S.n_colours = w->n_colours; S.left = w->left;
 	S.claim = w->claim; S.taken = w->taken;
 	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
 	S.jw_order = w->d_jw_order; S.jw_col = w->d_jw_col; S.jw_cbase = w->d_jw_cbase; S.jw_ncol = w->d_jw_ncol;
 	S.bar = w->bar;
-	S.tail_list = w->tail_list; S.tail_n = w->tail_n; S.tail_state = w->tail_n + 2;
-	S.colour_tail_max = w->colour_tail_max; S.solve_tail_max = w->solve_tail_max;
 	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;
 	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
-	S.large_index = w->d_large_index; S.n_parts = GRID_PARTS * w->n_large_worlds;
+	S.large_index = w->d_large_index; S.n_large_worlds = w->n_large_worlds;
 	S.pbound = w->pbound; S.phist = w->phist; S.bpart = w->bpart; S.pcol = w->pcol; S.pinfo = w->pinfo;
-	S.boundary_tail_max = w->boundary_tail_max;
 	S.rec = nullptr; S.rec_stride = 0; S.jrec = nullptr;
 	return S;
 }

@@ -1198,16 +1198,21 @@ static int cmp_jsolve(const void *x, const void *y)
  * bodies are cut into GRID_PARTS strips along x holding equal shares of
  * them: a PART_BINS-bin histogram of x between the set's least and greatest
  * x, each bin going to strip floor(GRID_PARTS * (bodies in earlier bins) /
- * (bodies)).  An arbiter of the grid set is interior to the strip of its
- * dynamic bodies, or on the boundary when they lie in two strips.
+ * (bodies)).  An arbiter of the grid set belongs to the strip of its
+ * dynamic bodies, to seam q when they lie in strips q and q + 1, or to the
+ * wide set when their strips are further apart.
  * Each unit is coloured on its own and the space is solved unit by unit:
- * the other islands (unit 0, together: they share no body), the strips in
- * order (1 + q), the boundary (1 + GRID_PARTS).                          */
+ * the other islands (unit 0, together: they share no body), the strips
+ * (1 + q), the even seams (UNIT_EVEN + q / 2), the odd seams
+ * (UNIT_ODD + q / 2), the wide set (UNIT_WIDE).                          */
 
 #define SMALL_WORLD_MAX_BODIES 4096
 #define ISLAND_MAX 2048
 #define GRID_PARTS 64
 #define PART_BINS 1024
+#define UNIT_EVEN (1 + GRID_PARTS)
+#define UNIT_ODD (UNIT_EVEN + GRID_PARTS / 2)
+#define UNIT_WIDE (UNIT_ODD + (GRID_PARTS - 1) / 2)
 
 static int uf_root(int32_t *parent, int x)
 {
@@ -1305,7 +1310,11 @@ static int grid_units(oworld *w, OWorldPart *P)
 		int r = uf_root(parent, (DYN(arb->ba) ? arb->ba : arb->bb) - P->b0);
 		if (icount[r] <= ISLAND_MAX && !ijoint[r]) continue;
 		int pa = DYN(arb->ba) ? part[arb->ba - P->b0] : -1, pb = DYN(arb->bb) ? part[arb->bb - P->b0] : -1;
-		arb->unit = (pa >= 0 && pb >= 0 && pa != pb) ? 1 + GRID_PARTS : 1 + (pa > pb ? pa : pb);
+		if (pa < 0 || pb < 0 || pa == pb) arb->unit = 1 + (pa > pb ? pa : pb);
+		else if (pa - pb == 1 || pb - pa == 1) {
+			int q = pa < pb ? pa : pb;
+			arb->unit = (q % 2 == 0 ? UNIT_EVEN : UNIT_ODD) + q / 2;
+		} else arb->unit = UNIT_WIDE;
 	}
 #undef DYN
 	free(parent); free(icount); free(ijoint); free(part);
@@ -1722,7 +1731,7 @@ static void step_world(oworld *w, OWorldPart *P, double dt, const int32_t *order
 			int32_t *ubb = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
 			uint64_t *upr = (uint64_t *)malloc((size_t)(n + 1) * sizeof(uint64_t));
 			P->ncol_arb = 0;
-			for (int u = 0; u <= 1 + GRID_PARTS; u++) {
+			for (int u = 0; u <= UNIT_WIDE; u++) {
 				int64_t m = 0;
 				for (int64_t k = 0; k < n; k++)
 					if (P->arbs[P->active[k]].unit == u) { idx[m] = k; uba[m] = ba[k]; ubb[m] = bb[k]; upr[m] = pr[k]; m++; }

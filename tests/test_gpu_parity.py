@@ -101,9 +101,9 @@ def test_batch_of_worlds_bit_exact():
 
 
 def _strips_seen(o):
-    """Strips holding arbiters and boundary arbiters in the oracle's last step."""
+    """Strips holding arbiters, and arbiters of seams or the wide set, in the oracle's last step."""
     u = o.units()
-    return len(np.unique(u[(u > 0) & (u <= 64)])), int(np.count_nonzero(u == 65))
+    return len(np.unique(u[(u > 0) & (u <= 64)])), int(np.count_nonzero(u > 64))
 
 
 def test_grid_solve_mid_size_world_bit_exact():
@@ -117,7 +117,7 @@ def test_grid_solve_mid_size_world_bit_exact():
 
 def test_grid_strips_dense_world_bit_exact():
     """A dense large space: one island of ~10k arbiters, solved on the grid in
-    64 strips and a boundary."""
+    64 strips, the seams between them and the wide set."""
     pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
     g, o = run_lockstep(pw, 60, check_every=1, arbiters_every=20)
     strips, boundary = _strips_seen(o)

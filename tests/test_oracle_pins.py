@@ -537,16 +537,22 @@ def _strip_units(pw, p, shapes, small_max=4096, island_max=2048, parts=64, bins=
     on_grid = grid_comp[arb_comp]
     pa = np.where(dyn[a], part[a], -1)
     pb = np.where(dyn[b], part[b], -1)
-    cross = (pa >= 0) & (pb >= 0) & (pa != pb)
-    unit[on_grid] = np.where(cross, 1 + parts, 1 + np.maximum(pa, pb))[on_grid]
+    both = (pa >= 0) & (pb >= 0)
+    gap = np.abs(pa - pb)
+    q = np.minimum(pa, pb)
+    seam = np.where(q % 2 == 0, 1 + parts + q // 2, 1 + parts + parts // 2 + q // 2)
+    u = np.where(both & (gap == 1), seam, 1 + np.maximum(pa, pb))
+    u = np.where(both & (gap > 1), 1 + parts + parts // 2 + (parts - 1) // 2, u)
+    unit[on_grid] = u[on_grid]
     return unit
 
 
 @pytest.mark.parametrize("scene", ["dense", "chain"])
 def test_large_space_strips_match_restatement(scene):
-    """A large space's solve units (islands, strips along x, the boundary)
-    against an independent restatement; each unit coloured on its own (the
-    Python claim rounds) and solved unit by unit, by colour inside a unit."""
+    """A large space's solve units (islands, strips along x, seams between
+    neighbouring strips, even ones first, and the wide set) against an
+    independent restatement; each unit coloured on its own (the Python claim
+    rounds) and solved unit by unit, by colour inside a unit."""
     from chipmunk2d_amd.scenes import chain_world
     if scene == "dense":
         pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
@@ -565,7 +571,10 @@ def test_large_space_strips_match_restatement(scene):
     assert np.array_equal(units, want)
     strips = np.unique(units[(units > 0) & (units <= 64)])
     assert len(strips) >= (32 if scene == "dense" else 16), len(strips)
-    assert np.count_nonzero(units == 65) > 0
+    assert len(np.unique(units[(units > 64) & (units <= 96)])) >= (24 if scene == "dense" else 4)   # even seams
+    assert len(np.unique(units[(units > 96) & (units < 128)])) >= (24 if scene == "dense" else 4)   # odd seams
+    if scene == "dense":
+        assert np.count_nonzero(units == 128) > 0   # arbiters spanning more than one strip border
     # solve order: by unit, then by colour
     order = units.astype(np.int64) * 4096 + A.colour
     assert np.all(np.diff(order) >= 0)

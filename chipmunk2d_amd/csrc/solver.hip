@@ -458,42 +458,36 @@ __device__ __forceinline__ void wave_add(unsigned long long *counter, unsigned l
 	if (__lane_id() == 0 && v) atomicAdd(counter, v);
 }
 
-// One colouring claim round (see the file comment) over the arbiters at
-// positions idx[0..n) of the solve set (idx == nullptr: 0..n-1).  Each
-// arbiter's dynamic bodies (-1 for none), priority and colour were gathered
-// into the position-indexed arrays cbody / cprio / ccol, so a round reads
-// them coalesced instead of scattered arbiter records.  Returns the number of
-// arbiters that took colour c.
-__device__ int colour_round(int32_t *taken, const int2 *cbody, const unsigned long long *cprio, int32_t *ccol,
-                            const int32_t *idx, int n, int tid, int stride, int c, unsigned long long *cl,
-                            unsigned long long *other, bool claim_phase)
+// One arbiter's part of a colouring round (see the file comment): its
+// dynamic bodies bd (-1 for none), priority and colour so far.
+// Claim phase: an uncoloured arbiter none of whose bodies colour c has taken
+// claims them with an atomic min of its priority.
+__device__ __forceinline__ void colour_claim(const int32_t *taken, int2 bd, unsigned long long prio, int col, int c,
+                                             unsigned long long *cl)
 {
-	int won = 0;
-	for (int j = tid; j < n; j += stride) {
-		const int k = idx ? idx[j] : j;
-		if (ccol[k] >= 0) continue;
-		const int2 bd = cbody[k];
-		const int ba = bd.x, bb = bd.y;
-		const unsigned long long prio = cprio[k];
-		if (claim_phase) {
-			if ((ba >= 0 && taken[ba] == c) || (bb >= 0 && taken[bb] == c)) continue;
-			if (ba >= 0) atomicMin(&cl[ba], prio);
-			if (bb >= 0) atomicMin(&cl[bb], prio);
-		} else {
-			const bool win = (ba < 0 || __hip_atomic_load(&cl[ba], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio) &&
-			                 (bb < 0 || __hip_atomic_load(&cl[bb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio);
-			if (win) {
-				ccol[k] = c;
-				if (ba >= 0) taken[ba] = c;
-				if (bb >= 0) taken[bb] = c;
-				won++;
-			} else {
-				if (ba >= 0) other[ba] = ~0ull;
-				if (bb >= 0) other[bb] = ~0ull;
-			}
-		}
+	if (col >= 0) return;
+	if ((bd.x >= 0 && taken[bd.x] == c) || (bd.y >= 0 && taken[bd.y] == c)) return;
+	if (bd.x >= 0) atomicMin(&cl[bd.x], prio);
+	if (bd.y >= 0) atomicMin(&cl[bd.y], prio);
+}
+// Resolve phase: an arbiter holding all its claims takes colour c and its
+// bodies; the others reset their bodies' claims for the next round.
+// Returns 1 when the arbiter took colour c.
+__device__ __forceinline__ int colour_resolve(int32_t *taken, int2 bd, unsigned long long prio, int &col, int c,
+                                              unsigned long long *cl, unsigned long long *other)
+{
+	if (col >= 0) return 0;
+	const bool win = (bd.x < 0 || __hip_atomic_load(&cl[bd.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio) &&
+	                 (bd.y < 0 || __hip_atomic_load(&cl[bd.y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == prio);
+	if (win) {
+		col = c;
+		if (bd.x >= 0) taken[bd.x] = c;
+		if (bd.y >= 0) taken[bd.y] = c;
+		return 1;
 	}
-	return won;
+	if (bd.x >= 0) other[bd.x] = ~0ull;
+	if (bd.y >= 0) other[bd.y] = ~0ull;
+	return 0;
 }
 
 // Run f(k) for every position k of colours [0, n_colours) in colour order,
@@ -567,7 +561,11 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 }
 
 // Colour the set's arbiters and sort them into colour order (T.order,
-// T.col_off) on one workgroup; returns the colour count.
+// T.col_off) on one workgroup; returns the colour count.  Thread t handles
+// positions t, t + blockDim.x, ...; the first COLOUR_REGS of them keep their
+// bodies, priority and colour in registers through the rounds, the rest in
+// the position-indexed arrays S.cbody / S.cprio / S.ccol.
+constexpr int COLOUR_REGS = 4;
 __device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T)
 {
 	const int tid = threadIdx.x, stride = blockDim.x, n = T.n;
@@ -577,16 +575,33 @@ __device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	int2 *cbody = S.cbody + T.order_base;
 	unsigned long long *cprio = S.cprio + T.order_base;
 	int32_t *ccol = S.ccol + T.order_base;
+	int2 rb[COLOUR_REGS];
+	unsigned long long rp[COLOUR_REGS];
+	int rc[COLOUR_REGS];
+	const int k_mem = tid + COLOUR_REGS * stride;   // first position kept in memory
 
 	// ---- colouring ----
-	for (int k = tid; k < n; k += stride) {
+	auto init = [&](int k, int2 &bd, unsigned long long &prio) {
 		const Arbiter &arb = S.act[arb_index(k)];
-		const int ba = dyn_or_neg(W, arb.ba), bb = dyn_or_neg(W, arb.bb);
-		cbody[k] = make_int2(ba, bb);
-		cprio[k] = arbiter_priority(W, arb);
+		bd = make_int2(dyn_or_neg(W, arb.ba), dyn_or_neg(W, arb.bb));
+		prio = arbiter_priority(W, arb);
+		if (bd.x >= 0) { claim0[bd.x] = ~0ull; claim1[bd.x] = ~0ull; taken[bd.x] = -1; }
+		if (bd.y >= 0) { claim0[bd.y] = ~0ull; claim1[bd.y] = ~0ull; taken[bd.y] = -1; }
+	};
+#pragma unroll
+	for (int r = 0; r < COLOUR_REGS; r++) {
+		rc[r] = -1;
+		rb[r] = make_int2(-1, -1);
+		rp[r] = 0;
+		if (tid + r * stride < n) init(tid + r * stride, rb[r], rp[r]);
+	}
+	for (int k = k_mem; k < n; k += stride) {
+		int2 bd;
+		unsigned long long prio;
+		init(k, bd, prio);
+		cbody[k] = bd;
+		cprio[k] = prio;
 		ccol[k] = -1;
-		if (ba >= 0) { claim0[ba] = ~0ull; claim1[ba] = ~0ull; taken[ba] = -1; }
-		if (bb >= 0) { claim0[bb] = ~0ull; claim1[bb] = ~0ull; taken[bb] = -1; }
 	}
 	for (int k = tid; k <= CPGPU_MAX_COLOURS; k += stride) T.col_off[k] = 0;
 	if (tid == 0) *T.left = n;
@@ -599,9 +614,22 @@ __device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		if (c >= CPGPU_MAX_COLOURS) { if (tid == 0) raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); break; }
 		unsigned long long *cl = (sub & 1) ? claim1 : claim0;
 		unsigned long long *other = (sub & 1) ? claim0 : claim1;
-		colour_round(taken, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, true);
+#pragma unroll
+		for (int r = 0; r < COLOUR_REGS; r++)
+			if (tid + r * stride < n) colour_claim(taken, rb[r], rp[r], rc[r], c, cl);
+		for (int k = k_mem; k < n; k += stride) colour_claim(taken, cbody[k], cprio[k], ccol[k], c, cl);
 		__syncthreads();
-		const int won = colour_round(taken, cbody, cprio, ccol, nullptr, n, tid, stride, c, cl, other, false);
+		int won = 0;
+#pragma unroll
+		for (int r = 0; r < COLOUR_REGS; r++)
+			if (tid + r * stride < n) won += colour_resolve(taken, rb[r], rp[r], rc[r], c, cl, other);
+		for (int k = k_mem; k < n; k += stride) {
+			int col = ccol[k];
+			if (colour_resolve(taken, cbody[k], cprio[k], col, c, cl, other)) {
+				ccol[k] = col;
+				won++;
+			}
+		}
 		if (won) atomicSub(T.left, won);
 		__syncthreads();
 		if (++sub % COLOUR_SUBROUNDS == 0) c++;
@@ -609,14 +637,20 @@ __device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	const int n_colours = (sub % COLOUR_SUBROUNDS == 0) ? c : c + 1;
 
 	// ---- colour sort ----
+	// f(position, colour) for this thread's positions
+	auto each = [&](auto f) {
+#pragma unroll
+		for (int r = 0; r < COLOUR_REGS; r++)
+			if (tid + r * stride < n) f(tid + r * stride, rc[r]);
+		for (int k = k_mem; k < n; k += stride) f(k, ccol[k]);
+	};
 	unsigned long long n_contacts = 0;
-	for (int k = tid; k < n; k += stride) {
+	each([&](int k, int col) {
 		const int a = arb_index(k);
-		const int col = ccol[k];
 		S.colour[a] = col;
 		if (col >= 0) atomicAdd(&T.col_off[col], 1);
 		n_contacts += S.act[a].count;
-	}
+	});
 	wave_add(&W.totals[2 + 2 * T.unit], n_contacts);
 	__syncthreads();
 	if (tid == 0) {
@@ -630,12 +664,9 @@ __device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 		T.col_off[n_colours] = run;
 	}
 	__syncthreads();
-	for (int k = tid; k < n; k += stride) {
-		const int col = ccol[k];
-		if (col < 0) continue;
-		const int pos = atomicAdd(&T.col_cursor[col], 1);
-		T.order[pos] = arb_index(k);
-	}
+	each([&](int k, int col) {
+		if (col >= 0) T.order[atomicAdd(&T.col_cursor[col], 1)] = arb_index(k);
+	});
 	return n_colours;
 }
 

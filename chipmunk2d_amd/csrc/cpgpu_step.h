@@ -140,8 +140,12 @@ struct SolveBuffers {
 	unsigned long long *pbound;    // [2 per large world] least and greatest x of the grid set (ordered keys)
 	int32_t *phist;                // [PART_BINS per large world] bin counts, then each bin's strip
 	int32_t *bpart;                // [n_bodies] strip of each grid-set body
+	int32_t *bslot;                // [n_bodies] its slot among its strip's bodies
+	int32_t *pcount;               // [GRID_PARTS per large world] bodies of each strip
 	int32_t *pcol;                 // [grid_units][CPGPU_MAX_COLOURS + 1] colour offsets of each unit
 	int2 *pinfo;                   // [grid_units] first sorted position and colour count of each unit
+	int32_t *udone;                // [grid_units] sweeps finished by each unit (dependency-driven schedule)
+	int grid_flow;                 // allow the dependency-driven schedule (else grid barriers between phases)
 	// recording (adjoint replay): rec[it * rec_stride + position in order], or nullptr
 	ArbRec *rec;
 	int rec_stride;

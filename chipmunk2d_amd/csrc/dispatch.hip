@@ -29,6 +29,7 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldGetBuckets(cpgpuWorld *world, int32_t *max_bodies, int32_t *threads, int32_t *n_worlds, int32_t *lds_bodies); \
 	int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies, const int32_t *threads); \
 	int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out); \
+	int cpgpuWorldSetGridSchedule(cpgpuWorld *world, int32_t schedule); \
 	int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every); \
 	int cpgpuWorldTapeEnd(cpgpuWorld *world); \
 	int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v, const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0, double *d_m_inv, double *d_i_inv, double *d_u); \
@@ -145,6 +146,13 @@ int cpgpuWorldGetBuckets(cpgpuWorld *world, int32_t *max_bodies, int32_t *thread
 	if (!world) return null_world();
 	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetBuckets(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), max_bodies, threads, n_worlds, lds_bodies);
 	return cpgpu64::cpgpuWorldGetBuckets(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), max_bodies, threads, n_worlds, lds_bodies);
+}
+
+int cpgpuWorldSetGridSchedule(cpgpuWorld *world, int32_t schedule)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldSetGridSchedule(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), schedule);
+	return cpgpu64::cpgpuWorldSetGridSchedule(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), schedule);
 }
 
 int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies, const int32_t *threads)

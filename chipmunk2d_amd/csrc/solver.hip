@@ -172,26 +172,36 @@ __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
 	}
 }
 
+// The impulse applications take the arbiter's body indices and whether they
+// are dynamic as arguments, so a caller that has them at hand (the grid
+// solve keeps them in LDS) loads the bodies beside the arbiter instead of
+// after it.
 template <class BA>
-__device__ void arb_apply_cached(const DevWorld &W, const BA &bodies, const Arbiter &arb)
+__device__ void arb_apply_cached(const DevWorld &W, const BA &bodies, const Arbiter &arb, int ia, int ib, bool da,
+                                 bool db)
 {
 	if (arb.state == CPGPU_ARB_FIRST_COLLISION) return;
 	const real dt_coef = W.ws[arb.world].dt_coef;
-	BV a = bodies.load(arb.ba), b = bodies.load(arb.bb);
+	BV a = bodies.load(ia), b = bodies.load(ib);
 	const V2 n = v2(arb.nx, arb.ny);
 	for (int i = 0; i < arb.count; i++) {
 		const Contact &con = arb.c[i];
 		const V2 j = vrotate(n, v2(con.jnAcc, con.jtAcc));
 		apply_impulses(a, b, v2(con.r1x, con.r1y), v2(con.r2x, con.r2y), vmult(j, dt_coef));
 	}
-	if (is_dynamic(W, arb.ba)) bodies.store(arb.ba, a);
-	if (is_dynamic(W, arb.bb)) bodies.store(arb.bb, b);
+	if (da) bodies.store(ia, a);
+	if (db) bodies.store(ib, b);
+}
+template <class BA>
+__device__ __forceinline__ void arb_apply_cached(const DevWorld &W, const BA &bodies, const Arbiter &arb)
+{
+	arb_apply_cached(W, bodies, arb, arb.ba, arb.bb, is_dynamic(W, arb.ba), is_dynamic(W, arb.bb));
 }
 
 template <class BA>
-__device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &arb)
+__device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &arb, int ia, int ib, bool da, bool db)
 {
-	BV a = bodies.load(arb.ba), b = bodies.load(arb.bb);
+	BV a = bodies.load(ia), b = bodies.load(ib);
 	const V2 n = v2(arb.nx, arb.ny);
 	const V2 surface_vr = v2(arb.svx, arb.svy);
 	const real friction = arb.u;
@@ -220,8 +230,13 @@ __device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &
 		apply_bias_impulse(b, jb, r2);
 		apply_impulses(a, b, r1, r2, vrotate(n, v2(con.jnAcc - jnOld, con.jtAcc - jtOld)));
 	}
-	if (is_dynamic(W, arb.ba)) bodies.store(arb.ba, a);
-	if (is_dynamic(W, arb.bb)) bodies.store(arb.bb, b);
+	if (da) bodies.store(ia, a);
+	if (db) bodies.store(ib, b);
+}
+template <class BA>
+__device__ __forceinline__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &arb)
+{
+	arb_apply_impulse(W, bodies, arb, arb.ba, arb.bb, is_dynamic(W, arb.ba), is_dynamic(W, arb.bb));
 }
 
 // ---- joints (cpPivotJoint.c:24-70, cpDampedSpring.c:24-77, cpSlideJoint.c:24-89, cpGearJoint.c:24-69) ----
@@ -449,6 +464,12 @@ struct SolveSet {
 	int order_base;            // position of order[0] in S.order (records are indexed by it)
 	int cset;                  // claim arrays used by the colouring (units coloured concurrently
 	                           // with shared bodies use different ones)
+	// colouring with the claim arrays in LDS (lclaim != nullptr): the set's dynamic bodies
+	// are numbered 0 .. by slot[body]
+	unsigned long long *lclaim;   // [2][lcap]
+	int32_t *ltaken;              // [lcap]
+	int lcap;
+	const int32_t *slot;
 };
 
 // a wave-reduced 64-bit add to a counter
@@ -566,11 +587,20 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 // bodies, priority and colour in registers through the rounds, the rest in
 // the position-indexed arrays S.cbody / S.cprio / S.ccol.
 constexpr int COLOUR_REGS = 4;
-__device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T)
+__device__ __forceinline__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T)
 {
 	const int tid = threadIdx.x, stride = blockDim.x, n = T.n;
-	unsigned long long *claim0 = S.claim + (size_t)T.cset * 2 * S.n_bodies, *claim1 = claim0 + S.n_bodies;
-	int32_t *taken = S.taken + (size_t)T.cset * S.n_bodies;
+	unsigned long long *claim0, *claim1;
+	int32_t *taken;
+	if (T.lclaim) {
+		claim0 = T.lclaim;
+		claim1 = T.lclaim + T.lcap;
+		taken = T.ltaken;
+	} else {
+		claim0 = S.claim + (size_t)T.cset * 2 * S.n_bodies;
+		claim1 = claim0 + S.n_bodies;
+		taken = S.taken + (size_t)T.cset * S.n_bodies;
+	}
 	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
 	int2 *cbody = S.cbody + T.order_base;
 	unsigned long long *cprio = S.cprio + T.order_base;
@@ -584,6 +614,7 @@ __device__ int colour_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	auto init = [&](int k, int2 &bd, unsigned long long &prio) {
 		const Arbiter &arb = S.act[arb_index(k)];
 		bd = make_int2(dyn_or_neg(W, arb.ba), dyn_or_neg(W, arb.bb));
+		if (T.lclaim) bd = make_int2(bd.x >= 0 ? T.slot[bd.x] : -1, bd.y >= 0 ? T.slot[bd.y] : -1);
 		prio = arbiter_priority(W, arb);
 		if (bd.x >= 0) { claim0[bd.x] = ~0ull; claim1[bd.x] = ~0ull; taken[bd.x] = -1; }
 		if (bd.y >= 0) { claim0[bd.y] = ~0ull; claim1[bd.y] = ~0ull; taken[bd.y] = -1; }
@@ -729,7 +760,7 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 	__shared__ int32_t col_cursor[CPGPU_MAX_COLOURS];
 	__shared__ int32_t left;
 	__shared__ int32_t next;
-	SolveSet T;
+	SolveSet T = {};
 	T.col_off = col_off;
 	T.col_cursor = col_cursor;
 	T.left = &left;
@@ -809,32 +840,74 @@ __device__ __forceinline__ int unit_claims(int u, int n_large)
 	return 3;
 }
 
-constexpr int GRID_LDS_UNITS = 4;   // units per workgroup whose colour offsets stay in LDS (the last slot is scratch)
+// Completion counters of the grid-set units (passes finished): thread 0
+// waits until a unit has finished `need` passes, giving up (and flagging a
+// barrier time-out) after about a second, or when another wait gave up.
+__device__ void unit_wait(const int32_t *done, int need, int32_t *err)
+{
+	long spins = 0;
+	while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+		__builtin_amdgcn_s_sleep(1);
+		if (++spins > (1L << 25)) {
+			atomicOr(err, CPGPU_ERR_BARRIER_TIMEOUT);
+			break;
+		}
+		if ((spins & 1023) == 0 &&
+		    (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & CPGPU_ERR_BARRIER_TIMEOUT))
+			break;
+	}
+}
 
-// The grid set of all large worlds on the co-resident grid.  Workgroup g
-// colours the units g, g + gridDim.x, ... and sweeps them phase by phase:
-// strips, even seams, odd seams, the wide set, each phase ending in a grid
-// barrier; joints follow the arbiters in every sweep, as in cpSpaceStep.
+constexpr int GRID_LDS_COLS = 2;   // columns per workgroup whose units' colour offsets stay in LDS
+constexpr int GRID_LDS_POS = 4096; // arbiters per workgroup whose index and bodies stay in LDS
+constexpr int GRID_LDS_BODIES = 1536;   // strips of at most this many bodies are coloured with claims in LDS
+
+// The grid set of all large worlds on the co-resident grid.  Column c =
+// l * GRID_PARTS + q holds strip q of the l-th large world and seam q to its
+// right; workgroup g colours and sweeps columns g, g + gridDim.x, ... (the
+// last workgroup also the wide set).  A sweep runs the strips, the even
+// seams, the odd seams, the wide set, then the joints, as in cpSpaceStep.
+// Without joints and wide set, units wait only for the units they share
+// bodies with (a strip for its two seams' previous sweep, an even seam for
+// its two strips, an odd seam for the even seams beside it) through
+// completion counters; otherwise grid barriers separate the phases.
 __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, const SolveBuffers S)
 {
-	__shared__ int32_t lcol[GRID_LDS_UNITS][CPGPU_MAX_COLOURS + 1];
-	__shared__ int2 linfo[GRID_LDS_UNITS];
+	constexpr int SCRATCH = 2 * GRID_LDS_COLS;   // LDS slot of units kept in global memory
+	__shared__ int32_t lcol[SCRATCH + 1][CPGPU_MAX_COLOURS + 1];
+	__shared__ int2 linfo[SCRATCH];
 	__shared__ int32_t lcursor[CPGPU_MAX_COLOURS];
 	__shared__ int32_t lleft;
+	// the units' arbiters in colour order: index, and body indices (~b for a body that is not dynamic)
+	__shared__ int32_t lpos[GRID_LDS_POS];
+	__shared__ int2 lpb[GRID_LDS_POS];
+	__shared__ int32_t lbase[SCRATCH];   // a unit's first entry in lpos / lpb, -1 when not kept there
+	__shared__ unsigned long long lclaim[2 * GRID_LDS_BODIES];
+	__shared__ int32_t ltaken[GRID_LDS_BODIES];
+	int lds_used = 0;
 	const int n_act = *S.n_act;
 	const int grid_first = S.large_first[0], wide_first = S.large_first[1];
 	const int n_large = S.n_large_worlds;
-	const int n_units = grid_units(n_large);
+	const int n_units = grid_units(n_large), n_cols = n_large * GRID_PARTS;
+	const int e0 = n_cols, o0 = e0 + n_large * EVEN_SEAMS;
 	const int stat = CPGPU_MAX_BUCKETS + 1;
 	const int g = blockIdx.x, G = gridDim.x;
 	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err, 0u};
 
+	// the seam unit of column c, -1 for a world's last strip
+	auto seam_of = [&](int c) {
+		const int l = c / GRID_PARTS, q = c % GRID_PARTS;
+		if (q == GRID_SEAMS) return -1;
+		return (q % 2 == 0 ? e0 + l * EVEN_SEAMS : o0 + l * ODD_SEAMS) + q / 2;
+	};
+	auto slot = [&](int i, int seam) { return i < GRID_LDS_COLS ? 2 * i + seam : SCRATCH; };
+
 	// ---- every unit coloured by its workgroup ----
-	for (int u = g, i = 0; u < n_units; u += G, i++) {
+	auto colour_unit = [&](int u, int s) {
 		const uint64_t key = unit_key(u, n_large);
 		const int start = lower_bound_u64(S.keys, grid_first, n_act, key);
 		const int end = key == WIDE_KEY ? n_act : lower_bound_u64(S.keys, start, n_act, key + 1);
-		int32_t *col = lcol[min(i, GRID_LDS_UNITS - 1)];
+		int32_t *col = lcol[s];
 		int nc = 0;
 		if (end > start) {
 			SolveSet P = {};
@@ -847,18 +920,45 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			P.left = &lleft;
 			P.unit = stat;
 			P.cset = unit_claims(u, n_large);
+			if (u < n_cols && S.pcount[u] <= GRID_LDS_BODIES) {
+				P.lclaim = lclaim;
+				P.ltaken = ltaken;
+				P.lcap = GRID_LDS_BODIES;
+				P.slot = S.bslot;
+			}
 			nc = colour_set(W, S, P);
 			__syncthreads();
 		}
-		if (i >= GRID_LDS_UNITS - 1) {
+		if (s == SCRATCH) {
 			int32_t *pc = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
 			for (int q = threadIdx.x; q <= nc; q += blockDim.x) pc[q] = col[q];
 			if (threadIdx.x == 0) S.pinfo[u] = make_int2(start, nc);
-		} else if (threadIdx.x == 0) {
-			linfo[i] = make_int2(start, nc);
+		} else {
+			const int n = end - start;
+			const bool keep = lds_used + n <= GRID_LDS_POS;
+			if (threadIdx.x == 0) {
+				linfo[s] = make_int2(start, nc);
+				lbase[s] = keep ? lds_used : -1;
+			}
+			if (keep) {
+				for (int k = threadIdx.x; k < n; k += blockDim.x) {
+					const int a = S.order[start + k];
+					const Arbiter &arb = S.act[a];
+					lpos[lds_used + k] = a;
+					lpb[lds_used + k] = make_int2(is_dynamic(W, arb.ba) ? arb.ba : ~arb.ba,
+					                              is_dynamic(W, arb.bb) ? arb.bb : ~arb.bb);
+				}
+				lds_used += n;
+			}
 		}
+		if (threadIdx.x == 0) S.udone[u] = 0;
 		__syncthreads();
+	};
+	for (int c = g, i = 0; c < n_cols; c += G, i++) {
+		colour_unit(c, slot(i, 0));
+		if (seam_of(c) >= 0) colour_unit(seam_of(c), slot(i, 1));
 	}
+	if (g == G - 1) colour_unit(n_units - 1, SCRATCH);
 	sy.sync();
 
 	// ---- joint pre-steps (cpSpaceStep.c:389-396) and, with joints, the velocity
@@ -882,40 +982,97 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 
 	// ---- sweeps ----
 	const GlobalBodies bodies{&W};
-	// this workgroup's units among [u0, u1), colour by colour, then a grid barrier
-	auto phase = [&](int u0, int u1, auto f) {
-		for (int u = u0 + ((g - u0) % G + G) % G; u < u1; u += G) {
-			const int i = (u - g) / G;
-			const bool lds = i < GRID_LDS_UNITS - 1;
-			const int2 info = lds ? linfo[i] : S.pinfo[u];
-			const int32_t *col = lds ? lcol[i] : S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
-			for (int q = 0; q < info.y; q++) {
-				const int c1 = col[q + 1];
-				for (int k = col[q] + (int)threadIdx.x; k < c1; k += (int)blockDim.x) f(info.x + k);
-				__syncthreads();
+	// f(arbiter, its bodies as kept in lpb) for the unit's arbiters, colour by colour
+	auto sweep_unit = [&](int u, int s, auto f) {
+		int2 info;
+		const int32_t *col;
+		int base = -1;
+		if (s < SCRATCH) {
+			info = linfo[s];
+			col = lcol[s];
+			base = lbase[s];
+		} else {
+			info = S.pinfo[u];
+			col = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
+		}
+		for (int q = 0; q < info.y; q++) {
+			const int c1 = col[q + 1];
+			for (int k = col[q] + (int)threadIdx.x; k < c1; k += (int)blockDim.x) {
+				if (base >= 0) {
+					f(lpos[base + k], lpb[base + k]);
+				} else {
+					const int a = S.order[info.x + k];
+					const Arbiter &arb = S.act[a];
+					f(a, make_int2(is_dynamic(W, arb.ba) ? arb.ba : ~arb.ba, is_dynamic(W, arb.bb) ? arb.bb : ~arb.bb));
+				}
+			}
+			__syncthreads();
+		}
+	};
+	// units a and b (-1: none) have finished `need` passes
+	auto wait_units = [&](int a, int b, int need) {
+		if (threadIdx.x == 0) {
+			if (a >= 0) unit_wait(&S.udone[a], need, W.err);
+			if (b >= 0) unit_wait(&S.udone[b], need, W.err);
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+		}
+		__syncthreads();
+	};
+	auto finish_unit = [&](int u, int passes) {
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+			__hip_atomic_store(&S.udone[u], passes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+	};
+	const bool wide = wide_first < n_act;
+	const bool flow = S.grid_flow && !wide && S.n_jcolours == 0;
+	// sweep p (0: cached impulses) of the arbiters with f, then of the joints with fj
+	auto sweep = [&](int p, auto f, auto fj) {
+		if (flow) {
+			for (int c = g, i = 0; c < n_cols; c += G, i++) {
+				if (p > 0) wait_units(c % GRID_PARTS > 0 ? seam_of(c - 1) : -1, seam_of(c), p);
+				sweep_unit(c, slot(i, 0), f);
+				finish_unit(c, p + 1);
+			}
+			for (int parity = 0; parity < 2; parity++) {
+				for (int c = g, i = 0; c < n_cols; c += G, i++) {
+					const int su = seam_of(c);
+					if (su < 0 || c % GRID_PARTS % 2 != parity) continue;
+					if (parity == 0) wait_units(c, c + 1, p + 1);
+					else wait_units(seam_of(c - 1), seam_of(c + 1), p + 1);
+					sweep_unit(su, slot(i, 1), f);
+					finish_unit(su, p + 1);
+				}
+			}
+		} else {
+			for (int c = g, i = 0; c < n_cols; c += G, i++) sweep_unit(c, slot(i, 0), f);
+			sy.sync();
+			for (int parity = 0; parity < 2; parity++) {
+				for (int c = g, i = 0; c < n_cols; c += G, i++)
+					if (seam_of(c) >= 0 && c % GRID_PARTS % 2 == parity) sweep_unit(seam_of(c), slot(i, 1), f);
+				sy.sync();
+			}
+			if (wide) {
+				if (g == G - 1) sweep_unit(n_units - 1, SCRATCH, f);
+				sy.sync();
 			}
 		}
-		sy.sync();
-	};
-	const int e0 = n_large * GRID_PARTS, o0 = e0 + n_large * EVEN_SEAMS, w0 = o0 + n_large * ODD_SEAMS;
-	auto sweep = [&](auto f, auto fj) {
-		phase(0, e0, f);
-		phase(e0, o0, f);
-		phase(o0, w0, f);
-		if (wide_first < n_act) phase(w0, n_units, f);
 		for (int jc = 0; jc < S.n_jcolours; jc++) {
 			for (int k = S.jcol_off[jc] + sy.sweep_tid(); k < S.jcol_off[jc + 1]; k += sy.stride) fj(W.joints[S.jorder[k]]);
 			sy.sync();
 		}
 	};
 	// cached impulses (cpSpaceStep.c:407-415)
-	sweep([&](int pos) { arb_apply_cached(W, bodies, S.act[S.order[pos]]); },
-	      [&](Joint &j) { joint_apply_cached(W, bodies, j); });
+	sweep(0, [&](int a, int2 e) {
+		arb_apply_cached(W, bodies, S.act[a], e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
+	}, [&](Joint &j) { joint_apply_cached(W, bodies, j); });
 	// the impulse iterations (cpSpaceStep.c:418-427)
 	for (int it = 0; it < S.max_iterations; it++) {
-		sweep([&](int pos) {
-			Arbiter &arb = S.act[S.order[pos]];
-			if (it < W.ws[arb.world].iterations) arb_apply_impulse(W, bodies, arb);
+		sweep(1 + it, [&](int a, int2 e) {
+			Arbiter &arb = S.act[a];
+			if (it < W.ws[arb.world].iterations)
+				arb_apply_impulse(W, bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
 		}, [&](Joint &j) {
 			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);
 		});
@@ -996,6 +1153,7 @@ __global__ void k_uf_init(const SolveBuffers S)
 	const int n_large = S.n_large_worlds;
 	for (int i = q; i < n_large * PART_BINS; i += gridDim.x * blockDim.x) {
 		S.phist[i] = 0;
+		if (i < n_large * GRID_PARTS) S.pcount[i] = 0;
 		if (i < n_large) { S.pbound[2 * i] = ~0ull; S.pbound[2 * i + 1] = 0ull; }
 	}
 	if (q >= S.n_large_bodies) return;
@@ -1152,15 +1310,38 @@ __global__ void __launch_bounds__(PART_BINS) k_part_scan(const SolveBuffers S)
 	h[i] = total > 0 ? (int)(before * GRID_PARTS / total) : 0;
 }
 
-__global__ void k_part_assign(const DevWorld W, const SolveBuffers S)
+// each grid-set body's strip, and its slot among the strip's bodies: the
+// workgroup counts its bodies per strip in LDS and reserves each strip's
+// slots with one atomic.  The large bodies are in body order and a large
+// world has more than a workgroup's bodies, so a workgroup's bodies lie in
+// at most two large worlds.
+constexpr int PART_ASSIGN_THREADS = 1024;
+__global__ void __launch_bounds__(PART_ASSIGN_THREADS) k_part_assign(const DevWorld W, const SolveBuffers S)
 {
+	__shared__ int32_t cnt[2][GRID_PARTS], base[2][GRID_PARTS];
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q >= S.n_large_bodies) return;
-	const int b = S.large_bodies[q];
-	if (!in_grid_set(W, S, b)) return;
-	const int l = S.large_index[W.b_world[b]];
-	const int bin = part_bin((double)W.b_p[b].x, key_x(S.pbound[2 * l]), key_x(S.pbound[2 * l + 1]));
-	S.bpart[b] = S.phist[l * PART_BINS + bin];
+	for (int i = threadIdx.x; i < 2 * GRID_PARTS; i += blockDim.x) cnt[i / GRID_PARTS][i % GRID_PARTS] = 0;
+	const int l0 = S.large_index[W.b_world[S.large_bodies[blockIdx.x * blockDim.x]]];
+	__syncthreads();
+	int b = -1, rel = -1, strip = 0, local = 0;
+	if (q < S.n_large_bodies) {
+		b = S.large_bodies[q];
+		if (in_grid_set(W, S, b)) {
+			const int l = S.large_index[W.b_world[b]];
+			const int bin = part_bin((double)W.b_p[b].x, key_x(S.pbound[2 * l]), key_x(S.pbound[2 * l + 1]));
+			strip = S.phist[l * PART_BINS + bin];
+			S.bpart[b] = strip;
+			rel = l - l0;
+			local = atomicAdd(&cnt[rel][strip], 1);
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < 2 * GRID_PARTS; i += blockDim.x) {
+		const int r = i / GRID_PARTS, st = i % GRID_PARTS;
+		if (cnt[r][st] > 0) base[r][st] = atomicAdd(&S.pcount[(l0 + r) * GRID_PARTS + st], cnt[r][st]);
+	}
+	__syncthreads();
+	if (rel >= 0) S.bslot[b] = base[rel][strip] + local;
 }
 
 // Sort keys of the solved arbiters; the pre-steps of the large worlds'
@@ -1276,7 +1457,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 				k_part_bounds<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 				k_part_hist<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 				k_part_scan<<<S.n_large_worlds, PART_BINS, 0, stream>>>(S);
-				k_part_assign<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
+				k_part_assign<<<(nb + PART_ASSIGN_THREADS - 1) / PART_ASSIGN_THREADS, PART_ASSIGN_THREADS, 0, stream>>>(W, S);
 			});
 		HIP_TRY(hipGetLastError());
 	}

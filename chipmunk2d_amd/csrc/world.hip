@@ -304,7 +304,10 @@ struct cpgpuWorld {
 	std::vector<double> h_joint_grads;  // [nc * 8] parameter adjoints of the last backward pass
 	int32_t *d_replay_n = nullptr;
 	// solve units of the grid set
-	int32_t *d_large_index = nullptr, *phist = nullptr, *bpart = nullptr, *pcol = nullptr;
+	int grid_flow = 1;
+	int32_t *udone = nullptr;
+	int32_t *d_large_index = nullptr, *phist = nullptr, *bpart = nullptr, *bslot = nullptr, *pcount = nullptr;
+	int32_t *pcol = nullptr;
 	unsigned long long *pbound = nullptr;
 	int2 *pinfo = nullptr;
 	int n_jcolours = 0;
@@ -696,6 +699,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		HIP_OR_FAIL(own_alloc(w, w->phist, (size_t)PART_BINS * n_large));
 		HIP_OR_FAIL(own_alloc(w, w->pcol, (size_t)grid_units(n_large) * (CPGPU_MAX_COLOURS + 1)));
 		HIP_OR_FAIL(own_alloc(w, w->pinfo, (size_t)grid_units(n_large)));
+		HIP_OR_FAIL(own_alloc(w, w->udone, (size_t)grid_units(n_large)));
+		HIP_OR_FAIL(own_alloc(w, w->pcount, (size_t)GRID_PARTS * n_large));
 
 		// large worlds' joints: by colour
 		int ncol_large = 0;
@@ -758,6 +763,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_upload(w, w->warb_off, (const int32_t *)nullptr, w->nw + 2));
 	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 2));
 	HIP_OR_FAIL(own_alloc(w, w->bpart, nb));
+	HIP_OR_FAIL(own_alloc(w, w->bslot, nb));
 	HIP_OR_FAIL(own_alloc(w, w->claim, CLAIM_SETS * 2 * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->taken, CLAIM_SETS * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->label, nb));
@@ -853,6 +859,8 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
 	S.large_index = w->d_large_index; S.n_large_worlds = w->n_large_worlds;
 	S.pbound = w->pbound; S.phist = w->phist; S.bpart = w->bpart; S.pcol = w->pcol; S.pinfo = w->pinfo;
+	S.udone = w->udone; S.grid_flow = w->grid_flow;
+	S.bslot = w->bslot; S.pcount = w->pcount;
 	S.rec = nullptr; S.rec_stride = 0; S.jrec = nullptr;
 	return S;
 }
@@ -1206,6 +1214,17 @@ int cpgpuWorldSetTiming(cpgpuWorld *w, int32_t enable)
 	if (rc != CPGPU_OK) return rc;
 	w->timer.enabled = enable != 0;
 	w->timer.acc.clear();
+	return CPGPU_OK;
+}
+
+int cpgpuWorldSetGridSchedule(cpgpuWorld *w, int32_t schedule)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (schedule != CPGPU_GRID_FLOW && schedule != CPGPU_GRID_PHASES)
+		return fail(CPGPU_ERR_INVALID, "unknown grid schedule " + std::to_string(schedule));
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	w->grid_flow = schedule == CPGPU_GRID_FLOW;
 	return CPGPU_OK;
 }
 

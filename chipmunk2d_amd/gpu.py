@@ -26,9 +26,13 @@ EXPORTS = [
     "cpgpuWorldCreate", "cpgpuWorldDestroy", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
     "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
-    "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
+    "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldSetGridSchedule",
+    "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
     "cpgpuWorldBackward", "cpgpuWorldGetJointGrads", "cpgpuGetLastError", "cpgpuVersion",
 ]
+
+
+GRID_FLOW, GRID_PHASES = 0, 1
 
 
 class CpgpuError(RuntimeError):
@@ -54,6 +58,7 @@ def lib():
         L.cpgpuWorldReadArbiters.argtypes = [_vp, ctypes.c_int64, _i64p, _ip, _ip, _ip, _ip, _dp, _dp, _u64p]
         L.cpgpuWorldReadConstraints.argtypes = [_vp, _dp, _ip]
         L.cpgpuWorldSetTiming.argtypes = [_vp, ctypes.c_int32]
+        L.cpgpuWorldSetGridSchedule.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
         L.cpgpuWorldGetSolveCounts.argtypes = [_vp, _i64p, _i64p]
         L.cpgpuWorldGetBuckets.argtypes = [_vp, _ip, _ip, _ip, _ip]
@@ -168,6 +173,10 @@ class GPUWorld:
         mb = np.ascontiguousarray(max_bodies, np.int32)
         th = np.ascontiguousarray(threads, np.int32)
         _check(self.L.cpgpuWorldSetBuckets(self.h, len(mb), ptr(mb, ctypes.c_int32), ptr(th, ctypes.c_int32)))
+
+    def set_grid_schedule(self, schedule):
+        """GRID_FLOW (dependency-driven, the default) or GRID_PHASES (grid barriers between phases)."""
+        _check(self.L.cpgpuWorldSetGridSchedule(self.h, int(schedule)))
 
     def autotune(self, dt, trial_steps=3):
         """Pick each size bucket's workgroup size by timing (advances the world 5 x trial_steps steps)."""

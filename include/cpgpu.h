@@ -231,6 +231,16 @@ int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies
  * timing every bucket's solve at each workgroup size, and keeps the fastest
  * size per bucket (reported in threads_out[CPGPU_MAX_BUCKETS]). */
 int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out);
+/* Schedule of the grid set of spaces with more than 4096 dynamic bodies
+ * (its islands of more than 2048 arbiters or with joints, cut into x-strips
+ * and the seams between them).  CPGPU_GRID_FLOW (the default): without
+ * joints or arbiters spanning non-neighbouring strips, every strip and seam
+ * waits only for the units it shares bodies with; CPGPU_GRID_PHASES: grid
+ * barriers between strips, even seams and odd seams.  Results are the same
+ * bit for bit; the schedule only changes the time. */
+#define CPGPU_GRID_FLOW 0
+#define CPGPU_GRID_PHASES 1
+int cpgpuWorldSetGridSchedule(cpgpuWorld *world, int32_t schedule);
 
 /* ---- differentiation (no counterpart in the reference: cpFloat is a plain
  * double there).  cpgpuWorldTapeBegin starts recording the rollout: every

@@ -137,6 +137,41 @@ def test_grid_strips_chain_with_joints_bit_exact():
     assert np.array_equal(jg, jo)
 
 
+@pytest.mark.parametrize("schedule", ["flow", "phases"])
+def test_grid_strips_flat_pile_schedules_bit_exact(schedule):
+    """A wide flat pile (no joints, no arbiter spanning non-neighbouring
+    strips): the dependency-driven schedule of strips and seams, and the
+    phase-barrier one, both bit-exact with the oracle."""
+    from chipmunk2d_amd.gpu import GRID_FLOW, GRID_PHASES
+    pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=400.0, seed=15).pack()
+    g = GPUWorld(pw)
+    g.set_grid_schedule(GRID_FLOW if schedule == "flow" else GRID_PHASES)
+    o = OracleWorld(pw)
+    seen = []
+    for s in range(160):
+        g.step(DT)
+        o.step(DT)
+        if s % 4 == 3 or s == 159:
+            assert_bodies_equal(g, o, "step %d" % s)
+        if s >= 100 and s % 20 == 19:
+            u = o.units()
+            seen.append((len(np.unique(u[(u > 0) & (u <= 64)])), int(np.count_nonzero((u > 64) & (u < 128))),
+                         int(np.count_nonzero(u == 128))))
+    assert_arbiters_equal(g, o)
+    print(seen)
+    assert all(st == 64 and seams > 500 and wide == 0 for st, seams, wide in seen), seen
+    gs, os_ = g.stats(), o.stats()
+    for k in ("candidate_pairs", "arbiters", "contacts", "cached_arbiters", "arbiter_colours"):
+        assert gs[k] == os_[k], (k, gs[k], os_[k])
+
+
+def test_grid_schedule_rejects_unknown_values():
+    from chipmunk2d_amd.gpu import CpgpuError
+    g = GPUWorld(pyramid_210(rows=3).pack())
+    with pytest.raises(CpgpuError):
+        g.set_grid_schedule(5)
+
+
 def test_mixed_batch_grid_and_workgroup_solves_bit_exact():
     """Two large spaces (grid sets in strips) and small spaces (a workgroup each) in one step."""
     rng = Rng(0x1A46E)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <map>
 #include <string>
 #include <vector>
@@ -15,9 +16,11 @@ namespace CPGPU_NS {
 // ---- per-kernel device timing with HIP events on the launch stream ----
 struct KernelTimer {
 	bool enabled = false;
+	bool solve_only = false;   // record only the solve kernels (names starting "solver" or "reverse")
 	struct Rec { const char *name; hipEvent_t a, b; };
 	std::vector<Rec> pending;
 	std::vector<hipEvent_t> pool;
+	std::vector<int> open;     // per open region: its record in pending, or -1 when not recorded
 	std::map<std::string, std::pair<double, long>> acc;   // name -> (ms, launches)
 	hipEvent_t take() {
 		if (pool.empty()) { hipEvent_t e; (void)hipEventCreate(&e); return e; }
@@ -25,13 +28,20 @@ struct KernelTimer {
 	}
 	void begin(const char *name, hipStream_t s) {
 		if (!enabled) return;
+		if (solve_only && strncmp(name, "solver", 6) != 0 && strncmp(name, "reverse", 7) != 0) {
+			open.push_back(-1);
+			return;
+		}
 		Rec r; r.name = name; r.a = take(); r.b = take();
 		(void)hipEventRecord(r.a, s);
+		open.push_back((int)pending.size());
 		pending.push_back(r);
 	}
 	void end(hipStream_t s) {
-		if (!enabled || pending.empty()) return;
-		(void)hipEventRecord(pending.back().b, s);
+		if (!enabled || open.empty()) return;
+		const int k = open.back();
+		open.pop_back();
+		if (k >= 0) (void)hipEventRecord(pending[k].b, s);
 	}
 	// after a stream sync: fold finished records into the totals
 	void collect() {

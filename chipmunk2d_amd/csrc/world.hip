@@ -1212,7 +1212,10 @@ int cpgpuWorldSetTiming(cpgpuWorld *w, int32_t enable)
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	int rc = cpgpuWorldSync(w);
 	if (rc != CPGPU_OK) return rc;
-	w->timer.enabled = enable != 0;
+	if (enable < CPGPU_TIMING_OFF || enable > CPGPU_TIMING_SOLVE)
+		return fail(CPGPU_ERR_INVALID, "unknown timing mode " + std::to_string(enable));
+	w->timer.enabled = enable != CPGPU_TIMING_OFF;
+	w->timer.solve_only = enable == CPGPU_TIMING_SOLVE;
 	w->timer.acc.clear();
 	return CPGPU_OK;
 }

@@ -148,8 +148,10 @@ class GPUWorld:
         _check(self.L.cpgpuWorldReadConstraints(self.h, ptr(jacc, ctypes.c_double), ptr(col, ctypes.c_int32)))
         return jacc, col
 
-    def set_timing(self, on=True):
-        _check(self.L.cpgpuWorldSetTiming(self.h, 1 if on else 0))
+    def set_timing(self, on=True, solve_only=False):
+        """Time kernel launches with HIP events: all of them, or (solve_only) only the solver's and
+        the adjoint's."""
+        _check(self.L.cpgpuWorldSetTiming(self.h, (2 if solve_only else 1) if on else 0))
 
     def timing(self):
         """dict kernel name -> (total ms, launches) since timing was enabled."""

@@ -206,7 +206,13 @@ int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour);
 
 /* ---- profiling ---- */
 
-/* Record HIP events around every kernel launch (0 = off, the default). */
+/* Record HIP events around kernel launches: CPGPU_TIMING_OFF (the default),
+ * CPGPU_TIMING_ALL (every launch; the events cost a few microseconds each),
+ * CPGPU_TIMING_SOLVE (only the velocity solver's and the adjoint's launches:
+ * cheap enough to time them inside a measured run). */
+#define CPGPU_TIMING_OFF 0
+#define CPGPU_TIMING_ALL 1
+#define CPGPU_TIMING_SOLVE 2
 int cpgpuWorldSetTiming(cpgpuWorld *world, int32_t enable);
 /* Per-kernel accumulated device time in ms and launch counts since timing was
  * enabled.  names is filled with a '\n'-separated list of kernel names.

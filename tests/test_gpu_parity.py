@@ -165,6 +165,23 @@ def test_grid_strips_flat_pile_schedules_bit_exact(schedule):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
 
 
+def test_solve_only_timing_records_only_solver_launches():
+    g = GPUWorld(concat([pyramid_210(rows=6).pack(), random_world(n_circles=2600, n_polys=2600, n_segments=400,
+                                                                  extent=140.0, seed=12).pack()]))
+    g.set_timing(True, solve_only=True)
+    g.step(DT, 3)
+    t = g.timing()
+    assert t and all(k.startswith("solver") for k in t), t
+    assert t["solver"][1] == 3
+    g.set_timing(True)
+    g.step(DT, 2)
+    t = g.timing()
+    assert "narrowphase" in t and t["solver"][1] == 2
+    from chipmunk2d_amd.gpu import CpgpuError, _check
+    with pytest.raises(CpgpuError):
+        _check(g.L.cpgpuWorldSetTiming(g.h, 3))
+
+
 def test_grid_schedule_rejects_unknown_values():
     from chipmunk2d_amd.gpu import CpgpuError
     g = GPUWorld(pyramid_210(rows=3).pack())

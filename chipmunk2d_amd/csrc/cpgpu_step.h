@@ -65,12 +65,9 @@ struct GridBarrier {
 // barriers; grid barriers separate the phases.  The solve order is
 // therefore: the strips, the even seams, the odd seams (each in its colour
 // order), then the wide set in its colour order.
-// Sorted arbiter keys: small world w -> w; large-world island with root
-// body r -> 1 << 40 | r; strip q of the l-th large world -> 2 << 40 | l << 8 | q;
-// its seam q -> 2 << 40 | l << 8 | (GRID_PARTS + q); wide set -> WIDE_KEY, the greatest key.
-#define ISLAND_KEY (1ull << 40)
-#define GRID_KEY (2ull << 40)
-#define WIDE_KEY (GRID_KEY | ((1ull << 40) - 1))
+// Sorted arbiter keys, dense so the radix sort has few digits: small world
+// w -> w; large-world island with root body r -> n_worlds + r; unit u of the
+// grid set (grid_units' order) -> n_worlds + n_bodies + u.
 #define GRID_PARTS 64
 #define GRID_SEAMS (GRID_PARTS - 1)
 #define EVEN_SEAMS ((GRID_SEAMS + 1) / 2)
@@ -109,6 +106,7 @@ struct SolveBuffers {
 	uint64_t *keys;            // [cap] the sorted keys
 	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world (the last: islands)
 	int32_t *large_first;      // [2] first sorted position of the grid set, of the wide set
+	int n_worlds;
 	// islands of the large worlds
 	int32_t *label;            // [n_bodies] union-find parent, then island root (the least body index)
 	int32_t *icount;           // [n_bodies] per root: arbiters of the island

@@ -819,18 +819,20 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 	}
 }
 
-// Key of grid-set unit u (the strips, even seams and odd seams of every large
-// world, then the wide set) and the claim arrays of its colouring.
-__device__ __forceinline__ uint64_t unit_key(int u, int n_large)
+// Sort key of grid-set unit u (the strips, even seams and odd seams of every
+// large world, then the wide set), the unit of an arbiter whose dynamic
+// bodies lie in strips pa and pb (-1: none) of the l-th large world, and the
+// claim arrays of a unit's colouring.
+__device__ __forceinline__ uint64_t unit_key(const SolveBuffers &S, int u)
 {
-	if (u < n_large * GRID_PARTS) return GRID_KEY | ((uint64_t)(u / GRID_PARTS) << 8) | (uint64_t)(u % GRID_PARTS);
-	u -= n_large * GRID_PARTS;
-	if (u < n_large * EVEN_SEAMS)
-		return GRID_KEY | ((uint64_t)(u / EVEN_SEAMS) << 8) | (uint64_t)(GRID_PARTS + 2 * (u % EVEN_SEAMS));
-	u -= n_large * EVEN_SEAMS;
-	if (u < n_large * ODD_SEAMS)
-		return GRID_KEY | ((uint64_t)(u / ODD_SEAMS) << 8) | (uint64_t)(GRID_PARTS + 1 + 2 * (u % ODD_SEAMS));
-	return WIDE_KEY;
+	return (uint64_t)S.n_worlds + (uint64_t)S.n_bodies + (uint64_t)u;
+}
+__device__ __forceinline__ int unit_of(int l, int pa, int pb, int n_large)
+{
+	if (pa < 0 || pb < 0 || pa == pb) return l * GRID_PARTS + max(pa, pb);
+	if (abs(pa - pb) != 1) return grid_units(n_large) - 1;
+	const int q = min(pa, pb), e0 = n_large * GRID_PARTS;
+	return q % 2 == 0 ? e0 + l * EVEN_SEAMS + q / 2 : e0 + n_large * EVEN_SEAMS + l * ODD_SEAMS + q / 2;
 }
 __device__ __forceinline__ int unit_claims(int u, int n_large)
 {
@@ -904,9 +906,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 
 	// ---- every unit coloured by its workgroup ----
 	auto colour_unit = [&](int u, int s) {
-		const uint64_t key = unit_key(u, n_large);
+		const uint64_t key = unit_key(S, u);
 		const int start = lower_bound_u64(S.keys, grid_first, n_act, key);
-		const int end = key == WIDE_KEY ? n_act : lower_bound_u64(S.keys, start, n_act, key + 1);
+		const int end = u == n_units - 1 ? n_act : lower_bound_u64(S.keys, start, n_act, key + 1);
 		int32_t *col = lcol[s];
 		int nc = 0;
 		if (end > start) {
@@ -1362,15 +1364,12 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 		} else {
 			const int r = island_of(W, S, arb.ba, arb.bb);
 			if (island_on_workgroup(S, r)) {
-				key = ISLAND_KEY | (unsigned long long)r;
+				key = (unsigned long long)S.n_worlds + (unsigned long long)r;
 			} else {
 				// the strip of its dynamic bodies, the seam of two neighbouring strips, or the wide set
 				const int pa = is_dynamic(W, arb.ba) ? S.bpart[arb.ba] : -1;
 				const int pb = is_dynamic(W, arb.bb) ? S.bpart[arb.bb] : -1;
-				const unsigned long long l = (unsigned long long)S.large_index[w] << 8;
-				if (pa < 0 || pb < 0 || pa == pb) key = GRID_KEY | l | (unsigned long long)max(pa, pb);
-				else if (abs(pa - pb) == 1) key = GRID_KEY | l | (unsigned long long)(GRID_PARTS + min(pa, pb));
-				else key = WIDE_KEY;
+				key = unit_key(S, unit_of(S.large_index[w], pa, pb, S.n_large_worlds));
 			}
 			arb_prestep(W, arb);
 		}
@@ -1412,8 +1411,8 @@ __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 	const int w = blockIdx.x * blockDim.x + threadIdx.x;
 	if (w <= n_worlds) S.warb_off[w] = lower_bound_u64(S.keys, 0, n, (uint64_t)w);
 	if (w == n_worlds + 1) {
-		S.large_first[0] = lower_bound_u64(S.keys, 0, n, GRID_KEY);
-		S.large_first[1] = lower_bound_u64(S.keys, 0, n, WIDE_KEY);
+		S.large_first[0] = lower_bound_u64(S.keys, 0, n, unit_key(S, 0));
+		S.large_first[1] = lower_bound_u64(S.keys, 0, n, unit_key(S, grid_units(S.n_large_worlds) - 1));
 	}
 }
 
@@ -1469,8 +1468,11 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
 		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
-		// keys: a small world's index, ISLAND_KEY | island root, a grid-set unit's key (below bit 42)
-		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, 42, sortbuf, stream, timer));
+		// keys below n_worlds + n_bodies + grid_units
+		const uint64_t key_end = (uint64_t)n_worlds + (uint64_t)S.n_bodies + (uint64_t)grid_units(S.n_large_worlds);
+		int key_bits = 1;
+		while ((1ull << key_bits) < key_end) key_bits++;
+		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, key_bits, sortbuf, stream, timer));
 	}
 	if (large && nb > 0) {
 		TIMED(timer, "island_integrate", stream,

@@ -860,6 +860,7 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.large_index = w->d_large_index; S.n_large_worlds = w->n_large_worlds;
 	S.pbound = w->pbound; S.phist = w->phist; S.bpart = w->bpart; S.pcol = w->pcol; S.pinfo = w->pinfo;
 	S.udone = w->udone; S.grid_flow = w->grid_flow;
+	S.n_worlds = w->nw;
 	S.bslot = w->bslot; S.pcount = w->pcount;
 	S.rec = nullptr; S.rec_stride = 0; S.jrec = nullptr;
 	return S;

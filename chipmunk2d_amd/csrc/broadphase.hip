@@ -23,8 +23,8 @@ namespace {
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 4;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
-constexpr int MORTON_BITS = 22;
-constexpr int WORLD_SHIFT = 44;
+constexpr int MORTON_BITS = 16;   // per axis: 65536 cells across a world's extent
+constexpr int WORLD_SHIFT = 2 * MORTON_BITS;
 constexpr int TRAVERSAL_STACK = 48;
 
 // order-preserving integer image of a box coordinate (as a double in both builds)
@@ -715,7 +715,7 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 		                                                (unsigned long long *)sortbuf.orand);
 	});
 	HIP_TRY(hipGetLastError());
-	// world index above the 44 Morton bits
+	// world index above the 2 * MORTON_BITS Morton bits
 	int world_bits = 0;
 	while ((1 << world_bits) < n_worlds) world_bits++;
 	HIP_TRY(radix_sort_pairs(t.keys, t.leaf_shape, n, nullptr, WORLD_SHIFT + world_bits, sortbuf, stream, timer));

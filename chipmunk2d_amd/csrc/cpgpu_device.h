@@ -297,7 +297,6 @@ struct DevWorld {
 #define CPGPU_ERR_BARRIER_TIMEOUT 16
 #define CPGPU_ERR_TABLE_FULL 32
 #define CPGPU_ERR_EPA_HULL 64
-#define CPGPU_ERR_ISLANDS 128
 #define CPGPU_ERR_ADJOINT 256
 
 CPD void raise_err(int32_t *err, int32_t code) { atomicOr(err, code); }

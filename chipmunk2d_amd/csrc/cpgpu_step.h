@@ -46,14 +46,9 @@ struct GridBarrier {
 // Worlds are solved in one of two ways, chosen per world by size:
 //   small worlds: one workgroup each (k_solve_worlds), every synchronisation
 //     a workgroup barrier;
-//   large worlds: split each step into islands, the connected components of
-//     their dynamic bodies under this step's arbiters and the joints.
-//     Islands are body-disjoint, so each can be solved on its own in its
-//     colour order with the same result as the whole set.  Joint-free
-//     islands of at most island_max arbiters take a workgroup each
-//     (k_solve_worlds again); the rest form the grid set, solved together
-//     on a co-resident grid (k_solve_grid).
-// The grid set of each large world is cut into GRID_PARTS strips along x
+//   large worlds: every arbiter and joint in the grid set, solved on a
+//     co-resident grid (k_solve_grid).
+// The grid set of each large world (its dynamic bodies) is cut into GRID_PARTS strips along x
 // with equal numbers of its dynamic bodies (quantiles of a PART_BINS-bin
 // histogram of their x between the set's least and greatest x).  An arbiter
 // whose dynamic bodies lie in one strip belongs to that strip; one joining
@@ -66,15 +61,14 @@ struct GridBarrier {
 // therefore: the strips, the even seams, the odd seams (each in its colour
 // order), then the wide set in its colour order.
 // Sorted arbiter keys, dense so the radix sort has few digits: small world
-// w -> w; large-world island with root body r -> n_worlds + r; unit u of the
-// grid set (grid_units' order) -> n_worlds + n_bodies + u.
+// w -> w; unit u of the grid set (grid_units' order) -> n_worlds + u.
 #define GRID_PARTS 64
 #define GRID_SEAMS (GRID_PARTS - 1)
 #define EVEN_SEAMS ((GRID_SEAMS + 1) / 2)
 #define ODD_SEAMS (GRID_SEAMS / 2)
 #define PART_BINS 1024
-// Claim arrays for concurrent colourings: strips and workgroup-solved units,
-// even seams, odd seams, the wide set.
+// Claim arrays for concurrent colourings: strips and small worlds, even
+// seams, odd seams, the wide set.
 #define CLAIM_SETS 4
 
 // Pre-state of one arbiter's impulse application in one sweep (rollout
@@ -104,16 +98,9 @@ struct SolveBuffers {
 	int32_t *order;            // [cap] active arbiters in colour order (per workgroup solve: its own segment)
 	int32_t *arb_by_world;     // [cap] active arbiters sorted by key; nullptr for a single small space
 	uint64_t *keys;            // [cap] the sorted keys
-	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world (the last: islands)
+	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world (the last: the grid set)
 	int32_t *large_first;      // [2] first sorted position of the grid set, of the wide set
 	int n_worlds;
-	// islands of the large worlds
-	int32_t *label;            // [n_bodies] union-find parent, then island root (the least body index)
-	int32_t *icount;           // [n_bodies] per root: arbiters of the island
-	int32_t *ijoint;           // [n_bodies] per root: the island holds a joint
-	int32_t *groups;           // [cap] first sorted position of each workgroup-solved island
-	int32_t *n_groups;         // [2] island count, work counter
-	int island_max;
 	int n_large_joints;        // joints of large worlds (jorder[0..n))
 	const uint8_t *world_small;
 	unsigned long long *claim; // [CLAIM_SETS][2][n_bodies]
@@ -152,11 +139,12 @@ struct SolveBuffers {
 	JointRec *jrec;
 };
 
-// Group the solved arbiters: islands of the large worlds (union-find), the
-// pre-steps and velocity integration of the workgroup-solved islands, then
-// the arbiters sorted by key with the small worlds' ranges and the island
-// list.  Skipped for a single small space.  n_act bounds the solved arbiters
-// from above (launch sizes); the kernels read the count *S.n_act.
+// Group the solved arbiters: the strips of the large worlds' grid sets, the
+// large worlds' arbiter pre-steps (and, without joints, their bodies'
+// velocity integration), then the arbiters sorted by key with the small
+// worlds' and the grid units' ranges.  Skipped for a single small space.
+// n_act bounds the solved arbiters from above (launch sizes); the kernels
+// read the count *S.n_act.
 hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
                         RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer);
 // solve units of the grid set: the strips, even seams and odd seams of every
@@ -170,8 +158,8 @@ __host__ __device__ inline int grid_units(int n_large_worlds)
 // size and room for the largest of its worlds' bodies in LDS (0: the bodies
 // stay in global memory).  Worlds are listed largest first.
 static_assert(CPGPU_MAX_BUCKETS == 4, "bucket count of the C ABI");
-// solve units counted in DevWorld::totals: the buckets, the islands, the grid
-static_assert(CPGPU_SOLVE_UNITS == CPGPU_MAX_BUCKETS + 2, "solve units: buckets, islands, grid");
+// solve units counted in DevWorld::totals: the buckets, the grid
+static_assert(CPGPU_SOLVE_UNITS == CPGPU_MAX_BUCKETS + 1, "solve units: buckets, grid");
 struct WorldBucket {
 	int max_bodies;            // worlds with at most this many bodies (static included)
 	int threads;               // 64, 128, 256, 512 or 1024
@@ -182,12 +170,10 @@ struct WorldBucket {
 // Launch the solvers: colouring, colour sort, pre-steps, velocity
 // integration, cached impulses and the impulse iterations.  Each bucket of
 // small worlds runs on its own auxiliary stream (all independent worlds);
-// grid > 0 launches the grid
-// solve with that many workgroups, on the main stream, while island_blocks
-// workgroups take the islands on aux[CPGPU_MAX_BUCKETS]; the main stream
-// waits for all of them.  events: CPGPU_MAX_BUCKETS + 2.
+// grid > 0 launches the grid solve with that many workgroups on the main
+// stream; the main stream waits for all of them.  events: CPGPU_MAX_BUCKETS + 2.
 hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBucket *buckets, int n_buckets,
-                       int island_blocks, int grid, hipStream_t stream, const hipStream_t *aux,
+                       int grid, hipStream_t stream, const hipStream_t *aux,
                        const hipEvent_t *events, KernelTimer *timer);
 // Largest dynamic LDS a small-world workgroup may take next to its static LDS.
 int solver_max_lds_bytes();

@@ -405,12 +405,6 @@ __device__ __forceinline__ void integrate_velocity(const DevWorld &W, int b)
 	W.b_t[b] = 0.0;
 }
 
-// an island of a large world is solved by a workgroup (else on the grid)
-__device__ __forceinline__ bool island_on_workgroup(const SolveBuffers &S, int root)
-{
-	return S.icount[root] <= S.island_max && S.ijoint[root] == 0;
-}
-
 // colouring priority: a bijective hash of the shape pair's indices within its
 // own space, so a space is coloured (and stepped) identically alone or in a batch
 __device__ __forceinline__ unsigned long long arbiter_priority(const DevWorld &W, const Arbiter &arb)
@@ -450,14 +444,11 @@ struct SolveSet {
 	int32_t *col_off;          // [CPGPU_MAX_COLOURS + 1]
 	int32_t *col_cursor;       // [CPGPU_MAX_COLOURS]
 	int32_t *left;
-	const int32_t *bodies;     // bodies to integrate: list, or nullptr for the range [body0, body0 + nb)
-	int body0, nb;
+	int body0, nb;             // bodies to integrate: [body0, body0 + nb)
 	const int32_t *jorder;     // joints in colour order
 	const int32_t *jcol_off;   // [njcol + 1]
 	int njcol;
 	int iterations;            // sweeps (elements of worlds with fewer sweeps skip the extra ones)
-	bool prestep;              // arbiter pre-steps here (else done before, with the bodies' integration)
-	bool island_filter;        // integrate only bodies whose island is not solved by a workgroup
 	BodyVel *lds_bodies;       // LDS room for lds_cap bodies, or nullptr
 	int lds_cap;
 	int unit;                  // solve unit whose contacts / bodies are counted (W.totals[2 + 2 unit ...])
@@ -708,8 +699,7 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
 	const int n_colours = colour_set(W, S, T);
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----
-	if (T.prestep)
-		for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);
+	for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);
 	sy.sync();
 	for (int jc = 0; jc < T.njcol; jc++) {
 		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) joint_prestep(W, W.joints[T.jorder[k]]);
@@ -718,15 +708,13 @@ __device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveS
 	// ---- velocity integration (cpSpaceStep.c:399-404, cpBody.c:493-509) ----
 	unsigned long long n_int = 0;
 	for (int q = tid; q < T.nb; q += stride) {
-		const int b = T.bodies ? T.bodies[q] : T.body0 + q;
-		if (T.island_filter && island_on_workgroup(S, S.label[b])) continue;
-		integrate_velocity(W, b);
+		integrate_velocity(W, T.body0 + q);
 		n_int++;
 	}
 	wave_add(&W.totals[3 + 2 * T.unit], n_int);
 	sy.sync();
 	// ---- impulses: on a copy of the bodies in LDS when the set is one world's range that fits ----
-	if (T.lds_bodies && !T.bodies && T.nb <= T.lds_cap) {
+	if (T.lds_bodies && T.nb <= T.lds_cap) {
 		for (int q = tid; q < T.nb; q += stride) T.lds_bodies[q] = W.b_vel[T.body0 + q];
 		sy.sync();
 		solve_impulses(W, S, T, sy, n_colours, LdsBodies{T.lds_bodies, T.body0});
@@ -747,31 +735,25 @@ __device__ __forceinline__ int lower_bound_u64(const uint64_t *keys, int lo, int
 	return lo;
 }
 
-// One workgroup per small world, then the remaining workgroups take the
-// islands of the large worlds from a work counter: every synchronisation is a
-// workgroup barrier.
+// One workgroup per small world: every synchronisation is a workgroup barrier.
 template <int THREADS>
 __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, const SolveBuffers S,
-                                                            const int32_t *worlds, int n_small, int lds_cap,
-                                                            int unit)
+                                                            const int32_t *worlds, int lds_cap, int unit)
 {
 	extern __shared__ BodyVel lds_bodies[];
 	__shared__ int32_t col_off[CPGPU_MAX_COLOURS + 1];
 	__shared__ int32_t col_cursor[CPGPU_MAX_COLOURS];
 	__shared__ int32_t left;
-	__shared__ int32_t next;
 	SolveSet T = {};
 	T.col_off = col_off;
 	T.col_cursor = col_cursor;
 	T.left = &left;
 	T.cset = 0;
-	T.bodies = nullptr;
-	T.island_filter = false;
 	T.lds_bodies = lds_cap > 0 ? lds_bodies : nullptr;
 	T.lds_cap = lds_cap;
 	T.unit = unit;
 	BlockSync sy{(int)threadIdx.x, (int)blockDim.x};
-	if ((int)blockIdx.x < n_small) {
+	{
 		const int w = worlds[blockIdx.x];
 		if (S.arb_by_world) {
 			T.list = S.arb_by_world + S.warb_off[w];
@@ -789,32 +771,6 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 		T.jcol_off = S.jw_col + S.jw_cbase[w];
 		T.njcol = S.jw_ncol[w];
 		T.iterations = W.ws[w].iterations;
-		T.prestep = true;
-		solve_set(W, S, T, sy);
-		return;
-	}
-	// islands: pre-steps and velocity integration were done before
-	const int n_groups = S.n_groups[0];
-	const int island_end = S.large_first[0];
-	T.body0 = 0;
-	T.nb = 0;
-	T.jorder = nullptr;
-	T.jcol_off = nullptr;
-	T.njcol = 0;
-	T.prestep = false;
-	while (true) {
-		if (threadIdx.x == 0) next = atomicAdd(&S.n_groups[1], 1);
-		__syncthreads();
-		const int g = next;
-		__syncthreads();
-		if (g >= n_groups) break;
-		const int start = S.groups[g];
-		const int end = lower_bound_u64(S.keys, start, island_end, S.keys[start] + 1);
-		T.list = S.arb_by_world + start;
-		T.n = end - start;
-		T.order = S.order + start;
-		T.order_base = start;
-		T.iterations = W.ws[S.act[T.list[0]].world].iterations;
 		solve_set(W, S, T, sy);
 	}
 }
@@ -825,7 +781,7 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 // claim arrays of a unit's colouring.
 __device__ __forceinline__ uint64_t unit_key(const SolveBuffers &S, int u)
 {
-	return (uint64_t)S.n_worlds + (uint64_t)S.n_bodies + (uint64_t)u;
+	return (uint64_t)S.n_worlds + (uint64_t)u;
 }
 __device__ __forceinline__ int unit_of(int l, int pa, int pb, int n_large)
 {
@@ -892,7 +848,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	const int n_large = S.n_large_worlds;
 	const int n_units = grid_units(n_large), n_cols = n_large * GRID_PARTS;
 	const int e0 = n_cols, o0 = e0 + n_large * EVEN_SEAMS;
-	const int stat = CPGPU_MAX_BUCKETS + 1;
+	const int stat = CPGPU_MAX_BUCKETS;
 	const int g = blockIdx.x, G = gridDim.x;
 	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err, 0u};
 
@@ -973,9 +929,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	if (S.n_large_joints > 0) {
 		unsigned long long n_int = 0;
 		for (int q = sy.tid; q < S.n_large_bodies; q += sy.stride) {
-			const int b = S.large_bodies[q];
-			if (island_on_workgroup(S, S.label[b])) continue;
-			integrate_velocity(W, b);
+			integrate_velocity(W, S.large_bodies[q]);
 			n_int++;
 		}
 		wave_add(&W.totals[3 + 2 * stat], n_int);
@@ -1103,147 +1057,25 @@ __device__ __forceinline__ int part_bin(double x, double lo, double hi)
 	return t >= (double)PART_BINS ? PART_BINS - 1 : (int)t;
 }
 
+// the grid set: the dynamic bodies of the large worlds
 __device__ __forceinline__ bool in_grid_set(const DevWorld &W, const SolveBuffers &S, int b)
 {
-	return is_dynamic(W, b) && !island_on_workgroup(S, S.label[b]);
+	return is_dynamic(W, b);
 }
 
-// ---- grouping the solved arbiters: islands and keys ----
+// ---- grouping the solved arbiters: strips and keys ----
 
 constexpr int GROUP_BLOCKS = 1024;
 
-// union-find over body indices; a root is hooked under the smaller root, so
-// every island's root is its least body index.
-//
-// The walks read and halve paths through this XCD's caches (workgroup-scope
-// relaxed atomics): a stale parent is still an ancestor, since a pointer only
-// ever moves to an ancestor, so a walk still ends at a body that was a root.
-// Only the hook is a device-scope compare-and-swap; when it finds that the
-// root was hooked meanwhile it returns the fresh parent and the walk goes on
-// from there.  Device-scope loads would go past L2 on every step of a walk.
-__device__ __forceinline__ int uf_find(int32_t *parent, int x)
-{
-	while (true) {
-		const int p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-		if (p == x) return x;
-		const int gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-		// path halving: x is no root, so only this pointer to an ancestor changes
-		if (gp != p) __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-		x = gp;
-	}
-}
-
-__device__ void uf_unite(int32_t *parent, int a, int b)
-{
-	while (true) {
-		a = uf_find(parent, a);
-		b = uf_find(parent, b);
-		if (a == b) return;
-		if (a < b) { const int t = a; a = b; b = t; }
-		int expected = a;
-		if (__hip_atomic_compare_exchange_strong(&parent[a], &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-		                                         __HIP_MEMORY_SCOPE_AGENT))
-			return;
-		a = expected;   // a was hooked meanwhile: continue from its parent
-	}
-}
-
-__global__ void k_uf_init(const SolveBuffers S)
+// clear the strips' bounds, histograms and body counts
+__global__ void k_part_init(const SolveBuffers S)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q == 0) { S.n_groups[0] = 0; S.n_groups[1] = 0; }
 	const int n_large = S.n_large_worlds;
 	for (int i = q; i < n_large * PART_BINS; i += gridDim.x * blockDim.x) {
 		S.phist[i] = 0;
 		if (i < n_large * GRID_PARTS) S.pcount[i] = 0;
 		if (i < n_large) { S.pbound[2 * i] = ~0ull; S.pbound[2 * i + 1] = 0ull; }
-	}
-	if (q >= S.n_large_bodies) return;
-	const int b = S.large_bodies[q];
-	S.label[b] = b;
-	S.icount[b] = 0;
-	S.ijoint[b] = 0;
-}
-
-// edges: arbiters and joints of large worlds joining two dynamic bodies
-__global__ void k_uf_link(const DevWorld W, const SolveBuffers S)
-{
-	const int n = *S.n_act;
-	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n + S.n_large_joints; k += gridDim.x * blockDim.x) {
-		int a, b;
-		if (k < n) {
-			const Arbiter &arb = S.act[k];
-			if (S.world_small[arb.world]) continue;
-			a = arb.ba; b = arb.bb;
-		} else {
-			const Joint &j = W.joints[S.jorder[k - n]];
-			a = j.a; b = j.b;
-		}
-		if (is_dynamic(W, a) && is_dynamic(W, b)) uf_unite(S.label, a, b);
-	}
-}
-
-// Every body's label becomes its root.  The walk only reads: a path-halving
-// store here could overwrite a label another lane has just set to its root
-// with a non-root ancestor.  Concurrent label writes are roots, so every
-// walk still ends at the root.
-__global__ void k_uf_flatten(const DevWorld W, const SolveBuffers S)
-{
-	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q >= S.n_large_bodies) return;
-	const int b = S.large_bodies[q];
-	int x = b;
-	while (true) {
-		const int p = __hip_atomic_load(&S.label[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-		if (p == x) break;
-		x = p;
-	}
-	__hip_atomic_store(&S.label[b], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-__device__ __forceinline__ int island_of(const DevWorld &W, const SolveBuffers &S, int a, int b)
-{
-	return S.label[is_dynamic(W, a) ? a : b];
-}
-
-// add 1 to icount[root] for every lane with root >= 0: lanes of a wave with
-// the same root (one large island: most of them) combine into one atomic
-__device__ __forceinline__ void count_by_root(int32_t *icount, int root)
-{
-	uint64_t todo = __ballot(root >= 0);
-	while (todo) {
-		const int leader = __ffsll((long long)todo) - 1;
-		const int r = __shfl(root, leader);
-		const uint64_t same = __ballot(root == r) & todo;
-		if (__lane_id() == leader) atomicAdd(&icount[r], (int)__popcll(same));
-		todo &= ~same;
-	}
-}
-
-__global__ void k_island_count(const DevWorld W, const SolveBuffers S)
-{
-	const int n = *S.n_act;
-	const int total = n + S.n_large_joints;
-	// every lane of a wave runs the same number of passes (the wave-wide ballots)
-	const int span = gridDim.x * blockDim.x;
-	const int passes = (total + span - 1) / span;
-	for (int p = 0; p < passes; p++) {
-		const int k = p * span + blockIdx.x * blockDim.x + threadIdx.x;
-		int root = -1;
-		if (k < n) {
-			const Arbiter &arb = S.act[k];
-			if (!S.world_small[arb.world]) {
-				root = island_of(W, S, arb.ba, arb.bb);
-				// both bodies of an arbiter must share the island: islands are solved concurrently
-				if (is_dynamic(W, arb.ba) && is_dynamic(W, arb.bb) && S.label[arb.ba] != S.label[arb.bb])
-					raise_err(W.err, CPGPU_ERR_ISLANDS);
-			}
-		}
-		count_by_root(S.icount, root);
-		if (k >= n && k < total) {
-			const Joint &j = W.joints[S.jorder[k - n]];
-			if (is_dynamic(W, j.a) || is_dynamic(W, j.b)) S.ijoint[island_of(W, S, j.a, j.b)] = 1;
-		}
 	}
 }
 
@@ -1362,15 +1194,10 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 		if (S.world_small[w]) {
 			key = (unsigned long long)w;
 		} else {
-			const int r = island_of(W, S, arb.ba, arb.bb);
-			if (island_on_workgroup(S, r)) {
-				key = (unsigned long long)S.n_worlds + (unsigned long long)r;
-			} else {
-				// the strip of its dynamic bodies, the seam of two neighbouring strips, or the wide set
-				const int pa = is_dynamic(W, arb.ba) ? S.bpart[arb.ba] : -1;
-				const int pb = is_dynamic(W, arb.bb) ? S.bpart[arb.bb] : -1;
-				key = unit_key(S, unit_of(S.large_index[w], pa, pb, S.n_large_worlds));
-			}
+			// the strip of its dynamic bodies, the seam of two neighbouring strips, or the wide set
+			const int pa = is_dynamic(W, arb.ba) ? S.bpart[arb.ba] : -1;
+			const int pb = is_dynamic(W, arb.bb) ? S.bpart[arb.bb] : -1;
+			key = unit_key(S, unit_of(S.large_index[w], pa, pb, S.n_large_worlds));
 			arb_prestep(W, arb);
 		}
 		S.keys[k] = key;
@@ -1392,15 +1219,14 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 	}
 }
 
-// velocity integration of the large worlds' bodies that no joint pre-step
-// touches first: all of them without joints, else those of the workgroup-solved
-// islands (after the pre-steps above, before any impulse)
-__global__ void k_island_integrate(const DevWorld W, const SolveBuffers S)
+// velocity integration of the large worlds' bodies when they have no joints
+// (after the pre-steps above, before any impulse; with joints the grid solve
+// integrates them after the joints' pre-steps)
+__global__ void k_large_integrate(const DevWorld W, const SolveBuffers S)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= S.n_large_bodies) return;
-	const int b = S.large_bodies[q];
-	if (S.n_large_joints == 0 || island_on_workgroup(S, S.label[b])) integrate_velocity(W, b);
+	integrate_velocity(W, S.large_bodies[q]);
 }
 
 // warb_off[w] = first sorted position of small world w (n_worlds + 1 entries);
@@ -1414,14 +1240,6 @@ __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 		S.large_first[0] = lower_bound_u64(S.keys, 0, n, unit_key(S, 0));
 		S.large_first[1] = lower_bound_u64(S.keys, 0, n, unit_key(S, grid_units(S.n_large_worlds) - 1));
 	}
-}
-
-// the first sorted position of every workgroup-solved island
-__global__ void k_island_groups(const SolveBuffers S, int n_worlds)
-{
-	const int lo = S.warb_off[n_worlds], hi = S.large_first[0];
-	for (int i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x)
-		if (i == lo || S.keys[i] != S.keys[i - 1]) S.groups[atomicAdd(&S.n_groups[0], 1)] = i;
 }
 
 } // namespace
@@ -1443,16 +1261,11 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 {
 	// n_act: an upper bound of *S.n_act (the kernels read the count)
 	const int nb = S.n_large_bodies;
-	const int edge_blocks = std::max(1, std::min((n_act + S.n_large_joints + 255) / 256, GROUP_BLOCKS));
 	if (large) {
-		TIMED(timer, "island_union", stream, {
-			k_uf_init<<<std::max(1, (nb + 255) / 256), 256, 0, stream>>>(S);
-			k_uf_link<<<edge_blocks, 256, 0, stream>>>(W, S);
-			if (nb > 0) k_uf_flatten<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
-		});
-		TIMED(timer, "island_count", stream, k_island_count<<<edge_blocks, 256, 0, stream>>>(W, S));
 		if (nb > 0)
 			TIMED(timer, "strips", stream, {
+				k_part_init<<<std::max(1, std::min((S.n_large_worlds * PART_BINS + 255) / 256, GROUP_BLOCKS)), 256, 0,
+				              stream>>>(S);
 				k_part_bounds<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 				k_part_hist<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 				k_part_scan<<<S.n_large_worlds, PART_BINS, 0, stream>>>(S);
@@ -1468,28 +1281,25 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
 		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
-		// keys below n_worlds + n_bodies + grid_units
-		const uint64_t key_end = (uint64_t)n_worlds + (uint64_t)S.n_bodies + (uint64_t)grid_units(S.n_large_worlds);
+		// keys below n_worlds + grid_units
+		const uint64_t key_end = (uint64_t)n_worlds + (uint64_t)grid_units(S.n_large_worlds);
 		int key_bits = 1;
 		while ((1ull << key_bits) < key_end) key_bits++;
 		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, key_bits, sortbuf, stream, timer));
 	}
-	if (large && nb > 0) {
-		TIMED(timer, "island_integrate", stream,
-		      k_island_integrate<<<(nb + 255) / 256, 256, 0, stream>>>(W, S));
+	if (large && nb > 0 && S.n_large_joints == 0) {
+		TIMED(timer, "large_integrate", stream,
+		      k_large_integrate<<<(nb + 255) / 256, 256, 0, stream>>>(W, S));
 	}
 	k_world_offsets<<<(n_worlds + 2 + 255) / 256, 256, 0, stream>>>(S, n_worlds);
-	if (large && n_act > 0)
-		k_island_groups<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(S, n_worlds);
 	return hipGetLastError();
 }
 
 template <int THREADS>
-static void launch_worlds(const DevWorld &W, const SolveBuffers &S, const int32_t *worlds, int blocks, int n_small,
-                          int lds_cap, int unit, hipStream_t st)
+static void launch_worlds(const DevWorld &W, const SolveBuffers &S, const int32_t *worlds, int blocks, int lds_cap,
+                          int unit, hipStream_t st)
 {
-	k_solve_worlds<THREADS><<<blocks, THREADS, (size_t)lds_cap * sizeof(BodyVel), st>>>(W, S, worlds, n_small,
-	                                                                                    lds_cap, unit);
+	k_solve_worlds<THREADS><<<blocks, THREADS, (size_t)lds_cap * sizeof(BodyVel), st>>>(W, S, worlds, lds_cap, unit);
 }
 
 static const char *bucket_names[CPGPU_MAX_BUCKETS] = {"solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2",
@@ -1505,7 +1315,7 @@ int solver_max_lds_bytes()
 }
 
 hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBucket *buckets, int n_buckets,
-                       int island_blocks, int grid, hipStream_t stream, const hipStream_t *aux,
+                       int grid, hipStream_t stream, const hipStream_t *aux,
                        const hipEvent_t *events, KernelTimer *timer)
 {
 	// fork: the buckets' streams start after everything queued so far
@@ -1516,23 +1326,14 @@ hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBuck
 		HIP_TRY(hipStreamWaitEvent(aux[q], events[0], 0));
 		TIMED(timer, bucket_names[q], aux[q], {
 			switch (B.threads) {
-			case 64: launch_worlds<64>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
-			case 128: launch_worlds<128>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
-			case 256: launch_worlds<256>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
-			case 512: launch_worlds<512>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
-			default: launch_worlds<1024>(W, S, B.worlds, B.n, B.n, B.lds_cap, q, aux[q]); break;
+			case 64: launch_worlds<64>(W, S, B.worlds, B.n, B.lds_cap, q, aux[q]); break;
+			case 128: launch_worlds<128>(W, S, B.worlds, B.n, B.lds_cap, q, aux[q]); break;
+			case 256: launch_worlds<256>(W, S, B.worlds, B.n, B.lds_cap, q, aux[q]); break;
+			case 512: launch_worlds<512>(W, S, B.worlds, B.n, B.lds_cap, q, aux[q]); break;
+			default: launch_worlds<1024>(W, S, B.worlds, B.n, B.lds_cap, q, aux[q]); break;
 			}
 		});
 		HIP_TRY(hipGetLastError());
-	}
-	if (island_blocks > 0) {
-		// n_small = 0: every workgroup takes islands from the work counter; beside the grid solve
-		hipStream_t is = aux[CPGPU_MAX_BUCKETS];
-		HIP_TRY(hipStreamWaitEvent(is, events[0], 0));
-		TIMED(timer, "solver_islands", is, launch_worlds<256>(W, S, nullptr, island_blocks, 0, 0, CPGPU_MAX_BUCKETS, is));
-		HIP_TRY(hipGetLastError());
-		HIP_TRY(hipEventRecord(events[1 + CPGPU_MAX_BUCKETS], is));
-		HIP_TRY(hipStreamWaitEvent(stream, events[1 + CPGPU_MAX_BUCKETS], 0));
 	}
 	if (grid > 0) {
 		void *args[] = {(void *)&W, (void *)&S};

@@ -276,7 +276,7 @@ struct cpgpuWorld {
 	int n_buckets = 0;
 	WorldBucket buckets[CPGPU_MAX_BUCKETS] = {};
 	int32_t *bucket_lists[CPGPU_MAX_BUCKETS] = {};
-	hipStream_t aux[CPGPU_MAX_BUCKETS + 1] = {};   // the last: the islands' solve
+	hipStream_t aux[CPGPU_MAX_BUCKETS] = {};
 	hipEvent_t fork_join[CPGPU_MAX_BUCKETS + 2] = {};
 	int32_t *d_large_bodies = nullptr;
 	int n_large_bodies = 0;
@@ -284,9 +284,6 @@ struct cpgpuWorld {
 	int32_t *arb_by_world = nullptr;
 	uint64_t *world_keys = nullptr;
 	int32_t *warb_off = nullptr, *large_first = nullptr;
-	// islands of the large worlds (see SolveBuffers)
-	int32_t *label = nullptr, *icount = nullptr, *ijoint = nullptr, *groups = nullptr, *n_groups = nullptr;
-	int island_max = 2048, island_blocks_max = 2048;
 	int n_large_joints = 0;
 	// rollout tape for the adjoint (cpgpuWorldTapeBegin)
 	struct Tape {
@@ -449,7 +446,6 @@ int check_device_error(cpgpuWorld *w)
 	if (e & CPGPU_ERR_BARRIER_TIMEOUT) m += " solver grid barrier timeout;";
 	if (e & CPGPU_ERR_TABLE_FULL) m += " pair table full;";
 	if (e & CPGPU_ERR_EPA_HULL) m += " EPA hull overflow;";
-	if (e & CPGPU_ERR_ISLANDS) m += " inconsistent island labels;";
 	if (e & CPGPU_ERR_ADJOINT) m += " adjoint: a contact's collision re-run differs from the rollout's;";
 	return fail(CPGPU_ERR_DEVICE, m);
 }
@@ -522,7 +518,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	w->precision = precision;
 	HIP_OR_FAIL(hipGetDevice(&w->device));
 	HIP_OR_FAIL(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
-	for (int q = 0; q <= CPGPU_MAX_BUCKETS; q++) HIP_OR_FAIL(hipStreamCreateWithFlags(&w->aux[q], hipStreamNonBlocking));
+	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++) HIP_OR_FAIL(hipStreamCreateWithFlags(&w->aux[q], hipStreamNonBlocking));
 	for (int q = 0; q < CPGPU_MAX_BUCKETS + 2; q++)
 		HIP_OR_FAIL(hipEventCreateWithFlags(&w->fork_join[q], hipEventDisableTiming));
 	w->nw = d->n_worlds; w->nb = d->n_bodies; w->ns = d->n_shapes; w->nc = d->n_constraints;
@@ -660,7 +656,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		const int max_col = colour_host(nb, ba, bbv, prio, w->h_jcolour);
 		(void)max_col;
 
-		// small worlds (one workgroup each) and large worlds (islands and the grid solve)
+		// small worlds (one workgroup each) and large worlds (the grid solve)
 		std::vector<uint8_t> small(w->nw);
 		std::vector<int32_t> large_bodies;
 		w->h_world_nb.assign(w->nw, 0);
@@ -766,10 +762,6 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_alloc(w, w->bslot, nb));
 	HIP_OR_FAIL(own_alloc(w, w->claim, CLAIM_SETS * 2 * (size_t)nb));
 	HIP_OR_FAIL(own_alloc(w, w->taken, CLAIM_SETS * (size_t)nb));
-	HIP_OR_FAIL(own_alloc(w, w->label, nb));
-	HIP_OR_FAIL(own_alloc(w, w->icount, nb));
-	HIP_OR_FAIL(own_alloc(w, w->ijoint, nb));
-	HIP_OR_FAIL(own_upload(w, w->n_groups, (const int32_t *)nullptr, 2));
 	w->max_iterations = 0;
 	for (auto &s : w->sp) w->max_iterations = std::max(w->max_iterations, (int)s.iterations);
 
@@ -816,7 +808,7 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	lbvh_free(w->static_tree);
 	radix_sort_free(w->sortbuf);
 	void *extra[] = {w->d_pairs, w->d_pairs_by_type, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
-	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys, w->groups,
+	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys,
 	                 w->cbody, w->cprio, w->ccol};
 	for (void *p : extra) if (p) (void)hipFree(p);
 	if (w->h_ws) (void)hipHostFree(w->h_ws);
@@ -825,7 +817,7 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	if (w->d_replay_n) (void)hipFree(w->d_replay_n);
 	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++)
 		if (w->bucket_lists[q]) (void)hipFree(w->bucket_lists[q]);
-	for (int q = 0; q <= CPGPU_MAX_BUCKETS; q++)
+	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++)
 		if (w->aux[q]) (void)hipStreamDestroy(w->aux[q]);
 	for (int q = 0; q < CPGPU_MAX_BUCKETS + 2; q++)
 		if (w->fork_join[q]) (void)hipEventDestroy(w->fork_join[q]);
@@ -844,9 +836,6 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
 	S.arb_by_world = grouped ? w->arb_by_world : nullptr;
 	S.keys = w->world_keys;
-	S.label = w->label; S.icount = w->icount; S.ijoint = w->ijoint;
-	S.groups = w->groups; S.n_groups = w->n_groups;
-	S.island_max = w->island_max;
 	S.n_large_joints = w->n_large_joints;
 	S.warb_off = warb_off; S.large_first = w->large_first;
 	S.world_small = w->d_world_small;
@@ -1004,14 +993,13 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	}
 	HIP_OR_FAIL(regrow(w->seen, w->seen_cap, (long)n_prev_recs + 1));
 	if (w->act_cap[cur] > w->colour_cap) {
-		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups, w->cbody, w->cprio, w->ccol};
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->cbody, w->cprio, w->ccol};
 		for (void *p : old) if (p) (void)hipFree(p);
 		w->colour_cap = w->act_cap[cur];
 		HIP_OR_FAIL(dalloc(w->colour, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->order, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->arb_by_world, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->world_keys, w->colour_cap));
-		HIP_OR_FAIL(dalloc(w->groups, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->cbody, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->cprio, w->colour_cap));
 		HIP_OR_FAIL(dalloc(w->ccol, w->colour_cap));
@@ -1033,15 +1021,13 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 
 	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
 	SolveBuffers S = solve_buffers(w, w->act[cur], B.n_act, w->colour, w->order, w->warb_off);
-	int island_blocks = 0;
 	if (grouped) {
-		// the solved arbiters by world and island; launches sized for the capacity (at most one solved
+		// the solved arbiters by world and grid unit; launches sized for the capacity (at most one solved
 		// arbiter per candidate pair), the kernels read the count
 		const int n_act_cap = std::max(n_pairs, 1);
 		HIP_OR_FAIL(solver_group(W, S, n_act_cap, w->nw, w->n_large_worlds > 0, w->sortbuf, w->stream, timer));
-		if (w->n_large_worlds > 0) island_blocks = std::min(n_act_cap, w->island_blocks_max);
 	}
-	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, island_blocks, w->n_large_worlds > 0 ? w->solver_grid : 0,
+	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, w->n_large_worlds > 0 ? w->solver_grid : 0,
 	                        w->stream, w->aux, w->fork_join, timer));
 
 	if (w->tape.on) {
@@ -1368,7 +1354,7 @@ static int replay_step(cpgpuWorld *w, int t, Arbiter *act, int32_t *colour, int3
 	S.jrec = jrec;
 	if (w->nw > 1) HIP_OR_FAIL(solver_group(W, S, n, w->nw, false, w->sortbuf, w->stream, nullptr));
 	TIMED(timer, "replay_solve", w->stream,
-	      HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, 0, 0, w->stream, w->aux, w->fork_join, nullptr)));
+	      HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, 0, w->stream, w->aux, w->fork_join, nullptr)));
 	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
 	return CPGPU_OK;
 }
@@ -1437,14 +1423,13 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	ADJ_TRY(grab((void **)&totals_saved, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS)));
 	// replay scratch sized for the largest step
 	if (max_n > w->colour_cap) {
-		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->groups, w->cbody, w->cprio, w->ccol};
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->cbody, w->cprio, w->ccol};
 		for (void *p : old) if (p) (void)hipFree(p);
 		w->colour_cap = max_n;
 		ADJ_TRY(dalloc(w->colour, w->colour_cap));
 		ADJ_TRY(dalloc(w->order, w->colour_cap));
 		ADJ_TRY(dalloc(w->arb_by_world, w->colour_cap));
 		ADJ_TRY(dalloc(w->world_keys, w->colour_cap));
-		ADJ_TRY(dalloc(w->groups, w->colour_cap));
 		ADJ_TRY(dalloc(w->cbody, w->colour_cap));
 		ADJ_TRY(dalloc(w->cprio, w->colour_cap));
 		ADJ_TRY(dalloc(w->ccol, w->colour_cap));

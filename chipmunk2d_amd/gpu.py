@@ -213,8 +213,7 @@ class GPUWorld:
         _check(self.L.cpgpuWorldGetJointGrads(self.h, ptr(out["joint"], ctypes.c_double)))
         return out
 
-    SOLVE_UNITS = ("solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2", "solver_worlds_b3", "solver_islands",
-                   "solver")
+    SOLVE_UNITS = ("solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2", "solver_worlds_b3", "solver")
 
     def solve_counts(self):
         """dict solver kernel name -> (contacts solved, bodies integrated) since creation."""

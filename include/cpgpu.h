@@ -238,8 +238,7 @@ int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies
  * size per bucket (reported in threads_out[CPGPU_MAX_BUCKETS]). */
 int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out);
 /* Schedule of the grid set of spaces with more than 4096 dynamic bodies
- * (its islands of more than 2048 arbiters or with joints, cut into x-strips
- * and the seams between them).  CPGPU_GRID_FLOW (the default): without
+ * (their dynamic bodies, cut into x-strips, with the seams between them).  CPGPU_GRID_FLOW (the default): without
  * joints or arbiters spanning non-neighbouring strips, every strip and seam
  * waits only for the units it shares bodies with; CPGPU_GRID_PHASES: grid
  * barriers between strips, even seams and odd seams.  Results are the same
@@ -280,10 +279,10 @@ int cpgpuWorldGetJointGrads(cpgpuWorld *world, double *d_params);
 
 /* Contacts solved and bodies velocity-integrated since creation by each
  * solve unit, in the order of the timing names solver_worlds_b0 ..
- * solver_worlds_b3 (size buckets of small worlds), solver_islands,
- * solver (grid): CPGPU_SOLVE_UNITS entries each (instrumentation for
+ * solver_worlds_b3 (size buckets of small worlds), solver (the grid set of
+ * the large spaces): CPGPU_SOLVE_UNITS entries each (instrumentation for
  * per-kernel byte counts; no counterpart in the reference). */
-#define CPGPU_SOLVE_UNITS 6
+#define CPGPU_SOLVE_UNITS 5
 int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodies);
 
 const char *cpgpuGetLastError(void);

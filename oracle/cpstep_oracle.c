@@ -1191,34 +1191,23 @@ static int cmp_jsolve(const void *x, const void *y)
 
 /* ------------------------------------------------------------------ */
 /* the device's solve units of a large space (no reference counterpart).
- * A space with more than SMALL_WORLD_MAX_BODIES non-static bodies is split
- * into islands: connected components of its dynamic bodies under the
- * step's arbiters and its joints.  Islands with more than ISLAND_MAX
- * arbiters or with a joint form the grid set.  The grid set's dynamic
- * bodies are cut into GRID_PARTS strips along x holding equal shares of
- * them: a PART_BINS-bin histogram of x between the set's least and greatest
- * x, each bin going to strip floor(GRID_PARTS * (bodies in earlier bins) /
- * (bodies)).  An arbiter of the grid set belongs to the strip of its
- * dynamic bodies, to seam q when they lie in strips q and q + 1, or to the
- * wide set when their strips are further apart.
- * Each unit is coloured on its own and the space is solved unit by unit:
- * the other islands (unit 0, together: they share no body), the strips
- * (1 + q), the even seams (UNIT_EVEN + q / 2), the odd seams
- * (UNIT_ODD + q / 2), the wide set (UNIT_WIDE).                          */
+ * A space with more than SMALL_WORLD_MAX_BODIES non-static bodies is solved
+ * as one grid set: its dynamic bodies are cut into GRID_PARTS strips along x
+ * holding equal shares of them: a PART_BINS-bin histogram of x between the
+ * bodies' least and greatest x, each bin going to strip
+ * floor(GRID_PARTS * (bodies in earlier bins) / (bodies)).  An arbiter
+ * belongs to the strip of its dynamic bodies, to seam q when they lie in
+ * strips q and q + 1, or to the wide set when their strips are further apart.
+ * Each unit is coloured on its own and the space is solved unit by unit: the
+ * strips (1 + q), the even seams (UNIT_EVEN + q / 2), the odd seams
+ * (UNIT_ODD + q / 2), the wide set (UNIT_WIDE).  A small space is one unit, 0. */
 
 #define SMALL_WORLD_MAX_BODIES 4096
-#define ISLAND_MAX 2048
 #define GRID_PARTS 64
 #define PART_BINS 1024
 #define UNIT_EVEN (1 + GRID_PARTS)
 #define UNIT_ODD (UNIT_EVEN + GRID_PARTS / 2)
 #define UNIT_WIDE (UNIT_ODD + (GRID_PARTS - 1) / 2)
-
-static int uf_root(int32_t *parent, int x)
-{
-	while (parent[x] != x) { parent[x] = parent[parent[x]]; x = parent[x]; }
-	return x;
-}
 
 /* a double's bits as an unsigned key of the same order (x + 0.0 turns -0.0 into +0.0) */
 static uint64_t x_key(double x)
@@ -1251,43 +1240,15 @@ static int grid_units(oworld *w, OWorldPart *P)
 	for (int64_t k = 0; k < P->nactive; k++) P->arbs[P->active[k]].unit = 0;
 	if (n_ns <= SMALL_WORLD_MAX_BODIES) return 0;
 	const int nbw = P->b1 - P->b0;
-	int32_t *parent = (int32_t *)malloc((size_t)nbw * sizeof(int32_t));
-	int32_t *icount = (int32_t *)calloc((size_t)nbw, sizeof(int32_t));
-	int32_t *ijoint = (int32_t *)calloc((size_t)nbw, sizeof(int32_t));
 	int32_t *part = (int32_t *)malloc((size_t)nbw * sizeof(int32_t));
 	int32_t hist[PART_BINS];
 #define DYN(b) (w->B[b].type == CPGPU_BODY_DYNAMIC)
-	for (int i = 0; i < nbw; i++) parent[i] = i;
-	for (int64_t k = 0; k < P->nactive; k++) {
-		const OArb *arb = &P->arbs[P->active[k]];
-		if (DYN(arb->ba) && DYN(arb->bb)) {
-			int ra = uf_root(parent, arb->ba - P->b0), rb = uf_root(parent, arb->bb - P->b0);
-			if (ra != rb) parent[ra > rb ? ra : rb] = ra < rb ? ra : rb;
-		}
-	}
-	for (int c = P->c0; c < P->c1; c++) {
-		const OJoint *j = &w->J[c];
-		if (DYN(j->a) && DYN(j->b)) {
-			int ra = uf_root(parent, j->a - P->b0), rb = uf_root(parent, j->b - P->b0);
-			if (ra != rb) parent[ra > rb ? ra : rb] = ra < rb ? ra : rb;
-		}
-	}
-	for (int64_t k = 0; k < P->nactive; k++) {
-		const OArb *arb = &P->arbs[P->active[k]];
-		icount[uf_root(parent, (DYN(arb->ba) ? arb->ba : arb->bb) - P->b0)]++;
-	}
-	for (int c = P->c0; c < P->c1; c++) {
-		const OJoint *j = &w->J[c];
-		if (DYN(j->a) || DYN(j->b)) ijoint[uf_root(parent, (DYN(j->a) ? j->a : j->b) - P->b0)] = 1;
-	}
-	/* the grid set's bodies and their strips */
+	/* the strips of the dynamic bodies */
 	uint64_t lo_k = UINT64_MAX, hi_k = 0;
 	int n_grid = 0;
 	for (int i = 0; i < nbw; i++) {
 		part[i] = -1;
 		if (!DYN(P->b0 + i)) continue;
-		int r = uf_root(parent, i);
-		if (icount[r] <= ISLAND_MAX && !ijoint[r]) continue;
 		part[i] = 0;
 		uint64_t key = x_key(w->B[P->b0 + i].p.x);
 		if (key < lo_k) lo_k = key;
@@ -1307,8 +1268,6 @@ static int grid_units(oworld *w, OWorldPart *P)
 		if (part[i] >= 0) part[i] = hist[part_bin(w->B[P->b0 + i].p.x, lo, hi)];
 	for (int64_t k = 0; k < P->nactive; k++) {
 		OArb *arb = &P->arbs[P->active[k]];
-		int r = uf_root(parent, (DYN(arb->ba) ? arb->ba : arb->bb) - P->b0);
-		if (icount[r] <= ISLAND_MAX && !ijoint[r]) continue;
 		int pa = DYN(arb->ba) ? part[arb->ba - P->b0] : -1, pb = DYN(arb->bb) ? part[arb->bb - P->b0] : -1;
 		if (pa < 0 || pb < 0 || pa == pb) arb->unit = 1 + (pa > pb ? pa : pb);
 		else if (pa - pb == 1 || pb - pa == 1) {
@@ -1317,7 +1276,7 @@ static int grid_units(oworld *w, OWorldPart *P)
 		} else arb->unit = UNIT_WIDE;
 	}
 #undef DYN
-	free(parent); free(icount); free(ijoint); free(part);
+	free(part);
 	return 1;
 }
 

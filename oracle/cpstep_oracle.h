@@ -52,10 +52,9 @@ int ow_write_bodies(oworld *w, const double *v, const double *wv, const double *
 /* Same layout as cpgpuWorldReadArbiters. */
 int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colour, int32_t *state,
                          int32_t *count, double *normal, double *contact, uint64_t *hash);
-/* solve unit of each arbiter of the last step (same order): 0 for a small
- * space and a large space's workgroup-solved islands; for its grid set
- * 1 + q for strip q (q < 64), 65 + q / 2 for even seam q, 97 + q / 2 for odd
- * seam q, 128 for the wide set */
+/* solve unit of each arbiter of the last step (same order): 0 in a small
+ * space; in a large space 1 + q for strip q (q < 64), 65 + q / 2 for even
+ * seam q, 97 + q / 2 for odd seam q, 128 for the wide set */
 int64_t ow_read_units(oworld *w, int64_t cap, int32_t *unit);
 int ow_read_constraints(oworld *w, double *jacc, int32_t *colour);
 void ow_stats(oworld *w, cpgpuStats *out);

@@ -107,8 +107,8 @@ def _strips_seen(o):
 
 
 def test_grid_solve_mid_size_world_bit_exact():
-    """More dynamic bodies than one workgroup takes, in small islands: each
-    island on a workgroup beside the (empty) grid solve."""
+    """More dynamic bodies than one workgroup takes, in small scattered
+    clusters: the grid solve's strips hold many independent clusters."""
     pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=260.0, seed=12).pack()
     assert pw.dynamic_body_count() > 4096
     g, o = run_lockstep(pw, 60, check_every=2, arbiters_every=30)
@@ -116,8 +116,8 @@ def test_grid_solve_mid_size_world_bit_exact():
 
 
 def test_grid_strips_dense_world_bit_exact():
-    """A dense large space: one island of ~10k arbiters, solved on the grid in
-    64 strips, the seams between them and the wide set."""
+    """A dense large space of ~10k arbiters, solved on the grid in 64 strips,
+    the seams between them and the wide set."""
     pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
     g, o = run_lockstep(pw, 60, check_every=1, arbiters_every=20)
     strips, boundary = _strips_seen(o)
@@ -125,8 +125,8 @@ def test_grid_strips_dense_world_bit_exact():
 
 
 def test_grid_strips_chain_with_joints_bit_exact():
-    """A large space whose chain island (pivots, springs, slides, gears) is on
-    the grid: strips of the links and the circles on them, then the joints."""
+    """A large space with a chain (pivots, springs, slides, gears) on the grid:
+    strips of the links and the circles, then the joints."""
     pw = chain_world(n_links=300, n_circles=3900, seed=9).pack()
     g, o = run_lockstep(pw, 100, check_every=2, arbiters_every=50)
     strips, boundary = _strips_seen(o)

@@ -498,31 +498,17 @@ def test_colouring_is_proper_and_matches_restatement():
     assert _python_colouring(ba, bb, prio, pw.n_bodies) == list(A.colour)
 
 
-def _strip_units(pw, p, shapes, small_max=4096, island_max=2048, parts=64, bins=1024):
+def _strip_units(pw, p, shapes, small_max=4096, parts=64, bins=1024):
     """Independent restatement of a large space's solve units (one space):
-    islands by scipy's connected components, strips by numpy quantiles."""
-    from scipy.sparse import coo_matrix
-    from scipy.sparse.csgraph import connected_components
+    strips by numpy histogram quantiles of the dynamic bodies' x."""
     bt = pw.a["body_type"]
     sb = pw.a["shape_body"]
     nb = pw.n_bodies
     if np.count_nonzero(bt != 2) <= small_max:
         return np.zeros(len(shapes), np.int32)
     a, b = sb[shapes[:, 0]].astype(np.int64), sb[shapes[:, 1]].astype(np.int64)
-    ja, jb = pw.a["cons_a"].astype(np.int64), pw.a["cons_b"].astype(np.int64)
     dyn = bt == BODY_DYNAMIC
-    ea = np.concatenate([a[dyn[a] & dyn[b]], ja[dyn[ja] & dyn[jb]]])
-    eb = np.concatenate([b[dyn[a] & dyn[b]], jb[dyn[ja] & dyn[jb]]])
-    g = coo_matrix((np.ones(len(ea)), (ea, eb)), shape=(nb, nb))
-    _, comp = connected_components(g, directed=False)
-    arb_comp = comp[np.where(dyn[a], a, b)]
-    icount = np.bincount(arb_comp, minlength=nb)
-    ijoint = np.zeros(nb, bool)
-    jdyn = dyn[ja] | dyn[jb]
-    ijoint[comp[np.where(dyn[ja], ja, jb)[jdyn]]] = True
-    grid_comp = (icount > island_max) | ijoint
-    grid_body = dyn & grid_comp[comp]
-    x = p[grid_body, 0] + 0.0
+    x = p[dyn, 0] + 0.0
     lo, hi = x.min(), x.max()
     if hi > lo:
         t = (x - lo) * (float(bins) / (hi - lo))
@@ -532,9 +518,7 @@ def _strip_units(pw, p, shapes, small_max=4096, island_max=2048, parts=64, bins=
     hist = np.bincount(binv, minlength=bins)
     before = np.concatenate([[0], np.cumsum(hist)[:-1]])
     part = np.full(nb, -1, np.int64)
-    part[grid_body] = (before * parts // len(x))[binv]
-    unit = np.zeros(len(shapes), np.int32)
-    on_grid = grid_comp[arb_comp]
+    part[dyn] = (before * parts // len(x))[binv]
     pa = np.where(dyn[a], part[a], -1)
     pb = np.where(dyn[b], part[b], -1)
     both = (pa >= 0) & (pb >= 0)
@@ -543,13 +527,12 @@ def _strip_units(pw, p, shapes, small_max=4096, island_max=2048, parts=64, bins=
     seam = np.where(q % 2 == 0, 1 + parts + q // 2, 1 + parts + parts // 2 + q // 2)
     u = np.where(both & (gap == 1), seam, 1 + np.maximum(pa, pb))
     u = np.where(both & (gap > 1), 1 + parts + parts // 2 + (parts - 1) // 2, u)
-    unit[on_grid] = u[on_grid]
-    return unit
+    return u.astype(np.int32)
 
 
 @pytest.mark.parametrize("scene", ["dense", "chain"])
 def test_large_space_strips_match_restatement(scene):
-    """A large space's solve units (islands, strips along x, seams between
+    """A large space's solve units (strips along x, seams between
     neighbouring strips, even ones first, and the wide set) against an
     independent restatement; each unit coloured on its own (the Python claim
     rounds) and solved unit by unit, by colour inside a unit."""
@@ -558,8 +541,7 @@ def test_large_space_strips_match_restatement(scene):
         pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
         steps = 6
     else:
-        # a 300-link chain (pivots, springs, slides, gears) under 3900 circles: its island is
-        # on the grid for its joints; circles landing on it put arbiters into the strips
+        # a 300-link chain (pivots, springs, slides, gears) under 3900 circles falling onto it
         pw = chain_world(n_links=300, n_circles=3900, seed=9).pack()
         steps = 90
     o = OracleWorld(pw)

@@ -104,6 +104,28 @@ def test_fp32_random_world_100k_first_steps():
     assert rep[0]["max_dp_rel"] < 2e-5, rep[0]
 
 
+def test_fp32_dense_large_world_on_the_grid():
+    """A dense large space (its grid set in 64 strips, seams and a wide set) in
+    FP32 beside FP64; both grid schedules give the same FP32 run."""
+    from chipmunk2d_amd.gpu import GRID_FLOW, GRID_PHASES
+    pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
+    rep = drift_report(pw, DT, checkpoints=(1, 20))
+    _show(rep)
+    assert all(r["finite"] for r in rep)
+    assert rep[0]["max_dp_rel"] < 2e-5, rep[0]
+    assert abs(rep[-1]["contacts_fp32"] - rep[-1]["contacts_fp64"]) <= 0.02 * rep[-1]["contacts_fp64"], rep[-1]
+    flat = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=400.0, seed=15).pack()
+    runs = []
+    for sched in (GRID_FLOW, GRID_PHASES):
+        g = GPUWorld(flat, FP32)
+        g.set_grid_schedule(sched)
+        g.step(DT, 130)
+        runs.append(g.bodies())
+        g.close()
+    for f in ("p", "v", "a", "w"):
+        assert np.array_equal(getattr(runs[0], f), getattr(runs[1], f)), f
+
+
 def test_adjoint_refuses_fp32_worlds():
     g = GPUWorld(pyramid_210(rows=3).pack(), FP32)
     with pytest.raises(CpgpuError, match="FP64"):

@@ -331,7 +331,7 @@ __global__ void k_world_ranges(const uint64_t *keys, int n, int32_t *first, int3
 }
 
 __global__ void k_karras(const uint64_t *keys, int n, int32_t *child, int32_t *parent, int32_t *range,
-                         const int32_t *first, const int32_t *last, int32_t *root)
+                         const int32_t *first, const int32_t *last, int32_t *root, PairNode *node)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n - 1) return;
@@ -359,6 +359,11 @@ __global__ void k_karras(const uint64_t *keys, int n, int32_t *child, int32_t *p
 	parent[right] = i;
 	range[2 * i] = lo;
 	range[2 * i + 1] = hi;
+	// the traversal's record: the left child ends at leaf gamma, the right one at hi (the refit adds the boxes)
+	node[i].child[0] = left;
+	node[i].child[1] = right;
+	node[i].hi[0] = gamma;
+	node[i].hi[1] = hi;
 	const int w = (int)(keys[lo] >> WORLD_SHIFT);
 	if (first[w] == lo && last[w] == hi) root[w] = i;
 }
@@ -366,6 +371,15 @@ __global__ void k_karras(const uint64_t *keys, int n, int32_t *child, int32_t *p
 __device__ __forceinline__ real4 box_union(real4 a, real4 b)
 {
 	return make_real4(fminc(a.x, b.x), fminc(a.y, b.y), fmaxc(a.z, b.z), fmaxc(a.w, b.w));
+}
+
+// node x's box, also into its parent's traversal record
+__device__ __forceinline__ void put_box(real4 *box, PairNode *node, const int32_t *child, const int32_t *parent, int x,
+                                        real4 b)
+{
+	box[x] = b;
+	const int p = parent[x];
+	if (p >= 0) node[p].box[child[2 * p] == x ? 0 : 1] = b;
 }
 
 __device__ __forceinline__ real4 shfl_up4(real4 v, int off)
@@ -387,8 +401,8 @@ constexpr int REFIT_CHUNK = 256;
 
 __global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, int n, const int32_t *leaf_shape,
                                                            const int32_t *child, const int32_t *parent,
-                                                           const int32_t *range, real4 *box, real4 *pre,
-                                                           real4 *suf, real4 *cbox)
+                                                           const int32_t *range, real4 *box, PairNode *node,
+                                                           real4 *pre, real4 *suf, real4 *cbox)
 {
 	__shared__ real4 lb[REFIT_CHUNK];
 	__shared__ real4 ib[REFIT_CHUNK];
@@ -401,7 +415,10 @@ __global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, i
 	const real4 b = valid ? W.s_bb[leaf_shape[i]] : make_real4(inf, inf, -inf, -inf);
 	lb[t] = b;
 	vis[t] = 0;
-	if (valid) box[(n - 1) + i] = b;
+	if (valid) {
+		if (n > 1) put_box(box, node, child, parent, (n - 1) + i, b);
+		else box[(n - 1) + i] = b;
+	}
 	const int lane = t & 63, wave = t >> 6;
 	real4 pf = b, sf = b;
 	for (int off = 1; off < 64; off <<= 1) {
@@ -429,7 +446,7 @@ __global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, i
 		const real4 b1 = c1 >= n - 1 ? lb[c1 - (n - 1) - lo] : ib[c1 - lo];
 		const real4 u = box_union(b0, b1);
 		ib[p - lo] = u;
-		box[p] = u;
+		put_box(box, node, child, parent, p, u);
 		p = parent[p];
 	}
 }
@@ -437,8 +454,9 @@ __global__ void __launch_bounds__(REFIT_CHUNK) k_refit_local(const DevWorld W, i
 // Pass 2: a node spanning chunks gets the union of the suffix of its first
 // chunk, the whole chunks between, and the prefix of its last chunk.  Nodes
 // above the world roots are never read and are skipped.
-__global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, const real4 *pre,
-                             const real4 *suf, const real4 *cbox, real4 *box)
+__global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, const int32_t *child,
+                             const int32_t *parent, const real4 *pre, const real4 *suf, const real4 *cbox, real4 *box,
+                             PairNode *node)
 {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= n - 1) return;
@@ -448,7 +466,7 @@ __global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, 
 	if ((keys[lo] >> WORLD_SHIFT) != (keys[hi] >> WORLD_SHIFT)) return;
 	real4 u = box_union(suf[lo], pre[hi]);
 	for (int c = cl + 1; c < ch; c++) u = box_union(u, cbox[c]);
-	box[p] = u;
+	put_box(box, node, child, parent, p, u);
 }
 
 // ---- traversal ----
@@ -527,11 +545,12 @@ __device__ __forceinline__ void emit_pair(const DevWorld &W, int s, int t, const
 
 
 // one dynamic shape's queries: pairs with later leaves of its world's dynamic
-// tree and with its world's static tree
+// tree and with its world's static tree; a popped node's children come in one
+// record (PairNode)
 __device__ __forceinline__ void traverse_pairs(const DevWorld &W, int q, int nd, const uint64_t *dkeys,
-                                               const int32_t *dshape, const int32_t *dchild, const int32_t *drange,
+                                               const int32_t *dshape, const PairNode *dnode, const int32_t *drange,
                                                const real4 *dbox, const int32_t *droot, int ns, const int32_t *sshape,
-                                               const int32_t *schild, const real4 *sbox, const int32_t *sroot,
+                                               const PairNode *snode, const real4 *sbox, const int32_t *sroot,
                                                int32_t *stk, const PairStage &st)
 {
 	const int s = dshape[q];
@@ -548,15 +567,15 @@ __device__ __forceinline__ void traverse_pairs(const DevWorld &W, int q, int nd,
 		}
 	}
 	while (sp > 0) {
-		const int node = stk[256 * --sp];
+		const PairNode nd_rec = dnode[stk[256 * --sp]];
 		for (int side = 0; side < 2; side++) {
-			const int c = dchild[2 * node + side];
+			const int c = nd_rec.child[side];
+			if (nd_rec.hi[side] <= q) continue;   // a leaf at or before q, or a subtree ending there
+			if (!box_overlap(bb, nd_rec.box[side])) continue;
 			if (c >= nd - 1) {
-				const int p = c - (nd - 1);
-				if (p <= q) continue;
-				const int t = dshape[p];
-				if (box_overlap(bb, dbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, st);
-			} else if (drange[2 * c + 1] > q && box_overlap(bb, dbox[c])) {
+				const int t = dshape[c - (nd - 1)];
+				if (pair_ok(W, s, t)) emit_pair(W, s, t, st);
+			} else {
 				if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
 				stk[256 * sp++] = c;
 			}
@@ -572,13 +591,14 @@ __device__ __forceinline__ void traverse_pairs(const DevWorld &W, int q, int nd,
 			stk[256 * sp++] = root;
 		}
 		while (sp > 0) {
-			const int node = stk[256 * --sp];
+			const PairNode nd_rec = snode[stk[256 * --sp]];
 			for (int side = 0; side < 2; side++) {
-				const int c = schild[2 * node + side];
+				const int c = nd_rec.child[side];
+				if (!box_overlap(bb, nd_rec.box[side])) continue;
 				if (c >= ns - 1) {
 					const int t = sshape[c - (ns - 1)];
-					if (box_overlap(bb, sbox[c]) && pair_ok(W, s, t)) emit_pair(W, s, t, st);
-				} else if (box_overlap(bb, sbox[c])) {
+					if (pair_ok(W, s, t)) emit_pair(W, s, t, st);
+				} else {
 					if (sp >= TRAVERSAL_STACK) { raise_err(W.err, CPGPU_ERR_TRAVERSAL_STACK); continue; }
 					stk[256 * sp++] = c;
 				}
@@ -588,9 +608,9 @@ __device__ __forceinline__ void traverse_pairs(const DevWorld &W, int q, int nd,
 }
 
 __global__ void __launch_bounds__(256) k_pairs(const DevWorld W, int nd, const uint64_t *dkeys,
-                                               const int32_t *dshape, const int32_t *dchild,
+                                               const int32_t *dshape, const PairNode *dnode,
                                                const int32_t *drange, const real4 *dbox, const int32_t *droot,
-                                               int ns, const int32_t *sshape, const int32_t *schild,
+                                               int ns, const int32_t *sshape, const PairNode *snode,
                                                const real4 *sbox, const int32_t *sroot, int2 *pairs, int cap,
                                                int32_t *count)
 {
@@ -604,7 +624,7 @@ __global__ void __launch_bounds__(256) k_pairs(const DevWorld W, int nd, const u
 	if (__lane_id() == 0) stage_n[wave] = 0;
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 	const PairStage st{stage_buf[wave], &stage_n[wave], pairs, cap, count};
-	if (q < nd) traverse_pairs(W, q, nd, dkeys, dshape, dchild, drange, dbox, droot, ns, sshape, schild, sbox, sroot,
+	if (q < nd) traverse_pairs(W, q, nd, dkeys, dshape, dnode, drange, dbox, droot, ns, sshape, snode, sbox, sroot,
 	                           &stk[0][lane], st);
 	// every lane of the wave is here again: write out what is left
 	const int staged = *st.n;
@@ -681,6 +701,7 @@ void lbvh_reserve(Lbvh &t, int n, int n_worlds)
 	(void)hipMalloc(&t.parent, sizeof(int32_t) * 2 * cap);
 	(void)hipMalloc(&t.range, sizeof(int32_t) * 2 * cap);
 	(void)hipMalloc(&t.box, sizeof(real4) * 2 * cap);
+	(void)hipMalloc(&t.node, sizeof(PairNode) * cap);
 	(void)hipMalloc(&t.pre, sizeof(real4) * cap);
 	(void)hipMalloc(&t.suf, sizeof(real4) * cap);
 	(void)hipMalloc(&t.cbox, sizeof(real4) * (cap / REFIT_CHUNK + 1));
@@ -692,7 +713,7 @@ void lbvh_reserve(Lbvh &t, int n, int n_worlds)
 
 void lbvh_free(Lbvh &t)
 {
-	void *ptrs[] = {t.keys, t.leaf_shape, t.child, t.parent, t.range, t.box, t.pre, t.suf, t.cbox, t.world_root,
+	void *ptrs[] = {t.keys, t.leaf_shape, t.child, t.parent, t.range, t.box, t.node, t.pre, t.suf, t.cbox, t.world_root,
 	                t.world_first, t.world_last};
 	for (void *p : ptrs) if (p) (void)hipFree(p);
 	t = Lbvh();
@@ -725,11 +746,13 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 		                                                        t.world_root);
 		if (n > 1)
 			k_karras<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.child, t.parent, t.range,
-			                                                      t.world_first, t.world_last, t.world_root);
+			                                                      t.world_first, t.world_last, t.world_root, t.node);
 		k_refit_local<<<blocks_for(n, REFIT_CHUNK), REFIT_CHUNK, 0, stream>>>(W, n, t.leaf_shape, t.child, t.parent,
-		                                                                    t.range, t.box, t.pre, t.suf, t.cbox);
+		                                                                    t.range, t.box, t.node, t.pre, t.suf,
+		                                                                    t.cbox);
 		if (n > REFIT_CHUNK)
-			k_refit_span<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.range, t.pre, t.suf, t.cbox, t.box);
+			k_refit_span<<<blocks_for(n - 1, 256), 256, 0, stream>>>(t.keys, n, t.range, t.child, t.parent, t.pre,
+			                                                         t.suf, t.cbox, t.box, t.node);
 	});
 	return hipGetLastError();
 }
@@ -740,9 +763,9 @@ hipError_t broadphase_pairs(const Lbvh &dyn, const Lbvh &stat, const DevWorld &W
 	HIP_TRY(hipMemsetAsync(count, 0, sizeof(int32_t), stream));
 	if (dyn.n == 0) return hipSuccess;
 	TIMED(timer, "bvh_pairs", stream, {
-		k_pairs<<<blocks_for(dyn.n, 256), 256, 0, stream>>>(W, dyn.n, dyn.keys, dyn.leaf_shape, dyn.child, dyn.range,
+		k_pairs<<<blocks_for(dyn.n, 256), 256, 0, stream>>>(W, dyn.n, dyn.keys, dyn.leaf_shape, dyn.node, dyn.range,
 		                                                    dyn.box, dyn.world_root, stat.n, stat.leaf_shape,
-		                                                    stat.child, stat.box, stat.world_root, pairs, cap, count);
+		                                                    stat.node, stat.box, stat.world_root, pairs, cap, count);
 	});
 	return hipGetLastError();
 }

@@ -87,6 +87,13 @@ hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n_cap, const int3
                             RadixSortBuffers &b, hipStream_t stream, KernelTimer *timer);
 
 // ---- LBVH (broadphase.hip) ----
+// an internal node as the pair traversal reads it: both children's boxes,
+// indices and last leaves (a leaf child's own leaf index), in one record
+struct PairNode {
+	real4 box[2];
+	int32_t child[2];
+	int32_t hi[2];
+};
 struct Lbvh {
 	int n = 0;                      // leaves
 	int cap = 0;
@@ -96,6 +103,7 @@ struct Lbvh {
 	int32_t *parent = nullptr;      // [2n-1]
 	int32_t *range = nullptr;       // [n-1][2] leaf range of internal nodes
 	real4 *box = nullptr;         // [2n-1]
+	PairNode *node = nullptr;     // [n-1]
 	real4 *pre = nullptr;         // [n] refit: union of the chunk's leaf boxes up to each leaf
 	real4 *suf = nullptr;         // [n] ... and from each leaf to the chunk's end
 	real4 *cbox = nullptr;        // [n / chunk + 1] union of each chunk's leaf boxes

@@ -1079,36 +1079,42 @@ __global__ void k_part_init(const SolveBuffers S)
 	}
 }
 
-// The grid set's least and greatest x per large world.  Lanes of one world
-// (nearly always the whole wave) combine their keys before the atomics.
-__global__ void k_part_bounds(const DevWorld W, const SolveBuffers S)
+// The grid set's least and greatest x per large world: waves reduce their
+// lanes, the workgroup its waves in LDS (its bodies lie in at most two large
+// worlds, as in k_part_assign), then one atomic pair per world.
+constexpr int PART_BOUNDS_THREADS = 1024;
+__global__ void __launch_bounds__(PART_BOUNDS_THREADS) k_part_bounds(const DevWorld W, const SolveBuffers S)
 {
+	__shared__ unsigned long long lmin[2], lmax[2];
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	int l = -1;
+	const int l0 = S.large_index[W.b_world[S.large_bodies[blockIdx.x * blockDim.x]]];
+	if (threadIdx.x < 2) { lmin[threadIdx.x] = ~0ull; lmax[threadIdx.x] = 0ull; }
+	__syncthreads();
+	int rel = -1;
 	unsigned long long k = 0;
 	if (q < S.n_large_bodies) {
 		const int b = S.large_bodies[q];
 		if (in_grid_set(W, S, b)) {
-			l = S.large_index[W.b_world[b]];
+			rel = S.large_index[W.b_world[b]] - l0;
 			k = x_key((double)W.b_p[b].x);
 		}
 	}
-	uint64_t todo = __ballot(l >= 0);
-	while (todo) {
-		const int leader = __ffsll((long long)todo) - 1;
-		const int r = __shfl(l, leader);
-		const bool mine = l == r;
-		unsigned long long mn = mine ? k : ~0ull, mx = mine ? k : 0ull;
+	for (int r = 0; r < 2; r++) {
+		unsigned long long mn = rel == r ? k : ~0ull, mx = rel == r ? k : 0ull;
 		for (int off = 32; off > 0; off >>= 1) {
 			const unsigned long long a = __shfl_xor(mn, off), b = __shfl_xor(mx, off);
 			mn = a < mn ? a : mn;
 			mx = b > mx ? b : mx;
 		}
-		if (__lane_id() == leader) {
-			atomicMin(&S.pbound[2 * r], mn);
-			atomicMax(&S.pbound[2 * r + 1], mx);
+		if (__lane_id() == 0 && mn <= mx) {
+			atomicMin(&lmin[r], mn);
+			atomicMax(&lmax[r], mx);
 		}
-		todo &= ~__ballot(mine);
+	}
+	__syncthreads();
+	if (threadIdx.x < 2 && lmin[threadIdx.x] <= lmax[threadIdx.x]) {
+		atomicMin(&S.pbound[2 * (l0 + threadIdx.x)], lmin[threadIdx.x]);
+		atomicMax(&S.pbound[2 * (l0 + threadIdx.x) + 1], lmax[threadIdx.x]);
 	}
 }
 
@@ -1266,7 +1272,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 			TIMED(timer, "strips", stream, {
 				k_part_init<<<std::max(1, std::min((S.n_large_worlds * PART_BINS + 255) / 256, GROUP_BLOCKS)), 256, 0,
 				              stream>>>(S);
-				k_part_bounds<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
+				k_part_bounds<<<(nb + PART_BOUNDS_THREADS - 1) / PART_BOUNDS_THREADS, PART_BOUNDS_THREADS, 0, stream>>>(W, S);
 				k_part_hist<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
 				k_part_scan<<<S.n_large_worlds, PART_BINS, 0, stream>>>(S);
 				k_part_assign<<<(nb + PART_ASSIGN_THREADS - 1) / PART_ASSIGN_THREADS, PART_ASSIGN_THREADS, 0, stream>>>(W, S);

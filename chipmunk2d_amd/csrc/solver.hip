@@ -4,9 +4,9 @@
 // arbiter pre-step, constraint pre-step, velocity integration, cached
 // impulses, then `iterations` sweeps of cpArbiterApplyImpulse and the joints'
 // applyImpulse.  The reference sweeps its arbiter list sequentially
-// (Gauss-Seidel).  Here the arbiters of one solve unit (a small world, an
-// island, a strip, seam or the wide set of a large world's grid set) are
-// coloured so that no two arbiters of a colour share a dynamic body; a sweep
+// (Gauss-Seidel).  Here the arbiters of one solve unit (a small world, or a
+// strip, seam or the wide set of a large world's grid set) are coloured so
+// that no two arbiters of a colour share a dynamic body; a sweep
 // runs colour after colour with a barrier between colours, so the result is
 // exactly the sequential sweep in colour order (arbiters of one colour touch
 // disjoint bodies, so their order inside the colour does not matter).
@@ -25,7 +25,7 @@ namespace CPGPU_NS {
 
 namespace {
 
-constexpr int GRID_THREADS = 1024;  // grid solve: few, large workgroups keep the barrier cheap
+constexpr int GRID_THREADS = 1024;  // grid solve: one workgroup per column of strips and seams
 
 // ---- grid barrier (all blocks co-resident: cooperative launch) ----
 // The arrival count starts at 0 with each launch (k_bar_reset) and only

@@ -249,12 +249,13 @@ __global__ void __launch_bounds__(256) k_pair_class_count(const DevWorld W, cons
 	if (threadIdx.x < PAIR_CLASSES && h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], h[threadIdx.x]);
 }
 
-// counts -> class starts (cursors)
+// counts -> class starts (cursors), the costliest classes (polygon pairs,
+// highest class numbers) first, so their long-running waves start first
 __global__ void k_pair_class_scan(int32_t *counts)
 {
 	if (threadIdx.x != 0) return;
 	int run = 0;
-	for (int c = 0; c < PAIR_CLASSES; c++) {
+	for (int c = PAIR_CLASSES - 1; c >= 0; c--) {
 		const int k = counts[c];
 		counts[PAIR_CLASSES + c] = run;
 		run += k;

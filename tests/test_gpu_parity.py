@@ -189,6 +189,23 @@ def test_grid_schedule_rejects_unknown_values():
         g.set_grid_schedule(5)
 
 
+def test_grid_strips_with_kinematic_bodies_bit_exact():
+    """Kinematic paddles sweeping through a dense large space: they join no
+    strip and constrain no colour, their contacts go with the dynamic body's
+    strip."""
+    sp = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=16)
+    for k in range(4):
+        kin = sp.add(Body.kinematic())
+        kin.position = (-60.0 + 30.0 * k, -40.0 + 20.0 * k)
+        kin.velocity = (25.0 if k % 2 == 0 else -25.0, 0.0)
+        kin.angular_velocity = 0.5
+        sp.add(Box(kin, 24.0, 3.0, 0.0))
+    pw = sp.pack()
+    g, o = run_lockstep(pw, 40, check_every=2, arbiters_every=20)
+    strips, others = _strips_seen(o)
+    assert strips == 64 and others > 1000, (strips, others)
+
+
 def test_mixed_batch_grid_and_workgroup_solves_bit_exact():
     """Two large spaces (grid sets in strips) and small spaces (a workgroup each) in one step."""
     rng = Rng(0x1A46E)

@@ -1324,9 +1324,10 @@ hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBuck
                        int grid, hipStream_t stream, const hipStream_t *aux,
                        const hipEvent_t *events, KernelTimer *timer)
 {
-	// fork: the buckets' streams start after everything queued so far
+	// fork: the buckets' streams start after everything queued so far; the
+	// buckets of the largest spaces (the longest workgroups) are queued first
 	HIP_TRY(hipEventRecord(events[0], stream));
-	for (int q = 0; q < n_buckets; q++) {
+	for (int q = n_buckets - 1; q >= 0; q--) {
 		const WorldBucket &B = buckets[q];
 		if (B.n == 0) continue;
 		HIP_TRY(hipStreamWaitEvent(aux[q], events[0], 0));

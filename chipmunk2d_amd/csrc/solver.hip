@@ -456,11 +456,12 @@ struct SolveSet {
 	int cset;                  // claim arrays used by the colouring (units coloured concurrently
 	                           // with shared bodies use different ones)
 	// colouring with the claim arrays in LDS (lclaim != nullptr): the set's dynamic bodies
-	// are numbered 0 .. by slot[body]
+	// are numbered 0 .. by slot[body], or by body - slot_base when slot is nullptr
 	unsigned long long *lclaim;   // [2][lcap]
 	int32_t *ltaken;              // [lcap]
 	int lcap;
 	const int32_t *slot;
+	int slot_base;
 };
 
 // a wave-reduced 64-bit add to a counter
@@ -605,7 +606,10 @@ __device__ __forceinline__ int colour_set(const DevWorld &W, const SolveBuffers 
 	auto init = [&](int k, int2 &bd, unsigned long long &prio) {
 		const Arbiter &arb = S.act[arb_index(k)];
 		bd = make_int2(dyn_or_neg(W, arb.ba), dyn_or_neg(W, arb.bb));
-		if (T.lclaim) bd = make_int2(bd.x >= 0 ? T.slot[bd.x] : -1, bd.y >= 0 ? T.slot[bd.y] : -1);
+		if (T.lclaim) {
+			auto slot_of = [&](int b) { return T.slot ? T.slot[b] : b - T.slot_base; };
+			bd = make_int2(bd.x >= 0 ? slot_of(bd.x) : -1, bd.y >= 0 ? slot_of(bd.y) : -1);
+		}
 		prio = arbiter_priority(W, arb);
 		if (bd.x >= 0) { claim0[bd.x] = ~0ull; claim1[bd.x] = ~0ull; taken[bd.x] = -1; }
 		if (bd.y >= 0) { claim0[bd.y] = ~0ull; claim1[bd.y] = ~0ull; taken[bd.y] = -1; }
@@ -693,10 +697,21 @@ __device__ __forceinline__ int colour_set(const DevWorld &W, const SolveBuffers 
 }
 
 template <class Sync>
-__device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &T, Sync &sy)
+__device__ void solve_set(const DevWorld &W, const SolveBuffers &S, const SolveSet &Tin, Sync &sy)
 {
-	const int tid = sy.tid, stride = sy.stride, n = T.n;
-	auto arb_index = [&](int k) { return T.list ? T.list[k] : k; };
+	const int tid = sy.tid, stride = sy.stride, n = Tin.n;
+	auto arb_index = [&](int k) { return Tin.list ? Tin.list[k] : k; };
+	SolveSet T = Tin;
+	if (T.lds_bodies && T.nb <= T.lds_cap) {
+		// the world's bodies are staged in LDS only after the colouring: until then the claim
+		// arrays live there (20 bytes a body, no more than a staged body's)
+		T.lclaim = reinterpret_cast<unsigned long long *>(T.lds_bodies);
+		T.lcap = T.nb;
+		T.ltaken = reinterpret_cast<int32_t *>(T.lclaim + 2 * T.nb);
+		T.slot = nullptr;
+		T.slot_base = T.body0;
+	}
+	static_assert(sizeof(BodyVel) >= 2 * sizeof(unsigned long long) + sizeof(int32_t), "claims fit a body's room");
 	const int n_colours = colour_set(W, S, T);
 	// ---- pre-steps (cpSpaceStep.c:383-396) ----
 	for (int k = tid; k < n; k += stride) arb_prestep(W, S.act[arb_index(k)]);

@@ -25,6 +25,9 @@
  * The deterministic graph colouring (the device solve order) has no reference
  * counterpart; it is pinned by its defining properties (a proper colouring,
  * every arbiter coloured, identical to an independent Python restatement).
+ * So has the split of a large space into solve units (strips, seams, the
+ * wide set), pinned against an independent numpy restatement, each unit's
+ * colouring against the Python claim rounds.
  */
 #include <math.h>
 #include <float.h>

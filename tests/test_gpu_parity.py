@@ -182,6 +182,26 @@ def test_solve_only_timing_records_only_solver_launches():
         _check(g.L.cpgpuWorldSetTiming(g.h, 3))
 
 
+def test_grid_flow_schedule_two_large_spaces_in_a_batch_bit_exact():
+    """Two wide flat piles and small spaces in one step: the completion counters
+    of both large spaces' strips and seams (no wide arbiters, no joints)."""
+    rng = Rng(0xF10)
+    a = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=400.0, seed=15).pack()
+    b = random_world(n_circles=2500, n_polys=2500, n_segments=300, extent=380.0, seed=17).pack()
+    pw = concat([pyramid_210(rows=6).pack(), a, random_pile(150, rng).pack(), b])
+    g = GPUWorld(pw)
+    o = OracleWorld(pw)
+    for s in range(125):
+        g.step(DT)
+        o.step(DT)
+        if s % 5 == 4:
+            assert_bodies_equal(g, o, "step %d" % s)
+    assert_arbiters_equal(g, o)
+    u = o.units()
+    assert np.count_nonzero(u == 128) == 0
+    assert len(np.unique(u[(u > 0) & (u <= 64)])) == 64 and np.count_nonzero((u > 64) & (u < 128)) > 500
+
+
 def test_grid_schedule_rejects_unknown_values():
     from chipmunk2d_amd.gpu import CpgpuError
     g = GPUWorld(pyramid_210(rows=3).pack())

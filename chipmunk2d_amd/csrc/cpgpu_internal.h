@@ -43,9 +43,15 @@ struct KernelTimer {
 		open.pop_back();
 		if (k >= 0) (void)hipEventRecord(pending[k].b, s);
 	}
-	// after a stream sync: fold finished records into the totals
+	// after a stream sync: fold finished records into the totals; records of
+	// another stream that has not reached them yet stay pending
 	void collect() {
+		std::vector<Rec> keep;
 		for (auto &r : pending) {
+			if (hipEventQuery(r.b) == hipErrorNotReady) {
+				keep.push_back(r);
+				continue;
+			}
 			float ms = 0.0f;
 			if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
 				auto &x = acc[r.name];
@@ -53,7 +59,8 @@ struct KernelTimer {
 			}
 			pool.push_back(r.a); pool.push_back(r.b);
 		}
-		pending.clear();
+		(void)hipGetLastError();   // a not-ready query is no error
+		pending.swap(keep);
 	}
 	~KernelTimer() {
 		for (auto &r : pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }

@@ -32,6 +32,13 @@ struct StepBuffers {
 
 #define PAIR_CLASSES 9         // type_a * 3 + type_b (type_a <= type_b)
 
+// The previous step's pair records in an open-addressing table by pair key
+// (B.tkeys / tvals / tmask, B.prev_recs, B.n_prev_recs, B.seen); they depend
+// only on the previous step, so this can run beside the broadphase.
+// n_prev_recs_cap bounds the records from above (the kernels read the count).
+hipError_t narrowphase_table(const DevWorld &W, const StepBuffers &B, int n_prev_recs_cap, hipStream_t stream,
+                             KernelTimer *timer);
+// The narrowphase over the candidate pairs (after narrowphase_table).
 hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 *pairs, const int32_t *n_pairs_dev,
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer);
 
@@ -139,12 +146,15 @@ struct SolveBuffers {
 	JointRec *jrec;
 };
 
-// Group the solved arbiters: the strips of the large worlds' grid sets, the
-// large worlds' arbiter pre-steps (and, without joints, their bodies'
-// velocity integration), then the arbiters sorted by key with the small
-// worlds' and the grid units' ranges.  Skipped for a single small space.
-// n_act bounds the solved arbiters from above (launch sizes); the kernels
-// read the count *S.n_act.
+// The strips of the large worlds' grid sets (each dynamic body's strip and
+// slot).  They depend only on the step's body positions, so they can run
+// beside the broadphase and the narrowphase.
+hipError_t solver_strips(const DevWorld &W, const SolveBuffers &S, hipStream_t stream, KernelTimer *timer);
+// Group the solved arbiters (after solver_strips): the large worlds' arbiter
+// pre-steps (and, without joints, their bodies' velocity integration), then
+// the arbiters sorted by key with the small worlds' and the grid units'
+// ranges.  Skipped for a single small space.  n_act bounds the solved
+// arbiters from above (launch sizes); the kernels read the count *S.n_act.
 hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
                         RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer);
 // solve units of the grid set: the strips, even seams and odd seams of every

@@ -290,19 +290,24 @@ inline int blocks_for(int n, int t) { return (n + t - 1) / t; }
 
 } // namespace
 
+hipError_t narrowphase_table(const DevWorld &W, const StepBuffers &B, int n_prev_recs_cap, hipStream_t stream,
+                             KernelTimer *timer)
+{
+	TIMED(timer, "pair_table", stream, {
+		k_table_clear<<<blocks_for(B.tmask + 1, 256), 256, 0, stream>>>(B.tkeys, B.tvals, B.tmask + 1);
+		if (n_prev_recs_cap > 0)
+			k_table_insert<<<std::min(blocks_for(n_prev_recs_cap, 256), 2048), 256, 0, stream>>>(
+			    B.prev_recs, B.n_prev_recs, B.tkeys, B.tvals, B.tmask, B.seen, W.err);
+	});
+	return hipGetLastError();
+}
+
 hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 *pairs, const int32_t *n_pairs_dev,
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer)
 {
 	HIP_TRY(hipMemsetAsync(B.n_recs, 0, sizeof(int32_t), stream));
 	HIP_TRY(hipMemsetAsync(B.n_act, 0, sizeof(int32_t), stream));
 	HIP_TRY(hipMemsetAsync(B.n_cached, 0, sizeof(int32_t), stream));
-	TIMED(timer, "pair_table", stream, {
-		k_table_clear<<<blocks_for(B.tmask + 1, 256), 256, 0, stream>>>(B.tkeys, B.tvals, B.tmask + 1);
-		if (n_prev_recs_host > 0)
-			k_table_insert<<<blocks_for(n_prev_recs_host, 256), 256, 0, stream>>>(B.prev_recs, B.n_prev_recs, B.tkeys,
-			                                                                      B.tvals, B.tmask, B.seen, W.err);
-	});
-	HIP_TRY(hipGetLastError());
 	if (n_pairs_host > 0) {
 		const int g = std::min(blocks_for(n_pairs_host, 256), 1024);
 		TIMED(timer, "pair_classes", stream, {

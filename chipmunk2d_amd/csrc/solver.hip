@@ -1277,23 +1277,26 @@ int solver_max_grid()
 	return per_cu * prop.multiProcessorCount;
 }
 
+hipError_t solver_strips(const DevWorld &W, const SolveBuffers &S, hipStream_t stream, KernelTimer *timer)
+{
+	const int nb = S.n_large_bodies;
+	if (S.n_large_worlds == 0 || nb == 0) return hipSuccess;
+	TIMED(timer, "strips", stream, {
+		k_part_init<<<std::max(1, std::min((S.n_large_worlds * PART_BINS + 255) / 256, GROUP_BLOCKS)), 256, 0, stream>>>(
+		    S);
+		k_part_bounds<<<(nb + PART_BOUNDS_THREADS - 1) / PART_BOUNDS_THREADS, PART_BOUNDS_THREADS, 0, stream>>>(W, S);
+		k_part_hist<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
+		k_part_scan<<<S.n_large_worlds, PART_BINS, 0, stream>>>(S);
+		k_part_assign<<<(nb + PART_ASSIGN_THREADS - 1) / PART_ASSIGN_THREADS, PART_ASSIGN_THREADS, 0, stream>>>(W, S);
+	});
+	return hipGetLastError();
+}
+
 hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int n_worlds, bool large,
                         RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer)
 {
 	// n_act: an upper bound of *S.n_act (the kernels read the count)
 	const int nb = S.n_large_bodies;
-	if (large) {
-		if (nb > 0)
-			TIMED(timer, "strips", stream, {
-				k_part_init<<<std::max(1, std::min((S.n_large_worlds * PART_BINS + 255) / 256, GROUP_BLOCKS)), 256, 0,
-				              stream>>>(S);
-				k_part_bounds<<<(nb + PART_BOUNDS_THREADS - 1) / PART_BOUNDS_THREADS, PART_BOUNDS_THREADS, 0, stream>>>(W, S);
-				k_part_hist<<<(nb + 255) / 256, 256, 0, stream>>>(W, S);
-				k_part_scan<<<S.n_large_worlds, PART_BINS, 0, stream>>>(S);
-				k_part_assign<<<(nb + PART_ASSIGN_THREADS - 1) / PART_ASSIGN_THREADS, PART_ASSIGN_THREADS, 0, stream>>>(W, S);
-			});
-		HIP_TRY(hipGetLastError());
-	}
 	if (n_act > 0) {
 		radix_sort_reserve(sortbuf, n_act);
 		HIP_TRY(hipMemsetAsync(sortbuf.orand, 0, sizeof(uint64_t), stream));

@@ -277,7 +277,7 @@ struct cpgpuWorld {
 	WorldBucket buckets[CPGPU_MAX_BUCKETS] = {};
 	int32_t *bucket_lists[CPGPU_MAX_BUCKETS] = {};
 	hipStream_t aux[CPGPU_MAX_BUCKETS] = {};
-	hipEvent_t fork_join[CPGPU_MAX_BUCKETS + 2] = {};
+	hipEvent_t fork_join[CPGPU_MAX_BUCKETS + 3] = {};   // the solver's fork and joins; the strips' fork and join
 	int32_t *d_large_bodies = nullptr;
 	int n_large_bodies = 0;
 	int max_iterations_large = 0;
@@ -519,7 +519,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(hipGetDevice(&w->device));
 	HIP_OR_FAIL(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
 	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++) HIP_OR_FAIL(hipStreamCreateWithFlags(&w->aux[q], hipStreamNonBlocking));
-	for (int q = 0; q < CPGPU_MAX_BUCKETS + 2; q++)
+	for (int q = 0; q < CPGPU_MAX_BUCKETS + 3; q++)
 		HIP_OR_FAIL(hipEventCreateWithFlags(&w->fork_join[q], hipEventDisableTiming));
 	w->nw = d->n_worlds; w->nb = d->n_bodies; w->ns = d->n_shapes; w->nc = d->n_constraints;
 	w->npv = d->n_poly_verts;
@@ -819,7 +819,7 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 		if (w->bucket_lists[q]) (void)hipFree(w->bucket_lists[q]);
 	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++)
 		if (w->aux[q]) (void)hipStreamDestroy(w->aux[q]);
-	for (int q = 0; q < CPGPU_MAX_BUCKETS + 2; q++)
+	for (int q = 0; q < CPGPU_MAX_BUCKETS + 3; q++)
 		if (w->fork_join[q]) (void)hipEventDestroy(w->fork_join[q]);
 	(void)hipStreamDestroy(w->stream);
 	delete w;
@@ -956,6 +956,20 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		                                                                             w->n_dyn_shapes, 1, 0));
 	HIP_OR_FAIL(hipGetLastError());
 
+	// the large worlds' strips from this step's positions, on an auxiliary stream beside the
+	// broadphase and the narrowphase
+	const int prev = w->cur ^ 1, cur = w->cur;
+	const bool strips = w->n_large_worlds > 0 && w->n_large_bodies > 0;
+	hipEvent_t strips_fork = w->fork_join[CPGPU_MAX_BUCKETS + 1], strips_join = w->fork_join[CPGPU_MAX_BUCKETS + 2];
+	if (strips) {
+		hipStream_t aux = w->aux[CPGPU_MAX_BUCKETS - 1];
+		HIP_OR_FAIL(hipEventRecord(strips_fork, w->stream));
+		HIP_OR_FAIL(hipStreamWaitEvent(aux, strips_fork, 0));
+		const SolveBuffers S0 = solve_buffers(w, w->act[cur], nullptr, w->colour, w->order, w->warb_off);
+		HIP_OR_FAIL(solver_strips(W, S0, aux, timer));
+		HIP_OR_FAIL(hipEventRecord(strips_join, aux));
+	}
+
 	// broadphase (one sync inside the radix sort)
 	HIP_OR_FAIL(lbvh_build(w->dyn_tree, W, w->d_dyn_shapes, w->n_dyn_shapes, w->nw, w->sortbuf, w->d_world_bounds,
 	                       w->stream, timer));
@@ -976,7 +990,6 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	w->total_pairs += n_pairs;
 
 	// capacities of this step's outputs from last step's counts
-	const int prev = w->cur ^ 1, cur = w->cur;
 	const int n_prev_recs = w->steps > 0 ? w->h_counts[3 * prev + 0] : 0;
 	const int n_prev_act = w->steps > 0 ? w->h_counts[3 * prev + 1] : 0;
 	const int n_prev_cached = w->steps > 0 ? w->h_counts[3 * prev + 2] : 0;
@@ -1017,10 +1030,12 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	HIP_OR_FAIL(regrow(w->d_pairs_by_type, w->pairs_by_type_cap, (long)n_pairs + 1));
 	B.pairs_by_type = w->d_pairs_by_type;
 	B.type_counts = w->d_type_counts;
+	HIP_OR_FAIL(narrowphase_table(W, B, n_prev_recs, w->stream, timer));
 	HIP_OR_FAIL(narrowphase_step(W, B, w->d_pairs, w->d_counters, n_pairs, n_prev_recs, w->stream, timer));
 
 	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
 	SolveBuffers S = solve_buffers(w, w->act[cur], B.n_act, w->colour, w->order, w->warb_off);
+	if (strips) HIP_OR_FAIL(hipStreamWaitEvent(w->stream, strips_join, 0));
 	if (grouped) {
 		// the solved arbiters by world and grid unit; launches sized for the capacity (at most one solved
 		// arbiter per candidate pair), the kernels read the count

@@ -330,6 +330,18 @@ __global__ void k_world_ranges(const uint64_t *keys, int n, int32_t *first, int3
 	if (f && l) root[w] = (n - 1) + i;
 }
 
+// a build's resets in one launch: no world has a root yet, the sort's key OR
+// and AND start at 0 and all ones, the whole tree's root has no parent
+__global__ void k_bvh_reset(int32_t *world_root, int n_worlds, unsigned long long *orand, int32_t *parent)
+{
+	for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_worlds; w += gridDim.x * blockDim.x) world_root[w] = -1;
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		orand[0] = 0ull;
+		orand[1] = ~0ull;
+		parent[0] = -1;
+	}
+}
+
 __global__ void k_karras(const uint64_t *keys, int n, int32_t *child, int32_t *parent, int32_t *range,
                          const int32_t *first, const int32_t *last, int32_t *root, PairNode *node)
 {
@@ -723,15 +735,15 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
                       RadixSortBuffers &sortbuf, real4 *world_bounds, hipStream_t stream, KernelTimer *timer)
 {
 	t.n = n;
-	HIP_TRY(hipMemsetAsync(t.world_root, 0xFF, sizeof(int32_t) * n_worlds, stream));
+	radix_sort_reserve(sortbuf, std::max(n, 1));   // before the reset: it writes the sort's key OR / AND
+	k_bvh_reset<<<std::min(blocks_for(n_worlds, 256), 64), 256, 0, stream>>>(
+	    t.world_root, n_worlds, (unsigned long long *)sortbuf.orand, t.parent);
+	HIP_TRY(hipGetLastError());
 	if (n == 0) return hipSuccess;
 	unsigned long long *wb = (unsigned long long *)world_bounds;
-	radix_sort_reserve(sortbuf, n);
 	TIMED(timer, "bvh_keys", stream, {
 		k_bounds_reset<<<blocks_for(n_worlds, 256), 256, 0, stream>>>(wb, n_worlds);
 		k_bounds<<<blocks_for(n, 256), 256, 0, stream>>>(W, shapes, n, wb);
-		(void)hipMemsetAsync(sortbuf.orand, 0, sizeof(uint64_t), stream);
-		(void)hipMemsetAsync(sortbuf.orand + 1, 0xFF, sizeof(uint64_t), stream);
 		k_keys<<<min(blocks_for(n, 256), KEYS_BLOCKS), 256, 0, stream>>>(W, shapes, n, wb, t.keys, t.leaf_shape,
 		                                                (unsigned long long *)sortbuf.orand);
 	});
@@ -741,7 +753,6 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 	while ((1 << world_bits) < n_worlds) world_bits++;
 	HIP_TRY(radix_sort_pairs(t.keys, t.leaf_shape, n, nullptr, WORLD_SHIFT + world_bits, sortbuf, stream, timer));
 	TIMED(timer, "bvh_build", stream, {
-		(void)hipMemsetAsync(t.parent, 0xFF, sizeof(int32_t) * 1, stream);
 		k_world_ranges<<<blocks_for(n, 256), 256, 0, stream>>>(t.keys, n, t.world_first, t.world_last,
 		                                                        t.world_root);
 		if (n > 1)

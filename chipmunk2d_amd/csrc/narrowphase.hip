@@ -305,9 +305,8 @@ hipError_t narrowphase_table(const DevWorld &W, const StepBuffers &B, int n_prev
 hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 *pairs, const int32_t *n_pairs_dev,
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer)
 {
-	HIP_TRY(hipMemsetAsync(B.n_recs, 0, sizeof(int32_t), stream));
-	HIP_TRY(hipMemsetAsync(B.n_act, 0, sizeof(int32_t), stream));
-	HIP_TRY(hipMemsetAsync(B.n_cached, 0, sizeof(int32_t), stream));
+	// the step's three counters are consecutive: records, solved arbiters, cached arbiters
+	HIP_TRY(hipMemsetAsync(B.n_recs, 0, 3 * sizeof(int32_t), stream));
 	if (n_pairs_host > 0) {
 		const int g = std::min(blocks_for(n_pairs_host, 256), 1024);
 		TIMED(timer, "pair_classes", stream, {

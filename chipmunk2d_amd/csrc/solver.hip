@@ -1199,6 +1199,13 @@ __global__ void __launch_bounds__(PART_ASSIGN_THREADS) k_part_assign(const DevWo
 	if (rel >= 0) S.bslot[b] = base[rel][strip] + local;
 }
 
+// the key OR and AND of the world sort start at 0 and all ones
+__global__ void k_orand_reset(unsigned long long *orand)
+{
+	orand[0] = 0ull;
+	orand[1] = ~0ull;
+}
+
 // Sort keys of the solved arbiters; the pre-steps of the large worlds'
 // arbiters (they only read velocities, before the joints' pre-steps and the
 // velocity integration, as in cpSpaceStep); OR / AND of the keys for the
@@ -1299,8 +1306,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 	const int nb = S.n_large_bodies;
 	if (n_act > 0) {
 		radix_sort_reserve(sortbuf, n_act);
-		HIP_TRY(hipMemsetAsync(sortbuf.orand, 0, sizeof(uint64_t), stream));
-		HIP_TRY(hipMemsetAsync(sortbuf.orand + 1, 0xFF, sizeof(uint64_t), stream));
+		k_orand_reset<<<1, 1, 0, stream>>>((unsigned long long *)sortbuf.orand);
 		TIMED(timer, "world_sort", stream,
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
 		          W, S, (unsigned long long *)sortbuf.orand));

@@ -1024,7 +1024,8 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	B.prev_recs = w->recs[prev]; B.n_prev_recs = w->d_step_counts + 3 * prev + 0;
 	B.seen = w->seen;
 	B.prev_act = w->act[prev]; B.prev_cached = w->cached[prev];
-	B.recs = w->recs[cur]; B.n_recs = w->d_step_counts + 3 * cur + 0; B.cap_recs = w->rec_cap[cur];
+	B.recs = w->recs[cur]; B.n_recs = w->d_step_counts + 3 * cur + 0; B.cap_recs = w->rec_cap[cur];   // n_recs, n_act and
+	// n_cached are consecutive (narrowphase_step clears the three at once)
 	B.act = w->act[cur]; B.n_act = w->d_step_counts + 3 * cur + 1; B.cap_act = w->act_cap[cur];
 	B.cached = w->cached[cur]; B.n_cached = w->d_step_counts + 3 * cur + 2; B.cap_cached = w->cached_cap[cur];
 	HIP_OR_FAIL(regrow(w->d_pairs_by_type, w->pairs_by_type_cap, (long)n_pairs + 1));

@@ -477,7 +477,17 @@ __global__ void k_refit_span(const uint64_t *keys, int n, const int32_t *range, 
 	if (cl == ch) return;
 	if ((keys[lo] >> WORLD_SHIFT) != (keys[hi] >> WORLD_SHIFT)) return;
 	real4 u = box_union(suf[lo], pre[hi]);
-	for (int c = cl + 1; c < ch; c++) u = box_union(u, cbox[c]);
+	// the whole chunks between, eight independent loads at a time (the top nodes span hundreds)
+	int c = cl + 1;
+	for (; c + 8 <= ch; c += 8) {
+		real4 b[8];
+#pragma unroll
+		for (int k = 0; k < 8; k++) b[k] = cbox[c + k];
+#pragma unroll
+		for (int k = 0; k < 4; k++) b[k] = box_union(b[k], b[k + 4]);
+		u = box_union(u, box_union(box_union(b[0], b[1]), box_union(b[2], b[3])));
+	}
+	for (; c < ch; c++) u = box_union(u, cbox[c]);
 	put_box(box, node, child, parent, p, u);
 }
 

@@ -198,38 +198,52 @@ __device__ __forceinline__ void arb_apply_cached(const DevWorld &W, const BA &bo
 	arb_apply_cached(W, bodies, arb, arb.ba, arb.bb, is_dynamic(W, arb.ba), is_dynamic(W, arb.bb));
 }
 
+// one contact's impulses (cpArbiterApplyImpulse's loop body): reads the
+// contact from `in`, writes its accumulated impulses to `out`
+__device__ __forceinline__ void contact_impulse(BV &a, BV &b, V2 n, V2 surface_vr, real friction, const Contact &in,
+                                                Contact &out)
+{
+	const real nMass = in.nMass;
+	const V2 r1 = v2(in.r1x, in.r1y), r2 = v2(in.r2x, in.r2y);
+	const V2 vb1 = vadd(a.vb, vmult(vperp(r1), a.wb));
+	const V2 vb2 = vadd(b.vb, vmult(vperp(r2), b.wb));
+	const V2 vr = vadd(rel_vel(a, b, r1, r2), surface_vr);
+	const real vbn = vdot(vsub(vb2, vb1), n);
+	const real vrn = vdot(vr, n);
+	const real vrt = vdot(vr, vperp(n));
+	const real jbn = (in.bias - vbn) * nMass;
+	const real jbnOld = in.jBias;
+	const real jBias = fmaxc(jbnOld + jbn, real(0.0));
+	const real jn = -(in.bounce + vrn) * nMass;
+	const real jnOld = in.jnAcc;
+	const real jnAcc = fmaxc(jnOld + jn, real(0.0));
+	const real jtMax = friction * jnAcc;
+	const real jt = -vrt * in.tMass;
+	const real jtOld = in.jtAcc;
+	const real jtAcc = fclampc(jtOld + jt, -jtMax, jtMax);
+	out.jBias = jBias;
+	out.jnAcc = jnAcc;
+	out.jtAcc = jtAcc;
+	const V2 jb = vmult(n, jBias - jbnOld);
+	apply_bias_impulse(a, vneg(jb), r1);
+	apply_bias_impulse(b, jb, r2);
+	apply_impulses(a, b, r1, r2, vrotate(n, v2(jnAcc - jnOld, jtAcc - jtOld)));
+}
+
+// Both contact slots are read before the count is known, so that every load
+// of the arbiter is in flight at once (a slot past the count is read and
+// left unused).
 template <class BA>
 __device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &arb, int ia, int ib, bool da, bool db)
 {
 	BV a = bodies.load(ia), b = bodies.load(ib);
+	const Contact c0 = arb.c[0], c1 = arb.c[1];
+	const int count = arb.count;
 	const V2 n = v2(arb.nx, arb.ny);
 	const V2 surface_vr = v2(arb.svx, arb.svy);
 	const real friction = arb.u;
-	for (int i = 0; i < arb.count; i++) {
-		Contact &con = arb.c[i];
-		const real nMass = con.nMass;
-		const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
-		const V2 vb1 = vadd(a.vb, vmult(vperp(r1), a.wb));
-		const V2 vb2 = vadd(b.vb, vmult(vperp(r2), b.wb));
-		const V2 vr = vadd(rel_vel(a, b, r1, r2), surface_vr);
-		const real vbn = vdot(vsub(vb2, vb1), n);
-		const real vrn = vdot(vr, n);
-		const real vrt = vdot(vr, vperp(n));
-		const real jbn = (con.bias - vbn) * nMass;
-		const real jbnOld = con.jBias;
-		con.jBias = fmaxc(jbnOld + jbn, real(0.0));
-		const real jn = -(con.bounce + vrn) * nMass;
-		const real jnOld = con.jnAcc;
-		con.jnAcc = fmaxc(jnOld + jn, real(0.0));
-		const real jtMax = friction * con.jnAcc;
-		const real jt = -vrt * con.tMass;
-		const real jtOld = con.jtAcc;
-		con.jtAcc = fclampc(jtOld + jt, -jtMax, jtMax);
-		const V2 jb = vmult(n, con.jBias - jbnOld);
-		apply_bias_impulse(a, vneg(jb), r1);
-		apply_bias_impulse(b, jb, r2);
-		apply_impulses(a, b, r1, r2, vrotate(n, v2(con.jnAcc - jnOld, con.jtAcc - jtOld)));
-	}
+	if (count > 0) contact_impulse(a, b, n, surface_vr, friction, c0, arb.c[0]);
+	if (count > 1) contact_impulse(a, b, n, surface_vr, friction, c1, arb.c[1]);
 	if (da) bodies.store(ia, a);
 	if (db) bodies.store(ib, b);
 }
@@ -855,6 +869,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	__shared__ int32_t lpos[GRID_LDS_POS];
 	__shared__ int2 lpb[GRID_LDS_POS];
 	__shared__ int32_t lbase[SCRATCH];   // a unit's first entry in lpos / lpb, -1 when not kept there
+	__shared__ int32_t liter[SCRATCH];   // iterations of the unit's space (a strip or seam lies in one space)
 	__shared__ unsigned long long lclaim[2 * GRID_LDS_BODIES];
 	__shared__ int32_t ltaken[GRID_LDS_BODIES];
 	int lds_used = 0;
@@ -912,6 +927,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			if (threadIdx.x == 0) {
 				linfo[s] = make_int2(start, nc);
 				lbase[s] = keep ? lds_used : -1;
+				liter[s] = n > 0 ? W.ws[S.act[S.arb_by_world[start]].world].iterations : 0;
 			}
 			if (keep) {
 				for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -953,15 +969,17 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 
 	// ---- sweeps ----
 	const GlobalBodies bodies{&W};
-	// f(arbiter, its bodies as kept in lpb) for the unit's arbiters, colour by colour
+	// f(arbiter, its bodies as kept in lpb, iterations of its space or -1 when the unit's
+	// arbiters may lie in several spaces) for the unit's arbiters, colour by colour
 	auto sweep_unit = [&](int u, int s, auto f) {
 		int2 info;
 		const int32_t *col;
-		int base = -1;
+		int base = -1, iters = -1;
 		if (s < SCRATCH) {
 			info = linfo[s];
 			col = lcol[s];
 			base = lbase[s];
+			iters = liter[s];
 		} else {
 			info = S.pinfo[u];
 			col = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
@@ -970,11 +988,12 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			const int c1 = col[q + 1];
 			for (int k = col[q] + (int)threadIdx.x; k < c1; k += (int)blockDim.x) {
 				if (base >= 0) {
-					f(lpos[base + k], lpb[base + k]);
+					f(lpos[base + k], lpb[base + k], iters);
 				} else {
 					const int a = S.order[info.x + k];
 					const Arbiter &arb = S.act[a];
-					f(a, make_int2(is_dynamic(W, arb.ba) ? arb.ba : ~arb.ba, is_dynamic(W, arb.bb) ? arb.bb : ~arb.bb));
+					f(a, make_int2(is_dynamic(W, arb.ba) ? arb.ba : ~arb.ba, is_dynamic(W, arb.bb) ? arb.bb : ~arb.bb),
+					  iters);
 				}
 			}
 			__syncthreads();
@@ -1035,14 +1054,14 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 		}
 	};
 	// cached impulses (cpSpaceStep.c:407-415)
-	sweep(0, [&](int a, int2 e) {
+	sweep(0, [&](int a, int2 e, int) {
 		arb_apply_cached(W, bodies, S.act[a], e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
 	}, [&](Joint &j) { joint_apply_cached(W, bodies, j); });
 	// the impulse iterations (cpSpaceStep.c:418-427)
 	for (int it = 0; it < S.max_iterations; it++) {
-		sweep(1 + it, [&](int a, int2 e) {
+		sweep(1 + it, [&](int a, int2 e, int iters) {
 			Arbiter &arb = S.act[a];
-			if (it < W.ws[arb.world].iterations)
+			if (it < (iters >= 0 ? iters : W.ws[arb.world].iterations))
 				arb_apply_impulse(W, bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
 		}, [&](Joint &j) {
 			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);

@@ -177,25 +177,20 @@ __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
 // solve keeps them in LDS) loads the bodies beside the arbiter instead of
 // after it.
 template <class BA>
-__device__ void arb_apply_cached(const DevWorld &W, const BA &bodies, const Arbiter &arb, int ia, int ib, bool da,
-                                 bool db)
+__device__ void arb_apply_cached(const BA &bodies, const Arbiter &arb, int ia, int ib, bool da, bool db, real dt_coef)
 {
-	if (arb.state == CPGPU_ARB_FIRST_COLLISION) return;
-	const real dt_coef = W.ws[arb.world].dt_coef;
+	// every load is issued before the state and count are known
 	BV a = bodies.load(ia), b = bodies.load(ib);
+	const Contact c0 = arb.c[0], c1 = arb.c[1];
+	const int count = arb.count;
+	if (arb.state == CPGPU_ARB_FIRST_COLLISION) return;
 	const V2 n = v2(arb.nx, arb.ny);
-	for (int i = 0; i < arb.count; i++) {
-		const Contact &con = arb.c[i];
-		const V2 j = vrotate(n, v2(con.jnAcc, con.jtAcc));
-		apply_impulses(a, b, v2(con.r1x, con.r1y), v2(con.r2x, con.r2y), vmult(j, dt_coef));
-	}
+	if (count > 0)
+		apply_impulses(a, b, v2(c0.r1x, c0.r1y), v2(c0.r2x, c0.r2y), vmult(vrotate(n, v2(c0.jnAcc, c0.jtAcc)), dt_coef));
+	if (count > 1)
+		apply_impulses(a, b, v2(c1.r1x, c1.r1y), v2(c1.r2x, c1.r2y), vmult(vrotate(n, v2(c1.jnAcc, c1.jtAcc)), dt_coef));
 	if (da) bodies.store(ia, a);
 	if (db) bodies.store(ib, b);
-}
-template <class BA>
-__device__ __forceinline__ void arb_apply_cached(const DevWorld &W, const BA &bodies, const Arbiter &arb)
-{
-	arb_apply_cached(W, bodies, arb, arb.ba, arb.bb, is_dynamic(W, arb.ba), is_dynamic(W, arb.bb));
 }
 
 // one contact's impulses (cpArbiterApplyImpulse's loop body): reads the
@@ -462,7 +457,8 @@ struct SolveSet {
 	const int32_t *jorder;     // joints in colour order
 	const int32_t *jcol_off;   // [njcol + 1]
 	int njcol;
-	int iterations;            // sweeps (elements of worlds with fewer sweeps skip the extra ones)
+	int iterations;            // sweeps of the set's space
+	real dt_coef;              // its cached-impulse scale (dt / previous dt, or 0 without warm starting)
 	BodyVel *lds_bodies;       // LDS room for lds_cap bodies, or nullptr
 	int lds_cap;
 	int unit;                  // solve unit whose contacts / bodies are counted (W.totals[2 + 2 unit ...])
@@ -558,7 +554,13 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
                                const BA &bodies)
 {
 	const int tid = sy.sweep_tid(), stride = sy.stride;
-	for_colours(sy, T.col_off, n_colours, [&](int k) { arb_apply_cached(W, bodies, S.act[T.order[k]]); });
+	// the sorted arbiters' bodies, written by the colour sort
+	const int2 *obody = S.cbody + T.order_base;
+	auto body_of = [](int e) { return e >= 0 ? e : ~e; };
+	for_colours(sy, T.col_off, n_colours, [&](int k) {
+		const int2 e = obody[k];
+		arb_apply_cached(bodies, S.act[T.order[k]], body_of(e.x), body_of(e.y), e.x >= 0, e.y >= 0, T.dt_coef);
+	});
 	for (int jc = 0; jc < T.njcol; jc++) {
 		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
 			if (S.jrec) record_joint(S.jrec + k, bodies, W.joints[T.jorder[k]]);
@@ -567,12 +569,12 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 		sy.sync();
 	}
 	for (int it = 0; it < T.iterations; it++) {
+		// every arbiter of the set is of its one space, whose iterations bound the loop
 		for_colours(sy, T.col_off, n_colours, [&](int k) {
+			const int2 e = obody[k];
 			Arbiter &arb = S.act[T.order[k]];
-			if (it < W.ws[arb.world].iterations) {
-				if (S.rec) record_arbiter(S.rec + (size_t)it * S.rec_stride + T.order_base + k, bodies, arb);
-				arb_apply_impulse(W, bodies, arb);
-			}
+			if (S.rec) record_arbiter(S.rec + (size_t)it * S.rec_stride + T.order_base + k, bodies, arb);
+			arb_apply_impulse(W, bodies, arb, body_of(e.x), body_of(e.y), e.x >= 0, e.y >= 0);
 		});
 		for (int jc = 0; jc < T.njcol; jc++) {
 			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
@@ -704,8 +706,16 @@ __device__ __forceinline__ int colour_set(const DevWorld &W, const SolveBuffers 
 		T.col_off[n_colours] = run;
 	}
 	__syncthreads();
+	// beside each sorted arbiter, its bodies (~b for one that is not dynamic) in the
+	// colouring's body array, which the colouring no longer reads
 	each([&](int k, int col) {
-		if (col >= 0) T.order[atomicAdd(&T.col_cursor[col], 1)] = arb_index(k);
+		if (col >= 0) {
+			const int a = arb_index(k);
+			const Arbiter &arb = S.act[a];
+			const int pos = atomicAdd(&T.col_cursor[col], 1);
+			T.order[pos] = a;
+			cbody[pos] = make_int2(is_dynamic(W, arb.ba) ? arb.ba : ~arb.ba, is_dynamic(W, arb.bb) ? arb.bb : ~arb.bb);
+		}
 	});
 	return n_colours;
 }
@@ -800,6 +810,7 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 		T.jcol_off = S.jw_col + S.jw_cbase[w];
 		T.njcol = S.jw_ncol[w];
 		T.iterations = W.ws[w].iterations;
+		T.dt_coef = W.ws[w].dt_coef;
 		solve_set(W, S, T, sy);
 	}
 }
@@ -870,6 +881,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	__shared__ int2 lpb[GRID_LDS_POS];
 	__shared__ int32_t lbase[SCRATCH];   // a unit's first entry in lpos / lpb, -1 when not kept there
 	__shared__ int32_t liter[SCRATCH];   // iterations of the unit's space (a strip or seam lies in one space)
+	__shared__ real ldt[SCRATCH];        // its cached-impulse scale
 	__shared__ unsigned long long lclaim[2 * GRID_LDS_BODIES];
 	__shared__ int32_t ltaken[GRID_LDS_BODIES];
 	int lds_used = 0;
@@ -927,7 +939,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			if (threadIdx.x == 0) {
 				linfo[s] = make_int2(start, nc);
 				lbase[s] = keep ? lds_used : -1;
-				liter[s] = n > 0 ? W.ws[S.act[S.arb_by_world[start]].world].iterations : 0;
+				const int wd = n > 0 ? S.act[S.arb_by_world[start]].world : 0;
+				liter[s] = W.ws[wd].iterations;
+				ldt[s] = W.ws[wd].dt_coef;
 			}
 			if (keep) {
 				for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -969,17 +983,16 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 
 	// ---- sweeps ----
 	const GlobalBodies bodies{&W};
-	// f(arbiter, its bodies as kept in lpb, iterations of its space or -1 when the unit's
+	// f(arbiter, its bodies as kept in lpb, the unit's LDS slot: SCRATCH when the unit's
 	// arbiters may lie in several spaces) for the unit's arbiters, colour by colour
 	auto sweep_unit = [&](int u, int s, auto f) {
 		int2 info;
 		const int32_t *col;
-		int base = -1, iters = -1;
+		int base = -1;
 		if (s < SCRATCH) {
 			info = linfo[s];
 			col = lcol[s];
 			base = lbase[s];
-			iters = liter[s];
 		} else {
 			info = S.pinfo[u];
 			col = S.pcol + (size_t)u * (CPGPU_MAX_COLOURS + 1);
@@ -988,12 +1001,12 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			const int c1 = col[q + 1];
 			for (int k = col[q] + (int)threadIdx.x; k < c1; k += (int)blockDim.x) {
 				if (base >= 0) {
-					f(lpos[base + k], lpb[base + k], iters);
+					f(lpos[base + k], lpb[base + k], s);
 				} else {
 					const int a = S.order[info.x + k];
 					const Arbiter &arb = S.act[a];
 					f(a, make_int2(is_dynamic(W, arb.ba) ? arb.ba : ~arb.ba, is_dynamic(W, arb.bb) ? arb.bb : ~arb.bb),
-					  iters);
+					  s);
 				}
 			}
 			__syncthreads();
@@ -1054,14 +1067,16 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 		}
 	};
 	// cached impulses (cpSpaceStep.c:407-415)
-	sweep(0, [&](int a, int2 e, int) {
-		arb_apply_cached(W, bodies, S.act[a], e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
+	sweep(0, [&](int a, int2 e, int s) {
+		const Arbiter &arb = S.act[a];
+		arb_apply_cached(bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0,
+		                 s < SCRATCH ? ldt[s] : W.ws[arb.world].dt_coef);
 	}, [&](Joint &j) { joint_apply_cached(W, bodies, j); });
 	// the impulse iterations (cpSpaceStep.c:418-427)
 	for (int it = 0; it < S.max_iterations; it++) {
-		sweep(1 + it, [&](int a, int2 e, int iters) {
+		sweep(1 + it, [&](int a, int2 e, int s) {
 			Arbiter &arb = S.act[a];
-			if (it < (iters >= 0 ? iters : W.ws[arb.world].iterations))
+			if (it < (s < SCRATCH ? liter[s] : W.ws[arb.world].iterations))
 				arb_apply_impulse(W, bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
 		}, [&](Joint &j) {
 			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);

@@ -324,11 +324,11 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 	}
 }
 
+// dt_coef: the joint's space's cached-impulse scale
 template <class BA>
-__device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Joint &j)
+__device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Joint &j, real dt_coef)
 {
 	if (j.type == CPGPU_JOINT_DAMPED_SPRING) return;
-	const real dt_coef = W.ws[j.world].dt_coef;
 	BV a = bodies.load(j.a), b = bodies.load(j.b);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
@@ -345,10 +345,10 @@ __device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Jo
 	if (is_dynamic(W, j.b)) bodies.store(j.b, b);
 }
 
+// dt: the joint's space's time step
 template <class BA>
-__device__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &j)
+__device__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &j, real dt)
 {
-	const real dt = W.ws[j.world].dt;
 	BV a = bodies.load(j.a), b = bodies.load(j.b);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
@@ -459,6 +459,7 @@ struct SolveSet {
 	int njcol;
 	int iterations;            // sweeps of the set's space
 	real dt_coef;              // its cached-impulse scale (dt / previous dt, or 0 without warm starting)
+	real dt;                   // its time step
 	BodyVel *lds_bodies;       // LDS room for lds_cap bodies, or nullptr
 	int lds_cap;
 	int unit;                  // solve unit whose contacts / bodies are counted (W.totals[2 + 2 unit ...])
@@ -564,7 +565,7 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 	for (int jc = 0; jc < T.njcol; jc++) {
 		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
 			if (S.jrec) record_joint(S.jrec + k, bodies, W.joints[T.jorder[k]]);
-			joint_apply_cached(W, bodies, W.joints[T.jorder[k]]);
+			joint_apply_cached(W, bodies, W.joints[T.jorder[k]], T.dt_coef);
 		}
 		sy.sync();
 	}
@@ -579,10 +580,8 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 		for (int jc = 0; jc < T.njcol; jc++) {
 			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
 				Joint &j = W.joints[T.jorder[k]];
-				if (it < W.ws[j.world].iterations) {
-					if (S.jrec) record_joint(S.jrec + (size_t)(it + 1) * W.n_joints + k, bodies, j);
-					joint_apply_impulse(W, bodies, j);
-				}
+				if (S.jrec) record_joint(S.jrec + (size_t)(it + 1) * W.n_joints + k, bodies, j);
+				joint_apply_impulse(W, bodies, j, T.dt);
 			}
 			sy.sync();
 		}
@@ -811,6 +810,7 @@ __global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, c
 		T.njcol = S.jw_ncol[w];
 		T.iterations = W.ws[w].iterations;
 		T.dt_coef = W.ws[w].dt_coef;
+		T.dt = W.ws[w].dt;
 		solve_set(W, S, T, sy);
 	}
 }
@@ -1071,7 +1071,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 		const Arbiter &arb = S.act[a];
 		arb_apply_cached(bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0,
 		                 s < SCRATCH ? ldt[s] : W.ws[arb.world].dt_coef);
-	}, [&](Joint &j) { joint_apply_cached(W, bodies, j); });
+	}, [&](Joint &j) { joint_apply_cached(W, bodies, j, W.ws[j.world].dt_coef); });
 	// the impulse iterations (cpSpaceStep.c:418-427)
 	for (int it = 0; it < S.max_iterations; it++) {
 		sweep(1 + it, [&](int a, int2 e, int s) {
@@ -1079,7 +1079,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			if (it < (s < SCRATCH ? liter[s] : W.ws[arb.world].iterations))
 				arb_apply_impulse(W, bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
 		}, [&](Joint &j) {
-			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j);
+			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j, W.ws[j.world].dt);
 		});
 	}
 }

@@ -118,6 +118,7 @@ struct SolveBuffers {
 	int n_jcolours;
 	// ... and every small world on its own
 	const int32_t *jw_order;   // joints sorted by (world, colour)
+	const int2 *jw_body;       // their bodies in that order (~b for a body that is not dynamic)
 	const int32_t *jw_col;     // per world: n colours + 1 offsets into jw_order
 	const int32_t *jw_cbase;   // [n_worlds] first entry of the world's offsets in jw_col
 	const int32_t *jw_ncol;    // [n_worlds]

@@ -324,12 +324,14 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 	}
 }
 
-// dt_coef: the joint's space's cached-impulse scale
+// dt_coef: the joint's space's cached-impulse scale; ia, ib: its bodies, da, db: whether
+// they are dynamic (given by a caller that has them at hand, so the bodies load beside the
+// joint instead of after it)
 template <class BA>
-__device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Joint &j, real dt_coef)
+__device__ void joint_apply_cached(const BA &bodies, const Joint &j, real dt_coef, int ia, int ib, bool da, bool db)
 {
+	BV a = bodies.load(ia), b = bodies.load(ib);
 	if (j.type == CPGPU_JOINT_DAMPED_SPRING) return;
-	BV a = bodies.load(j.a), b = bodies.load(j.b);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
 		apply_impulses(a, b, r1, r2, vmult(v2(j.jAx, j.jAy), dt_coef));
@@ -341,15 +343,20 @@ __device__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Jo
 		a.w -= jj * a.i_inv * ratio_inv;
 		b.w += jj * b.i_inv;
 	}
-	if (is_dynamic(W, j.a)) bodies.store(j.a, a);
-	if (is_dynamic(W, j.b)) bodies.store(j.b, b);
+	if (da) bodies.store(ia, a);
+	if (db) bodies.store(ib, b);
+}
+template <class BA>
+__device__ __forceinline__ void joint_apply_cached(const DevWorld &W, const BA &bodies, const Joint &j, real dt_coef)
+{
+	joint_apply_cached(bodies, j, dt_coef, j.a, j.b, is_dynamic(W, j.a), is_dynamic(W, j.b));
 }
 
-// dt: the joint's space's time step
+// dt: the joint's space's time step; ia, ib, da, db as for joint_apply_cached
 template <class BA>
-__device__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &j, real dt)
+__device__ void joint_apply_impulse(const BA &bodies, Joint &j, real dt, int ia, int ib, bool da, bool db)
 {
-	BV a = bodies.load(j.a), b = bodies.load(j.b);
+	BV a = bodies.load(ia), b = bodies.load(ib);
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
 	if (j.type == CPGPU_JOINT_PIVOT) {
 		const V2 vr = rel_vel(a, b, r1, r2);
@@ -389,8 +396,13 @@ __device__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &
 		a.w -= jj * a.i_inv * ratio_inv;
 		b.w += jj * b.i_inv;
 	}
-	if (is_dynamic(W, j.a)) bodies.store(j.a, a);
-	if (is_dynamic(W, j.b)) bodies.store(j.b, b);
+	if (da) bodies.store(ia, a);
+	if (db) bodies.store(ib, b);
+}
+template <class BA>
+__device__ __forceinline__ void joint_apply_impulse(const DevWorld &W, const BA &bodies, Joint &j, real dt)
+{
+	joint_apply_impulse(bodies, j, dt, j.a, j.b, is_dynamic(W, j.a), is_dynamic(W, j.b));
 }
 
 // ---- the solve routine, shared by the workgroup-per-world and grid kernels ----
@@ -564,8 +576,9 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 	});
 	for (int jc = 0; jc < T.njcol; jc++) {
 		for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
+			const int2 e = S.jw_body[k];
 			if (S.jrec) record_joint(S.jrec + k, bodies, W.joints[T.jorder[k]]);
-			joint_apply_cached(W, bodies, W.joints[T.jorder[k]], T.dt_coef);
+			joint_apply_cached(bodies, W.joints[T.jorder[k]], T.dt_coef, body_of(e.x), body_of(e.y), e.x >= 0, e.y >= 0);
 		}
 		sy.sync();
 	}
@@ -579,9 +592,10 @@ __device__ void solve_impulses(const DevWorld &W, const SolveBuffers &S, const S
 		});
 		for (int jc = 0; jc < T.njcol; jc++) {
 			for (int k = T.jcol_off[jc] + tid; k < T.jcol_off[jc + 1]; k += stride) {
+				const int2 e = S.jw_body[k];
 				Joint &j = W.joints[T.jorder[k]];
 				if (S.jrec) record_joint(S.jrec + (size_t)(it + 1) * W.n_joints + k, bodies, j);
-				joint_apply_impulse(W, bodies, j, T.dt);
+				joint_apply_impulse(bodies, j, T.dt, body_of(e.x), body_of(e.y), e.x >= 0, e.y >= 0);
 			}
 			sy.sync();
 		}

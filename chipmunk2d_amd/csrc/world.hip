@@ -266,6 +266,7 @@ struct cpgpuWorld {
 	int32_t *taken = nullptr;
 	int32_t *d_jorder = nullptr, *d_jcol_off = nullptr;
 	int32_t *d_jw_order = nullptr, *d_jw_col = nullptr, *d_jw_cbase = nullptr, *d_jw_ncol = nullptr;
+	int2 *d_jw_body = nullptr;
 	// small / large worlds
 	int small_world_max_bodies = 4096;
 	uint8_t *d_world_small = nullptr;
@@ -738,6 +739,14 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		HIP_OR_FAIL(own_upload(w, w->d_jorder, jorder.data(), jorder.size()));
 		HIP_OR_FAIL(own_upload(w, w->d_jcol_off, joff.data(), joff.size()));
 		HIP_OR_FAIL(own_upload(w, w->d_jw_order, jw_order.data(), jw_order.size()));
+		// beside each, its bodies (~b for a body that is not dynamic): body types are fixed
+		std::vector<int2> jw_body(nc);
+		for (int p = 0; p < nc; p++) {
+			const Joint &j = J[jw_order[p]];
+			const bool da = d->body_type[j.a] == CPGPU_BODY_DYNAMIC, db = d->body_type[j.b] == CPGPU_BODY_DYNAMIC;
+			jw_body[p] = make_int2(da ? j.a : ~j.a, db ? j.b : ~j.b);
+		}
+		HIP_OR_FAIL(own_upload(w, w->d_jw_body, jw_body.data(), jw_body.size()));
 		HIP_OR_FAIL(own_upload(w, w->d_jw_col, jw_col.data(), jw_col.size()));
 		HIP_OR_FAIL(own_upload(w, w->d_jw_cbase, jw_cbase.data(), jw_cbase.size()));
 		HIP_OR_FAIL(own_upload(w, w->d_jw_ncol, jw_ncol.data(), jw_ncol.size()));
@@ -841,7 +850,7 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.world_small = w->d_world_small;
 	S.claim = w->claim; S.taken = w->taken;
 	S.jorder = w->d_jorder; S.jcol_off = w->d_jcol_off; S.n_jcolours = w->n_jcolours;
-	S.jw_order = w->d_jw_order; S.jw_col = w->d_jw_col; S.jw_cbase = w->d_jw_cbase; S.jw_ncol = w->d_jw_ncol;
+	S.jw_order = w->d_jw_order; S.jw_body = w->d_jw_body; S.jw_col = w->d_jw_col; S.jw_cbase = w->d_jw_cbase; S.jw_ncol = w->d_jw_ncol;
 	S.bar = w->bar;
 	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;

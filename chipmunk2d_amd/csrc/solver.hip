@@ -470,7 +470,7 @@ struct SolveSet {
 	const int32_t *jcol_off;   // [njcol + 1]
 	int njcol;
 	int iterations;            // sweeps of the set's space
-	real dt_coef;              // its cached-impulse scale (dt / previous dt, or 0 without warm starting)
+	real dt_coef;              // its cached-impulse scale (dt / previous dt, 0 on the space's first step)
 	real dt;                   // its time step
 	BodyVel *lds_bodies;       // LDS room for lds_cap bodies, or nullptr
 	int lds_cap;
@@ -997,8 +997,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 
 	// ---- sweeps ----
 	const GlobalBodies bodies{&W};
-	// f(arbiter, its bodies as kept in lpb, the unit's LDS slot: SCRATCH when the unit's
-	// arbiters may lie in several spaces) for the unit's arbiters, colour by colour
+	// f(arbiter, its bodies as kept in lpb, the unit's LDS slot: SCRATCH for a unit kept in
+	// global memory, whose arbiters' spaces are then read per arbiter) for the unit's
+	// arbiters, colour by colour
 	auto sweep_unit = [&](int u, int s, auto f) {
 		int2 info;
 		const int32_t *col;

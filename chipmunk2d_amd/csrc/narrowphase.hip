@@ -16,6 +16,8 @@
 #include "narrowphase.h"
 #include "cpgpu_step.h"
 
+#include <type_traits>
+
 namespace CPGPU_NS {
 
 namespace {
@@ -91,9 +93,15 @@ __device__ __forceinline__ int32_t carry_arbiter(const StepBuffers &B, const Dev
 	if (ticks >= ws.persistence) return -1;
 	const int m = append_index(B.n_cached);
 	if (m >= B.cap_cached) { raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); return -1; }
-	Arbiter a = *old;
-	a.state = CPGPU_ARB_CACHED;
-	B.cached[m] = a;
+	// copied in 16- (or 8-) byte pieces straight from the old arbiter (a local copy of the
+	// whole record would sit in scratch memory), then marked cached
+	using Piece = typename std::conditional<sizeof(Arbiter) % 16 == 0, uint4, uint2>::type;
+	static_assert(sizeof(Arbiter) % sizeof(Piece) == 0, "arbiters copy in whole pieces");
+	const Piece *src = reinterpret_cast<const Piece *>(old);
+	Piece *dst = reinterpret_cast<Piece *>(&B.cached[m]);
+#pragma unroll
+	for (int i = 0; i < (int)(sizeof(Arbiter) / sizeof(Piece)); i++) dst[i] = src[i];
+	B.cached[m].state = CPGPU_ARB_CACHED;
 	return m | CACHED_TAG;
 }
 

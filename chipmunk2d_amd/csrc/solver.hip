@@ -789,8 +789,10 @@ __device__ __forceinline__ int lower_bound_u64(const uint64_t *keys, int lo, int
 
 // One workgroup per small world: every synchronisation is a workgroup barrier.
 template <int THREADS>
-__global__ void __launch_bounds__(THREADS, 4) k_solve_worlds(const DevWorld W, const SolveBuffers S,
-                                                            const int32_t *worlds, int lds_cap, int unit)
+// At least 4 waves a SIMD for the 1024-thread size (which needs them); the smaller sizes,
+// whose occupancy their spaces' LDS bodies bound, may take the registers to run unspilled.
+__global__ void __launch_bounds__(THREADS, THREADS >= 1024 ? 4 : 2)
+	k_solve_worlds(const DevWorld W, const SolveBuffers S, const int32_t *worlds, int lds_cap, int unit)
 {
 	extern __shared__ BodyVel lds_bodies[];
 	__shared__ int32_t col_off[CPGPU_MAX_COLOURS + 1];

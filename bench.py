@@ -146,10 +146,18 @@ def measured_traffic(workload, kernel, precision="fp64"):
     return None
 
 
-def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60, precision=0):
+def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60, precision=0, bucket_threads=None):
     """Warm-up (and, for batches, the autotuner, once the copy has run `autotune_at` steps so that piles have
-    formed) on a throwaway copy of the world, then `steps` timed steps on a fresh copy with the tuned tiling."""
+    formed) on a throwaway copy of the world, then `steps` timed steps on a fresh copy with the tuned tiling.
+    bucket_threads (four workgroup sizes) fixes the buckets' sizes in place of the autotuner."""
     from chipmunk2d_amd.gpu import GPUWorld
+
+    def fix_buckets(g):
+        if bucket_threads:
+            g.set_buckets([b[0] for b in g.buckets()], bucket_threads)
+
+    if bucket_threads:
+        autotune_steps = 0
     tiling = None
     if warmup > 0 or autotune_steps > 0:
         g = GPUWorld(pw, precision)
@@ -164,6 +172,7 @@ def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60, precision=0
     g = GPUWorld(pw, precision)
     if autotune_steps > 0:
         g.set_buckets([b[0] for b in tiling], [b[1] for b in tiling])
+    fix_buckets(g)
     tiling = g.buckets()
     g.sync()
     # inside the measured run only the solver's launches carry HIP events
@@ -184,6 +193,7 @@ def run_gpu(pw, steps, warmup, dt, autotune_steps=0, autotune_at=60, precision=0
     g = GPUWorld(pw, precision)
     if autotune_steps > 0:
         g.set_buckets([b[0] for b in tiling], [b[1] for b in tiling])
+    fix_buckets(g)
     g.set_timing(True)
     g.step(dt, steps)
     g.sync()
@@ -300,6 +310,11 @@ def main():
     ap.add_argument("--cpu-batch-sample-steps", type=int, default=20,
                     help="steps of a batch rollout the reference CPU step is timed on (all worlds)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain-baseline", action="store_true",
+                    help="skip the batch's comparison with its spaces stepped one at a time")
+    ap.add_argument("--bucket-threads", default=None,
+                    help="four comma-separated workgroup sizes for the small-space buckets, in place of the "
+                         "autotuner (to profile the sizes a measured run chose)")
     ap.add_argument("--autotune-steps", type=int, default=3,
                     help="steps per trial of the batch tiling autotuner (on the warm-up copy; 0: default tiling)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
@@ -338,8 +353,15 @@ def main():
                       "fwd_adjoint_steps_per_s": round(args.steps / wall, 2),
                       "forward_only_body_steps_per_s": round(n_dyn * args.steps / t_fwd, 1)}
     else:
+        bucket_threads = [int(x) for x in args.bucket_threads.split(",")] if args.bucket_threads else None
+        if bucket_threads is not None and len(bucket_threads) != 4:
+            print("--bucket-threads takes four workgroup sizes", file=sys.stderr)
+            sys.exit(2)
+        if bucket_threads:
+            autotune = 0
         wall, t_steps, timing, profile, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt,
-                                                                               autotune, precision=precision)
+                                                                               autotune, precision=precision,
+                                                                               bucket_threads=bucket_threads)
     if tiling:
         config["tiling"] = {"autotuned": bool(autotune), "buckets": [
             {"bucket": i, "max_bodies": b[0], "threads": b[1], "worlds": b[2], "lds_bodies": b[3]}
@@ -393,7 +415,7 @@ def main():
                                 "note": "cpHastySpace clamps its solver to 2 threads (cpHastySpace.c:387,534)"},
             }
     plain = None
-    if args.workload == "batch_4096":
+    if args.workload == "batch_4096" and not args.no_plain_baseline:
         # the same spaces stepped one at a time by the plain per-space GPU step (each its own world)
         rate, bodies, secs = run_plain_per_space(pw, 64, 20, dt)
         plain = {"value": round(rate, 1), "unit": "body-steps/s",

@@ -328,6 +328,15 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would step its own "
               "slice of worlds independently); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    bucket_threads = None
+    if args.bucket_threads:
+        try:
+            bucket_threads = [int(x) for x in args.bucket_threads.split(",")]
+        except ValueError:
+            bucket_threads = []
+        if len(bucket_threads) != 4 or any(t not in (64, 128, 256, 512, 1024) for t in bucket_threads):
+            print("--bucket-threads takes four workgroup sizes, each 64, 128, 256, 512 or 1024", file=sys.stderr)
+            sys.exit(2)
 
     dt = 1.0 / 60.0
     pw, config = build_workload(args.workload)
@@ -353,10 +362,6 @@ def main():
                       "fwd_adjoint_steps_per_s": round(args.steps / wall, 2),
                       "forward_only_body_steps_per_s": round(n_dyn * args.steps / t_fwd, 1)}
     else:
-        bucket_threads = [int(x) for x in args.bucket_threads.split(",")] if args.bucket_threads else None
-        if bucket_threads is not None and len(bucket_threads) != 4:
-            print("--bucket-threads takes four workgroup sizes", file=sys.stderr)
-            sys.exit(2)
         if bucket_threads:
             autotune = 0
         wall, t_steps, timing, profile, stats, solve_counts, tiling = run_gpu(pw, args.steps, args.warmup, dt,

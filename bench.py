@@ -315,7 +315,7 @@ def main():
     ap.add_argument("--bucket-threads", default=None,
                     help="four comma-separated workgroup sizes for the small-space buckets, in place of the "
                          "autotuner (to profile the sizes a measured run chose)")
-    ap.add_argument("--autotune-steps", type=int, default=3,
+    ap.add_argument("--autotune-steps", type=int, default=6,
                     help="steps per trial of the batch tiling autotuner (on the warm-up copy; 0: default tiling)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
                     help="world precision (CPGPU_FP64: bit-for-bit the reference's double build; CPGPU_FP32)")

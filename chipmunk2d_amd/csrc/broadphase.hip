@@ -659,19 +659,26 @@ inline int blocks_for(int n, int t) { return (n + t - 1) / t; }
 
 // ---------------------------------------------------------------------------
 
-void radix_sort_reserve(RadixSortBuffers &b, int n)
+hipError_t radix_sort_reserve(RadixSortBuffers &b, int n)
 {
-	if (n <= b.cap) return;
+	if (n <= b.cap) return hipSuccess;
 	radix_sort_free(b);
 	int cap = 1024;
 	while (cap < n) cap *= 2;
 	const int nblocks = blocks_for(cap, RS_TILE);
-	(void)hipMalloc(&b.keys_alt, sizeof(uint64_t) * cap);
-	(void)hipMalloc(&b.vals_alt, sizeof(int32_t) * cap);
-	(void)hipMalloc(&b.block_hist, sizeof(uint32_t) * 256 * nblocks);
-	(void)hipMalloc(&b.orand, sizeof(uint64_t) * 2);
-	(void)hipMalloc(&b.bin_total, sizeof(uint32_t) * 256);
+	// on a failed allocation every buffer is released and cap stays 0, so the
+	// next reserve tries again and no kernel is given a stale pointer
+	hipError_t e = hipMalloc(&b.keys_alt, sizeof(uint64_t) * cap);
+	if (e == hipSuccess) e = hipMalloc(&b.vals_alt, sizeof(int32_t) * cap);
+	if (e == hipSuccess) e = hipMalloc(&b.block_hist, sizeof(uint32_t) * 256 * nblocks);
+	if (e == hipSuccess) e = hipMalloc(&b.orand, sizeof(uint64_t) * 2);
+	if (e == hipSuccess) e = hipMalloc(&b.bin_total, sizeof(uint32_t) * 256);
+	if (e != hipSuccess) {
+		radix_sort_free(b);
+		return e;
+	}
 	b.cap = cap;
+	return hipSuccess;
 }
 
 void radix_sort_free(RadixSortBuffers &b)
@@ -711,26 +718,31 @@ hipError_t radix_sort_pairs(uint64_t *keys, int32_t *vals, int n_cap, const int3
 	return hipSuccess;
 }
 
-void lbvh_reserve(Lbvh &t, int n, int n_worlds)
+hipError_t lbvh_reserve(Lbvh &t, int n, int n_worlds)
 {
-	if (n <= t.cap && t.world_root) return;
+	if (n <= t.cap && t.world_root) return hipSuccess;
 	lbvh_free(t);
 	int cap = 1024;
 	while (cap < n) cap *= 2;
-	(void)hipMalloc(&t.keys, sizeof(uint64_t) * cap);
-	(void)hipMalloc(&t.leaf_shape, sizeof(int32_t) * cap);
-	(void)hipMalloc(&t.child, sizeof(int32_t) * 2 * cap);
-	(void)hipMalloc(&t.parent, sizeof(int32_t) * 2 * cap);
-	(void)hipMalloc(&t.range, sizeof(int32_t) * 2 * cap);
-	(void)hipMalloc(&t.box, sizeof(real4) * 2 * cap);
-	(void)hipMalloc(&t.node, sizeof(PairNode) * cap);
-	(void)hipMalloc(&t.pre, sizeof(real4) * cap);
-	(void)hipMalloc(&t.suf, sizeof(real4) * cap);
-	(void)hipMalloc(&t.cbox, sizeof(real4) * (cap / REFIT_CHUNK + 1));
-	(void)hipMalloc(&t.world_root, sizeof(int32_t) * n_worlds);
-	(void)hipMalloc(&t.world_first, sizeof(int32_t) * n_worlds);
-	(void)hipMalloc(&t.world_last, sizeof(int32_t) * n_worlds);
+	hipError_t e = hipMalloc(&t.keys, sizeof(uint64_t) * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.leaf_shape, sizeof(int32_t) * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.child, sizeof(int32_t) * 2 * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.parent, sizeof(int32_t) * 2 * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.range, sizeof(int32_t) * 2 * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.box, sizeof(real4) * 2 * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.node, sizeof(PairNode) * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.pre, sizeof(real4) * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.suf, sizeof(real4) * cap);
+	if (e == hipSuccess) e = hipMalloc(&t.cbox, sizeof(real4) * (cap / REFIT_CHUNK + 1));
+	if (e == hipSuccess) e = hipMalloc(&t.world_root, sizeof(int32_t) * n_worlds);
+	if (e == hipSuccess) e = hipMalloc(&t.world_first, sizeof(int32_t) * n_worlds);
+	if (e == hipSuccess) e = hipMalloc(&t.world_last, sizeof(int32_t) * n_worlds);
+	if (e != hipSuccess) {
+		lbvh_free(t);   // cap stays 0 and world_root null: the next reserve tries again
+		return e;
+	}
 	t.cap = cap;
+	return hipSuccess;
 }
 
 void lbvh_free(Lbvh &t)
@@ -745,7 +757,7 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
                       RadixSortBuffers &sortbuf, real4 *world_bounds, hipStream_t stream, KernelTimer *timer)
 {
 	t.n = n;
-	radix_sort_reserve(sortbuf, std::max(n, 1));   // before the reset: it writes the sort's key OR / AND
+	HIP_TRY(radix_sort_reserve(sortbuf, std::max(n, 1)));   // before the reset: it writes the sort's key OR / AND
 	k_bvh_reset<<<std::min(blocks_for(n_worlds, 256), 64), 256, 0, stream>>>(
 	    t.world_root, n_worlds, (unsigned long long *)sortbuf.orand, t.parent);
 	HIP_TRY(hipGetLastError());

@@ -83,7 +83,7 @@ struct RadixSortBuffers {
 	uint32_t *bin_total = nullptr;    // [256] per-digit totals, then their exclusive scan
 	int cap = 0;
 };
-void radix_sort_reserve(RadixSortBuffers &b, int n);
+hipError_t radix_sort_reserve(RadixSortBuffers &b, int n);
 void radix_sort_free(RadixSortBuffers &b);
 // Sorts (keys, vals) ascending by key, stable, over the low key_bits bits:
 // *n_dev elements (n_dev == nullptr: n_cap), at most n_cap.  b.orand holds
@@ -118,7 +118,7 @@ struct Lbvh {
 	int32_t *world_first = nullptr; // [n_worlds]
 	int32_t *world_last = nullptr;  // [n_worlds]
 };
-void lbvh_reserve(Lbvh &t, int n, int n_worlds);
+hipError_t lbvh_reserve(Lbvh &t, int n, int n_worlds);
 void lbvh_free(Lbvh &t);
 // Build over the shapes in `shapes` (n entries): keys from bb centres
 // quantised in per-world bounds, sorted, Karras hierarchy, boxes refitted.

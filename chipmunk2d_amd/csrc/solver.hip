@@ -1356,7 +1356,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 	// n_act: an upper bound of *S.n_act (the kernels read the count)
 	const int nb = S.n_large_bodies;
 	if (n_act > 0) {
-		radix_sort_reserve(sortbuf, n_act);
+		HIP_TRY(radix_sort_reserve(sortbuf, n_act));
 		k_orand_reset<<<1, 1, 0, stream>>>((unsigned long long *)sortbuf.orand);
 		TIMED(timer, "world_sort", stream,
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(

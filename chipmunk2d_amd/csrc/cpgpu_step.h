@@ -42,6 +42,12 @@ hipError_t narrowphase_table(const DevWorld &W, const StepBuffers &B, int n_prev
 hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 *pairs, const int32_t *n_pairs_dev,
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer);
 
+// cpShapesCollide (cpShape.c:260-283) for n shape pairs (device arrays) on the
+// shapes' cached geometry: count[n] and sets[n][12] {normal, then per point
+// pointA, pointB, distance}, written as doubles for either build.
+hipError_t shapes_collide(const DevWorld &W, const int2 *pairs, int n, int32_t *count, double *sets,
+                          hipStream_t stream);
+
 #define CPGPU_MAX_COLOURS 512
 #define COLOUR_SUBROUNDS 3
 

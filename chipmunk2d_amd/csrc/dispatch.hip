@@ -4,6 +4,7 @@
 // the bit-exact FP64 step) and namespace cpgpu32 (real = float, the FP32
 // mode).  A world remembers its precision in its first field; each entry
 // point forwards to the build the world was created with.
+#include <stddef.h>
 #include <stdint.h>
 
 #include <string>
@@ -24,6 +25,10 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out); \
 	int cpgpuWorldReadArbiters(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *shapes, int32_t *colour, int32_t *state, int32_t *count, double *normal, double *contact, uint64_t *hash); \
 	int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour); \
+	int cpgpuWorldReadPairs(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs); \
+	int cpgpuWorldShapesCollide(cpgpuWorld *world, int64_t n, const int32_t *pairs, int32_t *count, double *sets); \
+	int cpgpuWorldShapesCollideIn(cpgpuWorld *world, int64_t n, const int32_t *pairs, int32_t *count, double *sets, void *scratch, size_t scratch_bytes); \
+	size_t cpgpuShapesCollideScratchBytes(int64_t n); \
 	int cpgpuWorldSetTiming(cpgpuWorld *world, int32_t enable); \
 	int cpgpuWorldGetTiming(cpgpuWorld *world, int32_t cap, int32_t *n_out, double *ms, int64_t *launches, char *names, int32_t names_cap); \
 	int cpgpuWorldGetBuckets(cpgpuWorld *world, int32_t *max_bodies, int32_t *threads, int32_t *n_worlds, int32_t *lds_bodies); \
@@ -118,6 +123,20 @@ int cpgpuWorldReadArbiters(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32
 	if (!world) return null_world();
 	if (is_fp32(world)) return cpgpu32::cpgpuWorldReadArbiters(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), cap, n_out, shapes, colour, state, count, normal, contact, hash);
 	return cpgpu64::cpgpuWorldReadArbiters(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), cap, n_out, shapes, colour, state, count, normal, contact, hash);
+}
+
+int cpgpuWorldReadPairs(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldReadPairs(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), cap, n_out, pairs);
+	return cpgpu64::cpgpuWorldReadPairs(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), cap, n_out, pairs);
+}
+
+int cpgpuWorldShapesCollide(cpgpuWorld *world, int64_t n, const int32_t *pairs, int32_t *count, double *sets)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldShapesCollide(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), n, pairs, count, sets);
+	return cpgpu64::cpgpuWorldShapesCollide(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), n, pairs, count, sets);
 }
 
 int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour)

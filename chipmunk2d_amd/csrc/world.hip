@@ -409,14 +409,15 @@ int validate(const cpgpuWorldDesc *d)
 		const int t = d->body_type[b];
 		if (t < 0 || t > 2) return fail(CPGPU_ERR_INVALID, "body type out of range");
 	}
+	// shapes and worlds walked together: world w owns [world_shape_start[w], world_shape_start[w + 1])
+	int sw = 0;
 	for (int s = 0; s < d->n_shapes; s++) {
+		while (s >= d->world_shape_start[sw + 1]) sw++;
 		const int t = d->shape_type[s];
 		if (t < 0 || t > 2) return fail(CPGPU_ERR_INVALID, "shape type out of range");
 		const int b = d->shape_body[s];
 		if (b < 0 || b >= d->n_bodies) return fail(CPGPU_ERR_INVALID, "shape body index out of range");
-		int w = 0;
-		while (!(d->world_shape_start[w] <= s && s < d->world_shape_start[w + 1])) w++;
-		if (body_world[b] != w) return fail(CPGPU_ERR_INVALID, "shape and its body belong to different worlds");
+		if (body_world[b] != sw) return fail(CPGPU_ERR_INVALID, "shape and its body belong to different worlds");
 		if (t == CPGPU_SHAPE_POLY) {
 			const int c = d->shape_poly_count[s], st = d->shape_poly_start[s];
 			if (c < 1 || c > CPGPU_MAX_POLY_VERTS)
@@ -783,11 +784,11 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(hipGetLastError());
 	// the static tree, built once
 	if (w->n_static_shapes > 0) {
-		lbvh_reserve(w->static_tree, w->n_static_shapes, w->nw);
+		HIP_OR_FAIL(lbvh_reserve(w->static_tree, w->n_static_shapes, w->nw));
 		HIP_OR_FAIL(lbvh_build(w->static_tree, W, w->d_static_shapes, w->n_static_shapes, w->nw, w->sortbuf,
 		                       w->d_world_bounds, w->stream, nullptr));
 	}
-	lbvh_reserve(w->dyn_tree, std::max(w->n_dyn_shapes, 1), w->nw);
+	HIP_OR_FAIL(lbvh_reserve(w->dyn_tree, std::max(w->n_dyn_shapes, 1), w->nw));
 	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
 
 	// initial capacities
@@ -1199,6 +1200,82 @@ int cpgpuWorldReadArbiters(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *
 		}
 	}
 	return CPGPU_OK;
+}
+
+int cpgpuWorldReadPairs(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *pairs)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int64_t n = w->steps > 0 ? w->last_pairs : 0;
+	if (n_out) *n_out = n;
+	if (n == 0 || cap <= 0 || !pairs) return CPGPU_OK;
+	std::vector<int2> h((size_t)n);
+	HIP_OR_FAIL(hipMemcpy(h.data(), w->d_pairs, sizeof(int2) * (size_t)n, hipMemcpyDeviceToHost));
+	// unordered pairs as (lower index, higher index), sorted: the device's order depends on the
+	// tree traversal's timing, the set does not
+	for (auto &p : h)
+		if (p.x > p.y) std::swap(p.x, p.y);
+	std::sort(h.begin(), h.end(), [](const int2 &x, const int2 &y) { return x.x != y.x ? x.x < y.x : x.y < y.y; });
+	for (int64_t q = 0; q < n && q < cap; q++) {
+		pairs[2 * q] = h[q].x;
+		pairs[2 * q + 1] = h[q].y;
+	}
+	return CPGPU_OK;
+}
+
+// Device bytes that cpgpuWorldShapesCollideIn stages n pairs in: the pairs,
+// the counts and the contact point sets, each 256-byte aligned.
+size_t cpgpuShapesCollideScratchBytes(int64_t n)
+{
+	auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+	return up(sizeof(int2) * (size_t)n) + up(sizeof(int32_t) * (size_t)n) + up(sizeof(double) * 12 * (size_t)n);
+}
+
+// cpShapesCollide for n pairs through the caller's device memory `scratch`
+// (at least cpgpuShapesCollideScratchBytes(n) bytes on the world's device):
+// the pairs are staged there, the kernel writes the results there, and they
+// are copied back before the call returns.  Allocates nothing.
+int cpgpuWorldShapesCollideIn(cpgpuWorld *w, int64_t n, const int32_t *pairs, int32_t *count, double *sets,
+                              void *scratch, size_t scratch_bytes)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (n < 0) return fail(CPGPU_ERR_INVALID, "negative pair count");
+	if (n == 0) return CPGPU_OK;
+	if (!pairs || !count || !sets) return fail(CPGPU_ERR_INVALID, "null pair, count or contact set array");
+	if (n > (int64_t)INT32_MAX / 12) return fail(CPGPU_ERR_INVALID, "too many pairs in one call");
+	if (!scratch || scratch_bytes < cpgpuShapesCollideScratchBytes(n))
+		return fail(CPGPU_ERR_CAPACITY, "staging memory too small for " + std::to_string(n) + " pairs");
+	for (int64_t k = 0; k < 2 * n; k++)
+		if (pairs[k] < 0 || pairs[k] >= w->ns)
+			return fail(CPGPU_ERR_INVALID, "shape index " + std::to_string(pairs[k]) + " out of range (pair " +
+			                                   std::to_string(k / 2) + ")");
+	int cur = -1;
+	HIP_OR_FAIL(hipGetDevice(&cur));
+	if (cur != w->device) return fail(CPGPU_ERR_INVALID, "call on the device the world was created on");
+	auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+	char *base = static_cast<char *>(scratch);
+	int2 *d_pairs = reinterpret_cast<int2 *>(base);
+	int32_t *d_count = reinterpret_cast<int32_t *>(base + up(sizeof(int2) * (size_t)n));
+	double *d_sets = reinterpret_cast<double *>(base + up(sizeof(int2) * (size_t)n) + up(sizeof(int32_t) * (size_t)n));
+	// queued behind the world's steps on its stream: the shapes' cached geometry is the last step's
+	HIP_OR_FAIL(hipMemcpyAsync(d_pairs, pairs, sizeof(int2) * (size_t)n, hipMemcpyHostToDevice, w->stream));
+	HIP_OR_FAIL(shapes_collide(w->W, d_pairs, (int)n, d_count, d_sets, w->stream));
+	HIP_OR_FAIL(hipMemcpyAsync(count, d_count, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, w->stream));
+	HIP_OR_FAIL(hipMemcpyAsync(sets, d_sets, sizeof(double) * 12 * (size_t)n, hipMemcpyDeviceToHost, w->stream));
+	return sync_world(w);
+}
+
+int cpgpuWorldShapesCollide(cpgpuWorld *w, int64_t n, const int32_t *pairs, int32_t *count, double *sets)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (n <= 0) return n < 0 ? fail(CPGPU_ERR_INVALID, "negative pair count") : CPGPU_OK;
+	// this form stages the call in device memory of its own
+	const size_t bytes = cpgpuShapesCollideScratchBytes(n);
+	void *dev = nullptr;
+	HIP_OR_FAIL(hipMalloc(&dev, bytes));
+	const int rc = cpgpuWorldShapesCollideIn(w, n, pairs, count, sets, dev, bytes);
+	(void)hipFree(dev);
+	return rc;
 }
 
 int cpgpuWorldReadConstraints(cpgpuWorld *w, double *jacc, int32_t *colour)

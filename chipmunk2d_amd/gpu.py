@@ -25,7 +25,7 @@ _vp = ctypes.c_void_p
 EXPORTS = [
     "cpgpuWorldCreate", "cpgpuWorldDestroy", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
-    "cpgpuWorldReadConstraints", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
+    "cpgpuWorldReadConstraints", "cpgpuWorldReadPairs", "cpgpuWorldShapesCollide", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
     "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldSetGridSchedule",
     "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
     "cpgpuWorldBackward", "cpgpuWorldGetJointGrads", "cpgpuGetLastError", "cpgpuVersion",
@@ -57,6 +57,8 @@ def lib():
         L.cpgpuWorldGetStats.argtypes = [_vp, ctypes.POINTER(Stats)]
         L.cpgpuWorldReadArbiters.argtypes = [_vp, ctypes.c_int64, _i64p, _ip, _ip, _ip, _ip, _dp, _dp, _u64p]
         L.cpgpuWorldReadConstraints.argtypes = [_vp, _dp, _ip]
+        L.cpgpuWorldReadPairs.argtypes = [_vp, ctypes.c_int64, _i64p, _ip]
+        L.cpgpuWorldShapesCollide.argtypes = [_vp, ctypes.c_int64, _ip, _ip, _dp]
         L.cpgpuWorldSetTiming.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldSetGridSchedule.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
@@ -140,6 +142,27 @@ class GPUWorld:
                                              ptr(A.count, ctypes.c_int32), ptr(A.normal, ctypes.c_double),
                                              ptr(A.contact, ctypes.c_double), ptr(A.hash, ctypes.c_uint64)))
         return A
+
+    def pairs(self):
+        """Candidate pairs of the last step as an (n, 2) int32 array of (lower, higher) global
+        shape indices, sorted."""
+        n = ctypes.c_int64()
+        _check(self.L.cpgpuWorldReadPairs(self.h, 0, ctypes.byref(n), None))
+        out = np.zeros((n.value, 2), np.int32)
+        m = ctypes.c_int64()
+        _check(self.L.cpgpuWorldReadPairs(self.h, n.value, ctypes.byref(m), ptr(out, ctypes.c_int32)))
+        return out
+
+    def shapes_collide(self, pairs):
+        """cpShapesCollide for each (a, b) row of `pairs` on the shapes' cached geometry: returns
+        (count [n] int32, sets [n, 12]: normal, then per point pointA, pointB, distance)."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        n = len(pairs)
+        count = np.zeros(n, np.int32)
+        sets = np.zeros((n, 12))
+        _check(self.L.cpgpuWorldShapesCollide(self.h, n, ptr(pairs, ctypes.c_int32), ptr(count, ctypes.c_int32),
+                                              ptr(sets, ctypes.c_double)))
+        return count, sets
 
     def constraints(self):
         n = self.packed.n_constraints

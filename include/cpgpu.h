@@ -201,6 +201,23 @@ int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out);
 int cpgpuWorldReadArbiters(cpgpuWorld *world, int64_t cap, int64_t *n_out,
                            int32_t *shapes, int32_t *colour, int32_t *state, int32_t *count,
                            double *normal, double *contact, uint64_t *hash);
+/* Copy the candidate pairs of the last step: the pairs that passed the exact
+ * QueryReject predicate (src/cpSpaceStep.c:204-232) and were handed to the
+ * narrowphase (cpSpaceCollideShapes, cpSpaceStep.c:235), as the reference's
+ * cpSpatialIndexReindexQuery (include/chipmunk/cpSpatialIndex.h:222-225)
+ * followed by QueryReject would report them.  Up to cap pairs go to
+ * pairs[cap][2] as global shape indices (lower, higher), sorted; the total
+ * goes to *n_out. */
+int cpgpuWorldReadPairs(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs);
+/* cpShapesCollide (include/chipmunk/cpShape.h:95, src/cpShape.c:260-283) for
+ * n pairs of the world's shapes, on their geometry as cached by the last step
+ * (by creation before the first): pairs[n][2] global shape indices (a, b);
+ * count[n] gets each contact point set's count (0..2) and sets[n][12]
+ * {normal.x, normal.y, then for each of the two points pointA.x, pointA.y,
+ * pointB.x, pointB.y, distance} (zeros past the count), in the caller's (a, b)
+ * order as cpShapesCollide returns them.  Blocks until the results are on the
+ * host.  Stages the call in device memory it allocates and frees. */
+int cpgpuWorldShapesCollide(cpgpuWorld *world, int64_t n, const int32_t *pairs, int32_t *count, double *sets);
 /* Copy constraint accumulated impulses ([n][2], see cons_jacc) and colours. */
 int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour);
 

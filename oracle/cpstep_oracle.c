@@ -922,6 +922,29 @@ int ow_collide(oworld *w, int32_t a, int32_t b, uint32_t id, int32_t *ab_out, ui
 	return info.count;
 }
 
+/* cpShapesCollide (cpShape.c:260-283): cpCollide with id 0 on the pair
+ * sorted by type (cpCollision.c:707-710); when that swapped the shapes the
+ * normal is negated and each point pair is given back in the caller's order;
+ * distance = (r2 - r1) . normal with the unswapped points. */
+int ow_shapes_collide(oworld *w, int32_t a, int32_t b, double *n_out, double *points_out)
+{
+	int32_t x = a, y = b;
+	if (w->S[a].type > w->S[b].type) { x = b; y = a; }
+	const int swapped = x != a;
+	CollInfo info;
+	collide(w, x, y, 0, &info);
+	const V n = swapped ? vneg(info.n) : info.n;
+	n_out[0] = n.x; n_out[1] = n.y;
+	for (int i = 0; i < info.count; i++) {
+		const V p1 = info.p1[i], p2 = info.p2[i];
+		const V pa = swapped ? p2 : p1, pb = swapped ? p1 : p2;
+		points_out[5 * i + 0] = pa.x; points_out[5 * i + 1] = pa.y;
+		points_out[5 * i + 2] = pb.x; points_out[5 * i + 3] = pb.y;
+		points_out[5 * i + 4] = vdot(vsub(p2, p1), n);
+	}
+	return info.count;
+}
+
 /* ------------------------------------------------------------------ */
 /* broadphase semantics: the exact QueryReject predicate
  * (cpSpaceStep.c:204-232, cpShapeFilterReject chipmunk_private.h:144-155)

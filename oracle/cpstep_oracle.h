@@ -75,6 +75,11 @@ void ow_set_transforms(oworld *w, const double *transform);
 int ow_collide(oworld *w, int32_t a, int32_t b, uint32_t id, int32_t *ab_out, uint32_t *id_out,
                double *n_out, double *points_out, uint64_t *hash_out);
 
+/* cpShapesCollide(a, b) (cpShape.c:260-283) on cached shapes: writes the
+ * contact point set's normal and per point {pointA.x, pointA.y, pointB.x,
+ * pointB.y, distance}; returns the count. */
+int ow_shapes_collide(oworld *w, int32_t a, int32_t b, double *n_out, double *points_out);
+
 /* Candidate pairs of the exact QueryReject predicate (cpSpaceStep.c:204-232)
  * for the current cached geometry: writes up to cap (a, b) pairs with a < b,
  * sorted; returns the total count. */

@@ -296,6 +296,21 @@ int rb_collide(rbWorld *w, int32_t a, int32_t b, uint32_t id, int32_t *ab_out, u
 	return info.count;
 }
 
+/* The reference's own cpShapesCollide (cpShape.c:260-283) on world-local
+ * shapes a and b: writes the contact point set's normal and, per point,
+ * {pointA.x, pointA.y, pointB.x, pointB.y, distance}; returns the count. */
+int rb_shapes_collide(rbWorld *w, int32_t a, int32_t b, double *n_out, double *points_out)
+{
+	cpContactPointSet set = cpShapesCollide(w->shapes[a], w->shapes[b]);
+	n_out[0] = set.normal.x; n_out[1] = set.normal.y;
+	for (int i = 0; i < set.count; i++) {
+		points_out[5 * i + 0] = set.points[i].pointA.x; points_out[5 * i + 1] = set.points[i].pointA.y;
+		points_out[5 * i + 2] = set.points[i].pointB.x; points_out[5 * i + 3] = set.points[i].pointB.y;
+		points_out[5 * i + 4] = set.points[i].distance;
+	}
+	return set.count;
+}
+
 /* The reference broadphase: cpSpatialIndexReindexQuery over the dynamic
  * BBTree (cpBBTree.c:638-654, which also collides against the static tree)
  * filtered by the same QueryReject predicate cpSpaceCollideShapes applies

@@ -52,6 +52,8 @@ def oracle_lib():
         lib.ow_set_transforms.argtypes = [_vp, _dp]
         lib.ow_collide.restype = ctypes.c_int
         lib.ow_collide.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _ip, _up, _dp, _dp, _u64p]
+        lib.ow_shapes_collide.restype = ctypes.c_int
+        lib.ow_shapes_collide.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
         lib.ow_candidate_pairs.restype = ctypes.c_int64
         lib.ow_candidate_pairs.argtypes = [_vp, ctypes.c_int64, _ip]
         lib.cp_sincos_oracle.argtypes = [_d, _dp, _dp]
@@ -79,6 +81,8 @@ def ref_lib(release=False):
         lib.rb_read_shape_bbs.argtypes = [_vp, _dp]
         lib.rb_collide.restype = ctypes.c_int
         lib.rb_collide.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _ip, _up, _dp, _dp, _u64p]
+        lib.rb_shapes_collide.restype = ctypes.c_int
+        lib.rb_shapes_collide.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
         lib.rb_candidate_pairs.restype = ctypes.c_int64
         lib.rb_candidate_pairs.argtypes = [_vp, ctypes.c_int64, _ip]
         lib.rb_body_update_position.argtypes = [_vp, ctypes.c_int, _d]
@@ -227,11 +231,26 @@ class OracleWorld:
     def collide(self, a, b, cid=0):
         return _collide(self.lib.ow_collide, self.h, a, b, cid)
 
+    def shapes_collide(self, a, b):
+        return _shapes_collide(self.lib.ow_shapes_collide, self.h, a, b)
+
     def candidate_pairs(self):
         n = self.lib.ow_candidate_pairs(self.h, 0, None)
         out = _arr((n, 2), np.int32)
         self.lib.ow_candidate_pairs(self.h, n, ptr(out, ctypes.c_int32))
         return out
+
+
+def _shapes_collide(fn, h, a, b):
+    """(count, set [12]) of cpShapesCollide in the library's layout: normal, then per point
+    pointA, pointB, distance; zeros past the count."""
+    n = _arr(2)
+    pts = _arr((2, 5))
+    count = fn(h, a, b, ptr(n, ctypes.c_double), ptr(pts, ctypes.c_double))
+    out = np.zeros(12)
+    out[:2] = n
+    out[2:2 + 5 * count] = pts[:count].ravel()
+    return count, out
 
 
 def _collide(fn, h, a, b, cid):
@@ -310,6 +329,9 @@ class RefWorld:
     def collide(self, a, b, cid=0):
         r = _collide(self.lib.rb_collide, self.h, a - self.s0, b - self.s0, cid)
         return r
+
+    def shapes_collide(self, a, b):
+        return _shapes_collide(self.lib.rb_shapes_collide, self.h, a - self.s0, b - self.s0)
 
     def candidate_pairs(self):
         n = self.lib.rb_candidate_pairs(self.h, 0, None)

@@ -32,14 +32,19 @@ def assert_bodies_equal(g, o, label=""):
                                                                             np.unravel_index(np.nanargmax(diff), diff.shape)))
 
 
+def _key_order(shapes):
+    """Order of arbiters by their unordered shape pair, and the sorted pairs."""
+    lo, hi = np.minimum(shapes[:, 0], shapes[:, 1]), np.maximum(shapes[:, 0], shapes[:, 1])
+    idx = np.lexsort((hi, lo))
+    return idx, np.stack([lo[idx], hi[idx]], axis=1)
+
+
 def assert_arbiters_equal(g, o):
     ga, oa = g.arbiters(), o.arbiters()
     assert ga.n == oa.n
-    gk = {(min(a, b), max(a, b)): k for k, (a, b) in enumerate(ga.shapes)}
-    ok = oa.by_key()
-    assert set(gk) == set(ok)
-    idx_g = np.array([gk[k] for k in sorted(gk)], dtype=np.int64)
-    idx_o = np.array([ok[k] for k in sorted(ok)], dtype=np.int64)
+    idx_g, keys_g = _key_order(ga.shapes)
+    idx_o, keys_o = _key_order(oa.shapes)
+    assert np.array_equal(keys_g, keys_o)
     assert np.array_equal(ga.shapes[idx_g], oa.shapes[idx_o])
     assert np.array_equal(ga.colour[idx_g], oa.colour[idx_o])
     assert np.array_equal(ga.state[idx_g], oa.state[idx_o])
@@ -49,8 +54,20 @@ def assert_arbiters_equal(g, o):
     assert np.array_equal(ga.hash[idx_g], oa.hash[idx_o])
 
 
-def run_lockstep(pw, steps, check_every=1, arbiters_every=0):
-    g = GPUWorld(pw)
+def assert_pairs_equal(g, o, label=""):
+    """The device's candidate pairs of the last step equal the oracle's as a set
+    (both sides report (lower, higher) shape indices, sorted)."""
+    gp, op = g.pairs(), o.candidate_pairs()
+    if not np.array_equal(gp, op):
+        gs, os_ = set(map(tuple, gp.tolist())), set(map(tuple, op.tolist()))
+        raise AssertionError("%s: candidate pairs differ: %d on the device, %d in the oracle, %d only on the device "
+                             "(e.g. %s), %d only in the oracle (e.g. %s)"
+                             % (label, len(gp), len(op), len(gs - os_), sorted(gs - os_)[:3], len(os_ - gs),
+                                sorted(os_ - gs)[:3]))
+
+
+def run_lockstep(pw, steps, check_every=1, arbiters_every=0, g=None):
+    g = g if g is not None else GPUWorld(pw)
     o = OracleWorld(pw)
     for s in range(steps):
         g.step(DT)
@@ -59,6 +76,8 @@ def run_lockstep(pw, steps, check_every=1, arbiters_every=0):
             assert_bodies_equal(g, o, "step %d" % s)
         if arbiters_every and (s + 1) % arbiters_every == 0:
             assert_arbiters_equal(g, o)
+            assert_pairs_equal(g, o, "step %d" % s)
+    assert_pairs_equal(g, o, "last step")
     gs, os_ = g.stats(), o.stats()
     for k in ("candidate_pairs", "arbiters", "contacts", "cached_arbiters", "arbiter_colours"):
         assert gs[k] == os_[k], (k, gs[k], os_[k])
@@ -158,6 +177,7 @@ def test_grid_strips_flat_pile_schedules_bit_exact(schedule):
             seen.append((len(np.unique(u[(u > 0) & (u <= 64)])), int(np.count_nonzero((u > 64) & (u < 128))),
                          int(np.count_nonzero(u == 128))))
     assert_arbiters_equal(g, o)
+    assert_pairs_equal(g, o)
     print(seen)
     assert all(st == 64 and seams > 500 and wide == 0 for st, seams, wide in seen), seen
     gs, os_ = g.stats(), o.stats()
@@ -197,6 +217,7 @@ def test_grid_flow_schedule_two_large_spaces_in_a_batch_bit_exact():
         if s % 5 == 4:
             assert_bodies_equal(g, o, "step %d" % s)
     assert_arbiters_equal(g, o)
+    assert_pairs_equal(g, o)
     u = o.units()
     assert np.count_nonzero(u == 128) == 0
     assert len(np.unique(u[(u > 0) & (u <= 64)])) == 64 and np.count_nonzero((u > 64) & (u < 128)) > 500
@@ -271,6 +292,34 @@ def test_filters_groups_masks_and_no_collide_joints():
         if k % 9 == 0:
             s.set_filter(categories=1, mask=2)
     run_lockstep(sp.pack(), 80, check_every=4, arbiters_every=40)
+
+
+def test_shapes_collide_bit_exact():
+    """cpShapesCollide (cpShape.c:260-283) on the device for every candidate pair in both
+    argument orders, plus far-apart pairs, at creation and after 30 steps: the contact point
+    sets equal the oracle's (itself pinned against the reference's cpShapesCollide)."""
+    pw = random_world(n_circles=300, n_polys=300, n_segments=80, extent=60.0, seed=23).pack()
+    g, o = GPUWorld(pw), OracleWorld(pw)
+    o.set_transforms(pw.a["body_transform"])
+    rng = np.random.default_rng(11)
+    for steps in (0, 30):
+        if steps:
+            g.step(DT, steps)
+            o.step(DT, steps)
+        cand = o.candidate_pairs()
+        far = rng.integers(0, pw.n_shapes, size=(200, 2)).astype(np.int32)
+        pairs = np.concatenate([cand, cand[:, ::-1], far]).astype(np.int32)
+        count, sets = g.shapes_collide(pairs)
+        touching = 0
+        for i, (a, b) in enumerate(pairs):
+            co, so = o.shapes_collide(int(a), int(b))
+            assert count[i] == co, (steps, a, b)
+            assert np.array_equal(sets[i, :2 + 5 * co], so[:2 + 5 * co]), (steps, a, b, sets[i], so)
+            assert not np.any(sets[i, 2 + 5 * co:]), (steps, a, b)
+            touching += co > 0
+        assert touching > 200, touching
+    with pytest.raises(Exception, match="out of range"):
+        g.shapes_collide([[0, pw.n_shapes]])
 
 
 # ---- known answers of the reference's own tests, on the device ----
@@ -387,6 +436,7 @@ def test_random_world_100k_first_steps_bit_exact():
         g.step(DT)
         o.step(DT)
         assert_bodies_equal(g, o, "step %d" % s)
+        assert_pairs_equal(g, o, "step %d" % s)
     assert_arbiters_equal(g, o)
     assert g.stats()["candidate_pairs"] == o.stats()["candidate_pairs"]
 

@@ -85,6 +85,33 @@ def test_collide_matches_reference_every_id():
     assert checked > 1000
 
 
+@needs_ref
+def test_shapes_collide_matches_reference():
+    """cpShapesCollide (cpShape.c:260-283) in both argument orders: the oracle's
+    contact point sets equal the reference's bit for bit (normal flipped and
+    points swapped when cpCollide reorders by type)."""
+    d = np.load(os.path.join(GOLDEN, "collide_pairs.npz"))
+    pw = load_world(d)
+    o = OracleWorld(pw)
+    r = RefWorld(pw)
+    o.set_transforms(pw.a["body_transform"])
+    r.set_transforms(pw.a["body_transform"])
+    types = pw.a["shape_type"]
+    checked, swapped, touching = 0, 0, 0
+    for k in range(0, len(d["c_a"]), 3):
+        a, b = int(d["c_a"][k]), int(d["c_b"][k])
+        for x, y in ((a, b), (b, a)):
+            co, so = o.shapes_collide(x, y)
+            cr, sr = r.shapes_collide(x, y)
+            assert co == cr, (x, y)
+            # the reference leaves points past the count unset: compare the normal and the set points
+            assert np.array_equal(so[:2 + 5 * co], sr[:2 + 5 * cr]), (x, y, so, sr)
+            checked += 1
+            swapped += types[x] > types[y]
+            touching += co > 0
+    assert checked > 500 and swapped > 100 and touching > 100, (checked, swapped, touching)
+
+
 def test_circle_circle_closed_form():
     """Two circles: normal along the centre line, points on the surfaces, hash 0."""
     space = Space()

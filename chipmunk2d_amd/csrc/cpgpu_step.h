@@ -32,6 +32,10 @@ struct StepBuffers {
 
 #define PAIR_CLASSES 9         // type_a * 3 + type_b (type_a <= type_b)
 
+// A pair record's arbiter reference: an index into the previous step's solved
+// arbiters, or, with this bit set, into its cached ones.
+constexpr int32_t CPGPU_CACHED_REF = 0x40000000;
+
 // The previous step's pair records in an open-addressing table by pair key
 // (B.tkeys / tvals / tmask, B.prev_recs, B.n_prev_recs, B.seen); they depend
 // only on the previous step, so this can run beside the broadphase.

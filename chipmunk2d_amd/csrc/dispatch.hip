@@ -22,6 +22,9 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldSync(cpgpuWorld *world); \
 	int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, double *w, double *v_bias, double *w_bias, double *transform); \
 	int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w, const double *f, const double *t); \
+	int cpgpuWorldWriteBodiesAt(cpgpuWorld *world, int32_t n, const int32_t *index, const double *p, const double *v, const double *f, const double *a, const double *w, const double *t, const double *transform); \
+	int cpgpuWorldSetSpaceParams(cpgpuWorld *world, int32_t space, const cpgpuSpaceParams *params); \
+	int cpgpuWorldCreateFrom(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld *old, const int32_t *shape_from, cpgpuWorld **out); \
 	int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out); \
 	int cpgpuWorldReadArbiters(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *shapes, int32_t *colour, int32_t *state, int32_t *count, double *normal, double *contact, uint64_t *hash); \
 	int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour); \
@@ -58,6 +61,8 @@ inline int null_world()
 	cpgpu_last_error = "null world";
 	return CPGPU_ERR_INVALID;
 }
+
+}
 } // namespace
 
 extern "C" {
@@ -67,6 +72,18 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld *
 	if (precision == CPGPU_FP32)
 		return cpgpu32::cpgpuWorldCreate(desc, precision, reinterpret_cast<cpgpu32::cpgpuWorld **>(out));
 	return cpgpu64::cpgpuWorldCreate(desc, precision, reinterpret_cast<cpgpu64::cpgpuWorld **>(out));
+}
+
+int cpgpuWorldCreateFrom(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld *old, const int32_t *shape_from,
+                         cpgpuWorld **out)
+{
+	if (out) *out = nullptr;
+	if (!old) return null_world();
+	if (is_fp32(old))
+		return cpgpu32::cpgpuWorldCreateFrom(desc, precision, reinterpret_cast<cpgpu32::cpgpuWorld *>(old), shape_from,
+		                                     reinterpret_cast<cpgpu32::cpgpuWorld **>(out));
+	return cpgpu64::cpgpuWorldCreateFrom(desc, precision, reinterpret_cast<cpgpu64::cpgpuWorld *>(old), shape_from,
+	                                     reinterpret_cast<cpgpu64::cpgpuWorld **>(out));
 }
 
 void cpgpuWorldDestroy(cpgpuWorld *world)
@@ -109,6 +126,24 @@ int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w, c
 	if (!world) return null_world();
 	if (is_fp32(world)) return cpgpu32::cpgpuWorldWriteBodies(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), v, w, f, t);
 	return cpgpu64::cpgpuWorldWriteBodies(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), v, w, f, t);
+}
+
+int cpgpuWorldWriteBodiesAt(cpgpuWorld *world, int32_t n, const int32_t *index, const double *p, const double *v,
+                            const double *f, const double *a, const double *w, const double *t, const double *transform)
+{
+	if (!world) return null_world();
+	if (is_fp32(world))
+		return cpgpu32::cpgpuWorldWriteBodiesAt(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), n, index, p, v, f, a, w, t,
+		                                        transform);
+	return cpgpu64::cpgpuWorldWriteBodiesAt(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), n, index, p, v, f, a, w, t,
+	                                        transform);
+}
+
+int cpgpuWorldSetSpaceParams(cpgpuWorld *world, int32_t space, const cpgpuSpaceParams *params)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldSetSpaceParams(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), space, params);
+	return cpgpu64::cpgpuWorldSetSpaceParams(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), space, params);
 }
 
 int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out)

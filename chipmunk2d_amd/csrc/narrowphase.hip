@@ -23,7 +23,6 @@ namespace CPGPU_NS {
 namespace {
 
 constexpr uint64_t EMPTY_KEY = ~0ull;
-constexpr int32_t CACHED_TAG = 0x40000000;
 
 __global__ void k_table_clear(uint64_t *tkeys, int32_t *tvals, int tcap)
 {
@@ -79,7 +78,7 @@ __device__ __forceinline__ int append_index(int32_t *counter)
 
 __device__ __forceinline__ const Arbiter *arb_ref(const StepBuffers &B, int32_t ref)
 {
-	return (ref & CACHED_TAG) ? &B.prev_cached[ref & ~CACHED_TAG] : &B.prev_act[ref];
+	return (ref & CPGPU_CACHED_REF) ? &B.prev_cached[ref & ~CPGPU_CACHED_REF] : &B.prev_act[ref];
 }
 
 // Keep an untouched arbiter for collisionPersistence steps as CACHED
@@ -102,7 +101,7 @@ __device__ __forceinline__ int32_t carry_arbiter(const StepBuffers &B, const Dev
 #pragma unroll
 	for (int i = 0; i < (int)(sizeof(Arbiter) / sizeof(Piece)); i++) dst[i] = src[i];
 	B.cached[m].state = CPGPU_ARB_CACHED;
-	return m | CACHED_TAG;
+	return m | CPGPU_CACHED_REF;
 }
 
 __device__ __forceinline__ void write_record(const StepBuffers &B, const DevWorld &W, uint64_t key, uint32_t id,

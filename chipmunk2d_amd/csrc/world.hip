@@ -37,6 +37,13 @@ int fail(int code, const std::string &msg)
 
 inline int blocks_for(long n, int t) { return (int)((n + t - 1) / t); }
 
+// the device's pair_key on the host: unordered shape pair, lower index high
+inline uint64_t pair_key_host(int32_t a, int32_t b)
+{
+	const uint32_t lo = (uint32_t)(a < b ? a : b), hi = (uint32_t)(a < b ? b : a);
+	return ((uint64_t)lo << 32) | hi;
+}
+
 template <typename T>
 hipError_t dalloc(T *&p, size_t n)
 {
@@ -1128,6 +1135,204 @@ int cpgpuWorldWriteBodies(cpgpuWorld *w, const double *v, const double *wv, cons
 	}
 	if (f) HIP_OR_FAIL(upload_real_into(reinterpret_cast<real *>(w->W.b_f), f, 2 * (size_t)nb));
 	if (t) HIP_OR_FAIL(upload_real_into(w->W.b_t, t, nb));
+	return CPGPU_OK;
+}
+
+// Rewrite the listed entries of a device array of `width` scalars per body.
+static hipError_t edit_rows(real *dev, int width, int nb, int32_t n, const int32_t *index, const double *src)
+{
+	std::vector<real> h((size_t)width * nb);
+	HIP_TRY(hipMemcpy(h.data(), dev, sizeof(real) * h.size(), hipMemcpyDeviceToHost));
+	for (int k = 0; k < n; k++)
+		for (int c = 0; c < width; c++) h[(size_t)width * index[k] + c] = (real)src[(size_t)width * k + c];
+	return hipMemcpy(dev, h.data(), sizeof(real) * h.size(), hipMemcpyHostToDevice);
+}
+
+int cpgpuWorldWriteBodiesAt(cpgpuWorld *w, int32_t n, const int32_t *index, const double *p, const double *v,
+                            const double *f, const double *a, const double *wv, const double *t,
+                            const double *transform)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (n < 0) return fail(CPGPU_ERR_INVALID, "negative body count");
+	if (n == 0) return CPGPU_OK;
+	if (!index) return fail(CPGPU_ERR_INVALID, "null body index list");
+	const bool pose = p || a || transform;
+	for (int k = 0; k < n; k++) {
+		if (index[k] < 0 || index[k] >= w->nb)
+			return fail(CPGPU_ERR_INVALID, "body index " + std::to_string(index[k]) + " out of range");
+		// static shapes are cached once, when the world is made (cpSpaceReindexStatic's role)
+		if (pose && w->body_type[index[k]] == CPGPU_BODY_STATIC)
+			return fail(CPGPU_ERR_INVALID, "a static body's pose changes only with a new world (body " +
+			                                   std::to_string(index[k]) + ")");
+	}
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int nb = w->nb;
+	if (p) HIP_OR_FAIL(edit_rows(reinterpret_cast<real *>(w->W.b_p), 2, nb, n, index, p));
+	if (a) HIP_OR_FAIL(edit_rows(w->W.b_a, 1, nb, n, index, a));
+	if (f) HIP_OR_FAIL(edit_rows(reinterpret_cast<real *>(w->W.b_f), 2, nb, n, index, f));
+	if (t) HIP_OR_FAIL(edit_rows(w->W.b_t, 1, nb, n, index, t));
+	if (transform) HIP_OR_FAIL(edit_rows(reinterpret_cast<real *>(w->W.b_T), 6, nb, n, index, transform));
+	if (v || wv) {
+		std::vector<BodyVel> vel(nb);
+		HIP_OR_FAIL(hipMemcpy(vel.data(), w->W.b_vel, sizeof(BodyVel) * nb, hipMemcpyDeviceToHost));
+		for (int k = 0; k < n; k++) {
+			BodyVel &b = vel[index[k]];
+			if (v) { b.vx = (real)v[2 * k]; b.vy = (real)v[2 * k + 1]; }
+			if (wv) b.w = (real)wv[k];
+		}
+		HIP_OR_FAIL(hipMemcpy(w->W.b_vel, vel.data(), sizeof(BodyVel) * nb, hipMemcpyHostToDevice));
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldSetSpaceParams(cpgpuWorld *w, int32_t space, const cpgpuSpaceParams *params)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (!params) return fail(CPGPU_ERR_INVALID, "null space parameters");
+	if (space < 0 || space >= w->nw) return fail(CPGPU_ERR_INVALID, "space index " + std::to_string(space) + " out of range");
+	if (params->iterations < 0) return fail(CPGPU_ERR_INVALID, "iterations must be >= 0");
+	if (w->tape.on) return fail(CPGPU_ERR_INVALID, "space parameters are fixed during a taped rollout");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	w->sp[space] = *params;
+	w->max_iterations = 0;
+	w->max_iterations_large = 0;
+	for (int k = 0; k < w->nw; k++) {
+		w->max_iterations = std::max(w->max_iterations, (int)w->sp[k].iterations);
+		if (!w->h_small[k]) w->max_iterations_large = std::max(w->max_iterations_large, (int)w->sp[k].iterations);
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldCreateFrom(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld *old, const int32_t *shape_from,
+                         cpgpuWorld **out)
+{
+	if (!out) return fail(CPGPU_ERR_INVALID, "null output pointer");
+	*out = nullptr;
+	if (!old) return fail(CPGPU_ERR_INVALID, "null previous world");
+	if (old->precision != precision) return fail(CPGPU_ERR_INVALID, "the new world must keep the old world's precision");
+	if (old->tape.on) return fail(CPGPU_ERR_INVALID, "a taped world cannot be packed again (cpgpuWorldTapeEnd first)");
+	int rc = validate(d);
+	if (rc != CPGPU_OK) return rc;
+	if (d->n_shapes > 0 && !shape_from) return fail(CPGPU_ERR_INVALID, "null shape map");
+	// new -> old and old -> new shape indices
+	std::vector<int32_t> inv(old->ns, -1);
+	for (int s = 0; s < d->n_shapes; s++) {
+		const int o = shape_from[s];
+		if (o < -1 || o >= old->ns) return fail(CPGPU_ERR_INVALID, "shape map entry " + std::to_string(s) + " out of range");
+		if (o < 0) continue;
+		if (inv[o] >= 0) return fail(CPGPU_ERR_INVALID, "shape map names old shape " + std::to_string(o) + " twice");
+		inv[o] = s;
+	}
+	rc = cpgpuWorldSync(old);
+	if (rc != CPGPU_OK) return rc;
+	cpgpuWorld *w = nullptr;
+	rc = cpgpuWorldCreate(d, precision, &w);
+	if (rc != CPGPU_OK) return rc;
+	auto abandon = [&](int code) { cpgpuWorldDestroy(w); return code; };
+#define MIGRATE_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
+	return abandon(fail(CPGPU_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_))); } while (0)
+	const int ns = w->ns, nso = old->ns;
+	std::vector<int32_t> shape_body_old(nso), shape_world(ns);
+	std::vector<uint8_t> static_old(nso), static_new(ns);
+	if (nso) MIGRATE_TRY(hipMemcpy(shape_body_old.data(), old->W.s_body, sizeof(int32_t) * nso, hipMemcpyDeviceToHost));
+	if (nso) MIGRATE_TRY(hipMemcpy(static_old.data(), old->W.s_static, nso, hipMemcpyDeviceToHost));
+	if (ns) MIGRATE_TRY(hipMemcpy(shape_world.data(), w->W.s_world, sizeof(int32_t) * ns, hipMemcpyDeviceToHost));
+	if (ns) MIGRATE_TRY(hipMemcpy(static_new.data(), w->W.s_static, ns, hipMemcpyDeviceToHost));
+	// a shape that stays in the dynamic tree keeps its fattened leaf box and the step of its
+	// last rebuild, so warm-start ids stay valid exactly as long as without the new pack
+	if (ns > 0 && nso > 0) {
+		std::vector<real4> fat_old(nso), fat(ns);
+		std::vector<uint32_t> reb_old(nso), reb(ns);
+		MIGRATE_TRY(hipMemcpy(fat_old.data(), old->W.s_fat, sizeof(real4) * nso, hipMemcpyDeviceToHost));
+		MIGRATE_TRY(hipMemcpy(reb_old.data(), old->W.s_rebuild, sizeof(uint32_t) * nso, hipMemcpyDeviceToHost));
+		MIGRATE_TRY(hipMemcpy(fat.data(), w->W.s_fat, sizeof(real4) * ns, hipMemcpyDeviceToHost));
+		MIGRATE_TRY(hipMemcpy(reb.data(), w->W.s_rebuild, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+		for (int s = 0; s < ns; s++) {
+			const int o = shape_from[s];
+			if (o < 0 || static_new[s] || static_old[o]) continue;
+			fat[s] = fat_old[o];
+			reb[s] = reb_old[o];
+		}
+		MIGRATE_TRY(hipMemcpy(w->W.s_fat, fat.data(), sizeof(real4) * ns, hipMemcpyHostToDevice));
+		MIGRATE_TRY(hipMemcpy(w->W.s_rebuild, reb.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+	}
+	// the persistent pair records and arbiters of the old world's last step, for every pair
+	// whose two shapes are both packed again, under their new indices
+	if (old->steps > 0) {
+		const int L = old->cur ^ 1;
+		const int n_recs = old->h_counts[3 * L], n_act = old->h_counts[3 * L + 1], n_cached = old->h_counts[3 * L + 2];
+		std::vector<PairRec> recs(n_recs);
+		std::vector<Arbiter> act(n_act), cached(n_cached);
+		if (n_recs) MIGRATE_TRY(hipMemcpy(recs.data(), old->recs[L], sizeof(PairRec) * n_recs, hipMemcpyDeviceToHost));
+		if (n_act) MIGRATE_TRY(hipMemcpy(act.data(), old->act[L], sizeof(Arbiter) * n_act, hipMemcpyDeviceToHost));
+		if (n_cached)
+			MIGRATE_TRY(hipMemcpy(cached.data(), old->cached[L], sizeof(Arbiter) * n_cached, hipMemcpyDeviceToHost));
+		auto remap = [&](std::vector<Arbiter> &v, std::vector<int32_t> &where) {
+			where.assign(v.size(), -1);
+			size_t keep = 0;
+			for (size_t m = 0; m < v.size(); m++) {
+				Arbiter a = v[m];
+				const int sa = inv[a.sa], sb = inv[a.sb];
+				if (sa < 0 || sb < 0 || shape_world[sa] != shape_world[sb]) continue;
+				a.sa = sa; a.sb = sb;
+				a.key = pair_key_host(sa, sb);
+				a.ba = d->shape_body[sa]; a.bb = d->shape_body[sb];
+				a.world = shape_world[sa];
+				for (int c = 0; c < 2; c++) { a.c[c].tape = -1; a.c[c].src = -1; }
+				where[m] = (int32_t)keep;
+				v[keep++] = a;
+			}
+			v.resize(keep);
+		};
+		std::vector<int32_t> act_at, cached_at;
+		remap(act, act_at);
+		remap(cached, cached_at);
+		size_t keep = 0;
+		for (int r = 0; r < n_recs; r++) {
+			PairRec rec = recs[r];
+			const int sa = inv[(int32_t)(rec.key >> 32)], sb = inv[(int32_t)(rec.key & 0xffffffffu)];
+			if (sa < 0 || sb < 0 || shape_world[sa] != shape_world[sb]) continue;
+			rec.key = pair_key_host(sa, sb);
+			if (rec.arb >= 0 && (rec.arb & CPGPU_CACHED_REF)) {
+				const int32_t m = cached_at[rec.arb & ~CPGPU_CACHED_REF];
+				rec.arb = m >= 0 ? (m | CPGPU_CACHED_REF) : -1;
+			} else if (rec.arb >= 0) {
+				rec.arb = act_at[rec.arb];
+			}
+			recs[keep++] = rec;
+		}
+		recs.resize(keep);
+		const int P = w->cur ^ 1;
+		MIGRATE_TRY(regrow(w->recs[P], w->rec_cap[P], (long)recs.size() + 1));
+		MIGRATE_TRY(regrow(w->act[P], w->act_cap[P], (long)act.size() + 1));
+		MIGRATE_TRY(regrow(w->cached[P], w->cached_cap[P], (long)cached.size() + 1));
+		if (!recs.empty())
+			MIGRATE_TRY(hipMemcpy(w->recs[P], recs.data(), sizeof(PairRec) * recs.size(), hipMemcpyHostToDevice));
+		if (!act.empty())
+			MIGRATE_TRY(hipMemcpy(w->act[P], act.data(), sizeof(Arbiter) * act.size(), hipMemcpyHostToDevice));
+		if (!cached.empty())
+			MIGRATE_TRY(hipMemcpy(w->cached[P], cached.data(), sizeof(Arbiter) * cached.size(), hipMemcpyHostToDevice));
+		const int32_t counts[3] = {(int32_t)recs.size(), (int32_t)act.size(), (int32_t)cached.size()};
+		MIGRATE_TRY(hipMemcpy(w->d_step_counts + 3 * P, counts, sizeof(counts), hipMemcpyHostToDevice));
+		for (int k = 0; k < 3; k++) w->h_counts[3 * P + k] = counts[k];
+		w->steps = old->steps;
+		w->total_pairs = old->total_pairs;
+		MIGRATE_TRY(hipMemcpy(w->W.totals, old->W.totals, sizeof(unsigned long long) * (2 + 2 * CPGPU_SOLVE_UNITS),
+		                      hipMemcpyDeviceToDevice));
+	}
+	// the old world's tiling and grid schedule
+	if (old->n_buckets == w->n_buckets) {
+		for (int q = 0; q < w->n_buckets; q++) {
+			w->buckets[q].max_bodies = old->buckets[q].max_bodies;
+			w->buckets[q].threads = old->buckets[q].threads;
+		}
+		MIGRATE_TRY(build_buckets(w));
+	}
+	w->grid_flow = old->grid_flow;
+#undef MIGRATE_TRY
+	*out = w;
 	return CPGPU_OK;
 }
 

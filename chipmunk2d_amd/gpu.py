@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .cabi import Stats, WorldDesc, ptr, FP64
+from .cabi import SpaceParams, Stats, WorldDesc, ptr, FP64
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcpgpu.so")
@@ -23,9 +23,11 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
 
 EXPORTS = [
-    "cpgpuWorldCreate", "cpgpuWorldDestroy", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
+    "cpgpuWorldCreate", "cpgpuWorldCreateFrom", "cpgpuWorldDestroy", "cpgpuWorldWriteBodiesAt",
+    "cpgpuWorldSetSpaceParams", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
-    "cpgpuWorldReadConstraints", "cpgpuWorldReadPairs", "cpgpuWorldShapesCollide", "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
+    "cpgpuWorldReadConstraints", "cpgpuWorldReadPairs", "cpgpuWorldShapesCollide",
+    "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
     "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldSetGridSchedule",
     "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
     "cpgpuWorldBackward", "cpgpuWorldGetJointGrads", "cpgpuGetLastError", "cpgpuVersion",
@@ -47,6 +49,9 @@ def lib():
             raise CpgpuError("%s is missing: build it with `python -m chipmunk2d_amd.build`" % LIB_PATH)
         L = ctypes.CDLL(LIB_PATH)
         L.cpgpuWorldCreate.argtypes = [ctypes.POINTER(WorldDesc), ctypes.c_int32, ctypes.POINTER(_vp)]
+        L.cpgpuWorldCreateFrom.argtypes = [ctypes.POINTER(WorldDesc), ctypes.c_int32, _vp, _ip, ctypes.POINTER(_vp)]
+        L.cpgpuWorldWriteBodiesAt.argtypes = [_vp, ctypes.c_int32, _ip] + [_dp] * 7
+        L.cpgpuWorldSetSpaceParams.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(SpaceParams)]
         L.cpgpuWorldDestroy.argtypes = [_vp]
         L.cpgpuWorldDestroy.restype = None
         L.cpgpuWorldStep.argtypes = [_vp, ctypes.c_double]
@@ -92,6 +97,21 @@ class GPUWorld:
         _check(self.L.cpgpuWorldCreate(ctypes.byref(self.desc), precision, ctypes.byref(h)))
         self.h = h
 
+    @classmethod
+    def from_old(cls, packed, old, shape_from, precision=FP64):
+        """cpgpuWorldCreateFrom: a world of `packed` that keeps the collision state of the
+        GPUWorld `old` (shape_from[s]: old index of shape s, -1 for a new shape)."""
+        self = cls.__new__(cls)
+        self.L = lib()
+        self.packed = packed
+        self.desc = packed.desc()
+        sf = np.ascontiguousarray(shape_from, dtype=np.int32)
+        h = _vp()
+        _check(self.L.cpgpuWorldCreateFrom(ctypes.byref(self.desc), precision, old.h, ptr(sf, ctypes.c_int32),
+                                           ctypes.byref(h)))
+        self.h = h
+        return self
+
     def close(self):
         if getattr(self, "h", None):
             self.L.cpgpuWorldDestroy(self.h)
@@ -122,6 +142,18 @@ class GPUWorld:
         args = [ptr(np.ascontiguousarray(x, dtype=np.float64), ctypes.c_double) if x is not None else None
                 for x in (v, w, f, t)]
         _check(self.L.cpgpuWorldWriteBodies(self.h, *args))
+
+    def write_bodies_at(self, index, p=None, v=None, f=None, a=None, w=None, t=None, transform=None):
+        """cpgpuWorldWriteBodiesAt: the listed bodies' fields (rows in index order)."""
+        index = np.ascontiguousarray(index, dtype=np.int32)
+        args = [None if x is None else ptr(np.ascontiguousarray(x, dtype=np.float64), ctypes.c_double)
+                for x in (p, v, f, a, w, t, transform)]
+        _check(self.L.cpgpuWorldWriteBodiesAt(self.h, len(index), ptr(index, ctypes.c_int32), *args))
+
+    def set_space_params(self, k, params):
+        """cpgpuWorldSetSpaceParams for space k from a dict of cpgpuSpaceParams fields."""
+        sp = SpaceParams(**params)
+        _check(self.L.cpgpuWorldSetSpaceParams(self.h, k, ctypes.byref(sp)))
 
     def stats(self):
         st = Stats()

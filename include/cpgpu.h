@@ -165,6 +165,22 @@ typedef struct cpgpuStats {
  * CPGPU_FP64 (bit-for-bit the reference's double arithmetic) or CPGPU_FP32. */
 int cpgpuWorldCreate(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld **out);
 void cpgpuWorldDestroy(cpgpuWorld *world);
+/* A new world from a description of the same spaces packed again after
+ * bodies, shapes or constraints were added or removed (cpSpaceAdd* /
+ * cpSpaceRemove*, src/cpSpace.c:417-571), which keeps the old world's
+ * persistent collision state: shape_from[s] is the old world's index of the
+ * description's shape s, or -1 for a new shape.  Every arbiter and GJK
+ * warm-start id of the old world's last step whose two shapes are both in
+ * the new description moves over (cached impulses, states, stamps), and every
+ * dynamic shape keeps its fattened tree box and the step of its last rebuild,
+ * so the next step warm-starts exactly as cpSpaceStep would after the same
+ * adds and removes; arbiters of removed shapes are dropped, as
+ * cpSpaceRemoveShape does through cpSpaceFilterArbiters (src/cpSpace.c:513-541).  Body state, joint
+ * accumulators and space parameters come from the description.  The old
+ * world is left as it was (the caller destroys it); same precision; not
+ * during a taped rollout. */
+int cpgpuWorldCreateFrom(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWorld *old, const int32_t *shape_from,
+                         cpgpuWorld **out);
 
 /* ---- the step (replaces cpSpaceStep, src/cpSpaceStep.c:335-445, and
  * cpHastySpaceStep, src/cpHastySpace.c:594-700) ---- */
@@ -185,6 +201,21 @@ int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, dou
  * cpBodySetForce setters); any pointer may be NULL. */
 int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w,
                           const double *f, const double *t);
+/* Overwrite the state of n bodies listed by global index, from host arrays of
+ * n entries each: positions p [n][2], velocities v [n][2], forces f [n][2],
+ * angles a, angular velocities w, torques t [n], transforms [n][6] (the
+ * cpBodySetPosition / SetVelocity / SetForce / SetAngle / SetAngularVelocity
+ * / SetTorque setters, src/cpBody.c:375-470, which also set the transform).
+ * Any pointer may be NULL.  A static body's pose is not written here: static
+ * shapes are cached once per world (cpSpaceReindexStatic's role), so a moved
+ * static body needs cpgpuWorldCreateFrom. */
+int cpgpuWorldWriteBodiesAt(cpgpuWorld *world, int32_t n, const int32_t *index, const double *p, const double *v,
+                            const double *f, const double *a, const double *w, const double *t,
+                            const double *transform);
+/* Replace one space's parameters (the cpSpaceSetGravity / SetIterations /
+ * SetDamping / SetCollisionSlop / SetCollisionBias / SetCollisionPersistence
+ * setters, src/cpSpace.c:240-300; stamp and curr_dt as the space holds them). */
+int cpgpuWorldSetSpaceParams(cpgpuWorld *world, int32_t space, const cpgpuSpaceParams *params);
 int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out);
 
 /* Copy the arbiters solved in the last step (cpSpace.arbiters, in solve

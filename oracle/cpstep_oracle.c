@@ -1822,6 +1822,124 @@ int ow_write_bodies(oworld *w, const double *v, const double *wv, const double *
 	return 0;
 }
 
+/* The setters between steps (cpBodySetPosition / SetVelocity / SetForce /
+ * SetAngle / SetAngularVelocity / SetTorque, cpBody.c:375-470, which also set
+ * the transform): the listed bodies' fields from n-entry arrays. */
+int ow_write_bodies_at(oworld *w, int32_t n, const int32_t *index, const double *p, const double *v,
+                       const double *f, const double *a, const double *wv, const double *t, const double *transform)
+{
+	for (int k = 0; k < n; k++)
+		if (index[k] < 0 || index[k] >= w->nb) return -1;
+	for (int k = 0; k < n; k++) {
+		OBody *b = &w->B[index[k]];
+		if (p) b->p = v2(p[2 * k], p[2 * k + 1]);
+		if (v) b->v = v2(v[2 * k], v[2 * k + 1]);
+		if (f) b->f = v2(f[2 * k], f[2 * k + 1]);
+		if (a) b->a = a[k];
+		if (wv) b->w = wv[k];
+		if (t) b->t = t[k];
+		if (transform) b->T = xf_from(transform + 6 * k);
+	}
+	return 0;
+}
+
+/* The cpSpace parameter setters (cpSpace.c:240-300). */
+int ow_set_space_params(oworld *w, int32_t space, const cpgpuSpaceParams *sp)
+{
+	if (space < 0 || space >= w->nw) return -1;
+	w->W[space].sp = *sp;
+	return 0;
+}
+
+/* A new oracle world from the spaces packed again after adds and removes
+ * (cpSpaceAdd* / cpSpaceRemove*, cpSpace.c:417-571), keeping the old world's
+ * collision state under the new shape indices: every cached arbiter and GJK
+ * warm-start id whose two shapes are packed again, and each dynamic shape's
+ * fattened leaf box and rebuild step.  Arbiters of removed shapes are
+ * dropped, as cpSpaceRemoveShape does through cpSpaceFilterArbiters
+ * (cpSpace.c:513-541).  shape_from[s] is the old index of shape s, or -1. */
+oworld *ow_new_from(const cpgpuWorldDesc *d, oworld *old, const int32_t *shape_from)
+{
+	oworld *w = ow_new(d);
+	int32_t *inv = (int32_t *)malloc((size_t)(old->ns ? old->ns : 1) * sizeof(int32_t));
+	for (int o = 0; o < old->ns; o++) inv[o] = -1;
+	for (int s = 0; s < w->ns; s++) if (shape_from[s] >= 0) inv[shape_from[s]] = s;
+	for (int s = 0; s < w->ns; s++) {
+		const int o = shape_from[s];
+		if (o < 0 || w->B[w->S[s].body].type == CPGPU_BODY_STATIC || old->B[old->S[o].body].type == CPGPU_BODY_STATIC)
+			continue;
+		memcpy(w->S[s].fat, old->S[o].fat, sizeof(w->S[s].fat));
+		w->S[s].rebuild_stamp = old->S[o].rebuild_stamp;
+	}
+	/* the new space of an old pair, or -1 when a shape is gone */
+#define NEW_PAIR_WORLD(a, b) ((inv[(a)] < 0 || inv[(b)] < 0 || w->S[inv[(a)]].world != w->S[inv[(b)]].world) \
+                              ? -1 : w->S[inv[(a)]].world)
+	int64_t *pid_count = (int64_t *)calloc((size_t)w->nw, sizeof(int64_t));
+	for (int wi = 0; wi < old->nw; wi++) {
+		OWorldPart *Po = &old->W[wi];
+		int64_t *at = (int64_t *)malloc((size_t)(Po->narb ? Po->narb : 1) * sizeof(int64_t));
+		for (int64_t i = 0; i < Po->narb; i++) {
+			const OArb *oa = &Po->arbs[i];
+			const int nw = NEW_PAIR_WORLD(oa->sa, oa->sb);
+			at[i] = -1;
+			if (nw < 0) continue;
+			OWorldPart *Pn = &w->W[nw];
+			const int64_t j = arb_append(Pn);
+			OArb *na = &Pn->arbs[j];
+			*na = *oa;
+			na->sa = inv[oa->sa]; na->sb = inv[oa->sb];
+			na->ba = w->S[na->sa].body; na->bb = w->S[na->sb].body;
+			na->key = pair_key(na->sa, na->sb);
+			na->prio = mix64(pair_key(na->sa - Pn->s0, na->sb - Pn->s0));
+			at[i] = j;
+		}
+		for (int64_t k = 0; k < Po->nactive; k++) {
+			const int64_t i = Po->active[k];
+			if (at[i] < 0) continue;
+			OWorldPart *Pn = &w->W[w->S[inv[Po->arbs[i].sa]].world];
+			Pn->active[Pn->nactive++] = at[i];
+		}
+		free(at);
+		for (int64_t k = 0; Po->pid_key && k <= Po->pid_mask; k++) {
+			if (Po->pid_key[k] == UINT64_MAX) continue;
+			const int nw = NEW_PAIR_WORLD((int32_t)(Po->pid_key[k] >> 32), (int32_t)(Po->pid_key[k] & 0xffffffffu));
+			if (nw >= 0) pid_count[nw]++;
+		}
+	}
+	for (int wi = 0; wi < w->nw; wi++) {
+		OWorldPart *Pn = &w->W[wi];
+		int64_t cap = 256;
+		while (cap < 2 * pid_count[wi] + 64) cap *= 2;
+		Pn->pid_key = (uint64_t *)malloc((size_t)cap * sizeof(uint64_t));
+		Pn->pid_id = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
+		Pn->pid_stamp = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
+		for (int64_t k = 0; k < cap; k++) Pn->pid_key[k] = UINT64_MAX;
+		Pn->pid_mask = cap - 1;
+		Pn->pid_n = 0;
+	}
+	for (int wi = 0; wi < old->nw; wi++) {
+		OWorldPart *Po = &old->W[wi];
+		for (int64_t k = 0; Po->pid_key && k <= Po->pid_mask; k++) {
+			if (Po->pid_key[k] == UINT64_MAX) continue;
+			const int32_t a = (int32_t)(Po->pid_key[k] >> 32), b = (int32_t)(Po->pid_key[k] & 0xffffffffu);
+			const int nw = NEW_PAIR_WORLD(a, b);
+			if (nw < 0) continue;
+			OWorldPart *Pn = &w->W[nw];
+			const uint64_t key = pair_key(inv[a], inv[b]);
+			uint64_t h = mix64(key) & (uint64_t)Pn->pid_mask;
+			while (Pn->pid_key[h] != UINT64_MAX) h = (h + 1) & (uint64_t)Pn->pid_mask;
+			Pn->pid_key[h] = key; Pn->pid_id[h] = Po->pid_id[k]; Pn->pid_stamp[h] = Po->pid_stamp[k];
+			Pn->pid_n++;
+		}
+	}
+#undef NEW_PAIR_WORLD
+	for (int wi = 0; wi < w->nw; wi++) tab_rebuild(&w->W[wi]);
+	free(pid_count);
+	free(inv);
+	w->steps = old->steps;
+	return w;
+}
+
 int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colour, int32_t *state,
                          int32_t *count, double *normal, double *contact, uint64_t *hash)
 {

@@ -49,6 +49,13 @@ int ow_read_bodies(oworld *w, double *p, double *v, double *a, double *wv,
                    double *v_bias, double *w_bias, double *transform);
 int ow_write_bodies(oworld *w, const double *v, const double *wv, const double *f, const double *t);
 
+/* Same as cpgpuWorldWriteBodiesAt, cpgpuWorldSetSpaceParams and
+ * cpgpuWorldCreateFrom (the old world is left as it was). */
+int ow_write_bodies_at(oworld *w, int32_t n, const int32_t *index, const double *p, const double *v,
+                       const double *f, const double *a, const double *wv, const double *t, const double *transform);
+int ow_set_space_params(oworld *w, int32_t space, const cpgpuSpaceParams *sp);
+oworld *ow_new_from(const cpgpuWorldDesc *desc, oworld *old, const int32_t *shape_from);
+
 /* Same layout as cpgpuWorldReadArbiters. */
 int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colour, int32_t *state,
                          int32_t *count, double *normal, double *contact, uint64_t *hash);

@@ -23,6 +23,7 @@
 typedef struct rbWorld {
 	cpSpace *space;
 	int hasty;
+	int owns_space;
 	int nb, ns, nc;
 	int b0, s0, c0;
 	cpBody **bodies;
@@ -39,17 +40,34 @@ static cpTransform xf(const double *t)
 
 /* Build world `wi` of the description as a cpSpace (hasty != 0: cpHastySpace
  * with that many threads). */
+static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi);
+
 rbWorld *rb_new(const cpgpuWorldDesc *d, int wi, int hasty)
 {
 	rbWorld *w = (rbWorld *)calloc(1, sizeof(rbWorld));
-	const cpgpuSpaceParams *sp = &d->space[wi];
 	w->hasty = hasty;
+	w->owns_space = 1;
 	if (hasty) {
 		w->space = cpHastySpaceNew();
 		cpHastySpaceSetThreads(w->space, (unsigned long)hasty);
 	} else {
 		w->space = cpSpaceNew();
 	}
+	return fill_space(w, d, wi);
+}
+
+/* Build world `wi` of the description, through the public API, into a space
+ * the caller made (for instance a cpGPUSpace) and keeps. */
+rbWorld *rb_new_in(cpSpace *space, const cpgpuWorldDesc *d, int wi)
+{
+	rbWorld *w = (rbWorld *)calloc(1, sizeof(rbWorld));
+	w->space = space;
+	return fill_space(w, d, wi);
+}
+
+static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi)
+{
+	const cpgpuSpaceParams *sp = &d->space[wi];
 	cpSpace *space = w->space;
 	cpSpaceSetIterations(space, sp->iterations);
 	cpSpaceSetGravity(space, cpv(sp->gravity_x, sp->gravity_y));
@@ -177,9 +195,19 @@ void rb_free(rbWorld *w)
 {
 	if (!w) return;
 	for (int k = 0; k < w->nc; k++) { cpSpaceRemoveConstraint(w->space, w->cons[k]); cpConstraintFree(w->cons[k]); }
-	for (int k = 0; k < w->ns; k++) { cpSpaceRemoveShape(w->space, w->shapes[k]); cpShapeFree(w->shapes[k]); }
-	for (int k = 0; k < w->nb; k++) { cpSpaceRemoveBody(w->space, w->bodies[k]); cpBodyFree(w->bodies[k]); }
-	if (w->hasty) cpHastySpaceFree(w->space); else cpSpaceFree(w->space);
+	for (int k = 0; k < w->ns; k++) {
+		if (!w->shapes[k]) continue;
+		cpSpaceRemoveShape(w->space, w->shapes[k]);
+		cpShapeFree(w->shapes[k]);
+	}
+	for (int k = 0; k < w->nb; k++) {
+		if (!w->bodies[k]) continue;
+		cpSpaceRemoveBody(w->space, w->bodies[k]);
+		cpBodyFree(w->bodies[k]);
+	}
+	if (w->owns_space) {
+		if (w->hasty) cpHastySpaceFree(w->space); else cpSpaceFree(w->space);
+	}
 	free(w->bodies); free(w->shapes); free(w->cons); free(w->ctype);
 	free(w);
 }
@@ -198,6 +226,7 @@ void rb_read_bodies(rbWorld *w, double *p, double *v, double *a, double *wv, dou
 {
 	for (int k = 0; k < w->nb; k++) {
 		cpBody *b = w->bodies[k];
+		if (!b) continue;   /* removed (rb_remove_body) */
 		if (p) { p[2 * k] = b->p.x; p[2 * k + 1] = b->p.y; }
 		if (v) { v[2 * k] = b->v.x; v[2 * k + 1] = b->v.y; }
 		if (a) a[k] = b->a;
@@ -393,4 +422,90 @@ int64_t rb_read_collision_log(int64_t cap, int32_t *out)
 	int64_t n = g_log_n < cap ? g_log_n : cap;
 	if (out && n) memcpy(out, g_log, (size_t)n * 4 * sizeof(int32_t));
 	return g_log_n;
+}
+
+/* cpSpaceRemoveShape / cpSpaceRemoveBody (cpSpace.c:523-555) of one object of
+ * the description, which is then freed and forgotten. */
+void rb_remove_shape(rbWorld *w, int k)
+{
+	cpSpaceRemoveShape(w->space, w->shapes[k]);
+	cpShapeFree(w->shapes[k]);
+	w->shapes[k] = NULL;
+}
+
+void rb_remove_body(rbWorld *w, int k)
+{
+	cpSpaceRemoveBody(w->space, w->bodies[k]);
+	cpBodyFree(w->bodies[k]);
+	w->bodies[k] = NULL;
+}
+
+/* ---- objects and raw state for the cpGPUSpace tests ---- */
+
+cpSpace *rb_space(rbWorld *w) { return w->space; }
+cpBody *rb_body(rbWorld *w, int k) { return w->bodies[k]; }
+cpShape *rb_shape(rbWorld *w, int k) { return w->shapes[k]; }
+cpConstraint *rb_constraint(rbWorld *w, int k) { return w->cons[k]; }
+
+/* The body's private state fields: p, v, f (2 each), a, w, t, v_bias (2),
+ * w_bias, transform (6): 17 doubles. */
+void rb_body_raw(const cpBody *b, double *out)
+{
+	out[0] = b->p.x; out[1] = b->p.y; out[2] = b->v.x; out[3] = b->v.y; out[4] = b->f.x; out[5] = b->f.y;
+	out[6] = b->a; out[7] = b->w; out[8] = b->t; out[9] = b->v_bias.x; out[10] = b->v_bias.y; out[11] = b->w_bias;
+	out[12] = b->transform.a; out[13] = b->transform.b; out[14] = b->transform.c;
+	out[15] = b->transform.d; out[16] = b->transform.tx; out[17] = b->transform.ty;
+}
+
+/* A constraint's accumulated impulse as the description stores it. */
+void rb_constraint_jacc(const cpConstraint *c, double *out)
+{
+	out[0] = out[1] = 0.0;
+	if (cpConstraintIsPivotJoint(c)) { out[0] = ((cpPivotJoint *)c)->jAcc.x; out[1] = ((cpPivotJoint *)c)->jAcc.y; }
+	else if (cpConstraintIsDampedSpring(c)) out[0] = ((cpDampedSpring *)c)->jAcc;
+	else if (cpConstraintIsSlideJoint(c)) out[0] = ((cpSlideJoint *)c)->jnAcc;
+	else if (cpConstraintIsGearJoint(c)) out[0] = ((cpGearJoint *)c)->jAcc;
+}
+
+typedef struct ArbList {
+	int n, cap;
+	const cpShape **a, **b;
+	int *count, *swapped;
+	double *jn, *jt, *impulse;
+	double *points;   /* per arbiter: normal, then per point pointA, pointB, distance */
+} ArbList;
+
+static void arb_visit(cpBody *body, cpArbiter *arb, void *data)
+{
+	ArbList *L = (ArbList *)data;
+	const int k = L->n++;
+	if (k >= L->cap) return;
+	CP_ARBITER_GET_SHAPES(arb, sa, sb);
+	L->a[k] = sa; L->b[k] = sb;
+	L->count[k] = cpArbiterGetCount(arb);
+	L->swapped[k] = arb->swapped;
+	L->jn[k] = L->jt[k] = 0.0;
+	for (int i = 0; i < arb->count; i++) { L->jn[k] += arb->contacts[i].jnAcc; L->jt[k] += arb->contacts[i].jtAcc; }
+	cpVect j = cpArbiterTotalImpulse(arb);
+	L->impulse[2 * k] = j.x; L->impulse[2 * k + 1] = j.y;
+	cpContactPointSet set = cpArbiterGetContactPointSet(arb);
+	double *q = L->points + 12 * k;
+	memset(q, 0, 12 * sizeof(double));
+	q[0] = set.normal.x; q[1] = set.normal.y;
+	for (int i = 0; i < set.count; i++) {
+		q[2 + 5 * i] = set.points[i].pointA.x; q[3 + 5 * i] = set.points[i].pointA.y;
+		q[4 + 5 * i] = set.points[i].pointB.x; q[5 + 5 * i] = set.points[i].pointB.y;
+		q[6 + 5 * i] = set.points[i].distance;
+	}
+}
+
+/* cpBodyEachArbiter (cpBody.c) over one body: each arbiter's shapes in the
+ * body's orientation (CP_ARBITER_GET_SHAPES), count, summed jnAcc / jtAcc,
+ * cpArbiterTotalImpulse and cpArbiterGetContactPointSet.  Returns the count. */
+int rb_body_arbiters(cpBody *body, int cap, const cpShape **a, const cpShape **b, int *count, int *swapped, double *jn,
+                     double *jt, double *impulse, double *points)
+{
+	ArbList L = {0, cap, a, b, count, swapped, jn, jt, impulse, points};
+	cpBodyEachArbiter(body, arb_visit, &L);
+	return L.n;
 }

@@ -9,10 +9,11 @@ import os
 
 import numpy as np
 
-from chipmunk2d_amd.cabi import Stats, WorldDesc, ptr
+from chipmunk2d_amd.cabi import PackedWorld, SpaceParams, Stats, WorldDesc, ptr
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+GPUSPACE_SO = os.path.join(ROOT, "oracle", "_ref", "libcpgpuspace.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref.so")
 REF_RELEASE_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref_release.so")
 
@@ -56,6 +57,10 @@ def oracle_lib():
         lib.ow_shapes_collide.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, _dp, _dp]
         lib.ow_candidate_pairs.restype = ctypes.c_int64
         lib.ow_candidate_pairs.argtypes = [_vp, ctypes.c_int64, _ip]
+        lib.ow_write_bodies_at.argtypes = [_vp, ctypes.c_int32, _ip] + [_dp] * 7
+        lib.ow_set_space_params.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(SpaceParams)]
+        lib.ow_new_from.restype = _vp
+        lib.ow_new_from.argtypes = [ctypes.POINTER(WorldDesc), _vp, _ip]
         lib.cp_sincos_oracle.argtypes = [_d, _dp, _dp]
         lib.cp_exp_oracle.restype = _d
         lib.cp_exp_oracle.argtypes = [_d]
@@ -92,6 +97,18 @@ def ref_lib(release=False):
         lib.rb_read_collision_log.argtypes = [ctypes.c_int64, _ip]
         lib.rb_space_stamp.restype = ctypes.c_int64
         lib.rb_space_stamp.argtypes = [_vp]
+        lib.rb_remove_shape.argtypes = lib.rb_remove_body.argtypes = [_vp, ctypes.c_int]
+        lib.rb_new_in.restype = _vp
+        lib.rb_new_in.argtypes = [_vp, ctypes.POINTER(WorldDesc), ctypes.c_int]
+        for name in ("rb_space", "rb_body", "rb_shape", "rb_constraint"):
+            getattr(lib, name).restype = _vp
+        lib.rb_space.argtypes = [_vp]
+        lib.rb_body.argtypes = lib.rb_shape.argtypes = lib.rb_constraint.argtypes = [_vp, ctypes.c_int]
+        lib.rb_body_raw.argtypes = [_vp, _dp]
+        lib.rb_constraint_jacc.argtypes = [_vp, _dp]
+        lib.rb_body_arbiters.restype = ctypes.c_int
+        lib.rb_body_arbiters.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _ip, _ip, _dp,
+                                         _dp, _dp, _dp]
         # a few reference entry points called directly
         lib.cpConvexHull.restype = ctypes.c_int
         lib.cpConvexHull.argtypes = [ctypes.c_int, _dp, _dp, _ip, _d]
@@ -189,6 +206,28 @@ class OracleWorld:
 
     def write_bodies(self, v=None, w=None, f=None, t=None):
         self.lib.ow_write_bodies(self.h, *[ptr(x, ctypes.c_double) if x is not None else None for x in (v, w, f, t)])
+
+    def write_bodies_at(self, index, p=None, v=None, f=None, a=None, w=None, t=None, transform=None):
+        index = np.ascontiguousarray(index, dtype=np.int32)
+        args = [None if x is None else ptr(np.ascontiguousarray(x, dtype=np.float64), ctypes.c_double)
+                for x in (p, v, f, a, w, t, transform)]
+        assert self.lib.ow_write_bodies_at(self.h, len(index), ptr(index, ctypes.c_int32), *args) == 0
+
+    def set_space_params(self, k, params):
+        sp = SpaceParams(**params)
+        assert self.lib.ow_set_space_params(self.h, k, ctypes.byref(sp)) == 0
+
+    @classmethod
+    def from_old(cls, packed, old, shape_from):
+        """The oracle's counterpart of cpgpuWorldCreateFrom: a world of `packed` that keeps
+        the collision state of `old` under the new shape indices."""
+        self = cls.__new__(cls)
+        self.lib = oracle_lib()
+        self.packed = packed
+        self.desc = packed.desc()
+        sf = np.ascontiguousarray(shape_from, dtype=np.int32)
+        self.h = self.lib.ow_new_from(ctypes.byref(self.desc), old.h, ptr(sf, ctypes.c_int32))
+        return self
 
     def stats(self):
         st = Stats()
@@ -349,3 +388,82 @@ def sincos(x):
     c = ctypes.c_double()
     lib.cp_sincos_oracle(x, ctypes.byref(s), ctypes.byref(c))
     return s.value, c.value
+
+
+def packed_from_desc(desc_ptr):
+    """A PackedWorld holding copies of the arrays a cpgpuWorldDesc points to."""
+    d = ctypes.cast(desc_ptr, ctypes.POINTER(WorldDesc)).contents
+
+    def arr(p, n, dtype, width=1):
+        shape = (n, width) if width > 1 else (n,)
+        if n == 0 or not p:
+            return np.zeros(shape, dtype)
+        return np.ctypeslib.as_array(p, shape=(n * width,)).astype(dtype).reshape(shape).copy()
+
+    nw, nb, ns, nc, nv = d.n_worlds, d.n_bodies, d.n_shapes, d.n_constraints, d.n_poly_verts
+    a = {"space": [{name: getattr(d.space[k], name) for name, _ in SpaceParams._fields_ if name != "_pad"}
+                   for k in range(nw)]}
+    a["world_body_start"] = arr(d.world_body_start, nw + 1, np.int32)
+    a["world_shape_start"] = arr(d.world_shape_start, nw + 1, np.int32)
+    a["world_constraint_start"] = arr(d.world_constraint_start, nw + 1, np.int32)
+    a["body_type"] = arr(d.body_type, nb, np.int32)
+    for n in ("m", "m_inv", "i", "i_inv", "a", "w", "t", "w_bias"):
+        a["body_" + n] = arr(getattr(d, "body_" + n), nb, np.float64)
+    for n in ("cog", "p", "v", "f", "v_bias"):
+        a["body_" + n] = arr(getattr(d, "body_" + n), nb, np.float64, 2)
+    a["body_transform"] = arr(d.body_transform, nb, np.float64, 6)
+    a["shape_type"] = arr(d.shape_type, ns, np.int32)
+    a["shape_body"] = arr(d.shape_body, ns, np.int32)
+    a["shape_hashid"] = arr(d.shape_hashid, ns, np.uint64)
+    a["shape_e"] = arr(d.shape_e, ns, np.float64)
+    a["shape_u"] = arr(d.shape_u, ns, np.float64)
+    a["shape_surface_v"] = arr(d.shape_surface_v, ns, np.float64, 2)
+    a["shape_group"] = arr(d.shape_group, ns, np.uint64)
+    a["shape_categories"] = arr(d.shape_categories, ns, np.uint32)
+    a["shape_mask"] = arr(d.shape_mask, ns, np.uint32)
+    a["shape_radius"] = arr(d.shape_radius, ns, np.float64)
+    a["shape_geom"] = arr(d.shape_geom, ns, np.float64, 10)
+    a["shape_poly_start"] = arr(d.shape_poly_start, ns, np.int32)
+    a["shape_poly_count"] = arr(d.shape_poly_count, ns, np.int32)
+    a["poly_verts"] = arr(d.poly_verts, nv, np.float64, 2)
+    a["poly_normals"] = arr(d.poly_normals, nv, np.float64, 2)
+    a["cons_type"] = arr(d.cons_type, nc, np.int32)
+    a["cons_a"] = arr(d.cons_a, nc, np.int32)
+    a["cons_b"] = arr(d.cons_b, nc, np.int32)
+    for n in ("max_force", "error_bias", "max_bias"):
+        a["cons_" + n] = arr(getattr(d, "cons_" + n), nc, np.float64)
+    a["cons_collide_bodies"] = arr(d.cons_collide_bodies, nc, np.int32)
+    a["cons_params"] = arr(d.cons_params, nc, np.float64, 8)
+    a["cons_jacc"] = arr(d.cons_jacc, nc, np.float64, 2)
+    return PackedWorld(a)
+
+
+def without(pw, shapes=(), bodies=()):
+    """The one-space description pw with the listed shapes and bodies taken out
+    (the others keep their order, indices renumbered); constraints must not use
+    a removed body.  Returns (description, kept shape indices, kept body indices)."""
+    a = pw.a
+    keep_s = np.array([k for k in range(pw.n_shapes) if k not in set(shapes)], np.int64)
+    keep_b = np.array([k for k in range(pw.n_bodies) if k not in set(bodies)], np.int64)
+    new_b = -np.ones(pw.n_bodies, np.int64)
+    new_b[keep_b] = np.arange(len(keep_b))
+    assert np.all(new_b[a["shape_body"][keep_s]] >= 0)
+    out = {"space": [dict(sp) for sp in a["space"]]}
+    for k, val in a.items():
+        if k == "space" or k.startswith("world_"):
+            continue
+        if k.startswith("body_"):
+            out[k] = np.ascontiguousarray(val[keep_b])
+        elif k.startswith("shape_"):
+            out[k] = np.ascontiguousarray(val[keep_s])
+        else:
+            out[k] = val.copy()
+    out["shape_body"] = new_b[a["shape_body"][keep_s]].astype(np.int32)
+    if pw.n_constraints:
+        out["cons_a"] = new_b[a["cons_a"]].astype(np.int32)
+        out["cons_b"] = new_b[a["cons_b"]].astype(np.int32)
+        assert np.all(out["cons_a"] >= 0) and np.all(out["cons_b"] >= 0)
+    out["world_body_start"] = np.array([0, len(keep_b)], np.int32)
+    out["world_shape_start"] = np.array([0, len(keep_s)], np.int32)
+    out["world_constraint_start"] = a["world_constraint_start"].copy()
+    return PackedWorld(out), keep_s, keep_b

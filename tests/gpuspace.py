@@ -1,0 +1,151 @@
+"""ctypes handles on the cpGPUSpace binding (oracle/_ref/libcpgpuspace.so,
+include/cpGPUSpace.h) and on the Chipmunk public API of the compiled
+reference it links (oracle/_ref/libcpref.so) -- test infrastructure.
+
+Both libraries are built by `make -C oracle` where the reference sources are
+present; the binding library is loaded after the reference library, so both
+use the one copy of Chipmunk.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from checkers import GPUSPACE_SO, ref_lib
+from chipmunk2d_amd.cabi import WorldDesc, ptr
+
+_d = ctypes.c_double
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class cpVect(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double)]
+
+
+class cpShapeFilter(ctypes.Structure):
+    _fields_ = [("group", ctypes.c_void_p), ("categories", ctypes.c_uint), ("mask", ctypes.c_uint)]
+
+
+class cpTransform(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("a", "b", "c", "d", "tx", "ty")]
+
+
+IDENTITY = cpTransform(1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+
+_cp = None
+_gs = None
+
+
+def cp():
+    """The reference's public API (the functions the tests call directly)."""
+    global _cp
+    if _cp is None:
+        L = ref_lib()
+        sig = {
+            "cpSpaceNew": (_vp, []), "cpSpaceFree": (None, [_vp]),
+            "cpSpaceSetGravity": (None, [_vp, cpVect]), "cpSpaceSetIterations": (None, [_vp, ctypes.c_int]),
+            "cpSpaceSetDamping": (None, [_vp, _d]), "cpSpaceSetCollisionSlop": (None, [_vp, _d]),
+            "cpSpaceGetStaticBody": (_vp, [_vp]),
+            "cpSpaceAddBody": (_vp, [_vp, _vp]), "cpSpaceAddShape": (_vp, [_vp, _vp]),
+            "cpSpaceAddConstraint": (_vp, [_vp, _vp]),
+            "cpSpaceRemoveBody": (None, [_vp, _vp]), "cpSpaceRemoveShape": (None, [_vp, _vp]),
+            "cpSpaceRemoveConstraint": (None, [_vp, _vp]),
+            "cpSpaceSetSleepTimeThreshold": (None, [_vp, _d]),
+            "cpBodyNew": (_vp, [_d, _d]), "cpBodyNewStatic": (_vp, []), "cpBodyFree": (None, [_vp]),
+            "cpBodySetPosition": (None, [_vp, cpVect]), "cpBodySetVelocity": (None, [_vp, cpVect]),
+            "cpBodySetAngle": (None, [_vp, _d]), "cpBodySetAngularVelocity": (None, [_vp, _d]),
+            "cpBodySetForce": (None, [_vp, cpVect]), "cpBodySetTorque": (None, [_vp, _d]),
+            "cpBodyApplyForceAtWorldPoint": (None, [_vp, cpVect, cpVect]),
+            "cpBodyApplyImpulseAtWorldPoint": (None, [_vp, cpVect, cpVect]),
+            "cpBodyGetPosition": (cpVect, [_vp]), "cpBodyGetVelocity": (cpVect, [_vp]),
+            "cpBodyGetAngle": (_d, [_vp]), "cpBodyGetAngularVelocity": (_d, [_vp]),
+            "cpCircleShapeNew": (_vp, [_vp, _d, cpVect]),
+            "cpSegmentShapeNew": (_vp, [_vp, cpVect, cpVect, _d]),
+            "cpBoxShapeNew": (_vp, [_vp, _d, _d, _d]),
+            "cpPolyShapeNew": (_vp, [_vp, ctypes.c_int, ctypes.POINTER(cpVect), cpTransform, _d]),
+            "cpShapeFree": (None, [_vp]),
+            "cpShapeSetFriction": (None, [_vp, _d]), "cpShapeSetElasticity": (None, [_vp, _d]),
+            "cpShapeSetSensor": (None, [_vp, ctypes.c_ubyte]),
+            "cpShapeGetBB": (ctypes.c_double * 4, [_vp]),
+            "cpPivotJointNew2": (_vp, [_vp, _vp, cpVect, cpVect]),
+            "cpDampedSpringNew": (_vp, [_vp, _vp, cpVect, cpVect, _d, _d, _d]),
+            "cpDampedSpringSetStiffness": (None, [_vp, _d]),
+            "cpPinJointNew": (_vp, [_vp, _vp, cpVect, cpVect]),
+            "cpConstraintFree": (None, [_vp]),
+            "cpMomentForCircle": (_d, [_d, _d, _d, cpVect]), "cpMomentForBox": (_d, [_d, _d, _d]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _cp = L
+    return _cp
+
+
+def gs():
+    """The cpGPUSpace binding."""
+    global _gs
+    if _gs is None:
+        cp()   # the reference first: the binding resolves Chipmunk to the same copy
+        L = ctypes.CDLL(GPUSPACE_SO)
+        sig = {
+            "cpGPUSpaceNew": (_vp, []), "cpGPUSpaceNewWithPrecision": (_vp, [ctypes.c_int]),
+            "cpGPUSpaceFree": (None, [_vp]), "cpGPUSpaceStep": (None, [_vp, _d]),
+            "cpGPUSpaceSyncHost": (None, [_vp]), "cpGPUSpaceGetWorld": (_vp, [_vp]),
+            "cpGPUSpaceGetDesc": (ctypes.POINTER(WorldDesc), [_vp]),
+            "cpGPUSpaceBodyIndex": (ctypes.c_int, [_vp, _vp]), "cpGPUSpaceShapeIndex": (ctypes.c_int, [_vp, _vp]),
+            "cpGPUSpaceConstraintIndex": (ctypes.c_int, [_vp, _vp]),
+            "cpGPUSpaceGetPackCount": (ctypes.c_int, [_vp]), "cpGPUSpaceGetShapeMap": (_ip, [_vp]),
+            "cpGPUSpaceDescribe": (ctypes.POINTER(WorldDesc), [_vp]),
+            "cpGPUBatchDescribe": (ctypes.POINTER(WorldDesc), [_vp]),
+            "cpGPUBatchNew": (_vp, [ctypes.POINTER(_vp), ctypes.c_int]),
+            "cpGPUBatchNewWithPrecision": (_vp, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
+            "cpGPUBatchFree": (None, [_vp]), "cpGPUBatchStep": (None, [_vp, _d]),
+            "cpGPUBatchSyncHost": (None, [_vp]), "cpGPUBatchAutotune": (None, [_vp, _d, ctypes.c_int]),
+            "cpGPUBatchGetWorld": (_vp, [_vp]), "cpGPUBatchGetDesc": (ctypes.POINTER(WorldDesc), [_vp]),
+            "cpGPUBatchGetPackCount": (ctypes.c_int, [_vp]), "cpGPUBatchGetShapeMap": (_ip, [_vp]),
+            "cpGPUSetAbortOnError": (None, [ctypes.c_int]), "cpGPUGetError": (ctypes.c_char_p, []),
+            "cpGPUClearError": (None, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _gs = L
+    return _gs
+
+
+def available():
+    return os.path.exists(GPUSPACE_SO)
+
+
+def v(x, y):
+    return cpVect(float(x), float(y))
+
+
+def body_raw(body):
+    """The body's private state: p (2), v (2), f (2), a, w, t, v_bias (2), w_bias, transform (6)."""
+    out = np.zeros(18)
+    ref_lib().rb_body_raw(body, ptr(out, ctypes.c_double))
+    return out
+
+
+RAW = {"p": slice(0, 2), "v": slice(2, 4), "f": slice(4, 6), "a": 6, "w": 7, "t": 8, "v_bias": slice(9, 11),
+       "w_bias": 11, "transform": slice(12, 18)}
+
+
+def body_arbiters(body, cap=64):
+    """cpBodyEachArbiter over one body (through the reference's own iterator)."""
+    a = (_vp * cap)()
+    b = (_vp * cap)()
+    count = np.zeros(cap, np.int32)
+    swapped = np.zeros(cap, np.int32)
+    jn, jt = np.zeros(cap), np.zeros(cap)
+    imp, pts = np.zeros((cap, 2)), np.zeros((cap, 12))
+    n = ref_lib().rb_body_arbiters(body, cap, a, b, ptr(count, ctypes.c_int32), ptr(swapped, ctypes.c_int32),
+                                   ptr(jn, ctypes.c_double), ptr(jt, ctypes.c_double), ptr(imp, ctypes.c_double),
+                                   ptr(pts, ctypes.c_double))
+    assert n <= cap, n
+    return [dict(a=a[k], b=b[k], count=int(count[k]), swapped=int(swapped[k]), jn=jn[k], jt=jt[k], impulse=imp[k],
+                 points=pts[k]) for k in range(n)]

@@ -1,0 +1,153 @@
+"""The cpGPUSpace binding's packer on the host (no device needed): Chipmunk
+spaces built through the reference's public API are described
+(cpGPUSpaceDescribe, cpGPUBatchDescribe) exactly as the packed description
+they were built from, and features the device does not run are refused with
+cpAssertHard's message instead of a pack."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import gpuspace
+from checkers import packed_from_desc, ref_lib
+from chipmunk2d_amd.scenes import chain_world, pyramid_210, random_world
+from gpuspace import cp, gs, v
+
+pytestmark = pytest.mark.skipif(not gpuspace.available(),
+                                reason="the binding is built only where the reference sources are present")
+
+
+@pytest.fixture(autouse=True)
+def _errors_return():
+    gs().cpGPUSetAbortOnError(0)
+    gs().cpGPUClearError()
+    yield
+    gs().cpGPUSetAbortOnError(1)
+
+
+def _binding_order(body_type):
+    """space->dynamicBodies (dynamic and kinematic), then space->staticBodies, each in the order added."""
+    return np.concatenate([np.nonzero(body_type != 2)[0], np.nonzero(body_type == 2)[0]])
+
+
+def _assert_describes(pw, d, j):
+    """d (the binding's description) equals pw with body k at index j[k]."""
+    a, b = pw.a, d.a
+    assert d.n_bodies == pw.n_bodies and sorted(j) == list(range(pw.n_bodies))
+    for f in ("body_type", "body_m", "body_m_inv", "body_i", "body_i_inv", "body_cog", "body_p", "body_v", "body_f",
+              "body_a", "body_w", "body_t", "body_v_bias", "body_w_bias", "body_transform"):
+        assert np.array_equal(b[f][j], a[f]), f
+    assert np.array_equal(b["shape_hashid"], a["shape_hashid"])
+    assert np.array_equal(b["shape_body"], j[a["shape_body"]])
+    for f in ("shape_type", "shape_e", "shape_u", "shape_surface_v", "shape_group", "shape_categories", "shape_mask",
+              "shape_radius", "shape_geom", "shape_poly_count"):
+        assert np.array_equal(b[f], a[f]), f
+    # polygon vertices and normals, shape by shape (the binding lays them out in shape order)
+    for k in np.nonzero(a["shape_type"] == 2)[0]:
+        n, s0, s1 = a["shape_poly_count"][k], a["shape_poly_start"][k], b["shape_poly_start"][k]
+        assert np.array_equal(b["poly_verts"][s1:s1 + n], a["poly_verts"][s0:s0 + n])
+        assert np.array_equal(b["poly_normals"][s1:s1 + n], a["poly_normals"][s0:s0 + n])
+    for f in ("cons_type", "cons_max_force", "cons_error_bias", "cons_max_bias", "cons_collide_bodies",
+              "cons_params", "cons_jacc"):
+        assert np.array_equal(b[f], a[f]), f
+    if pw.n_constraints:
+        assert np.array_equal(b["cons_a"], j[a["cons_a"]]) and np.array_equal(b["cons_b"], j[a["cons_b"]])
+    for k in ("iterations", "collision_persistence", "gravity_x", "gravity_y", "damping", "collision_slop",
+              "collision_bias", "curr_dt", "stamp"):
+        assert d.a["space"][0][k] == pw.a["space"][0][k], k
+
+
+@pytest.mark.parametrize("scene", ["pyramid", "random", "chain"])
+def test_describe_equals_the_description_the_space_was_built_from(scene):
+    pw = {"pyramid": lambda: pyramid_210(rows=8), "random": lambda: random_world(
+        n_circles=80, n_polys=80, n_segments=20, extent=40.0, seed=9), "chain": lambda: chain_world(
+        15, 40, seed=5)}[scene]().pack()
+    space = gs().cpGPUSpaceNew()
+    desc = pw.desc()
+    rb = ref_lib().rb_new_in(space, ctypes.byref(desc), 0)
+    d = gs().cpGPUSpaceDescribe(space)
+    assert d, gs().cpGPUGetError()
+    d = packed_from_desc(d)
+    bodies = [ref_lib().rb_body(rb, k) for k in range(pw.n_bodies)]
+    j = np.array([0] * pw.n_bodies)
+    order = _binding_order(pw.a["body_type"])
+    j[order] = np.arange(pw.n_bodies)
+    _assert_describes(pw, d, j)
+    assert gs().cpGPUSpaceGetPackCount(space) == 0 and not gs().cpGPUSpaceGetWorld(space)
+    # the index queries answer only for a pack (none yet)
+    assert gs().cpGPUSpaceBodyIndex(space, bodies[0]) == -1
+    ref_lib().rb_free(rb)
+    gs().cpGPUSpaceFree(space)
+
+
+def test_batch_describes_its_spaces_in_order():
+    packs = [pyramid_210(rows=4).pack(), chain_world(6, 10, seed=2).pack()]
+    descs = [p.desc() for p in packs]
+    rbs = [ref_lib().rb_new(ctypes.byref(d), 0, 0) for d in descs]
+    spaces = (ctypes.c_void_p * 2)(*[ref_lib().rb_space(r) for r in rbs])
+    batch = gs().cpGPUBatchNew(spaces, 2)
+    d = packed_from_desc(gs().cpGPUBatchDescribe(batch))
+    assert d.n_worlds == 2
+    assert list(d.a["world_body_start"]) == [0, packs[0].n_bodies, packs[0].n_bodies + packs[1].n_bodies]
+    assert list(d.a["world_shape_start"]) == [0, packs[0].n_shapes, packs[0].n_shapes + packs[1].n_shapes]
+    # hashids are per space, as cpSpaceAddShape numbers them
+    assert d.a["shape_hashid"][packs[0].n_shapes] == 0
+    gs().cpGPUBatchFree(batch)
+    for r in rbs:
+        ref_lib().rb_free(r)
+
+
+def _space_with_one_circle():
+    C = cp()
+    space = gs().cpGPUSpaceNew()
+    body = C.cpBodyNew(1.0, 1.0)
+    C.cpSpaceAddBody(space, body)
+    shape = C.cpCircleShapeNew(body, 1.0, v(0, 0))
+    C.cpSpaceAddShape(space, shape)
+    return space, body, shape
+
+
+def _free(space, body, shape):
+    C = cp()
+    C.cpSpaceRemoveShape(space, shape)
+    C.cpSpaceRemoveBody(space, body)
+    C.cpShapeFree(shape)
+    C.cpBodyFree(body)
+    gs().cpGPUSpaceFree(space)
+
+
+@pytest.mark.parametrize("feature", ["sensor", "sleeping", "pin joint"])
+def test_unsupported_features_are_refused_at_pack_time(feature):
+    """Sensors, sleeping and joints other than pivot, damped spring, slide and
+    gear are refused when the space is packed (the step does nothing); the
+    message is cpAssertHard's (which aborts when cpGPUSetAbortOnError is on)."""
+    C = cp()
+    space, body, shape = _space_with_one_circle()
+    extra = []
+    if feature == "sensor":
+        C.cpShapeSetSensor(shape, 1)
+        msg = "sensors"
+    elif feature == "sleeping":
+        C.cpSpaceSetSleepTimeThreshold(space, 1.0)
+        msg = "sleeping"
+    elif feature == "pin joint":
+        other = C.cpBodyNew(1.0, 1.0)
+        C.cpSpaceAddBody(space, other)
+        pin = C.cpPinJointNew(body, other, v(0, 0), v(0, 0))
+        C.cpSpaceAddConstraint(space, pin)
+        extra = [("c", pin), ("b", other)]
+        msg = "only pivot"
+    assert not gs().cpGPUSpaceDescribe(space)
+    assert msg in gs().cpGPUGetError().decode(), gs().cpGPUGetError()
+    gs().cpGPUClearError()
+    gs().cpGPUSpaceStep(space, 1.0 / 60.0)
+    assert msg in gs().cpGPUGetError().decode()
+    assert not gs().cpGPUSpaceGetWorld(space)
+    for kind, obj in extra:
+        if kind == "c":
+            C.cpSpaceRemoveConstraint(space, obj)
+            C.cpConstraintFree(obj)
+        else:
+            C.cpSpaceRemoveBody(space, obj)
+            C.cpBodyFree(obj)
+    _free(space, body, shape)

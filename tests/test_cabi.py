@@ -86,3 +86,19 @@ def test_batch_packing_offsets():
     assert pw.a["shape_body"][a.n_shapes] == a.n_bodies + b.a["shape_body"][0]
     # hashids stay local to each space, as cpSpaceAddShape assigns them
     assert pw.a["shape_hashid"][a.n_shapes] == 0
+
+
+def test_binding_exports_every_symbol_of_its_header():
+    """oracle/_ref/libcpgpuspace.so (the cpGPUSpace binding, built against the
+    reference's headers) exports every function include/cpGPUSpace.h declares."""
+    import gpuspace
+    if not gpuspace.available():
+        pytest.skip("the binding is built only where the reference sources are present")
+    text = open(os.path.join(ROOT, "include", "cpGPUSpace.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b(cpGPU[A-Za-z]+)\s*\(", text)))
+    assert len(names) >= 20
+    gpuspace.cp()
+    L = ctypes.CDLL(gpuspace.GPUSPACE_SO)
+    for name in names:
+        assert hasattr(L, name), name

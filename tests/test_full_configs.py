@@ -129,6 +129,15 @@ def _world_losses(p, targets, dyn, world_of_body, n_worlds):
     return np.bincount(world_of_body, weights=np.sum(d * d, axis=1), minlength=n_worlds)
 
 
+def _world_loss_diffs(p_plus, p_minus, targets, dyn, world_of_body, n_worlds):
+    """L(+) - L(-) per space, as the sum of (p+ - p-) . (p+ + p- - 2 target).
+    Subtracting the two losses instead would round the difference at the
+    loss's own magnitude: with L ~ 1e4 that is ~1e-12, as large as the whole
+    difference of a gradient of 1e-3 at eps 1e-6."""
+    d = np.sum((p_plus - p_minus) * (p_plus + p_minus - 2.0 * targets), axis=1) * dyn
+    return np.bincount(world_of_body, weights=d, minlength=n_worlds)
+
+
 def test_grad_1024_adjoint_matches_central_differences():
     """The loss is the sum over bodies of |p_T - target|^2.  Space k perturbs
     one parameter (kind k mod 3: an initial velocity component, a mass, a
@@ -179,16 +188,17 @@ def test_grad_1024_adjoint_matches_central_differences():
                 u[idx] += h
         return G.with_params(pw, v0, m, u)
 
-    L, same = {}, np.ones(n_worlds, bool)
+    P, same = {}, np.ones(n_worlds, bool)
     for h in (eps, -eps, 2 * eps, -2 * eps):
         gh, sig = _rollout(perturbed(h), steps, world_of_shape)
-        L[h] = _world_losses(gh.bodies().p, targets, dyn.astype(np.float64), world_of_body, n_worlds)
+        P[h] = gh.bodies().p
         gh.close()
         same &= np.all(sig == sig0, axis=1)
     _log("four perturbed rollouts; %d of %d spaces keep their contact set" % (same.sum(), n_worlds), t0)
 
-    fd1 = (L[eps] - L[-eps]) / (2 * eps)
-    fd2 = (L[2 * eps] - L[-2 * eps]) / (4 * eps)
+    dynf = dyn.astype(np.float64)
+    fd1 = _world_loss_diffs(P[eps], P[-eps], targets, dynf, world_of_body, n_worlds) / (2 * eps)
+    fd2 = _world_loss_diffs(P[2 * eps], P[-2 * eps], targets, dynf, world_of_body, n_worlds) / (4 * eps)
     ad = np.array([grads[kind][idx] for kind, idx in cases])
     report = {kind: [] for kind in kinds}
     rejected = {kind: 0 for kind in kinds}

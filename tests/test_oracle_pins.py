@@ -15,7 +15,7 @@ import pytest
 from chipmunk2d_amd.cabi import PackedWorld, BODY_DYNAMIC
 from chipmunk2d_amd.scenes import pyramid_210, random_world, Rng
 from chipmunk2d_amd.space import Space, Body, Circle, Segment, Poly, Box, moment_for_circle, moment_for_box
-from checkers import OracleWorld, RefWorld, ref_available, sincos, oracle_lib
+from checkers import OracleWorld, RefWorld, ref_available, sincos, oracle_lib, without
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 needs_ref = pytest.mark.skipif(not ref_available(), reason="compiled reference oracle/_ref/libcpref.so not built")
@@ -254,6 +254,78 @@ def test_replay_matches_reference_live(seed):
     assert np.array_equal(ra.hash, oa.hash)
     assert np.array_equal(ra.state, oa.state)
     assert r.cached_arbiters() == o.stats()["cached_arbiters"]
+
+
+@needs_ref
+def test_new_pack_after_removes_matches_reference():
+    """cpSpaceRemoveShape / cpSpaceRemoveBody (cpSpace.c:523-555) mid-run: the
+    oracle packed again without the removed objects (ow_new_from, the
+    counterpart of cpgpuWorldCreateFrom) keeps every surviving arbiter, cached
+    impulse and warm-start id exactly as the reference keeps them, so the
+    replay stays bit-exact; the arbiters of the removed shapes are gone.  A new
+    pack with nothing removed changes nothing at all."""
+    dt = 1 / 60
+    pw = random_world(n_circles=250, n_polys=250, n_segments=100, extent=100.0, seed=6).pack()
+    o = OracleWorld(pw)
+    r = RefWorld(pw)
+    o.libm_trig(True)
+    for s in range(60):
+        coll = r.step_logged(dt)
+        o.replay(r.arbiters(), coll, dt)
+    A = r.arbiters()
+    # four dynamic shapes that are touching something now, with their bodies
+    touching = sorted({int(x) for x in A.shapes.ravel() if pw.a["body_type"][pw.a["shape_body"][x]] == BODY_DYNAMIC})
+    gone = touching[::max(1, len(touching) // 4)][:4]
+    gone_bodies = sorted({int(pw.a["shape_body"][k]) for k in gone})
+    assert len(gone) == 4 and len(gone_bodies) == 4
+
+    def current(pwx, keep_b):
+        """pwx with the oracle's body state and the reference's stamp and dt."""
+        b = o.bodies()
+        for f, key in (("p", "body_p"), ("v", "body_v"), ("a", "body_a"), ("w", "body_w"), ("v_bias", "body_v_bias"),
+                       ("w_bias", "body_w_bias"), ("transform", "body_transform")):
+            pwx.a[key] = np.ascontiguousarray(getattr(b, f)[keep_b])
+        pwx.a["body_f"] = np.zeros_like(pwx.a["body_f"])
+        pwx.a["body_t"] = np.zeros_like(pwx.a["body_t"])
+        pwx.a["space"][0].update(stamp=int(r.lib.rb_space_stamp(r.h)), curr_dt=dt)
+        return pwx
+
+    # a new pack that removes nothing
+    same, keep_s0, keep_b0 = without(pw)
+    o_same = OracleWorld.from_old(current(same, keep_b0), o, keep_s0)
+    o_same.libm_trig(True)
+    for k in gone:
+        r.lib.rb_remove_shape(r.h, k)
+    for k in gone_bodies:
+        r.lib.rb_remove_body(r.h, k)
+    pw2, keep_s, keep_b = without(pw, shapes=gone, bodies=gone_bodies)
+    o2 = OracleWorld.from_old(current(pw2, keep_b), o, keep_s)
+    o2.libm_trig(True)
+    new_s = -np.ones(pw.n_shapes, np.int64)
+    new_s[keep_s] = np.arange(len(keep_s))
+    for s in range(60):
+        coll = r.step_logged(dt)
+        ra = r.arbiters()
+        assert not np.isin(ra.shapes, gone).any()
+        coll[:, :2] = new_s[coll[:, :2]]
+        ra.shapes = new_s[ra.shapes].astype(np.int32)
+        o2.replay(ra, coll, dt)
+        if s < 20:
+            # the unchanged pack, stepped on its own, stays with the original world
+            o.step(dt)
+            o_same.step(dt)
+            ob, sb = o.bodies(), o_same.bodies()
+            for f in ob.FIELDS:
+                assert np.array_equal(getattr(ob, f), getattr(sb, f)), (s, f)
+    rb, ob = r.bodies(), o2.bodies()
+    for f in rb.FIELDS:
+        assert np.array_equal(getattr(rb, f)[keep_b], getattr(ob, f)), f
+    ra, oa = r.arbiters(), o2.arbiters()
+    assert ra.n == oa.n and ra.n > 100
+    assert np.array_equal(new_s[ra.shapes], oa.shapes)
+    assert np.array_equal(ra.contact, oa.contact)
+    assert np.array_equal(ra.state, oa.state)
+    assert r.cached_arbiters() == o2.stats()["cached_arbiters"]
 
 
 # ---------------------------------------------------------------------------

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY (the checker of the HIP step library; see the
  * header).  Compiled with gcc -O2 -ffp-contract=off -fno-fast-math so every
- * + - * / sqrt is a single IEEE double operation, the same as the reference's
+ * + - * / sqrt is a single IEEE num operation, the same as the reference's
  * strict build and the device library.
  *
  * Every routine cites the reference file:line it restates.  How each part is
@@ -38,34 +38,45 @@
 
 #include "cpstep_oracle.h"
 
+/* The scalar of the step's state and arithmetic: double here.  dual_step.cpp
+ * compiles this file again as C++ with a dual number (value and tangent) in
+ * its place, for the forward-mode derivative of the same step; NUM_VAL(x) is
+ * the value part, for the decisions and outputs that take a double. */
+#ifndef ORACLE_NUM
+typedef double num;
+#define NUM_VAL(x) (x)
+#else
+typedef ORACLE_NUM num;
+#endif
+
 /* ------------------------------------------------------------------ */
 /* scalar and vector helpers: include/chipmunk/chipmunk_types.h:96-147,
  * cpVect.h, cpBB.h, cpTransform.h                                     */
 
-typedef struct { double x, y; } V;
+typedef struct { num x, y; } V;
 
-static inline V v2(double x, double y) { V r = {x, y}; return r; }
+static inline V v2(num x, num y) { V r = {x, y}; return r; }
 static inline V vadd(V a, V b) { return v2(a.x + b.x, a.y + b.y); }
 static inline V vsub(V a, V b) { return v2(a.x - b.x, a.y - b.y); }
 static inline V vneg(V a) { return v2(-a.x, -a.y); }
-static inline V vmult(V a, double s) { return v2(a.x * s, a.y * s); }
-static inline double vdot(V a, V b) { return a.x * b.x + a.y * b.y; }
-static inline double vcross(V a, V b) { return a.x * b.y - a.y * b.x; }
+static inline V vmult(V a, num s) { return v2(a.x * s, a.y * s); }
+static inline num vdot(V a, V b) { return a.x * b.x + a.y * b.y; }
+static inline num vcross(V a, V b) { return a.x * b.y - a.y * b.x; }
 static inline V vperp(V a) { return v2(-a.y, a.x); }
 static inline V vrperp(V a) { return v2(a.y, -a.x); }
 static inline V vrotate(V a, V b) { return v2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-static inline double vlengthsq(V a) { return vdot(a, a); }
-static inline double vlength(V a) { return sqrt(vdot(a, a)); }
+static inline num vlengthsq(V a) { return vdot(a, a); }
+static inline num vlength(V a) { return sqrt(vdot(a, a)); }
 static inline V vnormalize(V a) { return vmult(a, 1.0 / (vlength(a) + DBL_MIN)); }
-static inline V vlerp(V a, V b, double t) { return vadd(vmult(a, 1.0 - t), vmult(b, t)); }
+static inline V vlerp(V a, V b, num t) { return vadd(vmult(a, 1.0 - t), vmult(b, t)); }
 static inline int veql(V a, V b) { return a.x == b.x && a.y == b.y; }
-static inline V vclamp(V v, double len) { return (vdot(v, v) > len * len) ? vmult(vnormalize(v), len) : v; }
-static inline double fmaxc(double a, double b) { return (a > b) ? a : b; }
-static inline double fminc(double a, double b) { return (a < b) ? a : b; }
-static inline double fclampc(double f, double lo, double hi) { return fminc(fmaxc(f, lo), hi); }
-static inline double fclamp01(double f) { return fmaxc(0.0, fminc(f, 1.0)); }
+static inline V vclamp(V v, num len) { return (vdot(v, v) > len * len) ? vmult(vnormalize(v), len) : v; }
+static inline num fmaxc(num a, num b) { return (a > b) ? a : b; }
+static inline num fminc(num a, num b) { return (a < b) ? a : b; }
+static inline num fclampc(num f, num lo, num hi) { return fminc(fmaxc(f, lo), hi); }
+static inline num fclamp01(num f) { return fmaxc(0.0, fminc(f, 1.0)); }
 
-typedef struct { double a, b, c, d, tx, ty; } XF;
+typedef struct { num a, b, c, d, tx, ty; } XF;
 static inline V xpoint(XF t, V p) { return v2(t.a * p.x + t.c * p.y + t.tx, t.b * p.x + t.d * p.y + t.ty); }
 static inline V xvect(XF t, V v) { return v2(t.a * v.x + t.c * v.y, t.b * v.x + t.d * v.y); }
 
@@ -77,10 +88,10 @@ static inline uint64_t hash_pair(uint64_t a, uint64_t b) { return a * HASH_COEF 
 /* Deterministic transcendentals shared (as an algorithm) with the device
  * library.  sin/cos: Cody-Waite reduction by pi/2 in three FMA steps, then
  * the Taylor series of sin and cos on [-pi/4, pi/4] (coefficients 1/n!
- * rounded to double), with the 1 - z/2 rounding error of cos carried
+ * rounded to num), with the 1 - z/2 rounding error of cos carried
  * separately.  exp: reduction by ln 2 and the Taylor series of exp on
  * [-ln2/2, ln2/2], scaled by ldexp.  Pure IEEE operations: identical bits
- * on every IEEE-754 double machine. */
+ * on every IEEE-754 num machine. */
 
 #define PIO2_1 0x1.921fb54442d18p+0
 #define PIO2_2 0x1.1a62633145c07p-54
@@ -158,42 +169,42 @@ double cp_exp_oracle(double x)
 
 typedef struct {
 	int type, world;
-	double m, m_inv, i, i_inv;
+	num m, m_inv, i, i_inv;
 	V cog, p, v, f;
-	double a, w, t;
+	num a, w, t;
 	XF T;
 	V v_bias;
-	double w_bias;
+	num w_bias;
 } OBody;
 
 typedef struct {
 	int type, body, world;
 	uint64_t hashid;
-	double e, u;
+	num e, u;
 	V sv;
 	uint64_t group;
 	uint32_t cats, mask;
-	double r;
+	num r;
 	V c, tc;                                 /* circle */
 	V a, b, n, ta, tb, tn, a_tan, b_tan;     /* segment */
 	int count;                               /* poly */
 	const V *lv, *ln;
 	V *wv, *wn;
-	double bb[4];                            /* l, b, r, t */
-	double fat[4];                           /* velocity-fattened leaf box of the dynamic BBTree */
+	num bb[4];                            /* l, b, r, t */
+	num fat[4];                           /* velocity-fattened leaf box of the dynamic BBTree */
 	uint32_t rebuild_stamp;                  /* step of the last leaf rebuild */
 } OShape;
 
 typedef struct {
 	V r1, r2;
-	double nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
+	num nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
 	uint64_t hash;
 } OContact;
 
 typedef struct {
 	int32_t sa, sb, ba, bb;
 	uint64_t key, prio;
-	double e, u;
+	num e, u;
 	V svr, n;
 	int count;
 	OContact con[2];
@@ -204,13 +215,13 @@ typedef struct {
 
 typedef struct {
 	int type, a, b;
-	double maxForce, errorBias, maxBias;
+	num maxForce, errorBias, maxBias;
 	int collide;
-	double P[8];
+	num P[8];
 	/* solver state (chipmunk_structs.h:270-391) */
 	V r1, r2, n, bias2, jAcc2;
-	double k11, k12, k21, k22;
-	double nMass, bias, jAcc, jnAcc, target_vrn, v_coef, iSum;
+	num k11, k12, k21, k22;
+	num nMass, bias, jAcc, jnAcc, target_vrn, v_coef, iSum;
 	int colour;
 	uint64_t prio;
 } OJoint;
@@ -356,8 +367,8 @@ oworld *ow_new(const cpgpuWorldDesc *d)
 		j->a = d->cons_a[i]; j->b = d->cons_b[i];
 		j->maxForce = d->cons_max_force[i]; j->errorBias = d->cons_error_bias[i]; j->maxBias = d->cons_max_bias[i];
 		j->collide = d->cons_collide_bodies[i];
-		memcpy(j->P, d->cons_params + 8 * i, 8 * sizeof(double));
-		double j0 = d->cons_jacc ? d->cons_jacc[2 * i] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * i + 1] : 0.0;
+		for (int q = 0; q < 8; q++) j->P[q] = d->cons_params[8 * i + q];
+		num j0 = d->cons_jacc ? d->cons_jacc[2 * i] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * i + 1] : 0.0;
 		j->jAcc2 = v2(j0, j1);
 		j->jAcc = j0; j->jnAcc = j0;
 		/* colouring priority from the constraint's index within its own space */
@@ -449,11 +460,11 @@ void ow_set_next_order(oworld *w, int64_t n, const int32_t *pairs)
 /* integration: cpBodyUpdatePosition / SetTransform (cpBody.c:511-522, 347-357)
  * and cpBodyUpdateVelocity (cpBody.c:493-509)                          */
 
-static void update_position(oworld *w, OBody *b, double dt)
+static void update_position(oworld *w, OBody *b, num dt)
 {
 	b->p = vadd(b->p, vmult(vadd(b->v, b->v_bias), dt));
 	b->a = b->a + (b->w + b->w_bias) * dt;
-	double sn, cs;
+	num sn, cs;
 	if (w->libm_trig) { cs = cos(b->a); sn = sin(b->a); }
 	else cp_sincos_oracle(b->a, &sn, &cs);
 	V c = b->cog;
@@ -465,7 +476,7 @@ static void update_position(oworld *w, OBody *b, double dt)
 	b->w_bias = 0.0;
 }
 
-static void update_velocity(OBody *b, V g, double damping, double dt)
+static void update_velocity(OBody *b, V g, num damping, num dt)
 {
 	if (b->type == CPGPU_BODY_KINEMATIC) return;
 	b->v = vadd(vmult(b->v, damping), vmult(vadd(g, vmult(b->f, b->m_inv)), dt));
@@ -487,20 +498,20 @@ static void cache_shape(oworld *w, OShape *s)
 		s->bb[0] = c.x - s->r; s->bb[1] = c.y - s->r; s->bb[2] = c.x + s->r; s->bb[3] = c.y + s->r;
 	} else if (s->type == CPGPU_SHAPE_SEGMENT) {
 		s->ta = xpoint(T, s->a); s->tb = xpoint(T, s->b); s->tn = xvect(T, s->n);
-		double l, r, bt, tp;
+		num l, r, bt, tp;
 		if (s->ta.x < s->tb.x) { l = s->ta.x; r = s->tb.x; } else { l = s->tb.x; r = s->ta.x; }
 		if (s->ta.y < s->tb.y) { bt = s->ta.y; tp = s->tb.y; } else { bt = s->tb.y; tp = s->ta.y; }
-		double rad = s->r;
+		num rad = s->r;
 		s->bb[0] = l - rad; s->bb[1] = bt - rad; s->bb[2] = r + rad; s->bb[3] = tp + rad;
 	} else {
-		double l = INFINITY, r = -INFINITY, bt = INFINITY, tp = -INFINITY;
+		num l = INFINITY, r = -INFINITY, bt = INFINITY, tp = -INFINITY;
 		for (int i = 0; i < s->count; i++) {
 			V v = xpoint(T, s->lv[i]);
 			V n = xvect(T, s->ln[i]);
 			s->wv[i] = v; s->wn[i] = n;
 			l = fminc(l, v.x); r = fmaxc(r, v.x); bt = fminc(bt, v.y); tp = fmaxc(tp, v.y);
 		}
-		double rad = s->r;
+		num rad = s->r;
 		s->bb[0] = l - rad; s->bb[1] = bt - rad; s->bb[2] = r + rad; s->bb[3] = tp + rad;
 	}
 }
@@ -511,10 +522,10 @@ static void cache_shape(oworld *w, OShape *s)
  * id of every pair of that shape restarts at 0 (cpBBTree.c:211,434). */
 static void fat_bb(oworld *w, OShape *s)
 {
-	const double *bb = s->bb;
+	const num *bb = s->bb;
 	V v = vmult(w->B[s->body].v, 0.1);
-	double x = (bb[2] - bb[0]) * 0.1;
-	double y = (bb[3] - bb[1]) * 0.1;
+	num x = (bb[2] - bb[0]) * 0.1;
+	num y = (bb[3] - bb[1]) * 0.1;
 	s->fat[0] = bb[0] + fminc(-x, v.x);
 	s->fat[1] = bb[1] + fminc(-y, v.y);
 	s->fat[2] = bb[2] + fmaxc(x, v.x);
@@ -523,7 +534,7 @@ static void fat_bb(oworld *w, OShape *s)
 
 static void leaf_update(oworld *w, OShape *s)
 {
-	const double *f = s->fat, *b = s->bb;
+	const num *f = s->fat, *b = s->bb;
 	int contains = f[0] <= b[0] && f[2] >= b[2] && f[1] <= b[1] && f[3] >= b[3];
 	if (!contains) {
 		fat_bb(w, s);
@@ -535,7 +546,8 @@ void ow_cache_all_shapes(oworld *w) { for (int i = 0; i < w->ns; i++) cache_shap
 
 void ow_read_shape_bbs(oworld *w, double *bb)
 {
-	for (int i = 0; i < w->ns; i++) memcpy(bb + 4 * i, w->S[i].bb, 4 * sizeof(double));
+	for (int i = 0; i < w->ns; i++)
+		for (int q = 0; q < 4; q++) bb[4 * i + q] = NUM_VAL(w->S[i].bb[q]);
 }
 
 void ow_set_transforms(oworld *w, const double *t)
@@ -550,16 +562,16 @@ typedef struct { V p; uint32_t index; } SupportPt;                  /* :81-91 */
 typedef struct { V a, b, ab; uint32_t id; } MinkPt;                  /* :117-132 */
 typedef struct { const OShape *s1, *s2; } Ctx;                       /* :134-137 */
 typedef struct { V p; uint64_t hash; } EdgePt;                       /* :147-151 */
-typedef struct { EdgePt a, b; double r; V n; } Edge;                 /* :154-158 */
-typedef struct { V a, b, n; double d; uint32_t id; } ClosestPts;     /* :211-221 */
+typedef struct { EdgePt a, b; num r; V n; } Edge;                 /* :154-158 */
+typedef struct { V a, b, n; num d; uint32_t id; } ClosestPts;     /* :211-221 */
 typedef struct { int count; uint32_t id; V n; V p1[2], p2[2]; uint64_t hash[2]; } CollInfo;
 
 static inline int poly_support_index(const OShape *s, V n)            /* :62-77 */
 {
-	double max = -INFINITY;
+	num max = -INFINITY;
 	int index = 0;
 	for (int i = 0; i < s->count; i++) {
-		double d = vdot(s->wv[i], n);
+		num d = vdot(s->wv[i], n);
 		if (d > max) { max = d; index = i; }
 	}
 	return index;
@@ -604,40 +616,40 @@ static inline int check_axis(V v0, V v1, V p, V n)                    /* cpRobus
 	return vdot(p, n) <= fmaxc(vdot(v0, n), vdot(v1, n));
 }
 
-static inline double closest_t(V a, V b)                              /* :193-199 */
+static inline num closest_t(V a, V b)                              /* :193-199 */
 {
 	V delta = vsub(b, a);
 	return -fclampc(vdot(delta, vadd(a, b)) / (vlengthsq(delta) + DBL_MIN), -1.0, 1.0);
 }
 
-static inline V lerp_t(V a, V b, double t)                            /* :201-207 */
+static inline V lerp_t(V a, V b, num t)                            /* :201-207 */
 {
-	double ht = 0.5 * t;
+	num ht = 0.5 * t;
 	return vadd(vmult(a, 0.5 - ht), vmult(b, 0.5 + ht));
 }
 
 static ClosestPts closest_points_new(MinkPt v0, MinkPt v1)            /* :223-258 */
 {
-	double t = closest_t(v0.ab, v1.ab);
+	num t = closest_t(v0.ab, v1.ab);
 	V p = lerp_t(v0.ab, v1.ab, t);
 	V pa = lerp_t(v0.a, v1.a, t);
 	V pb = lerp_t(v0.b, v1.b, t);
 	uint32_t id = ((v0.id & 0xFFFF) << 16) | (v1.id & 0xFFFF);
 	V delta = vsub(v1.ab, v0.ab);
 	V n = vnormalize(vrperp(delta));
-	double d = vdot(n, p);
+	num d = vdot(n, p);
 	ClosestPts r;
 	r.a = pa; r.b = pb; r.id = id;
 	if (d <= 0.0 || (-1.0 < t && t < 1.0)) { r.n = n; r.d = d; }
 	else {
-		double d2 = vlength(p);
+		num d2 = vlength(p);
 		r.n = vmult(p, 1.0 / (d2 + DBL_MIN));
 		r.d = d2;
 	}
 	return r;
 }
 
-static inline double closest_dist(V v0, V v1)                          /* :262-266 */
+static inline num closest_dist(V v0, V v1)                          /* :262-266 */
 {
 	return vlengthsq(lerp_t(v0, v1, closest_t(v0, v1)));
 }
@@ -654,9 +666,9 @@ static ClosestPts epa(const Ctx *ctx, MinkPt v0, MinkPt v1, MinkPt v2)  /* :270-
 	hull[0] = v0; hull[1] = v1; hull[2] = v2;
 	for (int iteration = 1;; iteration++) {
 		int mini = 0;
-		double minDist = INFINITY;
+		num minDist = INFINITY;
 		for (int j = 0, i = count - 1; j < count; i = j, j++) {
-			double d = closest_dist(hull[i].ab, hull[j].ab);
+			num d = closest_dist(hull[i].ab, hull[j].ab);
 			if (d < minDist) { minDist = d; mini = i; }
 		}
 		MinkPt e0 = hull[mini];
@@ -690,7 +702,7 @@ static ClosestPts gjk_recurse(const Ctx *ctx, MinkPt v0, MinkPt v1)    /* :348-3
 			MinkPt tmp = v0; v0 = v1; v1 = tmp;
 			continue;
 		}
-		double t = closest_t(v0.ab, v1.ab);
+		num t = closest_t(v0.ab, v1.ab);
 		V n = (-1.0 < t && t < 1.0) ? vperp(vsub(v1.ab, v0.ab)) : vneg(lerp_t(v0.ab, v1.ab, t));
 		MinkPt p = support(ctx, n);
 		if (check_point_greater(p.ab, v0.ab, v2(0.0, 0.0)) && check_point_greater(v1.ab, p.ab, v2(0.0, 0.0))) {
@@ -715,7 +727,7 @@ static SupportPt shape_point(const OShape *s, int i)                   /* :395-4
 	return r;
 }
 
-static inline V bb_center(const double *bb) { return vlerp(v2(bb[0], bb[1]), v2(bb[2], bb[3]), 0.5); }
+static inline V bb_center(const num *bb) { return vlerp(v2(bb[0], bb[1]), v2(bb[2], bb[3]), 0.5); }
 
 static ClosestPts gjk(const Ctx *ctx, uint32_t *id)                    /* :416-472 */
 {
@@ -743,25 +755,25 @@ static inline void push_contact(CollInfo *info, V p1, V p2, uint64_t hash)  /* :
 
 static void contact_points(Edge e1, Edge e2, ClosestPts points, CollInfo *info)  /* :477-518 */
 {
-	double mindist = e1.r + e2.r;
+	num mindist = e1.r + e2.r;
 	if (points.d <= mindist) {
 		V n = info->n = points.n;
-		double d_e1_a = vcross(e1.a.p, n);
-		double d_e1_b = vcross(e1.b.p, n);
-		double d_e2_a = vcross(e2.a.p, n);
-		double d_e2_b = vcross(e2.b.p, n);
-		double e1_denom = 1.0 / (d_e1_b - d_e1_a + DBL_MIN);
-		double e2_denom = 1.0 / (d_e2_b - d_e2_a + DBL_MIN);
+		num d_e1_a = vcross(e1.a.p, n);
+		num d_e1_b = vcross(e1.b.p, n);
+		num d_e2_a = vcross(e2.a.p, n);
+		num d_e2_b = vcross(e2.b.p, n);
+		num e1_denom = 1.0 / (d_e1_b - d_e1_a + DBL_MIN);
+		num e2_denom = 1.0 / (d_e2_b - d_e2_a + DBL_MIN);
 		{
 			V p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_b - d_e1_a) * e1_denom)));
 			V p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_a - d_e2_a) * e2_denom)));
-			double dist = vdot(vsub(p2, p1), n);
+			num dist = vdot(vsub(p2, p1), n);
 			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.a.hash, e2.b.hash));
 		}
 		{
 			V p1 = vadd(vmult(n, e1.r), vlerp(e1.a.p, e1.b.p, fclamp01((d_e2_a - d_e1_a) * e1_denom)));
 			V p2 = vadd(vmult(n, -e2.r), vlerp(e2.a.p, e2.b.p, fclamp01((d_e1_b - d_e2_a) * e2_denom)));
-			double dist = vdot(vsub(p2, p1), n);
+			num dist = vdot(vsub(p2, p1), n);
 			if (dist <= 0.0) push_contact(info, p1, p2, hash_pair(e1.b.hash, e2.a.hash));
 		}
 	}
@@ -809,11 +821,11 @@ static inline V body_rot(oworld *w, const OShape *s) { XF T = w->B[s->body].T; r
 
 static void circle_circle(const OShape *c1, const OShape *c2, CollInfo *info)  /* :525-537 */
 {
-	double mindist = c1->r + c2->r;
+	num mindist = c1->r + c2->r;
 	V delta = vsub(c2->tc, c1->tc);
-	double distsq = vlengthsq(delta);
+	num distsq = vlengthsq(delta);
 	if (distsq < mindist * mindist) {
-		double dist = sqrt(distsq);
+		num dist = sqrt(distsq);
 		V n = info->n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : v2(1.0, 0.0));
 		push_contact(info, vadd(c1->tc, vmult(n, c1->r)), vadd(c2->tc, vmult(n, -c2->r)), 0);
 	}
@@ -823,13 +835,13 @@ static void circle_segment(oworld *w, const OShape *circle, const OShape *seg, C
 {
 	V seg_a = seg->ta, seg_b = seg->tb, center = circle->tc;
 	V seg_delta = vsub(seg_b, seg_a);
-	double closest_tt = fclamp01(vdot(seg_delta, vsub(center, seg_a)) / vlengthsq(seg_delta));
+	num closest_tt = fclamp01(vdot(seg_delta, vsub(center, seg_a)) / vlengthsq(seg_delta));
 	V closest = vadd(seg_a, vmult(seg_delta, closest_tt));
-	double mindist = circle->r + seg->r;
+	num mindist = circle->r + seg->r;
 	V delta = vsub(closest, center);
-	double distsq = vlengthsq(delta);
+	num distsq = vlengthsq(delta);
 	if (distsq < mindist * mindist) {
-		double dist = sqrt(distsq);
+		num dist = sqrt(distsq);
 		V n = info->n = (dist != 0.0 ? vmult(delta, 1.0 / dist) : seg->tn);
 		V rot = body_rot(w, seg);
 		if ((closest_tt != 0.0 || vdot(n, vrotate(seg->a_tan, rot)) >= 0.0) &&
@@ -913,10 +925,10 @@ int ow_collide(oworld *w, int32_t a, int32_t b, uint32_t id, int32_t *ab_out, ui
 	collide(w, a, b, id, &info);
 	ab_out[0] = a; ab_out[1] = b;
 	*id_out = info.id;
-	n_out[0] = info.n.x; n_out[1] = info.n.y;
+	n_out[0] = NUM_VAL(info.n.x); n_out[1] = NUM_VAL(info.n.y);
 	for (int i = 0; i < info.count; i++) {
-		points_out[4 * i + 0] = info.p1[i].x; points_out[4 * i + 1] = info.p1[i].y;
-		points_out[4 * i + 2] = info.p2[i].x; points_out[4 * i + 3] = info.p2[i].y;
+		points_out[4 * i + 0] = NUM_VAL(info.p1[i].x); points_out[4 * i + 1] = NUM_VAL(info.p1[i].y);
+		points_out[4 * i + 2] = NUM_VAL(info.p2[i].x); points_out[4 * i + 3] = NUM_VAL(info.p2[i].y);
 		hash_out[i] = info.hash[i];
 	}
 	return info.count;
@@ -934,13 +946,13 @@ int ow_shapes_collide(oworld *w, int32_t a, int32_t b, double *n_out, double *po
 	CollInfo info;
 	collide(w, x, y, 0, &info);
 	const V n = swapped ? vneg(info.n) : info.n;
-	n_out[0] = n.x; n_out[1] = n.y;
+	n_out[0] = NUM_VAL(n.x); n_out[1] = NUM_VAL(n.y);
 	for (int i = 0; i < info.count; i++) {
 		const V p1 = info.p1[i], p2 = info.p2[i];
 		const V pa = swapped ? p2 : p1, pb = swapped ? p1 : p2;
-		points_out[5 * i + 0] = pa.x; points_out[5 * i + 1] = pa.y;
-		points_out[5 * i + 2] = pb.x; points_out[5 * i + 3] = pb.y;
-		points_out[5 * i + 4] = vdot(vsub(p2, p1), n);
+		points_out[5 * i + 0] = NUM_VAL(pa.x); points_out[5 * i + 1] = NUM_VAL(pa.y);
+		points_out[5 * i + 2] = NUM_VAL(pb.x); points_out[5 * i + 3] = NUM_VAL(pb.y);
+		points_out[5 * i + 4] = NUM_VAL(vdot(vsub(p2, p1), n));
 	}
 	return info.count;
 }
@@ -951,7 +963,7 @@ int ow_shapes_collide(oworld *w, int32_t a, int32_t b, double *n_out, double *po
  * over dynamic x dynamic and dynamic x static shapes (cpBBTree.c:638-654,
  * cpSpatialIndex.c:61-68), by sort-and-sweep on the left edge.          */
 
-static inline int bb_intersects(const double *a, const double *b)   /* cpBB.h:58-61 */
+static inline int bb_intersects(const num *a, const num *b)   /* cpBB.h:58-61 */
 {
 	return a[0] <= b[2] && b[0] <= a[2] && a[1] <= b[3] && b[1] <= a[3];
 }
@@ -988,11 +1000,11 @@ static int pair_ok(oworld *w, const OWorldPart *P, int i, int j)
 	return 1;
 }
 
-static const double *g_sort_bbs;
+static const num *g_sort_bbs;
 static int cmp_by_left(const void *x, const void *y)
 {
 	int i = *(const int *)x, j = *(const int *)y;
-	double a = g_sort_bbs[4 * i], b = g_sort_bbs[4 * j];
+	num a = g_sort_bbs[4 * i], b = g_sort_bbs[4 * j];
 	if (a < b) return -1;
 	if (a > b) return 1;
 	return i < j ? -1 : (i > j ? 1 : 0);
@@ -1011,16 +1023,16 @@ static int cmp_pair(const void *x, const void *y)
 static IPair *world_pairs(oworld *w, OWorldPart *P, int64_t *n_out)
 {
 	int n = P->s1 - P->s0;
-	double *bbs = (double *)malloc((size_t)(4 * (n ? n : 1)) * sizeof(double));
+	num *bbs = (num *)malloc((size_t)(4 * (n ? n : 1)) * sizeof(num));
 	int *idx = (int *)malloc((size_t)(n ? n : 1) * sizeof(int));
-	for (int k = 0; k < n; k++) { memcpy(bbs + 4 * k, w->S[P->s0 + k].bb, 4 * sizeof(double)); idx[k] = k; }
+	for (int k = 0; k < n; k++) { memcpy(bbs + 4 * k, w->S[P->s0 + k].bb, 4 * sizeof(num)); idx[k] = k; }
 	g_sort_bbs = bbs;
 	qsort(idx, (size_t)n, sizeof(int), cmp_by_left);
 	int64_t cap = 1024, np = 0;
 	IPair *pairs = (IPair *)malloc((size_t)cap * sizeof(IPair));
 	for (int x = 0; x < n; x++) {
 		int i = idx[x];
-		double right = bbs[4 * i + 2];
+		num right = bbs[4 * i + 2];
 		for (int y = x + 1; y < n; y++) {
 			int j = idx[y];
 			if (!(bbs[4 * j] <= right)) break;
@@ -1235,7 +1247,7 @@ static int cmp_jsolve(const void *x, const void *y)
 #define UNIT_ODD (UNIT_EVEN + GRID_PARTS / 2)
 #define UNIT_WIDE (UNIT_ODD + (GRID_PARTS - 1) / 2)
 
-/* a double's bits as an unsigned key of the same order (x + 0.0 turns -0.0 into +0.0) */
+/* a num's bits as an unsigned key of the same order (x + 0.0 turns -0.0 into +0.0) */
 static uint64_t x_key(double x)
 {
 	double y = x + 0.0;
@@ -1276,7 +1288,7 @@ static int grid_units(oworld *w, OWorldPart *P)
 		part[i] = -1;
 		if (!DYN(P->b0 + i)) continue;
 		part[i] = 0;
-		uint64_t key = x_key(w->B[P->b0 + i].p.x);
+		uint64_t key = x_key(NUM_VAL(w->B[P->b0 + i].p.x));
 		if (key < lo_k) lo_k = key;
 		if (key > hi_k) hi_k = key;
 		n_grid++;
@@ -1284,14 +1296,14 @@ static int grid_units(oworld *w, OWorldPart *P)
 	double lo = key_x(lo_k), hi = key_x(hi_k);
 	memset(hist, 0, sizeof hist);
 	for (int i = 0; i < nbw; i++)
-		if (part[i] >= 0) hist[part_bin(w->B[P->b0 + i].p.x, lo, hi)]++;
+		if (part[i] >= 0) hist[part_bin(NUM_VAL(w->B[P->b0 + i].p.x), lo, hi)]++;
 	for (int q = 0, before = 0; q < PART_BINS; q++) {
 		int cnt = hist[q];
 		hist[q] = n_grid > 0 ? (int)((int64_t)before * GRID_PARTS / n_grid) : 0;
 		before += cnt;
 	}
 	for (int i = 0; i < nbw; i++)
-		if (part[i] >= 0) part[i] = hist[part_bin(w->B[P->b0 + i].p.x, lo, hi)];
+		if (part[i] >= 0) part[i] = hist[part_bin(NUM_VAL(w->B[P->b0 + i].p.x), lo, hi)];
 	for (int64_t k = 0; k < P->nactive; k++) {
 		OArb *arb = &P->arbs[P->active[k]];
 		int pa = DYN(arb->ba) ? part[arb->ba - P->b0] : -1, pb = DYN(arb->bb) ? part[arb->bb - P->b0] : -1;
@@ -1335,18 +1347,18 @@ static inline void apply_bias_impulse(OBody *body, V j, V r)           /* :199-2
 	body->w_bias += body->i_inv * vcross(r, j);
 }
 
-static inline double k_scalar_body(const OBody *b, V r, V n)           /* :212-217 */
+static inline num k_scalar_body(const OBody *b, V r, V n)           /* :212-217 */
 {
-	double rcn = vcross(r, n);
+	num rcn = vcross(r, n);
 	return b->m_inv + b->i_inv * rcn * rcn;
 }
 
-static inline double k_scalar(const OBody *a, const OBody *b, V r1, V r2, V n)  /* :219-226 */
+static inline num k_scalar(const OBody *a, const OBody *b, V r1, V r2, V n)  /* :219-226 */
 {
 	return k_scalar_body(a, r1, n) + k_scalar_body(b, r2, n);
 }
 
-static void arb_prestep(oworld *w, OArb *arb, double dt, double slop, double bias)
+static void arb_prestep(oworld *w, OArb *arb, num dt, num slop, num bias)
 {
 	const OBody *a = &w->B[arb->ba], *b = &w->B[arb->bb];
 	V n = arb->n;
@@ -1355,14 +1367,14 @@ static void arb_prestep(oworld *w, OArb *arb, double dt, double slop, double bia
 		OContact *con = &arb->con[i];
 		con->nMass = 1.0 / k_scalar(a, b, con->r1, con->r2, n);
 		con->tMass = 1.0 / k_scalar(a, b, con->r1, con->r2, vperp(n));
-		double dist = vdot(vadd(vsub(con->r2, con->r1), body_delta), n);
+		num dist = vdot(vadd(vsub(con->r2, con->r1), body_delta), n);
 		con->bias = -bias * fminc(0.0, dist + slop) / dt;
 		con->jBias = 0.0;
 		con->bounce = vdot(rel_vel(a, b, con->r1, con->r2), n) * arb->e;
 	}
 }
 
-static void arb_apply_cached(oworld *w, OArb *arb, double dt_coef)
+static void arb_apply_cached(oworld *w, OArb *arb, num dt_coef)
 {
 	if (arb->state == CPGPU_ARB_FIRST_COLLISION) return;
 	OBody *a = &w->B[arb->ba], *b = &w->B[arb->bb];
@@ -1379,26 +1391,26 @@ static void arb_apply_impulse(oworld *w, OArb *arb)
 	OBody *a = &w->B[arb->ba], *b = &w->B[arb->bb];
 	V n = arb->n;
 	V surface_vr = arb->svr;
-	double friction = arb->u;
+	num friction = arb->u;
 	for (int i = 0; i < arb->count; i++) {
 		OContact *con = &arb->con[i];
-		double nMass = con->nMass;
+		num nMass = con->nMass;
 		V r1 = con->r1, r2 = con->r2;
 		V vb1 = vadd(a->v_bias, vmult(vperp(r1), a->w_bias));
 		V vb2 = vadd(b->v_bias, vmult(vperp(r2), b->w_bias));
 		V vr = vadd(rel_vel(a, b, r1, r2), surface_vr);
-		double vbn = vdot(vsub(vb2, vb1), n);
-		double vrn = vdot(vr, n);
-		double vrt = vdot(vr, vperp(n));
-		double jbn = (con->bias - vbn) * nMass;
-		double jbnOld = con->jBias;
+		num vbn = vdot(vsub(vb2, vb1), n);
+		num vrn = vdot(vr, n);
+		num vrt = vdot(vr, vperp(n));
+		num jbn = (con->bias - vbn) * nMass;
+		num jbnOld = con->jBias;
 		con->jBias = fmaxc(jbnOld + jbn, 0.0);
-		double jn = -(con->bounce + vrn) * nMass;
-		double jnOld = con->jnAcc;
+		num jn = -(con->bounce + vrn) * nMass;
+		num jnOld = con->jnAcc;
 		con->jnAcc = fmaxc(jnOld + jn, 0.0);
-		double jtMax = friction * con->jnAcc;
-		double jt = -vrt * con->tMass;
-		double jtOld = con->jtAcc;
+		num jtMax = friction * con->jnAcc;
+		num jt = -vrt * con->tMass;
+		num jtOld = con->jtAcc;
 		con->jtAcc = fclampc(jtOld + jt, -jtMax, jtMax);
 		V jb = vmult(n, con->jBias - jbnOld);
 		apply_bias_impulse(a, vneg(jb), r1);
@@ -1413,9 +1425,9 @@ static void arb_apply_impulse(oworld *w, OArb *arb)
  * chipmunk_private.h:264-268 (libm pow, evaluated on the host);
  * k_tensor chipmunk_private.h:228-262                                   */
 
-static double bias_coef_host(double errorBias, double dt) { return 1.0 - pow(errorBias, dt); }
+static num bias_coef_host(num errorBias, num dt) { return 1.0 - pow(NUM_VAL(errorBias), NUM_VAL(dt)); }
 
-static void joint_prestep(oworld *w, OJoint *j, double dt)
+static void joint_prestep(oworld *w, OJoint *j, num dt)
 {
 	OBody *a = &w->B[j->a], *b = &w->B[j->b];
 	V anchorA = v2(j->P[0], j->P[1]), anchorB = v2(j->P[2], j->P[3]);
@@ -1423,60 +1435,60 @@ static void joint_prestep(oworld *w, OJoint *j, double dt)
 		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
 		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
 		/* k_tensor */
-		double m_sum = a->m_inv + b->m_inv;
-		double k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
-		double a_i_inv = a->i_inv;
-		double r1xsq = j->r1.x * j->r1.x * a_i_inv;
-		double r1ysq = j->r1.y * j->r1.y * a_i_inv;
-		double r1nxy = -j->r1.x * j->r1.y * a_i_inv;
+		num m_sum = a->m_inv + b->m_inv;
+		num k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
+		num a_i_inv = a->i_inv;
+		num r1xsq = j->r1.x * j->r1.x * a_i_inv;
+		num r1ysq = j->r1.y * j->r1.y * a_i_inv;
+		num r1nxy = -j->r1.x * j->r1.y * a_i_inv;
 		k11 += r1ysq; k12 += r1nxy; k21 += r1nxy; k22 += r1xsq;
-		double b_i_inv = b->i_inv;
-		double r2xsq = j->r2.x * j->r2.x * b_i_inv;
-		double r2ysq = j->r2.y * j->r2.y * b_i_inv;
-		double r2nxy = -j->r2.x * j->r2.y * b_i_inv;
+		num b_i_inv = b->i_inv;
+		num r2xsq = j->r2.x * j->r2.x * b_i_inv;
+		num r2ysq = j->r2.y * j->r2.y * b_i_inv;
+		num r2nxy = -j->r2.x * j->r2.y * b_i_inv;
 		k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
-		double det = k11 * k22 - k12 * k21;
-		double det_inv = 1.0 / det;
+		num det = k11 * k22 - k12 * k21;
+		num det_inv = 1.0 / det;
 		j->k11 = k22 * det_inv; j->k12 = -k12 * det_inv;
 		j->k21 = -k21 * det_inv; j->k22 = k11 * det_inv;
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
 		j->bias2 = vclamp(vmult(delta, -bias_coef_host(j->errorBias, dt) / dt), j->maxBias);
 	} else if (j->type == CPGPU_JOINT_DAMPED_SPRING) {
-		double restLength = j->P[4], stiffness = j->P[5], damping = j->P[6];
+		num restLength = j->P[4], stiffness = j->P[5], damping = j->P[6];
 		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
 		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
-		double dist = vlength(delta);
+		num dist = vlength(delta);
 		j->n = vmult(delta, 1.0 / (dist != 0.0 ? dist : INFINITY));
-		double k = k_scalar(a, b, j->r1, j->r2, j->n);
+		num k = k_scalar(a, b, j->r1, j->r2, j->n);
 		j->nMass = 1.0 / k;
 		j->target_vrn = 0.0;
 		j->v_coef = 1.0 - (w->libm_trig ? exp(-damping * dt * k) : cp_exp_oracle(-damping * dt * k));
-		double f_spring = (restLength - dist) * stiffness;
-		double j_spring = j->jAcc = f_spring * dt;
+		num f_spring = (restLength - dist) * stiffness;
+		num j_spring = j->jAcc = f_spring * dt;
 		apply_impulses(a, b, j->r1, j->r2, vmult(j->n, j_spring));
 	} else if (j->type == CPGPU_JOINT_SLIDE) {
-		double mn = j->P[4], mx = j->P[5];
+		num mn = j->P[4], mx = j->P[5];
 		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
 		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
-		double dist = vlength(delta);
-		double pdist = 0.0;
+		num dist = vlength(delta);
+		num pdist = 0.0;
 		if (dist > mx) { pdist = dist - mx; j->n = vnormalize(delta); }
 		else if (dist < mn) { pdist = mn - dist; j->n = vneg(vnormalize(delta)); }
 		else { j->n = v2(0.0, 0.0); j->jnAcc = 0.0; }
 		j->nMass = 1.0 / k_scalar(a, b, j->r1, j->r2, j->n);
-		double maxBias = j->maxBias;
+		num maxBias = j->maxBias;
 		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * pdist / dt, -maxBias, maxBias);
 	} else {
-		double phase = j->P[0], ratio = j->P[1], ratio_inv = j->P[2];
+		num phase = j->P[0], ratio = j->P[1], ratio_inv = j->P[2];
 		j->iSum = 1.0 / (a->i_inv * ratio_inv + ratio * b->i_inv);
-		double maxBias = j->maxBias;
+		num maxBias = j->maxBias;
 		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * (b->a * ratio - a->a - phase) / dt, -maxBias, maxBias);
 	}
 }
 
-static void joint_apply_cached(oworld *w, OJoint *j, double dt_coef)
+static void joint_apply_cached(oworld *w, OJoint *j, num dt_coef)
 {
 	OBody *a = &w->B[j->a], *b = &w->B[j->b];
 	if (j->type == CPGPU_JOINT_PIVOT) {
@@ -1485,14 +1497,14 @@ static void joint_apply_cached(oworld *w, OJoint *j, double dt_coef)
 		V jj = vmult(j->n, j->jnAcc * dt_coef);
 		apply_impulses(a, b, j->r1, j->r2, jj);
 	} else if (j->type == CPGPU_JOINT_GEAR) {
-		double ratio_inv = j->P[2];
-		double jj = j->jAcc * dt_coef;
+		num ratio_inv = j->P[2];
+		num jj = j->jAcc * dt_coef;
 		a->w -= jj * a->i_inv * ratio_inv;
 		b->w += jj * b->i_inv;
 	}
 }
 
-static void joint_apply_impulse(oworld *w, OJoint *j, double dt)
+static void joint_apply_impulse(oworld *w, OJoint *j, num dt)
 {
 	OBody *a = &w->B[j->a], *b = &w->B[j->b];
 	if (j->type == CPGPU_JOINT_PIVOT) {
@@ -1506,28 +1518,28 @@ static void joint_apply_impulse(oworld *w, OJoint *j, double dt)
 		apply_impulses(a, b, j->r1, j->r2, jj);
 	} else if (j->type == CPGPU_JOINT_DAMPED_SPRING) {
 		V n = j->n;
-		double vrn = vdot(rel_vel(a, b, j->r1, j->r2), n);
-		double v_damp = (j->target_vrn - vrn) * j->v_coef;
+		num vrn = vdot(rel_vel(a, b, j->r1, j->r2), n);
+		num v_damp = (j->target_vrn - vrn) * j->v_coef;
 		j->target_vrn = vrn + v_damp;
-		double j_damp = v_damp * j->nMass;
+		num j_damp = v_damp * j->nMass;
 		j->jAcc += j_damp;
 		apply_impulses(a, b, j->r1, j->r2, vmult(j->n, j_damp));
 	} else if (j->type == CPGPU_JOINT_SLIDE) {
 		if (veql(j->n, v2(0.0, 0.0))) return;
 		V n = j->n;
 		V vr = rel_vel(a, b, j->r1, j->r2);
-		double vrn = vdot(vr, n);
-		double jn = (j->bias - vrn) * j->nMass;
-		double jnOld = j->jnAcc;
+		num vrn = vdot(vr, n);
+		num jn = (j->bias - vrn) * j->nMass;
+		num jnOld = j->jnAcc;
 		j->jnAcc = fclampc(jnOld + jn, -j->maxForce * dt, 0.0);
 		jn = j->jnAcc - jnOld;
 		apply_impulses(a, b, j->r1, j->r2, vmult(n, jn));
 	} else {
-		double ratio = j->P[1], ratio_inv = j->P[2];
-		double wr = b->w * ratio - a->w;
-		double jMax = j->maxForce * dt;
-		double jj = (j->bias - wr) * j->iSum;
-		double jOld = j->jAcc;
+		num ratio = j->P[1], ratio_inv = j->P[2];
+		num wr = b->w * ratio - a->w;
+		num jMax = j->maxForce * dt;
+		num jj = (j->bias - wr) * j->iSum;
+		num jOld = j->jAcc;
 		j->jAcc = fclampc(jOld + jj, -jMax, jMax);
 		jj = j->jAcc - jOld;
 		a->w -= jj * a->i_inv * ratio_inv;
@@ -1538,12 +1550,12 @@ static void joint_apply_impulse(oworld *w, OJoint *j, double dt)
 /* ------------------------------------------------------------------ */
 /* the step: cpSpaceStep (cpSpaceStep.c:335-445)                       */
 
-static void step_world(oworld *w, OWorldPart *P, double dt, const int32_t *order, int64_t order_n)
+static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, int64_t order_n)
 {
 	cpgpuSpaceParams *sp = &P->sp;
 	sp->stamp++;
-	double prev_dt = sp->curr_dt;
-	sp->curr_dt = dt;
+	num prev_dt = sp->curr_dt;
+	sp->curr_dt = NUM_VAL(dt);
 
 	/* :351-360 reset the arbiters solved last step */
 	for (int64_t k = 0; k < P->nactive; k++) P->arbs[P->active[k]].state = CPGPU_ARB_NORMAL;
@@ -1756,18 +1768,18 @@ static void step_world(oworld *w, OWorldPart *P, double dt, const int32_t *order
 	}
 
 	/* :383-387 prestep the arbiters */
-	double slop = sp->collision_slop;
-	double biasCoef = 1.0 - pow(sp->collision_bias, dt);
+	num slop = sp->collision_slop;
+	num biasCoef = 1.0 - pow(sp->collision_bias, NUM_VAL(dt));
 	for (int64_t k = 0; k < P->nactive; k++) arb_prestep(w, &P->arbs[P->active[k]], dt, slop, biasCoef);
 	/* :389-396 prestep the constraints (in solve order) */
 	int64_t nj = P->c1 - P->c0;
 	for (int64_t k = 0; k < nj; k++) joint_prestep(w, &w->J[P->jorder[k]], dt);
 	/* :399-404 integrate velocities */
-	double damping = pow(sp->damping, dt);
+	num damping = pow(sp->damping, NUM_VAL(dt));
 	V g = v2(sp->gravity_x, sp->gravity_y);
 	for (int b = P->b0; b < P->b1; b++) if (w->B[b].type != CPGPU_BODY_STATIC) update_velocity(&w->B[b], g, damping, dt);
 	/* :407-415 cached impulses */
-	double dt_coef = (prev_dt == 0.0 ? 0.0 : dt / prev_dt);
+	num dt_coef = (prev_dt == 0.0 ? 0.0 : dt / prev_dt);
 	for (int64_t k = 0; k < P->nactive; k++) arb_apply_cached(w, &P->arbs[P->active[k]], dt_coef);
 	for (int64_t k = 0; k < nj; k++) joint_apply_cached(w, &w->J[P->jorder[k]], dt_coef);
 	/* :418-427 the impulse solver */
@@ -1796,15 +1808,16 @@ int ow_read_bodies(oworld *w, double *p, double *v, double *a, double *wv,
 {
 	for (int i = 0; i < w->nb; i++) {
 		OBody *b = &w->B[i];
-		if (p) { p[2 * i] = b->p.x; p[2 * i + 1] = b->p.y; }
-		if (v) { v[2 * i] = b->v.x; v[2 * i + 1] = b->v.y; }
-		if (a) a[i] = b->a;
-		if (wv) wv[i] = b->w;
-		if (v_bias) { v_bias[2 * i] = b->v_bias.x; v_bias[2 * i + 1] = b->v_bias.y; }
-		if (w_bias) w_bias[i] = b->w_bias;
+		if (p) { p[2 * i] = NUM_VAL(b->p.x); p[2 * i + 1] = NUM_VAL(b->p.y); }
+		if (v) { v[2 * i] = NUM_VAL(b->v.x); v[2 * i + 1] = NUM_VAL(b->v.y); }
+		if (a) a[i] = NUM_VAL(b->a);
+		if (wv) wv[i] = NUM_VAL(b->w);
+		if (v_bias) { v_bias[2 * i] = NUM_VAL(b->v_bias.x); v_bias[2 * i + 1] = NUM_VAL(b->v_bias.y); }
+		if (w_bias) w_bias[i] = NUM_VAL(b->w_bias);
 		if (transform) {
 			double *t = transform + 6 * i;
-			t[0] = b->T.a; t[1] = b->T.b; t[2] = b->T.c; t[3] = b->T.d; t[4] = b->T.tx; t[5] = b->T.ty;
+			t[0] = NUM_VAL(b->T.a); t[1] = NUM_VAL(b->T.b); t[2] = NUM_VAL(b->T.c);
+			t[3] = NUM_VAL(b->T.d); t[4] = NUM_VAL(b->T.tx); t[5] = NUM_VAL(b->T.ty);
 		}
 	}
 	return 0;
@@ -1953,15 +1966,15 @@ int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colou
 			if (colour) colour[n] = arb->colour;
 			if (state) state[n] = arb->state;
 			if (count) count[n] = arb->count;
-			if (normal) { normal[2 * n] = arb->n.x; normal[2 * n + 1] = arb->n.y; }
+			if (normal) { normal[2 * n] = NUM_VAL(arb->n.x); normal[2 * n + 1] = NUM_VAL(arb->n.y); }
 			for (int i = 0; i < 2; i++) {
 				OContact *c = &arb->con[i];
 				int valid = i < arb->count;
 				if (contact) {
 					double *o = contact + 24 * n + 12 * i;
-					double vals[12] = {c->r1.x, c->r1.y, c->r2.x, c->r2.y, c->nMass, c->tMass, c->bounce,
+					num vals[12] = {c->r1.x, c->r1.y, c->r2.x, c->r2.y, c->nMass, c->tMass, c->bounce,
 					                   c->jnAcc, c->jtAcc, c->jBias, c->bias, 0.0};
-					for (int q = 0; q < 12; q++) o[q] = valid ? vals[q] : 0.0;
+					for (int q = 0; q < 12; q++) o[q] = valid ? NUM_VAL(vals[q]) : 0.0;
 				}
 				if (hash) hash[2 * n + i] = valid ? c->hash : 0;
 			}
@@ -1987,9 +2000,9 @@ int ow_read_constraints(oworld *w, double *jacc, int32_t *colour)
 	for (int i = 0; i < w->nc; i++) {
 		OJoint *j = &w->J[i];
 		if (jacc) {
-			if (j->type == CPGPU_JOINT_PIVOT) { jacc[2 * i] = j->jAcc2.x; jacc[2 * i + 1] = j->jAcc2.y; }
-			else if (j->type == CPGPU_JOINT_SLIDE) { jacc[2 * i] = j->jnAcc; jacc[2 * i + 1] = 0.0; }
-			else { jacc[2 * i] = j->jAcc; jacc[2 * i + 1] = 0.0; }
+			if (j->type == CPGPU_JOINT_PIVOT) { jacc[2 * i] = NUM_VAL(j->jAcc2.x); jacc[2 * i + 1] = NUM_VAL(j->jAcc2.y); }
+			else if (j->type == CPGPU_JOINT_SLIDE) { jacc[2 * i] = NUM_VAL(j->jnAcc); jacc[2 * i + 1] = 0.0; }
+			else { jacc[2 * i] = NUM_VAL(j->jAcc); jacc[2 * i + 1] = 0.0; }
 		}
 		if (colour) colour[i] = j->colour;
 	}

@@ -14,6 +14,7 @@ from chipmunk2d_amd.cabi import PackedWorld, SpaceParams, Stats, WorldDesc, ptr
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 GPUSPACE_SO = os.path.join(ROOT, "oracle", "_ref", "libcpgpuspace.so")
+DUAL_SO = os.path.join(ROOT, "oracle", "libdualstep.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref.so")
 REF_RELEASE_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref_release.so")
 
@@ -467,3 +468,56 @@ def without(pw, shapes=(), bodies=()):
     out["world_shape_start"] = np.array([0, len(keep_s)], np.int32)
     out["world_constraint_start"] = a["world_constraint_start"].copy()
     return PackedWorld(out), keep_s, keep_b
+
+
+_dual = None
+
+
+def dual_lib():
+    global _dual
+    if _dual is None:
+        lib = ctypes.CDLL(DUAL_SO)
+        lib.dual_new.restype = _vp
+        lib.dual_new.argtypes = [ctypes.POINTER(WorldDesc)]
+        lib.dual_free.argtypes = [_vp]
+        lib.dual_step.argtypes = [_vp, _d]
+        lib.dual_set_libm_trig.argtypes = [_vp, ctypes.c_int]
+        lib.dual_seed.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        lib.dual_read_bodies.argtypes = [_vp] + [_dp] * 8
+        _dual = lib
+    return _dual
+
+
+class DualWorld:
+    """The oracle's step on dual numbers (oracle/libdualstep.so): the rollout
+    and its directional derivative along a seeded direction of the initial
+    velocities, angular velocities, masses and friction coefficients."""
+
+    def __init__(self, packed, dv=None, dw=None, dm=None, du=None):
+        self.lib = dual_lib()
+        self.packed = packed
+        self.desc = packed.desc()
+        self.h = self.lib.dual_new(ctypes.byref(self.desc))
+        seeds = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (dv, dw, dm, du)]
+        self._seeds = seeds
+        self.lib.dual_seed(self.h, *[None if x is None else ptr(x, ctypes.c_double) for x in seeds])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.dual_free(self.h)
+            self.h = None
+
+    def step(self, dt, n=1):
+        for _ in range(n):
+            self.lib.dual_step(self.h, dt)
+
+    def bodies(self):
+        """(values, tangents): dicts of p, v [n,2], a, w [n]."""
+        n = self.packed.n_bodies
+        val = {"p": _arr((n, 2)), "v": _arr((n, 2)), "a": _arr(n), "w": _arr(n)}
+        tan = {k: np.zeros_like(x) for k, x in val.items()}
+        args = []
+        for k in ("p", "v", "a", "w"):
+            args += [ptr(val[k], ctypes.c_double), ptr(tan[k], ctypes.c_double)]
+        self.lib.dual_read_bodies(self.h, *args)
+        return val, tan

@@ -220,3 +220,52 @@ def test_grad_1024_adjoint_matches_central_differences():
         assert len(report[kind]) >= 30, (kind, len(report[kind]), rejected[kind])
         assert max(x[3] for x in report[kind]) < 1e-6, (kind, sorted(report[kind], key=lambda x: -x[3])[:5])
     assert np.all(np.isfinite(L0))
+
+
+def test_grad_1024_adjoint_matches_dual_numbers():
+    """The same 120-step rollout of 1024 x 64 bodies, differentiated forward on
+    the CPU by the oracle's step on dual numbers (oracle/dual_step.cpp): its
+    value parts are the device's positions bit for bit, and for random
+    directions e of the initial velocities, of the masses and of the friction
+    coefficients, each space's directional derivative sum 2 (p_T - target) .
+    dp_T/de equals the adjoint's e . grad -- no finite differences, every
+    space (the contact set is the rollout's in both)."""
+    from checkers import DualWorld
+    t0 = time.time()
+    steps, every = 120, 10
+    pw, targets = grad_batch(1024, 64)
+    a = pw.a
+    n_worlds = pw.n_worlds
+    world_of_body = np.repeat(np.arange(n_worlds), np.diff(a["world_body_start"]))
+    world_of_shape = np.repeat(np.arange(n_worlds), np.diff(a["world_shape_start"]))
+    dyn = (a["body_type"] == G.DYNAMIC).astype(np.float64)
+    g = GPUWorld(pw)
+    g.tape_begin(every)
+    g.step(DT, steps)
+    p = g.bodies().p
+    grads = G.param_grads(pw, g.backward(g_p=2.0 * (p - targets) * dyn[:, None]))
+    g.close()
+    _log("taped rollout and backward", t0)
+    rng = np.random.default_rng(0xD0A1)
+    directions = {"v0": dict(dv=rng.normal(size=(pw.n_bodies, 2)) * dyn[:, None]),
+                  "m": dict(dm=rng.normal(size=pw.n_bodies) * dyn),
+                  "u": dict(du=rng.normal(size=pw.n_shapes) * dyn[a["shape_body"]])}
+    for kind, seed in directions.items():
+        d = DualWorld(pw, **seed)
+        d.step(DT, steps)
+        val, tan = d.bodies()
+        assert np.array_equal(val["p"], p), kind
+        dual = np.bincount(world_of_body, weights=np.sum(2.0 * (p - targets) * dyn[:, None] * tan["p"], axis=1),
+                           minlength=n_worlds)
+        if kind == "v0":
+            e_grad = np.sum(seed["dv"] * grads["v0"], axis=1)
+            adj = np.bincount(world_of_body, weights=e_grad, minlength=n_worlds)
+        elif kind == "m":
+            adj = np.bincount(world_of_body, weights=seed["dm"] * grads["m"], minlength=n_worlds)
+        else:
+            adj = np.bincount(world_of_shape, weights=seed["du"] * grads["u"], minlength=n_worlds)
+        # relative to the space's own derivative (1e-3 floor)
+        rel = np.abs(adj - dual) / np.maximum(np.abs(dual), 1e-3)
+        _log("%s: %d spaces, max rel diff %.2e, median %.2e, 99th pct %.2e" %
+             (kind, n_worlds, rel.max(), np.median(rel), np.percentile(rel, 99)), t0)
+        assert rel.max() < 1e-6, (kind, int(np.argmax(rel)), rel.max(), adj[np.argmax(rel)], dual[np.argmax(rel)])

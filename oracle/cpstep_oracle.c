@@ -1065,6 +1065,284 @@ int64_t ow_candidate_pairs(oworld *w, int64_t cap, int32_t *out)
 }
 
 /* ------------------------------------------------------------------ */
+/* spatial queries: cpSpaceQuery.c over the shapes as the last step cached
+ * them, with the shape queries of cpShape.c and cpPolyShape.c.  The
+ * reference finds candidates through its spatial indexes and checks each
+ * exactly; here every shape of the space is checked, which gives the same
+ * results (the exact checks decide) -- except that a segment query with a
+ * radius also finds shapes within the radius whose boxes the zero-width
+ * segment misses, which the reference's BBTree prunes (cpBBTree.c
+ * SubtreeSegmentQuery tests node boxes with cpBBSegmentQuery).  Of equal
+ * nearest distances or first alphas the lowest shape index is kept. */
+
+#define MAGIC_EPSILON 1e-5   /* chipmunk_private.h:32 */
+
+typedef struct { int hit; V point; num d; V grad; } PQInfo;            /* cpPointQueryInfo */
+typedef struct { int hit; V point, normal; num alpha; } SQInfo;        /* cpSegmentQueryInfo */
+
+static inline V closest_on_segment(V p, V a, V b)                      /* chipmunk.h:182-188 */
+{
+	V delta = vsub(a, b);
+	num t = fclamp01(vdot(delta, vsub(p, b)) / vlengthsq(delta));
+	return vadd(b, vmult(delta, t));
+}
+
+/* cpShapePointQuery (cpShape.c:223-234) through the class functions:
+ * circle cpShape.c:299-312, segment cpShape.c:408-424, poly cpPolyShape.c:67-103 */
+static PQInfo point_query(const OShape *s, V p)
+{
+	PQInfo q;
+	q.hit = 1;
+	if (s->type == CPGPU_SHAPE_CIRCLE) {
+		V delta = vsub(p, s->tc);
+		num d = vlength(delta), r = s->r;
+		num r_over_d = d > 0.0 ? r / d : r;
+		q.point = vadd(s->tc, vmult(delta, r_over_d));
+		q.d = d - r;
+		q.grad = d > MAGIC_EPSILON ? vmult(delta, 1.0 / d) : v2(0.0, 1.0);
+	} else if (s->type == CPGPU_SHAPE_SEGMENT) {
+		V closest = closest_on_segment(p, s->ta, s->tb);
+		V delta = vsub(p, closest);
+		num d = vlength(delta), r = s->r;
+		V g = vmult(delta, 1.0 / d);
+		q.point = d != 0.0 ? vadd(closest, vmult(g, r)) : closest;
+		q.d = d - r;
+		q.grad = d > MAGIC_EPSILON ? g : s->n;   /* the segment's body-local normal, as the reference reads it */
+	} else {
+		int count = s->count;
+		V v0 = s->wv[count - 1];
+		num minDist = INFINITY;
+		V closestPoint = v2(0.0, 0.0), closestNormal = v2(0.0, 0.0);
+		int outside = 0;
+		for (int i = 0; i < count; i++) {
+			V v1 = s->wv[i];
+			outside = outside || (vdot(s->wn[i], vsub(p, v1)) > 0.0);
+			V closest = closest_on_segment(p, v0, v1);
+			num dist = vlength(vsub(p, closest));
+			if (dist < minDist) { minDist = dist; closestPoint = closest; closestNormal = s->wn[i]; }
+			v0 = v1;
+		}
+		num dist = outside ? minDist : -minDist;
+		V g = vmult(vsub(p, closestPoint), 1.0 / dist);
+		q.point = vadd(closestPoint, vmult(g, s->r));
+		q.d = dist - s->r;
+		q.grad = minDist > MAGIC_EPSILON ? g : closestNormal;
+	}
+	return q;
+}
+
+/* CircleSegmentQuery (chipmunk_private.h:121-142) */
+static void circle_segment_query(V center, num r1, V a, V b, num r2, SQInfo *info)
+{
+	V da = vsub(a, center), db = vsub(b, center);
+	num rsum = r1 + r2;
+	num qa = vdot(da, da) - 2.0 * vdot(da, db) + vdot(db, db);
+	num qb = vdot(da, db) - vdot(da, da);
+	num det = qb * qb - qa * (vdot(da, da) - rsum * rsum);
+	if (det >= 0.0) {
+		num t = (-qb - sqrt(det)) / qa;
+		if (0.0 <= t && t <= 1.0) {
+			V n = vnormalize(vlerp(da, db, t));
+			info->hit = 1;
+			info->point = vsub(vlerp(a, b, t), vmult(n, r2));
+			info->normal = n;
+			info->alpha = t;
+		}
+	}
+}
+
+static inline SQInfo sq_blank(V b) { SQInfo q; q.hit = 0; q.point = b; q.normal = v2(0.0, 0.0); q.alpha = 1.0; return q; }
+
+/* cpShapeSegmentQuery (cpShape.c:238-257) through the class functions:
+ * circle cpShape.c:315-318, segment cpShape.c:426-466, poly cpPolyShape.c:106-145 */
+static SQInfo segment_query(const OShape *s, V a, V b, num radius)
+{
+	SQInfo info = sq_blank(b);
+	PQInfo nearest = point_query(s, a);
+	if (nearest.d <= radius) {
+		info.hit = 1;
+		info.alpha = 0.0;
+		info.normal = vnormalize(vsub(a, nearest.point));
+		return info;
+	}
+	if (s->type == CPGPU_SHAPE_CIRCLE) {
+		circle_segment_query(s->tc, s->r, a, b, radius, &info);
+	} else if (s->type == CPGPU_SHAPE_SEGMENT) {
+		V n = s->tn;
+		num d = vdot(vsub(s->ta, a), n);
+		num r = s->r + radius;
+		V flipped_n = d > 0.0 ? vneg(n) : n;
+		V seg_offset = vsub(vmult(flipped_n, r), a);
+		V seg_a = vadd(s->ta, seg_offset), seg_b = vadd(s->tb, seg_offset);
+		V delta = vsub(b, a);
+		if (vcross(delta, seg_a) * vcross(delta, seg_b) <= 0.0) {
+			num d_offset = d + (d > 0.0 ? -r : r);
+			num ad = -d_offset;
+			num bd = vdot(delta, n) - d_offset;
+			if (ad * bd < 0.0) {
+				num t = ad / (ad - bd);
+				info.hit = 1;
+				info.point = vsub(vlerp(a, b, t), vmult(flipped_n, radius));
+				info.normal = flipped_n;
+				info.alpha = t;
+			}
+		} else if (r != 0.0) {
+			SQInfo info1 = sq_blank(b), info2 = sq_blank(b);
+			circle_segment_query(s->ta, s->r, a, b, radius, &info1);
+			circle_segment_query(s->tb, s->r, a, b, radius, &info2);
+			info = info1.alpha < info2.alpha ? info1 : info2;
+		}
+	} else {
+		int count = s->count;
+		num r = s->r, rsum = r + radius;
+		for (int i = 0; i < count; i++) {
+			V n = s->wn[i];
+			num an = vdot(a, n);
+			num d = an - vdot(s->wv[i], n) - rsum;
+			if (d < 0.0) continue;
+			num bn = vdot(b, n);
+			num t = d / fmaxc(an - bn, DBL_MIN);
+			if (t < 0.0 || 1.0 < t) continue;
+			V point = vlerp(a, b, t);
+			num dt = vcross(n, point);
+			num dtMin = vcross(n, s->wv[(i - 1 + count) % count]);
+			num dtMax = vcross(n, s->wv[i]);
+			if (dtMin <= dt && dt <= dtMax) {
+				info.hit = 1;
+				info.point = vsub(vlerp(a, b, t), vmult(n, radius));
+				info.normal = n;
+				info.alpha = t;
+			}
+		}
+		if (rsum > 0.0) {
+			for (int i = 0; i < count; i++) {
+				SQInfo circle_info = sq_blank(b);
+				circle_segment_query(s->wv[i], r, a, b, radius, &circle_info);
+				if (circle_info.alpha < info.alpha) info = circle_info;
+			}
+		}
+	}
+	return info;
+}
+
+static inline int query_filter_reject(const OShape *s, uint64_t group, uint32_t cats, uint32_t mask)
+{                                                                      /* chipmunk_private.h:144-155 */
+	return (s->group != 0 && s->group == group) || (s->cats & mask) == 0 || (cats & s->mask) == 0;
+}
+
+static const OShape *world_shapes(oworld *w, int world, int *s0, int *s1)
+{
+	*s0 = w->W[world].s0;
+	*s1 = w->W[world].s1;
+	return w->S;
+}
+
+/* cpSpacePointQueryNearest (cpSpaceQuery.c:72-98): the shape (-1: none) and
+ * {point.x, point.y, distance, gradient.x, gradient.y} */
+int32_t ow_point_query_nearest(oworld *w, int32_t world, const double *p, double max_distance, uint64_t group,
+                               uint32_t cats, uint32_t mask, double *info)
+{
+	int s0, s1;
+	const OShape *S = world_shapes(w, world, &s0, &s1);
+	V pt = v2(p[0], p[1]);
+	int32_t best = -1;
+	PQInfo out;
+	out.hit = 0; out.point = v2(0.0, 0.0); out.d = max_distance; out.grad = v2(0.0, 0.0);
+	for (int i = s0; i < s1; i++) {
+		if (query_filter_reject(&S[i], group, cats, mask)) continue;
+		PQInfo q = point_query(&S[i], pt);
+		if (q.d < out.d) { out = q; best = i; }
+	}
+	info[0] = NUM_VAL(out.point.x); info[1] = NUM_VAL(out.point.y); info[2] = NUM_VAL(out.d);
+	info[3] = NUM_VAL(out.grad.x); info[4] = NUM_VAL(out.grad.y);
+	return best;
+}
+
+/* cpSpaceSegmentQueryFirst (cpSpaceQuery.c:154-175): the shape (-1: none)
+ * and {point.x, point.y, normal.x, normal.y, alpha} */
+int32_t ow_segment_query_first(oworld *w, int32_t world, const double *a, const double *b, double radius,
+                               uint64_t group, uint32_t cats, uint32_t mask, double *info)
+{
+	int s0, s1;
+	const OShape *S = world_shapes(w, world, &s0, &s1);
+	V va = v2(a[0], a[1]), vb = v2(b[0], b[1]);
+	int32_t best = -1;
+	SQInfo out = sq_blank(vb);
+	for (int i = s0; i < s1; i++) {
+		if (query_filter_reject(&S[i], group, cats, mask)) continue;
+		SQInfo q = segment_query(&S[i], va, vb, radius);
+		if (q.hit && q.alpha < out.alpha) { out = q; best = i; }
+	}
+	info[0] = NUM_VAL(out.point.x); info[1] = NUM_VAL(out.point.y); info[2] = NUM_VAL(out.normal.x);
+	info[3] = NUM_VAL(out.normal.y); info[4] = NUM_VAL(out.alpha);
+	return best;
+}
+
+/* cpSpacePointQuery (cpSpaceQuery.c:33-57): every shape closer than
+ * max_distance, in index order, with {point, distance, gradient}; returns
+ * the count (up to cap written) */
+int64_t ow_point_query(oworld *w, int32_t world, const double *p, double max_distance, uint64_t group, uint32_t cats,
+                       uint32_t mask, int64_t cap, int32_t *shapes, double *info)
+{
+	int s0, s1;
+	const OShape *S = world_shapes(w, world, &s0, &s1);
+	V pt = v2(p[0], p[1]);
+	int64_t n = 0;
+	for (int i = s0; i < s1; i++) {
+		if (query_filter_reject(&S[i], group, cats, mask)) continue;
+		PQInfo q = point_query(&S[i], pt);
+		if (!(q.d < max_distance)) continue;
+		if (n < cap) {
+			shapes[n] = i;
+			double *o = info + 5 * n;
+			o[0] = NUM_VAL(q.point.x); o[1] = NUM_VAL(q.point.y); o[2] = NUM_VAL(q.d);
+			o[3] = NUM_VAL(q.grad.x); o[4] = NUM_VAL(q.grad.y);
+		}
+		n++;
+	}
+	return n;
+}
+
+/* cpSpaceSegmentQuery (cpSpaceQuery.c:107-135): every shape the segment hits */
+int64_t ow_segment_query(oworld *w, int32_t world, const double *a, const double *b, double radius, uint64_t group,
+                         uint32_t cats, uint32_t mask, int64_t cap, int32_t *shapes, double *info)
+{
+	int s0, s1;
+	const OShape *S = world_shapes(w, world, &s0, &s1);
+	V va = v2(a[0], a[1]), vb = v2(b[0], b[1]);
+	int64_t n = 0;
+	for (int i = s0; i < s1; i++) {
+		if (query_filter_reject(&S[i], group, cats, mask)) continue;
+		SQInfo q = segment_query(&S[i], va, vb, radius);
+		if (!q.hit) continue;
+		if (n < cap) {
+			shapes[n] = i;
+			double *o = info + 5 * n;
+			o[0] = NUM_VAL(q.point.x); o[1] = NUM_VAL(q.point.y); o[2] = NUM_VAL(q.normal.x);
+			o[3] = NUM_VAL(q.normal.y); o[4] = NUM_VAL(q.alpha);
+		}
+		n++;
+	}
+	return n;
+}
+
+/* cpSpaceBBQuery (cpSpaceQuery.c:186-206): every shape whose box meets bb {l, b, r, t} */
+int64_t ow_bb_query(oworld *w, int32_t world, const double *bb, uint64_t group, uint32_t cats, uint32_t mask,
+                    int64_t cap, int32_t *shapes)
+{
+	int s0, s1;
+	const OShape *S = world_shapes(w, world, &s0, &s1);
+	int64_t n = 0;
+	for (int i = s0; i < s1; i++) {
+		if (query_filter_reject(&S[i], group, cats, mask)) continue;
+		if (!(bb[0] <= S[i].bb[2] && S[i].bb[0] <= bb[2] && bb[1] <= S[i].bb[3] && S[i].bb[1] <= bb[3])) continue;
+		if (n < cap) shapes[n] = i;
+		n++;
+	}
+	return n;
+}
+
+/* ------------------------------------------------------------------ */
 /* arbiter persistence: the cachedArbiters hash set of cpSpace
  * (cpSpaceStep.c:248-251, cpHashSet.c:152-176) keyed by the unordered
  * shape pair, eviction by cpSpaceArbiterSetFilter (cpSpaceStep.c:292-325). */

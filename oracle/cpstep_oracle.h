@@ -92,6 +92,24 @@ int ow_shapes_collide(oworld *w, int32_t a, int32_t b, double *n_out, double *po
  * sorted; returns the total count. */
 int64_t ow_candidate_pairs(oworld *w, int64_t cap, int32_t *pairs);
 
+/* Spatial queries (cpSpaceQuery.c) on the shapes as the last step cached
+ * them, in space `world`, filter {group, categories, mask}: the nearest
+ * shape to a point (cpSpacePointQueryNearest; info {point, distance,
+ * gradient}), the first shape along a segment (cpSpaceSegmentQueryFirst;
+ * info {point, normal, alpha}), and all shapes near a point, along a segment
+ * or meeting a box (cpSpacePointQuery, cpSpaceSegmentQuery, cpSpaceBBQuery),
+ * in shape order with up to cap results written.  Every shape is checked. */
+int32_t ow_point_query_nearest(oworld *w, int32_t world, const double *p, double max_distance, uint64_t group,
+                               uint32_t cats, uint32_t mask, double *info);
+int32_t ow_segment_query_first(oworld *w, int32_t world, const double *a, const double *b, double radius,
+                               uint64_t group, uint32_t cats, uint32_t mask, double *info);
+int64_t ow_point_query(oworld *w, int32_t world, const double *p, double max_distance, uint64_t group, uint32_t cats,
+                       uint32_t mask, int64_t cap, int32_t *shapes, double *info);
+int64_t ow_segment_query(oworld *w, int32_t world, const double *a, const double *b, double radius, uint64_t group,
+                         uint32_t cats, uint32_t mask, int64_t cap, int32_t *shapes, double *info);
+int64_t ow_bb_query(oworld *w, int32_t world, const double *bb, uint64_t group, uint32_t cats, uint32_t mask,
+                    int64_t cap, int32_t *shapes);
+
 /* The deterministic sin/cos shared with the device library. */
 void cp_sincos_oracle(double x, double *s, double *c);
 double cp_exp_oracle(double x);

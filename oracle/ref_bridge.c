@@ -509,3 +509,94 @@ int rb_body_arbiters(cpBody *body, int cap, const cpShape **a, const cpShape **b
 	cpBodyEachArbiter(body, arb_visit, &L);
 	return L.n;
 }
+
+/* ---- spatial queries through the reference's public API (cpSpaceQuery.c) ---- */
+
+static cpShapeFilter query_filter(uint64_t group, uint32_t cats, uint32_t mask)
+{
+	return cpShapeFilterNew((cpGroup)(uintptr_t)group, cats, mask);
+}
+
+/* cpSpacePointQueryNearest: description index of the shape (-1: none), info
+ * {point.x, point.y, distance, gradient.x, gradient.y} */
+int32_t rb_point_query_nearest(rbWorld *w, const double *p, double max_distance, uint64_t group, uint32_t cats,
+                               uint32_t mask, double *info)
+{
+	cpPointQueryInfo q;
+	cpShape *s = cpSpacePointQueryNearest(w->space, cpv(p[0], p[1]), max_distance, query_filter(group, cats, mask), &q);
+	info[0] = q.point.x; info[1] = q.point.y; info[2] = q.distance; info[3] = q.gradient.x; info[4] = q.gradient.y;
+	return s ? (int32_t)(intptr_t)s->userData : -1;
+}
+
+/* cpSpaceSegmentQueryFirst: shape index (-1: none), info {point, normal, alpha} */
+int32_t rb_segment_query_first(rbWorld *w, const double *a, const double *b, double radius, uint64_t group,
+                               uint32_t cats, uint32_t mask, double *info)
+{
+	cpSegmentQueryInfo q;
+	cpShape *s = cpSpaceSegmentQueryFirst(w->space, cpv(a[0], a[1]), cpv(b[0], b[1]), radius,
+	                                      query_filter(group, cats, mask), &q);
+	info[0] = q.point.x; info[1] = q.point.y; info[2] = q.normal.x; info[3] = q.normal.y; info[4] = q.alpha;
+	return s ? (int32_t)(intptr_t)s->userData : -1;
+}
+
+typedef struct QueryHits {
+	int64_t n, cap;
+	int32_t *shapes;
+	double *info;
+} QueryHits;
+
+static void point_hit(cpShape *shape, cpVect point, cpFloat distance, cpVect gradient, void *data)
+{
+	QueryHits *h = (QueryHits *)data;
+	if (h->n < h->cap) {
+		h->shapes[h->n] = (int32_t)(intptr_t)shape->userData;
+		double *o = h->info + 5 * h->n;
+		o[0] = point.x; o[1] = point.y; o[2] = distance; o[3] = gradient.x; o[4] = gradient.y;
+	}
+	h->n++;
+}
+
+static void segment_hit(cpShape *shape, cpVect point, cpVect normal, cpFloat alpha, void *data)
+{
+	QueryHits *h = (QueryHits *)data;
+	if (h->n < h->cap) {
+		h->shapes[h->n] = (int32_t)(intptr_t)shape->userData;
+		double *o = h->info + 5 * h->n;
+		o[0] = point.x; o[1] = point.y; o[2] = normal.x; o[3] = normal.y; o[4] = alpha;
+	}
+	h->n++;
+}
+
+static void bb_hit(cpShape *shape, void *data)
+{
+	QueryHits *h = (QueryHits *)data;
+	if (h->n < h->cap) h->shapes[h->n] = (int32_t)(intptr_t)shape->userData;
+	h->n++;
+}
+
+/* cpSpacePointQuery / cpSpaceSegmentQuery / cpSpaceBBQuery: every shape the
+ * callback saw, in the order it saw them; returns the count */
+int64_t rb_point_query(rbWorld *w, const double *p, double max_distance, uint64_t group, uint32_t cats, uint32_t mask,
+                       int64_t cap, int32_t *shapes, double *info)
+{
+	QueryHits h = {0, cap, shapes, info};
+	cpSpacePointQuery(w->space, cpv(p[0], p[1]), max_distance, query_filter(group, cats, mask), point_hit, &h);
+	return h.n;
+}
+
+int64_t rb_segment_query(rbWorld *w, const double *a, const double *b, double radius, uint64_t group, uint32_t cats,
+                         uint32_t mask, int64_t cap, int32_t *shapes, double *info)
+{
+	QueryHits h = {0, cap, shapes, info};
+	cpSpaceSegmentQuery(w->space, cpv(a[0], a[1]), cpv(b[0], b[1]), radius, query_filter(group, cats, mask),
+	                    segment_hit, &h);
+	return h.n;
+}
+
+int64_t rb_bb_query(rbWorld *w, const double *bb, uint64_t group, uint32_t cats, uint32_t mask, int64_t cap,
+                    int32_t *shapes)
+{
+	QueryHits h = {0, cap, shapes, NULL};
+	cpSpaceBBQuery(w->space, cpBBNew(bb[0], bb[1], bb[2], bb[3]), query_filter(group, cats, mask), bb_hit, &h);
+	return h.n;
+}

@@ -59,6 +59,17 @@ def oracle_lib():
         lib.ow_candidate_pairs.restype = ctypes.c_int64
         lib.ow_candidate_pairs.argtypes = [_vp, ctypes.c_int64, _ip]
         lib.ow_write_bodies_at.argtypes = [_vp, ctypes.c_int32, _ip] + [_dp] * 7
+        _q = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+        lib.ow_point_query_nearest.restype = ctypes.c_int32
+        lib.ow_point_query_nearest.argtypes = [_vp, ctypes.c_int32, _dp, _d] + _q + [_dp]
+        lib.ow_segment_query_first.restype = ctypes.c_int32
+        lib.ow_segment_query_first.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _d] + _q + [_dp]
+        lib.ow_point_query.restype = ctypes.c_int64
+        lib.ow_point_query.argtypes = [_vp, ctypes.c_int32, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
+        lib.ow_segment_query.restype = ctypes.c_int64
+        lib.ow_segment_query.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
+        lib.ow_bb_query.restype = ctypes.c_int64
+        lib.ow_bb_query.argtypes = [_vp, ctypes.c_int32, _dp] + _q + [ctypes.c_int64, _ip]
         lib.ow_set_space_params.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(SpaceParams)]
         lib.ow_new_from.restype = _vp
         lib.ow_new_from.argtypes = [ctypes.POINTER(WorldDesc), _vp, _ip]
@@ -99,6 +110,17 @@ def ref_lib(release=False):
         lib.rb_space_stamp.restype = ctypes.c_int64
         lib.rb_space_stamp.argtypes = [_vp]
         lib.rb_remove_shape.argtypes = lib.rb_remove_body.argtypes = [_vp, ctypes.c_int]
+        _q = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+        lib.rb_point_query_nearest.restype = ctypes.c_int32
+        lib.rb_point_query_nearest.argtypes = [_vp, _dp, _d] + _q + [_dp]
+        lib.rb_segment_query_first.restype = ctypes.c_int32
+        lib.rb_segment_query_first.argtypes = [_vp, _dp, _dp, _d] + _q + [_dp]
+        lib.rb_point_query.restype = ctypes.c_int64
+        lib.rb_point_query.argtypes = [_vp, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
+        lib.rb_segment_query.restype = ctypes.c_int64
+        lib.rb_segment_query.argtypes = [_vp, _dp, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
+        lib.rb_bb_query.restype = ctypes.c_int64
+        lib.rb_bb_query.argtypes = [_vp, _dp] + _q + [ctypes.c_int64, _ip]
         lib.rb_new_in.restype = _vp
         lib.rb_new_in.argtypes = [_vp, ctypes.POINTER(WorldDesc), ctypes.c_int]
         for name in ("rb_space", "rb_body", "rb_shape", "rb_constraint"):
@@ -521,3 +543,61 @@ class DualWorld:
             args += [ptr(val[k], ctypes.c_double), ptr(tan[k], ctypes.c_double)]
         self.lib.dual_read_bodies(self.h, *args)
         return val, tan
+
+
+ALL = (0, 0xFFFFFFFF, 0xFFFFFFFF)   # CP_SHAPE_FILTER_ALL: group, categories, mask
+
+
+def _v(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+class Queries:
+    """Spatial queries (cpSpaceQuery.c) with one calling convention for the oracle
+    (`ow_*`, a space index first) and the reference (`rb_*`, one space); shape
+    indices are the description's, hits sorted by shape."""
+
+    def __init__(self, fn_prefix, lib, h, world=None, cap=4096):
+        self.lib, self.h, self.world, self.cap, self.pre = lib, h, world, cap, fn_prefix
+
+    def _args(self):
+        return [self.h] if self.world is None else [self.h, self.world]
+
+    def point_nearest(self, p, max_distance, filt=ALL):
+        info = np.zeros(5)
+        s = getattr(self.lib, self.pre + "point_query_nearest")(*self._args(), ptr(_v(p), ctypes.c_double),
+                                                                 max_distance, *filt, ptr(info, ctypes.c_double))
+        return int(s), info
+
+    def segment_first(self, a, b, radius, filt=ALL):
+        info = np.zeros(5)
+        s = getattr(self.lib, self.pre + "segment_query_first")(*self._args(), ptr(_v(a), ctypes.c_double),
+                                                                 ptr(_v(b), ctypes.c_double), radius, *filt,
+                                                                 ptr(info, ctypes.c_double))
+        return int(s), info
+
+    def _hits(self, name, args, with_info=True):
+        shapes = np.zeros(self.cap, np.int32)
+        info = np.zeros((self.cap, 5))
+        extra = [ptr(shapes, ctypes.c_int32)] + ([ptr(info, ctypes.c_double)] if with_info else [])
+        n = getattr(self.lib, self.pre + name)(*self._args(), *args, self.cap, *extra)
+        assert n <= self.cap
+        order = np.argsort(shapes[:n], kind="stable")
+        return shapes[:n][order], info[:n][order]
+
+    def point_all(self, p, max_distance, filt=ALL):
+        return self._hits("point_query", [ptr(_v(p), ctypes.c_double), max_distance, *filt])
+
+    def segment_all(self, a, b, radius, filt=ALL):
+        return self._hits("segment_query", [ptr(_v(a), ctypes.c_double), ptr(_v(b), ctypes.c_double), radius, *filt])
+
+    def bb_all(self, bb, filt=ALL):
+        return self._hits("bb_query", [ptr(_v(bb), ctypes.c_double), *filt], with_info=False)[0]
+
+
+def oracle_queries(o, world=0):
+    return Queries("ow_", o.lib, o.h, world)
+
+
+def ref_queries(r):
+    return Queries("rb_", r.lib, r.h)

@@ -15,7 +15,7 @@ import pytest
 from chipmunk2d_amd.cabi import PackedWorld, BODY_DYNAMIC
 from chipmunk2d_amd.scenes import pyramid_210, random_world, Rng
 from chipmunk2d_amd.space import Space, Body, Circle, Segment, Poly, Box, moment_for_circle, moment_for_box
-from checkers import OracleWorld, RefWorld, ref_available, sincos, oracle_lib, without
+from checkers import OracleWorld, RefWorld, ref_available, sincos, oracle_lib, without, oracle_queries, ref_queries
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 needs_ref = pytest.mark.skipif(not ref_available(), reason="compiled reference oracle/_ref/libcpref.so not built")
@@ -326,6 +326,57 @@ def test_new_pack_after_removes_matches_reference():
     assert np.array_equal(ra.contact, oa.contact)
     assert np.array_equal(ra.state, oa.state)
     assert r.cached_arbiters() == o2.stats()["cached_arbiters"]
+
+
+@needs_ref
+def test_spatial_queries_match_reference():
+    """cpSpacePointQueryNearest, cpSpaceSegmentQueryFirst, cpSpacePointQuery,
+    cpSpaceSegmentQuery and cpSpaceBBQuery (cpSpaceQuery.c) of the compiled
+    reference against the oracle's, on a pile after 40 steps in lockstep:
+    the same shapes with bit-identical points, distances, gradients, normals
+    and alphas.  With a segment radius the reference's BBTree prunes by the
+    zero-width segment, so there its hits are a subset of the oracle's."""
+    dt = 1 / 60
+    pw = random_world(n_circles=200, n_polys=200, n_segments=60, extent=60.0, seed=12).pack()
+    o, r = OracleWorld(pw), RefWorld(pw)
+    o.libm_trig(True)
+    for _ in range(40):
+        coll = r.step_logged(dt)
+        o.replay(r.arbiters(), coll, dt)
+    qo, qr = oracle_queries(o), ref_queries(r)
+    rng = np.random.default_rng(3)
+    lo, hi = pw.a["body_p"].min(axis=0) - 5, pw.a["body_p"].max(axis=0) + 5
+    found = {"nearest": 0, "first": 0, "point": 0, "segment": 0, "bb": 0}
+    for k in range(150):
+        p = rng.uniform(lo, hi)
+        filt = (0, 0xFFFFFFFF, 0xFFFFFFFF) if k % 10 else (0, 0xFFFFFFFF, 0)   # every tenth rejects everything
+        for maxd in (0.25, 2.0, 6.0):
+            so, io = qo.point_nearest(p, maxd, filt)
+            sr, ir = qr.point_nearest(p, maxd, filt)
+            assert so == sr and np.array_equal(io, ir), (k, maxd, so, sr, io, ir)
+            found["nearest"] += so >= 0
+            ho, hio = qo.point_all(p, maxd, filt)
+            hr, hir = qr.point_all(p, maxd, filt)
+            assert np.array_equal(ho, hr) and np.array_equal(hio, hir), (k, maxd)
+            found["point"] += len(ho)
+        a, b = p, p + rng.normal(size=2) * 15.0
+        so, io = qo.segment_first(a, b, 0.0, filt)
+        sr, ir = qr.segment_first(a, b, 0.0, filt)
+        assert so == sr and np.array_equal(io, ir), (k, so, sr, io, ir)
+        found["first"] += so >= 0
+        ho, hio = qo.segment_all(a, b, 0.0, filt)
+        hr, hir = qr.segment_all(a, b, 0.0, filt)
+        assert np.array_equal(ho, hr) and np.array_equal(hio, hir), k
+        found["segment"] += len(ho)
+        ho, hio = qo.segment_all(a, b, 0.75, filt)
+        hr, hir = qr.segment_all(a, b, 0.75, filt)
+        common = np.isin(ho, hr)
+        assert np.all(np.isin(hr, ho)) and np.array_equal(hio[common], hir), k
+        bb = np.concatenate([np.minimum(a, b), np.maximum(a, b)])[[0, 1, 2, 3]]
+        bo, br = qo.bb_all(bb, filt), qr.bb_all(bb, filt)
+        assert np.array_equal(bo, br), k
+        found["bb"] += len(bo)
+    assert min(found.values()) > 20, found
 
 
 # ---------------------------------------------------------------------------

@@ -236,6 +236,14 @@ def measured_traffic(workload, kernel, steps, precision="fp64"):
     return None
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print("[bench %7.1f s] %s" % (time.time() - _T0, msg), file=sys.stderr, flush=True)
+
+
+_T0 = time.time()
+
+
 # ---- device runs ----
 
 def _advance(g, dt, n, chunk=50):
@@ -246,6 +254,8 @@ def _advance(g, dt, n, chunk=50):
         g.step(dt, k)
         g.sync()
         done += k
+        if n > chunk:
+            progress("  %d of %d untimed steps" % (done, n))
 
 
 def run_gpu(pw, steps, warmup, start, dt, autotune_steps=0, autotune_at=60, precision=0, bucket_threads=None):
@@ -260,6 +270,7 @@ def run_gpu(pw, steps, warmup, start, dt, autotune_steps=0, autotune_at=60, prec
     if bucket_threads:
         autotune_steps = 0
     tiling = None
+    progress("warm-up (%d steps)%s" % (warmup, ", autotuner" if autotune_steps else ""))
     if warmup > 0 or autotune_steps > 0:
         g = GPUWorld(pw, precision)
         if warmup > 0:
@@ -284,6 +295,7 @@ def run_gpu(pw, steps, warmup, start, dt, autotune_steps=0, autotune_at=60, prec
         st = g.stats()
         return st, g.solve_counts()
 
+    progress("the measured run: %d untimed steps, then %d timed" % (start, steps))
     g = fresh()
     tiling = g.buckets()
     state0 = g.bodies() if start > 0 else None
@@ -307,6 +319,7 @@ def run_gpu(pw, steps, warmup, start, dt, autotune_steps=0, autotune_at=60, prec
     for k in ("total_arbiters", "total_contacts", "total_pairs"):
         stats[k] = st1[k] - st0[k]
     counts = {k: (sc1[k][0] - sc0[k][0], sc1[k][1] - sc0[k][1]) for k in sc1}
+    progress("timed window: %.3f s; the per-kernel table run" % (t1 - t0))
     # the per-kernel table: the same window again, every launch timed
     g = fresh()
     g.set_timing(True)
@@ -378,6 +391,7 @@ def run_plain_per_space(pw, n_sample, steps, start, dt):
     window's `steps` steps each.  Returns (body-steps/s, bodies in the sample, seconds)."""
     from chipmunk2d_amd import scenes
     from chipmunk2d_amd.gpu import GPUWorld
+    progress("the batch's spaces one at a time (plain per-space step)")
     specs = scenes.batch_specs(n_worlds=pw.n_worlds)
     stride = max(1, pw.n_worlds // n_sample)
     spaces = [scenes.build_batch_world(specs[k]).pack() for k in range(0, pw.n_worlds, stride)][:n_sample]
@@ -500,6 +514,8 @@ def run_cpu_pinned(pw, start, n_dyn, cores, threads=1, batch=False, hasty=0, rel
     json.dump({"npz": npz, "dt": DT, "target_seconds": target, "runs": runs, "threads": threads, "batch": batch,
                "hasty": hasty, "release": release}, open(spec, "w"))
     cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--cpu-child", spec]
+    progress("CPU baseline on cores %s (%s%s%s)" % (",".join(str(c) for c in cores), "batch " if batch else "",
+                                                     "hasty " if hasty else "", "release" if release else "strict"))
     pinned = "taskset"
     try:
         subprocess.run(["taskset", "-V"], capture_output=True, check=True)
@@ -561,6 +577,7 @@ def cpu_baselines(workload, pw, state, start, n_dyn, gpu_rate, k_steps):
 # ---- one workload ----
 
 def measure(workload, args, precision, bucket_threads, cpu=True, plain=True, pw_config=None):
+    progress("building %s" % workload)
     pw, config = pw_config or build_workload(workload)
     steps = args.steps if args.steps is not None else config["rollout_steps"]
     start = window_of(workload, steps)

@@ -275,6 +275,7 @@ static cpCollisionPreSolveFunc default_pre_solve;
 static cpCollisionPostSolveFunc default_post_solve;
 static cpCollisionSeparateFunc default_separate;
 static cpDampedSpringForceFunc default_spring_force;
+static cpDampedRotarySpringTorqueFunc default_spring_torque;
 static int defaults_known;
 
 static void learn_defaults(void)
@@ -288,6 +289,9 @@ static void learn_defaults(void)
 	cpSpaceFree(s);
 	cpConstraint *c = cpDampedSpringNew(NULL, NULL, cpvzero, cpvzero, 0.0, 0.0, 0.0);
 	default_spring_force = ((cpDampedSpring *)c)->springForceFunc;
+	cpConstraintFree(c);
+	c = cpDampedRotarySpringNew(NULL, NULL, 0.0, 0.0, 0.0);
+	default_spring_torque = ((cpDampedRotarySpring *)c)->springTorqueFunc;
 	cpConstraintFree(c);
 	defaults_known = 1;
 }
@@ -479,20 +483,60 @@ static int read_constraint(Desc *D, int k, cpConstraint *c, int a, int b)
 		D->cons_type[k] = CPGPU_JOINT_GEAR;
 		P[0] = j->phase; P[1] = j->ratio; P[2] = j->ratio_inv;
 		J[0] = j->jAcc;
+	} else if (cpConstraintIsPinJoint(c)) {
+		const cpPinJoint *j = (const cpPinJoint *)c;
+		D->cons_type[k] = CPGPU_JOINT_PIN;
+		P[0] = j->anchorA.x; P[1] = j->anchorA.y; P[2] = j->anchorB.x; P[3] = j->anchorB.y;
+		P[4] = j->dist;
+		J[0] = j->jnAcc;
+	} else if (cpConstraintIsGrooveJoint(c)) {
+		const cpGrooveJoint *j = (const cpGrooveJoint *)c;
+		D->cons_type[k] = CPGPU_JOINT_GROOVE;
+		P[0] = j->grv_a.x; P[1] = j->grv_a.y; P[2] = j->grv_b.x; P[3] = j->grv_b.y;
+		P[4] = j->anchorB.x; P[5] = j->anchorB.y; P[6] = j->grv_n.x; P[7] = j->grv_n.y;
+		J[0] = j->jAcc.x; J[1] = j->jAcc.y;
+	} else if (cpConstraintIsRotaryLimitJoint(c)) {
+		const cpRotaryLimitJoint *j = (const cpRotaryLimitJoint *)c;
+		D->cons_type[k] = CPGPU_JOINT_ROTARY_LIMIT;
+		P[0] = j->min; P[1] = j->max;
+		J[0] = j->jAcc;
+	} else if (cpConstraintIsRatchetJoint(c)) {
+		const cpRatchetJoint *j = (const cpRatchetJoint *)c;
+		D->cons_type[k] = CPGPU_JOINT_RATCHET;
+		P[0] = j->phase; P[1] = j->ratchet;
+		J[0] = j->jAcc; J[1] = j->angle;
+	} else if (cpConstraintIsSimpleMotor(c)) {
+		const cpSimpleMotor *j = (const cpSimpleMotor *)c;
+		D->cons_type[k] = CPGPU_JOINT_SIMPLE_MOTOR;
+		P[0] = j->rate;
+		J[0] = j->jAcc;
+	} else if (cpConstraintIsDampedRotarySpring(c)) {
+		const cpDampedRotarySpring *j = (const cpDampedRotarySpring *)c;
+		if (j->springTorqueFunc != default_spring_torque)
+			return report("cpGPUSpace: a custom damped-rotary-spring torque is not supported on the device", NULL);
+		D->cons_type[k] = CPGPU_JOINT_DAMPED_ROTARY_SPRING;
+		P[0] = j->restAngle; P[1] = j->stiffness; P[2] = j->damping;
+		J[0] = j->jAcc;
 	} else {
-		return report("cpGPUSpace: only pivot, damped-spring, slide and gear joints run on the device", NULL);
+		return report("cpGPUSpace: a constraint class the device does not run", NULL);
 	}
 	return 0;
 }
 
-static void write_jacc(const Desc *D, int k, cpConstraint *c)
+/* the carried state J of a constraint of packed type `type` into its struct */
+static void write_jacc(int type, const double *J, cpConstraint *c)
 {
-	const double *J = D->cons_jacc + 2 * k;
-	switch (D->cons_type[k]) {
+	switch (type) {
 	case CPGPU_JOINT_PIVOT: ((cpPivotJoint *)c)->jAcc = cpv(J[0], J[1]); break;
 	case CPGPU_JOINT_DAMPED_SPRING: ((cpDampedSpring *)c)->jAcc = J[0]; break;
 	case CPGPU_JOINT_SLIDE: ((cpSlideJoint *)c)->jnAcc = J[0]; break;
 	case CPGPU_JOINT_GEAR: ((cpGearJoint *)c)->jAcc = J[0]; break;
+	case CPGPU_JOINT_PIN: ((cpPinJoint *)c)->jnAcc = J[0]; break;
+	case CPGPU_JOINT_GROOVE: ((cpGrooveJoint *)c)->jAcc = cpv(J[0], J[1]); break;
+	case CPGPU_JOINT_ROTARY_LIMIT: ((cpRotaryLimitJoint *)c)->jAcc = J[0]; break;
+	case CPGPU_JOINT_RATCHET: ((cpRatchetJoint *)c)->jAcc = J[0]; ((cpRatchetJoint *)c)->angle = J[1]; break;
+	case CPGPU_JOINT_SIMPLE_MOTOR: ((cpSimpleMotor *)c)->jAcc = J[0]; break;
+	case CPGPU_JOINT_DAMPED_ROTARY_SPRING: ((cpDampedRotarySpring *)c)->jAcc = J[0]; break;
 	}
 }
 
@@ -730,7 +774,7 @@ static int repack(Pack *pk)
 	pk->packs++;
 	/* the host structs take the state the device now holds */
 	for (int k = 0; k < F->nb; k++) write_body(F, k, F->bodies[k]);
-	for (int k = 0; k < F->nc; k++) write_jacc(F, k, F->cons[k]);
+	for (int k = 0; k < F->nc; k++) write_jacc(F->cons_type[k], F->cons_jacc + 2 * k, F->cons[k]);
 	swap_desc(C, F);
 	Maps t = pk->cur_at;
 	pk->cur_at = pk->fresh_at;
@@ -875,16 +919,81 @@ static void push_arbiter(cpBody *body, cpArbiter *arb)
 	body->arbiterList = arb;
 }
 
+/* ---- the binding's arbiters in the spaces' arbiter containers ----
+ * A step of the reference leaves its arbiters in space->cachedArbiters and
+ * space->arbiters (cpSpaceStep.c:250-260, 420-436) as well as threaded onto the
+ * bodies.  cpSpaceRemoveShape and cpSpaceRemoveBody unthread an arbiter only
+ * through space->cachedArbiters (cpSpaceFilterArbiters, cpSpace.c:483-520), so
+ * the arbiters the binding threads are registered there too: otherwise a body
+ * removed and freed would stay on its partners' arbiter lists. */
+
+static int is_ours(const Pack *pk, const void *p)
+{
+	const cpArbiter *a = (const cpArbiter *)p;
+	return pk->arbs && a >= pk->arbs && a < pk->arbs + pk->cap_arb;
+}
+
+static void drop_ours(cpArray *arr, const Pack *pk)
+{
+	int j = 0;
+	for (int i = 0; i < arr->num; i++)
+		if (!is_ours(pk, arr->arr[i])) arr->arr[j++] = arr->arr[i];
+	arr->num = j;
+}
+
+struct ForgetCtx {
+	const Pack *pk;
+	cpSpace *space;
+};
+
+/* every arbiter leaves the set: the binding's own are dropped; one a step of the
+ * reference left behind (a space stepped by cpSpaceStep before it was batched) is
+ * retired as cachedArbitersFilter retires it (cpSpace.c:503-505) */
+static cpBool forget_arbiter(void *elt, void *data)
+{
+	const struct ForgetCtx *c = (const struct ForgetCtx *)data;
+	cpArbiter *arb = (cpArbiter *)elt;
+	if (!is_ours(c->pk, arb)) {
+		cpArbiterUnthread(arb);
+		cpArrayDeleteObj(c->space->arbiters, arb);
+		cpArrayPush(c->space->pooledArbiters, arb);
+	}
+	return cpFalse;
+}
+
+/* The spaces' arbiter containers and the live bodies' arbiter lists emptied of
+ * every arbiter (the binding's are about to be replaced or freed). */
+static void forget_arbiters(Pack *pk)
+{
+	for (int w = 0; w < pk->n_spaces; w++) {
+		cpSpace *sp = pk->spaces[w];
+		struct ForgetCtx c = {pk, sp};
+		cpHashSetFilter(sp->cachedArbiters, forget_arbiter, &c);
+		drop_ours(sp->arbiters, pk);
+		drop_ours(sp->pooledArbiters, pk);
+		cpArray *arrays[2] = {sp->dynamicBodies, sp->staticBodies};
+		for (int q = 0; q < 2; q++)
+			for (int i = 0; i < arrays[q]->num; i++) ((cpBody *)arrays[q]->arr[i])->arbiterList = NULL;
+		sp->staticBody->arbiterList = NULL;
+	}
+	pk->narb = 0;
+}
+
+static void *same_arbiter(const void *shapes, void *arb)
+{
+	(void)shapes;
+	return arb;
+}
+
 /* The last step's arbiters as cpArbiters threaded onto their bodies, in the
  * device's solve order, as cpSpaceProcessComponents threads space->arbiters
- * (cpSpaceComponent.c:253-267). */
+ * (cpSpaceComponent.c:253-267), and entered into their space's cachedArbiters
+ * and arbiters. */
 static int sync_arbiters(Pack *pk)
 {
 	const Desc *C = &pk->cur;
 	const Maps *live = &pk->fresh_at;
-	for (int k = 0; k < C->nb; k++)
-		if (map_get(&live->body, C->bodies[k]) >= 0) C->bodies[k]->arbiterList = NULL;
-	pk->narb = 0;
+	forget_arbiters(pk);
 	int64_t n = 0;
 	int rc = cpgpuWorldReadArbiters(pk->world, 0, &n, NULL, NULL, NULL, NULL, NULL, NULL, NULL);
 	if (rc) return lib_error("cpgpuWorldReadArbiters", rc);
@@ -937,6 +1046,10 @@ static int sync_arbiters(Pack *pk)
 			}
 			push_arbiter(arb->body_a, arb);
 			push_arbiter(arb->body_b, arb);
+			const cpShape *pair[2] = {a, b};
+			cpHashSetInsert(a->space->cachedArbiters, CP_HASH_PAIR((cpHashValue)a, (cpHashValue)b), pair, same_arbiter,
+			                arb);
+			cpArrayPush(a->space->arbiters, arb);
 		}
 	}
 	free(shapes); free(state); free(count); free(normal); free(contact); free(hash);
@@ -973,7 +1086,8 @@ static int pack_sync(Pack *pk)
 		const int fk = map_get(&pk->fresh_at.cons, c);
 		if (!same_f64(pk->fresh.cons_jacc + 2 * fk, S->cons_jacc + 2 * k, 2)) continue;
 		memcpy(S->cons_jacc + 2 * k, pk->rb_jacc + 2 * k, 2 * sizeof(double));
-		write_jacc(S, k, c);
+		/* the type is the current pack's (the shadow keeps only state) */
+		write_jacc(C->cons_type[k], S->cons_jacc + 2 * k, c);
 	}
 	/* cached shape data and boxes from the new transforms (cpShapeUpdate, cpShape.c:216-220) */
 	for (int k = 0; k < C->ns; k++) {
@@ -989,14 +1103,8 @@ static void pack_release(Pack *pk)
 {
 	if (pk->world) {
 		(void)cpgpuWorldSync(pk->world);
-		/* unthread the host arbiters from the bodies still in their spaces */
-		for (int w = 0; w < pk->n_spaces; w++) {
-			cpSpace *sp = pk->spaces[w];
-			cpArray *arrays[2] = {sp->dynamicBodies, sp->staticBodies};
-			for (int q = 0; q < 2; q++)
-				for (int i = 0; i < arrays[q]->num; i++) ((cpBody *)arrays[q]->arr[i])->arbiterList = NULL;
-			sp->staticBody->arbiterList = NULL;
-		}
+		/* the host arbiters out of the spaces and off the bodies still in them */
+		forget_arbiters(pk);
 		cpgpuWorldDestroy(pk->world);
 		pk->world = NULL;
 	}

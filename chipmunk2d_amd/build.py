@@ -17,7 +17,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcpgpu.so")
-SOURCES = ["world.hip", "broadphase.hip", "narrowphase.hip", "solver.hip", "adjoint.hip"]
+SOURCES = ["world.hip", "broadphase.hip", "narrowphase.hip", "solver.hip", "adjoint.hip", "query.hip"]
 VARIANTS = {"64": ["-DCPGPU_NS=cpgpu64", "-DCPGPU_REAL=double"],
             "32": ["-DCPGPU_NS=cpgpu32", "-DCPGPU_REAL=float"]}
 UNITS = [(src, v) for v in VARIANTS for src in SOURCES] + [("dispatch.hip", None)]

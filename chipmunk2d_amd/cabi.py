@@ -14,6 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 BODY_DYNAMIC, BODY_KINEMATIC, BODY_STATIC = 0, 1, 2
 SHAPE_CIRCLE, SHAPE_SEGMENT, SHAPE_POLY = 0, 1, 2
 JOINT_PIVOT, JOINT_DAMPED_SPRING, JOINT_SLIDE, JOINT_GEAR = 0, 1, 2, 3
+JOINT_PIN, JOINT_GROOVE, JOINT_ROTARY_LIMIT, JOINT_RATCHET, JOINT_SIMPLE_MOTOR, JOINT_DAMPED_ROTARY_SPRING = \
+    4, 5, 6, 7, 8, 9
 ARB_FIRST_COLLISION, ARB_NORMAL, ARB_IGNORE, ARB_CACHED = 0, 1, 2, 3
 FP64, FP32 = 0, 1
 

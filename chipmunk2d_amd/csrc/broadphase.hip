@@ -39,6 +39,8 @@ __device__ __forceinline__ double double_of(unsigned long long u)
 	return __longlong_as_double((long long)u);
 }
 
+__global__ void k_zero_count(int32_t *count) { *count = 0; }
+
 __global__ void k_bounds_reset(unsigned long long *wb, int n_worlds)
 {
 	const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -793,8 +795,8 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 hipError_t broadphase_pairs(const Lbvh &dyn, const Lbvh &stat, const DevWorld &W, int2 *pairs, int cap,
                             int32_t *count, hipStream_t stream, KernelTimer *timer)
 {
-	HIP_TRY(hipMemsetAsync(count, 0, sizeof(int32_t), stream));
-	if (dyn.n == 0) return hipSuccess;
+	k_zero_count<<<1, 1, 0, stream>>>(count);   // a kernel, not a memset: the step may be captured in a graph
+	if (dyn.n == 0) return hipGetLastError();
 	TIMED(timer, "bvh_pairs", stream, {
 		k_pairs<<<blocks_for(dyn.n, 256), 256, 0, stream>>>(W, dyn.n, dyn.keys, dyn.leaf_shape, dyn.node, dyn.range,
 		                                                    dyn.box, dyn.world_root, stat.n, stat.leaf_shape,

@@ -243,6 +243,8 @@ struct Joint {
 	real k11, k12, k21, k22;
 	real bx, by, bias, nMass, jAx, jAy, jAcc, jnAcc, target_vrn, v_coef, iSum;
 	real bias_coef;
+	real clamp;   // groove: the groove end the anchor rests on (+1, -1), or 0 between them
+	real angle;   // ratchet: the angle of the last notch passed, carried between steps
 };
 
 struct DevWorld {

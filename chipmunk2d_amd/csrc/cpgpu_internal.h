@@ -132,6 +132,31 @@ hipError_t lbvh_build(Lbvh &t, const DevWorld &W, const int32_t *shapes, int n, 
 hipError_t broadphase_pairs(const Lbvh &dyn, const Lbvh &stat, const DevWorld &W, int2 *pairs, int cap,
                             int32_t *count, hipStream_t stream, KernelTimer *timer);
 
+// ---- spatial queries (query.hip) ----
+// the two trees of the last step as the queries walk them
+struct QueryTrees {
+	int nd, ns;                          // leaves of the dynamic and the static tree
+	const int32_t *droot, *sroot;        // per space: root node (-1: none)
+	const int32_t *dshape, *sshape;      // leaf -> shape
+	const PairNode *dnode, *snode;
+	const real4 *dbox, *sbox;
+};
+// a batch of queries: space index (NULL: 0), points or segment starts a [n][2]
+// (boxes [n][4] for box queries), segment ends b [n][2], the per-query scalar
+// (max distance or segment radius), filters (NULL: all)
+struct QueryIn {
+	const int32_t *space;
+	const double *a, *b, *scalar;
+	const cpgpuShapeFilter *filter;
+};
+// nearest point (segment == false) or first along a segment: shape_out [n], info_out [n][5]
+hipError_t query_nearest_or_first(const DevWorld &W, const QueryTrees &T, int n, const QueryIn &in, bool segment,
+                                  int32_t *shape_out, double *info_out, hipStream_t stream);
+// every hit: kind 0 point, 1 segment, 2 box; (query, shape) pairs and their five numbers appended
+// up to cap, the total in *count
+hipError_t query_all(const DevWorld &W, const QueryTrees &T, int n, const QueryIn &in, int kind, int2 *hits,
+                     double *info, int64_t cap, unsigned long long *count, hipStream_t stream);
+
 #define HIP_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
 
 } // namespace CPGPU_NS

@@ -42,7 +42,9 @@ constexpr int32_t CPGPU_CACHED_REF = 0x40000000;
 // n_prev_recs_cap bounds the records from above (the kernels read the count).
 hipError_t narrowphase_table(const DevWorld &W, const StepBuffers &B, int n_prev_recs_cap, hipStream_t stream,
                              KernelTimer *timer);
-// The narrowphase over the candidate pairs (after narrowphase_table).
+// The narrowphase over the candidate pairs (after narrowphase_table):
+// n_pairs_host bounds the pairs from above (the launch size); the kernels
+// read the count from n_pairs_dev.
 hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 *pairs, const int32_t *n_pairs_dev,
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer);
 

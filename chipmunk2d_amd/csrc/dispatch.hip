@@ -38,6 +38,12 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldSetBuckets(cpgpuWorld *world, int32_t n, const int32_t *max_bodies, const int32_t *threads); \
 	int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out); \
 	int cpgpuWorldSetGridSchedule(cpgpuWorld *world, int32_t schedule); \
+	int cpgpuWorldSetGraph(cpgpuWorld *world, int32_t enable); \
+	int cpgpuWorldQueryPointNearest(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points, const double *max_distance, const cpgpuShapeFilter *filter, int32_t *shape_out, double *info_out); \
+	int cpgpuWorldQuerySegmentFirst(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts, const double *ends, const double *radius, const cpgpuShapeFilter *filter, int32_t *shape_out, double *info_out); \
+	int cpgpuWorldQueryPointAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points, const double *max_distance, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits, double *info); \
+	int cpgpuWorldQuerySegmentAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts, const double *ends, const double *radius, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits, double *info); \
+	int cpgpuWorldQueryBBAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *boxes, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits); \
 	int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every); \
 	int cpgpuWorldTapeEnd(cpgpuWorld *world); \
 	int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v, const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0, double *d_m_inv, double *d_i_inv, double *d_u); \
@@ -62,7 +68,6 @@ inline int null_world()
 	return CPGPU_ERR_INVALID;
 }
 
-}
 } // namespace
 
 extern "C" {
@@ -223,6 +228,13 @@ int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_
 	return cpgpu64::cpgpuWorldAutotune(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), dt, trial_steps, threads_out);
 }
 
+int cpgpuWorldSetGraph(cpgpuWorld *world, int32_t enable)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldSetGraph(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), enable);
+	return cpgpu64::cpgpuWorldSetGraph(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), enable);
+}
+
 int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every)
 {
 	if (!world) return null_world();
@@ -257,6 +269,58 @@ int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodi
 	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetSolveCounts(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), contacts, bodies);
 	return cpgpu64::cpgpuWorldGetSolveCounts(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), contacts, bodies);
 }
+
+#define CPGPU_FORWARD(call64, call32) \
+	if (!world) return null_world(); \
+	if (is_fp32(world)) return call32; \
+	return call64
+
+#define W64 reinterpret_cast<cpgpu64::cpgpuWorld *>(world)
+#define W32 reinterpret_cast<cpgpu32::cpgpuWorld *>(world)
+
+int cpgpuWorldQueryPointNearest(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points,
+                                const double *max_distance, const cpgpuShapeFilter *filter, int32_t *shape_out,
+                                double *info_out)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldQueryPointNearest(W64, n, space, points, max_distance, filter, shape_out, info_out),
+	              cpgpu32::cpgpuWorldQueryPointNearest(W32, n, space, points, max_distance, filter, shape_out, info_out));
+}
+
+int cpgpuWorldQuerySegmentFirst(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts,
+                                const double *ends, const double *radius, const cpgpuShapeFilter *filter,
+                                int32_t *shape_out, double *info_out)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldQuerySegmentFirst(W64, n, space, starts, ends, radius, filter, shape_out, info_out),
+	              cpgpu32::cpgpuWorldQuerySegmentFirst(W32, n, space, starts, ends, radius, filter, shape_out, info_out));
+}
+
+int cpgpuWorldQueryPointAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points,
+                            const double *max_distance, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out,
+                            int32_t *hits, double *info)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldQueryPointAll(W64, n, space, points, max_distance, filter, cap, n_out, hits, info),
+	              cpgpu32::cpgpuWorldQueryPointAll(W32, n, space, points, max_distance, filter, cap, n_out, hits, info));
+}
+
+int cpgpuWorldQuerySegmentAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts,
+                              const double *ends, const double *radius, const cpgpuShapeFilter *filter, int64_t cap,
+                              int64_t *n_out, int32_t *hits, double *info)
+{
+	CPGPU_FORWARD(
+	    cpgpu64::cpgpuWorldQuerySegmentAll(W64, n, space, starts, ends, radius, filter, cap, n_out, hits, info),
+	    cpgpu32::cpgpuWorldQuerySegmentAll(W32, n, space, starts, ends, radius, filter, cap, n_out, hits, info));
+}
+
+int cpgpuWorldQueryBBAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *boxes,
+                         const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldQueryBBAll(W64, n, space, boxes, filter, cap, n_out, hits),
+	              cpgpu32::cpgpuWorldQueryBBAll(W32, n, space, boxes, filter, cap, n_out, hits));
+}
+
+#undef W64
+#undef W32
+#undef CPGPU_FORWARD
 
 const char *cpgpuGetLastError(void) { return cpgpu_last_error.c_str(); }
 const char *cpgpuVersion(void) { return "cpgpu 0.2 (gfx950; Chipmunk2D 7.0.3 cpSpaceStep semantics; FP64 and FP32 builds)"; }

@@ -114,11 +114,13 @@ __device__ __forceinline__ void write_record(const StepBuffers &B, const DevWorl
 	B.recs[k] = r;
 }
 
+// n_pairs bounds the pairs from above; the count itself is read from n_dev
+// (at most n_pairs of them were stored)
 __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const StepBuffers B, const int2 *pairs,
-                                                     int n_pairs)
+                                                     int n_pairs, const int32_t *n_dev)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n_pairs) return;
+	if (i >= min(n_pairs, *n_dev)) return;
 	const int2 pr = pairs[i];
 	const int ia = pr.x, ib = pr.y;
 	const uint64_t key = pair_key(ia, ib);
@@ -245,8 +247,9 @@ __device__ __forceinline__ int pair_class(const DevWorld &W, int2 pr)
 }
 
 __global__ void __launch_bounds__(256) k_pair_class_count(const DevWorld W, const int2 *pairs, int n,
-                                                          int32_t *counts)
+                                                          const int32_t *n_dev, int32_t *counts)
 {
+	n = min(n, *n_dev);
 	__shared__ int32_t h[PAIR_CLASSES];
 	if (threadIdx.x < PAIR_CLASSES) h[threadIdx.x] = 0;
 	__syncthreads();
@@ -272,9 +275,10 @@ __global__ void k_pair_class_scan(int32_t *counts)
 // each workgroup counts its pairs per class in LDS, reserves one range per
 // class, then writes its pairs into them
 __global__ void __launch_bounds__(256) k_pair_class_scatter(const DevWorld W, const int2 *pairs, int n,
-                                                            int32_t *cursors, int2 *out)
+                                                            const int32_t *n_dev, int32_t *cursors, int2 *out)
 {
 	__shared__ int32_t h[PAIR_CLASSES], base[PAIR_CLASSES];
+	n = min(n, *n_dev);
 	const int per_block = (n + gridDim.x - 1) / gridDim.x;
 	const int lo = blockIdx.x * per_block, hi = min(n, lo + per_block);
 	if (threadIdx.x < PAIR_CLASSES) h[threadIdx.x] = 0;
@@ -352,18 +356,20 @@ hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer)
 {
 	// the step's three counters are consecutive: records, solved arbiters, cached arbiters
-	HIP_TRY(hipMemsetAsync(B.n_recs, 0, 3 * sizeof(int32_t), stream));
+	// (cleared by a kernel: the step may be captured in a graph)
+	k_fill_i32<<<1, 32, 0, stream>>>(B.n_recs, 3, 0);
 	if (n_pairs_host > 0) {
 		const int g = std::min(blocks_for(n_pairs_host, 256), 1024);
 		TIMED(timer, "pair_classes", stream, {
 			k_fill_i32<<<1, 2 * PAIR_CLASSES, 0, stream>>>(B.type_counts, 2 * PAIR_CLASSES, 0);
-			k_pair_class_count<<<g, 256, 0, stream>>>(W, pairs, n_pairs_host, B.type_counts);
+			k_pair_class_count<<<g, 256, 0, stream>>>(W, pairs, n_pairs_host, n_pairs_dev, B.type_counts);
 			k_pair_class_scan<<<1, 64, 0, stream>>>(B.type_counts);
-			k_pair_class_scatter<<<g, 256, 0, stream>>>(W, pairs, n_pairs_host, B.type_counts + PAIR_CLASSES,
-			                                            B.pairs_by_type);
+			k_pair_class_scatter<<<g, 256, 0, stream>>>(W, pairs, n_pairs_host, n_pairs_dev,
+			                                            B.type_counts + PAIR_CLASSES, B.pairs_by_type);
 		});
 		TIMED(timer, "narrowphase", stream,
-		      k_narrowphase<<<blocks_for(n_pairs_host, 256), 256, 0, stream>>>(W, B, B.pairs_by_type, n_pairs_host));
+		      k_narrowphase<<<blocks_for(n_pairs_host, 256), 256, 0, stream>>>(W, B, B.pairs_by_type, n_pairs_host,
+		                                                                      n_pairs_dev));
 		HIP_TRY(hipGetLastError());
 	}
 	if (n_prev_recs_host > 0) {

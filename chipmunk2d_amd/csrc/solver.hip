@@ -248,7 +248,37 @@ __device__ __forceinline__ void arb_apply_impulse(const DevWorld &W, const BA &b
 	arb_apply_impulse(W, bodies, arb, arb.ba, arb.bb, is_dynamic(W, arb.ba), is_dynamic(W, arb.bb));
 }
 
-// ---- joints (cpPivotJoint.c:24-70, cpDampedSpring.c:24-77, cpSlideJoint.c:24-89, cpGearJoint.c:24-69) ----
+// ---- joints (cpPivotJoint.c:24-70, cpDampedSpring.c:24-77, cpSlideJoint.c:24-89, cpGearJoint.c:24-69,
+// cpPinJoint.c:24-71, cpGrooveJoint.c:24-93, cpRotaryLimitJoint.c:24-83, cpRatchetJoint.c:24-89,
+// cpSimpleMotor.c:24-67, cpDampedRotarySpring.c:24-72) ----
+
+// k_tensor (chipmunk_private.h:228-262): the inverse 2x2 effective mass of a point joint
+__device__ __forceinline__ void k_tensor(const BV &a, const BV &b, V2 r1, V2 r2, Joint &j)
+{
+	const real m_sum = a.m_inv + b.m_inv;
+	real k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
+	const real a_i_inv = a.i_inv;
+	const real r1xsq = r1.x * r1.x * a_i_inv;
+	const real r1ysq = r1.y * r1.y * a_i_inv;
+	const real r1nxy = -r1.x * r1.y * a_i_inv;
+	k11 += r1ysq; k12 += r1nxy; k21 += r1nxy; k22 += r1xsq;
+	const real b_i_inv = b.i_inv;
+	const real r2xsq = r2.x * r2.x * b_i_inv;
+	const real r2ysq = r2.y * r2.y * b_i_inv;
+	const real r2nxy = -r2.x * r2.y * b_i_inv;
+	k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
+	const real det = k11 * k22 - k12 * k21;
+	const real det_inv = 1.0 / det;
+	j.k11 = k22 * det_inv; j.k12 = -k12 * det_inv;
+	j.k21 = -k21 * det_inv; j.k22 = k11 * det_inv;
+}
+
+// an angular impulse jj between the bodies (rotary limit, ratchet, simple motor)
+__device__ __forceinline__ void apply_angular(BV &a, BV &b, real jj)
+{
+	a.w -= jj * a.i_inv;
+	b.w += jj * b.i_inv;
+}
 
 __device__ void joint_prestep(const DevWorld &W, Joint &j)
 {
@@ -263,22 +293,7 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 		const V2 r1 = xvect(Ta, vsub(anchorA, v2(cogA.x, cogA.y)));
 		const V2 r2 = xvect(Tb, vsub(anchorB, v2(cogB.x, cogB.y)));
 		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
-		const real m_sum = a.m_inv + b.m_inv;
-		real k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
-		const real a_i_inv = a.i_inv;
-		const real r1xsq = r1.x * r1.x * a_i_inv;
-		const real r1ysq = r1.y * r1.y * a_i_inv;
-		const real r1nxy = -r1.x * r1.y * a_i_inv;
-		k11 += r1ysq; k12 += r1nxy; k21 += r1nxy; k22 += r1xsq;
-		const real b_i_inv = b.i_inv;
-		const real r2xsq = r2.x * r2.x * b_i_inv;
-		const real r2ysq = r2.y * r2.y * b_i_inv;
-		const real r2nxy = -r2.x * r2.y * b_i_inv;
-		k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
-		const real det = k11 * k22 - k12 * k21;
-		const real det_inv = 1.0 / det;
-		j.k11 = k22 * det_inv; j.k12 = -k12 * det_inv;
-		j.k21 = -k21 * det_inv; j.k22 = k11 * det_inv;
+		k_tensor(a, b, r1, r2, j);
 		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(v2(pA.x, pA.y), r1));
 		const V2 bias = vclamp(vmult(delta, -bc / dt), j.maxBias);
 		j.bx = bias.x; j.by = bias.y;
@@ -316,11 +331,71 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 		j.nMass = 1.0 / k_scalar(a, b, r1, r2, n);
 		const real maxBias = j.maxBias;
 		j.bias = fclampc(-bc * pdist / dt, -maxBias, maxBias);
-	} else {
+	} else if (j.type == CPGPU_JOINT_GEAR) {
 		const real phase = j.P[0], ratio = j.P[1], ratio_inv = j.P[2];
 		j.iSum = 1.0 / (a.i_inv * ratio_inv + ratio * b.i_inv);
 		const real maxBias = j.maxBias;
 		j.bias = fclampc(-bc * (W.b_a[j.b] * ratio - W.b_a[j.a] - phase) / dt, -maxBias, maxBias);
+	} else if (j.type == CPGPU_JOINT_PIN) {
+		const V2 r1 = xvect(Ta, vsub(anchorA, v2(cogA.x, cogA.y)));
+		const V2 r2 = xvect(Tb, vsub(anchorB, v2(cogB.x, cogB.y)));
+		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
+		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(v2(pA.x, pA.y), r1));
+		const real dist = vlength(delta);
+		const V2 n = vmult(delta, 1.0 / (dist != 0.0 ? dist : __builtin_inf()));
+		j.nx = n.x; j.ny = n.y;
+		j.nMass = 1.0 / k_scalar(a, b, r1, r2, n);
+		const real maxBias = j.maxBias;
+		j.bias = fclampc(-bc * (dist - j.P[4]) / dt, -maxBias, maxBias);
+	} else if (j.type == CPGPU_JOINT_GROOVE) {
+		// P: the groove's ends on body A, the anchor on body B, the groove's normal
+		const V2 ta = xpoint(Ta, v2(j.P[0], j.P[1])), tb = xpoint(Ta, v2(j.P[2], j.P[3]));
+		const V2 n = xvect(Ta, v2(j.P[6], j.P[7]));
+		const real d = vdot(ta, n);
+		j.nx = n.x; j.ny = n.y;
+		const V2 r2 = xvect(Tb, vsub(v2(j.P[4], j.P[5]), v2(cogB.x, cogB.y)));
+		const V2 p_a = v2(pA.x, pA.y);
+		const real td = vcross(vadd(v2(pB.x, pB.y), r2), n);
+		V2 r1;
+		if (td <= vcross(ta, n)) { j.clamp = 1.0; r1 = vsub(ta, p_a); }
+		else if (td >= vcross(tb, n)) { j.clamp = -1.0; r1 = vsub(tb, p_a); }
+		else { j.clamp = 0.0; r1 = vsub(vadd(vmult(vperp(n), -td), vmult(n, d)), p_a); }
+		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
+		k_tensor(a, b, r1, r2, j);
+		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(p_a, r1));
+		const V2 bias = vclamp(vmult(delta, -bc / dt), j.maxBias);
+		j.bx = bias.x; j.by = bias.y;
+	} else if (j.type == CPGPU_JOINT_ROTARY_LIMIT || j.type == CPGPU_JOINT_RATCHET) {
+		const real delta = W.b_a[j.b] - W.b_a[j.a];
+		real pdist = 0.0;
+		if (j.type == CPGPU_JOINT_ROTARY_LIMIT) {
+			if (delta > j.P[1]) pdist = j.P[1] - delta;
+			else if (delta < j.P[0]) pdist = j.P[0] - delta;
+		} else {
+			const real phase = j.P[0], ratchet = j.P[1];
+			const real diff = j.angle - delta;
+			if (diff * ratchet > 0.0) pdist = diff;
+			else j.angle = floor((delta - phase) / ratchet) * ratchet + phase;
+		}
+		j.iSum = 1.0 / (a.i_inv + b.i_inv);
+		const real maxBias = j.maxBias;
+		j.bias = fclampc(-bc * pdist / dt, -maxBias, maxBias);
+		if (!j.bias) j.jAcc = 0.0;
+	} else if (j.type == CPGPU_JOINT_SIMPLE_MOTOR) {
+		j.iSum = 1.0 / (a.i_inv + b.i_inv);
+	} else {
+		// damped rotary spring, the default torque (relativeAngle - restAngle) stiffness
+		const real restAngle = j.P[0], stiffness = j.P[1], damping = j.P[2];
+		const real moment = a.i_inv + b.i_inv;
+		j.iSum = 1.0 / moment;
+		j.v_coef = 1.0 - cp_exp(-damping * dt * moment);
+		j.target_vrn = 0.0;
+		const real j_spring = ((W.b_a[j.a] - W.b_a[j.b]) - restAngle) * stiffness * dt;
+		j.jAcc = j_spring;
+		a.w -= j_spring * a.i_inv;
+		b.w += j_spring * b.i_inv;
+		if (is_dynamic(W, j.a)) store_bv(W, j.a, a);
+		if (is_dynamic(W, j.b)) store_bv(W, j.b, b);
 	}
 }
 
@@ -331,17 +406,19 @@ template <class BA>
 __device__ void joint_apply_cached(const BA &bodies, const Joint &j, real dt_coef, int ia, int ib, bool da, bool db)
 {
 	BV a = bodies.load(ia), b = bodies.load(ib);
-	if (j.type == CPGPU_JOINT_DAMPED_SPRING) return;
+	if (j.type == CPGPU_JOINT_DAMPED_SPRING || j.type == CPGPU_JOINT_DAMPED_ROTARY_SPRING) return;
 	const V2 r1 = v2(j.r1x, j.r1y), r2 = v2(j.r2x, j.r2y);
-	if (j.type == CPGPU_JOINT_PIVOT) {
+	if (j.type == CPGPU_JOINT_PIVOT || j.type == CPGPU_JOINT_GROOVE) {
 		apply_impulses(a, b, r1, r2, vmult(v2(j.jAx, j.jAy), dt_coef));
-	} else if (j.type == CPGPU_JOINT_SLIDE) {
+	} else if (j.type == CPGPU_JOINT_SLIDE || j.type == CPGPU_JOINT_PIN) {
 		apply_impulses(a, b, r1, r2, vmult(v2(j.nx, j.ny), j.jnAcc * dt_coef));
-	} else {
+	} else if (j.type == CPGPU_JOINT_GEAR) {
 		const real ratio_inv = j.P[2];
 		const real jj = j.jAcc * dt_coef;
 		a.w -= jj * a.i_inv * ratio_inv;
 		b.w += jj * b.i_inv;
+	} else {
+		apply_angular(a, b, j.jAcc * dt_coef);
 	}
 	if (da) bodies.store(ia, a);
 	if (db) bodies.store(ib, b);
@@ -385,7 +462,7 @@ __device__ void joint_apply_impulse(const BA &bodies, Joint &j, real dt, int ia,
 		j.jnAcc = fclampc(jnOld + jn, -j.maxForce * dt, 0.0);
 		jn = j.jnAcc - jnOld;
 		apply_impulses(a, b, r1, r2, vmult(n, jn));
-	} else {
+	} else if (j.type == CPGPU_JOINT_GEAR) {
 		const real ratio = j.P[1], ratio_inv = j.P[2];
 		const real wr = b.w * ratio - a.w;
 		const real jMax = j.maxForce * dt;
@@ -395,6 +472,56 @@ __device__ void joint_apply_impulse(const BA &bodies, Joint &j, real dt, int ia,
 		jj = j.jAcc - jOld;
 		a.w -= jj * a.i_inv * ratio_inv;
 		b.w += jj * b.i_inv;
+	} else if (j.type == CPGPU_JOINT_PIN) {
+		const V2 n = v2(j.nx, j.ny);
+		const real vrn = vdot(rel_vel(a, b, r1, r2), n);
+		const real jnMax = j.maxForce * dt;
+		real jn = (j.bias - vrn) * j.nMass;
+		const real jnOld = j.jnAcc;
+		j.jnAcc = fclampc(jnOld + jn, -jnMax, jnMax);
+		jn = j.jnAcc - jnOld;
+		apply_impulses(a, b, r1, r2, vmult(n, jn));
+	} else if (j.type == CPGPU_JOINT_GROOVE) {
+		const V2 vr = rel_vel(a, b, r1, r2);
+		const V2 d = vsub(v2(j.bx, j.by), vr);
+		const V2 jj = v2(d.x * j.k11 + d.y * j.k12, d.x * j.k21 + d.y * j.k22);
+		const V2 jOld = v2(j.jAx, j.jAy);
+		// grooveConstrain (cpGrooveJoint.c:74-79)
+		const V2 jc = vadd(jOld, jj), n = v2(j.nx, j.ny);
+		const V2 jClamp = (j.clamp * vcross(jc, n) > 0.0) ? jc : vmult(n, vdot(jc, n) / vdot(n, n));
+		const V2 jAcc = vclamp(jClamp, j.maxForce * dt);
+		j.jAx = jAcc.x; j.jAy = jAcc.y;
+		apply_impulses(a, b, r1, r2, vsub(jAcc, jOld));
+	} else if (j.type == CPGPU_JOINT_ROTARY_LIMIT || j.type == CPGPU_JOINT_RATCHET) {
+		if (!j.bias) return;
+		const real wr = b.w - a.w;
+		const real jMax = j.maxForce * dt;
+		const real jj = -(j.bias + wr) * j.iSum;
+		const real jOld = j.jAcc;
+		if (j.type == CPGPU_JOINT_RATCHET) {
+			const real ratchet = j.P[1];
+			j.jAcc = fclampc((jOld + jj) * ratchet, 0.0, jMax * fabs(ratchet)) / ratchet;
+		} else if (j.bias < 0.0) {
+			j.jAcc = fclampc(jOld + jj, 0.0, jMax);
+		} else {
+			j.jAcc = fclampc(jOld + jj, -jMax, 0.0);
+		}
+		apply_angular(a, b, j.jAcc - jOld);
+	} else if (j.type == CPGPU_JOINT_SIMPLE_MOTOR) {
+		const real wr = b.w - a.w + j.P[0];
+		const real jMax = j.maxForce * dt;
+		const real jj = -wr * j.iSum;
+		const real jOld = j.jAcc;
+		j.jAcc = fclampc(jOld + jj, -jMax, jMax);
+		apply_angular(a, b, j.jAcc - jOld);
+	} else {
+		const real wrn = a.w - b.w;
+		const real w_damp = (j.target_vrn - wrn) * j.v_coef;
+		j.target_vrn = wrn + w_damp;
+		const real j_damp = w_damp * j.iSum;
+		j.jAcc += j_damp;
+		a.w += j_damp * a.i_inv;
+		b.w -= j_damp * b.i_inv;
 	}
 	if (da) bodies.store(ia, a);
 	if (db) bodies.store(ib, b);

@@ -239,6 +239,7 @@ struct cpgpuWorld {
 	real *d_joint_bias_coef = nullptr;
 	double last_joint_dt = -1.0;
 	std::vector<double> joint_error_bias;
+	bool other_joints = false;   // a joint the adjoint has no reverse for (pin, groove, rotary ones)
 	// broadphase
 	Lbvh dyn_tree, static_tree;
 	RadixSortBuffers sortbuf;
@@ -327,6 +328,18 @@ struct cpgpuWorld {
 	int64_t total_pairs = 0;
 	bool pending = false;
 	KernelTimer timer;
+	// steps captured in HIP graphs (cpgpuWorldSetGraph): one executable graph per arbiter slot
+	bool graph_on = false;
+	bool graph_dirty = true;
+	double graph_dt = 0.0;
+	hipGraphExec_t gexec[2] = {nullptr, nullptr};
+	hipEvent_t graph_done = nullptr;        // after the last captured step
+	double *d_curr_dt = nullptr;            // per world: the previous step's dt
+	unsigned long long *d_graph_pairs = nullptr;   // candidate pairs summed over captured steps
+	long graph_max_pairs = 0;
+	int graph_rec_cap = 0, graph_cached_cap = 0;
+	int64_t graph_steps = 0;
+	bool dyn_tree_built = false;            // the dynamic tree over the shapes' cached boxes
 };
 
 namespace {
@@ -434,7 +447,7 @@ int validate(const cpgpuWorldDesc *d)
 	}
 	for (int c = 0; c < d->n_constraints; c++) {
 		const int t = d->cons_type[c];
-		if (t < 0 || t > 3) return fail(CPGPU_ERR_INVALID, "constraint type out of range");
+		if (t < 0 || t >= CPGPU_JOINT_TYPES) return fail(CPGPU_ERR_INVALID, "constraint type out of range");
 		if (d->cons_a[c] < 0 || d->cons_a[c] >= d->n_bodies || d->cons_b[c] < 0 || d->cons_b[c] >= d->n_bodies)
 			return fail(CPGPU_ERR_INVALID, "constraint body index out of range");
 		if (body_world[d->cons_a[c]] != body_world[d->cons_b[c]])
@@ -473,6 +486,7 @@ int sync_world(cpgpuWorld *w)
 	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
 	w->timer.collect();
 	w->pending = false;
+	if (w->graph_on && w->graph_steps > 0) w->last_pairs = w->h_counts[6];   // a captured step's candidate pairs
 	return check_device_error(w);
 }
 
@@ -656,6 +670,8 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 			for (int k = 0; k < 8; k++) j.P[k] = (real)d->cons_params[8 * c + k];
 			const double j0 = d->cons_jacc ? d->cons_jacc[2 * c] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * c + 1] : 0.0;
 			j.jAx = (real)j0; j.jAy = (real)j1; j.jAcc = (real)j0; j.jnAcc = (real)j0;
+			j.angle = (real)j1;
+			if (j.type > CPGPU_JOINT_GEAR) w->other_joints = true;
 			w->joint_error_bias[c] = d->cons_error_bias[c];
 			ba[c] = d->body_type[j.a] == CPGPU_BODY_DYNAMIC ? j.a : -1;
 			bbv[c] = d->body_type[j.b] == CPGPU_BODY_DYNAMIC ? j.b : -1;
@@ -828,6 +844,11 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys,
 	                 w->cbody, w->cprio, w->ccol};
 	for (void *p : extra) if (p) (void)hipFree(p);
+	for (int slot = 0; slot < 2; slot++)
+		if (w->gexec[slot]) (void)hipGraphExecDestroy(w->gexec[slot]);
+	if (w->graph_done) (void)hipEventDestroy(w->graph_done);
+	if (w->d_curr_dt) (void)hipFree(w->d_curr_dt);
+	if (w->d_graph_pairs) (void)hipFree(w->d_graph_pairs);
 	if (w->h_ws) (void)hipHostFree(w->h_ws);
 	if (w->h_counts) (void)hipHostFree(w->h_counts);
 	tape_clear(w);
@@ -919,21 +940,21 @@ static hipError_t set_joint_bias(cpgpuWorld *w, double dt)
 	return hipSuccess;
 }
 
-int cpgpuWorldStep(cpgpuWorld *w, double dt)
+// The per-world scalars of this step (cpSpaceStep.c:341-344, 383-384, 399, 407),
+// pow on the host.  advance: this is the step itself (stamp++, dt becomes the
+// space's curr_dt); otherwise the scalars a captured step starts from (the
+// stamp and dt of the last step, advanced on the device by k_advance_worlds).
+static void fill_world_steps(cpgpuWorld *w, double dt, bool advance)
 {
-	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
-	if (dt == 0.0) return CPGPU_OK;   // cpSpaceStep.c:339
-	DevWorld &W = w->W;
-	KernelTimer *timer = w->timer.enabled ? &w->timer : nullptr;
-
-	// per-world scalars of this step (cpSpaceStep.c:341-344, 383-384, 399, 407); pow on the host
 	for (int k = 0; k < w->nw; k++) {
 		cpgpuSpaceParams &s = w->sp[k];
-		s.stamp++;
 		const double prev_dt = s.curr_dt;
-		s.curr_dt = dt;
+		if (advance) {
+			s.stamp++;
+			s.curr_dt = dt;
+		}
 		WorldStep &ws = w->h_ws[k];
-		ws.dt = dt;
+		ws.dt = advance ? dt : prev_dt;
 		ws.dt_coef = (prev_dt == 0.0 ? 0.0 : dt / prev_dt);
 		ws.damping = pow(s.damping, dt);
 		ws.bias_coef = 1.0 - pow(s.collision_bias, dt);
@@ -943,6 +964,121 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		ws.stamp = s.stamp;
 		ws.persistence = s.collision_persistence;
 	}
+}
+
+// Integration, shape caching and the broadphase of a step (cpSpaceStep.c:364-372):
+// launch sizes depend only on the world, not on this step's counts.
+static hipError_t enqueue_front(cpgpuWorld *w, KernelTimer *timer)
+{
+	DevWorld &W = w->W;
+	// integrate positions, cache the dynamic shapes, update leaf boxes
+	if (w->n_dyn_bodies > 0)
+		TIMED(timer, "integrate_positions", w->stream,
+		      k_update_positions<<<blocks_for(w->n_dyn_bodies, 256), 256, 0, w->stream>>>(W, w->d_dyn_bodies,
+		                                                                                 w->n_dyn_bodies));
+	if (w->n_dyn_shapes > 0)
+		TIMED(timer, "cache_shapes", w->stream,
+		      k_cache_shapes<<<blocks_for(w->n_dyn_shapes, 256), 256, 0, w->stream>>>(W, w->d_dyn_shapes,
+		                                                                             w->n_dyn_shapes, 1, 0));
+	HIP_TRY(hipGetLastError());
+
+	// the large worlds' strips from this step's positions, on an auxiliary stream beside the
+	// broadphase and the narrowphase
+	const int cur = w->cur;
+	if (w->n_large_worlds > 0 && w->n_large_bodies > 0) {
+		hipEvent_t strips_fork = w->fork_join[CPGPU_MAX_BUCKETS + 1], strips_join = w->fork_join[CPGPU_MAX_BUCKETS + 2];
+		hipStream_t aux = w->aux[CPGPU_MAX_BUCKETS - 1];
+		HIP_TRY(hipEventRecord(strips_fork, w->stream));
+		HIP_TRY(hipStreamWaitEvent(aux, strips_fork, 0));
+		const SolveBuffers S0 = solve_buffers(w, w->act[cur], nullptr, w->colour, w->order, w->warb_off);
+		HIP_TRY(solver_strips(W, S0, aux, timer));
+		HIP_TRY(hipEventRecord(strips_join, aux));
+	}
+
+	// broadphase
+	HIP_TRY(lbvh_build(w->dyn_tree, W, w->d_dyn_shapes, w->n_dyn_shapes, w->nw, w->sortbuf, w->d_world_bounds,
+	                   w->stream, timer));
+	return broadphase_pairs(w->dyn_tree, w->static_tree, W, w->d_pairs, w->pair_cap, w->d_counters, w->stream, timer);
+}
+
+// The narrowphase, arbiter persistence and the solve of a step
+// (cpSpaceStep.c:372-440).  n_pairs and n_prev_recs bound this step's candidate
+// pairs and last step's records from above: the kernels read the counts.
+static hipError_t enqueue_back(cpgpuWorld *w, int n_pairs, int n_prev_recs, KernelTimer *timer)
+{
+	DevWorld &W = w->W;
+	const int prev = w->cur ^ 1, cur = w->cur;
+	StepBuffers B;
+	B.tkeys = w->tkeys; B.tvals = w->tvals; B.tmask = w->tcap - 1;
+	B.prev_recs = w->recs[prev]; B.n_prev_recs = w->d_step_counts + 3 * prev + 0;
+	B.seen = w->seen;
+	B.prev_act = w->act[prev]; B.prev_cached = w->cached[prev];
+	B.recs = w->recs[cur]; B.n_recs = w->d_step_counts + 3 * cur + 0; B.cap_recs = w->rec_cap[cur];   // n_recs, n_act and
+	// n_cached are consecutive (narrowphase_step clears the three at once)
+	B.act = w->act[cur]; B.n_act = w->d_step_counts + 3 * cur + 1; B.cap_act = w->act_cap[cur];
+	B.cached = w->cached[cur]; B.n_cached = w->d_step_counts + 3 * cur + 2; B.cap_cached = w->cached_cap[cur];
+	B.pairs_by_type = w->d_pairs_by_type;
+	B.type_counts = w->d_type_counts;
+	HIP_TRY(narrowphase_table(W, B, n_prev_recs, w->stream, timer));
+	HIP_TRY(narrowphase_step(W, B, w->d_pairs, w->d_counters, n_pairs, n_prev_recs, w->stream, timer));
+
+	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
+	SolveBuffers S = solve_buffers(w, w->act[cur], B.n_act, w->colour, w->order, w->warb_off);
+	if (w->n_large_worlds > 0 && w->n_large_bodies > 0)
+		HIP_TRY(hipStreamWaitEvent(w->stream, w->fork_join[CPGPU_MAX_BUCKETS + 2], 0));
+	if (grouped) {
+		// the solved arbiters by world and grid unit; launches sized for the capacity (at most one solved
+		// arbiter per candidate pair), the kernels read the count
+		HIP_TRY(solver_group(W, S, std::max(n_pairs, 1), w->nw, w->n_large_worlds > 0, w->sortbuf, w->stream, timer));
+	}
+	return solver_step(W, S, w->buckets, w->n_buckets, w->n_large_worlds > 0 ? w->solver_grid : 0, w->stream, w->aux,
+	                   w->fork_join, timer);
+}
+
+// Buffers of a step's outputs in arbiter slot `slot`: records, solved and
+// cached arbiters (with the colouring's buffers and the pairs grouped by type
+// beside the solved ones), and the table over last step's prev_recs records.
+static hipError_t reserve_step(cpgpuWorld *w, int slot, long recs, long act, long cached, long prev_recs)
+{
+	HIP_TRY(regrow(w->recs[slot], w->rec_cap[slot], recs));
+	HIP_TRY(regrow(w->act[slot], w->act_cap[slot], act));
+	HIP_TRY(regrow(w->cached[slot], w->cached_cap[slot], cached));
+	const int tcap = pow2_at_least(2L * prev_recs + 64);
+	if (tcap > w->tcap) {
+		if (w->tkeys) (void)hipFree(w->tkeys);
+		if (w->tvals) (void)hipFree(w->tvals);
+		w->tkeys = nullptr; w->tvals = nullptr; w->tcap = 0;
+		HIP_TRY(dalloc(w->tkeys, tcap));
+		HIP_TRY(dalloc(w->tvals, tcap));
+		w->tcap = tcap;
+	}
+	HIP_TRY(regrow(w->seen, w->seen_cap, prev_recs + 1));
+	if (w->act_cap[slot] > w->colour_cap) {
+		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->cbody, w->cprio, w->ccol};
+		for (void *p : old) if (p) (void)hipFree(p);
+		w->colour_cap = w->act_cap[slot];
+		HIP_TRY(dalloc(w->colour, w->colour_cap));
+		HIP_TRY(dalloc(w->order, w->colour_cap));
+		HIP_TRY(dalloc(w->arb_by_world, w->colour_cap));
+		HIP_TRY(dalloc(w->world_keys, w->colour_cap));
+		HIP_TRY(dalloc(w->cbody, w->colour_cap));
+		HIP_TRY(dalloc(w->cprio, w->colour_cap));
+		HIP_TRY(dalloc(w->ccol, w->colour_cap));
+	}
+	return regrow(w->d_pairs_by_type, w->pairs_by_type_cap, act);
+}
+
+static int graph_step(cpgpuWorld *w, double dt);
+
+int cpgpuWorldStep(cpgpuWorld *w, double dt)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (dt == 0.0) return CPGPU_OK;   // cpSpaceStep.c:339
+	if (w->graph_on) return graph_step(w, dt);
+	DevWorld &W = w->W;
+	KernelTimer *timer = w->timer.enabled ? &w->timer : nullptr;
+
+	fill_world_steps(w, dt, true);
 	HIP_OR_FAIL(hipMemcpyAsync(w->d_ws, w->h_ws, sizeof(WorldStep) * w->nw, hipMemcpyHostToDevice, w->stream));
 	if (w->tape.on) {
 		if (w->tape.steps > 0 && w->tape.steps % w->tape.every == 0) {
@@ -962,36 +1098,9 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 		HIP_OR_FAIL(hipMemcpyAsync(jk, W.joints, sizeof(Joint) * (size_t)w->nc, hipMemcpyDeviceToDevice, w->stream));
 	}
 
-	// integrate positions, cache the dynamic shapes, update leaf boxes
-	if (w->n_dyn_bodies > 0)
-		TIMED(timer, "integrate_positions", w->stream,
-		      k_update_positions<<<blocks_for(w->n_dyn_bodies, 256), 256, 0, w->stream>>>(W, w->d_dyn_bodies,
-		                                                                                 w->n_dyn_bodies));
-	if (w->n_dyn_shapes > 0)
-		TIMED(timer, "cache_shapes", w->stream,
-		      k_cache_shapes<<<blocks_for(w->n_dyn_shapes, 256), 256, 0, w->stream>>>(W, w->d_dyn_shapes,
-		                                                                             w->n_dyn_shapes, 1, 0));
-	HIP_OR_FAIL(hipGetLastError());
-
-	// the large worlds' strips from this step's positions, on an auxiliary stream beside the
-	// broadphase and the narrowphase
-	const int prev = w->cur ^ 1, cur = w->cur;
-	const bool strips = w->n_large_worlds > 0 && w->n_large_bodies > 0;
-	hipEvent_t strips_fork = w->fork_join[CPGPU_MAX_BUCKETS + 1], strips_join = w->fork_join[CPGPU_MAX_BUCKETS + 2];
-	if (strips) {
-		hipStream_t aux = w->aux[CPGPU_MAX_BUCKETS - 1];
-		HIP_OR_FAIL(hipEventRecord(strips_fork, w->stream));
-		HIP_OR_FAIL(hipStreamWaitEvent(aux, strips_fork, 0));
-		const SolveBuffers S0 = solve_buffers(w, w->act[cur], nullptr, w->colour, w->order, w->warb_off);
-		HIP_OR_FAIL(solver_strips(W, S0, aux, timer));
-		HIP_OR_FAIL(hipEventRecord(strips_join, aux));
-	}
-
-	// broadphase (one sync inside the radix sort)
-	HIP_OR_FAIL(lbvh_build(w->dyn_tree, W, w->d_dyn_shapes, w->n_dyn_shapes, w->nw, w->sortbuf, w->d_world_bounds,
-	                       w->stream, timer));
-	HIP_OR_FAIL(broadphase_pairs(w->dyn_tree, w->static_tree, W, w->d_pairs, w->pair_cap, w->d_counters, w->stream,
-	                             timer));
+	HIP_OR_FAIL(enqueue_front(w, timer));
+	w->dyn_tree_built = true;
+	// the pair count, read back to size this step's buffers
 	int rc = sync_world(w);
 	if (rc != CPGPU_OK) return rc;
 	int n_pairs = w->h_counts[6];
@@ -1007,65 +1116,19 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	w->total_pairs += n_pairs;
 
 	// capacities of this step's outputs from last step's counts
+	const int prev = w->cur ^ 1, cur = w->cur;
 	const int n_prev_recs = w->steps > 0 ? w->h_counts[3 * prev + 0] : 0;
 	const int n_prev_act = w->steps > 0 ? w->h_counts[3 * prev + 1] : 0;
 	const int n_prev_cached = w->steps > 0 ? w->h_counts[3 * prev + 2] : 0;
-	HIP_OR_FAIL(regrow(w->recs[cur], w->rec_cap[cur], (long)n_pairs + n_prev_recs + 1));
-	HIP_OR_FAIL(regrow(w->act[cur], w->act_cap[cur], (long)n_pairs + 1));
-	HIP_OR_FAIL(regrow(w->cached[cur], w->cached_cap[cur], (long)n_prev_act + n_prev_cached + 1));
-	const int tcap = pow2_at_least(2L * n_prev_recs + 64);
-	if (tcap > w->tcap) {
-		if (w->tkeys) (void)hipFree(w->tkeys);
-		if (w->tvals) (void)hipFree(w->tvals);
-		HIP_OR_FAIL(dalloc(w->tkeys, tcap));
-		HIP_OR_FAIL(dalloc(w->tvals, tcap));
-		w->tcap = tcap;
-	}
-	HIP_OR_FAIL(regrow(w->seen, w->seen_cap, (long)n_prev_recs + 1));
-	if (w->act_cap[cur] > w->colour_cap) {
-		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->cbody, w->cprio, w->ccol};
-		for (void *p : old) if (p) (void)hipFree(p);
-		w->colour_cap = w->act_cap[cur];
-		HIP_OR_FAIL(dalloc(w->colour, w->colour_cap));
-		HIP_OR_FAIL(dalloc(w->order, w->colour_cap));
-		HIP_OR_FAIL(dalloc(w->arb_by_world, w->colour_cap));
-		HIP_OR_FAIL(dalloc(w->world_keys, w->colour_cap));
-		HIP_OR_FAIL(dalloc(w->cbody, w->colour_cap));
-		HIP_OR_FAIL(dalloc(w->cprio, w->colour_cap));
-		HIP_OR_FAIL(dalloc(w->ccol, w->colour_cap));
-	}
+	HIP_OR_FAIL(reserve_step(w, cur, (long)n_pairs + n_prev_recs + 1, (long)n_pairs + 1, (long)n_prev_act + n_prev_cached + 1,
+	                         n_prev_recs));
 	if (w->steps == 0) HIP_OR_FAIL(hipMemsetAsync(w->d_step_counts + 3 * prev, 0, 3 * sizeof(int32_t), w->stream));
-
-	StepBuffers B;
-	B.tkeys = w->tkeys; B.tvals = w->tvals; B.tmask = w->tcap - 1;
-	B.prev_recs = w->recs[prev]; B.n_prev_recs = w->d_step_counts + 3 * prev + 0;
-	B.seen = w->seen;
-	B.prev_act = w->act[prev]; B.prev_cached = w->cached[prev];
-	B.recs = w->recs[cur]; B.n_recs = w->d_step_counts + 3 * cur + 0; B.cap_recs = w->rec_cap[cur];   // n_recs, n_act and
-	// n_cached are consecutive (narrowphase_step clears the three at once)
-	B.act = w->act[cur]; B.n_act = w->d_step_counts + 3 * cur + 1; B.cap_act = w->act_cap[cur];
-	B.cached = w->cached[cur]; B.n_cached = w->d_step_counts + 3 * cur + 2; B.cap_cached = w->cached_cap[cur];
-	HIP_OR_FAIL(regrow(w->d_pairs_by_type, w->pairs_by_type_cap, (long)n_pairs + 1));
-	B.pairs_by_type = w->d_pairs_by_type;
-	B.type_counts = w->d_type_counts;
-	HIP_OR_FAIL(narrowphase_table(W, B, n_prev_recs, w->stream, timer));
-	HIP_OR_FAIL(narrowphase_step(W, B, w->d_pairs, w->d_counters, n_pairs, n_prev_recs, w->stream, timer));
-
-	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
-	SolveBuffers S = solve_buffers(w, w->act[cur], B.n_act, w->colour, w->order, w->warb_off);
-	if (strips) HIP_OR_FAIL(hipStreamWaitEvent(w->stream, strips_join, 0));
-	if (grouped) {
-		// the solved arbiters by world and grid unit; launches sized for the capacity (at most one solved
-		// arbiter per candidate pair), the kernels read the count
-		const int n_act_cap = std::max(n_pairs, 1);
-		HIP_OR_FAIL(solver_group(W, S, n_act_cap, w->nw, w->n_large_worlds > 0, w->sortbuf, w->stream, timer));
-	}
-	HIP_OR_FAIL(solver_step(W, S, w->buckets, w->n_buckets, w->n_large_worlds > 0 ? w->solver_grid : 0,
-	                        w->stream, w->aux, w->fork_join, timer));
+	HIP_OR_FAIL(enqueue_back(w, n_pairs, n_prev_recs, timer));
 
 	if (w->tape.on) {
 		// record the solved arbiters (their contacts learn where they were recorded)
-		HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, B.n_act, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
+		const int32_t *n_act_dev = w->d_step_counts + 3 * cur + 1;
+		HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 10, n_act_dev, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
 		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
 		const int n_act = w->h_counts[10];
 		HIP_OR_FAIL(tape_reserve(w, w->tape.n + n_act));
@@ -1079,6 +1142,156 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	w->cur ^= 1;
 	w->steps++;
 	w->pending = true;
+	return CPGPU_OK;
+}
+
+// ---- steps captured in HIP graphs (cpgpuWorldSetGraph) ----
+
+namespace {
+
+// The per-step scalars of cpSpaceStep (cpSpaceStep.c:341-344) advanced on the
+// device for a captured step: stamp + 1, dt_coef = dt / the previous dt (0
+// after a first step), and dt, whose pow scalars the host wrote once.
+__global__ void k_advance_worlds(WorldStep *ws, double *curr_dt, int n, double dt)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n) return;
+	const double prev = curr_dt[k];
+	ws[k].dt = (real)dt;
+	ws[k].dt_coef = (real)(prev == 0.0 ? 0.0 : dt / prev);
+	ws[k].stamp += 1u;
+	curr_dt[k] = dt;
+}
+
+// A captured step sizes its launches for its buffers' capacities: more
+// candidate pairs than that raise the error word; the pair totals of captured
+// steps add up on the device.
+__global__ void k_graph_pairs(const int32_t *n_pairs, int cap, int32_t *err, unsigned long long *total)
+{
+	const int n = *n_pairs;
+	if (n > cap) raise_err(err, CPGPU_ERR_PAIRS_OVERFLOW);
+	*total += (unsigned long long)n;
+}
+
+} // namespace
+
+// Capacities of a captured step: room for twice the most pairs, records and
+// cached arbiters seen so far (the counts of the last finished step), and for
+// two pairs per dynamic shape at least.
+static hipError_t graph_reserve(cpgpuWorld *w)
+{
+	const int prev = w->cur ^ 1;
+	const long last_recs = w->steps > 0 ? w->h_counts[3 * prev] : 0;
+	const long last_cached = w->steps > 0 ? w->h_counts[3 * prev + 2] : 0;
+	const long seen_pairs = std::max<long>(w->last_pairs, w->graph_max_pairs);
+	const long pairs = std::max<long>({2 * seen_pairs + 1024, (long)w->pair_cap, 2L * w->n_dyn_shapes});
+	const long recs = std::max<long>({2 * pairs, 2 * last_recs + 1024, (long)w->graph_rec_cap});
+	const long cached = std::max<long>({pairs, 2 * last_cached + 1024, (long)w->graph_cached_cap});
+	HIP_TRY(regrow(w->d_pairs, w->pair_cap, pairs));
+	for (int slot = 0; slot < 2; slot++) HIP_TRY(reserve_step(w, slot, recs, (long)w->pair_cap + 1, cached, recs));
+	w->graph_rec_cap = std::min(w->rec_cap[0], w->rec_cap[1]);
+	w->graph_cached_cap = std::min(w->cached_cap[0], w->cached_cap[1]);
+	return radix_sort_reserve(w->sortbuf, std::max(w->pair_cap, std::max(w->n_dyn_shapes, 1)));
+}
+
+static int graph_capture(cpgpuWorld *w, double dt)
+{
+	int rc = sync_world(w);
+	if (rc != CPGPU_OK) return rc;
+	for (int slot = 0; slot < 2; slot++)
+		if (w->gexec[slot]) { (void)hipGraphExecDestroy(w->gexec[slot]); w->gexec[slot] = nullptr; }
+	HIP_OR_FAIL(graph_reserve(w));
+	if (w->nc > 0 && dt != w->last_joint_dt) HIP_OR_FAIL(set_joint_bias(w, dt));
+	const int prev = w->cur ^ 1;
+	if (w->steps == 0) HIP_OR_FAIL(hipMemsetAsync(w->d_step_counts + 3 * prev, 0, 3 * sizeof(int32_t), w->stream));
+	// the scalars the first captured step advances from
+	fill_world_steps(w, dt, false);
+	std::vector<double> curr(w->nw);
+	for (int k = 0; k < w->nw; k++) curr[k] = w->sp[k].curr_dt;
+	HIP_OR_FAIL(hipMemcpyAsync(w->d_ws, w->h_ws, sizeof(WorldStep) * w->nw, hipMemcpyHostToDevice, w->stream));
+	HIP_OR_FAIL(hipMemcpyAsync(w->d_curr_dt, curr.data(), sizeof(double) * w->nw, hipMemcpyHostToDevice, w->stream));
+	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+	// one graph per arbiter slot: step k reads slot k % 2's predecessor and writes its own
+	const int cur0 = w->cur;
+	for (int q = 0; q < 2; q++) {
+		w->cur = cur0 ^ q;
+		hipGraph_t g = nullptr;
+		HIP_OR_FAIL(hipStreamBeginCapture(w->stream, hipStreamCaptureModeRelaxed));
+		hipError_t e = hipSuccess;
+		k_advance_worlds<<<blocks_for(w->nw, 256), 256, 0, w->stream>>>(w->d_ws, w->d_curr_dt, w->nw, dt);
+		e = hipGetLastError();
+		if (e == hipSuccess) e = enqueue_front(w, nullptr);
+		if (e == hipSuccess) {
+			k_graph_pairs<<<1, 1, 0, w->stream>>>(w->d_counters, w->pair_cap, w->W.err, w->d_graph_pairs);
+			e = hipGetLastError();
+		}
+		if (e == hipSuccess) e = enqueue_back(w, w->pair_cap, w->rec_cap[w->cur ^ 1], nullptr);
+		if (e == hipSuccess) queue_counters(w);
+		hipError_t e2 = hipStreamEndCapture(w->stream, &g);
+		if (e == hipSuccess) e = e2;
+		if (e == hipSuccess) e = hipGraphInstantiate(&w->gexec[w->cur], g, nullptr, nullptr, 0);
+		if (g) (void)hipGraphDestroy(g);
+		if (e != hipSuccess) {
+			w->cur = cur0;
+			(void)hipGetLastError();
+			return fail(CPGPU_ERR_HIP, std::string("capturing the step in a graph: ") + hipGetErrorString(e));
+		}
+	}
+	w->cur = cur0;
+	w->graph_dt = dt;
+	w->graph_dirty = false;
+	return CPGPU_OK;
+}
+
+static int graph_step(cpgpuWorld *w, double dt)
+{
+	// the counts of the last finished captured step, copied by its last nodes: capacities with
+	// less than half to spare are grown, and the step captured again
+	if (w->graph_done && hipEventQuery(w->graph_done) == hipSuccess) {
+		const int last = w->cur ^ 1;
+		const long pairs = w->h_counts[6], recs = w->h_counts[3 * last], cached = w->h_counts[3 * last + 2];
+		w->graph_max_pairs = std::max<long>(w->graph_max_pairs, pairs);
+		if (2 * pairs > w->pair_cap || 2 * recs > w->graph_rec_cap || 2 * cached > w->graph_cached_cap)
+			w->graph_dirty = true;
+	}
+	(void)hipGetLastError();   // a not-ready query is no error
+	if (w->graph_dirty || dt != w->graph_dt || !w->gexec[w->cur]) {
+		const int rc = graph_capture(w, dt);
+		if (rc != CPGPU_OK) return rc;
+	}
+	for (int k = 0; k < w->nw; k++) {
+		w->sp[k].stamp++;
+		w->sp[k].curr_dt = dt;
+	}
+	HIP_OR_FAIL(hipGraphLaunch(w->gexec[w->cur], w->stream));
+	w->dyn_tree_built = true;
+	HIP_OR_FAIL(hipEventRecord(w->graph_done, w->stream));
+	w->cur ^= 1;
+	w->steps++;
+	w->graph_steps++;
+	w->pending = true;
+	return CPGPU_OK;
+}
+
+int cpgpuWorldSetGraph(cpgpuWorld *w, int32_t enable)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (enable) {
+		if (w->tape.on) return fail(CPGPU_ERR_INVALID, "a taped rollout is stepped without graphs");
+		if (w->timer.enabled) return fail(CPGPU_ERR_INVALID, "timed steps (cpgpuWorldSetTiming) are stepped without graphs");
+		if (w->n_large_worlds > 0)
+			return fail(CPGPU_ERR_INVALID, "spaces of more than 4096 dynamic bodies (the grid solve) are stepped "
+			                               "without graphs");
+	}
+	int rc = sync_world(w);
+	if (rc != CPGPU_OK) return rc;
+	if (enable && !w->graph_done) {
+		HIP_OR_FAIL(hipEventCreateWithFlags(&w->graph_done, hipEventDisableTiming));
+		HIP_OR_FAIL(dalloc(w->d_curr_dt, w->nw));
+		HIP_OR_FAIL(upload(w->d_graph_pairs, (const unsigned long long *)nullptr, 1));
+	}
+	w->graph_on = enable != 0;
+	w->graph_dirty = true;
 	return CPGPU_OK;
 }
 
@@ -1196,6 +1409,7 @@ int cpgpuWorldSetSpaceParams(cpgpuWorld *w, int32_t space, const cpgpuSpaceParam
 	int rc = cpgpuWorldSync(w);
 	if (rc != CPGPU_OK) return rc;
 	w->sp[space] = *params;
+	w->graph_dirty = true;
 	w->max_iterations = 0;
 	w->max_iterations_large = 0;
 	for (int k = 0; k < w->nw; k++) {
@@ -1363,6 +1577,11 @@ int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
 	out->total_arbiters = (int64_t)totals[0];
 	out->total_contacts = (int64_t)totals[1];
 	out->total_pairs = w->total_pairs;
+	if (w->d_graph_pairs) {
+		unsigned long long g = 0;
+		HIP_OR_FAIL(hipMemcpy(&g, w->d_graph_pairs, sizeof(g), hipMemcpyDeviceToHost));
+		out->total_pairs += (int64_t)g;
+	}
 	return CPGPU_OK;
 }
 
@@ -1483,6 +1702,181 @@ int cpgpuWorldShapesCollide(cpgpuWorld *w, int64_t n, const int32_t *pairs, int3
 	return rc;
 }
 
+// ---- spatial queries (cpSpaceQuery.c) on the last step's trees ----
+
+static int query_trees(cpgpuWorld *w, QueryTrees &T)
+{
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	if (!w->dyn_tree_built && w->n_dyn_shapes > 0) {
+		// before the first step: the tree over the boxes cached at creation
+		HIP_OR_FAIL(lbvh_build(w->dyn_tree, w->W, w->d_dyn_shapes, w->n_dyn_shapes, w->nw, w->sortbuf,
+		                       w->d_world_bounds, w->stream, nullptr));
+		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+		w->dyn_tree_built = true;
+	}
+	T.nd = w->n_dyn_shapes;
+	T.ns = w->n_static_shapes;
+	T.droot = w->dyn_tree.world_root; T.sroot = w->static_tree.world_root;
+	T.dshape = w->dyn_tree.leaf_shape; T.sshape = w->static_tree.leaf_shape;
+	T.dnode = w->dyn_tree.node; T.snode = w->static_tree.node;
+	T.dbox = w->dyn_tree.box; T.sbox = w->static_tree.box;
+	return CPGPU_OK;
+}
+
+// Stages a batch of queries in one device allocation: inputs (space, a [n][2
+// or 4], b [n][2], scalar [n], filter [n]) then outputs (out_bytes).
+struct QueryStage {
+	char *base = nullptr;
+	QueryIn in{};
+	char *out = nullptr;
+	~QueryStage() { if (base) (void)hipFree(base); }
+};
+
+static int query_stage(cpgpuWorld *w, int32_t n, const int32_t *space, const double *a, int a_width, const double *b,
+                       const double *scalar, const cpgpuShapeFilter *filter, size_t out_bytes, QueryStage &S)
+{
+	if (n < 0) return fail(CPGPU_ERR_INVALID, "negative query count");
+	if (n > 0 && (!a || (a_width == 2 && !scalar)))
+		return fail(CPGPU_ERR_INVALID, "null query points, segments, boxes or distances");
+	for (int32_t q = 0; space && q < n; q++)
+		if (space[q] < 0 || space[q] >= w->nw)
+			return fail(CPGPU_ERR_INVALID, "query " + std::to_string(q) + " names space " + std::to_string(space[q]) +
+			                                   " of " + std::to_string(w->nw));
+	auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+	const size_t nn = (size_t)std::max(n, 1);
+	const size_t sz_space = up(sizeof(int32_t) * nn), sz_a = up(sizeof(double) * a_width * nn);
+	const size_t sz_b = up(sizeof(double) * 2 * nn), sz_s = up(sizeof(double) * nn);
+	const size_t sz_f = up(sizeof(cpgpuShapeFilter) * nn);
+	HIP_OR_FAIL(hipMalloc(&S.base, sz_space + sz_a + sz_b + sz_s + sz_f + up(out_bytes)));
+	char *p = S.base;
+	int32_t *d_space = space ? reinterpret_cast<int32_t *>(p) : nullptr; p += sz_space;
+	double *d_a = reinterpret_cast<double *>(p); p += sz_a;
+	double *d_b = b ? reinterpret_cast<double *>(p) : nullptr; p += sz_b;
+	double *d_s = scalar ? reinterpret_cast<double *>(p) : nullptr; p += sz_s;
+	cpgpuShapeFilter *d_f = filter ? reinterpret_cast<cpgpuShapeFilter *>(p) : nullptr; p += sz_f;
+	S.out = p;
+	if (n > 0) {
+		if (space) HIP_OR_FAIL(hipMemcpy(d_space, space, sizeof(int32_t) * n, hipMemcpyHostToDevice));
+		HIP_OR_FAIL(hipMemcpy(d_a, a, sizeof(double) * a_width * n, hipMemcpyHostToDevice));
+		if (b) HIP_OR_FAIL(hipMemcpy(d_b, b, sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+		if (scalar) HIP_OR_FAIL(hipMemcpy(d_s, scalar, sizeof(double) * n, hipMemcpyHostToDevice));
+		if (filter) HIP_OR_FAIL(hipMemcpy(d_f, filter, sizeof(cpgpuShapeFilter) * n, hipMemcpyHostToDevice));
+	}
+	S.in.space = d_space; S.in.a = d_a; S.in.b = d_b; S.in.scalar = d_s; S.in.filter = d_f;
+	return CPGPU_OK;
+}
+
+static int query_one_each(cpgpuWorld *w, int32_t n, const int32_t *space, const double *a, const double *b,
+                          const double *scalar, const cpgpuShapeFilter *filter, bool segment, int32_t *shape_out,
+                          double *info_out)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (n > 0 && (!shape_out || !info_out)) return fail(CPGPU_ERR_INVALID, "null query outputs");
+	if (segment && n > 0 && !b) return fail(CPGPU_ERR_INVALID, "null segment ends");
+	QueryTrees T;
+	int rc = query_trees(w, T);
+	if (rc != CPGPU_OK) return rc;
+	QueryStage S;
+	rc = query_stage(w, n, space, a, 2, b, scalar, filter, (sizeof(int32_t) + 5 * sizeof(double)) * (size_t)std::max(n, 1) + 256, S);
+	if (rc != CPGPU_OK || n == 0) return rc;
+	int32_t *d_shape = reinterpret_cast<int32_t *>(S.out);
+	double *d_info = reinterpret_cast<double *>(S.out + ((sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255));
+	HIP_OR_FAIL(query_nearest_or_first(w->W, T, n, S.in, segment, d_shape, d_info, w->stream));
+	rc = sync_world(w);
+	if (rc != CPGPU_OK) return rc;
+	HIP_OR_FAIL(hipMemcpy(shape_out, d_shape, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+	HIP_OR_FAIL(hipMemcpy(info_out, d_info, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
+	return CPGPU_OK;
+}
+
+static int query_every(cpgpuWorld *w, int32_t n, const int32_t *space, const double *a, int a_width, const double *b,
+                       const double *scalar, const cpgpuShapeFilter *filter, int kind, int64_t cap, int64_t *n_out,
+                       int32_t *hits, double *info)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (cap < 0) return fail(CPGPU_ERR_INVALID, "negative hit capacity");
+	if (cap > 0 && !hits) return fail(CPGPU_ERR_INVALID, "null hit array");
+	if (kind == 1 && n > 0 && !b) return fail(CPGPU_ERR_INVALID, "null segment ends");
+	QueryTrees T;
+	int rc = query_trees(w, T);
+	if (rc != CPGPU_OK) return rc;
+	// the device keeps room for at least 4096 hits, so a first call with cap 0 learns the total
+	const int64_t dcap = std::max<int64_t>(cap, 4096);
+	const size_t hit_bytes = (sizeof(int2) * (size_t)dcap + 255) & ~(size_t)255;
+	const size_t info_bytes = (sizeof(double) * 5 * (size_t)dcap + 255) & ~(size_t)255;
+	QueryStage S;
+	rc = query_stage(w, n, space, a, a_width, b, scalar, filter, hit_bytes + info_bytes + 256, S);
+	if (rc != CPGPU_OK) return rc;
+	if (n == 0) {
+		if (n_out) *n_out = 0;
+		return CPGPU_OK;
+	}
+	int2 *d_hits = reinterpret_cast<int2 *>(S.out);
+	double *d_info = reinterpret_cast<double *>(S.out + hit_bytes);
+	unsigned long long *d_count = reinterpret_cast<unsigned long long *>(S.out + hit_bytes + info_bytes);
+	HIP_OR_FAIL(hipMemsetAsync(d_count, 0, sizeof(unsigned long long), w->stream));
+	HIP_OR_FAIL(query_all(w->W, T, n, S.in, kind, d_hits, kind == 2 ? nullptr : d_info, dcap, d_count, w->stream));
+	rc = sync_world(w);
+	if (rc != CPGPU_OK) return rc;
+	unsigned long long total = 0;
+	HIP_OR_FAIL(hipMemcpy(&total, d_count, sizeof(total), hipMemcpyDeviceToHost));
+	if (n_out) *n_out = (int64_t)total;
+	const int64_t got = std::min<int64_t>((int64_t)total, dcap);
+	if (cap == 0 || got == 0) return CPGPU_OK;
+	std::vector<int2> h(got);
+	std::vector<double> hi(kind == 2 ? 0 : 5 * got);
+	HIP_OR_FAIL(hipMemcpy(h.data(), d_hits, sizeof(int2) * got, hipMemcpyDeviceToHost));
+	if (kind != 2) HIP_OR_FAIL(hipMemcpy(hi.data(), d_info, sizeof(double) * 5 * got, hipMemcpyDeviceToHost));
+	// by query, then shape: the device appended them in no fixed order
+	std::vector<int64_t> order(got);
+	for (int64_t k = 0; k < got; k++) order[k] = k;
+	std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+		return h[x].x != h[y].x ? h[x].x < h[y].x : h[x].y < h[y].y;
+	});
+	for (int64_t k = 0; k < got && k < cap; k++) {
+		hits[2 * k] = h[order[k]].x;
+		hits[2 * k + 1] = h[order[k]].y;
+		if (info && kind != 2)
+			for (int i = 0; i < 5; i++) info[5 * k + i] = hi[5 * order[k] + i];
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldQueryPointNearest(cpgpuWorld *w, int32_t n, const int32_t *space, const double *points,
+                                const double *max_distance, const cpgpuShapeFilter *filter, int32_t *shape_out,
+                                double *info_out)
+{
+	return query_one_each(w, n, space, points, nullptr, max_distance, filter, false, shape_out, info_out);
+}
+
+int cpgpuWorldQuerySegmentFirst(cpgpuWorld *w, int32_t n, const int32_t *space, const double *starts,
+                                const double *ends, const double *radius, const cpgpuShapeFilter *filter,
+                                int32_t *shape_out, double *info_out)
+{
+	return query_one_each(w, n, space, starts, ends, radius, filter, true, shape_out, info_out);
+}
+
+int cpgpuWorldQueryPointAll(cpgpuWorld *w, int32_t n, const int32_t *space, const double *points,
+                            const double *max_distance, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out,
+                            int32_t *hits, double *info)
+{
+	return query_every(w, n, space, points, 2, nullptr, max_distance, filter, 0, cap, n_out, hits, info);
+}
+
+int cpgpuWorldQuerySegmentAll(cpgpuWorld *w, int32_t n, const int32_t *space, const double *starts,
+                              const double *ends, const double *radius, const cpgpuShapeFilter *filter, int64_t cap,
+                              int64_t *n_out, int32_t *hits, double *info)
+{
+	return query_every(w, n, space, starts, 2, ends, radius, filter, 1, cap, n_out, hits, info);
+}
+
+int cpgpuWorldQueryBBAll(cpgpuWorld *w, int32_t n, const int32_t *space, const double *boxes,
+                         const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits)
+{
+	return query_every(w, n, space, boxes, 4, nullptr, nullptr, filter, 2, cap, n_out, hits, nullptr);
+}
+
 int cpgpuWorldReadConstraints(cpgpuWorld *w, double *jacc, int32_t *colour)
 {
 	int rc = cpgpuWorldSync(w);
@@ -1492,8 +1886,9 @@ int cpgpuWorldReadConstraints(cpgpuWorld *w, double *jacc, int32_t *colour)
 	for (int c = 0; c < w->nc; c++) {
 		const Joint &j = J[c];
 		if (jacc) {
-			if (j.type == CPGPU_JOINT_PIVOT) { jacc[2 * c] = j.jAx; jacc[2 * c + 1] = j.jAy; }
-			else if (j.type == CPGPU_JOINT_SLIDE) { jacc[2 * c] = j.jnAcc; jacc[2 * c + 1] = 0.0; }
+			if (j.type == CPGPU_JOINT_PIVOT || j.type == CPGPU_JOINT_GROOVE) { jacc[2 * c] = j.jAx; jacc[2 * c + 1] = j.jAy; }
+			else if (j.type == CPGPU_JOINT_SLIDE || j.type == CPGPU_JOINT_PIN) { jacc[2 * c] = j.jnAcc; jacc[2 * c + 1] = 0.0; }
+			else if (j.type == CPGPU_JOINT_RATCHET) { jacc[2 * c] = j.jAcc; jacc[2 * c + 1] = j.angle; }
 			else { jacc[2 * c] = j.jAcc; jacc[2 * c + 1] = 0.0; }
 		}
 		if (colour) colour[c] = w->h_jcolour[c];
@@ -1508,6 +1903,9 @@ int cpgpuWorldSetTiming(cpgpuWorld *w, int32_t enable)
 	if (rc != CPGPU_OK) return rc;
 	if (enable < CPGPU_TIMING_OFF || enable > CPGPU_TIMING_SOLVE)
 		return fail(CPGPU_ERR_INVALID, "unknown timing mode " + std::to_string(enable));
+	if (enable != CPGPU_TIMING_OFF && w->graph_on)
+		return fail(CPGPU_ERR_INVALID, "steps captured in graphs are timed as a whole (cpgpuWorldSetGraph 0 to time "
+		                               "each launch)");
 	w->timer.enabled = enable != CPGPU_TIMING_OFF;
 	w->timer.solve_only = enable == CPGPU_TIMING_SOLVE;
 	w->timer.acc.clear();
@@ -1522,6 +1920,7 @@ int cpgpuWorldSetGridSchedule(cpgpuWorld *w, int32_t schedule)
 	int rc = cpgpuWorldSync(w);
 	if (rc != CPGPU_OK) return rc;
 	w->grid_flow = schedule == CPGPU_GRID_FLOW;
+	w->graph_dirty = true;
 	return CPGPU_OK;
 }
 
@@ -1556,6 +1955,7 @@ int cpgpuWorldSetBuckets(cpgpuWorld *w, int32_t n, const int32_t *max_bodies, co
 		w->buckets[q].threads = threads[q];
 	}
 	HIP_OR_FAIL(build_buckets(w));
+	w->graph_dirty = true;
 	return CPGPU_OK;
 }
 
@@ -1569,6 +1969,9 @@ int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *t
 	static const char *names[CPGPU_MAX_BUCKETS] = {"solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2",
 	                                               "solver_worlds_b3"};
 	const bool was_timing = w->timer.enabled;
+	const bool was_graph = w->graph_on;
+	w->graph_on = false;   // trials time every bucket's launches: eager steps
+	w->graph_dirty = true;
 	auto saved = w->timer.acc;
 	double best[CPGPU_MAX_BUCKETS];
 	int best_t[CPGPU_MAX_BUCKETS];
@@ -1580,10 +1983,10 @@ int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *t
 		w->timer.acc.clear();
 		for (int k = 0; k < trial_steps; k++) {
 			rc = cpgpuWorldStep(w, dt);
-			if (rc != CPGPU_OK) { w->timer.enabled = was_timing; return rc; }
+			if (rc != CPGPU_OK) { w->timer.enabled = was_timing; w->graph_on = was_graph; return rc; }
 		}
 		rc = cpgpuWorldSync(w);
-		if (rc != CPGPU_OK) { w->timer.enabled = was_timing; return rc; }
+		if (rc != CPGPU_OK) { w->timer.enabled = was_timing; w->graph_on = was_graph; return rc; }
 		for (int q = 0; q < w->n_buckets; q++) {
 			auto it = w->timer.acc.find(names[q]);
 			if (it == w->timer.acc.end() || it->second.second == 0) continue;
@@ -1596,6 +1999,7 @@ int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *t
 		for (int q = 0; q < CPGPU_MAX_BUCKETS; q++) threads_out[q] = q < w->n_buckets ? best_t[q] : 0;
 	w->timer.acc = saved;
 	w->timer.enabled = was_timing;
+	w->graph_on = was_graph;
 	return CPGPU_OK;
 }
 
@@ -1604,6 +2008,9 @@ int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	if (checkpoint_every < 1) return fail(CPGPU_ERR_INVALID, "checkpoint_every must be positive");
 	if (sizeof(real) != 8) return fail(CPGPU_ERR_INVALID, "the adjoint needs a CPGPU_FP64 world");
+	if (w->graph_on) return fail(CPGPU_ERR_INVALID, "a taped rollout is stepped without graphs (cpgpuWorldSetGraph 0)");
+	if (w->other_joints)
+		return fail(CPGPU_ERR_INVALID, "the adjoint covers pivot, damped-spring, slide and gear joints only");
 	if (w->n_large_worlds > 0)
 		return fail(CPGPU_ERR_INVALID, "the adjoint supports spaces solved one workgroup each (at most " +
 		                                   std::to_string(w->small_world_max_bodies) + " dynamic bodies)");

@@ -28,7 +28,9 @@ EXPORTS = [
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
     "cpgpuWorldReadConstraints", "cpgpuWorldReadPairs", "cpgpuWorldShapesCollide",
     "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
-    "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldSetGridSchedule",
+    "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldSetGridSchedule", "cpgpuWorldSetGraph",
+    "cpgpuWorldQueryPointNearest", "cpgpuWorldQuerySegmentFirst", "cpgpuWorldQueryPointAll", "cpgpuWorldQuerySegmentAll",
+    "cpgpuWorldQueryBBAll",
     "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
     "cpgpuWorldBackward", "cpgpuWorldGetJointGrads", "cpgpuGetLastError", "cpgpuVersion",
 ]
@@ -39,6 +41,29 @@ GRID_FLOW, GRID_PHASES = 0, 1
 
 class CpgpuError(RuntimeError):
     pass
+
+
+class ShapeFilter(ctypes.Structure):
+    """cpgpuShapeFilter (cpShapeFilter: group, categories, mask)."""
+    _fields_ = [("group", ctypes.c_uint64), ("categories", ctypes.c_uint32), ("mask", ctypes.c_uint32)]
+
+
+def _filters(filters, n):
+    """None (every shape) or one (group, categories, mask) per query."""
+    if filters is None:
+        return None
+    arr = (ShapeFilter * max(n, 1))()
+    for k, (g, c, m) in enumerate(filters):
+        arr[k] = ShapeFilter(g, c, m)
+    return arr
+
+
+def _f64(x, shape):
+    return np.ascontiguousarray(np.broadcast_to(np.asarray(x, dtype=np.float64), shape))
+
+
+def _space(space, n):
+    return None if space is None else np.ascontiguousarray(np.broadcast_to(np.asarray(space, np.int32), (n,)))
 
 
 def lib():
@@ -66,6 +91,14 @@ def lib():
         L.cpgpuWorldShapesCollide.argtypes = [_vp, ctypes.c_int64, _ip, _ip, _dp]
         L.cpgpuWorldSetTiming.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldSetGridSchedule.argtypes = [_vp, ctypes.c_int32]
+        L.cpgpuWorldSetGraph.argtypes = [_vp, ctypes.c_int32]
+        _fp = ctypes.POINTER(ShapeFilter)
+        L.cpgpuWorldQueryPointNearest.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _dp, _fp, _ip, _dp]
+        L.cpgpuWorldQuerySegmentFirst.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _dp, _dp, _fp, _ip, _dp]
+        L.cpgpuWorldQueryPointAll.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _dp, _fp, ctypes.c_int64, _i64p, _ip, _dp]
+        L.cpgpuWorldQuerySegmentAll.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _dp, _dp, _fp, ctypes.c_int64, _i64p,
+                                                _ip, _dp]
+        L.cpgpuWorldQueryBBAll.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _fp, ctypes.c_int64, _i64p, _ip]
         L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
         L.cpgpuWorldGetSolveCounts.argtypes = [_vp, _i64p, _i64p]
         L.cpgpuWorldGetBuckets.argtypes = [_vp, _ip, _ip, _ip, _ip]
@@ -192,8 +225,7 @@ class GPUWorld:
         n = len(pairs)
         count = np.zeros(n, np.int32)
         sets = np.zeros((n, 12))
-        _check(self.L.cpgpuWorldShapesCollide(self.h, n, ptr(pairs, ctypes.c_int32), ptr(count, ctypes.c_int32),
-                                              ptr(sets, ctypes.c_double)))
+        _check(self.L.cpgpuWorldShapesCollide(self.h, n, ptr(pairs, ctypes.c_int32), ptr(count, ctypes.c_int32), ptr(sets, ctypes.c_double)))
         return count, sets
 
     def constraints(self):
@@ -234,6 +266,72 @@ class GPUWorld:
     def set_grid_schedule(self, schedule):
         """GRID_FLOW (dependency-driven, the default) or GRID_PHASES (grid barriers between phases)."""
         _check(self.L.cpgpuWorldSetGridSchedule(self.h, int(schedule)))
+
+    def set_graph(self, on=True):
+        """Capture each step in a HIP graph (cpgpuWorldSetGraph)."""
+        _check(self.L.cpgpuWorldSetGraph(self.h, 1 if on else 0))
+
+    # ---- spatial queries (cpSpaceQuery.c) on the last step's shapes ----
+
+    def query_point_nearest(self, points, max_distance, space=None, filters=None):
+        """cpSpacePointQueryNearest per point: (shape [n], -1 for none; info [n, 5]: point, distance,
+        gradient)."""
+        p = _f64(points, (len(points), 2))
+        n = len(p)
+        d, sp, f = _f64(max_distance, (n,)), _space(space, n), _filters(filters, n)
+        shape, info = np.zeros(n, np.int32), np.zeros((n, 5))
+        _check(self.L.cpgpuWorldQueryPointNearest(self.h, n, ptr(sp, ctypes.c_int32), ptr(p, ctypes.c_double),
+                                                  ptr(d, ctypes.c_double), f, ptr(shape, ctypes.c_int32),
+                                                  ptr(info, ctypes.c_double)))
+        return shape, info
+
+    def query_segment_first(self, starts, ends, radius, space=None, filters=None):
+        """cpSpaceSegmentQueryFirst per segment: (shape [n]; info [n, 5]: point, normal, alpha)."""
+        a = _f64(starts, (len(starts), 2))
+        n = len(a)
+        b, r, sp, f = _f64(ends, (n, 2)), _f64(radius, (n,)), _space(space, n), _filters(filters, n)
+        shape, info = np.zeros(n, np.int32), np.zeros((n, 5))
+        _check(self.L.cpgpuWorldQuerySegmentFirst(self.h, n, ptr(sp, ctypes.c_int32), ptr(a, ctypes.c_double),
+                                                  ptr(b, ctypes.c_double), ptr(r, ctypes.c_double), f,
+                                                  ptr(shape, ctypes.c_int32), ptr(info, ctypes.c_double)))
+        return shape, info
+
+    def _query_all(self, fn, args, with_info=True):
+        total = ctypes.c_int64()
+        _check(fn(self.h, *args, 0, ctypes.byref(total), None, None) if with_info else
+               fn(self.h, *args, 0, ctypes.byref(total), None))
+        cap = total.value
+        hits, info = np.zeros((cap, 2), np.int32), np.zeros((cap, 5))
+        if with_info:
+            _check(fn(self.h, *args, cap, ctypes.byref(total), ptr(hits, ctypes.c_int32), ptr(info, ctypes.c_double)))
+            return hits, info
+        _check(fn(self.h, *args, cap, ctypes.byref(total), ptr(hits, ctypes.c_int32)))
+        return hits
+
+    def query_point_all(self, points, max_distance, space=None, filters=None):
+        """cpSpacePointQuery: hits [k, 2] (query, shape) sorted, info [k, 5] (point, distance, gradient)."""
+        p = _f64(points, (len(points), 2))
+        n = len(p)
+        d, sp = _f64(max_distance, (n,)), _space(space, n)
+        return self._query_all(self.L.cpgpuWorldQueryPointAll, [n, ptr(sp, ctypes.c_int32), ptr(p, ctypes.c_double),
+                                                                ptr(d, ctypes.c_double), _filters(filters, n)])
+
+    def query_segment_all(self, starts, ends, radius, space=None, filters=None):
+        """cpSpaceSegmentQuery: hits [k, 2] (query, shape) sorted, info [k, 5] (point, normal, alpha)."""
+        a = _f64(starts, (len(starts), 2))
+        n = len(a)
+        b, r, sp = _f64(ends, (n, 2)), _f64(radius, (n,)), _space(space, n)
+        return self._query_all(self.L.cpgpuWorldQuerySegmentAll,
+                               [n, ptr(sp, ctypes.c_int32), ptr(a, ctypes.c_double), ptr(b, ctypes.c_double),
+                                ptr(r, ctypes.c_double), _filters(filters, n)])
+
+    def query_bb_all(self, boxes, space=None, filters=None):
+        """cpSpaceBBQuery: hits [k, 2] (query, shape) sorted; boxes [n, 4] l, b, r, t."""
+        bb = _f64(boxes, (len(boxes), 4))
+        n = len(bb)
+        return self._query_all(self.L.cpgpuWorldQueryBBAll, [n, ptr(_space(space, n), ctypes.c_int32),
+                                                             ptr(bb, ctypes.c_double), _filters(filters, n)],
+                               with_info=False)
 
     def autotune(self, dt, trial_steps=3):
         """Pick each size bucket's workgroup size by timing (advances the world 5 x trial_steps steps)."""

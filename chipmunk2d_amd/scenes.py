@@ -15,13 +15,17 @@ Workloads (sizes as named):
                     segment, with target positions (differentiation)
   chain_world       a pivot-joint chain with springs, slides and gears, and
                     falling circles
+  machine_world     mechanisms with every joint class (pin, groove, rotary
+                    limit, ratchet, simple motor, damped rotary spring beside
+                    pivot, spring, slide, gear), and falling circles
 """
 import math
 
 import numpy as np
 
 from .space import Space, Body, Circle, Segment, Poly, Box, moment_for_box, moment_for_circle, PivotJoint, \
-    DampedSpring, SlideJoint, GearJoint
+    DampedSpring, SlideJoint, GearJoint, PinJoint, GrooveJoint, RotaryLimitJoint, RatchetJoint, SimpleMotor, \
+    DampedRotarySpring
 
 MASK64 = (1 << 64) - 1
 
@@ -30,6 +34,7 @@ SEED_RANDOM_WORLD = 0xC4100002
 SEED_BATCH = 0xC4100003
 SEED_GRAD = 0xC4100004
 SEED_CHAINS = 0xC4100005
+SEED_MACHINES = 0xC4100006
 
 
 def _rotl(x, k):
@@ -272,6 +277,99 @@ def chain_world(n_links=200, n_circles=300, seed=SEED_CHAINS, rng=None):
         shape.density = 1.0
         shape.friction = 0.5
         body.position = (rng.uniform(x0, -x0), rng.uniform(10.0, 10.0 + 0.5 * n_links))
+    return space
+
+
+def machine_world(n_units=8, n_circles=60, seed=SEED_MACHINES, rng=None):
+    """n_units mechanisms side by side (12 apart) on a ground segment, each using
+    every joint class, and n_circles circles (radius U[0.4,1.2]) falling on them.
+    Gravity (0,-100), 10 iterations.  Unit k at x = 12 k:
+      * a two-link pendulum of circles hung from the static body by pin joints,
+        the links joined by a damped rotary spring (rest angle U[-1,1]);
+      * a wheel on a pivot driven by a simple motor (rate U[-4,4], max force
+        U[2e3, 2e4]) and an arm on the same pivot held to the wheel by a rotary
+        limit joint ([-0.4, 0.4]);
+      * a slider on a tilted groove of the static body (it slides into the
+        groove's lower end) and a second slider on a groove along the arm;
+      * a paddle on a pivot with a ratchet (notch pi/8 or -pi/8, phase U[0,1])
+        against the static body, wound by a damped rotary spring;
+      * a gear joint (ratio U[0.5,2]) between the paddle and the wheel every
+        other unit, a damped spring and a slide joint between the pendulum's
+        tip and the slider.
+    Max bias and error bias vary between units."""
+    if rng is None:
+        rng = Rng(seed)
+    space = Space()
+    space.iterations = 10
+    space.gravity = (0.0, -100.0)
+    st = space.static_body
+    width = 12.0 * n_units + 20.0
+    ground = space.add(Segment(st, (-20.0, -2.0), (width, -2.0), 0.5))
+    ground.friction = 0.8
+
+    def circle_body(pos, r, mass=1.0):
+        body = space.add(Body(mass, moment_for_circle(mass, 0.0, r)))
+        body.position = pos
+        sh = space.add(Circle(body, r))
+        sh.friction = 0.5
+        return body
+
+    def box_body(pos, w, h, mass=1.0, angle=0.0):
+        body = space.add(Body(mass, moment_for_box(mass, w, h)))
+        body.position = pos
+        body.angle = angle
+        sh = space.add(Box(body, w, h, 0.0))
+        sh.friction = 0.6
+        return body
+
+    def tune(j, k):
+        if k % 3 == 1:
+            j.max_bias = 40.0
+        if k % 4 == 2:
+            j.error_bias = math.pow(1.0 - 0.2, 60.0)
+        return space.add(j)
+
+    for k in range(n_units):
+        x = 12.0 * k
+        # pendulum: pin joints, a damped rotary spring between the links
+        p1 = circle_body((x + 3.0, 22.0), 0.8)
+        p2 = circle_body((x + 5.5, 19.0), 0.6, mass=0.5)
+        tune(PinJoint(st, p1, (x, 24.0), (0.0, 0.0)), k)
+        tune(PinJoint(p1, p2, (0.3, 0.0), (0.0, 0.4)), k)
+        tune(DampedRotarySpring(p1, p2, rng.uniform(-1.0, 1.0), rng.uniform(20.0, 200.0), rng.uniform(1.0, 20.0)), k)
+        # wheel, motor, arm held by a rotary limit
+        wheel = circle_body((x, 6.0), 2.0, mass=2.0)
+        tune(PivotJoint(st, wheel, (x, 6.0)), k)
+        motor = tune(SimpleMotor(st, wheel, rng.uniform(-4.0, 4.0)), k)
+        motor.max_force = rng.uniform(2.0e3, 2.0e4)
+        arm = box_body((x + 3.0, 6.0), 6.0, 0.5, mass=0.8)
+        piv = tune(PivotJoint(wheel, arm, (x, 6.0)), k)
+        piv.collide_bodies = False
+        tune(RotaryLimitJoint(wheel, arm, -0.4, 0.4), k)
+        # sliders on grooves: one in the static body, one along the arm
+        s1 = box_body((x + 2.0, 14.0), 1.5, 1.0, mass=0.6)
+        tune(GrooveJoint(st, s1, (x - 3.0, 16.0), (x + 4.0, 12.5), (0.0, 0.0)), k)
+        s2 = circle_body((x + 4.0, 6.8), 0.4, mass=0.3)
+        g2 = tune(GrooveJoint(arm, s2, (-2.5, 0.8), (2.5, 0.8), (0.0, 0.0)), k)
+        g2.collide_bodies = False
+        # paddle: pivot, ratchet against the static body, wound by a rotary spring
+        paddle = box_body((x + 8.0, 10.0), 3.0, 0.6, mass=1.0, angle=rng.uniform(-0.5, 0.5))
+        tune(PivotJoint(st, paddle, (x + 8.0, 10.0)), k)
+        notch = math.pi / 8 if k % 2 == 0 else -math.pi / 8
+        tune(RatchetJoint(st, paddle, rng.uniform(0.0, 1.0), notch), k)
+        tune(DampedRotarySpring(st, paddle, rng.uniform(-3.0, 3.0), rng.uniform(50.0, 400.0), rng.uniform(2.0, 30.0)), k)
+        if k % 2 == 0:
+            tune(GearJoint(paddle, wheel, 0.0, rng.uniform(0.5, 2.0)), k)
+        tune(DampedSpring(p2, s1, (0.0, 0.0), (0.0, 0.0), rng.uniform(3.0, 6.0), rng.uniform(10.0, 100.0),
+                          rng.uniform(0.5, 5.0)), k)
+        tune(SlideJoint(p2, s1, (0.0, 0.0), (0.0, 0.0), 2.0, 7.0), k)
+    for _ in range(n_circles):
+        r = rng.uniform(0.4, 1.2)
+        body = space.add(Body(0.0, 0.0))
+        sh = space.add(Circle(body, r))
+        sh.density = 1.0
+        sh.friction = 0.5
+        body.position = (rng.uniform(-5.0, width - 20.0), rng.uniform(26.0, 60.0))
     return space
 
 

@@ -18,7 +18,8 @@ import math
 import numpy as np
 
 from .cabi import (BODY_DYNAMIC, BODY_KINEMATIC, BODY_STATIC, SHAPE_CIRCLE, SHAPE_SEGMENT, SHAPE_POLY,
-                   JOINT_PIVOT, JOINT_DAMPED_SPRING, JOINT_SLIDE, JOINT_GEAR, PackedWorld)
+                   JOINT_PIVOT, JOINT_DAMPED_SPRING, JOINT_SLIDE, JOINT_GEAR, JOINT_PIN, JOINT_GROOVE,
+                   JOINT_ROTARY_LIMIT, JOINT_RATCHET, JOINT_SIMPLE_MOTOR, JOINT_DAMPED_ROTARY_SPRING, PackedWorld)
 
 INF = float("inf")
 CP_PI = 3.14159265358979323846264338327950288
@@ -603,6 +604,113 @@ class GearJoint(Constraint):
 
     def params(self):
         return [self.phase, self.ratio, self.ratio_inv]
+
+
+class PinJoint(Constraint):
+    """cpPinJoint (src/cpPinJoint.c): the distance between the anchors is held at the
+    one they have when the joint is made (cpPinJointInit, cpPinJoint.c:98-116)."""
+    type = JOINT_PIN
+
+    def __init__(self, a, b, anchor_a, anchor_b):
+        super().__init__(a, b)
+        self.anchor_a = (float(anchor_a[0]), float(anchor_a[1]))
+        self.anchor_b = (float(anchor_b[0]), float(anchor_b[1]))
+        self.dist = _length(_sub(_xpoint(b.transform, self.anchor_b), _xpoint(a.transform, self.anchor_a)))
+        self.jn_acc = 0.0
+
+    def params(self):
+        return [*self.anchor_a, *self.anchor_b, self.dist]
+
+    def jacc(self):
+        return (self.jn_acc, 0.0)
+
+
+class GrooveJoint(Constraint):
+    """cpGrooveJoint (src/cpGrooveJoint.c): body B's anchor slides along the groove
+    from groove_a to groove_b on body A."""
+    type = JOINT_GROOVE
+
+    def __init__(self, a, b, groove_a, groove_b, anchor_b):
+        super().__init__(a, b)
+        self.groove_a = (float(groove_a[0]), float(groove_a[1]))
+        self.groove_b = (float(groove_b[0]), float(groove_b[1]))
+        self.anchor_b = (float(anchor_b[0]), float(anchor_b[1]))
+        self.j_acc = (0.0, 0.0)
+
+    @property
+    def groove_n(self):
+        # cpvperp(cpvnormalize(groove_b - groove_a)) (cpGrooveJoint.c:126, cpVect.h:159-163)
+        n = _normalize((self.groove_b[0] - self.groove_a[0], self.groove_b[1] - self.groove_a[1]))
+        return (-n[1], n[0])
+
+    def params(self):
+        return [*self.groove_a, *self.groove_b, *self.anchor_b, *self.groove_n]
+
+    def jacc(self):
+        return self.j_acc
+
+
+class RotaryLimitJoint(Constraint):
+    """cpRotaryLimitJoint (src/cpRotaryLimitJoint.c): b.angle - a.angle kept in [min, max]."""
+    type = JOINT_ROTARY_LIMIT
+
+    def __init__(self, a, b, mn, mx):
+        super().__init__(a, b)
+        self.min, self.max = float(mn), float(mx)
+        self.j_acc = 0.0
+
+    def params(self):
+        return [self.min, self.max]
+
+    def jacc(self):
+        return (self.j_acc, 0.0)
+
+
+class RatchetJoint(Constraint):
+    """cpRatchetJoint (src/cpRatchetJoint.c): a rotary ratchet with notches every
+    `ratchet` radians from `phase`; the angle starts at b.angle - a.angle
+    (cpRatchetJointInit, cpRatchetJoint.c:111-124) and is carried between steps."""
+    type = JOINT_RATCHET
+
+    def __init__(self, a, b, phase, ratchet):
+        super().__init__(a, b)
+        self.phase, self.ratchet = float(phase), float(ratchet)
+        self.angle = b.a - a.a
+        self.j_acc = 0.0
+
+    def params(self):
+        return [self.phase, self.ratchet]
+
+    def jacc(self):
+        return (self.j_acc, self.angle)
+
+
+class SimpleMotor(Constraint):
+    """cpSimpleMotor (src/cpSimpleMotor.c): holds b.w - a.w at -rate."""
+    type = JOINT_SIMPLE_MOTOR
+
+    def __init__(self, a, b, rate):
+        super().__init__(a, b)
+        self.rate = float(rate)
+        self.j_acc = 0.0
+
+    def params(self):
+        return [self.rate]
+
+    def jacc(self):
+        return (self.j_acc, 0.0)
+
+
+class DampedRotarySpring(Constraint):
+    """cpDampedRotarySpring (src/cpDampedRotarySpring.c) with the default spring torque."""
+    type = JOINT_DAMPED_ROTARY_SPRING
+
+    def __init__(self, a, b, rest_angle, stiffness, damping):
+        super().__init__(a, b)
+        self.rest_angle, self.stiffness, self.damping = float(rest_angle), float(stiffness), float(damping)
+
+    def params(self):
+        return [self.rest_angle, self.stiffness, self.damping]
 
 
 class Space:

@@ -59,7 +59,10 @@ CP_EXPORT void cpGPUSpaceStep(cpSpace *space, cpFloat dt);
  * (cpShapeUpdate); every constraint's accumulated impulse; and the arbiters
  * of the last step, threaded onto their bodies' arbiter lists as
  * cpSpaceProcessComponents does (src/cpSpaceComponent.c:253-267), so that
- * cpBodyEachArbiter and the cpArbiter getters work. */
+ * cpBodyEachArbiter and the cpArbiter getters work.  They are also entered in
+ * the space's cachedArbiters and arbiters, as a step of the reference leaves
+ * them, so that cpSpaceRemoveShape / cpSpaceRemoveBody unthread them from the
+ * remaining bodies (cpSpaceFilterArbiters, src/cpSpace.c:483-520). */
 CP_EXPORT void cpGPUSpaceSyncHost(cpSpace *space);
 /* The device world behind the space (NULL before the first step), for the
  * cpgpu.h calls that have no cpSpace counterpart (timing, readbacks). */
@@ -86,7 +89,7 @@ CP_EXPORT const cpgpuWorldDesc *cpGPUSpaceDescribe(cpSpace *space);
 typedef struct cpGPUBatch cpGPUBatch;
 /* The n spaces (plain cpSpaces, created with cpSpaceNew) are stepped as one
  * device world; each keeps its own parameters and never interacts with the
- * others.  The batch does not own the spaces. */
+ * others.  The batch does not own the spaces: free the batch before them. */
 CP_EXPORT cpGPUBatch *cpGPUBatchNew(cpSpace **spaces, int n);
 CP_EXPORT cpGPUBatch *cpGPUBatchNewWithPrecision(cpSpace **spaces, int n, int precision);
 CP_EXPORT void cpGPUBatchFree(cpGPUBatch *batch);

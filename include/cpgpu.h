@@ -46,11 +46,20 @@ extern "C" {
 #define CPGPU_SHAPE_POLY 2
 
 /* Constraint types (the joint classes of src/cpPivotJoint.c, cpDampedSpring.c,
- * cpSlideJoint.c, cpGearJoint.c). */
+ * cpSlideJoint.c, cpGearJoint.c, cpPinJoint.c, cpGrooveJoint.c,
+ * cpRotaryLimitJoint.c, cpRatchetJoint.c, cpSimpleMotor.c,
+ * cpDampedRotarySpring.c). */
 #define CPGPU_JOINT_PIVOT 0
 #define CPGPU_JOINT_DAMPED_SPRING 1
 #define CPGPU_JOINT_SLIDE 2
 #define CPGPU_JOINT_GEAR 3
+#define CPGPU_JOINT_PIN 4
+#define CPGPU_JOINT_GROOVE 5
+#define CPGPU_JOINT_ROTARY_LIMIT 6
+#define CPGPU_JOINT_RATCHET 7
+#define CPGPU_JOINT_SIMPLE_MOTOR 8
+#define CPGPU_JOINT_DAMPED_ROTARY_SPRING 9
+#define CPGPU_JOINT_TYPES 10
 
 /* Arbiter states, same numbering as enum cpArbiterState (chipmunk_structs.h:83-95). */
 #define CPGPU_ARB_FIRST_COLLISION 0
@@ -131,13 +140,22 @@ typedef struct cpgpuWorldDesc {
 	const double *cons_max_force, *cons_error_bias, *cons_max_bias;
 	const int32_t *cons_collide_bodies;  /* 0 or 1 */
 	/* [n][8] joint parameters:
-	 *   pivot  {anchorA.x, anchorA.y, anchorB.x, anchorB.y}
-	 *   spring {anchorA.x, anchorA.y, anchorB.x, anchorB.y, restLength, stiffness, damping}
-	 *   slide  {anchorA.x, anchorA.y, anchorB.x, anchorB.y, min, max}
-	 *   gear   {phase, ratio, ratio_inv} */
+	 *   pivot   {anchorA.x, anchorA.y, anchorB.x, anchorB.y}
+	 *   spring  {anchorA.x, anchorA.y, anchorB.x, anchorB.y, restLength, stiffness, damping}
+	 *   slide   {anchorA.x, anchorA.y, anchorB.x, anchorB.y, min, max}
+	 *   gear    {phase, ratio, ratio_inv}
+	 *   pin     {anchorA.x, anchorA.y, anchorB.x, anchorB.y, dist}
+	 *   groove  {grv_a.x, grv_a.y, grv_b.x, grv_b.y, anchorB.x, anchorB.y, grv_n.x, grv_n.y}
+	 *           (grv_n = cpvperp(cpvnormalize(grv_b - grv_a)), as cpGrooveJointInit keeps it)
+	 *   rotary limit {min, max}
+	 *   ratchet {phase, ratchet}
+	 *   simple motor {rate}
+	 *   damped rotary spring {restAngle, stiffness, damping} (the default spring torque) */
 	const double *cons_params;
-	/* [n][2] accumulated impulse carried between steps (pivot jAcc.x/.y,
-	 * spring jAcc, slide jnAcc, gear jAcc); NULL means zero. */
+	/* [n][2] state carried between steps: the accumulated impulse (pivot and
+	 * groove jAcc.x/.y, slide and pin jnAcc, every other joint's jAcc) and, for
+	 * a ratchet, its angle in [1] (cpRatchetJointInit starts it at
+	 * b->a - a->a); NULL means zero. */
 	const double *cons_jacc;
 } cpgpuWorldDesc;
 
@@ -249,6 +267,50 @@ int cpgpuWorldReadPairs(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t 
  * order as cpShapesCollide returns them.  Blocks until the results are on the
  * host.  Stages the call in device memory it allocates and frees. */
 int cpgpuWorldShapesCollide(cpgpuWorld *world, int64_t n, const int32_t *pairs, int32_t *count, double *sets);
+/* ---- spatial queries (replace cpSpacePointQueryNearest, cpSpaceSegmentQueryFirst,
+ * cpSpacePointQuery, cpSpaceSegmentQuery and cpSpaceBBQuery, src/cpSpaceQuery.c:23-206)
+ * on the shapes as the last step cached them (as cached at creation before
+ * the first step), n queries at once, one device lane each.  space[n] names
+ * each query's space (NULL: space 0); filter[n] is its cpShapeFilter (NULL:
+ * CP_SHAPE_FILTER_ALL).  Every shape whose exact check passes is found, so
+ * results are those of checking every shape; of equal nearest distances or
+ * first alphas the lowest shape index is kept, and a segment with a radius
+ * also finds shapes within the radius that the reference's tree prunes (it
+ * tests node boxes against the zero-width segment).  Each call blocks until
+ * its results are on the host and stages its arrays in device memory it
+ * allocates and frees. */
+typedef struct cpgpuShapeFilter {
+	uint64_t group;        /* cpShapeFilter.group */
+	uint32_t categories;   /* cpShapeFilter.categories */
+	uint32_t mask;         /* cpShapeFilter.mask */
+} cpgpuShapeFilter;
+/* cpSpacePointQueryNearest: points [n][2], max_distance [n]; shape_out [n]
+ * (global shape index, -1 for none), info_out [n][5] {point.x, point.y,
+ * distance, gradient.x, gradient.y} (cpPointQueryInfo; with no shape
+ * {0, 0, max_distance, 0, 0}). */
+int cpgpuWorldQueryPointNearest(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points,
+                                const double *max_distance, const cpgpuShapeFilter *filter, int32_t *shape_out,
+                                double *info_out);
+/* cpSpaceSegmentQueryFirst: starts, ends [n][2], radius [n]; shape_out [n],
+ * info_out [n][5] {point.x, point.y, normal.x, normal.y, alpha}
+ * (cpSegmentQueryInfo; with no shape {end, 0, 0, 1}). */
+int cpgpuWorldQuerySegmentFirst(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts,
+                                const double *ends, const double *radius, const cpgpuShapeFilter *filter,
+                                int32_t *shape_out, double *info_out);
+/* cpSpacePointQuery / cpSpaceSegmentQuery / cpSpaceBBQuery: every hit of
+ * every query, hits [cap][2] {query, shape} sorted by query then shape, with
+ * info [cap][5] as above (point queries: {point, distance, gradient};
+ * segment queries: {point, normal, alpha}; box queries: none, info may be
+ * NULL).  The total goes to *n_out; when it exceeds cap, call again with
+ * room for it.  Boxes are {l, b, r, t} [n][4]. */
+int cpgpuWorldQueryPointAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points,
+                            const double *max_distance, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out,
+                            int32_t *hits, double *info);
+int cpgpuWorldQuerySegmentAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts,
+                              const double *ends, const double *radius, const cpgpuShapeFilter *filter, int64_t cap,
+                              int64_t *n_out, int32_t *hits, double *info);
+int cpgpuWorldQueryBBAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *boxes,
+                         const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits);
 /* Copy constraint accumulated impulses ([n][2], see cons_jacc) and colours. */
 int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour);
 
@@ -295,6 +357,22 @@ int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_
 #define CPGPU_GRID_PHASES 1
 int cpgpuWorldSetGridSchedule(cpgpuWorld *world, int32_t schedule);
 
+/* Steps captured in HIP graphs (the north star's one launch sequence per step
+ * for a batch; no counterpart in the reference).  With graphs on, each
+ * cpgpuWorldStep launches one executable graph that holds the whole step --
+ * the per-space scalars (stamp, dt_coef) advanced on the device, integration,
+ * broadphase, narrowphase, persistence and the bucketed solve -- with no host
+ * round trip: launches are sized for buffer capacities of twice the largest
+ * counts seen, the kernels read the counts on the device.  The graphs are
+ * captured on the first step and again when dt, the tiling, a space's
+ * parameters or (seen after a step has finished) a count above half its
+ * capacity call for it.  A step that still outgrew a capacity raises the
+ * device error word (reported by the next synchronising call).  Results are
+ * bit for bit those of steps without graphs.  Not with a taped rollout, with
+ * per-launch timing, or for spaces of more than 4096 dynamic bodies (the grid
+ * solve). */
+int cpgpuWorldSetGraph(cpgpuWorld *world, int32_t enable);
+
 /* ---- differentiation (no counterpart in the reference: cpFloat is a plain
  * double there).  cpgpuWorldTapeBegin starts recording the rollout: every
  * later cpgpuWorldStep records its solved arbiters, and the bodies' state is
@@ -310,7 +388,8 @@ int cpgpuWorldSetGridSchedule(cpgpuWorld *world, int32_t schedule);
  * set and the clamp branches of the rollout are held fixed (derivatives
  * where they change are one-sided).  Joints (pivot, damped spring, slide,
  * gear) are differentiated too; their accumulated impulses carry adjoints
- * between steps like the contacts' warm starts.  Spaces must be solved one
+ * between steps like the contacts' warm starts; a world with any other
+ * joint type is refused.  Spaces must be solved one
  * workgroup each; polygons of at most 8 vertices; CPGPU_FP64 worlds; no
  * velocity or force writes during a taped rollout. */
 int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every);

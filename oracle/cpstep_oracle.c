@@ -222,6 +222,8 @@ typedef struct {
 	V r1, r2, n, bias2, jAcc2;
 	num k11, k12, k21, k22;
 	num nMass, bias, jAcc, jnAcc, target_vrn, v_coef, iSum;
+	num clamp;   /* groove: which end of the groove the anchor rests on (+1, -1) or 0 */
+	num angle;   /* ratchet: the angle of the last notch passed (carried between steps) */
 	int colour;
 	uint64_t prio;
 } OJoint;
@@ -371,6 +373,7 @@ oworld *ow_new(const cpgpuWorldDesc *d)
 		num j0 = d->cons_jacc ? d->cons_jacc[2 * i] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * i + 1] : 0.0;
 		j->jAcc2 = v2(j0, j1);
 		j->jAcc = j0; j->jnAcc = j0;
+		j->angle = j1;
 		/* colouring priority from the constraint's index within its own space */
 		j->prio = mix64((uint64_t)(uint32_t)(i - w->W[w->B[j->a].world].c0) | 0x8000000000000000ull);
 	}
@@ -1699,11 +1702,42 @@ static void arb_apply_impulse(oworld *w, OArb *arb)
 
 /* ------------------------------------------------------------------ */
 /* joints: src/cpPivotJoint.c:24-70, cpDampedSpring.c:24-77,
- * cpSlideJoint.c:24-89, cpGearJoint.c:24-69; bias_coef
+ * cpSlideJoint.c:24-89, cpGearJoint.c:24-69, cpPinJoint.c:24-71,
+ * cpGrooveJoint.c:24-93, cpRotaryLimitJoint.c:24-83, cpRatchetJoint.c:24-89,
+ * cpSimpleMotor.c:24-67, cpDampedRotarySpring.c:24-72; bias_coef
  * chipmunk_private.h:264-268 (libm pow, evaluated on the host);
  * k_tensor chipmunk_private.h:228-262                                   */
 
 static num bias_coef_host(num errorBias, num dt) { return 1.0 - pow(NUM_VAL(errorBias), NUM_VAL(dt)); }
+
+/* k_tensor (chipmunk_private.h:228-262): the inverse of the 2x2 effective mass */
+static void k_tensor(const OBody *a, const OBody *b, V r1, V r2, OJoint *j)
+{
+	num m_sum = a->m_inv + b->m_inv;
+	num k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
+	num a_i_inv = a->i_inv;
+	num r1xsq = r1.x * r1.x * a_i_inv;
+	num r1ysq = r1.y * r1.y * a_i_inv;
+	num r1nxy = -r1.x * r1.y * a_i_inv;
+	k11 += r1ysq; k12 += r1nxy; k21 += r1nxy; k22 += r1xsq;
+	num b_i_inv = b->i_inv;
+	num r2xsq = r2.x * r2.x * b_i_inv;
+	num r2ysq = r2.y * r2.y * b_i_inv;
+	num r2nxy = -r2.x * r2.y * b_i_inv;
+	k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
+	num det = k11 * k22 - k12 * k21;
+	num det_inv = 1.0 / det;
+	j->k11 = k22 * det_inv; j->k12 = -k12 * det_inv;
+	j->k21 = -k21 * det_inv; j->k22 = k11 * det_inv;
+}
+
+/* the angular impulse j between a and b (rotary limit, ratchet, simple motor:
+ * a->w -= j a->i_inv, b->w += j b->i_inv) */
+static inline void apply_angular(OBody *a, OBody *b, num jj)
+{
+	a->w -= jj * a->i_inv;
+	b->w += jj * b->i_inv;
+}
 
 static void joint_prestep(oworld *w, OJoint *j, num dt)
 {
@@ -1712,23 +1746,7 @@ static void joint_prestep(oworld *w, OJoint *j, num dt)
 	if (j->type == CPGPU_JOINT_PIVOT) {
 		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
 		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
-		/* k_tensor */
-		num m_sum = a->m_inv + b->m_inv;
-		num k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
-		num a_i_inv = a->i_inv;
-		num r1xsq = j->r1.x * j->r1.x * a_i_inv;
-		num r1ysq = j->r1.y * j->r1.y * a_i_inv;
-		num r1nxy = -j->r1.x * j->r1.y * a_i_inv;
-		k11 += r1ysq; k12 += r1nxy; k21 += r1nxy; k22 += r1xsq;
-		num b_i_inv = b->i_inv;
-		num r2xsq = j->r2.x * j->r2.x * b_i_inv;
-		num r2ysq = j->r2.y * j->r2.y * b_i_inv;
-		num r2nxy = -j->r2.x * j->r2.y * b_i_inv;
-		k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
-		num det = k11 * k22 - k12 * k21;
-		num det_inv = 1.0 / det;
-		j->k11 = k22 * det_inv; j->k12 = -k12 * det_inv;
-		j->k21 = -k21 * det_inv; j->k22 = k11 * det_inv;
+		k_tensor(a, b, j->r1, j->r2, j);
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
 		j->bias2 = vclamp(vmult(delta, -bias_coef_host(j->errorBias, dt) / dt), j->maxBias);
 	} else if (j->type == CPGPU_JOINT_DAMPED_SPRING) {
@@ -1758,11 +1776,78 @@ static void joint_prestep(oworld *w, OJoint *j, num dt)
 		j->nMass = 1.0 / k_scalar(a, b, j->r1, j->r2, j->n);
 		num maxBias = j->maxBias;
 		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * pdist / dt, -maxBias, maxBias);
-	} else {
+	} else if (j->type == CPGPU_JOINT_GEAR) {
 		num phase = j->P[0], ratio = j->P[1], ratio_inv = j->P[2];
 		j->iSum = 1.0 / (a->i_inv * ratio_inv + ratio * b->i_inv);
 		num maxBias = j->maxBias;
 		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * (b->a * ratio - a->a - phase) / dt, -maxBias, maxBias);
+	} else if (j->type == CPGPU_JOINT_PIN) {
+		num dist_rest = j->P[4];
+		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
+		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
+		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
+		num dist = vlength(delta);
+		j->n = vmult(delta, 1.0 / (dist != 0.0 ? dist : INFINITY));
+		j->nMass = 1.0 / k_scalar(a, b, j->r1, j->r2, j->n);
+		num maxBias = j->maxBias;
+		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * (dist - dist_rest) / dt, -maxBias, maxBias);
+	} else if (j->type == CPGPU_JOINT_GROOVE) {
+		/* P: groove ends {a.x, a.y, b.x, b.y} and anchorB on body B, groove normal n */
+		V grv_a = v2(j->P[0], j->P[1]), grv_b = v2(j->P[2], j->P[3]);
+		V anchor = v2(j->P[4], j->P[5]), grv_n = v2(j->P[6], j->P[7]);
+		V ta = xpoint(a->T, grv_a), tb = xpoint(a->T, grv_b);
+		V n = xvect(a->T, grv_n);
+		num d = vdot(ta, n);
+		j->n = n;
+		j->r2 = xvect(b->T, vsub(anchor, b->cog));
+		num td = vcross(vadd(b->p, j->r2), n);
+		if (td <= vcross(ta, n)) {
+			j->clamp = 1.0;
+			j->r1 = vsub(ta, a->p);
+		} else if (td >= vcross(tb, n)) {
+			j->clamp = -1.0;
+			j->r1 = vsub(tb, a->p);
+		} else {
+			j->clamp = 0.0;
+			j->r1 = vsub(vadd(vmult(vperp(n), -td), vmult(n, d)), a->p);
+		}
+		k_tensor(a, b, j->r1, j->r2, j);
+		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
+		j->bias2 = vclamp(vmult(delta, -bias_coef_host(j->errorBias, dt) / dt), j->maxBias);
+	} else if (j->type == CPGPU_JOINT_ROTARY_LIMIT) {
+		num mn = j->P[0], mx = j->P[1];
+		num dist = b->a - a->a;
+		num pdist = 0.0;
+		if (dist > mx) pdist = mx - dist;
+		else if (dist < mn) pdist = mn - dist;
+		j->iSum = 1.0 / (a->i_inv + b->i_inv);
+		num maxBias = j->maxBias;
+		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * pdist / dt, -maxBias, maxBias);
+		if (!j->bias) j->jAcc = 0.0;
+	} else if (j->type == CPGPU_JOINT_RATCHET) {
+		num phase = j->P[0], ratchet = j->P[1];
+		num delta = b->a - a->a;
+		num diff = j->angle - delta;
+		num pdist = 0.0;
+		if (diff * ratchet > 0.0) pdist = diff;
+		else j->angle = floor((delta - phase) / ratchet) * ratchet + phase;
+		j->iSum = 1.0 / (a->i_inv + b->i_inv);
+		num maxBias = j->maxBias;
+		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * pdist / dt, -maxBias, maxBias);
+		if (!j->bias) j->jAcc = 0.0;
+	} else if (j->type == CPGPU_JOINT_SIMPLE_MOTOR) {
+		j->iSum = 1.0 / (a->i_inv + b->i_inv);
+	} else {
+		/* damped rotary spring with the default torque (relativeAngle - restAngle) stiffness */
+		num restAngle = j->P[0], stiffness = j->P[1], damping = j->P[2];
+		num moment = a->i_inv + b->i_inv;
+		j->iSum = 1.0 / moment;
+		j->v_coef = 1.0 - (w->libm_trig ? exp(-damping * dt * moment) : cp_exp_oracle(-damping * dt * moment));
+		j->target_vrn = 0.0;
+		num j_spring = ((a->a - b->a) - restAngle) * stiffness * dt;
+		j->jAcc = j_spring;
+		a->w -= j_spring * a->i_inv;
+		b->w += j_spring * b->i_inv;
 	}
 }
 
@@ -1779,6 +1864,12 @@ static void joint_apply_cached(oworld *w, OJoint *j, num dt_coef)
 		num jj = j->jAcc * dt_coef;
 		a->w -= jj * a->i_inv * ratio_inv;
 		b->w += jj * b->i_inv;
+	} else if (j->type == CPGPU_JOINT_PIN) {
+		apply_impulses(a, b, j->r1, j->r2, vmult(j->n, j->jnAcc * dt_coef));
+	} else if (j->type == CPGPU_JOINT_GROOVE) {
+		apply_impulses(a, b, j->r1, j->r2, vmult(j->jAcc2, dt_coef));
+	} else if (j->type != CPGPU_JOINT_DAMPED_SPRING && j->type != CPGPU_JOINT_DAMPED_ROTARY_SPRING) {
+		apply_angular(a, b, j->jAcc * dt_coef);
 	}
 }
 
@@ -1812,7 +1903,7 @@ static void joint_apply_impulse(oworld *w, OJoint *j, num dt)
 		j->jnAcc = fclampc(jnOld + jn, -j->maxForce * dt, 0.0);
 		jn = j->jnAcc - jnOld;
 		apply_impulses(a, b, j->r1, j->r2, vmult(n, jn));
-	} else {
+	} else if (j->type == CPGPU_JOINT_GEAR) {
 		num ratio = j->P[1], ratio_inv = j->P[2];
 		num wr = b->w * ratio - a->w;
 		num jMax = j->maxForce * dt;
@@ -1822,6 +1913,57 @@ static void joint_apply_impulse(oworld *w, OJoint *j, num dt)
 		jj = j->jAcc - jOld;
 		a->w -= jj * a->i_inv * ratio_inv;
 		b->w += jj * b->i_inv;
+	} else if (j->type == CPGPU_JOINT_PIN) {
+		V n = j->n;
+		num vrn = vdot(rel_vel(a, b, j->r1, j->r2), n);
+		num jnMax = j->maxForce * dt;
+		num jn = (j->bias - vrn) * j->nMass;
+		num jnOld = j->jnAcc;
+		j->jnAcc = fclampc(jnOld + jn, -jnMax, jnMax);
+		jn = j->jnAcc - jnOld;
+		apply_impulses(a, b, j->r1, j->r2, vmult(n, jn));
+	} else if (j->type == CPGPU_JOINT_GROOVE) {
+		V vr = rel_vel(a, b, j->r1, j->r2);
+		V d = vsub(j->bias2, vr);
+		V jj = v2(j->k11 * d.x + j->k12 * d.y, j->k21 * d.x + j->k22 * d.y);
+		V jOld = j->jAcc2;
+		/* grooveConstrain (cpGrooveJoint.c:74-79) */
+		V jc = vadd(jOld, jj);
+		V n = j->n;
+		V jClamp = (j->clamp * vcross(jc, n) > 0.0) ? jc : vmult(n, vdot(jc, n) / vdot(n, n));
+		j->jAcc2 = vclamp(jClamp, j->maxForce * dt);
+		jj = vsub(j->jAcc2, jOld);
+		apply_impulses(a, b, j->r1, j->r2, jj);
+	} else if (j->type == CPGPU_JOINT_ROTARY_LIMIT || j->type == CPGPU_JOINT_RATCHET) {
+		if (!j->bias) return;
+		num wr = b->w - a->w;
+		num jMax = j->maxForce * dt;
+		num jj = -(j->bias + wr) * j->iSum;
+		num jOld = j->jAcc;
+		if (j->type == CPGPU_JOINT_RATCHET) {
+			num ratchet = j->P[1];
+			j->jAcc = fclampc((jOld + jj) * ratchet, 0.0, jMax * fabs(ratchet)) / ratchet;
+		} else if (j->bias < 0.0) {
+			j->jAcc = fclampc(jOld + jj, 0.0, jMax);
+		} else {
+			j->jAcc = fclampc(jOld + jj, -jMax, 0.0);
+		}
+		apply_angular(a, b, j->jAcc - jOld);
+	} else if (j->type == CPGPU_JOINT_SIMPLE_MOTOR) {
+		num wr = b->w - a->w + j->P[0];
+		num jMax = j->maxForce * dt;
+		num jj = -wr * j->iSum;
+		num jOld = j->jAcc;
+		j->jAcc = fclampc(jOld + jj, -jMax, jMax);
+		apply_angular(a, b, j->jAcc - jOld);
+	} else {
+		num wrn = a->w - b->w;
+		num w_damp = (j->target_vrn - wrn) * j->v_coef;
+		j->target_vrn = wrn + w_damp;
+		num j_damp = w_damp * j->iSum;
+		j->jAcc += j_damp;
+		a->w += j_damp * a->i_inv;
+		b->w -= j_damp * b->i_inv;
 	}
 }
 
@@ -2279,7 +2421,9 @@ int ow_read_constraints(oworld *w, double *jacc, int32_t *colour)
 		OJoint *j = &w->J[i];
 		if (jacc) {
 			if (j->type == CPGPU_JOINT_PIVOT) { jacc[2 * i] = NUM_VAL(j->jAcc2.x); jacc[2 * i + 1] = NUM_VAL(j->jAcc2.y); }
-			else if (j->type == CPGPU_JOINT_SLIDE) { jacc[2 * i] = NUM_VAL(j->jnAcc); jacc[2 * i + 1] = 0.0; }
+			else if (j->type == CPGPU_JOINT_GROOVE) { jacc[2 * i] = NUM_VAL(j->jAcc2.x); jacc[2 * i + 1] = NUM_VAL(j->jAcc2.y); }
+			else if (j->type == CPGPU_JOINT_SLIDE || j->type == CPGPU_JOINT_PIN) { jacc[2 * i] = NUM_VAL(j->jnAcc); jacc[2 * i + 1] = 0.0; }
+			else if (j->type == CPGPU_JOINT_RATCHET) { jacc[2 * i] = NUM_VAL(j->jAcc); jacc[2 * i + 1] = NUM_VAL(j->angle); }
 			else { jacc[2 * i] = NUM_VAL(j->jAcc); jacc[2 * i + 1] = 0.0; }
 		}
 		if (colour) colour[i] = j->colour;

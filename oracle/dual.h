@@ -60,5 +60,7 @@ inline Dual exp(Dual a)
 	const double e = ::exp(a.v);
 	return Dual(e, e * a.d);
 }
+/* piecewise constant: zero derivative between the integers (a ratchet's notch) */
+inline Dual floor(Dual a) { return Dual(::floor(a.v), 0.0); }
 
 #endif

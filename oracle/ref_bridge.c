@@ -168,6 +168,16 @@ static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi)
 		if (t == CPGPU_JOINT_PIVOT) c = cpPivotJointNew2(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]));
 		else if (t == CPGPU_JOINT_DAMPED_SPRING) c = cpDampedSpringNew(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]), P[4], P[5], P[6]);
 		else if (t == CPGPU_JOINT_SLIDE) c = cpSlideJointNew(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]), P[4], P[5]);
+		else if (t == CPGPU_JOINT_PIN) {
+			c = cpPinJointNew(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]));
+			cpPinJointSetDist(c, P[4]);
+		}
+		/* the groove's normal is the reference's own (cpGrooveJointInit), not the description's */
+		else if (t == CPGPU_JOINT_GROOVE) c = cpGrooveJointNew(a, b, cpv(P[0], P[1]), cpv(P[2], P[3]), cpv(P[4], P[5]));
+		else if (t == CPGPU_JOINT_ROTARY_LIMIT) c = cpRotaryLimitJointNew(a, b, P[0], P[1]);
+		else if (t == CPGPU_JOINT_RATCHET) c = cpRatchetJointNew(a, b, P[0], P[1]);
+		else if (t == CPGPU_JOINT_SIMPLE_MOTOR) c = cpSimpleMotorNew(a, b, P[0]);
+		else if (t == CPGPU_JOINT_DAMPED_ROTARY_SPRING) c = cpDampedRotarySpringNew(a, b, P[0], P[1], P[2]);
 		else {
 			c = cpGearJointNew(a, b, P[0], P[1]);
 			((cpGearJoint *)c)->ratio_inv = P[2];
@@ -176,11 +186,18 @@ static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi)
 		cpConstraintSetErrorBias(c, d->cons_error_bias[i]);
 		cpConstraintSetMaxBias(c, d->cons_max_bias[i]);
 		cpConstraintSetCollideBodies(c, (cpBool)d->cons_collide_bodies[i]);
-		if (d->cons_jacc) {
-			double j0 = d->cons_jacc[2 * i], j1 = d->cons_jacc[2 * i + 1];
+		{
+			/* the carried state; no description state is zero (a ratchet's angle too) */
+			double j0 = d->cons_jacc ? d->cons_jacc[2 * i] : 0.0, j1 = d->cons_jacc ? d->cons_jacc[2 * i + 1] : 0.0;
 			if (t == CPGPU_JOINT_PIVOT) ((cpPivotJoint *)c)->jAcc = cpv(j0, j1);
 			else if (t == CPGPU_JOINT_DAMPED_SPRING) ((cpDampedSpring *)c)->jAcc = j0;
 			else if (t == CPGPU_JOINT_SLIDE) ((cpSlideJoint *)c)->jnAcc = j0;
+			else if (t == CPGPU_JOINT_PIN) ((cpPinJoint *)c)->jnAcc = j0;
+			else if (t == CPGPU_JOINT_GROOVE) ((cpGrooveJoint *)c)->jAcc = cpv(j0, j1);
+			else if (t == CPGPU_JOINT_ROTARY_LIMIT) ((cpRotaryLimitJoint *)c)->jAcc = j0;
+			else if (t == CPGPU_JOINT_RATCHET) { ((cpRatchetJoint *)c)->jAcc = j0; cpRatchetJointSetAngle(c, j1); }
+			else if (t == CPGPU_JOINT_SIMPLE_MOTOR) ((cpSimpleMotor *)c)->jAcc = j0;
+			else if (t == CPGPU_JOINT_DAMPED_ROTARY_SPRING) ((cpDampedRotarySpring *)c)->jAcc = j0;
 			else ((cpGearJoint *)c)->jAcc = j0;
 		}
 		c->userData = (cpDataPointer)(intptr_t)i;
@@ -286,6 +303,12 @@ void rb_read_constraints(rbWorld *w, double *jacc)
 		if (t == CPGPU_JOINT_PIVOT) { j0 = ((cpPivotJoint *)c)->jAcc.x; j1 = ((cpPivotJoint *)c)->jAcc.y; }
 		else if (t == CPGPU_JOINT_DAMPED_SPRING) j0 = ((cpDampedSpring *)c)->jAcc;
 		else if (t == CPGPU_JOINT_SLIDE) j0 = ((cpSlideJoint *)c)->jnAcc;
+		else if (t == CPGPU_JOINT_PIN) j0 = ((cpPinJoint *)c)->jnAcc;
+		else if (t == CPGPU_JOINT_GROOVE) { j0 = ((cpGrooveJoint *)c)->jAcc.x; j1 = ((cpGrooveJoint *)c)->jAcc.y; }
+		else if (t == CPGPU_JOINT_ROTARY_LIMIT) j0 = ((cpRotaryLimitJoint *)c)->jAcc;
+		else if (t == CPGPU_JOINT_RATCHET) { j0 = ((cpRatchetJoint *)c)->jAcc; j1 = cpRatchetJointGetAngle(c); }
+		else if (t == CPGPU_JOINT_SIMPLE_MOTOR) j0 = ((cpSimpleMotor *)c)->jAcc;
+		else if (t == CPGPU_JOINT_DAMPED_ROTARY_SPRING) j0 = ((cpDampedRotarySpring *)c)->jAcc;
 		else j0 = ((cpGearJoint *)c)->jAcc;
 		jacc[2 * k] = j0; jacc[2 * k + 1] = j1;
 	}
@@ -465,6 +488,12 @@ void rb_constraint_jacc(const cpConstraint *c, double *out)
 	else if (cpConstraintIsDampedSpring(c)) out[0] = ((cpDampedSpring *)c)->jAcc;
 	else if (cpConstraintIsSlideJoint(c)) out[0] = ((cpSlideJoint *)c)->jnAcc;
 	else if (cpConstraintIsGearJoint(c)) out[0] = ((cpGearJoint *)c)->jAcc;
+	else if (cpConstraintIsPinJoint(c)) out[0] = ((cpPinJoint *)c)->jnAcc;
+	else if (cpConstraintIsGrooveJoint(c)) { out[0] = ((cpGrooveJoint *)c)->jAcc.x; out[1] = ((cpGrooveJoint *)c)->jAcc.y; }
+	else if (cpConstraintIsRotaryLimitJoint(c)) out[0] = ((cpRotaryLimitJoint *)c)->jAcc;
+	else if (cpConstraintIsRatchetJoint(c)) { out[0] = ((cpRatchetJoint *)c)->jAcc; out[1] = cpRatchetJointGetAngle(c); }
+	else if (cpConstraintIsSimpleMotor(c)) out[0] = ((cpSimpleMotor *)c)->jAcc;
+	else if (cpConstraintIsDampedRotarySpring(c)) out[0] = ((cpDampedRotarySpring *)c)->jAcc;
 }
 
 typedef struct ArbList {

@@ -14,6 +14,7 @@ from chipmunk2d_amd.cabi import PackedWorld, SpaceParams, Stats, WorldDesc, ptr
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 GPUSPACE_SO = os.path.join(ROOT, "oracle", "_ref", "libcpgpuspace.so")
+GPUSPACE_CPU_SO = os.path.join(ROOT, "oracle", "_ref", "libcpgpuspace_cpu.so")
 DUAL_SO = os.path.join(ROOT, "oracle", "libdualstep.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref.so")
 REF_RELEASE_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref_release.so")

@@ -2,16 +2,19 @@
 include/cpGPUSpace.h) and on the Chipmunk public API of the compiled
 reference it links (oracle/_ref/libcpref.so) -- test infrastructure.
 
-Both libraries are built by `make -C oracle` where the reference sources are
-present; the binding library is loaded after the reference library, so both
-use the one copy of Chipmunk.
+The binding comes in two builds: over the device library ("gpu") and over the
+oracle's CPU step ("cpu", oracle/_ref/libcpgpuspace_cpu.so, whose world
+handles answer the same cpgpuWorld* readers), so its host logic is tested
+without a GPU.  Both are built by `make -C oracle` where the reference sources
+are present; each binding library is loaded after the reference library, so
+all of them use the one copy of Chipmunk.
 """
 import ctypes
 import os
 
 import numpy as np
 
-from checkers import GPUSPACE_SO, ref_lib
+from checkers import GPUSPACE_SO, GPUSPACE_CPU_SO, ref_lib
 from chipmunk2d_amd.cabi import WorldDesc, ptr
 
 _d = ctypes.c_double
@@ -34,7 +37,7 @@ class cpTransform(ctypes.Structure):
 IDENTITY = cpTransform(1.0, 0.0, 0.0, 1.0, 0.0, 0.0)
 
 _cp = None
-_gs = None
+_gs = {}
 
 
 def cp():
@@ -72,6 +75,10 @@ def cp():
             "cpDampedSpringNew": (_vp, [_vp, _vp, cpVect, cpVect, _d, _d, _d]),
             "cpDampedSpringSetStiffness": (None, [_vp, _d]),
             "cpPinJointNew": (_vp, [_vp, _vp, cpVect, cpVect]),
+            "cpDampedRotarySpringNew": (_vp, [_vp, _vp, _d, _d, _d]),
+            "cpDampedRotarySpringSetSpringTorqueFunc": (None, [_vp, _vp]),
+            "cpSimpleMotorSetRate": (None, [_vp, _d]),
+            "cpSimpleMotorNew": (_vp, [_vp, _vp, _d]),
             "cpConstraintFree": (None, [_vp]),
             "cpMomentForCircle": (_d, [_d, _d, _d, cpVect]), "cpMomentForBox": (_d, [_d, _d, _d]),
         }
@@ -83,12 +90,11 @@ def cp():
     return _cp
 
 
-def gs():
-    """The cpGPUSpace binding."""
-    global _gs
-    if _gs is None:
+def gs(kind="gpu"):
+    """The cpGPUSpace binding over the device library ("gpu") or over the oracle ("cpu")."""
+    if kind not in _gs:
         cp()   # the reference first: the binding resolves Chipmunk to the same copy
-        L = ctypes.CDLL(GPUSPACE_SO)
+        L = ctypes.CDLL({"gpu": GPUSPACE_SO, "cpu": GPUSPACE_CPU_SO}[kind])
         sig = {
             "cpGPUSpaceNew": (_vp, []), "cpGPUSpaceNewWithPrecision": (_vp, [ctypes.c_int]),
             "cpGPUSpaceFree": (None, [_vp]), "cpGPUSpaceStep": (None, [_vp, _d]),
@@ -108,16 +114,27 @@ def gs():
             "cpGPUSetAbortOnError": (None, [ctypes.c_int]), "cpGPUGetError": (ctypes.c_char_p, []),
             "cpGPUClearError": (None, []),
         }
+        if kind == "cpu":
+            # the oracle-backed world's readers, under the device library's signatures
+            _i64p = ctypes.POINTER(ctypes.c_int64)
+            _dp = ctypes.POINTER(ctypes.c_double)
+            sig.update({
+                "cpgpuWorldReadArbiters": (ctypes.c_int, [_vp, ctypes.c_int64, _i64p, _ip, _ip, _ip, _ip, _dp, _dp,
+                                                          ctypes.POINTER(ctypes.c_uint64)]),
+                "cpgpuWorldReadConstraints": (ctypes.c_int, [_vp, _dp, _ip]),
+                "cpgpuWorldSync": (ctypes.c_int, [_vp]),
+                "cpgpuGetLastError": (ctypes.c_char_p, []),
+            })
         for name, (res, args) in sig.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _gs = L
-    return _gs
+        _gs[kind] = L
+    return _gs[kind]
 
 
-def available():
-    return os.path.exists(GPUSPACE_SO)
+def available(kind="gpu"):
+    return os.path.exists({"gpu": GPUSPACE_SO, "cpu": GPUSPACE_CPU_SO}[kind])
 
 
 def v(x, y):

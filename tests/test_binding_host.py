@@ -10,7 +10,7 @@ import pytest
 
 import gpuspace
 from checkers import packed_from_desc, ref_lib
-from chipmunk2d_amd.scenes import chain_world, pyramid_210, random_world
+from chipmunk2d_amd.scenes import chain_world, machine_world, pyramid_210, random_world
 from gpuspace import cp, gs, v
 
 pytestmark = pytest.mark.skipif(not gpuspace.available(),
@@ -57,11 +57,11 @@ def _assert_describes(pw, d, j):
         assert d.a["space"][0][k] == pw.a["space"][0][k], k
 
 
-@pytest.mark.parametrize("scene", ["pyramid", "random", "chain"])
+@pytest.mark.parametrize("scene", ["pyramid", "random", "chain", "machines"])
 def test_describe_equals_the_description_the_space_was_built_from(scene):
     pw = {"pyramid": lambda: pyramid_210(rows=8), "random": lambda: random_world(
         n_circles=80, n_polys=80, n_segments=20, extent=40.0, seed=9), "chain": lambda: chain_world(
-        15, 40, seed=5)}[scene]().pack()
+        15, 40, seed=5), "machines": lambda: machine_world(3, 10)}[scene]().pack()
     space = gs().cpGPUSpaceNew()
     desc = pw.desc()
     rb = ref_lib().rb_new_in(space, ctypes.byref(desc), 0)
@@ -116,11 +116,16 @@ def _free(space, body, shape):
     gs().cpGPUSpaceFree(space)
 
 
-@pytest.mark.parametrize("feature", ["sensor", "sleeping", "pin joint"])
+_TORQUE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
+_custom_torque = _TORQUE(lambda c, angle: 2.0 * angle)
+
+
+@pytest.mark.parametrize("feature", ["sensor", "sleeping", "custom spring torque"])
 def test_unsupported_features_are_refused_at_pack_time(feature):
-    """Sensors, sleeping and joints other than pivot, damped spring, slide and
-    gear are refused when the space is packed (the step does nothing); the
-    message is cpAssertHard's (which aborts when cpGPUSetAbortOnError is on)."""
+    """Sensors, sleeping and a damped rotary spring with a torque function of
+    the caller's are refused when the space is packed (the step does nothing);
+    the message is cpAssertHard's (which aborts when cpGPUSetAbortOnError is
+    on)."""
     C = cp()
     space, body, shape = _space_with_one_circle()
     extra = []
@@ -130,13 +135,14 @@ def test_unsupported_features_are_refused_at_pack_time(feature):
     elif feature == "sleeping":
         C.cpSpaceSetSleepTimeThreshold(space, 1.0)
         msg = "sleeping"
-    elif feature == "pin joint":
+    elif feature == "custom spring torque":
         other = C.cpBodyNew(1.0, 1.0)
         C.cpSpaceAddBody(space, other)
-        pin = C.cpPinJointNew(body, other, v(0, 0), v(0, 0))
-        C.cpSpaceAddConstraint(space, pin)
-        extra = [("c", pin), ("b", other)]
-        msg = "only pivot"
+        spring = C.cpDampedRotarySpringNew(body, other, 0.0, 10.0, 1.0)
+        C.cpDampedRotarySpringSetSpringTorqueFunc(spring, _custom_torque)
+        C.cpSpaceAddConstraint(space, spring)
+        extra = [("c", spring), ("b", other)]
+        msg = "custom damped-rotary-spring torque"
     assert not gs().cpGPUSpaceDescribe(space)
     assert msg in gs().cpGPUGetError().decode(), gs().cpGPUGetError()
     gs().cpGPUClearError()

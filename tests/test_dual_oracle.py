@@ -12,7 +12,7 @@ differences); it is pinned here before it checks anything:
 import numpy as np
 import pytest
 
-from chipmunk2d_amd.scenes import Rng, chain_world, grad_world, pyramid_210, random_world
+from chipmunk2d_amd.scenes import Rng, chain_world, grad_world, machine_world, pyramid_210, random_world
 from chipmunk2d_amd.cabi import concat
 from chipmunk2d_amd import grad as G
 from chipmunk2d_amd.space import Space, Body, Circle, moment_for_circle
@@ -31,12 +31,13 @@ def _grad_batch(n_worlds, n_bodies, seed):
     return concat(packs), np.vstack(targets)
 
 
-@pytest.mark.parametrize("scene", ["pyramid", "random", "chain", "grad"])
+@pytest.mark.parametrize("scene", ["pyramid", "random", "chain", "grad", "machines"])
 def test_value_parts_equal_the_double_oracle(scene):
     pw = {"pyramid": lambda: pyramid_210(rows=10).pack(),
           "random": lambda: random_world(n_circles=150, n_polys=150, n_segments=30, extent=50.0, seed=81).pack(),
           "chain": lambda: chain_world(20, 40, seed=9).pack(),
-          "grad": lambda: _grad_batch(4, 16, 0xD1)[0]}[scene]()
+          "grad": lambda: _grad_batch(4, 16, 0xD1)[0],
+          "machines": lambda: machine_world(n_units=4, n_circles=30).pack()}[scene]()
     rng = np.random.default_rng(1)
     o = OracleWorld(pw)
     d = DualWorld(pw, dv=rng.normal(size=(pw.n_bodies, 2)), dm=rng.normal(size=pw.n_bodies),

@@ -140,8 +140,8 @@ def _world_loss_diffs(p_plus, p_minus, targets, dyn, world_of_body, n_worlds):
 
 def test_grad_1024_adjoint_matches_central_differences():
     """The loss is the sum over bodies of |p_T - target|^2.  Space k perturbs
-    one parameter (kind k mod 3: an initial velocity component, a mass, a
-    friction coefficient), so the k-th space's loss difference is the central
+    one parameter (an initial velocity component in half of the spaces, a
+    mass or a friction coefficient in a quarter each), so the k-th space's loss difference is the central
     difference of that parameter.  Spaces whose contact set changes under any
     perturbation are left out (the contact set is held fixed by the adjoint),
     and so are those whose differences at eps and 2 eps disagree (a clamp of
@@ -168,7 +168,8 @@ def test_grad_1024_adjoint_matches_central_differences():
     cases = []
     for k in range(n_worlds):
         bodies = np.nonzero(dyn & (world_of_body == k))[0]
-        kind = kinds[k % 3]
+        # velocities, whose central differences are the noisiest, in half of the spaces
+        kind = ("v0", "m", "v0", "u")[k % 4]
         if kind == "v0":
             cases.append((kind, (int(rng.choice(bodies)), int(rng.integers(2)))))
         elif kind == "m":

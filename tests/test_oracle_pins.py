@@ -748,3 +748,39 @@ def test_joints_replay_matches_reference():
     jo, _ = o.constraints()
     assert np.array_equal(jr, jo)
     assert np.count_nonzero(jr) > 50
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [0xC4100006, 77])
+def test_every_joint_class_replay_matches_reference(seed):
+    """Pin, groove, rotary limit, ratchet, simple motor and damped rotary spring
+    (with pivot, spring, slide and gear) in machine_world: bodies, accumulated
+    impulses and ratchet angles equal the reference's after every step, and
+    every joint class carried an impulse at some step (the ratchet's angle moved,
+    the rotary limit pushed both ways, the slide joints reached a limit)."""
+    from chipmunk2d_amd.scenes import machine_world
+    pw = machine_world(n_units=8, n_circles=60, seed=seed).pack()
+    o = OracleWorld(pw)
+    r = RefWorld(pw)
+    o.libm_trig(True)
+    o.joint_list_order(True)
+    t = pw.a["cons_type"]
+    assert np.array_equal(np.unique(t), np.arange(10))
+    active = np.zeros(10, int)
+    angle = r.constraints()[t == 7, 1].copy()
+    moved = limit_signs = 0
+    for s in range(300):
+        coll = r.step_logged(1 / 60)
+        o.replay(r.arbiters(), coll, 1 / 60)
+        rb, ob = r.bodies(), o.bodies()
+        for f in rb.FIELDS:
+            assert np.array_equal(getattr(rb, f), getattr(ob, f)), (s, f)
+        jr = r.constraints()
+        jo, _ = o.constraints()
+        assert np.array_equal(jr, jo), s
+        active += np.bincount(t[np.any(jr != 0.0, axis=1)], minlength=10)
+        moved += np.count_nonzero(jr[t == 7, 1] != angle)
+        angle = jr[t == 7, 1].copy()
+        limit_signs |= (1 if np.any(jr[t == 6, 0] > 0) else 0) | (2 if np.any(jr[t == 6, 0] < 0) else 0)
+    assert np.all(active > 0), active
+    assert moved > 0 and limit_signs == 3, (moved, limit_signs)

@@ -39,6 +39,8 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldAutotune(cpgpuWorld *world, double dt, int32_t trial_steps, int32_t *threads_out); \
 	int cpgpuWorldSetGridSchedule(cpgpuWorld *world, int32_t schedule); \
 	int cpgpuWorldSetGraph(cpgpuWorld *world, int32_t enable); \
+	int cpgpuWorldSaveState(cpgpuWorld *world, void *buf, int64_t cap, int64_t *size_out); \
+	int cpgpuWorldLoadState(cpgpuWorld *world, const void *buf, int64_t size); \
 	int cpgpuWorldQueryPointNearest(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points, const double *max_distance, const cpgpuShapeFilter *filter, int32_t *shape_out, double *info_out); \
 	int cpgpuWorldQuerySegmentFirst(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts, const double *ends, const double *radius, const cpgpuShapeFilter *filter, int32_t *shape_out, double *info_out); \
 	int cpgpuWorldQueryPointAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points, const double *max_distance, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits, double *info); \
@@ -277,6 +279,17 @@ int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodi
 
 #define W64 reinterpret_cast<cpgpu64::cpgpuWorld *>(world)
 #define W32 reinterpret_cast<cpgpu32::cpgpuWorld *>(world)
+
+int cpgpuWorldSaveState(cpgpuWorld *world, void *buf, int64_t cap, int64_t *size_out)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldSaveState(W64, buf, cap, size_out),
+	              cpgpu32::cpgpuWorldSaveState(W32, buf, cap, size_out));
+}
+
+int cpgpuWorldLoadState(cpgpuWorld *world, const void *buf, int64_t size)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldLoadState(W64, buf, size), cpgpu32::cpgpuWorldLoadState(W32, buf, size));
+}
 
 int cpgpuWorldQueryPointNearest(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points,
                                 const double *max_distance, const cpgpuShapeFilter *filter, int32_t *shape_out,

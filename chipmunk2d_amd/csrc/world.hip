@@ -1550,6 +1550,148 @@ int cpgpuWorldCreateFrom(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld 
 	return CPGPU_OK;
 }
 
+// ---- snapshots of the state a step carries forward (cpgpuWorldSaveState / LoadState) ----
+
+namespace {
+
+struct SnapshotHeader {
+	char magic[8];               // "cpgpuSS1"
+	int32_t version;             // 1
+	int32_t precision;           // CPGPU_FP64 / CPGPU_FP32
+	int32_t n_worlds, n_bodies, n_shapes, n_constraints;
+	int32_t n_recs, n_act, n_cached;   // the last step's pair records, solved and cached arbiters
+	int32_t real_bytes;
+	int32_t sizes[4];            // sizeof PairRec, Arbiter, Joint, BodyVel of the build that wrote it
+	int32_t pad;
+	int64_t steps, total_pairs, last_pairs;
+};
+
+constexpr char SNAP_MAGIC[8] = {'c', 'p', 'g', 'p', 'u', 'S', 'S', '1'};
+constexpr int SNAP_TOTALS = 2 + 2 * CPGPU_SOLVE_UNITS;
+
+// the sections after the header, in order: {device pointer, bytes}
+struct SnapSection {
+	void *dev;
+	size_t bytes;
+};
+
+std::vector<SnapSection> snap_sections(cpgpuWorld *w, int n_recs, int n_act, int n_cached)
+{
+	const int P = w->cur ^ 1;
+	DevWorld &W = w->W;
+	const size_t nb = (size_t)w->nb, ns = (size_t)w->ns;
+	return {
+	    {W.b_p, sizeof(real2) * nb},         {W.b_a, sizeof(real) * nb},       {W.b_f, sizeof(real2) * nb},
+	    {W.b_t, sizeof(real) * nb},          {W.b_T, sizeof(XF) * nb},         {W.b_vel, sizeof(BodyVel) * nb},
+	    {W.s_fat, sizeof(real4) * ns},       {W.s_rebuild, sizeof(uint32_t) * ns},
+	    {W.joints, sizeof(Joint) * (size_t)w->nc},
+	    {w->recs[P], sizeof(PairRec) * (size_t)n_recs}, {w->act[P], sizeof(Arbiter) * (size_t)n_act},
+	    {w->cached[P], sizeof(Arbiter) * (size_t)n_cached}, {W.totals, sizeof(unsigned long long) * SNAP_TOTALS},
+	};
+}
+
+} // namespace
+
+int cpgpuWorldSaveState(cpgpuWorld *w, void *buf, int64_t cap, int64_t *size_out)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (cap < 0) return fail(CPGPU_ERR_INVALID, "negative capacity");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int L = w->cur ^ 1;
+	SnapshotHeader h;
+	memset(&h, 0, sizeof(h));
+	memcpy(h.magic, SNAP_MAGIC, 8);
+	h.version = 1;
+	h.precision = w->precision;
+	h.n_worlds = w->nw; h.n_bodies = w->nb; h.n_shapes = w->ns; h.n_constraints = w->nc;
+	h.n_recs = w->steps > 0 ? w->h_counts[3 * L] : 0;
+	h.n_act = w->steps > 0 ? w->h_counts[3 * L + 1] : 0;
+	h.n_cached = w->steps > 0 ? w->h_counts[3 * L + 2] : 0;
+	h.real_bytes = (int32_t)sizeof(real);
+	h.sizes[0] = sizeof(PairRec); h.sizes[1] = sizeof(Arbiter); h.sizes[2] = sizeof(Joint); h.sizes[3] = sizeof(BodyVel);
+	h.steps = w->steps; h.total_pairs = w->total_pairs; h.last_pairs = w->last_pairs;
+	const auto sec = snap_sections(w, h.n_recs, h.n_act, h.n_cached);
+	size_t size = sizeof(h) + sizeof(cpgpuSpaceParams) * (size_t)w->nw;
+	for (const auto &q : sec) size += q.bytes;
+	if (size_out) *size_out = (int64_t)size;
+	if (!buf) return CPGPU_OK;
+	if ((size_t)cap < size)
+		return fail(CPGPU_ERR_CAPACITY, "snapshot needs " + std::to_string(size) + " bytes, " + std::to_string(cap) +
+		                                    " given");
+	char *o = static_cast<char *>(buf);
+	memcpy(o, &h, sizeof(h)); o += sizeof(h);
+	memcpy(o, w->sp.data(), sizeof(cpgpuSpaceParams) * (size_t)w->nw); o += sizeof(cpgpuSpaceParams) * (size_t)w->nw;
+	for (const auto &q : sec) {
+		if (q.bytes) HIP_OR_FAIL(hipMemcpy(o, q.dev, q.bytes, hipMemcpyDeviceToHost));
+		o += q.bytes;
+	}
+	return CPGPU_OK;
+}
+
+int cpgpuWorldLoadState(cpgpuWorld *w, const void *buf, int64_t size)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (!buf || size < (int64_t)sizeof(SnapshotHeader)) return fail(CPGPU_ERR_INVALID, "snapshot too short");
+	if (w->tape.on) return fail(CPGPU_ERR_INVALID, "a taped world cannot load a snapshot (cpgpuWorldTapeEnd first)");
+	SnapshotHeader h;
+	memcpy(&h, buf, sizeof(h));
+	if (memcmp(h.magic, SNAP_MAGIC, 8) != 0 || h.version != 1) return fail(CPGPU_ERR_INVALID, "not a cpgpu snapshot");
+	if (h.precision != w->precision || h.real_bytes != (int32_t)sizeof(real))
+		return fail(CPGPU_ERR_INVALID, "the snapshot was saved from a world of the other precision");
+	if (h.sizes[0] != (int32_t)sizeof(PairRec) || h.sizes[1] != (int32_t)sizeof(Arbiter) ||
+	    h.sizes[2] != (int32_t)sizeof(Joint) || h.sizes[3] != (int32_t)sizeof(BodyVel))
+		return fail(CPGPU_ERR_INVALID, "the snapshot was saved by another build of the library");
+	if (h.n_worlds != w->nw || h.n_bodies != w->nb || h.n_shapes != w->ns || h.n_constraints != w->nc)
+		return fail(CPGPU_ERR_INVALID, "the snapshot is of a world of another description (counts differ)");
+	if (h.n_recs < 0 || h.n_act < 0 || h.n_cached < 0 || h.steps < 0)
+		return fail(CPGPU_ERR_INVALID, "negative counts in the snapshot");
+	int rc = cpgpuWorldSync(w);
+	if (rc != CPGPU_OK) return rc;
+	const int P = w->cur ^ 1;
+	HIP_OR_FAIL(regrow(w->recs[P], w->rec_cap[P], (long)h.n_recs + 1));
+	HIP_OR_FAIL(regrow(w->act[P], w->act_cap[P], (long)h.n_act + 1));
+	HIP_OR_FAIL(regrow(w->cached[P], w->cached_cap[P], (long)h.n_cached + 1));
+	const auto sec = snap_sections(w, h.n_recs, h.n_act, h.n_cached);
+	size_t need = sizeof(h) + sizeof(cpgpuSpaceParams) * (size_t)w->nw;
+	for (const auto &q : sec) need += q.bytes;
+	if ((size_t)size != need)
+		return fail(CPGPU_ERR_INVALID, "snapshot of " + std::to_string(size) + " bytes, its header describes " +
+		                                   std::to_string(need));
+	const char *in = static_cast<const char *>(buf) + sizeof(h);
+	std::vector<cpgpuSpaceParams> sp(w->nw);
+	memcpy(sp.data(), in, sizeof(cpgpuSpaceParams) * (size_t)w->nw);
+	in += sizeof(cpgpuSpaceParams) * (size_t)w->nw;
+	for (const auto &q : sec) {
+		if (q.bytes) HIP_OR_FAIL(hipMemcpy(q.dev, in, q.bytes, hipMemcpyHostToDevice));
+		in += q.bytes;
+	}
+	const int32_t counts[3] = {h.n_recs, h.n_act, h.n_cached};
+	HIP_OR_FAIL(hipMemcpy(w->d_step_counts + 3 * P, counts, sizeof(counts), hipMemcpyHostToDevice));
+	for (int k = 0; k < 3; k++) w->h_counts[3 * P + k] = counts[k];
+	w->sp = sp;
+	w->max_iterations = 0;
+	w->max_iterations_large = 0;
+	for (int k = 0; k < w->nw; k++) {
+		w->max_iterations = std::max(w->max_iterations, (int)w->sp[k].iterations);
+		if (!w->h_small[k]) w->max_iterations_large = std::max(w->max_iterations_large, (int)w->sp[k].iterations);
+	}
+	w->steps = h.steps;
+	w->total_pairs = h.total_pairs;
+	w->last_pairs = h.last_pairs;
+	w->last_joint_dt = -1.0;   // the joints' bias coefficients are recomputed with the next step's dt
+	w->graph_dirty = true;
+	w->dyn_tree_built = false;
+	// the dynamic shapes' cached geometry from the loaded transforms (their leaf boxes are loaded)
+	if (w->n_dyn_shapes > 0) {
+		k_cache_shapes<<<blocks_for(w->n_dyn_shapes, 256), 256, 0, w->stream>>>(w->W, w->d_dyn_shapes, w->n_dyn_shapes,
+		                                                                        0, 0);
+		HIP_OR_FAIL(hipGetLastError());
+		HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+	}
+	return CPGPU_OK;
+}
+
 int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
 {
 	int rc = cpgpuWorldSync(w);

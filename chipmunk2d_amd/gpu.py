@@ -29,6 +29,7 @@ EXPORTS = [
     "cpgpuWorldReadConstraints", "cpgpuWorldReadPairs", "cpgpuWorldShapesCollide",
     "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
     "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldSetGridSchedule", "cpgpuWorldSetGraph",
+    "cpgpuWorldSaveState", "cpgpuWorldLoadState",
     "cpgpuWorldQueryPointNearest", "cpgpuWorldQuerySegmentFirst", "cpgpuWorldQueryPointAll", "cpgpuWorldQuerySegmentAll",
     "cpgpuWorldQueryBBAll",
     "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
@@ -92,6 +93,8 @@ def lib():
         L.cpgpuWorldSetTiming.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldSetGridSchedule.argtypes = [_vp, ctypes.c_int32]
         L.cpgpuWorldSetGraph.argtypes = [_vp, ctypes.c_int32]
+        L.cpgpuWorldSaveState.argtypes = [_vp, _vp, ctypes.c_int64, _i64p]
+        L.cpgpuWorldLoadState.argtypes = [_vp, _vp, ctypes.c_int64]
         _fp = ctypes.POINTER(ShapeFilter)
         L.cpgpuWorldQueryPointNearest.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _dp, _fp, _ip, _dp]
         L.cpgpuWorldQuerySegmentFirst.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _dp, _dp, _fp, _ip, _dp]
@@ -227,6 +230,19 @@ class GPUWorld:
         sets = np.zeros((n, 12))
         _check(self.L.cpgpuWorldShapesCollide(self.h, n, ptr(pairs, ctypes.c_int32), ptr(count, ctypes.c_int32), ptr(sets, ctypes.c_double)))
         return count, sets
+
+    def save_state(self):
+        """cpgpuWorldSaveState: the state the next step starts from, as bytes."""
+        n = ctypes.c_int64()
+        _check(self.L.cpgpuWorldSaveState(self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(self.L.cpgpuWorldSaveState(self.h, buf, n.value, ctypes.byref(n)))
+        return buf.raw
+
+    def load_state(self, data):
+        """cpgpuWorldLoadState of bytes from save_state (a world of the same description)."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        _check(self.L.cpgpuWorldLoadState(self.h, buf, len(data)))
 
     def constraints(self):
         n = self.packed.n_constraints

@@ -236,6 +236,27 @@ int cpgpuWorldWriteBodiesAt(cpgpuWorld *world, int32_t n, const int32_t *index, 
 int cpgpuWorldSetSpaceParams(cpgpuWorld *world, int32_t space, const cpgpuSpaceParams *params);
 int cpgpuWorldGetStats(cpgpuWorld *world, cpgpuStats *out);
 
+/* ---- snapshots (checkpoint and resume; no counterpart in the reference,
+ * where a cpSpace is saved by its owner's own serialisation) ----
+ * Everything a step carries into the next, copied to a host buffer: the
+ * spaces' parameters with their stamps and last dt, the bodies' state
+ * (positions, angles, forces, torques, transforms, velocities and bias
+ * velocities), the dynamic tree's leaf boxes and rebuild stamps, the joints
+ * (parameters and accumulated impulses), the last step's pair records with
+ * their GJK warm-start ids and its solved and cached arbiters (contacts,
+ * impulses, stamps), and the running totals.  Loaded into a world created
+ * from the same description (same precision and build of this library), the
+ * next steps are bit for bit those of the world it was saved from.
+ * Layout: a header (magic "cpgpuSS1", version 1, precision, the counts and
+ * the struct sizes of the build), then cpgpuSpaceParams[n_worlds], then the
+ * device arrays in order -- positions, angles, forces, torques, transforms,
+ * velocity records, leaf boxes, rebuild stamps, joints, pair records, solved
+ * arbiters, cached arbiters, totals -- in the build's own layouts.
+ * cpgpuWorldSaveState: with buf NULL only *size_out is set (the bytes
+ * needed); otherwise cap must hold them (else CPGPU_ERR_CAPACITY). */
+int cpgpuWorldSaveState(cpgpuWorld *world, void *buf, int64_t cap, int64_t *size_out);
+int cpgpuWorldLoadState(cpgpuWorld *world, const void *buf, int64_t size);
+
 /* Copy the arbiters solved in the last step (cpSpace.arbiters, in solve
  * order: by colour, then by key).  Arrays have room for `cap` arbiters;
  * returns the number of arbiters through *n_out.  Any pointer may be NULL.

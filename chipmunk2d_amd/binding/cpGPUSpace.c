@@ -148,6 +148,7 @@ typedef struct Desc {
 	int32_t *shape_type, *shape_body, *shape_pstart, *shape_pcount;
 	uint64_t *shape_hashid, *shape_group;
 	uint32_t *shape_cats, *shape_mask;
+	uint8_t *shape_sensor;
 	double *shape_e, *shape_u, *shape_sv, *shape_r, *shape_geom;
 	double *pverts, *pnorms;
 	int32_t *cons_type, *cons_a, *cons_b, *cons_collide;
@@ -189,6 +190,7 @@ static void desc_reserve(Desc *D, int nw, int nb, int ns, int nc, int npv)
 		D->shape_group = realloc(D->shape_group, (size_t)c * sizeof(uint64_t));
 		D->shape_cats = realloc(D->shape_cats, (size_t)c * sizeof(uint32_t));
 		D->shape_mask = realloc(D->shape_mask, (size_t)c * sizeof(uint32_t));
+		D->shape_sensor = realloc(D->shape_sensor, (size_t)c * sizeof(uint8_t));
 		D->shape_e = realloc(D->shape_e, (size_t)c * sizeof(double));
 		D->shape_u = realloc(D->shape_u, (size_t)c * sizeof(double));
 		D->shape_sv = realloc(D->shape_sv, (size_t)c * 2 * sizeof(double));
@@ -223,7 +225,8 @@ static void desc_free(Desc *D)
 {
 	void *p[] = {D->bodies, D->shapes, D->cons, D->sp, D->wbs, D->wss, D->wcs, D->body_type, D->body_m, D->body_m_inv,
 	             D->body_i, D->body_i_inv, D->body_cog, D->shape_type, D->shape_body, D->shape_pstart, D->shape_pcount,
-	             D->shape_hashid, D->shape_group, D->shape_cats, D->shape_mask, D->shape_e, D->shape_u, D->shape_sv,
+	             D->shape_hashid, D->shape_group, D->shape_cats, D->shape_mask, D->shape_sensor, D->shape_e, D->shape_u,
+	             D->shape_sv,
 	             D->shape_r, D->shape_geom, D->pverts, D->pnorms, D->cons_type, D->cons_a, D->cons_b, D->cons_collide,
 	             D->cons_max_force, D->cons_error_bias, D->cons_max_bias, D->cons_params, D->cons_jacc};
 	for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); k++) free(p[k]);
@@ -253,6 +256,7 @@ static void desc_link(Desc *D)
 	d->shape_type = D->shape_type; d->shape_body = D->shape_body; d->shape_hashid = D->shape_hashid;
 	d->shape_e = D->shape_e; d->shape_u = D->shape_u; d->shape_surface_v = D->shape_sv;
 	d->shape_group = D->shape_group; d->shape_categories = D->shape_cats; d->shape_mask = D->shape_mask;
+	d->shape_sensor = D->shape_sensor;
 	d->shape_radius = D->shape_r; d->shape_geom = D->shape_geom;
 	d->shape_poly_start = D->shape_pstart; d->shape_poly_count = D->shape_pcount;
 	d->n_poly_verts = D->npv;
@@ -396,7 +400,6 @@ static void write_body(const Desc *D, int k, cpBody *b)
 
 static int read_shape(Desc *D, int k, cpShape *sh, int body)
 {
-	if (sh->sensor) return report("cpGPUSpace: sensors are not supported on the device", NULL);
 	memset(D->shape_geom + 10 * k, 0, 10 * sizeof(double));
 	D->shape_type[k] = (int32_t)sh->klass->type;
 	D->shape_body[k] = body;
@@ -407,6 +410,7 @@ static int read_shape(Desc *D, int k, cpShape *sh, int body)
 	D->shape_group[k] = (uint64_t)sh->filter.group;
 	D->shape_cats[k] = (uint32_t)sh->filter.categories;
 	D->shape_mask[k] = (uint32_t)sh->filter.mask;
+	D->shape_sensor[k] = sh->sensor ? 1 : 0;
 	D->shape_pstart[k] = 0;
 	D->shape_pcount[k] = 0;
 	double *g = D->shape_geom + 10 * k;
@@ -654,6 +658,7 @@ static int same_structure(const Desc *A, const Desc *B)
 	       same_f64(A->shape_e, B->shape_e, A->ns) && same_f64(A->shape_u, B->shape_u, A->ns) &&
 	       same_f64(A->shape_sv, B->shape_sv, 2 * A->ns) && same_u64(A->shape_group, B->shape_group, A->ns) &&
 	       same_u32(A->shape_cats, B->shape_cats, A->ns) && same_u32(A->shape_mask, B->shape_mask, A->ns) &&
+	       (A->ns == 0 || !memcmp(A->shape_sensor, B->shape_sensor, (size_t)A->ns)) &&
 	       same_f64(A->shape_r, B->shape_r, A->ns) && same_f64(A->shape_geom, B->shape_geom, 10 * A->ns) &&
 	       same_i32(A->shape_pstart, B->shape_pstart, A->ns) && same_i32(A->shape_pcount, B->shape_pcount, A->ns) &&
 	       same_f64(A->pverts, B->pverts, 2 * A->npv) && same_f64(A->pnorms, B->pnorms, 2 * A->npv) &&

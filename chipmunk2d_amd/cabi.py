@@ -69,6 +69,7 @@ class WorldDesc(ctypes.Structure):
         ("cons_max_force", _dp), ("cons_error_bias", _dp), ("cons_max_bias", _dp),
         ("cons_collide_bodies", _ip),
         ("cons_params", _dp), ("cons_jacc", _dp),
+        ("shape_sensor", ctypes.POINTER(ctypes.c_uint8)),
     ]
 
 
@@ -167,6 +168,8 @@ class PackedWorld:
         for n in ["max_force", "error_bias", "max_bias", "params", "jacc"]:
             setattr(d, "cons_" + n, ptr(a["cons_" + n], ctypes.c_double))
         d.cons_collide_bodies = ptr(a["cons_collide_bodies"], ctypes.c_int32)
+        if "shape_sensor" in a and np.any(a["shape_sensor"]):
+            d.shape_sensor = ptr(a["shape_sensor"], ctypes.c_uint8)
         self._desc = d
         return d
 

@@ -501,11 +501,13 @@ __device__ __forceinline__ bool box_overlap(real4 a, real4 b)
 }
 
 // QueryReject without the box test (cpSpaceStep.c:219-232), plus the
-// both-infinite-mass rejection of cpSpaceCollideShapes (:281).
+// both-infinite-mass and sensor rejections of cpSpaceCollideShapes (:268-281):
+// with the default handler neither kind of pair is ever solved.
 __device__ __forceinline__ bool pair_ok(const DevWorld &W, int s, int t)
 {
 	const int ba = W.s_body[s], bb = W.s_body[t];
 	if (ba == bb) return false;
+	if (W.s_sensor && (W.s_sensor[s] | W.s_sensor[t])) return false;
 	const uint64_t ga = W.s_group[s], gb = W.s_group[t];
 	if ((ga != 0 && ga == gb) || (W.s_cats[s] & W.s_mask[t]) == 0 || (W.s_cats[t] & W.s_mask[s]) == 0) return false;
 	for (int k = W.nc_start[ba]; k < W.nc_start[ba + 1]; k++)

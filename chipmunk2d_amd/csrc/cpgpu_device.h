@@ -267,6 +267,7 @@ struct DevWorld {
 	const real2 *s_sv;
 	const uint64_t *s_group;
 	const uint32_t *s_cats, *s_mask;
+	const uint8_t *s_sensor;     // 1 for a sensor shape; nullptr when the world has none
 	const real *s_r;
 	const real *s_geom;          // [n][10] body-local geometry
 	const int32_t *s_pstart, *s_pcount;

@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(QUERY_THREADS) k_point_nearest(const DevWorld 
 	PQ out;
 	out.point = v2(0.0, 0.0); out.d = maxd; out.grad = v2(0.0, 0.0);
 	walk_trees(T, w, qbox, &stk[0][threadIdx.x], W.err, [&](int s) {
-		if (filter_reject(W, s, f)) return;
+		if (filter_reject(W, s, f) || (W.s_sensor && W.s_sensor[s])) return;   // cpSpaceQuery.c:64
 		const PQ r = point_query(W, load_shape(W, s), s, p);
 		if (r.d < out.d || (best >= 0 && r.d == out.d && s < best)) { out = r; best = s; }
 	});
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(QUERY_THREADS) k_segment_first(const DevWorld 
 	int best = -1;
 	SQ out = sq_blank(b);
 	walk_trees(T, w, segment_box(a, b, radius), &stk[0][threadIdx.x], W.err, [&](int s) {
-		if (filter_reject(W, s, f)) return;
+		if (filter_reject(W, s, f) || (W.s_sensor && W.s_sensor[s])) return;   // cpSpaceQuery.c:145
 		const SQ r = segment_query(W, load_shape(W, s), s, a, b, radius);
 		if (r.hit && (r.alpha < out.alpha || (best >= 0 && r.alpha == out.alpha && s < best))) { out = r; best = s; }
 	});

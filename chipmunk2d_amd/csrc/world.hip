@@ -605,6 +605,16 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		HIP_OR_FAIL(own_upload(w, pg, d->shape_group, ns)); W.s_group = pg;
 		HIP_OR_FAIL(own_upload(w, pcat, d->shape_categories, ns)); W.s_cats = pcat;
 		HIP_OR_FAIL(own_upload(w, pmask, d->shape_mask, ns)); W.s_mask = pmask;
+		W.s_sensor = nullptr;
+		if (d->shape_sensor) {
+			bool any = false;
+			for (int s = 0; s < ns && !any; s++) any = d->shape_sensor[s] != 0;
+			if (any) {
+				uint8_t *psens;
+				HIP_OR_FAIL(own_upload(w, psens, d->shape_sensor, ns));
+				W.s_sensor = psens;
+			}
+		}
 		HIP_OR_FAIL(own_upload_real(w, pr, d->shape_radius, ns)); W.s_r = pr;
 		HIP_OR_FAIL(own_upload_real(w, pgeom, d->shape_geom, 10 * (size_t)ns)); W.s_geom = pgeom;
 		std::vector<int32_t> pstart(ns, 0), pcount(ns, 0);

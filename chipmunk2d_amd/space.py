@@ -742,8 +742,6 @@ class Space:
                 self.add(body)
                 self._static_added = True
             assert body.space is self, "the shape's body must be added to the space before the shape"
-            if obj.sensor:
-                raise ValueError("sensors are not supported by the device step")
             obj.hashid = len(self.shapes)
             obj.space = self
             self.shapes.append(obj)
@@ -788,6 +786,7 @@ class Space:
         a["shape_group"] = np.array([s.group for s in self.shapes], dtype=np.uint64).reshape(ns)
         a["shape_categories"] = np.array([s.categories for s in self.shapes], dtype=np.uint32).reshape(ns)
         a["shape_mask"] = np.array([s.mask for s in self.shapes], dtype=np.uint32).reshape(ns)
+        a["shape_sensor"] = np.array([1 if s.sensor else 0 for s in self.shapes], dtype=np.uint8).reshape(ns)
         a["shape_radius"] = np.array([s.r for s in self.shapes], dtype=np.float64).reshape(ns)
         geom = np.zeros((ns, 10), dtype=np.float64)
         pstart = np.zeros(ns, dtype=np.int32)

@@ -23,10 +23,11 @@
  * arbiters (warm starts and GJK ids) for every shape pair that survives.
  *
  * Supported on the device: dynamic, kinematic and static bodies with the
- * default integrators; circles, segments and convex polygons with the default
- * collision handler; pivot, damped-spring (default force), slide and gear
- * joints without pre/post-solve callbacks; sleeping off.  Anything else is
- * reported as an error at pack time (sensors, custom handlers, other joints,
+ * default integrators; circles, segments and convex polygons, sensors among
+ * them, with the default collision handler; every joint class of the
+ * reference (springs with their default force and torque) without pre/post-
+ * solve callbacks; sleeping off.  Anything else is reported as an error at
+ * pack time (custom handlers, custom spring functions, constraint callbacks,
  * sleeping), as cpAssertHard would (include/chipmunk/chipmunk.h:48-58).
  *
  * Errors: by default a failed call prints the message and aborts, like

@@ -157,6 +157,13 @@ typedef struct cpgpuWorldDesc {
 	 * a ratchet, its angle in [1] (cpRatchetJointInit starts it at
 	 * b->a - a->a); NULL means zero. */
 	const double *cons_jacc;
+
+	/* [n_shapes] 1 for a sensor shape (cpShape.sensor, src/cpShape.c:120-130), else 0;
+	 * NULL: no sensors.  With the default collision handler a sensor's contacts are
+	 * never solved (cpSpaceStep.c:268-270), so its pairs are dropped with the
+	 * infinite-mass pairs before the narrowphase (and are not candidate pairs);
+	 * the nearest-point and first-segment queries skip sensors (cpSpaceQuery.c:64,145). */
+	const uint8_t *shape_sensor;
 } cpgpuWorldDesc;
 
 #define CPGPU_MAX_POLY_VERTS 64

@@ -184,6 +184,7 @@ typedef struct {
 	V sv;
 	uint64_t group;
 	uint32_t cats, mask;
+	int sensor;                              /* cpShape.sensor */
 	num r;
 	V c, tc;                                 /* circle */
 	V a, b, n, ta, tb, tn, a_tan, b_tan;     /* segment */
@@ -346,6 +347,7 @@ oworld *ow_new(const cpgpuWorldDesc *d)
 		s->e = d->shape_e[i]; s->u = d->shape_u[i];
 		s->sv = v2(d->shape_surface_v[2 * i], d->shape_surface_v[2 * i + 1]);
 		s->group = d->shape_group[i]; s->cats = d->shape_categories[i]; s->mask = d->shape_mask[i];
+		s->sensor = d->shape_sensor ? d->shape_sensor[i] != 0 : 0;
 		s->r = d->shape_radius[i];
 		if (s->type == CPGPU_SHAPE_CIRCLE) {
 			s->c = v2(g[0], g[1]);
@@ -987,9 +989,10 @@ static int nocollide_has(const OWorldPart *P, int ba, int bb)
 	return lo < P->n_nocollide && P->nocollide[lo] == k;
 }
 
-/* Pair admitted by the device library: QueryReject passes and not both
- * bodies of infinite mass (cpSpaceStep.c:281; such pairs never reach the
- * solver, so they are dropped before the narrowphase). */
+/* Pair admitted by the device library: QueryReject passes, not both bodies
+ * of infinite mass and neither shape a sensor (cpSpaceStep.c:268-281; with
+ * the default handler such pairs never reach the solver, so they are dropped
+ * before the narrowphase). */
 static int pair_ok(oworld *w, const OWorldPart *P, int i, int j)
 {
 	const OShape *a = &w->S[i], *b = &w->S[j];
@@ -1000,6 +1003,7 @@ static int pair_ok(oworld *w, const OWorldPart *P, int i, int j)
 	if (filter_reject(a, b)) return 0;
 	if (nocollide_has(P, a->body, b->body)) return 0;
 	if (ba->m == INFINITY && bb->m == INFINITY) return 0;
+	if (a->sensor || b->sensor) return 0;
 	return 1;
 }
 
@@ -1252,7 +1256,7 @@ int32_t ow_point_query_nearest(oworld *w, int32_t world, const double *p, double
 	PQInfo out;
 	out.hit = 0; out.point = v2(0.0, 0.0); out.d = max_distance; out.grad = v2(0.0, 0.0);
 	for (int i = s0; i < s1; i++) {
-		if (query_filter_reject(&S[i], group, cats, mask)) continue;
+		if (query_filter_reject(&S[i], group, cats, mask) || S[i].sensor) continue;   /* cpSpaceQuery.c:64 */
 		PQInfo q = point_query(&S[i], pt);
 		if (q.d < out.d) { out = q; best = i; }
 	}
@@ -1272,7 +1276,7 @@ int32_t ow_segment_query_first(oworld *w, int32_t world, const double *a, const 
 	int32_t best = -1;
 	SQInfo out = sq_blank(vb);
 	for (int i = s0; i < s1; i++) {
-		if (query_filter_reject(&S[i], group, cats, mask)) continue;
+		if (query_filter_reject(&S[i], group, cats, mask) || S[i].sensor) continue;   /* cpSpaceQuery.c:145 */
 		SQInfo q = segment_query(&S[i], va, vb, radius);
 		if (q.hit && q.alpha < out.alpha) { out = q; best = i; }
 	}

@@ -138,6 +138,7 @@ static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi)
 		cpShapeSetFriction(s, d->shape_u[i]);
 		cpShapeSetSurfaceVelocity(s, cpv(d->shape_surface_v[2 * i], d->shape_surface_v[2 * i + 1]));
 		cpShapeSetFilter(s, cpShapeFilterNew((cpGroup)d->shape_group[i], d->shape_categories[i], d->shape_mask[i]));
+		if (d->shape_sensor && d->shape_sensor[i]) cpShapeSetSensor(s, cpTrue);
 		s->userData = (cpDataPointer)(intptr_t)i;
 		cpSpaceAddShape(space, s);
 		if (s->hashid != d->shape_hashid[i]) {

@@ -445,6 +445,7 @@ def packed_from_desc(desc_ptr):
     a["shape_group"] = arr(d.shape_group, ns, np.uint64)
     a["shape_categories"] = arr(d.shape_categories, ns, np.uint32)
     a["shape_mask"] = arr(d.shape_mask, ns, np.uint32)
+    a["shape_sensor"] = arr(d.shape_sensor, ns, np.uint8)
     a["shape_radius"] = arr(d.shape_radius, ns, np.float64)
     a["shape_geom"] = arr(d.shape_geom, ns, np.float64, 10)
     a["shape_poly_start"] = arr(d.shape_poly_start, ns, np.int32)

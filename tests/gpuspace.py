@@ -55,6 +55,7 @@ def cp():
             "cpSpaceRemoveBody": (None, [_vp, _vp]), "cpSpaceRemoveShape": (None, [_vp, _vp]),
             "cpSpaceRemoveConstraint": (None, [_vp, _vp]),
             "cpSpaceSetSleepTimeThreshold": (None, [_vp, _d]),
+            "cpSpaceAddCollisionHandler": (_vp, [_vp, ctypes.c_size_t, ctypes.c_size_t]),
             "cpBodyNew": (_vp, [_d, _d]), "cpBodyNewStatic": (_vp, []), "cpBodyFree": (None, [_vp]),
             "cpBodySetPosition": (None, [_vp, cpVect]), "cpBodySetVelocity": (None, [_vp, cpVect]),
             "cpBodySetAngle": (None, [_vp, _d]), "cpBodySetAngularVelocity": (None, [_vp, _d]),

@@ -40,7 +40,7 @@ def _assert_describes(pw, d, j):
     assert np.array_equal(b["shape_hashid"], a["shape_hashid"])
     assert np.array_equal(b["shape_body"], j[a["shape_body"]])
     for f in ("shape_type", "shape_e", "shape_u", "shape_surface_v", "shape_group", "shape_categories", "shape_mask",
-              "shape_radius", "shape_geom", "shape_poly_count"):
+              "shape_sensor", "shape_radius", "shape_geom", "shape_poly_count"):
         assert np.array_equal(b[f], a[f]), f
     # polygon vertices and normals, shape by shape (the binding lays them out in shape order)
     for k in np.nonzero(a["shape_type"] == 2)[0]:
@@ -57,11 +57,18 @@ def _assert_describes(pw, d, j):
         assert d.a["space"][0][k] == pw.a["space"][0][k], k
 
 
-@pytest.mark.parametrize("scene", ["pyramid", "random", "chain", "machines"])
+def _with_sensors(sp):
+    for k, shape in enumerate(sp.shapes):
+        shape.sensor = k % 5 == 2
+    return sp
+
+
+@pytest.mark.parametrize("scene", ["pyramid", "random", "chain", "machines", "sensors"])
 def test_describe_equals_the_description_the_space_was_built_from(scene):
     pw = {"pyramid": lambda: pyramid_210(rows=8), "random": lambda: random_world(
         n_circles=80, n_polys=80, n_segments=20, extent=40.0, seed=9), "chain": lambda: chain_world(
-        15, 40, seed=5), "machines": lambda: machine_world(3, 10)}[scene]().pack()
+        15, 40, seed=5), "machines": lambda: machine_world(3, 10), "sensors": lambda: _with_sensors(random_world(
+            n_circles=40, n_polys=40, n_segments=10, extent=30.0, seed=10))}[scene]().pack()
     space = gs().cpGPUSpaceNew()
     desc = pw.desc()
     rb = ref_lib().rb_new_in(space, ctypes.byref(desc), 0)
@@ -120,18 +127,18 @@ _TORQUE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
 _custom_torque = _TORQUE(lambda c, angle: 2.0 * angle)
 
 
-@pytest.mark.parametrize("feature", ["sensor", "sleeping", "custom spring torque"])
+@pytest.mark.parametrize("feature", ["custom handler", "sleeping", "custom spring torque"])
 def test_unsupported_features_are_refused_at_pack_time(feature):
-    """Sensors, sleeping and a damped rotary spring with a torque function of
-    the caller's are refused when the space is packed (the step does nothing);
-    the message is cpAssertHard's (which aborts when cpGPUSetAbortOnError is
-    on)."""
+    """A collision handler of the caller's, sleeping and a damped rotary spring
+    with a torque function of the caller's are refused when the space is packed
+    (the step does nothing); the message is cpAssertHard's (which aborts when
+    cpGPUSetAbortOnError is on)."""
     C = cp()
     space, body, shape = _space_with_one_circle()
     extra = []
-    if feature == "sensor":
-        C.cpShapeSetSensor(shape, 1)
-        msg = "sensors"
+    if feature == "custom handler":
+        C.cpSpaceAddCollisionHandler(space, 1, 2)
+        msg = "default collision handler"
     elif feature == "sleeping":
         C.cpSpaceSetSleepTimeThreshold(space, 1.0)
         msg = "sleeping"

@@ -464,3 +464,19 @@ def test_batch_tiling_and_autotuner_keep_results_bit_identical():
     for f in a.FIELDS:
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
         assert np.array_equal(getattr(a, f), getattr(c, f)), f
+
+
+def test_sensors_bit_exact():
+    """A fifth of the shapes are sensors (cpShape.sensor): their pairs are never solved
+    (cpSpaceStep.c:268-270) and are not candidate pairs; the rest steps bit for bit."""
+    sp = random_world(n_circles=400, n_polys=400, n_segments=60, extent=60.0, seed=55)
+    rng = Rng(56)
+    for shape in sp.shapes:
+        if rng.uniform(0.0, 1.0) < 0.2:
+            shape.sensor = True
+    pw = sp.pack()
+    assert pw.a["shape_sensor"].sum() > 100
+    g, o = run_lockstep(pw, 120, check_every=4, arbiters_every=40)
+    sensor = pw.a["shape_sensor"].astype(bool)
+    pairs = g.pairs()
+    assert not np.any(sensor[pairs[:, 0]] | sensor[pairs[:, 1]])

@@ -784,3 +784,32 @@ def test_every_joint_class_replay_matches_reference(seed):
         limit_signs |= (1 if np.any(jr[t == 6, 0] > 0) else 0) | (2 if np.any(jr[t == 6, 0] < 0) else 0)
     assert np.all(active > 0), active
     assert moved > 0 and limit_signs == 3, (moved, limit_signs)
+
+
+@needs_ref
+def test_sensors_replay_matches_reference():
+    """A fifth of the shapes (static segments among them) are sensors: the
+    reference collides their pairs but never solves them (cpSpaceStep.c:268-270);
+    the oracle, which drops them before the narrowphase, stays bit-exact with it
+    while sensors overlap other shapes on most steps."""
+    sp = random_world(n_circles=200, n_polys=200, n_segments=30, extent=50.0, seed=5)
+    rng = Rng(6)
+    for s in sp.shapes:
+        if rng.uniform(0.0, 1.0) < 0.2:
+            s.sensor = True
+    pw = sp.pack()
+    sensor = pw.a["shape_sensor"].astype(bool)
+    assert sensor.sum() > 50 and sensor[pw.a["body_type"][pw.a["shape_body"]] == 2].any()
+    o = OracleWorld(pw)
+    r = RefWorld(pw)
+    o.libm_trig(True)
+    touching = 0
+    for s in range(200):
+        coll = r.step_logged(1 / 60)
+        o.replay(r.arbiters(), coll, 1 / 60)
+        touching += bool(np.any(sensor[coll[:, 0]] | sensor[coll[:, 1]]))
+        if s % 20 == 19:
+            rb, ob = r.bodies(), o.bodies()
+            for f in rb.FIELDS:
+                assert np.array_equal(getattr(rb, f), getattr(ob, f)), (s, f)
+    assert touching > 150, touching

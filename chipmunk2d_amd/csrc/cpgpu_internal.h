@@ -148,14 +148,17 @@ struct QueryIn {
 	const int32_t *space;
 	const double *a, *b, *scalar;
 	const cpgpuShapeFilter *filter;
+	const int32_t *shape;          // shape queries: the query shapes (shapes of the world)
 };
 // nearest point (segment == false) or first along a segment: shape_out [n], info_out [n][5]
 hipError_t query_nearest_or_first(const DevWorld &W, const QueryTrees &T, int n, const QueryIn &in, bool segment,
                                   int32_t *shape_out, double *info_out, hipStream_t stream);
 // every hit: kind 0 point, 1 segment, 2 box; (query, shape) pairs and their five numbers appended
 // up to cap, the total in *count
+// kind 3: cpSpaceShapeQuery of in.shape[q]: (query, shape) pairs with counts[k] and sets[k][12]
 hipError_t query_all(const DevWorld &W, const QueryTrees &T, int n, const QueryIn &in, int kind, int2 *hits,
-                     double *info, int64_t cap, unsigned long long *count, hipStream_t stream);
+                     double *info, int64_t cap, unsigned long long *count, hipStream_t stream,
+                     int32_t *n_pts = nullptr, double *sets = nullptr);
 
 #define HIP_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
 

@@ -67,7 +67,9 @@ struct GridBarrier {
 //     a workgroup barrier;
 //   large worlds: every arbiter and joint in the grid set, solved on a
 //     co-resident grid (k_solve_grid).
-// The grid set of each large world (its dynamic bodies) is cut into GRID_PARTS strips along x
+// The grid set of each large world (its dynamic bodies) is cut into `parts` strips along x
+// (grid_parts_for: about GRID_STRIP_BODIES bodies a strip, between GRID_PARTS_MIN and
+// GRID_PARTS_MAX; one count for every large world of the description, from the largest)
 // with equal numbers of its dynamic bodies (quantiles of a PART_BINS-bin
 // histogram of their x between the set's least and greatest x).  An arbiter
 // whose dynamic bodies lie in one strip belongs to that strip; one joining
@@ -81,11 +83,22 @@ struct GridBarrier {
 // order), then the wide set in its colour order.
 // Sorted arbiter keys, dense so the radix sort has few digits: small world
 // w -> w; unit u of the grid set (grid_units' order) -> n_worlds + u.
-#define GRID_PARTS 64
-#define GRID_SEAMS (GRID_PARTS - 1)
-#define EVEN_SEAMS ((GRID_SEAMS + 1) / 2)
-#define ODD_SEAMS (GRID_SEAMS / 2)
+#define GRID_PARTS_MIN 64
+#define GRID_PARTS_MAX 256
+#define GRID_STRIP_BODIES 384
+// strips of at most this many bodies keep them in LDS during their sweeps
+#define GRID_RES_BODIES 512
 #define PART_BINS 1024
+
+// strips of a description whose largest grid set has n dynamic bodies
+__host__ __device__ inline int grid_parts_for(long n)
+{
+	const long p = (n + GRID_STRIP_BODIES - 1) / GRID_STRIP_BODIES;
+	return (int)(p < GRID_PARTS_MIN ? GRID_PARTS_MIN : (p > GRID_PARTS_MAX ? GRID_PARTS_MAX : p));
+}
+// seams between `parts` strips, the even ones (0, 2, ...) and the odd ones
+__host__ __device__ inline int even_seams(int parts) { return parts / 2; }
+__host__ __device__ inline int odd_seams(int parts) { return (parts - 1) / 2; }
 // Claim arrays for concurrent colourings: strips and small worlds, even
 // seams, odd seams, the wide set.
 #define CLAIM_SETS 4
@@ -142,11 +155,13 @@ struct SolveBuffers {
 	// solve units of the grid set
 	const int32_t *large_index;    // [n_worlds] ordinal among the large worlds, -1 for a small world
 	int n_large_worlds;
+	int parts;                     // strips of each large world's grid set (grid_parts_for)
 	unsigned long long *pbound;    // [2 per large world] least and greatest x of the grid set (ordered keys)
 	int32_t *phist;                // [PART_BINS per large world] bin counts, then each bin's strip
 	int32_t *bpart;                // [n_bodies] strip of each grid-set body
 	int32_t *bslot;                // [n_bodies] its slot among its strip's bodies
-	int32_t *pcount;               // [GRID_PARTS per large world] bodies of each strip
+	int32_t *pcount;               // [parts per large world] bodies of each strip
+	int32_t *sbody;                // [parts per large world][GRID_RES_BODIES] each strip's bodies by slot
 	int32_t *pcol;                 // [grid_units][CPGPU_MAX_COLOURS + 1] colour offsets of each unit
 	int2 *pinfo;                   // [grid_units] first sorted position and colour count of each unit
 	int32_t *udone;                // [grid_units] sweeps finished by each unit (dependency-driven schedule)
@@ -172,9 +187,9 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
                         RadixSortBuffers &sortbuf, hipStream_t stream, KernelTimer *timer);
 // solve units of the grid set: the strips, even seams and odd seams of every
 // large world, then the wide set
-__host__ __device__ inline int grid_units(int n_large_worlds)
+__host__ __device__ inline int grid_units(int n_large_worlds, int parts)
 {
-	return n_large_worlds * (GRID_PARTS + GRID_SEAMS) + 1;
+	return n_large_worlds * (2 * parts - 1) + 1;
 }
 
 // Small worlds are launched in size buckets, each with its own workgroup

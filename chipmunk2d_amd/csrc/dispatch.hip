@@ -46,6 +46,7 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldQueryPointAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *points, const double *max_distance, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits, double *info); \
 	int cpgpuWorldQuerySegmentAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *starts, const double *ends, const double *radius, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits, double *info); \
 	int cpgpuWorldQueryBBAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *boxes, const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits); \
+	int cpgpuWorldQueryShapes(cpgpuWorld *world, int32_t n, const int32_t *shapes, int64_t cap, int64_t *n_out, int32_t *hits, int32_t *counts, double *sets); \
 	int cpgpuWorldTapeBegin(cpgpuWorld *world, int32_t checkpoint_every); \
 	int cpgpuWorldTapeEnd(cpgpuWorld *world); \
 	int cpgpuWorldBackward(cpgpuWorld *world, const double *g_p, const double *g_a, const double *g_v, const double *g_w, double *d_p0, double *d_a0, double *d_v0, double *d_w0, double *d_m_inv, double *d_i_inv, double *d_u); \
@@ -279,6 +280,13 @@ int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodi
 
 #define W64 reinterpret_cast<cpgpu64::cpgpuWorld *>(world)
 #define W32 reinterpret_cast<cpgpu32::cpgpuWorld *>(world)
+
+int cpgpuWorldQueryShapes(cpgpuWorld *world, int32_t n, const int32_t *shapes, int64_t cap, int64_t *n_out,
+                          int32_t *hits, int32_t *counts, double *sets)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldQueryShapes(W64, n, shapes, cap, n_out, hits, counts, sets),
+	              cpgpu32::cpgpuWorldQueryShapes(W32, n, shapes, cap, n_out, hits, counts, sets));
+}
 
 int cpgpuWorldSaveState(cpgpuWorld *world, void *buf, int64_t cap, int64_t *size_out)
 {

@@ -455,4 +455,83 @@ CPD void collide(const ShapeViewT<T> &a, const ShapeViewT<T> &b, uint32_t id, Co
 	}
 }
 
+// cpShapeUpdate(shape, body->transform) (cpShape.c:217-220) into a view of
+// its own, leaving the world's cached copy alone: the shape at its body's
+// present transform, with the cacheData arithmetic of cache_shape (cpShape.c:
+// 291-296, 378-405; cpPolyShape.c:39-64); a polygon's world vertices and
+// normals go to P
+struct PosedPoly {
+	V2 v[CPGPU_MAX_POLY_VERTS], n[CPGPU_MAX_POLY_VERTS];
+};
+
+CPD ShapeView pose_shape(const DevWorld &W, int s, PosedPoly &P)
+{
+	ShapeView v = load_shape(W, s);
+	const XF T = W.b_T[W.s_body[s]];
+	const real r = v.r;
+	const real *g = W.s_geom + 10 * (size_t)s;
+	if (v.type == CPGPU_SHAPE_CIRCLE) {
+		v.tc = xpoint(T, v2(g[0], g[1]));
+		v.bb[0] = v.tc.x - r; v.bb[1] = v.tc.y - r; v.bb[2] = v.tc.x + r; v.bb[3] = v.tc.y + r;
+	} else if (v.type == CPGPU_SHAPE_SEGMENT) {
+		v.ta = xpoint(T, v2(g[0], g[1]));
+		v.tb = xpoint(T, v2(g[2], g[3]));
+		v.tn = xvect(T, v2(g[4], g[5]));
+		real l, rr, bt, tp;
+		if (v.ta.x < v.tb.x) { l = v.ta.x; rr = v.tb.x; } else { l = v.tb.x; rr = v.ta.x; }
+		if (v.ta.y < v.tb.y) { bt = v.ta.y; tp = v.tb.y; } else { bt = v.tb.y; tp = v.ta.y; }
+		v.bb[0] = l - r; v.bb[1] = bt - r; v.bb[2] = rr + r; v.bb[3] = tp + r;
+	} else {
+		const int st = W.s_pstart[s];
+		real l = real(__builtin_inf()), rr = real(-__builtin_inf()), bt = real(__builtin_inf()), tp = real(-__builtin_inf());
+		for (int i = 0; i < v.count; i++) {
+			const real2 lv = W.pv_local[st + i], ln = W.pn_local[st + i];
+			P.v[i] = xpoint(T, v2(lv.x, lv.y));
+			P.n[i] = xvect(T, v2(ln.x, ln.y));
+			l = fminc(l, P.v[i].x); rr = fmaxc(rr, P.v[i].x); bt = fminc(bt, P.v[i].y); tp = fmaxc(tp, P.v[i].y);
+		}
+		v.wv = P.v;
+		v.wn = P.n;
+		v.bb[0] = l - r; v.bb[1] = bt - r; v.bb[2] = rr + r; v.bb[3] = tp + r;
+	}
+	return v;
+}
+
+// cpShapesCollide (cpShape.c:260-283) of shapes a and b, as its contact point
+// set: cpCollide with id 0 on the pair sorted by type (cpCollision.c:707-710);
+// when that swapped the shapes the normal is negated and each contact's
+// points are given back in the caller's order; distance = (r2 - r1) . normal
+// with the unswapped points.  Writes o[12] = {normal.x, normal.y, then per
+// point pointA.x, pointA.y, pointB.x, pointB.y, distance} (zeros past the
+// count) and returns the count.
+CPD int shapes_collide_views(const ShapeView &a, const ShapeView &b, double *o, int32_t *err)
+{
+	const bool swapped = a.type > b.type;
+	CollInfo info;
+	if (swapped) collide(b, a, 0u, info, err);
+	else collide(a, b, 0u, info, err);
+	const V2 nrm = swapped ? vneg(info.n) : info.n;
+	o[0] = (double)nrm.x;
+	o[1] = (double)nrm.y;
+	for (int c = 0; c < 2; c++) {
+		double *q = o + 2 + 5 * c;
+		if (c < info.count) {
+			const V2 p1 = info.p1[c], p2 = info.p2[c];
+			const V2 pa = swapped ? p2 : p1, pb = swapped ? p1 : p2;
+			q[0] = (double)pa.x; q[1] = (double)pa.y;
+			q[2] = (double)pb.x; q[3] = (double)pb.y;
+			q[4] = (double)vdot(vsub(p2, p1), nrm);
+		} else {
+			q[0] = q[1] = q[2] = q[3] = q[4] = 0.0;
+		}
+	}
+	return info.count;
+}
+
+// shapes_collide_views of shapes x and y on their cached geometry
+CPD int shapes_collide_set(const DevWorld &W, int x, int y, double *o, int32_t *err)
+{
+	return shapes_collide_views(load_shape(W, x), load_shape(W, y), o, err);
+}
+
 } // namespace CPGPU_NS

@@ -297,43 +297,15 @@ __global__ void __launch_bounds__(256) k_pair_class_scatter(const DevWorld W, co
 	}
 }
 
-// cpShapesCollide (cpShape.c:260-283) for a list of shape pairs on the
-// shapes' cached geometry: cpCollide with id 0 on the pair sorted by type
-// (cpCollision.c:707-710); when that swapped the shapes the normal is
-// negated and each contact's points are given back in the caller's order;
-// distance = (r2 - r1) . normal with the unswapped points.  Writes, per pair,
-// the count and {normal.x, normal.y, then per point pointA.x, pointA.y,
-// pointB.x, pointB.y, distance} (zeros past the count).
+// cpShapesCollide for a list of shape pairs: per pair the count and the
+// contact point set (shapes_collide_set)
 __global__ void __launch_bounds__(256) k_shapes_collide(const DevWorld W, const int2 *pairs, int n, int32_t *count,
                                                         double *sets)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	const int2 pr = pairs[i];
-	int x = pr.x, y = pr.y;
-	if (W.s_type[x] > W.s_type[y]) { x = pr.y; y = pr.x; }
-	const bool swapped = x != pr.x;
-	const ShapeView sa = load_shape(W, x);
-	const ShapeView sb = load_shape(W, y);
-	CollInfo info;
-	collide(sa, sb, 0u, info, W.err);
-	const V2 nrm = swapped ? vneg(info.n) : info.n;
-	double *o = sets + 12 * (size_t)i;
-	o[0] = (double)nrm.x;
-	o[1] = (double)nrm.y;
-	for (int c = 0; c < 2; c++) {
-		double *q = o + 2 + 5 * c;
-		if (c < info.count) {
-			const V2 p1 = info.p1[c], p2 = info.p2[c];
-			const V2 pa = swapped ? p2 : p1, pb = swapped ? p1 : p2;
-			q[0] = (double)pa.x; q[1] = (double)pa.y;
-			q[2] = (double)pb.x; q[3] = (double)pb.y;
-			q[4] = (double)vdot(vsub(p2, p1), nrm);
-		} else {
-			q[0] = q[1] = q[2] = q[3] = q[4] = 0.0;
-		}
-	}
-	count[i] = info.count;
+	count[i] = shapes_collide_set(W, pr.x, pr.y, sets + 12 * (size_t)i, W.err);
 }
 
 inline int blocks_for(int n, int t) { return (n + t - 1) / t; }

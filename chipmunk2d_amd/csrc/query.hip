@@ -301,12 +301,15 @@ __global__ void __launch_bounds__(QUERY_THREADS) k_segment_first(const DevWorld 
 	o[3] = (double)out.normal.y; o[4] = (double)out.alpha;
 }
 
-// every hit of every query, appended (query, shape) with its five numbers
+// every hit of every query, appended (query, shape) with its five numbers, or
+// (shape queries) its contact count and contact point set
 struct HitsOut {
 	int2 *hit;
 	double *info;
 	int64_t cap;
 	unsigned long long *count;
+	int32_t *n_pts;
+	double *set;
 };
 
 CPD void append_hit(const HitsOut &H, int q, int s, const double *v)
@@ -318,17 +321,49 @@ CPD void append_hit(const HitsOut &H, int q, int s, const double *v)
 		for (int i = 0; i < 5; i++) H.info[5 * k + i] = v[i];
 }
 
+CPD void append_contact_hit(const HitsOut &H, int q, int s, int n, const double *set)
+{
+	const unsigned long long k = atomicAdd(H.count, 1ull);
+	if ((int64_t)k >= H.cap) return;
+	H.hit[k] = make_int2(q, s);
+	H.n_pts[k] = n;
+	for (int i = 0; i < 12; i++) H.set[12 * k + i] = set[i];
+}
+
 // kind 0: cpSpacePointQuery (cpSpaceQuery.c:33-57); 1: cpSpaceSegmentQuery
-// (cpSpaceQuery.c:107-135); 2: cpSpaceBBQuery (cpSpaceQuery.c:186-206)
+// (cpSpaceQuery.c:107-135); 2: cpSpaceBBQuery (cpSpaceQuery.c:186-206);
+// 3: cpSpaceShapeQuery (cpSpaceQuery.c:231-246) of a shape of the world, put
+// at its body's present transform as cpShapeUpdate does there (the other
+// shapes stay at their cached pose): every other shape of its space that
+// passes the two shapes' filters (cpShapeFilterReject(a->filter, b->filter))
+// and touches it, with the contact point set of cpShapesCollide(query shape,
+// shape)
 __global__ void __launch_bounds__(QUERY_THREADS) k_query_all(const DevWorld W, const QueryTrees T, int n,
                                                               const QueryIn in, int kind, HitsOut H)
 {
 	__shared__ int32_t stk[QUERY_STACK][QUERY_THREADS];
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= n) return;
+	int32_t *stack = &stk[0][threadIdx.x];
+	if (kind == 3) {
+		const int a = in.shape[q];
+		const uint64_t ga = W.s_group[a];
+		const uint32_t ca = W.s_cats[a], ma = W.s_mask[a];
+		PosedPoly P;
+		const ShapeView va = pose_shape(W, a, P);
+		const real4 qbox = make_real4(va.bb[0], va.bb[1], va.bb[2], va.bb[3]);
+		walk_trees(T, W.s_world[a], qbox, stack, W.err, [&](int s) {
+			if (s == a) return;
+			const uint64_t gb = W.s_group[s];
+			if ((ga != 0 && ga == gb) || (ca & W.s_mask[s]) == 0 || (W.s_cats[s] & ma) == 0) return;
+			double set[12];
+			const int m = shapes_collide_views(va, load_shape(W, s), set, W.err);
+			if (m > 0) append_contact_hit(H, q, s, m, set);
+		});
+		return;
+	}
 	const int w = in.space ? in.space[q] : 0;
 	const cpgpuShapeFilter f = query_filter(in.filter, q);
-	int32_t *stack = &stk[0][threadIdx.x];
 	if (kind == 0) {
 		const V2 p = v2((real)in.a[2 * q], (real)in.a[2 * q + 1]);
 		const real maxd = (real)in.scalar[q];
@@ -383,10 +418,11 @@ hipError_t query_nearest_or_first(const DevWorld &W, const QueryTrees &T, int n,
 }
 
 hipError_t query_all(const DevWorld &W, const QueryTrees &T, int n, const QueryIn &in, int kind, int2 *hits,
-                     double *info, int64_t cap, unsigned long long *count, hipStream_t stream)
+                     double *info, int64_t cap, unsigned long long *count, hipStream_t stream, int32_t *n_pts,
+                     double *sets)
 {
 	if (n <= 0) return hipSuccess;
-	HitsOut H{hits, info, cap, count};
+	HitsOut H{hits, info, cap, count, n_pts, sets};
 	k_query_all<<<blocks_for(n, QUERY_THREADS), QUERY_THREADS, 0, stream>>>(W, T, n, in, kind, H);
 	return hipGetLastError();
 }

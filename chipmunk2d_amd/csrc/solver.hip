@@ -176,12 +176,13 @@ __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
 // are dynamic as arguments, so a caller that has them at hand (the grid
 // solve keeps them in LDS) loads the bodies beside the arbiter instead of
 // after it.
-template <class BA>
-__device__ void arb_apply_cached(const BA &bodies, const Arbiter &arb, int ia, int ib, bool da, bool db, real dt_coef)
+// ARB: Arbiter, or the grid solve's ArbSolve (the same fields the sweeps use, kept in LDS)
+template <class BA, class ARB>
+__device__ void arb_apply_cached(const BA &bodies, const ARB &arb, int ia, int ib, bool da, bool db, real dt_coef)
 {
 	// every load is issued before the state and count are known
 	BV a = bodies.load(ia), b = bodies.load(ib);
-	const Contact c0 = arb.c[0], c1 = arb.c[1];
+	const auto c0 = arb.c[0], c1 = arb.c[1];
 	const int count = arb.count;
 	if (arb.state == CPGPU_ARB_FIRST_COLLISION) return;
 	const V2 n = v2(arb.nx, arb.ny);
@@ -195,8 +196,9 @@ __device__ void arb_apply_cached(const BA &bodies, const Arbiter &arb, int ia, i
 
 // one contact's impulses (cpArbiterApplyImpulse's loop body): reads the
 // contact from `in`, writes its accumulated impulses to `out`
-__device__ __forceinline__ void contact_impulse(BV &a, BV &b, V2 n, V2 surface_vr, real friction, const Contact &in,
-                                                Contact &out)
+template <class CIN, class COUT>
+__device__ __forceinline__ void contact_impulse(BV &a, BV &b, V2 n, V2 surface_vr, real friction, const CIN &in,
+                                                COUT &out)
 {
 	const real nMass = in.nMass;
 	const V2 r1 = v2(in.r1x, in.r1y), r2 = v2(in.r2x, in.r2y);
@@ -228,11 +230,11 @@ __device__ __forceinline__ void contact_impulse(BV &a, BV &b, V2 n, V2 surface_v
 // Both contact slots are read before the count is known, so that every load
 // of the arbiter is in flight at once (a slot past the count is read and
 // left unused).
-template <class BA>
-__device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, Arbiter &arb, int ia, int ib, bool da, bool db)
+template <class BA, class ARB>
+__device__ void arb_apply_impulse(const DevWorld &W, const BA &bodies, ARB &arb, int ia, int ib, bool da, bool db)
 {
 	BV a = bodies.load(ia), b = bodies.load(ib);
-	const Contact c0 = arb.c[0], c1 = arb.c[1];
+	const auto c0 = arb.c[0], c1 = arb.c[1];
 	const int count = arb.count;
 	const V2 n = v2(arb.nx, arb.ny);
 	const V2 surface_vr = v2(arb.svx, arb.svy);
@@ -966,18 +968,19 @@ __device__ __forceinline__ uint64_t unit_key(const SolveBuffers &S, int u)
 {
 	return (uint64_t)S.n_worlds + (uint64_t)u;
 }
-__device__ __forceinline__ int unit_of(int l, int pa, int pb, int n_large)
+__device__ __forceinline__ int unit_of(int l, int pa, int pb, int n_large, int parts)
 {
-	if (pa < 0 || pb < 0 || pa == pb) return l * GRID_PARTS + max(pa, pb);
-	if (abs(pa - pb) != 1) return grid_units(n_large) - 1;
-	const int q = min(pa, pb), e0 = n_large * GRID_PARTS;
-	return q % 2 == 0 ? e0 + l * EVEN_SEAMS + q / 2 : e0 + n_large * EVEN_SEAMS + l * ODD_SEAMS + q / 2;
+	if (pa < 0 || pb < 0 || pa == pb) return l * parts + max(pa, pb);
+	if (abs(pa - pb) != 1) return grid_units(n_large, parts) - 1;
+	const int q = min(pa, pb), e0 = n_large * parts;
+	return q % 2 == 0 ? e0 + l * even_seams(parts) + q / 2
+	                  : e0 + n_large * even_seams(parts) + l * odd_seams(parts) + q / 2;
 }
-__device__ __forceinline__ int unit_claims(int u, int n_large)
+__device__ __forceinline__ int unit_claims(int u, int n_large, int parts)
 {
-	if (u < n_large * GRID_PARTS) return 0;
-	if (u < n_large * (GRID_PARTS + EVEN_SEAMS)) return 1;
-	if (u < n_large * (GRID_PARTS + GRID_SEAMS)) return 2;
+	if (u < n_large * parts) return 0;
+	if (u < n_large * (parts + even_seams(parts))) return 1;
+	if (u < n_large * (2 * parts - 1)) return 2;
 	return 3;
 }
 
@@ -1000,11 +1003,61 @@ __device__ void unit_wait(const int32_t *done, int need, int32_t *err)
 }
 
 constexpr int GRID_LDS_COLS = 2;   // columns per workgroup whose units' colour offsets stay in LDS
-constexpr int GRID_LDS_POS = 4096; // arbiters per workgroup whose index and bodies stay in LDS
+constexpr int GRID_LDS_POS = 1024; // arbiters per workgroup whose index and bodies stay in LDS (colours not kept)
+constexpr int GRID_RES_COLOURS = 64;   // colours of the first column's units that may be kept in LDS
 constexpr int GRID_LDS_BODIES = 1536;   // strips of at most this many bodies are coloured with claims in LDS
 
+// The part of an arbiter the sweeps read and write (the names of Arbiter's
+// fields, so the same impulse code runs on it), kept in LDS by the grid solve
+// for the colours it keeps of a workgroup's first column.  ia / ib: the LDS
+// slot of a body of the workgroup's strip kept in LDS, else ~(its global
+// index); flags: bit 0 / 1 when body a / b is dynamic.
+struct ContactSolve {
+	real r1x, r1y, r2x, r2y;
+	real nMass, tMass, bounce, jnAcc, jtAcc, jBias, bias;
+};
+struct ArbSolve {
+	ContactSolve c[2];
+	real nx, ny, u, svx, svy;
+	int32_t count, state;
+	int32_t ia, ib;
+	int32_t flags;
+};
+
+// bodies by ArbSolve's encoding: an LDS slot, or ~(global index)
+struct MixedBodies {
+	BodyVel *lds;
+	const DevWorld *W;
+	__device__ __forceinline__ BV load(int e) const
+	{
+		if (e < 0) return load_bv(*W, ~e);
+		const BodyVel &q = lds[e];
+		BV r;
+		r.v = v2(q.vx, q.vy); r.w = q.w; r.vb = v2(q.vbx, q.vby); r.wb = q.wb; r.m_inv = q.m_inv; r.i_inv = q.i_inv;
+		return r;
+	}
+	__device__ __forceinline__ void store(int e, const BV &r) const
+	{
+		if (e < 0) { store_bv(*W, ~e, r); return; }
+		BodyVel &q = lds[e];
+		q.vx = r.v.x; q.vy = r.v.y; q.w = r.w; q.vbx = r.vb.x; q.vby = r.vb.y; q.wb = r.wb;
+	}
+};
+
+// Dynamic LDS of k_solve_grid: first the colouring's claim arrays; after the
+// colouring the same bytes hold the first column's strip bodies (BodyVel and
+// global index by slot, GRID_RES_BODIES of them) and then ArbSolve records.
+__host__ __device__ constexpr size_t grid_claim_bytes()
+{
+	return 2 * GRID_LDS_BODIES * sizeof(unsigned long long) + GRID_LDS_BODIES * sizeof(int32_t);
+}
+__host__ __device__ constexpr size_t grid_body_bytes()
+{
+	return ((GRID_RES_BODIES * (sizeof(BodyVel) + sizeof(int32_t)) + 15) / 16) * 16;
+}
+
 // The grid set of all large worlds on the co-resident grid.  Column c =
-// l * GRID_PARTS + q holds strip q of the l-th large world and seam q to its
+// l * parts + q holds strip q of the l-th large world and seam q to its
 // right; workgroup g colours and sweeps columns g, g + gridDim.x, ... (the
 // last workgroup also the wide set).  A sweep runs the strips, the even
 // seams, the odd seams, the wide set, then the joints, as in cpSpaceStep.
@@ -1012,8 +1065,16 @@ constexpr int GRID_LDS_BODIES = 1536;   // strips of at most this many bodies ar
 // bodies with (a strip for its two seams' previous sweep, an even seam for
 // its two strips, an odd seam for the even seams beside it) through
 // completion counters; otherwise grid barriers separate the phases.
-__global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, const SolveBuffers S)
+// The first column of each workgroup keeps what fits of its units in LDS
+// through the sweeps: a strip whose arbiters all fit with its bodies keeps
+// both (the bodies loaded before each of its sweeps and written back after
+// it, for the seams); otherwise the smallest colours of the seam and the strip
+// keep their arbiters.  Kept arbiters' accumulated impulses are written back
+// after the last sweep.  Every value is read and written in the same order as
+// from global memory.
+__global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, const SolveBuffers S, int lds_bytes)
 {
+	extern __shared__ __align__(16) unsigned char dyn[];
 	constexpr int SCRATCH = 2 * GRID_LDS_COLS;   // LDS slot of units kept in global memory
 	__shared__ int32_t lcol[SCRATCH + 1][CPGPU_MAX_COLOURS + 1];
 	__shared__ int2 linfo[SCRATCH];
@@ -1025,23 +1086,31 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	__shared__ int32_t lbase[SCRATCH];   // a unit's first entry in lpos / lpb, -1 when not kept there
 	__shared__ int32_t liter[SCRATCH];   // iterations of the unit's space (a strip or seam lies in one space)
 	__shared__ real ldt[SCRATCH];        // its cached-impulse scale
-	__shared__ unsigned long long lclaim[2 * GRID_LDS_BODIES];
-	__shared__ int32_t ltaken[GRID_LDS_BODIES];
+	// first column's strip (0) and seam (1): each colour's first ArbSolve record, -1 for a colour
+	// whose arbiters stay in global memory
+	__shared__ int32_t lres[2][GRID_RES_COLOURS];
+	__shared__ int32_t lnb;              // first column's strip bodies kept in LDS (0: none)
+	unsigned long long *lclaim = reinterpret_cast<unsigned long long *>(dyn);
+	int32_t *ltaken = reinterpret_cast<int32_t *>(dyn + 2 * GRID_LDS_BODIES * sizeof(unsigned long long));
+	BodyVel *lbody = reinterpret_cast<BodyVel *>(dyn);
+	int32_t *lbidx = reinterpret_cast<int32_t *>(dyn + GRID_RES_BODIES * sizeof(BodyVel));
 	int lds_used = 0;
 	const int n_act = *S.n_act;
 	const int grid_first = S.large_first[0], wide_first = S.large_first[1];
 	const int n_large = S.n_large_worlds;
-	const int n_units = grid_units(n_large), n_cols = n_large * GRID_PARTS;
-	const int e0 = n_cols, o0 = e0 + n_large * EVEN_SEAMS;
+	const int parts = S.parts;
+	const int n_units = grid_units(n_large, parts), n_cols = n_large * parts;
+	const int e0 = n_cols, o0 = e0 + n_large * even_seams(parts);
 	const int stat = CPGPU_MAX_BUCKETS;
 	const int g = blockIdx.x, G = gridDim.x;
 	GridSync sy{(int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x), S.bar, W.err, 0u};
+	const bool claims_fit = lds_bytes >= (int)grid_claim_bytes();
 
 	// the seam unit of column c, -1 for a world's last strip
 	auto seam_of = [&](int c) {
-		const int l = c / GRID_PARTS, q = c % GRID_PARTS;
-		if (q == GRID_SEAMS) return -1;
-		return (q % 2 == 0 ? e0 + l * EVEN_SEAMS : o0 + l * ODD_SEAMS) + q / 2;
+		const int l = c / parts, q = c % parts;
+		if (q == parts - 1) return -1;
+		return (q % 2 == 0 ? e0 + l * even_seams(parts) : o0 + l * odd_seams(parts)) + q / 2;
 	};
 	auto slot = [&](int i, int seam) { return i < GRID_LDS_COLS ? 2 * i + seam : SCRATCH; };
 
@@ -1062,8 +1131,8 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			P.col_cursor = lcursor;
 			P.left = &lleft;
 			P.unit = stat;
-			P.cset = unit_claims(u, n_large);
-			if (u < n_cols && S.pcount[u] <= GRID_LDS_BODIES) {
+			P.cset = unit_claims(u, n_large, parts);
+			if (u < n_cols && S.pcount[u] <= GRID_LDS_BODIES && claims_fit) {
 				P.lclaim = lclaim;
 				P.ltaken = ltaken;
 				P.lcap = GRID_LDS_BODIES;
@@ -1105,6 +1174,71 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 		if (seam_of(c) >= 0) colour_unit(seam_of(c), slot(i, 1));
 	}
 	if (g == G - 1) colour_unit(n_units - 1, SCRATCH);
+
+	// ---- the first column's units into LDS (the claim arrays are done with) ----
+	// A strip whose bodies and arbiters all fit keeps both (its bodies are its own: no other
+	// unit of its sweep touches them).  Otherwise colours are kept from the last, the smallest,
+	// back: the seam's, then the strip's, while records fit; the bodies stay in global memory.
+	const int c0 = g < n_cols ? g : -1;
+	if (threadIdx.x == 0) {
+		lnb = 0;
+		for (int s = 0; s < 2; s++)
+			for (int q = 0; q < GRID_RES_COLOURS; q++) lres[s][q] = -1;
+		if (c0 >= 0) {
+			const int nc0 = linfo[0].y, nc1 = seam_of(c0) >= 0 ? linfo[1].y : 0;
+			const int n0 = nc0 > 0 ? lcol[0][nc0] : 0;
+			const int nb = S.pcount[c0];
+			const int rec = (int)sizeof(ArbSolve);
+			const int cap_b = lds_bytes >= (int)grid_body_bytes() ? (lds_bytes - (int)grid_body_bytes()) / rec : 0;
+			int used = 0, budget = lds_bytes / rec;
+			if (S.sbody && lds_bytes >= (int)grid_body_bytes() && nb > 0 && nb <= GRID_RES_BODIES && n0 > 0 &&
+			    nc0 <= GRID_RES_COLOURS && n0 <= cap_b) {
+				lnb = nb;
+				budget = cap_b;
+				for (int q = 0; q < nc0; q++) lres[0][q] = lcol[0][q];
+				used = n0;
+			}
+			for (int s = 1; s >= 0; s--) {
+				if (s == 0 && lnb > 0) continue;
+				for (int q = min(s ? nc1 : nc0, GRID_RES_COLOURS) - 1; q >= 0; q--) {
+					const int sz = lcol[s][q + 1] - lcol[s][q];
+					if (used + sz > budget) break;
+					lres[s][q] = used;
+					used += sz;
+				}
+			}
+		}
+	}
+	__syncthreads();
+	const int nbr = lnb;
+	ArbSolve *larb = reinterpret_cast<ArbSolve *>(dyn + (nbr > 0 ? grid_body_bytes() : 0));
+	for (int q = threadIdx.x; q < nbr; q += blockDim.x) lbidx[q] = S.sbody[(size_t)c0 * GRID_RES_BODIES + q];
+	// f(record, arbiter index) for every kept arbiter
+	auto each_kept = [&](auto f) {
+		for (int s = 0; s < 2; s++)
+			for (int q = 0; q < GRID_RES_COLOURS; q++) {
+				if (lres[s][q] < 0) continue;
+				const int k0 = lcol[s][q], k1 = lcol[s][q + 1], first = lres[s][q], start = linfo[s].x;
+				for (int k = k0 + (int)threadIdx.x; k < k1; k += (int)blockDim.x)
+					f(larb[first + k - k0], S.order[start + k], s);
+			}
+	};
+	each_kept([&](ArbSolve &r, int a, int s) {
+		const Arbiter &arb = S.act[a];
+		for (int q = 0; q < 2; q++) {
+			const Contact &c = arb.c[q];
+			r.c[q].r1x = c.r1x; r.c[q].r1y = c.r1y; r.c[q].r2x = c.r2x; r.c[q].r2y = c.r2y;
+			r.c[q].nMass = c.nMass; r.c[q].tMass = c.tMass; r.c[q].bounce = c.bounce;
+			r.c[q].jnAcc = c.jnAcc; r.c[q].jtAcc = c.jtAcc; r.c[q].jBias = c.jBias; r.c[q].bias = c.bias;
+		}
+		r.nx = arb.nx; r.ny = arb.ny; r.u = arb.u; r.svx = arb.svx; r.svy = arb.svy;
+		r.count = arb.count; r.state = arb.state;
+		const bool da = is_dynamic(W, arb.ba), db = is_dynamic(W, arb.bb);
+		// a strip arbiter's dynamic bodies are the strip's: in LDS when the strip's bodies are
+		r.ia = (s == 0 && nbr > 0 && da) ? S.bslot[arb.ba] : ~arb.ba;
+		r.ib = (s == 0 && nbr > 0 && db) ? S.bslot[arb.bb] : ~arb.bb;
+		r.flags = (da ? 1 : 0) | (db ? 2 : 0);
+	});
 	sy.sync();
 
 	// ---- joint pre-steps (cpSpaceStep.c:389-396) and, with joints, the velocity
@@ -1126,10 +1260,20 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 
 	// ---- sweeps ----
 	const GlobalBodies bodies{&W};
+	const MixedBodies mixed{lbody, &W};
+	// the first column's strip bodies into LDS before its sweep, and back after it
+	auto strip_bodies_in = [&]() {
+		for (int q = threadIdx.x; q < nbr; q += blockDim.x) lbody[q] = W.b_vel[lbidx[q]];
+		__syncthreads();
+	};
+	auto strip_bodies_out = [&]() {
+		__syncthreads();
+		for (int q = threadIdx.x; q < nbr; q += blockDim.x) W.b_vel[lbidx[q]] = lbody[q];
+	};
 	// f(arbiter, its bodies as kept in lpb, the unit's LDS slot: SCRATCH for a unit kept in
 	// global memory, whose arbiters' spaces are then read per arbiter) for the unit's
-	// arbiters, colour by colour
-	auto sweep_unit = [&](int u, int s, auto f) {
+	// arbiters, colour by colour; fl(record, slot) for the arbiters of a colour kept in LDS
+	auto sweep_unit = [&](int u, int s, auto f, auto fl) {
 		int2 info;
 		const int32_t *col;
 		int base = -1;
@@ -1143,8 +1287,12 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 		}
 		for (int q = 0; q < info.y; q++) {
 			const int c1 = col[q + 1];
+			const bool kept = s < 2 && q < GRID_RES_COLOURS && lres[s][q] >= 0;
+			const int rofs = kept ? lres[s][q] - col[q] : 0;
 			for (int k = col[q] + (int)threadIdx.x; k < c1; k += (int)blockDim.x) {
-				if (base >= 0) {
+				if (kept) {
+					fl(larb[rofs + k], s);
+				} else if (base >= 0) {
 					f(lpos[base + k], lpb[base + k], s);
 				} else {
 					const int a = S.order[info.x + k];
@@ -1155,6 +1303,13 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			}
 			__syncthreads();
 		}
+	};
+	// a strip's sweep: its bodies through LDS when it is the first column's and they are kept there
+	auto sweep_strip = [&](int c, int i, auto f, auto fl) {
+		const bool in_lds = i == 0 && nbr > 0;
+		if (in_lds) strip_bodies_in();
+		sweep_unit(c, slot(i, 0), f, fl);
+		if (in_lds) strip_bodies_out();
 	};
 	// units a and b (-1: none) have finished `need` passes
 	auto wait_units = [&](int a, int b, int need) {
@@ -1174,34 +1329,37 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	};
 	const bool wide = wide_first < n_act;
 	const bool flow = S.grid_flow && !wide && S.n_jcolours == 0;
-	// sweep p (0: cached impulses) of the arbiters with f, then of the joints with fj
-	auto sweep = [&](int p, auto f, auto fj) {
+	// sweep p (0: cached impulses) of the arbiters with f / fl, then of the joints with fj
+	auto sweep = [&](int p, auto f, auto fl, auto fj) {
 		if (flow) {
 			for (int c = g, i = 0; c < n_cols; c += G, i++) {
-				if (p > 0) wait_units(c % GRID_PARTS > 0 ? seam_of(c - 1) : -1, seam_of(c), p);
-				sweep_unit(c, slot(i, 0), f);
+				if (p > 0) wait_units(c % parts > 0 ? seam_of(c - 1) : -1, seam_of(c), p);
+				sweep_strip(c, i, f, fl);
 				finish_unit(c, p + 1);
 			}
 			for (int parity = 0; parity < 2; parity++) {
 				for (int c = g, i = 0; c < n_cols; c += G, i++) {
 					const int su = seam_of(c);
-					if (su < 0 || c % GRID_PARTS % 2 != parity) continue;
+					if (su < 0 || c % parts % 2 != parity) continue;
+					// an even seam waits for its two strips; an odd seam for the even seams beside
+					// it, which waited for its strips -- or, beside a space's last strip (an odd
+					// strip count), for that strip itself
 					if (parity == 0) wait_units(c, c + 1, p + 1);
-					else wait_units(seam_of(c - 1), seam_of(c + 1), p + 1);
-					sweep_unit(su, slot(i, 1), f);
+					else wait_units(seam_of(c - 1), seam_of(c + 1) >= 0 ? seam_of(c + 1) : c + 1, p + 1);
+					sweep_unit(su, slot(i, 1), f, fl);
 					finish_unit(su, p + 1);
 				}
 			}
 		} else {
-			for (int c = g, i = 0; c < n_cols; c += G, i++) sweep_unit(c, slot(i, 0), f);
+			for (int c = g, i = 0; c < n_cols; c += G, i++) sweep_strip(c, i, f, fl);
 			sy.sync();
 			for (int parity = 0; parity < 2; parity++) {
 				for (int c = g, i = 0; c < n_cols; c += G, i++)
-					if (seam_of(c) >= 0 && c % GRID_PARTS % 2 == parity) sweep_unit(seam_of(c), slot(i, 1), f);
+					if (seam_of(c) >= 0 && c % parts % 2 == parity) sweep_unit(seam_of(c), slot(i, 1), f, fl);
 				sy.sync();
 			}
 			if (wide) {
-				if (g == G - 1) sweep_unit(n_units - 1, SCRATCH, f);
+				if (g == G - 1) sweep_unit(n_units - 1, SCRATCH, f, fl);
 				sy.sync();
 			}
 		}
@@ -1215,6 +1373,8 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 		const Arbiter &arb = S.act[a];
 		arb_apply_cached(bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0,
 		                 s < SCRATCH ? ldt[s] : W.ws[arb.world].dt_coef);
+	}, [&](const ArbSolve &r, int s) {
+		arb_apply_cached(mixed, r, r.ia, r.ib, r.flags & 1, r.flags & 2, ldt[s]);
 	}, [&](Joint &j) { joint_apply_cached(W, bodies, j, W.ws[j.world].dt_coef); });
 	// the impulse iterations (cpSpaceStep.c:418-427)
 	for (int it = 0; it < S.max_iterations; it++) {
@@ -1222,10 +1382,21 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 			Arbiter &arb = S.act[a];
 			if (it < (s < SCRATCH ? liter[s] : W.ws[arb.world].iterations))
 				arb_apply_impulse(W, bodies, arb, e.x >= 0 ? e.x : ~e.x, e.y >= 0 ? e.y : ~e.y, e.x >= 0, e.y >= 0);
+		}, [&](ArbSolve &r, int s) {
+			if (it < liter[s]) arb_apply_impulse(W, mixed, r, r.ia, r.ib, r.flags & 1, r.flags & 2);
 		}, [&](Joint &j) {
 			if (it < W.ws[j.world].iterations) joint_apply_impulse(W, bodies, j, W.ws[j.world].dt);
 		});
 	}
+
+	// ---- the kept arbiters' accumulated impulses back to the arbiters ----
+	__syncthreads();
+	each_kept([&](const ArbSolve &r, int a, int) {
+		Arbiter &arb = S.act[a];
+		for (int q = 0; q < 2; q++) {
+			arb.c[q].jnAcc = r.c[q].jnAcc; arb.c[q].jtAcc = r.c[q].jtAcc; arb.c[q].jBias = r.c[q].jBias;
+		}
+	});
 }
 
 // ---- strips of the grid set ----
@@ -1267,7 +1438,7 @@ __global__ void k_part_init(const SolveBuffers S)
 	const int n_large = S.n_large_worlds;
 	for (int i = q; i < n_large * PART_BINS; i += gridDim.x * blockDim.x) {
 		S.phist[i] = 0;
-		if (i < n_large * GRID_PARTS) S.pcount[i] = 0;
+		if (i < n_large * S.parts) S.pcount[i] = 0;
 		if (i < n_large) { S.pbound[2 * i] = ~0ull; S.pbound[2 * i + 1] = 0ull; }
 	}
 }
@@ -1324,7 +1495,7 @@ __global__ void k_part_hist(const DevWorld W, const SolveBuffers S)
 }
 
 // one workgroup (PART_BINS threads) per large world: each bin's strip is
-// GRID_PARTS times the fraction of the set's bodies in the bins before it
+// `parts` times the fraction of the set's bodies in the bins before it
 __global__ void __launch_bounds__(PART_BINS) k_part_scan(const SolveBuffers S)
 {
 	__shared__ int32_t sc[PART_BINS];
@@ -1340,7 +1511,7 @@ __global__ void __launch_bounds__(PART_BINS) k_part_scan(const SolveBuffers S)
 		__syncthreads();
 	}
 	const long long total = sc[PART_BINS - 1], before = sc[i] - own;
-	h[i] = total > 0 ? (int)(before * GRID_PARTS / total) : 0;
+	h[i] = total > 0 ? (int)(before * S.parts / total) : 0;
 }
 
 // each grid-set body's strip, and its slot among the strip's bodies: the
@@ -1351,9 +1522,10 @@ __global__ void __launch_bounds__(PART_BINS) k_part_scan(const SolveBuffers S)
 constexpr int PART_ASSIGN_THREADS = 1024;
 __global__ void __launch_bounds__(PART_ASSIGN_THREADS) k_part_assign(const DevWorld W, const SolveBuffers S)
 {
-	__shared__ int32_t cnt[2][GRID_PARTS], base[2][GRID_PARTS];
+	__shared__ int32_t cnt[2][GRID_PARTS_MAX], base[2][GRID_PARTS_MAX];
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	for (int i = threadIdx.x; i < 2 * GRID_PARTS; i += blockDim.x) cnt[i / GRID_PARTS][i % GRID_PARTS] = 0;
+	const int parts = S.parts;
+	for (int i = threadIdx.x; i < 2 * parts; i += blockDim.x) cnt[i / parts][i % parts] = 0;
 	const int l0 = S.large_index[W.b_world[S.large_bodies[blockIdx.x * blockDim.x]]];
 	__syncthreads();
 	int b = -1, rel = -1, strip = 0, local = 0;
@@ -1369,12 +1541,16 @@ __global__ void __launch_bounds__(PART_ASSIGN_THREADS) k_part_assign(const DevWo
 		}
 	}
 	__syncthreads();
-	for (int i = threadIdx.x; i < 2 * GRID_PARTS; i += blockDim.x) {
-		const int r = i / GRID_PARTS, st = i % GRID_PARTS;
-		if (cnt[r][st] > 0) base[r][st] = atomicAdd(&S.pcount[(l0 + r) * GRID_PARTS + st], cnt[r][st]);
+	for (int i = threadIdx.x; i < 2 * parts; i += blockDim.x) {
+		const int r = i / parts, st = i % parts;
+		if (cnt[r][st] > 0) base[r][st] = atomicAdd(&S.pcount[(l0 + r) * parts + st], cnt[r][st]);
 	}
 	__syncthreads();
-	if (rel >= 0) S.bslot[b] = base[rel][strip] + local;
+	if (rel >= 0) {
+		const int sl = base[rel][strip] + local;
+		S.bslot[b] = sl;
+		if (sl < GRID_RES_BODIES) S.sbody[((size_t)(l0 + rel) * parts + strip) * GRID_RES_BODIES + sl] = b;
+	}
 }
 
 // the key OR and AND of the world sort start at 0 and all ones
@@ -1403,7 +1579,7 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 			// the strip of its dynamic bodies, the seam of two neighbouring strips, or the wide set
 			const int pa = is_dynamic(W, arb.ba) ? S.bpart[arb.ba] : -1;
 			const int pb = is_dynamic(W, arb.bb) ? S.bpart[arb.bb] : -1;
-			key = unit_key(S, unit_of(S.large_index[w], pa, pb, S.n_large_worlds));
+			key = unit_key(S, unit_of(S.large_index[w], pa, pb, S.n_large_worlds, S.parts));
 			arb_prestep(W, arb);
 		}
 		S.keys[k] = key;
@@ -1444,11 +1620,26 @@ __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 	if (w <= n_worlds) S.warb_off[w] = lower_bound_u64(S.keys, 0, n, (uint64_t)w);
 	if (w == n_worlds + 1) {
 		S.large_first[0] = lower_bound_u64(S.keys, 0, n, unit_key(S, 0));
-		S.large_first[1] = lower_bound_u64(S.keys, 0, n, unit_key(S, grid_units(S.n_large_worlds) - 1));
+		S.large_first[1] = lower_bound_u64(S.keys, 0, n, unit_key(S, grid_units(S.n_large_worlds, S.parts) - 1));
 	}
 }
 
 } // namespace
+
+// dynamic LDS of k_solve_grid: what a workgroup may hold beside the kernel's static LDS
+// (asked once: the library runs on one kind of device)
+static int grid_lds_bytes()
+{
+	static int cached = -1;
+	if (cached >= 0) return cached;
+	int dev = 0, lds = 0;
+	(void)hipGetDevice(&dev);
+	(void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+	hipFuncAttributes attr;
+	if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(k_solve_grid)) != hipSuccess) return cached = 0;
+	const int dyn = lds - (int)attr.sharedSizeBytes;
+	return cached = dyn > 0 ? dyn & ~15 : 0;
+}
 
 int solver_max_grid()
 {
@@ -1457,7 +1648,7 @@ int solver_max_grid()
 	(void)hipGetDevice(&dev);
 	(void)hipGetDeviceProperties(&prop, dev);
 	(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_solve_grid),
-	                                                   GRID_THREADS, 0);
+	                                                   GRID_THREADS, (size_t)grid_lds_bytes());
 	if (per_cu < 1) per_cu = 1;
 	return per_cu * prop.multiProcessorCount;
 }
@@ -1490,7 +1681,7 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
 		// keys below n_worlds + grid_units
-		const uint64_t key_end = (uint64_t)n_worlds + (uint64_t)grid_units(S.n_large_worlds);
+		const uint64_t key_end = (uint64_t)n_worlds + (uint64_t)grid_units(S.n_large_worlds, S.parts);
 		int key_bits = 1;
 		while ((1ull << key_bits) < key_end) key_bits++;
 		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, key_bits, sortbuf, stream, timer));
@@ -1545,12 +1736,19 @@ hipError_t solver_step(const DevWorld &W, const SolveBuffers &S, const WorldBuck
 		HIP_TRY(hipGetLastError());
 	}
 	if (grid > 0) {
-		void *args[] = {(void *)&W, (void *)&S};
+		int lds_bytes = grid_lds_bytes();
+		void *args[] = {(void *)&W, (void *)&S, (void *)&lds_bytes};
 		hipError_t e = hipSuccess;
-		k_bar_reset<<<1, 1, 0, stream>>>(S.bar);
-		TIMED(timer, "solver", stream,
-		      e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_grid), dim3(grid),
-		                                     dim3(GRID_THREADS), args, 0, stream));
+		for (int attempt = 0; attempt < 2; attempt++) {
+			k_bar_reset<<<1, 1, 0, stream>>>(S.bar);
+			TIMED(timer, "solver", stream,
+			      e = hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_grid), dim3(grid),
+			                                     dim3(GRID_THREADS), args, (size_t)lds_bytes, stream));
+			if (e == hipSuccess || lds_bytes == 0) break;
+			// a device that refuses the dynamic LDS: the same solve, claims and arbiters in global memory
+			(void)hipGetLastError();
+			lds_bytes = 0;
+		}
 		HIP_TRY(e);
 	}
 	// join

@@ -312,7 +312,7 @@ struct cpgpuWorld {
 	// solve units of the grid set
 	int grid_flow = 1;
 	int32_t *udone = nullptr;
-	int32_t *d_large_index = nullptr, *phist = nullptr, *bpart = nullptr, *bslot = nullptr, *pcount = nullptr;
+	int32_t *d_large_index = nullptr, *phist = nullptr, *bpart = nullptr, *bslot = nullptr, *pcount = nullptr, *sbody = nullptr;
 	int32_t *pcol = nullptr;
 	unsigned long long *pbound = nullptr;
 	int2 *pinfo = nullptr;
@@ -320,6 +320,7 @@ struct cpgpuWorld {
 	std::vector<int32_t> h_jcolour;
 	GridBarrier *bar = nullptr;
 	int solver_grid = 0;
+	int grid_parts = GRID_PARTS_MIN;   // strips of every large world's grid set
 	int max_iterations = 0;
 	// host mirror of counters after the last sync
 	int32_t *h_counts = nullptr;        // pinned: [0..5] step counts, [6] n_pairs, [7] err
@@ -709,6 +710,15 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 			if (d->body_type[b] != CPGPU_BODY_STATIC && !small[body_world[b]]) large_bodies.push_back(b);
 		w->n_large_worlds = w->nw - w->n_small;
 		w->n_large_bodies = (int)large_bodies.size();
+		// one strip count for every large world, from the largest grid set (its dynamic bodies)
+		long max_grid = 0;
+		for (int k = 0; k < w->nw; k++) {
+			if (small[k]) continue;
+			long n = 0;
+			for (int b = d->world_body_start[k]; b < d->world_body_start[k + 1]; b++) n += d->body_type[b] == CPGPU_BODY_DYNAMIC;
+			max_grid = std::max(max_grid, n);
+		}
+		w->grid_parts = grid_parts_for(max_grid);
 		w->max_iterations_large = 0;
 		for (int k = 0; k < w->nw; k++)
 			if (!small[k]) w->max_iterations_large = std::max(w->max_iterations_large, (int)w->sp[k].iterations);
@@ -728,10 +738,12 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		const int n_large = std::max(w->n_large_worlds, 1);
 		HIP_OR_FAIL(own_alloc(w, w->pbound, 2 * (size_t)n_large));
 		HIP_OR_FAIL(own_alloc(w, w->phist, (size_t)PART_BINS * n_large));
-		HIP_OR_FAIL(own_alloc(w, w->pcol, (size_t)grid_units(n_large) * (CPGPU_MAX_COLOURS + 1)));
-		HIP_OR_FAIL(own_alloc(w, w->pinfo, (size_t)grid_units(n_large)));
-		HIP_OR_FAIL(own_alloc(w, w->udone, (size_t)grid_units(n_large)));
-		HIP_OR_FAIL(own_alloc(w, w->pcount, (size_t)GRID_PARTS * n_large));
+		const int n_units = grid_units(n_large, w->grid_parts);
+		HIP_OR_FAIL(own_alloc(w, w->pcol, (size_t)n_units * (CPGPU_MAX_COLOURS + 1)));
+		HIP_OR_FAIL(own_alloc(w, w->pinfo, (size_t)n_units));
+		HIP_OR_FAIL(own_alloc(w, w->udone, (size_t)n_units));
+		HIP_OR_FAIL(own_alloc(w, w->pcount, (size_t)w->grid_parts * n_large));
+		HIP_OR_FAIL(own_alloc(w, w->sbody, (size_t)w->grid_parts * n_large * GRID_RES_BODIES));
 
 		// large worlds' joints: by colour
 		int ncol_large = 0;
@@ -833,8 +845,9 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 		HIP_OR_FAIL(regrow(w->act[k], w->act_cap[k], base));
 		HIP_OR_FAIL(regrow(w->cached[k], w->cached_cap[k], base));
 	}
-	// the grid solve: few large workgroups keep its barriers cheap
-	w->solver_grid = std::min(solver_max_grid(), 64);
+	// the grid solve: one workgroup per column (a strip and the seam to its right) of one large
+	// world, as many as are co-resident; few large workgroups keep its barriers cheap
+	w->solver_grid = std::min(solver_max_grid(), w->grid_parts);
 	if (w->solver_grid < 1) w->solver_grid = 1;
 	*out = w;
 	return CPGPU_OK;
@@ -894,11 +907,11 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.max_iterations = w->max_iterations_large;
 	S.n_bodies = w->nb;
 	S.large_bodies = w->d_large_bodies; S.n_large_bodies = w->n_large_bodies;
-	S.large_index = w->d_large_index; S.n_large_worlds = w->n_large_worlds;
+	S.large_index = w->d_large_index; S.n_large_worlds = w->n_large_worlds; S.parts = w->grid_parts;
 	S.pbound = w->pbound; S.phist = w->phist; S.bpart = w->bpart; S.pcol = w->pcol; S.pinfo = w->pinfo;
 	S.udone = w->udone; S.grid_flow = w->grid_flow;
 	S.n_worlds = w->nw;
-	S.bslot = w->bslot; S.pcount = w->pcount;
+	S.bslot = w->bslot; S.pcount = w->pcount; S.sbody = w->sbody;
 	S.rec = nullptr; S.rec_stride = 0; S.jrec = nullptr;
 	return S;
 }
@@ -2027,6 +2040,71 @@ int cpgpuWorldQueryBBAll(cpgpuWorld *w, int32_t n, const int32_t *space, const d
                          const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits)
 {
 	return query_every(w, n, space, boxes, 4, nullptr, nullptr, filter, 2, cap, n_out, hits, nullptr);
+}
+
+int cpgpuWorldQueryShapes(cpgpuWorld *w, int32_t n, const int32_t *shapes, int64_t cap, int64_t *n_out, int32_t *hits,
+                          int32_t *counts, double *sets)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (n < 0) return fail(CPGPU_ERR_INVALID, "negative query count");
+	if (cap < 0) return fail(CPGPU_ERR_INVALID, "negative hit capacity");
+	if (n > 0 && !shapes) return fail(CPGPU_ERR_INVALID, "null query shapes");
+	if (cap > 0 && (!hits || !counts || !sets)) return fail(CPGPU_ERR_INVALID, "null hit, count or contact set array");
+	for (int32_t q = 0; q < n; q++)
+		if (shapes[q] < 0 || shapes[q] >= w->ns)
+			return fail(CPGPU_ERR_INVALID, "query " + std::to_string(q) + " names shape " + std::to_string(shapes[q]) +
+			                                   " of " + std::to_string(w->ns));
+	QueryTrees T;
+	int rc = query_trees(w, T);
+	if (rc != CPGPU_OK) return rc;
+	if (n == 0) {
+		if (n_out) *n_out = 0;
+		return CPGPU_OK;
+	}
+	const int64_t dcap = std::max<int64_t>(cap, 4096);
+	auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+	const size_t b_q = up(sizeof(int32_t) * (size_t)n), b_hit = up(sizeof(int2) * (size_t)dcap);
+	const size_t b_cnt = up(sizeof(int32_t) * (size_t)dcap), b_set = up(sizeof(double) * 12 * (size_t)dcap);
+	char *base = nullptr;
+	HIP_OR_FAIL(hipMalloc(&base, b_q + b_hit + b_cnt + b_set + 256));
+	struct Free { char *p; ~Free() { (void)hipFree(p); } } release{base};
+	int32_t *d_q = reinterpret_cast<int32_t *>(base);
+	int2 *d_hits = reinterpret_cast<int2 *>(base + b_q);
+	int32_t *d_cnt = reinterpret_cast<int32_t *>(base + b_q + b_hit);
+	double *d_set = reinterpret_cast<double *>(base + b_q + b_hit + b_cnt);
+	unsigned long long *d_count = reinterpret_cast<unsigned long long *>(base + b_q + b_hit + b_cnt + b_set);
+	HIP_OR_FAIL(hipMemcpy(d_q, shapes, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+	HIP_OR_FAIL(hipMemsetAsync(d_count, 0, sizeof(unsigned long long), w->stream));
+	QueryIn in{};
+	in.shape = d_q;
+	HIP_OR_FAIL(query_all(w->W, T, n, in, 3, d_hits, nullptr, dcap, d_count, w->stream, d_cnt, d_set));
+	rc = sync_world(w);
+	if (rc != CPGPU_OK) return rc;
+	unsigned long long total = 0;
+	HIP_OR_FAIL(hipMemcpy(&total, d_count, sizeof(total), hipMemcpyDeviceToHost));
+	if (n_out) *n_out = (int64_t)total;
+	const int64_t got = std::min<int64_t>((int64_t)total, dcap);
+	if (cap == 0 || got == 0) return CPGPU_OK;
+	std::vector<int2> h(got);
+	std::vector<int32_t> hc(got);
+	std::vector<double> hs(12 * got);
+	HIP_OR_FAIL(hipMemcpy(h.data(), d_hits, sizeof(int2) * got, hipMemcpyDeviceToHost));
+	HIP_OR_FAIL(hipMemcpy(hc.data(), d_cnt, sizeof(int32_t) * got, hipMemcpyDeviceToHost));
+	HIP_OR_FAIL(hipMemcpy(hs.data(), d_set, sizeof(double) * 12 * got, hipMemcpyDeviceToHost));
+	// by query, then shape: the device appended them in no fixed order
+	std::vector<int64_t> order(got);
+	for (int64_t k = 0; k < got; k++) order[k] = k;
+	std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+		return h[x].x != h[y].x ? h[x].x < h[y].x : h[x].y < h[y].y;
+	});
+	for (int64_t k = 0; k < got && k < cap; k++) {
+		const int64_t o = order[k];
+		hits[2 * k] = h[o].x;
+		hits[2 * k + 1] = h[o].y;
+		counts[k] = hc[o];
+		for (int i = 0; i < 12; i++) sets[12 * k + i] = hs[12 * o + i];
+	}
+	return CPGPU_OK;
 }
 
 int cpgpuWorldReadConstraints(cpgpuWorld *w, double *jacc, int32_t *colour)

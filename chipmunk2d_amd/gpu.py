@@ -31,7 +31,7 @@ EXPORTS = [
     "cpgpuWorldGetBuckets", "cpgpuWorldSetBuckets", "cpgpuWorldAutotune", "cpgpuWorldSetGridSchedule", "cpgpuWorldSetGraph",
     "cpgpuWorldSaveState", "cpgpuWorldLoadState",
     "cpgpuWorldQueryPointNearest", "cpgpuWorldQuerySegmentFirst", "cpgpuWorldQueryPointAll", "cpgpuWorldQuerySegmentAll",
-    "cpgpuWorldQueryBBAll",
+    "cpgpuWorldQueryBBAll", "cpgpuWorldQueryShapes",
     "cpgpuWorldTapeBegin", "cpgpuWorldTapeEnd",
     "cpgpuWorldBackward", "cpgpuWorldGetJointGrads", "cpgpuGetLastError", "cpgpuVersion",
 ]
@@ -102,6 +102,7 @@ def lib():
         L.cpgpuWorldQuerySegmentAll.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _dp, _dp, _fp, ctypes.c_int64, _i64p,
                                                 _ip, _dp]
         L.cpgpuWorldQueryBBAll.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _fp, ctypes.c_int64, _i64p, _ip]
+        L.cpgpuWorldQueryShapes.argtypes = [_vp, ctypes.c_int32, _ip, ctypes.c_int64, _i64p, _ip, _ip, _dp]
         L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
         L.cpgpuWorldGetSolveCounts.argtypes = [_vp, _i64p, _i64p]
         L.cpgpuWorldGetBuckets.argtypes = [_vp, _ip, _ip, _ip, _ip]
@@ -348,6 +349,22 @@ class GPUWorld:
         return self._query_all(self.L.cpgpuWorldQueryBBAll, [n, ptr(_space(space, n), ctypes.c_int32),
                                                              ptr(bb, ctypes.c_double), _filters(filters, n)],
                                with_info=False)
+
+    def query_shapes(self, shapes):
+        """cpSpaceShapeQuery of each shape of the world at its body's present
+        transform: hits [k, 2] (query, shape) sorted, contact counts [k] and
+        contact point sets [k, 12] (normal, then per point pointA, pointB,
+        distance)."""
+        q = np.ascontiguousarray(shapes, dtype=np.int32).reshape(-1)
+        n = len(q)
+        total = ctypes.c_int64()
+        fn = self.L.cpgpuWorldQueryShapes
+        _check(fn(self.h, n, ptr(q, ctypes.c_int32), 0, ctypes.byref(total), None, None, None))
+        cap = total.value
+        hits, counts, sets = np.zeros((cap, 2), np.int32), np.zeros(cap, np.int32), np.zeros((cap, 12))
+        _check(fn(self.h, n, ptr(q, ctypes.c_int32), cap, ctypes.byref(total), ptr(hits, ctypes.c_int32),
+                  ptr(counts, ctypes.c_int32), ptr(sets, ctypes.c_double)))
+        return hits, counts, sets
 
     def autotune(self, dt, trial_steps=3):
         """Pick each size bucket's workgroup size by timing (advances the world 5 x trial_steps steps)."""

@@ -339,6 +339,24 @@ int cpgpuWorldQuerySegmentAll(cpgpuWorld *world, int32_t n, const int32_t *space
                               int64_t *n_out, int32_t *hits, double *info);
 int cpgpuWorldQueryBBAll(cpgpuWorld *world, int32_t n, const int32_t *space, const double *boxes,
                          const cpgpuShapeFilter *filter, int64_t cap, int64_t *n_out, int32_t *hits);
+/* cpSpaceShapeQuery (cpSpaceQuery.c:231-246) with shapes of the world as the
+ * query shapes.  As there, the query shape is first put at its body's
+ * present transform (cpShapeUpdate), the other shapes keep the pose the last
+ * step cached; unlike there, the world's cached copy of the query shape is
+ * left as it was.  For each shapes[q]: every other shape of its space whose
+ * box meets the query shape's, that passes the two shapes' filters and that
+ * touches it, with the cpShapesCollide(query shape, shape) contact point set
+ * -- hits [cap][2] {query, shape} sorted by query then shape, counts [cap],
+ * sets [cap][12] {normal.x, normal.y, then per point pointA.x, pointA.y,
+ * pointB.x, pointB.y, distance}; the total goes to *n_out.  (The reference's
+ * dynamic tree prunes by fattened leaf boxes and may also report degenerate
+ * contacts -- zero normal -- with shapes whose own boxes miss the query
+ * shape's.)  The reference's return value is true when the query shape is no
+ * sensor and the last touching shape it visits is none: without sensors, when
+ * there is a hit.  (A shape not in the space is queried by adding it as a
+ * sensor: it then touches nothing in the step.) */
+int cpgpuWorldQueryShapes(cpgpuWorld *world, int32_t n, const int32_t *shapes, int64_t cap, int64_t *n_out,
+                          int32_t *hits, int32_t *counts, double *sets);
 /* Copy constraint accumulated impulses ([n][2], see cons_jacc) and colours. */
 int cpgpuWorldReadConstraints(cpgpuWorld *world, double *jacc, int32_t *colour);
 

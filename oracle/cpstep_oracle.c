@@ -265,7 +265,10 @@ struct oworld {
 	uint64_t *coll_key;
 	int joint_list_order;
 	int64_t steps;
+	int grid_parts;     /* strips of every large space's grid set (grid_parts_of) */
 };
+
+static int grid_parts_of(const oworld *w);
 
 /* splitmix64 finalizer: a bijection on 64-bit integers; the colouring priority. */
 static inline uint64_t mix64(uint64_t z)
@@ -397,6 +400,7 @@ oworld *ow_new(const cpgpuWorldDesc *d)
 		P->jorder = (int32_t *)calloc((size_t)(P->c1 - P->c0 + 1), sizeof(int32_t));
 		for (int c = P->c0; c < P->c1; c++) P->jorder[c - P->c0] = c;
 	}
+	w->grid_parts = grid_parts_of(w);
 	return w;
 }
 
@@ -1349,6 +1353,50 @@ int64_t ow_bb_query(oworld *w, int32_t world, const double *bb, uint64_t group, 
 	return n;
 }
 
+/* cpSpaceShapeQuery (cpSpaceQuery.c:231-246) of shape s, every shape
+ * checked: s is first put at its body's present transform (cpShapeUpdate,
+ * cpShape.c:217-220; the other shapes keep their cached pose), then the shapes
+ * of its space other than s whose box meets s's new box, that pass
+ * cpShapeFilterReject(s->filter, shape->filter) (chipmunk_private.h:144-155)
+ * and that touch it, in shape order, with cpShapesCollide(s, shape)'s count
+ * and contact point set {normal, then per point pointA, pointB, distance}.
+ * The reference's dynamic tree prunes by a shape's fattened leaf box instead
+ * of its own; the two differ only where cpCollide reports a contact between
+ * shapes whose boxes are apart, which degenerate input does (collinear
+ * segments: a zero normal).  Unlike the reference, s's cached pose is given
+ * back afterwards. */
+int64_t ow_shape_query(oworld *w, int32_t s, int64_t cap, int32_t *shapes, int32_t *counts, double *sets)
+{
+	OShape *a = &w->S[s];
+	const OShape saved = *a;
+	V wv[CPGPU_MAX_POLY_VERTS], wn[CPGPU_MAX_POLY_VERTS];
+	const int poly = a->type == CPGPU_SHAPE_POLY;
+	if (poly)
+		for (int i = 0; i < a->count; i++) { wv[i] = a->wv[i]; wn[i] = a->wn[i]; }
+	cache_shape(w, a);
+	int s0, s1;
+	world_shapes(w, w->B[a->body].world, &s0, &s1);
+	int64_t n = 0;
+	for (int i = s0; i < s1; i++) {
+		const OShape *b = &w->S[i];
+		if (i == s || !bb_intersects(a->bb, b->bb)) continue;
+		if ((a->group != 0 && a->group == b->group) || (a->cats & b->mask) == 0 || (b->cats & a->mask) == 0) continue;
+		double set[12] = {0};
+		const int m = ow_shapes_collide(w, s, i, set, set + 2);
+		if (m == 0) continue;
+		if (n < cap) {
+			shapes[n] = i;
+			counts[n] = m;
+			for (int k = 0; k < 12; k++) sets[12 * n + k] = set[k];
+		}
+		n++;
+	}
+	if (poly)
+		for (int i = 0; i < a->count; i++) { a->wv[i] = wv[i]; a->wn[i] = wn[i]; }
+	*a = saved;
+	return n;
+}
+
 /* ------------------------------------------------------------------ */
 /* arbiter persistence: the cachedArbiters hash set of cpSpace
  * (cpSpaceStep.c:248-251, cpHashSet.c:152-176) keyed by the unordered
@@ -1515,22 +1563,44 @@ static int cmp_jsolve(const void *x, const void *y)
 /* ------------------------------------------------------------------ */
 /* the device's solve units of a large space (no reference counterpart).
  * A space with more than SMALL_WORLD_MAX_BODIES non-static bodies is solved
- * as one grid set: its dynamic bodies are cut into GRID_PARTS strips along x
+ * as one grid set: its dynamic bodies are cut into P strips along x
  * holding equal shares of them: a PART_BINS-bin histogram of x between the
  * bodies' least and greatest x, each bin going to strip
- * floor(GRID_PARTS * (bodies in earlier bins) / (bodies)).  An arbiter
+ * floor(P * (bodies in earlier bins) / (bodies)).  P is one count for every
+ * large space of the description: about GRID_STRIP_BODIES dynamic bodies a
+ * strip in the largest, between GRID_PARTS_MIN and GRID_PARTS_MAX.  An arbiter
  * belongs to the strip of its dynamic bodies, to seam q when they lie in
  * strips q and q + 1, or to the wide set when their strips are further apart.
  * Each unit is coloured on its own and the space is solved unit by unit: the
- * strips (1 + q), the even seams (UNIT_EVEN + q / 2), the odd seams
- * (UNIT_ODD + q / 2), the wide set (UNIT_WIDE).  A small space is one unit, 0. */
+ * strips (1 + q), the even seams (1 + P + q / 2), the odd seams
+ * (1 + P + P / 2 + q / 2), the wide set (1 + P + P / 2 + (P - 1) / 2).  A small
+ * space is one unit, 0. */
 
 #define SMALL_WORLD_MAX_BODIES 4096
-#define GRID_PARTS 64
+#define GRID_PARTS_MIN 64
+#define GRID_PARTS_MAX 256
+#define GRID_STRIP_BODIES 384
 #define PART_BINS 1024
-#define UNIT_EVEN (1 + GRID_PARTS)
-#define UNIT_ODD (UNIT_EVEN + GRID_PARTS / 2)
-#define UNIT_WIDE (UNIT_ODD + (GRID_PARTS - 1) / 2)
+#define UNIT_EVEN(P) (1 + (P))
+#define UNIT_ODD(P) (UNIT_EVEN(P) + (P) / 2)
+#define UNIT_WIDE(P) (UNIT_ODD(P) + ((P) - 1) / 2)
+
+/* strips of every large space of the description, from its largest grid set */
+static int grid_parts_of(const oworld *w)
+{
+	long max_grid = 0;
+	for (int wi = 0; wi < w->nw; wi++) {
+		const OWorldPart *P = &w->W[wi];
+		long n_ns = 0, n_dyn = 0;
+		for (int b = P->b0; b < P->b1; b++) {
+			n_ns += w->B[b].type != CPGPU_BODY_STATIC;
+			n_dyn += w->B[b].type == CPGPU_BODY_DYNAMIC;
+		}
+		if (n_ns > SMALL_WORLD_MAX_BODIES && n_dyn > max_grid) max_grid = n_dyn;
+	}
+	long p = (max_grid + GRID_STRIP_BODIES - 1) / GRID_STRIP_BODIES;
+	return (int)(p < GRID_PARTS_MIN ? GRID_PARTS_MIN : (p > GRID_PARTS_MAX ? GRID_PARTS_MAX : p));
+}
 
 /* a num's bits as an unsigned key of the same order (x + 0.0 turns -0.0 into +0.0) */
 static uint64_t x_key(double x)
@@ -1584,7 +1654,7 @@ static int grid_units(oworld *w, OWorldPart *P)
 		if (part[i] >= 0) hist[part_bin(NUM_VAL(w->B[P->b0 + i].p.x), lo, hi)]++;
 	for (int q = 0, before = 0; q < PART_BINS; q++) {
 		int cnt = hist[q];
-		hist[q] = n_grid > 0 ? (int)((int64_t)before * GRID_PARTS / n_grid) : 0;
+		hist[q] = n_grid > 0 ? (int)((int64_t)before * w->grid_parts / n_grid) : 0;
 		before += cnt;
 	}
 	for (int i = 0; i < nbw; i++)
@@ -1595,8 +1665,8 @@ static int grid_units(oworld *w, OWorldPart *P)
 		if (pa < 0 || pb < 0 || pa == pb) arb->unit = 1 + (pa > pb ? pa : pb);
 		else if (pa - pb == 1 || pb - pa == 1) {
 			int q = pa < pb ? pa : pb;
-			arb->unit = (q % 2 == 0 ? UNIT_EVEN : UNIT_ODD) + q / 2;
-		} else arb->unit = UNIT_WIDE;
+			arb->unit = (q % 2 == 0 ? UNIT_EVEN(w->grid_parts) : UNIT_ODD(w->grid_parts)) + q / 2;
+		} else arb->unit = UNIT_WIDE(w->grid_parts);
 	}
 #undef DYN
 	free(part);
@@ -2152,14 +2222,29 @@ static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, i
 			int32_t *ubb = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
 			uint64_t *upr = (uint64_t *)malloc((size_t)(n + 1) * sizeof(uint64_t));
 			P->ncol_arb = 0;
-			for (int u = 0; u <= UNIT_WIDE; u++) {
+			/* the active arbiters grouped by unit, in active order within a unit */
+			const int n_units = UNIT_WIDE(w->grid_parts) + 1;
+			int64_t *ustart = (int64_t *)calloc((size_t)n_units + 1, sizeof(int64_t));
+			int64_t *byunit = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
+			for (int64_t k = 0; k < n; k++) ustart[P->arbs[P->active[k]].unit + 1]++;
+			for (int u = 0; u < n_units; u++) ustart[u + 1] += ustart[u];
+			for (int64_t k = 0; k < n; k++) {
+				const int u = P->arbs[P->active[k]].unit;
+				byunit[ustart[u]++] = k;
+			}
+			for (int u = n_units; u > 0; u--) ustart[u] = ustart[u - 1];
+			ustart[0] = 0;
+			for (int u = 0; u < n_units; u++) {
 				int64_t m = 0;
-				for (int64_t k = 0; k < n; k++)
-					if (P->arbs[P->active[k]].unit == u) { idx[m] = k; uba[m] = ba[k]; ubb[m] = bb[k]; upr[m] = pr[k]; m++; }
+				for (int64_t j = ustart[u]; j < ustart[u + 1]; j++) {
+					const int64_t k = byunit[j];
+					idx[m] = k; uba[m] = ba[k]; ubb[m] = bb[k]; upr[m] = pr[k]; m++;
+				}
 				int nc = colour_elements(w, m, uba, ubb, upr, col);
 				if (nc > P->ncol_arb) P->ncol_arb = nc;
 				for (int64_t q = 0; q < m; q++) P->arbs[P->active[idx[q]]].colour = col[q];
 			}
+			free(ustart); free(byunit);
 			free(idx); free(uba); free(ubb); free(upr);
 		}
 		g_sort_arbs = P->arbs;

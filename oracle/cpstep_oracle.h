@@ -63,6 +63,10 @@ int64_t ow_read_arbiters(oworld *w, int64_t cap, int32_t *shapes, int32_t *colou
  * space; in a large space 1 + q for strip q (q < 64), 65 + q / 2 for even
  * seam q, 97 + q / 2 for odd seam q, 128 for the wide set */
 int64_t ow_read_units(oworld *w, int64_t cap, int32_t *unit);
+/* cpSpaceShapeQuery of shape s (a shape of the world) at its cached pose:
+ * the touching shapes in shape order with counts [cap] and contact point
+ * sets [cap][12]; returns their number. */
+int64_t ow_shape_query(oworld *w, int32_t s, int64_t cap, int32_t *shapes, int32_t *counts, double *sets);
 int ow_read_constraints(oworld *w, double *jacc, int32_t *colour);
 void ow_stats(oworld *w, cpgpuStats *out);
 

@@ -623,6 +623,41 @@ int64_t rb_segment_query(rbWorld *w, const double *a, const double *b, double ra
 	return h.n;
 }
 
+typedef struct ShapeHits {
+	int64_t n, cap;
+	int32_t *shapes, *counts;
+	double *sets;
+} ShapeHits;
+
+static void shape_hit(cpShape *shape, cpContactPointSet *set, void *data)
+{
+	ShapeHits *h = (ShapeHits *)data;
+	if (h->n < h->cap) {
+		h->shapes[h->n] = (int32_t)(intptr_t)shape->userData;
+		h->counts[h->n] = set->count;
+		double *o = h->sets + 12 * h->n;
+		memset(o, 0, 12 * sizeof(double));
+		o[0] = set->normal.x; o[1] = set->normal.y;
+		for (int i = 0; i < set->count; i++) {
+			o[2 + 5 * i + 0] = set->points[i].pointA.x; o[2 + 5 * i + 1] = set->points[i].pointA.y;
+			o[2 + 5 * i + 2] = set->points[i].pointB.x; o[2 + 5 * i + 3] = set->points[i].pointB.y;
+			o[2 + 5 * i + 4] = set->points[i].distance;
+		}
+	}
+	h->n++;
+}
+
+/* cpSpaceShapeQuery with the world's shape k (local index): every shape the
+ * callback saw, in the order it saw them; *any: its return value */
+int64_t rb_shape_query(rbWorld *w, int32_t k, int64_t cap, int32_t *shapes, int32_t *counts, double *sets,
+                       int32_t *any)
+{
+	ShapeHits h = {0, cap, shapes, counts, sets};
+	const cpBool r = cpSpaceShapeQuery(w->space, w->shapes[k], shape_hit, &h);
+	if (any) *any = r ? 1 : 0;
+	return h.n;
+}
+
 int64_t rb_bb_query(rbWorld *w, const double *bb, uint64_t group, uint32_t cats, uint32_t mask, int64_t cap,
                     int32_t *shapes)
 {

@@ -71,6 +71,8 @@ def oracle_lib():
         lib.ow_segment_query.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
         lib.ow_bb_query.restype = ctypes.c_int64
         lib.ow_bb_query.argtypes = [_vp, ctypes.c_int32, _dp] + _q + [ctypes.c_int64, _ip]
+        lib.ow_shape_query.restype = ctypes.c_int64
+        lib.ow_shape_query.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp]
         lib.ow_set_space_params.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(SpaceParams)]
         lib.ow_new_from.restype = _vp
         lib.ow_new_from.argtypes = [ctypes.POINTER(WorldDesc), _vp, _ip]
@@ -122,6 +124,8 @@ def ref_lib(release=False):
         lib.rb_segment_query.argtypes = [_vp, _dp, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
         lib.rb_bb_query.restype = ctypes.c_int64
         lib.rb_bb_query.argtypes = [_vp, _dp] + _q + [ctypes.c_int64, _ip]
+        lib.rb_shape_query.restype = ctypes.c_int64
+        lib.rb_shape_query.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _ip]
         lib.rb_new_in.restype = _vp
         lib.rb_new_in.argtypes = [_vp, ctypes.POINTER(WorldDesc), ctypes.c_int]
         for name in ("rb_space", "rb_body", "rb_shape", "rb_constraint"):
@@ -595,6 +599,22 @@ class Queries:
 
     def bb_all(self, bb, filt=ALL):
         return self._hits("bb_query", [ptr(_v(bb), ctypes.c_double), *filt], with_info=False)[0]
+
+    def shape(self, s):
+        """cpSpaceShapeQuery of shape s: the touching shapes, their contact
+        counts and point sets [n, 12], sorted by shape; and, for the reference,
+        the function's return value (None for the oracle)."""
+        shapes = np.zeros(self.cap, np.int32)
+        counts = np.zeros(self.cap, np.int32)
+        sets = np.zeros((self.cap, 12))
+        any_ = np.zeros(1, np.int32)
+        args = [self.h, s, self.cap, ptr(shapes, ctypes.c_int32), ptr(counts, ctypes.c_int32), ptr(sets, ctypes.c_double)]
+        if self.pre == "rb_":
+            args.append(ptr(any_, ctypes.c_int32))
+        n = getattr(self.lib, self.pre + "shape_query")(*args)
+        assert n <= self.cap
+        order = np.argsort(shapes[:n], kind="stable")
+        return shapes[:n][order], counts[:n][order], sets[:n][order], (bool(any_[0]) if self.pre == "rb_" else None)
 
 
 def oracle_queries(o, world=0):

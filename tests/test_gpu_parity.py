@@ -143,6 +143,27 @@ def test_grid_strips_dense_world_bit_exact():
     assert strips == 64 and boundary > 1000, (strips, boundary)
 
 
+def test_grid_strips_crowded_world_bit_exact():
+    """A crowded large space (12k bodies, ~70k arbiters): strips of 400-700
+    arbiters, so the grid solve keeps some strips whole in LDS with their
+    bodies and of others only the smallest colours."""
+    pw = random_world(n_circles=6000, n_polys=6000, n_segments=400, extent=110.0, seed=21).pack()
+    g, o = run_lockstep(pw, 24, check_every=3, arbiters_every=12)
+    u = o.units()
+    per_strip = np.bincount(u[(u > 0) & (u <= 64)], minlength=65)[1:]
+    print("strip arbiters: min %d max %d" % (per_strip.min(), per_strip.max()), flush=True)
+    assert per_strip.max() > 600 and per_strip.min() < 500, per_strip
+
+
+def test_grid_strips_wide_world_runtime_strip_count_bit_exact():
+    """30k dynamic bodies spread wide: the grid set is cut into 79 strips
+    (about 384 bodies each), one workgroup per strip."""
+    pw = random_world(n_circles=15000, n_polys=15000, n_segments=1000, extent=800.0, seed=13).pack()
+    g, o = run_lockstep(pw, 30, check_every=3, arbiters_every=15)
+    u = o.units()
+    assert len(np.unique(u[(u > 0) & (u <= 79)])) == 79 and np.count_nonzero(u > 79) > 0
+
+
 def test_grid_strips_chain_with_joints_bit_exact():
     """A large space with a chain (pivots, springs, slides, gears) on the grid:
     strips of the links and the circles, then the joints."""

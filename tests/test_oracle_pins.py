@@ -379,6 +379,53 @@ def test_spatial_queries_match_reference():
     assert min(found.values()) > 20, found
 
 
+@needs_ref
+def test_shape_query_matches_reference():
+    """cpSpaceShapeQuery (cpSpaceQuery.c:231-246) of the compiled reference
+    against the oracle's, after 30 lockstep steps of a pile with filters: the
+    same touching shapes with the same contact counts and bit-identical
+    normals, points and distances, and cpTrue returned exactly when something
+    touched.  The query shape is put at its body's present transform (which
+    the reference keeps, the oracle does not), so a target that an earlier
+    query re-posed is left out of the comparison.  Every shape of the space is
+    queried: circles, segments and polygons, dynamic and static.  Where the
+    reference's fattened leaf boxes let it report more, each extra is a
+    degenerate contact (zero normal) with a shape whose own box misses the
+    query shape's."""
+    dt = 1 / 60
+    pw = random_world(n_circles=220, n_polys=220, n_segments=50, extent=40.0, seed=14).pack()
+    rng = np.random.default_rng(4)
+    pw.a["shape_categories"][:] = rng.choice([1, 2, 3], size=pw.n_shapes)
+    pw.a["shape_mask"][:] = rng.choice([0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFE], size=pw.n_shapes)
+    o, r = OracleWorld(pw), RefWorld(pw)
+    o.libm_trig(True)
+    for _ in range(30):
+        coll = r.step_logged(dt)
+        o.replay(r.arbiters(), coll, dt)
+    qo, qr = oracle_queries(o), ref_queries(r)
+    queried = set()
+    hits = points = extras = 0
+    for s in rng.permutation(pw.n_shapes):
+        so, co, seto, _ = qo.shape(int(s))
+        sr, cr, setr, anyr = qr.shape(int(s))
+        keep_o = ~np.isin(so, list(queried))
+        keep_r = ~np.isin(sr, list(queried))
+        extra = keep_r & ~np.isin(sr, so)
+        assert np.all(setr[extra, :2] == 0.0), (s, sr[extra], setr[extra])
+        keep_r &= ~extra
+        assert anyr == (len(sr) > 0), s
+        assert np.array_equal(so[keep_o], sr[keep_r]), (s, so, sr)
+        assert np.array_equal(co[keep_o], cr[keep_r]), s
+        assert np.array_equal(seto[keep_o], setr[keep_r]), s
+        assert s not in so
+        queried.add(int(s))
+        hits += int(keep_o.sum())
+        points += int(co[keep_o].sum())
+        extras += int(extra.sum())
+    print("shape queries: %d hits, %d contact points, %d degenerate extras of the reference" % (hits, points, extras))
+    assert hits > 200 and points > hits, (hits, points)
+
+
 # ---------------------------------------------------------------------------
 # broadphase semantics
 
@@ -648,7 +695,12 @@ def test_colouring_is_proper_and_matches_restatement():
     assert _python_colouring(ba, bb, prio, pw.n_bodies) == list(A.colour)
 
 
-def _strip_units(pw, p, shapes, small_max=4096, parts=64, bins=1024):
+def _strip_parts(n_dyn, per_strip=384, lo=64, hi=256):
+    """Strips of a grid set of n_dyn dynamic bodies: about per_strip bodies a strip, within [lo, hi]."""
+    return int(min(max(-(-n_dyn // per_strip), lo), hi))
+
+
+def _strip_units(pw, p, shapes, small_max=4096, bins=1024):
     """Independent restatement of a large space's solve units (one space):
     strips by numpy histogram quantiles of the dynamic bodies' x."""
     bt = pw.a["body_type"]
@@ -656,6 +708,7 @@ def _strip_units(pw, p, shapes, small_max=4096, parts=64, bins=1024):
     nb = pw.n_bodies
     if np.count_nonzero(bt != 2) <= small_max:
         return np.zeros(len(shapes), np.int32)
+    parts = _strip_parts(int(np.count_nonzero(bt == BODY_DYNAMIC)))
     a, b = sb[shapes[:, 0]].astype(np.int64), sb[shapes[:, 1]].astype(np.int64)
     dyn = bt == BODY_DYNAMIC
     x = p[dyn, 0] + 0.0
@@ -680,14 +733,18 @@ def _strip_units(pw, p, shapes, small_max=4096, parts=64, bins=1024):
     return u.astype(np.int32)
 
 
-@pytest.mark.parametrize("scene", ["dense", "chain"])
+@pytest.mark.parametrize("scene", ["dense", "chain", "wide"])
 def test_large_space_strips_match_restatement(scene):
     """A large space's solve units (strips along x, seams between
     neighbouring strips, even ones first, and the wide set) against an
     independent restatement; each unit coloured on its own (the Python claim
-    rounds) and solved unit by unit, by colour inside a unit."""
+    rounds) and solved unit by unit, by colour inside a unit.  The strip count
+    follows the grid set's size: 64 for the 5k-body spaces, 79 for 30k bodies."""
     from chipmunk2d_amd.scenes import chain_world
-    if scene == "dense":
+    if scene == "wide":
+        pw = random_world(n_circles=15000, n_polys=15000, n_segments=1000, extent=800.0, seed=13).pack()
+        steps = 25
+    elif scene == "dense":
         pw = random_world(n_circles=2600, n_polys=2600, n_segments=400, extent=140.0, seed=12).pack()
         steps = 6
     else:
@@ -701,6 +758,12 @@ def test_large_space_strips_match_restatement(scene):
     units = o.units()
     want = _strip_units(pw, o.bodies().p, A.shapes)
     assert np.array_equal(units, want)
+    if scene == "wide":
+        P = _strip_parts(int(np.count_nonzero(pw.a["body_type"] == BODY_DYNAMIC)))
+        assert P == 79
+        assert len(np.unique(units[(units > 0) & (units <= P)])) == P
+        assert np.count_nonzero(units > P) > 0
+        return
     strips = np.unique(units[(units > 0) & (units <= 64)])
     assert len(strips) >= (32 if scene == "dense" else 16), len(strips)
     assert len(np.unique(units[(units > 64) & (units <= 96)])) >= (24 if scene == "dense" else 4)   # even seams

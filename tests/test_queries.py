@@ -5,10 +5,13 @@ reference's cpSpaceQuery -- needs an MI355X.
 After lockstep steps (the device's shapes are the oracle's bit for bit),
 thousands of queries at once: nearest point and first along a segment (shape
 and every number of the info), and every hit near a point, along a segment
-(with and without a radius) and in a box.  Also: a batch of spaces (each
-query in its own space), queries before the first step, filters, and the
-100k-shape world.
+(with and without a radius) and in a box; and every shape of the world
+queried with itself (cpSpaceShapeQuery: the touching shapes, contact counts
+and point sets).  Also: a batch of spaces (each query in its own space),
+queries before the first step, filters, and the 100k-shape world.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -17,10 +20,12 @@ from chipmunk2d_amd.gpu import CpgpuError, GPUWorld
 from chipmunk2d_amd.scenes import pyramid_210, random_world
 from checkers import OracleWorld, oracle_queries
 
+# held off the GPU (run only on the CPU emulation of the library, CPGPU_EMULATE=1)
 pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="a GPU run of this file, with the cpGPUSpace, graph, query and "
-                                      "allocation-free tests, hung the device before its output came back; "
-                                      "off the GPU until the hang's cause is shown")]
+              pytest.mark.skipif(os.environ.get("CPGPU_EMULATE") != "1",
+                                 reason="a GPU run of this file, with the cpGPUSpace, graph, query and "
+                                        "allocation-free tests, hung the device before its output came back; "
+                                        "off the GPU until the hang's cause is shown")]
 
 DT = 1.0 / 60.0
 ALL = (0, 0xFFFFFFFF, 0xFFFFFFFF)
@@ -115,6 +120,41 @@ def test_queries_on_the_100k_world():
         assert np.array_equal(hits[hits[:, 0] == q, 1], Q.bb_all(np.concatenate([p[q] - 3.0, p[q] + 3.0])))
     print("100k world: %d nearest-point queries, %d box hits" % (n, len(hits)), flush=True)
     assert np.count_nonzero(s_near >= 0) > n // 10
+
+
+def _check_shape_queries(g, o, pw):
+    hits, counts, sets = g.query_shapes(np.arange(pw.n_shapes))
+    Q = oracle_queries(o)
+    starts = np.searchsorted(hits[:, 0], np.arange(pw.n_shapes + 1))
+    for s in range(pw.n_shapes):
+        k0, k1 = starts[s], starts[s + 1]
+        so, co, seto, _ = Q.shape(s)
+        assert np.array_equal(hits[k0:k1, 1], so), ("shape query", s, hits[k0:k1, 1], so)
+        assert np.array_equal(counts[k0:k1], co) and np.array_equal(sets[k0:k1], seto), ("shape query", s)
+    return len(hits)
+
+
+def test_shape_queries_match_the_oracle():
+    pw = random_world(n_circles=500, n_polys=500, n_segments=100, extent=55.0, seed=94).pack()
+    rng = np.random.default_rng(12)
+    pw.a["shape_categories"][:] = rng.choice([1, 2, 3], size=pw.n_shapes)
+    pw.a["shape_mask"][:] = rng.choice([0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFE], size=pw.n_shapes)
+    g, o = GPUWorld(pw), OracleWorld(pw)
+    n0 = _check_shape_queries(g, o, pw)          # before the first step
+    for _ in range(25):
+        g.step(DT)
+        o.step(DT)
+    assert np.array_equal(g.bodies().p, o.bodies().p)
+    n1 = _check_shape_queries(g, o, pw)
+    print("shape queries of %d shapes bit-exact: %d hits before stepping, %d after" % (pw.n_shapes, n0, n1), flush=True)
+    assert n1 > 500, n1
+    batch = concat([pyramid_210(rows=6).pack(), pyramid_210(rows=4, x_offset=2.0).pack()])
+    gb, ob = GPUWorld(batch), OracleWorld(batch)
+    gb.step(DT, 10)
+    ob.step(DT, 10)
+    assert _check_shape_queries(gb, ob, batch) > 20
+    with pytest.raises(CpgpuError, match="names shape"):
+        g.query_shapes([pw.n_shapes])
 
 
 def test_fp32_world_queries_run():

@@ -18,6 +18,10 @@ JOINT_PIN, JOINT_GROOVE, JOINT_ROTARY_LIMIT, JOINT_RATCHET, JOINT_SIMPLE_MOTOR, 
     4, 5, 6, 7, 8, 9
 ARB_FIRST_COLLISION, ARB_NORMAL, ARB_IGNORE, ARB_CACHED = 0, 1, 2, 3
 FP64, FP32 = 0, 1
+EVENT_BEGIN, EVENT_PRE_SOLVE, EVENT_POST_SOLVE, EVENT_SEPARATE = 1, 2, 4, 8
+WILDCARD = 0xFFFFFFFF
+# a description's collision handlers: {space (-1: every space), type_a, type_b, events}
+HANDLER_DTYPE = np.dtype([("space", np.int32), ("type_a", np.uint32), ("type_b", np.uint32), ("events", np.uint32)])
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
@@ -70,7 +74,30 @@ class WorldDesc(ctypes.Structure):
         ("cons_collide_bodies", _ip),
         ("cons_params", _dp), ("cons_jacc", _dp),
         ("shape_sensor", ctypes.POINTER(ctypes.c_uint8)),
+        ("shape_ctype", _up),
+        ("n_handlers", ctypes.c_int32),
+        ("handlers", ctypes.c_void_p),
     ]
+
+
+class CallbackArbiter(ctypes.Structure):
+    """cpgpuCallbackArbiter: a touching arbiter whose begin or preSolve is due."""
+    _fields_ = [("slot", ctypes.c_int32), ("space", ctypes.c_int32), ("shape_a", ctypes.c_int32),
+                ("shape_b", ctypes.c_int32), ("state", ctypes.c_int32), ("events", ctypes.c_int32),
+                ("count", ctypes.c_int32), ("_pad", ctypes.c_int32), ("normal", ctypes.c_double * 2),
+                ("e", ctypes.c_double), ("u", ctypes.c_double), ("surface_vr", ctypes.c_double * 2),
+                ("contact", (ctypes.c_double * 6) * 2), ("hash", ctypes.c_uint64 * 2)]
+
+
+class CallbackDecision(ctypes.Structure):
+    """cpgpuCallbackDecision: what begin / preSolve decided for one arbiter."""
+    _fields_ = [("slot", ctypes.c_int32), ("solve", ctypes.c_int32), ("ignore", ctypes.c_int32),
+                ("_pad", ctypes.c_int32), ("e", ctypes.c_double), ("u", ctypes.c_double),
+                ("surface_vr", ctypes.c_double * 2)]
+
+
+HANDLER_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(CallbackArbiter),
+                              ctypes.POINTER(CallbackDecision))
 
 
 class Stats(ctypes.Structure):
@@ -170,6 +197,12 @@ class PackedWorld:
         d.cons_collide_bodies = ptr(a["cons_collide_bodies"], ctypes.c_int32)
         if "shape_sensor" in a and np.any(a["shape_sensor"]):
             d.shape_sensor = ptr(a["shape_sensor"], ctypes.c_uint8)
+        if "shape_ctype" in a and np.any(a["shape_ctype"]):
+            d.shape_ctype = ptr(a["shape_ctype"], ctypes.c_uint32)
+        if "handlers" in a and len(a["handlers"]):
+            self._handlers = np.ascontiguousarray(a["handlers"], dtype=HANDLER_DTYPE)
+            d.n_handlers = len(self._handlers)
+            d.handlers = self._handlers.ctypes.data
         self._desc = d
         return d
 
@@ -187,7 +220,8 @@ class PackedWorld:
 def concat(worlds):
     """Pack several PackedWorld objects (each possibly multi-world) into one."""
     out = {"space": []}
-    keys = [k for k in worlds[0].a if k != "space" and not k.startswith("world_")]
+    keys = [k for k in worlds[0].a if k not in ("space", "handlers") and not k.startswith("world_")]
+    handlers = []
     for k in keys:
         out[k] = []
     wb, ws, wc = [0], [0], [0]
@@ -204,6 +238,11 @@ def concat(worlds):
             elif k == "shape_poly_start":
                 v = v + voff
             out[k].append(v)
+        # a world's handlers for every one of its spaces become handlers of each, by index
+        for h in a.get("handlers", np.zeros(0, HANDLER_DTYPE)):
+            spaces = range(len(a["space"])) if h["space"] < 0 else [int(h["space"])]
+            for sp in spaces:
+                handlers.append((sp + len(wb) - 1, h["type_a"], h["type_b"], h["events"]))
         for w in range(len(a["space"])):
             wb.append(int(a["world_body_start"][w + 1]) + boff)
             ws.append(int(a["world_shape_start"][w + 1]) + soff)
@@ -217,4 +256,5 @@ def concat(worlds):
     out["world_body_start"] = np.array(wb, dtype=np.int32)
     out["world_shape_start"] = np.array(ws, dtype=np.int32)
     out["world_constraint_start"] = np.array(wc, dtype=np.int32)
+    out["handlers"] = np.array(handlers, dtype=HANDLER_DTYPE)
     return PackedWorld(out)

@@ -507,12 +507,15 @@ __device__ __forceinline__ bool pair_ok(const DevWorld &W, int s, int t)
 {
 	const int ba = W.s_body[s], bb = W.s_body[t];
 	if (ba == bb) return false;
-	if (W.s_sensor && (W.s_sensor[s] | W.s_sensor[t])) return false;
 	const uint64_t ga = W.s_group[s], gb = W.s_group[t];
 	if ((ga != 0 && ga == gb) || (W.s_cats[s] & W.s_mask[t]) == 0 || (W.s_cats[t] & W.s_mask[s]) == 0) return false;
 	for (int k = W.nc_start[ba]; k < W.nc_start[ba + 1]; k++)
 		if (W.nc_list[k] == bb) return false;
-	if (W.b_m[ba] == __builtin_inf() && W.b_m[bb] == __builtin_inf()) return false;
+	// a pair never solved (cpSpaceStep.c:268-272) matters only to handlers that see it
+	if ((W.s_sensor && (W.s_sensor[s] | W.s_sensor[t])) ||
+	    (W.b_m[ba] == __builtin_inf() && W.b_m[bb] == __builtin_inf()))
+		return W.n_handlers > 0 &&
+		       (pair_events(W, s, t) & (CPGPU_EVENT_BEGIN | CPGPU_EVENT_PRE_SOLVE | CPGPU_EVENT_SEPARATE)) != 0;
 	return true;
 }
 

@@ -220,7 +220,7 @@ struct Arbiter {
 	uint32_t stamp;
 	int32_t world;
 	uint32_t id_in;            // GJK warm-start id the collision started from
-	int32_t _pad;
+	int32_t events;            // the pair's collision handler events (CPGPU_EVENT_*)
 	real nx, ny, e, u, svx, svy;
 	Contact c[2];
 };
@@ -268,6 +268,9 @@ struct DevWorld {
 	const uint64_t *s_group;
 	const uint32_t *s_cats, *s_mask;
 	const uint8_t *s_sensor;     // 1 for a sensor shape; nullptr when the world has none
+	const uint32_t *s_ctype;     // collision type; nullptr when the description has no handlers
+	const cpgpuHandlerDesc *handlers;
+	int n_handlers;
 	const real *s_r;
 	const real *s_geom;          // [n][10] body-local geometry
 	const int32_t *s_pstart, *s_pcount;
@@ -292,6 +295,26 @@ struct DevWorld {
 	// cumulative counters: [0] arbiters solved, [1] contacts solved
 	unsigned long long *totals;
 };
+
+// The events of the collision handlers that match shapes s and t: the pair's own
+// handler and the wildcard handlers of either type (cpSpaceLookupHandler,
+// cpSpace.c:382-399; cpArbiterUpdate, cpArbiter.c:402-409; the default
+// handlers call the wildcard ones, cpSpace.c:63-103).
+__device__ __forceinline__ uint32_t pair_events(const DevWorld &W, int s, int t)
+{
+	uint32_t ev = 0;
+	const uint32_t ta = W.s_ctype[s], tb = W.s_ctype[t];
+	const int space = W.s_world[s];
+	for (int k = 0; k < W.n_handlers; k++) {
+		const cpgpuHandlerDesc h = W.handlers[k];
+		if (h.space >= 0 && h.space != space) continue;
+		bool match;
+		if (h.type_b == CPGPU_WILDCARD) match = h.type_a == CPGPU_WILDCARD || h.type_a == ta || h.type_a == tb;
+		else match = (h.type_a == ta && h.type_b == tb) || (h.type_a == tb && h.type_b == ta);
+		if (match) ev |= h.events;
+	}
+	return ev;
+}
 
 #define CPGPU_ERR_PAIRS_OVERFLOW 1
 #define CPGPU_ERR_ARBITERS_OVERFLOW 2

@@ -28,6 +28,13 @@ struct StepBuffers {
 	// candidate pairs grouped by their shapes' type pair (the narrowphase's input order)
 	int2 *pairs_by_type;       // [pair capacity]
 	int32_t *type_counts;      // [2 * PAIR_CLASSES]: counts, then cursors
+	// collision handler callbacks: the solved-list slots of arbiters whose begin or preSolve
+	// is due (nullptr in a world without handlers), and the pairs that separated
+	int32_t *cb;               // [cap_act]
+	int32_t *n_cb;
+	int32_t *seps;             // [cap_seps][3] {space, shape a, shape b}
+	int32_t *n_seps;
+	int cap_seps;
 };
 
 #define PAIR_CLASSES 9         // type_a * 3 + type_b (type_a <= type_b)
@@ -51,6 +58,12 @@ hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 
 // cpShapesCollide (cpShape.c:260-283) for n shape pairs (device arrays) on the
 // shapes' cached geometry: count[n] and sets[n][12] {normal, then per point
 // pointA, pointB, distance}, written as doubles for either build.
+// collision handler callbacks over the step's due arbiters (B.cb, B.n_cb; launches sized
+// for cap, the kernels read the count): default decisions, the host's view, the decisions applied
+hipError_t callbacks_init(const StepBuffers &B, int cap, cpgpuCallbackDecision *dec, hipStream_t stream);
+hipError_t callbacks_pack(const StepBuffers &B, int cap, cpgpuCallbackArbiter *out, hipStream_t stream);
+hipError_t callbacks_apply(const DevWorld &W, const StepBuffers &B, int cap, const cpgpuCallbackDecision *dec,
+                           hipStream_t stream);
 hipError_t shapes_collide(const DevWorld &W, const int2 *pairs, int n, int32_t *count, double *sets,
                           hipStream_t stream);
 
@@ -131,7 +144,7 @@ struct SolveBuffers {
 	int32_t *arb_by_world;     // [cap] active arbiters sorted by key; nullptr for a single small space
 	uint64_t *keys;            // [cap] the sorted keys
 	int32_t *warb_off;         // [n_worlds + 1] first sorted position of each small world (the last: the grid set)
-	int32_t *large_first;      // [2] first sorted position of the grid set, of the wide set
+	int32_t *large_first;      // [3] first sorted position of the grid set, of the wide set; end of the solved
 	int n_worlds;
 	int n_large_joints;        // joints of large worlds (jorder[0..n))
 	const uint8_t *world_small;

@@ -19,6 +19,11 @@ thread_local std::string cpgpu_last_error;
 	void cpgpuWorldDestroy(cpgpuWorld *world); \
 	int cpgpuWorldStep(cpgpuWorld *world, double dt); \
 	int cpgpuWorldStepN(cpgpuWorld *world, double dt, int32_t n_steps); \
+	int cpgpuWorldStepBegin(cpgpuWorld *world, double dt, int64_t *n_callbacks); \
+	int cpgpuWorldReadCallbacks(cpgpuWorld *world, int64_t cap, int64_t *n_out, cpgpuCallbackArbiter *out); \
+	int cpgpuWorldWriteDecisions(cpgpuWorld *world, int64_t n, const cpgpuCallbackDecision *decisions); \
+	int cpgpuWorldStepEnd(cpgpuWorld *world); \
+	int cpgpuWorldReadSeparations(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs); \
 	int cpgpuWorldSync(cpgpuWorld *world); \
 	int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, double *w, double *v_bias, double *w_bias, double *transform); \
 	int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w, const double *f, const double *t); \
@@ -280,6 +285,34 @@ int cpgpuWorldGetSolveCounts(cpgpuWorld *world, int64_t *contacts, int64_t *bodi
 
 #define W64 reinterpret_cast<cpgpu64::cpgpuWorld *>(world)
 #define W32 reinterpret_cast<cpgpu32::cpgpuWorld *>(world)
+
+int cpgpuWorldStepBegin(cpgpuWorld *world, double dt, int64_t *n_callbacks)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldStepBegin(W64, dt, n_callbacks), cpgpu32::cpgpuWorldStepBegin(W32, dt, n_callbacks));
+}
+
+int cpgpuWorldReadCallbacks(cpgpuWorld *world, int64_t cap, int64_t *n_out, cpgpuCallbackArbiter *out)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldReadCallbacks(W64, cap, n_out, out),
+	              cpgpu32::cpgpuWorldReadCallbacks(W32, cap, n_out, out));
+}
+
+int cpgpuWorldWriteDecisions(cpgpuWorld *world, int64_t n, const cpgpuCallbackDecision *decisions)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldWriteDecisions(W64, n, decisions),
+	              cpgpu32::cpgpuWorldWriteDecisions(W32, n, decisions));
+}
+
+int cpgpuWorldStepEnd(cpgpuWorld *world)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldStepEnd(W64), cpgpu32::cpgpuWorldStepEnd(W32));
+}
+
+int cpgpuWorldReadSeparations(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs)
+{
+	CPGPU_FORWARD(cpgpu64::cpgpuWorldReadSeparations(W64, cap, n_out, pairs),
+	              cpgpu32::cpgpuWorldReadSeparations(W32, cap, n_out, pairs));
+}
 
 int cpgpuWorldQueryShapes(cpgpuWorld *world, int32_t n, const int32_t *shapes, int64_t cap, int64_t *n_out,
                           int32_t *hits, int32_t *counts, double *sets)

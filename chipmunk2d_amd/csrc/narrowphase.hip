@@ -89,6 +89,13 @@ __device__ __forceinline__ int32_t carry_arbiter(const StepBuffers &B, const Dev
 	const Arbiter *old = arb_ref(B, ref);
 	const WorldStep ws = W.ws[old->world];
 	const uint32_t ticks = ws.stamp - old->stamp;
+	// touched the step before, not this one: separate (cpSpaceStep.c:306-310)
+	if (ticks == 1u && (old->events & CPGPU_EVENT_SEPARATE) && B.seps) {
+		const int k = atomicAdd(B.n_seps, 1);
+		if (k < B.cap_seps) {
+			B.seps[3 * k] = old->world; B.seps[3 * k + 1] = old->sa; B.seps[3 * k + 2] = old->sb;
+		}
+	}
 	if (ticks >= ws.persistence) return -1;
 	const int m = append_index(B.n_cached);
 	if (m >= B.cap_cached) { raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW); return -1; }
@@ -102,6 +109,13 @@ __device__ __forceinline__ int32_t carry_arbiter(const StepBuffers &B, const Dev
 	for (int i = 0; i < (int)(sizeof(Arbiter) / sizeof(Piece)); i++) dst[i] = src[i];
 	B.cached[m].state = CPGPU_ARB_CACHED;
 	return m | CPGPU_CACHED_REF;
+}
+
+// a sensor's pair or one of two infinite masses: never solved (cpSpaceStep.c:268-272)
+__device__ __forceinline__ bool never_solved(const DevWorld &W, int s, int t)
+{
+	return (W.s_sensor && (W.s_sensor[s] | W.s_sensor[t])) ||
+	       (W.b_m[W.s_body[s]] == __builtin_inf() && W.b_m[W.s_body[t]] == __builtin_inf());
 }
 
 __device__ __forceinline__ void write_record(const StepBuffers &B, const DevWorld &W, uint64_t key, uint32_t id,
@@ -158,11 +172,13 @@ __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const Ste
 			a.world = world;
 			a.stamp = ws.stamp;
 			a.id_in = id;
-			a._pad = 0;
+			a.events = W.n_handlers > 0 ? (int32_t)pair_events(W, ia, ib) : 0;
 			// state: an arbiter solved last step was reset to NORMAL at the start of
-			// this step (cpSpaceStep.c:351-360); a cached one becomes FIRST_COLLISION
+			// this step (cpSpaceStep.c:351-360), an ignored one stays ignored while its
+			// shapes touch; a cached one becomes FIRST_COLLISION
 			if (po == nullptr) a.state = CPGPU_ARB_FIRST_COLLISION;
-			else a.state = (po->stamp == ws.stamp - 1u) ? CPGPU_ARB_NORMAL : CPGPU_ARB_FIRST_COLLISION;
+			else if (po->stamp == ws.stamp - 1u) a.state = po->state == CPGPU_ARB_IGNORE ? CPGPU_ARB_IGNORE : CPGPU_ARB_NORMAL;
+			else a.state = CPGPU_ARB_FIRST_COLLISION;
 			const real2 pA = W.b_p[a.ba], pB = W.b_p[a.bb];
 			const int old_count = po ? po->count : 0;
 			for (int c = 0; c < 2; c++) {
@@ -196,8 +212,23 @@ __global__ void __launch_bounds__(256) k_narrowphase(const DevWorld W, const Ste
 			const V2 surface_vr = v2(svB.x - svA.x, svB.y - svA.y);
 			const V2 svr = vsub(surface_vr, vmult(info.n, vdot(surface_vr, info.n)));
 			a.svx = svr.x; a.svy = svr.y;
+			// begin (on a first collision) and preSolve are the host's (cpSpaceStep.c:255-266);
+			// an ignored arbiter, or one never solved that no callback sees, keeps no contacts
+			// (:276-282)
+			const bool due = a.state != CPGPU_ARB_IGNORE &&
+			                 ((a.events & CPGPU_EVENT_PRE_SOLVE) ||
+			                  (a.state == CPGPU_ARB_FIRST_COLLISION && (a.events & CPGPU_EVENT_BEGIN)));
+			if (a.state == CPGPU_ARB_IGNORE || (!due && never_solved(W, ia, ib))) {
+				a.count = 0;
+				if (a.state != CPGPU_ARB_IGNORE) a.state = CPGPU_ARB_NORMAL;
+			}
 			B.act[m] = a;
 			arb_out = m;
+			if (due && B.cb) {
+				const int k = append_index(B.n_cb);
+				if (k < B.cap_act) B.cb[k] = m;
+				else raise_err(W.err, CPGPU_ERR_ARBITERS_OVERFLOW);
+			}
 		}
 	} else if (old >= 0) {
 		arb_out = carry_arbiter(B, W, old);
@@ -226,6 +257,65 @@ __global__ void k_carry(const DevWorld W, const StepBuffers B)
 		if (rec.arb >= 0) arb = carry_arbiter(B, W, rec.arb);
 		if (arb < 0 && !id_ok) continue;
 		write_record(B, W, rec.key, rec.id, rec.id_stamp, arb);
+	}
+}
+
+// ---- collision handler callbacks (cpSpaceStep.c:253-284) ----
+
+// the due arbiters' default decisions: solved, values unchanged
+__global__ void k_callbacks_init(const StepBuffers B, cpgpuCallbackDecision *dec)
+{
+	const int n = min(*B.n_cb, B.cap_act);
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+		const Arbiter &a = B.act[B.cb[k]];
+		cpgpuCallbackDecision d;
+		d.slot = k; d.solve = 1; d.ignore = 0; d._pad = 0;
+		d.e = (double)a.e; d.u = (double)a.u;
+		d.surface_vr[0] = (double)a.svx; d.surface_vr[1] = (double)a.svy;
+		dec[k] = d;
+	}
+}
+
+// the arbiters as the host's callbacks see them
+__global__ void k_callbacks_pack(const StepBuffers B, cpgpuCallbackArbiter *out)
+{
+	const int n = min(*B.n_cb, B.cap_act);
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+		const Arbiter &a = B.act[B.cb[k]];
+		cpgpuCallbackArbiter v;
+		v.slot = k; v.space = a.world; v.shape_a = a.sa; v.shape_b = a.sb;
+		v.state = a.state; v.events = a.events; v.count = a.count; v._pad = 0;
+		v.normal[0] = (double)a.nx; v.normal[1] = (double)a.ny;
+		v.e = (double)a.e; v.u = (double)a.u;
+		v.surface_vr[0] = (double)a.svx; v.surface_vr[1] = (double)a.svy;
+		for (int c = 0; c < 2; c++) {
+			const Contact &q = a.c[c];
+			const bool live = c < a.count;
+			v.contact[c][0] = live ? (double)q.r1x : 0.0; v.contact[c][1] = live ? (double)q.r1y : 0.0;
+			v.contact[c][2] = live ? (double)q.r2x : 0.0; v.contact[c][3] = live ? (double)q.r2y : 0.0;
+			v.contact[c][4] = live ? (double)q.jnAcc : 0.0; v.contact[c][5] = live ? (double)q.jtAcc : 0.0;
+			v.hash[c] = live ? q.hash : 0;
+		}
+		out[k] = v;
+	}
+}
+
+// what begin and preSolve decided: an ignored arbiter stays ignored; one not solved keeps no
+// contacts and, unless ignored, counts as normal (:268-282)
+__global__ void k_callbacks_apply(const DevWorld W, const StepBuffers B, const cpgpuCallbackDecision *dec)
+{
+	const int n = min(*B.n_cb, B.cap_act);
+	for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+		Arbiter &a = B.act[B.cb[k]];
+		const cpgpuCallbackDecision d = dec[k];
+		if (d.ignore) a.state = CPGPU_ARB_IGNORE;
+		if (d.solve && !d.ignore && !never_solved(W, a.sa, a.sb)) {
+			a.e = (real)d.e; a.u = (real)d.u;
+			a.svx = (real)d.surface_vr[0]; a.svy = (real)d.surface_vr[1];
+		} else {
+			a.count = 0;
+			if (a.state != CPGPU_ARB_IGNORE) a.state = CPGPU_ARB_NORMAL;
+		}
 	}
 }
 
@@ -328,8 +418,9 @@ hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 
                             int n_pairs_host, int n_prev_recs_host, hipStream_t stream, KernelTimer *timer)
 {
 	// the step's three counters are consecutive: records, solved arbiters, cached arbiters
-	// (cleared by a kernel: the step may be captured in a graph)
+	// (cleared by a kernel: the step may be captured in a graph); then the callbacks' two
 	k_fill_i32<<<1, 32, 0, stream>>>(B.n_recs, 3, 0);
+	if (B.n_cb) k_fill_i32<<<1, 32, 0, stream>>>(B.n_cb, 2, 0);
 	if (n_pairs_host > 0) {
 		const int g = std::min(blocks_for(n_pairs_host, 256), 1024);
 		TIMED(timer, "pair_classes", stream, {
@@ -350,6 +441,28 @@ hipError_t narrowphase_step(const DevWorld &W, const StepBuffers &B, const int2 
 		HIP_TRY(hipGetLastError());
 	}
 	return hipSuccess;
+}
+
+hipError_t callbacks_init(const StepBuffers &B, int cap, cpgpuCallbackDecision *dec, hipStream_t stream)
+{
+	if (cap <= 0) return hipSuccess;
+	k_callbacks_init<<<std::min(blocks_for(cap, 256), 1024), 256, 0, stream>>>(B, dec);
+	return hipGetLastError();
+}
+
+hipError_t callbacks_pack(const StepBuffers &B, int cap, cpgpuCallbackArbiter *out, hipStream_t stream)
+{
+	if (cap <= 0) return hipSuccess;
+	k_callbacks_pack<<<std::min(blocks_for(cap, 256), 1024), 256, 0, stream>>>(B, out);
+	return hipGetLastError();
+}
+
+hipError_t callbacks_apply(const DevWorld &W, const StepBuffers &B, int cap, const cpgpuCallbackDecision *dec,
+                           hipStream_t stream)
+{
+	if (cap <= 0) return hipSuccess;
+	k_callbacks_apply<<<std::min(blocks_for(cap, 256), 1024), 256, 0, stream>>>(W, B, dec);
+	return hipGetLastError();
 }
 
 hipError_t shapes_collide(const DevWorld &W, const int2 *pairs, int n, int32_t *count, double *sets,

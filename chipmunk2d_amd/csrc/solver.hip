@@ -964,6 +964,10 @@ __global__ void __launch_bounds__(THREADS, THREADS >= 1024 ? 4 : 2)
 // large world, then the wide set), the unit of an arbiter whose dynamic
 // bodies lie in strips pa and pb (-1: none) of the l-th large world, and the
 // claim arrays of a unit's colouring.
+__device__ __forceinline__ uint64_t excluded_key(const SolveBuffers &S)
+{
+	return (uint64_t)S.n_worlds + (uint64_t)grid_units(S.n_large_worlds, S.parts);
+}
 __device__ __forceinline__ uint64_t unit_key(const SolveBuffers &S, int u)
 {
 	return (uint64_t)S.n_worlds + (uint64_t)u;
@@ -1095,7 +1099,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_solve_grid(const DevWorld W, c
 	BodyVel *lbody = reinterpret_cast<BodyVel *>(dyn);
 	int32_t *lbidx = reinterpret_cast<int32_t *>(dyn + GRID_RES_BODIES * sizeof(BodyVel));
 	int lds_used = 0;
-	const int n_act = *S.n_act;
+	const int n_act = S.large_first[2];   // the solved arbiters' end (unsolved ones sort after)
 	const int grid_first = S.large_first[0], wide_first = S.large_first[1];
 	const int n_large = S.n_large_worlds;
 	const int parts = S.parts;
@@ -1573,7 +1577,10 @@ __global__ void __launch_bounds__(256) k_arb_keys(const DevWorld W, const SolveB
 		Arbiter &arb = S.act[k];
 		const int w = arb.world;
 		unsigned long long key;
-		if (S.world_small[w]) {
+		if (arb.count == 0) {
+			// touching but not solved (a collision handler's or a sensor's): after every set
+			key = excluded_key(S);
+		} else if (S.world_small[w]) {
 			key = (unsigned long long)w;
 		} else {
 			// the strip of its dynamic bodies, the seam of two neighbouring strips, or the wide set
@@ -1612,7 +1619,8 @@ __global__ void k_large_integrate(const DevWorld W, const SolveBuffers S)
 }
 
 // warb_off[w] = first sorted position of small world w (n_worlds + 1 entries);
-// large_first = first position of the grid set, of the wide set
+// large_first = first position of the grid set, of the wide set, and the end of
+// the solved arbiters (the unsolved ones sort after them)
 __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 {
 	const int n = *S.n_act;
@@ -1621,6 +1629,7 @@ __global__ void k_world_offsets(const SolveBuffers S, int n_worlds)
 	if (w == n_worlds + 1) {
 		S.large_first[0] = lower_bound_u64(S.keys, 0, n, unit_key(S, 0));
 		S.large_first[1] = lower_bound_u64(S.keys, 0, n, unit_key(S, grid_units(S.n_large_worlds, S.parts) - 1));
+		S.large_first[2] = lower_bound_u64(S.keys, 0, n, excluded_key(S));
 	}
 }
 
@@ -1680,10 +1689,10 @@ hipError_t solver_group(const DevWorld &W, const SolveBuffers &S, int n_act, int
 		      k_arb_keys<<<std::min((n_act + 255) / 256, GROUP_BLOCKS), 256, 0, stream>>>(
 		          W, S, (unsigned long long *)sortbuf.orand));
 		HIP_TRY(hipGetLastError());
-		// keys below n_worlds + grid_units
+		// keys up to n_worlds + grid_units (the unsolved arbiters' key)
 		const uint64_t key_end = (uint64_t)n_worlds + (uint64_t)grid_units(S.n_large_worlds, S.parts);
 		int key_bits = 1;
-		while ((1ull << key_bits) < key_end) key_bits++;
+		while ((1ull << key_bits) <= key_end) key_bits++;
 		HIP_TRY(radix_sort_pairs(S.keys, S.arb_by_world, n_act, S.n_act, key_bits, sortbuf, stream, timer));
 	}
 	if (large && nb > 0 && S.n_large_joints == 0) {

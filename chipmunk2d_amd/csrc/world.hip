@@ -259,6 +259,19 @@ struct cpgpuWorld {
 	int cached_cap[2] = {0, 0};
 	int32_t *d_step_counts = nullptr;   // [2][3]: n_recs, n_act, n_cached per slot
 	int cur = 0;
+	// collision handler callbacks, answered between cpgpuWorldStepBegin and cpgpuWorldStepEnd
+	int n_handlers = 0;
+	bool in_step = false;                   // a step begun and not ended
+	bool zero_step = false;                 // ... with dt 0: nothing to do
+	int32_t *d_cb = nullptr;                // [cb_cap] solved-list slots of the due arbiters
+	cpgpuCallbackDecision *d_dec = nullptr; // [cb_cap]
+	cpgpuCallbackArbiter *d_cbview = nullptr;
+	int cb_cap = 0, dec_cap = 0, cbview_cap = 0;
+	int32_t *d_seps = nullptr;              // [seps_words / 3][3]
+	int seps_words = 0;
+	int32_t *d_hcounts = nullptr;           // [0] due arbiters, [1] separations
+	int n_cb = 0;
+	int step_pairs = 0, step_prev_recs = 0; // bounds of the begun step, for its second half
 	uint64_t *tkeys = nullptr;
 	int32_t *tvals = nullptr;
 	int tcap = 0;
@@ -423,6 +436,21 @@ int validate(const cpgpuWorldDesc *d)
 		if (d->space[w].iterations < 0) return fail(CPGPU_ERR_INVALID, "iterations must be >= 0");
 	}
 	if (d->n_worlds >= (1 << 20)) return fail(CPGPU_ERR_INVALID, "at most 2^20 - 1 worlds");
+	if (d->n_handlers < 0 || (d->n_handlers > 0 && !d->handlers))
+		return fail(CPGPU_ERR_INVALID, "bad collision handler list");
+	if (d->n_handlers > 256) return fail(CPGPU_ERR_INVALID, "at most 256 collision handlers");
+	for (int k = 0; k < d->n_handlers; k++) {
+		const cpgpuHandlerDesc &h = d->handlers[k];
+		if (h.space < -1 || h.space >= d->n_worlds)
+			return fail(CPGPU_ERR_INVALID, "collision handler " + std::to_string(k) + " names space " +
+			                                   std::to_string(h.space));
+		if (h.type_a == CPGPU_WILDCARD && h.type_b != CPGPU_WILDCARD)
+			return fail(CPGPU_ERR_INVALID, "a wildcard handler gives its type first");
+	}
+	if (d->n_handlers > 0 && d->shape_ctype)
+		for (int s = 0; s < d->n_shapes; s++)
+			if (d->shape_ctype[s] == CPGPU_WILDCARD)
+				return fail(CPGPU_ERR_INVALID, "a shape's collision type is the wildcard");
 	std::vector<int> body_world(d->n_bodies);
 	for (int w = 0; w < d->n_worlds; w++)
 		for (int b = d->world_body_start[w]; b < d->world_body_start[w + 1]; b++) body_world[b] = w;
@@ -615,6 +643,21 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 				HIP_OR_FAIL(own_upload(w, psens, d->shape_sensor, ns));
 				W.s_sensor = psens;
 			}
+		}
+		w->n_handlers = (d->n_handlers > 0 && d->handlers) ? d->n_handlers : 0;
+		W.n_handlers = w->n_handlers;
+		W.handlers = nullptr;
+		W.s_ctype = nullptr;
+		if (w->n_handlers > 0) {
+			cpgpuHandlerDesc *phd;
+			HIP_OR_FAIL(own_upload(w, phd, d->handlers, (size_t)w->n_handlers));
+			W.handlers = phd;
+			std::vector<uint32_t> ct(ns, 0u);
+			if (d->shape_ctype) ct.assign(d->shape_ctype, d->shape_ctype + ns);
+			uint32_t *pct;
+			HIP_OR_FAIL(own_upload(w, pct, ct.data(), (size_t)ns));
+			W.s_ctype = pct;
+			HIP_OR_FAIL(own_upload(w, w->d_hcounts, (const int32_t *)nullptr, 2));
 		}
 		HIP_OR_FAIL(own_upload_real(w, pr, d->shape_radius, ns)); W.s_r = pr;
 		HIP_OR_FAIL(own_upload_real(w, pgeom, d->shape_geom, 10 * (size_t)ns)); W.s_geom = pgeom;
@@ -812,7 +855,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_alloc(w, w->d_joint_bias_coef, nc));
 	HIP_OR_FAIL(own_upload(w, w->bar, (const GridBarrier *)nullptr, 1));
 	HIP_OR_FAIL(own_upload(w, w->warb_off, (const int32_t *)nullptr, w->nw + 2));
-	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 2));
+	HIP_OR_FAIL(own_upload(w, w->large_first, (const int32_t *)nullptr, 3));
 	HIP_OR_FAIL(own_alloc(w, w->bpart, nb));
 	HIP_OR_FAIL(own_alloc(w, w->bslot, nb));
 	HIP_OR_FAIL(own_alloc(w, w->claim, CLAIM_SETS * 2 * (size_t)nb));
@@ -865,7 +908,7 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	radix_sort_free(w->sortbuf);
 	void *extra[] = {w->d_pairs, w->d_pairs_by_type, w->recs[0], w->recs[1], w->act[0], w->act[1], w->cached[0], w->cached[1],
 	                 w->tkeys, w->tvals, w->seen, w->colour, w->order, w->arb_by_world, w->world_keys,
-	                 w->cbody, w->cprio, w->ccol};
+	                 w->cbody, w->cprio, w->ccol, w->d_cb, w->d_dec, w->d_cbview, w->d_seps};
 	for (void *p : extra) if (p) (void)hipFree(p);
 	for (int slot = 0; slot < 2; slot++)
 		if (w->gexec[slot]) (void)hipGraphExecDestroy(w->gexec[slot]);
@@ -894,7 +937,8 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 	S.act = act; S.n_act = n_act;
 	S.colour = colour; S.order = order;
 	S.cbody = w->cbody; S.cprio = w->cprio; S.ccol = w->ccol;
-	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
+	// with collision handlers the arbiters a step does not solve are sorted out of every set
+	const bool grouped = w->nw > 1 || w->n_large_worlds > 0 || w->n_handlers > 0;
 	S.arb_by_world = grouped ? w->arb_by_world : nullptr;
 	S.keys = w->world_keys;
 	S.n_large_joints = w->n_large_joints;
@@ -1024,12 +1068,9 @@ static hipError_t enqueue_front(cpgpuWorld *w, KernelTimer *timer)
 	return broadphase_pairs(w->dyn_tree, w->static_tree, W, w->d_pairs, w->pair_cap, w->d_counters, w->stream, timer);
 }
 
-// The narrowphase, arbiter persistence and the solve of a step
-// (cpSpaceStep.c:372-440).  n_pairs and n_prev_recs bound this step's candidate
-// pairs and last step's records from above: the kernels read the counts.
-static hipError_t enqueue_back(cpgpuWorld *w, int n_pairs, int n_prev_recs, KernelTimer *timer)
+// The buffers of the step in slot w->cur.
+static StepBuffers step_buffers(cpgpuWorld *w)
 {
-	DevWorld &W = w->W;
 	const int prev = w->cur ^ 1, cur = w->cur;
 	StepBuffers B;
 	B.tkeys = w->tkeys; B.tvals = w->tvals; B.tmask = w->tcap - 1;
@@ -1042,10 +1083,36 @@ static hipError_t enqueue_back(cpgpuWorld *w, int n_pairs, int n_prev_recs, Kern
 	B.cached = w->cached[cur]; B.n_cached = w->d_step_counts + 3 * cur + 2; B.cap_cached = w->cached_cap[cur];
 	B.pairs_by_type = w->d_pairs_by_type;
 	B.type_counts = w->d_type_counts;
+	B.cb = nullptr; B.n_cb = nullptr; B.seps = nullptr; B.n_seps = nullptr; B.cap_seps = 0;
+	if (w->n_handlers > 0) {
+		B.cb = w->d_cb; B.n_cb = w->d_hcounts;
+		B.seps = w->d_seps; B.n_seps = w->d_hcounts + 1; B.cap_seps = w->seps_words / 3;
+	}
+	return B;
+}
+
+// The narrowphase and arbiter persistence of a step (cpSpaceStep.c:372-380):
+// up to where the collision handlers' begin and preSolve are due.  n_pairs and
+// n_prev_recs bound this step's candidate pairs and last step's records from
+// above: the kernels read the counts.
+static hipError_t enqueue_narrow(cpgpuWorld *w, int n_pairs, int n_prev_recs, KernelTimer *timer)
+{
+	DevWorld &W = w->W;
+	const StepBuffers B = step_buffers(w);
 	HIP_TRY(narrowphase_table(W, B, n_prev_recs, w->stream, timer));
 	HIP_TRY(narrowphase_step(W, B, w->d_pairs, w->d_counters, n_pairs, n_prev_recs, w->stream, timer));
+	if (w->n_handlers > 0) HIP_TRY(callbacks_init(B, w->act_cap[w->cur], w->d_dec, w->stream));
+	return hipSuccess;
+}
 
-	const bool grouped = w->nw > 1 || w->n_large_worlds > 0;
+// The solve of a step (cpSpaceStep.c:382-440), after the handlers' decisions.
+static hipError_t enqueue_solve(cpgpuWorld *w, int n_pairs, KernelTimer *timer)
+{
+	DevWorld &W = w->W;
+	const int cur = w->cur;
+	const StepBuffers B = step_buffers(w);
+	if (w->n_handlers > 0) HIP_TRY(callbacks_apply(W, B, w->act_cap[cur], w->d_dec, w->stream));
+	const bool grouped = w->nw > 1 || w->n_large_worlds > 0 || w->n_handlers > 0;
 	SolveBuffers S = solve_buffers(w, w->act[cur], B.n_act, w->colour, w->order, w->warb_off);
 	if (w->n_large_worlds > 0 && w->n_large_bodies > 0)
 		HIP_TRY(hipStreamWaitEvent(w->stream, w->fork_join[CPGPU_MAX_BUCKETS + 2], 0));
@@ -1056,6 +1123,13 @@ static hipError_t enqueue_back(cpgpuWorld *w, int n_pairs, int n_prev_recs, Kern
 	}
 	return solver_step(W, S, w->buckets, w->n_buckets, w->n_large_worlds > 0 ? w->solver_grid : 0, w->stream, w->aux,
 	                   w->fork_join, timer);
+}
+
+// The narrowphase, arbiter persistence and the solve of a step (cpSpaceStep.c:372-440).
+static hipError_t enqueue_back(cpgpuWorld *w, int n_pairs, int n_prev_recs, KernelTimer *timer)
+{
+	HIP_TRY(enqueue_narrow(w, n_pairs, n_prev_recs, timer));
+	return enqueue_solve(w, n_pairs, timer);
 }
 
 // Buffers of a step's outputs in arbiter slot `slot`: records, solved and
@@ -1076,6 +1150,12 @@ static hipError_t reserve_step(cpgpuWorld *w, int slot, long recs, long act, lon
 		w->tcap = tcap;
 	}
 	HIP_TRY(regrow(w->seen, w->seen_cap, prev_recs + 1));
+	if (w->n_handlers > 0) {
+		HIP_TRY(regrow(w->d_cb, w->cb_cap, w->act_cap[slot]));
+		HIP_TRY(regrow(w->d_dec, w->dec_cap, w->act_cap[slot]));
+		// only the arbiters touched last step can separate: at most its solved list
+		HIP_TRY(regrow(w->d_seps, w->seps_words, 3L * (w->act_cap[slot ^ 1] + 1)));
+	}
 	if (w->act_cap[slot] > w->colour_cap) {
 		void *old[] = {w->colour, w->order, w->arb_by_world, w->world_keys, w->cbody, w->cprio, w->ccol};
 		for (void *p : old) if (p) (void)hipFree(p);
@@ -1093,11 +1173,10 @@ static hipError_t reserve_step(cpgpuWorld *w, int slot, long recs, long act, lon
 
 static int graph_step(cpgpuWorld *w, double dt);
 
-int cpgpuWorldStep(cpgpuWorld *w, double dt)
+// The first half of a step (cpSpaceStep.c:341-380): up to where the collision
+// handlers' begin and preSolve are due.
+static int step_begin(cpgpuWorld *w, double dt)
 {
-	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
-	if (dt == 0.0) return CPGPU_OK;   // cpSpaceStep.c:339
-	if (w->graph_on) return graph_step(w, dt);
 	DevWorld &W = w->W;
 	KernelTimer *timer = w->timer.enabled ? &w->timer : nullptr;
 
@@ -1146,7 +1225,18 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	HIP_OR_FAIL(reserve_step(w, cur, (long)n_pairs + n_prev_recs + 1, (long)n_pairs + 1, (long)n_prev_act + n_prev_cached + 1,
 	                         n_prev_recs));
 	if (w->steps == 0) HIP_OR_FAIL(hipMemsetAsync(w->d_step_counts + 3 * prev, 0, 3 * sizeof(int32_t), w->stream));
-	HIP_OR_FAIL(enqueue_back(w, n_pairs, n_prev_recs, timer));
+	HIP_OR_FAIL(enqueue_narrow(w, n_pairs, n_prev_recs, timer));
+	w->step_pairs = n_pairs;
+	w->step_prev_recs = n_prev_recs;
+	return CPGPU_OK;
+}
+
+// The second half: the solve, with the decisions of the handlers, and the tape.
+static int step_end(cpgpuWorld *w)
+{
+	KernelTimer *timer = w->timer.enabled ? &w->timer : nullptr;
+	const int cur = w->cur;
+	HIP_OR_FAIL(enqueue_solve(w, w->step_pairs, timer));
 
 	if (w->tape.on) {
 		// record the solved arbiters (their contacts learn where they were recorded)
@@ -1165,6 +1255,113 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	w->cur ^= 1;
 	w->steps++;
 	w->pending = true;
+	return CPGPU_OK;
+}
+
+int cpgpuWorldStep(cpgpuWorld *w, double dt)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (w->in_step) return fail(CPGPU_ERR_INVALID, "a step begun with cpgpuWorldStepBegin is not ended");
+	if (dt == 0.0) return CPGPU_OK;   // cpSpaceStep.c:339
+	if (w->graph_on) return graph_step(w, dt);
+	const int rc = step_begin(w, dt);
+	if (rc != CPGPU_OK) return rc;
+	return step_end(w);
+}
+
+// ---- collision handler callbacks: a step in two halves around the host ----
+
+int cpgpuWorldStepBegin(cpgpuWorld *w, double dt, int64_t *n_callbacks)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (n_callbacks) *n_callbacks = 0;
+	if (w->in_step) return fail(CPGPU_ERR_INVALID, "a step begun with cpgpuWorldStepBegin is not ended");
+	if (w->graph_on) return fail(CPGPU_ERR_INVALID, "a step with callbacks is stepped without graphs");
+	w->n_cb = 0;
+	if (dt == 0.0) {   // cpSpaceStep.c:339: nothing happens
+		w->in_step = true;
+		w->zero_step = true;
+		return CPGPU_OK;
+	}
+	int rc = step_begin(w, dt);
+	if (rc != CPGPU_OK) return rc;
+	w->in_step = true;
+	w->zero_step = false;
+	if (w->n_handlers > 0) {
+		HIP_OR_FAIL(hipMemcpyAsync(w->h_counts + 11, w->d_hcounts, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream));
+		rc = sync_world(w);
+		if (rc != CPGPU_OK) return rc;
+		w->n_cb = std::min(w->h_counts[11], w->act_cap[w->cur]);
+	}
+	if (n_callbacks) *n_callbacks = w->n_cb;
+	return CPGPU_OK;
+}
+
+int cpgpuWorldReadCallbacks(cpgpuWorld *w, int64_t cap, int64_t *n_out, cpgpuCallbackArbiter *out)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (!w->in_step) return fail(CPGPU_ERR_INVALID, "no step begun with cpgpuWorldStepBegin");
+	if (cap < 0) return fail(CPGPU_ERR_INVALID, "negative capacity");
+	if (cap > 0 && !out) return fail(CPGPU_ERR_INVALID, "null output array");
+	if (n_out) *n_out = w->n_cb;
+	const int n = (int)std::min<int64_t>(cap, w->n_cb);
+	if (n <= 0 || w->zero_step) return CPGPU_OK;
+	HIP_OR_FAIL(regrow(w->d_cbview, w->cbview_cap, w->n_cb));
+	HIP_OR_FAIL(callbacks_pack(step_buffers(w), w->n_cb, w->d_cbview, w->stream));
+	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+	HIP_OR_FAIL(hipMemcpy(out, w->d_cbview, sizeof(cpgpuCallbackArbiter) * (size_t)n, hipMemcpyDeviceToHost));
+	return CPGPU_OK;
+}
+
+int cpgpuWorldWriteDecisions(cpgpuWorld *w, int64_t n, const cpgpuCallbackDecision *decisions)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (!w->in_step) return fail(CPGPU_ERR_INVALID, "no step begun with cpgpuWorldStepBegin");
+	if (n < 0 || (n > 0 && !decisions)) return fail(CPGPU_ERR_INVALID, "bad decision list");
+	for (int64_t k = 0; k < n; k++)
+		if (decisions[k].slot < 0 || decisions[k].slot >= w->n_cb)
+			return fail(CPGPU_ERR_INVALID, "decision " + std::to_string(k) + " names slot " +
+			                                   std::to_string(decisions[k].slot) + " of " + std::to_string(w->n_cb));
+	if (n == 0) return CPGPU_OK;
+	std::vector<cpgpuCallbackDecision> h((size_t)w->n_cb);
+	HIP_OR_FAIL(hipStreamSynchronize(w->stream));
+	HIP_OR_FAIL(hipMemcpy(h.data(), w->d_dec, sizeof(cpgpuCallbackDecision) * h.size(), hipMemcpyDeviceToHost));
+	for (int64_t k = 0; k < n; k++) {
+		cpgpuCallbackDecision d = decisions[k];
+		d.solve = d.solve ? 1 : 0;
+		d.ignore = d.ignore ? 1 : 0;
+		h[(size_t)d.slot] = d;
+	}
+	HIP_OR_FAIL(hipMemcpy(w->d_dec, h.data(), sizeof(cpgpuCallbackDecision) * h.size(), hipMemcpyHostToDevice));
+	return CPGPU_OK;
+}
+
+int cpgpuWorldStepEnd(cpgpuWorld *w)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (!w->in_step) return fail(CPGPU_ERR_INVALID, "no step begun with cpgpuWorldStepBegin");
+	w->in_step = false;
+	if (w->zero_step) return CPGPU_OK;
+	return step_end(w);
+}
+
+int cpgpuWorldReadSeparations(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *pairs)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	int rc = sync_world(w);
+	if (rc != CPGPU_OK) return rc;
+	if (cap < 0) return fail(CPGPU_ERR_INVALID, "negative capacity");
+	if (cap > 0 && !pairs) return fail(CPGPU_ERR_INVALID, "null output array");
+	if (w->n_handlers == 0 || w->steps == 0) {
+		if (n_out) *n_out = 0;
+		return CPGPU_OK;
+	}
+	int32_t n = 0;
+	HIP_OR_FAIL(hipMemcpy(&n, w->d_hcounts + 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+	if (n > w->seps_words / 3) return fail(CPGPU_ERR_ARBITERS_OVERFLOW, "more separations than their buffer holds");
+	if (n_out) *n_out = n;
+	const int64_t m = std::min<int64_t>(cap, n);
+	if (m > 0) HIP_OR_FAIL(hipMemcpy(pairs, w->d_seps, sizeof(int32_t) * 3 * (size_t)m, hipMemcpyDeviceToHost));
 	return CPGPU_OK;
 }
 
@@ -1300,6 +1497,8 @@ int cpgpuWorldSetGraph(cpgpuWorld *w, int32_t enable)
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	if (enable) {
+		if (w->n_handlers > 0) return fail(CPGPU_ERR_INVALID, "spaces with collision handlers are stepped without graphs");
+		if (w->in_step) return fail(CPGPU_ERR_INVALID, "a step begun with cpgpuWorldStepBegin is not ended");
 		if (w->tape.on) return fail(CPGPU_ERR_INVALID, "a taped rollout is stepped without graphs");
 		if (w->timer.enabled) return fail(CPGPU_ERR_INVALID, "timed steps (cpgpuWorldSetTiming) are stepped without graphs");
 		if (w->n_large_worlds > 0)
@@ -1732,8 +1931,14 @@ int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
 			HIP_OR_FAIL(hipMemcpy(arbs.data(), w->act[last], sizeof(Arbiter) * out->arbiters, hipMemcpyDeviceToHost));
 			HIP_OR_FAIL(hipMemcpy(col.data(), w->colour, sizeof(int32_t) * out->arbiters, hipMemcpyDeviceToHost));
 		}
+		// with collision handlers, arbiters touching but not solved have count 0 and no colour
+		if (w->n_handlers > 0) {
+			out->arbiters = 0;
+			for (auto &a : arbs) out->arbiters += a.count > 0;
+		}
 		for (auto &a : arbs) out->contacts += a.count;
-		for (int32_t c : col) out->arbiter_colours = std::max(out->arbiter_colours, c + 1);
+		for (size_t k = 0; k < col.size(); k++)
+			if (arbs[k].count > 0) out->arbiter_colours = std::max(out->arbiter_colours, col[k] + 1);
 	}
 	out->constraint_colours = w->n_jcolours;
 	out->steps = w->steps;
@@ -1757,19 +1962,26 @@ int cpgpuWorldReadArbiters(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *
 	if (rc != CPGPU_OK) return rc;
 	const int last = w->cur ^ 1;
 	const int n = w->steps > 0 ? w->h_counts[3 * last + 1] : 0;
-	if (n_out) *n_out = n;
-	if (n == 0 || cap <= 0) return CPGPU_OK;
+	// with collision handlers, arbiters touching but not solved (count 0) are not among the step's
+	if (n_out && w->n_handlers == 0) *n_out = n;
+	if (n == 0 || (cap <= 0 && w->n_handlers == 0)) {
+		if (n_out && n == 0) *n_out = 0;
+		return CPGPU_OK;
+	}
 	std::vector<Arbiter> arbs(n);
 	std::vector<int32_t> col(n);
 	HIP_OR_FAIL(hipMemcpy(arbs.data(), w->act[last], sizeof(Arbiter) * n, hipMemcpyDeviceToHost));
 	HIP_OR_FAIL(hipMemcpy(col.data(), w->colour, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-	std::vector<int> idx(n);
-	for (int k = 0; k < n; k++) idx[k] = k;
+	std::vector<int> idx;
+	idx.reserve(n);
+	for (int k = 0; k < n; k++)
+		if (arbs[k].count > 0) idx.push_back(k);
+	if (n_out) *n_out = (int64_t)idx.size();
 	std::sort(idx.begin(), idx.end(), [&](int x, int y) {
 		if (col[x] != col[y]) return col[x] < col[y];
 		return arbs[x].key < arbs[y].key;
 	});
-	for (int q = 0; q < n && q < cap; q++) {
+	for (int q = 0; q < (int)idx.size() && q < cap; q++) {
 		const Arbiter &a = arbs[idx[q]];
 		if (shapes) { shapes[2 * q] = a.sa; shapes[2 * q + 1] = a.sb; }
 		if (colour) colour[q] = col[idx[q]];
@@ -2241,6 +2453,8 @@ int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
 	if (w->graph_on) return fail(CPGPU_ERR_INVALID, "a taped rollout is stepped without graphs (cpgpuWorldSetGraph 0)");
 	if (w->other_joints)
 		return fail(CPGPU_ERR_INVALID, "the adjoint covers pivot, damped-spring, slide and gear joints only");
+	if (w->n_handlers > 0)
+		return fail(CPGPU_ERR_INVALID, "the adjoint covers spaces without collision handlers only");
 	if (w->n_large_worlds > 0)
 		return fail(CPGPU_ERR_INVALID, "the adjoint supports spaces solved one workgroup each (at most " +
 		                                   std::to_string(w->small_world_max_bodies) + " dynamic bodies)");

@@ -9,7 +9,8 @@ import os
 
 import numpy as np
 
-from .cabi import SpaceParams, Stats, WorldDesc, ptr, FP64
+from .cabi import (SpaceParams, Stats, WorldDesc, ptr, FP64, CallbackArbiter, CallbackDecision, ARB_FIRST_COLLISION,
+                   ARB_IGNORE, EVENT_BEGIN, EVENT_PRE_SOLVE)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcpgpu.so")
@@ -24,6 +25,8 @@ _vp = ctypes.c_void_p
 
 EXPORTS = [
     "cpgpuWorldCreate", "cpgpuWorldCreateFrom", "cpgpuWorldDestroy", "cpgpuWorldWriteBodiesAt",
+    "cpgpuWorldStepBegin", "cpgpuWorldReadCallbacks", "cpgpuWorldWriteDecisions", "cpgpuWorldStepEnd",
+    "cpgpuWorldReadSeparations",
     "cpgpuWorldSetSpaceParams", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
     "cpgpuWorldReadConstraints", "cpgpuWorldReadPairs", "cpgpuWorldShapesCollide",
@@ -103,6 +106,11 @@ def lib():
                                                 _ip, _dp]
         L.cpgpuWorldQueryBBAll.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _fp, ctypes.c_int64, _i64p, _ip]
         L.cpgpuWorldQueryShapes.argtypes = [_vp, ctypes.c_int32, _ip, ctypes.c_int64, _i64p, _ip, _ip, _dp]
+        L.cpgpuWorldStepBegin.argtypes = [_vp, ctypes.c_double, _i64p]
+        L.cpgpuWorldReadCallbacks.argtypes = [_vp, ctypes.c_int64, _i64p, ctypes.POINTER(CallbackArbiter)]
+        L.cpgpuWorldWriteDecisions.argtypes = [_vp, ctypes.c_int64, ctypes.POINTER(CallbackDecision)]
+        L.cpgpuWorldStepEnd.argtypes = [_vp]
+        L.cpgpuWorldReadSeparations.argtypes = [_vp, ctypes.c_int64, _i64p, _ip]
         L.cpgpuWorldGetTiming.argtypes = [_vp, ctypes.c_int32, _ip, _dp, _i64p, ctypes.c_char_p, ctypes.c_int32]
         L.cpgpuWorldGetSolveCounts.argtypes = [_vp, _i64p, _i64p]
         L.cpgpuWorldGetBuckets.argtypes = [_vp, _ip, _ip, _ip, _ip]
@@ -165,6 +173,60 @@ class GPUWorld:
 
     def sync(self):
         _check(self.L.cpgpuWorldSync(self.h))
+
+    # ---- collision handler callbacks ----
+
+    def step_begin(self, dt):
+        """The first half of a step: returns the arbiters whose begin or preSolve is due
+        (a ctypes array of CallbackArbiter)."""
+        n = ctypes.c_int64()
+        _check(self.L.cpgpuWorldStepBegin(self.h, dt, ctypes.byref(n)))
+        out = (CallbackArbiter * max(n.value, 1))()
+        m = ctypes.c_int64()
+        if n.value:
+            _check(self.L.cpgpuWorldReadCallbacks(self.h, n.value, ctypes.byref(m), out))
+        return out[:n.value]
+
+    def step_end(self, decisions=()):
+        """Hand back the decisions (CallbackDecision) and finish the step."""
+        if decisions:
+            arr = (CallbackDecision * len(decisions))(*decisions)
+            _check(self.L.cpgpuWorldWriteDecisions(self.h, len(decisions), arr))
+        _check(self.L.cpgpuWorldStepEnd(self.h))
+
+    def step_with_callbacks(self, dt, begin=None, pre_solve=None):
+        """One step, its due arbiters answered as cpSpaceCollideShapes calls a handler
+        (cpSpaceStep.c:255-266): begin on a first collision (false, or an ignore, ignores
+        the pair until it separates), then preSolve unless ignored, which sees what begin
+        set.  begin / pre_solve(view, decision) edit the decision (solve, ignore, e, u,
+        surface_vr) in place; None answers with the defaults."""
+        decisions = []
+        for v in self.step_begin(dt):
+            d = CallbackDecision(slot=v.slot, solve=1, ignore=0, e=v.e, u=v.u)
+            d.surface_vr[0], d.surface_vr[1] = v.surface_vr[0], v.surface_vr[1]
+            ignored = False
+            if v.state == ARB_FIRST_COLLISION and (v.events & EVENT_BEGIN) and begin:
+                begin(v, d)
+                if not d.solve or d.ignore:
+                    d.solve, d.ignore, ignored = 0, 1, True
+            if not ignored and (v.events & EVENT_PRE_SOLVE) and pre_solve:
+                v.e, v.u = d.e, d.u
+                v.surface_vr[0], v.surface_vr[1] = d.surface_vr[0], d.surface_vr[1]
+                d.solve = 1
+                pre_solve(v, d)
+            decisions.append(d)
+        self.step_end(decisions)
+
+    def separations(self):
+        """Pairs whose arbiter separated in the last step, among handler pairs with a
+        separate event: (n, 3) int32 {space, shape_a, shape_b}."""
+        n = ctypes.c_int64()
+        _check(self.L.cpgpuWorldReadSeparations(self.h, 0, ctypes.byref(n), None))
+        out = np.zeros((n.value, 3), np.int32)
+        m = ctypes.c_int64()
+        if n.value:
+            _check(self.L.cpgpuWorldReadSeparations(self.h, n.value, ctypes.byref(m), ptr(out, ctypes.c_int32)))
+        return out
 
     def bodies(self):
         from types import SimpleNamespace

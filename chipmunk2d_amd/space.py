@@ -19,7 +19,8 @@ import numpy as np
 
 from .cabi import (BODY_DYNAMIC, BODY_KINEMATIC, BODY_STATIC, SHAPE_CIRCLE, SHAPE_SEGMENT, SHAPE_POLY,
                    JOINT_PIVOT, JOINT_DAMPED_SPRING, JOINT_SLIDE, JOINT_GEAR, JOINT_PIN, JOINT_GROOVE,
-                   JOINT_ROTARY_LIMIT, JOINT_RATCHET, JOINT_SIMPLE_MOTOR, JOINT_DAMPED_ROTARY_SPRING, PackedWorld)
+                   JOINT_ROTARY_LIMIT, JOINT_RATCHET, JOINT_SIMPLE_MOTOR, JOINT_DAMPED_ROTARY_SPRING, PackedWorld,
+                   HANDLER_DTYPE, WILDCARD)
 
 INF = float("inf")
 CP_PI = 3.14159265358979323846264338327950288
@@ -408,6 +409,7 @@ class Shape:
         self.categories = ALL_CATEGORIES
         self.mask = ALL_CATEGORIES
         self.sensor = False
+        self.collision_type = 0
         self.mass_m = 0.0
         self.mass_i = mass_i
         self.mass_cog = mass_cog
@@ -728,6 +730,21 @@ class Space:
         self.constraints = []
         self.static_body = Body.static()
         self._static_added = False
+        # collision handlers: (type_a, type_b, events); WILDCARD as type_b makes a
+        # wildcard handler of type_a, as both the default handler
+        self.handlers = []
+
+    def add_collision_handler(self, type_a, type_b, events):
+        """cpSpaceAddCollisionHandler (cpSpace.c:382-399) with the CPGPU_EVENT_* callbacks it has."""
+        self.handlers.append((int(type_a), int(type_b), int(events)))
+
+    def add_wildcard_handler(self, type_a, events):
+        """cpSpaceAddWildcardHandler (cpSpace.c:401-413)."""
+        self.handlers.append((int(type_a), WILDCARD, int(events)))
+
+    def add_default_handler(self, events):
+        """cpSpaceAddDefaultCollisionHandler (cpSpace.c:373-380)."""
+        self.handlers.append((WILDCARD, WILDCARD, int(events)))
 
     def add(self, obj):
         """cpSpaceAddBody / AddShape / AddConstraint (cpSpace.c:417-474)."""
@@ -787,6 +804,8 @@ class Space:
         a["shape_categories"] = np.array([s.categories for s in self.shapes], dtype=np.uint32).reshape(ns)
         a["shape_mask"] = np.array([s.mask for s in self.shapes], dtype=np.uint32).reshape(ns)
         a["shape_sensor"] = np.array([1 if s.sensor else 0 for s in self.shapes], dtype=np.uint8).reshape(ns)
+        a["shape_ctype"] = np.array([s.collision_type for s in self.shapes], dtype=np.uint32).reshape(ns)
+        a["handlers"] = np.array([(-1, ta, tb, ev) for ta, tb, ev in self.handlers], dtype=HANDLER_DTYPE)
         a["shape_radius"] = np.array([s.r for s in self.shapes], dtype=np.float64).reshape(ns)
         geom = np.zeros((ns, 10), dtype=np.float64)
         pstart = np.zeros(ns, dtype=np.int32)

@@ -164,7 +164,40 @@ typedef struct cpgpuWorldDesc {
 	 * infinite-mass pairs before the narrowphase (and are not candidate pairs);
 	 * the nearest-point and first-segment queries skip sensors (cpSpaceQuery.c:64,145). */
 	const uint8_t *shape_sensor;
+
+	/* Collision handlers (cpSpaceAddCollisionHandler, cpSpaceAddWildcardHandler,
+	 * cpSpaceAddDefaultCollisionHandler; src/cpSpace.c:382-413).  shape_ctype
+	 * [n_shapes]: each shape's collision type (cpShape.type, as 32 bits; NULL:
+	 * all 0).  handlers [n_handlers]: which callbacks exist for which types; a
+	 * pair of shapes takes the events of every entry that matches it (see
+	 * cpgpuHandlerDesc).  A pair with events goes through the narrowphase even
+	 * when it is never solved (a sensor, two infinite masses), as in
+	 * cpSpaceCollideShapes; its begin and preSolve are answered by the host
+	 * between cpgpuWorldStepBegin and cpgpuWorldStepEnd, its postSolve and
+	 * separate after the step. */
+	const uint32_t *shape_ctype;
+	int32_t n_handlers;
+	const struct cpgpuHandlerDesc *handlers;
 } cpgpuWorldDesc;
+
+/* collision handler events */
+#define CPGPU_EVENT_BEGIN 1
+#define CPGPU_EVENT_PRE_SOLVE 2
+#define CPGPU_EVENT_POST_SOLVE 4
+#define CPGPU_EVENT_SEPARATE 8
+/* the wildcard collision type (CP_WILDCARD_COLLISION_TYPE) */
+#define CPGPU_WILDCARD 0xFFFFFFFFu
+
+/* One handler: the space it belongs to (-1: every space of the description)
+ * and its types.  {a, b} matches the pairs of types a and b in either order;
+ * {a, CPGPU_WILDCARD} every pair with a shape of type a (a wildcard handler);
+ * {CPGPU_WILDCARD, CPGPU_WILDCARD} every pair (a default handler).  events:
+ * the CPGPU_EVENT_* callbacks it has that are not the do-nothing defaults. */
+typedef struct cpgpuHandlerDesc {
+	int32_t space;
+	uint32_t type_a, type_b;
+	uint32_t events;
+} cpgpuHandlerDesc;
 
 #define CPGPU_MAX_POLY_VERTS 64
 
@@ -214,6 +247,57 @@ int cpgpuWorldCreateFrom(const cpgpuWorldDesc *desc, int32_t precision, cpgpuWor
 int cpgpuWorldStep(cpgpuWorld *world, double dt);
 /* Advance every world by dt, n_steps times. */
 int cpgpuWorldStepN(cpgpuWorld *world, double dt, int32_t n_steps);
+
+/* ---- collision handler callbacks: a step in two halves around the host ----
+ *
+ * cpgpuWorldStepBegin runs a step up to the point where cpSpaceCollideShapes
+ * (cpSpaceStep.c:253-284) would call begin and preSolve: positions, shapes,
+ * broadphase, narrowphase and cpArbiterUpdate.  *n_callbacks gets the number
+ * of touching arbiters whose handler has a begin (due on a first collision)
+ * or a preSolve; cpgpuWorldReadCallbacks copies them out, the host runs the
+ * callbacks, cpgpuWorldWriteDecisions hands back what they decided (an
+ * arbiter left out keeps the defaults: solved, values unchanged), and
+ * cpgpuWorldStepEnd finishes the step (cpSpaceStep.c:377-445).  An arbiter
+ * that is not solved -- rejected by preSolve, ignored, a sensor's, or between
+ * two infinite masses -- keeps no contacts (its count is 0, as
+ * cpSpaceCollideShapes leaves it), is not among the step's arbiters, and an
+ * ignored one stays ignored, with no begin and no preSolve, until its shapes
+ * separate.  cpgpuWorldStep is the two halves with the default decisions.
+ * After the step, postSolve is due for the solved arbiters of handler pairs
+ * with that event (cpgpuWorldReadArbiters gives their impulses), separate for
+ * the pairs cpgpuWorldReadSeparations lists. */
+typedef struct cpgpuCallbackArbiter {
+	int32_t slot;             /* the arbiter's handle for its decision */
+	int32_t space;
+	int32_t shape_a, shape_b; /* in the arbiter's order (cpCollide's a and b) */
+	int32_t state;            /* CPGPU_ARB_FIRST_COLLISION (begin is due) or CPGPU_ARB_NORMAL */
+	int32_t events;           /* the pair's CPGPU_EVENT_* */
+	int32_t count;
+	int32_t _pad;
+	double normal[2];
+	double e, u, surface_vr[2];
+	/* per contact: r1.x, r1.y, r2.x, r2.y (offsets from the bodies' positions),
+	 * the warm-started jnAcc, jtAcc */
+	double contact[2][6];
+	uint64_t hash[2];
+} cpgpuCallbackArbiter;
+
+typedef struct cpgpuCallbackDecision {
+	int32_t slot;
+	int32_t solve;   /* 0: not this step (begin or preSolve returned false) */
+	int32_t ignore;  /* 1: cpArbiterIgnore was called (not solved until the shapes separate) */
+	int32_t _pad;
+	double e, u, surface_vr[2];   /* the arbiter's values for the solve (cpArbiterSet*) */
+} cpgpuCallbackDecision;
+
+int cpgpuWorldStepBegin(cpgpuWorld *world, double dt, int64_t *n_callbacks);
+int cpgpuWorldReadCallbacks(cpgpuWorld *world, int64_t cap, int64_t *n_out, cpgpuCallbackArbiter *out);
+int cpgpuWorldWriteDecisions(cpgpuWorld *world, int64_t n, const cpgpuCallbackDecision *decisions);
+int cpgpuWorldStepEnd(cpgpuWorld *world);
+/* The pairs whose arbiter separated in the last step (cpSpaceArbiterSetFilter,
+ * cpSpaceStep.c:306-310: touched the step before, not in this one) and whose
+ * handler has a separate event: pairs [cap][3] {space, shape_a, shape_b}. */
+int cpgpuWorldReadSeparations(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs);
 /* Wait for queued steps and check the device error word. */
 int cpgpuWorldSync(cpgpuWorld *world);
 

@@ -185,6 +185,7 @@ typedef struct {
 	uint64_t group;
 	uint32_t cats, mask;
 	int sensor;                              /* cpShape.sensor */
+	uint32_t ctype;                          /* cpShape.type (collision type) */
 	num r;
 	V c, tc;                                 /* circle */
 	V a, b, n, ta, tb, tn, a_tan, b_tan;     /* segment */
@@ -212,6 +213,7 @@ typedef struct {
 	uint32_t id, stamp;
 	int state, colour;
 	int unit;   /* solve unit of a large space (grid_units) */
+	uint32_t events;   /* the pair's collision handler events (CPGPU_EVENT_*) */
 } OArb;
 
 typedef struct {
@@ -266,6 +268,13 @@ struct oworld {
 	int joint_list_order;
 	int64_t steps;
 	int grid_parts;     /* strips of every large space's grid set (grid_parts_of) */
+	/* collision handlers: the description's table, the host's answers */
+	cpgpuHandlerDesc *handlers;
+	int n_handlers;
+	ow_handler_fn hfn;
+	void *hctx;
+	int32_t *seps;      /* the last step's separations {space, a, b} */
+	int64_t nseps, sepcap;
 };
 
 static int grid_parts_of(const oworld *w);
@@ -351,6 +360,7 @@ oworld *ow_new(const cpgpuWorldDesc *d)
 		s->sv = v2(d->shape_surface_v[2 * i], d->shape_surface_v[2 * i + 1]);
 		s->group = d->shape_group[i]; s->cats = d->shape_categories[i]; s->mask = d->shape_mask[i];
 		s->sensor = d->shape_sensor ? d->shape_sensor[i] != 0 : 0;
+		s->ctype = d->shape_ctype ? d->shape_ctype[i] : 0u;
 		s->r = d->shape_radius[i];
 		if (s->type == CPGPU_SHAPE_CIRCLE) {
 			s->c = v2(g[0], g[1]);
@@ -401,7 +411,54 @@ oworld *ow_new(const cpgpuWorldDesc *d)
 		for (int c = P->c0; c < P->c1; c++) P->jorder[c - P->c0] = c;
 	}
 	w->grid_parts = grid_parts_of(w);
+	if (d->n_handlers > 0 && d->handlers) {
+		w->n_handlers = d->n_handlers;
+		w->handlers = (cpgpuHandlerDesc *)malloc((size_t)d->n_handlers * sizeof(cpgpuHandlerDesc));
+		memcpy(w->handlers, d->handlers, (size_t)d->n_handlers * sizeof(cpgpuHandlerDesc));
+	}
 	return w;
+}
+
+void ow_set_handler_fn(oworld *w, ow_handler_fn fn, void *ctx)
+{
+	w->hfn = fn;
+	w->hctx = ctx;
+}
+
+/* The events of the handlers that match a pair of shapes: the pair's own
+ * handler and the wildcard handlers of either type (cpSpaceLookupHandler,
+ * cpSpace.c:382-399; cpArbiterUpdate, cpArbiter.c:402-409; the default
+ * handlers call the wildcard ones, cpSpace.c:63-103). */
+static uint32_t pair_events(const oworld *w, const OShape *a, const OShape *b)
+{
+	uint32_t ev = 0;
+	const uint32_t ta = a->ctype, tb = b->ctype;
+	for (int k = 0; k < w->n_handlers; k++) {
+		const cpgpuHandlerDesc *h = &w->handlers[k];
+		if (h->space >= 0 && h->space != a->world) continue;
+		int match;
+		if (h->type_b == CPGPU_WILDCARD) match = h->type_a == CPGPU_WILDCARD || h->type_a == ta || h->type_a == tb;
+		else match = (h->type_a == ta && h->type_b == tb) || (h->type_a == tb && h->type_b == ta);
+		if (match) ev |= h->events;
+	}
+	return ev;
+}
+
+static void sep_push(oworld *w, int32_t space, int32_t a, int32_t b)
+{
+	if (w->nseps == w->sepcap) {
+		w->sepcap = w->sepcap ? 2 * w->sepcap : 64;
+		w->seps = (int32_t *)realloc(w->seps, (size_t)w->sepcap * 3 * sizeof(int32_t));
+	}
+	int32_t *e = w->seps + 3 * w->nseps++;
+	e[0] = space; e[1] = a; e[2] = b;
+}
+
+int64_t ow_read_separations(oworld *w, int64_t cap, int32_t *pairs)
+{
+	for (int64_t k = 0; k < w->nseps && k < cap; k++)
+		for (int q = 0; q < 3; q++) pairs[3 * k + q] = w->seps[3 * k + q];
+	return w->nseps;
 }
 
 void ow_free(oworld *w)
@@ -414,6 +471,7 @@ void ow_free(oworld *w)
 	}
 	free(w->W); free(w->B); free(w->S); free(w->J); free(w->polymem); free(w->next_pairs);
 	free(w->coll); free(w->coll_key);
+	free(w->handlers); free(w->seps);
 	free(w);
 }
 
@@ -1006,9 +1064,35 @@ static int pair_ok(oworld *w, const OWorldPart *P, int i, int j)
 	if (a->body == b->body) return 0;
 	if (filter_reject(a, b)) return 0;
 	if (nocollide_has(P, a->body, b->body)) return 0;
-	if (ba->m == INFINITY && bb->m == INFINITY) return 0;
-	if (a->sensor || b->sensor) return 0;
+	/* a pair that is never solved (cpSpaceStep.c:268-272) matters only to handlers
+	 * that see it: begin, preSolve or separate */
+	if ((ba->m == INFINITY && bb->m == INFINITY) || a->sensor || b->sensor)
+		return w->n_handlers > 0 &&
+		       (pair_events(w, a, b) & (CPGPU_EVENT_BEGIN | CPGPU_EVENT_PRE_SOLVE | CPGPU_EVENT_SEPARATE)) != 0;
 	return 1;
+}
+
+/* the arbiter as a handler callback sees it */
+static void cb_view(const oworld *w, int world, const OArb *arb, cpgpuCallbackArbiter *v)
+{
+	memset(v, 0, sizeof(*v));
+	v->slot = -1;
+	v->space = world;
+	v->shape_a = arb->sa; v->shape_b = arb->sb;
+	v->state = arb->state;
+	v->events = (int32_t)arb->events;
+	v->count = arb->count;
+	v->normal[0] = NUM_VAL(arb->n.x); v->normal[1] = NUM_VAL(arb->n.y);
+	v->e = NUM_VAL(arb->e); v->u = NUM_VAL(arb->u);
+	v->surface_vr[0] = NUM_VAL(arb->svr.x); v->surface_vr[1] = NUM_VAL(arb->svr.y);
+	for (int c = 0; c < arb->count; c++) {
+		const OContact *con = &arb->con[c];
+		double *o = v->contact[c];
+		o[0] = NUM_VAL(con->r1.x); o[1] = NUM_VAL(con->r1.y); o[2] = NUM_VAL(con->r2.x); o[3] = NUM_VAL(con->r2.y);
+		o[4] = NUM_VAL(con->jnAcc); o[5] = NUM_VAL(con->jtAcc);
+		v->hash[c] = con->hash;
+	}
+	(void)w;
 }
 
 static const num *g_sort_bbs;
@@ -2112,7 +2196,6 @@ static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, i
 		collide(w, ia, ib, id, &info);
 		pid_put(P, key, info.id);   /* cpSpaceCollideShapes returns the new id to the tree (:245) */
 		if (info.count == 0) continue;
-		P->ncontacts += info.count;
 		if (ai < 0) {
 			/* cpArbiterInit (cpArbiter.c:316-350) */
 			ai = arb_append(P);
@@ -2151,8 +2234,44 @@ static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, i
 		V surface_vr = vsub(sb->sv, sa->sv);
 		arb->svr = vsub(surface_vr, vmult(info.n, vdot(surface_vr, info.n)));
 		if (arb->state == CPGPU_ARB_CACHED) arb->state = CPGPU_ARB_FIRST_COLLISION;
-		/* :280 admit; :287 stamp */
-		P->active[P->nactive++] = ai;
+		/* :255-284 begin (on a first collision; false ignores the pair until it
+		 * separates), preSolve, then admission: not ignored, accepted, no sensor,
+		 * not two infinite masses */
+		arb->events = w->n_handlers > 0 ? pair_events(w, sa, sb) : 0u;
+		int solve = 1;
+		const int due_begin = arb->state == CPGPU_ARB_FIRST_COLLISION && (arb->events & CPGPU_EVENT_BEGIN);
+		const int due_pre = arb->state != CPGPU_ARB_IGNORE && (arb->events & CPGPU_EVENT_PRE_SOLVE);
+		if (arb->state == CPGPU_ARB_IGNORE) {
+			solve = 0;
+		} else if (w->hfn && (due_begin || due_pre)) {
+			cpgpuCallbackArbiter v;
+			cpgpuCallbackDecision dec = {-1, 1, 0, 0, NUM_VAL(arb->e), NUM_VAL(arb->u),
+			                             {NUM_VAL(arb->svr.x), NUM_VAL(arb->svr.y)}};
+			if (due_begin) {
+				cb_view(w, (int)(P - w->W), arb, &v);
+				w->hfn(w->hctx, CPGPU_EVENT_BEGIN, &v, &dec);
+				if (!dec.solve || dec.ignore) arb->state = CPGPU_ARB_IGNORE;
+			}
+			if (arb->state != CPGPU_ARB_IGNORE && (arb->events & CPGPU_EVENT_PRE_SOLVE)) {
+				arb->e = dec.e; arb->u = dec.u; arb->svr.x = dec.surface_vr[0]; arb->svr.y = dec.surface_vr[1];
+				cb_view(w, (int)(P - w->W), arb, &v);
+				dec.solve = 1;
+				w->hfn(w->hctx, CPGPU_EVENT_PRE_SOLVE, &v, &dec);
+				if (dec.ignore) arb->state = CPGPU_ARB_IGNORE;
+			}
+			arb->e = dec.e; arb->u = dec.u; arb->svr.x = dec.surface_vr[0]; arb->svr.y = dec.surface_vr[1];
+			if (!dec.solve || arb->state == CPGPU_ARB_IGNORE) solve = 0;
+		}
+		if (sa->sensor || sb->sensor || (w->B[sa->body].m == INFINITY && w->B[sb->body].m == INFINITY)) solve = 0;
+		if (solve) {
+			P->active[P->nactive++] = ai;
+			P->ncontacts += info.count;
+		} else {
+			/* :276-282 the contacts are dropped; a pair not ignored counts as normal */
+			arb->count = 0;
+			if (arb->state != CPGPU_ARB_IGNORE) arb->state = CPGPU_ARB_NORMAL;
+		}
+		/* :287 stamp */
 		arb->stamp = sp->stamp;
 		/* keep the lookup index current for arbiters created this step */
 		if (P->narb * 2 + 16 > P->tabmask + 1) tab_rebuild(P);
@@ -2176,7 +2295,20 @@ static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, i
 			if (a->type == CPGPU_BODY_STATIC && b->type == CPGPU_BODY_STATIC) {
 				evict = 0;
 			} else {
-				if (ticks >= 1 && arb->state != CPGPU_ARB_CACHED) arb->state = CPGPU_ARB_CACHED;
+				if (ticks >= 1 && arb->state != CPGPU_ARB_CACHED) {
+					/* :306-310 separate */
+					arb->state = CPGPU_ARB_CACHED;
+					if (arb->events & CPGPU_EVENT_SEPARATE) {
+						sep_push(w, (int32_t)(P - w->W), arb->sa, arb->sb);
+						if (w->hfn) {
+							cpgpuCallbackArbiter v;
+							cpgpuCallbackDecision dec;
+							memset(&dec, 0, sizeof(dec));
+							cb_view(w, (int)(P - w->W), arb, &v);
+							w->hfn(w->hctx, CPGPU_EVENT_SEPARATE, &v, &dec);
+						}
+					}
+				}
 				if (ticks >= sp->collision_persistence) evict = 1;
 			}
 			if (evict) { remap[i] = -1; continue; }
@@ -2296,11 +2428,24 @@ static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, i
 		for (int64_t k = 0; k < P->nactive; k++) arb_apply_impulse(w, &P->arbs[P->active[k]]);
 		for (int64_t k = 0; k < nj; k++) joint_apply_impulse(w, &w->J[P->jorder[k]], dt);
 	}
+
+	/* :437-442 postSolve of the solved arbiters */
+	if (w->hfn)
+		for (int64_t k = 0; k < P->nactive; k++) {
+			const OArb *arb = &P->arbs[P->active[k]];
+			if (!(arb->events & CPGPU_EVENT_POST_SOLVE)) continue;
+			cpgpuCallbackArbiter v;
+			cpgpuCallbackDecision dec;
+			memset(&dec, 0, sizeof(dec));
+			cb_view(w, (int)(P - w->W), arb, &v);
+			w->hfn(w->hctx, CPGPU_EVENT_POST_SOLVE, &v, &dec);
+		}
 }
 
 void ow_step(oworld *w, double dt)
 {
 	if (dt == 0.0) return;   /* cpSpaceStep.c:339 */
+	w->nseps = 0;
 	for (int wi = 0; wi < w->nw; wi++) {
 		step_world(w, &w->W[wi], dt, w->next_n > 0 ? w->next_pairs : NULL, w->next_n > 0 ? w->next_n : 0);
 	}

@@ -19,11 +19,25 @@
 
 typedef struct oworld oworld;
 
+/* Collision handler callbacks (cpCollisionHandler's begin, preSolve,
+ * postSolve, separate): event is one CPGPU_EVENT_*.  For begin and preSolve
+ * the function reads the arbiter and writes its decision -- dec arrives with
+ * solve = 1, ignore = 0 and the arbiter's e, u and surface velocity; dec->solve
+ * is the callback's return value.  For postSolve the arbiter carries the
+ * solved impulses; for separate only the shapes and the space matter and dec
+ * is unused. */
+typedef void (*ow_handler_fn)(void *ctx, int32_t event, const cpgpuCallbackArbiter *arb, cpgpuCallbackDecision *dec);
+
 oworld *ow_new(const cpgpuWorldDesc *desc);
 void ow_free(oworld *w);
 
 /* One cpSpaceStep over every world. */
 void ow_step(oworld *w, double dt);
+/* The function that answers the description's collision handlers (NULL: the
+ * default answers -- every arbiter solved, values unchanged). */
+void ow_set_handler_fn(oworld *w, ow_handler_fn fn, void *ctx);
+/* The last step's separations of pairs with a separate event: {space, a, b} [cap][3]. */
+int64_t ow_read_separations(oworld *w, int64_t cap, int32_t *pairs);
 
 /* 1: body rotations use libm cos/sin (cpBody.c:350 via cpVect.h:104-107) and
  * the damped spring libm exp (cpDampedSpring.c:47) exactly as the reference

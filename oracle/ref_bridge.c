@@ -30,6 +30,10 @@ typedef struct rbWorld {
 	cpShape **shapes;
 	cpConstraint **cons;
 	int *ctype;
+	int wi;
+	/* collision handler callbacks answered by the caller (the oracle's ow_handler_fn form) */
+	void (*hfn)(void *ctx, int32_t event, const cpgpuCallbackArbiter *arb, cpgpuCallbackDecision *dec);
+	void *hctx;
 } rbWorld;
 
 static cpTransform xf(const double *t)
@@ -65,9 +69,103 @@ rbWorld *rb_new_in(cpSpace *space, const cpgpuWorldDesc *d, int wi)
 	return fill_space(w, d, wi);
 }
 
+/* ---- collision handlers: each callback hands the arbiter to w->hfn ---- */
+
+static void rb_view(rbWorld *w, cpArbiter *arb, cpgpuCallbackArbiter *v)
+{
+	memset(v, 0, sizeof(*v));
+	v->slot = -1;
+	v->space = w->wi;
+	v->shape_a = (int32_t)(intptr_t)arb->a->userData;
+	v->shape_b = (int32_t)(intptr_t)arb->b->userData;
+	v->state = arb->state;
+	v->count = arb->count;
+	v->normal[0] = arb->n.x; v->normal[1] = arb->n.y;
+	v->e = arb->e; v->u = arb->u;
+	v->surface_vr[0] = arb->surface_vr.x; v->surface_vr[1] = arb->surface_vr.y;
+	for (int c = 0; c < arb->count; c++) {
+		const struct cpContact *con = &arb->contacts[c];
+		double *o = v->contact[c];
+		o[0] = con->r1.x; o[1] = con->r1.y; o[2] = con->r2.x; o[3] = con->r2.y;
+		o[4] = con->jnAcc; o[5] = con->jtAcc;
+		v->hash[c] = (uint64_t)con->hash;
+	}
+}
+
+static cpBool rb_decide(cpArbiter *arb, int32_t event, rbWorld *w)
+{
+	if (!w->hfn) return cpTrue;
+	cpgpuCallbackArbiter v;
+	rb_view(w, arb, &v);
+	cpgpuCallbackDecision dec = {-1, 1, 0, 0, arb->e, arb->u, {arb->surface_vr.x, arb->surface_vr.y}};
+	w->hfn(w->hctx, event, &v, &dec);
+	cpArbiterSetRestitution(arb, dec.e);
+	cpArbiterSetFriction(arb, dec.u);
+	cpArbiterSetSurfaceVelocity(arb, cpv(dec.surface_vr[0], dec.surface_vr[1]));
+	if (dec.ignore) cpArbiterIgnore(arb);
+	return dec.solve ? cpTrue : cpFalse;
+}
+
+static cpBool rb_begin(cpArbiter *arb, cpSpace *space, cpDataPointer data)
+{
+	(void)space;
+	return rb_decide(arb, CPGPU_EVENT_BEGIN, (rbWorld *)data);
+}
+static cpBool rb_pre_solve(cpArbiter *arb, cpSpace *space, cpDataPointer data)
+{
+	(void)space;
+	return rb_decide(arb, CPGPU_EVENT_PRE_SOLVE, (rbWorld *)data);
+}
+static void rb_notify(cpArbiter *arb, int32_t event, rbWorld *w)
+{
+	if (!w->hfn) return;
+	cpgpuCallbackArbiter v;
+	cpgpuCallbackDecision dec;
+	memset(&dec, 0, sizeof(dec));
+	rb_view(w, arb, &v);
+	w->hfn(w->hctx, event, &v, &dec);
+}
+static void rb_post_solve(cpArbiter *arb, cpSpace *space, cpDataPointer data)
+{
+	(void)space;
+	rb_notify(arb, CPGPU_EVENT_POST_SOLVE, (rbWorld *)data);
+}
+static void rb_separate(cpArbiter *arb, cpSpace *space, cpDataPointer data)
+{
+	(void)space;
+	rb_notify(arb, CPGPU_EVENT_SEPARATE, (rbWorld *)data);
+}
+
+/* the description's handlers of space wi, through cpSpaceAddCollisionHandler,
+ * cpSpaceAddWildcardHandler and cpSpaceAddDefaultCollisionHandler */
+static void rb_add_handlers(rbWorld *w, const cpgpuWorldDesc *d, int wi)
+{
+	for (int k = 0; k < d->n_handlers; k++) {
+		const cpgpuHandlerDesc *h = &d->handlers[k];
+		if (h->space >= 0 && h->space != wi) continue;
+		cpCollisionHandler *ch;
+		if (h->type_b == CPGPU_WILDCARD && h->type_a == CPGPU_WILDCARD) ch = cpSpaceAddDefaultCollisionHandler(w->space);
+		else if (h->type_b == CPGPU_WILDCARD) ch = cpSpaceAddWildcardHandler(w->space, (cpCollisionType)h->type_a);
+		else ch = cpSpaceAddCollisionHandler(w->space, (cpCollisionType)h->type_a, (cpCollisionType)h->type_b);
+		ch->userData = w;
+		if (h->events & CPGPU_EVENT_BEGIN) ch->beginFunc = rb_begin;
+		if (h->events & CPGPU_EVENT_PRE_SOLVE) ch->preSolveFunc = rb_pre_solve;
+		if (h->events & CPGPU_EVENT_POST_SOLVE) ch->postSolveFunc = rb_post_solve;
+		if (h->events & CPGPU_EVENT_SEPARATE) ch->separateFunc = rb_separate;
+	}
+}
+
+void rb_set_handler_fn(rbWorld *w, void (*fn)(void *, int32_t, const cpgpuCallbackArbiter *, cpgpuCallbackDecision *),
+                       void *ctx)
+{
+	w->hfn = fn;
+	w->hctx = ctx;
+}
+
 static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi)
 {
 	const cpgpuSpaceParams *sp = &d->space[wi];
+	w->wi = wi;
 	cpSpace *space = w->space;
 	cpSpaceSetIterations(space, sp->iterations);
 	cpSpaceSetGravity(space, cpv(sp->gravity_x, sp->gravity_y));
@@ -139,6 +237,7 @@ static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi)
 		cpShapeSetSurfaceVelocity(s, cpv(d->shape_surface_v[2 * i], d->shape_surface_v[2 * i + 1]));
 		cpShapeSetFilter(s, cpShapeFilterNew((cpGroup)d->shape_group[i], d->shape_categories[i], d->shape_mask[i]));
 		if (d->shape_sensor && d->shape_sensor[i]) cpShapeSetSensor(s, cpTrue);
+		if (d->shape_ctype) cpShapeSetCollisionType(s, (cpCollisionType)d->shape_ctype[i]);
 		s->userData = (cpDataPointer)(intptr_t)i;
 		cpSpaceAddShape(space, s);
 		if (s->hashid != d->shape_hashid[i]) {
@@ -206,6 +305,7 @@ static rbWorld *fill_space(rbWorld *w, const cpgpuWorldDesc *d, int wi)
 		w->cons[k] = c;
 		w->ctype[k] = t;
 	}
+	if (d->n_handlers > 0 && d->handlers) rb_add_handlers(w, d, wi);
 	return w;
 }
 

@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from chipmunk2d_amd.cabi import PackedWorld, SpaceParams, Stats, WorldDesc, ptr
+from chipmunk2d_amd.cabi import HANDLER_FN, PackedWorld, SpaceParams, Stats, WorldDesc, ptr
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -71,6 +71,9 @@ def oracle_lib():
         lib.ow_segment_query.argtypes = [_vp, ctypes.c_int32, _dp, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
         lib.ow_bb_query.restype = ctypes.c_int64
         lib.ow_bb_query.argtypes = [_vp, ctypes.c_int32, _dp] + _q + [ctypes.c_int64, _ip]
+        lib.ow_set_handler_fn.argtypes = [_vp, HANDLER_FN, _vp]
+        lib.ow_read_separations.restype = ctypes.c_int64
+        lib.ow_read_separations.argtypes = [_vp, ctypes.c_int64, _ip]
         lib.ow_shape_query.restype = ctypes.c_int64
         lib.ow_shape_query.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp]
         lib.ow_set_space_params.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(SpaceParams)]
@@ -124,6 +127,7 @@ def ref_lib(release=False):
         lib.rb_segment_query.argtypes = [_vp, _dp, _dp, _d] + _q + [ctypes.c_int64, _ip, _dp]
         lib.rb_bb_query.restype = ctypes.c_int64
         lib.rb_bb_query.argtypes = [_vp, _dp] + _q + [ctypes.c_int64, _ip]
+        lib.rb_set_handler_fn.argtypes = [_vp, HANDLER_FN, _vp]
         lib.rb_shape_query.restype = ctypes.c_int64
         lib.rb_shape_query.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp, _ip]
         lib.rb_new_in.restype = _vp
@@ -205,6 +209,20 @@ class OracleWorld:
     def step(self, dt, n=1):
         for _ in range(n):
             self.lib.ow_step(self.h, dt)
+
+    def set_handler_fn(self, fn):
+        """Answer the description's collision handlers with fn(event, view, decision)
+        (None: the defaults)."""
+        self._hfn = HANDLER_FN(lambda ctx, ev, v, d: fn(ev, v.contents, d.contents)) if fn else HANDLER_FN()
+        self.lib.ow_set_handler_fn(self.h, self._hfn, None)
+
+    def separations(self):
+        """The last step's separations of pairs with a separate event: (n, 3) {space, a, b}."""
+        n = self.lib.ow_read_separations(self.h, 0, None)
+        out = np.zeros((n, 3), np.int32)
+        if n:
+            self.lib.ow_read_separations(self.h, n, ptr(out, ctypes.c_int32))
+        return out
 
     def libm_trig(self, on=True):
         self.lib.ow_set_libm_trig(self.h, 1 if on else 0)
@@ -347,6 +365,12 @@ class RefWorld:
         if getattr(self, "h", None):
             self.lib.rb_free(self.h)
             self.h = None
+
+    def set_handler_fn(self, fn):
+        """Answer the collision handlers installed from the description with
+        fn(event, view, decision) (None: the defaults)."""
+        self._hfn = HANDLER_FN(lambda ctx, ev, v, d: fn(ev, v.contents, d.contents)) if fn else HANDLER_FN()
+        self.lib.rb_set_handler_fn(self.h, self._hfn, None)
 
     def step(self, dt, n=1):
         self.lib.rb_step(self.h, dt, n)

@@ -149,10 +149,17 @@ typedef struct Desc {
 	uint64_t *shape_hashid, *shape_group;
 	uint32_t *shape_cats, *shape_mask;
 	uint8_t *shape_sensor;
+	uint32_t *shape_ctype;               /* collision type, as an index into types */
 	double *shape_e, *shape_u, *shape_sv, *shape_r, *shape_geom;
 	double *pverts, *pnorms;
 	int32_t *cons_type, *cons_a, *cons_b, *cons_collide;
 	double *cons_max_force, *cons_error_bias, *cons_max_bias, *cons_params, *cons_jacc;
+	/* collision handlers: the ones that are not Chipmunk's do-nothing defaults, with
+	 * their types as indices into types */
+	cpgpuHandlerDesc *handlers;
+	int nh, cap_h;
+	cpCollisionType *types;
+	int ntypes, cap_t;
 	cpgpuWorldDesc d;
 } Desc;
 
@@ -191,6 +198,7 @@ static void desc_reserve(Desc *D, int nw, int nb, int ns, int nc, int npv)
 		D->shape_cats = realloc(D->shape_cats, (size_t)c * sizeof(uint32_t));
 		D->shape_mask = realloc(D->shape_mask, (size_t)c * sizeof(uint32_t));
 		D->shape_sensor = realloc(D->shape_sensor, (size_t)c * sizeof(uint8_t));
+		D->shape_ctype = realloc(D->shape_ctype, (size_t)c * sizeof(uint32_t));
 		D->shape_e = realloc(D->shape_e, (size_t)c * sizeof(double));
 		D->shape_u = realloc(D->shape_u, (size_t)c * sizeof(double));
 		D->shape_sv = realloc(D->shape_sv, (size_t)c * 2 * sizeof(double));
@@ -228,7 +236,8 @@ static void desc_free(Desc *D)
 	             D->shape_hashid, D->shape_group, D->shape_cats, D->shape_mask, D->shape_sensor, D->shape_e, D->shape_u,
 	             D->shape_sv,
 	             D->shape_r, D->shape_geom, D->pverts, D->pnorms, D->cons_type, D->cons_a, D->cons_b, D->cons_collide,
-	             D->cons_max_force, D->cons_error_bias, D->cons_max_bias, D->cons_params, D->cons_jacc};
+	             D->cons_max_force, D->cons_error_bias, D->cons_max_bias, D->cons_params, D->cons_jacc,
+	             D->shape_ctype, D->handlers, D->types};
 	for (size_t k = 0; k < sizeof(p) / sizeof(p[0]); k++) free(p[k]);
 	for (int f = 0; f < BODY_FIELDS; f++) free(D->state[f]);
 	memset(D, 0, sizeof(*D));
@@ -257,6 +266,9 @@ static void desc_link(Desc *D)
 	d->shape_e = D->shape_e; d->shape_u = D->shape_u; d->shape_surface_v = D->shape_sv;
 	d->shape_group = D->shape_group; d->shape_categories = D->shape_cats; d->shape_mask = D->shape_mask;
 	d->shape_sensor = D->shape_sensor;
+	d->shape_ctype = D->nh > 0 ? D->shape_ctype : NULL;
+	d->n_handlers = D->nh;
+	d->handlers = D->nh > 0 ? D->handlers : NULL;
 	d->shape_radius = D->shape_r; d->shape_geom = D->shape_geom;
 	d->shape_poly_start = D->shape_pstart; d->shape_poly_count = D->shape_pcount;
 	d->n_poly_verts = D->npv;
@@ -271,13 +283,13 @@ static void desc_link(Desc *D)
 
 /* ---- reading one space ---- */
 
-/* the default handler's callbacks and the damped spring's default force, taken
- * from freshly made objects (their functions are private to cpSpace.c and
- * cpDampedSpring.c) */
-static cpCollisionBeginFunc default_begin;
-static cpCollisionPreSolveFunc default_pre_solve;
-static cpCollisionPostSolveFunc default_post_solve;
-static cpCollisionSeparateFunc default_separate;
+/* the damped springs' default force and torque, taken from freshly made objects
+ * (their functions are private to cpDampedSpring.c and cpDampedRotarySpring.c) */
+/* the defaults that only call the wildcard handlers: DefaultBegin ... DefaultSeparate (cpSpace.c:66-90) */
+static cpCollisionBeginFunc wild_begin;
+static cpCollisionPreSolveFunc wild_pre_solve;
+static cpCollisionPostSolveFunc wild_post_solve;
+static cpCollisionSeparateFunc wild_separate;
 static cpDampedSpringForceFunc default_spring_force;
 static cpDampedRotarySpringTorqueFunc default_spring_torque;
 static int defaults_known;
@@ -286,10 +298,11 @@ static void learn_defaults(void)
 {
 	if (defaults_known) return;
 	cpSpace *s = cpSpaceNew();
-	default_begin = s->defaultHandler.beginFunc;
-	default_pre_solve = s->defaultHandler.preSolveFunc;
-	default_post_solve = s->defaultHandler.postSolveFunc;
-	default_separate = s->defaultHandler.separateFunc;
+	const cpCollisionHandler *h = cpSpaceAddCollisionHandler(s, 1, 2);
+	wild_begin = h->beginFunc;
+	wild_pre_solve = h->preSolveFunc;
+	wild_post_solve = h->postSolveFunc;
+	wild_separate = h->separateFunc;
 	cpSpaceFree(s);
 	cpConstraint *c = cpDampedSpringNew(NULL, NULL, cpvzero, cpvzero, 0.0, 0.0, 0.0);
 	default_spring_force = ((cpDampedSpring *)c)->springForceFunc;
@@ -300,11 +313,6 @@ static void learn_defaults(void)
 	defaults_known = 1;
 }
 
-static int handler_is_default(const cpCollisionHandler *h)
-{
-	return h->beginFunc == default_begin && h->preSolveFunc == default_pre_solve && h->postSolveFunc == default_post_solve &&
-	       h->separateFunc == default_separate;
-}
 
 typedef struct Collect {
 	cpShape **list;
@@ -398,6 +406,54 @@ static void write_body(const Desc *D, int k, cpBody *b)
 	for (int f = 0; f < BODY_FIELDS; f++) set_body_field(b, f, D->state[f] + (size_t)body_width[f] * k);
 }
 
+/* the index of a collision type in D->types (added when new) */
+static uint32_t type_index(Desc *D, cpCollisionType t)
+{
+	if (t == CP_WILDCARD_COLLISION_TYPE) return CPGPU_WILDCARD;
+	for (int i = 0; i < D->ntypes; i++)
+		if (D->types[i] == t) return (uint32_t)i;
+	GROW(D->types, D->cap_t, D->ntypes + 1);
+	D->types[D->ntypes] = t;
+	return (uint32_t)D->ntypes++;
+}
+
+/* The callbacks of a handler that are the caller's: neither Chipmunk's do-nothing
+ * ones (cpSpace.c:100-106) nor the defaults that only call the wildcard handlers
+ * (cpSpace.c:66-90), whose own entries bring their events. */
+static uint32_t handler_events(const cpCollisionHandler *h)
+{
+	const cpCollisionHandler *none = &cpCollisionHandlerDoNothing;
+	uint32_t ev = 0;
+	if (h->beginFunc != none->beginFunc && h->beginFunc != wild_begin) ev |= CPGPU_EVENT_BEGIN;
+	if (h->preSolveFunc != none->preSolveFunc && h->preSolveFunc != wild_pre_solve) ev |= CPGPU_EVENT_PRE_SOLVE;
+	if (h->postSolveFunc != none->postSolveFunc && h->postSolveFunc != wild_post_solve) ev |= CPGPU_EVENT_POST_SOLVE;
+	if (h->separateFunc != none->separateFunc && h->separateFunc != wild_separate) ev |= CPGPU_EVENT_SEPARATE;
+	return ev;
+}
+
+static void add_handler(Desc *D, int space, uint32_t ta, uint32_t tb, uint32_t events)
+{
+	if (!events) return;
+	GROW(D->handlers, D->cap_h, D->nh + 1);
+	cpgpuHandlerDesc *h = &D->handlers[D->nh++];
+	h->space = space; h->type_a = ta; h->type_b = tb; h->events = events;
+}
+
+struct HandlerCtx {
+	Desc *D;
+	int space;
+};
+
+/* cpSpaceAddCollisionHandler and cpSpaceAddWildcardHandler entries of cpSpace.collisionHandlers */
+static void collect_handler(void *elt, void *data)
+{
+	const struct HandlerCtx *c = (const struct HandlerCtx *)data;
+	const cpCollisionHandler *h = (const cpCollisionHandler *)elt;
+	cpCollisionType a = h->typeA, b = h->typeB;
+	if (a == CP_WILDCARD_COLLISION_TYPE && b != CP_WILDCARD_COLLISION_TYPE) { a = h->typeB; b = h->typeA; }
+	add_handler(c->D, c->space, type_index(c->D, a), type_index(c->D, b), handler_events(h));
+}
+
 static int read_shape(Desc *D, int k, cpShape *sh, int body)
 {
 	memset(D->shape_geom + 10 * k, 0, 10 * sizeof(double));
@@ -411,6 +467,7 @@ static int read_shape(Desc *D, int k, cpShape *sh, int body)
 	D->shape_cats[k] = (uint32_t)sh->filter.categories;
 	D->shape_mask[k] = (uint32_t)sh->filter.mask;
 	D->shape_sensor[k] = sh->sensor ? 1 : 0;
+	D->shape_ctype[k] = type_index(D, sh->type);
 	D->shape_pstart[k] = 0;
 	D->shape_pcount[k] = 0;
 	double *g = D->shape_geom + 10 * k;
@@ -550,6 +607,8 @@ static int collect(Pack *pk, Desc *D, Maps *M)
 	PtrMap *bm = &M->body, *sm = &M->shape, *cm = &M->cons;
 	learn_defaults();
 	D->nw = pk->n_spaces;
+	D->nh = 0;
+	D->ntypes = 0;
 	D->nb = D->ns = D->nc = D->npv = 0;
 	desc_reserve(D, D->nw, 16, 16, 16, 16);
 	int nb = 0, ns = 0, nc = 0;
@@ -565,8 +624,10 @@ static int collect(Pack *pk, Desc *D, Maps *M)
 	for (int w = 0; w < pk->n_spaces; w++) {
 		cpSpace *sp = pk->spaces[w];
 		if (sp->sleepTimeThreshold != INFINITY) return report("cpGPUSpace: sleeping is not supported on the device", NULL);
-		if (sp->usesWildcards || cpHashSetCount(sp->collisionHandlers) > 0 || !handler_is_default(&sp->defaultHandler))
-			return report("cpGPUSpace: only the default collision handler is supported on the device", NULL);
+		/* the collision handlers (cpSpace.c:382-413): pair and wildcard ones, then the default */
+		struct HandlerCtx hc = {D, w};
+		cpHashSetEach(sp->collisionHandlers, collect_handler, &hc);
+		add_handler(D, w, CPGPU_WILDCARD, CPGPU_WILDCARD, handler_events(&sp->defaultHandler));
 		if (sp->locked) return report("cpGPUSpace: the space is locked", NULL);
 		cpgpuSpaceParams *P = &D->sp[w];
 		memset(P, 0, sizeof(*P));
@@ -659,6 +720,8 @@ static int same_structure(const Desc *A, const Desc *B)
 	       same_f64(A->shape_sv, B->shape_sv, 2 * A->ns) && same_u64(A->shape_group, B->shape_group, A->ns) &&
 	       same_u32(A->shape_cats, B->shape_cats, A->ns) && same_u32(A->shape_mask, B->shape_mask, A->ns) &&
 	       (A->ns == 0 || !memcmp(A->shape_sensor, B->shape_sensor, (size_t)A->ns)) &&
+	       same_u32(A->shape_ctype, B->shape_ctype, A->ns) && A->nh == B->nh &&
+	       (A->nh == 0 || !memcmp(A->handlers, B->handlers, (size_t)A->nh * sizeof(cpgpuHandlerDesc))) &&
 	       same_f64(A->shape_r, B->shape_r, A->ns) && same_f64(A->shape_geom, B->shape_geom, 10 * A->ns) &&
 	       same_i32(A->shape_pstart, B->shape_pstart, A->ns) && same_i32(A->shape_pcount, B->shape_pcount, A->ns) &&
 	       same_f64(A->pverts, B->pverts, 2 * A->npv) && same_f64(A->pnorms, B->pnorms, 2 * A->npv) &&
@@ -903,14 +966,187 @@ static void host_after_steps(Pack *pk, double dt, int steps)
 	}
 }
 
+/* ---- collision handler callbacks around a device step ---- */
+
+/* cpSpaceLookupHandler (cpArbiter.c:348-354) */
+static cpCollisionHandler *lookup_handler(cpSpace *space, cpCollisionType a, cpCollisionType b, cpCollisionHandler *dflt)
+{
+	cpCollisionType types[] = {a, b};
+	cpCollisionHandler *h = (cpCollisionHandler *)cpHashSetFind(space->collisionHandlers, CP_HASH_PAIR(a, b), types);
+	return h ? h : dflt;
+}
+
+/* An arbiter's handlers as cpArbiterUpdate chooses them (cpArbiter.c:399-409). */
+static void set_handlers(cpArbiter *arb)
+{
+	const cpShape *a = arb->a, *b = arb->b;
+	cpSpace *space = a->space;
+	cpCollisionHandler *dflt = &space->defaultHandler;
+	cpCollisionHandler *h = arb->handler = lookup_handler(space, a->type, b->type, dflt);
+	const cpBool swapped = arb->swapped = (a->type != h->typeA && h->typeA != CP_WILDCARD_COLLISION_TYPE);
+	arb->handlerA = arb->handlerB = &cpCollisionHandlerDoNothing;
+	if (h != dflt || space->usesWildcards) {
+		arb->handlerA = lookup_handler(space, swapped ? b->type : a->type, CP_WILDCARD_COLLISION_TYPE,
+		                               &cpCollisionHandlerDoNothing);
+		arb->handlerB = lookup_handler(space, swapped ? a->type : b->type, CP_WILDCARD_COLLISION_TYPE,
+		                               &cpCollisionHandlerDoNothing);
+	}
+}
+
+/* A cpArbiter for a callback on shapes a and b. */
+static void callback_arbiter(cpArbiter *arb, struct cpContact *con, cpShape *a, cpShape *b, int count)
+{
+	memset(arb, 0, sizeof(*arb));
+	memset(con, 0, 2 * sizeof(*con));
+	arb->a = a; arb->b = b;
+	arb->body_a = a->body; arb->body_b = b->body;
+	arb->contacts = con;
+	arb->count = count;
+	arb->stamp = a->space->stamp;
+	set_handlers(arb);
+}
+
+/* begin and preSolve of the step's due arbiters, as cpSpaceCollideShapes calls
+ * them (cpSpaceStep.c:255-266), with the spaces locked; their answers go back
+ * to the device */
+static int run_callbacks(Pack *pk, int64_t n)
+{
+	const Desc *C = &pk->cur;
+	cpgpuCallbackArbiter *due = (cpgpuCallbackArbiter *)malloc((size_t)n * sizeof(*due));
+	cpgpuCallbackDecision *dec = (cpgpuCallbackDecision *)malloc((size_t)n * sizeof(*dec));
+	int64_t got = 0;
+	int rc = cpgpuWorldReadCallbacks(pk->world, n, &got, due);
+	if (rc == 0) {
+		for (int w = 0; w < pk->n_spaces; w++) pk->spaces[w]->locked++;
+		for (int64_t k = 0; k < n; k++) {
+			const cpgpuCallbackArbiter *v = &due[k];
+			cpShape *a = C->shapes[v->shape_a], *b = C->shapes[v->shape_b];
+			cpSpace *space = a->space;
+			cpArbiter arb;
+			struct cpContact con[2];
+			callback_arbiter(&arb, con, a, b, v->count);
+			arb.n = cpv(v->normal[0], v->normal[1]);
+			arb.e = v->e; arb.u = v->u;
+			arb.surface_vr = cpv(v->surface_vr[0], v->surface_vr[1]);
+			arb.state = (enum cpArbiterState)v->state;
+			for (int c = 0; c < v->count && c < 2; c++) {
+				con[c].r1 = cpv(v->contact[c][0], v->contact[c][1]);
+				con[c].r2 = cpv(v->contact[c][2], v->contact[c][3]);
+				con[c].jnAcc = v->contact[c][4];
+				con[c].jtAcc = v->contact[c][5];
+				con[c].hash = (cpHashValue)v->hash[c];
+			}
+			cpCollisionHandler *h = arb.handler;
+			if (arb.state == CP_ARBITER_STATE_FIRST_COLLISION && !h->beginFunc(&arb, space, h->userData))
+				cpArbiterIgnore(&arb);
+			cpBool solve = cpFalse;
+			if (arb.state != CP_ARBITER_STATE_IGNORE)
+				solve = h->preSolveFunc(&arb, space, h->userData) && arb.state != CP_ARBITER_STATE_IGNORE;
+			cpgpuCallbackDecision *d = &dec[k];
+			d->slot = v->slot;
+			d->solve = solve ? 1 : 0;
+			d->ignore = arb.state == CP_ARBITER_STATE_IGNORE;
+			d->_pad = 0;
+			d->e = arb.e; d->u = arb.u;
+			d->surface_vr[0] = arb.surface_vr.x; d->surface_vr[1] = arb.surface_vr.y;
+		}
+		for (int w = 0; w < pk->n_spaces; w++) pk->spaces[w]->locked--;
+		rc = cpgpuWorldWriteDecisions(pk->world, n, dec);
+		if (rc) rc = lib_error("cpgpuWorldWriteDecisions", rc);
+	} else {
+		rc = lib_error("cpgpuWorldReadCallbacks", rc);
+	}
+	free(due);
+	free(dec);
+	return rc ? -1 : 0;
+}
+
+/* separate for the pairs that separated (cpSpaceStep.c:306-310), then postSolve
+ * for the solved arbiters (:437-442) */
+static int after_callbacks(Pack *pk)
+{
+	const Desc *C = &pk->cur;
+	uint32_t events = 0;
+	for (int k = 0; k < C->nh; k++) events |= C->handlers[k].events;
+	int64_t n = 0;
+	int rc;
+	if (events & CPGPU_EVENT_SEPARATE) {
+		rc = cpgpuWorldReadSeparations(pk->world, 0, &n, NULL);
+		if (rc) return lib_error("cpgpuWorldReadSeparations", rc);
+		if (n > 0) {
+			int32_t *pairs = (int32_t *)malloc((size_t)n * 3 * sizeof(int32_t));
+			rc = cpgpuWorldReadSeparations(pk->world, n, &n, pairs);
+			for (int64_t k = 0; rc == 0 && k < n; k++) {
+				cpShape *a = C->shapes[pairs[3 * k + 1]], *b = C->shapes[pairs[3 * k + 2]];
+				cpArbiter arb;
+				struct cpContact con[2];
+				callback_arbiter(&arb, con, a, b, 0);
+				arb.state = CP_ARBITER_STATE_CACHED;
+				arb.handler->separateFunc(&arb, a->space, arb.handler->userData);
+			}
+			free(pairs);
+			if (rc) return lib_error("cpgpuWorldReadSeparations", rc);
+		}
+	}
+	if (!(events & CPGPU_EVENT_POST_SOLVE)) return 0;
+	rc = cpgpuWorldReadArbiters(pk->world, 0, &n, NULL, NULL, NULL, NULL, NULL, NULL, NULL);
+	if (rc) return lib_error("cpgpuWorldReadArbiters", rc);
+	if (n == 0) return 0;
+	int32_t *shapes = (int32_t *)malloc((size_t)n * 2 * sizeof(int32_t));
+	int32_t *state = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+	int32_t *count = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+	double *normal = (double *)malloc((size_t)n * 2 * sizeof(double));
+	double *contact = (double *)malloc((size_t)n * 24 * sizeof(double));
+	uint64_t *hash = (uint64_t *)malloc((size_t)n * 2 * sizeof(uint64_t));
+	rc = cpgpuWorldReadArbiters(pk->world, n, &n, shapes, NULL, state, count, normal, contact, hash);
+	for (int64_t i = 0; rc == 0 && i < n; i++) {
+		cpShape *a = C->shapes[shapes[2 * i]], *b = C->shapes[shapes[2 * i + 1]];
+		cpArbiter arb;
+		struct cpContact con[2];
+		callback_arbiter(&arb, con, a, b, count[i]);
+		arb.n = cpv(normal[2 * i], normal[2 * i + 1]);
+		arb.e = a->e * b->e;
+		arb.u = a->u * b->u;
+		arb.state = (enum cpArbiterState)state[i];
+		for (int c = 0; c < count[i] && c < 2; c++) {
+			const double *q = contact + 24 * i + 12 * c;
+			con[c].r1 = cpv(q[0], q[1]); con[c].r2 = cpv(q[2], q[3]);
+			con[c].nMass = q[4]; con[c].tMass = q[5]; con[c].bounce = q[6];
+			con[c].jnAcc = q[7]; con[c].jtAcc = q[8]; con[c].jBias = q[9]; con[c].bias = q[10];
+			con[c].hash = (cpHashValue)hash[2 * i + c];
+		}
+		arb.handler->postSolveFunc(&arb, a->space, arb.handler->userData);
+	}
+	free(shapes); free(state); free(count); free(normal); free(contact); free(hash);
+	if (rc) return lib_error("cpgpuWorldReadArbiters", rc);
+	return 0;
+}
+
+static int pack_sync_state(Pack *pk);
+
 static int pack_step(Pack *pk, double dt)
 {
 	if (dt == 0.0) return 0;   /* cpSpaceStep.c:339 */
 	if (prepare(pk)) return -1;
-	const int rc = cpgpuWorldStep(pk->world, dt);
-	if (rc) return lib_error("cpgpuWorldStep", rc);
+	if (pk->cur.nh == 0) {
+		const int rc = cpgpuWorldStep(pk->world, dt);
+		if (rc) return lib_error("cpgpuWorldStep", rc);
+		host_after_steps(pk, dt, 1);
+		return 0;
+	}
+	/* with collision handlers: the step in two halves around the callbacks, which see
+	 * the bodies at this step's positions and the spaces' new stamps */
+	int64_t n = 0;
+	int rc = cpgpuWorldStepBegin(pk->world, dt, &n);
+	if (rc) return lib_error("cpgpuWorldStepBegin", rc);
 	host_after_steps(pk, dt, 1);
-	return 0;
+	if (n > 0 && (pack_sync_state(pk) || run_callbacks(pk, n))) {
+		(void)cpgpuWorldStepEnd(pk->world);
+		return -1;
+	}
+	rc = cpgpuWorldStepEnd(pk->world);
+	if (rc) return lib_error("cpgpuWorldStepEnd", rc);
+	return after_callbacks(pk);
 }
 
 /* ---- after a step: device state back into the structs ---- */
@@ -1035,8 +1271,7 @@ static int sync_arbiters(Pack *pk)
 			arb->u = a->u * b->u;
 			const cpVect surface_vr = cpvsub(b->surfaceV, a->surfaceV);
 			arb->surface_vr = cpvsub(surface_vr, cpvmult(arb->n, cpvdot(surface_vr, arb->n)));
-			arb->handler = &a->space->defaultHandler;
-			arb->swapped = cpFalse;
+			set_handlers(arb);
 			arb->stamp = a->space->stamp;
 			arb->state = (enum cpArbiterState)state[i];
 			for (int c = 0; c < 2; c++) {
@@ -1062,7 +1297,9 @@ static int sync_arbiters(Pack *pk)
 	return 0;
 }
 
-static int pack_sync(Pack *pk)
+/* The device's body state and joint impulses into the structs (the fields the
+ * caller has not changed), and the shapes' cached data. */
+static int pack_sync_state(Pack *pk)
 {
 	if (!pk->world) return 0;
 	/* the objects now in the spaces: only those are written (a removed body may be freed) */
@@ -1101,7 +1338,13 @@ static int pack_sync(Pack *pk)
 		if (cpBodyGetType(sh->body) == CP_BODY_TYPE_STATIC) continue;
 		cpShapeUpdate(sh, sh->body->transform);
 	}
-	return sync_arbiters(pk);
+	return 0;
+}
+
+static int pack_sync(Pack *pk)
+{
+	if (pack_sync_state(pk)) return -1;
+	return pk->world ? sync_arbiters(pk) : 0;
 }
 
 static void pack_release(Pack *pk)

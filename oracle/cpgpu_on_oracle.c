@@ -8,7 +8,8 @@
  * packing again after adds and removes, threading arbiters onto bodies --
  * runs on the CPU, under the compiler's address checks, without a GPU.
  *
- * Only what the binding needs: no timing, graphs, tape or queries.  Each call
+ * Only what the binding needs: no timing, graphs, tape, queries or handler
+ * callbacks.  Each call
  * validates its arguments as the device library does where the binding could
  * get them wrong (indices, counts), and records a message for
  * cpgpuGetLastError.
@@ -172,4 +173,36 @@ int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *t
 	if (threads_out)
 		for (int q = 0; q < CPGPU_MAX_BUCKETS; q++) threads_out[q] = 0;
 	return CPGPU_OK;
+}
+
+/* Collision handler callbacks need the step in two halves, which the oracle's
+ * step does not stop between: the binding's handler flows run over the device
+ * library (and its CPU emulation) only. */
+int cpgpuWorldStepBegin(cpgpuWorld *w, double dt, int64_t *n_callbacks)
+{
+	(void)w; (void)dt;
+	if (n_callbacks) *n_callbacks = 0;
+	return fail(CPGPU_ERR_INVALID, "collision handler callbacks need the device library");
+}
+int cpgpuWorldReadCallbacks(cpgpuWorld *w, int64_t cap, int64_t *n_out, cpgpuCallbackArbiter *out)
+{
+	(void)w; (void)cap; (void)out;
+	if (n_out) *n_out = 0;
+	return fail(CPGPU_ERR_INVALID, "collision handler callbacks need the device library");
+}
+int cpgpuWorldWriteDecisions(cpgpuWorld *w, int64_t n, const cpgpuCallbackDecision *d)
+{
+	(void)w; (void)n; (void)d;
+	return fail(CPGPU_ERR_INVALID, "collision handler callbacks need the device library");
+}
+int cpgpuWorldStepEnd(cpgpuWorld *w)
+{
+	(void)w;
+	return fail(CPGPU_ERR_INVALID, "collision handler callbacks need the device library");
+}
+int cpgpuWorldReadSeparations(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *pairs)
+{
+	(void)w; (void)cap; (void)pairs;
+	if (n_out) *n_out = 0;
+	return fail(CPGPU_ERR_INVALID, "collision handler callbacks need the device library");
 }

@@ -359,3 +359,44 @@ def joints_sync_to_host(kind):
     jo, _ = o2.constraints()
     assert np.array_equal(host_jacc(), jo[at])
     g.free()
+
+
+def handlers_run_the_callers_callbacks(kind):
+    """Collision handlers set through the reference's API (ref_bridge installs
+    them from the description: a pair handler with every callback, pair
+    handlers with preSolve, a wildcard handler for a few sensors) run as the
+    reference runs them: the binding calls their begin and preSolve between the
+    device's two half-steps, then separate and postSolve.  The same
+    deterministic answers (handler_rules.Rule) from the oracle stepping the
+    binding's description: each step the same pairs get each callback, and the
+    bodies stay bit-exact."""
+    from chipmunk2d_amd.cabi import EVENT_BEGIN, EVENT_POST_SOLVE, EVENT_PRE_SOLVE, EVENT_SEPARATE, HANDLER_FN
+    from handler_rules import Rule, handler_space
+    pw = handler_space(n_circles=90, n_polys=90, n_segments=20, extent=35.0, seed=41).pack()
+    g = GPUSpace(pw, kind)
+    rg = Rule()
+    hfn = HANDLER_FN(lambda ctx, ev, vv, dd: rg(ev, vv.contents, dd.contents))
+    ref_lib().rb_set_handler_fn(g.rb, hfn, None)
+    g.step()
+    # the rule sees the description's shape indices on both sides
+    assert [g.B.cpGPUSpaceShapeIndex(g.space, sh) for sh in g.shapes] == list(range(pw.n_shapes))
+    o = OracleWorld(g.desc())
+    assert len(o.packed.a["handlers"]) == 4
+    ro = Rule()
+    o.set_handler_fn(ro)
+    o.step(DT)
+    kinds = (EVENT_BEGIN, EVENT_PRE_SOLVE, EVENT_POST_SOLVE, EVENT_SEPARATE)
+    for s in range(1, 90):
+        rg.step = ro.step = s
+        g.step()
+        o.step(DT)
+        for k in kinds:
+            eg = {e for e in rg.events(k) if e[0] == s}
+            eo = {e for e in ro.events(k) if e[0] == s}
+            assert eg == eo, (s, k, sorted(eg ^ eo)[:6])
+        if s % 15 == 14:
+            g.sync()
+            _assert_host_equals_oracle(g, o, "step %d" % s)
+    counts = {k: len(rg.events(k)) for k in kinds}
+    assert all(c > 10 for c in counts.values()), counts
+    g.free()

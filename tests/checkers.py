@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from chipmunk2d_amd.cabi import HANDLER_FN, PackedWorld, SpaceParams, Stats, WorldDesc, ptr
+from chipmunk2d_amd.cabi import HANDLER_DTYPE, HANDLER_FN, PackedWorld, SpaceParams, Stats, WorldDesc, ptr
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
@@ -474,6 +474,10 @@ def packed_from_desc(desc_ptr):
     a["shape_categories"] = arr(d.shape_categories, ns, np.uint32)
     a["shape_mask"] = arr(d.shape_mask, ns, np.uint32)
     a["shape_sensor"] = arr(d.shape_sensor, ns, np.uint8)
+    a["shape_ctype"] = arr(d.shape_ctype, ns, np.uint32)
+    nh = d.n_handlers
+    a["handlers"] = (np.ctypeslib.as_array(ctypes.cast(d.handlers, ctypes.POINTER(ctypes.c_uint32)), shape=(4 * nh,))
+                     .copy().view(HANDLER_DTYPE) if nh > 0 else np.zeros(0, HANDLER_DTYPE))
     a["shape_radius"] = arr(d.shape_radius, ns, np.float64)
     a["shape_geom"] = arr(d.shape_geom, ns, np.float64, 10)
     a["shape_poly_start"] = arr(d.shape_poly_start, ns, np.int32)

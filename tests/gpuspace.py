@@ -40,6 +40,13 @@ _cp = None
 _gs = {}
 
 
+class CollisionHandler(ctypes.Structure):
+    """struct cpCollisionHandler (include/chipmunk/cpSpace.h)."""
+    _fields_ = [("typeA", ctypes.c_size_t), ("typeB", ctypes.c_size_t), ("beginFunc", ctypes.c_void_p),
+                ("preSolveFunc", ctypes.c_void_p), ("postSolveFunc", ctypes.c_void_p),
+                ("separateFunc", ctypes.c_void_p), ("userData", ctypes.c_void_p)]
+
+
 def cp():
     """The reference's public API (the functions the tests call directly)."""
     global _cp
@@ -55,7 +62,10 @@ def cp():
             "cpSpaceRemoveBody": (None, [_vp, _vp]), "cpSpaceRemoveShape": (None, [_vp, _vp]),
             "cpSpaceRemoveConstraint": (None, [_vp, _vp]),
             "cpSpaceSetSleepTimeThreshold": (None, [_vp, _d]),
-            "cpSpaceAddCollisionHandler": (_vp, [_vp, ctypes.c_size_t, ctypes.c_size_t]),
+            "cpSpaceAddCollisionHandler": (ctypes.POINTER(CollisionHandler), [_vp, ctypes.c_size_t, ctypes.c_size_t]),
+            "cpSpaceAddWildcardHandler": (ctypes.POINTER(CollisionHandler), [_vp, ctypes.c_size_t]),
+            "cpSpaceAddDefaultCollisionHandler": (ctypes.POINTER(CollisionHandler), [_vp]),
+            "cpShapeSetCollisionType": (None, [_vp, ctypes.c_size_t]),
             "cpBodyNew": (_vp, [_d, _d]), "cpBodyNewStatic": (_vp, []), "cpBodyFree": (None, [_vp]),
             "cpBodySetPosition": (None, [_vp, cpVect]), "cpBodySetVelocity": (None, [_vp, cpVect]),
             "cpBodySetAngle": (None, [_vp, _d]), "cpBodySetAngularVelocity": (None, [_vp, _d]),

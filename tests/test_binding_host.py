@@ -123,23 +123,53 @@ def _free(space, body, shape):
     gs().cpGPUSpaceFree(space)
 
 
+_PRE_SOLVE = ctypes.CFUNCTYPE(ctypes.c_ubyte, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+_my_pre_solve = _PRE_SOLVE(lambda arb, space, data: 1)
+_SEPARATE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+_my_separate = _SEPARATE(lambda arb, space, data: None)
+
+
+def test_collision_handlers_are_packed_with_the_callers_callbacks():
+    """The handler table of the description lists, per handler, only the
+    callbacks that are the caller's: not the do-nothing ones, not the defaults
+    that only call the wildcard handlers.  Collision types become indices."""
+    from checkers import packed_from_desc
+    from chipmunk2d_amd.cabi import EVENT_PRE_SOLVE, EVENT_SEPARATE, WILDCARD
+    C = cp()
+    space, body, shape = _space_with_one_circle()
+    C.cpShapeSetCollisionType(shape, 70)
+    h = C.cpSpaceAddCollisionHandler(space, 70, 90)          # DefaultBegin ... : none of them the caller's
+    h.contents.preSolveFunc = ctypes.cast(_my_pre_solve, ctypes.c_void_p)
+    w = C.cpSpaceAddWildcardHandler(space, 90)               # AlwaysCollide / DoNothing but the caller's separate
+    w.contents.separateFunc = ctypes.cast(_my_separate, ctypes.c_void_p)
+    C.cpSpaceAddDefaultCollisionHandler(space)               # the wildcard-calling defaults: no entry
+    d = gs().cpGPUSpaceDescribe(space)
+    assert d, gs().cpGPUGetError()
+    pw = packed_from_desc(d)
+    hs = sorted((int(x["space"]), int(x["type_a"]), int(x["type_b"]), int(x["events"])) for x in pw.a["handlers"])
+    t70 = int(pw.a["shape_ctype"][0])
+    assert t70 != WILDCARD
+    assert len(hs) == 2, hs
+    pre = [x for x in hs if x[3] == EVENT_PRE_SOLVE]
+    sep = [x for x in hs if x[3] == EVENT_SEPARATE]
+    assert len(pre) == 1 and pre[0][1] == t70 and pre[0][2] not in (t70, WILDCARD)
+    assert len(sep) == 1 and sep[0][1] == pre[0][2] and sep[0][2] == WILDCARD
+    _free(space, body, shape)
+
+
 _TORQUE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
 _custom_torque = _TORQUE(lambda c, angle: 2.0 * angle)
 
 
-@pytest.mark.parametrize("feature", ["custom handler", "sleeping", "custom spring torque"])
+@pytest.mark.parametrize("feature", ["sleeping", "custom spring torque"])
 def test_unsupported_features_are_refused_at_pack_time(feature):
-    """A collision handler of the caller's, sleeping and a damped rotary spring
-    with a torque function of the caller's are refused when the space is packed
-    (the step does nothing); the message is cpAssertHard's (which aborts when
-    cpGPUSetAbortOnError is on)."""
+    """Sleeping and a damped rotary spring with a torque function of the
+    caller's are refused when the space is packed (the step does nothing); the
+    message is cpAssertHard's (which aborts when cpGPUSetAbortOnError is on)."""
     C = cp()
     space, body, shape = _space_with_one_circle()
     extra = []
-    if feature == "custom handler":
-        C.cpSpaceAddCollisionHandler(space, 1, 2)
-        msg = "default collision handler"
-    elif feature == "sleeping":
+    if feature == "sleeping":
         C.cpSpaceSetSleepTimeThreshold(space, 1.0)
         msg = "sleeping"
     elif feature == "custom spring torque":

@@ -44,3 +44,7 @@ def test_batch_of_plain_spaces_and_alone():
 
 def test_every_joint_class_syncs_to_host():
     F.joints_sync_to_host("gpu")
+
+
+def test_collision_handlers_run_the_callers_callbacks():
+    F.handlers_run_the_callers_callbacks("gpu")

@@ -1007,8 +1007,7 @@ static void callback_arbiter(cpArbiter *arb, struct cpContact *con, cpShape *a, 
 }
 
 /* begin and preSolve of the step's due arbiters, as cpSpaceCollideShapes calls
- * them (cpSpaceStep.c:255-266), with the spaces locked; their answers go back
- * to the device */
+ * them (cpSpaceStep.c:255-266); their answers go back to the device */
 static int run_callbacks(Pack *pk, int64_t n)
 {
 	const Desc *C = &pk->cur;
@@ -1017,7 +1016,6 @@ static int run_callbacks(Pack *pk, int64_t n)
 	int64_t got = 0;
 	int rc = cpgpuWorldReadCallbacks(pk->world, n, &got, due);
 	if (rc == 0) {
-		for (int w = 0; w < pk->n_spaces; w++) pk->spaces[w]->locked++;
 		for (int64_t k = 0; k < n; k++) {
 			const cpgpuCallbackArbiter *v = &due[k];
 			cpShape *a = C->shapes[v->shape_a], *b = C->shapes[v->shape_b];
@@ -1050,7 +1048,6 @@ static int run_callbacks(Pack *pk, int64_t n)
 			d->e = arb.e; d->u = arb.u;
 			d->surface_vr[0] = arb.surface_vr.x; d->surface_vr[1] = arb.surface_vr.y;
 		}
-		for (int w = 0; w < pk->n_spaces; w++) pk->spaces[w]->locked--;
 		rc = cpgpuWorldWriteDecisions(pk->world, n, dec);
 		if (rc) rc = lib_error("cpgpuWorldWriteDecisions", rc);
 	} else {
@@ -1135,18 +1132,21 @@ static int pack_step(Pack *pk, double dt)
 		return 0;
 	}
 	/* with collision handlers: the step in two halves around the callbacks, which see
-	 * the bodies at this step's positions and the spaces' new stamps */
+	 * the bodies at this step's positions and the spaces' new stamps, with the spaces
+	 * locked as cpSpaceStep locks them; unlocking runs the post-step callbacks the
+	 * handlers queued (cpSpaceStep.c:368-445, cpSpaceUnlock :69-104) */
 	int64_t n = 0;
 	int rc = cpgpuWorldStepBegin(pk->world, dt, &n);
 	if (rc) return lib_error("cpgpuWorldStepBegin", rc);
 	host_after_steps(pk, dt, 1);
-	if (n > 0 && (pack_sync_state(pk) || run_callbacks(pk, n))) {
-		(void)cpgpuWorldStepEnd(pk->world);
-		return -1;
-	}
+	int failed = n > 0 && pack_sync_state(pk);
+	for (int w = 0; w < pk->n_spaces; w++) cpSpaceLock(pk->spaces[w]);
+	if (!failed && n > 0) failed = run_callbacks(pk, n);
 	rc = cpgpuWorldStepEnd(pk->world);
-	if (rc) return lib_error("cpgpuWorldStepEnd", rc);
-	return after_callbacks(pk);
+	if (!failed && rc) failed = lib_error("cpgpuWorldStepEnd", rc);
+	if (!failed) failed = after_callbacks(pk);
+	for (int w = 0; w < pk->n_spaces; w++) cpSpaceUnlock(pk->spaces[w], failed ? cpFalse : cpTrue);
+	return failed ? -1 : 0;
 }
 
 /* ---- after a step: device state back into the structs ---- */

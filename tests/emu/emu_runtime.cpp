@@ -383,10 +383,16 @@ using emu::last_error;
 
 static hipError_t ok() { return hipSuccess; }
 
+// allocation failures on demand: the n-th device allocation from now fails (0: none)
+static std::atomic<long> malloc_countdown(0);
+extern "C" void emu_fail_malloc_after(long n) { malloc_countdown = n; }
+extern "C" long emu_malloc_countdown() { return malloc_countdown; }
+
 hipError_t hipMalloc(void **p, size_t bytes)
 {
 	if (!p) return hipErrorInvalidValue;
 	*p = nullptr;
+	if (malloc_countdown > 0 && --malloc_countdown == 0) return hipErrorOutOfMemory;
 	if (bytes == 0) return ok();
 	void *q = nullptr;
 	if (posix_memalign(&q, 256, (bytes + 255) & ~(size_t)255) != 0) return hipErrorOutOfMemory;

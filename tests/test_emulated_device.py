@@ -78,3 +78,37 @@ def test_shape_queries_bit_exact(emulated):
         o.step(DT)
     assert np.array_equal(g.bodies().p, o.bodies().p)
     assert _check_shape_queries(g, o, pw) > 50
+
+
+def test_device_allocation_failures_are_reported(emulated):
+    """Every device allocation of a world's creation and first steps made to fail
+    in turn (the emulation's hipMalloc fails on demand): each one comes back as
+    an error from the C ABI -- no null or stale buffer is ever launched on -- and
+    a world made afterwards steps normally."""
+    import ctypes
+    from chipmunk2d_amd.scenes import pyramid_210, random_world
+    from chipmunk2d_amd.cabi import concat
+    L = ctypes.CDLL(EMU_SO)
+    L.emu_fail_malloc_after.argtypes = [ctypes.c_long]
+    L.emu_malloc_countdown.restype = ctypes.c_long
+    pw = concat([pyramid_210(rows=5).pack(), random_world(n_circles=60, n_polys=60, n_segments=10, extent=25.0,
+                                                          seed=26).pack()])
+    failed, k = 0, 1
+    while True:
+        L.emu_fail_malloc_after(k)
+        try:
+            g = emulated.GPUWorld(pw)
+            g.step(DT, 4)
+            g.bodies()
+            g.close()
+        except emulated.CpgpuError:
+            failed += 1
+        left = L.emu_malloc_countdown()
+        L.emu_fail_malloc_after(0)
+        if left > 0:   # fewer than k allocations: every one has been made to fail
+            break
+        k += 1
+    assert failed == k - 1 and k > 20, (failed, k)
+    g = emulated.GPUWorld(pw)
+    g.step(DT, 4)
+    assert np.all(np.isfinite(g.bodies().p))

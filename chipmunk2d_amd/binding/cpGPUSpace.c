@@ -1127,9 +1127,11 @@ static int pack_step(Pack *pk, double dt)
 	if (prepare(pk)) return -1;
 	if (pk->cur.nh == 0) {
 		const int rc = cpgpuWorldStep(pk->world, dt);
-		if (rc) return lib_error("cpgpuWorldStep", rc);
+		if (rc && rc != CPGPU_ERR_FAULT) return lib_error("cpgpuWorldStep", rc);
 		host_after_steps(pk, dt, 1);
-		return 0;
+		/* a fault: the step ran in full and failed a check of the reference's
+		 * debug build, reported (by default: aborting) as that build does */
+		return rc ? lib_error("cpgpuWorldStep", rc) : 0;
 	}
 	/* with collision handlers: the step in two halves around the callbacks, which see
 	 * the bodies at this step's positions and the spaces' new stamps, with the spaces
@@ -1143,9 +1145,10 @@ static int pack_step(Pack *pk, double dt)
 	for (int w = 0; w < pk->n_spaces; w++) cpSpaceLock(pk->spaces[w]);
 	if (!failed && n > 0) failed = run_callbacks(pk, n);
 	rc = cpgpuWorldStepEnd(pk->world);
-	if (!failed && rc) failed = lib_error("cpgpuWorldStepEnd", rc);
+	if (!failed && rc && rc != CPGPU_ERR_FAULT) failed = lib_error("cpgpuWorldStepEnd", rc);
 	if (!failed) failed = after_callbacks(pk);
 	for (int w = 0; w < pk->n_spaces; w++) cpSpaceUnlock(pk->spaces[w], failed ? cpFalse : cpTrue);
+	if (!failed && rc == CPGPU_ERR_FAULT) failed = lib_error("cpgpuWorldStepEnd", rc);   /* as above */
 	return failed ? -1 : 0;
 }
 

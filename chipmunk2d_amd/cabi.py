@@ -20,6 +20,8 @@ ARB_FIRST_COLLISION, ARB_NORMAL, ARB_IGNORE, ARB_CACHED = 0, 1, 2, 3
 FP64, FP32 = 0, 1
 EVENT_BEGIN, EVENT_PRE_SOLVE, EVENT_POST_SOLVE, EVENT_SEPARATE = 1, 2, 4, 8
 WILDCARD = 0xFFFFFFFF
+# the checks a debug build of the reference makes during cpSpaceStep (cpgpuWorldGetFaults)
+FAULT_UNSOLVABLE, FAULT_BODY_STATE, FAULT_BODY_MASS = 128, 512, 1024
 # a description's collision handlers: {space (-1: every space), type_a, type_b, events}
 HANDLER_DTYPE = np.dtype([("space", np.int32), ("type_a", np.uint32), ("type_b", np.uint32), ("events", np.uint32)])
 

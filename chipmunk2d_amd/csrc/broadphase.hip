@@ -68,8 +68,11 @@ __global__ void __launch_bounds__(256) k_bounds(const DevWorld W, const int32_t 
 		const int s = shapes[i];
 		const V2 c = box_centre(W.s_bb[s]);
 		w = W.s_world[s];
-		x0 = x1 = ord_of(c.x);
-		y0 = y1 = ord_of(c.y);
+		// a centre that is not finite (a body whose state overflowed) bounds nothing
+		if (__builtin_isfinite(c.x) && __builtin_isfinite(c.y)) {
+			x0 = x1 = ord_of(c.x);
+			y0 = y1 = ord_of(c.y);
+		}
 	}
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int w_first = __shfl(w, 0);
@@ -382,9 +385,15 @@ __global__ void k_karras(const uint64_t *keys, int n, int32_t *child, int32_t *p
 	if (first[w] == lo && last[w] == hi) root[w] = i;
 }
 
+// min and max that drop a NaN operand: a leaf box with a NaN side (a body whose
+// state went NaN; such a box overlaps nothing) leaves the boxes of the nodes
+// above it as the union of the others, so no other pair is pruned away
+__device__ __forceinline__ real min_nn(real a, real b) { return (a < b || b != b) ? a : b; }
+__device__ __forceinline__ real max_nn(real a, real b) { return (a > b || b != b) ? a : b; }
+
 __device__ __forceinline__ real4 box_union(real4 a, real4 b)
 {
-	return make_real4(fminc(a.x, b.x), fminc(a.y, b.y), fmaxc(a.z, b.z), fmaxc(a.w, b.w));
+	return make_real4(min_nn(a.x, b.x), min_nn(a.y, b.y), max_nn(a.z, b.z), max_nn(a.w, b.w));
 }
 
 // node x's box, also into its parent's traversal record

@@ -327,4 +327,25 @@ __device__ __forceinline__ uint32_t pair_events(const DevWorld &W, int s, int t)
 
 CPD void raise_err(int32_t *err, int32_t code) { atomicOr(err, code); }
 
+// The reference's debug-build step checks (CPGPU_FAULT_*) share the word.
+#define CPGPU_FAULTS (CPGPU_FAULT_UNSOLVABLE | CPGPU_FAULT_BODY_STATE | CPGPU_FAULT_BODY_MASS)
+
+// a body failed one of them: err[1] keeps the lowest such index b as INT32_MAX - b (0: none)
+CPD void raise_body_fault(int32_t *err, int32_t code, int b)
+{
+	atomicOr(err, code);
+	atomicMax(err + 1, 0x7fffffff - b);
+}
+
+// cpBodySanityCheck (cpBody.c:110-125) over what the device keeps of a body:
+// its moment only as the inverse, which carries the same NaN and sign tests
+CPD bool body_sane(real m, real m_inv, real i_inv, real px, real py, real vx, real vy, real fx, real fy, real a,
+                   real w, real t)
+{
+	return m == m && m_inv == m_inv && i_inv == i_inv && m >= real(0.0) && i_inv >= real(0.0) &&
+	       __builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(vx) && __builtin_isfinite(vy) &&
+	       __builtin_isfinite(fx) && __builtin_isfinite(fy) && __builtin_isfinite(a) && __builtin_isfinite(w) &&
+	       __builtin_isfinite(t);
+}
+
 } // namespace CPGPU_NS

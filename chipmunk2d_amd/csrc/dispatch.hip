@@ -25,6 +25,7 @@ thread_local std::string cpgpu_last_error;
 	int cpgpuWorldStepEnd(cpgpuWorld *world); \
 	int cpgpuWorldReadSeparations(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs); \
 	int cpgpuWorldSync(cpgpuWorld *world); \
+	int cpgpuWorldGetFaults(cpgpuWorld *world, int32_t *faults, int32_t *first_body); \
 	int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, double *w, double *v_bias, double *w_bias, double *transform); \
 	int cpgpuWorldWriteBodies(cpgpuWorld *world, const double *v, const double *w, const double *f, const double *t); \
 	int cpgpuWorldWriteBodiesAt(cpgpuWorld *world, int32_t n, const int32_t *index, const double *p, const double *v, const double *f, const double *a, const double *w, const double *t, const double *transform); \
@@ -125,6 +126,13 @@ int cpgpuWorldSync(cpgpuWorld *world)
 	if (!world) return null_world();
 	if (is_fp32(world)) return cpgpu32::cpgpuWorldSync(reinterpret_cast<cpgpu32::cpgpuWorld *>(world));
 	return cpgpu64::cpgpuWorldSync(reinterpret_cast<cpgpu64::cpgpuWorld *>(world));
+}
+
+int cpgpuWorldGetFaults(cpgpuWorld *world, int32_t *faults, int32_t *first_body)
+{
+	if (!world) return null_world();
+	if (is_fp32(world)) return cpgpu32::cpgpuWorldGetFaults(reinterpret_cast<cpgpu32::cpgpuWorld *>(world), faults, first_body);
+	return cpgpu64::cpgpuWorldGetFaults(reinterpret_cast<cpgpu64::cpgpuWorld *>(world), faults, first_body);
 }
 
 int cpgpuWorldReadBodies(cpgpuWorld *world, double *p, double *v, double *a, double *w, double *v_bias, double *w_bias, double *transform)

@@ -151,6 +151,13 @@ __device__ __forceinline__ real k_scalar(const BV &a, const BV &b, V2 r1, V2 r2,
 	return k_scalar_body(a, r1, n) + k_scalar_body(b, r2, n);
 }
 
+// k_scalar's "Unsolvable collision or constraint." check (chipmunk_private.h:223)
+__device__ __forceinline__ real solvable(const DevWorld &W, real k)
+{
+	if (k == real(0.0)) raise_err(W.err, CPGPU_FAULT_UNSOLVABLE);
+	return k;
+}
+
 // ---- arbiters: cpArbiterPreStep / ApplyCachedImpulse / ApplyImpulse (cpArbiter.c:416-498) ----
 
 __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
@@ -163,8 +170,8 @@ __device__ void arb_prestep(const DevWorld &W, Arbiter &arb)
 	for (int i = 0; i < arb.count; i++) {
 		Contact &con = arb.c[i];
 		const V2 r1 = v2(con.r1x, con.r1y), r2 = v2(con.r2x, con.r2y);
-		con.nMass = 1.0 / k_scalar(a, b, r1, r2, n);
-		con.tMass = 1.0 / k_scalar(a, b, r1, r2, vperp(n));
+		con.nMass = 1.0 / solvable(W, k_scalar(a, b, r1, r2, n));
+		con.tMass = 1.0 / solvable(W, k_scalar(a, b, r1, r2, vperp(n)));
 		const real dist = vdot(vadd(vsub(r2, r1), body_delta), n);
 		con.bias = -ws.bias_coef * fminc(real(0.0), dist + ws.slop) / ws.dt;
 		con.jBias = 0.0;
@@ -254,8 +261,9 @@ __device__ __forceinline__ void arb_apply_impulse(const DevWorld &W, const BA &b
 // cpPinJoint.c:24-71, cpGrooveJoint.c:24-93, cpRotaryLimitJoint.c:24-83, cpRatchetJoint.c:24-89,
 // cpSimpleMotor.c:24-67, cpDampedRotarySpring.c:24-72) ----
 
-// k_tensor (chipmunk_private.h:228-262): the inverse 2x2 effective mass of a point joint
-__device__ __forceinline__ void k_tensor(const BV &a, const BV &b, V2 r1, V2 r2, Joint &j)
+// k_tensor (chipmunk_private.h:228-262): the inverse 2x2 effective mass of a
+// point joint, with its "Unsolvable constraint." check (:255)
+__device__ __forceinline__ void k_tensor(const DevWorld &W, const BV &a, const BV &b, V2 r1, V2 r2, Joint &j)
 {
 	const real m_sum = a.m_inv + b.m_inv;
 	real k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
@@ -270,6 +278,7 @@ __device__ __forceinline__ void k_tensor(const BV &a, const BV &b, V2 r1, V2 r2,
 	const real r2nxy = -r2.x * r2.y * b_i_inv;
 	k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
 	const real det = k11 * k22 - k12 * k21;
+	if (det == real(0.0)) raise_err(W.err, CPGPU_FAULT_UNSOLVABLE);
 	const real det_inv = 1.0 / det;
 	j.k11 = k22 * det_inv; j.k12 = -k12 * det_inv;
 	j.k21 = -k21 * det_inv; j.k22 = k11 * det_inv;
@@ -295,7 +304,7 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 		const V2 r1 = xvect(Ta, vsub(anchorA, v2(cogA.x, cogA.y)));
 		const V2 r2 = xvect(Tb, vsub(anchorB, v2(cogB.x, cogB.y)));
 		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
-		k_tensor(a, b, r1, r2, j);
+		k_tensor(W, a, b, r1, r2, j);
 		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(v2(pA.x, pA.y), r1));
 		const V2 bias = vclamp(vmult(delta, -bc / dt), j.maxBias);
 		j.bx = bias.x; j.by = bias.y;
@@ -308,7 +317,7 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 		const real dist = vlength(delta);
 		const V2 n = vmult(delta, 1.0 / (dist != 0.0 ? dist : __builtin_inf()));
 		j.nx = n.x; j.ny = n.y;
-		const real k = k_scalar(a, b, r1, r2, n);
+		const real k = solvable(W, k_scalar(a, b, r1, r2, n));
 		j.nMass = 1.0 / k;
 		j.target_vrn = 0.0;
 		j.v_coef = 1.0 - cp_exp(-damping * dt * k);
@@ -330,7 +339,7 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 		else if (dist < mn) { pdist = mn - dist; n = vneg(vnormalize(delta)); }
 		else { n = v2(0.0, 0.0); j.jnAcc = 0.0; }
 		j.nx = n.x; j.ny = n.y;
-		j.nMass = 1.0 / k_scalar(a, b, r1, r2, n);
+		j.nMass = 1.0 / solvable(W, k_scalar(a, b, r1, r2, n));
 		const real maxBias = j.maxBias;
 		j.bias = fclampc(-bc * pdist / dt, -maxBias, maxBias);
 	} else if (j.type == CPGPU_JOINT_GEAR) {
@@ -346,7 +355,7 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 		const real dist = vlength(delta);
 		const V2 n = vmult(delta, 1.0 / (dist != 0.0 ? dist : __builtin_inf()));
 		j.nx = n.x; j.ny = n.y;
-		j.nMass = 1.0 / k_scalar(a, b, r1, r2, n);
+		j.nMass = 1.0 / solvable(W, k_scalar(a, b, r1, r2, n));
 		const real maxBias = j.maxBias;
 		j.bias = fclampc(-bc * (dist - j.P[4]) / dt, -maxBias, maxBias);
 	} else if (j.type == CPGPU_JOINT_GROOVE) {
@@ -363,7 +372,7 @@ __device__ void joint_prestep(const DevWorld &W, Joint &j)
 		else if (td >= vcross(tb, n)) { j.clamp = -1.0; r1 = vsub(tb, p_a); }
 		else { j.clamp = 0.0; r1 = vsub(vadd(vmult(vperp(n), -td), vmult(n, d)), p_a); }
 		j.r1x = r1.x; j.r1y = r1.y; j.r2x = r2.x; j.r2y = r2.y;
-		k_tensor(a, b, r1, r2, j);
+		k_tensor(W, a, b, r1, r2, j);
 		const V2 delta = vsub(vadd(v2(pB.x, pB.y), r2), vadd(p_a, r1));
 		const V2 bias = vclamp(vmult(delta, -bc / dt), j.maxBias);
 		j.bx = bias.x; j.by = bias.y;
@@ -546,6 +555,11 @@ __device__ __forceinline__ void integrate_velocity(const DevWorld &W, int b)
 	const WorldStep ws = W.ws[W.b_world[b]];
 	BodyVel bv = W.b_vel[b];
 	const real2 f = W.b_f[b];
+	const real m = W.b_m[b];
+	// "Body's mass and moment must be positive to simulate." (cpBody.c:498); the
+	// moment is positive when its inverse lies in [0, inf)
+	if (!(m > real(0.0) && bv.i_inv >= real(0.0) && bv.i_inv < __builtin_inf()))
+		raise_body_fault(W.err, CPGPU_FAULT_BODY_MASS, b);
 	const V2 v = vadd(vmult(v2(bv.vx, bv.vy), ws.damping),
 	                  vmult(vadd(v2(ws.gx, ws.gy), vmult(v2(f.x, f.y), bv.m_inv)), ws.dt));
 	bv.vx = v.x; bv.vy = v.y;
@@ -553,6 +567,11 @@ __device__ __forceinline__ void integrate_velocity(const DevWorld &W, int b)
 	W.b_vel[b] = bv;
 	W.b_f[b] = make_real2(0.0, 0.0);
 	W.b_t[b] = 0.0;
+	// the sanity check after the update (cpBody.c:508): position and angle are
+	// as the position update checked them this step, force and torque now zero
+	if (!body_sane(m, bv.m_inv, bv.i_inv, real(0.0), real(0.0), bv.vx, bv.vy, real(0.0), real(0.0), real(0.0), bv.w,
+	               real(0.0)))
+		raise_body_fault(W.err, CPGPU_FAULT_BODY_STATE, b);
 }
 
 // colouring priority: a bijective hash of the shape pair's indices within its

@@ -87,6 +87,10 @@ __global__ void k_update_positions(const DevWorld W, const int32_t *bodies, int 
 	W.b_T[b] = T;
 	bv.vbx = real(0.0); bv.vby = real(0.0); bv.wb = real(0.0);
 	W.b_vel[b] = bv;
+	// the sanity check after the update (cpBody.c:521)
+	const real2 f = W.b_f[b];
+	if (!body_sane(W.b_m[b], bv.m_inv, bv.i_inv, p.x, p.y, bv.vx, bv.vy, f.x, f.y, a, bv.w, W.b_t[b]))
+		raise_body_fault(W.err, CPGPU_FAULT_BODY_STATE, b);
 }
 
 __global__ void k_set_joint_bias(Joint *joints, const real *bc, int n)
@@ -336,7 +340,8 @@ struct cpgpuWorld {
 	int grid_parts = GRID_PARTS_MIN;   // strips of every large world's grid set
 	int max_iterations = 0;
 	// host mirror of counters after the last sync
-	int32_t *h_counts = nullptr;        // pinned: [0..5] step counts, [6] n_pairs, [7] err
+	int32_t *h_counts = nullptr;        // pinned: [0..5] step counts, [6] n_pairs, [7] err, [8] first failing body
+	int32_t faults_reported = 0;        // the CPGPU_FAULT_* a synchronising call has already returned
 	int64_t steps = 0;
 	int64_t last_pairs = 0;
 	int64_t total_pairs = 0;
@@ -485,9 +490,10 @@ int validate(const cpgpuWorldDesc *d)
 	return CPGPU_OK;
 }
 
+// The error word's conditions other than the faults: the call that waited fails.
 int check_device_error(cpgpuWorld *w)
 {
-	const int32_t e = w->h_counts[7];
+	const int32_t e = w->h_counts[7] & ~CPGPU_FAULTS;
 	if (!e) return CPGPU_OK;
 	std::string m = "device error during step:";
 	if (e & CPGPU_ERR_PAIRS_OVERFLOW) m += " pair buffer overflow;";
@@ -501,12 +507,29 @@ int check_device_error(cpgpuWorld *w)
 	return fail(CPGPU_ERR_DEVICE, m);
 }
 
+// The reference's debug-build checks (CPGPU_FAULT_*) raised since the last
+// report, as of the last wait: returned once, by a step when it has enqueued
+// all its work or by cpgpuWorldSync, never by a reader.
+int report_faults(cpgpuWorld *w)
+{
+	const int32_t e = w->h_counts[7] & CPGPU_FAULTS & ~w->faults_reported;
+	if (!e) return CPGPU_OK;
+	w->faults_reported |= e;
+	std::string m = "the step failed the reference's checks:";
+	if (e & CPGPU_FAULT_UNSOLVABLE) m += " Unsolvable collision or constraint (an effective mass of zero);";
+	if (e & CPGPU_FAULT_BODY_MASS) m += " Body's mass and moment must be positive to simulate;";
+	if (e & CPGPU_FAULT_BODY_STATE) m += " Body's state is invalid (NaN or infinite, or a negative mass or moment);";
+	if (e & (CPGPU_FAULT_BODY_MASS | CPGPU_FAULT_BODY_STATE))
+		m += " first body: " + std::to_string(0x7fffffff - w->h_counts[8]) + ";";
+	return fail(CPGPU_ERR_FAULT, m);
+}
+
 // queue a copy of the step counters and the error word into pinned memory
 void queue_counters(cpgpuWorld *w)
 {
 	(void)hipMemcpyAsync(w->h_counts, w->d_step_counts, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
 	(void)hipMemcpyAsync(w->h_counts + 6, w->d_counters, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
-	(void)hipMemcpyAsync(w->h_counts + 7, w->W.err, sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
+	(void)hipMemcpyAsync(w->h_counts + 7, w->W.err, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, w->stream);
 }
 
 int sync_world(cpgpuWorld *w)
@@ -517,6 +540,13 @@ int sync_world(cpgpuWorld *w)
 	w->pending = false;
 	if (w->graph_on && w->graph_steps > 0) w->last_pairs = w->h_counts[6];   // a captured step's candidate pairs
 	return check_device_error(w);
+}
+
+// sync_world for the readers behind the C ABI
+int wait_world(cpgpuWorld *w)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	return sync_world(w);
 }
 
 } // namespace
@@ -847,7 +877,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(hipHostMalloc((void **)&w->h_ws, sizeof(WorldStep) * w->nw, hipHostMallocDefault));
 	HIP_OR_FAIL(hipHostMalloc((void **)&w->h_counts, sizeof(int32_t) * 16, hipHostMallocDefault));
 	memset(w->h_counts, 0, sizeof(int32_t) * 16);
-	HIP_OR_FAIL(own_upload(w, W.err, (const int32_t *)nullptr, 1));
+	HIP_OR_FAIL(own_upload(w, W.err, (const int32_t *)nullptr, 2));
 	HIP_OR_FAIL(own_upload(w, W.totals, (const unsigned long long *)nullptr, 2 + 2 * CPGPU_SOLVE_UNITS));
 	HIP_OR_FAIL(own_upload(w, w->d_counters, (const int32_t *)nullptr, 4));
 	HIP_OR_FAIL(own_upload(w, w->d_step_counts, (const int32_t *)nullptr, 6));
@@ -1264,9 +1294,10 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 	if (w->in_step) return fail(CPGPU_ERR_INVALID, "a step begun with cpgpuWorldStepBegin is not ended");
 	if (dt == 0.0) return CPGPU_OK;   // cpSpaceStep.c:339
 	if (w->graph_on) return graph_step(w, dt);
-	const int rc = step_begin(w, dt);
+	int rc = step_begin(w, dt);
+	if (rc == CPGPU_OK) rc = step_end(w);
 	if (rc != CPGPU_OK) return rc;
-	return step_end(w);
+	return report_faults(w);   // faults seen by the step's wait, now that it is enqueued in full
 }
 
 // ---- collision handler callbacks: a step in two halves around the host ----
@@ -1342,7 +1373,9 @@ int cpgpuWorldStepEnd(cpgpuWorld *w)
 	if (!w->in_step) return fail(CPGPU_ERR_INVALID, "no step begun with cpgpuWorldStepBegin");
 	w->in_step = false;
 	if (w->zero_step) return CPGPU_OK;
-	return step_end(w);
+	const int rc = step_end(w);
+	if (rc != CPGPU_OK) return rc;
+	return report_faults(w);
 }
 
 int cpgpuWorldReadSeparations(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *pairs)
@@ -1490,7 +1523,7 @@ static int graph_step(cpgpuWorld *w, double dt)
 	w->steps++;
 	w->graph_steps++;
 	w->pending = true;
-	return CPGPU_OK;
+	return report_faults(w);   // faults seen by a capture's wait
 }
 
 int cpgpuWorldSetGraph(cpgpuWorld *w, int32_t enable)
@@ -1529,13 +1562,22 @@ int cpgpuWorldStepN(cpgpuWorld *w, double dt, int32_t n_steps)
 int cpgpuWorldSync(cpgpuWorld *w)
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
-	return sync_world(w);
+	const int rc = sync_world(w);
+	return rc != CPGPU_OK ? rc : report_faults(w);
+}
+
+int cpgpuWorldGetFaults(cpgpuWorld *w, int32_t *faults, int32_t *first_body)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	if (faults) *faults = w->h_counts[7] & CPGPU_FAULTS;
+	if (first_body) *first_body = w->h_counts[8] > 0 ? 0x7fffffff - w->h_counts[8] : -1;
+	return CPGPU_OK;
 }
 
 int cpgpuWorldReadBodies(cpgpuWorld *w, double *p, double *v, double *a, double *wv, double *v_bias, double *w_bias,
                          double *transform)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int nb = w->nb;
 	if (p) HIP_OR_FAIL(download_real(p, reinterpret_cast<const real *>(w->W.b_p), 2 * (size_t)nb));
@@ -1556,7 +1598,7 @@ int cpgpuWorldReadBodies(cpgpuWorld *w, double *p, double *v, double *a, double 
 
 int cpgpuWorldWriteBodies(cpgpuWorld *w, const double *v, const double *wv, const double *f, const double *t)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int nb = w->nb;
 	if (v || wv) {
@@ -1600,7 +1642,7 @@ int cpgpuWorldWriteBodiesAt(cpgpuWorld *w, int32_t n, const int32_t *index, cons
 			return fail(CPGPU_ERR_INVALID, "a static body's pose changes only with a new world (body " +
 			                                   std::to_string(index[k]) + ")");
 	}
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int nb = w->nb;
 	if (p) HIP_OR_FAIL(edit_rows(reinterpret_cast<real *>(w->W.b_p), 2, nb, n, index, p));
@@ -1628,7 +1670,7 @@ int cpgpuWorldSetSpaceParams(cpgpuWorld *w, int32_t space, const cpgpuSpaceParam
 	if (space < 0 || space >= w->nw) return fail(CPGPU_ERR_INVALID, "space index " + std::to_string(space) + " out of range");
 	if (params->iterations < 0) return fail(CPGPU_ERR_INVALID, "iterations must be >= 0");
 	if (w->tape.on) return fail(CPGPU_ERR_INVALID, "space parameters are fixed during a taped rollout");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	w->sp[space] = *params;
 	w->graph_dirty = true;
@@ -1661,7 +1703,7 @@ int cpgpuWorldCreateFrom(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld 
 		if (inv[o] >= 0) return fail(CPGPU_ERR_INVALID, "shape map names old shape " + std::to_string(o) + " twice");
 		inv[o] = s;
 	}
-	rc = cpgpuWorldSync(old);
+	rc = wait_world(old);
 	if (rc != CPGPU_OK) return rc;
 	cpgpuWorld *w = nullptr;
 	rc = cpgpuWorldCreate(d, precision, &w);
@@ -1818,7 +1860,7 @@ int cpgpuWorldSaveState(cpgpuWorld *w, void *buf, int64_t cap, int64_t *size_out
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	if (cap < 0) return fail(CPGPU_ERR_INVALID, "negative capacity");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int L = w->cur ^ 1;
 	SnapshotHeader h;
@@ -1868,7 +1910,7 @@ int cpgpuWorldLoadState(cpgpuWorld *w, const void *buf, int64_t size)
 		return fail(CPGPU_ERR_INVALID, "the snapshot is of a world of another description (counts differ)");
 	if (h.n_recs < 0 || h.n_act < 0 || h.n_cached < 0 || h.steps < 0)
 		return fail(CPGPU_ERR_INVALID, "negative counts in the snapshot");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int P = w->cur ^ 1;
 	HIP_OR_FAIL(regrow(w->recs[P], w->rec_cap[P], (long)h.n_recs + 1));
@@ -1916,7 +1958,7 @@ int cpgpuWorldLoadState(cpgpuWorld *w, const void *buf, int64_t size)
 
 int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int last = w->cur ^ 1;
 	memset(out, 0, sizeof(*out));
@@ -1958,7 +2000,7 @@ int cpgpuWorldGetStats(cpgpuWorld *w, cpgpuStats *out)
 int cpgpuWorldReadArbiters(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *shapes, int32_t *colour,
                            int32_t *state, int32_t *count, double *normal, double *contact, uint64_t *hash)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int last = w->cur ^ 1;
 	const int n = w->steps > 0 ? w->h_counts[3 * last + 1] : 0;
@@ -2005,7 +2047,7 @@ int cpgpuWorldReadArbiters(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *
 
 int cpgpuWorldReadPairs(cpgpuWorld *w, int64_t cap, int64_t *n_out, int32_t *pairs)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	const int64_t n = w->steps > 0 ? w->last_pairs : 0;
 	if (n_out) *n_out = n;
@@ -2083,7 +2125,7 @@ int cpgpuWorldShapesCollide(cpgpuWorld *w, int64_t n, const int32_t *pairs, int3
 
 static int query_trees(cpgpuWorld *w, QueryTrees &T)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	if (!w->dyn_tree_built && w->n_dyn_shapes > 0) {
 		// before the first step: the tree over the boxes cached at creation
@@ -2321,7 +2363,7 @@ int cpgpuWorldQueryShapes(cpgpuWorld *w, int32_t n, const int32_t *shapes, int64
 
 int cpgpuWorldReadConstraints(cpgpuWorld *w, double *jacc, int32_t *colour)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	std::vector<Joint> J(w->nc);
 	if (w->nc) HIP_OR_FAIL(hipMemcpy(J.data(), w->W.joints, sizeof(Joint) * w->nc, hipMemcpyDeviceToHost));
@@ -2341,7 +2383,7 @@ int cpgpuWorldReadConstraints(cpgpuWorld *w, double *jacc, int32_t *colour)
 int cpgpuWorldSetTiming(cpgpuWorld *w, int32_t enable)
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	if (enable < CPGPU_TIMING_OFF || enable > CPGPU_TIMING_SOLVE)
 		return fail(CPGPU_ERR_INVALID, "unknown timing mode " + std::to_string(enable));
@@ -2359,7 +2401,7 @@ int cpgpuWorldSetGridSchedule(cpgpuWorld *w, int32_t schedule)
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	if (schedule != CPGPU_GRID_FLOW && schedule != CPGPU_GRID_PHASES)
 		return fail(CPGPU_ERR_INVALID, "unknown grid schedule " + std::to_string(schedule));
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	w->grid_flow = schedule == CPGPU_GRID_FLOW;
 	w->graph_dirty = true;
@@ -2389,7 +2431,7 @@ int cpgpuWorldSetBuckets(cpgpuWorld *w, int32_t n, const int32_t *max_bodies, co
 		if (!valid_threads(threads[q])) return fail(CPGPU_ERR_INVALID, "bucket threads must be 64..1024, a power of 2");
 		if (q > 0 && max_bodies[q] <= max_bodies[q - 1]) return fail(CPGPU_ERR_INVALID, "bucket limits must rise");
 	}
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	w->n_buckets = n;
 	for (int q = 0; q < n; q++) {
@@ -2405,7 +2447,7 @@ int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *t
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	if (trial_steps < 1) return fail(CPGPU_ERR_INVALID, "trial_steps must be positive");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	static const int candidates[] = {64, 128, 256, 512, 1024};
 	static const char *names[CPGPU_MAX_BUCKETS] = {"solver_worlds_b0", "solver_worlds_b1", "solver_worlds_b2",
@@ -2427,7 +2469,7 @@ int cpgpuWorldAutotune(cpgpuWorld *w, double dt, int32_t trial_steps, int32_t *t
 			rc = cpgpuWorldStep(w, dt);
 			if (rc != CPGPU_OK) { w->timer.enabled = was_timing; w->graph_on = was_graph; return rc; }
 		}
-		rc = cpgpuWorldSync(w);
+		rc = wait_world(w);
 		if (rc != CPGPU_OK) { w->timer.enabled = was_timing; w->graph_on = was_graph; return rc; }
 		for (int q = 0; q < w->n_buckets; q++) {
 			auto it = w->timer.acc.find(names[q]);
@@ -2458,7 +2500,7 @@ int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
 	if (w->n_large_worlds > 0)
 		return fail(CPGPU_ERR_INVALID, "the adjoint supports spaces solved one workgroup each (at most " +
 		                                   std::to_string(w->small_world_max_bodies) + " dynamic bodies)");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	tape_clear(w);
 	auto &T = w->tape;
@@ -2479,13 +2521,13 @@ int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
 	HIP_OR_FAIL(adjoint_state_save(w->W, ck, w->stream));
 	if (!w->d_replay_n) HIP_OR_FAIL(hipMalloc(&w->d_replay_n, sizeof(int32_t)));
 	T.on = true;
-	return cpgpuWorldSync(w);
+	return wait_world(w);
 }
 
 int cpgpuWorldTapeEnd(cpgpuWorld *w)
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	tape_clear(w);
 	return CPGPU_OK;
@@ -2524,7 +2566,7 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	auto &T = w->tape;
 	if (!T.on) return fail(CPGPU_ERR_INVALID, "no tape: call cpgpuWorldTapeBegin before the rollout");
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	DevWorld &W = w->W;
 	KernelTimer *timer = w->timer.enabled ? &w->timer : nullptr;
@@ -2697,7 +2739,7 @@ int cpgpuWorldBackward(cpgpuWorld *w, const double *g_p, const double *g_a, cons
 
 int cpgpuWorldGetSolveCounts(cpgpuWorld *w, int64_t *contacts, int64_t *bodies)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	unsigned long long t[2 + 2 * CPGPU_SOLVE_UNITS];
 	HIP_OR_FAIL(hipMemcpy(t, w->W.totals, sizeof(t), hipMemcpyDeviceToHost));
@@ -2720,7 +2762,7 @@ int cpgpuWorldGetJointGrads(cpgpuWorld *w, double *d_params)
 int cpgpuWorldGetTiming(cpgpuWorld *w, int32_t cap, int32_t *n_out, double *ms, int64_t *launches, char *names,
                         int32_t names_cap)
 {
-	int rc = cpgpuWorldSync(w);
+	int rc = wait_world(w);
 	if (rc != CPGPU_OK) return rc;
 	std::string all;
 	int k = 0;

@@ -28,6 +28,7 @@ EXPORTS = [
     "cpgpuWorldStepBegin", "cpgpuWorldReadCallbacks", "cpgpuWorldWriteDecisions", "cpgpuWorldStepEnd",
     "cpgpuWorldReadSeparations",
     "cpgpuWorldSetSpaceParams", "cpgpuWorldStep", "cpgpuWorldStepN", "cpgpuWorldSync",
+    "cpgpuWorldGetFaults",
     "cpgpuWorldReadBodies", "cpgpuWorldWriteBodies", "cpgpuWorldGetStats", "cpgpuWorldReadArbiters",
     "cpgpuWorldReadConstraints", "cpgpuWorldReadPairs", "cpgpuWorldShapesCollide",
     "cpgpuWorldSetTiming", "cpgpuWorldGetTiming", "cpgpuWorldGetSolveCounts",
@@ -86,6 +87,7 @@ def lib():
         L.cpgpuWorldStep.argtypes = [_vp, ctypes.c_double]
         L.cpgpuWorldStepN.argtypes = [_vp, ctypes.c_double, ctypes.c_int32]
         L.cpgpuWorldSync.argtypes = [_vp]
+        L.cpgpuWorldGetFaults.argtypes = [_vp, _ip, _ip]
         L.cpgpuWorldReadBodies.argtypes = [_vp] + [_dp] * 7
         L.cpgpuWorldWriteBodies.argtypes = [_vp] + [_dp] * 4
         L.cpgpuWorldGetStats.argtypes = [_vp, ctypes.POINTER(Stats)]
@@ -173,6 +175,14 @@ class GPUWorld:
 
     def sync(self):
         _check(self.L.cpgpuWorldSync(self.h))
+
+    def faults(self):
+        """(CPGPU_FAULT_* bits, lowest failing body or -1) as of the last
+        synchronising call: the checks a debug build of the reference makes
+        during cpSpaceStep."""
+        f, b = ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(self.L.cpgpuWorldGetFaults(self.h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
 
     # ---- collision handler callbacks ----
 

@@ -34,6 +34,8 @@ extern "C" {
 #define CPGPU_ERR_HIP -2          /* a HIP runtime call failed */
 #define CPGPU_ERR_CAPACITY -3     /* a fixed device capacity was exceeded */
 #define CPGPU_ERR_DEVICE -4       /* the device-side error word was raised during a step */
+#define CPGPU_ERR_FAULT -5        /* the step ran in full and failed one of the checks a debug build of the
+                                     reference makes (CPGPU_FAULT_*, cpgpuWorldGetFaults) */
 
 /* Body types, same numbering as cpBodyType (include/chipmunk/cpBody.h:30-45). */
 #define CPGPU_BODY_DYNAMIC 0
@@ -300,6 +302,28 @@ int cpgpuWorldStepEnd(cpgpuWorld *world);
 int cpgpuWorldReadSeparations(cpgpuWorld *world, int64_t cap, int64_t *n_out, int32_t *pairs);
 /* Wait for queued steps and check the device error word. */
 int cpgpuWorldSync(cpgpuWorld *world);
+
+/* The conditions a debug build of the reference asserts during cpSpaceStep
+ * (a release build, -DNDEBUG, checks none of them and steps on with
+ * infinities and NaNs).  The device checks them at the same points and steps
+ * on as the release build does.  A fault is returned once, as CPGPU_ERR_FAULT
+ * with the reference's message, by the first step or cpgpuWorldSync after the
+ * device raised it, once that call has done all its work: a fault of a step's
+ * position update by the step itself, one of its solve (effective masses,
+ * velocity update) by the next step or the cpgpuWorldSync before it.  Readers
+ * never return it.  The bits stay set for the world's lifetime. */
+#define CPGPU_FAULT_UNSOLVABLE 128  /* an effective mass of zero: "Unsolvable collision or constraint."
+                                       (k_scalar, include/chipmunk/chipmunk_private.h:223) or "Unsolvable
+                                       constraint." (k_tensor, :255) */
+#define CPGPU_FAULT_BODY_STATE 512  /* cpBodySanityCheck after a position or velocity update (src/cpBody.c:110-125,
+                                       508, 521): a NaN or negative mass or moment, a NaN or infinite
+                                       position, velocity, force, angle, angular velocity or torque */
+#define CPGPU_FAULT_BODY_MASS 1024  /* "Body's mass and moment must be positive to simulate." (src/cpBody.c:498) */
+/* The CPGPU_FAULT_* raised so far (as of the last call that waited for the
+ * device) and the lowest body index that failed a body check (-1: none).
+ * The device keeps a body's moment only as its inverse and checks it through
+ * that. */
+int cpgpuWorldGetFaults(cpgpuWorld *world, int32_t *faults, int32_t *first_body);
 
 /* ---- state readback (the role of the cpBody getters after cpSpaceStep) ---- */
 

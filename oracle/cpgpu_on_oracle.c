@@ -25,6 +25,7 @@ struct cpgpuWorld {
 	oworld *o;
 	int precision;
 	int32_t nb, ns, nc;
+	int32_t faults_reported;   /* the CPGPU_FAULT_* a step has already returned */
 };
 
 static char last_error[256];
@@ -119,10 +120,28 @@ int cpgpuWorldStep(cpgpuWorld *w, double dt)
 {
 	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
 	ow_step(w->o, dt);
-	return CPGPU_OK;
+	/* the reference's debug-build checks, returned once as the device library does */
+	int32_t first;
+	const int32_t f = ow_faults(w->o, &first) & ~w->faults_reported;
+	if (!f) return CPGPU_OK;
+	w->faults_reported |= f;
+	char msg[200];
+	snprintf(msg, sizeof(msg), "the step failed the reference's checks:%s%s%s first body: %d;",
+	         f & CPGPU_FAULT_UNSOLVABLE ? " Unsolvable collision or constraint (an effective mass of zero);" : "",
+	         f & CPGPU_FAULT_BODY_MASS ? " Body's mass and moment must be positive to simulate;" : "",
+	         f & CPGPU_FAULT_BODY_STATE ? " Body's state is invalid;" : "", first);
+	return fail(CPGPU_ERR_FAULT, msg);
 }
 
 int cpgpuWorldSync(cpgpuWorld *w) { return w ? CPGPU_OK : fail(CPGPU_ERR_INVALID, "null world"); }
+
+int cpgpuWorldGetFaults(cpgpuWorld *w, int32_t *faults, int32_t *first_body)
+{
+	if (!w) return fail(CPGPU_ERR_INVALID, "null world");
+	const int32_t f = ow_faults(w->o, first_body);
+	if (faults) *faults = f;
+	return CPGPU_OK;
+}
 
 int cpgpuWorldReadBodies(cpgpuWorld *w, double *p, double *v, double *a, double *wv, double *v_bias, double *w_bias,
                          double *transform)

@@ -275,6 +275,9 @@ struct oworld {
 	void *hctx;
 	int32_t *seps;      /* the last step's separations {space, a, b} */
 	int64_t nseps, sepcap;
+	/* the conditions a debug build of the reference asserts during a step
+	 * (CPGPU_FAULT_*), kept from step to step, and the lowest body that failed */
+	int32_t faults, fault_body;
 };
 
 static int grid_parts_of(const oworld *w);
@@ -310,6 +313,7 @@ static void fat_bb(oworld *w, OShape *s);
 oworld *ow_new(const cpgpuWorldDesc *d)
 {
 	oworld *w = (oworld *)calloc(1, sizeof(oworld));
+	w->fault_body = -1;
 	w->nw = d->n_worlds;
 	w->nb = d->n_bodies;
 	w->ns = d->n_shapes;
@@ -527,6 +531,25 @@ void ow_set_next_order(oworld *w, int64_t n, const int32_t *pairs)
 /* integration: cpBodyUpdatePosition / SetTransform (cpBody.c:511-522, 347-357)
  * and cpBodyUpdateVelocity (cpBody.c:493-509)                          */
 
+/* cpBodySanityCheck (cpBody.c:110-125): what a debug build of the reference
+ * asserts of a body after each of its integrations (cpBody.c:508,521) */
+static int fin(num x) { return isfinite(NUM_VAL(x)); }
+
+static int body_sane(const OBody *b)
+{
+	return NUM_VAL(b->m) == NUM_VAL(b->m) && NUM_VAL(b->m_inv) == NUM_VAL(b->m_inv) &&
+	       NUM_VAL(b->i) == NUM_VAL(b->i) && NUM_VAL(b->i_inv) == NUM_VAL(b->i_inv) &&
+	       NUM_VAL(b->m) >= 0.0 && NUM_VAL(b->i) >= 0.0 &&
+	       fin(b->p.x) && fin(b->p.y) && fin(b->v.x) && fin(b->v.y) && fin(b->f.x) && fin(b->f.y) &&
+	       fin(b->a) && fin(b->w) && fin(b->t);
+}
+
+static void body_fault(oworld *w, int32_t code, int body)
+{
+	w->faults |= code;
+	if (w->fault_body < 0 || body < w->fault_body) w->fault_body = body;
+}
+
 static void update_position(oworld *w, OBody *b, num dt)
 {
 	b->p = vadd(b->p, vmult(vadd(b->v, b->v_bias), dt));
@@ -541,15 +564,19 @@ static void update_position(oworld *w, OBody *b, num dt)
 	b->T.ty = b->p.y - (c.x * sn + c.y * cs);
 	b->v_bias = v2(0.0, 0.0);
 	b->w_bias = 0.0;
+	if (!body_sane(b)) body_fault(w, CPGPU_FAULT_BODY_STATE, (int)(b - w->B));
 }
 
-static void update_velocity(OBody *b, V g, num damping, num dt)
+static void update_velocity(oworld *w, OBody *b, V g, num damping, num dt)
 {
 	if (b->type == CPGPU_BODY_KINEMATIC) return;
+	/* :498 "Body's mass and moment must be positive to simulate." */
+	if (!(NUM_VAL(b->m) > 0.0 && NUM_VAL(b->i) > 0.0)) body_fault(w, CPGPU_FAULT_BODY_MASS, (int)(b - w->B));
 	b->v = vadd(vmult(b->v, damping), vmult(vadd(g, vmult(b->f, b->m_inv)), dt));
 	b->w = b->w * damping + b->t * b->i_inv * dt;
 	b->f = v2(0.0, 0.0);
 	b->t = 0.0;
+	if (!body_sane(b)) body_fault(w, CPGPU_FAULT_BODY_STATE, (int)(b - w->B));
 }
 
 /* ------------------------------------------------------------------ */
@@ -1120,7 +1147,16 @@ static IPair *world_pairs(oworld *w, OWorldPart *P, int64_t *n_out)
 	int n = P->s1 - P->s0;
 	num *bbs = (num *)malloc((size_t)(4 * (n ? n : 1)) * sizeof(num));
 	int *idx = (int *)malloc((size_t)(n ? n : 1) * sizeof(int));
-	for (int k = 0; k < n; k++) { memcpy(bbs + 4 * k, w->S[P->s0 + k].bb, 4 * sizeof(num)); idx[k] = k; }
+	int m = 0;
+	for (int k = 0; k < n; k++) {
+		const num *bb = w->S[P->s0 + k].bb;
+		memcpy(bbs + 4 * k, bb, 4 * sizeof(num));
+		/* a box with a NaN side overlaps nothing (cpBBIntersects compares
+		 * every side), so it stays out of the sweep, whose order it would break */
+		if (!isnan(NUM_VAL(bb[0])) && !isnan(NUM_VAL(bb[1])) && !isnan(NUM_VAL(bb[2])) && !isnan(NUM_VAL(bb[3])))
+			idx[m++] = k;
+	}
+	n = m;
 	g_sort_bbs = bbs;
 	qsort(idx, (size_t)n, sizeof(int), cmp_by_left);
 	int64_t cap = 1024, np = 0;
@@ -1792,9 +1828,12 @@ static inline num k_scalar_body(const OBody *b, V r, V n)           /* :212-217 
 	return b->m_inv + b->i_inv * rcn * rcn;
 }
 
-static inline num k_scalar(const OBody *a, const OBody *b, V r1, V r2, V n)  /* :219-226 */
+/* :219-226, with its "Unsolvable collision or constraint." check (:223) */
+static inline num k_scalar(oworld *w, const OBody *a, const OBody *b, V r1, V r2, V n)
 {
-	return k_scalar_body(a, r1, n) + k_scalar_body(b, r2, n);
+	num value = k_scalar_body(a, r1, n) + k_scalar_body(b, r2, n);
+	if (NUM_VAL(value) == 0.0) w->faults |= CPGPU_FAULT_UNSOLVABLE;
+	return value;
 }
 
 static void arb_prestep(oworld *w, OArb *arb, num dt, num slop, num bias)
@@ -1804,8 +1843,8 @@ static void arb_prestep(oworld *w, OArb *arb, num dt, num slop, num bias)
 	V body_delta = vsub(b->p, a->p);
 	for (int i = 0; i < arb->count; i++) {
 		OContact *con = &arb->con[i];
-		con->nMass = 1.0 / k_scalar(a, b, con->r1, con->r2, n);
-		con->tMass = 1.0 / k_scalar(a, b, con->r1, con->r2, vperp(n));
+		con->nMass = 1.0 / k_scalar(w, a, b, con->r1, con->r2, n);
+		con->tMass = 1.0 / k_scalar(w, a, b, con->r1, con->r2, vperp(n));
 		num dist = vdot(vadd(vsub(con->r2, con->r1), body_delta), n);
 		con->bias = -bias * fminc(0.0, dist + slop) / dt;
 		con->jBias = 0.0;
@@ -1868,8 +1907,9 @@ static void arb_apply_impulse(oworld *w, OArb *arb)
 
 static num bias_coef_host(num errorBias, num dt) { return 1.0 - pow(NUM_VAL(errorBias), NUM_VAL(dt)); }
 
-/* k_tensor (chipmunk_private.h:228-262): the inverse of the 2x2 effective mass */
-static void k_tensor(const OBody *a, const OBody *b, V r1, V r2, OJoint *j)
+/* k_tensor (chipmunk_private.h:228-262): the inverse of the 2x2 effective
+ * mass, with its "Unsolvable constraint." check (:255) */
+static void k_tensor(oworld *w, const OBody *a, const OBody *b, V r1, V r2, OJoint *j)
 {
 	num m_sum = a->m_inv + b->m_inv;
 	num k11 = m_sum, k12 = 0.0, k21 = 0.0, k22 = m_sum;
@@ -1884,6 +1924,7 @@ static void k_tensor(const OBody *a, const OBody *b, V r1, V r2, OJoint *j)
 	num r2nxy = -r2.x * r2.y * b_i_inv;
 	k11 += r2ysq; k12 += r2nxy; k21 += r2nxy; k22 += r2xsq;
 	num det = k11 * k22 - k12 * k21;
+	if (NUM_VAL(det) == 0.0) w->faults |= CPGPU_FAULT_UNSOLVABLE;
 	num det_inv = 1.0 / det;
 	j->k11 = k22 * det_inv; j->k12 = -k12 * det_inv;
 	j->k21 = -k21 * det_inv; j->k22 = k11 * det_inv;
@@ -1904,7 +1945,7 @@ static void joint_prestep(oworld *w, OJoint *j, num dt)
 	if (j->type == CPGPU_JOINT_PIVOT) {
 		j->r1 = xvect(a->T, vsub(anchorA, a->cog));
 		j->r2 = xvect(b->T, vsub(anchorB, b->cog));
-		k_tensor(a, b, j->r1, j->r2, j);
+		k_tensor(w, a, b, j->r1, j->r2, j);
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
 		j->bias2 = vclamp(vmult(delta, -bias_coef_host(j->errorBias, dt) / dt), j->maxBias);
 	} else if (j->type == CPGPU_JOINT_DAMPED_SPRING) {
@@ -1914,7 +1955,7 @@ static void joint_prestep(oworld *w, OJoint *j, num dt)
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
 		num dist = vlength(delta);
 		j->n = vmult(delta, 1.0 / (dist != 0.0 ? dist : INFINITY));
-		num k = k_scalar(a, b, j->r1, j->r2, j->n);
+		num k = k_scalar(w, a, b, j->r1, j->r2, j->n);
 		j->nMass = 1.0 / k;
 		j->target_vrn = 0.0;
 		j->v_coef = 1.0 - (w->libm_trig ? exp(-damping * dt * k) : cp_exp_oracle(-damping * dt * k));
@@ -1931,7 +1972,7 @@ static void joint_prestep(oworld *w, OJoint *j, num dt)
 		if (dist > mx) { pdist = dist - mx; j->n = vnormalize(delta); }
 		else if (dist < mn) { pdist = mn - dist; j->n = vneg(vnormalize(delta)); }
 		else { j->n = v2(0.0, 0.0); j->jnAcc = 0.0; }
-		j->nMass = 1.0 / k_scalar(a, b, j->r1, j->r2, j->n);
+		j->nMass = 1.0 / k_scalar(w, a, b, j->r1, j->r2, j->n);
 		num maxBias = j->maxBias;
 		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * pdist / dt, -maxBias, maxBias);
 	} else if (j->type == CPGPU_JOINT_GEAR) {
@@ -1946,7 +1987,7 @@ static void joint_prestep(oworld *w, OJoint *j, num dt)
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
 		num dist = vlength(delta);
 		j->n = vmult(delta, 1.0 / (dist != 0.0 ? dist : INFINITY));
-		j->nMass = 1.0 / k_scalar(a, b, j->r1, j->r2, j->n);
+		j->nMass = 1.0 / k_scalar(w, a, b, j->r1, j->r2, j->n);
 		num maxBias = j->maxBias;
 		j->bias = fclampc(-bias_coef_host(j->errorBias, dt) * (dist - dist_rest) / dt, -maxBias, maxBias);
 	} else if (j->type == CPGPU_JOINT_GROOVE) {
@@ -1969,7 +2010,7 @@ static void joint_prestep(oworld *w, OJoint *j, num dt)
 			j->clamp = 0.0;
 			j->r1 = vsub(vadd(vmult(vperp(n), -td), vmult(n, d)), a->p);
 		}
-		k_tensor(a, b, j->r1, j->r2, j);
+		k_tensor(w, a, b, j->r1, j->r2, j);
 		V delta = vsub(vadd(b->p, j->r2), vadd(a->p, j->r1));
 		j->bias2 = vclamp(vmult(delta, -bias_coef_host(j->errorBias, dt) / dt), j->maxBias);
 	} else if (j->type == CPGPU_JOINT_ROTARY_LIMIT) {
@@ -2418,7 +2459,7 @@ static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, i
 	/* :399-404 integrate velocities */
 	num damping = pow(sp->damping, NUM_VAL(dt));
 	V g = v2(sp->gravity_x, sp->gravity_y);
-	for (int b = P->b0; b < P->b1; b++) if (w->B[b].type != CPGPU_BODY_STATIC) update_velocity(&w->B[b], g, damping, dt);
+	for (int b = P->b0; b < P->b1; b++) if (w->B[b].type != CPGPU_BODY_STATIC) update_velocity(w, &w->B[b], g, damping, dt);
 	/* :407-415 cached impulses */
 	num dt_coef = (prev_dt == 0.0 ? 0.0 : dt / prev_dt);
 	for (int64_t k = 0; k < P->nactive; k++) arb_apply_cached(w, &P->arbs[P->active[k]], dt_coef);
@@ -2440,6 +2481,12 @@ static void step_world(oworld *w, OWorldPart *P, num dt, const int32_t *order, i
 			cb_view(w, (int)(P - w->W), arb, &v);
 			w->hfn(w->hctx, CPGPU_EVENT_POST_SOLVE, &v, &dec);
 		}
+}
+
+int32_t ow_faults(const oworld *w, int32_t *first_body)
+{
+	if (first_body) *first_body = w->fault_body;
+	return w->faults;
 }
 
 void ow_step(oworld *w, double dt)

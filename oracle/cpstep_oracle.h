@@ -38,6 +38,10 @@ void ow_step(oworld *w, double dt);
 void ow_set_handler_fn(oworld *w, ow_handler_fn fn, void *ctx);
 /* The last step's separations of pairs with a separate event: {space, a, b} [cap][3]. */
 int64_t ow_read_separations(oworld *w, int64_t cap, int32_t *pairs);
+/* The CPGPU_FAULT_* raised since the world was made (the checks a debug build
+ * of the reference makes during a step) and the lowest body that failed a body
+ * check (-1: none). */
+int32_t ow_faults(const oworld *w, int32_t *first_body);
 
 /* 1: body rotations use libm cos/sin (cpBody.c:350 via cpVect.h:104-107) and
  * the damped spring libm exp (cpDampedSpring.c:47) exactly as the reference

@@ -400,3 +400,30 @@ def handlers_run_the_callers_callbacks(kind):
     counts = {k: len(rg.events(k)) for k in kinds}
     assert all(c > 10 for c in counts.values()), counts
     g.free()
+
+
+def faults_report_like_a_debug_build(kind):
+    """A body whose velocity is NaN: cpGPUSpaceStep runs the step in full and
+    reports the check a debug build of the reference fails there (aborting,
+    by default, as that build does); the report comes once, and the space
+    steps and syncs on as a release build would."""
+    from fault_scenes import build
+    pw, bad, fault, step, message = build("nan_velocity")
+    g = GPUSpace(pw, kind)
+    g.B.cpGPUSpaceStep(g.space, DT)
+    err = g.B.cpGPUGetError().decode()
+    assert "returned -5" in err and "Body's state is invalid" in err, err
+    g.B.cpGPUClearError()
+    o = OracleWorld(g.desc())   # the binding's own description, in its body order
+    o.step(DT)
+    g.step()
+    o.step(DT)
+    g.sync()
+    ob = o.bodies()
+    for k, b in enumerate(g.bodies):
+        raw, j = body_raw(b), g.index(b)
+        if k == bad:
+            assert np.isnan(raw[RAW["p"]][0]) and np.isnan(ob.p[j][0])
+        else:
+            assert np.array_equal(raw[RAW["p"]], ob.p[j]) and np.array_equal(raw[RAW["v"]], ob.v[j]), k
+    g.free()

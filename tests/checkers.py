@@ -18,6 +18,8 @@ GPUSPACE_CPU_SO = os.path.join(ROOT, "oracle", "_ref", "libcpgpuspace_cpu.so")
 DUAL_SO = os.path.join(ROOT, "oracle", "libdualstep.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref.so")
 REF_RELEASE_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref_release.so")
+# the strict build with the reference's assertions on: it aborts where a debug build would
+REF_DEBUG_SO = os.path.join(ROOT, "oracle", "_ref", "libcpref_debug.so")
 
 _d = ctypes.c_double
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -74,6 +76,8 @@ def oracle_lib():
         lib.ow_set_handler_fn.argtypes = [_vp, HANDLER_FN, _vp]
         lib.ow_read_separations.restype = ctypes.c_int64
         lib.ow_read_separations.argtypes = [_vp, ctypes.c_int64, _ip]
+        lib.ow_faults.restype = ctypes.c_int32
+        lib.ow_faults.argtypes = [_vp, _ip]
         lib.ow_shape_query.restype = ctypes.c_int64
         lib.ow_shape_query.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64, _ip, _ip, _dp]
         lib.ow_set_space_params.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(SpaceParams)]
@@ -86,8 +90,8 @@ def oracle_lib():
     return _oracle
 
 
-def ref_lib(release=False):
-    path = REF_RELEASE_SO if release else REF_SO
+def ref_lib(release=False, debug=False):
+    path = REF_RELEASE_SO if release else REF_DEBUG_SO if debug else REF_SO
     if path not in _refs:
         lib = ctypes.CDLL(path)
         lib.rb_new.restype = _vp
@@ -224,6 +228,11 @@ class OracleWorld:
             self.lib.ow_read_separations(self.h, n, ptr(out, ctypes.c_int32))
         return out
 
+    def faults(self):
+        """(CPGPU_FAULT_* bits, lowest failing body or -1) raised so far."""
+        b = ctypes.c_int32(0)
+        return int(self.lib.ow_faults(self.h, ctypes.byref(b))), b.value
+
     def libm_trig(self, on=True):
         self.lib.ow_set_libm_trig(self.h, 1 if on else 0)
 
@@ -353,8 +362,8 @@ def _collide(fn, h, a, b, cid):
 class RefWorld:
     """World `w` of a packed description built as a real cpSpace."""
 
-    def __init__(self, packed, w=0, hasty_threads=0, release=False):
-        self.lib = ref_lib(release)
+    def __init__(self, packed, w=0, hasty_threads=0, release=False, debug=False):
+        self.lib = ref_lib(release, debug)
         self.packed = packed
         self.desc = packed.desc()
         self.w = w

@@ -42,3 +42,7 @@ def test_batch_of_plain_spaces_and_alone():
 
 def test_every_joint_class_syncs_to_host():
     F.joints_sync_to_host("cpu")
+
+
+def test_faults_report_like_a_debug_build():
+    F.faults_report_like_a_debug_build("cpu")

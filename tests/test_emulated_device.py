@@ -112,3 +112,12 @@ def test_device_allocation_failures_are_reported(emulated):
     g = emulated.GPUWorld(pw)
     g.step(DT, 4)
     assert np.all(np.isfinite(g.bodies().p))
+
+
+def test_faults_reported_as_the_oracle(emulated):
+    """The reference's debug-build checks: an unsolvable joint, a NaN state, a
+    position overflowing some steps in; the same faults, step and first body
+    as the oracle, and the bodies the fault does not touch still bit-exact."""
+    from test_faults import test_device_reports_the_oracles_faults as check
+    for name in ("kinematic_pivot", "nan_velocity", "runaway"):
+        check(name)

@@ -48,3 +48,7 @@ def test_every_joint_class_syncs_to_host():
 
 def test_collision_handlers_run_the_callers_callbacks():
     F.handlers_run_the_callers_callbacks("gpu")
+
+
+def test_faults_report_like_a_debug_build():
+    F.faults_report_like_a_debug_build("gpu")

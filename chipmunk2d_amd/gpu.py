@@ -4,8 +4,10 @@ The library is built in-tree (chipmunk2d_amd/libcpgpu.so, see build.py).  If
 it is missing this module raises on import of `lib()`: there is no CPU
 fallback for the step.
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -133,6 +135,19 @@ def _check(rc):
         raise CpgpuError("cpgpu error %d: %s" % (rc, lib().cpgpuGetLastError().decode()))
 
 
+# Worlds still open when the interpreter exits are destroyed by an atexit
+# handler, while the HIP runtime is certainly alive, rather than by __del__
+# during module teardown, whose order against the runtime's (and a
+# profiler's) own exit handlers is not fixed.
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_worlds():
+    for w in list(_live):
+        w.close()
+
+
 class GPUWorld:
     """One or more packed spaces resident on the current HIP device."""
 
@@ -143,6 +158,7 @@ class GPUWorld:
         h = _vp()
         _check(self.L.cpgpuWorldCreate(ctypes.byref(self.desc), precision, ctypes.byref(h)))
         self.h = h
+        _live.add(self)
 
     @classmethod
     def from_old(cls, packed, old, shape_from, precision=FP64):
@@ -157,6 +173,7 @@ class GPUWorld:
         _check(self.L.cpgpuWorldCreateFrom(ctypes.byref(self.desc), precision, old.h, ptr(sf, ctypes.c_int32),
                                            ctypes.byref(h)))
         self.h = h
+        _live.add(self)
         return self
 
     def close(self):

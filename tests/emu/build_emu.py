@@ -29,7 +29,12 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "chipmunk2d_amd", "csrc")
-BUILD = os.path.join(HERE, "_build")
+# EMU_ASAN=1: an AddressSanitizer build into _build_asan, with allocations of
+# their exact size, run with libasan preloaded (python itself is not built with it):
+#   ASAN_OPTIONS=detect_leaks=0 LD_PRELOAD=$(gcc -print-file-name=libasan.so) python ...
+ASAN = os.environ.get("EMU_ASAN") == "1"
+SAN = ["-fsanitize=address", "-fno-omit-frame-pointer", "-DEMU_EXACT_ALLOC"] if ASAN else []
+BUILD = os.path.join(HERE, "_build_asan" if ASAN else "_build")
 OUT = os.path.join(BUILD, "libcpgpu_emu.so")
 SOURCES = ["world.hip", "broadphase.hip", "narrowphase.hip", "solver.hip", "adjoint.hip", "query.hip"]
 VARIANTS = {"64": ("cpgpu64", "double"), "32": ("cpgpu32", "float")}
@@ -143,7 +148,7 @@ def build(verbose=True):
         s, v = unit
         obj = os.path.join(BUILD, "obj", os.path.splitext(s)[0] + ("_fp" + v if v else "") + ".o")
         cmd = ["g++", "-x", "c++", "-std=c++17", "-O2", "-g", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
-               "-Wno-attributes", "-Wno-unused-result", "-I", HERE]
+               "-Wno-attributes", "-Wno-unused-result", "-I", HERE] + SAN
         if v:
             ns, real = VARIANTS[v]
             cmd += ["-DCPGPU_NS=" + ns, "-DCPGPU_REAL=" + real, "-DEMU_GRID_STATIC_LDS=%d" % lds[v]]
@@ -160,17 +165,18 @@ def build(verbose=True):
     if failed:
         raise RuntimeError("g++ failed on %s" % ", ".join("%s (fp%s)" % (s, v) for s, v, *_ in failed))
     rt = os.path.join(BUILD, "obj", "emu_runtime.o")
-    r = subprocess.run(["g++", "-std=c++17", "-O2", "-g", "-fPIC", "-I", HERE, "-c", os.path.join(HERE, "emu_runtime.cpp"),
-                        "-o", rt], capture_output=True, text=True)
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-g", "-fPIC", "-I", HERE] + SAN +
+                       ["-c", os.path.join(HERE, "emu_runtime.cpp"), "-o", rt], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("emu_runtime.cpp:\n" + r.stderr)
     tmp = OUT + ".tmp"
-    r = subprocess.run(["g++", "-shared", "-o", tmp] + [x[2] for x in results] + [rt, "-lpthread"],
+    r = subprocess.run(["g++", "-shared"] + SAN[:1] + ["-o", tmp] + [x[2] for x in results] + [rt, "-lpthread"],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, OUT)
-    build_binding()
+    if not ASAN:
+        build_binding()
     return OUT
 
 

@@ -395,7 +395,12 @@ hipError_t hipMalloc(void **p, size_t bytes)
 	if (malloc_countdown > 0 && --malloc_countdown == 0) return hipErrorOutOfMemory;
 	if (bytes == 0) return ok();
 	void *q = nullptr;
-	if (posix_memalign(&q, 256, (bytes + 255) & ~(size_t)255) != 0) return hipErrorOutOfMemory;
+#ifdef EMU_EXACT_ALLOC
+	const size_t size = bytes;   // the AddressSanitizer build: a read past the allocation is reported
+#else
+	const size_t size = (bytes + 255) & ~(size_t)255;
+#endif
+	if (posix_memalign(&q, 256, size) != 0) return hipErrorOutOfMemory;
 	memset(q, 0xA5, bytes);   // device memory holds no zeros it was not given
 	*p = q;
 	return ok();

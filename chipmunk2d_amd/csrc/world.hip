@@ -324,6 +324,12 @@ struct cpgpuWorld {
 		std::vector<BodyState *> ckpt;  // bodies at steps 0, every, 2 every, ...
 		std::vector<Joint *> jckpt;     // joints at the same steps
 	} tape;
+	// a finished tape's buffers, kept for the next tape: a loop that tapes every
+	// rollout allocates only on its first (the sizes depend only on the world)
+	std::vector<BodyState *> spare_ckpt;
+	std::vector<Joint *> spare_jckpt;
+	Arbiter *spare_arb = nullptr;
+	long spare_arb_cap = 0;
 	std::vector<double> h_joint_grads;  // [nc * 8] parameter adjoints of the last backward pass
 	int32_t *d_replay_n = nullptr;
 	// solve units of the grid set
@@ -926,7 +932,7 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	return CPGPU_OK;
 }
 
-static void tape_clear(cpgpuWorld *w);
+static void tape_clear(cpgpuWorld *w, bool release);
 
 void cpgpuWorldDestroy(cpgpuWorld *w)
 {
@@ -947,7 +953,7 @@ void cpgpuWorldDestroy(cpgpuWorld *w)
 	if (w->d_graph_pairs) (void)hipFree(w->d_graph_pairs);
 	if (w->h_ws) (void)hipHostFree(w->h_ws);
 	if (w->h_counts) (void)hipHostFree(w->h_counts);
-	tape_clear(w);
+	tape_clear(w, true);
 	if (w->d_replay_n) (void)hipFree(w->d_replay_n);
 	for (int q = 0; q < CPGPU_MAX_BUCKETS; q++)
 		if (w->bucket_lists[q]) (void)hipFree(w->bucket_lists[q]);
@@ -993,6 +999,12 @@ static SolveBuffers solve_buffers(cpgpuWorld *w, Arbiter *act, const int32_t *n_
 static hipError_t tape_reserve(cpgpuWorld *w, long n)
 {
 	auto &T = w->tape;
+	if (!T.arb && w->spare_arb) {   // the last tape's records buffer (holds nothing yet)
+		T.arb = w->spare_arb;
+		T.cap = w->spare_arb_cap;
+		w->spare_arb = nullptr;
+		w->spare_arb_cap = 0;
+	}
 	if (n <= T.cap) return hipSuccess;
 	long cap = std::max<long>(1024, T.cap);
 	while (cap < n) cap *= 2;
@@ -1006,13 +1018,35 @@ static hipError_t tape_reserve(cpgpuWorld *w, long n)
 	return hipSuccess;
 }
 
-static void tape_clear(cpgpuWorld *w)
+// end the tape, its buffers kept for the next one (release: freed)
+static void tape_clear(cpgpuWorld *w, bool release = false)
 {
 	auto &T = w->tape;
-	if (T.arb) (void)hipFree(T.arb);
-	for (BodyState *c : T.ckpt) (void)hipFree(c);
-	for (Joint *c : T.jckpt) (void)hipFree(c);
+	if (T.arb && (w->spare_arb || release)) (void)hipFree(T.arb);
+	else if (T.arb) { w->spare_arb = T.arb; w->spare_arb_cap = T.cap; }
+	w->spare_ckpt.insert(w->spare_ckpt.end(), T.ckpt.begin(), T.ckpt.end());
+	w->spare_jckpt.insert(w->spare_jckpt.end(), T.jckpt.begin(), T.jckpt.end());
 	T = cpgpuWorld::Tape();
+	if (release) {
+		if (w->spare_arb) (void)hipFree(w->spare_arb);
+		w->spare_arb = nullptr;
+		w->spare_arb_cap = 0;
+		for (BodyState *c : w->spare_ckpt) (void)hipFree(c);
+		for (Joint *c : w->spare_jckpt) (void)hipFree(c);
+		w->spare_ckpt.clear();
+		w->spare_jckpt.clear();
+	}
+}
+
+// a checkpoint buffer of n elements, a kept one if there is one
+template <class E> static hipError_t take_spare(std::vector<E *> &spare, E *&p, size_t n)
+{
+	if (!spare.empty()) {
+		p = spare.back();
+		spare.pop_back();
+		return hipSuccess;
+	}
+	return hipMalloc(&p, sizeof(E) * n);
 }
 
 // bias_coef (chipmunk_private.h:264-268) per joint for step dt, libm pow on the
@@ -1215,7 +1249,7 @@ static int step_begin(cpgpuWorld *w, double dt)
 	if (w->tape.on) {
 		if (w->tape.steps > 0 && w->tape.steps % w->tape.every == 0) {
 			BodyState *ck = nullptr;
-			HIP_OR_FAIL(hipMalloc(&ck, sizeof(BodyState) * (size_t)w->nb));
+			HIP_OR_FAIL(take_spare(w->spare_ckpt, ck, (size_t)w->nb));
 			w->tape.ckpt.push_back(ck);
 			HIP_OR_FAIL(adjoint_state_save(W, ck, w->stream));
 		}
@@ -1225,7 +1259,7 @@ static int step_begin(cpgpuWorld *w, double dt)
 	if (w->tape.on && w->nc > 0 && w->tape.steps % w->tape.every == 0) {
 		// the joints at the checkpoint, with this step's bias coefficients
 		Joint *jk = nullptr;
-		HIP_OR_FAIL(hipMalloc(&jk, sizeof(Joint) * (size_t)w->nc));
+		HIP_OR_FAIL(take_spare(w->spare_jckpt, jk, (size_t)w->nc));
 		w->tape.jckpt.push_back(jk);
 		HIP_OR_FAIL(hipMemcpyAsync(jk, W.joints, sizeof(Joint) * (size_t)w->nc, hipMemcpyDeviceToDevice, w->stream));
 	}
@@ -2516,7 +2550,7 @@ int cpgpuWorldTapeBegin(cpgpuWorld *w, int32_t checkpoint_every)
 		T.n = (long)n_act + n_cached;
 	}
 	BodyState *ck = nullptr;
-	HIP_OR_FAIL(hipMalloc(&ck, sizeof(BodyState) * (size_t)w->nb));
+	HIP_OR_FAIL(take_spare(w->spare_ckpt, ck, (size_t)w->nb));
 	T.ckpt.push_back(ck);
 	HIP_OR_FAIL(adjoint_state_save(w->W, ck, w->stream));
 	if (!w->d_replay_n) HIP_OR_FAIL(hipMalloc(&w->d_replay_n, sizeof(int32_t)));

@@ -38,4 +38,9 @@ g, o = G.GPUWorld(pw), OracleWorld(pw)
 for _ in range(5):
     g.step(1/60); o.step(1/60)
 _check_shape_queries(g, o, pw)
+print("tapes", flush=True)
+import test_adjoint
+test_adjoint.GPUWorld = G.GPUWorld
+test_adjoint.test_a_second_tape_reuses_the_first_ones_buffers()
+test_adjoint.test_backward_leaves_the_world_unchanged()
 print("ALL OK", flush=True)

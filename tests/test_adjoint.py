@@ -90,6 +90,31 @@ def test_backward_leaves_the_world_unchanged():
         assert np.array_equal(getattr(x, f), getattr(y, f)), f
 
 
+def test_a_second_tape_reuses_the_first_ones_buffers():
+    """A world taped twice (the second tape on the checkpoint and record
+    buffers the first one left, with a different interval) gives the
+    gradients of a fresh world taped once from the same state."""
+    pw, _ = small_batch(seed=0xADA4)
+    g_p = np.ones((pw.n_bodies, 2))
+    a = GPUWorld(pw)
+    a.tape_begin(3)
+    a.step(DT, 12)
+    a.backward(g_p=g_p)
+    a.tape_end()
+    a.tape_begin(5)
+    a.step(DT, 20)
+    second = a.backward(g_p=g_p)
+    a.tape_end()
+    b = GPUWorld(pw)
+    b.step(DT, 12)
+    b.tape_begin(5)
+    b.step(DT, 20)
+    fresh = b.backward(g_p=g_p)
+    b.tape_end()
+    for k in fresh:
+        assert np.array_equal(second[k], fresh[k]), k
+
+
 def _linear_loss(pw, weights, steps):
     """L = sum of weights . p_T, and the rollout's contact counts."""
     g = GPUWorld(pw)

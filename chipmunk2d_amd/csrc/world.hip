@@ -887,7 +887,9 @@ int cpgpuWorldCreate(const cpgpuWorldDesc *d, int32_t precision, cpgpuWorld **ou
 	HIP_OR_FAIL(own_upload(w, W.totals, (const unsigned long long *)nullptr, 2 + 2 * CPGPU_SOLVE_UNITS));
 	HIP_OR_FAIL(own_upload(w, w->d_counters, (const int32_t *)nullptr, 4));
 	HIP_OR_FAIL(own_upload(w, w->d_step_counts, (const int32_t *)nullptr, 6));
-	HIP_OR_FAIL(own_alloc(w, w->d_world_bounds, w->nw));
+	// four ordered 64-bit keys per world (k_bounds_reset / k_bounds), whatever the size of
+	// real: one real4 in FP64, two in FP32
+	HIP_OR_FAIL(own_alloc(w, w->d_world_bounds, (size_t)w->nw * (4 * sizeof(unsigned long long) / sizeof(real4))));
 	HIP_OR_FAIL(own_alloc(w, w->d_joint_bias_coef, nc));
 	HIP_OR_FAIL(own_upload(w, w->bar, (const GridBarrier *)nullptr, 1));
 	HIP_OR_FAIL(own_upload(w, w->warb_off, (const int32_t *)nullptr, w->nw + 2));

@@ -43,4 +43,13 @@ import test_adjoint
 test_adjoint.GPUWorld = G.GPUWorld
 test_adjoint.test_a_second_tape_reuses_the_first_ones_buffers()
 test_adjoint.test_backward_leaves_the_world_unchanged()
+print("fp32", flush=True)
+from chipmunk2d_amd.cabi import FP32
+from chipmunk2d_amd.scenes import chains_packed
+for pw in (chains_packed(n_worlds=64, processes=4),
+           random_world(n_circles=2600, n_polys=2600, n_segments=300, extent=70.0, seed=23).pack()):
+    g = G.GPUWorld(pw, FP32)
+    g.step(1 / 60, 3)
+    g.bodies()
+    g.close()
 print("ALL OK", flush=True)
